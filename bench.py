@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-updates/s of the five-equation WENO5-HLLC TVD-RK3 hyperbolic step on one GPU.
+
+A step is one full TVD-RK3 step of the level (device ComputeDt, three ghost fills, three fused
+stage kernels), replayed from a hipGraph, with the state resident in HBM. Prints one JSON line.
+
+Workloads (--workload):
+  advected-interface-2d   256^2 periodic, two ideal gases (gamma 1.4/rho 1 in a circle r = 0.25,
+                          gamma 1.6/rho 10 outside), u = v = p = 1, WENO5 + HLLC + RK3, CFL 0.5
+Also reported: the dominant kernel's roofline position (measured with HIP events on the stream it
+runs on) and the oracle (CPU restatement, OpenMP) timed on this host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6      # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_HBM_GBS = 8000.0
+
+
+def workload(name):
+    from compas_amd import cases
+    if name == "advected-interface-2d":
+        c = cases.advected_interface_2d(n=256)
+        c["config"] = {"workload": name, "grid": [256, 256], "boxes": len(c["boxes"]),
+                       "scheme": "WENO5", "riemann": "HLLC", "integrator": "TVD-RK3", "cfl": 0.5,
+                       "nstate": 6, "bc": "periodic"}
+        c["default_steps"] = 100
+        return c
+    raise SystemExit("unknown workload %r" % name)
+
+
+def cpu_baseline(c, seconds, threads):
+    """The oracle (reference-structure CPU restatement) on the same initial state, timed on this
+    host: as many RK3 steps as fit in about `seconds` (at least one)."""
+    import numpy as np
+    import oracle
+    from compas_amd import Layout
+    oracle.set_num_threads(threads)
+    P, G = c["params"], c["geom"]
+    lay = Layout(P.dim, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    cells = int(np.prod(c["shape"]))
+    cmax = oracle.level_initial_cmax(P, G, lay, buf)
+    dt = oracle.compute_dt(G, cmax, c["cfl"], 1, 0.0)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        oracle.level_rk3_step(P, G, lay, buf, dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 200:
+            break
+    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": "%d RK3 steps of the %s workload (%d cells) in %.1f s" % (steps, c["name"], cells, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="advected-interface-2d")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (every box of a "
+              "level runs on one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from compas_amd import Layout, Level, roofline
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    c = workload(args.workload)
+    K = args.steps if args.steps is not None else c["default_steps"]
+    W = args.warmup
+    P, G = c["params"], c["geom"]
+    lay = Layout(P.dim, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    Un = torch.from_numpy(buf).to(dev)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    cmax = torch.zeros(1, dtype=torch.float64, device=dev)
+    lev = Level(P, G, lay)
+    stream = torch.cuda.Stream()
+    cells = int(np.prod(c["shape"]))
+
+    with torch.cuda.stream(stream):
+        lev.advance(Un, U1, U2, W, c["cfl"], dt, cmax, use_graph=True, resume=False, stream=stream)
+        stream.synchronize()
+        assert lev.check_errors(stream) == 0
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        lev.advance(Un, U1, U2, K, c["cfl"], dt, cmax, use_graph=True, resume=True, stream=stream)
+        ev1.record(stream)
+        stream.synchronize()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        err = lev.check_errors(stream)
+        assert err == 0, "device error word %d" % err
+        wall = t1 - t0
+        ev_ms = ev0.elapsed_time(ev1)
+
+        # dominant kernel: the fused stage kernel (stage 2 form: reads Us and Un, writes Uout)
+        lev.FillPatch(U1, stream=stream)
+        k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        k0.record(stream)
+        for _ in range(args.kernel_reps):
+            lev.rk3_stage(1, U1, Un, U2, dt, cmax, stream=stream)
+        k1.record(stream)
+        stream.synchronize()
+        stage_ms = k0.elapsed_time(k1) / args.kernel_reps
+        assert lev.check_errors(stream) == 0
+
+    flops_stage = roofline.flops_per_cell_stage(P) * cells
+    achieved = flops_stage / (stage_ms * 1e-3) / 1e12
+    bytes_update = roofline.bytes_per_cell_update(P)
+    rec = {
+        "metric": "cell-updates/s (5Eq WENO5-HLLC RK3)",
+        "value": cells * K / wall,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded initial state of the named case)",
+        "config": c["config"],
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_stage (fused WENO5 + HLLC + divergence + RK3 stage)",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel_ms": stage_ms,
+            "flops_per_cell_stage": roofline.flops_per_cell_stage(P),
+            "hbm_GBps_algorithmic": bytes_update / 3.0 * cells / (stage_ms * 1e-3) / 1e9,
+        },
+        "event_ms_per_step": ev_ms / K,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        rec["cpu_baseline"] = cpu_baseline(c, args.cpu_seconds, threads)
+    lev.close()
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

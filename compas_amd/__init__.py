@@ -1,0 +1,262 @@
+"""compas_amd - MI355X (gfx950) five-equation finite-volume RHS behind a C ABI.
+
+Python side of the library: parameter/geometry builders that map COMPAS's input keys onto the C
+structs, level-buffer packing, and a Level object whose methods carry the names of the
+reference's per-level operations (FillPatch, compute_dUdt_FV, ThirdOrderSSPRK, ComputeDt).
+All numerics run in libcompas_amd.so; nothing here computes a flux.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import cmp_box, cmp_geometry, cmp_params, check
+
+__all__ = ["make_params", "make_geometry", "split_domain", "Layout", "Level", "capi",
+           "SCHEMES", "BC"]
+
+# FiniteVolume.Scheme -> (ID_WENO, ID_Limiter, ID_Poly, Num_Grow)
+# (include/Operators_FV/FiniteVolume_Parameter.H:71-139)
+SCHEMES = {
+    "GODUNOV": (0, 0, 0, 1),
+    "MUSCL": (0, 1, 1, 2),
+    "MUSCL-Minmod": (0, 1, 1, 2),
+    "MUSCL-vanLeer": (0, 2, 1, 2),
+    "MUSCL-Superbee": (0, 3, 1, 2),
+    "MUSCL-MC": (0, 4, 1, 2),
+    "WENO5": (1, 5, 4, 3),
+    "WENO5B": (1, 6, 4, 3),
+}
+
+# run.lo_bc / run.hi_bc codes (source/Class_Compressible5Eq_PhaseField.cpp:169-183)
+BC = {"Interior": 0, "Inflow": 1, "Outflow": 2, "Symmetry": 3, "SlipWall": 4, "NoSlipWall": 5}
+
+RIEMANN = {"HLL": 1, "HLLC": 2}
+
+
+def eos_coefficients(eos, gamma, pinf=(0.0, 0.0), b=(0.0, 0.0), q=(0.0, 0.0)):
+    """A, B, C, D per phase as Parm::Initialize sets them (source/Parm.cpp:124-156).
+
+    eos: 0 ideal gas, 1 stiffened gas, 3 NASG (covolume b in C, heat of formation q in D)."""
+    A = [1.0 / (g - 1.0) for g in gamma]
+    if eos == 0:
+        return A, [0.0, 0.0], [0.0, 0.0], [0.0, 0.0]
+    Bc = [gamma[k] * pinf[k] / (gamma[k] - 1.0) for k in range(2)]
+    if eos == 1:
+        return A, Bc, [0.0, 0.0], [0.0, 0.0]
+    if eos == 3:
+        return A, Bc, list(b), list(q)
+    raise ValueError("EOS %r not supported on this path" % (eos,))
+
+
+def make_params(dim, gamma=(1.4, 1.4), pinf=(0.0, 0.0), eos=None, scheme="WENO5", riemann="HLLC",
+                source_term=0, b=(0.0, 0.0), q=(0.0, 0.0), thinc=0, thinc_use_normal=1,
+                thinc_eps=1.0e-4, thinc_beta=2.3, thinc_normal_threshold=0.25):
+    if eos is None:
+        eos = 1 if any(p != 0.0 for p in pinf) else 0
+    A, B, C, D = eos_coefficients(eos, gamma, pinf, b, q)
+    w, lim, poly, _ = SCHEMES[scheme]
+    p = cmp_params()
+    p.dim = dim
+    p.nstate = 4 + dim
+    for k in range(2):
+        p.eos_A[k], p.eos_B[k], p.eos_C[k], p.eos_D[k] = A[k], B[k], C[k], D[k]
+    p.id_weno, p.id_limiter, p.id_poly, p.id_bound = w, lim, poly, 0
+    p.id_thinc, p.thinc_use_normal = thinc, thinc_use_normal
+    p.thinc_eps, p.thinc_beta, p.thinc_normal_threshold = thinc_eps, thinc_beta, thinc_normal_threshold
+    p.source_term = source_term
+    p.id_riemann = RIEMANN[riemann]
+    return p
+
+
+def scheme_ghosts(scheme):
+    return SCHEMES[scheme][3]
+
+
+def make_geometry(n_cell, prob_lo, prob_hi, periodic, lo_bc=None, hi_bc=None):
+    dim = len(n_cell)
+    g = cmp_geometry()
+    g.dim = dim
+    for d in range(3):
+        if d < dim:
+            g.domain_lo[d] = 0
+            g.domain_hi[d] = n_cell[d] - 1
+            g.is_periodic[d] = int(periodic[d])
+            g.dx[d] = (prob_hi[d] - prob_lo[d]) / n_cell[d]
+            g.prob_lo[d] = prob_lo[d]
+            g.lo_bc[d] = 0 if periodic[d] else (lo_bc[d] if lo_bc else BC["Outflow"])
+            g.hi_bc[d] = 0 if periodic[d] else (hi_bc[d] if hi_bc else BC["Outflow"])
+        else:
+            g.domain_lo[d] = g.domain_hi[d] = 0
+            g.dx[d] = 1.0
+    return g
+
+
+def split_domain(n_cell, max_grid_size):
+    """Uniform split of the domain into boxes of at most max_grid_size cells a side
+    (AMReX's maxSize chopping of a single-level BoxArray)."""
+    dim = len(n_cell)
+    mg = max_grid_size if isinstance(max_grid_size, (list, tuple)) else [max_grid_size] * dim
+    edges = []
+    for d in range(dim):
+        nb = -(-n_cell[d] // mg[d])
+        base, extra = divmod(n_cell[d], nb)
+        cuts = [0]
+        for i in range(nb):
+            cuts.append(cuts[-1] + base + (1 if i < extra else 0))
+        edges.append(cuts)
+    boxes = []
+    ranges = [range(len(e) - 1) for e in edges]
+    import itertools
+    for idx in itertools.product(*reversed(ranges)):
+        idx = tuple(reversed(idx))
+        lo = [edges[d][idx[d]] for d in range(dim)]
+        hi = [edges[d][idx[d] + 1] - 1 for d in range(dim)]
+        boxes.append((lo, hi))
+    return boxes
+
+
+class Layout:
+    """Level-buffer layout: boxes with ng ghosts, ncomp components, fabs concatenated."""
+
+    def __init__(self, dim, boxes, ng, ncomp):
+        self.dim, self.ng, self.ncomp = dim, ng, ncomp
+        self.nbox = len(boxes)
+        self.cboxes = (cmp_box * self.nbox)()
+        for b, (lo, hi) in enumerate(boxes):
+            for d in range(3):
+                self.cboxes[b].lo[d] = lo[d] if d < dim else 0
+                self.cboxes[b].hi[d] = hi[d] if d < dim else 0
+        total = ctypes.c_longlong(0)
+        lib = capi.load()
+        check(lib.cmp_level_layout(dim, self.nbox, self.cboxes, ng, ncomp, ctypes.byref(total)),
+              "cmp_level_layout")
+        self.total = total.value
+        self.boxes = [(list(lo), list(hi)) for lo, hi in boxes]
+
+    def fab_shape(self, b):
+        lo, hi = self.boxes[b]
+        ext = [hi[d] - lo[d] + 1 + 2 * self.ng for d in range(self.dim)]
+        return (self.ncomp,) + tuple(reversed(ext))
+
+    def fab_view(self, buf, b):
+        """numpy view of box b's fab in a level buffer: [n, (k,) j, i]."""
+        off = self.cboxes[b].offset
+        shape = self.fab_shape(b)
+        size = int(np.prod(shape))
+        return buf[off:off + size].reshape(shape)
+
+    def pack(self, field, buf=None):
+        """Global valid field [n, (k,) j, i] -> level buffer (ghosts zero)."""
+        if buf is None:
+            buf = np.zeros(self.total, dtype=np.float64)
+        g = self.ng
+        for b, (lo, hi) in enumerate(self.boxes):
+            v = self.fab_view(buf, b)
+            sl_dst = (slice(None),) + tuple(slice(g, g + hi[d] - lo[d] + 1) for d in reversed(range(self.dim)))
+            sl_src = (slice(None),) + tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(self.dim)))
+            v[sl_dst] = field[sl_src]
+        return buf
+
+    def unpack(self, buf, shape):
+        """Level buffer -> global valid field [n, (k,) j, i]."""
+        out = np.empty((self.ncomp,) + tuple(shape), dtype=np.float64)
+        g = self.ng
+        for b, (lo, hi) in enumerate(self.boxes):
+            v = self.fab_view(buf, b)
+            sl_src = (slice(None),) + tuple(slice(g, g + hi[d] - lo[d] + 1) for d in reversed(range(self.dim)))
+            sl_dst = (slice(None),) + tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(self.dim)))
+            out[sl_dst] = v[sl_src]
+        return out
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Level:
+    """One level's boxes on the GPU. Buffers are torch CUDA tensors owned by the caller."""
+
+    def __init__(self, params, geom, layout):
+        self.lib = capi.load()
+        self.params, self.geom, self.layout = params, geom, layout
+        h = ctypes.c_void_p()
+        check(self.lib.cmp_level_create(ctypes.byref(params), ctypes.byref(geom), layout.nbox,
+                                        layout.cboxes, layout.ng, ctypes.byref(h)), "cmp_level_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.cmp_level_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # FillPatchSingleLevel
+    def FillPatch(self, U, stream=None):
+        check(self.lib.cmp_level_fill_ghosts(self.h, _ptr(U), _stream(stream)), "cmp_level_fill_ghosts")
+
+    # compute_dUdt_FV on every box (ghosts of U filled)
+    def compute_dUdt_FV(self, dUdt, U, cmax=None, stream=None):
+        check(self.lib.cmp_level_rhs(self.h, _ptr(U), _ptr(dUdt), _ptr(cmax), _stream(stream)),
+              "cmp_level_rhs")
+
+    def rk3_stage(self, stage, Us, Un, Uout, dt_dev, cmax=None, stream=None):
+        check(self.lib.cmp_level_rk3_stage(self.h, stage, _ptr(Us), _ptr(Un), _ptr(Uout), _ptr(dt_dev),
+                                           _ptr(cmax), _stream(stream)), "cmp_level_rk3_stage")
+
+    def ThirdOrderSSPRK(self, Un, U1, U2, dt_dev, cmax=None, stream=None):
+        check(self.lib.cmp_level_rk3_step(self.h, _ptr(Un), _ptr(U1), _ptr(U2), _ptr(dt_dev),
+                                          _ptr(cmax), _stream(stream)), "cmp_level_rk3_step")
+
+    def CalculateInitialcmax(self, U, cmax, stream=None):
+        check(self.lib.cmp_level_initial_cmax(self.h, _ptr(U), _ptr(cmax), _stream(stream)),
+              "cmp_level_initial_cmax")
+
+    def ComputeDt(self, cmax, dt_dev, cfl, limiter=1, stream=None):
+        check(self.lib.cmp_level_compute_dt(self.h, _ptr(cmax), _ptr(dt_dev), float(cfl), int(limiter),
+                                            _stream(stream)), "cmp_level_compute_dt")
+
+    def advance(self, Un, U1, U2, nsteps, cfl, dt_dev, cmax, dt_hist=None, limiter=1, use_graph=True,
+                resume=False, stream=None):
+        check(self.lib.cmp_level_advance(self.h, _ptr(Un), _ptr(U1), _ptr(U2), int(nsteps), float(cfl),
+                                         int(limiter), _ptr(dt_dev), _ptr(cmax), _ptr(dt_hist),
+                                         int(use_graph), int(bool(resume)), _stream(stream)),
+              "cmp_level_advance")
+
+    def check_errors(self, stream=None):
+        """Synchronises and returns the device error word (0 when clean)."""
+        return self.lib.cmp_level_check(self.h, _stream(stream))
+
+    def info(self):
+        n, x, y, z = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int())
+        check(self.lib.cmp_level_info(self.h, ctypes.byref(n), ctypes.byref(x), ctypes.byref(y),
+                                      ctypes.byref(z)), "cmp_level_info")
+        return {"ntiles": n.value, "tile": (x.value, y.value, z.value)}
+
+
+def rhs_box(params, geom, box, ng, U, dUdt, cmax=None, fluxes=(None, None, None),
+            fluxnc=(None, None, None), stream=None):
+    """Per-box compute_dUdt_FV through cmp_rhs_box (torch CUDA tensors)."""
+    lib = capi.load()
+    b = cmp_box()
+    lo, hi = box
+    for d in range(3):
+        b.lo[d] = lo[d] if d < len(lo) else 0
+        b.hi[d] = hi[d] if d < len(hi) else 0
+    check(lib.cmp_rhs_box(ctypes.byref(params), ctypes.byref(geom), ctypes.byref(b), ng, _ptr(U),
+                          _ptr(dUdt), _ptr(cmax), _ptr(fluxes[0]), _ptr(fluxes[1]), _ptr(fluxes[2]),
+                          _ptr(fluxnc[0]), _ptr(fluxnc[1]), _ptr(fluxnc[2]), _stream(stream)),
+          "cmp_rhs_box")

@@ -1,0 +1,132 @@
+"""ctypes mirror of include/compas_amd.h.
+
+The shared library is the product: this module only declares its structs and entry points.
+Loading fails loudly when libcompas_amd.so has not been built; there is no fallback path.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcompas_amd.so")
+
+CMP_OK = 0
+CMP_ERR_ARG = 1
+CMP_ERR_UNSUPPORTED = 2
+CMP_ERR_HIP = 3
+CMP_ERR_NAN = 4
+CMP_ERR_COVOLUME = 5
+CMP_ERR_BC = 6
+CMP_ERR_LAYOUT = 7
+
+ERROR_NAMES = {
+    CMP_OK: "CMP_OK",
+    CMP_ERR_ARG: "CMP_ERR_ARG",
+    CMP_ERR_UNSUPPORTED: "CMP_ERR_UNSUPPORTED",
+    CMP_ERR_HIP: "CMP_ERR_HIP",
+    CMP_ERR_NAN: "CMP_ERR_NAN",
+    CMP_ERR_COVOLUME: "CMP_ERR_COVOLUME",
+    CMP_ERR_BC: "CMP_ERR_BC",
+    CMP_ERR_LAYOUT: "CMP_ERR_LAYOUT",
+}
+
+
+class CmpError(RuntimeError):
+    def __init__(self, code, where=""):
+        self.code = code
+        super().__init__("%s failed: %s (%d)" % (where, ERROR_NAMES.get(code, "?"), code))
+
+
+class cmp_params(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("nstate", ctypes.c_int),
+        ("eos_A", ctypes.c_double * 2),
+        ("eos_B", ctypes.c_double * 2),
+        ("eos_C", ctypes.c_double * 2),
+        ("eos_D", ctypes.c_double * 2),
+        ("id_weno", ctypes.c_int),
+        ("id_limiter", ctypes.c_int),
+        ("id_poly", ctypes.c_int),
+        ("id_bound", ctypes.c_int),
+        ("id_thinc", ctypes.c_int),
+        ("thinc_use_normal", ctypes.c_int),
+        ("thinc_eps", ctypes.c_double),
+        ("thinc_beta", ctypes.c_double),
+        ("thinc_normal_threshold", ctypes.c_double),
+        ("source_term", ctypes.c_int),
+        ("id_riemann", ctypes.c_int),
+    ]
+
+
+class cmp_geometry(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int),
+        ("domain_lo", ctypes.c_int * 3),
+        ("domain_hi", ctypes.c_int * 3),
+        ("is_periodic", ctypes.c_int * 3),
+        ("lo_bc", ctypes.c_int * 3),
+        ("hi_bc", ctypes.c_int * 3),
+        ("dx", ctypes.c_double * 3),
+        ("prob_lo", ctypes.c_double * 3),
+    ]
+
+
+class cmp_box(ctypes.Structure):
+    _fields_ = [
+        ("lo", ctypes.c_int * 3),
+        ("hi", ctypes.c_int * 3),
+        ("offset", ctypes.c_longlong),
+    ]
+
+
+P = ctypes.POINTER
+vp = ctypes.c_void_p
+dp = ctypes.c_void_p   # device pointers travel as integers (torch data_ptr())
+
+# name -> (restype, argtypes); exactly the entry points include/compas_amd.h declares
+SIGNATURES = {
+    "cmp_version": (ctypes.c_char_p, []),
+    "cmp_abi_sizes": (ctypes.c_int, [P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]),
+    "cmp_fab_size": (ctypes.c_longlong, [ctypes.c_int, P(cmp_box), ctypes.c_int, ctypes.c_int]),
+    "cmp_level_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(cmp_box), ctypes.c_int,
+                                        ctypes.c_int, P(ctypes.c_longlong)]),
+    "cmp_rhs_box": (ctypes.c_int, [P(cmp_params), P(cmp_geometry), P(cmp_box), ctypes.c_int,
+                                   dp, dp, dp, dp, dp, dp, dp, dp, dp, vp]),
+    "cmp_level_create": (ctypes.c_int, [P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box),
+                                        ctypes.c_int, P(vp)]),
+    "cmp_level_destroy": (ctypes.c_int, [vp]),
+    "cmp_level_fill_ghosts": (ctypes.c_int, [vp, dp, vp]),
+    "cmp_level_rhs": (ctypes.c_int, [vp, dp, dp, dp, vp]),
+    "cmp_level_rk3_stage": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp, dp, dp, vp]),
+    "cmp_level_rk3_step": (ctypes.c_int, [vp, dp, dp, dp, dp, dp, vp]),
+    "cmp_level_initial_cmax": (ctypes.c_int, [vp, dp, dp, vp]),
+    "cmp_level_compute_dt": (ctypes.c_int, [vp, dp, dp, ctypes.c_double, ctypes.c_int, vp]),
+    "cmp_level_advance": (ctypes.c_int, [vp, dp, dp, dp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                         dp, dp, dp, ctypes.c_int, ctypes.c_int, vp]),
+    "cmp_level_check": (ctypes.c_int, [vp, vp]),
+    "cmp_level_info": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                      P(ctypes.c_int)]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libcompas_amd.so and declares its prototypes. Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libcompas_amd.so is not built (%s); run python -m compas_amd.build" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code, where):
+    if code != CMP_OK:
+        raise CmpError(code, where)

@@ -1,0 +1,147 @@
+"""Initial states of the benchmark and test cases (synthetic, seeded, FP64).
+
+Cell-centred evaluation (the reference's run.QuadratureIC = 0, include/InitData.H:66-98). The
+primitive->conservative map used here is the EOS of Parm::Initialize (source/Parm.cpp:124-156),
+rho e = A p + B + rho D with A, B mixed by volume fraction; CPU and GPU start from the same arrays.
+
+Conventions: the state's a1 component is the volume fraction of phase 1; in the advected-interface
+case phase 1 (gamma 1.4, rho 1) is inside the circle and phase 2 (gamma 1.6, rho 10) outside; in the
+water-air case phase 1 is water (the droplet) and phase 2 air.
+"""
+import math
+
+import numpy as np
+
+from . import make_geometry, make_params, split_domain
+
+
+def prim_to_cons(params, W):
+    """W = [a1r1, a2r2, u.., p, a1] (leading axis) -> conservative state (same shape)."""
+    dim = params.dim
+    a1r1, a2r2 = W[0], W[1]
+    vel = [W[2 + m] for m in range(dim)]
+    p, a1 = W[2 + dim], W[3 + dim]
+    a2 = 1.0 - a1
+    A, B, C, D = params.eos_A, params.eos_B, params.eos_C, params.eos_D
+    phi1 = np.where(C[0] > 0.0, a1 - C[0] * a1r1, a1)
+    phi2 = np.where(C[1] > 0.0, a2 - C[1] * a2r2, a2)
+    AP = phi1 * A[0] + phi2 * A[1]
+    BP = phi1 * B[0] + phi2 * B[1]
+    rho = a1r1 + a2r2
+    rho_e = AP * p + BP + (a1r1 * D[0] + a2r2 * D[1])
+    ke = sum(v * v for v in vel)
+    U = np.empty_like(W)
+    U[0], U[1] = a1r1, a2r2
+    for m in range(dim):
+        U[2 + m] = rho * vel[m]
+    U[2 + dim] = rho_e + 0.5 * rho * ke
+    U[3 + dim] = a1
+    return U
+
+
+def _centres(geom, shape):
+    dim = geom.dim
+    axes = []
+    for d in range(dim):
+        n = shape[dim - 1 - d]
+        axes.append(geom.prob_lo[d] + (np.arange(n) + 0.5) * geom.dx[d])
+    grids = np.meshgrid(*reversed(axes), indexing="ij")   # [(k,) j, i] ordering
+    return list(reversed(grids))                          # x, y, (z)
+
+
+def advected_interface_2d(n=256, max_grid_size=None, scheme="WENO5", riemann="HLLC"):
+    """2D two-phase advected interface: [0,1]^2 periodic, u = v = 1, p = 1, circle r = 0.25 at
+    (0.5, 0.5), alpha = 0.5 (1 + tanh((0.25 - r)/(2 dx))) clipped to [1e-8, 1 - 1e-8]."""
+    params = make_params(2, gamma=(1.4, 1.6), scheme=scheme, riemann=riemann, source_term=0)
+    geom = make_geometry((n, n), (0.0, 0.0), (1.0, 1.0), (1, 1))
+    x, y = _centres(geom, (n, n))
+    r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2)
+    a1 = np.clip(0.5 * (1.0 + np.tanh((0.25 - r) / (2.0 * geom.dx[0]))), 1e-8, 1.0 - 1e-8)
+    W = np.empty((6, n, n))
+    W[0] = a1 * 1.0
+    W[1] = (1.0 - a1) * 10.0
+    W[2] = 1.0
+    W[3] = 1.0
+    W[4] = 1.0
+    W[5] = a1
+    U = prim_to_cons(params, W)
+    boxes = split_domain((n, n), max_grid_size or n)
+    return dict(params=params, geom=geom, U=U, boxes=boxes, shape=(n, n), cfl=0.5,
+                name="advected-interface-2d")
+
+
+def rankine_hugoniot(gamma, p1, rho1, mach):
+    """Post-shock (p2, rho2, u2) of a shock moving at Mach `mach` into gas at rest
+    (ideal gas; the relations of include/PhysicsFunctions.H:15-34)."""
+    g = gamma
+    M2 = mach * mach
+    p2 = p1 * (1.0 + 2.0 * g / (g + 1.0) * (M2 - 1.0))
+    rho2 = rho1 * (g + 1.0) * M2 / ((g - 1.0) * M2 + 2.0)
+    c1 = math.sqrt(g * p1 / rho1)
+    u2 = mach * c1 * (1.0 - rho1 / rho2)
+    return p2, rho2, u2
+
+
+def water_air_droplet_3d(n=256, box=32, scheme="WENO5", riemann="HLLC"):
+    """3D water-air shock-droplet: unit cube, outflow BCs, water (stiffened gas gamma 4.4,
+    p_inf 6e8 Pa, rho 1000) droplet r = 0.1 at the centre, air (gamma 1.4, rho 1.2), p = 1e5 Pa,
+    Mach 2.4 shock in air at x = 0.25 moving in +x. Interface: tanh profile of width dx."""
+    params = make_params(3, gamma=(4.4, 1.4), pinf=(6.0e8, 0.0), scheme=scheme, riemann=riemann,
+                         source_term=0)
+    geom = make_geometry((n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (0, 0, 0))
+    x, y, z = _centres(geom, (n, n, n))
+    r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2)
+    a1 = np.clip(0.5 * (1.0 + np.tanh((0.1 - r) / (2.0 * geom.dx[0]))), 1e-8, 1.0 - 1e-8)
+    p0, rho_w, rho_a = 1.0e5, 1000.0, 1.2
+    p2, rho2, u2 = rankine_hugoniot(1.4, p0, rho_a, 2.4)
+    post = x < 0.25
+    W = np.empty((7, n, n, n))
+    W[0] = a1 * rho_w
+    W[1] = (1.0 - a1) * np.where(post, rho2, rho_a)
+    W[2] = np.where(post, u2, 0.0)
+    W[3] = 0.0
+    W[4] = 0.0
+    W[5] = np.where(post, p2, p0)
+    W[6] = a1
+    U = prim_to_cons(params, W)
+    boxes = split_domain((n, n, n), box)
+    return dict(params=params, geom=geom, U=U, boxes=boxes, shape=(n, n, n), cfl=0.4,
+                name="water-air-droplet-3d")
+
+
+def random_smooth(dim, n, seed, periodic=True, max_grid_size=None, scheme="WENO5", riemann="HLLC",
+                  gamma=(1.4, 1.6), pinf=(0.0, 0.0), source_term=0, amp=0.1):
+    """Seeded smooth two-phase state for parity tests: a few random Fourier modes on top of a
+    uniform state, with a tanh interface across the middle."""
+    rng = np.random.default_rng(seed)
+    params = make_params(dim, gamma=gamma, pinf=pinf, scheme=scheme, riemann=riemann,
+                         source_term=source_term)
+    shape = (n,) * dim
+    geom = make_geometry(shape, (0.0,) * dim, (1.0,) * dim, (int(periodic),) * dim)
+    X = _centres(geom, tuple(reversed(shape)))
+
+    def modes():
+        f = np.zeros_like(X[0])
+        for _ in range(3):
+            k = rng.integers(1, 3, size=dim)
+            ph = rng.uniform(0, 2 * np.pi, size=dim)
+            term = np.ones_like(X[0])
+            for d in range(dim):
+                term = term * np.sin(2 * np.pi * k[d] * X[d] + ph[d])
+            f = f + rng.uniform(-1, 1) * term
+        return f / 3.0
+
+    xi = X[0] - 0.5 + 0.05 * modes()
+    a1 = np.clip(0.5 * (1.0 + np.tanh(np.sin(2 * np.pi * xi) / (4.0 * geom.dx[0]))), 1e-8, 1 - 1e-8)
+    pscale = 1.0 + max(pinf)
+    W = np.empty((4 + dim,) + tuple(reversed(shape)))
+    W[0] = a1 * (1.0 + amp * modes())
+    W[1] = (1.0 - a1) * (5.0 + amp * 5.0 * modes())
+    for m in range(dim):
+        W[2 + m] = 0.5 + amp * modes()
+    W[2 + dim] = pscale * (1.0 + amp * modes())
+    W[3 + dim] = a1
+    U = prim_to_cons(params, W)
+    boxes = split_domain(shape, max_grid_size or n)
+    return dict(params=params, geom=geom, U=U, boxes=boxes, shape=tuple(reversed(shape)), cfl=0.4,
+                name="random-smooth-%dd" % dim)

@@ -1,0 +1,179 @@
+/*
+ * compas_amd.h - C ABI of the MI355X (gfx950) five-equation finite-volume right-hand side.
+ *
+ * Plain C: POD structs, raw device pointers, sizes and an opaque stream (hipStream_t passed as
+ * void*). Every entry point returns an int status (CMP_OK = 0) and never aborts the process.
+ *
+ * Memory layout (same as AMReX Array4, which the reference's kernels index): a "fab" of a box
+ * with ng ghost cells stores (i,j,k,n) with i fastest, then j, k, then component n; the index
+ * range is [lo-ng, hi+ng] in each active dimension and a single cell in inactive ones. A level
+ * buffer concatenates the fabs of its boxes; cmp_box.offset is the element offset of a box's fab.
+ * Face arrays (optional outputs) are nodal in their direction and carry no ghosts.
+ *
+ * State vector (PHYSICS == FIVEEQS, include/Physics_Compressible5Eq/Variables_Compressible5Eq.H:21-30):
+ *   n = 0 a1rho1, 1 a2rho2, 2..1+dim rho*u_d, 2+dim rho*E, 3+dim a1      (nstate = 4+dim)
+ */
+#ifndef COMPAS_AMD_H
+#define COMPAS_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------------- */
+#define CMP_OK               0
+#define CMP_ERR_ARG          1  /* bad argument (null pointer, size, dim, ghost width)                  */
+#define CMP_ERR_UNSUPPORTED  2  /* scheme combination the reference aborts on, or not built here        */
+#define CMP_ERR_HIP          3  /* a HIP runtime call failed                                            */
+#define CMP_ERR_NAN          4  /* a NaN appeared in the state or the wave speed                        */
+#define CMP_ERR_COVOLUME     5  /* NASG covolume bound 1 - b rho <= 0 (EquationOfState_NASG.H:28-32)   */
+#define CMP_ERR_BC           6  /* boundary type without a fill rule here (Inflow/ext_dir)              */
+#define CMP_ERR_LAYOUT       7  /* boxes overlap, leave the domain, or do not cover it                  */
+
+/* Scheme IDs follow FiniteVolume_Parameter::Initialize (include/Operators_FV/FiniteVolume_Parameter.H:71-139):
+ *   GODUNOV  id_weno=0 id_limiter=0 id_poly=0  ng>=1
+ *   MUSCL-*  id_weno=0 id_limiter=1..4 (minmod, vanLeer, superbee, MC) id_poly=1  ng>=2
+ *   WENO5    id_weno=1 id_limiter=5 id_poly=4  ng>=3
+ *   WENO5B   id_weno=1 id_limiter=6 id_poly=4  ng>=3
+ * Riemann IDs follow Physics_Parameter (include/Physics_Compressible5Eq/Physics_Parameter_Compressible5Eq.H:16-27):
+ *   1 HLL, 2 HLLC.
+ * Boundary codes follow run.lo_bc/run.hi_bc (source/Class_Compressible5Eq_PhaseField.cpp:169-183):
+ *   0 Interior, 1 Inflow, 2 Outflow, 3 Symmetry, 4 SlipWall, 5 NoSlipWall. */
+
+/* Parameters read by every kernel: the subset of the reference's Parm that this path uses. */
+typedef struct cmp_params {
+    int    dim;            /* AMREX_SPACEDIM, 2 or 3                                                 */
+    int    nstate;         /* 4 + dim                                                                */
+    /* EOS coefficients of phase 1 and 2 as set by Parm::Initialize (source/Parm.cpp:124-156):
+     *   rho_k e_k = (1 - b_k rho_k)(A_k p + B_k) + rho_k D_k, b_k = C_k (NASG covolume, 0 otherwise) */
+    double eos_A[2];
+    double eos_B[2];
+    double eos_C[2];
+    double eos_D[2];
+    int    id_weno;
+    int    id_limiter;
+    int    id_poly;
+    int    id_bound;       /* 1 (bound-preserving WENO5BP/MUSCLBP) is not built here                 */
+    int    id_thinc;       /* THINC interface sharpening (FiniteVolume.THINC)                         */
+    int    thinc_use_normal;
+    double thinc_eps;
+    double thinc_beta;
+    double thinc_normal_threshold;
+    int    source_term;    /* Physics.source_term: ID_K, 0 or 1                                      */
+    int    id_riemann;     /* 1 HLL, 2 HLLC                                                           */
+} cmp_params;
+
+typedef struct cmp_geometry {
+    int    dim;
+    int    domain_lo[3];
+    int    domain_hi[3];
+    int    is_periodic[3];
+    int    lo_bc[3];
+    int    hi_bc[3];
+    double dx[3];
+    double prob_lo[3];
+} cmp_geometry;
+
+typedef struct cmp_box {
+    int       lo[3];
+    int       hi[3];
+    long long offset;      /* element offset of this box's fab inside the level buffer              */
+} cmp_box;
+
+/* ---- layout helpers (host only, no GPU) ------------------------------------------------------- */
+
+/* Elements of one fab: prod_d (hi_d - lo_d + 1 + 2 ng) over active dims, times ncomp. */
+long long cmp_fab_size(int dim, const cmp_box* box, int ng, int ncomp);
+
+/* Assigns consecutive offsets to the boxes of a level; *total receives the level buffer length. */
+int cmp_level_layout(int dim, int nbox, cmp_box* boxes, int ng, int ncomp, long long* total);
+
+/* sizeof(cmp_params), sizeof(cmp_geometry), sizeof(cmp_box): lets bindings check their mirrors. */
+int cmp_abi_sizes(int* params_size, int* geometry_size, int* box_size);
+
+/* Library version string. */
+const char* cmp_version(void);
+
+/* ---- one box ---------------------------------------------------------------------------------- */
+
+/* Right-hand side of one box from its ghost-filled state, i.e. COMPAS's per-box
+ * ComputeFaceFluxes + FluxDivergence (source/Class_Compressible_PhaseField.cpp:3152-3391, called by
+ * compute_dUdt_FV :3473-3537, per face FVM_Conservative2FluxHyperbolic_K,
+ * include/Operators_Compressible5Eq/FVM_Conservative2FluxHyperbolic_K_Compressible5Eq.H:15-170).
+ *   U        device, fab of `box` with ng ghosts, nstate comps (ghosts already filled)
+ *   dUdt     device, valid box (no ghosts), nstate comps; overwritten
+ *   cmax     device scalar or NULL: max-accumulated with the wave speed of every face that is the
+ *            low face of a valid cell (the reference's c_max(i,j,k) = max over faces (i,j,k),
+ *            reduced over valid cells, :3212 and :4490)
+ *   flux_d   device or NULL: nodal face fluxes of direction d, nstate comps (the `fluxes` MultiFab)
+ *   fluxnc_d device or NULL: nodal face normal velocity u*_n, 1 comp (the `fluxes_nc` MultiFab)
+ *   stream   hipStream_t or NULL for the default stream */
+int cmp_rhs_box(const cmp_params* params, const cmp_geometry* geom, const cmp_box* box, int ng,
+                const double* U, double* dUdt, double* cmax,
+                double* flux_x, double* flux_y, double* flux_z,
+                double* fluxnc_x, double* fluxnc_y, double* fluxnc_z,
+                void* stream);
+
+/* ---- one level (all boxes of a level advance together) ---------------------------------------- */
+
+typedef struct cmp_level cmp_level;
+
+/* Builds the device work-lists of a level: tile list, ghost-fill map, box lookup. The boxes must
+ * carry offsets from cmp_level_layout (nstate comps, ng ghosts). Allocates device memory for these
+ * tables only; state buffers stay owned by the caller. */
+int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbox,
+                     const cmp_box* boxes, int ng, cmp_level** out);
+int cmp_level_destroy(cmp_level* level);
+
+/* Same-level ghost fill: neighbour boxes, periodic images, then the physical BCs
+ * (FillPatchSingleLevel, source/COMPAS.cpp:1647-1678; BC map source/Class_Compressible5Eq_PhaseField.cpp:162-244). */
+int cmp_level_fill_ghosts(cmp_level* level, double* U, void* stream);
+
+/* Right-hand side of every box of the level (ghosts of U already filled); dUdt is a level buffer
+ * with the same layout as U (its ghost cells are not written). */
+int cmp_level_rhs(cmp_level* level, const double* U, double* dUdt, double* cmax, void* stream);
+
+/* One TVD-RK3 stage (ThirdOrderSSPRK, source/Class_Compressible_PhaseField.cpp:3938-4107), fused:
+ *   stage 0: Uout = Us + dt L(Us)
+ *   stage 1: Uout = (1/4 Us + 3/4 Un) + (dt/4) L(Us)
+ *   stage 2: Uout = (2/3 Us + 1/3 Un) + (2 dt/3) L(Us)
+ * Us must have its ghosts filled; dt is read from device memory (*dt_dev); cmax (device) is
+ * max-accumulated. Uout may alias Un (each cell reads Un before it writes Uout) but not Us. */
+int cmp_level_rk3_stage(cmp_level* level, int stage, const double* Us, const double* Un,
+                        double* Uout, const double* dt_dev, double* cmax, void* stream);
+
+/* One full RK3 step in place on Un with two scratch level buffers (ghost fills included). */
+int cmp_level_rk3_step(cmp_level* level, double* Un, double* U1, double* U2,
+                       const double* dt_dev, double* cmax, void* stream);
+
+/* Wave speed of the initial state (CalculateInitialcmax, :4502-4605): fills ghosts of U, sets
+ * *cmax to the max face wave speed over low faces of valid cells. */
+int cmp_level_initial_cmax(cmp_level* level, double* U, double* cmax, void* stream);
+
+/* Device-side ComputeDt (source/COMPAS.cpp:2250-2302, EstTimeStep :4473-4499) for one level:
+ *   est = min_d cfl*dx_d/(*cmax);  dt = (limiter && *dt_dev > 0) ? min(est, 1.1 * *dt_dev) : est
+ * writes *dt_dev and resets *cmax to 0 for the next step's stages. */
+int cmp_level_compute_dt(cmp_level* level, double* cmax, double* dt_dev, double cfl,
+                         int limiter, void* stream);
+
+/* nsteps time steps: per step ComputeDt then one RK3 step. With resume == 0 the run starts at
+ * t = 0: step 0 takes its wave speed from the initial state (CalculateInitialcmax) and no
+ * previous dt limits it; with resume != 0 it continues from *cmax and *dt_dev of an earlier call
+ * on the same buffers. dt_hist (device, nsteps doubles, or NULL) receives the dt of each step of
+ * this call. With use_graph != 0 one step is captured into a hipGraph once and replayed. */
+int cmp_level_advance(cmp_level* level, double* Un, double* U1, double* U2, int nsteps,
+                      double cfl, int limiter, double* dt_dev, double* cmax, double* dt_hist,
+                      int use_graph, int resume, void* stream);
+
+/* Reads and clears the level's device error word (CMP_ERR_NAN, CMP_ERR_COVOLUME or CMP_OK).
+ * Synchronises the stream. */
+int cmp_level_check(cmp_level* level, void* stream);
+
+/* Number of tiles in the level work-list and the tile shape used (for reporting). */
+int cmp_level_info(const cmp_level* level, int* ntiles, int* tile_x, int* tile_y, int* tile_z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COMPAS_AMD_H */

@@ -1,0 +1,31 @@
+"""Builds the oracle (CPU restatement, test infrastructure) as oracle/_build/libcpu_ref.so.
+
+gcc, C99, -O2, no FMA contraction (-ffp-contract=off), no fast-math, OpenMP for the timed CPU
+baseline. The reference itself (COMPAS) cannot be built here: it needs AMReX 26.01, which is not
+vendored and not installed, so there is no oracle/_ref build.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(BUILD, "libcpu_ref.so")
+
+CFLAGS = ["-O2", "-std=c99", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+          "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = LIB + ".tmp"
+    cmd = ["gcc"] + CFLAGS + ["-shared", "-o", tmp, os.path.join(HERE, "cpu_ref.c"), "-lm"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,800 @@
+/*
+ * cpu_ref.c - CPU restatement (the oracle) of COMPAS's five-equation hyperbolic RHS, FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg, never by the product library. It follows the reference's structure on purpose (per face:
+ * cons->prim of every stencil cell, reconstruction, prim->cons, Riemann solver with its own
+ * cons->prim; stored face fluxes; a separate divergence pass; separate RK LinComb/Saxpy passes),
+ * so that it is both the checker and a faithful CPU baseline of the reference's algorithm.
+ *
+ * Operation order follows the cited reference lines; build with -ffp-contract=off (no FMA),
+ * like x86-64 g++ builds the reference. std::min/std::max semantics (not fmin/fmax) are kept.
+ *
+ * Parity pinning of each function (what checks it independently) is listed in tests/ and
+ * DESIGN.md; every function here has at least one such check.
+ */
+#include "cpu_ref.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* std::min(a,b) = (b < a) ? b : a ; std::max(a,b) = (a < b) ? b : a  */
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+
+/* Sign (include/Tools/Sign.H:13-25) */
+static inline double ref_sign(double x) { return (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0); }
+
+#define ERR_SET(err, code) do { if ((err) && *(err) == 0) *(err) = (code); } while (0)
+
+/* ------------------------------------------------------------------------------------------------
+ * EOS and thermodynamics
+ * ---------------------------------------------------------------------------------------------- */
+
+/* NASG_Phi (include/EquationOfState_NASG.H:34-41): phi = alpha - b alpha rho for b > 0, else alpha;
+ * the covolume check aborts in the reference, here it sets CMP_ERR_COVOLUME. */
+double ref_nasg_phi(double alpha, double alpha_rho, double b, int* err)
+{
+    if (b > 0.0 && alpha > 0.0) {
+        double xi = 1.0 - b * (alpha_rho / alpha);
+        if (!(xi > 0.0)) ERR_SET(err, CMP_ERR_COVOLUME);
+    }
+    return (b > 0.0) ? alpha - b * alpha_rho : alpha;
+}
+
+/* EquationOfState_Mixture, pressure-based branch
+ * (include/Physics_Compressible5Eq/EquationOfState_Mixture_Compressible5Eq.H:15-54). */
+double ref_eos_mixture(const cmp_params* P, double input, double a1, double a1r1, double a2r2,
+                       int output_internal_energy, int* err)
+{
+    double a2 = 1.0 - a1;
+    double phi1 = ref_nasg_phi(a1, a1r1, P->eos_C[0], err);
+    double phi2 = ref_nasg_phi(a2, a2r2, P->eos_C[1], err);
+    double rho_D = a1r1 * P->eos_D[0] + a2r2 * P->eos_D[1];
+    double AP = phi1 * P->eos_A[0] + phi2 * P->eos_A[1];
+    double BP = phi1 * P->eos_B[0] + phi2 * P->eos_B[1];
+    if (output_internal_energy)
+        return AP * input + BP + rho_D;
+    return (input - rho_D - BP) / AP;
+}
+
+/* Conservative2Primitive (include/Physics_Compressible5Eq/Conservative2Primitive_Compressible5Eq.H:15-61) */
+void ref_cons2prim(const cmp_params* P, const double* U, double* W, int* err)
+{
+    const int d = P->dim;
+    double a1r1 = U[0], a2r2 = U[1];
+    double rE = U[2 + d], a1 = U[3 + d];
+    double rho = a1r1 + a2r2;
+    double u = U[2] / rho;
+    double v = U[3] / rho;
+    double ke;
+    double w = 0.0;
+    if (d == 3) {
+        w = U[4] / rho;
+        ke = u * u + v * v + w * w;
+    } else {
+        ke = u * u + v * v;
+    }
+    double rho_e = rE - (0.5) * rho * ke;
+    double p = ref_eos_mixture(P, rho_e, a1, a1r1, a2r2, 0, err);
+    W[0] = a1r1; W[1] = a2r2; W[2] = u; W[3] = v;
+    if (d == 3) W[4] = w;
+    W[2 + d] = p; W[3 + d] = a1;
+}
+
+/* Primitive2Conservative (include/Physics_Compressible5Eq/Primitive2Conservative_Compressible5Eq.H:15-62) */
+void ref_prim2cons(const cmp_params* P, const double* W, double* U, int* err)
+{
+    const int d = P->dim;
+    double a1r1 = W[0], a2r2 = W[1];
+    double u = W[2], v = W[3], w = (d == 3) ? W[4] : 0.0;
+    double p = W[2 + d], a1 = W[3 + d];
+    double rho = a1r1 + a2r2;
+    double ru = rho * u, rv = rho * v, rw = rho * w;
+    double rho_e = ref_eos_mixture(P, p, a1, a1r1, a2r2, 1, err);
+    double ke = (d == 3) ? (u * u + v * v + w * w) : (u * u + v * v);
+    double rE = rho_e + (0.5) * rho * ke;
+    U[0] = a1r1; U[1] = a2r2; U[2] = ru; U[3] = rv;
+    if (d == 3) U[4] = rw;
+    U[2 + d] = rE; U[3 + d] = a1;
+}
+
+/* rho_c2_Phase (include/Physics_Compressible5Eq/rho_c2_Phase_Compressible5Eq.H:17-23) */
+double ref_rho_c2_phase(double p, double A, double B) { return ((A + 1.0) * p + B) / A; }
+
+/* rho_c2_Mixture (include/Physics_Compressible5Eq/rho_c2_Mixture_Compressible5Eq.H:15-49) */
+static double ref_rho_c2_mixture(const cmp_params* P, double r1c1, double r2c2, double a1,
+                                 double a1r1, double a2r2, int id_k, int* err)
+{
+    double a2 = 1.0 - a1;
+    double phi1 = ref_nasg_phi(a1, a1r1, P->eos_C[0], err);
+    double phi2 = ref_nasg_phi(a2, a2r2, P->eos_C[1], err);
+    if (id_k == 0) {
+        double AP_rho_c2 = a1 * P->eos_A[0] * r1c1 + a2 * P->eos_A[1] * r2c2;
+        double AP = phi1 * P->eos_A[0] + phi2 * P->eos_A[1];
+        return AP_rho_c2 / AP;
+    }
+    return 1.0 / (phi1 / r1c1 + phi2 / r2c2);
+}
+
+/* SoundSpeed (include/Physics_Compressible5Eq/SoundSpeed_Compressible5Eq.H:15-54) */
+double ref_sound_speed(const cmp_params* P, double p, double a1, double a1r1, double a2r2, int id_k,
+                       int* err)
+{
+    double r1c1 = ref_rho_c2_phase(p, P->eos_A[0], P->eos_B[0]);
+    double r2c2 = ref_rho_c2_phase(p, P->eos_A[1], P->eos_B[1]);
+    double rho_c2 = ref_rho_c2_mixture(P, r1c1, r2c2, a1, a1r1, a2r2, id_k, err);
+    double rho = a1r1 + a2r2;
+    return sqrt(rho_c2 / rho);
+}
+
+/* NonConservative (include/Physics_Compressible5Eq/NonConservative_Compressible5Eq.H:15-58) */
+void ref_nonconservative_K(const cmp_params* P, double p, double a1, double a1r1, double a2r2,
+                           int id_k, double* K, int* err)
+{
+    if (id_k == 0) { K[0] = 0.0; K[1] = 0.0; return; }
+    double r1c1 = ref_rho_c2_phase(p, P->eos_A[0], P->eos_B[0]);
+    double r2c2 = ref_rho_c2_phase(p, P->eos_A[1], P->eos_B[1]);
+    double rho_c2 = ref_rho_c2_mixture(P, r1c1, r2c2, a1, a1r1, a2r2, id_k, err);
+    K[0] = (rho_c2 / r1c1 - 1.0);
+    K[1] = (rho_c2 / r2c2 - 1.0);
+}
+
+/* Conservative2Flux (include/Physics_Compressible5Eq/Conservative2Flux_Compressible5Eq.H:15-80) */
+void ref_cons2flux(const cmp_params* P, const double* U, int dir, double* F, int* err)
+{
+    const int d = P->dim;
+    double W[REF_NMAX];
+    ref_cons2prim(P, U, W, err);
+    double p = W[2 + d];
+    double V = W[2 + dir];
+    F[0] = W[0] * V;
+    F[1] = W[1] * V;
+    for (int m = 0; m < d; ++m) F[2 + m] = U[2 + m] * V;
+    F[2 + d] = (U[2 + d] + p) * V;
+    F[3 + d] = W[3 + d] * V;
+    F[2 + dir] += p;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Scalar reconstruction
+ * ---------------------------------------------------------------------------------------------- */
+
+/* WENO5 candidate values and ratio-form nonlinear weights (include/Operators_FV/WENO5.H:12-75):
+ * p = 2, no epsilon, a NaN ratio (0/0) replaced by 1. id_face = +1 gives the value at the high
+ * face of cell c, -1 at its low face. */
+void ref_weno5_weights(double f_mm, double f_m, double f_c, double f_p, double f_pp, int id_face,
+                       double* f, double* W)
+{
+    double d0 = 0.0, d1, d2 = 0.0;
+    if (id_face == 1) {
+        f[0] = f_c + (2.0 / 3.0) * (f_p - f_c) - (1.0 / 6.0) * (f_pp - f_p);
+        f[1] = f_c + (1.0 / 6.0) * (f_c - f_m) + (1.0 / 3.0) * (f_p - f_c);
+        f[2] = f_c - (1.0 / 3.0) * (f_m - f_mm) + (5.0 / 6.0) * (f_c - f_m);
+        d0 = 3.0 / 10.0;
+        d2 = 1.0 / 10.0;
+    }
+    if (id_face == -1) {
+        f[0] = f_c - (5.0 / 6.0) * (f_p - f_c) + (1.0 / 3.0) * (f_pp - f_p);
+        f[1] = f_c - (1.0 / 3.0) * (f_c - f_m) - (1.0 / 6.0) * (f_p - f_c);
+        f[2] = f_c + (1.0 / 6.0) * (f_m - f_mm) - (2.0 / 3.0) * (f_c - f_m);
+        d0 = 1.0 / 10.0;
+        d2 = 3.0 / 10.0;
+    }
+    d1 = 1.0 - d0 - d2;
+
+    double IS0 = (10.0 / 3.0) * (f_p - f_c) * (f_p - f_c) - (11.0 / 3.0) * (f_p - f_c) * (f_pp - f_p)
+               + (4.0 / 3.0) * (f_pp - f_p) * (f_pp - f_p);
+    double IS1 = -(5.0 / 3.0) * (f_c - f_m) * (f_p - f_c) + (4.0 / 3.0) * (f_c - f_m) * (f_c - f_m)
+               + (4.0 / 3.0) * (f_p - f_c) * (f_p - f_c);
+    double IS2 = (10.0 / 3.0) * (f_c - f_m) * (f_c - f_m) - (11.0 / 3.0) * (f_m - f_mm) * (f_c - f_m)
+               + (4.0 / 3.0) * (f_m - f_mm) * (f_m - f_mm);
+
+    double IS01 = IS0 / IS1;
+    if (isnan(IS01)) IS01 = 1.0;
+    double IS10 = 1.0 / IS01;
+    double IS02 = IS0 / IS2;
+    if (isnan(IS02)) IS02 = 1.0;
+    double IS20 = 1.0 / IS02;
+    double IS12 = IS1 / IS2;
+    if (isnan(IS12)) IS12 = 1.0;
+    double IS21 = 1.0 / IS12;
+
+    W[0] = d0 / (d0 + (IS01 * IS01) * d1 + (IS02 * IS02) * d2);
+    W[1] = d1 / ((IS10 * IS10) * d0 + d1 + (IS12 * IS12) * d2);
+    W[2] = d2 / ((IS20 * IS20) * d0 + (IS21 * IS21) * d1 + d2);
+}
+
+/* Reconstruction2Face_WENO5_D + Reconstruction2Face_WENO
+ * (include/Operators_FV/Reconstruction2Face_WENO5_D.H:12-37, Reconstruction2Face_WENO.H:13-22) */
+double ref_weno5(double f_mm, double f_m, double f_c, double f_p, double f_pp, int id_face)
+{
+    double f[3], W[3];
+    ref_weno5_weights(f_mm, f_m, f_c, f_p, f_pp, id_face, f, W);
+    double face = 0.0;
+    for (int s = 0; s < 3; ++s) face += W[s] * f[s];
+    return face;
+}
+
+/* Limiters (include/Operators_FV/Limiter_{Minmod,vanLeer,Superbee,MC}.H) and Slope_TVD
+ * (include/Operators_FV/Slope_TVD.H:12-43) */
+static double ref_slope_tvd(double sm, double sc, double sp, int id_limiter)
+{
+    if (id_limiter == 1)
+        return smin(smax(sm, 0.0), smax(sp, 0.0)) + smax(smin(sm, 0.0), smin(sp, 0.0));
+    if (id_limiter == 2) {
+        if (sm * sp > 0.0) return 2.0 / (1.0 / sm + 1.0 / sp);
+        return 0.0;
+    }
+    if (id_limiter == 3) {
+        double am = fabs(sm), ap = fabs(sp);
+        double m = smin(smin(am, ap), (0.5) * smax(am, ap));
+        return (ref_sign(sm) + ref_sign(sp)) * m;
+    }
+    if (id_limiter == 4) {
+        /* std::min({c+, 2a+, 2b+}) + std::max({c-, 2a-, 2b-}); the initializer-list forms keep
+         * the first of equal elements, which is value-identical */
+        double hi = smin(smin(smax(sc, 0.0), smax(2.0 * sm, 0.0)), smax(2.0 * sp, 0.0));
+        double lo = smax(smax(smin(sc, 0.0), smin(2.0 * sm, 0.0)), smin(2.0 * sp, 0.0));
+        return hi + lo;
+    }
+    return 0.0;
+}
+
+/* Reconstruction2Face_MUSCL_D + Reconstruction2Face_Linear
+ * (include/Operators_FV/Reconstruction2Face_MUSCL_D.H:12-36, Reconstruction2Face_Linear.H) */
+double ref_muscl(double f_m, double f_c, double f_p, double dx, int id_face, int id_limiter)
+{
+    double sm = (f_c - f_m) / dx;
+    double sc = (f_p - f_m) / (2.0 * dx);
+    double sp = (f_p - f_c) / dx;
+    double s = ref_slope_tvd(sm, sc, sp, id_limiter);
+    return f_c + (double)id_face * 0.5 * s * dx;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Riemann solvers
+ * ---------------------------------------------------------------------------------------------- */
+
+/* Riemann_HLLC (include/Physics_Compressible5Eq/Riemann_HLLC_Compressible5Eq.H:16-194) and
+ * Riemann_HLL (include/Physics_Compressible5Eq/Riemann_HLL_Compressible5Eq.H:16-133), dispatched as
+ * RiemannSolver (RiemannSolver_Compressible5Eq.H:16-47). */
+void ref_riemann(const cmp_params* P, const double* UL, const double* UR, int dir,
+                 double* flux, double* vel, double* cmax, int* err)
+{
+    const int d = P->dim, N = P->nstate, id_k = P->source_term;
+    double WL[REF_NMAX], WR[REF_NMAX], FL[REF_NMAX], FR[REF_NMAX];
+    ref_cons2prim(P, UL, WL, err);
+    ref_cons2prim(P, UR, WR, err);
+    ref_cons2flux(P, UL, dir, FL, err);
+    ref_cons2flux(P, UR, dir, FR, err);
+    double pL = WL[2 + d], pR = WR[2 + d];
+    double cL = ref_sound_speed(P, pL, WL[3 + d], WL[0], WL[1], id_k, err);
+    double cR = ref_sound_speed(P, pR, WR[3 + d], WR[0], WR[1], id_k, err);
+    double VL = WL[2 + dir], VR = WR[2 + dir];
+
+    if (P->id_riemann == 2) {
+        double V_avg = 0.5 * (VL + VR);
+        double c_avg = 0.5 * (cL + cR);
+        double SL = smin(V_avg - c_avg, VL - cL);
+        double SR = smax(V_avg + c_avg, VR + cR);
+        double Sm = smin(0.0, SL);
+        double Sp = smax(0.0, SR);
+        double rhoL = WL[0] + WL[1];
+        double rhoR = WR[0] + WR[1];
+        double S_Star = (pR - pL + rhoL * VL * (SL - VL) - rhoR * VR * (SR - VR))
+                      / (rhoL * (SL - VL) - rhoR * (SR - VR));
+        double s = ref_sign(S_Star);
+        for (int m = 0; m < d; ++m)
+            vel[m] = 0.5 * (1.0 + s) * WL[2 + m] + 0.5 * (1.0 - s) * WR[2 + m];
+        vel[dir] += 0.5 * (1.0 + s) * Sm * ((SL - VL) / (SL - S_Star) - 1.0)
+                  + 0.5 * (1.0 - s) * Sp * ((SR - VR) / (SR - S_Star) - 1.0);
+        for (int n = 0; n < N; ++n) {
+            double CLs = (SL - VL) / (SL - S_Star) * UL[n];
+            double CRs = (SR - VR) / (SR - S_Star) * UR[n];
+            if (n == 2 + dir) {
+                CLs += (SL - VL) / (SL - S_Star) * rhoL * (S_Star - VL);
+                CRs += (SR - VR) / (SR - S_Star) * rhoR * (S_Star - VR);
+            } else if (n == 2 + d) {
+                CLs += (SL - VL) / (SL - S_Star) * (S_Star - VL) * (rhoL * S_Star + pL / (SL - VL));
+                CRs += (SR - VR) / (SR - S_Star) * (S_Star - VR) * (rhoR * S_Star + pR / (SR - VR));
+            }
+            flux[n] = 0.5 * (1.0 + s) * (FL[n] + Sm * (CLs - UL[n]))
+                    + 0.5 * (1.0 - s) * (FR[n] + Sp * (CRs - UR[n]));
+        }
+    } else {
+        /* std::max({0, VL+cL, VR+cR}) / std::min({0, VL-cL, VR-cR}) */
+        double Sp = smax(smax(0.0, VL + cL), VR + cR);
+        double Sm = smin(smin(0.0, VL - cL), VR - cR);
+        for (int m = 0; m < d; ++m)
+            vel[m] = (Sp * WL[2 + m] - Sm * WR[2 + m]) / (Sp - Sm);
+        for (int n = 0; n < N; ++n)
+            flux[n] = (Sp * FL[n] - Sm * FR[n] + Sp * Sm * (UR[n] - UL[n])) / (Sp - Sm);
+    }
+    double qL = (d == 3) ? (WL[2] * WL[2] + WL[3] * WL[3] + WL[4] * WL[4]) : (WL[2] * WL[2] + WL[3] * WL[3]);
+    double qR = (d == 3) ? (WR[2] * WR[2] + WR[3] * WR[3] + WR[4] * WR[4]) : (WR[2] * WR[2] + WR[3] * WR[3]);
+    *cmax = smax(cL + sqrt(qL), cR + sqrt(qR));
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Fab access
+ * ---------------------------------------------------------------------------------------------- */
+
+typedef struct {
+    int lo[3];      /* lowest stored index per dim (lo - ng, or 0 for inactive dims) */
+    int ext[3];     /* stored extent per dim */
+    long long sj, sk, sn;
+} fab_t;
+
+static fab_t make_fab(int dim, const cmp_box* b, int ng)
+{
+    fab_t f;
+    for (int d = 0; d < 3; ++d) {
+        if (d < dim) { f.lo[d] = b->lo[d] - ng; f.ext[d] = b->hi[d] - b->lo[d] + 1 + 2 * ng; }
+        else         { f.lo[d] = b->lo[d];      f.ext[d] = 1; }
+    }
+    f.sj = f.ext[0];
+    f.sk = f.sj * f.ext[1];
+    f.sn = f.sk * f.ext[2];
+    return f;
+}
+
+static inline long long fab_idx(const fab_t* f, int i, int j, int k)
+{
+    return (long long)(i - f->lo[0]) + f->sj * (j - f->lo[1]) + f->sk * (k - f->lo[2]);
+}
+
+long long cmp_fab_size(int dim, const cmp_box* box, int ng, int ncomp);
+
+/* FVM_Conservative2FluxHyperbolic_K, ID_Bound == 0 branch without THINC
+ * (include/Operators_Compressible5Eq/FVM_Conservative2FluxHyperbolic_K_Compressible5Eq.H:15-170):
+ * reconstruction dispatch FVM_Reconstruction2FaceLR_Conservative_K (…FaceLR_Conservative_K…H:15-82),
+ * the per-side primitive reconstruction FVM_Reconstruction2Face_Conservative_{WENO5,MUSCL,Godunov}_D
+ * (cons->prim of every stencil cell, scalar reconstruction per primitive, prim->cons), the Riemann
+ * solver, and FluxNC[0] = u*_n (:165-168). L uses cells i-3..i+1 (id_face +1), R uses i-2..i+2 (-1). */
+int ref_face_flux(const cmp_params* P, const cmp_box* box, int ng, const double* U,
+                  int i, int j, int k, int dir, double dx_dir,
+                  double* flux, double* fluxnc, double* cmax)
+{
+    const int N = P->nstate;
+    int err = 0;
+    fab_t f = make_fab(P->dim, box, ng);
+    int off[3] = {0, 0, 0};
+    off[dir] = 1;
+    /* stencil cells s = -3..+2 relative to the face's high cell */
+    double Wst[6][REF_NMAX], Ust[6][REF_NMAX];
+    int smin_ = -3, smax_ = 2;
+    if (P->id_weno == 0 && P->id_limiter == 0) { smin_ = -1; smax_ = 0; }
+    else if (P->id_weno == 0) { smin_ = -2; smax_ = 1; }
+    for (int s = smin_; s <= smax_; ++s) {
+        long long c = fab_idx(&f, i + s * off[0], j + s * off[1], k + s * off[2]);
+        for (int n = 0; n < N; ++n) Ust[s + 3][n] = U[c + n * f.sn];
+    }
+    double UL[REF_NMAX], UR[REF_NMAX], WLf[REF_NMAX], WRf[REF_NMAX];
+    if (P->id_weno == 1 && P->id_limiter == 5) {
+        /* L: cons->prim of cells -3..+1 (FVM_Reconstruction2Face_Conservative_WENO5_D_Compressible5Eq.H:25-44) */
+        for (int s = -3; s <= 1; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
+        for (int n = 0; n < N; ++n)
+            WLf[n] = ref_weno5(Wst[0][n], Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], 1);
+        ref_prim2cons(P, WLf, UL, &err);
+        /* R: cons->prim of cells -2..+2 (recomputed, as the reference does) */
+        for (int s = -2; s <= 2; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
+        for (int n = 0; n < N; ++n)
+            WRf[n] = ref_weno5(Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], Wst[5][n], -1);
+        ref_prim2cons(P, WRf, UR, &err);
+    } else if (P->id_weno == 0 && P->id_limiter >= 1 && P->id_limiter <= 4) {
+        for (int s = -2; s <= 0; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
+        for (int n = 0; n < N; ++n)
+            WLf[n] = ref_muscl(Wst[1][n], Wst[2][n], Wst[3][n], dx_dir, 1, P->id_limiter);
+        ref_prim2cons(P, WLf, UL, &err);
+        for (int s = -1; s <= 1; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
+        for (int n = 0; n < N; ++n)
+            WRf[n] = ref_muscl(Wst[2][n], Wst[3][n], Wst[4][n], dx_dir, -1, P->id_limiter);
+        ref_prim2cons(P, WRf, UR, &err);
+    } else if (P->id_weno == 0 && P->id_limiter == 0) {
+        ref_cons2prim(P, Ust[2], WLf, &err);
+        ref_prim2cons(P, WLf, UL, &err);
+        ref_cons2prim(P, Ust[3], WRf, &err);
+        ref_prim2cons(P, WRf, UR, &err);
+    } else {
+        return CMP_ERR_UNSUPPORTED;
+    }
+    double vel[3];
+    ref_riemann(P, UL, UR, dir, flux, vel, cmax, &err);
+    *fluxnc = vel[dir];
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * One box: ComputeFaceFluxes (source/Class_Compressible_PhaseField.cpp:3152-3254) and
+ * FluxDivergence (:3263-3391) with FVM_SurfaceIntegral (include/Operators_FV/FVM_SurfaceIntegral.H:17-33),
+ * FVM_Divergence_{K,D} (include/Operators_FV/FVM_Divergence_K.H:12-42, _D.H:27-29) and
+ * FVM_SurfaceIntegral_NC (include/Operators_Compressible5Eq/FVM_SurfaceIntegralNC_Compressible5Eq.H:15-52).
+ * ---------------------------------------------------------------------------------------------- */
+
+int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
+                const double* U, double* dUdt, double* cmax, double** flux_out, double** fluxnc_out)
+{
+    const int dim = P->dim, N = P->nstate;
+    int nv[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) nv[d] = box->hi[d] - box->lo[d] + 1;
+    long long ncell = (long long)nv[0] * nv[1] * nv[2];
+    double* F[3] = {NULL, NULL, NULL};
+    double* FN[3] = {NULL, NULL, NULL};
+    int err = 0;
+    double cm = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        int fe[3] = {nv[0], nv[1], nv[2]};
+        fe[d] += 1;
+        long long nface = (long long)fe[0] * fe[1] * fe[2];
+        F[d] = (double*)malloc(sizeof(double) * nface * N);
+        FN[d] = (double*)malloc(sizeof(double) * nface);
+        int e = 0;
+        double cmd = 0.0;
+        int lo2 = (dim == 3) ? box->lo[2] : 0;
+#pragma omp parallel for collapse(2) reduction(max:cmd) reduction(max:e) schedule(static)
+        for (int kk = 0; kk < fe[2]; ++kk)
+            for (int jj = 0; jj < fe[1]; ++jj)
+                for (int ii = 0; ii < fe[0]; ++ii) {
+                    int i = box->lo[0] + ii, j = box->lo[1] + jj, k = lo2 + kk;
+                    double fl[REF_NMAX], fnc, c;
+                    int r = ref_face_flux(P, box, ng, U, i, j, k, d, G->dx[d], fl, &fnc, &c);
+                    if (r > e) e = r;
+                    long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                    for (int n = 0; n < N; ++n) F[d][q + n * nface] = fl[n];
+                    FN[d][q] = fnc;
+                    /* c_max(i,j,k) of a valid cell takes the faces (i,j,k) (:3212); the valid-cell
+                     * reduction (:4490) therefore skips the box's high face */
+                    int idx[3] = {ii, jj, kk};
+                    if (idx[d] < nv[d]) {
+                        if (isnan(c)) { if (CMP_ERR_NAN > e) e = CMP_ERR_NAN; }
+                        else if (cmd < c) cmd = c;
+                    }
+                }
+        if (e && !err) err = e;
+        if (cm < cmd) cm = cmd;
+        if (flux_out && flux_out[d]) memcpy(flux_out[d], F[d], sizeof(double) * nface * N);
+        if (fluxnc_out && fluxnc_out[d]) memcpy(fluxnc_out[d], FN[d], sizeof(double) * nface);
+    }
+    /* divergence, then the non-conservative term, per valid cell */
+    fab_t f = make_fab(dim, box, ng);
+    long long sf[3][3];   /* face strides per direction */
+    long long nf[3];
+    for (int d = 0; d < dim; ++d) {
+        int fe[3] = {nv[0], nv[1], nv[2]};
+        fe[d] += 1;
+        sf[d][0] = 1; sf[d][1] = fe[0]; sf[d][2] = (long long)fe[0] * fe[1];
+        nf[d] = (long long)fe[0] * fe[1] * fe[2];
+    }
+    const int iA = 3 + dim;
+    int e2 = 0;
+#pragma omp parallel for collapse(2) reduction(max:e2) schedule(static)
+    for (int kk = 0; kk < nv[2]; ++kk)
+        for (int jj = 0; jj < nv[1]; ++jj)
+            for (int ii = 0; ii < nv[0]; ++ii) {
+                long long cell = ii + (long long)nv[0] * (jj + (long long)nv[1] * kk);
+                int idx[3] = {ii, jj, kk};
+                for (int n = 0; n < N; ++n) {
+                    double div = 0.0;
+                    for (int d = 0; d < dim; ++d) {
+                        long long qm = idx[0] * sf[d][0] + idx[1] * sf[d][1] + idx[2] * sf[d][2];
+                        long long qp = qm + sf[d][d];
+                        double term = (F[d][qp + n * nf[d]] - F[d][qm + n * nf[d]]) / G->dx[d];
+                        div = (d == 0) ? term : div + term;
+                    }
+                    dUdt[cell + n * ncell] = -div;
+                }
+                /* FVM_SurfaceIntegral_NC */
+                int i = box->lo[0] + ii, j = box->lo[1] + jj, k = ((dim == 3) ? box->lo[2] : 0) + kk;
+                long long c = fab_idx(&f, i, j, k);
+                double Uc[REF_NMAX];
+                for (int n = 0; n < N; ++n) Uc[n] = U[c + n * f.sn];
+                int le = 0;
+                int id_k = P->source_term;
+                double a1 = Uc[iA];
+                double phi1 = (id_k == 1) ? ref_nasg_phi(a1, Uc[0], P->eos_C[0], &le) : a1;
+                double K[2] = {0.0, 0.0};
+                if (id_k == 1) {
+                    double Wc[REF_NMAX];
+                    ref_cons2prim(P, Uc, Wc, &le);
+                    ref_nonconservative_K(P, Wc[2 + dim], Wc[3 + dim], Wc[0], Wc[1], id_k, K, &le);
+                }
+                double divu = 0.0;
+                for (int d = 0; d < dim; ++d) {
+                    long long qm = idx[0] * sf[d][0] + idx[1] * sf[d][1] + idx[2] * sf[d][2];
+                    long long qp = qm + sf[d][d];
+                    double term = (FN[d][qp] - FN[d][qm]) / G->dx[d];
+                    divu = (d == 0) ? term : divu + term;
+                }
+                dUdt[cell + iA * ncell] += phi1 * (1.0 + K[0]) * divu;
+                if (le > e2) e2 = le;
+            }
+    if (e2 && !err) err = e2;
+    for (int d = 0; d < dim; ++d) { free(F[d]); free(FN[d]); }
+    if (cmax && (*cmax < cm || isnan(cm))) *cmax = cm;
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Level ghost fill: FillPatchSingleLevel semantics (source/COMPAS.cpp:1647-1678) with the BC map
+ * of SetPhysicsBC (source/Class_Compressible5Eq_PhaseField.cpp:162-244) and AMReX's BCType rules:
+ *   int_dir      periodic image or the valid cell of the box that owns it
+ *   foextrap     nearest in-domain cell along that direction (Outflow)
+ *   reflect_even mirror image across the boundary (Symmetry/SlipWall/NoSlipWall, non-normal comps)
+ *   reflect_odd  mirror image with the sign flipped (normal momentum; tangential for NoSlipWall)
+ * A ghost cell outside the domain in several directions composes the per-direction maps.
+ * ---------------------------------------------------------------------------------------------- */
+
+static int box_contains(int dim, const cmp_box* b, const int* c)
+{
+    for (int d = 0; d < dim; ++d)
+        if (c[d] < b->lo[d] || c[d] > b->hi[d]) return 0;
+    return 1;
+}
+
+int ref_level_fill_ghosts(const cmp_params* P, const cmp_geometry* G, int nbox,
+                          const cmp_box* boxes, int ng, double* U)
+{
+    const int dim = P->dim, N = P->nstate;
+    int err = 0;
+    for (int b = 0; b < nbox; ++b) {
+        const cmp_box* bx = &boxes[b];
+        fab_t f = make_fab(dim, bx, ng);
+        double* base = U + bx->offset;
+        int lo2 = f.lo[2], hi2 = f.lo[2] + f.ext[2] - 1;
+        int e = 0;
+#pragma omp parallel for collapse(2) reduction(max:e) schedule(static)
+        for (int k = lo2; k <= hi2; ++k)
+            for (int j = f.lo[1]; j <= f.lo[1] + f.ext[1] - 1; ++j)
+                for (int i = f.lo[0]; i <= f.lo[0] + f.ext[0] - 1; ++i) {
+                    int c[3] = {i, j, k};
+                    if (box_contains(dim, bx, c)) continue;
+                    /* map to a source cell inside the domain */
+                    int src[3] = {i, j, k};
+                    double sgn[REF_NMAX];
+                    for (int n = 0; n < N; ++n) sgn[n] = 1.0;
+                    int ok = 1;
+                    for (int d = 0; d < dim; ++d) {
+                        int dlo = G->domain_lo[d], dhi = G->domain_hi[d];
+                        if (src[d] >= dlo && src[d] <= dhi) continue;
+                        int per = dhi - dlo + 1;
+                        if (G->is_periodic[d]) {
+                            while (src[d] < dlo) src[d] += per;
+                            while (src[d] > dhi) src[d] -= per;
+                            continue;
+                        }
+                        int code = (src[d] < dlo) ? G->lo_bc[d] : G->hi_bc[d];
+                        if (code == 2 || code == 0) {          /* foextrap (Interior on a non-periodic side: treat as outflow) */
+                            src[d] = (src[d] < dlo) ? dlo : dhi;
+                        } else if (code >= 3 && code <= 5) { /* reflect */
+                            src[d] = (src[d] < dlo) ? (2 * dlo - 1 - src[d]) : (2 * dhi + 1 - src[d]);
+                            /* normal momentum odd; tangential odd only for NoSlipWall */
+                            for (int m = 0; m < dim; ++m) {
+                                int odd = (m == d) || (code == 5);
+                                if (odd) sgn[2 + m] = -sgn[2 + m];
+                            }
+                        } else {
+                            ok = 0;
+                        }
+                    }
+                    if (!ok) { if (CMP_ERR_BC > e) e = CMP_ERR_BC; continue; }
+                    int owner = -1;
+                    for (int q = 0; q < nbox; ++q)
+                        if (box_contains(dim, &boxes[q], src)) { owner = q; break; }
+                    if (owner < 0) { if (CMP_ERR_LAYOUT > e) e = CMP_ERR_LAYOUT; continue; }
+                    fab_t fs = make_fab(dim, &boxes[owner], ng);
+                    const double* sb = U + boxes[owner].offset;
+                    long long si = fab_idx(&fs, src[0], src[1], src[2]);
+                    long long di = fab_idx(&f, i, j, k);
+                    for (int n = 0; n < N; ++n) base[di + n * f.sn] = sgn[n] * sb[si + n * fs.sn];
+                }
+        if (e > err) err = e;
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Level RHS, RK3 and time step
+ * ---------------------------------------------------------------------------------------------- */
+
+static long long valid_cells(int dim, const cmp_box* b)
+{
+    long long n = 1;
+    for (int d = 0; d < dim; ++d) n *= (b->hi[d] - b->lo[d] + 1);
+    return n;
+}
+
+/* compute_dUdt_FV over all boxes; dUdt is a level buffer with U's layout (valid part written) */
+int ref_level_rhs(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                  int ng, const double* U, double* dUdt, double* cmax)
+{
+    const int dim = P->dim, N = P->nstate;
+    int err = 0;
+    for (int b = 0; b < nbox; ++b) {
+        long long nc = valid_cells(dim, &boxes[b]);
+        double* tmp = (double*)malloc(sizeof(double) * nc * N);
+        int e = ref_rhs_box(P, G, &boxes[b], ng, U + boxes[b].offset, tmp, cmax, NULL, NULL);
+        if (e && !err) err = e;
+        fab_t f = make_fab(dim, &boxes[b], ng);
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = boxes[b].hi[d] - boxes[b].lo[d] + 1;
+        int lo2 = (dim == 3) ? boxes[b].lo[2] : 0;
+        for (int n = 0; n < N; ++n)
+            for (int kk = 0; kk < nv[2]; ++kk)
+                for (int jj = 0; jj < nv[1]; ++jj)
+                    for (int ii = 0; ii < nv[0]; ++ii) {
+                        long long q = fab_idx(&f, boxes[b].lo[0] + ii, boxes[b].lo[1] + jj, lo2 + kk);
+                        dUdt[boxes[b].offset + q + n * f.sn] = tmp[ii + (long long)nv[0] * (jj + (long long)nv[1] * kk) + n * nc];
+                    }
+        free(tmp);
+    }
+    return err;
+}
+
+/* ThirdOrderSSPRK (source/Class_Compressible_PhaseField.cpp:3938-4107) on one level:
+ *   stage 1: Uborder = FillPatch(Un);  U1 = LinComb(1, Uborder, dt, dUdt)               (:4012)
+ *   stage 2: Uborder = FillPatch(U1);  U2 = LinComb(1/4, Uborder, 3/4, Un); Saxpy(dt/4)  (:4046-4047)
+ *   stage 3: Uborder = FillPatch(U2);  U3 = LinComb(2/3, Uborder, 1/3, Un); Saxpy(2dt/3) (:4083-4084)
+ * AMReX LinComb: dst = a*x + b*y; Saxpy: dst += a*src. c_max accumulates over the three stages. */
+int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
+                       const cmp_box* boxes, int ng, double* Un, double dt, double* cmax)
+{
+    const int dim = P->dim, N = P->nstate;
+    long long total = 0;
+    for (int b = 0; b < nbox; ++b) total += cmp_fab_size(dim, &boxes[b], ng, N);
+    double* Ub = (double*)malloc(sizeof(double) * total);     /* Uborder */
+    double* Unew = (double*)malloc(sizeof(double) * total);
+    double* L = (double*)malloc(sizeof(double) * total);
+    int err = 0, e;
+    const double w2 = 0.25 * dt, w3 = (2.0 / 3.0) * dt;
+    for (int stage = 0; stage < 3; ++stage) {
+        memcpy(Ub, stage == 0 ? Un : Unew, sizeof(double) * total);
+        e = ref_level_fill_ghosts(P, G, nbox, boxes, ng, Ub); if (e && !err) err = e;
+        e = ref_level_rhs(P, G, nbox, boxes, ng, Ub, L, cmax); if (e && !err) err = e;
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &boxes[b], ng);
+            int nv[3] = {1, 1, 1};
+            for (int d = 0; d < dim; ++d) nv[d] = boxes[b].hi[d] - boxes[b].lo[d] + 1;
+            int lo2 = (dim == 3) ? boxes[b].lo[2] : 0;
+            long long o = boxes[b].offset;
+            for (int n = 0; n < N; ++n)
+#pragma omp parallel for collapse(2) schedule(static)
+                for (int kk = 0; kk < nv[2]; ++kk)
+                    for (int jj = 0; jj < nv[1]; ++jj)
+                        for (int ii = 0; ii < nv[0]; ++ii) {
+                            long long q = o + fab_idx(&f, boxes[b].lo[0] + ii, boxes[b].lo[1] + jj, lo2 + kk) + n * f.sn;
+                            if (stage == 0) {
+                                Unew[q] = 1.0 * Ub[q] + dt * L[q];
+                            } else if (stage == 1) {
+                                Unew[q] = 0.25 * Ub[q] + 0.75 * Un[q];
+                                Unew[q] += w2 * L[q];
+                            } else {
+                                Unew[q] = (2.0 / 3.0) * Ub[q] + (1.0 / 3.0) * Un[q];
+                                Unew[q] += w3 * L[q];
+                            }
+                        }
+        }
+    }
+    /* the new state's valid cells replace Un's; Un's ghost cells are left as they were */
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &boxes[b], ng);
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = boxes[b].hi[d] - boxes[b].lo[d] + 1;
+        int lo2 = (dim == 3) ? boxes[b].lo[2] : 0;
+        for (int n = 0; n < N; ++n)
+            for (int kk = 0; kk < nv[2]; ++kk)
+                for (int jj = 0; jj < nv[1]; ++jj)
+                    for (int ii = 0; ii < nv[0]; ++ii) {
+                        long long q = boxes[b].offset + fab_idx(&f, boxes[b].lo[0] + ii, boxes[b].lo[1] + jj, lo2 + kk) + n * f.sn;
+                        Un[q] = Unew[q];
+                    }
+    }
+    free(Ub); free(Unew); free(L);
+    return err;
+}
+
+/* CalculateInitialcmax (source/Class_Compressible_PhaseField.cpp:4502-4605): FillPatch into a
+ * bordered copy, then the face kernel over every box without storing fluxes. */
+int ref_level_initial_cmax(const cmp_params* P, const cmp_geometry* G, int nbox,
+                           const cmp_box* boxes, int ng, const double* U, double* cmax)
+{
+    const int dim = P->dim, N = P->nstate;
+    long long total = 0;
+    for (int b = 0; b < nbox; ++b) total += cmp_fab_size(dim, &boxes[b], ng, N);
+    double* Ub = (double*)malloc(sizeof(double) * total);
+    double* L = (double*)malloc(sizeof(double) * total);
+    memcpy(Ub, U, sizeof(double) * total);
+    *cmax = 0.0;
+    int err = ref_level_fill_ghosts(P, G, nbox, boxes, ng, Ub);
+    int e = ref_level_rhs(P, G, nbox, boxes, ng, Ub, L, cmax);
+    if (e && !err) err = e;
+    free(Ub); free(L);
+    return err;
+}
+
+/* EstTimeStep (source/Class_Compressible_PhaseField.cpp:4473-4499): min over dims of cfl*dx/cmax */
+double ref_est_dt(const cmp_geometry* G, double cmax, double cfl)
+{
+    double dt_est = DBL_MAX;
+    for (int d = 0; d < G->dim; ++d) {
+        double e = cfl * G->dx[d] / cmax;
+        dt_est = (e < dt_est) ? e : dt_est;   /* amrex::min */
+    }
+    return dt_est;
+}
+
+/* ComputeDt for one level without stop_time clipping (source/COMPAS.cpp:2250-2302):
+ * dt = min(est, 1.1 dt_prev) when the change limiter is on and dt_prev > 0. */
+double ref_compute_dt(const cmp_geometry* G, double cmax, double cfl, int limiter, double dt_prev)
+{
+    double dt_tmp = ref_est_dt(G, cmax, cfl);
+    double dt_0 = dt_tmp;
+    const double change_max = 1.1;
+    if (limiter && dt_prev > 0.0) dt_tmp = smin(dt_tmp, change_max * dt_prev);
+    dt_0 = smin(dt_0, 1 * dt_tmp);
+    return dt_0;
+}
+
+/* Evolve's step loop for a single level (source/Class_Compressible_PhaseField.cpp:202-263):
+ * ComputeDt from the previous step's c_max (the initial state's at step 0), then one RK3 step. */
+int ref_level_advance(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                      int ng, double* U, int nsteps, double cfl, int limiter, double* dt_hist)
+{
+    double cmax = 0.0, dt = 0.0;
+    int err = ref_level_initial_cmax(P, G, nbox, boxes, ng, U, &cmax);
+    for (int s = 0; s < nsteps && !err; ++s) {
+        /* at t = 0 the previous dt is set to the estimate itself (:2258-2260) */
+        dt = ref_compute_dt(G, cmax, cfl, limiter, s == 0 ? ref_est_dt(G, cmax, cfl) : dt);
+        if (dt_hist) dt_hist[s] = dt;
+        cmax = 0.0;   /* c_max reset at stage 0 (:3161-3164) */
+        err = ref_level_rk3_step(P, G, nbox, boxes, ng, U, dt, &cmax);
+        if (isnan(cmax)) err = CMP_ERR_NAN;
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Layout helpers (the same definitions as the product library's; the oracle keeps its own copy so
+ * that it links nothing from it)
+ * ---------------------------------------------------------------------------------------------- */
+
+long long cmp_fab_size(int dim, const cmp_box* box, int ng, int ncomp)
+{
+    long long n = ncomp;
+    for (int d = 0; d < dim; ++d) n *= (box->hi[d] - box->lo[d] + 1 + 2 * ng);
+    return n;
+}
+
+int cmp_level_layout(int dim, int nbox, cmp_box* boxes, int ng, int ncomp, long long* total)
+{
+    long long off = 0;
+    for (int b = 0; b < nbox; ++b) {
+        boxes[b].offset = off;
+        off += cmp_fab_size(dim, &boxes[b], ng, ncomp);
+    }
+    if (total) *total = off;
+    return CMP_OK;
+}
+
+int ref_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ref_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}

@@ -1,0 +1,78 @@
+/*
+ * cpu_ref.h - CPU restatement (the oracle) of COMPAS's five-equation hyperbolic right-hand side.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+ * may load this library, and only as the checker or the timed CPU baseline. The product path
+ * (libcompas_amd.so) never links or calls it.
+ *
+ * Every function restates one reference function in the same operation order (no FMA
+ * contraction: built with -ffp-contract=off), and says which file:line it follows. Data layout
+ * and the POD structs are those of include/compas_amd.h.
+ */
+#ifndef CPU_REF_H
+#define CPU_REF_H
+
+#include "../include/compas_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REF_NMAX 7
+
+/* pointwise physics */
+double ref_nasg_phi(double alpha, double alpha_rho, double b, int* err);
+double ref_eos_mixture(const cmp_params* P, double input, double a1, double a1r1, double a2r2,
+                       int output_internal_energy, int* err);
+void   ref_cons2prim(const cmp_params* P, const double* U, double* W, int* err);
+void   ref_prim2cons(const cmp_params* P, const double* W, double* U, int* err);
+double ref_rho_c2_phase(double p, double A, double B);
+double ref_sound_speed(const cmp_params* P, double p, double a1, double a1r1, double a2r2, int id_k,
+                       int* err);
+void   ref_nonconservative_K(const cmp_params* P, double p, double a1, double a1r1, double a2r2,
+                             int id_k, double* K, int* err);
+void   ref_cons2flux(const cmp_params* P, const double* U, int dir, double* F, int* err);
+
+/* scalar reconstruction */
+double ref_weno5(double f_mm, double f_m, double f_c, double f_p, double f_pp, int id_face);
+void   ref_weno5_weights(double f_mm, double f_m, double f_c, double f_p, double f_pp, int id_face,
+                         double* cand, double* w);
+double ref_muscl(double f_m, double f_c, double f_p, double dx, int id_face, int id_limiter);
+
+/* Riemann solvers: flux[nstate], vel[dim], *cmax */
+void ref_riemann(const cmp_params* P, const double* UL, const double* UR, int dir,
+                 double* flux, double* vel, double* cmax, int* err);
+
+/* per-face kernel on a box fab: U is the fab base of `box` (ng ghosts); (i,j,k) the face index
+ * (face i sits between cells i-1 and i in direction dir) */
+int ref_face_flux(const cmp_params* P, const cmp_box* box, int ng, const double* U,
+                  int i, int j, int k, int dir, double dx_dir,
+                  double* flux, double* fluxnc, double* cmax);
+
+/* one box: ComputeFaceFluxes + FluxDivergence. dUdt: valid box, nstate comps. cmax (scalar or NULL)
+ * max-accumulated over low faces of valid cells. flux[d]/fluxnc[d] (NULL allowed) nodal arrays. */
+int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
+                const double* U, double* dUdt, double* cmax, double** flux, double** fluxnc);
+
+/* level operations on level buffers laid out by cmp_level_layout */
+int ref_level_fill_ghosts(const cmp_params* P, const cmp_geometry* G, int nbox,
+                          const cmp_box* boxes, int ng, double* U);
+int ref_level_rhs(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                  int ng, const double* U, double* dUdt, double* cmax);
+int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
+                       const cmp_box* boxes, int ng, double* Un, double dt, double* cmax);
+int ref_level_initial_cmax(const cmp_params* P, const cmp_geometry* G, int nbox,
+                           const cmp_box* boxes, int ng, const double* U, double* cmax);
+double ref_est_dt(const cmp_geometry* G, double cmax, double cfl);
+double ref_compute_dt(const cmp_geometry* G, double cmax, double cfl, int limiter, double dt_prev);
+int ref_level_advance(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                      int ng, double* U, int nsteps, double cfl, int limiter, double* dt_hist);
+
+/* number of OpenMP threads the level functions use (1 when built without OpenMP) */
+int ref_num_threads(void);
+void ref_set_num_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

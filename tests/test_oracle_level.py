@@ -1,0 +1,223 @@
+"""Pins the oracle's box and level operations (divergence, non-conservative term, ghost fill, RK3,
+time step) with invariants and closed forms that do not restate its code: exact zero RHS of a
+uniform state, telescoping conservation, symmetry, translation invariance, the five-equation
+consistency of a uniform volume fraction under compression, 2D/3D agreement, numpy's pad modes,
+third-order convergence in time, and the CFL formula with the textbook sound speed. CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from compas_amd import Layout, cases, make_geometry, make_params, scheme_ghosts
+
+SCHEMES = [("WENO5", "HLLC"), ("WENO5", "HLL"), ("MUSCL", "HLLC"), ("MUSCL-MC", "HLL"),
+           ("GODUNOV", "HLLC")]
+
+
+def _box_rhs(c, scheme):
+    P, G = c["params"], c["geom"]
+    ng = scheme_ghosts(scheme)
+    lay = Layout(P.dim, c["boxes"], ng, P.nstate)
+    buf = lay.pack(c["U"])
+    oracle.level_fill_ghosts(P, G, lay, buf)
+    dUdt, cmax = oracle.level_rhs(P, G, lay, buf)
+    return lay.unpack(dUdt, c["shape"]), cmax
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("scheme,riemann", SCHEMES)
+def test_uniform_state_has_zero_rhs(dim, scheme, riemann):
+    n = 12 if dim == 2 else 8
+    c = cases.random_smooth(dim, n, seed=0, scheme=scheme, riemann=riemann, amp=0.0)
+    c["U"][:] = c["U"][(slice(None),) + (0,) * dim][(slice(None),) + (None,) * dim]
+    r, _ = _box_rhs(c, scheme)
+    assert np.all(r == 0.0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("scheme,riemann", SCHEMES)
+def test_periodic_conservation(dim, scheme, riemann):
+    """Sum over a periodic box of dU/dt of every conservative component telescopes to zero."""
+    n = 16 if dim == 2 else 8
+    c = cases.random_smooth(dim, n, seed=4, scheme=scheme, riemann=riemann, max_grid_size=n // 2)
+    r, _ = _box_rhs(c, scheme)
+    for q in range(2 + dim + 1):
+        scale = np.sum(np.abs(r[q])) + 1e-300
+        assert abs(np.sum(r[q])) <= 1e-13 * scale, (q, np.sum(r[q]), scale)
+
+
+@pytest.mark.parametrize("scheme,riemann", SCHEMES)
+def test_xy_symmetry(scheme, riemann):
+    """Transposing the 2D state (and swapping u, v) transposes dU/dt."""
+    c = cases.random_smooth(2, 16, seed=5, scheme=scheme, riemann=riemann)
+    r, _ = _box_rhs(c, scheme)
+    ct = dict(c)
+    Ut = np.transpose(c["U"], (0, 2, 1)).copy()
+    Ut[[2, 3]] = Ut[[3, 2]]
+    ct["U"] = Ut
+    rt, _ = _box_rhs(ct, scheme)
+    rtt = np.transpose(rt, (0, 2, 1)).copy()
+    rtt[[2, 3]] = rtt[[3, 2]]
+    for q in range(6):
+        assert np.allclose(rtt[q], r[q], rtol=1e-13, atol=1e-13 * np.max(np.abs(r[q])))
+
+
+def test_translation_invariance_is_exact():
+    c = cases.random_smooth(2, 16, seed=6)
+    r, _ = _box_rhs(c, "WENO5")
+    c2 = dict(c)
+    c2["U"] = np.roll(c["U"], (3, -5), axis=(1, 2)).copy()
+    r2, _ = _box_rhs(c2, "WENO5")
+    assert np.array_equal(np.roll(r, (3, -5), axis=(1, 2)), r2)
+
+
+@pytest.mark.parametrize("riemann", ["HLLC", "HLL"])
+def test_uniform_alpha_stays_uniform_under_compression(riemann):
+    """d(alpha)/dt + u.grad(alpha) = 0 (K = 0): with alpha uniform and div(u) != 0 the
+    conservative alpha flux and the alpha div(u*) term cancel to round-off. A wrong sign or index
+    of the non-conservative term leaves 2 alpha div(u) or moves it to another component."""
+    c = cases.random_smooth(2, 16, seed=8, riemann=riemann, amp=0.3)
+    U = c["U"]
+    a1 = 0.37
+    rho = U[0] + U[1]
+    W_u, W_v = U[2] / rho, U[3] / rho
+    P = c["params"]
+    W = np.stack([a1 * 1.0 * np.ones_like(rho), (1 - a1) * 5.0 * np.ones_like(rho), W_u, W_v,
+                  np.ones_like(rho), a1 * np.ones_like(rho)])
+    c["U"] = cases.prim_to_cons(P, W)
+    r, _ = _box_rhs(c, "WENO5")
+    divu_scale = np.max(np.abs(r[0])) / a1   # the mass equation carries rho1 alpha1 div(u)
+    assert divu_scale > 1e-2
+    assert np.max(np.abs(r[5])) <= 1e-13 * divu_scale
+
+
+def test_2d_and_3d_agree_on_z_invariant_state():
+    """A 3D state that does not vary in z (w = 0) has the 2D right-hand side in every z plane."""
+    c2 = cases.random_smooth(2, 12, seed=9)
+    r2, _ = _box_rhs(c2, "WENO5")
+    P3 = make_params(3, gamma=(1.4, 1.6))
+    G3 = make_geometry((12, 12, 4), (0, 0, 0), (1, 1, 1.0 / 3.0), (1, 1, 1))
+    U2 = c2["U"]
+    U3 = np.zeros((7, 4, 12, 12))
+    for k in range(4):
+        U3[0:4, k] = U2[0:4]
+        U3[4, k] = 0.0
+        U3[5:7, k] = U2[4:6]
+    c3 = dict(params=P3, geom=G3, U=U3, boxes=[([0, 0, 0], [11, 11, 3])], shape=(4, 12, 12))
+    r3, _ = _box_rhs(c3, "WENO5")
+    for k in range(4):
+        for (q2, q3) in [(0, 0), (1, 1), (2, 2), (3, 3), (4, 5), (5, 6)]:
+            assert np.allclose(r3[q3, k], r2[q2], rtol=1e-14, atol=1e-14 * np.max(np.abs(r2[q2])))
+        assert np.all(r3[4, k] == 0.0)
+
+
+@pytest.mark.parametrize("bc,mode,odd", [(None, "wrap", False), (2, "edge", False),
+                                         (3, "symmetric", True), (4, "symmetric", True),
+                                         (5, "symmetric", True)])
+def test_ghost_fill_matches_numpy_pad(bc, mode, odd):
+    """Single box covering the domain: periodic = wrap, Outflow = edge, Symmetry/SlipWall =
+    symmetric with the normal momentum negated, NoSlipWall = symmetric with all momenta negated."""
+    n, ng = 10, 3
+    c = cases.random_smooth(2, n, seed=10, periodic=(bc is None))
+    G = c["geom"] if bc is None else make_geometry((n, n), (0, 0), (1, 1), (0, 0), [bc, bc], [bc, bc])
+    P = c["params"]
+    lay = Layout(2, c["boxes"], ng, 6)
+    buf = lay.pack(c["U"])
+    oracle.level_fill_ghosts(P, G, lay, buf)
+    fab = lay.fab_view(buf, 0)
+    for q in range(6):
+        # rows (y) first, then columns (x): cross-shaped ghost regions only (corners compose)
+        padx = np.pad(c["U"][q], ((0, 0), (ng, ng)), mode=mode)
+        pady = np.pad(c["U"][q], ((ng, ng), (0, 0)), mode=mode)
+        if odd and q in (2, 3):
+            sx = -1.0 if (q == 2 or bc == 5) else 1.0
+            sy = -1.0 if (q == 3 or bc == 5) else 1.0
+            padx[:, :ng] *= sx
+            padx[:, -ng:] *= sx
+            pady[:ng, :] *= sy
+            pady[-ng:, :] *= sy
+        assert np.array_equal(fab[q, ng:-ng, :], padx)
+        assert np.array_equal(fab[q, :, ng:-ng], pady)
+
+
+def test_ghost_fill_independent_of_box_split():
+    n = 24
+    c = cases.random_smooth(2, n, seed=12, periodic=False)
+    G = make_geometry((n, n), (0, 0), (1, 1), (0, 0), [3, 2], [5, 4])
+    P = c["params"]
+    one = Layout(2, c["boxes"], 3, 6)
+    b1 = one.pack(c["U"])
+    oracle.level_fill_ghosts(P, G, one, b1)
+    boxes = [([0, 0], [7, 23]), ([8, 0], [23, 9]), ([8, 10], [15, 23]), ([16, 10], [23, 23])]
+    many = Layout(2, boxes, 3, 6)
+    bm = many.pack(c["U"])
+    oracle.level_fill_ghosts(P, G, many, bm)
+    big = one.fab_view(b1, 0)
+    for b, (lo, hi) in enumerate(boxes):
+        f = many.fab_view(bm, b)
+        sub = big[:, lo[1]:hi[1] + 7, lo[0]:hi[0] + 7]
+        # compare the cross-shaped part (valid rows with x ghosts, valid columns with y ghosts)
+        assert np.array_equal(f[:, 3:-3, :], sub[:, 3:-3, :])
+        assert np.array_equal(f[:, :, 3:-3], sub[:, :, 3:-3])
+
+
+def _density_wave(n, cfl_dt):
+    """alpha1 = 1 single-gas density wave at u = 1, v = 0.5, p = 1 on [0,1]^2: exact solution is a
+    translation."""
+    P = make_params(2, gamma=(1.4, 1.6))
+    G = make_geometry((n, n), (0, 0), (1, 1), (1, 1))
+    h = 1.0 / n
+    x = (np.arange(n) + 0.5) * h
+
+    def avg(t):
+        # cell averages of 1 + 0.2 sin(2 pi (x - t)) sin(2 pi (y - 0.5 t)) (exact)
+        fx = (np.cos(2 * np.pi * (x - 0.5 * h - t)) - np.cos(2 * np.pi * (x + 0.5 * h - t))) / (2 * np.pi * h)
+        fy = (np.cos(2 * np.pi * (x - 0.5 * h - 0.5 * t)) - np.cos(2 * np.pi * (x + 0.5 * h - 0.5 * t))) / (2 * np.pi * h)
+        return 1.0 + 0.2 * np.outer(fy, fx)
+
+    rho = avg(0.0)
+    W = np.stack([rho, np.zeros_like(rho), np.ones_like(rho), 0.5 * np.ones_like(rho),
+                  np.ones_like(rho), np.ones_like(rho)])
+    U = cases.prim_to_cons(P, W)
+    return P, G, U, avg
+
+
+def test_rk3_third_order_in_time():
+    """On a fixed mesh, the difference from a tiny-dt solution (the time error alone) falls like
+    dt^3; a wrong stage coefficient drops the order to 1 or 2. The spatial error is checked too:
+    the tiny-dt solution is within WENO5's error of the exact translation."""
+    n, T = 32, 0.2
+
+    def run(nsteps):
+        P, G, U, avg = _density_wave(n, None)
+        lay = Layout(2, [([0, 0], [n - 1, n - 1])], 3, 6)
+        buf = lay.pack(U)
+        for _ in range(nsteps):
+            oracle.level_rk3_step(P, G, lay, buf, T / nsteps)
+        return lay.unpack(buf, (n, n))[0], avg
+
+    ref, avg = run(512)
+    assert np.max(np.abs(ref - avg(T))) < 2e-5
+    errs = [np.max(np.abs(run(s)[0] - ref)) for s in (16, 32, 64)]
+    o = math.log2(errs[0] / errs[1]), math.log2(errs[1] / errs[2])
+    assert o[0] > 2.7 and o[1] > 2.7, (errs, o)
+
+
+def test_dt_formula_and_wave_speed():
+    """Uniform state: c_max = c + |u| with the textbook ideal-gas c, dt = cfl min(dx)/c_max, and
+    the 1.1 growth limiter."""
+    P = make_params(2, gamma=(1.4, 1.6))
+    G = make_geometry((8, 16), (0, 0), (1, 1), (1, 1))
+    rho, u, v, p = 1.0, 0.6, -0.8, 1.0
+    W = np.zeros((6, 16, 8))
+    W[0], W[1], W[2], W[3], W[4], W[5] = rho, 0.0, u, v, p, 1.0
+    lay = Layout(2, [([0, 0], [7, 15])], 3, 6)
+    buf = lay.pack(cases.prim_to_cons(P, W))
+    cmax = oracle.level_initial_cmax(P, G, lay, buf)
+    assert cmax == pytest.approx(math.sqrt(1.4 * p / rho) + 1.0, rel=1e-14)
+    dt = oracle.compute_dt(G, cmax, 0.4, 1, 0.0)
+    assert dt == pytest.approx(0.4 * (1.0 / 16) / cmax, rel=1e-15)
+    assert oracle.compute_dt(G, cmax, 0.4, 1, 0.5 * dt) == pytest.approx(0.55 * dt, rel=1e-15)
+    assert oracle.compute_dt(G, cmax, 0.4, 0, 0.5 * dt) == dt
