@@ -140,7 +140,9 @@ def test_one_rk3_step_matches_oracle(gpu, dim, scheme, riemann, mg):
 
 def test_advected_interface_100_steps(gpu):
     """The advected-interface case (WENO5 + HLLC + RK3, CFL 0.5) at 64^2 on a 2x2 box split:
-    L1 <= 1e-9 after 100 steps, totals conserved, u and p uniform, alpha in its clip range."""
+    L1 <= 1e-9 after 100 steps, totals conserved, u and p uniform. Plain WENO5 has no bound
+    limiter, so alpha may leave [0, 1] slightly, in the reference as here: the GPU's extremes must
+    match the oracle's (the [0, 1] bound is checked for WENO5B)."""
     c = cases.advected_interface_2d(n=64, max_grid_size=32)
     lay = Layout(2, c["boxes"], 3, 6)
     buf = lay.pack(c["U"])
@@ -153,7 +155,7 @@ def test_advected_interface_100_steps(gpu):
     U0 = c["U"]
     for n in range(5):   # masses, momenta, energy: periodic box, exact telescoping
         assert abs(got[n].sum() - U0[n].sum()) <= 1e-13 * np.abs(U0[n]).sum()
-    assert got[5].min() > 0.0 and got[5].max() < 1.0
+    assert abs(got[5].min() - ref[5].min()) <= 1e-10 and abs(got[5].max() - ref[5].max()) <= 1e-10
     u = got[2] / (got[0] + got[1])
     assert np.max(np.abs(u - 1.0)) < 1e-12
 
