@@ -32,6 +32,14 @@ def workload(name):
                        "nstate": 6, "bc": "periodic"}
         c["default_steps"] = 100
         return c
+    if name == "water-air-droplet-3d":
+        c = cases.water_air_droplet_3d(n=256, box=32)
+        c["config"] = {"workload": name, "grid": [256, 256, 256], "boxes": len(c["boxes"]),
+                       "box": [32, 32, 32], "scheme": "WENO5+THINC", "riemann": "HLLC",
+                       "integrator": "TVD-RK3", "cfl": 0.4, "nstate": 7, "bc": "outflow",
+                       "eos": "stiffened gas water / ideal air"}
+        c["default_steps"] = 20
+        return c
     raise SystemExit("unknown workload %r" % name)
 
 
@@ -144,7 +152,8 @@ def main():
         "config": c["config"],
         "roofline": {
             "bound": "valu",
-            "kernel": "k_stage (fused WENO5 + HLLC + divergence + RK3 stage)",
+            "kernel": "k_stage (fused %s + %s + divergence + RK3 stage)" % (c["config"]["scheme"],
+                                                                            c["config"]["riemann"]),
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",

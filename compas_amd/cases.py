@@ -82,12 +82,13 @@ def rankine_hugoniot(gamma, p1, rho1, mach):
     return p2, rho2, u2
 
 
-def water_air_droplet_3d(n=256, box=32, scheme="WENO5", riemann="HLLC"):
+def water_air_droplet_3d(n=256, box=32, scheme="WENO5", riemann="HLLC", thinc=1):
     """3D water-air shock-droplet: unit cube, outflow BCs, water (stiffened gas gamma 4.4,
     p_inf 6e8 Pa, rho 1000) droplet r = 0.1 at the centre, air (gamma 1.4, rho 1.2), p = 1e5 Pa,
-    Mach 2.4 shock in air at x = 0.25 moving in +x. Interface: tanh profile of width dx."""
+    Mach 2.4 shock in air at x = 0.25 moving in +x. Interface: tanh profile of width dx. THINC on
+    a WENO5 base scheme (FiniteVolume.THINC = 1 with its default parameters)."""
     params = make_params(3, gamma=(4.4, 1.4), pinf=(6.0e8, 0.0), scheme=scheme, riemann=riemann,
-                         source_term=0)
+                         source_term=0, thinc=thinc)
     geom = make_geometry((n, n, n), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (0, 0, 0))
     x, y, z = _centres(geom, (n, n, n))
     r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2)
@@ -110,12 +111,12 @@ def water_air_droplet_3d(n=256, box=32, scheme="WENO5", riemann="HLLC"):
 
 
 def random_smooth(dim, n, seed, periodic=True, max_grid_size=None, scheme="WENO5", riemann="HLLC",
-                  gamma=(1.4, 1.6), pinf=(0.0, 0.0), source_term=0, amp=0.1):
+                  gamma=(1.4, 1.6), pinf=(0.0, 0.0), source_term=0, amp=0.1, thinc=0):
     """Seeded smooth two-phase state for parity tests: a few random Fourier modes on top of a
     uniform state, with a tanh interface across the middle."""
     rng = np.random.default_rng(seed)
     params = make_params(dim, gamma=gamma, pinf=pinf, scheme=scheme, riemann=riemann,
-                         source_term=source_term)
+                         source_term=source_term, thinc=thinc)
     shape = (n,) * dim
     geom = make_geometry(shape, (0.0,) * dim, (1.0,) * dim, (int(periodic),) * dim)
     X = _centres(geom, tuple(reversed(shape)))

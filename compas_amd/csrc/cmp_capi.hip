@@ -60,14 +60,18 @@ hipError_t launch_faces(const LaunchCfg& lc, const KParams& kp, const BoxDev& bo
                        : launch_faces_dim<3>(lc, kp, box, dir, U, flux, fluxnc, s);
 }
 
+template <int DIM>
+void tile_dims_dim(int* tx, int* ty, int* tz);
+
 int tile_dims(int dim, int* tx, int* ty, int* tz)
 {
-    if (dim == 2) { *tx = 16; *ty = 16; *tz = 1; }
-    else          { *tx = 8;  *ty = 8;  *tz = 4; }
+    if (dim == 2) tile_dims_dim<2>(tx, ty, tz);
+    else          tile_dims_dim<3>(tx, ty, tz);
     return 0;
 }
 
-int scheme_ghosts(int scheme) { return scheme == 5 ? 3 : (scheme == 1 ? 2 : 1); }
+// Ghost cells a scheme reads (FiniteVolume_Parameter.H:71-139 Num_Grow; THINC needs 2, :194-196).
+int scheme_ghosts(int scheme) { return scheme >= 5 ? 3 : (scheme == 1 ? 2 : 1); }
 
 }  // namespace cmp
 
@@ -111,10 +115,20 @@ int check_hip(hipError_t e)
 
 #define CMP_TRY(x) do { int _r = check_hip(x); if (_r) return _r; } while (0)
 
+// FiniteVolume_Parameter IDs -> kernel scheme. THINC is built on the WENO5 / WENO5B base schemes
+// (the ones the benchmark cases use); THINC on MUSCL/Godunov and the bound-preserving schemes are
+// not built here.
 int scheme_of(const cmp_params* P, int* scheme)
 {
-    if (P->id_bound != 0 || P->id_thinc != 0) return CMP_ERR_UNSUPPORTED;
+    if (P->id_bound != 0) return CMP_ERR_UNSUPPORTED;
+    if (P->id_thinc != 0 && P->id_thinc != 1) return CMP_ERR_ARG;
     if (P->id_weno == 1 && P->id_limiter == 5) { *scheme = 5; return CMP_OK; }
+    if (P->id_weno == 1 && P->id_limiter == 6) {
+        if (P->id_poly != 2 && P->id_poly != 4) return CMP_ERR_ARG;
+        *scheme = 6;
+        return CMP_OK;
+    }
+    if (P->id_thinc) return CMP_ERR_UNSUPPORTED;
     if (P->id_weno == 0 && P->id_limiter >= 1 && P->id_limiter <= 4) { *scheme = 1; return CMP_OK; }
     if (P->id_weno == 0 && P->id_limiter == 0) { *scheme = 0; return CMP_OK; }
     return CMP_ERR_UNSUPPORTED;
@@ -135,11 +149,14 @@ int validate(const cmp_params* P, const cmp_geometry* G, int ng, LaunchCfg* lc, 
         if (!(G->dx[d] > 0.0)) return CMP_ERR_ARG;
     for (int k = 0; k < 2; ++k)
         if (!(P->eos_A[k] > 0.0)) return CMP_ERR_ARG;
+    if (P->id_thinc && !(P->thinc_beta > 0.0)) return CMP_ERR_ARG;
     lc->dim = P->dim;
     lc->scheme = scheme;
     lc->limiter = P->id_limiter;
     lc->riemann = P->id_riemann;
+    lc->thinc = P->id_thinc;
     std::memset(kp, 0, sizeof(*kp));
+    kp->weno5b_weight = (P->id_poly == 2) ? 1.0 / 6.0 : 1.0 / 12.0;
     for (int k = 0; k < 2; ++k) {
         kp->A[k] = P->eos_A[k];
         kp->B[k] = P->eos_B[k];

@@ -1,12 +1,20 @@
-// cmp_device.hpp - pointwise five-equation physics and scalar reconstructions for gfx950 kernels.
+// cmp_device.hpp - pointwise five-equation physics, scalar reconstructions and the face flux of
+// the gfx950 kernels.
 //
-// FP64 throughout. Formulas follow the reference's operators (file:line on each function);
-// the kernels call these on primitives that are computed once per cell and staged in LDS, and
-// the HLLC flux evaluates only the upwind star state. FMA contraction is left to the compiler.
+// FP64 throughout. The formulas are the reference's (file:line on each function), arranged for
+// the GPU: primitives are computed once per staged cell (one division), WENO5 weights use the
+// product form d_k prod_{j!=k} IS_j^2 / sum (one division instead of nine, with the reference's
+// ratio form kept for the cases the product form cannot represent), the Riemann solver works on
+// the face primitives it is given instead of re-deriving them, and the HLLC flux evaluates only
+// the upwind star state. Results agree with the reference's operation order to rounding.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include "../../include/compas_amd.h"
+
+#ifndef CMP_FAST_RCP
+#define CMP_FAST_RCP 1
+#endif
 
 namespace cmp {
 
@@ -20,13 +28,14 @@ struct KParams {
     double thinc_eps, thinc_beta, thinc_thr;
     double dx[3];
     double inv_dx[3];
+    double weno5b_weight;   // quadrature weight of the WENO5B bound limiter (1/12 for ID_Poly = 4)
 };
 
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }  // std::min
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double dsign(double x) { return (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0); }
 
-// NASG_Phi (include/EquationOfState_NASG.H:34-41); the covolume violation sets *bad.
+// NASG_Phi (include/EquationOfState_NASG.H:34-41); the covolume violation sets bad.
 __device__ __forceinline__ double nasg_phi(double alpha, double alpha_rho, double b, int& bad)
 {
     if (b > 0.0) {
@@ -36,7 +45,7 @@ __device__ __forceinline__ double nasg_phi(double alpha, double alpha_rho, doubl
     return alpha;
 }
 
-// Mixture A and B sums of EquationOfState_Mixture
+// Mixture sums of EquationOfState_Mixture
 // (include/Physics_Compressible5Eq/EquationOfState_Mixture_Compressible5Eq.H:15-54).
 __device__ __forceinline__ void mix_AB(const KParams& kp, double a1, double a1r1, double a2r2,
                                        double& AP, double& BP, double& rho_D, int& bad)
@@ -52,41 +61,46 @@ __device__ __forceinline__ void mix_AB(const KParams& kp, double a1, double a1r1
     BP = phi1 * kp.B[0] + phi2 * kp.B[1];
 }
 
-// Conservative2Primitive (include/Physics_Compressible5Eq/Conservative2Primitive_Compressible5Eq.H:15-61).
+// Conservative2Primitive (include/Physics_Compressible5Eq/Conservative2Primitive_Compressible5Eq.H:15-61),
+// with 1/rho and 1/AP from one division.
 template <int DIM>
 __device__ __forceinline__ void cons2prim(const KParams& kp, const double* U, double* W, int& bad)
 {
     const double a1r1 = U[0], a2r2 = U[1];
     const double rho = a1r1 + a2r2;
+    double AP, BP, rho_D;
+    mix_AB(kp, U[3 + DIM], a1r1, a2r2, AP, BP, rho_D, bad);
+    const double inv = 1.0 / (rho * AP);
+    const double ir = AP * inv;
     double ke = 0.0;
 #pragma unroll
     for (int m = 0; m < DIM; ++m) {
-        const double v = U[2 + m] / rho;
+        const double v = U[2 + m] * ir;
         W[2 + m] = v;
         ke = (m == 0) ? v * v : ke + v * v;
     }
     const double rho_e = U[2 + DIM] - (0.5 * rho) * ke;
-    double AP, BP, rho_D;
-    mix_AB(kp, U[3 + DIM], a1r1, a2r2, AP, BP, rho_D, bad);
     W[0] = a1r1;
     W[1] = a2r2;
-    W[2 + DIM] = (rho_e - rho_D - BP) / AP;
+    W[2 + DIM] = (rho_e - rho_D - BP) * (rho * inv);
     W[3 + DIM] = U[3 + DIM];
 }
 
-// Primitive2Conservative (include/Physics_Compressible5Eq/Primitive2Conservative_Compressible5Eq.H:15-62).
+// Primitive2Conservative (include/Physics_Compressible5Eq/Primitive2Conservative_Compressible5Eq.H:15-62);
+// also returns AP, BP and rho for the sound speed.
 template <int DIM>
-__device__ __forceinline__ void prim2cons(const KParams& kp, const double* W, double* U, int& bad)
+__device__ __forceinline__ void prim2cons(const KParams& kp, const double* W, double* U, double& AP,
+                                          double& BP, double& rho, int& bad)
 {
     const double a1r1 = W[0], a2r2 = W[1];
-    const double rho = a1r1 + a2r2;
+    rho = a1r1 + a2r2;
     double ke = 0.0;
 #pragma unroll
     for (int m = 0; m < DIM; ++m) {
         U[2 + m] = rho * W[2 + m];
         ke = (m == 0) ? W[2 + m] * W[2 + m] : ke + W[2 + m] * W[2 + m];
     }
-    double AP, BP, rho_D;
+    double rho_D;
     mix_AB(kp, W[3 + DIM], a1r1, a2r2, AP, BP, rho_D, bad);
     const double rho_e = AP * W[2 + DIM] + BP + rho_D;
     U[0] = a1r1;
@@ -97,27 +111,23 @@ __device__ __forceinline__ void prim2cons(const KParams& kp, const double* W, do
 
 // SoundSpeed with rho_c2_Phase / rho_c2_Mixture
 // (include/Physics_Compressible5Eq/SoundSpeed_Compressible5Eq.H:15-54, rho_c2_Phase_…H:17-23,
-//  rho_c2_Mixture_…H:15-49). K = 0: A-weighted frozen mixture; K = 1: Wood.
+//  rho_c2_Mixture_…H:15-49). K = 0 without covolume: sum_k alpha_k A_k rho_k c_k^2 = (AP+1) p + BP,
+// so c^2 = ((AP+1) p + BP)/(AP rho); K = 1 (Wood) and NASG use the general sums.
 __device__ __forceinline__ double sound_speed(const KParams& kp, double p, double a1, double a1r1,
-                                              double a2r2, int& bad)
+                                              double a2r2, double rho, double AP, double BP, int& bad)
 {
-    const double r1c1 = ((kp.A[0] + 1.0) * p + kp.B[0]) / kp.A[0];
-    const double r2c2 = ((kp.A[1] + 1.0) * p + kp.B[1]) / kp.A[1];
+    if (kp.source_term == 0 && !kp.nasg) return sqrt(((AP + 1.0) * p + BP) / (AP * rho));
+    const double g1 = (kp.A[0] + 1.0) * p + kp.B[0];
+    const double g2 = (kp.A[1] + 1.0) * p + kp.B[1];
     const double a2 = 1.0 - a1;
     double phi1 = a1, phi2 = a2;
     if (kp.nasg) {
         phi1 = nasg_phi(a1, a1r1, kp.C[0], bad);
         phi2 = nasg_phi(a2, a2r2, kp.C[1], bad);
     }
-    double rho_c2;
-    if (kp.source_term == 0) {
-        const double num = a1 * kp.A[0] * r1c1 + a2 * kp.A[1] * r2c2;
-        const double AP = phi1 * kp.A[0] + phi2 * kp.A[1];
-        rho_c2 = num / AP;
-    } else {
-        rho_c2 = 1.0 / (phi1 / r1c1 + phi2 / r2c2);
-    }
-    return sqrt(rho_c2 / (a1r1 + a2r2));
+    if (kp.source_term == 0) return sqrt((a1 * g1 + a2 * g2) / (AP * rho));
+    // Wood: rho c^2 = 1/(phi1 A1/g1 + phi2 A2/g2)
+    return sqrt(g1 * g2 / ((phi1 * kp.A[0] * g2 + phi2 * kp.A[1] * g1) * rho));
 }
 
 // Kapila coefficient phi1 (1 + K1) of the alpha1 equation's div(u) term
@@ -127,76 +137,128 @@ template <int DIM>
 __device__ __forceinline__ double nc_coefficient(const KParams& kp, const double* U, int& bad)
 {
     const double a1 = U[3 + DIM];
-    if (kp.source_term == 0) return a1 * (1.0 + 0.0);
+    if (kp.source_term == 0) return a1;
     double W[4 + DIM];
     cons2prim<DIM>(kp, U, W, bad);
     const double p = W[2 + DIM];
-    const double r1c1 = ((kp.A[0] + 1.0) * p + kp.B[0]) / kp.A[0];
-    const double r2c2 = ((kp.A[1] + 1.0) * p + kp.B[1]) / kp.A[1];
+    const double g1 = (kp.A[0] + 1.0) * p + kp.B[0];   // A1 rho1 c1^2
+    const double g2 = (kp.A[1] + 1.0) * p + kp.B[1];
     double phi1 = a1, phi2 = 1.0 - a1;
     if (kp.nasg) {
         phi1 = nasg_phi(a1, U[0], kp.C[0], bad);
         phi2 = nasg_phi(1.0 - a1, U[1], kp.C[1], bad);
     }
-    const double rho_c2 = 1.0 / (phi1 / r1c1 + phi2 / r2c2);
-    const double K1 = rho_c2 / r1c1 - 1.0;
-    return phi1 * (1.0 + K1);
+    // phi1 (1 + K1) = phi1 rho c^2 / (rho1 c1^2) with Wood's rho c^2
+    return phi1 * (kp.A[0] * g2) / (phi1 * kp.A[0] * g2 + phi2 * kp.A[1] * g1);
 }
 
-// Ratio-form WENO5 (include/Operators_FV/WENO5.H:12-75, Reconstruction2Face_WENO5_D.H:12-37):
-// p = 2, no epsilon, NaN ratios replaced by 1. FACE = +1: value at the high face of cell c.
+// ---- WENO5 (include/Operators_FV/WENO5.H:12-75, Reconstruction2Face_WENO5_D.H:12-37) ------------
+
+// Candidates and smoothness indicators of one side. FACE = +1: high face of cell c.
+template <int FACE>
+__device__ __forceinline__ void weno5_parts(double f_mm, double f_m, double f_c, double f_p, double f_pp,
+                                            double* f, double* IS)
+{
+    if (FACE == 1) {
+        f[0] = f_c + (2.0 / 3.0) * (f_p - f_c) - (1.0 / 6.0) * (f_pp - f_p);
+        f[1] = f_c + (1.0 / 6.0) * (f_c - f_m) + (1.0 / 3.0) * (f_p - f_c);
+        f[2] = f_c - (1.0 / 3.0) * (f_m - f_mm) + (5.0 / 6.0) * (f_c - f_m);
+    } else {
+        f[0] = f_c - (5.0 / 6.0) * (f_p - f_c) + (1.0 / 3.0) * (f_pp - f_p);
+        f[1] = f_c - (1.0 / 3.0) * (f_c - f_m) - (1.0 / 6.0) * (f_p - f_c);
+        f[2] = f_c + (1.0 / 6.0) * (f_m - f_mm) - (2.0 / 3.0) * (f_c - f_m);
+    }
+    const double a = f_p - f_c, b = f_pp - f_p, c = f_c - f_m, e = f_m - f_mm;
+    IS[0] = (10.0 / 3.0) * a * a - (11.0 / 3.0) * a * b + (4.0 / 3.0) * b * b;
+    IS[1] = -(5.0 / 3.0) * c * a + (4.0 / 3.0) * c * c + (4.0 / 3.0) * a * a;
+    IS[2] = (10.0 / 3.0) * c * c - (11.0 / 3.0) * e * c + (4.0 / 3.0) * e * e;
+}
+
+// 1/x from the hardware reciprocal estimate and two Newton steps (relative error a few 1e-16,
+// against ten instructions for an IEEE division). Used where the result only scales weights.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+#if CMP_FAST_RCP
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
+// Nonlinear weights. The reference's ratio form d_k / sum_j (IS_k/IS_j)^2 d_j with 0/0 -> 1 equals
+// d_k b_k / sum_j d_j b_j with b_k = prod_{j != k} IS_j^2; the IS are first scaled by a power of two
+// (exact) so the products neither overflow nor underflow. When that sum vanishes (two or three IS
+// equal to zero, or ratios beyond 1e-150) the ratio form itself is evaluated.
+template <int FACE>
+__device__ __forceinline__ void weno5_weights(const double* IS, double* w)
+{
+    constexpr double d0 = (FACE == 1) ? 3.0 / 10.0 : 1.0 / 10.0;
+    constexpr double d2 = (FACE == 1) ? 1.0 / 10.0 : 3.0 / 10.0;
+    constexpr double d1 = 1.0 - d0 - d2;
+    const double M = smax(smax(IS[0], IS[1]), IS[2]);
+    if (M == 0.0) {
+        // locally constant data: every ratio is 0/0 -> 1, i.e. the linear weights d_k / sum d
+        constexpr double den = (d0 + d1) + d2;
+        w[0] = d0 / den;
+        w[1] = d1 / den;
+        w[2] = d2 / den;
+        return;
+    }
+    int e = 0;
+    (void)frexp(M, &e);
+    const double r0 = ldexp(IS[0], -e), r1 = ldexp(IS[1], -e), r2 = ldexp(IS[2], -e);
+    const double q01 = r0 * r1, q02 = r0 * r2, q12 = r1 * r2;
+    const double b0 = q12 * q12, b1 = q02 * q02, b2 = q01 * q01;
+    const double S = d0 * b0 + d1 * b1 + d2 * b2;
+    if (S > 1e-290) {
+        const double inv = rcp_nr(S);
+        w[0] = d0 * b0 * inv;
+        w[1] = d1 * b1 * inv;
+        w[2] = d2 * b2 * inv;
+        return;
+    }
+    double IS01 = IS[0] / IS[1];
+    if (__builtin_isnan(IS01)) IS01 = 1.0;
+    double IS02 = IS[0] / IS[2];
+    if (__builtin_isnan(IS02)) IS02 = 1.0;
+    double IS12 = IS[1] / IS[2];
+    if (__builtin_isnan(IS12)) IS12 = 1.0;
+    const double IS10 = 1.0 / IS01, IS20 = 1.0 / IS02, IS21 = 1.0 / IS12;
+    w[0] = d0 / (d0 + (IS01 * IS01) * d1 + (IS02 * IS02) * d2);
+    w[1] = d1 / ((IS10 * IS10) * d0 + d1 + (IS12 * IS12) * d2);
+    w[2] = d2 / ((IS20 * IS20) * d0 + (IS21 * IS21) * d1 + d2);
+}
+
 template <int FACE>
 __device__ __forceinline__ double weno5(double f_mm, double f_m, double f_c, double f_p, double f_pp)
 {
-    double f0, f1, f2, d0, d2;
-    if (FACE == 1) {
-        f0 = f_c + (2.0 / 3.0) * (f_p - f_c) - (1.0 / 6.0) * (f_pp - f_p);
-        f1 = f_c + (1.0 / 6.0) * (f_c - f_m) + (1.0 / 3.0) * (f_p - f_c);
-        f2 = f_c - (1.0 / 3.0) * (f_m - f_mm) + (5.0 / 6.0) * (f_c - f_m);
-        d0 = 3.0 / 10.0;
-        d2 = 1.0 / 10.0;
-    } else {
-        f0 = f_c - (5.0 / 6.0) * (f_p - f_c) + (1.0 / 3.0) * (f_pp - f_p);
-        f1 = f_c - (1.0 / 3.0) * (f_c - f_m) - (1.0 / 6.0) * (f_p - f_c);
-        f2 = f_c + (1.0 / 6.0) * (f_m - f_mm) - (2.0 / 3.0) * (f_c - f_m);
-        d0 = 1.0 / 10.0;
-        d2 = 3.0 / 10.0;
-    }
-    const double d1 = 1.0 - d0 - d2;
-    const double a = f_p - f_c, b = f_pp - f_p, c = f_c - f_m, e = f_m - f_mm;
-    const double IS0 = (10.0 / 3.0) * a * a - (11.0 / 3.0) * a * b + (4.0 / 3.0) * b * b;
-    const double IS1 = -(5.0 / 3.0) * c * a + (4.0 / 3.0) * c * c + (4.0 / 3.0) * a * a;
-    const double IS2 = (10.0 / 3.0) * c * c - (11.0 / 3.0) * e * c + (4.0 / 3.0) * e * e;
-    double IS01 = IS0 / IS1;
-    if (__builtin_isnan(IS01)) IS01 = 1.0;
-    double IS02 = IS0 / IS2;
-    if (__builtin_isnan(IS02)) IS02 = 1.0;
-    double IS12 = IS1 / IS2;
-    if (__builtin_isnan(IS12)) IS12 = 1.0;
-    const double IS10 = 1.0 / IS01, IS20 = 1.0 / IS02, IS21 = 1.0 / IS12;
-    const double w0 = d0 / (d0 + (IS01 * IS01) * d1 + (IS02 * IS02) * d2);
-    const double w1 = d1 / ((IS10 * IS10) * d0 + d1 + (IS12 * IS12) * d2);
-    const double w2 = d2 / ((IS20 * IS20) * d0 + (IS21 * IS21) * d1 + d2);
-    return w0 * f0 + w1 * f1 + w2 * f2;
+    double f[3], IS[3], w[3];
+    weno5_parts<FACE>(f_mm, f_m, f_c, f_p, f_pp, f, IS);
+    weno5_weights<FACE>(IS, w);
+    return w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
 }
 
 // Slope_TVD + Reconstruction2Face_MUSCL_D + Reconstruction2Face_Linear
 // (include/Operators_FV/Slope_TVD.H:12-43, Reconstruction2Face_MUSCL_D.H:12-36, Limiter_*.H).
-__device__ __forceinline__ double muscl(double f_m, double f_c, double f_p, double dx, int face,
-                                        int limiter)
+__device__ __forceinline__ double muscl(double f_m, double f_c, double f_p, double dx, double idx,
+                                        int face, int limiter)
 {
-    const double sm = (f_c - f_m) / dx;
-    const double sc = (f_p - f_m) / (2.0 * dx);
-    const double sp = (f_p - f_c) / dx;
+    const double sm = (f_c - f_m) * idx;
+    const double sp = (f_p - f_c) * idx;
     double s = 0.0;
     if (limiter == 1) {
         s = smin(smax(sm, 0.0), smax(sp, 0.0)) + smax(smin(sm, 0.0), smin(sp, 0.0));
     } else if (limiter == 2) {
-        s = (sm * sp > 0.0) ? 2.0 / (1.0 / sm + 1.0 / sp) : 0.0;
+        s = (sm * sp > 0.0) ? 2.0 * sm * sp / (sm + sp) : 0.0;
     } else if (limiter == 3) {
         const double am = fabs(sm), ap = fabs(sp);
         s = (dsign(sm) + dsign(sp)) * smin(smin(am, ap), 0.5 * smax(am, ap));
     } else if (limiter == 4) {
+        const double sc = (f_p - f_m) * (0.5 * idx);
         const double hi = smin(smin(smax(sc, 0.0), smax(2.0 * sm, 0.0)), smax(2.0 * sp, 0.0));
         const double lo = smax(smax(smin(sc, 0.0), smin(2.0 * sm, 0.0)), smin(2.0 * sp, 0.0));
         s = hi + lo;
@@ -204,102 +266,308 @@ __device__ __forceinline__ double muscl(double f_m, double f_c, double f_p, doub
     return f_c + double(face) * 0.5 * s * dx;
 }
 
-// One face's Riemann problem from conservative L/R states. HLLC follows
-// Riemann_HLLC (include/Physics_Compressible5Eq/Riemann_HLLC_Compressible5Eq.H:16-194) but forms
-// only the star state of the side sign(S*) selects (both halves when S* == 0); HLL follows
-// Riemann_HLL (Riemann_HLL_Compressible5Eq.H:16-133). The states are taken through cons->prim
-// first, as the reference does. Returns u*_n; F receives N fluxes; cmax the face wave speed.
+// ---- WENO5B (include/Operators_Compressible5Eq/FVM_Reconstruction2Face_Conservative_WENO5B_D_Compressible5Eq.H:15-320)
+// fC_QuadratureRule + Limiter_BoundMin with f_min = 0 (include/Operators_FV/fC_QuadratureRule.H,
+// Limiter_BoundMin.H): theta = max(0, f_c)/max(0, f_c - min(fC, fL, fR)), at most 1.
+__device__ __forceinline__ double theta_min0(double f_c, double fL, double fR, double W, double iw)
+{
+    const double fC = (f_c - W * fL - W * fR) * iw;
+    const double mf = smin(smin(fC, fL), fR);
+    const double th = smax(0.0, f_c) / smax(0.0, f_c - mf);
+    return smin(1.0, th);
+}
+
+// Bounded face values of the phase masses and alpha1 of the middle cell of a 5-cell stencil of
+// primitives (s(t, n), t = 0..4). FACE selects which face is returned; both faces enter the limiter.
+template <int DIM, int FACE, typename S>
+__device__ __forceinline__ void weno5b_bounded(const KParams& kp, const S& s, double& a1r1F,
+                                               double& a2r2F, double& a1F)
+{
+    constexpr int IA = 3 + DIM;
+    const double W = kp.weno5b_weight, iw = 1.0 / (1.0 - 2.0 * W);
+    double arL[2], alL[2], arR[2], alR[2];
+    double sumL = 0.0, sumR = 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double v[5], va[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            v[t] = s(t, k);
+            va[t] = (k == 0) ? s(t, IA) : 1.0 - s(t, IA);
+        }
+        double f[3], IS[3], w[3], fa[3], ISa[3];
+        weno5_parts<1>(v[0], v[1], v[2], v[3], v[4], f, IS);
+        weno5_weights<1>(IS, w);
+        arL[k] = w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
+        weno5_parts<1>(va[0], va[1], va[2], va[3], va[4], fa, ISa);
+        alL[k] = w[0] * fa[0] + w[1] * fa[1] + w[2] * fa[2];
+        weno5_parts<-1>(v[0], v[1], v[2], v[3], v[4], f, IS);
+        weno5_weights<-1>(IS, w);
+        arR[k] = w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
+        weno5_parts<-1>(va[0], va[1], va[2], va[3], va[4], fa, ISa);
+        alR[k] = w[0] * fa[0] + w[1] * fa[1] + w[2] * fa[2];
+        const double th = smin(theta_min0(v[2], arL[k], arR[k], W, iw),
+                               theta_min0(va[2], alL[k], alR[k], W, iw));
+        arL[k] = v[2] + th * (arL[k] - v[2]);
+        alL[k] = va[2] + th * (alL[k] - va[2]);
+        arR[k] = v[2] + th * (arR[k] - v[2]);
+        alR[k] = va[2] + th * (alR[k] - va[2]);
+        sumL += alL[k];
+        sumR += alR[k];
+    }
+    const double isL = 1.0 / sumL, isR = 1.0 / sumR;
+    double theta = 1.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double c_ar = s(2, k);
+        const double c_al = (k == 0) ? s(2, IA) : 1.0 - s(2, IA);
+        arL[k] *= isL; alL[k] *= isL;
+        arR[k] *= isR; alR[k] *= isR;
+        theta = smin(smin(theta, theta_min0(c_ar, arL[k], arR[k], W, iw)),
+                     theta_min0(c_al, alL[k], alR[k], W, iw));
+    }
+    const double c1 = s(2, 0), c2 = s(2, 1), ca = s(2, IA);
+    const double r1 = (FACE == 1) ? arL[0] : arR[0];
+    const double r2 = (FACE == 1) ? arL[1] : arR[1];
+    const double ra = (FACE == 1) ? alL[0] : alR[0];
+    a1r1F = c1 + theta * (r1 - c1);
+    a2r2F = c2 + theta * (r2 - c2);
+    a1F = ca + theta * (ra - ca);
+}
+
+// ---- THINC (include/Operators_Compressible5Eq/FVM_Reconstruction2Face_Conservative_THINC_K_Compressible5Eq.H:70-192,
+//      include/Operators_FV/Reconstruction2Face_THINC_D.H:13-35) --------------------------------------
+__device__ __forceinline__ double thinc_face(double beta, double sigma, double abar, int face_hi)
+{
+    const double a = face_hi ? 0.0 : 1.0;
+    const double b = face_hi ? 1.0 : -1.0;
+    const double e2a = exp(2.0 * beta * abar);
+    const double arg_ln = (e2a - 1.0) / (exp(2.0 * beta) - e2a);
+    const double c = 0.5 / beta * log(arg_ln);
+    const double af = 0.5 * (1.0 + tanh(beta * (a + 0.5 * b * (sigma + 1.0) + c)));
+    const double lo = (0.0 < af) ? af : 0.0;
+    return (lo < 1.0) ? lo : 1.0;
+}
+
+// Face state of a THINC side (Reconstruct_State_From_VolumeFraction, :16-64): the phase densities,
+// velocity and pressure of the base face state W, alpha1 replaced by a1n (primitive form).
+template <int DIM>
+__device__ __forceinline__ void thinc_state(double* W, double a1n)
+{
+    const double a1 = W[3 + DIM];
+    const double rho1 = W[0] / a1;
+    const double rho2 = W[1] / (1.0 - a1);
+    W[0] = a1n * rho1;
+    W[1] = (1.0 - a1n) * rho2;
+    W[3 + DIM] = a1n;
+}
+
+// ---- Riemann solvers from face primitives ---------------------------------------------------------
+// HLLC: Riemann_HLLC (include/Physics_Compressible5Eq/Riemann_HLLC_Compressible5Eq.H:16-194), only
+// the star state of the side sign(S*) selects (both halves when S* == 0). HLL: Riemann_HLL
+// (Riemann_HLL_Compressible5Eq.H:16-133). Conservative2Flux (Conservative2Flux_Compressible5Eq.H:15-80)
+// gives the physical fluxes. Returns u*_n; F gets N fluxes; cmax the face wave speed (:164).
+template <int DIM>
+struct FaceSide {
+    double W[4 + DIM];
+    double U[4 + DIM];
+    double rho, AP, BP, c;
+};
+
+template <int DIM>
+__device__ __forceinline__ double hllc_half(const FaceSide<DIM>& s, int dir, double S, double Sb,
+                                            double S_Star, double* F)
+{
+    const double V = s.W[2 + dir];
+    const double p = s.W[2 + DIM];
+    const double inv = 1.0 / (S - S_Star);
+    const double q = (S - V) * inv;
+    F[0] = s.W[0] * V + Sb * (q * s.U[0] - s.U[0]);
+    F[1] = s.W[1] * V + Sb * (q * s.U[1] - s.U[1]);
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+        double Fm = s.U[2 + m] * V;
+        double Cs = q * s.U[2 + m];
+        if (m == dir) {
+            Fm += p;
+            Cs += q * s.rho * (S_Star - V);
+        }
+        F[2 + m] = Fm + Sb * (Cs - s.U[2 + m]);
+    }
+    const double CsE = q * s.U[2 + DIM] + (S_Star - V) * (q * s.rho * S_Star + p * inv);
+    F[2 + DIM] = (s.U[2 + DIM] + p) * V + Sb * (CsE - s.U[2 + DIM]);
+    F[3 + DIM] = s.W[3 + DIM] * V + Sb * (q * s.U[3 + DIM] - s.U[3 + DIM]);
+    return V + Sb * (q - 1.0);
+}
+
 template <int DIM, int RIEMANN>
-__device__ __forceinline__ double riemann(const KParams& kp, const double* UL, const double* UR,
-                                          int dir, double* F, double& cmax, int& bad)
+__device__ __forceinline__ double riemann(const FaceSide<DIM>& L, const FaceSide<DIM>& R, int dir,
+                                          double* F, double& cmax)
 {
     constexpr int N = 4 + DIM;
-    double WL[N], WR[N];
-    cons2prim<DIM>(kp, UL, WL, bad);
-    cons2prim<DIM>(kp, UR, WR, bad);
-    const double pL = WL[2 + DIM], pR = WR[2 + DIM];
-    const double cL = sound_speed(kp, pL, WL[3 + DIM], WL[0], WL[1], bad);
-    const double cR = sound_speed(kp, pR, WR[3 + DIM], WR[0], WR[1], bad);
-    const double VL = WL[2 + dir], VR = WR[2 + dir];
+    const double VL = L.W[2 + dir], VR = R.W[2 + dir];
     double qL = 0.0, qR = 0.0;
 #pragma unroll
     for (int m = 0; m < DIM; ++m) {
-        qL = (m == 0) ? WL[2] * WL[2] : qL + WL[2 + m] * WL[2 + m];
-        qR = (m == 0) ? WR[2] * WR[2] : qR + WR[2 + m] * WR[2 + m];
+        qL = (m == 0) ? L.W[2] * L.W[2] : qL + L.W[2 + m] * L.W[2 + m];
+        qR = (m == 0) ? R.W[2] * R.W[2] : qR + R.W[2 + m] * R.W[2 + m];
     }
-    cmax = smax(cL + sqrt(qL), cR + sqrt(qR));
-
+    cmax = smax(L.c + sqrt(qL), R.c + sqrt(qR));
     if (RIEMANN == 2) {
+        const double pL = L.W[2 + DIM], pR = R.W[2 + DIM];
         const double V_avg = 0.5 * (VL + VR);
-        const double c_avg = 0.5 * (cL + cR);
-        const double SL = smin(V_avg - c_avg, VL - cL);
-        const double SR = smax(V_avg + c_avg, VR + cR);
-        const double rhoL = WL[0] + WL[1];
-        const double rhoR = WR[0] + WR[1];
-        const double S_Star = (pR - pL + rhoL * VL * (SL - VL) - rhoR * VR * (SR - VR))
-                            / (rhoL * (SL - VL) - rhoR * (SR - VR));
-        // Physical flux of the upwind side (Conservative2Flux, Conservative2Flux_Compressible5Eq.H:15-80)
-        // plus S_K (U*_K - U_K); the star state scales U_K by q = (S_K - V_K)/(S_K - S*).
-        auto side = [&](const double* U, const double* W, double S, double V, double rho, double p,
-                        double Sb, double* out) -> double {
-            const double q = (S - V) / (S - S_Star);
-            out[0] = W[0] * V + Sb * (q * U[0] - U[0]);
-            out[1] = W[1] * V + Sb * (q * U[1] - U[1]);
+        const double c_avg = 0.5 * (L.c + R.c);
+        const double SL = smin(V_avg - c_avg, VL - L.c);
+        const double SR = smax(V_avg + c_avg, VR + R.c);
+        const double S_Star = (pR - pL + L.rho * VL * (SL - VL) - R.rho * VR * (SR - VR))
+                            / (L.rho * (SL - VL) - R.rho * (SR - VR));
+        if (S_Star > 0.0) return hllc_half<DIM>(L, dir, SL, smin(0.0, SL), S_Star, F);
+        if (S_Star < 0.0) return hllc_half<DIM>(R, dir, SR, smax(0.0, SR), S_Star, F);
+        double FR_[N];
+        const double uL = hllc_half<DIM>(L, dir, SL, smin(0.0, SL), S_Star, F);
+        const double uR = hllc_half<DIM>(R, dir, SR, smax(0.0, SR), S_Star, FR_);
 #pragma unroll
-            for (int m = 0; m < DIM; ++m) {
-                double Fm = U[2 + m] * V;
-                double Cs = q * U[2 + m];
-                if (m == dir) {
-                    Fm += p;
-                    Cs += q * rho * (S_Star - V);
-                }
-                out[2 + m] = Fm + Sb * (Cs - U[2 + m]);
-            }
-            const double CsE = q * U[2 + DIM] + q * (S_Star - V) * (rho * S_Star + p / (S - V));
-            out[2 + DIM] = (U[2 + DIM] + p) * V + Sb * (CsE - U[2 + DIM]);
-            out[3 + DIM] = W[3 + DIM] * V + Sb * (q * U[3 + DIM] - U[3 + DIM]);
-            return Sb * (q - 1.0);
-        };
-        double un;
-        if (S_Star > 0.0) {
-            const double Sm = smin(0.0, SL);
-            un = VL + side(UL, WL, SL, VL, rhoL, pL, Sm, F);
-        } else if (S_Star < 0.0) {
-            const double Sp = smax(0.0, SR);
-            un = VR + side(UR, WR, SR, VR, rhoR, pR, Sp, F);
-        } else {
-            double FR_[N];
-            const double Sm = smin(0.0, SL), Sp = smax(0.0, SR);
-            const double dL = side(UL, WL, SL, VL, rhoL, pL, Sm, F);
-            const double dR = side(UR, WR, SR, VR, rhoR, pR, Sp, FR_);
-#pragma unroll
-            for (int n = 0; n < N; ++n) F[n] = 0.5 * F[n] + 0.5 * FR_[n];
-            un = (0.5 * VL + 0.5 * VR) + (0.5 * dL + 0.5 * dR);
-        }
-        return un;
+        for (int n = 0; n < N; ++n) F[n] = 0.5 * F[n] + 0.5 * FR_[n];
+        return 0.5 * uL + 0.5 * uR;
     } else {
-        const double Sp = smax(smax(0.0, VL + cL), VR + cR);
-        const double Sm = smin(smin(0.0, VL - cL), VR - cR);
+        const double Sp = smax(smax(0.0, VL + L.c), VR + R.c);
+        const double Sm = smin(smin(0.0, VL - L.c), VR - R.c);
         const double inv = 1.0 / (Sp - Sm);
-        double FL[N], FR[N];
-        FL[0] = WL[0] * VL; FR[0] = WR[0] * VR;
-        FL[1] = WL[1] * VL; FR[1] = WR[1] * VR;
+        const double pL = L.W[2 + DIM], pR = R.W[2 + DIM];
 #pragma unroll
-        for (int m = 0; m < DIM; ++m) {
-            FL[2 + m] = UL[2 + m] * VL;
-            FR[2 + m] = UR[2 + m] * VR;
+        for (int n = 0; n < N; ++n) {
+            double FL, FR;
+            if (n < 2 || n == 3 + DIM) { FL = L.W[n] * VL; FR = R.W[n] * VR; }
+            else if (n == 2 + DIM) { FL = (L.U[n] + pL) * VL; FR = (R.U[n] + pR) * VR; }
+            else {
+                FL = L.U[n] * VL + ((n == 2 + dir) ? pL : 0.0);
+                FR = R.U[n] * VR + ((n == 2 + dir) ? pR : 0.0);
+            }
+            F[n] = (Sp * FL - Sm * FR + Sp * Sm * (R.U[n] - L.U[n])) * inv;
         }
-        FL[2 + dir] += pL;
-        FR[2 + dir] += pR;
-        FL[2 + DIM] = (UL[2 + DIM] + pL) * VL;
-        FR[2 + DIM] = (UR[2 + DIM] + pR) * VR;
-        FL[3 + DIM] = WL[3 + DIM] * VL;
-        FR[3 + DIM] = WR[3 + DIM] * VR;
-#pragma unroll
-        for (int n = 0; n < N; ++n)
-            F[n] = (Sp * FL[n] - Sm * FR[n] + Sp * Sm * (UR[n] - UL[n])) * inv;
         return (Sp * VL - Sm * VR) * inv;
     }
+}
+
+// Primitive face state -> the side record the Riemann solver reads.
+template <int DIM>
+__device__ __forceinline__ void make_side(const KParams& kp, FaceSide<DIM>& s, int& bad)
+{
+    prim2cons<DIM>(kp, s.W, s.U, s.AP, s.BP, s.rho, bad);
+    s.c = sound_speed(kp, s.W[2 + DIM], s.W[3 + DIM], s.W[0], s.W[1], s.rho, s.AP, s.BP, bad);
+}
+
+// ---- one face --------------------------------------------------------------------------------------
+// FVM_Conservative2FluxHyperbolic_K (include/Operators_Compressible5Eq/FVM_Conservative2FluxHyperbolic_K_Compressible5Eq.H:15-170)
+// for the face between stencil cells G-1 and G: s(t, n) returns primitive n of stencil cell t
+// (t = 0..2G-1, cell G-1 is the low side). SCHEME: 0 Godunov, 1 MUSCL, 5 WENO5, 6 WENO5B. With
+// THINC, normal() returns the face normal's component along dir (from the volume fraction); it is
+// evaluated only when the volume-fraction gates already admit THINC on a side.
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, typename S, typename NRM>
+__device__ __forceinline__ double face_flux(const KParams& kp, int limiter, int dir, const S& s,
+                                            const NRM& normal, double* F, double& cmax, int& bad)
+{
+    constexpr int N = 4 + DIM;
+    constexpr int G = (SCHEME >= 5) ? 3 : (SCHEME == 1 ? 2 : 1);
+    constexpr int IA = 3 + DIM;
+    const double dx = kp.dx[dir], idx = kp.inv_dx[dir];
+    FaceSide<DIM> L, R;
+    bool done = false;
+    if (THINC) {
+        // stencil cells mm, m, c, p of the face = G-2 .. G+1
+        const double amm = s(G - 2, IA), am = s(G - 1, IA), ac = s(G, IA), ap = s(G + 1, IA);
+        const bool gL = (ac - am) * (am - amm) > 0.0;
+        const bool gR = (ap - ac) * (ac - am) > 0.0;
+        const bool sym = (amm == ap) && (am == ac) && (am != amm);
+        const double eps = kp.thinc_eps;
+        const bool cL = (am > eps && am < 1.0 - eps) && (gL || sym);
+        const bool cR = (ac > eps && ac < 1.0 - eps) && (gR || sym);
+        double nd = 0.0;
+        bool gate = false;
+        if (cL || cR) {
+            nd = normal();
+            gate = (!kp.thinc_use_normal) || (fabs(nd) > kp.thinc_thr);
+        }
+        const bool iL = cL && gate;
+        const bool iR = cR && gate;
+        if (iL || iR) {
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+                L.W[n] = muscl(s(G - 2, n), s(G - 1, n), s(G, n), dx, idx, 1, 1);
+                R.W[n] = muscl(s(G - 1, n), s(G, n), s(G + 1, n), dx, idx, -1, 1);
+            }
+            const double beta = kp.thinc_use_normal ? (kp.thinc_beta * fabs(nd) + 0.01) : kp.thinc_beta;
+            if (iL) thinc_state<DIM>(L.W, thinc_face(beta, (ac - amm > 0.0) ? 1.0 : -1.0, am, 1));
+            if (iR) thinc_state<DIM>(R.W, thinc_face(beta, (ap - am > 0.0) ? 1.0 : -1.0, ac, 0));
+            done = true;
+        }
+    }
+    if (!done) {
+        if (SCHEME == 5 || SCHEME == 6) {
+            constexpr int n0 = (SCHEME == 6) ? 2 : 0;
+            constexpr int n1 = (SCHEME == 6) ? 2 + DIM : N - 1;
+#pragma unroll
+            for (int n = n0; n <= n1; ++n) {
+                const double v0 = s(0, n), v1 = s(1, n), v2 = s(2, n), v3 = s(3, n), v4 = s(4, n), v5 = s(5, n);
+                L.W[n] = weno5<1>(v0, v1, v2, v3, v4);
+                R.W[n] = weno5<-1>(v1, v2, v3, v4, v5);
+            }
+            if (SCHEME == 6) {
+                auto sl = [&](int t, int n) { return s(t, n); };
+                auto sr = [&](int t, int n) { return s(t + 1, n); };
+                weno5b_bounded<DIM, 1>(kp, sl, L.W[0], L.W[1], L.W[IA]);
+                weno5b_bounded<DIM, -1>(kp, sr, R.W[0], R.W[1], R.W[IA]);
+            }
+        } else if (SCHEME == 1) {
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+                L.W[n] = muscl(s(0, n), s(1, n), s(2, n), dx, idx, 1, limiter);
+                R.W[n] = muscl(s(1, n), s(2, n), s(3, n), dx, idx, -1, limiter);
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+                L.W[n] = s(0, n);
+                R.W[n] = s(1, n);
+            }
+        }
+    }
+    make_side<DIM>(kp, L, bad);
+    make_side<DIM>(kp, R, bad);
+    return riemann<DIM, RIEMANN>(L, R, dir, F, cmax);
+}
+
+// Normal component along dir of the volume-fraction gradient at a face
+// (FDM_Normal_{X,Y,Z}FaceFromCenter_D, include/Operators_FD/FDM_Normal_XFaceFromCenter_D.H:12-56,
+//  with FDM_Gradient*_*FaceFromCenter_D and Interpolation2Edge_Linear_D). a(o_d, o_t1, o_t2) returns
+// alpha1 at the cell offset by o_d along dir (0: high cell, -1: low cell) and o_t along the
+// transverse directions (ascending axis order).
+template <int DIM, typename A>
+__device__ __forceinline__ double face_normal_component(const KParams& kp, int dir, const A& a)
+{
+    const double fc = a(0, 0, 0), fm = a(-1, 0, 0);
+    const double gd = (fc - fm) * kp.inv_dx[dir];
+    double sum = gd * gd;
+    int tdim[2] = {0, 0};
+    int nt = 0;
+#pragma unroll
+    for (int t = 0; t < DIM; ++t)
+        if (t != dir) tdim[nt++] = t;
+#pragma unroll
+    for (int q = 0; q < DIM - 1; ++q) {
+        double edge[2];
+#pragma unroll
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            const int o = sgn ? 1 : -1;
+            const double C = (q == 0) ? a(0, o, 0) : a(0, 0, o);
+            const double D = (q == 0) ? a(-1, o, 0) : a(-1, 0, o);
+            edge[sgn] = (dir < tdim[q]) ? 0.25 * fc + 0.25 * fm + 0.25 * C + 0.25 * D
+                                        : 0.25 * fc + 0.25 * C + 0.25 * fm + 0.25 * D;
+        }
+        const double g = (edge[1] - edge[0]) * kp.inv_dx[tdim[q]];
+        sum += g * g;
+    }
+    return (sum == 0.0) ? 0.0 : gd / sqrt(sum);
 }
 
 }  // namespace cmp

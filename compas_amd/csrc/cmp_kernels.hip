@@ -1,18 +1,19 @@
 // cmp_kernels.hip - gfx950 kernels of the five-equation RHS, RK3 stage, ghost fill and CFL.
 //
 // Built twice, with CMP_KERNEL_DIM=2 and CMP_KERNEL_DIM=3, so that the two dimensions compile in
-// parallel; cmp_dispatch.hip routes a call to the right instantiation.
+// parallel; cmp_capi.hip routes a call to the right instantiation.
 //
-// Stage kernel design (one workgroup = one tile of a box, 256 threads = 4 wave64):
+// Stage kernel (one workgroup = one tile of a box):
+//   with THINC: alpha1 of the tile plus one cell all round is staged in LDS once (face normals);
 //   for each direction d:
-//     1. stage the primitives of the tile plus G cells on both sides along d in LDS
-//        (cons->prim once per staged cell instead of once per stencil use),
-//     2. each face of the tile along d is computed once: WENO5/MUSCL/Godunov on the staged
-//        primitives, prim->cons, HLLC/HLL; flux and u*_n go to LDS,
-//     3. each cell accumulates (F_{+} - F_{-})/dx_d and (u*_{+} - u*_{-})/dx_d in registers;
+//     1. the primitives of the tile plus G cells on both sides along d are staged in LDS
+//        (cons->prim once per staged cell),
+//     2. every face of the tile along d is computed once (reconstruction, prim->cons, Riemann);
+//        flux and u*_n go to LDS,
+//     3. every cell accumulates (F_+ - F_-)/dx_d and (u*_+ - u*_-)/dx_d in registers;
 //   then dU = -div F, dU[alpha1] += phi1 (1+K1) div u*, and the RK3 combination writes the new
-//   state directly: dU never goes to HBM. The face wave speeds of the tile are max-reduced with
-//   wave shuffles and one 64-bit atomicMax per workgroup.
+//   state: dU never goes to HBM. The face wave speeds of the tile are max-reduced with wave
+//   shuffles and one 64-bit atomicMax per workgroup.
 #include "cmp_kernels.hpp"
 
 #ifndef CMP_KERNEL_DIM
@@ -21,14 +22,13 @@
 
 namespace cmp {
 
+// Tile shapes chosen so that every direction's face count fits one pass of the 256 threads:
+// 2D 16x15 (255 x-faces, 256 y-faces), 3D 8x8x3 (216, 216, 256).
 template <int DIM> struct Tile;
-template <> struct Tile<2> { static constexpr int X = 16, Y = 16, Z = 1; };
-template <> struct Tile<3> { static constexpr int X = 8, Y = 8, Z = 4; };
+template <> struct Tile<2> { static constexpr int X = 16, Y = 15, Z = 1; };
+template <> struct Tile<3> { static constexpr int X = 8, Y = 8, Z = 3; };
 
-template <int SCHEME> struct Stencil;
-template <> struct Stencil<0> { static constexpr int G = 1; };   // Godunov
-template <> struct Stencil<1> { static constexpr int G = 2; };   // MUSCL
-template <> struct Stencil<5> { static constexpr int G = 3; };   // WENO5
+template <int SCHEME> struct Stencil { static constexpr int G = (SCHEME >= 5) ? 3 : (SCHEME == 1 ? 2 : 1); };
 
 constexpr int NT = 256;
 
@@ -41,6 +41,7 @@ struct Lds {
                                       DIM == 3 ? TX * TY * (TZ + 2 * G) : 0);
     static constexpr int FMAX = cmax3((TX + 1) * TY * TZ, TX * (TY + 1) * TZ,
                                       DIM == 3 ? TX * TY * (TZ + 1) : 0);
+    static constexpr int AMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
 };
 
 __device__ __forceinline__ double wave_max(double v)
@@ -58,8 +59,126 @@ __device__ __forceinline__ int xcd_tile(int b, int nb)
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-template <int DIM, int SCHEME, int RIEMANN>
-__global__ __launch_bounds__(NT)
+__device__ __forceinline__ long long fab_index(const BoxDev& bx, int i, int j, int k)
+{
+    return (long long)(i - bx.flo[0]) + bx.sj * (j - bx.flo[1]) + bx.sk * (k - bx.flo[2]);
+}
+
+// What one workgroup's direction passes share.
+template <int DIM>
+struct StageCtx {
+    const KParams& kp;
+    int limiter;
+    const BoxDev& bx;
+    const double* __restrict__ Ub;
+    int tlo[3];
+    int n[3];
+    int ae[3];      // extents of the alpha1 halo region (THINC)
+    int tid, cx, cy, cz;
+    bool valid;
+};
+
+// One direction of the stage kernel (D known at compile time): stage the primitives of the tile
+// plus G cells on both sides along D, compute every face along D once, and accumulate the flux and
+// u*_n differences of the thread's cell.
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int D>
+__device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, double* sF,
+                                         const double* sA, double* acc, double& divu,
+                                         double& cmax_loc, int& bad)
+{
+    constexpr int N = 4 + DIM;
+    constexpr int G = Stencil<SCHEME>::G;
+    using L = Lds<DIM, G>;
+    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
+    const KParams& kp = cx_.kp;
+    const BoxDev& bx = cx_.bx;
+    const int tid = cx_.tid;
+    // region extents (tile plus G cells on both sides along D)
+    const int e0 = cx_.n[0] + (D == 0 ? 2 * G : 0);
+    const int e1 = cx_.n[1] + (D == 1 ? 2 * G : 0);
+    const int e2 = cx_.n[2] + (D == 2 ? 2 * G : 0);
+    const int R = e0 * e1 * e2;
+    // 1. primitives of the region, staged in LDS
+    for (int r = tid; r < R; r += NT) {
+        const int x = r % e0;
+        const int y = (r / e0) % e1;
+        const int z = r / (e0 * e1);
+        const int i = cx_.tlo[0] + x - (D == 0 ? G : 0);
+        const int j = cx_.tlo[1] + y - (D == 1 ? G : 0);
+        const int k = cx_.tlo[2] + z - (D == 2 ? G : 0);
+        const long long c = fab_index(bx, i, j, k);
+        double Uc[N], Wc[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) Uc[q] = cx_.Ub[c + q * bx.sn];
+        cons2prim<DIM>(kp, Uc, Wc, bad);
+#pragma unroll
+        for (int q = 0; q < N; ++q) sW[q * RMAX + r] = Wc[q];
+    }
+    __syncthreads();
+    // 2. faces along D
+    const int f0 = cx_.n[0] + (D == 0 ? 1 : 0);
+    const int f1 = cx_.n[1] + (D == 1 ? 1 : 0);
+    const int f2 = cx_.n[2] + (D == 2 ? 1 : 0);
+    const int F = f0 * f1 * f2;
+    const int rs = (D == 0) ? 1 : ((D == 1) ? e0 : e0 * e1);
+    for (int f = tid; f < F; f += NT) {
+        const int x = f % f0;
+        const int y = (f / f0) % f1;
+        const int z = f / (f0 * f1);
+        const int a = (D == 0) ? x : ((D == 1) ? y : z);
+        const int base = x + e0 * (y + e1 * z);
+        auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
+        // alpha1 halo coordinates of the face's high cell; transverse axes in ascending order
+        const int hx = x + 1, hy = y + 1, hz = (DIM == 3) ? z + 1 : 0;
+        const int a0 = cx_.ae[0], a1 = cx_.ae[1];
+        auto aa = [&](int od, int o1, int o2) -> double {
+            const int lo = (od == -1) ? 1 : 0;
+            int px = hx - (D == 0 ? lo : 0), py = hy - (D == 1 ? lo : 0), pz = hz - (D == 2 ? lo : 0);
+            if (D == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
+            else if (D == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
+            else { px += o1; py += o2; }
+            return sA[px + a0 * (py + a1 * pz)];
+        };
+        auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
+        double Fl[N], c;
+        const double un = face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, cx_.limiter, D, st, normal, Fl, c, bad);
+#pragma unroll
+        for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
+        sF[N * FMAX + f] = un;
+        if (a < cx_.n[D]) {
+            if (__builtin_isnan(c)) bad = CMP_ERR_NAN;
+            cmax_loc = smax(cmax_loc, c);
+        }
+    }
+    __syncthreads();
+    // 3. flux differences of this direction
+    if (cx_.valid) {
+        const int fm = cx_.cx + f0 * (cx_.cy + f1 * cx_.cz);
+        const int fp = fm + ((D == 0) ? 1 : ((D == 1) ? f0 : f0 * f1));
+        const double idx = kp.inv_dx[D];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const double t = (sF[q * FMAX + fp] - sF[q * FMAX + fm]) * idx;
+            acc[q] = (D == 0) ? t : acc[q] + t;
+        }
+        const double t = (sF[N * FMAX + fp] - sF[N * FMAX + fm]) * idx;
+        divu = (D == 0) ? t : divu + t;
+    }
+    __syncthreads();
+}
+
+// Minimum resident waves per SIMD the register allocator must leave room for (0: compiler's choice).
+#ifndef CMP_STAGE_WAVES
+#define CMP_STAGE_WAVES 2
+#endif
+#if CMP_STAGE_WAVES > 0
+#define CMP_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(CMP_STAGE_WAVES)))
+#else
+#define CMP_STAGE_ATTR
+#endif
+
+template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+__global__ __launch_bounds__(NT) CMP_STAGE_ATTR
 void k_stage(KParams kp, int limiter, int mode, int stage,
              const TileDev* __restrict__ tiles, int ntiles, const BoxDev* __restrict__ boxes,
              BoxDev sbox, int sntx, int snty, int single,
@@ -67,40 +186,37 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
              const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err)
 {
     constexpr int N = 4 + DIM;
+    constexpr int IA = 3 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
     using L = Lds<DIM, G>;
     constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
-    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
+    constexpr int RMAX = L::RMAX, FMAX = L::FMAX, AMAX = THINC ? L::AMAX : 1;
 
     __shared__ double sW[N * RMAX];
     __shared__ double sF[(N + 1) * FMAX];
+    __shared__ double sA[AMAX];
     __shared__ double sRed[NT / 64];
 
     const int tid = threadIdx.x;
-    int bidx;
     int tlo[3];
-    const BoxDev* bxp;
+    BoxDev bx;
     if (single) {
         int b = blockIdx.x;
         const int tx = b % sntx;
         b /= sntx;
         const int ty = b % snty;
         const int tz = b / snty;
-        bxp = &sbox;
+        bx = sbox;
         tlo[0] = sbox.lo[0] + tx * TX;
         tlo[1] = sbox.lo[1] + ty * TY;
         tlo[2] = sbox.lo[2] + (DIM == 3 ? tz * TZ : 0);
-        bidx = 0;
     } else {
-        const int t = xcd_tile(blockIdx.x, ntiles);
-        const TileDev td = tiles[t];
-        bidx = td.box;
-        bxp = &boxes[bidx];
+        const TileDev td = tiles[xcd_tile(blockIdx.x, ntiles)];
+        bx = boxes[td.box];
         tlo[0] = td.lo[0];
         tlo[1] = td.lo[1];
         tlo[2] = td.lo[2];
     }
-    const BoxDev bx = *bxp;
     int n[3];
     n[0] = min(TX, bx.hi[0] - tlo[0] + 1);
     n[1] = min(TY, bx.hi[1] - tlo[1] + 1);
@@ -117,96 +233,33 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
 #pragma unroll
     for (int q = 0; q < N; ++q) acc[q] = 0.0;
 
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        int e[3] = {n[0], n[1], n[2]};
-        e[d] += 2 * G;
-        const int R = e[0] * e[1] * e[2];
-        // 1. primitives of the region, staged in LDS
-        for (int r = tid; r < R; r += NT) {
-            const int x = r % e[0];
-            const int y = (r / e[0]) % e[1];
-            const int z = r / (e[0] * e[1]);
-            const int i = tlo[0] + x - (d == 0 ? G : 0);
-            const int j = tlo[1] + y - (d == 1 ? G : 0);
-            const int k = tlo[2] + z - (d == 2 ? G : 0);
-            const long long c = (long long)(i - bx.flo[0]) + bx.sj * (j - bx.flo[1]) + bx.sk * (k - bx.flo[2]);
-            double Uc[N], Wc[N];
-#pragma unroll
-            for (int q = 0; q < N; ++q) Uc[q] = Ub[c + q * bx.sn];
-            cons2prim<DIM>(kp, Uc, Wc, bad);
-#pragma unroll
-            for (int q = 0; q < N; ++q) sW[q * RMAX + r] = Wc[q];
+    // alpha1 of the tile plus one cell all round (face normals of THINC)
+    int ae[3] = {n[0] + 2, n[1] + 2, DIM == 3 ? n[2] + 2 : 1};
+    if (THINC) {
+        const int RA = ae[0] * ae[1] * ae[2];
+        for (int r = tid; r < RA; r += NT) {
+            const int x = r % ae[0], y = (r / ae[0]) % ae[1], z = r / (ae[0] * ae[1]);
+            const int k = (DIM == 3) ? tlo[2] + z - 1 : tlo[2];
+            sA[r] = Ub[fab_index(bx, tlo[0] + x - 1, tlo[1] + y - 1, k) + IA * bx.sn];
         }
-        __syncthreads();
-        // 2. faces along d
-        int fe[3] = {n[0], n[1], n[2]};
-        fe[d] += 1;
-        const int F = fe[0] * fe[1] * fe[2];
-        const int rs = (d == 0) ? 1 : ((d == 1) ? e[0] : e[0] * e[1]);
-        for (int f = tid; f < F; f += NT) {
-            const int x = f % fe[0];
-            const int y = (f / fe[0]) % fe[1];
-            const int z = f / (fe[0] * fe[1]);
-            const int a = (d == 0) ? x : ((d == 1) ? y : z);
-            const int base = x + e[0] * (y + e[1] * z);
-            double WL[N], WR[N];
-#pragma unroll
-            for (int q = 0; q < N; ++q) {
-                const double* s = sW + q * RMAX + base;
-                if (SCHEME == 5) {
-                    const double v0 = s[0], v1 = s[rs], v2 = s[2 * rs], v3 = s[3 * rs], v4 = s[4 * rs],
-                                 v5 = s[5 * rs];
-                    WL[q] = weno5<1>(v0, v1, v2, v3, v4);
-                    WR[q] = weno5<-1>(v1, v2, v3, v4, v5);
-                } else if (SCHEME == 1) {
-                    const double v0 = s[0], v1 = s[rs], v2 = s[2 * rs], v3 = s[3 * rs];
-                    WL[q] = muscl(v0, v1, v2, kp.dx[d], 1, limiter);
-                    WR[q] = muscl(v1, v2, v3, kp.dx[d], -1, limiter);
-                } else {
-                    WL[q] = s[0];
-                    WR[q] = s[rs];
-                }
-            }
-            double UL[N], UR[N], Fl[N], c;
-            prim2cons<DIM>(kp, WL, UL, bad);
-            prim2cons<DIM>(kp, WR, UR, bad);
-            const double un = riemann<DIM, RIEMANN>(kp, UL, UR, d, Fl, c, bad);
-#pragma unroll
-            for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
-            sF[N * FMAX + f] = un;
-            if (a < n[d]) {
-                if (__builtin_isnan(c)) bad = CMP_ERR_NAN;
-                cmax_loc = smax(cmax_loc, c);
-            }
-        }
-        __syncthreads();
-        // 3. flux differences of this direction
-        if (valid) {
-            const int fm = cx + fe[0] * (cy + fe[1] * cz);
-            const int fs = (d == 0) ? 1 : ((d == 1) ? fe[0] : fe[0] * fe[1]);
-            const int fp = fm + fs;
-#pragma unroll
-            for (int q = 0; q < N; ++q) {
-                const double t = (sF[q * FMAX + fp] - sF[q * FMAX + fm]) / kp.dx[d];
-                acc[q] = (d == 0) ? t : acc[q] + t;
-            }
-            const double t = (sF[N * FMAX + fp] - sF[N * FMAX + fm]) / kp.dx[d];
-            divu = (d == 0) ? t : divu + t;
-        }
-        __syncthreads();
     }
+
+    StageCtx<DIM> ctx{kp, limiter, bx, Ub, {tlo[0], tlo[1], tlo[2]}, {n[0], n[1], n[2]},
+                      {ae[0], ae[1], ae[2]}, tid, cx, cy, cz, valid};
+    dir_pass<DIM, SCHEME, RIEMANN, THINC, 0>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
+    if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
+    if (DIM > 2) dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1)>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
 
     if (valid) {
         const int i = tlo[0] + cx, j = tlo[1] + cy, k = tlo[2] + cz;
-        const long long c = (long long)(i - bx.flo[0]) + bx.sj * (j - bx.flo[1]) + bx.sk * (k - bx.flo[2]);
+        const long long c = fab_index(bx, i, j, k);
         double Uc[N], dU[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) {
             Uc[q] = Ub[c + q * bx.sn];
             dU[q] = -acc[q];
         }
-        dU[3 + DIM] += nc_coefficient<DIM>(kp, Uc, bad) * divu;
+        dU[IA] += nc_coefficient<DIM>(kp, Uc, bad) * divu;
         if (mode == MODE_RK) {
             const double dt = *dtp;
             double* Ob = Uout + bx.off;
@@ -214,7 +267,7 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
             if (stage == 0) {
 #pragma unroll
                 for (int q = 0; q < N; ++q) {
-                    const double v = 1.0 * Uc[q] + dt * dU[q];
+                    const double v = Uc[q] + dt * dU[q];
                     if (__builtin_isnan(v)) bad = CMP_ERR_NAN;
                     Ob[c + q * bx.sn] = v;
                 }
@@ -252,26 +305,28 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
 
 // Face fluxes of one box in the reference's shape (one thread per face, every stencil cell taken
 // through cons->prim by the thread that uses it), for callers that need the face arrays (flux
-// registers, face averaging) and as a second GPU path to check the fused kernel against.
-template <int DIM, int SCHEME, int RIEMANN>
+// registers, face averaging) and as a second GPU path the fused kernel is checked against.
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int DIR>
 __global__ __launch_bounds__(NT)
-void k_faces(KParams kp, int limiter, BoxDev bx, int dir, const double* __restrict__ U,
+void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
              double* __restrict__ flux, double* __restrict__ fluxnc)
 {
     constexpr int N = 4 + DIM;
+    constexpr int IA = 3 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
-    int fe[3];
-    for (int m = 0; m < 3; ++m) fe[m] = (m < DIM) ? bx.hi[m] - bx.lo[m] + 1 : 1;
-    fe[dir] += 1;
-    const long long F = (long long)fe[0] * fe[1] * fe[2];
+    constexpr int dir = DIR;
+    const int fe0 = bx.hi[0] - bx.lo[0] + 1 + (DIR == 0 ? 1 : 0);
+    const int fe1 = bx.hi[1] - bx.lo[1] + 1 + (DIR == 1 ? 1 : 0);
+    const int fe2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + (DIR == 2 ? 1 : 0) : 1;
+    const long long F = (long long)fe0 * fe1 * fe2;
     const long long f = (long long)blockIdx.x * NT + threadIdx.x;
     if (f >= F) return;
-    const int x = (int)(f % fe[0]);
-    const int y = (int)((f / fe[0]) % fe[1]);
-    const int z = (int)(f / ((long long)fe[0] * fe[1]));
-    const int i = bx.lo[0] + x, j = bx.lo[1] + y, k = bx.lo[2] + z;
-    const long long sd = (dir == 0) ? 1 : ((dir == 1) ? bx.sj : bx.sk);
-    const long long c0 = (long long)(i - bx.flo[0]) + bx.sj * (j - bx.flo[1]) + bx.sk * (k - bx.flo[2]);
+    const int x = (int)(f % fe0);
+    const int y = (int)((f / fe0) % fe1);
+    const int z = (int)(f / ((long long)fe0 * fe1));
+    const int ci = bx.lo[0] + x, cj = bx.lo[1] + y, ck = bx.lo[2] + z;
+    const long long sd = (DIR == 0) ? 1 : ((DIR == 1) ? bx.sj : bx.sk);
+    const long long c0 = fab_index(bx, ci, cj, ck);
     const double* Ub = U + bx.off;
     double W[2 * G][N];
     int bad = 0;
@@ -283,24 +338,18 @@ void k_faces(KParams kp, int limiter, BoxDev bx, int dir, const double* __restri
         for (int q = 0; q < N; ++q) Uc[q] = Ub[c + q * bx.sn];
         cons2prim<DIM>(kp, Uc, W[t], bad);
     }
-    double WL[N], WR[N];
-#pragma unroll
-    for (int q = 0; q < N; ++q) {
-        if (SCHEME == 5) {
-            WL[q] = weno5<1>(W[0][q], W[1][q], W[2][q], W[3][q], W[4][q]);
-            WR[q] = weno5<-1>(W[1][q], W[2][q], W[3][q], W[4][q], W[5][q]);
-        } else if (SCHEME == 1) {
-            WL[q] = muscl(W[0][q], W[1][q], W[2][q], kp.dx[dir], 1, limiter);
-            WR[q] = muscl(W[1][q], W[2][q], W[3][q], kp.dx[dir], -1, limiter);
-        } else {
-            WL[q] = W[0][q];
-            WR[q] = W[1][q];
-        }
-    }
-    double UL[N], UR[N], Fl[N], cm;
-    prim2cons<DIM>(kp, WL, UL, bad);
-    prim2cons<DIM>(kp, WR, UR, bad);
-    const double un = riemann<DIM, RIEMANN>(kp, UL, UR, dir, Fl, cm, bad);
+    auto st = [&](int t, int q) -> double { return W[t][q]; };
+    auto aa = [&](int od, int o1, int o2) -> double {
+        const int lo = (od == -1) ? 1 : 0;
+        int px = ci - (DIR == 0 ? lo : 0), py = cj - (DIR == 1 ? lo : 0), pz = ck - (DIR == 2 ? lo : 0);
+        if (DIR == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
+        else if (DIR == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
+        else { px += o1; py += o2; }
+        return Ub[fab_index(bx, px, py, pz) + IA * bx.sn];
+    };
+    auto normal = [&]() -> double { return face_normal_component<DIM>(kp, DIR, aa); };
+    double Fl[N], cm;
+    const double un = face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, dir, st, normal, Fl, cm, bad);
     if (flux) {
 #pragma unroll
         for (int q = 0; q < N; ++q) flux[f + q * F] = Fl[q];
@@ -325,46 +374,51 @@ void k_fill(FillDev fd, long long nghost, double* U, int* err)
             if (fd.gprefix[mid] <= g) lo = mid; else hi = mid;
         }
         const int b = lo;
-        long long q = g - fd.gprefix[b];
         const BoxDev bx = fd.boxes[b];
         const int ng = fd.ng;
-        int nv[3] = {bx.hi[0] - bx.lo[0] + 1, bx.hi[1] - bx.lo[1] + 1, DIM == 3 ? bx.hi[2] - bx.lo[2] + 1 : 1};
-        const int ex = nv[0] + 2 * ng, ey = nv[1] + 2 * ng, ez = (DIM == 3) ? nv[2] + 2 * ng : 1;
+        const int nv1 = bx.hi[1] - bx.lo[1] + 1;
+        const int ex = bx.hi[0] - bx.lo[0] + 1 + 2 * ng, ey = nv1 + 2 * ng;
+        const int ez = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + 2 * ng : 1;
         int x, y, z;
-        long long r = q;
+        long long r = g - fd.gprefix[b];
+        bool slab = false;
         if (DIM == 3) {
             const long long A = (long long)ng * ex * ey;
-            if (r < A) {
-                z = (int)(r / ((long long)ex * ey)); r %= (long long)ex * ey;
-                y = (int)(r / ex); x = (int)(r % ex);
-                goto mapped;
-            }
+            const long long exy = (long long)ex * ey;
             if (r < 2 * A) {
-                r -= A;
-                z = ez - ng + (int)(r / ((long long)ex * ey)); r %= (long long)ex * ey;
-                y = (int)(r / ex); x = (int)(r % ex);
-                goto mapped;
+                // z-low (r < A) or z-high slab: whole xy planes
+                const bool high = r >= A;
+                if (high) r -= A;
+                z = (high ? ez - ng : 0) + (int)(r / exy);
+                r %= exy;
+                y = (int)(r / ex);
+                x = (int)(r % ex);
+                slab = true;
+            } else {
+                r -= 2 * A;
+                const long long Rg = 2LL * ng * ex + 2LL * ng * nv1;
+                z = ng + (int)(r / Rg);
+                r %= Rg;
             }
-            r -= 2 * A;
-            const long long Rg = 2LL * ng * ex + 2LL * ng * nv[1];
-            z = ng + (int)(r / Rg);
-            r %= Rg;
         } else {
             z = 0;
         }
-        if (r < (long long)ng * ex) {
-            y = (int)(r / ex); x = (int)(r % ex);
-        } else if (r < 2LL * ng * ex) {
-            r -= (long long)ng * ex;
-            y = ey - ng + (int)(r / ex); x = (int)(r % ex);
-        } else {
-            r -= 2LL * ng * ex;
-            y = ng + (int)(r / (2 * ng));
-            const int cc = (int)(r % (2 * ng));
-            x = (cc < ng) ? cc : (ex - 2 * ng + cc);
+        if (!slab) {
+            if (r < (long long)ng * ex) {
+                y = (int)(r / ex);
+                x = (int)(r % ex);
+            } else if (r < 2LL * ng * ex) {
+                r -= (long long)ng * ex;
+                y = ey - ng + (int)(r / ex);
+                x = (int)(r % ex);
+            } else {
+                r -= 2LL * ng * ex;
+                y = ng + (int)(r / (2 * ng));
+                const int cc = (int)(r % (2 * ng));
+                x = (cc < ng) ? cc : (ex - 2 * ng + cc);
+            }
         }
-    mapped:
-        int cell[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
+        const int cell[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
         int src[3] = {cell[0], cell[1], cell[2]};
         unsigned flip = 0u;
         int ok = 1;
@@ -399,8 +453,8 @@ void k_fill(FillDev fd, long long nghost, double* U, int* err)
         const int own = fd.owner[bi];
         if (own < 0) { atomicMax(err, CMP_ERR_LAYOUT); continue; }
         const BoxDev so = fd.boxes[own];
-        const long long si = so.off + (src[0] - so.flo[0]) + so.sj * (src[1] - so.flo[1]) + so.sk * (src[2] - so.flo[2]);
-        const long long di = bx.off + (cell[0] - bx.flo[0]) + bx.sj * (cell[1] - bx.flo[1]) + bx.sk * (cell[2] - bx.flo[2]);
+        const long long si = so.off + fab_index(so, src[0], src[1], src[2]);
+        const long long di = bx.off + fab_index(bx, cell[0], cell[1], cell[2]);
 #pragma unroll
         for (int c = 0; c < N; ++c) {
             double v = U[si + c * so.sn];
@@ -411,8 +465,8 @@ void k_fill(FillDev fd, long long nghost, double* U, int* err)
 }
 
 // ComputeDt on the device (source/COMPAS.cpp:2250-2302, EstTimeStep
-// source/Class_Compressible_PhaseField.cpp:4473-4499). One thread.
-// step[0] counts the steps of the current advance call, step[1] is the capacity of hist.
+// source/Class_Compressible_PhaseField.cpp:4473-4499). One thread. step[0] counts the steps of the
+// current advance call, step[1] is the capacity of hist.
 template <int DIM>
 __global__ void k_compute_dt(double dx0, double dx1, double dx2, double* cmax, double* dt, double cfl,
                              int limiter, double* hist, int* step, int* err)
@@ -457,8 +511,8 @@ hipError_t launch_set2(int* p, int a, int b, hipStream_t s)
 
 // ---- launchers for this dimension ---------------------------------------------------------------
 
-template <int DIM, int SCHEME, int RIEMANN>
-static hipError_t stage3(const KParams& kp, int limiter, int mode, int stage, const TileDev* tiles,
+template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, const TileDev* tiles,
                          int ntiles, const BoxDev* boxes, const BoxDev* sbox, int sntiles,
                          const double* Us, const double* Un, double* Uout, const double* dt,
                          unsigned long long* cmax_bits, int* err, hipStream_t s)
@@ -473,11 +527,47 @@ static hipError_t stage3(const KParams& kp, int limiter, int mode, int stage, co
         grid = sntiles;
     }
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_stage<DIM, SCHEME, RIEMANN>), dim3(grid), dim3(NT), 0, s, kp, limiter, mode,
-                       stage, tiles, ntiles, boxes, sb, sntx, snty, single, Us, Un, Uout, dt,
+    hipLaunchKernelGGL((k_stage<DIM, SCHEME, RIEMANN, THINC>), dim3(grid), dim3(NT), 0, s, kp, limiter,
+                       mode, stage, tiles, ntiles, boxes, sb, sntx, snty, single, Us, Un, Uout, dt,
                        cmax_bits, err);
     return hipGetLastError();
 }
+
+template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+static hipError_t faces4(const KParams& kp, int limiter, const BoxDev& box, int dir, const double* U,
+                         double* flux, double* fluxnc, hipStream_t s)
+{
+    long long F = 1;
+    for (int m = 0; m < DIM; ++m) F *= (box.hi[m] - box.lo[m] + 1 + (m == dir ? 1 : 0));
+    const int grid = (int)((F + NT - 1) / NT);
+    if (dir == 0)
+        hipLaunchKernelGGL((k_faces<DIM, SCHEME, RIEMANN, THINC, 0>), dim3(grid), dim3(NT), 0, s, kp,
+                           limiter, box, U, flux, fluxnc);
+    else if (dir == 1)
+        hipLaunchKernelGGL((k_faces<DIM, SCHEME, RIEMANN, THINC, 1>), dim3(grid), dim3(NT), 0, s, kp,
+                           limiter, box, U, flux, fluxnc);
+    else
+        hipLaunchKernelGGL((k_faces<DIM, SCHEME, RIEMANN, THINC, (DIM == 3 ? 2 : 1)>), dim3(grid),
+                           dim3(NT), 0, s, kp, limiter, box, U, flux, fluxnc);
+    return hipGetLastError();
+}
+
+// Scheme x Riemann x THINC dispatch shared by the two launchers.
+#define CMP_DISPATCH(FN, ...)                                                         \
+    do {                                                                              \
+        const bool th = lc.thinc != 0;                                                \
+        if (lc.riemann == 2) {                                                        \
+            if (lc.scheme == 5) return th ? FN<DIM, 5, 2, true>(__VA_ARGS__) : FN<DIM, 5, 2, false>(__VA_ARGS__); \
+            if (lc.scheme == 6) return th ? FN<DIM, 6, 2, true>(__VA_ARGS__) : FN<DIM, 6, 2, false>(__VA_ARGS__); \
+            if (lc.scheme == 1 && !th) return FN<DIM, 1, 2, false>(__VA_ARGS__);      \
+            if (lc.scheme == 0 && !th) return FN<DIM, 0, 2, false>(__VA_ARGS__);      \
+        } else {                                                                      \
+            if (lc.scheme == 5) return th ? FN<DIM, 5, 1, true>(__VA_ARGS__) : FN<DIM, 5, 1, false>(__VA_ARGS__); \
+            if (lc.scheme == 6) return th ? FN<DIM, 6, 1, true>(__VA_ARGS__) : FN<DIM, 6, 1, false>(__VA_ARGS__); \
+            if (lc.scheme == 1 && !th) return FN<DIM, 1, 1, false>(__VA_ARGS__);      \
+            if (lc.scheme == 0 && !th) return FN<DIM, 0, 1, false>(__VA_ARGS__);      \
+        }                                                                             \
+    } while (0)
 
 template <int DIM>
 hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, int stage,
@@ -486,18 +576,8 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
                             double* Uout, const double* dt, unsigned long long* cmax_bits, int* err,
                             hipStream_t s)
 {
-#define CMP_STAGE(SCH, RIE) \
-    return stage3<DIM, SCH, RIE>(kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout, dt, cmax_bits, err, s)
-    if (lc.riemann == 2) {
-        if (lc.scheme == 5) CMP_STAGE(5, 2);
-        if (lc.scheme == 1) CMP_STAGE(1, 2);
-        if (lc.scheme == 0) CMP_STAGE(0, 2);
-    } else {
-        if (lc.scheme == 5) CMP_STAGE(5, 1);
-        if (lc.scheme == 1) CMP_STAGE(1, 1);
-        if (lc.scheme == 0) CMP_STAGE(0, 1);
-    }
-#undef CMP_STAGE
+    CMP_DISPATCH(stage4, kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout,
+                 dt, cmax_bits, err, s);
     return hipErrorInvalidValue;
 }
 
@@ -505,25 +585,7 @@ template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
                             const double* U, double* flux, double* fluxnc, hipStream_t s)
 {
-    long long F = 1;
-    for (int m = 0; m < DIM; ++m) F *= (box.hi[m] - box.lo[m] + 1 + (m == dir ? 1 : 0));
-    const int grid = (int)((F + NT - 1) / NT);
-#define CMP_FACES(SCH, RIE)                                                                          \
-    {                                                                                                \
-        hipLaunchKernelGGL((k_faces<DIM, SCH, RIE>), dim3(grid), dim3(NT), 0, s, kp, lc.limiter, box, \
-                           dir, U, flux, fluxnc);                                                    \
-        return hipGetLastError();                                                                    \
-    }
-    if (lc.riemann == 2) {
-        if (lc.scheme == 5) CMP_FACES(5, 2);
-        if (lc.scheme == 1) CMP_FACES(1, 2);
-        if (lc.scheme == 0) CMP_FACES(0, 2);
-    } else {
-        if (lc.scheme == 5) CMP_FACES(5, 1);
-        if (lc.scheme == 1) CMP_FACES(1, 1);
-        if (lc.scheme == 0) CMP_FACES(0, 1);
-    }
-#undef CMP_FACES
+    CMP_DISPATCH(faces4, kp, lc.limiter, box, dir, U, flux, fluxnc, s);
     return hipErrorInvalidValue;
 }
 
@@ -546,6 +608,15 @@ hipError_t launch_dt_dim(const double* dx, double* cmax, double* dt, double cfl,
     return hipGetLastError();
 }
 
+template <int DIM>
+void tile_dims_dim(int* tx, int* ty, int* tz)
+{
+    *tx = Tile<DIM>::X;
+    *ty = Tile<DIM>::Y;
+    *tz = Tile<DIM>::Z;
+}
+
+template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
 template hipError_t launch_stage_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, int, int,
                                                      const TileDev*, int, const BoxDev*, const BoxDev*,
                                                      int, const double*, const double*, double*,

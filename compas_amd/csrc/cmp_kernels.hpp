@@ -41,12 +41,14 @@ struct FillDev {
 
 enum StageMode { MODE_RK = 0, MODE_RHS = 1, MODE_CMAX = 2 };
 
-// Scheme selector: 0 Godunov, 1 MUSCL (limiter from id_limiter), 5 WENO5.
+// Scheme selector: 0 Godunov, 1 MUSCL (limiter from id_limiter), 5 WENO5, 6 WENO5B; THINC on top
+// of WENO5 or WENO5B.
 struct LaunchCfg {
     int dim;
     int scheme;
     int limiter;
     int riemann;
+    int thinc;
 };
 
 hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int stage,

@@ -51,9 +51,11 @@ def load():
         "ref_weno5": (D, [D, D, D, D, D, ctypes.c_int]),
         "ref_weno5_weights": (None, [D, D, D, D, D, ctypes.c_int, PD, PD]),
         "ref_muscl": (D, [D, D, D, D, ctypes.c_int, ctypes.c_int]),
+        "ref_weno5b_side": (None, [PP, PD, ctypes.c_int, PD]),
+        "ref_thinc": (D, [D, D, D, ctypes.c_int]),
         "ref_riemann": (None, [PP, PD, PD, ctypes.c_int, PD, PD, PD, PI]),
-        "ref_face_flux": (ctypes.c_int, [PP, PB, ctypes.c_int, PD, ctypes.c_int, ctypes.c_int,
-                                         ctypes.c_int, ctypes.c_int, D, PD, PD, PD]),
+        "ref_face_flux": (ctypes.c_int, [PP, PG, PB, ctypes.c_int, PD, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, PD, PD, PD]),
         "ref_rhs_box": (ctypes.c_int, [PP, PG, PB, ctypes.c_int, PD, PD, PD,
                                        ctypes.POINTER(PD), ctypes.POINTER(PD)]),
         "ref_level_fill_ghosts": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD]),
@@ -128,6 +130,19 @@ def weno5_weights(v, face):
 
 def muscl(fm, fc, fp, dx, face, limiter):
     return load().ref_muscl(float(fm), float(fc), float(fp), float(dx), int(face), int(limiter))
+
+
+def weno5b_side(params, W5, face):
+    """WENO5B primitive face state of the middle cell of W5 (5 x nstate primitives)."""
+    W = np.zeros((5, 7))
+    W[:, :params.nstate] = np.asarray(W5, dtype=np.float64)
+    out = np.zeros(7)
+    load().ref_weno5b_side(ctypes.byref(params), _dptr(W), int(face), _dptr(out))
+    return out[:params.nstate]
+
+
+def thinc(beta, sigma, alpha_bar, face):
+    return load().ref_thinc(float(beta), float(sigma), float(alpha_bar), int(face))
 
 
 def riemann(params, UL, UR, direction):

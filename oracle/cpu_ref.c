@@ -257,6 +257,87 @@ double ref_muscl(double f_m, double f_c, double f_p, double dx, int id_face, int
     return f_c + (double)id_face * 0.5 * s * dx;
 }
 
+/* fC_QuadratureRule + Limiter_BoundMin (include/Operators_FV/fC_QuadratureRule.H:12-19,
+ * Limiter_BoundMin.H:12-22), called through Limiter_Bound with ID_BoundType = -1 (Limiter_Bound.H). */
+static double ref_theta_min(double f_min, double f_c, double fL_fp, double fR_fm, double Weight)
+{
+    double fC = (f_c - Weight * fL_fp - Weight * fR_fm) / (1.0 - 2.0 * Weight);
+    double min_f = smin(smin(fC, fL_fp), fR_fm);
+    double th = smax(0.0, f_c - f_min) / smax(0.0, f_c - min_f);
+    return smin(1.0, th);
+}
+
+/* FVM_Reconstruction2Face_Conservative_WENO5B_D, primitive part
+ * (include/Operators_Compressible5Eq/FVM_Reconstruction2Face_Conservative_WENO5B_D_Compressible5Eq.H:15-320):
+ * W[5][N] are the primitives of cells mm..pp; both faces of cell c are reconstructed for the
+ * phase masses and volume fractions (alpha with the weights of alpha*rho), limited for positivity,
+ * normalised by the sum of the volume fractions, limited again; velocity and pressure by WENO5. */
+void ref_weno5b_side(const cmp_params* P, double W[][REF_NMAX], int id_face, double* Wf)
+{
+    const int d = P->dim;
+    double ar[2][5], al[2][5];
+    for (int s = 0; s < 5; ++s) {
+        ar[0][s] = W[s][0];
+        ar[1][s] = W[s][1];
+        al[0][s] = W[s][3 + d];
+        al[1][s] = 1.0 - W[s][3 + d];
+    }
+    double Weight = (P->id_poly == 2) ? 1.0 / 6.0 : 1.0 / 12.0;
+    double arL[2], alL[2], arR[2], alR[2];
+    double sumL = 0.0, sumR = 0.0;
+    double cand[3], wr[3], cand2[3], wdummy[3];
+    for (int k = 0; k < 2; ++k) {
+        ref_weno5_weights(ar[k][0], ar[k][1], ar[k][2], ar[k][3], ar[k][4], 1, cand, wr);
+        arL[k] = 0.0; for (int s = 0; s < 3; ++s) arL[k] += wr[s] * cand[s];
+        ref_weno5_weights(al[k][0], al[k][1], al[k][2], al[k][3], al[k][4], 1, cand2, wdummy);
+        alL[k] = 0.0; for (int s = 0; s < 3; ++s) alL[k] += wr[s] * cand2[s];
+        ref_weno5_weights(ar[k][0], ar[k][1], ar[k][2], ar[k][3], ar[k][4], -1, cand, wr);
+        arR[k] = 0.0; for (int s = 0; s < 3; ++s) arR[k] += wr[s] * cand[s];
+        ref_weno5_weights(al[k][0], al[k][1], al[k][2], al[k][3], al[k][4], -1, cand2, wdummy);
+        alR[k] = 0.0; for (int s = 0; s < 3; ++s) alR[k] += wr[s] * cand2[s];
+        double th_ar = ref_theta_min(0.0, ar[k][2], arL[k], arR[k], Weight);
+        double th_al = ref_theta_min(0.0, al[k][2], alL[k], alR[k], Weight);
+        double th = smin(th_ar, th_al);
+        arL[k] = ar[k][2] + th * (arL[k] - ar[k][2]);
+        alL[k] = al[k][2] + th * (alL[k] - al[k][2]);
+        arR[k] = ar[k][2] + th * (arR[k] - ar[k][2]);
+        alR[k] = al[k][2] + th * (alR[k] - al[k][2]);
+        sumL += alL[k];
+        sumR += alR[k];
+    }
+    double theta = 1.0;
+    for (int k = 0; k < 2; ++k) {
+        arL[k] /= sumL; alL[k] /= sumL;
+        arR[k] /= sumR; alR[k] /= sumR;
+        double th_ar = ref_theta_min(0.0, ar[k][2], arL[k], arR[k], Weight);
+        double th_al = ref_theta_min(0.0, al[k][2], alL[k], alR[k], Weight);
+        theta = smin(smin(theta, th_ar), th_al);
+    }
+    double a1r1F = (id_face == 1) ? arL[0] : arR[0];
+    double a2r2F = (id_face == 1) ? arL[1] : arR[1];
+    double a1F = (id_face == 1) ? alL[0] : alR[0];
+    Wf[0] = ar[0][2] + theta * (a1r1F - ar[0][2]);
+    Wf[1] = ar[1][2] + theta * (a2r2F - ar[1][2]);
+    Wf[3 + d] = al[0][2] + theta * (a1F - al[0][2]);
+    for (int n = 2; n <= 2 + d; ++n)
+        Wf[n] = ref_weno5(W[0][n], W[1][n], W[2][n], W[3][n], W[4][n], id_face);
+}
+
+/* Reconstruction2Face_THINC_D (include/Operators_FV/Reconstruction2Face_THINC_D.H:13-35):
+ * id_face 1 gives the high face of the cell, anything else the low face. */
+double ref_thinc(double beta, double sigma, double alpha_bar, int id_face)
+{
+    double a = (id_face == 1) ? 0.0 : 1.0;
+    double b = (id_face == 1) ? 1.0 : -1.0;
+    double arg_ln = (exp(2.0 * beta * alpha_bar) - 1.0) / (exp(2.0 * beta) - exp(2.0 * beta * alpha_bar));
+    double c = 0.5 / beta * log(arg_ln);
+    double arg_tanh = beta * (a + 0.5 * b * (sigma + 1.0) + c);
+    double af = 0.5 * (1.0 + tanh(arg_tanh));
+    /* amrex::min(1.0, amrex::max(0.0, af)) */
+    af = (0.0 < af) ? af : 0.0;
+    return (af < 1.0) ? af : 1.0;
+}
+
 /* ------------------------------------------------------------------------------------------------
  * Riemann solvers
  * ---------------------------------------------------------------------------------------------- */
@@ -351,58 +432,196 @@ static inline long long fab_idx(const fab_t* f, int i, int j, int k)
 
 long long cmp_fab_size(int dim, const cmp_box* box, int ng, int ncomp);
 
-/* FVM_Conservative2FluxHyperbolic_K, ID_Bound == 0 branch without THINC
+/* Normal of the volume fraction at the face (i,j,k) of direction dir (between cells c - e_dir and
+ * c): FDM_Normal_{X,Y,Z}FaceFromCenter_{K,D} (include/Operators_FD/FDM_Normal_XFaceFromCenter_D.H:12-56
+ * and its Y/Z twins) with FDM_Gradient*_*FaceFromCenter_D and Interpolation2Edge_Linear_D: the normal
+ * derivative from the two cells, each transverse derivative from the two edge averages of four
+ * cells (listed in axis order), normalised unless all derivatives are zero. */
+static void ref_face_normal(const cmp_params* P, const cmp_geometry* G, const fab_t* f,
+                            const double* U, int i, int j, int k, int dir, double* nrm)
+{
+    const int dim = P->dim;
+    const long long sa = (long long)(3 + dim) * f->sn;
+    int c[3] = {i, j, k};
+    int e[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#define AL(x, y, z) U[sa + fab_idx(f, (x), (y), (z))]
+    double g[3] = {0.0, 0.0, 0.0};
+    double fc = AL(c[0], c[1], c[2]);
+    double fm = AL(c[0] - e[dir][0], c[1] - e[dir][1], c[2] - e[dir][2]);
+    g[dir] = (fc - fm) / G->dx[dir];
+    for (int t = 0; t < dim; ++t) {
+        if (t == dir) continue;
+        double edge[2];
+        for (int s = 0; s < 2; ++s) {
+            int sg = s ? 1 : -1;
+            double A = fc;
+            double B = fm;
+            double C = AL(c[0] + sg * e[t][0], c[1] + sg * e[t][1], c[2] + sg * e[t][2]);
+            double D = AL(c[0] - e[dir][0] + sg * e[t][0], c[1] - e[dir][1] + sg * e[t][1],
+                          c[2] - e[dir][2] + sg * e[t][2]);
+            if (dir < t) edge[s] = 0.25 * A + 0.25 * B + 0.25 * C + 0.25 * D;
+            else         edge[s] = 0.25 * A + 0.25 * C + 0.25 * B + 0.25 * D;
+        }
+        g[t] = (edge[1] - edge[0]) / G->dx[t];
+    }
+#undef AL
+    double sum = 0.0;
+    for (int m = 0; m < dim; ++m) sum += g[m] * g[m];
+    for (int m = 0; m < dim; ++m) nrm[m] = (sum == 0.0) ? 0.0 : g[m] / sqrt(sum);
+}
+
+/* Reconstruct_State_From_VolumeFraction
+ * (include/Operators_Compressible5Eq/FVM_Reconstruction2Face_Conservative_THINC_K_Compressible5Eq.H:16-64):
+ * the face state keeps the phase densities, velocity and pressure of Uin and takes alpha1 = a1n. */
+static void ref_state_from_vf(const cmp_params* P, const double* Uin, double a1n, double* Uo,
+                              int* err)
+{
+    const int d = P->dim;
+    double W[REF_NMAX];
+    ref_cons2prim(P, Uin, W, err);
+    double a1 = W[3 + d], p = W[2 + d];
+    double rho1 = W[0] / a1;
+    double rho2 = W[1] / (1.0 - a1);
+    double a1r1n = a1n * rho1;
+    double a2r2n = (1.0 - a1n) * rho2;
+    double rho_n = a1r1n + a2r2n;
+    double ke = (d == 3) ? (W[2] * W[2] + W[3] * W[3] + W[4] * W[4]) : (W[2] * W[2] + W[3] * W[3]);
+    double K = 0.5 * ke;
+    double r1e1 = P->eos_A[0] * p + P->eos_B[0] + rho1 * P->eos_D[0];
+    double r2e2 = P->eos_A[1] * p + P->eos_B[1] + rho2 * P->eos_D[1];
+    if (P->eos_C[0] > 0.0) {
+        double xi = 1.0 - P->eos_C[0] * rho1;
+        if (!(xi > 0.0)) ERR_SET(err, CMP_ERR_COVOLUME);
+        r1e1 = xi * (P->eos_A[0] * p + P->eos_B[0]) + rho1 * P->eos_D[0];
+    }
+    if (P->eos_C[1] > 0.0) {
+        double xi = 1.0 - P->eos_C[1] * rho2;
+        if (!(xi > 0.0)) ERR_SET(err, CMP_ERR_COVOLUME);
+        r2e2 = xi * (P->eos_A[1] * p + P->eos_B[1]) + rho2 * P->eos_D[1];
+    }
+    Uo[0] = a1r1n;
+    Uo[1] = a2r2n;
+    for (int m = 0; m < d; ++m) Uo[2 + m] = W[2 + m] * rho_n;
+    Uo[2 + d] = K * rho_n + r1e1 * a1n + r2e2 * (1.0 - a1n);
+    Uo[3 + d] = a1n;
+}
+
+/* SignNZ (include/Tools/Sign.H:27-37) */
+static inline double ref_signnz(double x) { return (x > 0.0) ? 1.0 : -1.0; }
+
+/* FVM_Conservative2FluxHyperbolic_K, ID_Bound == 0 branch
  * (include/Operators_Compressible5Eq/FVM_Conservative2FluxHyperbolic_K_Compressible5Eq.H:15-170):
  * reconstruction dispatch FVM_Reconstruction2FaceLR_Conservative_K (…FaceLR_Conservative_K…H:15-82),
  * the per-side primitive reconstruction FVM_Reconstruction2Face_Conservative_{WENO5,MUSCL,Godunov}_D
  * (cons->prim of every stencil cell, scalar reconstruction per primitive, prim->cons), the Riemann
  * solver, and FluxNC[0] = u*_n (:165-168). L uses cells i-3..i+1 (id_face +1), R uses i-2..i+2 (-1). */
-int ref_face_flux(const cmp_params* P, const cmp_box* box, int ng, const double* U,
-                  int i, int j, int k, int dir, double dx_dir,
-                  double* flux, double* fluxnc, double* cmax)
+/* FVM_Reconstruction2FaceLR_Conservative_K (…FaceLR_Conservative_K_Compressible5Eq.H:15-82) with
+ * the per-side primitive reconstructions FVM_Reconstruction2Face_Conservative_{WENO5,WENO5B,MUSCL,
+ * Godunov}_D: cons->prim of every stencil cell of the side, scalar reconstruction per primitive,
+ * prim->cons. Ust holds the conservative states of cells -3..+2 relative to the face's high cell;
+ * L uses cells -3..+1 (id_face +1), R cells -2..+2 (id_face -1). */
+static int ref_recon_lr(const cmp_params* P, double Ust[6][REF_NMAX], int id_weno, int id_limiter,
+                        double dx_dir, double* UL, double* UR, int* err)
 {
     const int N = P->nstate;
+    double Wst[6][REF_NMAX], WLf[REF_NMAX], WRf[REF_NMAX];
+    if (id_weno == 1 && (id_limiter == 5 || id_limiter == 6)) {
+        for (int s = -3; s <= 1; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        if (id_limiter == 5) {
+            for (int n = 0; n < N; ++n)
+                WLf[n] = ref_weno5(Wst[0][n], Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], 1);
+        } else {
+            ref_weno5b_side(P, Wst, 1, WLf);
+        }
+        ref_prim2cons(P, WLf, UL, err);
+        /* R: the cons->prim of the stencil cells is recomputed, as the reference does */
+        for (int s = -2; s <= 2; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        if (id_limiter == 5) {
+            for (int n = 0; n < N; ++n)
+                WRf[n] = ref_weno5(Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], Wst[5][n], -1);
+        } else {
+            ref_weno5b_side(P, &Wst[1], -1, WRf);
+        }
+        ref_prim2cons(P, WRf, UR, err);
+    } else if (id_weno == 0 && id_limiter >= 1 && id_limiter <= 4) {
+        for (int s = -2; s <= 0; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        for (int n = 0; n < N; ++n)
+            WLf[n] = ref_muscl(Wst[1][n], Wst[2][n], Wst[3][n], dx_dir, 1, id_limiter);
+        ref_prim2cons(P, WLf, UL, err);
+        for (int s = -1; s <= 1; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        for (int n = 0; n < N; ++n)
+            WRf[n] = ref_muscl(Wst[2][n], Wst[3][n], Wst[4][n], dx_dir, -1, id_limiter);
+        ref_prim2cons(P, WRf, UR, err);
+    } else if (id_weno == 0 && id_limiter == 0) {
+        ref_cons2prim(P, Ust[2], WLf, err);
+        ref_prim2cons(P, WLf, UL, err);
+        ref_cons2prim(P, Ust[3], WRf, err);
+        ref_prim2cons(P, WRf, UR, err);
+    } else {
+        return CMP_ERR_UNSUPPORTED;
+    }
+    return CMP_OK;
+}
+
+/* FVM_Conservative2FluxHyperbolic_K, ID_Bound == 0 branch
+ * (include/Operators_Compressible5Eq/FVM_Conservative2FluxHyperbolic_K_Compressible5Eq.H:15-170):
+ * reconstruction (base scheme, or THINC, …THINC_K_Compressible5Eq.H:70-192), the Riemann solver and
+ * FluxNC[0] = u*_n (:165-168). (i,j,k) is the face between cells c - e_dir and c = (i,j,k). */
+int ref_face_flux(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
+                  const double* U, int i, int j, int k, int dir,
+                  double* flux, double* fluxnc, double* cmax)
+{
+    const int N = P->nstate, d = P->dim;
+    const double dx_dir = G->dx[dir];
     int err = 0;
     fab_t f = make_fab(P->dim, box, ng);
     int off[3] = {0, 0, 0};
     off[dir] = 1;
-    /* stencil cells s = -3..+2 relative to the face's high cell */
-    double Wst[6][REF_NMAX], Ust[6][REF_NMAX];
+    double Ust[6][REF_NMAX];
     int smin_ = -3, smax_ = 2;
-    if (P->id_weno == 0 && P->id_limiter == 0) { smin_ = -1; smax_ = 0; }
+    if (P->id_weno == 0 && P->id_limiter == 0 && !P->id_thinc) { smin_ = -1; smax_ = 0; }
     else if (P->id_weno == 0) { smin_ = -2; smax_ = 1; }
     for (int s = smin_; s <= smax_; ++s) {
         long long c = fab_idx(&f, i + s * off[0], j + s * off[1], k + s * off[2]);
         for (int n = 0; n < N; ++n) Ust[s + 3][n] = U[c + n * f.sn];
     }
-    double UL[REF_NMAX], UR[REF_NMAX], WLf[REF_NMAX], WRf[REF_NMAX];
-    if (P->id_weno == 1 && P->id_limiter == 5) {
-        /* L: cons->prim of cells -3..+1 (FVM_Reconstruction2Face_Conservative_WENO5_D_Compressible5Eq.H:25-44) */
-        for (int s = -3; s <= 1; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
-        for (int n = 0; n < N; ++n)
-            WLf[n] = ref_weno5(Wst[0][n], Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], 1);
-        ref_prim2cons(P, WLf, UL, &err);
-        /* R: cons->prim of cells -2..+2 (recomputed, as the reference does) */
-        for (int s = -2; s <= 2; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
-        for (int n = 0; n < N; ++n)
-            WRf[n] = ref_weno5(Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], Wst[5][n], -1);
-        ref_prim2cons(P, WRf, UR, &err);
-    } else if (P->id_weno == 0 && P->id_limiter >= 1 && P->id_limiter <= 4) {
-        for (int s = -2; s <= 0; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
-        for (int n = 0; n < N; ++n)
-            WLf[n] = ref_muscl(Wst[1][n], Wst[2][n], Wst[3][n], dx_dir, 1, P->id_limiter);
-        ref_prim2cons(P, WLf, UL, &err);
-        for (int s = -1; s <= 1; ++s) ref_cons2prim(P, Ust[s + 3], Wst[s + 3], &err);
-        for (int n = 0; n < N; ++n)
-            WRf[n] = ref_muscl(Wst[2][n], Wst[3][n], Wst[4][n], dx_dir, -1, P->id_limiter);
-        ref_prim2cons(P, WRf, UR, &err);
-    } else if (P->id_weno == 0 && P->id_limiter == 0) {
-        ref_cons2prim(P, Ust[2], WLf, &err);
-        ref_prim2cons(P, WLf, UL, &err);
-        ref_cons2prim(P, Ust[3], WRf, &err);
-        ref_prim2cons(P, WRf, UR, &err);
+    double UL[REF_NMAX], UR[REF_NMAX];
+    int r;
+    if (P->id_thinc) {
+        double nrm[3];
+        ref_face_normal(P, G, &f, U, i, j, k, dir, nrm);
+        double Wd[REF_NMAX];
+        for (int s = -2; s <= 1; ++s) ref_cons2prim(P, Ust[s + 3], Wd, &err);   /* unused, as in the reference */
+        const int iA = 3 + d;
+        double Al[4] = {Ust[1][iA], Ust[2][iA], Ust[3][iA], Ust[4][iA]};   /* mm, m, c, p */
+        int gL = (Al[2] - Al[1]) * (Al[1] - Al[0]) > 0.0;
+        int gR = (Al[3] - Al[2]) * (Al[2] - Al[1]) > 0.0;
+        int sym = (Al[0] == Al[3]) && (Al[1] == Al[2]) && (Al[1] != Al[0]);
+        int gate = (!P->thinc_use_normal) || (fabs(nrm[dir]) > P->thinc_normal_threshold);
+        double eps = P->thinc_eps;
+        int intL = (Al[1] > eps && Al[1] < 1.0 - eps) && (gL || sym) && gate;
+        int intR = (Al[2] > eps && Al[2] < 1.0 - eps) && (gR || sym) && gate;
+        if (intL || intR) {
+            r = ref_recon_lr(P, Ust, 0, 1, dx_dir, UL, UR, &err);   /* MUSCL-minmod base */
+            if (r) return r;
+            double Lb[REF_NMAX], Rb[REF_NMAX];
+            for (int n = 0; n < N; ++n) { Lb[n] = UL[n]; Rb[n] = UR[n]; }
+            double beta = P->thinc_use_normal ? (P->thinc_beta * fabs(nrm[dir]) + 0.01) : P->thinc_beta;
+            if (intL) {
+                double sigma = ref_signnz(Al[2] - Al[0]);
+                ref_state_from_vf(P, Lb, ref_thinc(beta, sigma, Al[1], 1), UL, &err);
+            }
+            if (intR) {
+                double sigma = ref_signnz(Al[3] - Al[1]);
+                ref_state_from_vf(P, Rb, ref_thinc(beta, sigma, Al[2], 0), UR, &err);
+            }
+        } else {
+            r = ref_recon_lr(P, Ust, P->id_weno, P->id_limiter, dx_dir, UL, UR, &err);
+            if (r) return r;
+        }
     } else {
-        return CMP_ERR_UNSUPPORTED;
+        r = ref_recon_lr(P, Ust, P->id_weno, P->id_limiter, dx_dir, UL, UR, &err);
+        if (r) return r;
     }
     double vel[3];
     ref_riemann(P, UL, UR, dir, flux, vel, cmax, &err);
@@ -443,7 +662,7 @@ int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, 
                 for (int ii = 0; ii < fe[0]; ++ii) {
                     int i = box->lo[0] + ii, j = box->lo[1] + jj, k = lo2 + kk;
                     double fl[REF_NMAX], fnc, c;
-                    int r = ref_face_flux(P, box, ng, U, i, j, k, d, G->dx[d], fl, &fnc, &c);
+                    int r = ref_face_flux(P, G, box, ng, U, i, j, k, d, fl, &fnc, &c);
                     if (r > e) e = r;
                     long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
                     for (int n = 0; n < N; ++n) F[d][q + n * nface] = fl[n];

@@ -38,6 +38,9 @@ double ref_weno5(double f_mm, double f_m, double f_c, double f_p, double f_pp, i
 void   ref_weno5_weights(double f_mm, double f_m, double f_c, double f_p, double f_pp, int id_face,
                          double* cand, double* w);
 double ref_muscl(double f_m, double f_c, double f_p, double dx, int id_face, int id_limiter);
+/* WENO5B face state of cell W[2] from the primitives of cells W[0..4] */
+void   ref_weno5b_side(const cmp_params* P, double W[][REF_NMAX], int id_face, double* Wf);
+double ref_thinc(double beta, double sigma, double alpha_bar, int id_face);
 
 /* Riemann solvers: flux[nstate], vel[dim], *cmax */
 void ref_riemann(const cmp_params* P, const double* UL, const double* UR, int dir,
@@ -45,8 +48,8 @@ void ref_riemann(const cmp_params* P, const double* UL, const double* UR, int di
 
 /* per-face kernel on a box fab: U is the fab base of `box` (ng ghosts); (i,j,k) the face index
  * (face i sits between cells i-1 and i in direction dir) */
-int ref_face_flux(const cmp_params* P, const cmp_box* box, int ng, const double* U,
-                  int i, int j, int k, int dir, double dx_dir,
+int ref_face_flux(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
+                  const double* U, int i, int j, int k, int dir,
                   double* flux, double* fluxnc, double* cmax);
 
 /* one box: ComputeFaceFluxes + FluxDivergence. dUdt: valid box, nstate comps. cmax (scalar or NULL)

@@ -29,19 +29,22 @@ def _fill_cpu(c, lay, buf):
 
 
 CASES_RHS = [
-    (2, "WENO5", "HLLC"), (2, "WENO5", "HLL"), (2, "MUSCL", "HLLC"), (2, "MUSCL-MC", "HLL"),
-    (2, "MUSCL-vanLeer", "HLLC"), (2, "MUSCL-Superbee", "HLLC"), (2, "GODUNOV", "HLLC"),
-    (3, "WENO5", "HLLC"), (3, "WENO5", "HLL"), (3, "MUSCL", "HLLC"),
+    (2, "WENO5", "HLLC", 0), (2, "WENO5", "HLL", 0), (2, "MUSCL", "HLLC", 0), (2, "MUSCL-MC", "HLL", 0),
+    (2, "MUSCL-vanLeer", "HLLC", 0), (2, "MUSCL-Superbee", "HLLC", 0), (2, "GODUNOV", "HLLC", 0),
+    (2, "WENO5B", "HLLC", 0), (2, "WENO5B", "HLL", 0), (2, "WENO5", "HLLC", 1), (2, "WENO5B", "HLLC", 1),
+    (3, "WENO5", "HLLC", 0), (3, "WENO5", "HLL", 0), (3, "MUSCL", "HLLC", 0), (3, "WENO5B", "HLLC", 0),
+    (3, "WENO5", "HLLC", 1), (3, "WENO5B", "HLL", 1),
 ]
 
 
-@pytest.mark.parametrize("dim,scheme,riemann", CASES_RHS)
+@pytest.mark.parametrize("dim,scheme,riemann,thinc", CASES_RHS)
 @pytest.mark.parametrize("source_term", [0, 1])
-def test_rhs_box_matches_oracle(gpu, dim, scheme, riemann, source_term):
+def test_rhs_box_matches_oracle(gpu, dim, scheme, riemann, thinc, source_term):
     import torch
     n = 24 if dim == 2 else 12
     c = cases.random_smooth(dim, n, seed=7 + dim, scheme=scheme, riemann=riemann,
-                            pinf=(0.0, 0.0) if dim == 2 else (3.0, 0.0), source_term=source_term)
+                            pinf=(0.0, 0.0) if dim == 2 else (3.0, 0.0), source_term=source_term,
+                            thinc=thinc)
     P, G = c["params"], c["geom"]
     ng = scheme_ghosts(scheme)
     lay = Layout(dim, c["boxes"], ng, P.nstate)
@@ -123,11 +126,14 @@ def _gpu_advance(c, lay, nsteps, use_graph=True):
     return lay.unpack(Un.cpu().numpy(), c["shape"]), hist.cpu().numpy()[:nsteps]
 
 
-@pytest.mark.parametrize("dim,scheme,riemann,mg", [(2, "WENO5", "HLLC", 16), (2, "WENO5", "HLL", 32),
-                                                   (2, "MUSCL", "HLLC", 32), (3, "WENO5", "HLLC", 8)])
-def test_one_rk3_step_matches_oracle(gpu, dim, scheme, riemann, mg):
+@pytest.mark.parametrize("dim,scheme,riemann,mg,thinc", [
+    (2, "WENO5", "HLLC", 16, 0), (2, "WENO5", "HLL", 32, 0), (2, "MUSCL", "HLLC", 32, 0),
+    (2, "WENO5B", "HLLC", 16, 0), (2, "WENO5", "HLLC", 16, 1), (3, "WENO5", "HLLC", 8, 0),
+    (3, "WENO5B", "HLLC", 8, 1)])
+def test_one_rk3_step_matches_oracle(gpu, dim, scheme, riemann, mg, thinc):
     n = 48 if dim == 2 else 16
-    c = cases.random_smooth(dim, n, seed=11, scheme=scheme, riemann=riemann, max_grid_size=mg)
+    c = cases.random_smooth(dim, n, seed=11, scheme=scheme, riemann=riemann, max_grid_size=mg,
+                            thinc=thinc)
     lay = Layout(dim, c["boxes"], scheme_ghosts(scheme), c["params"].nstate)
     buf = lay.pack(c["U"])
     hist_ref = oracle.level_advance(c["params"], c["geom"], lay, buf, 1, c["cfl"])
@@ -182,10 +188,20 @@ def test_box_split_and_graph_are_bit_identical(gpu):
     assert np.array_equal(ha, hb) and np.array_equal(ha, hd) and np.array_equal(ha, he)
 
 
-def test_water_air_3d_small_matches_oracle(gpu):
-    """Stiffened-gas water/air at shock pressure ratios, outflow BCs, 3D, 24^3 in 8^3 boxes:
-    one step <= 1e-12, then 20 steps compared in L1."""
-    c = cases.water_air_droplet_3d(n=24, box=8)
+@pytest.mark.parametrize("thinc", [0, 1])
+def test_water_air_3d_small_matches_oracle(gpu, thinc):
+    """Stiffened-gas water/air at shock pressure ratios, outflow BCs, 3D, 24^3 in 8^3 boxes, with
+    and without THINC: one step <= 1e-12, then 20 steps compared in L1.
+
+    THINC's gates include an exact equality test of neighbouring volume fractions (the two-cell
+    plateau, FVM_Reconstruction2Face_Conservative_THINC_K_Compressible5Eq.H:147-150). The droplet
+    is centred on a face, so mirror cells start bit-identical; once round-off breaks that symmetry
+    differently on two machines the gate flips and the trajectories part. With THINC the
+    multi-step check is therefore step-wise: from the oracle's own state after 5, 10 and 20 steps,
+    one GPU step must match one oracle step to 1e-12 (identical inputs, identical gate decisions),
+    and the 20-step trajectories must agree to 1e-3 in L1."""
+    import torch
+    c = cases.water_air_droplet_3d(n=24, box=8, thinc=thinc)
     lay = Layout(3, c["boxes"], 3, 7)
     buf = lay.pack(c["U"])
     oracle.level_advance(c["params"], c["geom"], lay, buf, 1, c["cfl"])
@@ -196,4 +212,28 @@ def test_water_air_3d_small_matches_oracle(gpu):
     oracle.level_advance(c["params"], c["geom"], lay, buf, 20, c["cfl"])
     ref = lay.unpack(buf, c["shape"])
     got, _ = _gpu_advance(c, lay, 20)
-    assert l1rel_per_var(got, ref).max() <= TOL_L1_100
+    assert l1rel_per_var(got, ref).max() <= (TOL_L1_100 if not thinc else 1e-3)
+    if not thinc:
+        return
+    P, G = c["params"], c["geom"]
+    dev = torch.device("cuda:0")
+    buf = lay.pack(c["U"])
+    lev = Level(P, G, lay)
+    cm = oracle.level_initial_cmax(P, G, lay, buf)
+    step = 0
+    for target in (5, 10, 20):
+        while step < target:   # the oracle's own trajectory (dt from the previous step's c_max)
+            cm = oracle.level_rk3_step(P, G, lay, buf, oracle.compute_dt(G, cm, c["cfl"], 0, 0.0))
+            step += 1
+        dt = oracle.compute_dt(G, cm, c["cfl"], 0, 0.0)
+        one = buf.copy()
+        oracle.level_rk3_step(P, G, lay, one, dt)
+        Un = torch.from_numpy(buf.copy()).to(dev)
+        U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+        dtd = torch.full((1,), dt, dtype=torch.float64, device=dev)
+        lev.ThirdOrderSSPRK(Un, U1, U2, dtd)
+        assert lev.check_errors() == 0
+        g1 = lay.unpack(Un.cpu().numpy(), c["shape"])
+        r1 = lay.unpack(one, c["shape"])
+        assert maxrel_per_var(g1, r1).max() <= TOL_STEP, (target, maxrel_per_var(g1, r1))
+    lev.close()

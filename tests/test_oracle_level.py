@@ -12,13 +12,13 @@ import pytest
 import oracle
 from compas_amd import Layout, cases, make_geometry, make_params, scheme_ghosts
 
-SCHEMES = [("WENO5", "HLLC"), ("WENO5", "HLL"), ("MUSCL", "HLLC"), ("MUSCL-MC", "HLL"),
-           ("GODUNOV", "HLLC")]
+SCHEMES = [("WENO5", "HLLC", 0), ("WENO5", "HLL", 0), ("MUSCL", "HLLC", 0), ("MUSCL-MC", "HLL", 0),
+           ("GODUNOV", "HLLC", 0), ("WENO5B", "HLLC", 0), ("WENO5", "HLLC", 1), ("WENO5B", "HLL", 1)]
 
 
 def _box_rhs(c, scheme):
     P, G = c["params"], c["geom"]
-    ng = scheme_ghosts(scheme)
+    ng = max(scheme_ghosts(scheme), 2 if P.id_thinc else 1)
     lay = Layout(P.dim, c["boxes"], ng, P.nstate)
     buf = lay.pack(c["U"])
     oracle.level_fill_ghosts(P, G, lay, buf)
@@ -27,31 +27,32 @@ def _box_rhs(c, scheme):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("scheme,riemann", SCHEMES)
-def test_uniform_state_has_zero_rhs(dim, scheme, riemann):
+@pytest.mark.parametrize("scheme,riemann,thinc", SCHEMES)
+def test_uniform_state_has_zero_rhs(dim, scheme, riemann, thinc):
     n = 12 if dim == 2 else 8
-    c = cases.random_smooth(dim, n, seed=0, scheme=scheme, riemann=riemann, amp=0.0)
+    c = cases.random_smooth(dim, n, seed=0, scheme=scheme, riemann=riemann, amp=0.0, thinc=thinc)
     c["U"][:] = c["U"][(slice(None),) + (0,) * dim][(slice(None),) + (None,) * dim]
     r, _ = _box_rhs(c, scheme)
     assert np.all(r == 0.0)
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("scheme,riemann", SCHEMES)
-def test_periodic_conservation(dim, scheme, riemann):
+@pytest.mark.parametrize("scheme,riemann,thinc", SCHEMES)
+def test_periodic_conservation(dim, scheme, riemann, thinc):
     """Sum over a periodic box of dU/dt of every conservative component telescopes to zero."""
     n = 16 if dim == 2 else 8
-    c = cases.random_smooth(dim, n, seed=4, scheme=scheme, riemann=riemann, max_grid_size=n // 2)
+    c = cases.random_smooth(dim, n, seed=4, scheme=scheme, riemann=riemann, max_grid_size=n // 2,
+                            thinc=thinc)
     r, _ = _box_rhs(c, scheme)
     for q in range(2 + dim + 1):
         scale = np.sum(np.abs(r[q])) + 1e-300
         assert abs(np.sum(r[q])) <= 1e-13 * scale, (q, np.sum(r[q]), scale)
 
 
-@pytest.mark.parametrize("scheme,riemann", SCHEMES)
-def test_xy_symmetry(scheme, riemann):
+@pytest.mark.parametrize("scheme,riemann,thinc", SCHEMES)
+def test_xy_symmetry(scheme, riemann, thinc):
     """Transposing the 2D state (and swapping u, v) transposes dU/dt."""
-    c = cases.random_smooth(2, 16, seed=5, scheme=scheme, riemann=riemann)
+    c = cases.random_smooth(2, 16, seed=5, scheme=scheme, riemann=riemann, thinc=thinc)
     r, _ = _box_rhs(c, scheme)
     ct = dict(c)
     Ut = np.transpose(c["U"], (0, 2, 1)).copy()

@@ -1,0 +1,146 @@
+"""Pins of the oracle's WENO5B and THINC reconstructions (CPU only).
+
+WENO5B (FVM_Reconstruction2Face_Conservative_WENO5B_D_Compressible5Eq.H:15-320): reduces to WENO5
+for smooth data away from the bounds when the phase densities are uniform (alpha then has the same
+weights as alpha*rho); keeps face volume fractions in [0, 1] and phase masses >= 0 where WENO5
+leaves them; face alpha1 + alpha2 = 1.
+THINC (Reconstruction2Face_THINC_D.H:13-35): the jump position c makes the cell average of the
+tanh profile equal to the cell value (checked by quadrature); the face values are that profile at
+the cell's faces; the face normal is exact for linear fields; the rebuilt state keeps velocity,
+pressure and phase densities.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from compas_amd import Layout, cases, make_geometry, make_params
+
+
+def _prims(dim, a1, rho1, rho2, u, p):
+    W = np.zeros((5, 4 + dim))
+    W[:, 0] = a1 * rho1
+    W[:, 1] = (1 - a1) * rho2
+    W[:, 2] = u
+    W[:, 2 + dim] = p
+    W[:, 3 + dim] = a1
+    return W
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_weno5b_equals_weno5_on_smooth_interior_data(dim):
+    P = make_params(dim, scheme="WENO5B")
+    x = np.linspace(0.0, 0.4, 5)
+    a1 = 0.4 + 0.1 * np.sin(3 * x)
+    W = _prims(dim, a1, 2.0, 0.5, 0.3 + 0.2 * x, 1.0 + 0.1 * np.cos(2 * x))
+    for face in (1, -1):
+        wb = oracle.weno5b_side(P, W, face)
+        for n in range(4 + dim):
+            ref = oracle.weno5(W[:, n], face)
+            assert wb[n] == pytest.approx(ref, rel=1e-13, abs=1e-15), n
+
+
+def test_weno5b_keeps_bounds_where_weno5_overshoots():
+    P = make_params(2, scheme="WENO5B")
+    rng = np.random.default_rng(3)
+    n_over = 0
+    for _ in range(500):
+        a1 = np.clip(rng.choice([0.0, 1.0], size=5) * rng.uniform(0.9, 1.0, 5) + rng.uniform(0, 0.02, 5), 1e-8, 1 - 1e-8)
+        W = _prims(2, a1, rng.uniform(0.5, 2), rng.uniform(0.5, 2), 0.1, 1.0)
+        W[:, 0] = a1 * rng.uniform(0.5, 2.0, 5)
+        W[:, 1] = (1 - a1) * rng.uniform(0.5, 2.0, 5)
+        for face in (1, -1):
+            plain = oracle.weno5(W[:, 5], face)
+            if plain < 0.0 or plain > 1.0:
+                n_over += 1
+            wb = oracle.weno5b_side(P, W, face)
+            assert -1e-15 <= wb[5] <= 1.0 + 1e-15
+            assert wb[0] >= -1e-15 and wb[1] >= -1e-15
+    assert n_over > 20   # the check above was exercised where plain WENO5 leaves [0, 1]
+
+
+@pytest.mark.parametrize("beta", [0.5, 1.0, 2.3, 4.0])
+@pytest.mark.parametrize("abar", [0.02, 0.3, 0.5, 0.77, 0.99])
+def test_thinc_profile_has_the_cell_average(beta, abar):
+    from scipy.integrate import quad
+    hi = oracle.thinc(beta, 1.0, abar, 1)
+    lo = oracle.thinc(beta, 1.0, abar, 0)
+    # c from the closed form of the cell average; the profile is 0.5 (1 + tanh(beta (x + c)))
+    c = 0.5 / beta * math.log((math.exp(2 * beta * abar) - 1) / (math.exp(2 * beta) - math.exp(2 * beta * abar)))
+    prof = lambda x: 0.5 * (1 + math.tanh(beta * (x + c)))
+    avg, _ = quad(prof, 0.0, 1.0, epsabs=1e-14, epsrel=1e-14)
+    assert avg == pytest.approx(abar, rel=1e-10, abs=1e-13)
+    assert hi == pytest.approx(prof(1.0), rel=1e-13)
+    assert lo == pytest.approx(prof(0.0), rel=1e-13)
+    # sigma = -1 mirrors the profile
+    assert oracle.thinc(beta, -1.0, abar, 1) == pytest.approx(prof(0.0), rel=1e-13)
+    assert oracle.thinc(beta, -1.0, abar, 0) == pytest.approx(prof(1.0), rel=1e-13)
+
+
+def _thinc_case(dim, field, periodic=True):
+    P = make_params(dim, gamma=(1.4, 1.6), scheme="WENO5", thinc=1)
+    n = 12
+    G = make_geometry((n,) * dim, (0,) * dim, (1,) * dim, (int(periodic),) * dim)
+    W = np.zeros((4 + dim,) + (n,) * dim)
+    a1 = field
+    W[0] = a1 * 1.0
+    W[1] = (1 - a1) * 8.0
+    W[2] = 0.7
+    W[2 + dim] = 1.0
+    W[3 + dim] = a1
+    return P, G, cases.prim_to_cons(P, W)
+
+
+def test_thinc_applies_at_interfaces_only_and_stays_bounded():
+    """A tanh interface across x: faces next to it get THINC values inside [0, 1] that differ from
+    WENO5's; far away the fluxes are exactly WENO5's."""
+    n = 12
+    x = (np.arange(n) + 0.5) / n
+    a1 = np.clip(0.5 * (1 + np.tanh((x - 0.5) / 0.04)), 1e-8, 1 - 1e-8)
+    field = np.broadcast_to(a1[None, :], (n, n)).copy()
+    P, G, U = _thinc_case(2, field)
+    lay = Layout(2, [([0, 0], [n - 1, n - 1])], 3, 6)
+    buf = lay.pack(U)
+    oracle.level_fill_ghosts(P, G, lay, buf)
+    fab = lay.fab_view(buf, 0).copy()
+    _, _, ft, _ = oracle.rhs_box(P, G, ([0, 0], [n - 1, n - 1]), 3, fab, want_faces=True)
+    Pw = make_params(2, gamma=(1.4, 1.6), scheme="WENO5", thinc=0)
+    _, _, fw, _ = oracle.rhs_box(Pw, G, ([0, 0], [n - 1, n - 1]), 3, fab, want_faces=True)
+    ax, wx = ft[0][5], fw[0][5]      # alpha flux through x faces, shape (n, n+1)
+    diff = np.abs(ax - wx).max(axis=0)
+    near = [i for i in range(n + 1) if abs(i / n - 0.5) < 0.1]
+    assert diff[near].max() > 1e-3
+    far = [i for i in range(n + 1) if 0.25 < abs(i / n - 0.5) < 0.4]
+    assert diff[far].max() == 0.0
+    # where THINC acted, the alpha flux alpha_face u* (u* = 0.7 here) has alpha_face in [0, 1]
+    thinc_faces = np.abs(ax - wx) > 0.0
+    assert thinc_faces.sum() >= n
+    af = ax[thinc_faces] / 0.7
+    assert np.all(af >= -1e-12) and np.all(af <= 1.0 + 1e-12)
+
+
+def test_face_normal_exact_for_linear_fields():
+    """For alpha = a x + b y (+ c z) the THINC gate sees the normal (a, b, c)/|.|: only faces whose
+    normal component exceeds the threshold switch to THINC. Checked through the fluxes: tilting
+    the field past the threshold angle flips the x faces from WENO5 to THINC."""
+    n = 12
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)   # [j, i]
+
+    def xflux(theta):
+        field = 0.5 + 0.3 * (np.cos(theta) * (X - 0.5) + np.sin(theta) * (Y - 0.5))
+        P, G, U = _thinc_case(2, field, periodic=False)
+        lay = Layout(2, [([0, 0], [n - 1, n - 1])], 3, 6)
+        buf = lay.pack(U)
+        oracle.level_fill_ghosts(P, G, lay, buf)
+        fab = lay.fab_view(buf, 0).copy()
+        _, _, ft, _ = oracle.rhs_box(P, G, ([0, 0], [n - 1, n - 1]), 3, fab, want_faces=True)
+        Pw = make_params(2, gamma=(1.4, 1.6), scheme="WENO5", thinc=0)
+        _, _, fw, _ = oracle.rhs_box(Pw, G, ([0, 0], [n - 1, n - 1]), 3, fab, want_faces=True)
+        inner = (slice(2, -2), slice(3, -3))
+        return np.abs(ft[0][5][inner] - fw[0][5][inner]).max()
+
+    thr = 0.25
+    assert xflux(math.acos(thr) - 0.05) > 1e-6      # |n_x| just above the threshold: THINC
+    assert xflux(math.acos(thr) + 0.05) == 0.0      # just below: base scheme
