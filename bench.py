@@ -1,10 +1,18 @@
 #!/usr/bin/env python3
-"""Benchmark: cell-updates/s of the five-equation WENO5-HLLC TVD-RK3 hyperbolic step on one GPU.
+"""Benchmark: cell-updates/s of the five-equation WENO5-HLLC TVD-RK3 step on one GPU.
 
-A step is one full TVD-RK3 step of the level (device ComputeDt, three ghost fills, three fused
-stage kernels), replayed from a hipGraph, with the state resident in HBM. Prints one JSON line.
+A step is one full time step of the level as the reference's Evolve loop takes it: device
+ComputeDt and the flow's TVD-RK3 step (three ghost fills, three fused stage kernels) replayed from
+a hipGraph, then, where the workload has interface regularization, the Phase-Field step
+(mobility from max|u|, the ACDI mechanism sub-steps, Grad_Q, and Phase-Field RK3 sub-steps until
+the flow step's dt is covered). The state stays resident in HBM. Prints one JSON line.
 
 Workloads (--workload):
+  water-air-droplet-3d    (default) 256^3 in 32^3 boxes, water droplet r = 0.1 (stiffened gas
+                          gamma 4.4, p_inf 6e8 Pa, rho 1000) in air (gamma 1.4, rho 1.2), p = 1e5 Pa,
+                          Mach 2.4 shock at x = 0.25, outflow BCs; WENO5 + THINC, HLLC, RK3 at
+                          CFL 0.4; ACDI regularization (explicit RC, Phase-Field CFL 0.3,
+                          eta = 0.5 dx)
   advected-interface-2d   256^2 periodic, two ideal gases (gamma 1.4/rho 1 in a circle r = 0.25,
                           gamma 1.6/rho 10 outside), u = v = p = 1, WENO5 + HLLC + RK3, CFL 0.5
 Also reported: the dominant kernel's roofline position (measured with HIP events on the stream it
@@ -24,20 +32,26 @@ PEAK_HBM_GBS = 8000.0
 
 
 def workload(name):
-    from compas_amd import cases
+    from compas_amd import cases, make_pf_params
     if name == "advected-interface-2d":
         c = cases.advected_interface_2d(n=256)
         c["config"] = {"workload": name, "grid": [256, 256], "boxes": len(c["boxes"]),
                        "scheme": "WENO5", "riemann": "HLLC", "integrator": "TVD-RK3", "cfl": 0.5,
-                       "nstate": 6, "bc": "periodic"}
+                       "nstate": 6, "bc": "periodic", "phase_field": None}
         c["default_steps"] = 100
+        c["pf"] = None
         return c
     if name == "water-air-droplet-3d":
         c = cases.water_air_droplet_3d(n=256, box=32)
+        # PhaseField.* of the reference's water-air example inputs (ACDI, explicit RC, cfl 0.3,
+        # Eta_Multiplier 0.5)
+        c["pf"] = make_pf_params(mechanism="ACDI", explicit_rc=1, cfl=0.3, eta_multiplier=0.5)
         c["config"] = {"workload": name, "grid": [256, 256, 256], "boxes": len(c["boxes"]),
                        "box": [32, 32, 32], "scheme": "WENO5+THINC", "riemann": "HLLC",
                        "integrator": "TVD-RK3", "cfl": 0.4, "nstate": 7, "bc": "outflow",
-                       "eos": "stiffened gas water / ideal air"}
+                       "eos": "stiffened gas water / ideal air",
+                       "phase_field": {"mechanism": "ACDI", "explicit_rc": 1, "flux": "UD mixture",
+                                       "cfl": 0.3, "eta_multiplier": 0.5}}
         c["default_steps"] = 20
         return c
     raise SystemExit("unknown workload %r" % name)
@@ -45,7 +59,8 @@ def workload(name):
 
 def cpu_baseline(c, seconds, threads):
     """The oracle (reference-structure CPU restatement) on the same initial state, timed on this
-    host: as many RK3 steps as fit in about `seconds` (at least one)."""
+    host: as many full time steps (flow RK3 step, then the Phase-Field step where the workload has
+    one) as fit in about `seconds`, at least one."""
     import numpy as np
     import oracle
     from compas_amd import Layout
@@ -60,21 +75,24 @@ def cpu_baseline(c, seconds, threads):
     steps = 0
     while True:
         oracle.level_rk3_step(P, G, lay, buf, dt)
+        if c["pf"] is not None:
+            oracle.level_pf_evolve(P, c["pf"], G, lay, buf, dt)
         steps += 1
         el = time.perf_counter() - t0
         if el >= seconds or steps >= 200:
             break
+    what = "flow + Phase-Field steps" if c["pf"] is not None else "RK3 steps"
     return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": oracle.num_threads(),
             "kind": "port",
-            "sample": "%d RK3 steps of the %s workload (%d cells) in %.1f s" % (steps, c["name"], cells, el)}
+            "sample": "%d %s of the %s workload (%d cells) in %.1f s" % (steps, what, c["name"], cells, el)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="advected-interface-2d")
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="water-air-droplet-3d")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
@@ -101,6 +119,8 @@ def main():
     dt = torch.zeros(1, dtype=torch.float64, device=dev)
     cmax = torch.zeros(1, dtype=torch.float64, device=dev)
     lev = Level(P, G, lay)
+    if c["pf"] is not None:
+        lev.pf_setup(c["pf"])
     stream = torch.cuda.Stream()
     cells = int(np.prod(c["shape"]))
 
@@ -133,11 +153,32 @@ def main():
         stage_ms = k0.elapsed_time(k1) / args.kernel_reps
         assert lev.check_errors(stream) == 0
 
+        pf_rec = None
+        if c["pf"] is not None:
+            # the Phase-Field step alone (host-synchronised: it reads max|u| and max|Grad_Q|)
+            t2 = time.perf_counter()
+            info = lev.PhaseField_Evolve(Un, U1, U2, dt, stream=stream)
+            stream.synchronize()
+            pf_ms = (time.perf_counter() - t2) * 1e3
+            # its stage kernel: the Phase-Field right-hand side with the step's Grad_Q
+            gq = torch.zeros(lev.faces.total, dtype=torch.float64, device=dev)
+            lev.PhaseField_GradQPF(Un, float(dt.item()), gq, stream=stream)
+            lev.FillPatch(U1, stream=stream)
+            k0.record(stream)
+            for _ in range(args.kernel_reps):
+                lev.PhaseField_compute_dUdt_FV(U2, U1, gq, stream=stream)
+            k1.record(stream)
+            stream.synchronize()
+            assert lev.check_errors(stream) == 0
+            pf_rec = {"ms_per_pf_step": pf_ms, "mechanism_substeps": info[3],
+                      "pf_rk3_substeps": info[5], "max_grad_q": info[4], "mobility": info[1],
+                      "pf_rhs_kernel_ms": k0.elapsed_time(k1) / args.kernel_reps}
+
     flops_stage = roofline.flops_per_cell_stage(P) * cells
     achieved = flops_stage / (stage_ms * 1e-3) / 1e12
     bytes_update = roofline.bytes_per_cell_update(P)
     rec = {
-        "metric": "cell-updates/s (5Eq WENO5-HLLC RK3)",
+        "metric": "cell-updates/s (5Eq WENO5-HLLC RK3), % roofline; p50/p99 case latency",
         "value": cells * K / wall,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -165,6 +206,8 @@ def main():
         },
         "event_ms_per_step": ev_ms / K,
     }
+    if pf_rec is not None:
+        rec["phase_field"] = pf_rec
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         rec["cpu_baseline"] = cpu_baseline(c, args.cpu_seconds, threads)

@@ -12,8 +12,8 @@ import numpy as np
 from . import capi
 from .capi import cmp_box, cmp_geometry, cmp_params, check
 
-__all__ = ["make_params", "make_geometry", "split_domain", "Layout", "Level", "capi",
-           "SCHEMES", "BC"]
+__all__ = ["make_params", "make_pf_params", "make_geometry", "split_domain", "Layout", "FaceLayout",
+           "Level", "capi", "SCHEMES", "BC"]
 
 # FiniteVolume.Scheme -> (ID_WENO, ID_Limiter, ID_Poly, Num_Grow)
 # (include/Operators_FV/FiniteVolume_Parameter.H:71-139)
@@ -71,6 +71,24 @@ def make_params(dim, gamma=(1.4, 1.4), pinf=(0.0, 0.0), eos=None, scheme="WENO5"
 
 def scheme_ghosts(scheme):
     return SCHEMES[scheme][3]
+
+
+# PhaseField.Mechanism strings with a path here -> ID_Mechanism
+# (include/Physics_PhaseField/PhaseField_Parameter.H:79-137)
+PF_MECHANISMS = {"ACDI": 3}
+
+
+def make_pf_params(mechanism="ACDI", explicit_rc=1, id_flux=0, flux_phase2flux=1, cfl=0.4,
+                   lambda_=0.0, eta=0.0, eta_factor=2.0, eta_multiplier=1.0, p_min=(0.0, 0.0),
+                   max_step=1000000, change_limiter=1):
+    """cmp_pf_params with PhaseField_Parameter's defaults (PhaseField_Parameter.H:31-53)."""
+    p = capi.cmp_pf_params()
+    p.mechanism = PF_MECHANISMS[mechanism]
+    p.explicit_rc, p.id_flux, p.flux_phase2flux = explicit_rc, id_flux, flux_phase2flux
+    p.cfl, p.lambda_, p.eta, p.eta_factor, p.eta_multiplier = cfl, lambda_, eta, eta_factor, eta_multiplier
+    p.p_min[0], p.p_min[1] = p_min
+    p.max_step, p.change_limiter = max_step, change_limiter
+    return p
 
 
 def make_geometry(n_cell, prob_lo, prob_hi, periodic, lo_bc=None, hi_bc=None):
@@ -170,6 +188,35 @@ class Layout:
         return out
 
 
+class FaceLayout:
+    """Face-buffer layout of a level (cmp_face_layout): per box and direction a nodal array of
+    ncomp components, box-major."""
+
+    def __init__(self, dim, boxes, ncomp=1):
+        self.dim, self.ncomp = dim, ncomp
+        self.boxes = [(list(lo), list(hi)) for lo, hi in boxes]
+        self.offsets = []
+        off = 0
+        for lo, hi in self.boxes:
+            row = []
+            for d in range(dim):
+                row.append(off)
+                off += ncomp * int(np.prod([hi[e] - lo[e] + 1 + (e == d) for e in range(dim)]))
+            self.offsets.append(row)
+        self.total = off
+
+    def face_shape(self, b, d):
+        lo, hi = self.boxes[b]
+        ext = [hi[e] - lo[e] + 1 + (e == d) for e in range(self.dim)]
+        return (self.ncomp,) + tuple(reversed(ext))
+
+    def face_view(self, buf, b, d):
+        """numpy view of box b's direction-d faces: [n, (k,) j, i]."""
+        shape = self.face_shape(b, d)
+        off = self.offsets[b][d]
+        return buf[off:off + int(np.prod(shape))].reshape(shape)
+
+
 def _ptr(t):
     if t is None:
         return None
@@ -245,6 +292,36 @@ class Level:
         check(self.lib.cmp_level_info(self.h, ctypes.byref(n), ctypes.byref(x), ctypes.byref(y),
                                       ctypes.byref(z)), "cmp_level_info")
         return {"ntiles": n.value, "tile": (x.value, y.value, z.value)}
+
+    # ---- Phase-Field regularization ----------------------------------------------------------
+    def pf_setup(self, pf):
+        self.pf = pf
+        self.faces = FaceLayout(self.layout.dim, self.layout.boxes, 1)
+        check(self.lib.cmp_level_pf_setup(self.h, ctypes.byref(pf)), "cmp_level_pf_setup")
+
+    # PhaseField_compute_dUdt_FV on every box (ghosts of U filled; gradq a face buffer)
+    def PhaseField_compute_dUdt_FV(self, dUdt, U, gradq, stream=None):
+        check(self.lib.cmp_level_pf_rhs(self.h, _ptr(U), _ptr(gradq), _ptr(dUdt), _stream(stream)),
+              "cmp_level_pf_rhs")
+
+    # MaxVelocity + PhaseFieldMechanism_Evolve + PhaseField_GradQPF; returns (max|Grad_Q|, info)
+    def PhaseField_GradQPF(self, U, dt, gradq, stream=None):
+        gm = ctypes.c_double(0.0)
+        info = (ctypes.c_double * 4)()
+        check(self.lib.cmp_level_pf_gradq(self.h, _ptr(U), float(dt), _ptr(gradq), ctypes.byref(gm),
+                                          info, _stream(stream)), "cmp_level_pf_gradq")
+        return gm.value, list(info)
+
+    def PhaseField_ThirdOrderSSPRK(self, Un, U1, U2, gradq, pf_dt, stream=None):
+        check(self.lib.cmp_level_pf_rk3_step(self.h, _ptr(Un), _ptr(U1), _ptr(U2), _ptr(gradq),
+                                             float(pf_dt), _stream(stream)), "cmp_level_pf_rk3_step")
+
+    def PhaseField_Evolve(self, Un, U1, U2, dt_dev, stream=None):
+        """Returns [max|u|, mobility, eta, mechanism sub-steps, max|Grad_Q|, Phase-Field sub-steps]."""
+        info = (ctypes.c_double * 6)()
+        check(self.lib.cmp_level_pf_evolve(self.h, _ptr(Un), _ptr(U1), _ptr(U2), _ptr(dt_dev), info,
+                                           _stream(stream)), "cmp_level_pf_evolve")
+        return list(info)
 
 
 def rhs_box(params, geom, box, ng, U, dUdt, cmax=None, fluxes=(None, None, None),

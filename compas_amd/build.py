@@ -41,6 +41,8 @@ def build(verbose=False):
     units = [
         ("cmp_kernels.hip", "k2.o", ["-DCMP_KERNEL_DIM=2"]),
         ("cmp_kernels.hip", "k3.o", ["-DCMP_KERNEL_DIM=3"]),
+        ("cmp_pf.hip", "pf2.o", ["-DCMP_KERNEL_DIM=2"]),
+        ("cmp_pf.hip", "pf3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_capi.hip", "capi.o", []),
     ]
     procs = []

@@ -79,6 +79,23 @@ class cmp_box(ctypes.Structure):
     ]
 
 
+class cmp_pf_params(ctypes.Structure):
+    _fields_ = [
+        ("mechanism", ctypes.c_int),
+        ("explicit_rc", ctypes.c_int),
+        ("id_flux", ctypes.c_int),
+        ("flux_phase2flux", ctypes.c_int),
+        ("cfl", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+        ("eta", ctypes.c_double),
+        ("eta_factor", ctypes.c_double),
+        ("eta_multiplier", ctypes.c_double),
+        ("p_min", ctypes.c_double * 2),
+        ("max_step", ctypes.c_int),
+        ("change_limiter", ctypes.c_int),
+    ]
+
+
 P = ctypes.POINTER
 vp = ctypes.c_void_p
 dp = ctypes.c_void_p   # device pointers travel as integers (torch data_ptr())
@@ -106,6 +123,15 @@ SIGNATURES = {
     "cmp_level_check": (ctypes.c_int, [vp, vp]),
     "cmp_level_info": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                       P(ctypes.c_int)]),
+    "cmp_face_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(cmp_box), ctypes.c_int,
+                                       P(ctypes.c_longlong), P(ctypes.c_longlong)]),
+    "cmp_abi_pf_size": (ctypes.c_int, [P(ctypes.c_int)]),
+    "cmp_level_pf_setup": (ctypes.c_int, [vp, P(cmp_pf_params)]),
+    "cmp_level_pf_rhs": (ctypes.c_int, [vp, dp, dp, dp, vp]),
+    "cmp_level_pf_gradq": (ctypes.c_int, [vp, dp, ctypes.c_double, dp, P(ctypes.c_double),
+                                          P(ctypes.c_double), vp]),
+    "cmp_level_pf_rk3_step": (ctypes.c_int, [vp, dp, dp, dp, dp, ctypes.c_double, vp]),
+    "cmp_level_pf_evolve": (ctypes.c_int, [vp, dp, dp, dp, dp, P(ctypes.c_double), vp]),
 }
 
 _lib = None
@@ -118,6 +144,10 @@ def load(path=LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise RuntimeError("libcompas_amd.so is not built (%s); run python -m compas_amd.build" % path)
+    # torch ships its own HIP runtime with the same soname (libamdhip64.so.7) as /opt/rocm's. Loaded
+    # first, it is the one the library binds to; loaded after the library, it would come in as a
+    # second runtime in the process and the two would disagree about the device.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

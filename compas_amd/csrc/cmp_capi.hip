@@ -17,33 +17,89 @@ namespace cmp {
 template <int DIM>
 hipError_t launch_stage_dim(const LaunchCfg&, const KParams&, int, int, const TileDev*, int,
                             const BoxDev*, const BoxDev*, int, const double*, const double*,
-                            double*, const double*, unsigned long long*, int*, hipStream_t);
+                            double*, const double*, unsigned long long*, int*, hipStream_t,
+                            const double*);
 template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg&, const KParams&, const BoxDev&, int, const double*,
                             double*, double*, hipStream_t);
 template <int DIM>
-hipError_t launch_fill_dim(const FillDev&, long long, double*, int*, hipStream_t);
+hipError_t launch_fill_dim(int, const FillDev&, long long, double*, int*, hipStream_t);
 template <int DIM>
 hipError_t launch_dt_dim(const double*, double*, double*, double, int, double*, int*, int*,
                          hipStream_t);
+template <int DIM>
+hipError_t launch_pf_extract_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_normals_dim(const PfGeom&, long long, const double*, double*, double, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_jfaces_dim(const PfGeom&, const long long*, const double*, const double*, double*,
+                                double*, double, double, double, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_update_dim(const PfGeom&, long long, double*, const double*, double, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_gradq_dim(const PfGeom&, const long long*, const double*, const double*, double*,
+                               double, double, unsigned long long*, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_maxvel_dim(const PfGeom&, long long, const double*, unsigned long long*,
+                                hipStream_t);
 
 hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int stage,
                         const TileDev* tiles, int ntiles, const BoxDev* boxes,
                         const BoxDev* single_box, int single_ntiles, const double* Us,
                         const double* Un, double* Uout, const double* dt,
-                        unsigned long long* cmax_bits, int* err, hipStream_t s)
+                        unsigned long long* cmax_bits, int* err, hipStream_t s, const double* gradq)
 {
     if (lc.dim == 2)
         return launch_stage_dim<2>(lc, kp, mode, stage, tiles, ntiles, boxes, single_box,
-                                   single_ntiles, Us, Un, Uout, dt, cmax_bits, err, s);
+                                   single_ntiles, Us, Un, Uout, dt, cmax_bits, err, s, gradq);
     return launch_stage_dim<3>(lc, kp, mode, stage, tiles, ntiles, boxes, single_box, single_ntiles,
-                               Us, Un, Uout, dt, cmax_bits, err, s);
+                               Us, Un, Uout, dt, cmax_bits, err, s, gradq);
 }
 
-hipError_t launch_fill(int dim, int /*N*/, const FillDev& fd, long long nghost, double* U, int* err,
+hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, double* U, int* err,
                        hipStream_t s)
 {
-    return dim == 2 ? launch_fill_dim<2>(fd, nghost, U, err, s) : launch_fill_dim<3>(fd, nghost, U, err, s);
+    return dim == 2 ? launch_fill_dim<2>(N, fd, nghost, U, err, s) : launch_fill_dim<3>(N, fd, nghost, U, err, s);
+}
+
+#define CMP_BY_DIM(dim, FN, ...) ((dim) == 2 ? FN<2>(__VA_ARGS__) : FN<3>(__VA_ARGS__))
+
+hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A,
+                             hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_extract_dim, g, maxcells, U, A, s);
+}
+
+hipError_t launch_pf_normals(int dim, const PfGeom& g, long long maxcells, const double* A, double* Nrm,
+                             double eta, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_normals_dim, g, maxcells, A, Nrm, eta, s);
+}
+
+hipError_t launch_pf_jfaces(int dim, const PfGeom& g, const long long* maxfaces, const double* A,
+                            const double* Nrm, double* J, double* H, double mob_face, double eta,
+                            double dt_m, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_jfaces_dim, g, maxfaces, A, Nrm, J, H, mob_face, eta, dt_m, s);
+}
+
+hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double* A, const double* J,
+                            double dt_m, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_update_dim, g, maxcells, A, J, dt_m, s);
+}
+
+hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* A,
+                           const double* H, double* gq, double eta, double eta2,
+                           unsigned long long* gmax_bits, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_gradq_dim, g, maxfaces, A, H, gq, eta, eta2, gmax_bits, s);
+}
+
+hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
+                            unsigned long long* bits, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_maxvel_dim, g, maxcells, U, bits, s);
 }
 
 hipError_t launch_compute_dt(int dim, const double* dx, double* cmax, double* dt, double cfl,
@@ -100,6 +156,29 @@ struct cmp_level {
     const void* gkey[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double gcfl = 0.0;
     int glimiter = -1;
+
+    // Phase-Field step (cmp_level_pf_setup)
+    bool pf_on = false;
+    cmp_pf_params pf{};
+    LaunchCfg pf_lc{};
+    KParams pf_kp{};
+    BoxDev* d_abox = nullptr;       // alpha1 layout: 1 ghost, 1 comp
+    BoxDev* d_nbox = nullptr;       // normals layout: 1 ghost, dim comps
+    long long* d_gprefix1 = nullptr;
+    long long nghost1 = 0;
+    FillDev fdA{}, fdN{};
+    PfGeom pg{};
+    long long maxcells = 0, maxfaces[3] = {0, 0, 0};
+    long long na = 0, nfaces = 0;
+    double* d_A0 = nullptr;         // alpha1 of the step, PF_bcs ghosts
+    double* d_A = nullptr;          // alpha1 advanced by the mechanism sub-steps
+    double* d_N = nullptr;          // cell normals
+    double* d_J = nullptr;          // face J of the current sub-step
+    double* d_H = nullptr;          // face Jhat
+    double* d_gq = nullptr;         // face Grad_Q (cmp_level_pf_evolve)
+    unsigned long long* d_bits = nullptr;   // [0] max|u|, [1] max|Grad_Q| as bit patterns
+    double* d_pfdt = nullptr;       // Phase-Field sub-step size read by the stage kernels
+    double* h_pin = nullptr;        // pinned host scalars
 };
 
 namespace {
@@ -201,6 +280,20 @@ BoxDev make_boxdev(int dim, const cmp_box& b, int ng)
     return d;
 }
 
+// Face-buffer offsets of cmp_face_layout: box-major, nodal in each direction.
+long long face_offsets(int dim, int nbox, const cmp_box* boxes, int ncomp, long long* off)
+{
+    long long o = 0;
+    for (int b = 0; b < nbox; ++b)
+        for (int d = 0; d < dim; ++d) {
+            long long n = ncomp;
+            for (int e = 0; e < dim; ++e) n *= (boxes[b].hi[e] - boxes[b].lo[e] + 1 + (e == d ? 1 : 0));
+            if (off) off[b * dim + d] = o;
+            o += n;
+        }
+    return o;
+}
+
 long long gcd_ll(long long a, long long b)
 {
     a = a < 0 ? -a : a;
@@ -220,6 +313,25 @@ int cmp_abi_sizes(int* params_size, int* geometry_size, int* box_size)
     if (params_size) *params_size = (int)sizeof(cmp_params);
     if (geometry_size) *geometry_size = (int)sizeof(cmp_geometry);
     if (box_size) *box_size = (int)sizeof(cmp_box);
+    return CMP_OK;
+}
+
+int cmp_abi_pf_size(int* pf_params_size)
+{
+    if (!pf_params_size) return CMP_ERR_ARG;
+    *pf_params_size = (int)sizeof(cmp_pf_params);
+    return CMP_OK;
+}
+
+int cmp_face_layout(int dim, int nbox, const cmp_box* boxes, int ncomp, long long* offsets,
+                    long long* total)
+{
+    if (!boxes || nbox < 0 || dim < 2 || dim > 3 || ncomp < 1) return CMP_ERR_ARG;
+    for (int b = 0; b < nbox; ++b)
+        for (int d = 0; d < dim; ++d)
+            if (boxes[b].hi[d] < boxes[b].lo[d]) return CMP_ERR_ARG;
+    const long long t = face_offsets(dim, nbox, boxes, ncomp, offsets);
+    if (total) *total = t;
     return CMP_OK;
 }
 
@@ -320,9 +432,14 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
     L->kp = kp;
     L->lc = lc;
 
-    // box table
+    // box table, with each box's place in a level face buffer (Phase-Field Grad_Q)
     std::vector<BoxDev> hb(nbox);
-    for (int b = 0; b < nbox; ++b) hb[b] = make_boxdev(dim, boxes[b], ng);
+    std::vector<long long> foff((size_t)nbox * dim);
+    face_offsets(dim, nbox, boxes, 1, foff.data());
+    for (int b = 0; b < nbox; ++b) {
+        hb[b] = make_boxdev(dim, boxes[b], ng);
+        for (int d = 0; d < dim; ++d) hb[b].goff[d] = foff[(size_t)b * dim + d];
+    }
     // tile work-list, box by box
     int tx, ty, tz;
     tile_dims(dim, &tx, &ty, &tz);
@@ -415,9 +532,32 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
     return CMP_OK;
 }
 
+static void pf_free(cmp_level* L)
+{
+    hipFree(L->d_abox);
+    hipFree(L->d_nbox);
+    hipFree(L->d_gprefix1);
+    hipFree(L->d_A0);
+    hipFree(L->d_A);
+    hipFree(L->d_N);
+    hipFree(L->d_J);
+    hipFree(L->d_H);
+    hipFree(L->d_gq);
+    hipFree(L->d_bits);
+    hipFree(L->d_pfdt);
+    if (L->h_pin) hipHostFree(L->h_pin);
+    L->d_abox = L->d_nbox = nullptr;
+    L->d_gprefix1 = nullptr;
+    L->d_A0 = L->d_A = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_pfdt = nullptr;
+    L->d_bits = nullptr;
+    L->h_pin = nullptr;
+    L->pf_on = false;
+}
+
 int cmp_level_destroy(cmp_level* L)
 {
     if (!L) return CMP_OK;
+    pf_free(L);
     if (L->gexec) hipGraphExecDestroy(L->gexec);
     if (L->graph) hipGraphDestroy(L->graph);
     if (L->own_stream) hipStreamDestroy(L->own_stream);
@@ -530,8 +670,10 @@ int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nste
     // dt_hist is indexed by the steps of this call and holds at most nsteps entries
     CMP_TRY(launch_set2(L->d_step, 0, dt_hist ? nsteps : 0, s));
     if (!use_graph || nsteps < 2) {
-        for (int n = 0; n < nsteps; ++n)
+        for (int n = 0; n < nsteps; ++n) {
             if ((r = level_step_with_hist(L, Un, U1, U2, cfl, limiter, dt_dev, cmax, dt_hist, s))) return r;
+            if (L->pf_on && (r = cmp_level_pf_evolve(L, Un, U1, U2, dt_dev, nullptr, stream))) return r;
+        }
         return CMP_OK;
     }
     // graph of one step, replayed; captured on a private stream when the caller gave the legacy
@@ -560,8 +702,244 @@ int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nste
         L->gcfl = cfl;
         L->glimiter = limiter;
     }
-    for (int n = 0; n < nsteps; ++n) CMP_TRY(hipGraphLaunch(L->gexec, cs));
+    for (int n = 0; n < nsteps; ++n) {
+        CMP_TRY(hipGraphLaunch(L->gexec, cs));
+        if (L->pf_on && (r = cmp_level_pf_evolve(L, Un, U1, U2, dt_dev, nullptr, cs))) return r;
+    }
     if (cs != s) CMP_TRY(hipStreamSynchronize(cs));
+    return CMP_OK;
+}
+
+// ---- Phase-Field step ---------------------------------------------------------------------------
+
+int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
+{
+    if (!L || !pf) return CMP_ERR_ARG;
+    if (pf->mechanism != CMP_PF_ACDI || pf->explicit_rc != 1 || pf->id_flux != 0)
+        return CMP_ERR_UNSUPPORTED;
+    if (pf->flux_phase2flux != 0 && pf->flux_phase2flux != 1) return CMP_ERR_ARG;
+    if (!(pf->cfl > 0.0) || !(pf->eta_factor > 0.0) || pf->max_step < 1) return CMP_ERR_ARG;
+    if (!(pf->eta > 0.0) && !(pf->eta_multiplier > 0.0)) return CMP_ERR_ARG;
+    if (L->nbox > 65535) return CMP_ERR_UNSUPPORTED;   // one grid row per box
+    pf_free(L);
+    const int dim = L->P.dim, nbox = L->nbox;
+    std::vector<cmp_box> ba(L->boxes), bn(L->boxes);
+    long long na = 0, nn = 0;
+    cmp_level_layout(dim, nbox, ba.data(), 1, 1, &na);
+    cmp_level_layout(dim, nbox, bn.data(), 1, dim, &nn);
+    std::vector<long long> foff((size_t)nbox * dim);
+    const long long nf = face_offsets(dim, nbox, L->boxes.data(), 1, foff.data());
+    std::vector<BoxDev> ha(nbox), hn(nbox);
+    std::vector<long long> gp(nbox + 1, 0);
+    long long maxcells = 0, maxfaces[3] = {0, 0, 0};
+    for (int b = 0; b < nbox; ++b) {
+        ha[b] = make_boxdev(dim, ba[b], 1);
+        hn[b] = make_boxdev(dim, bn[b], 1);
+        for (int d = 0; d < dim; ++d) ha[b].goff[d] = hn[b].goff[d] = foff[(size_t)b * dim + d];
+        long long fab = 1, val = 1;
+        for (int d = 0; d < dim; ++d) {
+            const long long n = ba[b].hi[d] - ba[b].lo[d] + 1;
+            fab *= n + 2;
+            val *= n;
+        }
+        gp[b + 1] = gp[b] + (fab - val);
+        maxcells = std::max(maxcells, val);
+        for (int d = 0; d < dim; ++d) {
+            long long f = 1;
+            for (int e = 0; e < dim; ++e) f *= ba[b].hi[e] - ba[b].lo[e] + 1 + (e == d ? 1 : 0);
+            maxfaces[d] = std::max(maxfaces[d], f);
+        }
+    }
+    int rc = CMP_OK;
+#define CMP_TRYP(x) do { rc = check_hip(x); if (rc) { pf_free(L); return rc; } } while (0)
+    CMP_TRYP(hipMalloc(&L->d_abox, sizeof(BoxDev) * nbox));
+    CMP_TRYP(hipMemcpy(L->d_abox, ha.data(), sizeof(BoxDev) * nbox, hipMemcpyHostToDevice));
+    CMP_TRYP(hipMalloc(&L->d_nbox, sizeof(BoxDev) * nbox));
+    CMP_TRYP(hipMemcpy(L->d_nbox, hn.data(), sizeof(BoxDev) * nbox, hipMemcpyHostToDevice));
+    CMP_TRYP(hipMalloc(&L->d_gprefix1, sizeof(long long) * (nbox + 1)));
+    CMP_TRYP(hipMemcpy(L->d_gprefix1, gp.data(), sizeof(long long) * (nbox + 1), hipMemcpyHostToDevice));
+    CMP_TRYP(hipMalloc(&L->d_A0, sizeof(double) * na));
+    CMP_TRYP(hipMalloc(&L->d_A, sizeof(double) * na));
+    CMP_TRYP(hipMalloc(&L->d_N, sizeof(double) * nn));
+    CMP_TRYP(hipMalloc(&L->d_J, sizeof(double) * nf));
+    CMP_TRYP(hipMalloc(&L->d_H, sizeof(double) * nf));
+    CMP_TRYP(hipMalloc(&L->d_gq, sizeof(double) * nf));
+    CMP_TRYP(hipMalloc(&L->d_bits, sizeof(unsigned long long) * 2));
+    CMP_TRYP(hipMalloc(&L->d_pfdt, sizeof(double)));
+    CMP_TRYP(hipHostMalloc((void**)&L->h_pin, sizeof(double) * 4, hipHostMallocDefault));
+    // the unfilled ghost cells of the normals are never read, but keep them defined
+    CMP_TRYP(hipMemset(L->d_N, 0, sizeof(double) * nn));
+#undef CMP_TRYP
+    L->na = na;
+    L->nfaces = nf;
+    L->maxcells = maxcells;
+    for (int d = 0; d < 3; ++d) L->maxfaces[d] = maxfaces[d];
+    L->nghost1 = gp[nbox];
+    // PF_bcs / LS_bcs / Normal_bcs: int_dir on periodic sides, foextrap elsewhere
+    // (source/Class_Compressible_PhaseField.cpp:471-488, 1516-1530, 1970-1986)
+    L->fdA = L->fd;
+    L->fdA.boxes = L->d_abox;
+    L->fdA.gprefix = L->d_gprefix1;
+    L->fdA.ng = 1;
+    for (int d = 0; d < 3; ++d) {
+        L->fdA.lo_bc[d] = 2;
+        L->fdA.hi_bc[d] = 2;
+    }
+    L->fdN = L->fdA;
+    L->fdN.boxes = L->d_nbox;
+    L->pg.hbox = L->d_boxes;
+    L->pg.abox = L->d_abox;
+    L->pg.nbox = nbox;
+    for (int d = 0; d < 3; ++d) L->pg.dx[d] = (d < dim) ? L->G.dx[d] : 1.0;
+    L->pf = *pf;
+    L->pf_lc = L->lc;
+    L->pf_lc.thinc = 0;
+    L->pf_lc.riemann = (pf->flux_phase2flux == 1) ? RIEMANN_PF_MIX : RIEMANN_PF_PHASIC;
+    L->pf_kp = L->kp;
+    L->pf_kp.thinc = 0;
+    L->pf_kp.pf_pmin[0] = pf->p_min[0];
+    L->pf_kp.pf_pmin[1] = pf->p_min[1];
+    L->pf_on = true;
+    return CMP_OK;
+}
+
+int cmp_level_pf_rhs(cmp_level* L, const double* U, const double* gradq, double* dUdt, void* stream)
+{
+    if (!L || !U || !gradq || !dUdt) return CMP_ERR_ARG;
+    if (!L->pf_on) return CMP_ERR_ARG;
+    CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RHS, 0, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0,
+                         U, nullptr, dUdt, nullptr, nullptr, L->d_err, (hipStream_t)stream, gradq));
+    return CMP_OK;
+}
+
+static inline double bits_to_double(const void* p)
+{
+    double v;
+    std::memcpy(&v, p, sizeof(double));
+    return v;
+}
+
+int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, double* gradq_max,
+                       double* info, void* stream)
+{
+    if (!L || !U || !gradq) return CMP_ERR_ARG;
+    if (!L->pf_on) return CMP_ERR_ARG;
+    if (!(dt > 0.0)) return CMP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const int dim = L->P.dim;
+    const cmp_pf_params& pf = L->pf;
+    CMP_TRY(launch_zero_u64(L->d_bits, 2, s));
+    CMP_TRY(launch_pf_maxvel(dim, L->pg, L->maxcells, U, L->d_bits, s));
+    CMP_TRY(launch_pf_extract(dim, L->pg, L->maxcells, U, L->d_A0, s));
+    CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A0, L->d_err, s));
+    CMP_TRY(hipMemcpyAsync(L->h_pin, L->d_bits, sizeof(double), hipMemcpyDeviceToHost, s));
+    CMP_TRY(hipStreamSynchronize(s));
+    const double maxv = bits_to_double(L->h_pin);
+    if (__builtin_isnan(maxv)) return CMP_ERR_NAN;
+    // Lambda, eta, mobility (PhaseField_Evolve, source/Class_Compressible_PhaseField.cpp:427-465)
+    double lambda = pf.lambda;
+    if (lambda <= 0.0) lambda = 1.0;
+    double eta = pf.eta;
+    if (eta <= 0.0) {
+        double m = L->G.dx[0];
+        for (int d = 1; d < dim; ++d) m = (m < L->G.dx[d]) ? L->G.dx[d] : m;
+        eta = pf.eta_multiplier * m;
+    }
+    double mob = 1.0;
+    mob *= 0.5 * eta * maxv;
+    // mechanism sub-steps (PhaseFieldMechanism_Evolve, :642-692)
+    double sdx = 1.0 / L->G.dx[0] / L->G.dx[0];
+    for (int d = 1; d < dim; ++d) sdx = sdx + 1.0 / L->G.dx[d] / L->G.dx[d];
+    double dt_m = 0.9 / (2.0 * mob * lambda) / (sdx + 1.0 / eta / eta);
+    const double mob_face = 0.5 * mob + 0.5 * mob;
+    CMP_TRY(hipMemcpyAsync(L->d_A, L->d_A0, sizeof(double) * L->na, hipMemcpyDeviceToDevice, s));
+    CMP_TRY(hipMemsetAsync(L->d_H, 0, sizeof(double) * L->nfaces, s));
+    double t = 0.0;
+    const double T = dt;
+    long long n = 0;
+    while (t < T) {
+        if (t + dt_m > T) dt_m = T - t;
+        if (!(dt_m > 0.0) || t + dt_m == t || n >= 10000000) return CMP_ERR_NAN;   // no progress
+        CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A, L->d_err, s));
+        CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_A, L->d_N, eta, s));
+        CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
+        CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, L->d_A, L->d_N, L->d_J, L->d_H, mob_face, eta,
+                                 dt_m, s));
+        CMP_TRY(launch_pf_update(dim, L->pg, L->maxcells, L->d_A, L->d_J, dt_m, s));
+        t += dt_m;
+        ++n;
+    }
+    CMP_TRY(launch_scale(L->d_H, L->nfaces, 1.0 / dt, s));
+    // explicit Grad_Q from the step's alpha1 (PhaseField_GradQPF, :2315-2403)
+    CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_A0, L->d_H, gradq, eta, pf.eta_factor * eta,
+                            L->d_bits + 1, s));
+    CMP_TRY(hipMemcpyAsync(L->h_pin + 1, L->d_bits + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    CMP_TRY(hipStreamSynchronize(s));
+    if (gradq_max) *gradq_max = bits_to_double(L->h_pin + 1);
+    if (info) {
+        info[0] = maxv;
+        info[1] = mob;
+        info[2] = eta;
+        info[3] = (double)n;
+    }
+    return CMP_OK;
+}
+
+int cmp_level_pf_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, const double* gradq,
+                          double pf_dt, void* stream)
+{
+    if (!L || !Un || !U1 || !U2 || !gradq) return CMP_ERR_ARG;
+    if (!L->pf_on) return CMP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    CMP_TRY(launch_setd(L->d_pfdt, pf_dt, s));
+    double* Us[3] = {Un, U1, U2};
+    double* Uo[3] = {U1, U2, Un};
+    for (int stage = 0; stage < 3; ++stage) {
+        int r = cmp_level_fill_ghosts(L, Us[stage], stream);
+        if (r) return r;
+        CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes,
+                             nullptr, 0, Us[stage], Un, Uo[stage], L->d_pfdt, nullptr, L->d_err, s, gradq));
+    }
+    return CMP_OK;
+}
+
+int cmp_level_pf_evolve(cmp_level* L, double* Un, double* U1, double* U2, const double* dt_dev,
+                        double* info, void* stream)
+{
+    if (!L || !Un || !U1 || !U2 || !dt_dev) return CMP_ERR_ARG;
+    if (!L->pf_on) return CMP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    CMP_TRY(hipMemcpyAsync(L->h_pin + 2, dt_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+    CMP_TRY(hipStreamSynchronize(s));
+    const double dt = L->h_pin[2];
+    double gm = 0.0, inf4[4] = {0.0, 0.0, 0.0, 0.0};
+    int r = cmp_level_pf_gradq(L, Un, dt, L->d_gq, &gm, inf4, stream);
+    if (r) return r;
+    // Phase-Field sub-steps (PhaseField_Evolve :541-590, PhaseField_ComputeDt :2037-2098)
+    const int dim = L->P.dim;
+    double dxm = L->G.dx[0];
+    for (int d = 1; d < dim; ++d) dxm = (L->G.dx[d] < dxm) ? L->G.dx[d] : dxm;
+    double pf_time = 0.0, prev = 0.0;
+    int n = 0;
+    for (int step = 0; step < L->pf.max_step && pf_time < dt; ++step) {
+        double tmp = L->pf.cfl * dxm / (gm + 1.0e-14);
+        if (pf_time == 0.0) prev = tmp;
+        double dt0 = tmp;
+        if (L->pf.change_limiter && prev > 0.0) tmp = (1.1 * prev < tmp) ? 1.1 * prev : tmp;
+        dt0 = (tmp < dt0) ? tmp : dt0;
+        const double eps = (1.0e-3) * dt0;
+        if (pf_time + dt0 > dt - eps) dt0 = dt - pf_time;
+        prev = dt0;
+        if ((r = cmp_level_pf_rk3_step(L, Un, U1, U2, L->d_gq, dt0, stream))) return r;
+        pf_time += dt0;
+        ++n;
+        if (pf_time >= dt - 1.0e-6 * dt) break;
+    }
+    if (info) {
+        for (int q = 0; q < 4; ++q) info[q] = inf4[q];
+        info[4] = gm;
+        info[5] = (double)n;
+    }
     return CMP_OK;
 }
 

@@ -12,10 +12,6 @@
 #include <hip/hip_runtime.h>
 #include "../../include/compas_amd.h"
 
-#ifndef CMP_FAST_RCP
-#define CMP_FAST_RCP 1
-#endif
-
 namespace cmp {
 
 // Kernel-side parameter block (passed by value as a kernel argument).
@@ -29,6 +25,7 @@ struct KParams {
     double dx[3];
     double inv_dx[3];
     double weno5b_weight;   // quadrature weight of the WENO5B bound limiter (1/12 for ID_Poly = 4)
+    double pf_pmin[2];      // phasic energy floors of the phasic Phase-Field formulation
 };
 
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }  // std::min
@@ -174,21 +171,6 @@ __device__ __forceinline__ void weno5_parts(double f_mm, double f_m, double f_c,
     IS[2] = (10.0 / 3.0) * c * c - (11.0 / 3.0) * e * c + (4.0 / 3.0) * e * e;
 }
 
-// 1/x from the hardware reciprocal estimate and two Newton steps (relative error a few 1e-16,
-// against ten instructions for an IEEE division). Used where the result only scales weights.
-__device__ __forceinline__ double rcp_nr(double x)
-{
-#if CMP_FAST_RCP
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-#else
-    return 1.0 / x;
-#endif
-}
-
 // Nonlinear weights. The reference's ratio form d_k / sum_j (IS_k/IS_j)^2 d_j with 0/0 -> 1 equals
 // d_k b_k / sum_j d_j b_j with b_k = prod_{j != k} IS_j^2; the IS are first scaled by a power of two
 // (exact) so the products neither overflow nor underflow. When that sum vanishes (two or three IS
@@ -215,7 +197,7 @@ __device__ __forceinline__ void weno5_weights(const double* IS, double* w)
     const double b0 = q12 * q12, b1 = q02 * q02, b2 = q01 * q01;
     const double S = d0 * b0 + d1 * b1 + d2 * b2;
     if (S > 1e-290) {
-        const double inv = rcp_nr(S);
+        const double inv = 1.0 / S;
         w[0] = d0 * b0 * inv;
         w[1] = d1 * b1 * inv;
         w[2] = d2 * b2 * inv;
@@ -463,15 +445,16 @@ __device__ __forceinline__ void make_side(const KParams& kp, FaceSide<DIM>& s, i
 // (t = 0..2G-1, cell G-1 is the low side). SCHEME: 0 Godunov, 1 MUSCL, 5 WENO5, 6 WENO5B. With
 // THINC, normal() returns the face normal's component along dir (from the volume fraction); it is
 // evaluated only when the volume-fraction gates already admit THINC on a side.
-template <int DIM, int SCHEME, int RIEMANN, bool THINC, typename S, typename NRM>
-__device__ __forceinline__ double face_flux(const KParams& kp, int limiter, int dir, const S& s,
-                                            const NRM& normal, double* F, double& cmax, int& bad)
+// Reconstruction part: the primitive face states WL (low side) and WR (high side).
+template <int DIM, int SCHEME, bool THINC, typename S, typename NRM>
+__device__ __forceinline__ void recon_face(const KParams& kp, int limiter, int dir, const S& s,
+                                           const NRM& normal, double* LW, double* RW)
 {
     constexpr int N = 4 + DIM;
     constexpr int G = (SCHEME >= 5) ? 3 : (SCHEME == 1 ? 2 : 1);
     constexpr int IA = 3 + DIM;
     const double dx = kp.dx[dir], idx = kp.inv_dx[dir];
-    FaceSide<DIM> L, R;
+    struct { double* W; } L{LW}, R{RW};
     bool done = false;
     if (THINC) {
         // stencil cells mm, m, c, p of the face = G-2 .. G+1
@@ -532,9 +515,168 @@ __device__ __forceinline__ double face_flux(const KParams& kp, int limiter, int 
             }
         }
     }
+}
+
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, typename S, typename NRM>
+__device__ __forceinline__ double face_flux(const KParams& kp, int limiter, int dir, const S& s,
+                                            const NRM& normal, double* F, double& cmax, int& bad)
+{
+    FaceSide<DIM> L, R;
+    recon_face<DIM, SCHEME, THINC>(kp, limiter, dir, s, normal, L.W, R.W);
     make_side<DIM>(kp, L, bad);
     make_side<DIM>(kp, R, bad);
     return riemann<DIM, RIEMANN>(L, R, dir, F, cmax);
+}
+
+// ---- phase field ------------------------------------------------------------------------------------
+// Mapping_Backward / Mapping_Forward (include/Physics_PhaseField/Mapping_Backward.H:12-29,
+// Mapping_Forward.H:12-26).
+__device__ __forceinline__ double pf_map_backward(double phi, double eta)
+{
+    const double ep = 1.0e-12;
+    double t = phi;
+    if (t < ep) t = ep;
+    else if (t > 1.0 - ep) t = 1.0 - ep;
+    return 1.4142135623730951 * eta * atanh(2.0 * t - 1.0);
+}
+
+__device__ __forceinline__ double pf_map_forward(double psi, double eta)
+{
+    return 0.5 + 0.5 * tanh(psi / (1.4142135623730951 * eta));
+}
+
+__device__ __forceinline__ double pf_map_forward_d(double psi, double eta)
+{
+    const double x = psi / (1.4142135623730951 * eta);
+    const double ch = cosh(x);
+    return 0.5 / (1.4142135623730951 * eta) / ch / ch;
+}
+
+// ACDI face flux J = M (grad phi - F'(psi_f) n_f) (FDM_PhaseFieldFlux_ACDI_D,
+// include/Operators_PhaseField/FDM_PhaseFieldFlux_ACDI_D.H:12-45; PhaseFieldFlux_ACDI.H:12-26).
+// psi_m, psi_c are Mapping_Backward of pf_m, pf_c.
+__device__ __forceinline__ double pf_acdi_flux(double pf_m, double pf_c, double psi_m, double psi_c,
+                                               double n_m, double n_c, double mob, double eta, double dx)
+{
+    const double grad = (pf_c - pf_m) / dx;
+    const double psi = 0.5 * psi_m + 0.5 * psi_c;
+    const double nrm = 0.5 * n_m + 0.5 * n_c;
+    return mob * (grad - pf_map_forward_d(psi, eta) * nrm);
+}
+
+// Explicit Grad_Q of one face (source/Class_Compressible_PhaseField.cpp:2358-2391,
+// include/Physics_PhaseField/PhaseFieldFlux2GradQPF.H:12-28).
+__device__ __forceinline__ double pf_gradq_face(double pf_m, double pf_c, double jhat, double eta,
+                                                double eta2)
+{
+    const double epsilon = 1.0e-3;
+    const double c = pf_map_forward(pf_map_backward(pf_c, eta), eta2);
+    const double m = pf_map_forward(pf_map_backward(pf_m, eta), eta2);
+    const double pf = 0.5 * m + 0.5 * c;
+    const double flux = (pf < epsilon || pf > 1.0 - epsilon) ? 0.0 : jhat;
+    return flux / (pf * (1.0 - pf) + epsilon);
+}
+
+// f_Upwind / f_Downwind (include/Operators_FV/f_Upwind.H:12-27, f_Downwind.H:12-27)
+__device__ __forceinline__ double f_up(double fL, double fR, double V)
+{
+    return (V > 0.0) ? fL : ((V < 0.0) ? fR : 0.5 * (fL + fR));
+}
+__device__ __forceinline__ double f_down(double fL, double fR, double V)
+{
+    return (V > 0.0) ? fR : ((V < 0.0) ? fL : 0.5 * (fL + fR));
+}
+
+// Phasic internal energy alpha_k rho_k e_k from p, floored at p_min
+// (EquationOfState_Phase, Conservative2ConservativePhase_Compressible5Eq.H:34-91).
+template <int DIM>
+__device__ __forceinline__ void pf_phase_energies(const KParams& kp, const double* W, double& e1,
+                                                  double& e2, int& bad)
+{
+    const double a1 = W[3 + DIM], a2 = 1.0 - a1, p = W[2 + DIM];
+    const double phi1 = nasg_phi(a1, W[0], kp.C[0], bad);
+    const double phi2 = nasg_phi(a2, W[1], kp.C[1], bad);
+    e1 = phi1 * kp.A[0] * p + phi1 * kp.B[0] + W[0] * kp.D[0];
+    const double e1m = phi1 * kp.A[0] * kp.pf_pmin[0] + phi1 * kp.B[0] + W[0] * kp.D[0];
+    e2 = phi2 * kp.A[1] * p + phi2 * kp.B[1] + W[1] * kp.D[1];
+    const double e2m = phi2 * kp.A[1] * kp.pf_pmin[1] + phi2 * kp.B[1] + W[1] * kp.D[1];
+    const double rho_e = e1 + e2;
+    if (e1 < e1m) { e1 = e1m; e2 = rho_e - e1; }
+    else if (e2 < e2m) { e2 = e2m; e1 = rho_e - e2; }
+}
+
+// Phase-Field flux fhat of one face from the primitive face states: FluxPhaseField
+// (include/Physics_Compressible5Eq_PhaseField/FluxPhaseField_Compressible5Eq_PhaseField.H:12-65) with
+// the UD phasic flux (…/FluxPhasePhaseField_UD_Compressible_TwoPhase_PhaseField.H:12-194) and the
+// mixture (FORM 0, FluxPhaseField_Mixture_Compressible5Eq_PhaseField.H:12-116) or phasic
+// (FORM 1, VectorPhase2Vector_Compressible5Eq.H:15-54) formulation. Grad_Q1 = g, Grad_Q2 = -g.
+template <int DIM, int FORM>
+__device__ __forceinline__ void pf_flux(const KParams& kp, const double* WL, const double* WR, double g,
+                                        double* F, int& bad)
+{
+    constexpr int IA = 3 + DIM;
+    const double g1 = g, g2 = -g;
+    const double a1L = WL[IA], a1R = WR[IA];
+    const double a2L = 1.0 - a1L, a2R = 1.0 - a1R;
+    const double a2u = f_up(a2L, a2R, g1);
+    const double a1u = f_up(a1L, a1R, g2);
+    const double J1r1 = f_down(WL[0], WR[0], g1) * a2u * g1;
+    const double J2r2 = f_down(WL[1], WR[1], g2) * a1u * g2;
+    const double J1 = f_down(a1L, a1R, g1) * a2u * g1;
+    F[0] = J1r1;
+    F[1] = J2r2;
+    F[IA] = J1;
+    if (FORM == 0) {
+        const double J2 = f_down(a2L, a2R, g2) * a1u * g2;
+        const double M = J1r1 + J2r2;
+        double ke = 0.0;
+#pragma unroll
+        for (int m = 0; m < DIM; ++m) {
+            const double u = f_down(WL[2 + m], WR[2 + m], M);
+            F[2 + m] = M * u;
+            ke = (m == 0) ? u * u : ke + u * u;
+        }
+        const double MeK = M * (0.5 * ke);
+        const double p1 = f_down(WL[2 + DIM], WR[2 + DIM], J1);
+        const double J1phi = (kp.C[0] > 0.0) ? J1 - kp.C[0] * J1r1 : J1;
+        const double J1e1 = kp.A[0] * J1phi * p1 + kp.B[0] * J1phi + kp.D[0] * J1r1;
+        const double p2 = f_down(WL[2 + DIM], WR[2 + DIM], J2);
+        const double J2phi = (kp.C[1] > 0.0) ? J2 - kp.C[1] * J2r2 : J2;
+        const double J2e2 = kp.A[1] * J2phi * p2 + kp.B[1] * J2phi + kp.D[1] * J2r2;
+        F[2 + DIM] = MeK + J1e1 + J2e2;
+    } else {
+        double keL = 0.0, keR = 0.0;
+#pragma unroll
+        for (int m = 0; m < DIM; ++m) {
+            const double uL = WL[2 + m], uR = WR[2 + m];
+            keL = (m == 0) ? uL * uL : keL + uL * uL;
+            keR = (m == 0) ? uR * uR : keR + uR * uR;
+            const double m1 = f_down(WL[0] * uL, WR[0] * uR, g1) * a2u * g1;
+            const double m2 = f_down(WL[1] * uL, WR[1] * uR, g2) * a1u * g2;
+            F[2 + m] = m1 + m2;
+        }
+        double e1L, e2L, e1R, e2R;
+        pf_phase_energies<DIM>(kp, WL, e1L, e2L, bad);
+        pf_phase_energies<DIM>(kp, WR, e1R, e2R, bad);
+        const double E1L = e1L + 0.5 * WL[0] * keL, E1R = e1R + 0.5 * WR[0] * keR;
+        const double E2L = e2L + 0.5 * WL[1] * keL, E2R = e2R + 0.5 * WR[1] * keR;
+        const double JE1 = f_down(E1L, E1R, g1) * a2u * g1;
+        const double JE2 = f_down(E2L, E2R, g2) * a1u * g2;
+        F[2 + DIM] = JE1 + JE2;
+    }
+}
+
+// FVM_Conservative2FluxPhaseField_K (ID_Bound 0): base reconstruction (no THINC on this path),
+// then the Phase-Field flux.
+template <int DIM, int SCHEME, int FORM, typename S>
+__device__ __forceinline__ void pf_face_flux(const KParams& kp, int limiter, int dir, const S& s, double g,
+                                             double* F, int& bad)
+{
+    constexpr int N = 4 + DIM;
+    double LW[N], RW[N];
+    auto no_normal = []() -> double { return 0.0; };
+    recon_face<DIM, SCHEME, false>(kp, limiter, dir, s, no_normal, LW, RW);
+    pf_flux<DIM, FORM>(kp, LW, RW, g, F, bad);
 }
 
 // Normal component along dir of the volume-fraction gradient at a face

@@ -30,8 +30,6 @@ template <> struct Tile<3> { static constexpr int X = 8, Y = 8, Z = 3; };
 
 template <int SCHEME> struct Stencil { static constexpr int G = (SCHEME >= 5) ? 3 : (SCHEME == 1 ? 2 : 1); };
 
-constexpr int NT = 256;
-
 constexpr int cmax3(int a, int b, int c) { return (a > b ? (a > c ? a : c) : (b > c ? b : c)); }
 
 template <int DIM, int G>
@@ -44,24 +42,12 @@ struct Lds {
     static constexpr int AMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
 };
 
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = smax(v, __shfl_xor(v, off, 64));
-    return v;
-}
-
 // Maps a workgroup to a tile so that the workgroups one XCD receives (blocks b, b+8, b+16, ...
 // under round-robin dispatch) hold consecutive tiles, whose halos overlap in that XCD's L2.
 __device__ __forceinline__ int xcd_tile(int b, int nb)
 {
     const int q = nb >> 3, r = nb & 7, x = b & 7;
     return x * q + (x < r ? x : r) + (b >> 3);
-}
-
-__device__ __forceinline__ long long fab_index(const BoxDev& bx, int i, int j, int k)
-{
-    return (long long)(i - bx.flo[0]) + bx.sj * (j - bx.flo[1]) + bx.sk * (k - bx.flo[2]);
 }
 
 // What one workgroup's direction passes share.
@@ -76,6 +62,7 @@ struct StageCtx {
     int ae[3];      // extents of the alpha1 halo region (THINC)
     int tid, cx, cy, cz;
     bool valid;
+    const double* __restrict__ gq;   // Grad_Q face buffer (Phase-Field flux only)
 };
 
 // One direction of the stage kernel (D known at compile time): stage the primitives of the tile
@@ -139,15 +126,28 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
             else { px += o1; py += o2; }
             return sA[px + a0 * (py + a1 * pz)];
         };
-        auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
-        double Fl[N], c;
-        const double un = face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, cx_.limiter, D, st, normal, Fl, c, bad);
+        double Fl[N];
+        if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
+            // Grad_Q of this face from the box's nodal face array of direction D
+            const int nv0 = bx.hi[0] - bx.lo[0] + 1 + (D == 0 ? 1 : 0);
+            const int nv1 = bx.hi[1] - bx.lo[1] + 1 + (D == 1 ? 1 : 0);
+            const long long gi = bx.goff[D] + (long long)(cx_.tlo[0] + x - bx.lo[0])
+                               + (long long)nv0 * ((cx_.tlo[1] + y - bx.lo[1])
+                                                   + (long long)nv1 * (cx_.tlo[2] + z - bx.lo[2]));
+            pf_face_flux<DIM, SCHEME, RIEMANN - RIEMANN_PF_MIX>(kp, cx_.limiter, D, st, cx_.gq[gi], Fl, bad);
 #pragma unroll
-        for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
-        sF[N * FMAX + f] = un;
-        if (a < cx_.n[D]) {
-            if (__builtin_isnan(c)) bad = CMP_ERR_NAN;
-            cmax_loc = smax(cmax_loc, c);
+            for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
+        } else {
+            auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
+            double c;
+            const double un = face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, cx_.limiter, D, st, normal, Fl, c, bad);
+#pragma unroll
+            for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
+            sF[N * FMAX + f] = un;
+            if (a < cx_.n[D]) {
+                if (__builtin_isnan(c)) bad = CMP_ERR_NAN;
+                cmax_loc = smax(cmax_loc, c);
+            }
         }
     }
     __syncthreads();
@@ -161,8 +161,10 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
             const double t = (sF[q * FMAX + fp] - sF[q * FMAX + fm]) * idx;
             acc[q] = (D == 0) ? t : acc[q] + t;
         }
-        const double t = (sF[N * FMAX + fp] - sF[N * FMAX + fm]) * idx;
-        divu = (D == 0) ? t : divu + t;
+        if constexpr (RIEMANN < RIEMANN_PF_MIX) {
+            const double t = (sF[N * FMAX + fp] - sF[N * FMAX + fm]) * idx;
+            divu = (D == 0) ? t : divu + t;
+        }
     }
     __syncthreads();
 }
@@ -183,7 +185,8 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
              const TileDev* __restrict__ tiles, int ntiles, const BoxDev* __restrict__ boxes,
              BoxDev sbox, int sntx, int snty, int single,
              const double* __restrict__ Us, const double* Un, double* Uout,
-             const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err)
+             const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err,
+             const double* __restrict__ gq)
 {
     constexpr int N = 4 + DIM;
     constexpr int IA = 3 + DIM;
@@ -245,7 +248,7 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
     }
 
     StageCtx<DIM> ctx{kp, limiter, bx, Ub, {tlo[0], tlo[1], tlo[2]}, {n[0], n[1], n[2]},
-                      {ae[0], ae[1], ae[2]}, tid, cx, cy, cz, valid};
+                      {ae[0], ae[1], ae[2]}, tid, cx, cy, cz, valid, gq};
     dir_pass<DIM, SCHEME, RIEMANN, THINC, 0>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
     if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
     if (DIM > 2) dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1)>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
@@ -254,12 +257,22 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
         const int i = tlo[0] + cx, j = tlo[1] + cy, k = tlo[2] + cz;
         const long long c = fab_index(bx, i, j, k);
         double Uc[N], dU[N];
+        if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
+            // faces hold fhat and the reference stores -fhat: -div(-fhat) = +div(fhat); the
+            // five-equation Phase-Field NC term is empty
 #pragma unroll
-        for (int q = 0; q < N; ++q) {
-            Uc[q] = Ub[c + q * bx.sn];
-            dU[q] = -acc[q];
+            for (int q = 0; q < N; ++q) {
+                Uc[q] = Ub[c + q * bx.sn];
+                dU[q] = acc[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                Uc[q] = Ub[c + q * bx.sn];
+                dU[q] = -acc[q];
+            }
+            dU[IA] += nc_coefficient<DIM>(kp, Uc, bad) * divu;
         }
-        dU[IA] += nc_coefficient<DIM>(kp, Uc, bad) * divu;
         if (mode == MODE_RK) {
             const double dt = *dtp;
             double* Ob = Uout + bx.off;
@@ -361,11 +374,10 @@ void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
 // search over ghost-count prefixes, the cell from the ghost shell's slab decomposition; the source
 // is mapped through the physical BCs (foextrap clamp, reflect mirror with odd momentum signs) and
 // periodic wrap, and read from the valid cells of the box owning it.
-template <int DIM>
+template <int DIM, int N>
 __global__ __launch_bounds__(NT)
 void k_fill(FillDev fd, long long nghost, double* U, int* err)
 {
-    constexpr int N = 4 + DIM;
     for (long long g = (long long)blockIdx.x * NT + threadIdx.x; g < nghost;
          g += (long long)gridDim.x * NT) {
         int lo = 0, hi = fd.nbox;
@@ -379,7 +391,7 @@ void k_fill(FillDev fd, long long nghost, double* U, int* err)
         const int nv1 = bx.hi[1] - bx.lo[1] + 1;
         const int ex = bx.hi[0] - bx.lo[0] + 1 + 2 * ng, ey = nv1 + 2 * ng;
         const int ez = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + 2 * ng : 1;
-        int x, y, z;
+        int x = 0, y = 0, z = 0;
         long long r = g - fd.gprefix[b];
         bool slab = false;
         if (DIM == 3) {
@@ -458,7 +470,7 @@ void k_fill(FillDev fd, long long nghost, double* U, int* err)
 #pragma unroll
         for (int c = 0; c < N; ++c) {
             double v = U[si + c * so.sn];
-            if (c >= 2 && c < 2 + DIM && ((flip >> (c - 2)) & 1u)) v = -v;
+            if (N == 4 + DIM && c >= 2 && c < 2 + DIM && ((flip >> (c - 2)) & 1u)) v = -v;
             U[di + c * bx.sn] = v;
         }
     }
@@ -515,7 +527,7 @@ template <int DIM, int SCHEME, int RIEMANN, bool THINC>
 static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, const TileDev* tiles,
                          int ntiles, const BoxDev* boxes, const BoxDev* sbox, int sntiles,
                          const double* Us, const double* Un, double* Uout, const double* dt,
-                         unsigned long long* cmax_bits, int* err, hipStream_t s)
+                         unsigned long long* cmax_bits, int* err, hipStream_t s, const double* gq)
 {
     BoxDev sb{};
     int sntx = 1, snty = 1, single = 0, grid = ntiles;
@@ -529,7 +541,7 @@ static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, co
     if (grid <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_stage<DIM, SCHEME, RIEMANN, THINC>), dim3(grid), dim3(NT), 0, s, kp, limiter,
                        mode, stage, tiles, ntiles, boxes, sb, sntx, snty, single, Us, Un, Uout, dt,
-                       cmax_bits, err);
+                       cmax_bits, err, gq);
     return hipGetLastError();
 }
 
@@ -574,10 +586,26 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
                             const TileDev* tiles, int ntiles, const BoxDev* boxes,
                             const BoxDev* sbox, int sntiles, const double* Us, const double* Un,
                             double* Uout, const double* dt, unsigned long long* cmax_bits, int* err,
-                            hipStream_t s)
+                            hipStream_t s, const double* gq)
 {
+    if (lc.riemann == RIEMANN_PF_MIX || lc.riemann == RIEMANN_PF_PHASIC) {
+        if (!gq || lc.thinc) return hipErrorInvalidValue;
+        const bool mix = lc.riemann == RIEMANN_PF_MIX;
+#define CMP_PF_ARGS kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout, dt, \
+                    cmax_bits, err, s, gq
+        if (lc.scheme == 5) return mix ? stage4<DIM, 5, RIEMANN_PF_MIX, false>(CMP_PF_ARGS)
+                                       : stage4<DIM, 5, RIEMANN_PF_PHASIC, false>(CMP_PF_ARGS);
+        if (lc.scheme == 6) return mix ? stage4<DIM, 6, RIEMANN_PF_MIX, false>(CMP_PF_ARGS)
+                                       : stage4<DIM, 6, RIEMANN_PF_PHASIC, false>(CMP_PF_ARGS);
+        if (lc.scheme == 1) return mix ? stage4<DIM, 1, RIEMANN_PF_MIX, false>(CMP_PF_ARGS)
+                                       : stage4<DIM, 1, RIEMANN_PF_PHASIC, false>(CMP_PF_ARGS);
+        if (lc.scheme == 0) return mix ? stage4<DIM, 0, RIEMANN_PF_MIX, false>(CMP_PF_ARGS)
+                                       : stage4<DIM, 0, RIEMANN_PF_PHASIC, false>(CMP_PF_ARGS);
+#undef CMP_PF_ARGS
+        return hipErrorInvalidValue;
+    }
     CMP_DISPATCH(stage4, kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout,
-                 dt, cmax_bits, err, s);
+                 dt, cmax_bits, err, s, nullptr);
     return hipErrorInvalidValue;
 }
 
@@ -590,12 +618,20 @@ hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev
 }
 
 template <int DIM>
-hipError_t launch_fill_dim(const FillDev& fd, long long nghost, double* U, int* err, hipStream_t s)
+hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, double* U, int* err,
+                           hipStream_t s)
 {
     if (nghost <= 0) return hipSuccess;
     long long blocks = (nghost + NT - 1) / NT;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL((k_fill<DIM>), dim3((unsigned)blocks), dim3(NT), 0, s, fd, nghost, U, err);
+    if (N == 4 + DIM)
+        hipLaunchKernelGGL((k_fill<DIM, 4 + DIM>), dim3((unsigned)blocks), dim3(NT), 0, s, fd, nghost, U, err);
+    else if (N == DIM)
+        hipLaunchKernelGGL((k_fill<DIM, DIM>), dim3((unsigned)blocks), dim3(NT), 0, s, fd, nghost, U, err);
+    else if (N == 1)
+        hipLaunchKernelGGL((k_fill<DIM, 1>), dim3((unsigned)blocks), dim3(NT), 0, s, fd, nghost, U, err);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -621,10 +657,10 @@ template hipError_t launch_stage_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KPa
                                                      const TileDev*, int, const BoxDev*, const BoxDev*,
                                                      int, const double*, const double*, double*,
                                                      const double*, unsigned long long*, int*,
-                                                     hipStream_t);
+                                                     hipStream_t, const double*);
 template hipError_t launch_faces_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev&,
                                                      int, const double*, double*, double*, hipStream_t);
-template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(const FillDev&, long long, double*, int*,
+template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(int, const FillDev&, long long, double*, int*,
                                                     hipStream_t);
 template hipError_t launch_dt_dim<CMP_KERNEL_DIM>(const double*, double*, double*, double, int,
                                                   double*, int*, int*, hipStream_t);

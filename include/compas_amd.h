@@ -80,6 +80,25 @@ typedef struct cmp_box {
     long long offset;      /* element offset of this box's fab inside the level buffer              */
 } cmp_box;
 
+/* Phase-field regularization parameters: the subset of PhaseField_Parameter
+ * (include/Physics_PhaseField/PhaseField_Parameter.H:29-144) that the ACDI path reads. */
+#define CMP_PF_ACDI 3      /* PhaseField.Mechanism = "ACDI" (ID_Mechanism 3, flux based)             */
+typedef struct cmp_pf_params {
+    int    mechanism;        /* ID_Mechanism; CMP_PF_ACDI is the one built here                      */
+    int    explicit_rc;      /* ID_ExplicitRC: 1 = Grad_Q from the time-averaged flux Jhat;
+                              * 0 (implicit, an elliptic solve) is not built here                    */
+    int    id_flux;          /* ID_Flux: 0 = UD flux                                                  */
+    int    flux_phase2flux;  /* ID_FluxPhase2Flux: 1 = mixture formulation, 0 = phasic              */
+    double cfl;              /* PhaseField.cfl of the Phase-Field sub-steps                           */
+    double lambda;           /* PhaseField.Lambda; <= 0 means 1                                       */
+    double eta;              /* PhaseField.Eta; <= 0 means eta_multiplier * max_d dx_d                */
+    double eta_factor;       /* PhaseField.Eta_Factor (re-sharpening width of Grad_Q's weight)        */
+    double eta_multiplier;   /* PhaseField.Eta_Multiplier                                             */
+    double p_min[2];         /* phasic energy floors (phasic formulation only)                        */
+    int    max_step;         /* PhaseField.max_step: cap on Phase-Field sub-steps per flow step       */
+    int    change_limiter;   /* timestep_change_limiter applied to the sub-step size                  */
+} cmp_pf_params;
+
 /* ---- layout helpers (host only, no GPU) ------------------------------------------------------- */
 
 /* Elements of one fab: prod_d (hi_d - lo_d + 1 + 2 ng) over active dims, times ncomp. */
@@ -88,8 +107,17 @@ long long cmp_fab_size(int dim, const cmp_box* box, int ng, int ncomp);
 /* Assigns consecutive offsets to the boxes of a level; *total receives the level buffer length. */
 int cmp_level_layout(int dim, int nbox, cmp_box* boxes, int ng, int ncomp, long long* total);
 
+/* Face-array layout of a level: for box b and direction d (d < dim) a nodal array of
+ * prod_e (hi_e - lo_e + 1 + [e == d]) faces times ncomp, no ghosts, placed box-major:
+ * offsets[b*dim + d]. *total receives the buffer length. */
+int cmp_face_layout(int dim, int nbox, const cmp_box* boxes, int ncomp, long long* offsets,
+                    long long* total);
+
 /* sizeof(cmp_params), sizeof(cmp_geometry), sizeof(cmp_box): lets bindings check their mirrors. */
 int cmp_abi_sizes(int* params_size, int* geometry_size, int* box_size);
+
+/* sizeof(cmp_pf_params). */
+int cmp_abi_pf_size(int* pf_params_size);
 
 /* Library version string. */
 const char* cmp_version(void);
@@ -160,7 +188,9 @@ int cmp_level_compute_dt(cmp_level* level, double* cmax, double* dt_dev, double 
  * t = 0: step 0 takes its wave speed from the initial state (CalculateInitialcmax) and no
  * previous dt limits it; with resume != 0 it continues from *cmax and *dt_dev of an earlier call
  * on the same buffers. dt_hist (device, nsteps doubles, or NULL) receives the dt of each step of
- * this call. With use_graph != 0 one step is captured into a hipGraph once and replayed. */
+ * this call. With use_graph != 0 one step is captured into a hipGraph once and replayed.
+ * After cmp_level_pf_setup every step ends with cmp_level_pf_evolve (Evolve,
+ * source/Class_Compressible_PhaseField.cpp:307-312); the graph then holds the flow part only. */
 int cmp_level_advance(cmp_level* level, double* Un, double* U1, double* U2, int nsteps,
                       double cfl, int limiter, double* dt_dev, double* cmax, double* dt_hist,
                       int use_graph, int resume, void* stream);
@@ -171,6 +201,46 @@ int cmp_level_check(cmp_level* level, void* stream);
 
 /* Number of tiles in the level work-list and the tile shape used (for reporting). */
 int cmp_level_info(const cmp_level* level, int* ntiles, int* tile_x, int* tile_y, int* tile_z);
+
+/* ---- phase-field interface regularization of a level ------------------------------------------
+ * Face buffers below follow cmp_face_layout with one component (NPHASE = 1 for two phases). */
+
+/* Enables the Phase-Field step on a level and allocates its device work buffers (two copies of
+ * alpha1 and the cell normals with one ghost cell; J, Jhat and Grad_Q on the faces). */
+int cmp_level_pf_setup(cmp_level* level, const cmp_pf_params* pf);
+
+/* Phase-Field right-hand side of every box, PhaseField_compute_dUdt_FV
+ * (source/Class_Compressible_PhaseField.cpp:951-1003): faces by PhaseField_ComputeFaceFluxes
+ * (:771-852) with FVM_Conservative2FluxPhaseField_K
+ * (include/Operators_Compressible5Eq_PhaseField/FVM_Conservative2FluxPhaseField_K_Compressible5Eq_PhaseField.H:15-74),
+ * stored as -fhat, and the conservative surface integral of PhaseField_FluxDivergence (:859-948).
+ * U ghosts filled; dUdt is a level buffer laid out like U (ghost cells not written). */
+int cmp_level_pf_rhs(cmp_level* level, const double* U, const double* gradq, double* dUdt,
+                     void* stream);
+
+/* Grad_Q for a flow step of size dt from the valid cells of U: MaxVelocity (:3541-3594), the
+ * mobility 0.5 eta max|u| (PhaseField_Evolve :425-465), the ACDI sub-steps of
+ * PhaseFieldMechanism_Evolve (:597-709, PhaseField_ACDI :1495-1644, PhaseField_Normal :1958-2032)
+ * accumulating Jhat, then PhaseField_GradQPF with the explicit RC (:2315-2403).
+ * gradq: device face buffer (written); *gradq_max (host) = max |Grad_Q| over all faces;
+ * info (host, 4 doubles, or NULL) = max|u|, mobility, eta, number of mechanism sub-steps.
+ * Synchronises the stream. */
+int cmp_level_pf_gradq(cmp_level* level, const double* U, double dt, double* gradq,
+                       double* gradq_max, double* info, void* stream);
+
+/* One Phase-Field TVD-RK3 sub-step of size pf_dt in place on Un (PhaseField_ThirdOrderSSPRK,
+ * :2620-2752): the stage combinations of cmp_level_rk3_stage with the Phase-Field right-hand side;
+ * U1, U2 are scratch level buffers. */
+int cmp_level_pf_rk3_step(cmp_level* level, double* Un, double* U1, double* U2,
+                          const double* gradq, double pf_dt, void* stream);
+
+/* The Phase-Field step that follows a flow step of size dt (PhaseField_Evolve, :425-594):
+ * cmp_level_pf_gradq, then sub-steps of PhaseField_ComputeDt (:2037-2098) size,
+ * pf.cfl min_d dx_d / (max|Grad_Q| + 1e-14), the last one cut to end at dt.
+ * dt is read from *dt_dev (device). info (host, 6 doubles, or NULL): the 4 of cmp_level_pf_gradq,
+ * max|Grad_Q| and the number of sub-steps. Synchronises the stream. */
+int cmp_level_pf_evolve(cmp_level* level, double* Un, double* U1, double* U2, const double* dt_dev,
+                        double* info, void* stream);
 
 #ifdef __cplusplus
 }

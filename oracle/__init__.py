@@ -31,12 +31,13 @@ def load():
         from . import build as _b
         _b.build()
     # the product library's struct mirrors are shared (plain PODs)
-    from compas_amd.capi import cmp_box, cmp_geometry, cmp_params
+    from compas_amd.capi import cmp_box, cmp_geometry, cmp_params, cmp_pf_params
     lib = ctypes.CDLL(LIB_PATH)
     D = ctypes.c_double
     PD = ctypes.POINTER(D)
     PI = ctypes.POINTER(ctypes.c_int)
     PP = ctypes.POINTER(cmp_params)
+    PF = ctypes.POINTER(cmp_pf_params)
     PG = ctypes.POINTER(cmp_geometry)
     PB = ctypes.POINTER(cmp_box)
     sigs = {
@@ -68,6 +69,25 @@ def load():
                                              ctypes.c_int, D, ctypes.c_int, PD]),
         "ref_num_threads": (ctypes.c_int, []),
         "ref_set_num_threads": (None, [ctypes.c_int]),
+        "ref_level_fill_pf": (ctypes.c_int, [ctypes.c_int, PG, ctypes.c_int, PB, ctypes.c_int,
+                                             ctypes.c_int, PD]),
+        "ref_pf_map_backward": (D, [D, D]),
+        "ref_pf_map_forward": (D, [D, D, ctypes.c_int]),
+        "ref_pf_acdi_flux": (D, [D, D, D, D, D, D, D]),
+        "ref_pf_gradq_face": (D, [D, D, D, D, D]),
+        "ref_pf_flux": (None, [PP, PF, PD, PD, D, PD, PD, PI]),
+        "ref_pf_coefficients": (None, [PF, PG, D, PD, PD, PD]),
+        "ref_pf_compute_dt": (D, [PF, PG, D, D, D, PD]),
+        "ref_level_pf_rhs": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, PD]),
+        "ref_level_pf_rk3_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, D]),
+        "ref_level_max_velocity": (D, [PP, ctypes.c_int, PB, ctypes.c_int, PD]),
+        "ref_level_pf_mechanism": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, D, D, D, D, PD, PI]),
+        "ref_level_pf_gradq": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, PD, D, PD]),
+        "ref_level_pf_gradq_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, D,
+                                                   PD, PD, PD]),
+        "ref_level_pf_evolve": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, D, PD]),
+        "ref_level_advance_pf": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD,
+                                                ctypes.c_int, D, ctypes.c_int, PD]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -243,6 +263,136 @@ def level_advance(params, geom, layout, buf, nsteps, cfl, limiter=1):
                                   int(limiter), _dptr(hist))
     if rc:
         raise RuntimeError("ref_level_advance failed: %d" % rc)
+    return hist[:nsteps]
+
+
+# ---- phase field ---------------------------------------------------------------------------------
+
+def pf_map_backward(phi, eta):
+    return load().ref_pf_map_backward(float(phi), float(eta))
+
+
+def pf_map_forward(psi, eta, deriv=0):
+    return load().ref_pf_map_forward(float(psi), float(eta), int(deriv))
+
+
+def pf_acdi_flux(pf_m, pf_c, n_m, n_c, mob, eta, dx):
+    return load().ref_pf_acdi_flux(*[float(x) for x in (pf_m, pf_c, n_m, n_c, mob, eta, dx)])
+
+
+def pf_gradq_face(pf_m, pf_c, jhat, eta, eta_factor):
+    return load().ref_pf_gradq_face(*[float(x) for x in (pf_m, pf_c, jhat, eta, eta_factor)])
+
+
+def pf_flux(params, pf, UL, UR, g):
+    """fhat (nstate) and the UD wave speed of one face."""
+    UL = np.ascontiguousarray(UL, dtype=np.float64)
+    UR = np.ascontiguousarray(UR, dtype=np.float64)
+    F = np.zeros(params.nstate)
+    c = ctypes.c_double(0.0)
+    err = ctypes.c_int(0)
+    load().ref_pf_flux(ctypes.byref(params), ctypes.byref(pf), _dptr(UL), _dptr(UR), float(g),
+                       _dptr(F), ctypes.byref(c), ctypes.byref(err))
+    return F, c.value
+
+
+def pf_coefficients(pf, geom, max_v):
+    L, e, m = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    load().ref_pf_coefficients(ctypes.byref(pf), ctypes.byref(geom), float(max_v), ctypes.byref(L),
+                               ctypes.byref(e), ctypes.byref(m))
+    return L.value, e.value, m.value
+
+
+def pf_compute_dt(pf, geom, gradq_max, pf_time, dt, prev):
+    p = ctypes.c_double(prev)
+    r = load().ref_pf_compute_dt(ctypes.byref(pf), ctypes.byref(geom), float(gradq_max), float(pf_time),
+                                 float(dt), ctypes.byref(p))
+    return r, p.value
+
+
+def level_fill_pf(geom, layout, buf):
+    """Ghost fill of a Phase-Field field level buffer (periodic or foextrap)."""
+    rc = load().ref_level_fill_pf(layout.dim, ctypes.byref(geom), layout.nbox, layout.cboxes,
+                                  layout.ng, layout.ncomp, _dptr(buf))
+    if rc:
+        raise RuntimeError("ref_level_fill_pf failed: %d" % rc)
+
+
+def level_pf_rhs(params, pf, geom, layout, buf, gradq):
+    dUdt = np.zeros_like(buf)
+    rc = load().ref_level_pf_rhs(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom), layout.nbox,
+                                 layout.cboxes, layout.ng, _dptr(buf), _dptr(gradq), _dptr(dUdt))
+    if rc:
+        raise RuntimeError("ref_level_pf_rhs failed: %d" % rc)
+    return dUdt
+
+
+def level_pf_rk3_step(params, pf, geom, layout, buf, gradq, pf_dt):
+    rc = load().ref_level_pf_rk3_step(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
+                                      layout.nbox, layout.cboxes, layout.ng, _dptr(buf), _dptr(gradq),
+                                      float(pf_dt))
+    if rc:
+        raise RuntimeError("ref_level_pf_rk3_step failed: %d" % rc)
+
+
+def level_max_velocity(params, layout, buf):
+    return load().ref_level_max_velocity(ctypes.byref(params), layout.nbox, layout.cboxes, layout.ng,
+                                         _dptr(buf))
+
+
+def level_pf_mechanism(params, pf, geom, alayout, alpha, dt, mob, eta, lam, nfaces):
+    """Jhat (face buffer of nfaces) and the number of sub-steps; alpha: 1 comp, 1 ghost, filled."""
+    jhat = np.zeros(nfaces)
+    n = ctypes.c_int(0)
+    rc = load().ref_level_pf_mechanism(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
+                                       alayout.nbox, alayout.cboxes, _dptr(alpha), float(dt), float(mob),
+                                       float(eta), float(lam), _dptr(jhat), ctypes.byref(n))
+    if rc:
+        raise RuntimeError("ref_level_pf_mechanism failed: %d" % rc)
+    return jhat, n.value
+
+
+def level_pf_gradq(params, pf, geom, alayout, alpha, jhat, eta):
+    gq = np.zeros_like(jhat)
+    rc = load().ref_level_pf_gradq(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
+                                   alayout.nbox, alayout.cboxes, _dptr(alpha), _dptr(jhat), float(eta),
+                                   _dptr(gq))
+    if rc:
+        raise RuntimeError("ref_level_pf_gradq failed: %d" % rc)
+    return gq
+
+
+def level_pf_gradq_step(params, pf, geom, layout, buf, dt, nfaces):
+    """(gradq face buffer, max|Grad_Q|, [max|u|, mobility, eta, mechanism sub-steps])."""
+    gq = np.zeros(nfaces)
+    gm = ctypes.c_double(0.0)
+    info = np.zeros(4)
+    rc = load().ref_level_pf_gradq_step(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
+                                        layout.nbox, layout.cboxes, layout.ng, _dptr(buf), float(dt),
+                                        _dptr(gq), ctypes.byref(gm), _dptr(info))
+    if rc:
+        raise RuntimeError("ref_level_pf_gradq_step failed: %d" % rc)
+    return gq, gm.value, list(info)
+
+
+def level_pf_evolve(params, pf, geom, layout, buf, dt):
+    """In place on buf; returns [max|u|, mobility, eta, mech sub-steps, max|Grad_Q|, PF sub-steps]."""
+    info = np.zeros(6)
+    rc = load().ref_level_pf_evolve(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
+                                    layout.nbox, layout.cboxes, layout.ng, _dptr(buf), float(dt),
+                                    _dptr(info))
+    if rc:
+        raise RuntimeError("ref_level_pf_evolve failed: %d" % rc)
+    return list(info)
+
+
+def level_advance_pf(params, pf, geom, layout, buf, nsteps, cfl, limiter=1):
+    hist = np.zeros(max(1, nsteps))
+    rc = load().ref_level_advance_pf(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
+                                     layout.nbox, layout.cboxes, layout.ng, _dptr(buf), int(nsteps),
+                                     float(cfl), int(limiter), _dptr(hist))
+    if rc:
+        raise RuntimeError("ref_level_advance_pf failed: %d" % rc)
     return hist[:nsteps]
 
 

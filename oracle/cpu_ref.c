@@ -756,10 +756,12 @@ static int box_contains(int dim, const cmp_box* b, const int* c)
     return 1;
 }
 
-int ref_level_fill_ghosts(const cmp_params* P, const cmp_geometry* G, int nbox,
-                          const cmp_box* boxes, int ng, double* U)
+/* pf_bcs != 0: the Phase-Field fields' rule (PF_bcs, LS_bcs and Normal_bcs of
+ * source/Class_Compressible_PhaseField.cpp:471-488, 1516-1530, 1970-1986): int_dir on periodic
+ * sides, foextrap on every other side, no sign changes. */
+static int fill_level(int dim, int N, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                      int ng, double* U, int pf_bcs)
 {
-    const int dim = P->dim, N = P->nstate;
     int err = 0;
     for (int b = 0; b < nbox; ++b) {
         const cmp_box* bx = &boxes[b];
@@ -788,6 +790,7 @@ int ref_level_fill_ghosts(const cmp_params* P, const cmp_geometry* G, int nbox,
                             continue;
                         }
                         int code = (src[d] < dlo) ? G->lo_bc[d] : G->hi_bc[d];
+                        if (pf_bcs) code = 2;
                         if (code == 2 || code == 0) {          /* foextrap (Interior on a non-periodic side: treat as outflow) */
                             src[d] = (src[d] < dlo) ? dlo : dhi;
                         } else if (code >= 3 && code <= 5) { /* reflect */
@@ -815,6 +818,18 @@ int ref_level_fill_ghosts(const cmp_params* P, const cmp_geometry* G, int nbox,
         if (e > err) err = e;
     }
     return err;
+}
+
+int ref_level_fill_ghosts(const cmp_params* P, const cmp_geometry* G, int nbox,
+                          const cmp_box* boxes, int ng, double* U)
+{
+    return fill_level(P->dim, P->nstate, G, nbox, boxes, ng, U, 0);
+}
+
+int ref_level_fill_pf(int dim, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
+                      int ncomp, double* A)
+{
+    return fill_level(dim, ncomp, G, nbox, boxes, ng, A, 1);
 }
 
 /* ------------------------------------------------------------------------------------------------
@@ -860,8 +875,19 @@ int ref_level_rhs(const cmp_params* P, const cmp_geometry* G, int nbox, const cm
  *   stage 2: Uborder = FillPatch(U1);  U2 = LinComb(1/4, Uborder, 3/4, Un); Saxpy(dt/4)  (:4046-4047)
  *   stage 3: Uborder = FillPatch(U2);  U3 = LinComb(2/3, Uborder, 1/3, Un); Saxpy(2dt/3) (:4083-4084)
  * AMReX LinComb: dst = a*x + b*y; Saxpy: dst += a*src. c_max accumulates over the three stages. */
-int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
-                       const cmp_box* boxes, int ng, double* Un, double dt, double* cmax)
+/* The right-hand side a RK3 step evaluates: the flow's (compute_dUdt_FV) or the Phase-Field's
+ * (PhaseField_compute_dUdt_FV with a fixed Grad_Q). */
+typedef struct {
+    const cmp_pf_params* pf;   /* NULL: flow right-hand side */
+    const double* gradq;
+} rk3_rhs_t;
+
+int ref_level_pf_rhs(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                     int nbox, const cmp_box* boxes, int ng, const double* U, const double* gradq,
+                     double* dUdt);
+
+static int rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                    int ng, double* Un, double dt, double* cmax, const rk3_rhs_t* R)
 {
     const int dim = P->dim, N = P->nstate;
     long long total = 0;
@@ -874,7 +900,9 @@ int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
     for (int stage = 0; stage < 3; ++stage) {
         memcpy(Ub, stage == 0 ? Un : Unew, sizeof(double) * total);
         e = ref_level_fill_ghosts(P, G, nbox, boxes, ng, Ub); if (e && !err) err = e;
-        e = ref_level_rhs(P, G, nbox, boxes, ng, Ub, L, cmax); if (e && !err) err = e;
+        if (R && R->pf) e = ref_level_pf_rhs(P, R->pf, G, nbox, boxes, ng, Ub, R->gradq, L);
+        else            e = ref_level_rhs(P, G, nbox, boxes, ng, Ub, L, cmax);
+        if (e && !err) err = e;
         for (int b = 0; b < nbox; ++b) {
             fab_t f = make_fab(dim, &boxes[b], ng);
             int nv[3] = {1, 1, 1};
@@ -915,6 +943,12 @@ int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
     }
     free(Ub); free(Unew); free(L);
     return err;
+}
+
+int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
+                       const cmp_box* boxes, int ng, double* Un, double dt, double* cmax)
+{
+    return rk3_step(P, G, nbox, boxes, ng, Un, dt, cmax, NULL);
 }
 
 /* CalculateInitialcmax (source/Class_Compressible_PhaseField.cpp:4502-4605): FillPatch into a
@@ -978,9 +1012,669 @@ int ref_level_advance(const cmp_params* P, const cmp_geometry* G, int nbox, cons
 }
 
 /* ------------------------------------------------------------------------------------------------
+ * Phase-field regularization: ACDI mechanism with the explicit RC, UD Phase-Field flux
+ * ---------------------------------------------------------------------------------------------- */
+
+/* Mapping_Backward (include/Physics_PhaseField/Mapping_Backward.H:12-29):
+ * psi = sqrt(2) eta atanh(2 phi - 1), phi clipped to [1e-12, 1 - 1e-12]. */
+double ref_pf_map_backward(double phi, double eta)
+{
+    const double ep = 1.0e-12;
+    double t = phi;
+    if (t < ep) t = ep;
+    else if (t > 1.0 - ep) t = 1.0 - ep;
+    return sqrt(2.0) * eta * atanh(2.0 * t - 1.0);
+}
+
+/* Mapping_Forward (include/Physics_PhaseField/Mapping_Forward.H:12-26):
+ * deriv 0: phi = 1/2 + 1/2 tanh(psi/(sqrt(2) eta)); deriv 1: dphi/dpsi. */
+double ref_pf_map_forward(double psi, double eta, int deriv)
+{
+    if (deriv == 0) return 0.5 + 0.5 * tanh(psi / (sqrt(2.0) * eta));
+    double x = psi / (sqrt(2.0) * eta);
+    return 0.5 / (sqrt(2.0) * eta) / cosh(x) / cosh(x);
+}
+
+/* FDM_PhaseFieldFlux_ACDI_D (include/Operators_PhaseField/FDM_PhaseFieldFlux_ACDI_D.H:12-45) with
+ * PhaseFieldFlux_ACDI (include/Physics_PhaseField/PhaseFieldFlux_ACDI.H:12-26): the face flux
+ * J = M (grad phi - F'(psi_f) n_f), psi and n averaged to the face, grad phi = (phi_c - phi_m)/dx. */
+double ref_pf_acdi_flux(double pf_m, double pf_c, double n_m, double n_c, double mob, double eta,
+                        double dx)
+{
+    double psi_c = ref_pf_map_backward(pf_c, eta);
+    double psi_m = ref_pf_map_backward(pf_m, eta);
+    double grad = (pf_c - pf_m) / dx;
+    double psi = 0.5 * psi_m + 0.5 * psi_c;
+    double nrm = 0.5 * n_m + 0.5 * n_c;
+    double F1 = ref_pf_map_forward(psi, eta, 1);
+    return mob * (grad - F1 * nrm);
+}
+
+/* The explicit Grad_Q of one face (source/Class_Compressible_PhaseField.cpp:2358-2391) with
+ * PhaseFieldFlux2GradQPF (include/Physics_PhaseField/PhaseFieldFlux2GradQPF.H:12-28): alpha re-sharpened
+ * through the mappings at eta and eta_factor*eta, averaged to the face, then
+ * Grad_Q = Jhat/(phi(1-phi) + 1e-3), with Jhat taken as 0 where phi is outside [1e-3, 1 - 1e-3]. */
+double ref_pf_gradq_face(double pf_m, double pf_c, double jhat, double eta, double eta_factor)
+{
+    const double epsilon = 1.0e-3;
+    double psi_c = ref_pf_map_backward(pf_c, eta);
+    double psi_m = ref_pf_map_backward(pf_m, eta);
+    double c = ref_pf_map_forward(psi_c, eta_factor * eta, 0);
+    double m = ref_pf_map_forward(psi_m, eta_factor * eta, 0);
+    double pf = 0.5 * m + 0.5 * c;
+    double flux = (pf < epsilon || pf > 1.0 - epsilon) ? 0.0 : jhat;
+    double w = pf * (1.0 - pf);
+    return flux / (w + epsilon);
+}
+
+/* f_Upwind, f_Downwind (include/Operators_FV/f_Upwind.H:12-27, f_Downwind.H:12-27) */
+static inline double f_up(double fL, double fR, double V)
+{
+    return (V > 0.0) ? fL : ((V < 0.0) ? fR : 0.5 * (fL + fR));
+}
+static inline double f_down(double fL, double fR, double V)
+{
+    return (V > 0.0) ? fR : ((V < 0.0) ? fL : 0.5 * (fL + fR));
+}
+
+/* EquationOfState_Phase, pressure to alpha_k rho_k e_k
+ * (include/Physics_Compressible5Eq/EquationOfState_Phase_Compressible5Eq.H:15-71) */
+static double ref_eos_phase_e(double p, double alpha, double alpha_rho, double A, double B,
+                              double C, double D, int* err)
+{
+    double phi = ref_nasg_phi(alpha, alpha_rho, C, err);
+    return phi * A * p + phi * B + alpha_rho * D;
+}
+
+/* Conservative2ConservativePhase (include/Physics_Compressible5Eq/Conservative2ConservativePhase_Compressible5Eq.H:15-111):
+ * the phasic vector [a1r1, a2r2, a1r1 u, a2r2 u, (v), (w), a1r1 E1, a2r2 E2, a1, a2] with the phasic
+ * internal energies floored at p_min. */
+static void ref_cons2phase(const cmp_params* P, const cmp_pf_params* PF, const double* U,
+                           double* Cp, int* err)
+{
+    const int d = P->dim;
+    double W[REF_NMAX];
+    ref_cons2prim(P, U, W, err);
+    double a1r1 = W[0], a2r2 = W[1], p = W[2 + d], a1 = W[3 + d];
+    double a2 = 1.0 - a1;
+    double e1 = ref_eos_phase_e(p, a1, a1r1, P->eos_A[0], P->eos_B[0], P->eos_C[0], P->eos_D[0], err);
+    double e1m = ref_eos_phase_e(PF->p_min[0], a1, a1r1, P->eos_A[0], P->eos_B[0], P->eos_C[0],
+                                 P->eos_D[0], err);
+    double e2 = ref_eos_phase_e(p, a2, a2r2, P->eos_A[1], P->eos_B[1], P->eos_C[1], P->eos_D[1], err);
+    double e2m = ref_eos_phase_e(PF->p_min[1], a2, a2r2, P->eos_A[1], P->eos_B[1], P->eos_C[1],
+                                 P->eos_D[1], err);
+    double rho_e = e1 + e2;
+    if (e1 < e1m) { e1 = e1m; e2 = rho_e - e1; }
+    else if (e2 < e2m) { e2 = e2m; e1 = rho_e - e2; }
+    double ke = W[2] * W[2];
+    for (int m = 1; m < d; ++m) ke = ke + W[2 + m] * W[2 + m];
+    Cp[0] = a1r1;
+    Cp[1] = a2r2;
+    for (int m = 0; m < d; ++m) { Cp[2 + 2 * m] = a1r1 * W[2 + m]; Cp[3 + 2 * m] = a2r2 * W[2 + m]; }
+    Cp[(1 + d) * 2] = e1 + 0.5 * a1r1 * ke;
+    Cp[(1 + d) * 2 + 1] = e2 + 0.5 * a2r2 * ke;
+    Cp[(2 + d) * 2] = a1;
+    Cp[(2 + d) * 2 + 1] = a2;
+}
+
+/* FluxPhaseField (include/Physics_Compressible5Eq_PhaseField/FluxPhaseField_Compressible5Eq_PhaseField.H:12-65)
+ * with the UD phasic flux FluxPhasePhaseField_UD
+ * (include/Physics_Compressible_TwoPhase_PhaseField/FluxPhasePhaseField_UD_Compressible_TwoPhase_PhaseField.H:12-194):
+ * Grad_Q1 = g, Grad_Q2 = -g; phase k's quantities are taken downwind of Grad_Qk and the other
+ * phase's volume fraction upwind of it. Then either the phasic sum (VectorPhase2Vector,
+ * include/Physics_Compressible5Eq/VectorPhase2Vector_Compressible5Eq.H:15-54) or the mixture form
+ * FluxPhaseField_Mixture (include/Physics_Compressible5Eq_PhaseField/FluxPhaseField_Mixture_Compressible5Eq_PhaseField.H:12-116). */
+void ref_pf_flux(const cmp_params* P, const cmp_pf_params* PF, const double* UL, const double* UR,
+                 double g, double* flux, double* cmax, int* err)
+{
+    const int d = P->dim;
+    const int nph = 2 * (3 + d);
+    double CL[2 * (3 + 3)], CR[2 * (3 + 3)], Jp[2 * (3 + 3)];
+    ref_cons2phase(P, PF, UL, CL, err);
+    ref_cons2phase(P, PF, UR, CR, err);
+    const int iA = (2 + d) * 2;
+    double g1 = g, g2 = -g1;
+    double c = 0.0;
+    double a2u = f_up(CL[iA + 1], CR[iA + 1], g1);
+    c = smax(c, fabs(a2u * g1));
+    double a1u = f_up(CL[iA], CR[iA], g2);
+    c = smax(c, fabs(a1u * g2));
+    /* phase 1 components are the even entries, phase 2 the odd ones */
+    for (int q = 0; q < nph; q += 2) {
+        Jp[q] = f_down(CL[q], CR[q], g1) * a2u * g1;
+        Jp[q + 1] = f_down(CL[q + 1], CR[q + 1], g2) * a1u * g2;
+    }
+    *cmax = c;
+    const int N = P->nstate;
+    if (PF->flux_phase2flux == 0) {
+        flux[0] = Jp[0];
+        flux[1] = Jp[1];
+        for (int m = 0; m < d; ++m) flux[2 + m] = Jp[2 + 2 * m] + Jp[3 + 2 * m];
+        flux[2 + d] = Jp[(1 + d) * 2] + Jp[(1 + d) * 2 + 1];
+        flux[3 + d] = Jp[iA];
+        return;
+    }
+    double J1r1 = Jp[0], J2r2 = Jp[1], J1 = Jp[iA], J2 = Jp[iA + 1];
+    double WL[REF_NMAX], WR[REF_NMAX];
+    ref_cons2prim(P, UL, WL, err);
+    ref_cons2prim(P, UR, WR, err);
+    double M = J1r1 + J2r2;
+    double uD[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < d; ++m) uD[m] = f_down(WL[2 + m], WR[2 + m], M);
+    double ke = uD[0] * uD[0];
+    for (int m = 1; m < d; ++m) ke = ke + uD[m] * uD[m];
+    double eK = 0.5 * ke;
+    double MeK = M * eK;
+    double p1 = f_down(WL[2 + d], WR[2 + d], J1);
+    double J1phi = (P->eos_C[0] > 0.0) ? J1 - P->eos_C[0] * J1r1 : J1;
+    double J1e1 = P->eos_A[0] * J1phi * p1 + P->eos_B[0] * J1phi + P->eos_D[0] * J1r1;
+    double p2 = f_down(WL[2 + d], WR[2 + d], J2);
+    double J2phi = (P->eos_C[1] > 0.0) ? J2 - P->eos_C[1] * J2r2 : J2;
+    double J2e2 = P->eos_A[1] * J2phi * p2 + P->eos_B[1] * J2phi + P->eos_D[1] * J2r2;
+    flux[0] = J1r1;
+    flux[1] = J2r2;
+    for (int m = 0; m < d; ++m) flux[2 + m] = M * uD[m];
+    flux[2 + d] = MeK + J1e1 + J2e2;
+    flux[3 + d] = J1;
+    (void)N;
+}
+
+/* FVM_Conservative2FluxPhaseField_K, ID_Bound == 0
+ * (include/Operators_Compressible5Eq_PhaseField/FVM_Conservative2FluxPhaseField_K_Compressible5Eq_PhaseField.H:15-74):
+ * the base reconstruction of FVM_Reconstruction2FaceLR_Conservative_K (no THINC on this path),
+ * then FluxPhaseField; FluxNC is zero. fhat is the flux before PhaseField_ComputeFaceFluxes
+ * stores it negated (source/Class_Compressible_PhaseField.cpp:840). */
+int ref_pf_face_flux(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                     const cmp_box* box, int ng, const double* U, int i, int j, int k, int dir,
+                     double gradq, double* fhat, double* cmax)
+{
+    const int N = P->nstate;
+    int err = 0;
+    fab_t f = make_fab(P->dim, box, ng);
+    int off[3] = {0, 0, 0};
+    off[dir] = 1;
+    double Ust[6][REF_NMAX];
+    int smin_ = -3, smax_ = 2;
+    if (P->id_weno == 0 && P->id_limiter == 0) { smin_ = -1; smax_ = 0; }
+    else if (P->id_weno == 0) { smin_ = -2; smax_ = 1; }
+    for (int s = smin_; s <= smax_; ++s) {
+        long long c = fab_idx(&f, i + s * off[0], j + s * off[1], k + s * off[2]);
+        for (int n = 0; n < N; ++n) Ust[s + 3][n] = U[c + n * f.sn];
+    }
+    double UL[REF_NMAX], UR[REF_NMAX];
+    int r = ref_recon_lr(P, Ust, P->id_weno, P->id_limiter, G->dx[dir], UL, UR, &err);
+    if (r) return r;
+    ref_pf_flux(P, PF, UL, UR, gradq, fhat, cmax, &err);
+    return err;
+}
+
+/* One box of PhaseField_ComputeFaceFluxes (:771-852) and PhaseField_FluxDivergence (:859-948):
+ * flux = -fhat on every face, dUdt = -sum_d (flux_+ - flux_-)/dx_d (FVM_SurfaceIntegral); the
+ * five-equation Phase-Field NC integral is empty
+ * (include/Operators_Compressible5Eq_PhaseField/FVM_PhaseField_SurfaceIntegralNC_Compressible5Eq_PhaseField.H).
+ * gq[d]: this box's nodal Grad_Q faces of direction d. */
+int ref_pf_rhs_box(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                   const cmp_box* box, int ng, const double* U, const double* const* gq,
+                   double* dUdt)
+{
+    const int dim = P->dim, N = P->nstate;
+    int nv[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) nv[d] = box->hi[d] - box->lo[d] + 1;
+    long long ncell = (long long)nv[0] * nv[1] * nv[2];
+    double* F[3] = {NULL, NULL, NULL};
+    long long sf[3][3], nf[3];
+    int err = 0;
+    for (int d = 0; d < dim; ++d) {
+        int fe[3] = {nv[0], nv[1], nv[2]};
+        fe[d] += 1;
+        long long nface = (long long)fe[0] * fe[1] * fe[2];
+        sf[d][0] = 1; sf[d][1] = fe[0]; sf[d][2] = (long long)fe[0] * fe[1];
+        nf[d] = nface;
+        F[d] = (double*)malloc(sizeof(double) * nface * N);
+        int e = 0;
+        int lo2 = (dim == 3) ? box->lo[2] : 0;
+#pragma omp parallel for collapse(2) reduction(max:e) schedule(static)
+        for (int kk = 0; kk < fe[2]; ++kk)
+            for (int jj = 0; jj < fe[1]; ++jj)
+                for (int ii = 0; ii < fe[0]; ++ii) {
+                    long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                    double fh[REF_NMAX], c;
+                    int r = ref_pf_face_flux(P, PF, G, box, ng, U, box->lo[0] + ii, box->lo[1] + jj,
+                                             lo2 + kk, d, gq[d][q], fh, &c);
+                    if (r > e) e = r;
+                    for (int n = 0; n < N; ++n) F[d][q + n * nface] = -fh[n];
+                }
+        if (e && !err) err = e;
+    }
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int kk = 0; kk < nv[2]; ++kk)
+        for (int jj = 0; jj < nv[1]; ++jj)
+            for (int ii = 0; ii < nv[0]; ++ii) {
+                long long cell = ii + (long long)nv[0] * (jj + (long long)nv[1] * kk);
+                int idx[3] = {ii, jj, kk};
+                for (int n = 0; n < N; ++n) {
+                    double div = 0.0;
+                    for (int d = 0; d < dim; ++d) {
+                        long long qm = idx[0] * sf[d][0] + idx[1] * sf[d][1] + idx[2] * sf[d][2];
+                        long long qp = qm + sf[d][d];
+                        double term = (F[d][qp + n * nf[d]] - F[d][qm + n * nf[d]]) / G->dx[d];
+                        div = (d == 0) ? term : div + term;
+                    }
+                    dUdt[cell + n * ncell] = -div;
+                }
+            }
+    for (int d = 0; d < dim; ++d) free(F[d]);
+    return err;
+}
+
+int cmp_face_layout(int dim, int nbox, const cmp_box* boxes, int ncomp, long long* offsets,
+                    long long* total);
+
+/* PhaseField_compute_dUdt_FV over every box; gradq is a face buffer (cmp_face_layout, 1 comp). */
+int ref_level_pf_rhs(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                     int nbox, const cmp_box* boxes, int ng, const double* U, const double* gradq,
+                     double* dUdt)
+{
+    const int dim = P->dim, N = P->nstate;
+    long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
+    cmp_face_layout(dim, nbox, boxes, 1, foff, NULL);
+    int err = 0;
+    for (int b = 0; b < nbox; ++b) {
+        long long nc = valid_cells(dim, &boxes[b]);
+        double* tmp = (double*)malloc(sizeof(double) * nc * N);
+        const double* gq[3] = {NULL, NULL, NULL};
+        for (int d = 0; d < dim; ++d) gq[d] = gradq + foff[b * dim + d];
+        int e = ref_pf_rhs_box(P, PF, G, &boxes[b], ng, U + boxes[b].offset, gq, tmp);
+        if (e && !err) err = e;
+        fab_t f = make_fab(dim, &boxes[b], ng);
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = boxes[b].hi[d] - boxes[b].lo[d] + 1;
+        int lo2 = (dim == 3) ? boxes[b].lo[2] : 0;
+        for (int n = 0; n < N; ++n)
+            for (int kk = 0; kk < nv[2]; ++kk)
+                for (int jj = 0; jj < nv[1]; ++jj)
+                    for (int ii = 0; ii < nv[0]; ++ii) {
+                        long long q = fab_idx(&f, boxes[b].lo[0] + ii, boxes[b].lo[1] + jj, lo2 + kk);
+                        dUdt[boxes[b].offset + q + n * f.sn] =
+                            tmp[ii + (long long)nv[0] * (jj + (long long)nv[1] * kk) + n * nc];
+                    }
+        free(tmp);
+    }
+    free(foff);
+    return err;
+}
+
+/* One Phase-Field RK3 sub-step (PhaseField_ThirdOrderSSPRK, :2620-2752): the stage combinations of
+ * ThirdOrderSSPRK with the Phase-Field right-hand side; ghosts by the flow's FillPatch. */
+int ref_level_pf_rk3_step(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                          int nbox, const cmp_box* boxes, int ng, double* Un, const double* gradq,
+                          double pf_dt)
+{
+    rk3_rhs_t R = {PF, gradq};
+    return rk3_step(P, G, nbox, boxes, ng, Un, pf_dt, NULL, &R);
+}
+
+/* MaxVelocity (:3541-3594): max over valid cells of |u|, u = rho u / (a1r1 + a2r2)
+ * (Conservative2Velocity, include/Physics_Compressible5Eq/Conservative2Velocity_Compressible5Eq.H:15-40). */
+double ref_level_max_velocity(const cmp_params* P, int nbox, const cmp_box* boxes, int ng,
+                              const double* U)
+{
+    const int dim = P->dim;
+    double mv = 0.0;
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &boxes[b], ng);
+        const double* B = U + boxes[b].offset;
+        int lo2 = (dim == 3) ? boxes[b].lo[2] : 0, hi2 = (dim == 3) ? boxes[b].hi[2] : 0;
+        for (int k = lo2; k <= hi2; ++k)
+            for (int j = boxes[b].lo[1]; j <= boxes[b].hi[1]; ++j)
+                for (int i = boxes[b].lo[0]; i <= boxes[b].hi[0]; ++i) {
+                    long long c = fab_idx(&f, i, j, k);
+                    double rho = B[c] + B[c + f.sn];
+                    double s = 0.0;
+                    for (int m = 0; m < dim; ++m) {
+                        double u = B[c + (2 + m) * f.sn] / rho;
+                        s = (m == 0) ? u * u : s + u * u;
+                    }
+                    double V = sqrt(s);
+                    if (mv < V || isnan(V)) mv = V;
+                }
+    }
+    return mv;
+}
+
+/* Boxes of a Phase-Field field: the level's boxes with offsets for ncomp comps and ng ghosts. */
+static cmp_box* pf_boxes(int dim, int nbox, const cmp_box* boxes, int ng, int ncomp, long long* total)
+{
+    cmp_box* bb = (cmp_box*)malloc(sizeof(cmp_box) * (size_t)nbox);
+    memcpy(bb, boxes, sizeof(cmp_box) * (size_t)nbox);
+    cmp_level_layout(dim, nbox, bb, ng, ncomp, total);
+    return bb;
+}
+
+/* The ACDI mechanism over one flow step of size dt (PhaseFieldMechanism_Evolve, :597-709, with
+ * PhaseField_Mechanism :1155-1233, PhaseField_ACDI :1495-1644 and PhaseField_Normal :1958-2032):
+ *   dt_m = 0.9/(2 M Lambda)/(sum_d 1/dx_d^2 + 1/eta^2), the last sub-step cut to end at dt;
+ *   per sub-step: ghosts of alpha (PF_bcs), psi = Mapping_Backward(alpha), cell normals of psi
+ *   (FDM_Normal_CenterFromCenter_D, include/Operators_FD/FDM_Normal_CenterFromCenter_D.H:12-63)
+ *   with their own ghost fill, J on every face, Jhat += dt_m J, alpha += dt_m div J
+ *   (FVM_Divergence_D, include/Operators_FV/FVM_Divergence_D.H);
+ *   finally Jhat *= 1/dt.
+ * alpha: the level's alpha1 with one ghost cell (offsets of cmp_level_layout for 1 comp, 1 ghost),
+ * ghosts filled; jhat: face buffer (1 comp), written. Returns the number of sub-steps in *nsub. */
+int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                           int nbox, const cmp_box* boxes, const double* alpha, double dt,
+                           double mob, double eta, double lambda, double* jhat, int* nsub)
+{
+    const int dim = P->dim;
+    long long ta, tn, tf;
+    cmp_box* ba = pf_boxes(dim, nbox, boxes, 1, 1, &ta);
+    cmp_box* bn = pf_boxes(dim, nbox, boxes, 1, dim, &tn);
+    long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
+    cmp_face_layout(dim, nbox, boxes, 1, foff, &tf);
+    double* A = (double*)malloc(sizeof(double) * ta);
+    double* Nrm = (double*)calloc((size_t)tn, sizeof(double));
+    double* J = (double*)malloc(sizeof(double) * tf);
+    memcpy(A, alpha, sizeof(double) * ta);
+    for (long long q = 0; q < tf; ++q) jhat[q] = 0.0;
+    (void)PF;   /* ACDI is the only mechanism restated here */
+    double s = 1.0 / G->dx[0] / G->dx[0];
+    for (int d = 1; d < dim; ++d) s = s + 1.0 / G->dx[d] / G->dx[d];
+    double dt_m = 0.9 / (2.0 * mob * lambda) / (s + 1.0 / eta / eta);
+    double t = 0.0, T = dt;
+    int n = 0, err = 0;
+    while (t < T) {
+        if (t + dt_m > T) dt_m = T - t;
+        int e = ref_level_fill_pf(dim, G, nbox, ba, 1, 1, A);
+        if (e && !err) err = e;
+        /* normals on valid cells from psi of the cell and its neighbours */
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &ba[b], 1);
+            const double* Ab = A + ba[b].offset;
+            double* Nb = Nrm + bn[b].offset;
+            int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                    for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                        int c3[3] = {i, j, k};
+                        long long c = fab_idx(&f, i, j, k);
+                        double fc = ref_pf_map_backward(Ab[c], eta);
+                        double g[3], sum = 0.0;
+                        for (int d = 0; d < dim; ++d) {
+                            int m3[3] = {c3[0], c3[1], c3[2]}, p3[3] = {c3[0], c3[1], c3[2]};
+                            m3[d] -= 1; p3[d] += 1;
+                            double fm = ref_pf_map_backward(Ab[fab_idx(&f, m3[0], m3[1], m3[2])], eta);
+                            double fp = ref_pf_map_backward(Ab[fab_idx(&f, p3[0], p3[1], p3[2])], eta);
+                            double gm = (fc - fm) / G->dx[d];
+                            double gp = (fp - fc) / G->dx[d];
+                            g[d] = 0.5 * gm + 0.5 * gp;
+                            sum += g[d] * g[d];
+                        }
+                        for (int d = 0; d < dim; ++d)
+                            Nb[c + d * f.sn] = (sum == 0.0) ? 0.0 : g[d] / sqrt(sum);
+                    }
+        }
+        e = ref_level_fill_pf(dim, G, nbox, bn, 1, dim, Nrm);
+        if (e && !err) err = e;
+        /* J on every face of every box, Jhat += dt_m J */
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &ba[b], 1);
+            const double* Ab = A + ba[b].offset;
+            const double* Nb = Nrm + bn[b].offset;
+            int nv[3] = {1, 1, 1};
+            for (int d = 0; d < dim; ++d) nv[d] = ba[b].hi[d] - ba[b].lo[d] + 1;
+            int lo2 = (dim == 3) ? ba[b].lo[2] : 0;
+            for (int d = 0; d < dim; ++d) {
+                int fe[3] = {nv[0], nv[1], nv[2]};
+                fe[d] += 1;
+                double* Jd = J + foff[b * dim + d];
+                double* Hd = jhat + foff[b * dim + d];
+#pragma omp parallel for collapse(2) schedule(static)
+                for (int kk = 0; kk < fe[2]; ++kk)
+                    for (int jj = 0; jj < fe[1]; ++jj)
+                        for (int ii = 0; ii < fe[0]; ++ii) {
+                            int c3[3] = {ba[b].lo[0] + ii, ba[b].lo[1] + jj, lo2 + kk};
+                            int m3[3] = {c3[0], c3[1], c3[2]};
+                            m3[d] -= 1;
+                            long long c = fab_idx(&f, c3[0], c3[1], c3[2]);
+                            long long m = fab_idx(&f, m3[0], m3[1], m3[2]);
+                            double mob_face = 0.5 * mob + 0.5 * mob;
+                            double Jf = ref_pf_acdi_flux(Ab[m], Ab[c], Nb[m + d * f.sn], Nb[c + d * f.sn],
+                                                         mob_face, eta, G->dx[d]);
+                            long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                            Jd[q] = Jf;
+                            Hd[q] += dt_m * Jf;
+                        }
+            }
+        }
+        /* alpha += dt_m div J on valid cells */
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &ba[b], 1);
+            double* Ab = A + ba[b].offset;
+            int nv[3] = {1, 1, 1};
+            for (int d = 0; d < dim; ++d) nv[d] = ba[b].hi[d] - ba[b].lo[d] + 1;
+            int lo2 = (dim == 3) ? ba[b].lo[2] : 0;
+            long long sf[3][3];
+            for (int d = 0; d < dim; ++d) {
+                int fe[3] = {nv[0], nv[1], nv[2]};
+                fe[d] += 1;
+                sf[d][0] = 1; sf[d][1] = fe[0]; sf[d][2] = (long long)fe[0] * fe[1];
+            }
+#pragma omp parallel for collapse(2) schedule(static)
+            for (int kk = 0; kk < nv[2]; ++kk)
+                for (int jj = 0; jj < nv[1]; ++jj)
+                    for (int ii = 0; ii < nv[0]; ++ii) {
+                        int idx[3] = {ii, jj, kk};
+                        double div = 0.0;
+                        for (int d = 0; d < dim; ++d) {
+                            const double* Jd = J + foff[b * dim + d];
+                            long long qm = idx[0] * sf[d][0] + idx[1] * sf[d][1] + idx[2] * sf[d][2];
+                            long long qp = qm + sf[d][d];
+                            double term = (Jd[qp] - Jd[qm]) / G->dx[d];
+                            div = (d == 0) ? term : div + term;
+                        }
+                        long long c = fab_idx(&f, ba[b].lo[0] + ii, ba[b].lo[1] + jj, lo2 + kk);
+                        Ab[c] = 1.0 * Ab[c] + dt_m * div;
+                    }
+        }
+        t += dt_m;
+        ++n;
+    }
+    double sc = 1.0 / dt;
+    for (long long q = 0; q < tf; ++q) jhat[q] *= sc;
+    if (nsub) *nsub = n;
+    free(A); free(Nrm); free(J); free(foff); free(ba); free(bn);
+    return err;
+}
+
+/* Grad_Q of every face from alpha (1 ghost, filled) and Jhat (explicit RC, :2315-2403). */
+int ref_level_pf_gradq(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                       int nbox, const cmp_box* boxes, const double* alpha, const double* jhat,
+                       double eta, double* gradq)
+{
+    const int dim = P->dim;
+    long long ta;
+    cmp_box* ba = pf_boxes(dim, nbox, boxes, 1, 1, &ta);
+    long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
+    cmp_face_layout(dim, nbox, boxes, 1, foff, NULL);
+    (void)G;
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &ba[b], 1);
+        const double* Ab = alpha + ba[b].offset;
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = ba[b].hi[d] - ba[b].lo[d] + 1;
+        int lo2 = (dim == 3) ? ba[b].lo[2] : 0;
+        for (int d = 0; d < dim; ++d) {
+            int fe[3] = {nv[0], nv[1], nv[2]};
+            fe[d] += 1;
+            const double* Hd = jhat + foff[b * dim + d];
+            double* Qd = gradq + foff[b * dim + d];
+            for (int kk = 0; kk < fe[2]; ++kk)
+                for (int jj = 0; jj < fe[1]; ++jj)
+                    for (int ii = 0; ii < fe[0]; ++ii) {
+                        int c3[3] = {ba[b].lo[0] + ii, ba[b].lo[1] + jj, lo2 + kk};
+                        int m3[3] = {c3[0], c3[1], c3[2]};
+                        m3[d] -= 1;
+                        long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                        Qd[q] = ref_pf_gradq_face(Ab[fab_idx(&f, m3[0], m3[1], m3[2])],
+                                                  Ab[fab_idx(&f, c3[0], c3[1], c3[2])], Hd[q], eta,
+                                                  PF->eta_factor);
+                    }
+        }
+    }
+    free(foff); free(ba);
+    return CMP_OK;
+}
+
+/* Resolved Lambda, eta and mobility of a Phase-Field step (PhaseField_Evolve, :427-465):
+ * Lambda <= 0 -> 1; eta <= 0 -> eta_multiplier max_d dx_d; mobility 1.0 (reset by Evolve each step,
+ * :236) times 0.5 eta max|u|. */
+void ref_pf_coefficients(const cmp_pf_params* PF, const cmp_geometry* G, double max_v,
+                         double* lambda, double* eta, double* mob)
+{
+    double L = PF->lambda;
+    if (L <= 0.0) L = 1.0;
+    double e = PF->eta;
+    if (e <= 0.0) {
+        double m = G->dx[0];
+        for (int d = 1; d < G->dim; ++d) m = (m < G->dx[d]) ? G->dx[d] : m;
+        e = PF->eta_multiplier * m;
+    }
+    double M = 1.0;
+    M *= 0.5 * e * max_v;
+    *lambda = L; *eta = e; *mob = M;
+}
+
+/* Grad_Q for a flow step of size dt: MaxVelocity, alpha1 with PF_bcs ghosts, the mechanism,
+ * the explicit Grad_Q. gradq: face buffer; info[4] = max|u|, mobility, eta, mechanism sub-steps. */
+int ref_level_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                            int nbox, const cmp_box* boxes, int ng, const double* U, double dt,
+                            double* gradq, double* gradq_max, double* info)
+{
+    const int dim = P->dim;
+    double mv = ref_level_max_velocity(P, nbox, boxes, ng, U);
+    double lambda, eta, mob;
+    ref_pf_coefficients(PF, G, mv, &lambda, &eta, &mob);
+    long long ta, tf;
+    cmp_box* ba = pf_boxes(dim, nbox, boxes, 1, 1, &ta);
+    double* A = (double*)calloc((size_t)ta, sizeof(double));
+    /* alpha1 of the valid cells, then its ghosts */
+    for (int b = 0; b < nbox; ++b) {
+        fab_t fu = make_fab(dim, &boxes[b], ng), fa = make_fab(dim, &ba[b], 1);
+        int lo2 = (dim == 3) ? boxes[b].lo[2] : 0, hi2 = (dim == 3) ? boxes[b].hi[2] : 0;
+        for (int k = lo2; k <= hi2; ++k)
+            for (int j = boxes[b].lo[1]; j <= boxes[b].hi[1]; ++j)
+                for (int i = boxes[b].lo[0]; i <= boxes[b].hi[0]; ++i)
+                    A[ba[b].offset + fab_idx(&fa, i, j, k)] =
+                        U[boxes[b].offset + fab_idx(&fu, i, j, k) + (3 + dim) * fu.sn];
+    }
+    int err = ref_level_fill_pf(dim, G, nbox, ba, 1, 1, A);
+    long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
+    cmp_face_layout(dim, nbox, boxes, 1, foff, &tf);
+    double* H = (double*)malloc(sizeof(double) * tf);
+    int nsub = 0;
+    int e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, H, &nsub);
+    if (e && !err) err = e;
+    ref_level_pf_gradq(P, PF, G, nbox, boxes, A, H, eta, gradq);
+    double gm = 0.0;
+    for (long long q = 0; q < tf; ++q) {
+        double a = fabs(gradq[q]);
+        if (gm < a || isnan(a)) gm = a;
+    }
+    if (gradq_max) *gradq_max = gm;
+    if (info) { info[0] = mv; info[1] = mob; info[2] = eta; info[3] = nsub; }
+    free(A); free(H); free(foff); free(ba);
+    return err;
+}
+
+/* PhaseField_ComputeDt (:2037-2098) for one level at Phase-Field time pf_time:
+ * tmp = cfl min_d dx_d/(max|Grad_Q| + 1e-14); with the change limiter tmp = min(tmp, 1.1 prev)
+ * (prev = tmp itself at pf_time 0); the step is cut so that it ends at dt when it would come
+ * within 1e-3 of its own size of dt. *prev receives the step. */
+double ref_pf_compute_dt(const cmp_pf_params* PF, const cmp_geometry* G, double gradq_max,
+                         double pf_time, double dt, double* prev)
+{
+    double dx = G->dx[0];
+    for (int d = 1; d < G->dim; ++d) dx = (G->dx[d] < dx) ? G->dx[d] : dx;
+    double tmp = PF->cfl * dx / (gradq_max + 1.0e-14);
+    if (pf_time == 0.0) *prev = tmp;
+    double dt0 = tmp;
+    if (PF->change_limiter && *prev > 0.0) tmp = smin(tmp, 1.1 * *prev);
+    dt0 = smin(dt0, 1 * tmp);
+    const double eps = (1.0e-3) * dt0;
+    if (pf_time + dt0 > dt - eps) dt0 = dt - pf_time;
+    *prev = dt0;
+    return dt0;
+}
+
+/* PhaseField_Evolve (:425-594) after a flow step of size dt, in place on U.
+ * info[6] (or NULL): max|u|, mobility, eta, mechanism sub-steps, max|Grad_Q|, Phase-Field sub-steps. */
+int ref_level_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                        int nbox, const cmp_box* boxes, int ng, double* U, double dt, double* info)
+{
+    const int dim = P->dim;
+    long long tf;
+    long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
+    cmp_face_layout(dim, nbox, boxes, 1, foff, &tf);
+    double* Q = (double*)malloc(sizeof(double) * tf);
+    double gm = 0.0, inf4[4];
+    int err = ref_level_pf_gradq_step(P, PF, G, nbox, boxes, ng, U, dt, Q, &gm, inf4);
+    double pf_time = 0.0, prev = 0.0;
+    int n = 0;
+    for (int step = 0; step < PF->max_step && pf_time < dt && !err; ++step) {
+        double pdt = ref_pf_compute_dt(PF, G, gm, pf_time, dt, &prev);
+        err = ref_level_pf_rk3_step(P, PF, G, nbox, boxes, ng, U, Q, pdt);
+        pf_time += pdt;
+        ++n;
+        if (pf_time >= dt - 1.0e-6 * dt) break;
+    }
+    if (info) {
+        for (int q = 0; q < 4; ++q) info[q] = inf4[q];
+        info[4] = gm;
+        info[5] = n;
+    }
+    free(Q); free(foff);
+    return err;
+}
+
+/* Evolve's step loop with the Phase-Field step (:202-312): ComputeDt, the flow's RK3 step, then
+ * PhaseField_Evolve with the same dt. */
+int ref_level_advance_pf(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                         int nbox, const cmp_box* boxes, int ng, double* U, int nsteps, double cfl,
+                         int limiter, double* dt_hist)
+{
+    double cmax = 0.0, dt = 0.0;
+    int err = ref_level_initial_cmax(P, G, nbox, boxes, ng, U, &cmax);
+    for (int s = 0; s < nsteps && !err; ++s) {
+        dt = ref_compute_dt(G, cmax, cfl, limiter, s == 0 ? ref_est_dt(G, cmax, cfl) : dt);
+        if (dt_hist) dt_hist[s] = dt;
+        cmax = 0.0;
+        err = ref_level_rk3_step(P, G, nbox, boxes, ng, U, dt, &cmax);
+        if (isnan(cmax)) err = CMP_ERR_NAN;
+        if (!err) err = ref_level_pf_evolve(P, PF, G, nbox, boxes, ng, U, dt, NULL);
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------------
  * Layout helpers (the same definitions as the product library's; the oracle keeps its own copy so
  * that it links nothing from it)
  * ---------------------------------------------------------------------------------------------- */
+
+int cmp_face_layout(int dim, int nbox, const cmp_box* boxes, int ncomp, long long* offsets,
+                    long long* total)
+{
+    long long off = 0;
+    for (int b = 0; b < nbox; ++b)
+        for (int d = 0; d < dim; ++d) {
+            long long n = ncomp;
+            for (int e = 0; e < dim; ++e) n *= (boxes[b].hi[e] - boxes[b].lo[e] + 1 + (e == d));
+            if (offsets) offsets[b * dim + d] = off;
+            off += n;
+        }
+    if (total) *total = off;
+    return CMP_OK;
+}
 
 long long cmp_fab_size(int dim, const cmp_box* box, int ng, int ncomp)
 {

@@ -71,6 +71,54 @@ double ref_compute_dt(const cmp_geometry* G, double cmax, double cfl, int limite
 int ref_level_advance(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
                       int ng, double* U, int nsteps, double cfl, int limiter, double* dt_hist);
 
+/* ghost fill of a Phase-Field field (ncomp comps, ng ghosts): periodic images or foextrap */
+int ref_level_fill_pf(int dim, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
+                      int ncomp, double* A);
+
+/* phase field: pointwise pieces */
+double ref_pf_map_backward(double phi, double eta);
+double ref_pf_map_forward(double psi, double eta, int deriv);
+double ref_pf_acdi_flux(double pf_m, double pf_c, double n_m, double n_c, double mob, double eta,
+                        double dx);
+double ref_pf_gradq_face(double pf_m, double pf_c, double jhat, double eta, double eta_factor);
+void   ref_pf_flux(const cmp_params* P, const cmp_pf_params* PF, const double* UL, const double* UR,
+                   double g, double* flux, double* cmax, int* err);
+int    ref_pf_face_flux(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                        const cmp_box* box, int ng, const double* U, int i, int j, int k, int dir,
+                        double gradq, double* fhat, double* cmax);
+int    ref_pf_rhs_box(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                      const cmp_box* box, int ng, const double* U, const double* const* gq,
+                      double* dUdt);
+void   ref_pf_coefficients(const cmp_pf_params* PF, const cmp_geometry* G, double max_v,
+                           double* lambda, double* eta, double* mob);
+double ref_pf_compute_dt(const cmp_pf_params* PF, const cmp_geometry* G, double gradq_max,
+                         double pf_time, double dt, double* prev);
+
+/* phase field: level operations. alpha buffers: 1 comp, 1 ghost (cmp_level_layout offsets);
+ * face buffers: cmp_face_layout with 1 comp. */
+int ref_level_pf_rhs(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                     int nbox, const cmp_box* boxes, int ng, const double* U, const double* gradq,
+                     double* dUdt);
+int ref_level_pf_rk3_step(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                          int nbox, const cmp_box* boxes, int ng, double* Un, const double* gradq,
+                          double pf_dt);
+double ref_level_max_velocity(const cmp_params* P, int nbox, const cmp_box* boxes, int ng,
+                              const double* U);
+int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                           int nbox, const cmp_box* boxes, const double* alpha, double dt,
+                           double mob, double eta, double lambda, double* jhat, int* nsub);
+int ref_level_pf_gradq(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                       int nbox, const cmp_box* boxes, const double* alpha, const double* jhat,
+                       double eta, double* gradq);
+int ref_level_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                            int nbox, const cmp_box* boxes, int ng, const double* U, double dt,
+                            double* gradq, double* gradq_max, double* info);
+int ref_level_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                        int nbox, const cmp_box* boxes, int ng, double* U, double dt, double* info);
+int ref_level_advance_pf(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                         int nbox, const cmp_box* boxes, int ng, double* U, int nsteps, double cfl,
+                         int limiter, double* dt_hist);
+
 /* number of OpenMP threads the level functions use (1 when built without OpenMP) */
 int ref_num_threads(void);
 void ref_set_num_threads(int n);

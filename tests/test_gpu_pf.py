@@ -1,0 +1,200 @@
+"""GPU parity of the Phase-Field path (through the C ABI) against the oracle on the same inputs:
+the Phase-Field right-hand side, Grad_Q from the ACDI mechanism, one Phase-Field RK3 sub-step,
+the whole Phase-Field step and flow steps followed by it.
+
+Bars (FP64): per-variable max relative difference <= 1e-12 for one evaluation or one step
+(Grad_Q: max |gpu - cpu| <= 1e-12 max |cpu| over all faces), <= 1e-9 in L1 after 10 steps; the
+sub-step counts must be equal.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from compas_amd import FaceLayout, Layout, Level, cases, make_pf_params, scheme_ghosts
+from tests._util import l1rel_per_var, maxrel_per_var
+from tests.test_oracle_pf import _global_faces_to_buffer, _interface_case, _periodic_random_faces
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-12
+
+
+def _dev(torch, a, device):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+PF_RHS_CASES = [
+    (2, "WENO5", 1, False), (2, "WENO5", 0, False), (2, "WENO5B", 1, False), (2, "MUSCL", 1, False),
+    (2, "GODUNOV", 0, False), (2, "WENO5", 1, True), (2, "WENO5", 0, True),
+    (3, "WENO5", 1, False), (3, "WENO5B", 0, False), (3, "WENO5", 1, True),
+]
+
+
+@pytest.mark.parametrize("dim,scheme,form,sg", PF_RHS_CASES)
+def test_pf_rhs_matches_oracle(gpu, dim, scheme, form, sg):
+    import torch
+    n = 24 if dim == 2 else 12
+    c = cases.random_smooth(dim, n, seed=21 + dim, scheme=scheme, max_grid_size=n // 2,
+                            gamma=(4.4, 1.4) if sg else (1.4, 1.6), pinf=(6.0e3, 0.0) if sg else (0.0, 0.0))
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(flux_phase2flux=form, p_min=(-1.0e3, -1.0e3))
+    lay = Layout(dim, c["boxes"], scheme_ghosts(scheme), P.nstate)
+    flay = FaceLayout(dim, c["boxes"])
+    buf = lay.pack(c["U"])
+    oracle.level_fill_ghosts(P, G, lay, buf)
+    gq = _global_faces_to_buffer(flay, _periodic_random_faces(c["shape"], dim, np.random.default_rng(4), 2.0))
+    ref = lay.unpack(oracle.level_pf_rhs(P, pf, G, lay, buf, gq), c["shape"])
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    U = _dev(torch, buf, gpu)
+    dU = torch.zeros_like(U)
+    lev.PhaseField_compute_dUdt_FV(dU, U, _dev(torch, gq, gpu))
+    assert lev.check_errors() == 0
+    got = lay.unpack(dU.cpu().numpy(), c["shape"])
+    lev.close()
+    rel = maxrel_per_var(got, ref)
+    assert rel.max() <= TOL_STEP, rel
+
+
+@pytest.mark.parametrize("dim,mg,periodic,sg", [(2, 16, True, False), (2, 32, False, True),
+                                                 (3, 8, True, False), (3, 16, False, True)])
+def test_pf_gradq_matches_oracle(gpu, dim, mg, periodic, sg):
+    import torch
+    n = 32 if dim == 2 else 16
+    eos = dict(gamma=(4.4, 1.4), pinf=(6.0e8, 0.0)) if sg else None
+    c = _interface_case(n, dim=dim, width=2.0, periodic=periodic, max_grid_size=mg, eos=eos)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(eta_multiplier=0.5)
+    lay = Layout(dim, c["boxes"], 3, P.nstate)
+    flay = FaceLayout(dim, c["boxes"])
+    buf = lay.pack(c["U"])
+    dt = 0.3 / n
+    ref, ref_gm, ref_info = oracle.level_pf_gradq_step(P, pf, G, lay, buf, dt, flay.total)
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    U = _dev(torch, buf, gpu)
+    gq = torch.zeros(flay.total, dtype=torch.float64, device=gpu)
+    gm, info = lev.PhaseField_GradQPF(U, dt, gq)
+    assert lev.check_errors() == 0
+    got = gq.cpu().numpy()
+    lev.close()
+    assert info[3] == ref_info[3] and info[3] >= 1
+    for q in range(3):
+        assert abs(info[q] - ref_info[q]) <= 1e-15 * abs(ref_info[q])
+    scale = np.max(np.abs(ref))
+    assert scale > 0.0
+    assert np.max(np.abs(got - ref)) <= TOL_STEP * scale
+    assert abs(gm - ref_gm) <= TOL_STEP * ref_gm
+
+
+@pytest.mark.parametrize("dim,form", [(2, 1), (2, 0), (3, 1)])
+def test_pf_rk3_step_matches_oracle(gpu, dim, form):
+    import torch
+    n = 32 if dim == 2 else 16
+    c = _interface_case(n, dim=dim, width=2.0, max_grid_size=n // 2)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(flux_phase2flux=form)
+    lay = Layout(dim, c["boxes"], 3, P.nstate)
+    flay = FaceLayout(dim, c["boxes"])
+    buf = lay.pack(c["U"])
+    gq, gm, _ = oracle.level_pf_gradq_step(P, pf, G, lay, buf, 0.01, flay.total)
+    pf_dt = 0.4 * G.dx[0] / gm
+    Un = _dev(torch, buf, gpu)
+    oracle.level_pf_rk3_step(P, pf, G, lay, buf, gq, pf_dt)
+    ref = lay.unpack(buf, c["shape"])
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    lev.PhaseField_ThirdOrderSSPRK(Un, U1, U2, _dev(torch, gq, gpu), pf_dt)
+    assert lev.check_errors() == 0
+    got = lay.unpack(Un.cpu().numpy(), c["shape"])
+    lev.close()
+    rel = maxrel_per_var(got, ref)
+    assert rel.max() <= TOL_STEP, rel
+
+
+@pytest.mark.parametrize("dim,form,sg", [(2, 1, False), (2, 0, True), (3, 1, True)])
+def test_pf_evolve_matches_oracle(gpu, dim, form, sg):
+    import torch
+    n = 32 if dim == 2 else 16
+    eos = dict(gamma=(4.4, 1.4), pinf=(6.0e8, 0.0)) if sg else None
+    c = _interface_case(n, dim=dim, width=2.5, periodic=not sg, max_grid_size=n // 2, eos=eos)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(flux_phase2flux=form, eta_multiplier=0.5, cfl=0.3)
+    lay = Layout(dim, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    dt = 2.0 / n   # about 8 mechanism and 4 Phase-Field sub-steps
+    Un = _dev(torch, buf, gpu)
+    ref_info = oracle.level_pf_evolve(P, pf, G, lay, buf, dt)
+    ref = lay.unpack(buf, c["shape"])
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dtd = torch.full((1,), dt, dtype=torch.float64, device=gpu)
+    info = lev.PhaseField_Evolve(Un, U1, U2, dtd)
+    assert lev.check_errors() == 0
+    got = lay.unpack(Un.cpu().numpy(), c["shape"])
+    lev.close()
+    assert info[3] == ref_info[3] and info[5] == ref_info[5] and info[5] >= 2
+    rel = maxrel_per_var(got, ref)
+    assert rel.max() <= TOL_STEP, rel
+
+
+def test_advance_with_pf_matches_oracle(gpu):
+    """Ten flow steps each followed by the Phase-Field step (Evolve's loop), advected interface
+    64^2 on a 2x2 split, eager and graph-replayed flow part: L1 <= 1e-9, equal dt history."""
+    import torch
+    c = cases.advected_interface_2d(n=64, max_grid_size=32)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(eta_multiplier=0.5, cfl=0.3)
+    lay = Layout(2, c["boxes"], 3, 6)
+    buf = lay.pack(c["U"])
+    hist_ref = oracle.level_advance_pf(P, pf, G, lay, buf, 10, c["cfl"])
+    ref = lay.unpack(buf, c["shape"])
+    for use_graph in (False, True):
+        lev = Level(P, G, lay)
+        lev.pf_setup(pf)
+        Un = _dev(torch, lay.pack(c["U"]), gpu)
+        U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+        dt = torch.zeros(1, dtype=torch.float64, device=gpu)
+        cm = torch.zeros(1, dtype=torch.float64, device=gpu)
+        hist = torch.zeros(10, dtype=torch.float64, device=gpu)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            lev.advance(Un, U1, U2, 10, c["cfl"], dt, cm, hist, use_graph=use_graph, stream=s)
+        s.synchronize()
+        assert lev.check_errors(s) == 0
+        got = lay.unpack(Un.cpu().numpy(), c["shape"])
+        lev.close()
+        assert np.max(np.abs(hist.cpu().numpy() - hist_ref) / hist_ref) <= 1e-12
+        l1 = l1rel_per_var(got, ref)
+        assert l1.max() <= 1e-9, (use_graph, l1)
+        for q in range(5):
+            assert abs(got[q].sum() - c["U"][q].sum()) <= 1e-13 * np.abs(c["U"][q]).sum()
+
+
+def test_pf_water_air_3d_step(gpu):
+    """3D water-air droplet (THINC on the flow fluxes, outflow BCs) at 24^3 in 8^3 boxes: one flow
+    step plus the Phase-Field step matches the oracle to 1e-12."""
+    import torch
+    c = cases.water_air_droplet_3d(n=24, box=8, thinc=1)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(eta_multiplier=0.5, cfl=0.3)
+    lay = Layout(3, c["boxes"], 3, 7)
+    buf = lay.pack(c["U"])
+    hist_ref = oracle.level_advance_pf(P, pf, G, lay, buf, 1, c["cfl"])
+    ref = lay.unpack(buf, c["shape"])
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    Un = _dev(torch, lay.pack(c["U"]), gpu)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dt = torch.zeros(1, dtype=torch.float64, device=gpu)
+    cm = torch.zeros(1, dtype=torch.float64, device=gpu)
+    hist = torch.zeros(1, dtype=torch.float64, device=gpu)
+    lev.advance(Un, U1, U2, 1, c["cfl"], dt, cm, hist, use_graph=False)
+    assert lev.check_errors() == 0
+    got = lay.unpack(Un.cpu().numpy(), c["shape"])
+    lev.close()
+    assert abs(hist.cpu().numpy()[0] - hist_ref[0]) <= 1e-14 * hist_ref[0]
+    rel = maxrel_per_var(got, ref)
+    assert rel.max() <= TOL_STEP, rel
