@@ -30,15 +30,15 @@ hipError_t launch_dt_dim(const double*, double*, double*, double, int, double*, 
 template <int DIM>
 hipError_t launch_pf_extract_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
 template <int DIM>
-hipError_t launch_pf_normals_dim(const PfGeom&, long long, const double*, double*, double, hipStream_t);
+hipError_t launch_pf_normals_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
 template <int DIM>
-hipError_t launch_pf_jfaces_dim(const PfGeom&, const long long*, const double*, const double*, double*,
-                                double*, double, double, double, hipStream_t);
+hipError_t launch_pf_jfaces_dim(const PfGeom&, const long long*, const double*, const double*,
+                                const double*, double*, double*, double, double, double, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_update_dim(const PfGeom&, long long, double*, const double*, double, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_gradq_dim(const PfGeom&, const long long*, const double*, const double*, double*,
-                               double, double, unsigned long long*, hipStream_t);
+                               unsigned long long*, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_maxvel_dim(const PfGeom&, long long, const double*, unsigned long long*,
                                 hipStream_t);
@@ -70,17 +70,17 @@ hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const
     return CMP_BY_DIM(dim, launch_pf_extract_dim, g, maxcells, U, A, s);
 }
 
-hipError_t launch_pf_normals(int dim, const PfGeom& g, long long maxcells, const double* A, double* Nrm,
-                             double eta, hipStream_t s)
+hipError_t launch_pf_normals(int dim, const PfGeom& g, long long maxcells, const double* Psi,
+                             double* Nrm, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_normals_dim, g, maxcells, A, Nrm, eta, s);
+    return CMP_BY_DIM(dim, launch_pf_normals_dim, g, maxcells, Psi, Nrm, s);
 }
 
 hipError_t launch_pf_jfaces(int dim, const PfGeom& g, const long long* maxfaces, const double* A,
-                            const double* Nrm, double* J, double* H, double mob_face, double eta,
-                            double dt_m, hipStream_t s)
+                            const double* Psi, const double* Nrm, double* J, double* H, double mob_face,
+                            double eta, double dt_m, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_jfaces_dim, g, maxfaces, A, Nrm, J, H, mob_face, eta, dt_m, s);
+    return CMP_BY_DIM(dim, launch_pf_jfaces_dim, g, maxfaces, A, Psi, Nrm, J, H, mob_face, eta, dt_m, s);
 }
 
 hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double* A, const double* J,
@@ -89,11 +89,10 @@ hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double
     return CMP_BY_DIM(dim, launch_pf_update_dim, g, maxcells, A, J, dt_m, s);
 }
 
-hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* A,
-                           const double* H, double* gq, double eta, double eta2,
-                           unsigned long long* gmax_bits, hipStream_t s)
+hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* S,
+                           const double* H, double* gq, unsigned long long* gmax_bits, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_gradq_dim, g, maxfaces, A, H, gq, eta, eta2, gmax_bits, s);
+    return CMP_BY_DIM(dim, launch_pf_gradq_dim, g, maxfaces, S, H, gq, gmax_bits, s);
 }
 
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
@@ -172,6 +171,7 @@ struct cmp_level {
     long long na = 0, nfaces = 0;
     double* d_A0 = nullptr;         // alpha1 of the step, PF_bcs ghosts
     double* d_A = nullptr;          // alpha1 advanced by the mechanism sub-steps
+    double* d_P = nullptr;          // psi of d_A; then the re-sharpened alpha1 of d_A0
     double* d_N = nullptr;          // cell normals
     double* d_J = nullptr;          // face J of the current sub-step
     double* d_H = nullptr;          // face Jhat
@@ -532,23 +532,17 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
     return CMP_OK;
 }
 
+// Releases what cmp_level_pf_setup allocated (errors of the frees are not actionable here).
 static void pf_free(cmp_level* L)
 {
-    hipFree(L->d_abox);
-    hipFree(L->d_nbox);
-    hipFree(L->d_gprefix1);
-    hipFree(L->d_A0);
-    hipFree(L->d_A);
-    hipFree(L->d_N);
-    hipFree(L->d_J);
-    hipFree(L->d_H);
-    hipFree(L->d_gq);
-    hipFree(L->d_bits);
-    hipFree(L->d_pfdt);
-    if (L->h_pin) hipHostFree(L->h_pin);
+    void* bufs[] = {L->d_abox, L->d_nbox, L->d_gprefix1, L->d_A0, L->d_A, L->d_P, L->d_N,
+                    L->d_J, L->d_H, L->d_gq, L->d_bits, L->d_pfdt};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (L->h_pin) (void)hipHostFree(L->h_pin);
     L->d_abox = L->d_nbox = nullptr;
     L->d_gprefix1 = nullptr;
-    L->d_A0 = L->d_A = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_pfdt = nullptr;
+    L->d_A0 = L->d_A = L->d_P = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_pfdt = nullptr;
     L->d_bits = nullptr;
     L->h_pin = nullptr;
     L->pf_on = false;
@@ -558,14 +552,12 @@ int cmp_level_destroy(cmp_level* L)
 {
     if (!L) return CMP_OK;
     pf_free(L);
-    if (L->gexec) hipGraphExecDestroy(L->gexec);
-    if (L->graph) hipGraphDestroy(L->graph);
-    if (L->own_stream) hipStreamDestroy(L->own_stream);
-    hipFree(L->d_boxes);
-    hipFree(L->d_tiles);
-    hipFree(L->d_gprefix);
-    hipFree(L->d_owner);
-    hipFree(L->d_err);
+    if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
+    if (L->graph) (void)hipGraphDestroy(L->graph);
+    if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
+    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
     delete L;
     return CMP_OK;
 }
@@ -688,13 +680,13 @@ int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nste
     bool same = L->gexec && L->gcfl == cfl && L->glimiter == limiter;
     for (int q = 0; q < 6 && same; ++q) same = (L->gkey[q] == key[q]);
     if (!same) {
-        if (L->gexec) { hipGraphExecDestroy(L->gexec); L->gexec = nullptr; }
-        if (L->graph) { hipGraphDestroy(L->graph); L->graph = nullptr; }
+        if (L->gexec) { CMP_TRY(hipGraphExecDestroy(L->gexec)); L->gexec = nullptr; }
+        if (L->graph) { CMP_TRY(hipGraphDestroy(L->graph)); L->graph = nullptr; }
         CMP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         r = level_step_with_hist(L, Un, U1, U2, cfl, limiter, dt_dev, cmax, dt_hist, cs);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(cs, &g);
-        if (r) { if (g) hipGraphDestroy(g); return r; }
+        if (r) { if (g) (void)hipGraphDestroy(g); return r; }
         CMP_TRY(e);
         L->graph = g;
         CMP_TRY(hipGraphInstantiate(&L->gexec, g, nullptr, nullptr, 0));
@@ -760,6 +752,7 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     CMP_TRYP(hipMemcpy(L->d_gprefix1, gp.data(), sizeof(long long) * (nbox + 1), hipMemcpyHostToDevice));
     CMP_TRYP(hipMalloc(&L->d_A0, sizeof(double) * na));
     CMP_TRYP(hipMalloc(&L->d_A, sizeof(double) * na));
+    CMP_TRYP(hipMalloc(&L->d_P, sizeof(double) * na));
     CMP_TRYP(hipMalloc(&L->d_N, sizeof(double) * nn));
     CMP_TRYP(hipMalloc(&L->d_J, sizeof(double) * nf));
     CMP_TRYP(hipMalloc(&L->d_H, sizeof(double) * nf));
@@ -861,18 +854,20 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
         if (t + dt_m > T) dt_m = T - t;
         if (!(dt_m > 0.0) || t + dt_m == t || n >= 10000000) return CMP_ERR_NAN;   // no progress
         CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A, L->d_err, s));
-        CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_A, L->d_N, eta, s));
+        CMP_TRY(launch_pf_map(L->d_A, L->d_P, L->na, eta, eta, 0, s));
+        CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_P, L->d_N, s));
         CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
-        CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, L->d_A, L->d_N, L->d_J, L->d_H, mob_face, eta,
-                                 dt_m, s));
+        CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, L->d_A, L->d_P, L->d_N, L->d_J, L->d_H, mob_face,
+                                 eta, dt_m, s));
         CMP_TRY(launch_pf_update(dim, L->pg, L->maxcells, L->d_A, L->d_J, dt_m, s));
         t += dt_m;
         ++n;
     }
     CMP_TRY(launch_scale(L->d_H, L->nfaces, 1.0 / dt, s));
-    // explicit Grad_Q from the step's alpha1 (PhaseField_GradQPF, :2315-2403)
-    CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_A0, L->d_H, gradq, eta, pf.eta_factor * eta,
-                            L->d_bits + 1, s));
+    // explicit Grad_Q from the step's alpha1 (PhaseField_GradQPF, :2315-2403), re-sharpened at
+    // eta_factor * eta
+    CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
+    CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, gradq, L->d_bits + 1, s));
     CMP_TRY(hipMemcpyAsync(L->h_pin + 1, L->d_bits + 1, sizeof(double), hipMemcpyDeviceToHost, s));
     CMP_TRY(hipStreamSynchronize(s));
     if (gradq_max) *gradq_max = bits_to_double(L->h_pin + 1);

@@ -565,13 +565,11 @@ __device__ __forceinline__ double pf_acdi_flux(double pf_m, double pf_c, double 
 }
 
 // Explicit Grad_Q of one face (source/Class_Compressible_PhaseField.cpp:2358-2391,
-// include/Physics_PhaseField/PhaseFieldFlux2GradQPF.H:12-28).
-__device__ __forceinline__ double pf_gradq_face(double pf_m, double pf_c, double jhat, double eta,
-                                                double eta2)
+// include/Physics_PhaseField/PhaseFieldFlux2GradQPF.H:12-28) from the re-sharpened cell values
+// m = F(B(alpha_m, eta), eta_factor eta), c likewise.
+__device__ __forceinline__ double pf_gradq_sharp(double m, double c, double jhat)
 {
     const double epsilon = 1.0e-3;
-    const double c = pf_map_forward(pf_map_backward(pf_c, eta), eta2);
-    const double m = pf_map_forward(pf_map_backward(pf_m, eta), eta2);
     const double pf = 0.5 * m + 0.5 * c;
     const double flux = (pf < epsilon || pf > 1.0 - epsilon) ? 0.0 : jhat;
     return flux / (pf * (1.0 - pf) + epsilon);

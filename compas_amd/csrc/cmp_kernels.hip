@@ -85,6 +85,27 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     const int e1 = cx_.n[1] + (D == 1 ? 2 * G : 0);
     const int e2 = cx_.n[2] + (D == 2 ? 2 * G : 0);
     const int R = e0 * e1 * e2;
+    const int f0 = cx_.n[0] + (D == 0 ? 1 : 0);
+    const int f1 = cx_.n[1] + (D == 1 ? 1 : 0);
+    const int f2 = cx_.n[2] + (D == 2 ? 1 : 0);
+    const int F = f0 * f1 * f2;
+    // Phase-Field flux: a face with Grad_Q == 0 has the flux 0 exactly (every term carries the
+    // factor Grad_Q), so its reconstruction is skipped, and a tile direction without any nonzero
+    // Grad_Q skips the staging as well (its flux differences are all zero)
+    const int gnv0 = bx.hi[0] - bx.lo[0] + 1 + (D == 0 ? 1 : 0);
+    const int gnv1 = bx.hi[1] - bx.lo[1] + 1 + (D == 1 ? 1 : 0);
+    auto gface = [&](int x, int y, int z) -> double {
+        const long long gi = bx.goff[D] + (long long)(cx_.tlo[0] + x - bx.lo[0])
+                           + (long long)gnv0 * ((cx_.tlo[1] + y - bx.lo[1])
+                                                + (long long)gnv1 * (cx_.tlo[2] + z - bx.lo[2]));
+        return cx_.gq[gi];
+    };
+    if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
+        int nz = 0;
+        for (int f = tid; f < F; f += NT)
+            nz |= (gface(f % f0, (f / f0) % f1, f / (f0 * f1)) != 0.0);
+        if (!__syncthreads_or(nz)) return;
+    }
     // 1. primitives of the region, staged in LDS
     for (int r = tid; r < R; r += NT) {
         const int x = r % e0;
@@ -103,10 +124,6 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     }
     __syncthreads();
     // 2. faces along D
-    const int f0 = cx_.n[0] + (D == 0 ? 1 : 0);
-    const int f1 = cx_.n[1] + (D == 1 ? 1 : 0);
-    const int f2 = cx_.n[2] + (D == 2 ? 1 : 0);
-    const int F = f0 * f1 * f2;
     const int rs = (D == 0) ? 1 : ((D == 1) ? e0 : e0 * e1);
     for (int f = tid; f < F; f += NT) {
         const int x = f % f0;
@@ -128,13 +145,13 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         };
         double Fl[N];
         if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
-            // Grad_Q of this face from the box's nodal face array of direction D
-            const int nv0 = bx.hi[0] - bx.lo[0] + 1 + (D == 0 ? 1 : 0);
-            const int nv1 = bx.hi[1] - bx.lo[1] + 1 + (D == 1 ? 1 : 0);
-            const long long gi = bx.goff[D] + (long long)(cx_.tlo[0] + x - bx.lo[0])
-                               + (long long)nv0 * ((cx_.tlo[1] + y - bx.lo[1])
-                                                   + (long long)nv1 * (cx_.tlo[2] + z - bx.lo[2]));
-            pf_face_flux<DIM, SCHEME, RIEMANN - RIEMANN_PF_MIX>(kp, cx_.limiter, D, st, cx_.gq[gi], Fl, bad);
+            const double g = gface(x, y, z);
+            if (g != 0.0) {
+                pf_face_flux<DIM, SCHEME, RIEMANN - RIEMANN_PF_MIX>(kp, cx_.limiter, D, st, g, Fl, bad);
+            } else {
+#pragma unroll
+                for (int q = 0; q < N; ++q) Fl[q] = 0.0;
+            }
 #pragma unroll
             for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
         } else {
@@ -310,7 +327,11 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
             double v = sRed[0];
 #pragma unroll
             for (int w = 1; w < NT / 64; ++w) v = smax(v, sRed[w]);
-            atomicMax(cmax_bits, (unsigned long long)__double_as_longlong(v));
+            // non-negative doubles order like their bit patterns; skip the atomic when the
+            // stored maximum is already as large
+            const unsigned long long vb = (unsigned long long)__double_as_longlong(v);
+            if (vb > __hip_atomic_load(cmax_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(cmax_bits, vb);
         }
     }
     if (bad && err) atomicMax(err, bad);

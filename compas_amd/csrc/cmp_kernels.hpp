@@ -92,16 +92,18 @@ struct PfGeom {
 
 hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A,
                              hipStream_t s);
-hipError_t launch_pf_normals(int dim, const PfGeom& g, long long maxcells, const double* A, double* Nrm,
-                             double eta, hipStream_t s);
+// psi (mode 0) or the re-sharpened alpha1 (mode 1) of every element of an alpha-layout buffer
+hipError_t launch_pf_map(const double* A, double* out, long long n, double eta, double eta2, int mode,
+                         hipStream_t s);
+hipError_t launch_pf_normals(int dim, const PfGeom& g, long long maxcells, const double* Psi,
+                             double* Nrm, hipStream_t s);
 hipError_t launch_pf_jfaces(int dim, const PfGeom& g, const long long* maxfaces, const double* A,
-                            const double* Nrm, double* J, double* H, double mob_face, double eta,
-                            double dt_m, hipStream_t s);
+                            const double* Psi, const double* Nrm, double* J, double* H, double mob_face,
+                            double eta, double dt_m, hipStream_t s);
 hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double* A, const double* J,
                             double dt_m, hipStream_t s);
-hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* A,
-                           const double* H, double* gq, double eta, double eta2,
-                           unsigned long long* gmax_bits, hipStream_t s);
+hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* S,
+                           const double* H, double* gq, unsigned long long* gmax_bits, hipStream_t s);
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
                             unsigned long long* bits, hipStream_t s);
 hipError_t launch_scale(double* p, long long n, double s_, hipStream_t s);

@@ -53,6 +53,9 @@ __device__ __forceinline__ long long face_index(const BoxDev& bx, int i, int j, 
     return bx.goff[D] + (long long)x + (long long)n0 * (y + (long long)n1 * z);
 }
 
+// Block max of non-negative v into *bits (the bit pattern of a non-negative double orders like the
+// double). The atomic is issued only when the block's value exceeds what *bits already holds, so
+// that a large grid does not queue one atomic per block on one address.
 __device__ __forceinline__ void block_max_to(double v, unsigned long long* bits)
 {
     __shared__ double red[NT / 64];
@@ -63,7 +66,8 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* bits)
         double m = red[0];
 #pragma unroll
         for (int w = 1; w < NT / 64; ++w) m = smax(m, red[w]);
-        atomicMax(bits, (unsigned long long)__double_as_longlong(m));
+        const unsigned long long mb = (unsigned long long)__double_as_longlong(m);
+        if (mb > __hip_atomic_load(bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(bits, mb);
     }
 }
 
@@ -79,24 +83,49 @@ __global__ __launch_bounds__(NT) void k_pf_extract(PfGeom g, const double* __res
     A[ab.off + fab_index(ab, i, j, k)] = U[hb.off + fab_index(hb, i, j, k) + (3 + DIM) * hb.sn];
 }
 
-// Cell normals of psi = Mapping_Backward(alpha1) on valid cells (PhaseField_Normal with
-// FDM_Normal_CenterFromCenter_D, include/Operators_FD/FDM_Normal_CenterFromCenter_D.H:12-63).
+// Pointwise maps of a whole alpha-layout buffer (valid and ghost cells): mode 0 psi =
+// Mapping_Backward(alpha, eta) (the LS field of PhaseField_ACDI, whose ghosts the reference fills
+// from the same cells the alpha ghosts come from); mode 1 the re-sharpened alpha of Grad_Q,
+// Mapping_Forward(Mapping_Backward(alpha, eta), eta2).
+#if CMP_KERNEL_DIM == 2
+__global__ __launch_bounds__(NT) void k_pf_map(const double* __restrict__ A, double* out, long long n,
+                                               double eta, double eta2, int mode)
+{
+    for (long long q = (long long)blockIdx.x * NT + threadIdx.x; q < n; q += (long long)gridDim.x * NT) {
+        const double psi = pf_map_backward(A[q], eta);
+        out[q] = mode ? pf_map_forward(psi, eta2) : psi;
+    }
+}
+
+hipError_t launch_pf_map(const double* A, double* out, long long n, double eta, double eta2, int mode,
+                         hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    long long blocks = (n + NT - 1) / NT;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_pf_map, dim3((unsigned)blocks), dim3(NT), 0, s, A, out, n, eta, eta2, mode);
+    return hipGetLastError();
+}
+#endif
+
+// Cell normals of psi on valid cells (PhaseField_Normal with FDM_Normal_CenterFromCenter_D,
+// include/Operators_FD/FDM_Normal_CenterFromCenter_D.H:12-63).
 template <int DIM>
-__global__ __launch_bounds__(NT) void k_pf_normals(PfGeom g, const double* __restrict__ A,
-                                                   double* Nrm, double eta)
+__global__ __launch_bounds__(NT) void k_pf_normals(PfGeom g, const double* __restrict__ Psi,
+                                                   double* Nrm)
 {
     const BoxDev ab = g.abox[blockIdx.y];
     int i, j, k;
     if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
-    const double* Ab = A + ab.off;
+    const double* Pb = Psi + ab.off;
     const long long c = fab_index(ab, i, j, k);
-    const double fc = pf_map_backward(Ab[c], eta);
+    const double fc = Pb[c];
     const long long st[3] = {1, ab.sj, ab.sk};
     double gr[DIM], sum = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        const double fm = pf_map_backward(Ab[c - st[d]], eta);
-        const double fp = pf_map_backward(Ab[c + st[d]], eta);
+        const double fm = Pb[c - st[d]];
+        const double fp = Pb[c + st[d]];
         gr[d] = 0.5 * ((fc - fm) / g.dx[d]) + 0.5 * ((fp - fc) / g.dx[d]);
         sum += gr[d] * gr[d];
     }
@@ -110,6 +139,7 @@ __global__ __launch_bounds__(NT) void k_pf_normals(PfGeom g, const double* __res
 // (PhaseField_ACDI, source/Class_Compressible_PhaseField.cpp:1609-1640; Saxpy of :679-683).
 template <int DIM, int D>
 __global__ __launch_bounds__(NT) void k_pf_jfaces(PfGeom g, const double* __restrict__ A,
+                                                  const double* __restrict__ Psi,
                                                   const double* __restrict__ Nrm, double* J, double* H,
                                                   double mob_face, double eta, double dt_m)
 {
@@ -120,10 +150,9 @@ __global__ __launch_bounds__(NT) void k_pf_jfaces(PfGeom g, const double* __rest
     const long long c = fab_index(ab, i, j, k);
     const long long m = c - ((D == 0) ? 1 : ((D == 1) ? ab.sj : ab.sk));
     const double* Ab = A + ab.off;
+    const double* Pb = Psi + ab.off;
     const double* Nb = Nrm + (long long)DIM * ab.off + D * ab.sn;
-    const double am = Ab[m], ac = Ab[c];
-    const double Jf = pf_acdi_flux(am, ac, pf_map_backward(am, eta), pf_map_backward(ac, eta), Nb[m],
-                                   Nb[c], mob_face, eta, g.dx[D]);
+    const double Jf = pf_acdi_flux(Ab[m], Ab[c], Pb[m], Pb[c], Nb[m], Nb[c], mob_face, eta, g.dx[D]);
     const long long f = ab.goff[D] + q;
     J[f] = Jf;
     H[f] = H[f] + dt_m * Jf;
@@ -146,11 +175,12 @@ __global__ __launch_bounds__(NT) void k_pf_update(PfGeom g, double* A, const dou
     A[c] = 1.0 * A[c] + dt_m * div;
 }
 
-// Explicit Grad_Q on every face of direction D from the step's alpha1 and Jhat, with max |Grad_Q|.
+// Explicit Grad_Q on every face of direction D from the step's re-sharpened alpha1 S (k_pf_map
+// mode 1) and Jhat, with max |Grad_Q|.
 template <int DIM, int D>
-__global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restrict__ A,
-                                                 const double* __restrict__ H, double* gq, double eta,
-                                                 double eta2, unsigned long long* gmax_bits)
+__global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restrict__ S,
+                                                 const double* __restrict__ H, double* gq,
+                                                 unsigned long long* gmax_bits)
 {
     const BoxDev ab = g.abox[blockIdx.y];
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
@@ -159,9 +189,9 @@ __global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restr
     if (face_of<DIM, D>(ab, q, i, j, k)) {
         const long long c = fab_index(ab, i, j, k);
         const long long m = c - ((D == 0) ? 1 : ((D == 1) ? ab.sj : ab.sk));
-        const double* Ab = A + ab.off;
+        const double* Sb = S + ab.off;
         const long long f = ab.goff[D] + q;
-        const double r = pf_gradq_face(Ab[m], Ab[c], H[f], eta, eta2);
+        const double r = pf_gradq_sharp(Sb[m], Sb[c], H[f]);
         gq[f] = r;
         v = fabs(r);
         if (__builtin_isnan(r)) v = __longlong_as_double(0x7ff8000000000000LL);
@@ -209,25 +239,25 @@ hipError_t launch_pf_extract_dim(const PfGeom& g, long long maxcells, const doub
 }
 
 template <int DIM>
-hipError_t launch_pf_normals_dim(const PfGeom& g, long long maxcells, const double* A, double* Nrm,
-                                 double eta, hipStream_t s)
+hipError_t launch_pf_normals_dim(const PfGeom& g, long long maxcells, const double* Psi, double* Nrm,
+                                 hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_normals<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, A, Nrm, eta);
+    hipLaunchKernelGGL((k_pf_normals<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, Psi, Nrm);
     return hipGetLastError();
 }
 
 template <int DIM>
 hipError_t launch_pf_jfaces_dim(const PfGeom& g, const long long* maxfaces, const double* A,
-                                const double* Nrm, double* J, double* H, double mob_face, double eta,
-                                double dt_m, hipStream_t s)
+                                const double* Psi, const double* Nrm, double* J, double* H,
+                                double mob_face, double eta, double dt_m, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_jfaces<DIM, 0>), grid_of(maxfaces[0], g.nbox), dim3(NT), 0, s, g, A, Nrm,
-                       J, H, mob_face, eta, dt_m);
-    hipLaunchKernelGGL((k_pf_jfaces<DIM, 1>), grid_of(maxfaces[1], g.nbox), dim3(NT), 0, s, g, A, Nrm,
-                       J, H, mob_face, eta, dt_m);
+    hipLaunchKernelGGL((k_pf_jfaces<DIM, 0>), grid_of(maxfaces[0], g.nbox), dim3(NT), 0, s, g, A, Psi,
+                       Nrm, J, H, mob_face, eta, dt_m);
+    hipLaunchKernelGGL((k_pf_jfaces<DIM, 1>), grid_of(maxfaces[1], g.nbox), dim3(NT), 0, s, g, A, Psi,
+                       Nrm, J, H, mob_face, eta, dt_m);
     if (DIM == 3)
         hipLaunchKernelGGL((k_pf_jfaces<DIM, (DIM == 3 ? 2 : 1)>), grid_of(maxfaces[2], g.nbox), dim3(NT),
-                           0, s, g, A, Nrm, J, H, mob_face, eta, dt_m);
+                           0, s, g, A, Psi, Nrm, J, H, mob_face, eta, dt_m);
     return hipGetLastError();
 }
 
@@ -240,17 +270,16 @@ hipError_t launch_pf_update_dim(const PfGeom& g, long long maxcells, double* A, 
 }
 
 template <int DIM>
-hipError_t launch_pf_gradq_dim(const PfGeom& g, const long long* maxfaces, const double* A,
-                               const double* H, double* gq, double eta, double eta2,
-                               unsigned long long* gmax_bits, hipStream_t s)
+hipError_t launch_pf_gradq_dim(const PfGeom& g, const long long* maxfaces, const double* S,
+                               const double* H, double* gq, unsigned long long* gmax_bits, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_gradq<DIM, 0>), grid_of(maxfaces[0], g.nbox), dim3(NT), 0, s, g, A, H, gq,
-                       eta, eta2, gmax_bits);
-    hipLaunchKernelGGL((k_pf_gradq<DIM, 1>), grid_of(maxfaces[1], g.nbox), dim3(NT), 0, s, g, A, H, gq,
-                       eta, eta2, gmax_bits);
+    hipLaunchKernelGGL((k_pf_gradq<DIM, 0>), grid_of(maxfaces[0], g.nbox), dim3(NT), 0, s, g, S, H, gq,
+                       gmax_bits);
+    hipLaunchKernelGGL((k_pf_gradq<DIM, 1>), grid_of(maxfaces[1], g.nbox), dim3(NT), 0, s, g, S, H, gq,
+                       gmax_bits);
     if (DIM == 3)
         hipLaunchKernelGGL((k_pf_gradq<DIM, (DIM == 3 ? 2 : 1)>), grid_of(maxfaces[2], g.nbox), dim3(NT),
-                           0, s, g, A, H, gq, eta, eta2, gmax_bits);
+                           0, s, g, S, H, gq, gmax_bits);
     return hipGetLastError();
 }
 
@@ -305,15 +334,15 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s)
 template hipError_t launch_pf_extract_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
                                                           double*, hipStream_t);
 template hipError_t launch_pf_normals_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
-                                                          double*, double, hipStream_t);
+                                                          double*, hipStream_t);
 template hipError_t launch_pf_jfaces_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
-                                                         const double*, double*, double*, double, double,
-                                                         double, hipStream_t);
+                                                         const double*, const double*, double*, double*,
+                                                         double, double, double, hipStream_t);
 template hipError_t launch_pf_update_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, double*, const double*,
                                                          double, hipStream_t);
 template hipError_t launch_pf_gradq_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
-                                                        const double*, double*, double, double,
-                                                        unsigned long long*, hipStream_t);
+                                                        const double*, double*, unsigned long long*,
+                                                        hipStream_t);
 template hipError_t launch_pf_maxvel_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
                                                          unsigned long long*, hipStream_t);
 
