@@ -215,11 +215,23 @@ __device__ __forceinline__ void weno5_weights(const double* IS, double* w)
     w[2] = d2 / ((IS20 * IS20) * d0 + (IS21 * IS21) * d1 + d2);
 }
 
+// Face value sum_k w_k f_k. Fast path: with t_k = d_k prod_{j != k} IS_j^2 the value is
+// (sum_k t_k f_k) / sum_k t_k, taken when that sum lies well inside the double range (no scaling,
+// one division); constant data (all IS zero), extreme IS ratios, NaN and inf go through
+// weno5_weights.
 template <int FACE>
 __device__ __forceinline__ double weno5(double f_mm, double f_m, double f_c, double f_p, double f_pp)
 {
-    double f[3], IS[3], w[3];
+    constexpr double d0 = (FACE == 1) ? 3.0 / 10.0 : 1.0 / 10.0;
+    constexpr double d2 = (FACE == 1) ? 1.0 / 10.0 : 3.0 / 10.0;
+    constexpr double d1 = 1.0 - d0 - d2;
+    double f[3], IS[3];
     weno5_parts<FACE>(f_mm, f_m, f_c, f_p, f_pp, f, IS);
+    const double q01 = IS[0] * IS[1], q02 = IS[0] * IS[2], q12 = IS[1] * IS[2];
+    const double t0 = d0 * (q12 * q12), t1 = d1 * (q02 * q02), t2 = d2 * (q01 * q01);
+    const double S = (t0 + t1) + t2;
+    if (S > 1e-290 && S < 1e290) return (t0 * f[0] + t1 * f[1] + t2 * f[2]) / S;
+    double w[3];
     weno5_weights<FACE>(IS, w);
     return w[0] * f[0] + w[1] * f[1] + w[2] * f[2];
 }
@@ -445,10 +457,41 @@ __device__ __forceinline__ void make_side(const KParams& kp, FaceSide<DIM>& s, i
 // (t = 0..2G-1, cell G-1 is the low side). SCHEME: 0 Godunov, 1 MUSCL, 5 WENO5, 6 WENO5B. With
 // THINC, normal() returns the face normal's component along dir (from the volume fraction); it is
 // evaluated only when the volume-fraction gates already admit THINC on a side.
-// Reconstruction part: the primitive face states WL (low side) and WR (high side).
-template <int DIM, int SCHEME, bool THINC, typename S, typename NRM>
-__device__ __forceinline__ void recon_face(const KParams& kp, int limiter, int dir, const S& s,
-                                           const NRM& normal, double* LW, double* RW)
+// THINC decision of one face (FVM_Reconstruction2Face_Conservative_THINC_K_Compressible5Eq.H:143-170):
+// the interface gates of each side and, for a side that passes them, its THINC volume fraction.
+// aL / aR stay negative for a side THINC leaves to the base scheme. G is the stencil half-width of
+// the accessor s (cells mm, m, c, p of the face are G-2 .. G+1).
+template <int DIM, int G, typename S, typename NRM>
+__device__ __forceinline__ void thinc_decide(const KParams& kp, const S& s, const NRM& normal,
+                                             double& aL, double& aR)
+{
+    constexpr int IA = 3 + DIM;
+    const double amm = s(G - 2, IA), am = s(G - 1, IA), ac = s(G, IA), ap = s(G + 1, IA);
+    const bool gL = (ac - am) * (am - amm) > 0.0;
+    const bool gR = (ap - ac) * (ac - am) > 0.0;
+    const bool sym = (amm == ap) && (am == ac) && (am != amm);
+    const double eps = kp.thinc_eps;
+    const bool cL = (am > eps && am < 1.0 - eps) && (gL || sym);
+    const bool cR = (ac > eps && ac < 1.0 - eps) && (gR || sym);
+    aL = -1.0;
+    aR = -1.0;
+    if (cL || cR) {
+        const double nd = normal();
+        const bool gate = (!kp.thinc_use_normal) || (fabs(nd) > kp.thinc_thr);
+        if (gate) {
+            const double beta = kp.thinc_use_normal ? (kp.thinc_beta * fabs(nd) + 0.01) : kp.thinc_beta;
+            if (cL) aL = thinc_face(beta, (ac - amm > 0.0) ? 1.0 : -1.0, am, 1);
+            if (cR) aR = thinc_face(beta, (ap - am > 0.0) ? 1.0 : -1.0, ac, 0);
+        }
+    }
+}
+
+// Reconstruction part: the primitive face states WL (low side) and WR (high side). With THINC,
+// aL / aR are thinc_decide's results for this face: where either side uses THINC, both sides take
+// the MUSCL-minmod base state and a THINC side replaces its volume fraction.
+template <int DIM, int SCHEME, bool THINC, typename S>
+__device__ __forceinline__ void recon_face_decided(const KParams& kp, int limiter, int dir, const S& s,
+                                                   double aL, double aR, double* LW, double* RW)
 {
     constexpr int N = 4 + DIM;
     constexpr int G = (SCHEME >= 5) ? 3 : (SCHEME == 1 ? 2 : 1);
@@ -456,35 +499,17 @@ __device__ __forceinline__ void recon_face(const KParams& kp, int limiter, int d
     const double dx = kp.dx[dir], idx = kp.inv_dx[dir];
     struct { double* W; } L{LW}, R{RW};
     bool done = false;
-    if (THINC) {
-        // stencil cells mm, m, c, p of the face = G-2 .. G+1
-        const double amm = s(G - 2, IA), am = s(G - 1, IA), ac = s(G, IA), ap = s(G + 1, IA);
-        const bool gL = (ac - am) * (am - amm) > 0.0;
-        const bool gR = (ap - ac) * (ac - am) > 0.0;
-        const bool sym = (amm == ap) && (am == ac) && (am != amm);
-        const double eps = kp.thinc_eps;
-        const bool cL = (am > eps && am < 1.0 - eps) && (gL || sym);
-        const bool cR = (ac > eps && ac < 1.0 - eps) && (gR || sym);
-        double nd = 0.0;
-        bool gate = false;
-        if (cL || cR) {
-            nd = normal();
-            gate = (!kp.thinc_use_normal) || (fabs(nd) > kp.thinc_thr);
-        }
-        const bool iL = cL && gate;
-        const bool iR = cR && gate;
-        if (iL || iR) {
+    if (THINC && (aL >= 0.0 || aR >= 0.0)) {
 #pragma unroll
-            for (int n = 0; n < N; ++n) {
-                L.W[n] = muscl(s(G - 2, n), s(G - 1, n), s(G, n), dx, idx, 1, 1);
-                R.W[n] = muscl(s(G - 1, n), s(G, n), s(G + 1, n), dx, idx, -1, 1);
-            }
-            const double beta = kp.thinc_use_normal ? (kp.thinc_beta * fabs(nd) + 0.01) : kp.thinc_beta;
-            if (iL) thinc_state<DIM>(L.W, thinc_face(beta, (ac - amm > 0.0) ? 1.0 : -1.0, am, 1));
-            if (iR) thinc_state<DIM>(R.W, thinc_face(beta, (ap - am > 0.0) ? 1.0 : -1.0, ac, 0));
-            done = true;
+        for (int n = 0; n < N; ++n) {
+            L.W[n] = muscl(s(G - 2, n), s(G - 1, n), s(G, n), dx, idx, 1, 1);
+            R.W[n] = muscl(s(G - 1, n), s(G, n), s(G + 1, n), dx, idx, -1, 1);
         }
+        if (aL >= 0.0) thinc_state<DIM>(L.W, aL);
+        if (aR >= 0.0) thinc_state<DIM>(R.W, aR);
+        done = true;
     }
+    (void)IA;
     if (!done) {
         if (SCHEME == 5 || SCHEME == 6) {
             constexpr int n0 = (SCHEME == 6) ? 2 : 0;
@@ -517,15 +542,38 @@ __device__ __forceinline__ void recon_face(const KParams& kp, int limiter, int d
     }
 }
 
+// Reconstruction of one face, THINC decision included.
+template <int DIM, int SCHEME, bool THINC, typename S, typename NRM>
+__device__ __forceinline__ void recon_face(const KParams& kp, int limiter, int dir, const S& s,
+                                           const NRM& normal, double* LW, double* RW)
+{
+    constexpr int G = (SCHEME >= 5) ? 3 : (SCHEME == 1 ? 2 : 1);
+    double aL = -1.0, aR = -1.0;
+    if (THINC) thinc_decide<DIM, G>(kp, s, normal, aL, aR);
+    recon_face_decided<DIM, SCHEME, THINC>(kp, limiter, dir, s, aL, aR, LW, RW);
+}
+
+// Flow flux of one face with the THINC decision already taken (aL, aR of thinc_decide).
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, typename S>
+__device__ __forceinline__ double face_flux_decided(const KParams& kp, int limiter, int dir, const S& s,
+                                                    double aL, double aR, double* F, double& cmax,
+                                                    int& bad)
+{
+    FaceSide<DIM> L, R;
+    recon_face_decided<DIM, SCHEME, THINC>(kp, limiter, dir, s, aL, aR, L.W, R.W);
+    make_side<DIM>(kp, L, bad);
+    make_side<DIM>(kp, R, bad);
+    return riemann<DIM, RIEMANN>(L, R, dir, F, cmax);
+}
+
 template <int DIM, int SCHEME, int RIEMANN, bool THINC, typename S, typename NRM>
 __device__ __forceinline__ double face_flux(const KParams& kp, int limiter, int dir, const S& s,
                                             const NRM& normal, double* F, double& cmax, int& bad)
 {
-    FaceSide<DIM> L, R;
-    recon_face<DIM, SCHEME, THINC>(kp, limiter, dir, s, normal, L.W, R.W);
-    make_side<DIM>(kp, L, bad);
-    make_side<DIM>(kp, R, bad);
-    return riemann<DIM, RIEMANN>(L, R, dir, F, cmax);
+    constexpr int G = (SCHEME >= 5) ? 3 : (SCHEME == 1 ? 2 : 1);
+    double aL = -1.0, aR = -1.0;
+    if (THINC) thinc_decide<DIM, G>(kp, s, normal, aL, aR);
+    return face_flux_decided<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, dir, s, aL, aR, F, cmax, bad);
 }
 
 // ---- phase field ------------------------------------------------------------------------------------

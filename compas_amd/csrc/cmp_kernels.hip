@@ -125,6 +125,36 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     __syncthreads();
     // 2. faces along D
     const int rs = (D == 0) ? 1 : ((D == 1) ? e0 : e0 * e1);
+    if constexpr (THINC && RIEMANN < RIEMANN_PF_MIX) {
+        // THINC decisions first (gates, face normal, tanh profile), parked in the face's own flux
+        // slots: the main pass below then carries only the base reconstructions, which keeps its
+        // register footprint at the THINC-free kernel's
+        for (int f = tid; f < F; f += NT) {
+            const int x = f % f0;
+            const int y = (f / f0) % f1;
+            const int z = f / (f0 * f1);
+            const int base = x + e0 * (y + e1 * z);
+            auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
+            // alpha1 halo coordinates of the face's high cell; transverse axes in ascending order
+            const int hx = x + 1, hy = y + 1, hz = (DIM == 3) ? z + 1 : 0;
+            const int a0 = cx_.ae[0], a1 = cx_.ae[1];
+            auto aa = [&](int od, int o1, int o2) -> double {
+                const int lo = (od == -1) ? 1 : 0;
+                int px = hx - (D == 0 ? lo : 0), py = hy - (D == 1 ? lo : 0), pz = hz - (D == 2 ? lo : 0);
+                if (D == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
+                else if (D == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
+                else { px += o1; py += o2; }
+                return sA[px + a0 * (py + a1 * pz)];
+            };
+            auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
+            double aL, aR;
+            thinc_decide<DIM, G>(kp, st, normal, aL, aR);
+            sF[0 * FMAX + f] = aL;
+            sF[1 * FMAX + f] = aR;
+        }
+    }
+    // the main pass maps faces to threads exactly as the THINC pass does: each thread reads back only
+    // the slots it wrote, so no barrier is needed between the two
     for (int f = tid; f < F; f += NT) {
         const int x = f % f0;
         const int y = (f / f0) % f1;
@@ -132,17 +162,6 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         const int a = (D == 0) ? x : ((D == 1) ? y : z);
         const int base = x + e0 * (y + e1 * z);
         auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
-        // alpha1 halo coordinates of the face's high cell; transverse axes in ascending order
-        const int hx = x + 1, hy = y + 1, hz = (DIM == 3) ? z + 1 : 0;
-        const int a0 = cx_.ae[0], a1 = cx_.ae[1];
-        auto aa = [&](int od, int o1, int o2) -> double {
-            const int lo = (od == -1) ? 1 : 0;
-            int px = hx - (D == 0 ? lo : 0), py = hy - (D == 1 ? lo : 0), pz = hz - (D == 2 ? lo : 0);
-            if (D == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
-            else if (D == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
-            else { px += o1; py += o2; }
-            return sA[px + a0 * (py + a1 * pz)];
-        };
         double Fl[N];
         if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
             const double g = gface(x, y, z);
@@ -155,9 +174,14 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
 #pragma unroll
             for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
         } else {
-            auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
+            double aL = -1.0, aR = -1.0;
+            if (THINC) {
+                aL = sF[0 * FMAX + f];
+                aR = sF[1 * FMAX + f];
+            }
             double c;
-            const double un = face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, cx_.limiter, D, st, normal, Fl, c, bad);
+            const double un = face_flux_decided<DIM, SCHEME, RIEMANN, THINC>(kp, cx_.limiter, D, st, aL, aR,
+                                                                           Fl, c, bad);
 #pragma unroll
             for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
             sF[N * FMAX + f] = un;
