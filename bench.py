@@ -87,6 +87,53 @@ def cpu_baseline(c, seconds, threads):
             "sample": "%d %s of the %s workload (%d cells) in %.1f s" % (steps, what, c["name"], cells, el)}
 
 
+def measure_traffic(workload_name, kernel_prefix, calib_mib=512):
+    """HBM bytes per launch of the dominant kernel from two rocprofv3 counter runs of this script
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for
+    a device-to-device copy of calib_mib MiB run first in the same process. The kernel's last
+    dispatch is the stage-timing launch the roofline's flop rate is measured on. None when
+    rocprofv3 is missing or a run fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+    rp = shutil.which("rocprofv3")
+    if rp is None:
+        return None
+    child = [sys.executable, os.path.abspath(__file__), "--workload", workload_name, "--steps", "1",
+             "--warmup", "1", "--no-cpu-baseline", "--kernel-reps", "2",
+             "--copy-calibration-mib", str(calib_mib), "--no-traffic"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    got = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="cmp_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "300", rp, "--pmc", ctr, "--output-format", "csv", "-d", d,
+               "-o", "run", "--"] + child
+        r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, env=env)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            return None
+        kern, copy = [], []
+        for row in csv.DictReader(open(files[0])):
+            if row["Counter_Name"] != ctr:
+                continue
+            name = row["Kernel_Name"]
+            if name.startswith(kernel_prefix):
+                kern.append((int(row["Dispatch_Id"]), float(row["Counter_Value"])))
+            elif "copyBuffer" in name:
+                copy.append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if not kern or not copy:
+            return None
+        scale = calib_mib * 1024.0 / max(copy)        # the calibration copy is the largest copy
+        got[ctr] = (max(kern)[1] * scale, scale)
+    fetch, fs = got["FETCH_SIZE"]
+    write, ws = got["WRITE_SIZE"]
+    return {"bytes": (fetch + write) * 1024.0, "fetch_bytes": fetch * 1024.0, "write_bytes": write * 1024.0,
+            "scale_fetch": fs, "scale_write": ws}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +146,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first: the reference kernel "
                          "of known traffic against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter runs that measure the dominant kernel's HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (every box of a "
@@ -219,6 +268,18 @@ def main():
     }
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
+    if not args.no_traffic:
+        scheme = {(1, 5): 5, (1, 6): 6, (0, 0): 0}.get((P.id_weno, P.id_limiter), 1)
+        prefix = "void cmp::k_stage<%d, %d, %d, %s>" % (P.dim, scheme, P.id_riemann,
+                                                         "true" if P.id_thinc else "false")
+        tr = measure_traffic(args.workload, prefix)
+        if tr is not None:
+            rec["roofline"]["traffic"] = tr["bytes"]
+            rec["roofline"]["traffic_detail"] = {
+                "fetch_bytes": tr["fetch_bytes"], "write_bytes": tr["write_bytes"],
+                "scale_fetch": tr["scale_fetch"], "scale_write": tr["scale_write"],
+                "algorithmic_bytes": 3 * P.nstate * 8 * cells,
+                "launch": "stage 2 form (reads the stage state with ghosts and U^n, writes U^(2))"}
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         rec["cpu_baseline"] = cpu_baseline(c, args.cpu_seconds, threads)
