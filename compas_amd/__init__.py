@@ -75,7 +75,7 @@ def scheme_ghosts(scheme):
 
 # PhaseField.Mechanism strings with a path here -> ID_Mechanism
 # (include/Physics_PhaseField/PhaseField_Parameter.H:79-137)
-PF_MECHANISMS = {"ACDI": 3}
+PF_MECHANISMS = {"CAC-Adv": 1, "ACDI": 3}
 
 
 def make_pf_params(mechanism="ACDI", explicit_rc=1, id_flux=0, flux_phase2flux=1, cfl=0.4,

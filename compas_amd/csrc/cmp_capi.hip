@@ -101,6 +101,96 @@ hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const 
     return CMP_BY_DIM(dim, launch_pf_maxvel_dim, g, maxcells, U, bits, s);
 }
 
+template <int DIM>
+hipError_t launch_pf_cac_mech_dim(const PfGeom&, long long, const double*, const double*, double*, double*,
+                                  double, double, double, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_cac_update_dim(const PfGeom&, long long, double*, const double*, const double*,
+                                    const double*, double, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_divjhat_dim(const PfGeom&, long long, const double*, const double*, double*, double,
+                                 hipStream_t);
+template <int DIM>
+hipError_t launch_pf_bcoef_dim(const PfGeom&, const long long*, const double*, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_cell_sum_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_cg_init_dim(const PfGeom&, long long, const double*, const double*, double, double*,
+                              double*, double*, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_cg_ap_dim(const PfGeom&, long long, const double*, const double*, const double*, double*,
+                            double*, hipStream_t);
+template <int DIM>
+hipError_t launch_cg_xr_dim(const PfGeom&, long long, const double*, double*, double*, const double*,
+                            const double*, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_cg_p_dim(const PfGeom&, long long, const double*, const double*, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_gradx_dim(const PfGeom&, const long long*, const double*, double*, unsigned long long*,
+                               hipStream_t);
+
+hipError_t launch_pf_cac_mech(int dim, const PfGeom& g, long long maxcells, const double* A,
+                              const double* Nrm, double* Mk, double* Wk, double mob, double lambda,
+                              double eta, double* part, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_cac_mech_dim, g, maxcells, A, Nrm, Mk, Wk, mob, lambda, eta, part, s);
+}
+
+hipError_t launch_pf_cac_update(int dim, const PfGeom& g, long long maxcells, double* A, const double* Mk,
+                                const double* Wk, const double* sums, double dt_m, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_cac_update_dim, g, maxcells, A, Mk, Wk, sums, dt_m, s);
+}
+
+hipError_t launch_pf_divjhat(int dim, const PfGeom& g, long long maxcells, const double* A, const double* A0,
+                             double* DJ, double dt, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_divjhat_dim, g, maxcells, A, A0, DJ, dt, s);
+}
+
+hipError_t launch_pf_bcoef(int dim, const PfGeom& g, const long long* maxfaces, const double* S, double* Bf,
+                           hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_bcoef_dim, g, maxfaces, S, Bf, s);
+}
+
+hipError_t launch_cell_sum(int dim, const PfGeom& g, long long maxcells, const double* F, double* part,
+                           hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_cell_sum_dim, g, maxcells, F, part, s);
+}
+
+hipError_t launch_cg_init(int dim, const PfGeom& g, long long maxcells, const double* rhs,
+                          const double* mean_sum, double ncell, double* X, double* R, double* P,
+                          double* part, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_cg_init_dim, g, maxcells, rhs, mean_sum, ncell, X, R, P, part, s);
+}
+
+hipError_t launch_cg_ap(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Bf,
+                        const double* P, double* AP, double* part, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_cg_ap_dim, g, maxcells, sc, Bf, P, AP, part, s);
+}
+
+hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const double* sc, double* X, double* R,
+                        const double* P, const double* AP, double* part, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_cg_xr_dim, g, maxcells, sc, X, R, P, AP, part, s);
+}
+
+hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* R,
+                       double* P, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_cg_p_dim, g, maxcells, sc, R, P, s);
+}
+
+hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X, double* gq,
+                           unsigned long long* gmax_bits, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_gradx_dim, g, maxfaces, X, gq, gmax_bits, s);
+}
+
 hipError_t launch_compute_dt(int dim, const double* dx, double* cmax, double* dt, double cfl,
                              int limiter, double* hist, int* step, int* err, hipStream_t s)
 {
@@ -179,6 +269,20 @@ struct cmp_level {
     unsigned long long* d_bits = nullptr;   // [0] max|u|, [1] max|Grad_Q| as bit patterns
     double* d_pfdt = nullptr;       // Phase-Field sub-step size read by the stage kernels
     double* h_pin = nullptr;        // pinned host scalars
+    // CAC mechanism and the implicit RC's solve (alpha layout unless noted)
+    double* d_Mk = nullptr;         // mech of every cell
+    double* d_Wk = nullptr;         // alpha (1 - alpha)
+    double* d_DJ = nullptr;         // Div_Jhat
+    double* d_X = nullptr;          // Q
+    double* d_R = nullptr;          // CG residual
+    double* d_Pv = nullptr;         // CG search direction (its ghosts filled like alpha's)
+    double* d_AP = nullptr;         // operator applied to d_Pv
+    double* d_Bf = nullptr;         // face coefficients b (face layout)
+    double* d_part = nullptr;       // block partial sums, 2 per block
+    double* d_sc = nullptr;         // CG_NSC device scalars
+    int nblocks = 0;                // blocks of a cell pass over the level
+    long long ncell = 0;            // valid cells of the level
+    int cg_iters = 0;               // iterations of the last implicit solve
 };
 
 namespace {
@@ -536,13 +640,16 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
 static void pf_free(cmp_level* L)
 {
     void* bufs[] = {L->d_abox, L->d_nbox, L->d_gprefix1, L->d_A0, L->d_A, L->d_P, L->d_N,
-                    L->d_J, L->d_H, L->d_gq, L->d_bits, L->d_pfdt};
+                    L->d_J, L->d_H, L->d_gq, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
+                    L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Bf, L->d_part, L->d_sc};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (L->h_pin) (void)hipHostFree(L->h_pin);
     L->d_abox = L->d_nbox = nullptr;
     L->d_gprefix1 = nullptr;
     L->d_A0 = L->d_A = L->d_P = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_pfdt = nullptr;
+    L->d_Mk = L->d_Wk = L->d_DJ = L->d_X = L->d_R = L->d_Pv = L->d_AP = L->d_Bf = nullptr;
+    L->d_part = L->d_sc = nullptr;
     L->d_bits = nullptr;
     L->h_pin = nullptr;
     L->pf_on = false;
@@ -707,8 +814,9 @@ int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nste
 int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
 {
     if (!L || !pf) return CMP_ERR_ARG;
-    if (pf->mechanism != CMP_PF_ACDI || pf->explicit_rc != 1 || pf->id_flux != 0)
+    if ((pf->mechanism != CMP_PF_ACDI && pf->mechanism != CMP_PF_CAC) || pf->id_flux != 0)
         return CMP_ERR_UNSUPPORTED;
+    if (pf->explicit_rc != 0 && pf->explicit_rc != 1) return CMP_ERR_ARG;
     if (pf->flux_phase2flux != 0 && pf->flux_phase2flux != 1) return CMP_ERR_ARG;
     if (!(pf->cfl > 0.0) || !(pf->eta_factor > 0.0) || pf->max_step < 1) return CMP_ERR_ARG;
     if (!(pf->eta > 0.0) && !(pf->eta_multiplier > 0.0)) return CMP_ERR_ARG;
@@ -762,7 +870,33 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     CMP_TRYP(hipHostMalloc((void**)&L->h_pin, sizeof(double) * 4, hipHostMallocDefault));
     // the unfilled ghost cells of the normals are never read, but keep them defined
     CMP_TRYP(hipMemset(L->d_N, 0, sizeof(double) * nn));
+    const long long nblk = ((maxcells + NT - 1) / NT) * nbox;
+    if (nblk > (1LL << 30)) { pf_free(L); return CMP_ERR_UNSUPPORTED; }
+    L->nblocks = (int)nblk;
+    CMP_TRYP(hipMalloc(&L->d_sc, sizeof(double) * CG_NSC));
+    CMP_TRYP(hipMemset(L->d_sc, 0, sizeof(double) * CG_NSC));
+    CMP_TRYP(hipMalloc(&L->d_part, sizeof(double) * 2 * nblk));
+    CMP_TRYP(hipMemset(L->d_part, 0, sizeof(double) * 2 * nblk));
+    if (pf->mechanism == CMP_PF_CAC) {
+        CMP_TRYP(hipMalloc(&L->d_Mk, sizeof(double) * na));
+        CMP_TRYP(hipMalloc(&L->d_Wk, sizeof(double) * na));
+    }
+    if (pf->explicit_rc == 0) {
+        double** cell[] = {&L->d_DJ, &L->d_X, &L->d_R, &L->d_Pv, &L->d_AP};
+        for (double** p : cell) {
+            CMP_TRYP(hipMalloc(p, sizeof(double) * na));
+            CMP_TRYP(hipMemset(*p, 0, sizeof(double) * na));
+        }
+        CMP_TRYP(hipMalloc(&L->d_Bf, sizeof(double) * nf));
+    }
 #undef CMP_TRYP
+    long long ncell = 0;
+    for (int b = 0; b < nbox; ++b) {
+        long long v = 1;
+        for (int d = 0; d < dim; ++d) v *= ba[b].hi[d] - ba[b].lo[d] + 1;
+        ncell += v;
+    }
+    L->ncell = ncell;
     L->na = na;
     L->nfaces = nf;
     L->maxcells = maxcells;
@@ -783,7 +917,12 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     L->pg.hbox = L->d_boxes;
     L->pg.abox = L->d_abox;
     L->pg.nbox = nbox;
-    for (int d = 0; d < 3; ++d) L->pg.dx[d] = (d < dim) ? L->G.dx[d] : 1.0;
+    for (int d = 0; d < 3; ++d) {
+        L->pg.dx[d] = (d < dim) ? L->G.dx[d] : 1.0;
+        L->pg.dlo[d] = (d < dim) ? L->G.domain_lo[d] : 0;
+        L->pg.dhi[d] = (d < dim) ? L->G.domain_hi[d] : 0;
+        L->pg.periodic[d] = (d < dim) ? L->G.is_periodic[d] : 1;
+    }
     L->pf = *pf;
     L->pf_lc = L->lc;
     L->pf_lc.thinc = 0;
@@ -810,6 +949,52 @@ static inline double bits_to_double(const void* p)
     double v;
     std::memcpy(&v, p, sizeof(double));
     return v;
+}
+
+// The implicit Grad_Q (PhaseField_GradQPF with Div_J, source/Class_Compressible_PhaseField.cpp:2133-2310):
+// Div_Jhat = (alpha_new - alpha)/dt (:703); b = W(phi_f) + 1e-3 on the faces from the re-sharpened
+// alpha; Q solves div(b grad Q) = Div_Jhat with periodic / homogeneous Neumann sides; Grad_Q = grad Q
+// on the faces, zero on Neumann domain faces. The reference hands the system to AMReX's MLMG (relative
+// residual 1e-10); here conjugate gradients on the zero-mean subspace run to the rounding floor
+// (r.r <= 1e-28 of the initial r.r, or 50 iterations without a new minimum), each iteration five
+// passes over the level and two fixed-order reductions, the stopping test on the device.
+static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq, hipStream_t s)
+{
+    const int dim = L->P.dim;
+    const int maxit = 100000;
+    double* sc = L->d_sc;
+    CMP_TRY(launch_pf_divjhat(dim, L->pg, L->maxcells, L->d_A, L->d_A0, L->d_DJ, dt, s));
+    CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, L->pf.eta_factor * eta, 1, s));
+    CMP_TRY(launch_pf_bcoef(dim, L->pg, L->maxfaces, L->d_P, L->d_Bf, s));
+    CMP_TRY(launch_cell_sum(dim, L->pg, L->maxcells, L->d_DJ, L->d_part, s));
+    CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_SUM, s));
+    CMP_TRY(launch_cg_init(dim, L->pg, L->maxcells, L->d_DJ, sc + CG_SUM, (double)L->ncell, L->d_X, L->d_R,
+                           L->d_Pv, L->d_part, s));
+    CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_RR, s));
+    CMP_TRY(launch_cg_start(sc, s));
+    // the host looks at the stop flag every `chunk` iterations; iterations queued past the stop
+    // return at once
+    const int chunk = 16;
+    for (long long queued = 0;; queued += chunk) {
+        for (int q = 0; q < chunk; ++q) {
+            CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_Pv, L->d_err, s));
+            CMP_TRY(launch_cg_ap(dim, L->pg, L->maxcells, sc, L->d_Bf, L->d_Pv, L->d_AP, L->d_part, s));
+            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_PAP, s));
+            CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_part, s));
+            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_RRNEW, s));
+            CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_R, L->d_Pv, s));
+            CMP_TRY(launch_cg_end(sc, maxit, s));
+        }
+        CMP_TRY(hipMemcpyAsync(L->h_pin + 2, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
+        CMP_TRY(hipMemcpyAsync(L->h_pin + 3, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, s));
+        CMP_TRY(hipStreamSynchronize(s));
+        if (L->h_pin[2] != 0.0) break;
+        if (queued > maxit) return CMP_ERR_NAN;   // the device stop test failed to fire
+    }
+    L->cg_iters = (int)L->h_pin[3];
+    CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_X, L->d_err, s));
+    CMP_TRY(launch_pf_gradx(dim, L->pg, L->maxfaces, L->d_X, gradq, L->d_bits + 1, s));
+    return CMP_OK;
 }
 
 int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, double* gradq_max,
@@ -845,8 +1030,10 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
     for (int d = 1; d < dim; ++d) sdx = sdx + 1.0 / L->G.dx[d] / L->G.dx[d];
     double dt_m = 0.9 / (2.0 * mob * lambda) / (sdx + 1.0 / eta / eta);
     const double mob_face = 0.5 * mob + 0.5 * mob;
+    const bool cac = (pf.mechanism == CMP_PF_CAC);
+    const bool explicit_rc = (pf.explicit_rc == 1);
     CMP_TRY(hipMemcpyAsync(L->d_A, L->d_A0, sizeof(double) * L->na, hipMemcpyDeviceToDevice, s));
-    CMP_TRY(hipMemsetAsync(L->d_H, 0, sizeof(double) * L->nfaces, s));
+    if (explicit_rc) CMP_TRY(hipMemsetAsync(L->d_H, 0, sizeof(double) * L->nfaces, s));
     double t = 0.0;
     const double T = dt;
     long long n = 0;
@@ -854,20 +1041,37 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
         if (t + dt_m > T) dt_m = T - t;
         if (!(dt_m > 0.0) || t + dt_m == t || n >= 10000000) return CMP_ERR_NAN;   // no progress
         CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A, L->d_err, s));
-        CMP_TRY(launch_pf_map(L->d_A, L->d_P, L->na, eta, eta, 0, s));
-        CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_P, L->d_N, s));
-        CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
-        CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, L->d_A, L->d_P, L->d_N, L->d_J, L->d_H, mob_face,
-                                 eta, dt_m, s));
-        CMP_TRY(launch_pf_update(dim, L->pg, L->maxcells, L->d_A, L->d_J, dt_m, s));
+        if (cac) {
+            // normals of alpha itself; mech and W per cell, their level sums, alpha += dt_m (mech + W L)
+            CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_A, L->d_N, s));
+            CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
+            CMP_TRY(launch_pf_cac_mech(dim, L->pg, L->maxcells, L->d_A, L->d_N, L->d_Mk, L->d_Wk, mob, lambda,
+                                       eta, L->d_part, s));
+            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, L->d_sc + CG_SUM, s));
+            CMP_TRY(launch_pf_cac_update(dim, L->pg, L->maxcells, L->d_A, L->d_Mk, L->d_Wk, L->d_sc + CG_SUM,
+                                         dt_m, s));
+        } else {
+            CMP_TRY(launch_pf_map(L->d_A, L->d_P, L->na, eta, eta, 0, s));
+            CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_P, L->d_N, s));
+            CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
+            // Jhat is accumulated for the explicit RC only
+            CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, L->d_A, L->d_P, L->d_N, L->d_J,
+                                     explicit_rc ? L->d_H : nullptr, mob_face, eta, dt_m, s));
+            CMP_TRY(launch_pf_update(dim, L->pg, L->maxcells, L->d_A, L->d_J, dt_m, s));
+        }
         t += dt_m;
         ++n;
     }
-    CMP_TRY(launch_scale(L->d_H, L->nfaces, 1.0 / dt, s));
-    // explicit Grad_Q from the step's alpha1 (PhaseField_GradQPF, :2315-2403), re-sharpened at
-    // eta_factor * eta
-    CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
-    CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, gradq, L->d_bits + 1, s));
+    if (explicit_rc) {
+        CMP_TRY(launch_scale(L->d_H, L->nfaces, 1.0 / dt, s));
+        // explicit Grad_Q from the step's alpha1 (PhaseField_GradQPF, :2315-2403), re-sharpened at
+        // eta_factor * eta
+        CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
+        CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, gradq, L->d_bits + 1, s));
+    } else {
+        int r = pf_gradq_implicit(L, dt, eta, gradq, s);
+        if (r) return r;
+    }
     CMP_TRY(hipMemcpyAsync(L->h_pin + 1, L->d_bits + 1, sizeof(double), hipMemcpyDeviceToHost, s));
     CMP_TRY(hipStreamSynchronize(s));
     if (gradq_max) *gradq_max = bits_to_double(L->h_pin + 1);

@@ -612,6 +612,36 @@ __device__ __forceinline__ double pf_acdi_flux(double pf_m, double pf_c, double 
     return mob * (grad - pf_map_forward_d(psi, eta) * nrm);
 }
 
+// CAC-Adv mechanism of one cell, mech = -M mu, mu = Lambda (g'(phi)/eta^2 - lap phi - |grad phi| kappa),
+// kappa = -div n with the normals averaged to the faces (FDM_PhaseFieldMechanism_ACAdv_D.H,
+// FDM_ChemicalPotential_Adv_D.H:12-126, ChemicalPotential_Adv.H, PhaseFieldMechanism_AC.H:12-17,
+// PotentialFunction_DoubleWell.H:22-23). fm/fp: phi at the neighbours along each direction;
+// nc/nm/np: component d of the normal at the cell and its neighbours along d.
+template <int DIM>
+__device__ __forceinline__ double pf_cac_mech(const double* dx, double f, const double* fm,
+                                              const double* fp, const double* nc, const double* nm,
+                                              const double* np, double mob, double lambda, double eta)
+{
+    double lap = 0.0, divn = 0.0, absg = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double gp = (fp[d] - f) / dx[d];
+        const double gm = (f - fm[d]) / dx[d];
+        const double t = (gp - gm) / dx[d];
+        lap = (d == 0) ? t : lap + t;
+        const double nfp = 0.5 * nc[d] + 0.5 * np[d];
+        const double nfm = 0.5 * nm[d] + 0.5 * nc[d];
+        const double tn = (nfp - nfm) / dx[d];
+        divn = (d == 0) ? tn : divn + tn;
+        const double g = 0.5 * gm + 0.5 * gp;
+        absg += g * g;
+    }
+    absg = sqrt(absg);
+    const double g1 = 2.0 * f * (1.0 - f) * (1.0 - 2.0 * f);
+    const double mu = lambda * ((1.0 / eta / eta) * g1 - lap - absg * (-divn));
+    return -mob * mu;
+}
+
 // Explicit Grad_Q of one face (source/Class_Compressible_PhaseField.cpp:2358-2391,
 // include/Physics_PhaseField/PhaseFieldFlux2GradQPF.H:12-28) from the re-sharpened cell values
 // m = F(B(alpha_m, eta), eta_factor eta), c likewise.

@@ -88,6 +88,8 @@ struct PfGeom {
     const BoxDev* abox;    // alpha layout (1 ghost, 1 comp); normals use DIM * off with the same strides
     int nbox;
     double dx[3];
+    int dlo[3], dhi[3];    // domain index range
+    int periodic[3];
 };
 
 hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A,
@@ -106,6 +108,55 @@ hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, 
                            const double* H, double* gq, unsigned long long* gmax_bits, hipStream_t s);
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
                             unsigned long long* bits, hipStream_t s);
+
+// CAC mechanism and the implicit Grad_Q. Block partial sums go to `part` (one or two per block,
+// gridDim.x * nbox blocks), reduced in a fixed order by launch_reduce_sums into sums[0..n-1].
+hipError_t launch_pf_cac_mech(int dim, const PfGeom& g, long long maxcells, const double* A,
+                              const double* Nrm, double* Mk, double* Wk, double mob, double lambda,
+                              double eta, double* part, hipStream_t s);
+hipError_t launch_pf_cac_update(int dim, const PfGeom& g, long long maxcells, double* A,
+                                const double* Mk, const double* Wk, const double* sums, double dt_m,
+                                hipStream_t s);
+hipError_t launch_pf_divjhat(int dim, const PfGeom& g, long long maxcells, const double* A,
+                             const double* A0, double* DJ, double dt, hipStream_t s);
+hipError_t launch_pf_bcoef(int dim, const PfGeom& g, const long long* maxfaces, const double* S,
+                           double* Bf, hipStream_t s);
+// block partial sums of a field over the valid cells
+hipError_t launch_cell_sum(int dim, const PfGeom& g, long long maxcells, const double* F, double* part,
+                           hipStream_t s);
+// CG pieces: r = -(rhs - mean_sum[0]/ncell), p = r, x = 0, partial of r.r
+hipError_t launch_cg_init(int dim, const PfGeom& g, long long maxcells, const double* rhs,
+                          const double* mean_sum, double ncell, double* X, double* R, double* P,
+                          double* part, hipStream_t s);
+// Device scalars of the CG solve (a double array of CG_NSC entries). The iteration kernels return at
+// once when sc[CG_DONE] is set, so the host only polls the flag every few iterations.
+enum {
+    CG_RR = 0,      // r.r of the current residual
+    CG_PAP = 1,     // p.Ap of this iteration
+    CG_RRNEW = 2,   // r.r after this iteration's update
+    CG_B2 = 3,      // r.r of the initial residual
+    CG_DONE = 4,    // 1 once the solve has stopped
+    CG_BEST = 5,    // smallest r.r so far
+    CG_STALL = 6,   // iterations since the last new minimum
+    CG_IT = 7,      // iterations done
+    CG_SUM = 8,     // two slots for level sums (CAC's sum mech, sum W; the mean of the rhs)
+    CG_NSC = 16
+};
+// ap = -div(b grad p) (p ghosts filled; Neumann domain faces carry no flux), partial of p.ap
+hipError_t launch_cg_ap(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Bf,
+                        const double* P, double* AP, double* part, hipStream_t s);
+// al = rr/pAp; x += al p; r -= al ap; partial of r.r (nothing when pAp <= 0)
+hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const double* sc, double* X,
+                        double* R, const double* P, const double* AP, double* part, hipStream_t s);
+// be = rr_new/rr; p = r + be p
+hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* R,
+                       double* P, hipStream_t s);
+hipError_t launch_cg_start(double* sc, hipStream_t s);
+hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s);
+// face gradient of x (ghosts filled), zero on Neumann domain faces, with max |.|
+hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X,
+                           double* gq, unsigned long long* gmax_bits, hipStream_t s);
+hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s);
 hipError_t launch_scale(double* p, long long n, double s_, hipStream_t s);
 hipError_t launch_setd(double* p, double v, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);

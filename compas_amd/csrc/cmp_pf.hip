@@ -1,4 +1,5 @@
-// cmp_pf.hip - gfx950 kernels of the Phase-Field step's mechanism and Grad_Q (ACDI, explicit RC).
+// cmp_pf.hip - gfx950 kernels of the Phase-Field step's mechanism (ACDI, CAC-Adv) and Grad_Q (explicit
+// RC, or the implicit RC's variable-coefficient Poisson solve by conjugate gradients).
 //
 // Built twice, with CMP_KERNEL_DIM=2 and CMP_KERNEL_DIM=3. Every kernel runs over all boxes of a
 // level at once: blockIdx.y is the box, blockIdx.x * 256 + threadIdx.x the cell or face inside it.
@@ -71,6 +72,41 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long* bits)
     }
 }
 
+// Face index (box face buffer of direction d, runtime d) of cell (i,j,k)'s low (off 0) or high
+// (off 1) face.
+__device__ __forceinline__ long long face_index_rt(const BoxDev& bx, int d, int i, int j, int k, int off)
+{
+    const int n0 = bx.hi[0] - bx.lo[0] + 1 + (d == 0), n1 = bx.hi[1] - bx.lo[1] + 1 + (d == 1);
+    const int x = i - bx.lo[0] + (d == 0 ? off : 0);
+    const int y = j - bx.lo[1] + (d == 1 ? off : 0);
+    const int z = k - bx.lo[2] + (d == 2 ? off : 0);
+    return bx.goff[d] + (long long)x + (long long)n0 * (y + (long long)n1 * z);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block sum of v into part[idx], in a fixed order (the same inputs give the same bits). Every thread
+// of the block must call it.
+__device__ __forceinline__ void block_sum_to(double v, double* part, long long idx)
+{
+    __shared__ double red[NT / 64];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = red[0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) s += red[w];
+        part[idx] = s;
+    }
+    __syncthreads();
+}
+
 }  // namespace
 
 // alpha1 of the flow state's valid cells -> the alpha buffer
@@ -135,7 +171,7 @@ __global__ __launch_bounds__(NT) void k_pf_normals(PfGeom g, const double* __res
     for (int d = 0; d < DIM; ++d) Nb[c + d * ab.sn] = (sum == 0.0) ? 0.0 : gr[d] / r;
 }
 
-// ACDI flux on every face of direction D: J stored, Jhat += dt_m J
+// ACDI flux on every face of direction D: J stored, Jhat += dt_m J (explicit RC)
 // (PhaseField_ACDI, source/Class_Compressible_PhaseField.cpp:1609-1640; Saxpy of :679-683).
 template <int DIM, int D>
 __global__ __launch_bounds__(NT) void k_pf_jfaces(PfGeom g, const double* __restrict__ A,
@@ -155,7 +191,7 @@ __global__ __launch_bounds__(NT) void k_pf_jfaces(PfGeom g, const double* __rest
     const double Jf = pf_acdi_flux(Ab[m], Ab[c], Pb[m], Pb[c], Nb[m], Nb[c], mob_face, eta, g.dx[D]);
     const long long f = ab.goff[D] + q;
     J[f] = Jf;
-    H[f] = H[f] + dt_m * Jf;
+    if (H) H[f] = H[f] + dt_m * Jf;   // H is null when Jhat is not needed (implicit RC)
 }
 
 // alpha1 += dt_m div J on valid cells (FVM_Divergence_K + LinComb, :685-688).
@@ -223,7 +259,305 @@ __global__ __launch_bounds__(NT) void k_pf_maxvel(PfGeom g, const double* __rest
     block_max_to(v, bits);
 }
 
+static dim3 grid_of_cells(long long per_box, int nbox)
+{
+    return dim3((unsigned)((per_box + NT - 1) / NT), (unsigned)nbox);
+}
+
+// ---- CAC mechanism (PhaseField_Mechanism, source/Class_Compressible_PhaseField.cpp:1007-1152) -----
+
+// mech and W = alpha (1 - alpha) of every valid cell, with block partial sums of both
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_pf_cac_mech(PfGeom g, const double* __restrict__ A,
+                                                    const double* __restrict__ Nrm, double* Mk, double* Wk,
+                                                    double mob, double lambda, double eta, double* part)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    double mk = 0.0, wk = 0.0;
+    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+        const double* Ab = A + ab.off;
+        const double* Nb = Nrm + (long long)DIM * ab.off;
+        const long long c = fab_index(ab, i, j, k);
+        const long long st[3] = {1, ab.sj, ab.sk};
+        double fm[DIM], fp[DIM], nc[DIM], nm[DIM], np[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            fm[d] = Ab[c - st[d]];
+            fp[d] = Ab[c + st[d]];
+            nc[d] = Nb[c + d * ab.sn];
+            nm[d] = Nb[c - st[d] + d * ab.sn];
+            np[d] = Nb[c + st[d] + d * ab.sn];
+        }
+        const double f = Ab[c];
+        mk = pf_cac_mech<DIM>(g.dx, f, fm, fp, nc, nm, np, mob, lambda, eta);
+        wk = f * (1.0 - f);
+        Mk[ab.off + c] = mk;
+        Wk[ab.off + c] = wk;
+    }
+    const long long blk = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    const long long nblk = (long long)gridDim.x * gridDim.y;
+    block_sum_to(mk, part, blk);
+    block_sum_to(wk, part, nblk + blk);
+}
+
+// alpha += dt_m (mech + W L), L = -sum mech / sum W (0 when sum W = 0, PhaseFieldMechanism_CAC.H)
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_pf_cac_update(PfGeom g, double* A, const double* __restrict__ Mk,
+                                                      const double* __restrict__ Wk,
+                                                      const double* __restrict__ sums, double dt_m)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    const double sw = sums[1];
+    const double L = (sw == 0.0) ? 0.0 : -sums[0] / sw;
+    const long long c = ab.off + fab_index(ab, i, j, k);
+    const double divj = Mk[c] + Wk[c] * L;
+    A[c] = 1.0 * A[c] + dt_m * divj;
+}
+
+// Div_Jhat = (1/dt) alpha_new + (-1/dt) alpha (LinComb of PhaseFieldMechanism_Evolve, :703)
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_pf_divjhat(PfGeom g, const double* __restrict__ A,
+                                                   const double* __restrict__ A0, double* DJ, double a,
+                                                   double b)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    const long long c = ab.off + fab_index(ab, i, j, k);
+    DJ[c] = a * A[c] + b * A0[c];
+}
+
+// ---- implicit Grad_Q (PhaseField_GradQPF with Div_J, :2133-2310) ---------------------------------
+
+// b = W(phi_f) + 1e-3 on every face of direction D, phi_f the face mean of the re-sharpened alpha
+template <int DIM, int D>
+__global__ __launch_bounds__(NT) void k_pf_bcoef(PfGeom g, const double* __restrict__ S, double* Bf)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
+    int i, j, k;
+    if (!face_of<DIM, D>(ab, q, i, j, k)) return;
+    const long long c = fab_index(ab, i, j, k);
+    const long long m = c - ((D == 0) ? 1 : ((D == 1) ? ab.sj : ab.sk));
+    const double* Sb = S + ab.off;
+    const double pf = 0.5 * Sb[m] + 0.5 * Sb[c];
+    Bf[ab.goff[D] + q] = pf * (1.0 - pf) + 1.0e-3;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_cg_init(PfGeom g, const double* __restrict__ rhs,
+                                                const double* __restrict__ mean_sum, double ncell,
+                                                double* X, double* R, double* P, double* part)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    double v = 0.0;
+    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+        const long long c = ab.off + fab_index(ab, i, j, k);
+        const double mean = mean_sum[0] / ncell;
+        const double r = -(rhs[c] - mean);
+        X[c] = 0.0;
+        R[c] = r;
+        P[c] = r;
+        v = r * r;
+    }
+    block_sum_to(v, part, (long long)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// the level sum of rhs over valid cells (for its mean)
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_cell_sum(PfGeom g, const double* __restrict__ F, double* part)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    double v = 0.0;
+    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) v = F[ab.off + fab_index(ab, i, j, k)];
+    block_sum_to(v, part, (long long)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_cg_ap(PfGeom g, const double* __restrict__ sc,
+                                              const double* __restrict__ Bf, const double* __restrict__ P,
+                                              double* AP, double* part)
+{
+    if (sc[CG_DONE] != 0.0) return;   // uniform over the grid
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    double v = 0.0;
+    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+        const double* Pb = P + ab.off;
+        const long long c = fab_index(ab, i, j, k);
+        const long long st[3] = {1, ab.sj, ab.sk};
+        const int co[3] = {i, j, k};
+        const double pc = Pb[c];
+        double acc = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const bool lo_dom = !g.periodic[d] && co[d] == g.dlo[d];
+            const bool hi_dom = !g.periodic[d] && co[d] == g.dhi[d];
+            const double fp_ = hi_dom ? 0.0 : Bf[face_index_rt(ab, d, i, j, k, 1)] * (Pb[c + st[d]] - pc) / g.dx[d];
+            const double fm_ = lo_dom ? 0.0 : Bf[face_index_rt(ab, d, i, j, k, 0)] * (pc - Pb[c - st[d]]) / g.dx[d];
+            const double t = (fp_ - fm_) / g.dx[d];
+            acc = (d == 0) ? t : acc + t;
+        }
+        AP[ab.off + c] = -acc;
+        v = pc * (-acc);
+    }
+    block_sum_to(v, part, (long long)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_cg_xr(PfGeom g, const double* __restrict__ sc, double* X, double* R,
+                                              const double* __restrict__ P, const double* __restrict__ AP,
+                                              double* part)
+{
+    if (sc[CG_DONE] != 0.0 || !(sc[CG_PAP] > 0.0)) return;
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    double v = 0.0;
+    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+        const long long c = ab.off + fab_index(ab, i, j, k);
+        const double al = sc[CG_RR] / sc[CG_PAP];
+        X[c] += al * P[c];
+        const double r = R[c] - al * AP[c];
+        R[c] = r;
+        v = r * r;
+    }
+    block_sum_to(v, part, (long long)blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_cg_p(PfGeom g, const double* __restrict__ sc,
+                                             const double* __restrict__ R, double* P)
+{
+    if (sc[CG_DONE] != 0.0 || !(sc[CG_PAP] > 0.0)) return;
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    const long long c = ab.off + fab_index(ab, i, j, k);
+    const double be = sc[CG_RRNEW] / sc[CG_RR];
+    P[c] = R[c] + be * P[c];
+}
+
+// Grad_Q = (x_c - x_m)/dx on every face of direction D; zero on Neumann domain faces
+template <int DIM, int D>
+__global__ __launch_bounds__(NT) void k_pf_gradx(PfGeom g, const double* __restrict__ X, double* gq,
+                                                 unsigned long long* gmax_bits)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
+    int i, j, k;
+    double v = 0.0;
+    if (face_of<DIM, D>(ab, q, i, j, k)) {
+        const int co = (D == 0) ? i : ((D == 1) ? j : k);
+        const bool on_dom = !g.periodic[D] && (co == g.dlo[D] || co == g.dhi[D] + 1);
+        const long long c = fab_index(ab, i, j, k);
+        const long long m = c - ((D == 0) ? 1 : ((D == 1) ? ab.sj : ab.sk));
+        const double* Xb = X + ab.off;
+        const double r = on_dom ? 0.0 : (Xb[c] - Xb[m]) / g.dx[D];
+        gq[ab.goff[D] + q] = r;
+        v = fabs(r);
+        if (__builtin_isnan(r)) v = __longlong_as_double(0x7ff8000000000000LL);
+    }
+    block_max_to(v, gmax_bits);
+}
+
 // ---- launchers -----------------------------------------------------------------------------------
+
+template <int DIM>
+hipError_t launch_pf_cac_mech_dim(const PfGeom& g, long long maxcells, const double* A, const double* Nrm,
+                                  double* Mk, double* Wk, double mob, double lambda, double eta,
+                                  double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pf_cac_mech<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, A, Nrm, Mk,
+                       Wk, mob, lambda, eta, part);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_pf_cac_update_dim(const PfGeom& g, long long maxcells, double* A, const double* Mk,
+                                    const double* Wk, const double* sums, double dt_m, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pf_cac_update<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, A, Mk, Wk,
+                       sums, dt_m);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_pf_divjhat_dim(const PfGeom& g, long long maxcells, const double* A, const double* A0,
+                                 double* DJ, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pf_divjhat<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, A, A0, DJ,
+                       1.0 / dt, -1.0 / dt);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_pf_bcoef_dim(const PfGeom& g, const long long* maxfaces, const double* S, double* Bf,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pf_bcoef<DIM, 0>), grid_of_cells(maxfaces[0], g.nbox), dim3(NT), 0, s, g, S, Bf);
+    hipLaunchKernelGGL((k_pf_bcoef<DIM, 1>), grid_of_cells(maxfaces[1], g.nbox), dim3(NT), 0, s, g, S, Bf);
+    if (DIM == 3)
+        hipLaunchKernelGGL((k_pf_bcoef<DIM, (DIM == 3 ? 2 : 1)>), grid_of_cells(maxfaces[2], g.nbox), dim3(NT),
+                           0, s, g, S, Bf);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_cell_sum_dim(const PfGeom& g, long long maxcells, const double* F, double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cell_sum<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, F, part);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_cg_init_dim(const PfGeom& g, long long maxcells, const double* rhs, const double* mean_sum,
+                              double ncell, double* X, double* R, double* P, double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg_init<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, rhs, mean_sum,
+                       ncell, X, R, P, part);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_cg_ap_dim(const PfGeom& g, long long maxcells, const double* sc, const double* Bf,
+                            const double* P, double* AP, double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg_ap<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, Bf, P, AP, part);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_cg_xr_dim(const PfGeom& g, long long maxcells, const double* sc, double* X, double* R,
+                            const double* P, const double* AP, double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg_xr<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, X, R, P, AP, part);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_cg_p_dim(const PfGeom& g, long long maxcells, const double* sc, const double* R, double* P,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg_p<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, R, P);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_pf_gradx_dim(const PfGeom& g, const long long* maxfaces, const double* X, double* gq,
+                               unsigned long long* bits, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pf_gradx<DIM, 0>), grid_of_cells(maxfaces[0], g.nbox), dim3(NT), 0, s, g, X, gq, bits);
+    hipLaunchKernelGGL((k_pf_gradx<DIM, 1>), grid_of_cells(maxfaces[1], g.nbox), dim3(NT), 0, s, g, X, gq, bits);
+    if (DIM == 3)
+        hipLaunchKernelGGL((k_pf_gradx<DIM, (DIM == 3 ? 2 : 1)>), grid_of_cells(maxfaces[2], g.nbox), dim3(NT), 0,
+                           s, g, X, gq, bits);
+    return hipGetLastError();
+}
 
 static dim3 grid_of(long long per_box, int nbox)
 {
@@ -329,6 +663,83 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s)
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p, n);
     return hipGetLastError();
 }
+
+// sums[v] = sum_b part[v * nblocks + b], one workgroup, a fixed order of additions
+__global__ __launch_bounds__(NT) void k_reduce_sums(const double* __restrict__ part, int nblocks, int nvals,
+                                                    double* sums)
+{
+    __shared__ double red[NT];
+    for (int v = 0; v < nvals; ++v) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += NT) s += part[(long long)v * nblocks + b];
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = NT / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) sums[v] = red[0];
+        __syncthreads();
+    }
+}
+
+// CG bookkeeping on the device, so that the iteration stops where the host loop of the solve's
+// restatement stops without a host round trip per iteration: start after r.r of the initial
+// residual landed in sc[CG_RR]; end after each iteration (pAp <= 0 stops without counting it;
+// r.r <= 1e-28 b.b, 50 iterations without a new minimum or maxit stop after it).
+__global__ void k_cg_start(double* sc)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rr = sc[CG_RR];
+    sc[CG_B2] = rr;
+    sc[CG_BEST] = rr;
+    sc[CG_STALL] = 0.0;
+    sc[CG_IT] = 0.0;
+    sc[CG_DONE] = (rr > 0.0) ? 0.0 : 1.0;   // a zero (or NaN) right-hand side: x = 0
+}
+
+__global__ void k_cg_end(double* sc, double maxit)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sc[CG_DONE] != 0.0) return;
+    if (!(sc[CG_PAP] > 0.0)) {
+        sc[CG_DONE] = 1.0;
+        return;
+    }
+    const double rr = sc[CG_RRNEW];
+    sc[CG_RR] = rr;
+    const double it = sc[CG_IT] + 1.0;
+    sc[CG_IT] = it;
+    bool done = false;
+    if (rr < sc[CG_BEST]) {
+        sc[CG_BEST] = rr;
+        sc[CG_STALL] = 0.0;
+    } else {
+        const double st = sc[CG_STALL] + 1.0;
+        sc[CG_STALL] = st;
+        done = st > 50.0;
+    }
+    if (!(rr > 1.0e-28 * sc[CG_B2]) || !(it < maxit)) done = true;
+    if (done) sc[CG_DONE] = 1.0;
+}
+
+hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_sums, dim3(1), dim3(NT), 0, s, part, nblocks, nvals, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_start(double* sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cg_end, dim3(1), dim3(64), 0, s, sc, (double)maxit);
+    return hipGetLastError();
+}
 #endif
 
 template hipError_t launch_pf_extract_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
@@ -345,5 +756,26 @@ template hipError_t launch_pf_gradq_dim<CMP_KERNEL_DIM>(const PfGeom&, const lon
                                                         hipStream_t);
 template hipError_t launch_pf_maxvel_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
                                                          unsigned long long*, hipStream_t);
+template hipError_t launch_pf_cac_mech_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
+                                                           const double*, double*, double*, double, double,
+                                                           double, double*, hipStream_t);
+template hipError_t launch_pf_cac_update_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, double*, const double*,
+                                                             const double*, const double*, double, hipStream_t);
+template hipError_t launch_pf_divjhat_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
+                                                          const double*, double*, double, hipStream_t);
+template hipError_t launch_pf_bcoef_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
+                                                        double*, hipStream_t);
+template hipError_t launch_cell_sum_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, double*,
+                                                        hipStream_t);
+template hipError_t launch_cg_init_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
+                                                       double, double*, double*, double*, double*, hipStream_t);
+template hipError_t launch_cg_ap_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
+                                                     const double*, double*, double*, hipStream_t);
+template hipError_t launch_cg_xr_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, double*, double*,
+                                                     const double*, const double*, double*, hipStream_t);
+template hipError_t launch_cg_p_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
+                                                    double*, hipStream_t);
+template hipError_t launch_pf_gradx_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*, double*,
+                                                        unsigned long long*, hipStream_t);
 
 }  // namespace cmp

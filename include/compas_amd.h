@@ -82,11 +82,12 @@ typedef struct cmp_box {
 
 /* Phase-field regularization parameters: the subset of PhaseField_Parameter
  * (include/Physics_PhaseField/PhaseField_Parameter.H:29-144) that the ACDI path reads. */
+#define CMP_PF_CAC  1      /* PhaseField.Mechanism = "CAC-Adv" (ID_Mechanism 1, not flux based)      */
 #define CMP_PF_ACDI 3      /* PhaseField.Mechanism = "ACDI" (ID_Mechanism 3, flux based)             */
 typedef struct cmp_pf_params {
-    int    mechanism;        /* ID_Mechanism; CMP_PF_ACDI is the one built here                      */
-    int    explicit_rc;      /* ID_ExplicitRC: 1 = Grad_Q from the time-averaged flux Jhat;
-                              * 0 (implicit, an elliptic solve) is not built here                    */
+    int    mechanism;        /* ID_Mechanism: CMP_PF_CAC or CMP_PF_ACDI                              */
+    int    explicit_rc;      /* ID_ExplicitRC: 1 = Grad_Q from the time-averaged flux Jhat (ACDI
+                              * only); 0 = Grad_Q from the elliptic solve div(b grad Q) = div Jhat  */
     int    id_flux;          /* ID_Flux: 0 = UD flux                                                  */
     int    flux_phase2flux;  /* ID_FluxPhase2Flux: 1 = mixture formulation, 0 = phasic              */
     double cfl;              /* PhaseField.cfl of the Phase-Field sub-steps                           */

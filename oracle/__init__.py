@@ -81,7 +81,9 @@ def load():
         "ref_level_pf_rhs": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, PD]),
         "ref_level_pf_rk3_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, D]),
         "ref_level_max_velocity": (D, [PP, ctypes.c_int, PB, ctypes.c_int, PD]),
-        "ref_level_pf_mechanism": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, D, D, D, D, PD, PI]),
+        "ref_level_pf_mechanism": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, D, D, D, D, PD, PD, PI]),
+        "ref_pf_cac_mech": (D, [ctypes.c_int, PD, D, PD, PD, PD, PD, PD, D, D, D]),
+        "ref_level_pf_gradq_implicit": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, PD, D, PD, PI]),
         "ref_level_pf_gradq": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, PD, D, PD]),
         "ref_level_pf_gradq_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, D,
                                                    PD, PD, PD]),
@@ -340,16 +342,42 @@ def level_max_velocity(params, layout, buf):
                                          _dptr(buf))
 
 
-def level_pf_mechanism(params, pf, geom, alayout, alpha, dt, mob, eta, lam, nfaces):
-    """Jhat (face buffer of nfaces) and the number of sub-steps; alpha: 1 comp, 1 ghost, filled."""
-    jhat = np.zeros(nfaces)
+def level_pf_mechanism(params, pf, geom, alayout, alpha, dt, mob, eta, lam, nfaces, implicit=False):
+    """Jhat (face buffer of nfaces), or Div_Jhat (alpha layout) with implicit=True, and the number
+    of sub-steps; alpha: 1 comp, 1 ghost, filled."""
+    jhat = None if implicit else np.zeros(nfaces)
+    dj = np.zeros_like(alpha) if implicit else None
     n = ctypes.c_int(0)
     rc = load().ref_level_pf_mechanism(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
                                        alayout.nbox, alayout.cboxes, _dptr(alpha), float(dt), float(mob),
-                                       float(eta), float(lam), _dptr(jhat), ctypes.byref(n))
+                                       float(eta), float(lam), _dptr(jhat), _dptr(dj), ctypes.byref(n))
     if rc:
         raise RuntimeError("ref_level_pf_mechanism failed: %d" % rc)
-    return jhat, n.value
+    return (dj if implicit else jhat), n.value
+
+
+def pf_cac_mech(dim, dx, f, fm, fp, nc, nm, np_, mob, lam, eta):
+    """CAC-Adv mechanism of one cell; dx, fm, fp, nc, nm, np_: sequences of dim values."""
+    def v3(x):
+        a = np.zeros(3)
+        a[:dim] = np.asarray(x, dtype=np.float64)[:dim]
+        return a
+    arr = [v3(x) for x in (dx, fm, fp, nc, nm, np_)]
+    return load().ref_pf_cac_mech(int(dim), _dptr(arr[0]), float(f), _dptr(arr[1]), _dptr(arr[2]),
+                                  _dptr(arr[3]), _dptr(arr[4]), _dptr(arr[5]), float(mob), float(lam),
+                                  float(eta))
+
+
+def level_pf_gradq_implicit(params, pf, geom, alayout, alpha, divjhat, eta, nfaces):
+    """(Grad_Q face buffer, CG iterations) of the implicit RC."""
+    gq = np.zeros(nfaces)
+    it = ctypes.c_int(0)
+    rc = load().ref_level_pf_gradq_implicit(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom),
+                                            alayout.nbox, alayout.cboxes, _dptr(alpha), _dptr(divjhat),
+                                            float(eta), _dptr(gq), ctypes.byref(it))
+    if rc:
+        raise RuntimeError("ref_level_pf_gradq_implicit failed: %d" % rc)
+    return gq, it.value
 
 
 def level_pf_gradq(params, pf, geom, alayout, alpha, jhat, eta):

@@ -1351,21 +1351,58 @@ static cmp_box* pf_boxes(int dim, int nbox, const cmp_box* boxes, int ng, int nc
     return bb;
 }
 
-/* The ACDI mechanism over one flow step of size dt (PhaseFieldMechanism_Evolve, :597-709, with
- * PhaseField_Mechanism :1155-1233, PhaseField_ACDI :1495-1644 and PhaseField_Normal :1958-2032):
+/* The CAC-Adv mechanism of one cell (FDM_PhaseFieldMechanism_ACAdv_{K,D},
+ * include/Operators_PhaseField/FDM_PhaseFieldMechanism_ACAdv_D.H; FDM_ChemicalPotential_Adv_D.H:12-126;
+ * ChemicalPotential_Adv.H; PhaseFieldMechanism_AC.H:12-17; PotentialFunction_DoubleWell.H:22-23):
+ *   mu = Lambda (g'(phi)/eta^2 - lap phi - |grad phi| kappa), kappa = -div n (n averaged to faces),
+ *   mech = -M mu.
+ * f, fm[d], fp[d]: phi at the cell and its neighbours; nc[d], nm[d], np[d]: component d of the
+ * normal at the cell and at its neighbours along d. */
+double ref_pf_cac_mech(int dim, const double* dx, double f, const double* fm, const double* fp,
+                       const double* nc, const double* nm, const double* np, double mob,
+                       double lambda, double eta)
+{
+    double lap = 0.0, divn = 0.0, absg = 0.0;
+    for (int d = 0; d < dim; ++d) {
+        double gp = (fp[d] - f) / dx[d];
+        double gm = (f - fm[d]) / dx[d];
+        double t = (gp - gm) / dx[d];
+        lap = (d == 0) ? t : lap + t;
+        double nfp = 0.5 * nc[d] + 0.5 * np[d];
+        double nfm = 0.5 * nm[d] + 0.5 * nc[d];
+        double tn = (nfp - nfm) / dx[d];
+        divn = (d == 0) ? tn : divn + tn;
+        double g = 0.5 * gm + 0.5 * gp;
+        absg += g * g;
+    }
+    absg = sqrt(absg);
+    double curv = -divn;
+    double g1 = 2.0 * f * (1.0 - f) * (1.0 - 2.0 * f);
+    double mu = lambda * ((1.0 / eta / eta) * g1 - lap - absg * curv);
+    return -mob * mu;
+}
+
+/* The mechanism over one flow step of size dt (PhaseFieldMechanism_Evolve, :597-709, with
+ * PhaseField_Mechanism :1007-1152 (CAC) / :1155-1233 (flux based), PhaseField_ACDI :1495-1644 and
+ * PhaseField_Normal :1958-2032):
  *   dt_m = 0.9/(2 M Lambda)/(sum_d 1/dx_d^2 + 1/eta^2), the last sub-step cut to end at dt;
- *   per sub-step: ghosts of alpha (PF_bcs), psi = Mapping_Backward(alpha), cell normals of psi
- *   (FDM_Normal_CenterFromCenter_D, include/Operators_FD/FDM_Normal_CenterFromCenter_D.H:12-63)
- *   with their own ghost fill, J on every face, Jhat += dt_m J, alpha += dt_m div J
- *   (FVM_Divergence_D, include/Operators_FV/FVM_Divergence_D.H);
- *   finally Jhat *= 1/dt.
+ *   per sub-step: ghosts of alpha (PF_bcs), cell normals (FDM_Normal_CenterFromCenter_D,
+ *   include/Operators_FD/FDM_Normal_CenterFromCenter_D.H:12-63) of psi = Mapping_Backward(alpha)
+ *   for ACDI and of alpha itself for CAC, with their own ghost fill; then
+ *     ACDI: J on every face, Jhat += dt_m J, alpha += dt_m div J (FVM_Divergence_D);
+ *     CAC: mech and W = alpha (1 - alpha) per cell, their sums over the level, L = -sum mech / sum W
+ *          (0 when sum W = 0, PhaseFieldMechanism_CAC.H:12-26), alpha += dt_m (mech + W L);
+ *   finally Jhat *= 1/dt (explicit RC) or Div_Jhat = (alpha_new - alpha)/dt (implicit RC).
  * alpha: the level's alpha1 with one ghost cell (offsets of cmp_level_layout for 1 comp, 1 ghost),
- * ghosts filled; jhat: face buffer (1 comp), written. Returns the number of sub-steps in *nsub. */
+ * ghosts filled; jhat: face buffer (1 comp) or NULL; divjhat: alpha layout (valid cells written)
+ * or NULL. Returns the number of sub-steps in *nsub. */
 int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                            int nbox, const cmp_box* boxes, const double* alpha, double dt,
-                           double mob, double eta, double lambda, double* jhat, int* nsub)
+                           double mob, double eta, double lambda, double* jhat, double* divjhat,
+                           int* nsub)
 {
     const int dim = P->dim;
+    const int cac = (PF->mechanism == CMP_PF_CAC);
     long long ta, tn, tf;
     cmp_box* ba = pf_boxes(dim, nbox, boxes, 1, 1, &ta);
     cmp_box* bn = pf_boxes(dim, nbox, boxes, 1, dim, &tn);
@@ -1374,9 +1411,11 @@ int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const c
     double* A = (double*)malloc(sizeof(double) * ta);
     double* Nrm = (double*)calloc((size_t)tn, sizeof(double));
     double* J = (double*)malloc(sizeof(double) * tf);
+    double* Mk = (double*)calloc((size_t)ta, sizeof(double));
+    double* Wk = (double*)calloc((size_t)ta, sizeof(double));
     memcpy(A, alpha, sizeof(double) * ta);
-    for (long long q = 0; q < tf; ++q) jhat[q] = 0.0;
-    (void)PF;   /* ACDI is the only mechanism restated here */
+    if (jhat)
+        for (long long q = 0; q < tf; ++q) jhat[q] = 0.0;
     double s = 1.0 / G->dx[0] / G->dx[0];
     for (int d = 1; d < dim; ++d) s = s + 1.0 / G->dx[d] / G->dx[d];
     double dt_m = 0.9 / (2.0 * mob * lambda) / (s + 1.0 / eta / eta);
@@ -1398,13 +1437,15 @@ int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const c
                     for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
                         int c3[3] = {i, j, k};
                         long long c = fab_idx(&f, i, j, k);
-                        double fc = ref_pf_map_backward(Ab[c], eta);
+                        double fc = cac ? Ab[c] : ref_pf_map_backward(Ab[c], eta);
                         double g[3], sum = 0.0;
                         for (int d = 0; d < dim; ++d) {
                             int m3[3] = {c3[0], c3[1], c3[2]}, p3[3] = {c3[0], c3[1], c3[2]};
                             m3[d] -= 1; p3[d] += 1;
-                            double fm = ref_pf_map_backward(Ab[fab_idx(&f, m3[0], m3[1], m3[2])], eta);
-                            double fp = ref_pf_map_backward(Ab[fab_idx(&f, p3[0], p3[1], p3[2])], eta);
+                            double am = Ab[fab_idx(&f, m3[0], m3[1], m3[2])];
+                            double ap = Ab[fab_idx(&f, p3[0], p3[1], p3[2])];
+                            double fm = cac ? am : ref_pf_map_backward(am, eta);
+                            double fp = cac ? ap : ref_pf_map_backward(ap, eta);
                             double gm = (fc - fm) / G->dx[d];
                             double gp = (fp - fc) / G->dx[d];
                             g[d] = 0.5 * gm + 0.5 * gp;
@@ -1416,6 +1457,53 @@ int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const c
         }
         e = ref_level_fill_pf(dim, G, nbox, bn, 1, dim, Nrm);
         if (e && !err) err = e;
+        if (cac) {
+            /* mech and W per valid cell, their level sums, then alpha += dt_m (mech + W L) */
+            double sm = 0.0, sw = 0.0;
+            for (int b = 0; b < nbox; ++b) {
+                fab_t f = make_fab(dim, &ba[b], 1);
+                const double* Ab = A + ba[b].offset;
+                const double* Nb = Nrm + bn[b].offset;
+                int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+                for (int k = lo2; k <= hi2; ++k)
+                    for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                        for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                            int c3[3] = {i, j, k};
+                            long long c = fab_idx(&f, i, j, k);
+                            double fm[3], fp[3], nc[3], nm[3], np[3];
+                            for (int d = 0; d < dim; ++d) {
+                                int m3[3] = {c3[0], c3[1], c3[2]}, p3[3] = {c3[0], c3[1], c3[2]};
+                                m3[d] -= 1; p3[d] += 1;
+                                long long m = fab_idx(&f, m3[0], m3[1], m3[2]);
+                                long long p = fab_idx(&f, p3[0], p3[1], p3[2]);
+                                fm[d] = Ab[m];
+                                fp[d] = Ab[p];
+                                nc[d] = Nb[c + d * f.sn];
+                                nm[d] = Nb[m + d * f.sn];
+                                np[d] = Nb[p + d * f.sn];
+                            }
+                            double mk = ref_pf_cac_mech(dim, G->dx, Ab[c], fm, fp, nc, nm, np, mob, lambda, eta);
+                            double wk = Ab[c] * (1.0 - Ab[c]);
+                            Mk[ba[b].offset + c] = mk;
+                            Wk[ba[b].offset + c] = wk;
+                            sm += mk;
+                            sw += wk;
+                        }
+            }
+            double L = (sw == 0.0) ? 0.0 : -sm / sw;
+            for (int b = 0; b < nbox; ++b) {
+                fab_t f = make_fab(dim, &ba[b], 1);
+                double* Ab = A + ba[b].offset;
+                int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+                for (int k = lo2; k <= hi2; ++k)
+                    for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                        for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                            long long c = fab_idx(&f, i, j, k);
+                            double divj = Mk[ba[b].offset + c] + Wk[ba[b].offset + c] * L;
+                            Ab[c] = 1.0 * Ab[c] + dt_m * divj;
+                        }
+            }
+        } else {
         /* J on every face of every box, Jhat += dt_m J */
         for (int b = 0; b < nbox; ++b) {
             fab_t f = make_fab(dim, &ba[b], 1);
@@ -1428,7 +1516,7 @@ int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const c
                 int fe[3] = {nv[0], nv[1], nv[2]};
                 fe[d] += 1;
                 double* Jd = J + foff[b * dim + d];
-                double* Hd = jhat + foff[b * dim + d];
+                double* Hd = jhat ? jhat + foff[b * dim + d] : NULL;
 #pragma omp parallel for collapse(2) schedule(static)
                 for (int kk = 0; kk < fe[2]; ++kk)
                     for (int jj = 0; jj < fe[1]; ++jj)
@@ -1443,7 +1531,7 @@ int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const c
                                                          mob_face, eta, G->dx[d]);
                             long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
                             Jd[q] = Jf;
-                            Hd[q] += dt_m * Jf;
+                            if (Hd) Hd[q] += dt_m * Jf;
                         }
             }
         }
@@ -1477,13 +1565,208 @@ int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const c
                         Ab[c] = 1.0 * Ab[c] + dt_m * div;
                     }
         }
+        }   /* flux-based mechanism */
         t += dt_m;
         ++n;
     }
     double sc = 1.0 / dt;
-    for (long long q = 0; q < tf; ++q) jhat[q] *= sc;
+    if (jhat)
+        for (long long q = 0; q < tf; ++q) jhat[q] *= sc;
+    if (divjhat) {
+        /* LinComb(Div_Jhat, 1/dt, PF_new, -1/dt, PF) (:703) */
+        const double a = 1.0 / dt, bq = -1.0 / dt;
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &ba[b], 1);
+            int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                    for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                        long long c = ba[b].offset + fab_idx(&f, i, j, k);
+                        divjhat[c] = a * A[c] + bq * alpha[c];
+                    }
+        }
+    }
     if (nsub) *nsub = n;
-    free(A); free(Nrm); free(J); free(foff); free(ba); free(bn);
+    free(A); free(Nrm); free(J); free(Mk); free(Wk); free(foff); free(ba); free(bn);
+    return err;
+}
+
+/* The implicit Grad_Q (PhaseField_GradQPF with Div_J, :2133-2310): b = W(phi_f) + 1e-3 on every
+ * face, phi_f the face mean of the alpha re-sharpened through the mappings at eta and
+ * eta_factor*eta; Q solves div(b grad Q) = Div_Jhat (MLMG's ABecLaplacian with a = 0, b = -1;
+ * periodic sides periodic, the others homogeneous Neumann, Q = 0 initially); Grad_Q = grad Q on the
+ * faces (getGradSolution), zero on Neumann domain faces.
+ * The reference's solver is AMReX 26.01's geometric multigrid (MLMG, not vendored in the
+ * reference) stopped at a relative residual of 1e-10 (LinearSystem_Parameter.H:31). Any converged
+ * solve of the same discrete system gives the same Grad_Q to within that tolerance; this restates
+ * the system and solves it by conjugate gradients on the zero-mean subspace down to the rounding
+ * floor (relative residual 1e-14, or no further decrease), which the GPU solve also reaches.
+ * alpha: 1 ghost, filled; divjhat: alpha layout; gradq: face buffer. *iters: CG iterations. */
+int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                                int nbox, const cmp_box* boxes, const double* alpha,
+                                const double* divjhat, double eta, double* gradq, int* iters)
+{
+    const int dim = P->dim;
+    long long ta, tf;
+    cmp_box* ba = pf_boxes(dim, nbox, boxes, 1, 1, &ta);
+    long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
+    cmp_face_layout(dim, nbox, boxes, 1, foff, &tf);
+    double* Bf = (double*)malloc(sizeof(double) * tf);
+    /* b on the faces */
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &ba[b], 1);
+        const double* Ab = alpha + ba[b].offset;
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = ba[b].hi[d] - ba[b].lo[d] + 1;
+        int lo2 = (dim == 3) ? ba[b].lo[2] : 0;
+        for (int d = 0; d < dim; ++d) {
+            int fe[3] = {nv[0], nv[1], nv[2]};
+            fe[d] += 1;
+            for (int kk = 0; kk < fe[2]; ++kk)
+                for (int jj = 0; jj < fe[1]; ++jj)
+                    for (int ii = 0; ii < fe[0]; ++ii) {
+                        int c3[3] = {ba[b].lo[0] + ii, ba[b].lo[1] + jj, lo2 + kk};
+                        int m3[3] = {c3[0], c3[1], c3[2]};
+                        m3[d] -= 1;
+                        double c = ref_pf_map_forward(ref_pf_map_backward(Ab[fab_idx(&f, c3[0], c3[1], c3[2])], eta),
+                                                      PF->eta_factor * eta, 0);
+                        double m = ref_pf_map_forward(ref_pf_map_backward(Ab[fab_idx(&f, m3[0], m3[1], m3[2])], eta),
+                                                      PF->eta_factor * eta, 0);
+                        double pf = 0.5 * m + 0.5 * c;
+                        long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                        Bf[foff[b * dim + d] + q] = pf * (1.0 - pf) + 1.0e-3;
+                    }
+        }
+    }
+    double* X = (double*)calloc((size_t)ta, sizeof(double));
+    double* R = (double*)calloc((size_t)ta, sizeof(double));
+    double* Pv = (double*)calloc((size_t)ta, sizeof(double));
+    double* AP = (double*)calloc((size_t)ta, sizeof(double));
+    long long ncell = 0;
+    double mean = 0.0;
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &ba[b], 1);
+        int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+        for (int k = lo2; k <= hi2; ++k)
+            for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                    mean += divjhat[ba[b].offset + fab_idx(&f, i, j, k)];
+                    ++ncell;
+                }
+    }
+    mean /= (double)ncell;
+    /* the operator is -div(b grad .): symmetric positive semi-definite; r = -(rhs - mean) */
+    double rr = 0.0, b2 = 0.0;
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &ba[b], 1);
+        int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+        for (int k = lo2; k <= hi2; ++k)
+            for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                    long long c = ba[b].offset + fab_idx(&f, i, j, k);
+                    R[c] = -(divjhat[c] - mean);
+                    Pv[c] = R[c];
+                    rr += R[c] * R[c];
+                }
+    }
+    b2 = rr;
+    int it = 0, err = 0;
+    const int maxit = 100000;
+    double best = rr;
+    int stall = 0;
+    while (rr > 1.0e-28 * b2 && it < maxit && b2 > 0.0) {
+        int e = ref_level_fill_pf(dim, G, nbox, ba, 1, 1, Pv);
+        if (e && !err) err = e;
+        double pAp = 0.0;
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &ba[b], 1);
+            int nv[3] = {1, 1, 1};
+            for (int d = 0; d < dim; ++d) nv[d] = ba[b].hi[d] - ba[b].lo[d] + 1;
+            int lo2 = (dim == 3) ? ba[b].lo[2] : 0;
+            for (int kk = 0; kk < nv[2]; ++kk)
+                for (int jj = 0; jj < nv[1]; ++jj)
+                    for (int ii = 0; ii < nv[0]; ++ii) {
+                        int c3[3] = {ba[b].lo[0] + ii, ba[b].lo[1] + jj, lo2 + kk};
+                        long long c = ba[b].offset + fab_idx(&f, c3[0], c3[1], c3[2]);
+                        int idx[3] = {ii, jj, kk};
+                        double acc = 0.0;
+                        for (int d = 0; d < dim; ++d) {
+                            int fe[3] = {nv[0], nv[1], nv[2]};
+                            fe[d] += 1;
+                            long long qm = idx[0] + (long long)fe[0] * (idx[1] + (long long)fe[1] * idx[2]);
+                            long long qp = qm + (d == 0 ? 1 : (d == 1 ? fe[0] : (long long)fe[0] * fe[1]));
+                            int m3[3] = {c3[0], c3[1], c3[2]}, p3[3] = {c3[0], c3[1], c3[2]};
+                            m3[d] -= 1; p3[d] += 1;
+                            double pm = Pv[ba[b].offset + fab_idx(&f, m3[0], m3[1], m3[2])];
+                            double pp = Pv[ba[b].offset + fab_idx(&f, p3[0], p3[1], p3[2])];
+                            int lo_dom = (c3[d] == G->domain_lo[d]) && !G->is_periodic[d];
+                            int hi_dom = (c3[d] == G->domain_hi[d]) && !G->is_periodic[d];
+                            double fp_ = hi_dom ? 0.0 : Bf[foff[b * dim + d] + qp] * (pp - Pv[c]) / G->dx[d];
+                            double fm_ = lo_dom ? 0.0 : Bf[foff[b * dim + d] + qm] * (Pv[c] - pm) / G->dx[d];
+                            double t = (fp_ - fm_) / G->dx[d];
+                            acc = (d == 0) ? t : acc + t;
+                        }
+                        AP[c] = -acc;
+                        pAp += Pv[c] * AP[c];
+                    }
+        }
+        if (!(pAp > 0.0)) break;
+        double al = rr / pAp, rr2 = 0.0;
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &ba[b], 1);
+            int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                    for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                        long long c = ba[b].offset + fab_idx(&f, i, j, k);
+                        X[c] += al * Pv[c];
+                        R[c] -= al * AP[c];
+                        rr2 += R[c] * R[c];
+                    }
+        }
+        double be = rr2 / rr;
+        rr = rr2;
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &ba[b], 1);
+            int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
+                    for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
+                        long long c = ba[b].offset + fab_idx(&f, i, j, k);
+                        Pv[c] = R[c] + be * Pv[c];
+                    }
+        }
+        ++it;
+        if (rr < best) { best = rr; stall = 0; }
+        else if (++stall > 50) break;   /* rounding floor reached */
+    }
+    /* Grad_Q on every face */
+    int e = ref_level_fill_pf(dim, G, nbox, ba, 1, 1, X);
+    if (e && !err) err = e;
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &ba[b], 1);
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = ba[b].hi[d] - ba[b].lo[d] + 1;
+        int lo2 = (dim == 3) ? ba[b].lo[2] : 0;
+        for (int d = 0; d < dim; ++d) {
+            int fe[3] = {nv[0], nv[1], nv[2]};
+            fe[d] += 1;
+            for (int kk = 0; kk < fe[2]; ++kk)
+                for (int jj = 0; jj < fe[1]; ++jj)
+                    for (int ii = 0; ii < fe[0]; ++ii) {
+                        int c3[3] = {ba[b].lo[0] + ii, ba[b].lo[1] + jj, lo2 + kk};
+                        int m3[3] = {c3[0], c3[1], c3[2]};
+                        m3[d] -= 1;
+                        long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                        int on_dom = !G->is_periodic[d] && (c3[d] == G->domain_lo[d] || c3[d] == G->domain_hi[d] + 1);
+                        double xc = X[ba[b].offset + fab_idx(&f, c3[0], c3[1], c3[2])];
+                        double xm = X[ba[b].offset + fab_idx(&f, m3[0], m3[1], m3[2])];
+                        gradq[foff[b * dim + d] + q] = on_dom ? 0.0 : (xc - xm) / G->dx[d];
+                    }
+        }
+    }
+    if (iters) *iters = it;
+    free(Bf); free(X); free(R); free(Pv); free(AP); free(foff); free(ba);
     return err;
 }
 
@@ -1572,10 +1855,19 @@ int ref_level_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, const 
     long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
     cmp_face_layout(dim, nbox, boxes, 1, foff, &tf);
     double* H = (double*)malloc(sizeof(double) * tf);
-    int nsub = 0;
-    int e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, H, &nsub);
-    if (e && !err) err = e;
-    ref_level_pf_gradq(P, PF, G, nbox, boxes, A, H, eta, gradq);
+    int nsub = 0, e;
+    if (PF->explicit_rc) {
+        e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, H, NULL, &nsub);
+        if (e && !err) err = e;
+        ref_level_pf_gradq(P, PF, G, nbox, boxes, A, H, eta, gradq);
+    } else {
+        double* DJ = (double*)calloc((size_t)ta, sizeof(double));
+        e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, NULL, DJ, &nsub);
+        if (e && !err) err = e;
+        e = ref_level_pf_gradq_implicit(P, PF, G, nbox, boxes, A, DJ, eta, gradq, NULL);
+        if (e && !err) err = e;
+        free(DJ);
+    }
     double gm = 0.0;
     for (long long q = 0; q < tf; ++q) {
         double a = fabs(gradq[q]);

@@ -104,9 +104,16 @@ int ref_level_pf_rk3_step(const cmp_params* P, const cmp_pf_params* PF, const cm
                           double pf_dt);
 double ref_level_max_velocity(const cmp_params* P, int nbox, const cmp_box* boxes, int ng,
                               const double* U);
+double ref_pf_cac_mech(int dim, const double* dx, double f, const double* fm, const double* fp,
+                       const double* nc, const double* nm, const double* np, double mob,
+                       double lambda, double eta);
 int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                            int nbox, const cmp_box* boxes, const double* alpha, double dt,
-                           double mob, double eta, double lambda, double* jhat, int* nsub);
+                           double mob, double eta, double lambda, double* jhat, double* divjhat,
+                           int* nsub);
+int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                                int nbox, const cmp_box* boxes, const double* alpha,
+                                const double* divjhat, double eta, double* gradq, int* iters);
 int ref_level_pf_gradq(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                        int nbox, const cmp_box* boxes, const double* alpha, const double* jhat,
                        double eta, double* gradq);

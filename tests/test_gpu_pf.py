@@ -1,6 +1,6 @@
 """GPU parity of the Phase-Field path (through the C ABI) against the oracle on the same inputs:
-the Phase-Field right-hand side, Grad_Q from the ACDI mechanism, one Phase-Field RK3 sub-step,
-the whole Phase-Field step and flow steps followed by it.
+the Phase-Field right-hand side, Grad_Q from the ACDI and CAC-Adv mechanisms with the explicit or
+implicit RC, one Phase-Field RK3 sub-step, the whole Phase-Field step and flow steps followed by it.
 
 Bars (FP64): per-variable max relative difference <= 1e-12 for one evaluation or one step
 (Grad_Q: max |gpu - cpu| <= 1e-12 max |cpu| over all faces), <= 1e-9 in L1 after 10 steps; the
@@ -198,3 +198,82 @@ def test_pf_water_air_3d_step(gpu):
     assert abs(hist.cpu().numpy()[0] - hist_ref[0]) <= 1e-14 * hist_ref[0]
     rel = maxrel_per_var(got, ref)
     assert rel.max() <= TOL_STEP, rel
+
+
+# (mechanism, explicit_rc, dim, periodic, stiffened gas)
+PF_IMPLICIT_CASES = [
+    ("CAC-Adv", 0, 2, True, False), ("CAC-Adv", 0, 2, False, True), ("ACDI", 0, 2, True, False),
+    ("ACDI", 0, 2, False, False), ("CAC-Adv", 0, 3, True, False), ("ACDI", 0, 3, False, True),
+    ("CAC-Adv", 1, 2, True, False),
+]
+
+
+@pytest.mark.parametrize("mech,rc,dim,periodic,sg", PF_IMPLICIT_CASES)
+def test_pf_gradq_cac_and_implicit_match_oracle(gpu, mech, rc, dim, periodic, sg):
+    """Grad_Q from the CAC-Adv mechanism and/or the implicit RC. The implicit RC solves a
+    variable-coefficient Poisson problem by CG to the rounding floor on both sides, with sums taken
+    in different orders: max |gpu - cpu| <= 1e-9 max |cpu| (an unconverged or wrong solve is off by
+    O(1)); the mechanism's sub-step count and coefficients must agree exactly. CAC with the explicit
+    RC accumulates no Jhat, so its Grad_Q is zero."""
+    import torch
+    n = 32 if dim == 2 else 16
+    eos = dict(gamma=(4.4, 1.4), pinf=(6.0e8, 0.0)) if sg else None
+    c = _interface_case(n, dim=dim, width=2.0, periodic=periodic, max_grid_size=n // 2, eos=eos)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(mechanism=mech, explicit_rc=rc, eta_multiplier=0.5)
+    lay = Layout(dim, c["boxes"], 3, P.nstate)
+    flay = FaceLayout(dim, c["boxes"])
+    buf = lay.pack(c["U"])
+    dt = 0.3 / n
+    ref, ref_gm, ref_info = oracle.level_pf_gradq_step(P, pf, G, lay, buf, dt, flay.total)
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    U = _dev(torch, buf, gpu)
+    gq = torch.full((flay.total,), float("nan"), dtype=torch.float64, device=gpu)
+    gm, info = lev.PhaseField_GradQPF(U, dt, gq)
+    assert lev.check_errors() == 0
+    got = gq.cpu().numpy()
+    lev.close()
+    assert info[3] == ref_info[3] and info[3] >= 1
+    for q in range(3):
+        assert abs(info[q] - ref_info[q]) <= 1e-15 * abs(ref_info[q])
+    assert np.all(np.isfinite(got))
+    if mech == "CAC-Adv" and rc == 1:
+        assert np.all(ref == 0.0) and np.all(got == 0.0) and gm == 0.0
+        return
+    scale = np.max(np.abs(ref))
+    assert scale > 0.0
+    err = np.max(np.abs(got - ref))
+    assert err <= 1e-9 * scale, err / scale
+    assert abs(gm - ref_gm) <= 1e-9 * ref_gm
+
+
+@pytest.mark.parametrize("mech,dim", [("CAC-Adv", 2), ("ACDI", 2), ("CAC-Adv", 3)])
+def test_pf_evolve_implicit_matches_oracle(gpu, mech, dim):
+    """The whole Phase-Field step with the implicit RC (CAC-Adv or ACDI): per-variable max relative
+    difference <= 1e-10 (the CG solution carries rounding of order 1e-13), equal sub-step counts,
+    totals conserved."""
+    import torch
+    n = 32 if dim == 2 else 16
+    c = _interface_case(n, dim=dim, width=2.5, periodic=True, max_grid_size=n // 2)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(mechanism=mech, explicit_rc=0, eta_multiplier=0.5, cfl=0.3)
+    lay = Layout(dim, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    dt = 2.0 / n
+    Un = _dev(torch, buf, gpu)
+    ref_info = oracle.level_pf_evolve(P, pf, G, lay, buf, dt)
+    ref = lay.unpack(buf, c["shape"])
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dtd = torch.full((1,), dt, dtype=torch.float64, device=gpu)
+    info = lev.PhaseField_Evolve(Un, U1, U2, dtd)
+    assert lev.check_errors() == 0
+    got = lay.unpack(Un.cpu().numpy(), c["shape"])
+    lev.close()
+    assert info[3] == ref_info[3] and info[5] == ref_info[5] and info[5] >= 1
+    rel = maxrel_per_var(got, ref)
+    assert rel.max() <= 1e-10, rel
+    for q in range(P.nstate):
+        assert abs(got[q].sum() - c["U"][q].sum()) <= 1e-12 * np.abs(c["U"][q]).sum()

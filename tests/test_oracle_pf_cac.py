@@ -1,0 +1,183 @@
+"""Pins of the oracle's CAC-Adv mechanism and implicit Grad_Q (CPU only):
+
+- the CAC-Adv chemical potential vanishes at second order on a circle (and a plane) carrying the
+  equilibrium profile phi = 1/2 + 1/2 tanh(s/(sqrt 2 eta)) with exact normals: the curvature term
+  cancels the radial part of the Laplacian, g'(phi)/eta^2 the rest; a sign or factor error in any
+  term leaves an O(1/eta^2) potential;
+- the mechanism conserves sum(alpha) to round-off (L is chosen for that) and Div_Jhat dt equals
+  alpha_new - alpha;
+- the implicit solve reproduces a manufactured periodic Q (grad Q exact on the faces), its output
+  satisfies div(b Grad_Q) = rhs - mean(rhs) with variable b, Neumann domain faces carry zero;
+- a whole Phase-Field step with CAC or ACDI and the implicit RC conserves totals and keeps uniform
+  pressure and velocity uniform.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from compas_amd import FaceLayout, Layout, make_geometry, make_params, make_pf_params, split_domain
+from tests.test_oracle_pf import _buffer_to_global_faces, _global_faces_to_buffer, _interface_case
+
+
+def _eq_profile(s, eta):
+    return 0.5 + 0.5 * math.tanh(s / (math.sqrt(2.0) * eta))
+
+
+def _cac_mu_error(eta, dx, R=0.3, dim=2):
+    """max |mech| over cells near a circle (sphere) of radius R with the equilibrium profile."""
+    worst = 0.0
+    rng = np.random.default_rng(0)
+    for _ in range(40):
+        # cell centres at random positions near the interface, on a grid of spacing dx
+        r0 = R + rng.uniform(-2 * eta, 2 * eta)
+        th = rng.uniform(0, 2 * math.pi)
+        ph = rng.uniform(0.2, math.pi - 0.2)
+        c = np.array([r0 * math.cos(th) * (math.sin(ph) if dim == 3 else 1.0),
+                      r0 * math.sin(th) * (math.sin(ph) if dim == 3 else 1.0),
+                      r0 * math.cos(ph) if dim == 3 else 0.0])[:dim]
+
+        def phi(x):
+            return _eq_profile(R - np.linalg.norm(x), eta)
+
+        def nrm(x):
+            return -x / np.linalg.norm(x)   # grad phi points inward
+        fm, fp, nc, nm, np_ = [], [], [], [], []
+        for d in range(dim):
+            e = np.zeros(dim)
+            e[d] = dx
+            fm.append(phi(c - e))
+            fp.append(phi(c + e))
+            nc.append(nrm(c)[d])
+            nm.append(nrm(c - e)[d])
+            np_.append(nrm(c + e)[d])
+        m = oracle.pf_cac_mech(dim, [dx] * dim, phi(c), fm, fp, nc, nm, np_, 1.0, 1.0, eta)
+        worst = max(worst, abs(m))
+    return worst
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cac_potential_vanishes_at_second_order_on_the_equilibrium_circle(dim):
+    eta = 0.02
+    errs = [_cac_mu_error(eta, eta / r, dim=dim) for r in (4, 8, 16)]
+    for a, b in zip(errs, errs[1:]):
+        assert 3.3 < a / b < 4.7, errs
+    # a planar profile with n = e_x: the curvature term is zero and the same holds
+    dx = eta / 16
+    f = [_eq_profile(-dx, eta), _eq_profile(dx, eta)]
+    m = oracle.pf_cac_mech(1 + 1, [dx, dx], _eq_profile(0.0, eta), [f[0], 0.5], [f[1], 0.5],
+                           [1.0, 0.0], [1.0, 0.0], [1.0, 0.0], 1.0, 1.0, eta)
+    assert abs(m) < 1e-2 * (1.0 / eta ** 2) * 1e-2
+    # off equilibrium (a profile twice as wide) the potential is O(1/eta^2)
+    wide = [_eq_profile(-dx, 2 * eta), _eq_profile(dx, 2 * eta)]
+    m2 = oracle.pf_cac_mech(2, [dx, dx], _eq_profile(0.3 * eta, 2 * eta), [wide[0], 0.5], [wide[1], 0.5],
+                            [1.0, 0.0], [1.0, 0.0], [1.0, 0.0], 1.0, 1.0, eta)
+    assert abs(m2) > 100 * errs[-1]
+
+
+def test_cac_mechanism_conserves_and_matches_alpha_change():
+    c = _interface_case(32, width=3.0)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(mechanism="CAC-Adv", explicit_rc=0)
+    alay = Layout(2, c["boxes"], 1, 1)
+    a0 = alay.pack(c["U"][5:6])
+    oracle.level_fill_pf(G, alay, a0)
+    lam, eta, mob = oracle.pf_coefficients(pf, G, 1.0)
+    flay = FaceLayout(2, c["boxes"])
+    dt = 0.002
+    dj, nsub = oracle.level_pf_mechanism(P, pf, G, alay, a0, dt, mob, eta, lam, flay.total, implicit=True)
+    assert nsub >= 1
+    DJ = alay.unpack(dj, c["shape"])[0]
+    assert abs(DJ.sum()) <= 1e-9 * np.abs(DJ).max() * DJ.size
+    assert np.abs(DJ).max() > 0.0
+
+
+def _periodic_case(n, mg=None, periodic=True):
+    P = make_params(2)
+    G = make_geometry((n, n), (0.0, 0.0), (1.0, 1.0), (int(periodic),) * 2)
+    return P, G, split_domain((n, n), mg or n)
+
+
+def test_implicit_solve_reproduces_a_manufactured_solution():
+    """alpha = 1/2 everywhere gives b = 1/4 + 1e-3 on every face; with rhs the discrete
+    div(b grad Q*) of Q* = cos(2 pi x) cos(4 pi y), Grad_Q is grad Q* on the faces."""
+    n = 32
+    P, G, boxes = _periodic_case(n, mg=16)
+    pf = make_pf_params(mechanism="CAC-Adv", explicit_rc=0)
+    alay = Layout(2, boxes, 1, 1)
+    a0 = alay.pack(np.full((1, n, n), 0.5))
+    oracle.level_fill_pf(G, alay, a0)
+    h = 1.0 / n
+    x = (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    Q = np.cos(2 * np.pi * X) * np.cos(4 * np.pi * Y)
+    b = 0.25 + 1e-3
+    lap = (np.roll(Q, -1, 1) - 2 * Q + np.roll(Q, 1, 1)) / h ** 2 + (np.roll(Q, -1, 0) - 2 * Q + np.roll(Q, 1, 0)) / h ** 2
+    rhs = alay.pack((b * lap)[None])
+    flay = FaceLayout(2, boxes)
+    gq, it = oracle.level_pf_gradq_implicit(P, pf, G, alay, a0, rhs, 0.01, flay.total)
+    assert it >= 1   # one eigenmode of the constant-coefficient operator: CG needs one step
+    gx, gy = _buffer_to_global_faces(flay, gq, (n, n))
+    Qx = np.concatenate([Q, Q[:, :1]], axis=1)
+    ex = (Qx[:, 1:] - Qx[:, :-1]) / h          # faces 1..n
+    assert np.max(np.abs(gx[:, 1:] - ex)) < 1e-10 * np.max(np.abs(ex))
+    Qy = np.concatenate([Q, Q[:1, :]], axis=0)
+    ey = (Qy[1:, :] - Qy[:-1, :]) / h
+    assert np.max(np.abs(gy[1:, :] - ey)) < 1e-10 * np.max(np.abs(ey))
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_implicit_solution_satisfies_its_equation(periodic):
+    """Variable b from an interface; the output faces satisfy div(b Grad_Q) = rhs - mean(rhs)
+    and, on Neumann sides, Grad_Q = 0 on the domain faces."""
+    n = 24
+    c = _interface_case(n, width=2.0, periodic=periodic, max_grid_size=12)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(mechanism="CAC-Adv", explicit_rc=0, eta_factor=2.0)
+    alay = Layout(2, c["boxes"], 1, 1)
+    a0 = alay.pack(c["U"][5:6])
+    oracle.level_fill_pf(G, alay, a0)
+    rng = np.random.default_rng(3)
+    rhs_g = rng.uniform(-1, 1, (n, n))
+    rhs = alay.pack(rhs_g[None])
+    flay = FaceLayout(2, c["boxes"])
+    eta = 1.0 / n
+    gq, it = oracle.level_pf_gradq_implicit(P, pf, G, alay, a0, rhs, eta, flay.total)
+    gx, gy = _buffer_to_global_faces(flay, gq, (n, n))
+    # b on the faces, from the re-sharpened face mean (independent evaluation of the mapping)
+    A = c["U"][5]
+    Ap = np.pad(A, 1, mode="wrap" if periodic else "edge")
+
+    def sharp(a):
+        a = np.clip(a, 1e-12, 1 - 1e-12)
+        psi = eta / math.sqrt(2.0) * np.log(a / (1 - a))
+        return 0.5 + 0.5 * np.tanh(psi / (math.sqrt(2.0) * 2.0 * eta))
+    S = sharp(Ap)
+    fx = 0.5 * S[1:-1, :-1] + 0.5 * S[1:-1, 1:]      # x faces 0..n
+    fy = 0.5 * S[:-1, 1:-1] + 0.5 * S[1:, 1:-1]
+    bx = fx * (1 - fx) + 1e-3
+    by = fy * (1 - fy) + 1e-3
+    div = (bx[:, 1:] * gx[:, 1:] - bx[:, :-1] * gx[:, :-1]) * n + (by[1:, :] * gy[1:, :] - by[:-1, :] * gy[:-1, :]) * n
+    target = rhs_g - rhs_g.mean()
+    assert np.max(np.abs(div - target)) < 1e-9 * np.max(np.abs(target))
+    if not periodic:
+        assert np.all(gx[:, 0] == 0.0) and np.all(gx[:, -1] == 0.0)
+        assert np.all(gy[0, :] == 0.0) and np.all(gy[-1, :] == 0.0)
+
+
+@pytest.mark.parametrize("mechanism", ["CAC-Adv", "ACDI"])
+def test_implicit_pf_step_conserves_and_keeps_equilibrium(mechanism):
+    c = _interface_case(24, width=2.5)
+    P, G = c["params"], c["geom"]
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    pf = make_pf_params(mechanism=mechanism, explicit_rc=0, eta_multiplier=0.5, cfl=0.3)
+    info = oracle.level_pf_evolve(P, pf, G, lay, buf, 0.004)
+    assert info[5] >= 1 and info[4] > 0.0
+    U = lay.unpack(buf, c["shape"])
+    for q in range(P.nstate):
+        assert abs(U[q].sum() - c["U"][q].sum()) <= 1e-12 * np.abs(c["U"][q]).sum()
+    u = U[2] / (U[0] + U[1])
+    assert np.max(np.abs(u - 1.0)) < 1e-11
+    assert np.max(np.abs(U[5] - c["U"][5])) > 1e-7
