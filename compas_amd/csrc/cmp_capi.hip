@@ -817,6 +817,8 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     if ((pf->mechanism != CMP_PF_ACDI && pf->mechanism != CMP_PF_CAC) || pf->id_flux != 0)
         return CMP_ERR_UNSUPPORTED;
     if (pf->explicit_rc != 0 && pf->explicit_rc != 1) return CMP_ERR_ARG;
+    // a mechanism that is not flux based has no Jhat (PhaseField_Parameter.H:138-140 aborts)
+    if (pf->mechanism == CMP_PF_CAC && pf->explicit_rc == 1) return CMP_ERR_UNSUPPORTED;
     if (pf->flux_phase2flux != 0 && pf->flux_phase2flux != 1) return CMP_ERR_ARG;
     if (!(pf->cfl > 0.0) || !(pf->eta_factor > 0.0) || pf->max_step < 1) return CMP_ERR_ARG;
     if (!(pf->eta > 0.0) && !(pf->eta_multiplier > 0.0)) return CMP_ERR_ARG;

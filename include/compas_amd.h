@@ -81,13 +81,15 @@ typedef struct cmp_box {
 } cmp_box;
 
 /* Phase-field regularization parameters: the subset of PhaseField_Parameter
- * (include/Physics_PhaseField/PhaseField_Parameter.H:29-144) that the ACDI path reads. */
+ * (include/Physics_PhaseField/PhaseField_Parameter.H:29-144) that the ACDI and CAC-Adv paths read. */
 #define CMP_PF_CAC  1      /* PhaseField.Mechanism = "CAC-Adv" (ID_Mechanism 1, not flux based)      */
 #define CMP_PF_ACDI 3      /* PhaseField.Mechanism = "ACDI" (ID_Mechanism 3, flux based)             */
 typedef struct cmp_pf_params {
     int    mechanism;        /* ID_Mechanism: CMP_PF_CAC or CMP_PF_ACDI                              */
     int    explicit_rc;      /* ID_ExplicitRC: 1 = Grad_Q from the time-averaged flux Jhat (ACDI
-                              * only); 0 = Grad_Q from the elliptic solve div(b grad Q) = div Jhat  */
+                              * only; CAC with 1 is CMP_ERR_UNSUPPORTED, as the reference aborts,
+                              * PhaseField_Parameter.H:138-140); 0 = Grad_Q from the elliptic solve
+                              * div(b grad Q) = div Jhat                                             */
     int    id_flux;          /* ID_Flux: 0 = UD flux                                                  */
     int    flux_phase2flux;  /* ID_FluxPhase2Flux: 1 = mixture formulation, 0 = phasic              */
     double cfl;              /* PhaseField.cfl of the Phase-Field sub-steps                           */
@@ -207,7 +209,8 @@ int cmp_level_info(const cmp_level* level, int* ntiles, int* tile_x, int* tile_y
  * Face buffers below follow cmp_face_layout with one component (NPHASE = 1 for two phases). */
 
 /* Enables the Phase-Field step on a level and allocates its device work buffers (two copies of
- * alpha1 and the cell normals with one ghost cell; J, Jhat and Grad_Q on the faces). */
+ * alpha1 and the cell normals with one ghost cell; J, Jhat and Grad_Q on the faces; for CAC the
+ * per-cell mech and W; for the implicit RC Div_Jhat, the CG vectors and b on the faces). */
 int cmp_level_pf_setup(cmp_level* level, const cmp_pf_params* pf);
 
 /* Phase-Field right-hand side of every box, PhaseField_compute_dUdt_FV
@@ -220,9 +223,12 @@ int cmp_level_pf_rhs(cmp_level* level, const double* U, const double* gradq, dou
                      void* stream);
 
 /* Grad_Q for a flow step of size dt from the valid cells of U: MaxVelocity (:3541-3594), the
- * mobility 0.5 eta max|u| (PhaseField_Evolve :425-465), the ACDI sub-steps of
- * PhaseFieldMechanism_Evolve (:597-709, PhaseField_ACDI :1495-1644, PhaseField_Normal :1958-2032)
- * accumulating Jhat, then PhaseField_GradQPF with the explicit RC (:2315-2403).
+ * mobility 0.5 eta max|u| (PhaseField_Evolve :425-465), the mechanism sub-steps of
+ * PhaseFieldMechanism_Evolve (:597-709; PhaseField_ACDI :1495-1644 or the CAC-Adv
+ * PhaseField_Mechanism :1007-1152, with PhaseField_Normal :1958-2032) accumulating Jhat or
+ * Div_Jhat, then PhaseField_GradQPF with the explicit RC (:2315-2403) or the implicit one
+ * (:2133-2310: div(b grad Q) = Div_Jhat solved by conjugate gradients to the rounding floor where
+ * the reference calls AMReX's MLMG).
  * gradq: device face buffer (written); *gradq_max (host) = max |Grad_Q| over all faces;
  * info (host, 4 doubles, or NULL) = max|u|, mobility, eta, number of mechanism sub-steps.
  * Synchronises the stream. */

@@ -1835,6 +1835,8 @@ int ref_level_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, const 
                             double* gradq, double* gradq_max, double* info)
 {
     const int dim = P->dim;
+    /* a mechanism that is not flux based has no Jhat: PhaseField_Parameter.H:138-140 aborts */
+    if (PF->mechanism == CMP_PF_CAC && PF->explicit_rc) return CMP_ERR_UNSUPPORTED;
     double mv = ref_level_max_velocity(P, nbox, boxes, ng, U);
     double lambda, eta, mob;
     ref_pf_coefficients(PF, G, mv, &lambda, &eta, &mob);

@@ -204,7 +204,6 @@ def test_pf_water_air_3d_step(gpu):
 PF_IMPLICIT_CASES = [
     ("CAC-Adv", 0, 2, True, False), ("CAC-Adv", 0, 2, False, True), ("ACDI", 0, 2, True, False),
     ("ACDI", 0, 2, False, False), ("CAC-Adv", 0, 3, True, False), ("ACDI", 0, 3, False, True),
-    ("CAC-Adv", 1, 2, True, False),
 ]
 
 
@@ -213,8 +212,7 @@ def test_pf_gradq_cac_and_implicit_match_oracle(gpu, mech, rc, dim, periodic, sg
     """Grad_Q from the CAC-Adv mechanism and/or the implicit RC. The implicit RC solves a
     variable-coefficient Poisson problem by CG to the rounding floor on both sides, with sums taken
     in different orders: max |gpu - cpu| <= 1e-9 max |cpu| (an unconverged or wrong solve is off by
-    O(1)); the mechanism's sub-step count and coefficients must agree exactly. CAC with the explicit
-    RC accumulates no Jhat, so its Grad_Q is zero."""
+    O(1)); the mechanism's sub-step count and coefficients must agree exactly."""
     import torch
     n = 32 if dim == 2 else 16
     eos = dict(gamma=(4.4, 1.4), pinf=(6.0e8, 0.0)) if sg else None
@@ -238,9 +236,6 @@ def test_pf_gradq_cac_and_implicit_match_oracle(gpu, mech, rc, dim, periodic, sg
     for q in range(3):
         assert abs(info[q] - ref_info[q]) <= 1e-15 * abs(ref_info[q])
     assert np.all(np.isfinite(got))
-    if mech == "CAC-Adv" and rc == 1:
-        assert np.all(ref == 0.0) and np.all(got == 0.0) and gm == 0.0
-        return
     scale = np.max(np.abs(ref))
     assert scale > 0.0
     err = np.max(np.abs(got - ref))
@@ -277,3 +272,15 @@ def test_pf_evolve_implicit_matches_oracle(gpu, mech, dim):
     assert rel.max() <= 1e-10, rel
     for q in range(P.nstate):
         assert abs(got[q].sum() - c["U"][q].sum()) <= 1e-12 * np.abs(c["U"][q]).sum()
+
+
+def test_pf_setup_rejects_cac_with_the_explicit_rc(gpu):
+    """CAC-Adv with the explicit RC is refused (the reference aborts, PhaseField_Parameter.H:138-140)
+    and leaves the level usable for a valid setup."""
+    c = _interface_case(16, width=2.0)
+    P, G = c["params"], c["geom"]
+    lev = Level(P, G, Layout(2, c["boxes"], 3, P.nstate))
+    with pytest.raises(RuntimeError, match="cmp_level_pf_setup"):
+        lev.pf_setup(make_pf_params(mechanism="CAC-Adv", explicit_rc=1))
+    lev.pf_setup(make_pf_params(mechanism="CAC-Adv", explicit_rc=0))
+    lev.close()

@@ -181,3 +181,15 @@ def test_implicit_pf_step_conserves_and_keeps_equilibrium(mechanism):
     u = U[2] / (U[0] + U[1])
     assert np.max(np.abs(u - 1.0)) < 1e-11
     assert np.max(np.abs(U[5] - c["U"][5])) > 1e-7
+
+
+def test_cac_with_the_explicit_rc_is_rejected():
+    """CAC-Adv is not flux based, so it has no Jhat: the reference aborts on it with the explicit RC
+    (PhaseField_Parameter.H:138-140); the oracle returns CMP_ERR_UNSUPPORTED."""
+    c = _interface_case(16, width=2.0)
+    P, G = c["params"], c["geom"]
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    flay = FaceLayout(2, c["boxes"])
+    pf = make_pf_params(mechanism="CAC-Adv", explicit_rc=1)
+    with pytest.raises(RuntimeError, match="failed: 2"):
+        oracle.level_pf_gradq_step(P, pf, G, lay, lay.pack(c["U"]), 0.01, flay.total)
