@@ -80,15 +80,25 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     const KParams& kp = cx_.kp;
     const BoxDev& bx = cx_.bx;
     const int tid = cx_.tid;
-    // region extents (tile plus G cells on both sides along D)
+    // Region (tile plus G cells on both sides along D) and face index spaces have the full tile's
+    // extents, known at compile time (divisions by constants, constant LDS strides); a tile clipped
+    // by its box leaves the positions beyond its cells unused.
+    constexpr int E0 = L::TX + (D == 0 ? 2 * G : 0);
+    constexpr int E1 = L::TY + (D == 1 ? 2 * G : 0);
+    constexpr int E2 = L::TZ + (D == 2 ? 2 * G : 0);
+    constexpr int R = E0 * E1 * E2;
+    constexpr int F0 = L::TX + (D == 0 ? 1 : 0);
+    constexpr int F1 = L::TY + (D == 1 ? 1 : 0);
+    constexpr int F2 = L::TZ + (D == 2 ? 1 : 0);
+    constexpr int F = F0 * F1 * F2;
+    static_assert(R <= RMAX && F <= FMAX, "LDS regions");
+    // extents actually used by this tile
     const int e0 = cx_.n[0] + (D == 0 ? 2 * G : 0);
     const int e1 = cx_.n[1] + (D == 1 ? 2 * G : 0);
     const int e2 = cx_.n[2] + (D == 2 ? 2 * G : 0);
-    const int R = e0 * e1 * e2;
     const int f0 = cx_.n[0] + (D == 0 ? 1 : 0);
     const int f1 = cx_.n[1] + (D == 1 ? 1 : 0);
     const int f2 = cx_.n[2] + (D == 2 ? 1 : 0);
-    const int F = f0 * f1 * f2;
     // Phase-Field flux: a face with Grad_Q == 0 has the flux 0 exactly (every term carries the
     // factor Grad_Q), so its reconstruction is skipped, and a tile direction without any nonzero
     // Grad_Q skips the staging as well (its flux differences are all zero)
@@ -102,15 +112,18 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     };
     if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
         int nz = 0;
-        for (int f = tid; f < F; f += NT)
-            nz |= (gface(f % f0, (f / f0) % f1, f / (f0 * f1)) != 0.0);
+        for (int f = tid; f < F; f += NT) {
+            const int x = f % F0, y = (f / F0) % F1, z = f / (F0 * F1);
+            if (x < f0 && y < f1 && z < f2) nz |= (gface(x, y, z) != 0.0);
+        }
         if (!__syncthreads_or(nz)) return;
     }
     // 1. primitives of the region, staged in LDS
     for (int r = tid; r < R; r += NT) {
-        const int x = r % e0;
-        const int y = (r / e0) % e1;
-        const int z = r / (e0 * e1);
+        const int x = r % E0;
+        const int y = (r / E0) % E1;
+        const int z = r / (E0 * E1);
+        if (x >= e0 || y >= e1 || z >= e2) continue;
         const int i = cx_.tlo[0] + x - (D == 0 ? G : 0);
         const int j = cx_.tlo[1] + y - (D == 1 ? G : 0);
         const int k = cx_.tlo[2] + z - (D == 2 ? G : 0);
@@ -124,20 +137,21 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     }
     __syncthreads();
     // 2. faces along D
-    const int rs = (D == 0) ? 1 : ((D == 1) ? e0 : e0 * e1);
+    constexpr int rs = (D == 0) ? 1 : ((D == 1) ? E0 : E0 * E1);
     if constexpr (THINC && RIEMANN < RIEMANN_PF_MIX) {
         // THINC decisions first (gates, face normal, tanh profile), parked in the face's own flux
         // slots: the main pass below then carries only the base reconstructions, which keeps its
         // register footprint at the THINC-free kernel's
         for (int f = tid; f < F; f += NT) {
-            const int x = f % f0;
-            const int y = (f / f0) % f1;
-            const int z = f / (f0 * f1);
-            const int base = x + e0 * (y + e1 * z);
+            const int x = f % F0;
+            const int y = (f / F0) % F1;
+            const int z = f / (F0 * F1);
+            if (x >= f0 || y >= f1 || z >= f2) continue;
+            const int base = x + E0 * (y + E1 * z);
             auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
             // alpha1 halo coordinates of the face's high cell; transverse axes in ascending order
             const int hx = x + 1, hy = y + 1, hz = (DIM == 3) ? z + 1 : 0;
-            const int a0 = cx_.ae[0], a1 = cx_.ae[1];
+            constexpr int a0 = L::TX + 2, a1 = L::TY + 2;
             auto aa = [&](int od, int o1, int o2) -> double {
                 const int lo = (od == -1) ? 1 : 0;
                 int px = hx - (D == 0 ? lo : 0), py = hy - (D == 1 ? lo : 0), pz = hz - (D == 2 ? lo : 0);
@@ -156,11 +170,12 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     // the main pass maps faces to threads exactly as the THINC pass does: each thread reads back only
     // the slots it wrote, so no barrier is needed between the two
     for (int f = tid; f < F; f += NT) {
-        const int x = f % f0;
-        const int y = (f / f0) % f1;
-        const int z = f / (f0 * f1);
+        const int x = f % F0;
+        const int y = (f / F0) % F1;
+        const int z = f / (F0 * F1);
+        if (x >= f0 || y >= f1 || z >= f2) continue;
         const int a = (D == 0) ? x : ((D == 1) ? y : z);
-        const int base = x + e0 * (y + e1 * z);
+        const int base = x + E0 * (y + E1 * z);
         auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
         double Fl[N];
         if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
@@ -194,8 +209,8 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     __syncthreads();
     // 3. flux differences of this direction
     if (cx_.valid) {
-        const int fm = cx_.cx + f0 * (cx_.cy + f1 * cx_.cz);
-        const int fp = fm + ((D == 0) ? 1 : ((D == 1) ? f0 : f0 * f1));
+        const int fm = cx_.cx + F0 * (cx_.cy + F1 * cx_.cz);
+        const int fp = fm + ((D == 0) ? 1 : ((D == 1) ? F0 : F0 * F1));
         const double idx = kp.inv_dx[D];
 #pragma unroll
         for (int q = 0; q < N; ++q) {
@@ -277,12 +292,14 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
 #pragma unroll
     for (int q = 0; q < N; ++q) acc[q] = 0.0;
 
-    // alpha1 of the tile plus one cell all round (face normals of THINC)
+    // alpha1 of the tile plus one cell all round (face normals of THINC), at the full tile's
+    // extents (positions beyond a clipped tile's halo stay unused)
     int ae[3] = {n[0] + 2, n[1] + 2, DIM == 3 ? n[2] + 2 : 1};
     if (THINC) {
-        const int RA = ae[0] * ae[1] * ae[2];
-        for (int r = tid; r < RA; r += NT) {
-            const int x = r % ae[0], y = (r / ae[0]) % ae[1], z = r / (ae[0] * ae[1]);
+        constexpr int A0 = TX + 2, A1 = TY + 2, A2 = (DIM == 3) ? TZ + 2 : 1;
+        for (int r = tid; r < A0 * A1 * A2; r += NT) {
+            const int x = r % A0, y = (r / A0) % A1, z = r / (A0 * A1);
+            if (x >= ae[0] || y >= ae[1] || z >= ae[2]) continue;
             const int k = (DIM == 3) ? tlo[2] + z - 1 : tlo[2];
             sA[r] = Ub[fab_index(bx, tlo[0] + x - 1, tlo[1] + y - 1, k) + IA * bx.sn];
         }
