@@ -8,8 +8,8 @@
 //   for each direction d:
 //     1. the primitives of the tile plus G cells on both sides along d are staged in LDS
 //        (cons->prim once per staged cell),
-//     2. every face of the tile along d is computed once (reconstruction, prim->cons, Riemann);
-//        flux and u*_n go to LDS,
+//     2. every face of the tile along d is computed once, one face per thread (reconstruction,
+//        prim->cons, Riemann); flux and u*_n then go to LDS over the staged primitives,
 //     3. every cell accumulates (F_+ - F_-)/dx_d and (u*_+ - u*_-)/dx_d in registers;
 //   then dU = -div F, dU[alpha1] += phi1 (1+K1) div u*, and the RK3 combination writes the new
 //   state: dU never goes to HBM. The face wave speeds of the tile are max-reduced with wave
@@ -136,48 +136,21 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         for (int q = 0; q < N; ++q) sW[q * RMAX + r] = Wc[q];
     }
     __syncthreads();
-    // 2. faces along D
+    // 2. faces along D: one face per thread (F <= NT for every tile shape). The face is computed
+    // into registers while the staged primitives are read; after a barrier the fluxes are written
+    // over the staged region, whose LDS the flux slots share.
+    static_assert(F <= NT, "one face per thread");
     constexpr int rs = (D == 0) ? 1 : ((D == 1) ? E0 : E0 * E1);
-    if constexpr (THINC && RIEMANN < RIEMANN_PF_MIX) {
-        // THINC decisions first (gates, face normal, tanh profile), parked in the face's own flux
-        // slots: the main pass below then carries only the base reconstructions, which keeps its
-        // register footprint at the THINC-free kernel's
-        for (int f = tid; f < F; f += NT) {
-            const int x = f % F0;
-            const int y = (f / F0) % F1;
-            const int z = f / (F0 * F1);
-            if (x >= f0 || y >= f1 || z >= f2) continue;
-            const int base = x + E0 * (y + E1 * z);
-            auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
-            // alpha1 halo coordinates of the face's high cell; transverse axes in ascending order
-            const int hx = x + 1, hy = y + 1, hz = (DIM == 3) ? z + 1 : 0;
-            constexpr int a0 = L::TX + 2, a1 = L::TY + 2;
-            auto aa = [&](int od, int o1, int o2) -> double {
-                const int lo = (od == -1) ? 1 : 0;
-                int px = hx - (D == 0 ? lo : 0), py = hy - (D == 1 ? lo : 0), pz = hz - (D == 2 ? lo : 0);
-                if (D == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
-                else if (D == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
-                else { px += o1; py += o2; }
-                return sA[px + a0 * (py + a1 * pz)];
-            };
-            auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
-            double aL, aR;
-            thinc_decide<DIM, G>(kp, st, normal, aL, aR);
-            sF[0 * FMAX + f] = aL;
-            sF[1 * FMAX + f] = aR;
-        }
-    }
-    // the main pass maps faces to threads exactly as the THINC pass does: each thread reads back only
-    // the slots it wrote, so no barrier is needed between the two
-    for (int f = tid; f < F; f += NT) {
-        const int x = f % F0;
-        const int y = (f / F0) % F1;
-        const int z = f / (F0 * F1);
-        if (x >= f0 || y >= f1 || z >= f2) continue;
+    const int x = tid % F0;
+    const int y = (tid / F0) % F1;
+    const int z = tid / (F0 * F1);
+    const bool has_face = (tid < F) && (x < f0) && (y < f1) && (z < f2);
+    double Fl[N];
+    double un = 0.0;
+    if (has_face) {
         const int a = (D == 0) ? x : ((D == 1) ? y : z);
         const int base = x + E0 * (y + E1 * z);
         auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
-        double Fl[N];
         if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
             const double g = gface(x, y, z);
             if (g != 0.0) {
@@ -186,25 +159,37 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
 #pragma unroll
                 for (int q = 0; q < N; ++q) Fl[q] = 0.0;
             }
-#pragma unroll
-            for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
         } else {
             double aL = -1.0, aR = -1.0;
-            if (THINC) {
-                aL = sF[0 * FMAX + f];
-                aR = sF[1 * FMAX + f];
+            if constexpr (THINC) {
+                // THINC decision (gates, face normal, tanh profile) ahead of the base reconstruction
+                // alpha1 halo coordinates of the face's high cell; transverse axes in ascending order
+                const int hx = x + 1, hy = y + 1, hz = (DIM == 3) ? z + 1 : 0;
+                constexpr int a0 = L::TX + 2, a1 = L::TY + 2;
+                auto aa = [&](int od, int o1, int o2) -> double {
+                    const int lo = (od == -1) ? 1 : 0;
+                    int px = hx - (D == 0 ? lo : 0), py = hy - (D == 1 ? lo : 0), pz = hz - (D == 2 ? lo : 0);
+                    if (D == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
+                    else if (D == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
+                    else { px += o1; py += o2; }
+                    return sA[px + a0 * (py + a1 * pz)];
+                };
+                auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
+                thinc_decide<DIM, G>(kp, st, normal, aL, aR);
             }
             double c;
-            const double un = face_flux_decided<DIM, SCHEME, RIEMANN, THINC>(kp, cx_.limiter, D, st, aL, aR,
-                                                                           Fl, c, bad);
-#pragma unroll
-            for (int q = 0; q < N; ++q) sF[q * FMAX + f] = Fl[q];
-            sF[N * FMAX + f] = un;
+            un = face_flux_decided<DIM, SCHEME, RIEMANN, THINC>(kp, cx_.limiter, D, st, aL, aR, Fl, c, bad);
             if (a < cx_.n[D]) {
                 if (__builtin_isnan(c)) bad = CMP_ERR_NAN;
                 cmax_loc = smax(cmax_loc, c);
             }
         }
+    }
+    __syncthreads();
+    if (has_face) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) sF[q * FMAX + tid] = Fl[q];
+        if constexpr (RIEMANN < RIEMANN_PF_MIX) sF[N * FMAX + tid] = un;
     }
     __syncthreads();
     // 3. flux differences of this direction
@@ -251,8 +236,11 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
     constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
     constexpr int RMAX = L::RMAX, FMAX = L::FMAX, AMAX = THINC ? L::AMAX : 1;
 
-    __shared__ double sW[N * RMAX];
-    __shared__ double sF[(N + 1) * FMAX];
+    // the staged primitives of a direction and, once its faces are computed, their fluxes
+    constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
+    __shared__ double sWF[SWF];
+    double* const sW = sWF;
+    double* const sF = sWF;
     __shared__ double sA[AMAX];
     __shared__ double sRed[NT / 64];
 
