@@ -23,7 +23,7 @@ template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg&, const KParams&, const BoxDev&, int, const double*,
                             double*, double*, hipStream_t);
 template <int DIM>
-hipError_t launch_fill_dim(int, const FillDev&, long long, double*, int*, hipStream_t);
+hipError_t launch_fill_dim(int, const FillDev&, long long, const int*, int, double*, int*, hipStream_t);
 template <int DIM>
 hipError_t launch_dt_dim(const double*, double*, double*, double, int, double*, int*, int*,
                          hipStream_t);
@@ -38,7 +38,7 @@ template <int DIM>
 hipError_t launch_pf_update_dim(const PfGeom&, long long, double*, const double*, double, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_gradq_dim(const PfGeom&, const long long*, const double*, const double*, double*,
-                               unsigned long long*, hipStream_t);
+                               unsigned long long*, int*, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_maxvel_dim(const PfGeom&, long long, const double*, unsigned long long*,
                                 hipStream_t);
@@ -57,9 +57,10 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 }
 
 hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, double* U, int* err,
-                       hipStream_t s)
+                       hipStream_t s, const int* list, int nlist)
 {
-    return dim == 2 ? launch_fill_dim<2>(N, fd, nghost, U, err, s) : launch_fill_dim<3>(N, fd, nghost, U, err, s);
+    return dim == 2 ? launch_fill_dim<2>(N, fd, nghost, list, nlist, U, err, s)
+                    : launch_fill_dim<3>(N, fd, nghost, list, nlist, U, err, s);
 }
 
 #define CMP_BY_DIM(dim, FN, ...) ((dim) == 2 ? FN<2>(__VA_ARGS__) : FN<3>(__VA_ARGS__))
@@ -90,9 +91,10 @@ hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double
 }
 
 hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* S,
-                           const double* H, double* gq, unsigned long long* gmax_bits, hipStream_t s)
+                           const double* H, double* gq, unsigned long long* gmax_bits, int* boxflag,
+                           hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_gradq_dim, g, maxfaces, S, H, gq, gmax_bits, s);
+    return CMP_BY_DIM(dim, launch_pf_gradq_dim, g, maxfaces, S, H, gq, gmax_bits, boxflag, s);
 }
 
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
@@ -127,7 +129,7 @@ template <int DIM>
 hipError_t launch_cg_p_dim(const PfGeom&, long long, const double*, const double*, double*, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_gradx_dim(const PfGeom&, const long long*, const double*, double*, unsigned long long*,
-                               hipStream_t);
+                               int*, hipStream_t);
 
 hipError_t launch_pf_cac_mech(int dim, const PfGeom& g, long long maxcells, const double* A,
                               const double* Nrm, double* Mk, double* Wk, double mob, double lambda,
@@ -186,9 +188,9 @@ hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const doubl
 }
 
 hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X, double* gq,
-                           unsigned long long* gmax_bits, hipStream_t s)
+                           unsigned long long* gmax_bits, int* boxflag, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_gradx_dim, g, maxfaces, X, gq, gmax_bits, s);
+    return CMP_BY_DIM(dim, launch_pf_gradx_dim, g, maxfaces, X, gq, gmax_bits, boxflag, s);
 }
 
 hipError_t launch_compute_dt(int dim, const double* dx, double* cmax, double* dt, double cfl,
@@ -283,6 +285,23 @@ struct cmp_level {
     int nblocks = 0;                // blocks of a cell pass over the level
     long long ncell = 0;            // valid cells of the level
     int cg_iters = 0;               // iterations of the last implicit solve
+    // Phase-Field sub-steps run on the boxes with a nonzero Grad_Q face and the boxes whose valid
+    // cells fill their ghosts; every other box has a zero Phase-Field flux on all its faces
+    std::vector<std::vector<int>> ghost_src;   // per box: boxes its ghost cells are filled from
+    int* d_boxflag = nullptr;       // per box: a nonzero Grad_Q face (set by the Grad_Q kernels)
+    int* h_boxflag = nullptr;       // pinned copy
+    TileDev* d_pf_tiles = nullptr;  // tiles of the selected boxes
+    TileDev* h_pf_tiles = nullptr;  // pinned staging of the list
+    int* d_pf_fill = nullptr;       // boxes whose ghosts the Phase-Field sub-steps fill
+    int* h_pf_fill = nullptr;       // pinned staging of the list
+    int pf_ntiles = 0, pf_nfill = 0;
+
+    // host copies of the level tables (tiles box by box, the owner map)
+    std::vector<TileDev> h_tiles;
+    std::vector<int> tile_start;    // first tile of each box, nbox + 1 entries
+    std::vector<int> h_owner;
+    long long bs = 1;
+    int nb[3] = {1, 1, 1};
 };
 
 namespace {
@@ -522,6 +541,8 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
             }
         }
         if (boxes[b].offset != expect) return CMP_ERR_LAYOUT;
+        // the kernels index the cells of one fab with 32-bit integers
+        if (cmp_fab_size(dim, &boxes[b], ng, 1) >= (1LL << 31)) return CMP_ERR_UNSUPPORTED;
         expect += cmp_fab_size(dim, &boxes[b], ng, params->nstate);
     }
     for (int d = 0; d < dim; ++d)
@@ -557,6 +578,9 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
                     ht.push_back(TileDev{b, {i, j, k}});
     }
     L->ntiles = (int)ht.size();
+    L->tile_start.assign(nbox + 1, 0);
+    for (const TileDev& t : ht) ++L->tile_start[t.box + 1];
+    for (int b = 0; b < nbox; ++b) L->tile_start[b + 1] += L->tile_start[b];
     // ghost-count prefix
     std::vector<long long> gp(nbox + 1, 0);
     for (int b = 0; b < nbox; ++b) {
@@ -621,9 +645,15 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
     fd.boxes = L->d_boxes;
     fd.gprefix = L->d_gprefix;
     fd.owner = L->d_owner;
+    L->h_tiles = std::move(ht);
+    L->h_owner = std::move(owner);
+    L->bs = bs;
+    for (int d = 0; d < 3; ++d) L->nb[d] = nb[d];
     fd.nbox = nbox;
     fd.ng = ng;
     fd.bs = (int)bs;
+    fd.maxghost = 0;
+    for (int b = 0; b < nbox; ++b) fd.maxghost = std::max(fd.maxghost, gp[b + 1] - gp[b]);
     for (int d = 0; d < 3; ++d) {
         fd.nb[d] = nb[d];
         fd.dlo[d] = geom->domain_lo[d];
@@ -641,10 +671,16 @@ static void pf_free(cmp_level* L)
 {
     void* bufs[] = {L->d_abox, L->d_nbox, L->d_gprefix1, L->d_A0, L->d_A, L->d_P, L->d_N,
                     L->d_J, L->d_H, L->d_gq, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
-                    L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Bf, L->d_part, L->d_sc};
+                    L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Bf, L->d_part, L->d_sc, L->d_boxflag,
+                    L->d_pf_tiles, L->d_pf_fill};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
-    if (L->h_pin) (void)hipHostFree(L->h_pin);
+    void* pins[] = {L->h_pin, L->h_boxflag, L->h_pf_tiles, L->h_pf_fill};
+    for (void* p : pins)
+        if (p) (void)hipHostFree(p);
+    L->d_boxflag = L->d_pf_fill = L->h_boxflag = L->h_pf_fill = nullptr;
+    L->d_pf_tiles = L->h_pf_tiles = nullptr;
+    L->ghost_src.clear();
     L->d_abox = L->d_nbox = nullptr;
     L->d_gprefix1 = nullptr;
     L->d_A0 = L->d_A = L->d_P = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_pfdt = nullptr;
@@ -811,6 +847,44 @@ int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nste
 
 // ---- Phase-Field step ---------------------------------------------------------------------------
 
+// For every box, the boxes whose valid cells its ng-deep ghost cells are filled from: the owners of
+// the blocks covering the box grown by ng, the grown range wrapped on periodic sides and clamped to
+// the domain elsewhere (foextrap and reflect sources lie within ng cells of the domain face).
+static std::vector<std::vector<int>> ghost_sources(const cmp_level* L)
+{
+    const int dim = L->P.dim, ng = L->ng;
+    std::vector<std::vector<int>> src(L->nbox);
+    std::vector<char> seen(L->nbox, 0);
+    for (int b = 0; b < L->nbox; ++b) {
+        std::vector<int> blk[3];
+        for (int d = 0; d < 3; ++d) {
+            if (d >= dim) { blk[d].push_back(0); continue; }
+            const int dlo = L->G.domain_lo[d], dhi = L->G.domain_hi[d], per = dhi - dlo + 1;
+            std::vector<char> hit(L->nb[d], 0);
+            for (int c = L->boxes[b].lo[d] - ng; c <= L->boxes[b].hi[d] + ng; ++c) {
+                int x = c;
+                if (L->G.is_periodic[d]) {
+                    x = ((x - dlo) % per + per) % per + dlo;
+                } else {
+                    x = std::min(std::max(x, dlo), dhi);
+                }
+                hit[(x - dlo) / L->bs] = 1;
+            }
+            for (int q = 0; q < L->nb[d]; ++q)
+                if (hit[q]) blk[d].push_back(q);
+        }
+        for (int k : blk[2])
+            for (int j : blk[1])
+                for (int i : blk[0]) {
+                    const long long q = i + (long long)L->nb[0] * (j + (long long)L->nb[1] * k);
+                    const int o = L->h_owner[q];
+                    if (o >= 0 && !seen[o]) { seen[o] = 1; src[b].push_back(o); }
+                }
+        for (int o : src[b]) seen[o] = 0;
+    }
+    return src;
+}
+
 int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
 {
     if (!L || !pf) return CMP_ERR_ARG;
@@ -891,7 +965,15 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
         }
         CMP_TRYP(hipMalloc(&L->d_Bf, sizeof(double) * nf));
     }
+    CMP_TRYP(hipMalloc(&L->d_boxflag, sizeof(int) * nbox));
+    CMP_TRYP(hipMemset(L->d_boxflag, 0, sizeof(int) * nbox));
+    CMP_TRYP(hipHostMalloc((void**)&L->h_boxflag, sizeof(int) * nbox, hipHostMallocDefault));
+    CMP_TRYP(hipMalloc(&L->d_pf_tiles, sizeof(TileDev) * std::max(1, L->ntiles)));
+    CMP_TRYP(hipHostMalloc((void**)&L->h_pf_tiles, sizeof(TileDev) * std::max(1, L->ntiles), hipHostMallocDefault));
+    CMP_TRYP(hipMalloc(&L->d_pf_fill, sizeof(int) * nbox));
+    CMP_TRYP(hipHostMalloc((void**)&L->h_pf_fill, sizeof(int) * nbox, hipHostMallocDefault));
 #undef CMP_TRYP
+    L->ghost_src = ghost_sources(L);
     long long ncell = 0;
     for (int b = 0; b < nbox; ++b) {
         long long v = 1;
@@ -910,6 +992,8 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     L->fdA.boxes = L->d_abox;
     L->fdA.gprefix = L->d_gprefix1;
     L->fdA.ng = 1;
+    L->fdA.maxghost = 0;
+    for (int b = 0; b < nbox; ++b) L->fdA.maxghost = std::max(L->fdA.maxghost, gp[b + 1] - gp[b]);
     for (int d = 0; d < 3; ++d) {
         L->fdA.lo_bc[d] = 2;
         L->fdA.hi_bc[d] = 2;
@@ -995,7 +1079,7 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
     }
     L->cg_iters = (int)L->h_pin[3];
     CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_X, L->d_err, s));
-    CMP_TRY(launch_pf_gradx(dim, L->pg, L->maxfaces, L->d_X, gradq, L->d_bits + 1, s));
+    CMP_TRY(launch_pf_gradx(dim, L->pg, L->maxfaces, L->d_X, gradq, L->d_bits + 1, L->d_boxflag, s));
     return CMP_OK;
 }
 
@@ -1009,6 +1093,7 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
     const int dim = L->P.dim;
     const cmp_pf_params& pf = L->pf;
     CMP_TRY(launch_zero_u64(L->d_bits, 2, s));
+    CMP_TRY(hipMemsetAsync(L->d_boxflag, 0, sizeof(int) * L->nbox, s));
     CMP_TRY(launch_pf_maxvel(dim, L->pg, L->maxcells, U, L->d_bits, s));
     CMP_TRY(launch_pf_extract(dim, L->pg, L->maxcells, U, L->d_A0, s));
     CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A0, L->d_err, s));
@@ -1069,12 +1154,13 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
         // explicit Grad_Q from the step's alpha1 (PhaseField_GradQPF, :2315-2403), re-sharpened at
         // eta_factor * eta
         CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
-        CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, gradq, L->d_bits + 1, s));
+        CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, gradq, L->d_bits + 1, L->d_boxflag, s));
     } else {
         int r = pf_gradq_implicit(L, dt, eta, gradq, s);
         if (r) return r;
     }
     CMP_TRY(hipMemcpyAsync(L->h_pin + 1, L->d_bits + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    CMP_TRY(hipMemcpyAsync(L->h_boxflag, L->d_boxflag, sizeof(int) * L->nbox, hipMemcpyDeviceToHost, s));
     CMP_TRY(hipStreamSynchronize(s));
     if (gradq_max) *gradq_max = bits_to_double(L->h_pin + 1);
     if (info) {
@@ -1086,21 +1172,54 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
     return CMP_OK;
 }
 
+// One Phase-Field RK3 sub-step over the given tiles; the ghosts of the `fill` boxes (all boxes when
+// fill is null) are filled before each stage.
+static int pf_rk3(cmp_level* L, double* Un, double* U1, double* U2, const double* gradq, double pf_dt,
+                  const TileDev* tiles, int ntiles, const int* fill, int nfill, hipStream_t s)
+{
+    CMP_TRY(launch_setd(L->d_pfdt, pf_dt, s));
+    double* Us[3] = {Un, U1, U2};
+    double* Uo[3] = {U1, U2, Un};
+    for (int stage = 0; stage < 3; ++stage) {
+        CMP_TRY(launch_fill(L->P.dim, L->P.nstate, L->fd, L->nghost, Us[stage], L->d_err, s, fill, nfill));
+        if (ntiles > 0)
+            CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RK, stage, tiles, ntiles, L->d_boxes, nullptr, 0,
+                                 Us[stage], Un, Uo[stage], L->d_pfdt, nullptr, L->d_err, s, gradq));
+    }
+    return CMP_OK;
+}
+
 int cmp_level_pf_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, const double* gradq,
                           double pf_dt, void* stream)
 {
     if (!L || !Un || !U1 || !U2 || !gradq) return CMP_ERR_ARG;
     if (!L->pf_on) return CMP_ERR_ARG;
-    hipStream_t s = (hipStream_t)stream;
-    CMP_TRY(launch_setd(L->d_pfdt, pf_dt, s));
-    double* Us[3] = {Un, U1, U2};
-    double* Uo[3] = {U1, U2, Un};
-    for (int stage = 0; stage < 3; ++stage) {
-        int r = cmp_level_fill_ghosts(L, Us[stage], stream);
-        if (r) return r;
-        CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes,
-                             nullptr, 0, Us[stage], Un, Uo[stage], L->d_pfdt, nullptr, L->d_err, s, gradq));
+    return pf_rk3(L, Un, U1, U2, gradq, pf_dt, L->d_tiles, L->ntiles, nullptr, 0, (hipStream_t)stream);
+}
+
+// The boxes the Phase-Field sub-steps of one step touch, from the box flags of the last Grad_Q:
+// the flagged boxes (ghosts filled, fluxes computed) and the boxes their ghosts are filled from
+// (RK combination with a zero flux, so that their stage states hold what the filled ghosts read).
+// A box outside both has a zero flux on every face; its state is left as it is.
+static int pf_select_boxes(cmp_level* L, hipStream_t s)
+{
+    std::vector<char> need(L->nbox, 0);
+    int nfill = 0;
+    for (int b = 0; b < L->nbox; ++b) {
+        if (!L->h_boxflag[b]) continue;
+        L->h_pf_fill[nfill++] = b;
+        need[b] = 1;
+        for (int o : L->ghost_src[b]) need[o] = 1;
     }
+    int nt = 0;
+    for (int b = 0; b < L->nbox; ++b) {
+        if (!need[b]) continue;
+        for (int t = L->tile_start[b]; t < L->tile_start[b + 1]; ++t) L->h_pf_tiles[nt++] = L->h_tiles[t];
+    }
+    L->pf_nfill = nfill;
+    L->pf_ntiles = nt;
+    if (nfill > 0) CMP_TRY(hipMemcpyAsync(L->d_pf_fill, L->h_pf_fill, sizeof(int) * nfill, hipMemcpyHostToDevice, s));
+    if (nt > 0) CMP_TRY(hipMemcpyAsync(L->d_pf_tiles, L->h_pf_tiles, sizeof(TileDev) * nt, hipMemcpyHostToDevice, s));
     return CMP_OK;
 }
 
@@ -1116,6 +1235,7 @@ int cmp_level_pf_evolve(cmp_level* L, double* Un, double* U1, double* U2, const 
     double gm = 0.0, inf4[4] = {0.0, 0.0, 0.0, 0.0};
     int r = cmp_level_pf_gradq(L, Un, dt, L->d_gq, &gm, inf4, stream);
     if (r) return r;
+    if ((r = pf_select_boxes(L, s))) return r;
     // Phase-Field sub-steps (PhaseField_Evolve :541-590, PhaseField_ComputeDt :2037-2098)
     const int dim = L->P.dim;
     double dxm = L->G.dx[0];
@@ -1131,7 +1251,9 @@ int cmp_level_pf_evolve(cmp_level* L, double* Un, double* U1, double* U2, const 
         const double eps = (1.0e-3) * dt0;
         if (pf_time + dt0 > dt - eps) dt0 = dt - pf_time;
         prev = dt0;
-        if ((r = cmp_level_pf_rk3_step(L, Un, U1, U2, L->d_gq, dt0, stream))) return r;
+        if (L->pf_nfill > 0 &&
+            (r = pf_rk3(L, Un, U1, U2, L->d_gq, dt0, L->d_pf_tiles, L->pf_ntiles, L->d_pf_fill, L->pf_nfill, s)))
+            return r;
         pf_time += dt0;
         ++n;
         if (pf_time >= dt - 1.0e-6 * dt) break;

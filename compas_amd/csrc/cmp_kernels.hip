@@ -426,57 +426,55 @@ void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
 // periodic wrap, and read from the valid cells of the box owning it.
 template <int DIM, int N>
 __global__ __launch_bounds__(NT)
-void k_fill(FillDev fd, long long nghost, double* U, int* err)
+void k_fill(FillDev fd, const int* __restrict__ list, int box0, double* U, int* err)
 {
-    for (long long g = (long long)blockIdx.x * NT + threadIdx.x; g < nghost;
-         g += (long long)gridDim.x * NT) {
-        int lo = 0, hi = fd.nbox;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (fd.gprefix[mid] <= g) lo = mid; else hi = mid;
-        }
-        const int b = lo;
-        const BoxDev bx = fd.boxes[b];
-        const int ng = fd.ng;
-        const int nv1 = bx.hi[1] - bx.lo[1] + 1;
-        const int ex = bx.hi[0] - bx.lo[0] + 1 + 2 * ng, ey = nv1 + 2 * ng;
-        const int ez = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + 2 * ng : 1;
+    // one grid row per box (of the level, or of `list`); 32-bit ghost indices inside the box
+    const int b = list ? list[box0 + (int)blockIdx.y] : box0 + (int)blockIdx.y;
+    const BoxDev bx = fd.boxes[b];
+    const int ng = fd.ng;
+    const int nv0 = bx.hi[0] - bx.lo[0] + 1, nv1 = bx.hi[1] - bx.lo[1] + 1;
+    const int nv2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
+    const int ex = nv0 + 2 * ng, ey = nv1 + 2 * ng;
+    const int ez = (DIM == 3) ? nv2 + 2 * ng : 1;
+    const int nghost = ex * ey * ez - nv0 * nv1 * nv2;
+    for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < nghost; g += (int)(gridDim.x * NT)) {
         int x = 0, y = 0, z = 0;
-        long long r = g - fd.gprefix[b];
+        int r = g;
         bool slab = false;
         if (DIM == 3) {
-            const long long A = (long long)ng * ex * ey;
-            const long long exy = (long long)ex * ey;
+            const int exy = ex * ey;
+            const int A = ng * exy;
             if (r < 2 * A) {
                 // z-low (r < A) or z-high slab: whole xy planes
                 const bool high = r >= A;
                 if (high) r -= A;
-                z = (high ? ez - ng : 0) + (int)(r / exy);
-                r %= exy;
-                y = (int)(r / ex);
-                x = (int)(r % ex);
+                z = (high ? ez - ng : 0) + r / exy;
+                r -= (r / exy) * exy;
+                y = r / ex;
+                x = r - y * ex;
                 slab = true;
             } else {
                 r -= 2 * A;
-                const long long Rg = 2LL * ng * ex + 2LL * ng * nv1;
-                z = ng + (int)(r / Rg);
-                r %= Rg;
+                const int Rg = 2 * ng * ex + 2 * ng * nv1;
+                const int q = r / Rg;
+                z = ng + q;
+                r -= q * Rg;
             }
-        } else {
-            z = 0;
         }
         if (!slab) {
-            if (r < (long long)ng * ex) {
-                y = (int)(r / ex);
-                x = (int)(r % ex);
-            } else if (r < 2LL * ng * ex) {
-                r -= (long long)ng * ex;
-                y = ey - ng + (int)(r / ex);
-                x = (int)(r % ex);
+            if (r < ng * ex) {
+                y = r / ex;
+                x = r - y * ex;
+            } else if (r < 2 * ng * ex) {
+                r -= ng * ex;
+                const int q = r / ex;
+                y = ey - ng + q;
+                x = r - q * ex;
             } else {
-                r -= 2LL * ng * ex;
-                y = ng + (int)(r / (2 * ng));
-                const int cc = (int)(r % (2 * ng));
+                r -= 2 * ng * ex;
+                const int q = r / (2 * ng);
+                y = ng + q;
+                const int cc = r - q * (2 * ng);
                 x = (cc < ng) ? cc : (ex - 2 * ng + cc);
             }
         }
@@ -509,7 +507,7 @@ void k_fill(FillDev fd, long long nghost, double* U, int* err)
         for (int d = 0; d < DIM; ++d)
             if (src[d] < fd.dlo[d] || src[d] > fd.dhi[d]) ok = 0;
         if (!ok) { atomicMax(err, CMP_ERR_LAYOUT); continue; }
-        long long bi = 0;
+        int bi = 0;
         for (int d = DIM - 1; d >= 0; --d)
             bi = bi * fd.nb[d] + (src[d] - fd.dlo[d]) / fd.bs;
         const int own = fd.owner[bi];
@@ -668,20 +666,26 @@ hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev
 }
 
 template <int DIM>
-hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, double* U, int* err,
+hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, const int* list, int nlist, double* U,
+                           int* err,
                            hipStream_t s)
 {
     if (nghost <= 0) return hipSuccess;
-    long long blocks = (nghost + NT - 1) / NT;
-    if (blocks > 65536) blocks = 65536;
-    if (N == 4 + DIM)
-        hipLaunchKernelGGL((k_fill<DIM, 4 + DIM>), dim3((unsigned)blocks), dim3(NT), 0, s, fd, nghost, U, err);
-    else if (N == DIM)
-        hipLaunchKernelGGL((k_fill<DIM, DIM>), dim3((unsigned)blocks), dim3(NT), 0, s, fd, nghost, U, err);
-    else if (N == 1)
-        hipLaunchKernelGGL((k_fill<DIM, 1>), dim3((unsigned)blocks), dim3(NT), 0, s, fd, nghost, U, err);
-    else
-        return hipErrorInvalidValue;
+    if (N != 4 + DIM && N != DIM && N != 1) return hipErrorInvalidValue;
+    // grid rows are boxes (at most 65535 per launch); grid columns cover the largest box's ghosts
+    long long cols = (fd.maxghost + NT - 1) / NT;
+    if (cols > 65536) cols = 65536;
+    const int nb = list ? nlist : fd.nbox;
+    for (int box0 = 0; box0 < nb; box0 += 65535) {
+        const int rows = (nb - box0 < 65535) ? nb - box0 : 65535;
+        const dim3 grid((unsigned)cols, (unsigned)rows);
+        if (N == 4 + DIM)
+            hipLaunchKernelGGL((k_fill<DIM, 4 + DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
+        else if (N == DIM)
+            hipLaunchKernelGGL((k_fill<DIM, DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
+        else
+            hipLaunchKernelGGL((k_fill<DIM, 1>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
+    }
     return hipGetLastError();
 }
 
@@ -710,7 +714,7 @@ template hipError_t launch_stage_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KPa
                                                      hipStream_t, const double*);
 template hipError_t launch_faces_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev&,
                                                      int, const double*, double*, double*, hipStream_t);
-template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(int, const FillDev&, long long, double*, int*,
+template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(int, const FillDev&, long long, const int*, int, double*, int*,
                                                     hipStream_t);
 template hipError_t launch_dt_dim<CMP_KERNEL_DIM>(const double*, double*, double*, double, int,
                                                   double*, int*, int*, hipStream_t);

@@ -49,6 +49,7 @@ struct FillDev {
     int nbox;
     int ng;
     int bs;
+    long long maxghost;         // ghost cells of the box with the most
     int nb[3];
     int dlo[3], dhi[3];
     int periodic[3];
@@ -77,9 +78,10 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
                         const double* gradq = nullptr);
 
 // N = 4 + dim: the flow state (reflecting BCs flip momenta); N = 1 or dim: a Phase-Field field
-// (alpha1 or the normals), filled without sign changes
+// (alpha1 or the normals), filled without sign changes. With `list`, only the ghosts of the nlist
+// boxes it names are filled.
 hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, double* U, int* err,
-                       hipStream_t s);
+                       hipStream_t s, const int* list = nullptr, int nlist = 0);
 
 // ---- Phase-Field mechanism kernels (grid.y = box) ----
 // PfGeom: per-level constants of the mechanism kernels.
@@ -104,8 +106,10 @@ hipError_t launch_pf_jfaces(int dim, const PfGeom& g, const long long* maxfaces,
                             double eta, double dt_m, hipStream_t s);
 hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double* A, const double* J,
                             double dt_m, hipStream_t s);
+// boxflag (nbox ints, or null): set to 1 for every box with a nonzero Grad_Q on one of its faces
 hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* S,
-                           const double* H, double* gq, unsigned long long* gmax_bits, hipStream_t s);
+                           const double* H, double* gq, unsigned long long* gmax_bits, int* boxflag,
+                           hipStream_t s);
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
                             unsigned long long* bits, hipStream_t s);
 
@@ -155,7 +159,7 @@ hipError_t launch_cg_start(double* sc, hipStream_t s);
 hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s);
 // face gradient of x (ghosts filled), zero on Neumann domain faces, with max |.|
 hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X,
-                           double* gq, unsigned long long* gmax_bits, hipStream_t s);
+                           double* gq, unsigned long long* gmax_bits, int* boxflag, hipStream_t s);
 hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s);
 hipError_t launch_scale(double* p, long long n, double s_, hipStream_t s);
 hipError_t launch_setd(double* p, double v, hipStream_t s);

@@ -216,7 +216,7 @@ __global__ __launch_bounds__(NT) void k_pf_update(PfGeom g, double* A, const dou
 template <int DIM, int D>
 __global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restrict__ S,
                                                  const double* __restrict__ H, double* gq,
-                                                 unsigned long long* gmax_bits)
+                                                 unsigned long long* gmax_bits, int* boxflag)
 {
     const BoxDev ab = g.abox[blockIdx.y];
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
@@ -231,6 +231,7 @@ __global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restr
         gq[f] = r;
         v = fabs(r);
         if (__builtin_isnan(r)) v = __longlong_as_double(0x7ff8000000000000LL);
+        if (boxflag && r != 0.0) boxflag[blockIdx.y] = 1;   // every writer stores the same value
     }
     block_max_to(v, gmax_bits);
 }
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(NT) void k_cg_p(PfGeom g, const double* __restrict_
 // Grad_Q = (x_c - x_m)/dx on every face of direction D; zero on Neumann domain faces
 template <int DIM, int D>
 __global__ __launch_bounds__(NT) void k_pf_gradx(PfGeom g, const double* __restrict__ X, double* gq,
-                                                 unsigned long long* gmax_bits)
+                                                 unsigned long long* gmax_bits, int* boxflag)
 {
     const BoxDev ab = g.abox[blockIdx.y];
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
@@ -461,6 +462,7 @@ __global__ __launch_bounds__(NT) void k_pf_gradx(PfGeom g, const double* __restr
         gq[ab.goff[D] + q] = r;
         v = fabs(r);
         if (__builtin_isnan(r)) v = __longlong_as_double(0x7ff8000000000000LL);
+        if (boxflag && r != 0.0) boxflag[blockIdx.y] = 1;
     }
     block_max_to(v, gmax_bits);
 }
@@ -549,13 +551,15 @@ hipError_t launch_cg_p_dim(const PfGeom& g, long long maxcells, const double* sc
 
 template <int DIM>
 hipError_t launch_pf_gradx_dim(const PfGeom& g, const long long* maxfaces, const double* X, double* gq,
-                               unsigned long long* bits, hipStream_t s)
+                               unsigned long long* bits, int* boxflag, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_gradx<DIM, 0>), grid_of_cells(maxfaces[0], g.nbox), dim3(NT), 0, s, g, X, gq, bits);
-    hipLaunchKernelGGL((k_pf_gradx<DIM, 1>), grid_of_cells(maxfaces[1], g.nbox), dim3(NT), 0, s, g, X, gq, bits);
+    hipLaunchKernelGGL((k_pf_gradx<DIM, 0>), grid_of_cells(maxfaces[0], g.nbox), dim3(NT), 0, s, g, X, gq, bits,
+                       boxflag);
+    hipLaunchKernelGGL((k_pf_gradx<DIM, 1>), grid_of_cells(maxfaces[1], g.nbox), dim3(NT), 0, s, g, X, gq, bits,
+                       boxflag);
     if (DIM == 3)
         hipLaunchKernelGGL((k_pf_gradx<DIM, (DIM == 3 ? 2 : 1)>), grid_of_cells(maxfaces[2], g.nbox), dim3(NT), 0,
-                           s, g, X, gq, bits);
+                           s, g, X, gq, bits, boxflag);
     return hipGetLastError();
 }
 
@@ -605,15 +609,16 @@ hipError_t launch_pf_update_dim(const PfGeom& g, long long maxcells, double* A, 
 
 template <int DIM>
 hipError_t launch_pf_gradq_dim(const PfGeom& g, const long long* maxfaces, const double* S,
-                               const double* H, double* gq, unsigned long long* gmax_bits, hipStream_t s)
+                               const double* H, double* gq, unsigned long long* gmax_bits, int* boxflag,
+                               hipStream_t s)
 {
     hipLaunchKernelGGL((k_pf_gradq<DIM, 0>), grid_of(maxfaces[0], g.nbox), dim3(NT), 0, s, g, S, H, gq,
-                       gmax_bits);
+                       gmax_bits, boxflag);
     hipLaunchKernelGGL((k_pf_gradq<DIM, 1>), grid_of(maxfaces[1], g.nbox), dim3(NT), 0, s, g, S, H, gq,
-                       gmax_bits);
+                       gmax_bits, boxflag);
     if (DIM == 3)
         hipLaunchKernelGGL((k_pf_gradq<DIM, (DIM == 3 ? 2 : 1)>), grid_of(maxfaces[2], g.nbox), dim3(NT),
-                           0, s, g, S, H, gq, gmax_bits);
+                           0, s, g, S, H, gq, gmax_bits, boxflag);
     return hipGetLastError();
 }
 
@@ -752,7 +757,7 @@ template hipError_t launch_pf_jfaces_dim<CMP_KERNEL_DIM>(const PfGeom&, const lo
 template hipError_t launch_pf_update_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, double*, const double*,
                                                          double, hipStream_t);
 template hipError_t launch_pf_gradq_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
-                                                        const double*, double*, unsigned long long*,
+                                                        const double*, double*, unsigned long long*, int*,
                                                         hipStream_t);
 template hipError_t launch_pf_maxvel_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
                                                          unsigned long long*, hipStream_t);
@@ -776,6 +781,6 @@ template hipError_t launch_cg_xr_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, c
 template hipError_t launch_cg_p_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
                                                     double*, hipStream_t);
 template hipError_t launch_pf_gradx_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*, double*,
-                                                        unsigned long long*, hipStream_t);
+                                                        unsigned long long*, int*, hipStream_t);
 
 }  // namespace cmp

@@ -284,3 +284,42 @@ def test_pf_setup_rejects_cac_with_the_explicit_rc(gpu):
         lev.pf_setup(make_pf_params(mechanism="CAC-Adv", explicit_rc=1))
     lev.pf_setup(make_pf_params(mechanism="CAC-Adv", explicit_rc=0))
     lev.close()
+
+
+def test_pf_evolve_skips_boxes_without_interface(gpu):
+    """3D water-air droplet at 64^3 in 512 boxes of 8^3: the Phase-Field sub-steps run only on the
+    boxes with a nonzero Grad_Q face and the boxes feeding their ghosts. Every other box has a zero
+    Phase-Field flux on all its faces: its state stays bit-identical on the GPU, while the oracle
+    (the reference's LinComb over every cell) moves it by rounding only. The whole state matches the
+    oracle to 1e-12."""
+    import torch
+    c = cases.water_air_droplet_3d(n=64, box=8, thinc=1)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(eta_multiplier=0.5, cfl=0.3)
+    lay = Layout(3, c["boxes"], 3, 7)
+    buf0 = lay.pack(c["U"])
+    buf = buf0.copy()
+    dt = 0.5 / 64 / 500.0
+    ref_info = oracle.level_pf_evolve(P, pf, G, lay, buf, dt)
+    ref = lay.unpack(buf, c["shape"])
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    Un = _dev(torch, buf0, gpu)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dtd = torch.full((1,), dt, dtype=torch.float64, device=gpu)
+    info = lev.PhaseField_Evolve(Un, U1, U2, dtd)
+    assert lev.check_errors() == 0
+    got = lay.unpack(Un.cpu().numpy(), c["shape"])
+    lev.close()
+    assert info[5] == ref_info[5] and info[5] >= 1
+    rel = maxrel_per_var(got, ref)
+    assert rel.max() <= TOL_STEP, rel
+    # the corner box (cells 0..7 in every direction) is far from the droplet (alpha at its clip)
+    init = c["U"]
+    corner = (slice(None), slice(0, 8), slice(0, 8), slice(0, 8))
+    assert np.all(init[6][corner[1:]] == 1e-8)
+    assert np.array_equal(got[corner], init[corner])
+    assert np.max(np.abs(ref[corner] - init[corner]) / np.maximum(np.abs(init[corner]), 1e-300)) <= 1e-15
+    # and the droplet's boxes did change
+    centre = (slice(None), slice(28, 36), slice(28, 36), slice(28, 36))
+    assert not np.array_equal(got[centre], init[centre])
