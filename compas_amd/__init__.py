@@ -34,32 +34,43 @@ BC = {"Interior": 0, "Inflow": 1, "Outflow": 2, "Symmetry": 3, "SlipWall": 4, "N
 RIEMANN = {"HLL": 1, "HLLC": 2}
 
 
-def eos_coefficients(eos, gamma, pinf=(0.0, 0.0), b=(0.0, 0.0), q=(0.0, 0.0)):
-    """A, B, C, D per phase as Parm::Initialize sets them (source/Parm.cpp:124-156).
+def eos_coefficients(eos, gamma, pinf=None, b=None, q=None):
+    """A, B, C, D per phase as Parm::Initialize sets them (source/Parm.cpp:124-156, :187-272).
 
     eos: 0 ideal gas, 1 stiffened gas, 3 NASG (covolume b in C, heat of formation q in D)."""
+    n = len(gamma)
+    zero = [0.0] * n
+    pinf = list(pinf) if pinf is not None else zero
     A = [1.0 / (g - 1.0) for g in gamma]
     if eos == 0:
-        return A, [0.0, 0.0], [0.0, 0.0], [0.0, 0.0]
-    Bc = [gamma[k] * pinf[k] / (gamma[k] - 1.0) for k in range(2)]
+        return A, zero, zero, zero
+    Bc = [gamma[k] * pinf[k] / (gamma[k] - 1.0) for k in range(n)]
     if eos == 1:
-        return A, Bc, [0.0, 0.0], [0.0, 0.0]
+        return A, Bc, zero, zero
     if eos == 3:
-        return A, Bc, list(b), list(q)
+        return A, Bc, list(b) if b is not None else zero, list(q) if q is not None else zero
     raise ValueError("EOS %r not supported on this path" % (eos,))
 
 
-def make_params(dim, gamma=(1.4, 1.4), pinf=(0.0, 0.0), eos=None, scheme="WENO5", riemann="HLLC",
-                source_term=0, b=(0.0, 0.0), q=(0.0, 0.0), thinc=0, thinc_use_normal=1,
-                thinc_eps=1.0e-4, thinc_beta=2.3, thinc_normal_threshold=0.25):
+def make_params(dim, gamma=(1.4, 1.4), pinf=None, eos=None, scheme="WENO5", riemann="HLLC",
+                source_term=0, b=None, q=None, thinc=0, thinc_use_normal=1,
+                thinc_eps=1.0e-4, thinc_beta=2.3, thinc_normal_threshold=0.25, nphase=0):
+    """cmp_params of the two-phase five-equation model (nphase 0, two entries in gamma/pinf/b/q)
+    or of the N-phase model (nphase 2 or 3, nphase entries)."""
+    nph = nphase if nphase else 2
+    if len(gamma) != nph:
+        raise ValueError("gamma needs %d entries" % nph)
+    if pinf is None:
+        pinf = (0.0,) * nph
     if eos is None:
         eos = 1 if any(p != 0.0 for p in pinf) else 0
     A, B, C, D = eos_coefficients(eos, gamma, pinf, b, q)
     w, lim, poly, _ = SCHEMES[scheme]
     p = cmp_params()
     p.dim = dim
-    p.nstate = 4 + dim
-    for k in range(2):
+    p.nphase = nphase
+    p.nstate = (2 * nphase + dim + 1) if nphase else 4 + dim
+    for k in range(nph):
         p.eos_A[k], p.eos_B[k], p.eos_C[k], p.eos_D[k] = A[k], B[k], C[k], D[k]
     p.id_weno, p.id_limiter, p.id_poly, p.id_bound = w, lim, poly, 0
     p.id_thinc, p.thinc_use_normal = thinc, thinc_use_normal

@@ -40,10 +40,11 @@ class cmp_params(ctypes.Structure):
     _fields_ = [
         ("dim", ctypes.c_int),
         ("nstate", ctypes.c_int),
-        ("eos_A", ctypes.c_double * 2),
-        ("eos_B", ctypes.c_double * 2),
-        ("eos_C", ctypes.c_double * 2),
-        ("eos_D", ctypes.c_double * 2),
+        ("nphase", ctypes.c_int),
+        ("eos_A", ctypes.c_double * 3),
+        ("eos_B", ctypes.c_double * 3),
+        ("eos_C", ctypes.c_double * 3),
+        ("eos_D", ctypes.c_double * 3),
         ("id_weno", ctypes.c_int),
         ("id_limiter", ctypes.c_int),
         ("id_poly", ctypes.c_int),

@@ -146,3 +146,129 @@ def random_smooth(dim, n, seed, periodic=True, max_grid_size=None, scheme="WENO5
     boxes = split_domain(shape, max_grid_size or n)
     return dict(params=params, geom=geom, U=U, boxes=boxes, shape=tuple(reversed(shape)), cfl=0.4,
                 name="random-smooth-%dd" % dim)
+
+
+# ---- the N-phase model (cmp_params.nphase = 2 or 3) --------------------------------------------------
+
+def np_prim_to_cons(params, W):
+    """N-phase primitives [alpha_k rho_k (NP), u.., p, alpha_k (NP)] -> conservative (same shape)."""
+    dim, np_ = params.dim, params.nphase
+    A, B, C, D = params.eos_A, params.eos_B, params.eos_C, params.eos_D
+    arho = [W[k] for k in range(np_)]
+    alpha = [W[np_ + dim + 1 + k] for k in range(np_)]
+    vel = [W[np_ + m] for m in range(dim)]
+    p = W[np_ + dim]
+    AP = sum(np.where(C[k] > 0.0, alpha[k] - C[k] * arho[k], alpha[k]) * A[k] for k in range(np_))
+    BP = sum(np.where(C[k] > 0.0, alpha[k] - C[k] * arho[k], alpha[k]) * B[k] for k in range(np_))
+    rho = sum(arho)
+    rho_e = AP * p + BP + sum(arho[k] * D[k] for k in range(np_))
+    U = np.empty_like(W)
+    for k in range(np_):
+        U[k] = arho[k]
+        U[np_ + dim + 1 + k] = alpha[k]
+    for m in range(dim):
+        U[np_ + m] = rho * vel[m]
+    U[np_ + dim] = rho_e + 0.5 * rho * sum(v * v for v in vel)
+    return U
+
+
+def nphase_from_two_phase(c, nphase=3, split=0.35):
+    """The same flow in the N-phase model: nphase 2 keeps both phases (alpha2 = 1 - alpha1 stored);
+    nphase 3 splits phase 2 into two phases with phase 2's EOS, a fraction `split` of its volume and
+    mass in the second one (identical densities: the mixture is the same material)."""
+    P2 = c["params"]
+    dim = P2.dim
+    U2 = c["U"]
+    W2 = _cons_to_prim_two(P2, U2)
+    a1r1, a2r2 = W2[0], W2[1]
+    a1 = W2[3 + dim]
+    gam = [1.0 + 1.0 / P2.eos_A[k] for k in range(2)]
+    pinf = [P2.eos_B[k] / (P2.eos_A[k] + 1.0) for k in range(2)]
+    if nphase == 2:
+        g, pi = gam, pinf
+        arho = [a1r1, a2r2]
+        alpha = [a1, 1.0 - a1]
+    else:
+        g, pi = [gam[0], gam[1], gam[1]], [pinf[0], pinf[1], pinf[1]]
+        arho = [a1r1, (1.0 - split) * a2r2, split * a2r2]
+        alpha = [a1, (1.0 - split) * (1.0 - a1), split * (1.0 - a1)]
+    scheme = [k for k, v in SCHEME_IDS.items() if v == (P2.id_weno, P2.id_limiter)][0]
+    riemann = "HLLC" if P2.id_riemann == 2 else "HLL"
+    P = make_params(dim, gamma=tuple(g), pinf=tuple(pi), scheme=scheme, riemann=riemann,
+                    source_term=P2.source_term, nphase=nphase)
+    W = np.empty((P.nstate,) + U2.shape[1:])
+    for k in range(nphase):
+        W[k] = arho[k]
+        W[nphase + dim + 1 + k] = alpha[k]
+    for m in range(dim):
+        W[nphase + m] = W2[2 + m]
+    W[nphase + dim] = W2[2 + dim]
+    out = dict(c)
+    out["params"] = P
+    out["U"] = np_prim_to_cons(P, W)
+    return out
+
+
+def _cons_to_prim_two(P, U):
+    dim = P.dim
+    a1r1, a2r2 = U[0], U[1]
+    rho = a1r1 + a2r2
+    vel = [U[2 + m] / rho for m in range(dim)]
+    a1 = U[3 + dim]
+    a2 = 1.0 - a1
+    A, B, C, D = P.eos_A, P.eos_B, P.eos_C, P.eos_D
+    phi1 = np.where(C[0] > 0.0, a1 - C[0] * a1r1, a1)
+    phi2 = np.where(C[1] > 0.0, a2 - C[1] * a2r2, a2)
+    AP = phi1 * A[0] + phi2 * A[1]
+    BP = phi1 * B[0] + phi2 * B[1]
+    rho_e = U[2 + dim] - 0.5 * rho * sum(v * v for v in vel)
+    W = np.empty_like(U)
+    W[0], W[1] = a1r1, a2r2
+    for m in range(dim):
+        W[2 + m] = vel[m]
+    W[2 + dim] = (rho_e - (a1r1 * D[0] + a2r2 * D[1]) - BP) / AP
+    W[3 + dim] = a1
+    return W
+
+
+SCHEME_IDS = {"WENO5": (1, 5), "WENO5B": (1, 6), "MUSCL": (0, 1), "GODUNOV": (0, 0)}
+
+
+def nphase_smooth(dim, n, seed, nphase=3, periodic=True, max_grid_size=None, scheme="WENO5",
+                  riemann="HLLC", gamma=(1.4, 1.67, 1.09), pinf=(0.0, 0.0, 0.0), source_term=0,
+                  amp=0.1):
+    """Seeded smooth N-phase state for parity tests: volume fractions from smooth positive weights
+    (normalised to sum 1), phase densities, velocity and pressure with a few random Fourier modes."""
+    rng = np.random.default_rng(seed)
+    params = make_params(dim, gamma=tuple(gamma[:nphase]), pinf=tuple(pinf[:nphase]), scheme=scheme,
+                         riemann=riemann, source_term=source_term, nphase=nphase)
+    shape = (n,) * dim
+    geom = make_geometry(shape, (0.0,) * dim, (1.0,) * dim, (int(periodic),) * dim)
+    X = _centres(geom, tuple(reversed(shape)))
+
+    def modes():
+        f = np.zeros_like(X[0])
+        for _ in range(3):
+            k = rng.integers(1, 3, size=dim)
+            ph = rng.uniform(0, 2 * np.pi, size=dim)
+            term = np.ones_like(X[0])
+            for d in range(dim):
+                term = term * np.sin(2 * np.pi * k[d] * X[d] + ph[d])
+            f = f + rng.uniform(-1, 1) * term
+        return f / 3.0
+
+    wts = [np.exp(3.0 * modes()) for _ in range(nphase)]
+    tot = sum(wts)
+    alpha = [w / tot for w in wts]
+    pscale = 1.0 + max(pinf[:nphase])
+    W = np.empty((2 * nphase + dim + 1,) + tuple(reversed(shape)))
+    for k in range(nphase):
+        W[k] = alpha[k] * ((1.0 + 2.0 * k) * (1.0 + amp * modes()))
+        W[nphase + dim + 1 + k] = alpha[k]
+    for m in range(dim):
+        W[nphase + m] = 0.5 + amp * modes()
+    W[nphase + dim] = pscale * (1.0 + amp * modes())
+    U = np_prim_to_cons(params, W)
+    boxes = split_domain(shape, max_grid_size or n)
+    return dict(params=params, geom=geom, U=U, boxes=boxes, shape=shape, cfl=0.4,
+                name="nphase-smooth")

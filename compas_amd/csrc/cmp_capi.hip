@@ -340,7 +340,16 @@ int validate(const cmp_params* P, const cmp_geometry* G, int ng, LaunchCfg* lc, 
 {
     if (!P || !G) return CMP_ERR_ARG;
     if (P->dim != 2 && P->dim != 3) return CMP_ERR_ARG;
-    if (G->dim != P->dim || P->nstate != 4 + P->dim) return CMP_ERR_ARG;
+    if (G->dim != P->dim) return CMP_ERR_ARG;
+    if (P->nphase == 0) {
+        if (P->nstate != 4 + P->dim) return CMP_ERR_ARG;
+    } else if (P->nphase == 2 || P->nphase == 3) {
+        if (P->nstate != 2 * P->nphase + P->dim + 1) return CMP_ERR_ARG;
+        // the N-phase path here: Godunov, MUSCL and WENO5 with the consistent limiter, HLL/HLLC
+        if (P->id_thinc || !(P->id_weno == 0 || P->id_limiter == 5)) return CMP_ERR_UNSUPPORTED;
+    } else {
+        return CMP_ERR_ARG;
+    }
     if (P->id_riemann != 1 && P->id_riemann != 2) return CMP_ERR_UNSUPPORTED;
     if (P->source_term != 0 && P->source_term != 1) return CMP_ERR_ARG;
     int scheme = 0;
@@ -349,7 +358,8 @@ int validate(const cmp_params* P, const cmp_geometry* G, int ng, LaunchCfg* lc, 
     if (ng < scheme_ghosts(scheme)) return CMP_ERR_ARG;
     for (int d = 0; d < P->dim; ++d)
         if (!(G->dx[d] > 0.0)) return CMP_ERR_ARG;
-    for (int k = 0; k < 2; ++k)
+    const int nph = P->nphase ? P->nphase : 2;
+    for (int k = 0; k < nph; ++k)
         if (!(P->eos_A[k] > 0.0)) return CMP_ERR_ARG;
     if (P->id_thinc && !(P->thinc_beta > 0.0)) return CMP_ERR_ARG;
     lc->dim = P->dim;
@@ -357,15 +367,16 @@ int validate(const cmp_params* P, const cmp_geometry* G, int ng, LaunchCfg* lc, 
     lc->limiter = P->id_limiter;
     lc->riemann = P->id_riemann;
     lc->thinc = P->id_thinc;
+    lc->nphase = P->nphase;
     std::memset(kp, 0, sizeof(*kp));
     kp->weno5b_weight = (P->id_poly == 2) ? 1.0 / 6.0 : 1.0 / 12.0;
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < nph; ++k) {
         kp->A[k] = P->eos_A[k];
         kp->B[k] = P->eos_B[k];
         kp->C[k] = P->eos_C[k];
         kp->D[k] = P->eos_D[k];
+        if (P->eos_C[k] > 0.0) kp->nasg = 1;
     }
-    kp->nasg = (P->eos_C[0] > 0.0 || P->eos_C[1] > 0.0) ? 1 : 0;
     kp->source_term = P->source_term;
     kp->thinc = P->id_thinc;
     kp->thinc_use_normal = P->thinc_use_normal;
@@ -891,6 +902,7 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     if ((pf->mechanism != CMP_PF_ACDI && pf->mechanism != CMP_PF_CAC) || pf->id_flux != 0)
         return CMP_ERR_UNSUPPORTED;
     if (pf->explicit_rc != 0 && pf->explicit_rc != 1) return CMP_ERR_ARG;
+    if (L->P.nphase != 0) return CMP_ERR_UNSUPPORTED;   // the N-phase Phase-Field flux is not built
     // a mechanism that is not flux based has no Jhat (PhaseField_Parameter.H:138-140 aborts)
     if (pf->mechanism == CMP_PF_CAC && pf->explicit_rc == 1) return CMP_ERR_UNSUPPORTED;
     if (pf->flux_phase2flux != 0 && pf->flux_phase2flux != 1) return CMP_ERR_ARG;

@@ -16,7 +16,7 @@ namespace cmp {
 
 // Kernel-side parameter block (passed by value as a kernel argument).
 struct KParams {
-    double A[2], B[2], C[2], D[2];
+    double A[CMP_MAX_PHASES], B[CMP_MAX_PHASES], C[CMP_MAX_PHASES], D[CMP_MAX_PHASES];
     int source_term;
     int nasg;          // any C_k > 0
     int thinc;

@@ -68,6 +68,7 @@ struct LaunchCfg {
     int limiter;
     int riemann;
     int thinc;
+    int nphase;     // 0: two-phase five-equation model; 2, 3: N-phase model
 };
 
 hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int stage,

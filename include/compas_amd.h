@@ -41,15 +41,23 @@ extern "C" {
  *   0 Interior, 1 Inflow, 2 Outflow, 3 Symmetry, 4 SlipWall, 5 NoSlipWall. */
 
 /* Parameters read by every kernel: the subset of the reference's Parm that this path uses. */
+#define CMP_MAX_PHASES 3
+
 typedef struct cmp_params {
     int    dim;            /* AMREX_SPACEDIM, 2 or 3                                                 */
-    int    nstate;         /* 4 + dim                                                                */
-    /* EOS coefficients of phase 1 and 2 as set by Parm::Initialize (source/Parm.cpp:124-156):
-     *   rho_k e_k = (1 - b_k rho_k)(A_k p + B_k) + rho_k D_k, b_k = C_k (NASG covolume, 0 otherwise) */
-    double eos_A[2];
-    double eos_B[2];
-    double eos_C[2];
-    double eos_D[2];
+    int    nstate;         /* 4 + dim (nphase 0), 2 nphase + dim + 1 (nphase 2 or 3)                  */
+    int    nphase;         /* 0: the two-phase five-equation model (PHYSICS == FIVEEQS), state
+                            * a1rho1 a2rho2 rho_u.. rhoE a1 (Variables_Compressible5Eq.H:21-30);
+                            * 2 or 3: the N-phase model (PHYSICS == FIVEEQS_NPHASE, NPHASE = nphase),
+                            * state alpha_k rho_k.. rho_u.. rhoE alpha_k..
+                            * (Variables_Compressible5Eq_NPhase.H:21-29)                              */
+    /* EOS coefficients per phase as set by Parm::Initialize (source/Parm.cpp:124-156, :187-272):
+     *   rho_k e_k = (1 - b_k rho_k)(A_k p + B_k) + rho_k D_k, b_k = C_k (NASG covolume, 0 otherwise);
+     * the first 2 (nphase 0) or nphase entries are read */
+    double eos_A[CMP_MAX_PHASES];
+    double eos_B[CMP_MAX_PHASES];
+    double eos_C[CMP_MAX_PHASES];
+    double eos_D[CMP_MAX_PHASES];
     int    id_weno;
     int    id_limiter;
     int    id_poly;

@@ -55,6 +55,15 @@ def load():
         "ref_weno5b_side": (None, [PP, PD, ctypes.c_int, PD]),
         "ref_thinc": (D, [D, D, D, ctypes.c_int]),
         "ref_riemann": (None, [PP, PD, PD, ctypes.c_int, PD, PD, PD, PI]),
+        "ref_np_eos_mixture": (D, [PP, D, PD, PD, ctypes.c_int, PI]),
+        "ref_np_cons2prim": (None, [PP, PD, PD, PI]),
+        "ref_np_prim2cons": (None, [PP, PD, PD, PI]),
+        "ref_np_sound_speed": (D, [PP, D, PD, PD, ctypes.c_int, PI]),
+        "ref_np_K": (None, [PP, D, PD, PD, ctypes.c_int, PD, PI]),
+        "ref_np_cons2flux": (None, [PP, PD, ctypes.c_int, PD, PI]),
+        "ref_np_riemann": (None, [PP, PD, PD, ctypes.c_int, PD, PD, PD, PI]),
+        "ref_np_limiter_consistent": (None, [ctypes.c_int, PD, PD]),
+        "ref_np_limit_consistent": (None, [PP, PD, PD, D, ctypes.c_int, PD, PI]),
         "ref_face_flux": (ctypes.c_int, [PP, PG, PB, ctypes.c_int, PD, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, PD, PD, PD]),
         "ref_rhs_box": (ctypes.c_int, [PP, PG, PB, ctypes.c_int, PD, PD, PD,
@@ -154,11 +163,14 @@ def muscl(fm, fc, fp, dx, face, limiter):
     return load().ref_muscl(float(fm), float(fc), float(fp), float(dx), int(face), int(limiter))
 
 
+REF_NMAX = 10   # row length of the oracle's stencil arrays (cpu_ref.h)
+
+
 def weno5b_side(params, W5, face):
     """WENO5B primitive face state of the middle cell of W5 (5 x nstate primitives)."""
-    W = np.zeros((5, 7))
+    W = np.zeros((5, REF_NMAX))
     W[:, :params.nstate] = np.asarray(W5, dtype=np.float64)
-    out = np.zeros(7)
+    out = np.zeros(REF_NMAX)
     load().ref_weno5b_side(ctypes.byref(params), _dptr(W), int(face), _dptr(out))
     return out[:params.nstate]
 
@@ -174,9 +186,79 @@ def riemann(params, UL, UR, direction):
     vel = np.zeros(3)
     cmax = ctypes.c_double(0.0)
     err = ctypes.c_int(0)
-    load().ref_riemann(ctypes.byref(params), _dptr(UL), _dptr(UR), direction, _dptr(F), _dptr(vel),
-                       ctypes.byref(cmax), ctypes.byref(err))
+    fn = load().ref_np_riemann if params.nphase else load().ref_riemann
+    fn(ctypes.byref(params), _dptr(UL), _dptr(UR), direction, _dptr(F), _dptr(vel), ctypes.byref(cmax),
+       ctypes.byref(err))
     return F, vel[:params.dim].copy(), cmax.value
+
+
+# ---- the N-phase model (params.nphase = 2 or 3) ----------------------------------------------------
+
+def np_eos_mixture(params, value, alpha, arho, output_internal_energy):
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    r = np.ascontiguousarray(arho, dtype=np.float64)
+    err = ctypes.c_int(0)
+    return load().ref_np_eos_mixture(ctypes.byref(params), float(value), _dptr(a), _dptr(r),
+                                     int(output_internal_energy), ctypes.byref(err))
+
+
+def np_cons2prim(params, U):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    W = np.empty_like(U)
+    err = ctypes.c_int(0)
+    load().ref_np_cons2prim(ctypes.byref(params), _dptr(U), _dptr(W), ctypes.byref(err))
+    return W
+
+
+def np_prim2cons(params, W):
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    U = np.empty_like(W)
+    err = ctypes.c_int(0)
+    load().ref_np_prim2cons(ctypes.byref(params), _dptr(W), _dptr(U), ctypes.byref(err))
+    return U
+
+
+def np_sound_speed(params, p, alpha, arho, id_k):
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    r = np.ascontiguousarray(arho, dtype=np.float64)
+    err = ctypes.c_int(0)
+    return load().ref_np_sound_speed(ctypes.byref(params), float(p), _dptr(a), _dptr(r), int(id_k),
+                                     ctypes.byref(err))
+
+
+def np_K(params, p, alpha, arho, id_k):
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    r = np.ascontiguousarray(arho, dtype=np.float64)
+    K = np.zeros(params.nphase)
+    err = ctypes.c_int(0)
+    load().ref_np_K(ctypes.byref(params), float(p), _dptr(a), _dptr(r), int(id_k), _dptr(K),
+                    ctypes.byref(err))
+    return K
+
+
+def np_cons2flux(params, U, direction):
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    F = np.empty_like(U)
+    err = ctypes.c_int(0)
+    load().ref_np_cons2flux(ctypes.byref(params), _dptr(U), int(direction), _dptr(F), ctypes.byref(err))
+    return F
+
+
+def np_limiter_consistent(slopes):
+    s = np.ascontiguousarray(slopes, dtype=np.float64)
+    th = np.zeros(len(s))
+    load().ref_np_limiter_consistent(len(s), _dptr(s), _dptr(th))
+    return th
+
+
+def np_limit_consistent(params, Uface, Ucell, dx, face):
+    Uf = np.ascontiguousarray(Uface, dtype=np.float64)
+    Uc = np.ascontiguousarray(Ucell, dtype=np.float64)
+    out = np.empty_like(Uf)
+    err = ctypes.c_int(0)
+    load().ref_np_limit_consistent(ctypes.byref(params), _dptr(Uf), _dptr(Uc), float(dx), int(face),
+                                   _dptr(out), ctypes.byref(err))
+    return out
 
 
 # ---- box and level wrappers ----------------------------------------------------------------------

@@ -403,6 +403,277 @@ void ref_riemann(const cmp_params* P, const double* UL, const double* UR, int di
 }
 
 /* ------------------------------------------------------------------------------------------------
+ * The N-phase five-equation model (PHYSICS == FIVEEQS_NPHASE, NPHASE = P->nphase): state
+ * alpha_k rho_k (k < NP), rho u_d, rho E, alpha_k (k < NP), N = 2 NP + dim + 1
+ * (include/Physics_Compressible5Eq_NPhase/Variables_Compressible5Eq_NPhase.H:21-29,
+ *  Component2Vector_Compressible5Eq_NPhase.H); primitives alpha_k rho_k, u_d, p, alpha_k.
+ * ---------------------------------------------------------------------------------------------- */
+
+/* EquationOfState_Mixture, pressure-based branch
+ * (include/Physics_Compressible5Eq_NPhase/EquationOfState_Mixture_Compressible5Eq_NPhase.H:15-67):
+ * every phase's own alpha_k enters AP and BP. */
+double ref_np_eos_mixture(const cmp_params* P, double input, const double* alpha, const double* arho,
+                          int output_internal_energy, int* err)
+{
+    double rho_D = 0.0, AP = 0.0, BP = 0.0;
+    for (int k = 0; k < P->nphase; ++k) rho_D += arho[k] * P->eos_D[k];
+    for (int k = 0; k < P->nphase; ++k) {
+        double phi = ref_nasg_phi(alpha[k], arho[k], P->eos_C[k], err);
+        AP += phi * P->eos_A[k];
+        BP += phi * P->eos_B[k];
+    }
+    if (output_internal_energy) return AP * input + BP + rho_D;
+    return (input - rho_D - BP) / AP;
+}
+
+/* Conservative2Primitive (…Conservative2Primitive_Compressible5Eq_NPhase.H:16-60) */
+void ref_np_cons2prim(const cmp_params* P, const double* U, double* W, int* err)
+{
+    const int d = P->dim, np = P->nphase;
+    double rho = 0.0;
+    for (int k = 0; k < np; ++k) rho += U[k];
+    double vel[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < d; ++m) vel[m] = U[np + m] / rho;
+    double ke = (d == 3) ? (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2])
+                         : (vel[0] * vel[0] + vel[1] * vel[1]);
+    double rho_e = U[np + d] - (0.5) * rho * ke;
+    double p = ref_np_eos_mixture(P, rho_e, U + np + d + 1, U, 0, err);
+    for (int k = 0; k < np; ++k) { W[k] = U[k]; W[np + d + 1 + k] = U[np + d + 1 + k]; }
+    for (int m = 0; m < d; ++m) W[np + m] = vel[m];
+    W[np + d] = p;
+}
+
+/* Primitive2Conservative (…Primitive2Conservative_Compressible5Eq_NPhase.H:15-62) */
+void ref_np_prim2cons(const cmp_params* P, const double* W, double* U, int* err)
+{
+    const int d = P->dim, np = P->nphase;
+    double rho = 0.0;
+    for (int k = 0; k < np; ++k) rho += W[k];
+    double mom[3];
+    for (int m = 0; m < d; ++m) mom[m] = rho * W[np + m];
+    double rho_e = ref_np_eos_mixture(P, W[np + d], W + np + d + 1, W, 1, err);
+    double ke = (d == 3) ? (W[np] * W[np] + W[np + 1] * W[np + 1] + W[np + 2] * W[np + 2])
+                         : (W[np] * W[np] + W[np + 1] * W[np + 1]);
+    double rE = rho_e + (0.5) * rho * ke;
+    for (int k = 0; k < np; ++k) { U[k] = W[k]; U[np + d + 1 + k] = W[np + d + 1 + k]; }
+    for (int m = 0; m < d; ++m) U[np + m] = mom[m];
+    U[np + d] = rE;
+}
+
+/* rho_c2_Mixture (…rho_c2_Mixture_Compressible5Eq_NPhase.H:15-52) of the phase values rc[k] =
+ * rho_c2_Phase(p, A_k, B_k) */
+static double ref_np_rho_c2_mixture(const cmp_params* P, const double* rc, const double* alpha,
+                                    const double* arho, int id_k, int* err)
+{
+    if (id_k == 0) {
+        double APrc = 0.0, AP = 0.0;
+        for (int k = 0; k < P->nphase; ++k) {
+            double phi = ref_nasg_phi(alpha[k], arho[k], P->eos_C[k], err);
+            APrc += alpha[k] * P->eos_A[k] * rc[k];
+            AP += phi * P->eos_A[k];
+        }
+        return APrc / AP;
+    }
+    double inv = 0.0;
+    for (int k = 0; k < P->nphase; ++k) {
+        double phi = ref_nasg_phi(alpha[k], arho[k], P->eos_C[k], err);
+        inv += phi / rc[k];
+    }
+    return 1.0 / inv;
+}
+
+/* SoundSpeed (…SoundSpeed_Compressible5Eq_NPhase.H:15-52) */
+double ref_np_sound_speed(const cmp_params* P, double p, const double* alpha, const double* arho,
+                          int id_k, int* err)
+{
+    double rc[CMP_MAX_PHASES];
+    for (int k = 0; k < P->nphase; ++k) rc[k] = ref_rho_c2_phase(p, P->eos_A[k], P->eos_B[k]);
+    double rho_c2 = ref_np_rho_c2_mixture(P, rc, alpha, arho, id_k, err);
+    double rho = 0.0;
+    for (int k = 0; k < P->nphase; ++k) rho += arho[k];
+    return sqrt(rho_c2 / rho);
+}
+
+/* NonConservative (…NonConservative_Compressible5Eq_NPhase.H:15-58): K_k */
+void ref_np_K(const cmp_params* P, double p, const double* alpha, const double* arho, int id_k,
+              double* K, int* err)
+{
+    if (id_k == 0) {
+        for (int k = 0; k < P->nphase; ++k) K[k] = 0.0;
+        return;
+    }
+    double rc[CMP_MAX_PHASES];
+    for (int k = 0; k < P->nphase; ++k) rc[k] = ref_rho_c2_phase(p, P->eos_A[k], P->eos_B[k]);
+    double rho_c2 = ref_np_rho_c2_mixture(P, rc, alpha, arho, id_k, err);
+    for (int k = 0; k < P->nphase; ++k) K[k] = (rho_c2 / rc[k] - 1.0);
+}
+
+/* Conservative2Flux (…Conservative2Flux_Compressible5Eq_NPhase.H:15-80) */
+void ref_np_cons2flux(const cmp_params* P, const double* U, int dir, double* F, int* err)
+{
+    const int d = P->dim, np = P->nphase;
+    double W[REF_NMAX];
+    ref_np_cons2prim(P, U, W, err);
+    double p = W[np + d], V = W[np + dir];
+    for (int k = 0; k < np; ++k) {
+        F[k] = W[k] * V;
+        F[np + d + 1 + k] = W[np + d + 1 + k] * V;
+    }
+    for (int m = 0; m < d; ++m) F[np + m] = U[np + m] * V;
+    F[np + d] = (U[np + d] + p) * V;
+    F[np + dir] += p;
+}
+
+/* Riemann_HLLC (…Riemann_HLLC_Compressible5Eq_NPhase.H:16-162) and Riemann_HLL
+ * (…Riemann_HLL_Compressible5Eq_NPhase.H) */
+void ref_np_riemann(const cmp_params* P, const double* UL, const double* UR, int dir, double* flux,
+                    double* vel, double* cmax, int* err)
+{
+    const int d = P->dim, N = P->nstate, np = P->nphase, id_k = P->source_term;
+    double WL[REF_NMAX], WR[REF_NMAX], FL[REF_NMAX], FR[REF_NMAX];
+    ref_np_cons2prim(P, UL, WL, err);
+    ref_np_cons2prim(P, UR, WR, err);
+    ref_np_cons2flux(P, UL, dir, FL, err);
+    ref_np_cons2flux(P, UR, dir, FR, err);
+    double pL = WL[np + d], pR = WR[np + d];
+    double cL = ref_np_sound_speed(P, pL, WL + np + d + 1, WL, id_k, err);
+    double cR = ref_np_sound_speed(P, pR, WR + np + d + 1, WR, id_k, err);
+    double VL = WL[np + dir], VR = WR[np + dir];
+    if (P->id_riemann == 2) {
+        double V_avg = 0.5 * (VL + VR);
+        double c_avg = 0.5 * (cL + cR);
+        double SL = smin(V_avg - c_avg, VL - cL);
+        double SR = smax(V_avg + c_avg, VR + cR);
+        double Sm = smin(0.0, SL);
+        double Sp = smax(0.0, SR);
+        double rhoL = 0.0, rhoR = 0.0;
+        for (int k = 0; k < np; ++k) { rhoL += WL[k]; rhoR += WR[k]; }
+        double S_Star = (pR - pL + rhoL * VL * (SL - VL) - rhoR * VR * (SR - VR))
+                      / (rhoL * (SL - VL) - rhoR * (SR - VR));
+        double s = ref_sign(S_Star);
+        for (int m = 0; m < d; ++m)
+            vel[m] = 0.5 * (1.0 + s) * WL[np + m] + 0.5 * (1.0 - s) * WR[np + m];
+        vel[dir] += 0.5 * (1.0 + s) * Sm * ((SL - VL) / (SL - S_Star) - 1.0)
+                  + 0.5 * (1.0 - s) * Sp * ((SR - VR) / (SR - S_Star) - 1.0);
+        for (int n = 0; n < N; ++n) {
+            double CLs = (SL - VL) / (SL - S_Star) * UL[n];
+            double CRs = (SR - VR) / (SR - S_Star) * UR[n];
+            if (n == np + dir) {
+                CLs += (SL - VL) / (SL - S_Star) * rhoL * (S_Star - VL);
+                CRs += (SR - VR) / (SR - S_Star) * rhoR * (S_Star - VR);
+            } else if (n == np + d) {
+                CLs += (SL - VL) / (SL - S_Star) * (S_Star - VL) * (rhoL * S_Star + pL / (SL - VL));
+                CRs += (SR - VR) / (SR - S_Star) * (S_Star - VR) * (rhoR * S_Star + pR / (SR - VR));
+            }
+            flux[n] = 0.5 * (1.0 + s) * (FL[n] + Sm * (CLs - UL[n]))
+                    + 0.5 * (1.0 - s) * (FR[n] + Sp * (CRs - UR[n]));
+        }
+    } else {
+        double Sp = smax(smax(0.0, VL + cL), VR + cR);
+        double Sm = smin(smin(0.0, VL - cL), VR - cR);
+        for (int m = 0; m < d; ++m)
+            vel[m] = (Sp * WL[np + m] - Sm * WR[np + m]) / (Sp - Sm);
+        for (int n = 0; n < N; ++n)
+            flux[n] = (Sp * FL[n] - Sm * FR[n] + Sp * Sm * (UR[n] - UL[n])) / (Sp - Sm);
+    }
+    double qL = 0.0, qR = 0.0;
+    for (int m = 0; m < d; ++m) {
+        qL = (m == 0) ? WL[np] * WL[np] : qL + WL[np + m] * WL[np + m];
+        qR = (m == 0) ? WR[np] * WR[np] : qR + WR[np + m] * WR[np + m];
+    }
+    *cmax = smax(cL + sqrt(qL), cR + sqrt(qR));
+}
+
+/* Limiter_Consistent (include/Operators_FV/Limiter_Consistent.H:12-39): the first phase of largest
+ * |slope| (std::max_element) takes theta = clip((s_max - S)/s_max), the others clip(s_max/(s_max - S)),
+ * S the sum of the slopes; theta = 1 for all when |s_max| or |S| is below 1e-10. */
+void ref_np_limiter_consistent(int np, const double* slope, double* theta)
+{
+    const double ep = 1.0e-10;
+    double S = 0.0;
+    int imax = 0;
+    for (int k = 0; k < np; ++k) S += slope[k];
+    for (int k = 1; k < np; ++k)
+        if (fabs(slope[imax]) < fabs(slope[k])) imax = k;
+    double smx = slope[imax];
+    if (fabs(smx) < ep || fabs(S) < ep) {
+        for (int k = 0; k < np; ++k) theta[k] = 1.0;
+        return;
+    }
+    double smS = smx - S;
+    for (int k = 0; k < np; ++k) theta[k] = smax(smin(smx / smS, 1.0), 0.0);
+    theta[imax] = smax(smin(smS / smx, 1.0), 0.0);
+}
+
+/* FVM_Reconstruction2Face_Conservative_LimiterConsistent (…LimiterConsistent_Compressible5Eq_NPhase.H:15-73)
+ * with FVM_Reconstruction2Face_LimiterConsistent (…:15-68) and Reconstruction2Face_Linear: the face
+ * state Uf (unlimited, conservative) and the cell state Uc -> the limited conservative face state. */
+void ref_np_limit_consistent(const cmp_params* P, const double* Uf, const double* Uc, double dx,
+                             int id_face, double* Uo, int* err)
+{
+    const int d = P->dim, np = P->nphase, ia = np + d + 1;
+    double Wf[REF_NMAX], Wc[REF_NMAX];
+    ref_np_cons2prim(P, Uf, Wf, err);
+    ref_np_cons2prim(P, Uc, Wc, err);
+    double sr[CMP_MAX_PHASES], sa[CMP_MAX_PHASES], th[CMP_MAX_PHASES];
+    for (int k = 0; k < np; ++k) {
+        sr[k] = (Wf[k] - Wc[k]) / ((double)id_face * 0.5 * dx);
+        sa[k] = (Wf[ia + k] - Wc[ia + k]) / ((double)id_face * 0.5 * dx);
+    }
+    ref_np_limiter_consistent(np, sa, th);
+    for (int k = 0; k < np; ++k) {
+        double slope = th[k] * sr[k];
+        Wf[k] = Wc[k] + (double)id_face * 0.5 * slope * dx;
+        slope = th[k] * sa[k];
+        Wf[ia + k] = Wc[ia + k] + (double)id_face * 0.5 * slope * dx;
+    }
+    ref_np_prim2cons(P, Wf, Uo, err);
+}
+
+/* FVM_Reconstruction2FaceLR_Conservative_K of the N-phase model
+ * (…FVM_Reconstruction2FaceLR_Conservative_K_Compressible5Eq_NPhase.H) with the per-side
+ * FVM_Reconstruction2Face_Conservative_{WENO5,MUSCL}_D (…WENO5_D_Compressible5Eq_NPhase.H:15-73,
+ * …MUSCL_D_…:15-74: scalar reconstruction of every primitive, prim->cons, the consistent limiter)
+ * and …Godunov_D (cons->prim->cons, no limiter). Ust as in ref_recon_lr. */
+static int ref_np_recon_lr(const cmp_params* P, double Ust[6][REF_NMAX], double dx_dir, double* UL,
+                           double* UR, int* err)
+{
+    const int N = P->nstate;
+    double Wst[6][REF_NMAX], WLf[REF_NMAX], WRf[REF_NMAX], Uu[REF_NMAX];
+    if (P->id_weno == 1 && P->id_limiter == 5) {
+        for (int s = -3; s <= 1; ++s) ref_np_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        for (int n = 0; n < N; ++n)
+            WLf[n] = ref_weno5(Wst[0][n], Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], 1);
+        ref_np_prim2cons(P, WLf, Uu, err);
+        ref_np_limit_consistent(P, Uu, Ust[2], dx_dir, 1, UL, err);
+        for (int s = -2; s <= 2; ++s) ref_np_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        for (int n = 0; n < N; ++n)
+            WRf[n] = ref_weno5(Wst[1][n], Wst[2][n], Wst[3][n], Wst[4][n], Wst[5][n], -1);
+        ref_np_prim2cons(P, WRf, Uu, err);
+        ref_np_limit_consistent(P, Uu, Ust[3], dx_dir, -1, UR, err);
+    } else if (P->id_weno == 0 && P->id_limiter >= 1 && P->id_limiter <= 4) {
+        for (int s = -2; s <= 0; ++s) ref_np_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        for (int n = 0; n < N; ++n)
+            WLf[n] = ref_muscl(Wst[1][n], Wst[2][n], Wst[3][n], dx_dir, 1, P->id_limiter);
+        ref_np_prim2cons(P, WLf, Uu, err);
+        ref_np_limit_consistent(P, Uu, Ust[2], dx_dir, 1, UL, err);
+        for (int s = -1; s <= 1; ++s) ref_np_cons2prim(P, Ust[s + 3], Wst[s + 3], err);
+        for (int n = 0; n < N; ++n)
+            WRf[n] = ref_muscl(Wst[2][n], Wst[3][n], Wst[4][n], dx_dir, -1, P->id_limiter);
+        ref_np_prim2cons(P, WRf, Uu, err);
+        ref_np_limit_consistent(P, Uu, Ust[3], dx_dir, -1, UR, err);
+    } else if (P->id_weno == 0 && P->id_limiter == 0) {
+        ref_np_cons2prim(P, Ust[2], WLf, err);
+        ref_np_prim2cons(P, WLf, UL, err);
+        ref_np_cons2prim(P, Ust[3], WRf, err);
+        ref_np_prim2cons(P, WRf, UR, err);
+    } else {
+        return CMP_ERR_UNSUPPORTED;
+    }
+    return CMP_OK;
+}
+
+/* ------------------------------------------------------------------------------------------------
  * Fab access
  * ---------------------------------------------------------------------------------------------- */
 
@@ -587,6 +858,17 @@ int ref_face_flux(const cmp_params* P, const cmp_geometry* G, const cmp_box* box
     }
     double UL[REF_NMAX], UR[REF_NMAX];
     int r;
+    if (P->nphase) {
+        /* FVM_Conservative2FluxHyperbolic_K of the N-phase model
+         * (…FVM_Conservative2FluxHyperbolic_K_Compressible5Eq_NPhase.H:16-150), ID_Bound == 0 */
+        if (P->id_thinc) return CMP_ERR_UNSUPPORTED;
+        r = ref_np_recon_lr(P, Ust, dx_dir, UL, UR, &err);
+        if (r) return r;
+        double vel[3];
+        ref_np_riemann(P, UL, UR, dir, flux, vel, cmax, &err);
+        *fluxnc = vel[dir];
+        return err;
+    }
     if (P->id_thinc) {
         double nrm[3];
         ref_face_normal(P, G, &f, U, i, j, k, dir, nrm);
@@ -715,6 +997,28 @@ int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, 
                 for (int n = 0; n < N; ++n) Uc[n] = U[c + n * f.sn];
                 int le = 0;
                 int id_k = P->source_term;
+                if (P->nphase) {
+                    /* …FVM_SurfaceIntegralNC_Compressible5Eq_NPhase.H:16-56: every alpha_k */
+                    const int np = P->nphase, ia = np + dim + 1;
+                    double Kn[CMP_MAX_PHASES];
+                    double Wc[REF_NMAX];
+                    ref_np_cons2prim(P, Uc, Wc, &le);
+                    ref_np_K(P, Wc[np + dim], Uc + ia, Uc, id_k, Kn, &le);
+                    double divu = 0.0;
+                    for (int d = 0; d < dim; ++d) {
+                        long long qm = idx[0] * sf[d][0] + idx[1] * sf[d][1] + idx[2] * sf[d][2];
+                        long long qp = qm + sf[d][d];
+                        double term = (FN[d][qp] - FN[d][qm]) / G->dx[d];
+                        divu = (d == 0) ? term : divu + term;
+                    }
+                    for (int q = 0; q < np; ++q) {
+                        double ai = Uc[ia + q];
+                        double phii = (id_k == 1) ? ref_nasg_phi(ai, Uc[q], P->eos_C[q], &le) : ai;
+                        dUdt[cell + (ia + q) * ncell] += phii * (1.0 + Kn[q]) * divu;
+                    }
+                    if (le > e2) e2 = le;
+                    continue;
+                }
                 double a1 = Uc[iA];
                 double phi1 = (id_k == 1) ? ref_nasg_phi(a1, Uc[0], P->eos_C[0], &le) : a1;
                 double K[2] = {0.0, 0.0};
@@ -759,7 +1063,7 @@ static int box_contains(int dim, const cmp_box* b, const int* c)
 /* pf_bcs != 0: the Phase-Field fields' rule (PF_bcs, LS_bcs and Normal_bcs of
  * source/Class_Compressible_PhaseField.cpp:471-488, 1516-1530, 1970-1986): int_dir on periodic
  * sides, foextrap on every other side, no sign changes. */
-static int fill_level(int dim, int N, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+static int fill_level(int dim, int N, int mom0, const cmp_geometry* G, int nbox, const cmp_box* boxes,
                       int ng, double* U, int pf_bcs)
 {
     int err = 0;
@@ -798,7 +1102,7 @@ static int fill_level(int dim, int N, const cmp_geometry* G, int nbox, const cmp
                             /* normal momentum odd; tangential odd only for NoSlipWall */
                             for (int m = 0; m < dim; ++m) {
                                 int odd = (m == d) || (code == 5);
-                                if (odd) sgn[2 + m] = -sgn[2 + m];
+                                if (odd) sgn[mom0 + m] = -sgn[mom0 + m];
                             }
                         } else {
                             ok = 0;
@@ -823,13 +1127,13 @@ static int fill_level(int dim, int N, const cmp_geometry* G, int nbox, const cmp
 int ref_level_fill_ghosts(const cmp_params* P, const cmp_geometry* G, int nbox,
                           const cmp_box* boxes, int ng, double* U)
 {
-    return fill_level(P->dim, P->nstate, G, nbox, boxes, ng, U, 0);
+    return fill_level(P->dim, P->nstate, P->nphase ? P->nphase : 2, G, nbox, boxes, ng, U, 0);
 }
 
 int ref_level_fill_pf(int dim, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
                       int ncomp, double* A)
 {
-    return fill_level(dim, ncomp, G, nbox, boxes, ng, A, 1);
+    return fill_level(dim, ncomp, 0, G, nbox, boxes, ng, A, 1);
 }
 
 /* ------------------------------------------------------------------------------------------------

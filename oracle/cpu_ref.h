@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-#define REF_NMAX 7
+#define REF_NMAX 10
 
 /* pointwise physics */
 double ref_nasg_phi(double alpha, double alpha_rho, double b, int* err);
@@ -32,6 +32,22 @@ double ref_sound_speed(const cmp_params* P, double p, double a1, double a1r1, do
 void   ref_nonconservative_K(const cmp_params* P, double p, double a1, double a1r1, double a2r2,
                              int id_k, double* K, int* err);
 void   ref_cons2flux(const cmp_params* P, const double* U, int dir, double* F, int* err);
+
+/* the N-phase model (P->nphase = 2 or 3): state alpha_k rho_k, rho u_d, rho E, alpha_k */
+double ref_np_eos_mixture(const cmp_params* P, double input, const double* alpha, const double* arho,
+                          int output_internal_energy, int* err);
+void   ref_np_cons2prim(const cmp_params* P, const double* U, double* W, int* err);
+void   ref_np_prim2cons(const cmp_params* P, const double* W, double* U, int* err);
+double ref_np_sound_speed(const cmp_params* P, double p, const double* alpha, const double* arho,
+                          int id_k, int* err);
+void   ref_np_K(const cmp_params* P, double p, const double* alpha, const double* arho, int id_k,
+                double* K, int* err);
+void   ref_np_cons2flux(const cmp_params* P, const double* U, int dir, double* F, int* err);
+void   ref_np_riemann(const cmp_params* P, const double* UL, const double* UR, int dir, double* flux,
+                      double* vel, double* cmax, int* err);
+void   ref_np_limiter_consistent(int np, const double* slope, double* theta);
+void   ref_np_limit_consistent(const cmp_params* P, const double* Uf, const double* Uc, double dx,
+                               int id_face, double* Uo, int* err);
 
 /* scalar reconstruction */
 double ref_weno5(double f_mm, double f_m, double f_c, double f_p, double f_pp, int id_face);
