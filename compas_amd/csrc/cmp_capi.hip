@@ -524,8 +524,11 @@ int cmp_rhs_box(const cmp_params* params, const cmp_geometry* geom, const cmp_bo
                          dUdt, nullptr, (unsigned long long*)cmax, nullptr, s));
     double* fx[3] = {flux_x, flux_y, flux_z};
     double* fn[3] = {fluxnc_x, fluxnc_y, fluxnc_z};
-    for (int d = 0; d < dim; ++d)
-        if (fx[d] || fn[d]) CMP_TRY(launch_faces(lc, kp, bd, d, U, fx[d], fn[d], s));
+    for (int d = 0; d < dim; ++d) {
+        if (!fx[d] && !fn[d]) continue;
+        if (lc.nphase) return CMP_ERR_UNSUPPORTED;   // face arrays of the N-phase model are not built
+        CMP_TRY(launch_faces(lc, kp, bd, d, U, fx[d], fn[d], s));
+    }
     return CMP_OK;
 }
 

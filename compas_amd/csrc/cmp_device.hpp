@@ -576,6 +576,323 @@ __device__ __forceinline__ double face_flux(const KParams& kp, int limiter, int 
     return face_flux_decided<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, dir, s, aL, aR, F, cmax, bad);
 }
 
+// ---- the N-phase model (include/Physics_Compressible5Eq_NPhase/, Operators_Compressible5Eq_NPhase/) ---
+// State alpha_k rho_k (k < NP), rho u_d, rho E, alpha_k (k < NP); primitives alpha_k rho_k, u_d, p,
+// alpha_k (Variables_Compressible5Eq_NPhase.H:21-29). Every phase's own alpha_k enters the mixture.
+template <int DIM, int NP>
+struct NpIdx {
+    static constexpr int N = 2 * NP + DIM + 1;
+    static constexpr int MOM = NP;            // first momentum component
+    static constexpr int E = NP + DIM;        // energy (conservative) / pressure (primitive)
+    static constexpr int A = NP + DIM + 1;    // first volume fraction
+};
+
+// Sums of EquationOfState_Mixture (…EquationOfState_Mixture_Compressible5Eq_NPhase.H:15-67).
+template <int NP>
+__device__ __forceinline__ void np_mix(const KParams& kp, const double* alpha, const double* arho,
+                                       double& AP, double& BP, double& rho_D, int& bad)
+{
+    rho_D = arho[0] * kp.D[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k) rho_D += arho[k] * kp.D[k];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const double phi = kp.nasg ? nasg_phi(alpha[k], arho[k], kp.C[k], bad) : alpha[k];
+        AP = (k == 0) ? phi * kp.A[k] : AP + phi * kp.A[k];
+        BP = (k == 0) ? phi * kp.B[k] : BP + phi * kp.B[k];
+    }
+}
+
+// Conservative2Primitive (…Conservative2Primitive_Compressible5Eq_NPhase.H:16-60), 1/rho and 1/AP
+// from one division.
+template <int DIM, int NP>
+__device__ __forceinline__ void np_cons2prim(const KParams& kp, const double* U, double* W, int& bad)
+{
+    using I = NpIdx<DIM, NP>;
+    double rho = U[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k) rho += U[k];
+    double AP, BP, rho_D;
+    np_mix<NP>(kp, U + I::A, U, AP, BP, rho_D, bad);
+    const double inv = 1.0 / (rho * AP);
+    const double ir = AP * inv;
+    double ke = 0.0;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+        const double v = U[I::MOM + m] * ir;
+        W[I::MOM + m] = v;
+        ke = (m == 0) ? v * v : ke + v * v;
+    }
+    const double rho_e = U[I::E] - (0.5 * rho) * ke;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        W[k] = U[k];
+        W[I::A + k] = U[I::A + k];
+    }
+    W[I::E] = (rho_e - rho_D - BP) * (rho * inv);
+}
+
+// Primitive2Conservative (…Primitive2Conservative_Compressible5Eq_NPhase.H:15-62); AP, BP, rho out.
+template <int DIM, int NP>
+__device__ __forceinline__ void np_prim2cons(const KParams& kp, const double* W, double* U, double& AP,
+                                             double& BP, double& rho, int& bad)
+{
+    using I = NpIdx<DIM, NP>;
+    rho = W[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k) rho += W[k];
+    double ke = 0.0;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+        U[I::MOM + m] = rho * W[I::MOM + m];
+        ke = (m == 0) ? W[I::MOM] * W[I::MOM] : ke + W[I::MOM + m] * W[I::MOM + m];
+    }
+    double rho_D;
+    np_mix<NP>(kp, W + I::A, W, AP, BP, rho_D, bad);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        U[k] = W[k];
+        U[I::A + k] = W[I::A + k];
+    }
+    U[I::E] = (AP * W[I::E] + BP + rho_D) + (0.5 * rho) * ke;
+}
+
+// SoundSpeed with rho_c2_Phase / rho_c2_Mixture (…SoundSpeed_Compressible5Eq_NPhase.H:15-52,
+// …rho_c2_Mixture_Compressible5Eq_NPhase.H:15-52): K = 0 rho c^2 = sum_k alpha_k A_k rho_k c_k^2 / AP
+// with A_k rho_k c_k^2 = (A_k + 1) p + B_k; K = 1 Wood's 1/sum_k phi_k/(rho_k c_k^2).
+template <int NP>
+__device__ __forceinline__ double np_sound_speed(const KParams& kp, double p, const double* alpha,
+                                                 const double* arho, double rho, double AP, int& bad)
+{
+    if (kp.source_term == 0) {
+        double s = alpha[0] * ((kp.A[0] + 1.0) * p + kp.B[0]);
+#pragma unroll
+        for (int k = 1; k < NP; ++k) s += alpha[k] * ((kp.A[k] + 1.0) * p + kp.B[k]);
+        return sqrt(s / (AP * rho));
+    }
+    double inv = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const double phi = kp.nasg ? nasg_phi(alpha[k], arho[k], kp.C[k], bad) : alpha[k];
+        inv += phi * kp.A[k] / ((kp.A[k] + 1.0) * p + kp.B[k]);
+    }
+    return sqrt(1.0 / (inv * rho));
+}
+
+// Kapila coefficients phi_k (1 + K_k) of the alpha_k equations (FVM_SurfaceIntegral_NC,
+// …FVM_SurfaceIntegralNC_Compressible5Eq_NPhase.H:16-56; NonConservative …NPhase.H:15-58).
+template <int DIM, int NP>
+__device__ __forceinline__ void np_nc_coefficients(const KParams& kp, const double* U, double* f, int& bad)
+{
+    using I = NpIdx<DIM, NP>;
+    if (kp.source_term == 0) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) f[k] = U[I::A + k];
+        return;
+    }
+    double W[I::N];
+    np_cons2prim<DIM, NP>(kp, U, W, bad);
+    const double p = W[I::E];
+    double g[NP], inv = 0.0, phi[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        g[k] = ((kp.A[k] + 1.0) * p + kp.B[k]) / kp.A[k];   // rho_k c_k^2
+        phi[k] = kp.nasg ? nasg_phi(U[I::A + k], U[k], kp.C[k], bad) : U[I::A + k];
+        inv += phi[k] / g[k];
+    }
+    const double rc2 = 1.0 / inv;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) f[k] = phi[k] * (1.0 + (rc2 / g[k] - 1.0));
+}
+
+// Limiter_Consistent (include/Operators_FV/Limiter_Consistent.H:12-39).
+template <int NP>
+__device__ __forceinline__ void np_limiter_consistent(const double* s, double* th)
+{
+    double S = s[0];
+    int imax = 0;
+#pragma unroll
+    for (int k = 1; k < NP; ++k) {
+        S += s[k];
+        if (fabs(s[imax]) < fabs(s[k])) imax = k;
+    }
+    double smx = s[0];
+#pragma unroll
+    for (int k = 1; k < NP; ++k)
+        if (k == imax) smx = s[k];
+    if (fabs(smx) < 1.0e-10 || fabs(S) < 1.0e-10) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) th[k] = 1.0;
+        return;
+    }
+    const double smS = smx - S;
+    const double to = smax(smin(smx / smS, 1.0), 0.0);
+    const double tm = smax(smin(smS / smx, 1.0), 0.0);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) th[k] = (k == imax) ? tm : to;
+}
+
+// FVM_Reconstruction2Face_Conservative_LimiterConsistent (…LimiterConsistent_Compressible5Eq_NPhase.H:15-73)
+// on a primitive face state W of the side whose cell primitives are c(n): prim->cons->prim as the
+// reference takes it, the alpha_k rho_k and alpha_k slopes to the face, theta from the alpha slopes,
+// the linear face values with theta times the slope.
+template <int DIM, int NP, typename C>
+__device__ __forceinline__ void np_limit_face(const KParams& kp, double* W, const C& c, double dx, int face,
+                                              int& bad)
+{
+    using I = NpIdx<DIM, NP>;
+    double U[I::N], AP, BP, rho;
+    np_prim2cons<DIM, NP>(kp, W, U, AP, BP, rho, bad);
+    np_cons2prim<DIM, NP>(kp, U, W, bad);
+    const double h = double(face) * 0.5 * dx;
+    double sr[NP], sa[NP], th[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        sr[k] = (W[k] - c(k)) / h;
+        sa[k] = (W[I::A + k] - c(I::A + k)) / h;
+    }
+    np_limiter_consistent<NP>(sa, th);
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        W[k] = c(k) + double(face) * 0.5 * (th[k] * sr[k]) * dx;
+        W[I::A + k] = c(I::A + k) + double(face) * 0.5 * (th[k] * sa[k]) * dx;
+    }
+}
+
+template <int DIM, int NP>
+struct FaceSideN {
+    double W[NpIdx<DIM, NP>::N];
+    double U[NpIdx<DIM, NP>::N];
+    double rho, AP, BP, c;
+};
+
+template <int DIM, int NP>
+__device__ __forceinline__ void np_make_side(const KParams& kp, FaceSideN<DIM, NP>& s, int& bad)
+{
+    using I = NpIdx<DIM, NP>;
+    np_prim2cons<DIM, NP>(kp, s.W, s.U, s.AP, s.BP, s.rho, bad);
+    s.c = np_sound_speed<NP>(kp, s.W[I::E], s.W + I::A, s.W, s.rho, s.AP, bad);
+}
+
+// One side's HLLC flux (Riemann_HLLC …NPhase.H:16-162) with the star state of the side S* picks.
+template <int DIM, int NP>
+__device__ __forceinline__ double np_hllc_half(const FaceSideN<DIM, NP>& s, int dir, double S, double Sb,
+                                               double S_Star, double* F)
+{
+    using I = NpIdx<DIM, NP>;
+    const double V = s.W[I::MOM + dir];
+    const double p = s.W[I::E];
+    const double inv = 1.0 / (S - S_Star);
+    const double q = (S - V) * inv;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        F[k] = s.W[k] * V + Sb * (q * s.U[k] - s.U[k]);
+        F[I::A + k] = s.W[I::A + k] * V + Sb * (q * s.U[I::A + k] - s.U[I::A + k]);
+    }
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+        double Fm = s.U[I::MOM + m] * V;
+        double Cs = q * s.U[I::MOM + m];
+        if (m == dir) {
+            Fm += p;
+            Cs += q * s.rho * (S_Star - V);
+        }
+        F[I::MOM + m] = Fm + Sb * (Cs - s.U[I::MOM + m]);
+    }
+    const double CsE = q * s.U[I::E] + (S_Star - V) * (q * s.rho * S_Star + p * inv);
+    F[I::E] = (s.U[I::E] + p) * V + Sb * (CsE - s.U[I::E]);
+    return V + Sb * (q - 1.0);
+}
+
+// HLLC (RIEMANN 2) or HLL (1, Riemann_HLL …NPhase.H) of the N-phase model; returns u*_n.
+template <int DIM, int RIEMANN, int NP>
+__device__ __forceinline__ double np_riemann(const FaceSideN<DIM, NP>& L, const FaceSideN<DIM, NP>& R, int dir,
+                                             double* F, double& cmax)
+{
+    using I = NpIdx<DIM, NP>;
+    constexpr int N = I::N;
+    const double VL = L.W[I::MOM + dir], VR = R.W[I::MOM + dir];
+    double qL = 0.0, qR = 0.0;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m) {
+        qL = (m == 0) ? L.W[I::MOM] * L.W[I::MOM] : qL + L.W[I::MOM + m] * L.W[I::MOM + m];
+        qR = (m == 0) ? R.W[I::MOM] * R.W[I::MOM] : qR + R.W[I::MOM + m] * R.W[I::MOM + m];
+    }
+    cmax = smax(L.c + sqrt(qL), R.c + sqrt(qR));
+    const double pL = L.W[I::E], pR = R.W[I::E];
+    if (RIEMANN == 2) {
+        const double V_avg = 0.5 * (VL + VR);
+        const double c_avg = 0.5 * (L.c + R.c);
+        const double SL = smin(V_avg - c_avg, VL - L.c);
+        const double SR = smax(V_avg + c_avg, VR + R.c);
+        const double S_Star = (pR - pL + L.rho * VL * (SL - VL) - R.rho * VR * (SR - VR))
+                            / (L.rho * (SL - VL) - R.rho * (SR - VR));
+        if (S_Star > 0.0) return np_hllc_half<DIM, NP>(L, dir, SL, smin(0.0, SL), S_Star, F);
+        if (S_Star < 0.0) return np_hllc_half<DIM, NP>(R, dir, SR, smax(0.0, SR), S_Star, F);
+        double FR_[N];
+        const double uL = np_hllc_half<DIM, NP>(L, dir, SL, smin(0.0, SL), S_Star, F);
+        const double uR = np_hllc_half<DIM, NP>(R, dir, SR, smax(0.0, SR), S_Star, FR_);
+#pragma unroll
+        for (int n = 0; n < N; ++n) F[n] = 0.5 * F[n] + 0.5 * FR_[n];
+        return 0.5 * uL + 0.5 * uR;
+    } else {
+        const double Sp = smax(smax(0.0, VL + L.c), VR + R.c);
+        const double Sm = smin(smin(0.0, VL - L.c), VR - R.c);
+        const double inv = 1.0 / (Sp - Sm);
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+            double FL, FR;
+            if (n < NP || n >= I::A) { FL = L.W[n] * VL; FR = R.W[n] * VR; }
+            else if (n == I::E) { FL = (L.U[n] + pL) * VL; FR = (R.U[n] + pR) * VR; }
+            else {
+                FL = L.U[n] * VL + ((n == I::MOM + dir) ? pL : 0.0);
+                FR = R.U[n] * VR + ((n == I::MOM + dir) ? pR : 0.0);
+            }
+            F[n] = (Sp * FL - Sm * FR + Sp * Sm * (R.U[n] - L.U[n])) * inv;
+        }
+        return (Sp * VL - Sm * VR) * inv;
+    }
+}
+
+// One face of the N-phase model (FVM_Conservative2FluxHyperbolic_K …NPhase.H:16-150): per side the
+// primitive reconstruction (WENO5 or MUSCL per primitive, then the consistent limiter; Godunov
+// without it), the Riemann solver; returns u*_n. s(t, n) as in face_flux.
+template <int DIM, int SCHEME, int RIEMANN, int NP, typename S>
+__device__ __forceinline__ double np_face_flux(const KParams& kp, int limiter, int dir, const S& s, double* F,
+                                               double& cmax, int& bad)
+{
+    constexpr int N = NpIdx<DIM, NP>::N;
+    const double dx = kp.dx[dir], idx = kp.inv_dx[dir];
+    FaceSideN<DIM, NP> L, R;
+    if (SCHEME == 5) {
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+            const double v0 = s(0, n), v1 = s(1, n), v2 = s(2, n), v3 = s(3, n), v4 = s(4, n), v5 = s(5, n);
+            L.W[n] = weno5<1>(v0, v1, v2, v3, v4);
+            R.W[n] = weno5<-1>(v1, v2, v3, v4, v5);
+        }
+        np_limit_face<DIM, NP>(kp, L.W, [&](int n) { return s(2, n); }, dx, 1, bad);
+        np_limit_face<DIM, NP>(kp, R.W, [&](int n) { return s(3, n); }, dx, -1, bad);
+    } else if (SCHEME == 1) {
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+            L.W[n] = muscl(s(0, n), s(1, n), s(2, n), dx, idx, 1, limiter);
+            R.W[n] = muscl(s(1, n), s(2, n), s(3, n), dx, idx, -1, limiter);
+        }
+        np_limit_face<DIM, NP>(kp, L.W, [&](int n) { return s(1, n); }, dx, 1, bad);
+        np_limit_face<DIM, NP>(kp, R.W, [&](int n) { return s(2, n); }, dx, -1, bad);
+    } else {
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+            L.W[n] = s(0, n);
+            R.W[n] = s(1, n);
+        }
+    }
+    np_make_side<DIM, NP>(kp, L, bad);
+    np_make_side<DIM, NP>(kp, R, bad);
+    return np_riemann<DIM, RIEMANN, NP>(L, R, dir, F, cmax);
+}
+
 // ---- phase field ------------------------------------------------------------------------------------
 // Mapping_Backward / Mapping_Forward (include/Physics_PhaseField/Mapping_Backward.H:12-29,
 // Mapping_Forward.H:12-26).

@@ -68,12 +68,12 @@ struct StageCtx {
 // One direction of the stage kernel (D known at compile time): stage the primitives of the tile
 // plus G cells on both sides along D, compute every face along D once, and accumulate the flux and
 // u*_n differences of the thread's cell.
-template <int DIM, int SCHEME, int RIEMANN, bool THINC, int D>
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int D, int NP>
 __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, double* sF,
                                          const double* sA, double* acc, double& divu,
                                          double& cmax_loc, int& bad)
 {
-    constexpr int N = 4 + DIM;
+    constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
     using L = Lds<DIM, G>;
     constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
@@ -131,7 +131,10 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         double Uc[N], Wc[N];
 #pragma unroll
         for (int q = 0; q < N; ++q) Uc[q] = cx_.Ub[c + q * bx.sn];
-        cons2prim<DIM>(kp, Uc, Wc, bad);
+        if constexpr (NP > 0)
+            np_cons2prim<DIM, NP>(kp, Uc, Wc, bad);
+        else
+            cons2prim<DIM>(kp, Uc, Wc, bad);
 #pragma unroll
         for (int q = 0; q < N; ++q) sW[q * RMAX + r] = Wc[q];
     }
@@ -158,6 +161,13 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
             } else {
 #pragma unroll
                 for (int q = 0; q < N; ++q) Fl[q] = 0.0;
+            }
+        } else if constexpr (NP > 0) {
+            double c;
+            un = np_face_flux<DIM, SCHEME, RIEMANN, NP>(kp, cx_.limiter, D, st, Fl, c, bad);
+            if (a < cx_.n[D]) {
+                if (__builtin_isnan(c)) bad = CMP_ERR_NAN;
+                cmax_loc = smax(cmax_loc, c);
             }
         } else {
             double aL = -1.0, aR = -1.0;
@@ -220,7 +230,7 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
 #define CMP_STAGE_ATTR
 #endif
 
-template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP>
 __global__ __launch_bounds__(NT) CMP_STAGE_ATTR
 void k_stage(KParams kp, int limiter, int mode, int stage,
              const TileDev* __restrict__ tiles, int ntiles, const BoxDev* __restrict__ boxes,
@@ -229,7 +239,7 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
              const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err,
              const double* __restrict__ gq)
 {
-    constexpr int N = 4 + DIM;
+    constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
     using L = Lds<DIM, G>;
@@ -295,9 +305,10 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
 
     StageCtx<DIM> ctx{kp, limiter, bx, Ub, {tlo[0], tlo[1], tlo[2]}, {n[0], n[1], n[2]},
                       {ae[0], ae[1], ae[2]}, tid, cx, cy, cz, valid, gq};
-    dir_pass<DIM, SCHEME, RIEMANN, THINC, 0>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
-    if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
-    if (DIM > 2) dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1)>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
+    dir_pass<DIM, SCHEME, RIEMANN, THINC, 0, NP>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
+    if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1, NP>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
+    if (DIM > 2)
+        dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1), NP>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
 
     if (valid) {
         const int i = tlo[0] + cx, j = tlo[1] + cy, k = tlo[2] + cz;
@@ -311,6 +322,17 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
                 Uc[q] = Ub[c + q * bx.sn];
                 dU[q] = acc[q];
             }
+        } else if constexpr (NP > 0) {
+            // every alpha_k takes phi_k (1 + K_k) div u* (…FVM_SurfaceIntegralNC_Compressible5Eq_NPhase.H)
+#pragma unroll
+            for (int q = 0; q < N; ++q) {
+                Uc[q] = Ub[c + q * bx.sn];
+                dU[q] = -acc[q];
+            }
+            double f[NP > 0 ? NP : 1];
+            np_nc_coefficients<DIM, NP>(kp, Uc, f, bad);
+#pragma unroll
+            for (int k = 0; k < NP; ++k) dU[NpIdx<DIM, NP>::A + k] += f[k] * divu;
         } else {
 #pragma unroll
             for (int q = 0; q < N; ++q) {
@@ -428,6 +450,9 @@ template <int DIM, int N>
 __global__ __launch_bounds__(NT)
 void k_fill(FillDev fd, const int* __restrict__ list, int box0, double* U, int* err)
 {
+    // first momentum component of a flow state (two-phase: after 2 masses; N-phase: after NP
+    // masses, N = 2 NP + DIM + 1); a Phase-Field field (1 or DIM comps) has none
+    constexpr int MOM = (N == 4 + DIM) ? 2 : ((N > 4 + DIM) ? (N - DIM - 1) / 2 : -1);
     // one grid row per box (of the level, or of `list`); 32-bit ghost indices inside the box
     const int b = list ? list[box0 + (int)blockIdx.y] : box0 + (int)blockIdx.y;
     const BoxDev bx = fd.boxes[b];
@@ -518,7 +543,7 @@ void k_fill(FillDev fd, const int* __restrict__ list, int box0, double* U, int* 
 #pragma unroll
         for (int c = 0; c < N; ++c) {
             double v = U[si + c * so.sn];
-            if (N == 4 + DIM && c >= 2 && c < 2 + DIM && ((flip >> (c - 2)) & 1u)) v = -v;
+            if (MOM >= 0 && c >= MOM && c < MOM + DIM && ((flip >> (c - MOM)) & 1u)) v = -v;
             U[di + c * bx.sn] = v;
         }
     }
@@ -571,7 +596,7 @@ hipError_t launch_set2(int* p, int a, int b, hipStream_t s)
 
 // ---- launchers for this dimension ---------------------------------------------------------------
 
-template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP = 0>
 static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, const TileDev* tiles,
                          int ntiles, const BoxDev* boxes, const BoxDev* sbox, int sntiles,
                          const double* Us, const double* Un, double* Uout, const double* dt,
@@ -587,7 +612,7 @@ static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, co
         grid = sntiles;
     }
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_stage<DIM, SCHEME, RIEMANN, THINC>), dim3(grid), dim3(NT), 0, s, kp, limiter,
+    hipLaunchKernelGGL((k_stage<DIM, SCHEME, RIEMANN, THINC, NP>), dim3(grid), dim3(NT), 0, s, kp, limiter,
                        mode, stage, tiles, ntiles, boxes, sb, sntx, snty, single, Us, Un, Uout, dt,
                        cmax_bits, err, gq);
     return hipGetLastError();
@@ -652,6 +677,25 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
 #undef CMP_PF_ARGS
         return hipErrorInvalidValue;
     }
+    if (lc.nphase) {
+        // the N-phase model: Godunov, MUSCL, WENO5 (consistent limiter) with HLL or HLLC
+        if (lc.thinc || (lc.riemann != 1 && lc.riemann != 2)) return hipErrorInvalidValue;
+#define CMP_NP_ARGS kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout, dt, \
+                    cmax_bits, err, s, nullptr
+#define CMP_NP_CASE(SC)                                                                               \
+        if (lc.scheme == SC) {                                                                        \
+            if (lc.nphase == 3) return lc.riemann == 2 ? stage4<DIM, SC, 2, false, 3>(CMP_NP_ARGS)    \
+                                                       : stage4<DIM, SC, 1, false, 3>(CMP_NP_ARGS);   \
+            if (lc.nphase == 2) return lc.riemann == 2 ? stage4<DIM, SC, 2, false, 2>(CMP_NP_ARGS)    \
+                                                       : stage4<DIM, SC, 1, false, 2>(CMP_NP_ARGS);   \
+        }
+        CMP_NP_CASE(5)
+        CMP_NP_CASE(1)
+        CMP_NP_CASE(0)
+#undef CMP_NP_CASE
+#undef CMP_NP_ARGS
+        return hipErrorInvalidValue;
+    }
     CMP_DISPATCH(stage4, kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout,
                  dt, cmax_bits, err, s, nullptr);
     return hipErrorInvalidValue;
@@ -661,6 +705,7 @@ template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
                             const double* U, double* flux, double* fluxnc, hipStream_t s)
 {
+    if (lc.nphase) return hipErrorInvalidValue;   // face arrays of the N-phase model are not built
     CMP_DISPATCH(faces4, kp, lc.limiter, box, dir, U, flux, fluxnc, s);
     return hipErrorInvalidValue;
 }
@@ -671,7 +716,7 @@ hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, const int
                            hipStream_t s)
 {
     if (nghost <= 0) return hipSuccess;
-    if (N != 4 + DIM && N != DIM && N != 1) return hipErrorInvalidValue;
+    if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM && N != DIM && N != 1) return hipErrorInvalidValue;
     // grid rows are boxes (at most 65535 per launch); grid columns cover the largest box's ghosts
     long long cols = (fd.maxghost + NT - 1) / NT;
     if (cols > 65536) cols = 65536;
@@ -681,6 +726,10 @@ hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, const int
         const dim3 grid((unsigned)cols, (unsigned)rows);
         if (N == 4 + DIM)
             hipLaunchKernelGGL((k_fill<DIM, 4 + DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
+        else if (N == 5 + DIM)
+            hipLaunchKernelGGL((k_fill<DIM, 5 + DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
+        else if (N == 7 + DIM)
+            hipLaunchKernelGGL((k_fill<DIM, 7 + DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
         else if (N == DIM)
             hipLaunchKernelGGL((k_fill<DIM, DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
         else

@@ -12,6 +12,10 @@
  *
  * State vector (PHYSICS == FIVEEQS, include/Physics_Compressible5Eq/Variables_Compressible5Eq.H:21-30):
  *   n = 0 a1rho1, 1 a2rho2, 2..1+dim rho*u_d, 2+dim rho*E, 3+dim a1      (nstate = 4+dim)
+ * or, with cmp_params.nphase = NP in {2, 3} (PHYSICS == FIVEEQS_NPHASE,
+ * include/Physics_Compressible5Eq_NPhase/Variables_Compressible5Eq_NPhase.H:21-29):
+ *   n = 0..NP-1 alpha_k rho_k, NP..NP+dim-1 rho*u_d, NP+dim rho*E, NP+dim+1.. alpha_k
+ *                                                                        (nstate = 2 NP + dim + 1)
  */
 #ifndef COMPAS_AMD_H
 #define COMPAS_AMD_H
