@@ -4,7 +4,6 @@
 // parallel; cmp_capi.hip routes a call to the right instantiation.
 //
 // Stage kernel (one workgroup = one tile of a box):
-//   with THINC: alpha1 of the tile plus one cell all round is staged in LDS once (face normals);
 //   for each direction d:
 //     1. the primitives of the tile plus G cells on both sides along d are staged in LDS
 //        (cons->prim once per staged cell),
@@ -39,7 +38,6 @@ struct Lds {
                                       DIM == 3 ? TX * TY * (TZ + 2 * G) : 0);
     static constexpr int FMAX = cmax3((TX + 1) * TY * TZ, TX * (TY + 1) * TZ,
                                       DIM == 3 ? TX * TY * (TZ + 1) : 0);
-    static constexpr int AMAX = (TX + 2) * (TY + 2) * (DIM == 3 ? TZ + 2 : 1);
 };
 
 // Maps a workgroup to a tile so that the workgroups one XCD receives (blocks b, b+8, b+16, ...
@@ -59,7 +57,6 @@ struct StageCtx {
     const double* __restrict__ Ub;
     int tlo[3];
     int n[3];
-    int ae[3];      // extents of the alpha1 halo region (THINC)
     int tid, cx, cy, cz;
     bool valid;
     const double* __restrict__ gq;   // Grad_Q face buffer (Phase-Field flux only)
@@ -69,9 +66,8 @@ struct StageCtx {
 // plus G cells on both sides along D, compute every face along D once, and accumulate the flux and
 // u*_n differences of the thread's cell.
 template <int DIM, int SCHEME, int RIEMANN, bool THINC, int D, int NP>
-__device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, double* sF,
-                                         const double* sA, double* acc, double& divu,
-                                         double& cmax_loc, int& bad)
+__device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, double* sF, double* acc,
+                                         double& divu, double& cmax_loc, int& bad)
 {
     constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
@@ -172,17 +168,19 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         } else {
             double aL = -1.0, aR = -1.0;
             if constexpr (THINC) {
-                // THINC decision (gates, face normal, tanh profile) ahead of the base reconstruction
-                // alpha1 halo coordinates of the face's high cell; transverse axes in ascending order
-                const int hx = x + 1, hy = y + 1, hz = (DIM == 3) ? z + 1 : 0;
-                constexpr int a0 = L::TX + 2, a1 = L::TY + 2;
+                // THINC decision (gates, face normal, tanh profile) ahead of the base reconstruction.
+                // The face normal is needed only where the volume-fraction gates admit THINC, so its
+                // alpha1 stencil (the face's two cells and their transverse neighbours, transverse
+                // axes in ascending order) is read from the state itself, which the staging pass has
+                // just brought into L2, instead of being staged for every face.
+                const int hi_ = cx_.tlo[0] + x, hj = cx_.tlo[1] + y, hk = cx_.tlo[2] + ((DIM == 3) ? z : 0);
                 auto aa = [&](int od, int o1, int o2) -> double {
                     const int lo = (od == -1) ? 1 : 0;
-                    int px = hx - (D == 0 ? lo : 0), py = hy - (D == 1 ? lo : 0), pz = hz - (D == 2 ? lo : 0);
-                    if (D == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
-                    else if (D == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
-                    else { px += o1; py += o2; }
-                    return sA[px + a0 * (py + a1 * pz)];
+                    int pi = hi_ - (D == 0 ? lo : 0), pj = hj - (D == 1 ? lo : 0), pk = hk - (D == 2 ? lo : 0);
+                    if (D == 0) { pj += o1; pk += (DIM == 3) ? o2 : 0; }
+                    else if (D == 1) { pi += o1; pk += (DIM == 3) ? o2 : 0; }
+                    else { pi += o1; pj += o2; }
+                    return cx_.Ub[fab_index(bx, pi, pj, pk) + (3 + DIM) * bx.sn];
                 };
                 auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
                 thinc_decide<DIM, G>(kp, st, normal, aL, aR);
@@ -244,14 +242,13 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
     constexpr int G = Stencil<SCHEME>::G;
     using L = Lds<DIM, G>;
     constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
-    constexpr int RMAX = L::RMAX, FMAX = L::FMAX, AMAX = THINC ? L::AMAX : 1;
+    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
 
     // the staged primitives of a direction and, once its faces are computed, their fluxes
     constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
     __shared__ double sWF[SWF];
     double* const sW = sWF;
     double* const sF = sWF;
-    __shared__ double sA[AMAX];
     __shared__ double sRed[NT / 64];
 
     const int tid = threadIdx.x;
@@ -290,25 +287,12 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
 #pragma unroll
     for (int q = 0; q < N; ++q) acc[q] = 0.0;
 
-    // alpha1 of the tile plus one cell all round (face normals of THINC), at the full tile's
-    // extents (positions beyond a clipped tile's halo stay unused)
-    int ae[3] = {n[0] + 2, n[1] + 2, DIM == 3 ? n[2] + 2 : 1};
-    if (THINC) {
-        constexpr int A0 = TX + 2, A1 = TY + 2, A2 = (DIM == 3) ? TZ + 2 : 1;
-        for (int r = tid; r < A0 * A1 * A2; r += NT) {
-            const int x = r % A0, y = (r / A0) % A1, z = r / (A0 * A1);
-            if (x >= ae[0] || y >= ae[1] || z >= ae[2]) continue;
-            const int k = (DIM == 3) ? tlo[2] + z - 1 : tlo[2];
-            sA[r] = Ub[fab_index(bx, tlo[0] + x - 1, tlo[1] + y - 1, k) + IA * bx.sn];
-        }
-    }
-
-    StageCtx<DIM> ctx{kp, limiter, bx, Ub, {tlo[0], tlo[1], tlo[2]}, {n[0], n[1], n[2]},
-                      {ae[0], ae[1], ae[2]}, tid, cx, cy, cz, valid, gq};
-    dir_pass<DIM, SCHEME, RIEMANN, THINC, 0, NP>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
-    if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1, NP>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
+    StageCtx<DIM> ctx{kp, limiter, bx, Ub, {tlo[0], tlo[1], tlo[2]}, {n[0], n[1], n[2]}, tid, cx, cy, cz,
+                      valid, gq};
+    dir_pass<DIM, SCHEME, RIEMANN, THINC, 0, NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
+    if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1, NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
     if (DIM > 2)
-        dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1), NP>(ctx, sW, sF, sA, acc, divu, cmax_loc, bad);
+        dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1), NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
 
     if (valid) {
         const int i = tlo[0] + cx, j = tlo[1] + cy, k = tlo[2] + cz;
