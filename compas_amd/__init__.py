@@ -323,6 +323,13 @@ class Level:
                                           info, _stream(stream)), "cmp_level_pf_gradq")
         return gm.value, list(info)
 
+    def solver_iterations(self):
+        """Iterations of the last implicit Grad_Q solve."""
+        it = ctypes.c_int(0)
+        check(self.lib.cmp_level_pf_solver_iterations(self.h, ctypes.byref(it)),
+              "cmp_level_pf_solver_iterations")
+        return it.value
+
     def PhaseField_ThirdOrderSSPRK(self, Un, U1, U2, gradq, pf_dt, stream=None):
         check(self.lib.cmp_level_pf_rk3_step(self.h, _ptr(Un), _ptr(U1), _ptr(U2), _ptr(gradq),
                                              float(pf_dt), _stream(stream)), "cmp_level_pf_rk3_step")

@@ -132,6 +132,7 @@ SIGNATURES = {
     "cmp_level_pf_gradq": (ctypes.c_int, [vp, dp, ctypes.c_double, dp, P(ctypes.c_double),
                                           P(ctypes.c_double), vp]),
     "cmp_level_pf_rk3_step": (ctypes.c_int, [vp, dp, dp, dp, dp, ctypes.c_double, vp]),
+    "cmp_level_pf_solver_iterations": (ctypes.c_int, [vp, P(ctypes.c_int)]),
     "cmp_level_pf_evolve": (ctypes.c_int, [vp, dp, dp, dp, dp, P(ctypes.c_double), vp]),
 }
 

@@ -117,14 +117,16 @@ hipError_t launch_pf_bcoef_dim(const PfGeom&, const long long*, const double*, d
 template <int DIM>
 hipError_t launch_cell_sum_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
 template <int DIM>
-hipError_t launch_cg_init_dim(const PfGeom&, long long, const double*, const double*, double, double*,
-                              double*, double*, double*, hipStream_t);
+hipError_t launch_cg_idiag_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
+template <int DIM>
+hipError_t launch_cg_init_dim(const PfGeom&, long long, const double*, const double*, double, const double*,
+                              double*, double*, double*, double*, double*, hipStream_t);
 template <int DIM>
 hipError_t launch_cg_ap_dim(const PfGeom&, long long, const double*, const double*, const double*, double*,
                             double*, hipStream_t);
 template <int DIM>
-hipError_t launch_cg_xr_dim(const PfGeom&, long long, const double*, double*, double*, const double*,
-                            const double*, double*, hipStream_t);
+hipError_t launch_cg_xr_dim(const PfGeom&, long long, const double*, const double*, double*, double*, double*,
+                            const double*, const double*, double*, hipStream_t);
 template <int DIM>
 hipError_t launch_cg_p_dim(const PfGeom&, long long, const double*, const double*, double*, hipStream_t);
 template <int DIM>
@@ -162,11 +164,17 @@ hipError_t launch_cell_sum(int dim, const PfGeom& g, long long maxcells, const d
     return CMP_BY_DIM(dim, launch_cell_sum_dim, g, maxcells, F, part, s);
 }
 
-hipError_t launch_cg_init(int dim, const PfGeom& g, long long maxcells, const double* rhs,
-                          const double* mean_sum, double ncell, double* X, double* R, double* P,
-                          double* part, hipStream_t s)
+hipError_t launch_cg_idiag(int dim, const PfGeom& g, long long maxcells, const double* Bf, double* ID,
+                           hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_cg_init_dim, g, maxcells, rhs, mean_sum, ncell, X, R, P, part, s);
+    return CMP_BY_DIM(dim, launch_cg_idiag_dim, g, maxcells, Bf, ID, s);
+}
+
+hipError_t launch_cg_init(int dim, const PfGeom& g, long long maxcells, const double* rhs,
+                          const double* mean_sum, double ncell, const double* ID, double* X, double* R,
+                          double* Z, double* P, double* part, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_cg_init_dim, g, maxcells, rhs, mean_sum, ncell, ID, X, R, Z, P, part, s);
 }
 
 hipError_t launch_cg_ap(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Bf,
@@ -175,16 +183,17 @@ hipError_t launch_cg_ap(int dim, const PfGeom& g, long long maxcells, const doub
     return CMP_BY_DIM(dim, launch_cg_ap_dim, g, maxcells, sc, Bf, P, AP, part, s);
 }
 
-hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const double* sc, double* X, double* R,
-                        const double* P, const double* AP, double* part, hipStream_t s)
+hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* ID,
+                        double* X, double* R, double* Z, const double* P, const double* AP, double* part,
+                        hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_cg_xr_dim, g, maxcells, sc, X, R, P, AP, part, s);
+    return CMP_BY_DIM(dim, launch_cg_xr_dim, g, maxcells, sc, ID, X, R, Z, P, AP, part, s);
 }
 
-hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* R,
+hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Z,
                        double* P, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_cg_p_dim, g, maxcells, sc, R, P, s);
+    return CMP_BY_DIM(dim, launch_cg_p_dim, g, maxcells, sc, Z, P, s);
 }
 
 hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X, double* gq,
@@ -279,6 +288,8 @@ struct cmp_level {
     double* d_R = nullptr;          // CG residual
     double* d_Pv = nullptr;         // CG search direction (its ghosts filled like alpha's)
     double* d_AP = nullptr;         // operator applied to d_Pv
+    double* d_Z = nullptr;          // preconditioned residual
+    double* d_ID = nullptr;         // inverse diagonal of the operator (Jacobi preconditioner)
     double* d_Bf = nullptr;         // face coefficients b (face layout)
     double* d_part = nullptr;       // block partial sums, 2 per block
     double* d_sc = nullptr;         // CG_NSC device scalars
@@ -685,7 +696,8 @@ static void pf_free(cmp_level* L)
 {
     void* bufs[] = {L->d_abox, L->d_nbox, L->d_gprefix1, L->d_A0, L->d_A, L->d_P, L->d_N,
                     L->d_J, L->d_H, L->d_gq, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
-                    L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Bf, L->d_part, L->d_sc, L->d_boxflag,
+                    L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Z, L->d_ID, L->d_Bf, L->d_part, L->d_sc,
+                    L->d_boxflag,
                     L->d_pf_tiles, L->d_pf_fill};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
@@ -699,6 +711,7 @@ static void pf_free(cmp_level* L)
     L->d_gprefix1 = nullptr;
     L->d_A0 = L->d_A = L->d_P = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_pfdt = nullptr;
     L->d_Mk = L->d_Wk = L->d_DJ = L->d_X = L->d_R = L->d_Pv = L->d_AP = L->d_Bf = nullptr;
+    L->d_Z = L->d_ID = nullptr;
     L->d_part = L->d_sc = nullptr;
     L->d_bits = nullptr;
     L->h_pin = nullptr;
@@ -973,7 +986,7 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
         CMP_TRYP(hipMalloc(&L->d_Wk, sizeof(double) * na));
     }
     if (pf->explicit_rc == 0) {
-        double** cell[] = {&L->d_DJ, &L->d_X, &L->d_R, &L->d_Pv, &L->d_AP};
+        double** cell[] = {&L->d_DJ, &L->d_X, &L->d_R, &L->d_Pv, &L->d_AP, &L->d_Z, &L->d_ID};
         for (double** p : cell) {
             CMP_TRYP(hipMalloc(p, sizeof(double) * na));
             CMP_TRYP(hipMemset(*p, 0, sizeof(double) * na));
@@ -1056,9 +1069,10 @@ static inline double bits_to_double(const void* p)
 // Div_Jhat = (alpha_new - alpha)/dt (:703); b = W(phi_f) + 1e-3 on the faces from the re-sharpened
 // alpha; Q solves div(b grad Q) = Div_Jhat with periodic / homogeneous Neumann sides; Grad_Q = grad Q
 // on the faces, zero on Neumann domain faces. The reference hands the system to AMReX's MLMG (relative
-// residual 1e-10); here conjugate gradients on the zero-mean subspace run to the rounding floor
-// (r.r <= 1e-28 of the initial r.r, or 50 iterations without a new minimum), each iteration five
-// passes over the level and two fixed-order reductions, the stopping test on the device.
+// residual 1e-10); here conjugate gradients with a Jacobi (diagonal) preconditioner on the zero-mean
+// subspace run to the rounding floor (r.r <= 1e-28 of the initial r.r, or 50 iterations without a
+// new minimum), each iteration five passes over the level and two fixed-order reductions, the
+// stopping test on the device.
 static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq, hipStream_t s)
 {
     const int dim = L->P.dim;
@@ -1069,9 +1083,10 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
     CMP_TRY(launch_pf_bcoef(dim, L->pg, L->maxfaces, L->d_P, L->d_Bf, s));
     CMP_TRY(launch_cell_sum(dim, L->pg, L->maxcells, L->d_DJ, L->d_part, s));
     CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_SUM, s));
-    CMP_TRY(launch_cg_init(dim, L->pg, L->maxcells, L->d_DJ, sc + CG_SUM, (double)L->ncell, L->d_X, L->d_R,
-                           L->d_Pv, L->d_part, s));
-    CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_RR, s));
+    CMP_TRY(launch_cg_idiag(dim, L->pg, L->maxcells, L->d_Bf, L->d_ID, s));
+    CMP_TRY(launch_cg_init(dim, L->pg, L->maxcells, L->d_DJ, sc + CG_SUM, (double)L->ncell, L->d_ID, L->d_X,
+                           L->d_R, L->d_Z, L->d_Pv, L->d_part, s));
+    CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RR, s));
     CMP_TRY(launch_cg_start(sc, s));
     // the host looks at the stop flag every `chunk` iterations; iterations queued past the stop
     // return at once
@@ -1081,9 +1096,10 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
             CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_Pv, L->d_err, s));
             CMP_TRY(launch_cg_ap(dim, L->pg, L->maxcells, sc, L->d_Bf, L->d_Pv, L->d_AP, L->d_part, s));
             CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_PAP, s));
-            CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_part, s));
-            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_RRNEW, s));
-            CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_R, L->d_Pv, s));
+            CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, L->d_ID, L->d_X, L->d_R, L->d_Z, L->d_Pv, L->d_AP,
+                                 L->d_part, s));
+            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RRNEW, s));
+            CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s));
             CMP_TRY(launch_cg_end(sc, maxit, s));
         }
         CMP_TRY(hipMemcpyAsync(L->h_pin + 2, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1095,6 +1111,13 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
     L->cg_iters = (int)L->h_pin[3];
     CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_X, L->d_err, s));
     CMP_TRY(launch_pf_gradx(dim, L->pg, L->maxfaces, L->d_X, gradq, L->d_bits + 1, L->d_boxflag, s));
+    return CMP_OK;
+}
+
+int cmp_level_pf_solver_iterations(const cmp_level* L, int* iterations)
+{
+    if (!L || !iterations) return CMP_ERR_ARG;
+    *iterations = L->cg_iters;
     return CMP_OK;
 }
 

@@ -129,32 +129,38 @@ hipError_t launch_pf_bcoef(int dim, const PfGeom& g, const long long* maxfaces, 
 // block partial sums of a field over the valid cells
 hipError_t launch_cell_sum(int dim, const PfGeom& g, long long maxcells, const double* F, double* part,
                            hipStream_t s);
-// CG pieces: r = -(rhs - mean_sum[0]/ncell), p = r, x = 0, partial of r.r
+// inverse diagonal of -div(b grad .) per cell (Jacobi preconditioner; Neumann domain faces excluded)
+hipError_t launch_cg_idiag(int dim, const PfGeom& g, long long maxcells, const double* Bf, double* ID,
+                           hipStream_t s);
+// CG pieces: r = -(rhs - mean_sum[0]/ncell), z = r/diag, p = z, x = 0, partials of r.r and r.z
 hipError_t launch_cg_init(int dim, const PfGeom& g, long long maxcells, const double* rhs,
-                          const double* mean_sum, double ncell, double* X, double* R, double* P,
-                          double* part, hipStream_t s);
+                          const double* mean_sum, double ncell, const double* ID, double* X, double* R,
+                          double* Z, double* P, double* part, hipStream_t s);
 // Device scalars of the CG solve (a double array of CG_NSC entries). The iteration kernels return at
 // once when sc[CG_DONE] is set, so the host only polls the flag every few iterations.
 enum {
     CG_RR = 0,      // r.r of the current residual
-    CG_PAP = 1,     // p.Ap of this iteration
-    CG_RRNEW = 2,   // r.r after this iteration's update
-    CG_B2 = 3,      // r.r of the initial residual
-    CG_DONE = 4,    // 1 once the solve has stopped
-    CG_BEST = 5,    // smallest r.r so far
-    CG_STALL = 6,   // iterations since the last new minimum
-    CG_IT = 7,      // iterations done
-    CG_SUM = 8,     // two slots for level sums (CAC's sum mech, sum W; the mean of the rhs)
+    CG_RZ = 1,      // r.z (z the preconditioned residual); the pair is reduced together
+    CG_PAP = 2,     // p.Ap of this iteration
+    CG_RRNEW = 3,   // r.r after this iteration's update
+    CG_RZNEW = 4,   // r.z after this iteration's update
+    CG_B2 = 5,      // r.r of the initial residual
+    CG_DONE = 6,    // 1 once the solve has stopped
+    CG_BEST = 7,    // smallest r.r so far
+    CG_STALL = 8,   // iterations since the last new minimum
+    CG_IT = 9,      // iterations done
+    CG_SUM = 10,    // two slots for level sums (CAC's sum mech, sum W; the mean of the rhs)
     CG_NSC = 16
 };
 // ap = -div(b grad p) (p ghosts filled; Neumann domain faces carry no flux), partial of p.ap
 hipError_t launch_cg_ap(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Bf,
                         const double* P, double* AP, double* part, hipStream_t s);
-// al = rr/pAp; x += al p; r -= al ap; partial of r.r (nothing when pAp <= 0)
-hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const double* sc, double* X,
-                        double* R, const double* P, const double* AP, double* part, hipStream_t s);
-// be = rr_new/rr; p = r + be p
-hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* R,
+// al = rz/pAp; x += al p; r -= al ap; z = r/diag; partials of r.r and r.z (nothing when pAp <= 0)
+hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* ID,
+                        double* X, double* R, double* Z, const double* P, const double* AP, double* part,
+                        hipStream_t s);
+// be = rz_new/rz; p = z + be p
+hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Z,
                        double* P, hipStream_t s);
 hipError_t launch_cg_start(double* sc, hipStream_t s);
 hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s);

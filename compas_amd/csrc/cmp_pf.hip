@@ -348,24 +348,50 @@ __global__ __launch_bounds__(NT) void k_pf_bcoef(PfGeom g, const double* __restr
     Bf[ab.goff[D] + q] = pf * (1.0 - pf) + 1.0e-3;
 }
 
+// 1/diag of -div(b grad .): sum over the cell's faces of b_f/dx_d^2, Neumann domain faces excluded
 template <int DIM>
-__global__ __launch_bounds__(NT) void k_cg_init(PfGeom g, const double* __restrict__ rhs,
-                                                const double* __restrict__ mean_sum, double ncell,
-                                                double* X, double* R, double* P, double* part)
+__global__ __launch_bounds__(NT) void k_cg_idiag(PfGeom g, const double* __restrict__ Bf, double* ID)
 {
     const BoxDev ab = g.abox[blockIdx.y];
     int i, j, k;
-    double v = 0.0;
+    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    const int co[3] = {i, j, k};
+    double dsum = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const bool lo_dom = !g.periodic[d] && co[d] == g.dlo[d];
+        const bool hi_dom = !g.periodic[d] && co[d] == g.dhi[d];
+        const double h2 = g.dx[d] * g.dx[d];
+        if (!hi_dom) dsum += Bf[face_index_rt(ab, d, i, j, k, 1)] / h2;
+        if (!lo_dom) dsum += Bf[face_index_rt(ab, d, i, j, k, 0)] / h2;
+    }
+    ID[ab.off + fab_index(ab, i, j, k)] = (dsum > 0.0) ? 1.0 / dsum : 0.0;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_cg_init(PfGeom g, const double* __restrict__ rhs,
+                                                const double* __restrict__ mean_sum, double ncell,
+                                                const double* __restrict__ ID, double* X, double* R,
+                                                double* Z, double* P, double* part)
+{
+    const BoxDev ab = g.abox[blockIdx.y];
+    int i, j, k;
+    double v = 0.0, w = 0.0;
     if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
         const long long c = ab.off + fab_index(ab, i, j, k);
         const double mean = mean_sum[0] / ncell;
         const double r = -(rhs[c] - mean);
+        const double z = r * ID[c];
         X[c] = 0.0;
         R[c] = r;
-        P[c] = r;
+        Z[c] = z;
+        P[c] = z;
         v = r * r;
+        w = r * z;
     }
-    block_sum_to(v, part, (long long)blockIdx.y * gridDim.x + blockIdx.x);
+    const long long blk = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    block_sum_to(v, part, blk);
+    block_sum_to(w, part, (long long)gridDim.x * gridDim.y + blk);
 }
 
 // the level sum of rhs over valid cells (for its mean)
@@ -411,36 +437,42 @@ __global__ __launch_bounds__(NT) void k_cg_ap(PfGeom g, const double* __restrict
 }
 
 template <int DIM>
-__global__ __launch_bounds__(NT) void k_cg_xr(PfGeom g, const double* __restrict__ sc, double* X, double* R,
+__global__ __launch_bounds__(NT) void k_cg_xr(PfGeom g, const double* __restrict__ sc,
+                                              const double* __restrict__ ID, double* X, double* R, double* Z,
                                               const double* __restrict__ P, const double* __restrict__ AP,
                                               double* part)
 {
     if (sc[CG_DONE] != 0.0 || !(sc[CG_PAP] > 0.0)) return;
     const BoxDev ab = g.abox[blockIdx.y];
     int i, j, k;
-    double v = 0.0;
+    double v = 0.0, w = 0.0;
     if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
         const long long c = ab.off + fab_index(ab, i, j, k);
-        const double al = sc[CG_RR] / sc[CG_PAP];
+        const double al = sc[CG_RZ] / sc[CG_PAP];
         X[c] += al * P[c];
         const double r = R[c] - al * AP[c];
+        const double z = r * ID[c];
         R[c] = r;
+        Z[c] = z;
         v = r * r;
+        w = r * z;
     }
-    block_sum_to(v, part, (long long)blockIdx.y * gridDim.x + blockIdx.x);
+    const long long blk = (long long)blockIdx.y * gridDim.x + blockIdx.x;
+    block_sum_to(v, part, blk);
+    block_sum_to(w, part, (long long)gridDim.x * gridDim.y + blk);
 }
 
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_cg_p(PfGeom g, const double* __restrict__ sc,
-                                             const double* __restrict__ R, double* P)
+                                             const double* __restrict__ Z, double* P)
 {
     if (sc[CG_DONE] != 0.0 || !(sc[CG_PAP] > 0.0)) return;
     const BoxDev ab = g.abox[blockIdx.y];
     int i, j, k;
     if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
     const long long c = ab.off + fab_index(ab, i, j, k);
-    const double be = sc[CG_RRNEW] / sc[CG_RR];
-    P[c] = R[c] + be * P[c];
+    const double be = sc[CG_RZNEW] / sc[CG_RZ];
+    P[c] = Z[c] + be * P[c];
 }
 
 // Grad_Q = (x_c - x_m)/dx on every face of direction D; zero on Neumann domain faces
@@ -517,11 +549,19 @@ hipError_t launch_cell_sum_dim(const PfGeom& g, long long maxcells, const double
 }
 
 template <int DIM>
+hipError_t launch_cg_idiag_dim(const PfGeom& g, long long maxcells, const double* Bf, double* ID, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg_idiag<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, Bf, ID);
+    return hipGetLastError();
+}
+
+template <int DIM>
 hipError_t launch_cg_init_dim(const PfGeom& g, long long maxcells, const double* rhs, const double* mean_sum,
-                              double ncell, double* X, double* R, double* P, double* part, hipStream_t s)
+                              double ncell, const double* ID, double* X, double* R, double* Z, double* P,
+                              double* part, hipStream_t s)
 {
     hipLaunchKernelGGL((k_cg_init<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, rhs, mean_sum,
-                       ncell, X, R, P, part);
+                       ncell, ID, X, R, Z, P, part);
     return hipGetLastError();
 }
 
@@ -534,18 +574,19 @@ hipError_t launch_cg_ap_dim(const PfGeom& g, long long maxcells, const double* s
 }
 
 template <int DIM>
-hipError_t launch_cg_xr_dim(const PfGeom& g, long long maxcells, const double* sc, double* X, double* R,
-                            const double* P, const double* AP, double* part, hipStream_t s)
+hipError_t launch_cg_xr_dim(const PfGeom& g, long long maxcells, const double* sc, const double* ID, double* X,
+                            double* R, double* Z, const double* P, const double* AP, double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_xr<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, X, R, P, AP, part);
+    hipLaunchKernelGGL((k_cg_xr<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, ID, X, R, Z, P, AP,
+                       part);
     return hipGetLastError();
 }
 
 template <int DIM>
-hipError_t launch_cg_p_dim(const PfGeom& g, long long maxcells, const double* sc, const double* R, double* P,
+hipError_t launch_cg_p_dim(const PfGeom& g, long long maxcells, const double* sc, const double* Z, double* P,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_p<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, R, P);
+    hipLaunchKernelGGL((k_cg_p<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, Z, P);
     return hipGetLastError();
 }
 
@@ -700,7 +741,7 @@ __global__ void k_cg_start(double* sc)
     sc[CG_BEST] = rr;
     sc[CG_STALL] = 0.0;
     sc[CG_IT] = 0.0;
-    sc[CG_DONE] = (rr > 0.0) ? 0.0 : 1.0;   // a zero (or NaN) right-hand side: x = 0
+    sc[CG_DONE] = (rr > 0.0 && sc[CG_RZ] > 0.0) ? 0.0 : 1.0;   // a zero (or NaN) right-hand side: x = 0
 }
 
 __global__ void k_cg_end(double* sc, double maxit)
@@ -713,6 +754,7 @@ __global__ void k_cg_end(double* sc, double maxit)
     }
     const double rr = sc[CG_RRNEW];
     sc[CG_RR] = rr;
+    sc[CG_RZ] = sc[CG_RZNEW];
     const double it = sc[CG_IT] + 1.0;
     sc[CG_IT] = it;
     bool done = false;
@@ -772,12 +814,16 @@ template hipError_t launch_pf_bcoef_dim<CMP_KERNEL_DIM>(const PfGeom&, const lon
                                                         double*, hipStream_t);
 template hipError_t launch_cell_sum_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, double*,
                                                         hipStream_t);
+template hipError_t launch_cg_idiag_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, double*,
+                                                        hipStream_t);
 template hipError_t launch_cg_init_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
-                                                       double, double*, double*, double*, double*, hipStream_t);
+                                                       double, const double*, double*, double*, double*, double*,
+                                                       double*, hipStream_t);
 template hipError_t launch_cg_ap_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
                                                      const double*, double*, double*, hipStream_t);
-template hipError_t launch_cg_xr_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, double*, double*,
-                                                     const double*, const double*, double*, hipStream_t);
+template hipError_t launch_cg_xr_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
+                                                     double*, double*, double*, const double*, const double*,
+                                                     double*, hipStream_t);
 template hipError_t launch_cg_p_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
                                                     double*, hipStream_t);
 template hipError_t launch_pf_gradx_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*, double*,

@@ -247,6 +247,10 @@ int cmp_level_pf_rhs(cmp_level* level, const double* U, const double* gradq, dou
 int cmp_level_pf_gradq(cmp_level* level, const double* U, double dt, double* gradq,
                        double* gradq_max, double* info, void* stream);
 
+/* Iterations of the last implicit Grad_Q solve of cmp_level_pf_gradq (0 with the explicit RC).
+ * No counterpart in the reference (MLMG reports through its verbosity); for diagnostics. */
+int cmp_level_pf_solver_iterations(const cmp_level* level, int* iterations);
+
 /* One Phase-Field TVD-RK3 sub-step of size pf_dt in place on Un (PhaseField_ThirdOrderSSPRK,
  * :2620-2752): the stage combinations of cmp_level_rk3_stage with the Phase-Field right-hand side;
  * U1, U2 are scratch level buffers. */
