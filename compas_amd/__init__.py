@@ -97,7 +97,10 @@ def make_pf_params(mechanism="ACDI", explicit_rc=1, id_flux=0, flux_phase2flux=1
     p.mechanism = PF_MECHANISMS[mechanism]
     p.explicit_rc, p.id_flux, p.flux_phase2flux = explicit_rc, id_flux, flux_phase2flux
     p.cfl, p.lambda_, p.eta, p.eta_factor, p.eta_multiplier = cfl, lambda_, eta, eta_factor, eta_multiplier
-    p.p_min[0], p.p_min[1] = p_min
+    if len(p_min) > 3:
+        raise ValueError("p_min holds at most 3 phases")
+    for k, v in enumerate(p_min):
+        p.p_min[k] = v
     p.max_step, p.change_limiter = max_step, change_limiter
     return p
 

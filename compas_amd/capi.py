@@ -91,7 +91,7 @@ class cmp_pf_params(ctypes.Structure):
         ("eta", ctypes.c_double),
         ("eta_factor", ctypes.c_double),
         ("eta_multiplier", ctypes.c_double),
-        ("p_min", ctypes.c_double * 2),
+        ("p_min", ctypes.c_double * 3),
         ("max_step", ctypes.c_int),
         ("change_limiter", ctypes.c_int),
     ]

@@ -1043,8 +1043,7 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     L->pf_lc.riemann = (pf->flux_phase2flux == 1) ? RIEMANN_PF_MIX : RIEMANN_PF_PHASIC;
     L->pf_kp = L->kp;
     L->pf_kp.thinc = 0;
-    L->pf_kp.pf_pmin[0] = pf->p_min[0];
-    L->pf_kp.pf_pmin[1] = pf->p_min[1];
+    for (int k = 0; k < CMP_MAX_PHASES; ++k) L->pf_kp.pf_pmin[k] = pf->p_min[k];
     L->pf_on = true;
     return CMP_OK;
 }

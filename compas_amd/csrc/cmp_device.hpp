@@ -25,7 +25,7 @@ struct KParams {
     double dx[3];
     double inv_dx[3];
     double weno5b_weight;   // quadrature weight of the WENO5B bound limiter (1/12 for ID_Poly = 4)
-    double pf_pmin[2];      // phasic energy floors of the phasic Phase-Field formulation
+    double pf_pmin[CMP_MAX_PHASES];    // phasic energy floors of the phasic Phase-Field formulation
 };
 
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }  // std::min

@@ -109,7 +109,8 @@ typedef struct cmp_pf_params {
     double eta;              /* PhaseField.Eta; <= 0 means eta_multiplier * max_d dx_d                */
     double eta_factor;       /* PhaseField.Eta_Factor (re-sharpening width of Grad_Q's weight)        */
     double eta_multiplier;   /* PhaseField.Eta_Multiplier                                             */
-    double p_min[2];         /* phasic energy floors (phasic formulation only)                        */
+    double p_min[CMP_MAX_PHASES]; /* phasic energy floors, one per phase (phasic formulation only;
+                              * the reference sets -B_k/A_k, or -B_k/(A_k+1) with the source term) */
     int    max_step;         /* PhaseField.max_step: cap on Phase-Field sub-steps per flow step       */
     int    change_limiter;   /* timestep_change_limiter applied to the sub-step size                  */
 } cmp_pf_params;

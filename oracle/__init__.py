@@ -63,6 +63,8 @@ def load():
         "ref_np_cons2flux": (None, [PP, PD, ctypes.c_int, PD, PI]),
         "ref_np_riemann": (None, [PP, PD, PD, ctypes.c_int, PD, PD, PD, PI]),
         "ref_np_limiter_consistent": (None, [ctypes.c_int, PD, PD]),
+        "ref_np_pf_flux": (None, [PP, PF, PD, PD, PD, PD, PD, PI]),
+        "ref_np_cons2phase": (None, [PP, PF, PD, PD, PI]),
         "ref_np_limit_consistent": (None, [PP, PD, PD, D, ctypes.c_int, PD, PI]),
         "ref_face_flux": (ctypes.c_int, [PP, PG, PB, ctypes.c_int, PD, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, PD, PD, PD]),
@@ -377,6 +379,28 @@ def pf_flux(params, pf, UL, UR, g):
     err = ctypes.c_int(0)
     load().ref_pf_flux(ctypes.byref(params), ctypes.byref(pf), _dptr(UL), _dptr(UR), float(g),
                        _dptr(F), ctypes.byref(c), ctypes.byref(err))
+    return F, c.value
+
+
+def np_cons2phase(params, pf, U):
+    """Phasic conservative vector Cp[q*nphase + k] (q: mass, momenta, energy, alpha) of one state."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    Cp = np.zeros((3 + params.dim) * params.nphase)
+    err = ctypes.c_int(0)
+    load().ref_np_cons2phase(ctypes.byref(params), ctypes.byref(pf), _dptr(U), _dptr(Cp), ctypes.byref(err))
+    return Cp
+
+
+def np_pf_flux(params, pf, UL, UR, g):
+    """N-phase model: fhat (nstate) and the UD wave speed of one face; g holds Grad_Q per phase."""
+    UL = np.ascontiguousarray(UL, dtype=np.float64)
+    UR = np.ascontiguousarray(UR, dtype=np.float64)
+    gg = np.ascontiguousarray(g, dtype=np.float64)
+    F = np.zeros(params.nstate)
+    c = ctypes.c_double(0.0)
+    err = ctypes.c_int(0)
+    load().ref_np_pf_flux(ctypes.byref(params), ctypes.byref(pf), _dptr(UL), _dptr(UR), _dptr(gg), _dptr(F),
+                          ctypes.byref(c), ctypes.byref(err))
     return F, c.value
 
 

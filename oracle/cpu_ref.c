@@ -1483,6 +1483,122 @@ void ref_pf_flux(const cmp_params* P, const cmp_pf_params* PF, const double* UL,
     (void)N;
 }
 
+/* Conservative2ConservativePhase of the N-phase model
+ * (include/Physics_Compressible5Eq_NPhase/Conservative2ConservativePhase_Compressible5Eq_NPhase.H):
+ * Cp[q*NP + k] for q = alpha_k rho_k, alpha_k rho_k u_d (dim), alpha_k rho_k E_k, alpha_k; a phase
+ * whose internal energy falls below its p_min value is clipped to it and the remainder shared among
+ * the unclipped phases by volume fraction, repeated until none falls below. */
+void ref_np_cons2phase(const cmp_params* P, const cmp_pf_params* PF, const double* U, double* Cp, int* err)
+{
+    const int d = P->dim, np = P->nphase, ia = np + d + 1;
+    double W[REF_NMAX];
+    ref_np_cons2prim(P, U, W, err);
+    double p = W[np + d];
+    double e[CMP_MAX_PHASES], em[CMP_MAX_PHASES];
+    int clipped[CMP_MAX_PHASES] = {0, 0, 0};
+    int neg = 0;
+    double rho_e = 0.0;
+    for (int k = 0; k < np; ++k) {
+        e[k] = ref_eos_phase_e(p, W[ia + k], W[k], P->eos_A[k], P->eos_B[k], P->eos_C[k], P->eos_D[k], err);
+        rho_e += e[k];
+        em[k] = ref_eos_phase_e(PF->p_min[k], W[ia + k], W[k], P->eos_A[k], P->eos_B[k], P->eos_C[k],
+                                P->eos_D[k], err);
+        if (e[k] < em[k]) neg = 1;
+    }
+    int guard = 0;
+    while (neg && guard++ < 2 * CMP_MAX_PHASES) {
+        for (int k = 0; k < np; ++k)
+            if (e[k] < em[k]) { e[k] = em[k]; clipped[k] = 1; }
+        double se = 0.0, sa = 0.0;
+        for (int k = 0; k < np; ++k) {
+            se += e[k];
+            if (!clipped[k]) sa += W[ia + k];
+        }
+        for (int k = 0; k < np; ++k)
+            if (!clipped[k]) e[k] += W[ia + k] / sa * (rho_e - se);
+        neg = 0;
+        for (int k = 0; k < np; ++k)
+            if (e[k] < em[k]) neg = 1;
+    }
+    double ke = W[np] * W[np];
+    for (int m = 1; m < d; ++m) ke = ke + W[np + m] * W[np + m];
+    for (int k = 0; k < np; ++k) {
+        Cp[k] = W[k];
+        for (int m = 0; m < d; ++m) Cp[(1 + m) * np + k] = W[k] * W[np + m];
+        Cp[(1 + d) * np + k] = (0.5) * W[k] * ke + e[k];
+        Cp[(2 + d) * np + k] = W[ia + k];
+    }
+}
+
+/* FluxPhaseField of the N-phase model (…FluxPhaseField_Compressible5Eq_NPhase_PhaseField.H:12-79) with
+ * the efficient UD phase flux (FluxPhasePhaseField_UD_Efficient,
+ * include/Physics_Compressible_NPhase_PhaseField/FluxPhasePhaseField_UD_Efficient_…H:12-232):
+ * per phase i, J_q,i = q_i (downwind of Grad_Q_i) (1 - alpha_i)(upwind of Grad_Q_i) Grad_Q_i; then
+ * with Sum_J = sum_i J_alpha,i every J_q,i -= q_i (upwind of Sum_J) Sum_J. The phasic sum
+ * (VectorPhase2Vector) or the mixture form (FluxPhaseField_Mixture …NPhase_PhaseField.H:12-104). */
+void ref_np_pf_flux(const cmp_params* P, const cmp_pf_params* PF, const double* UL, const double* UR,
+                    const double* g, double* flux, double* cmax, int* err)
+{
+    const int d = P->dim, np = P->nphase, nq = 3 + d;
+    double CL[(3 + 3) * CMP_MAX_PHASES], CR[(3 + 3) * CMP_MAX_PHASES], Jp[(3 + 3) * CMP_MAX_PHASES];
+    ref_np_cons2phase(P, PF, UL, CL, err);
+    ref_np_cons2phase(P, PF, UR, CR, err);
+    const int qa = (2 + d) * np;   /* alpha_k entries */
+    double c = 0.0, SumJ = 0.0;
+    for (int i = 0; i < np; ++i) {
+        double gi = g[i];
+        double aju = f_up(1.0 - CL[qa + i], 1.0 - CR[qa + i], gi);
+        double ci = fabs(aju * gi);
+        c = smax(c, ci);
+        for (int q = 0; q < nq; ++q)
+            Jp[q * np + i] = f_down(CL[q * np + i], CR[q * np + i], gi) * aju * gi;
+        SumJ += Jp[qa + i];
+    }
+    c += fabs(SumJ);
+    for (int i = 0; i < np; ++i)
+        for (int q = 0; q < nq; ++q)
+            Jp[q * np + i] += -f_up(CL[q * np + i], CR[q * np + i], SumJ) * SumJ;
+    *cmax = c;
+    if (PF->flux_phase2flux == 0) {
+        for (int k = 0; k < np; ++k) {
+            flux[k] = Jp[k];
+            flux[np + d + 1 + k] = Jp[qa + k];
+        }
+        for (int m = 0; m < d; ++m) {
+            double t = 0.0;
+            for (int k = 0; k < np; ++k) t += Jp[(1 + m) * np + k];
+            flux[np + m] = t;
+        }
+        double te = 0.0;
+        for (int k = 0; k < np; ++k) te += Jp[(1 + d) * np + k];
+        flux[np + d] = te;
+        return;
+    }
+    double WL[REF_NMAX], WR[REF_NMAX];
+    ref_np_cons2prim(P, UL, WL, err);
+    ref_np_cons2prim(P, UR, WR, err);
+    double M = 0.0;
+    for (int k = 0; k < np; ++k) M += Jp[k];
+    double uD[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < d; ++m) uD[m] = f_down(WL[np + m], WR[np + m], M);
+    double ke = uD[0] * uD[0];
+    for (int m = 1; m < d; ++m) ke = ke + uD[m] * uD[m];
+    double MeK = M * (0.5 * ke);
+    double Je = 0.0;
+    for (int k = 0; k < np; ++k) {
+        double Jk = Jp[qa + k];
+        double pk = f_down(WL[np + d], WR[np + d], Jk);
+        double Jphi = (P->eos_C[k] > 0.0) ? Jk - P->eos_C[k] * Jp[k] : Jk;
+        Je += P->eos_A[k] * Jphi * pk + P->eos_B[k] * Jphi + P->eos_D[k] * Jp[k];
+    }
+    for (int k = 0; k < np; ++k) {
+        flux[k] = Jp[k];
+        flux[np + d + 1 + k] = Jp[qa + k];
+    }
+    for (int m = 0; m < d; ++m) flux[np + m] = M * uD[m];
+    flux[np + d] = MeK + Je;
+}
+
 /* FVM_Conservative2FluxPhaseField_K, ID_Bound == 0
  * (include/Operators_Compressible5Eq_PhaseField/FVM_Conservative2FluxPhaseField_K_Compressible5Eq_PhaseField.H:15-74):
  * the base reconstruction of FVM_Reconstruction2FaceLR_Conservative_K (no THINC on this path),
@@ -1490,7 +1606,7 @@ void ref_pf_flux(const cmp_params* P, const cmp_pf_params* PF, const double* UL,
  * stores it negated (source/Class_Compressible_PhaseField.cpp:840). */
 int ref_pf_face_flux(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                      const cmp_box* box, int ng, const double* U, int i, int j, int k, int dir,
-                     double gradq, double* fhat, double* cmax)
+                     const double* gradq, double* fhat, double* cmax)
 {
     const int N = P->nstate;
     int err = 0;
@@ -1506,9 +1622,16 @@ int ref_pf_face_flux(const cmp_params* P, const cmp_pf_params* PF, const cmp_geo
         for (int n = 0; n < N; ++n) Ust[s + 3][n] = U[c + n * f.sn];
     }
     double UL[REF_NMAX], UR[REF_NMAX];
+    if (P->nphase) {
+        /* …FVM_Conservative2FluxPhaseField_K_Compressible5Eq_NPhase_PhaseField.H:15-79 */
+        int r = ref_np_recon_lr(P, Ust, G->dx[dir], UL, UR, &err);
+        if (r) return r;
+        ref_np_pf_flux(P, PF, UL, UR, gradq, fhat, cmax, &err);
+        return err;
+    }
     int r = ref_recon_lr(P, Ust, P->id_weno, P->id_limiter, G->dx[dir], UL, UR, &err);
     if (r) return r;
-    ref_pf_flux(P, PF, UL, UR, gradq, fhat, cmax, &err);
+    ref_pf_flux(P, PF, UL, UR, gradq[0], fhat, cmax, &err);
     return err;
 }
 
@@ -1522,6 +1645,7 @@ int ref_pf_rhs_box(const cmp_params* P, const cmp_pf_params* PF, const cmp_geome
                    double* dUdt)
 {
     const int dim = P->dim, N = P->nstate;
+    const int npf = P->nphase ? P->nphase : 1;   /* Grad_Q components per face */
     int nv[3] = {1, 1, 1};
     for (int d = 0; d < dim; ++d) nv[d] = box->hi[d] - box->lo[d] + 1;
     long long ncell = (long long)nv[0] * nv[1] * nv[2];
@@ -1542,9 +1666,10 @@ int ref_pf_rhs_box(const cmp_params* P, const cmp_pf_params* PF, const cmp_geome
             for (int jj = 0; jj < fe[1]; ++jj)
                 for (int ii = 0; ii < fe[0]; ++ii) {
                     long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
-                    double fh[REF_NMAX], c;
+                    double fh[REF_NMAX], c, gv[CMP_MAX_PHASES];
+                    for (int m = 0; m < npf; ++m) gv[m] = gq[d][q + m * nface];
                     int r = ref_pf_face_flux(P, PF, G, box, ng, U, box->lo[0] + ii, box->lo[1] + jj,
-                                             lo2 + kk, d, gq[d][q], fh, &c);
+                                             lo2 + kk, d, gv, fh, &c);
                     if (r > e) e = r;
                     for (int n = 0; n < N; ++n) F[d][q + n * nface] = -fh[n];
                 }
@@ -1574,14 +1699,15 @@ int ref_pf_rhs_box(const cmp_params* P, const cmp_pf_params* PF, const cmp_geome
 int cmp_face_layout(int dim, int nbox, const cmp_box* boxes, int ncomp, long long* offsets,
                     long long* total);
 
-/* PhaseField_compute_dUdt_FV over every box; gradq is a face buffer (cmp_face_layout, 1 comp). */
+/* PhaseField_compute_dUdt_FV over every box; gradq is a face buffer (cmp_face_layout with one
+ * component per volume fraction: 1 for the two-phase model, nphase for the N-phase model). */
 int ref_level_pf_rhs(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                      int nbox, const cmp_box* boxes, int ng, const double* U, const double* gradq,
                      double* dUdt)
 {
     const int dim = P->dim, N = P->nstate;
     long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
-    cmp_face_layout(dim, nbox, boxes, 1, foff, NULL);
+    cmp_face_layout(dim, nbox, boxes, P->nphase ? P->nphase : 1, foff, NULL);
     int err = 0;
     for (int b = 0; b < nbox; ++b) {
         long long nc = valid_cells(dim, &boxes[b]);
@@ -1633,10 +1759,12 @@ double ref_level_max_velocity(const cmp_params* P, int nbox, const cmp_box* boxe
             for (int j = boxes[b].lo[1]; j <= boxes[b].hi[1]; ++j)
                 for (int i = boxes[b].lo[0]; i <= boxes[b].hi[0]; ++i) {
                     long long c = fab_idx(&f, i, j, k);
-                    double rho = B[c] + B[c + f.sn];
+                    const int nm = P->nphase ? P->nphase : 2;   /* masses, then momenta */
+                    double rho = B[c];
+                    for (int q = 1; q < nm; ++q) rho += B[c + q * f.sn];
                     double s = 0.0;
                     for (int m = 0; m < dim; ++m) {
-                        double u = B[c + (2 + m) * f.sn] / rho;
+                        double u = B[c + (nm + m) * f.sn] / rho;
                         s = (m == 0) ? u * u : s + u * u;
                     }
                     double V = sqrt(s);
@@ -2144,44 +2272,63 @@ int ref_level_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, const 
     double mv = ref_level_max_velocity(P, nbox, boxes, ng, U);
     double lambda, eta, mob;
     ref_pf_coefficients(PF, G, mv, &lambda, &eta, &mob);
-    long long ta, tf;
+    /* one volume fraction (two-phase model) or one per phase (N-phase model); PhaseField_Mechanism
+     * and PhaseField_ACDI loop over the phases (:1036-1150, :1495-1644), each phase's Grad_Q is the
+     * phase's component of the face buffer */
+    const int npf = P->nphase ? P->nphase : 1;
+    const int ia0 = P->nphase ? P->nphase + dim + 1 : 3 + dim;
+    long long ta, tf, tfn;
     cmp_box* ba = pf_boxes(dim, nbox, boxes, 1, 1, &ta);
     double* A = (double*)calloc((size_t)ta, sizeof(double));
-    /* alpha1 of the valid cells, then its ghosts */
-    for (int b = 0; b < nbox; ++b) {
-        fab_t fu = make_fab(dim, &boxes[b], ng), fa = make_fab(dim, &ba[b], 1);
-        int lo2 = (dim == 3) ? boxes[b].lo[2] : 0, hi2 = (dim == 3) ? boxes[b].hi[2] : 0;
-        for (int k = lo2; k <= hi2; ++k)
-            for (int j = boxes[b].lo[1]; j <= boxes[b].hi[1]; ++j)
-                for (int i = boxes[b].lo[0]; i <= boxes[b].hi[0]; ++i)
-                    A[ba[b].offset + fab_idx(&fa, i, j, k)] =
-                        U[boxes[b].offset + fab_idx(&fu, i, j, k) + (3 + dim) * fu.sn];
-    }
-    int err = ref_level_fill_pf(dim, G, nbox, ba, 1, 1, A);
     long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
+    long long* foffn = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
     cmp_face_layout(dim, nbox, boxes, 1, foff, &tf);
+    cmp_face_layout(dim, nbox, boxes, npf, foffn, &tfn);
     double* H = (double*)malloc(sizeof(double) * tf);
-    int nsub = 0, e;
-    if (PF->explicit_rc) {
-        e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, H, NULL, &nsub);
+    double* gq1 = (double*)malloc(sizeof(double) * tf);
+    int err = 0, nsub = 0, e;
+    for (int ph = 0; ph < npf; ++ph) {
+        /* alpha of the valid cells, then its ghosts */
+        for (int b = 0; b < nbox; ++b) {
+            fab_t fu = make_fab(dim, &boxes[b], ng), fa = make_fab(dim, &ba[b], 1);
+            int lo2 = (dim == 3) ? boxes[b].lo[2] : 0, hi2 = (dim == 3) ? boxes[b].hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = boxes[b].lo[1]; j <= boxes[b].hi[1]; ++j)
+                    for (int i = boxes[b].lo[0]; i <= boxes[b].hi[0]; ++i)
+                        A[ba[b].offset + fab_idx(&fa, i, j, k)] =
+                            U[boxes[b].offset + fab_idx(&fu, i, j, k) + (ia0 + ph) * fu.sn];
+        }
+        e = ref_level_fill_pf(dim, G, nbox, ba, 1, 1, A);
         if (e && !err) err = e;
-        ref_level_pf_gradq(P, PF, G, nbox, boxes, A, H, eta, gradq);
-    } else {
-        double* DJ = (double*)calloc((size_t)ta, sizeof(double));
-        e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, NULL, DJ, &nsub);
-        if (e && !err) err = e;
-        e = ref_level_pf_gradq_implicit(P, PF, G, nbox, boxes, A, DJ, eta, gradq, NULL);
-        if (e && !err) err = e;
-        free(DJ);
+        if (PF->explicit_rc) {
+            e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, H, NULL, &nsub);
+            if (e && !err) err = e;
+            ref_level_pf_gradq(P, PF, G, nbox, boxes, A, H, eta, gq1);
+        } else {
+            double* DJ = (double*)calloc((size_t)ta, sizeof(double));
+            e = ref_level_pf_mechanism(P, PF, G, nbox, boxes, A, dt, mob, eta, lambda, NULL, DJ, &nsub);
+            if (e && !err) err = e;
+            e = ref_level_pf_gradq_implicit(P, PF, G, nbox, boxes, A, DJ, eta, gq1, NULL);
+            if (e && !err) err = e;
+            free(DJ);
+        }
+        /* into component ph of the face buffer */
+        for (int b = 0; b < nbox; ++b)
+            for (int d = 0; d < dim; ++d) {
+                long long nf = 1;
+                for (int m = 0; m < dim; ++m) nf *= boxes[b].hi[m] - boxes[b].lo[m] + 1 + (m == d ? 1 : 0);
+                for (long long q = 0; q < nf; ++q)
+                    gradq[foffn[b * dim + d] + ph * nf + q] = gq1[foff[b * dim + d] + q];
+            }
     }
     double gm = 0.0;
-    for (long long q = 0; q < tf; ++q) {
+    for (long long q = 0; q < tfn; ++q) {
         double a = fabs(gradq[q]);
         if (gm < a || isnan(a)) gm = a;
     }
     if (gradq_max) *gradq_max = gm;
     if (info) { info[0] = mv; info[1] = mob; info[2] = eta; info[3] = nsub; }
-    free(A); free(H); free(foff); free(ba);
+    free(A); free(H); free(gq1); free(foff); free(foffn); free(ba);
     return err;
 }
 
@@ -2213,7 +2360,7 @@ int ref_level_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, const cmp_
     const int dim = P->dim;
     long long tf;
     long long* foff = (long long*)malloc(sizeof(long long) * (size_t)(nbox * dim + 1));
-    cmp_face_layout(dim, nbox, boxes, 1, foff, &tf);
+    cmp_face_layout(dim, nbox, boxes, P->nphase ? P->nphase : 1, foff, &tf);
     double* Q = (double*)malloc(sizeof(double) * tf);
     double gm = 0.0, inf4[4];
     int err = ref_level_pf_gradq_step(P, PF, G, nbox, boxes, ng, U, dt, Q, &gm, inf4);

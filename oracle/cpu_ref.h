@@ -99,9 +99,13 @@ double ref_pf_acdi_flux(double pf_m, double pf_c, double n_m, double n_c, double
 double ref_pf_gradq_face(double pf_m, double pf_c, double jhat, double eta, double eta_factor);
 void   ref_pf_flux(const cmp_params* P, const cmp_pf_params* PF, const double* UL, const double* UR,
                    double g, double* flux, double* cmax, int* err);
+void   ref_np_cons2phase(const cmp_params* P, const cmp_pf_params* PF, const double* U, double* Cp,
+                         int* err);
+void   ref_np_pf_flux(const cmp_params* P, const cmp_pf_params* PF, const double* UL, const double* UR,
+                      const double* g, double* flux, double* cmax, int* err);
 int    ref_pf_face_flux(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                         const cmp_box* box, int ng, const double* U, int i, int j, int k, int dir,
-                        double gradq, double* fhat, double* cmax);
+                        const double* gradq, double* fhat, double* cmax);
 int    ref_pf_rhs_box(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                       const cmp_box* box, int ng, const double* U, const double* const* gq,
                       double* dUdt);
