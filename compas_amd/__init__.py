@@ -310,7 +310,8 @@ class Level:
     # ---- Phase-Field regularization ----------------------------------------------------------
     def pf_setup(self, pf):
         self.pf = pf
-        self.faces = FaceLayout(self.layout.dim, self.layout.boxes, 1)
+        # Grad_Q: one component per volume fraction
+        self.faces = FaceLayout(self.layout.dim, self.layout.boxes, self.params.nphase or 1)
         check(self.lib.cmp_level_pf_setup(self.h, ctypes.byref(pf)), "cmp_level_pf_setup")
 
     # PhaseField_compute_dUdt_FV on every box (ghosts of U filled; gradq a face buffer)

@@ -28,7 +28,9 @@ template <int DIM>
 hipError_t launch_dt_dim(const double*, double*, double*, double, int, double*, int*, int*,
                          hipStream_t);
 template <int DIM>
-hipError_t launch_pf_extract_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
+hipError_t launch_pf_extract_dim(const PfGeom&, long long, const double*, double*, int, hipStream_t);
+template <int DIM>
+hipError_t launch_pf_scatter_dim(const PfGeom&, const long long*, const double*, double*, int, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_normals_dim(const PfGeom&, long long, const double*, double*, hipStream_t);
 template <int DIM>
@@ -40,7 +42,7 @@ template <int DIM>
 hipError_t launch_pf_gradq_dim(const PfGeom&, const long long*, const double*, const double*, double*,
                                unsigned long long*, int*, hipStream_t);
 template <int DIM>
-hipError_t launch_pf_maxvel_dim(const PfGeom&, long long, const double*, unsigned long long*,
+hipError_t launch_pf_maxvel_dim(const PfGeom&, long long, const double*, unsigned long long*, int,
                                 hipStream_t);
 
 hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int stage,
@@ -65,10 +67,16 @@ hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, doub
 
 #define CMP_BY_DIM(dim, FN, ...) ((dim) == 2 ? FN<2>(__VA_ARGS__) : FN<3>(__VA_ARGS__))
 
-hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A,
+hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A, int comp,
                              hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_extract_dim, g, maxcells, U, A, s);
+    return CMP_BY_DIM(dim, launch_pf_extract_dim, g, maxcells, U, A, comp, s);
+}
+
+hipError_t launch_pf_scatter(int dim, const PfGeom& g, const long long* maxfaces, const double* src, double* dst,
+                             int comp, hipStream_t s)
+{
+    return CMP_BY_DIM(dim, launch_pf_scatter_dim, g, maxfaces, src, dst, comp, s);
 }
 
 hipError_t launch_pf_normals(int dim, const PfGeom& g, long long maxcells, const double* Psi,
@@ -98,9 +106,9 @@ hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, 
 }
 
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
-                            unsigned long long* bits, hipStream_t s)
+                            unsigned long long* bits, int nmass, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_maxvel_dim, g, maxcells, U, bits, s);
+    return CMP_BY_DIM(dim, launch_pf_maxvel_dim, g, maxcells, U, bits, nmass, s);
 }
 
 template <int DIM>
@@ -276,7 +284,8 @@ struct cmp_level {
     double* d_N = nullptr;          // cell normals
     double* d_J = nullptr;          // face J of the current sub-step
     double* d_H = nullptr;          // face Jhat
-    double* d_gq = nullptr;         // face Grad_Q (cmp_level_pf_evolve)
+    double* d_gq = nullptr;         // face Grad_Q (cmp_level_pf_evolve), one component per volume fraction
+    double* d_gq1 = nullptr;        // one phase's Grad_Q (N-phase model)
     unsigned long long* d_bits = nullptr;   // [0] max|u|, [1] max|Grad_Q| as bit patterns
     double* d_pfdt = nullptr;       // Phase-Field sub-step size read by the stage kernels
     double* h_pin = nullptr;        // pinned host scalars
@@ -585,7 +594,7 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
     // box table, with each box's place in a level face buffer (Phase-Field Grad_Q)
     std::vector<BoxDev> hb(nbox);
     std::vector<long long> foff((size_t)nbox * dim);
-    face_offsets(dim, nbox, boxes, 1, foff.data());
+    face_offsets(dim, nbox, boxes, params->nphase ? params->nphase : 1, foff.data());
     for (int b = 0; b < nbox; ++b) {
         hb[b] = make_boxdev(dim, boxes[b], ng);
         for (int d = 0; d < dim; ++d) hb[b].goff[d] = foff[(size_t)b * dim + d];
@@ -695,7 +704,7 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
 static void pf_free(cmp_level* L)
 {
     void* bufs[] = {L->d_abox, L->d_nbox, L->d_gprefix1, L->d_A0, L->d_A, L->d_P, L->d_N,
-                    L->d_J, L->d_H, L->d_gq, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
+                    L->d_J, L->d_H, L->d_gq, L->d_gq1, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
                     L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Z, L->d_ID, L->d_Bf, L->d_part, L->d_sc,
                     L->d_boxflag,
                     L->d_pf_tiles, L->d_pf_fill};
@@ -709,7 +718,7 @@ static void pf_free(cmp_level* L)
     L->ghost_src.clear();
     L->d_abox = L->d_nbox = nullptr;
     L->d_gprefix1 = nullptr;
-    L->d_A0 = L->d_A = L->d_P = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_pfdt = nullptr;
+    L->d_A0 = L->d_A = L->d_P = L->d_N = L->d_J = L->d_H = L->d_gq = L->d_gq1 = L->d_pfdt = nullptr;
     L->d_Mk = L->d_Wk = L->d_DJ = L->d_X = L->d_R = L->d_Pv = L->d_AP = L->d_Bf = nullptr;
     L->d_Z = L->d_ID = nullptr;
     L->d_part = L->d_sc = nullptr;
@@ -918,7 +927,6 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     if ((pf->mechanism != CMP_PF_ACDI && pf->mechanism != CMP_PF_CAC) || pf->id_flux != 0)
         return CMP_ERR_UNSUPPORTED;
     if (pf->explicit_rc != 0 && pf->explicit_rc != 1) return CMP_ERR_ARG;
-    if (L->P.nphase != 0) return CMP_ERR_UNSUPPORTED;   // the N-phase Phase-Field flux is not built
     // a mechanism that is not flux based has no Jhat (PhaseField_Parameter.H:138-140 aborts)
     if (pf->mechanism == CMP_PF_CAC && pf->explicit_rc == 1) return CMP_ERR_UNSUPPORTED;
     if (pf->flux_phase2flux != 0 && pf->flux_phase2flux != 1) return CMP_ERR_ARG;
@@ -968,7 +976,11 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     CMP_TRYP(hipMalloc(&L->d_N, sizeof(double) * nn));
     CMP_TRYP(hipMalloc(&L->d_J, sizeof(double) * nf));
     CMP_TRYP(hipMalloc(&L->d_H, sizeof(double) * nf));
-    CMP_TRYP(hipMalloc(&L->d_gq, sizeof(double) * nf));
+    // Grad_Q of the level: one component per volume fraction; the N-phase model computes each
+    // phase's into d_gq1 first
+    const int npf = L->P.nphase ? L->P.nphase : 1;
+    CMP_TRYP(hipMalloc(&L->d_gq, sizeof(double) * nf * npf));
+    if (npf > 1) CMP_TRYP(hipMalloc(&L->d_gq1, sizeof(double) * nf));
     CMP_TRYP(hipMalloc(&L->d_bits, sizeof(unsigned long long) * 2));
     CMP_TRYP(hipMalloc(&L->d_pfdt, sizeof(double)));
     CMP_TRYP(hipHostMalloc((void**)&L->h_pin, sizeof(double) * 4, hipHostMallocDefault));
@@ -1131,9 +1143,8 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
     const cmp_pf_params& pf = L->pf;
     CMP_TRY(launch_zero_u64(L->d_bits, 2, s));
     CMP_TRY(hipMemsetAsync(L->d_boxflag, 0, sizeof(int) * L->nbox, s));
-    CMP_TRY(launch_pf_maxvel(dim, L->pg, L->maxcells, U, L->d_bits, s));
-    CMP_TRY(launch_pf_extract(dim, L->pg, L->maxcells, U, L->d_A0, s));
-    CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A0, L->d_err, s));
+    const int nph = L->P.nphase;
+    CMP_TRY(launch_pf_maxvel(dim, L->pg, L->maxcells, U, L->d_bits, nph ? nph : 2, s));
     CMP_TRY(hipMemcpyAsync(L->h_pin, L->d_bits, sizeof(double), hipMemcpyDeviceToHost, s));
     CMP_TRY(hipStreamSynchronize(s));
     const double maxv = bits_to_double(L->h_pin);
@@ -1156,45 +1167,56 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
     const double mob_face = 0.5 * mob + 0.5 * mob;
     const bool cac = (pf.mechanism == CMP_PF_CAC);
     const bool explicit_rc = (pf.explicit_rc == 1);
-    CMP_TRY(hipMemcpyAsync(L->d_A, L->d_A0, sizeof(double) * L->na, hipMemcpyDeviceToDevice, s));
-    if (explicit_rc) CMP_TRY(hipMemsetAsync(L->d_H, 0, sizeof(double) * L->nfaces, s));
-    double t = 0.0;
-    const double T = dt;
+    // one volume fraction (two-phase model) or each phase's in turn (N-phase model: PhaseField_Mechanism
+    // and PhaseField_ACDI loop over the phases, :1036-1150, :1495-1644); Grad_Q's max and the box
+    // flags gather over the phases
+    const int npf = nph ? nph : 1;
     long long n = 0;
-    while (t < T) {
-        if (t + dt_m > T) dt_m = T - t;
-        if (!(dt_m > 0.0) || t + dt_m == t || n >= 10000000) return CMP_ERR_NAN;   // no progress
-        CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A, L->d_err, s));
-        if (cac) {
-            // normals of alpha itself; mech and W per cell, their level sums, alpha += dt_m (mech + W L)
-            CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_A, L->d_N, s));
-            CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
-            CMP_TRY(launch_pf_cac_mech(dim, L->pg, L->maxcells, L->d_A, L->d_N, L->d_Mk, L->d_Wk, mob, lambda,
-                                       eta, L->d_part, s));
-            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, L->d_sc + CG_SUM, s));
-            CMP_TRY(launch_pf_cac_update(dim, L->pg, L->maxcells, L->d_A, L->d_Mk, L->d_Wk, L->d_sc + CG_SUM,
-                                         dt_m, s));
-        } else {
-            CMP_TRY(launch_pf_map(L->d_A, L->d_P, L->na, eta, eta, 0, s));
-            CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_P, L->d_N, s));
-            CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
-            // Jhat is accumulated for the explicit RC only
-            CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, L->d_A, L->d_P, L->d_N, L->d_J,
-                                     explicit_rc ? L->d_H : nullptr, mob_face, eta, dt_m, s));
-            CMP_TRY(launch_pf_update(dim, L->pg, L->maxcells, L->d_A, L->d_J, dt_m, s));
+    for (int ph = 0; ph < npf; ++ph) {
+        double* out = (npf == 1) ? gradq : L->d_gq1;
+        CMP_TRY(launch_pf_extract(dim, L->pg, L->maxcells, U, L->d_A0, nph ? nph + dim + 1 + ph : 3 + dim, s));
+        CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A0, L->d_err, s));
+        CMP_TRY(hipMemcpyAsync(L->d_A, L->d_A0, sizeof(double) * L->na, hipMemcpyDeviceToDevice, s));
+        if (explicit_rc) CMP_TRY(hipMemsetAsync(L->d_H, 0, sizeof(double) * L->nfaces, s));
+        double t = 0.0, dt_s = dt_m;
+        const double T = dt;
+        n = 0;
+        while (t < T) {
+            if (t + dt_s > T) dt_s = T - t;
+            if (!(dt_s > 0.0) || t + dt_s == t || n >= 10000000) return CMP_ERR_NAN;   // no progress
+            CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A, L->d_err, s));
+            if (cac) {
+                // normals of alpha itself; mech and W per cell, their level sums, alpha += dt_m (mech + W L)
+                CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_A, L->d_N, s));
+                CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
+                CMP_TRY(launch_pf_cac_mech(dim, L->pg, L->maxcells, L->d_A, L->d_N, L->d_Mk, L->d_Wk, mob, lambda,
+                                           eta, L->d_part, s));
+                CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, L->d_sc + CG_SUM, s));
+                CMP_TRY(launch_pf_cac_update(dim, L->pg, L->maxcells, L->d_A, L->d_Mk, L->d_Wk, L->d_sc + CG_SUM,
+                                             dt_s, s));
+            } else {
+                CMP_TRY(launch_pf_map(L->d_A, L->d_P, L->na, eta, eta, 0, s));
+                CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, L->d_P, L->d_N, s));
+                CMP_TRY(launch_fill(dim, dim, L->fdN, L->nghost1, L->d_N, L->d_err, s));
+                // Jhat is accumulated for the explicit RC only
+                CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, L->d_A, L->d_P, L->d_N, L->d_J,
+                                         explicit_rc ? L->d_H : nullptr, mob_face, eta, dt_s, s));
+                CMP_TRY(launch_pf_update(dim, L->pg, L->maxcells, L->d_A, L->d_J, dt_s, s));
+            }
+            t += dt_s;
+            ++n;
         }
-        t += dt_m;
-        ++n;
-    }
-    if (explicit_rc) {
-        CMP_TRY(launch_scale(L->d_H, L->nfaces, 1.0 / dt, s));
-        // explicit Grad_Q from the step's alpha1 (PhaseField_GradQPF, :2315-2403), re-sharpened at
-        // eta_factor * eta
-        CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
-        CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, gradq, L->d_bits + 1, L->d_boxflag, s));
-    } else {
-        int r = pf_gradq_implicit(L, dt, eta, gradq, s);
-        if (r) return r;
+        if (explicit_rc) {
+            CMP_TRY(launch_scale(L->d_H, L->nfaces, 1.0 / dt, s));
+            // explicit Grad_Q from the step's alpha (PhaseField_GradQPF, :2315-2403), re-sharpened at
+            // eta_factor * eta
+            CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
+            CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, out, L->d_bits + 1, L->d_boxflag, s));
+        } else {
+            int r = pf_gradq_implicit(L, dt, eta, out, s);
+            if (r) return r;
+        }
+        if (npf > 1) CMP_TRY(launch_pf_scatter(dim, L->pg, L->maxfaces, out, gradq, ph, s));
     }
     CMP_TRY(hipMemcpyAsync(L->h_pin + 1, L->d_bits + 1, sizeof(double), hipMemcpyDeviceToHost, s));
     CMP_TRY(hipMemcpyAsync(L->h_boxflag, L->d_boxflag, sizeof(int) * L->nbox, hipMemcpyDeviceToHost, s));

@@ -854,40 +854,49 @@ __device__ __forceinline__ double np_riemann(const FaceSideN<DIM, NP>& L, const 
     }
 }
 
-// One face of the N-phase model (FVM_Conservative2FluxHyperbolic_K …NPhase.H:16-150): per side the
-// primitive reconstruction (WENO5 or MUSCL per primitive, then the consistent limiter; Godunov
-// without it), the Riemann solver; returns u*_n. s(t, n) as in face_flux.
-template <int DIM, int SCHEME, int RIEMANN, int NP, typename S>
-__device__ __forceinline__ double np_face_flux(const KParams& kp, int limiter, int dir, const S& s, double* F,
-                                               double& cmax, int& bad)
+// Primitive face states of the N-phase model (FVM_Reconstruction2FaceLR_Conservative_K …NPhase):
+// WENO5 or MUSCL per primitive, then the consistent limiter; Godunov without it. s(t, n) as in
+// face_flux.
+template <int DIM, int SCHEME, int NP, typename S>
+__device__ __forceinline__ void np_recon_face(const KParams& kp, int limiter, int dir, const S& s, double* LW,
+                                              double* RW, int& bad)
 {
     constexpr int N = NpIdx<DIM, NP>::N;
     const double dx = kp.dx[dir], idx = kp.inv_dx[dir];
-    FaceSideN<DIM, NP> L, R;
     if (SCHEME == 5) {
 #pragma unroll
         for (int n = 0; n < N; ++n) {
             const double v0 = s(0, n), v1 = s(1, n), v2 = s(2, n), v3 = s(3, n), v4 = s(4, n), v5 = s(5, n);
-            L.W[n] = weno5<1>(v0, v1, v2, v3, v4);
-            R.W[n] = weno5<-1>(v1, v2, v3, v4, v5);
+            LW[n] = weno5<1>(v0, v1, v2, v3, v4);
+            RW[n] = weno5<-1>(v1, v2, v3, v4, v5);
         }
-        np_limit_face<DIM, NP>(kp, L.W, [&](int n) { return s(2, n); }, dx, 1, bad);
-        np_limit_face<DIM, NP>(kp, R.W, [&](int n) { return s(3, n); }, dx, -1, bad);
+        np_limit_face<DIM, NP>(kp, LW, [&](int n) { return s(2, n); }, dx, 1, bad);
+        np_limit_face<DIM, NP>(kp, RW, [&](int n) { return s(3, n); }, dx, -1, bad);
     } else if (SCHEME == 1) {
 #pragma unroll
         for (int n = 0; n < N; ++n) {
-            L.W[n] = muscl(s(0, n), s(1, n), s(2, n), dx, idx, 1, limiter);
-            R.W[n] = muscl(s(1, n), s(2, n), s(3, n), dx, idx, -1, limiter);
+            LW[n] = muscl(s(0, n), s(1, n), s(2, n), dx, idx, 1, limiter);
+            RW[n] = muscl(s(1, n), s(2, n), s(3, n), dx, idx, -1, limiter);
         }
-        np_limit_face<DIM, NP>(kp, L.W, [&](int n) { return s(1, n); }, dx, 1, bad);
-        np_limit_face<DIM, NP>(kp, R.W, [&](int n) { return s(2, n); }, dx, -1, bad);
+        np_limit_face<DIM, NP>(kp, LW, [&](int n) { return s(1, n); }, dx, 1, bad);
+        np_limit_face<DIM, NP>(kp, RW, [&](int n) { return s(2, n); }, dx, -1, bad);
     } else {
 #pragma unroll
         for (int n = 0; n < N; ++n) {
-            L.W[n] = s(0, n);
-            R.W[n] = s(1, n);
+            LW[n] = s(0, n);
+            RW[n] = s(1, n);
         }
     }
+}
+
+// One face of the N-phase model (FVM_Conservative2FluxHyperbolic_K …NPhase.H:16-150): the face
+// states, then the Riemann solver; returns u*_n.
+template <int DIM, int SCHEME, int RIEMANN, int NP, typename S>
+__device__ __forceinline__ double np_face_flux(const KParams& kp, int limiter, int dir, const S& s, double* F,
+                                               double& cmax, int& bad)
+{
+    FaceSideN<DIM, NP> L, R;
+    np_recon_face<DIM, SCHEME, NP>(kp, limiter, dir, s, L.W, R.W, bad);
     np_make_side<DIM, NP>(kp, L, bad);
     np_make_side<DIM, NP>(kp, R, bad);
     return np_riemann<DIM, RIEMANN, NP>(L, R, dir, F, cmax);
@@ -1070,6 +1079,142 @@ __device__ __forceinline__ void pf_face_flux(const KParams& kp, int limiter, int
     auto no_normal = []() -> double { return 0.0; };
     recon_face<DIM, SCHEME, false>(kp, limiter, dir, s, no_normal, LW, RW);
     pf_flux<DIM, FORM>(kp, LW, RW, g, F, bad);
+}
+
+// ---- the N-phase Phase-Field flux ---------------------------------------------------------------
+// Phasic internal energies alpha_k rho_k e_k of one face state at its pressure
+// (Conservative2ConservativePhase_Compressible5Eq_NPhase.H:43-120): a phase below its p_min energy
+// is set to it and the energy it gained is taken from the other phases in proportion to their
+// volume fractions; repeated until none is below (each pass sets at least one more phase, so at
+// most NP passes).
+template <int DIM, int NP>
+__device__ __forceinline__ void np_pf_phase_energies(const KParams& kp, const double* W, double* e, int& bad)
+{
+    using I = NpIdx<DIM, NP>;
+    const double p = W[I::E];
+    double em[NP];
+    bool set[NP];
+    double rho_e = 0.0;
+    bool low = false;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const double phi = nasg_phi(W[I::A + k], W[k], kp.C[k], bad);
+        e[k] = phi * kp.A[k] * p + phi * kp.B[k] + W[k] * kp.D[k];
+        em[k] = phi * kp.A[k] * kp.pf_pmin[k] + phi * kp.B[k] + W[k] * kp.D[k];
+        rho_e += e[k];
+        set[k] = false;
+        low = low || (e[k] < em[k]);
+    }
+    for (int pass = 0; low && pass < NP; ++pass) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+            if (e[k] < em[k]) { e[k] = em[k]; set[k] = true; }
+        double se = 0.0, sa = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            se += e[k];
+            if (!set[k]) sa += W[I::A + k];
+        }
+        low = false;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            if (!set[k]) e[k] += W[I::A + k] / sa * (rho_e - se);
+        }
+#pragma unroll
+        for (int k = 0; k < NP; ++k) low = low || (e[k] < em[k]);
+    }
+}
+
+// FluxPhaseField of the N-phase model (…FluxPhaseField_Compressible5Eq_NPhase_PhaseField.H:12-79)
+// from primitive face states, with the efficient UD phase flux
+// (FluxPhasePhaseField_UD_Efficient_Compressible_NPhase_PhaseField.H:12-232): for each phase i the
+// phase-i quantities downwind of Grad_Q_i times (1 - alpha_i) upwind of it times Grad_Q_i; then,
+// with Sum_J the sum of the volume-fraction fluxes, every flux less the quantity upwind of Sum_J
+// times Sum_J. FORM 0: mixture momentum and energy (FluxPhaseField_Mixture …NPhase_PhaseField.H:12-104);
+// FORM 1: phasic sums (VectorPhase2Vector).
+template <int DIM, int FORM, int NP>
+__device__ __forceinline__ void np_pf_flux(const KParams& kp, const double* WL, const double* WR, const double* g,
+                                           double* F, int& bad)
+{
+    using I = NpIdx<DIM, NP>;
+    double aju[NP], Jr[NP], Ja[NP];
+    double SumJ = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        aju[i] = f_up(1.0 - WL[I::A + i], 1.0 - WR[I::A + i], g[i]);
+        Jr[i] = f_down(WL[i], WR[i], g[i]) * aju[i] * g[i];
+        Ja[i] = f_down(WL[I::A + i], WR[I::A + i], g[i]) * aju[i] * g[i];
+        SumJ += Ja[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+        Jr[i] = Jr[i] - f_up(WL[i], WR[i], SumJ) * SumJ;
+        Ja[i] = Ja[i] - f_up(WL[I::A + i], WR[I::A + i], SumJ) * SumJ;
+        F[i] = Jr[i];
+        F[I::A + i] = Ja[i];
+    }
+    if (FORM == 0) {
+        double M = 0.0;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) M += Jr[i];
+        double ke = 0.0;
+#pragma unroll
+        for (int m = 0; m < DIM; ++m) {
+            const double u = f_down(WL[I::MOM + m], WR[I::MOM + m], M);
+            F[I::MOM + m] = M * u;
+            ke = (m == 0) ? u * u : ke + u * u;
+        }
+        const double MeK = M * (0.5 * ke);
+        double Je = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            const double pk = f_down(WL[I::E], WR[I::E], Ja[k]);
+            const double Jphi = (kp.C[k] > 0.0) ? Ja[k] - kp.C[k] * Jr[k] : Ja[k];
+            Je += kp.A[k] * Jphi * pk + kp.B[k] * Jphi + kp.D[k] * Jr[k];
+        }
+        F[I::E] = MeK + Je;
+    } else {
+        double keL = 0.0, keR = 0.0;
+#pragma unroll
+        for (int m = 0; m < DIM; ++m) {
+            const double uL = WL[I::MOM + m], uR = WR[I::MOM + m];
+            keL = (m == 0) ? uL * uL : keL + uL * uL;
+            keR = (m == 0) ? uR * uR : keR + uR * uR;
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < NP; ++i) {
+                double Ji = f_down(WL[i] * uL, WR[i] * uR, g[i]) * aju[i] * g[i];
+                Ji = Ji - f_up(WL[i] * uL, WR[i] * uR, SumJ) * SumJ;
+                t += Ji;
+            }
+            F[I::MOM + m] = t;
+        }
+        double eL[NP], eR[NP];
+        np_pf_phase_energies<DIM, NP>(kp, WL, eL, bad);
+        np_pf_phase_energies<DIM, NP>(kp, WR, eR, bad);
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < NP; ++i) {
+            const double EL = 0.5 * WL[i] * keL + eL[i], ER = 0.5 * WR[i] * keR + eR[i];
+            double Ji = f_down(EL, ER, g[i]) * aju[i] * g[i];
+            Ji = Ji - f_up(EL, ER, SumJ) * SumJ;
+            t += Ji;
+        }
+        F[I::E] = t;
+    }
+}
+
+// FVM_Conservative2FluxPhaseField_K of the N-phase model (ID_Bound 0,
+// …FVM_Conservative2FluxPhaseField_K_Compressible5Eq_NPhase_PhaseField.H:15-79): the model's face
+// states, then its Phase-Field flux; g holds Grad_Q of every phase on this face.
+template <int DIM, int SCHEME, int FORM, int NP, typename S>
+__device__ __forceinline__ void np_pf_face_flux(const KParams& kp, int limiter, int dir, const S& s,
+                                                const double* g, double* F, int& bad)
+{
+    constexpr int N = NpIdx<DIM, NP>::N;
+    double LW[N], RW[N];
+    np_recon_face<DIM, SCHEME, NP>(kp, limiter, dir, s, LW, RW, bad);
+    np_pf_flux<DIM, FORM, NP>(kp, LW, RW, g, F, bad);
 }
 
 // Normal component along dir of the volume-fraction gradient at a face

@@ -98,19 +98,26 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     // Phase-Field flux: a face with Grad_Q == 0 has the flux 0 exactly (every term carries the
     // factor Grad_Q), so its reconstruction is skipped, and a tile direction without any nonzero
     // Grad_Q skips the staging as well (its flux differences are all zero)
+    // Grad_Q has one component per volume fraction (1, or NP for the N-phase model), each the
+    // box's whole face block of direction D
+    constexpr int NG = NP ? NP : 1;
     const int gnv0 = bx.hi[0] - bx.lo[0] + 1 + (D == 0 ? 1 : 0);
     const int gnv1 = bx.hi[1] - bx.lo[1] + 1 + (D == 1 ? 1 : 0);
-    auto gface = [&](int x, int y, int z) -> double {
+    const int gnv2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + (D == 2 ? 1 : 0) : 1;
+    const long long gcs = (long long)gnv0 * gnv1 * gnv2;
+    auto gface = [&](int x, int y, int z, int m) -> double {
         const long long gi = bx.goff[D] + (long long)(cx_.tlo[0] + x - bx.lo[0])
                            + (long long)gnv0 * ((cx_.tlo[1] + y - bx.lo[1])
                                                 + (long long)gnv1 * (cx_.tlo[2] + z - bx.lo[2]));
-        return cx_.gq[gi];
+        return cx_.gq[gi + m * gcs];
     };
     if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
         int nz = 0;
         for (int f = tid; f < F; f += NT) {
             const int x = f % F0, y = (f / F0) % F1, z = f / (F0 * F1);
-            if (x < f0 && y < f1 && z < f2) nz |= (gface(x, y, z) != 0.0);
+            if (x < f0 && y < f1 && z < f2)
+#pragma unroll
+                for (int m = 0; m < NG; ++m) nz |= (gface(x, y, z, m) != 0.0);
         }
         if (!__syncthreads_or(nz)) return;
     }
@@ -151,9 +158,19 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         const int base = x + E0 * (y + E1 * z);
         auto st = [&](int t, int q) -> double { return sW[q * RMAX + base + t * rs]; };
         if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
-            const double g = gface(x, y, z);
-            if (g != 0.0) {
-                pf_face_flux<DIM, SCHEME, RIEMANN - RIEMANN_PF_MIX>(kp, cx_.limiter, D, st, g, Fl, bad);
+            double g[NG];
+            bool any = false;
+#pragma unroll
+            for (int m = 0; m < NG; ++m) {
+                g[m] = gface(x, y, z, m);
+                any = any || (g[m] != 0.0);
+            }
+            if (any) {
+                if constexpr (NP > 0)
+                    np_pf_face_flux<DIM, SCHEME, RIEMANN - RIEMANN_PF_MIX, (NP > 0 ? NP : 2)>(kp, cx_.limiter, D,
+                                                                                          st, g, Fl, bad);
+                else
+                    pf_face_flux<DIM, SCHEME, RIEMANN - RIEMANN_PF_MIX>(kp, cx_.limiter, D, st, g[0], Fl, bad);
             } else {
 #pragma unroll
                 for (int q = 0; q < N; ++q) Fl[q] = 0.0;
@@ -650,6 +667,22 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
         const bool mix = lc.riemann == RIEMANN_PF_MIX;
 #define CMP_PF_ARGS kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout, dt, \
                     cmax_bits, err, s, gq
+        if (lc.nphase) {
+            // the N-phase model's Phase-Field flux: its schemes (Godunov, MUSCL, WENO5 with the
+            // consistent limiter), mixture or phasic form
+#define CMP_NPF_CASE(SC)                                                                              \
+            if (lc.scheme == SC) {                                                                    \
+                if (lc.nphase == 3) return mix ? stage4<DIM, SC, RIEMANN_PF_MIX, false, 3>(CMP_PF_ARGS)    \
+                                               : stage4<DIM, SC, RIEMANN_PF_PHASIC, false, 3>(CMP_PF_ARGS); \
+                if (lc.nphase == 2) return mix ? stage4<DIM, SC, RIEMANN_PF_MIX, false, 2>(CMP_PF_ARGS)    \
+                                               : stage4<DIM, SC, RIEMANN_PF_PHASIC, false, 2>(CMP_PF_ARGS); \
+            }
+            CMP_NPF_CASE(5)
+            CMP_NPF_CASE(1)
+            CMP_NPF_CASE(0)
+#undef CMP_NPF_CASE
+            return hipErrorInvalidValue;
+        }
         if (lc.scheme == 5) return mix ? stage4<DIM, 5, RIEMANN_PF_MIX, false>(CMP_PF_ARGS)
                                        : stage4<DIM, 5, RIEMANN_PF_PHASIC, false>(CMP_PF_ARGS);
         if (lc.scheme == 6) return mix ? stage4<DIM, 6, RIEMANN_PF_MIX, false>(CMP_PF_ARGS)

@@ -95,8 +95,12 @@ struct PfGeom {
     int periodic[3];
 };
 
-hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A,
+// volume fraction `comp` of the flow state -> an alpha-layout buffer
+hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A, int comp,
                              hipStream_t s);
+// a one-component face buffer (abox offsets) -> component comp of a multi-component one (hbox offsets)
+hipError_t launch_pf_scatter(int dim, const PfGeom& g, const long long* maxfaces, const double* src, double* dst,
+                             int comp, hipStream_t s);
 // psi (mode 0) or the re-sharpened alpha1 (mode 1) of every element of an alpha-layout buffer
 hipError_t launch_pf_map(const double* A, double* out, long long n, double eta, double eta2, int mode,
                          hipStream_t s);
@@ -112,7 +116,7 @@ hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, 
                            const double* H, double* gq, unsigned long long* gmax_bits, int* boxflag,
                            hipStream_t s);
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
-                            unsigned long long* bits, hipStream_t s);
+                            unsigned long long* bits, int nmass, hipStream_t s);
 
 // CAC mechanism and the implicit Grad_Q. Block partial sums go to `part` (one or two per block,
 // gridDim.x * nbox blocks), reduced in a fixed order by launch_reduce_sums into sums[0..n-1].

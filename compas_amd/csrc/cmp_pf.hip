@@ -109,14 +109,31 @@ __device__ __forceinline__ void block_sum_to(double v, double* part, long long i
 
 }  // namespace
 
-// alpha1 of the flow state's valid cells -> the alpha buffer
+// One volume fraction (state component comp) of the flow state's valid cells -> the alpha buffer
 template <int DIM>
-__global__ __launch_bounds__(NT) void k_pf_extract(PfGeom g, const double* __restrict__ U, double* A)
+__global__ __launch_bounds__(NT) void k_pf_extract(PfGeom g, const double* __restrict__ U, double* A, int comp)
 {
     const BoxDev hb = g.hbox[blockIdx.y], ab = g.abox[blockIdx.y];
     int i, j, k;
     if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
-    A[ab.off + fab_index(ab, i, j, k)] = U[hb.off + fab_index(hb, i, j, k) + (3 + DIM) * hb.sn];
+    A[ab.off + fab_index(ab, i, j, k)] = U[hb.off + fab_index(hb, i, j, k) + comp * hb.sn];
+}
+
+// One phase's Grad_Q (a one-component face buffer, abox offsets) -> component comp of the level's
+// Grad_Q face buffer (hbox offsets, ncomp components per box and direction)
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_pf_scatter(PfGeom g, const double* __restrict__ src, double* dst,
+                                                   int comp)
+{
+    const BoxDev hb = g.hbox[blockIdx.y], ab = g.abox[blockIdx.y];
+    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        long long nf = 1;
+#pragma unroll
+        for (int e = 0; e < DIM; ++e) nf *= ab.hi[e] - ab.lo[e] + 1 + (e == d ? 1 : 0);
+        if (q < nf) dst[hb.goff[d] + comp * nf + q] = src[ab.goff[d] + q];
+    }
 }
 
 // Pointwise maps of a whole alpha-layout buffer (valid and ghost cells): mode 0 psi =
@@ -238,8 +255,9 @@ __global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restr
 
 // max over valid cells of |u| (MaxVelocity, :3541-3579; Conservative2Velocity_Compressible5Eq.H:15-40)
 template <int DIM>
+// nmass partial densities come first (2, or the N-phase model's NP), then the momenta
 __global__ __launch_bounds__(NT) void k_pf_maxvel(PfGeom g, const double* __restrict__ U,
-                                                  unsigned long long* bits)
+                                                  unsigned long long* bits, int nmass)
 {
     const BoxDev hb = g.hbox[blockIdx.y];
     int i, j, k;
@@ -247,11 +265,12 @@ __global__ __launch_bounds__(NT) void k_pf_maxvel(PfGeom g, const double* __rest
     if (cell_of<DIM>(hb, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
         const double* Ub = U + hb.off;
         const long long c = fab_index(hb, i, j, k);
-        const double rho = Ub[c] + Ub[c + hb.sn];
+        double rho = Ub[c];
+        for (int q = 1; q < nmass; ++q) rho += Ub[c + q * hb.sn];
         double s = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
-            const double u = Ub[c + (2 + d) * hb.sn] / rho;
+            const double u = Ub[c + (nmass + d) * hb.sn] / rho;
             s = (d == 0) ? u * u : s + u * u;
         }
         v = sqrt(s);
@@ -610,10 +629,20 @@ static dim3 grid_of(long long per_box, int nbox)
 }
 
 template <int DIM>
-hipError_t launch_pf_extract_dim(const PfGeom& g, long long maxcells, const double* U, double* A,
+hipError_t launch_pf_extract_dim(const PfGeom& g, long long maxcells, const double* U, double* A, int comp,
                                  hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_extract<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, U, A);
+    hipLaunchKernelGGL((k_pf_extract<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, U, A, comp);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_pf_scatter_dim(const PfGeom& g, const long long* maxfaces, const double* src, double* dst,
+                                 int comp, hipStream_t s)
+{
+    long long m = 0;
+    for (int d = 0; d < DIM; ++d) m = (maxfaces[d] > m) ? maxfaces[d] : m;
+    hipLaunchKernelGGL((k_pf_scatter<DIM>), grid_of(m, g.nbox), dim3(NT), 0, s, g, src, dst, comp);
     return hipGetLastError();
 }
 
@@ -665,9 +694,9 @@ hipError_t launch_pf_gradq_dim(const PfGeom& g, const long long* maxfaces, const
 
 template <int DIM>
 hipError_t launch_pf_maxvel_dim(const PfGeom& g, long long maxcells, const double* U,
-                                unsigned long long* bits, hipStream_t s)
+                                unsigned long long* bits, int nmass, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_maxvel<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, U, bits);
+    hipLaunchKernelGGL((k_pf_maxvel<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, U, bits, nmass);
     return hipGetLastError();
 }
 
@@ -790,7 +819,9 @@ hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s)
 #endif
 
 template hipError_t launch_pf_extract_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
-                                                          double*, hipStream_t);
+                                                          double*, int, hipStream_t);
+template hipError_t launch_pf_scatter_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
+                                                          double*, int, hipStream_t);
 template hipError_t launch_pf_normals_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
                                                           double*, hipStream_t);
 template hipError_t launch_pf_jfaces_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
@@ -802,7 +833,7 @@ template hipError_t launch_pf_gradq_dim<CMP_KERNEL_DIM>(const PfGeom&, const lon
                                                         const double*, double*, unsigned long long*, int*,
                                                         hipStream_t);
 template hipError_t launch_pf_maxvel_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
-                                                         unsigned long long*, hipStream_t);
+                                                         unsigned long long*, int, hipStream_t);
 template hipError_t launch_pf_cac_mech_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
                                                            const double*, double*, double*, double, double,
                                                            double, double*, hipStream_t);

@@ -219,7 +219,9 @@ int cmp_level_check(cmp_level* level, void* stream);
 int cmp_level_info(const cmp_level* level, int* ntiles, int* tile_x, int* tile_y, int* tile_z);
 
 /* ---- phase-field interface regularization of a level ------------------------------------------
- * Face buffers below follow cmp_face_layout with one component (NPHASE = 1 for two phases). */
+ * Grad_Q face buffers below follow cmp_face_layout with one component per volume fraction: one for
+ * the two-phase model (NPHASE = 1), nphase for the N-phase model (component k is phase k's Grad_Q,
+ * computed from alpha_k as the two-phase model computes it from alpha1). */
 
 /* Enables the Phase-Field step on a level and allocates its device work buffers (two copies of
  * alpha1 and the cell normals with one ghost cell; J, Jhat and Grad_Q on the faces; for CAC the
@@ -241,7 +243,7 @@ int cmp_level_pf_rhs(cmp_level* level, const double* U, const double* gradq, dou
  * PhaseField_Mechanism :1007-1152, with PhaseField_Normal :1958-2032) accumulating Jhat or
  * Div_Jhat, then PhaseField_GradQPF with the explicit RC (:2315-2403) or the implicit one
  * (:2133-2310: div(b grad Q) = Div_Jhat solved by conjugate gradients to the rounding floor where
- * the reference calls AMReX's MLMG).
+ * the reference calls AMReX's MLMG); for the N-phase model once per phase.
  * gradq: device face buffer (written); *gradq_max (host) = max |Grad_Q| over all faces;
  * info (host, 4 doubles, or NULL) = max|u|, mobility, eta, number of mechanism sub-steps.
  * Synchronises the stream. */
