@@ -18,7 +18,7 @@ template <int DIM>
 hipError_t launch_stage_dim(const LaunchCfg&, const KParams&, int, int, const TileDev*, int,
                             const BoxDev*, const BoxDev*, int, const double*, const double*,
                             double*, const double*, unsigned long long*, int*, hipStream_t,
-                            const double*);
+                            const double*, const FillDev*);
 template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg&, const KParams&, const BoxDev&, int, const double*,
                             double*, double*, hipStream_t);
@@ -49,13 +49,14 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
                         const TileDev* tiles, int ntiles, const BoxDev* boxes,
                         const BoxDev* single_box, int single_ntiles, const double* Us,
                         const double* Un, double* Uout, const double* dt,
-                        unsigned long long* cmax_bits, int* err, hipStream_t s, const double* gradq)
+                        unsigned long long* cmax_bits, int* err, hipStream_t s, const double* gradq,
+                        const FillDev* fuse)
 {
     if (lc.dim == 2)
         return launch_stage_dim<2>(lc, kp, mode, stage, tiles, ntiles, boxes, single_box,
-                                   single_ntiles, Us, Un, Uout, dt, cmax_bits, err, s, gradq);
+                                   single_ntiles, Us, Un, Uout, dt, cmax_bits, err, s, gradq, fuse);
     return launch_stage_dim<3>(lc, kp, mode, stage, tiles, ntiles, boxes, single_box, single_ntiles,
-                               Us, Un, Uout, dt, cmax_bits, err, s, gradq);
+                               Us, Un, Uout, dt, cmax_bits, err, s, gradq, fuse);
 }
 
 hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, double* U, int* err,
@@ -250,6 +251,7 @@ struct cmp_level {
     KParams kp;
     LaunchCfg lc;
     BoxDev* d_boxes = nullptr;
+    FillDev* d_fd = nullptr;        // device copy of fd (the stage kernel's fused ghost reads)
     TileDev* d_tiles = nullptr;
     int ntiles = 0;
     long long* d_gprefix = nullptr;
@@ -312,8 +314,6 @@ struct cmp_level {
     int* h_boxflag = nullptr;       // pinned copy
     TileDev* d_pf_tiles = nullptr;  // tiles of the selected boxes
     TileDev* h_pf_tiles = nullptr;  // pinned staging of the list
-    int* d_pf_fill = nullptr;       // boxes whose ghosts the Phase-Field sub-steps fill
-    int* h_pf_fill = nullptr;       // pinned staging of the list
     int pf_ntiles = 0, pf_nfill = 0;
 
     // host copies of the level tables (tiles box by box, the owner map)
@@ -609,7 +609,8 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
         for (int k = klo; k <= khi; k += (dim == 3 ? tz : 1))
             for (int j = B.lo[1]; j <= B.hi[1]; j += ty)
                 for (int i = B.lo[0]; i <= B.hi[0]; i += tx)
-                    ht.push_back(TileDev{b, {i, j, k}});
+                    ht.push_back(TileDev{b, {i, j, k}, {{HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP},
+                                                         {HALO_LOOKUP, HALO_LOOKUP}}});
     }
     L->ntiles = (int)ht.size();
     L->tile_start.assign(nbox + 1, 0);
@@ -658,6 +659,55 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
                     owner[q] = b;
                 }
     }
+    // halo sources of the tiles on box faces (the ghost fill's rules, resolved per tile side): the
+    // ng cells beyond the face, across the tile's transverse extent, lie in one box (after the
+    // periodic wrap), or beyond a non-periodic domain face whose BC clamps (codes 0, 2) or
+    // mirrors (3, 4: normal momentum negated; 5: all momenta) within the tile's own box; anything
+    // else is looked up cell by cell
+    {
+        auto owner_of = [&](const int* c) -> int {
+            long long q = 0;
+            for (int d = dim - 1; d >= 0; --d) q = q * nb[d] + (c[d] - geom->domain_lo[d]) / bs;
+            return owner[q];
+        };
+        for (TileDev& t : ht) {
+            const cmp_box& B = boxes[t.box];
+            int tlo[3] = {0, 0, 0}, thi[3] = {0, 0, 0};
+            const int te[3] = {tx, ty, tz};
+            for (int d = 0; d < dim; ++d) {
+                tlo[d] = t.lo[d];
+                thi[d] = std::min(t.lo[d] + te[d] - 1, B.hi[d]);
+            }
+            for (int d = 0; d < dim; ++d)
+                for (int side = 0; side < 2; ++side) {
+                    if ((side == 0 && tlo[d] != B.lo[d]) || (side == 1 && thi[d] != B.hi[d])) continue;
+                    const int dlo = geom->domain_lo[d], dhi = geom->domain_hi[d];
+                    int rlo[3], rhi[3];
+                    for (int e = 0; e < 3; ++e) { rlo[e] = tlo[e]; rhi[e] = thi[e]; }
+                    if (side == 0) { rlo[d] = B.lo[d] - ng; rhi[d] = B.lo[d] - 1; }
+                    else { rlo[d] = B.hi[d] + 1; rhi[d] = B.hi[d] + ng; }
+                    const bool below = rhi[d] < dlo, above = rlo[d] > dhi;
+                    const bool inside = rlo[d] >= dlo && rhi[d] <= dhi;
+                    int code = HALO_LOOKUP;
+                    if (inside || ((below || above) && geom->is_periodic[d])) {
+                        if (below) { rlo[d] += dhi - dlo + 1; rhi[d] += dhi - dlo + 1; }
+                        if (above) { rlo[d] -= dhi - dlo + 1; rhi[d] -= dhi - dlo + 1; }
+                        const int o = owner_of(rlo);
+                        bool ok = o >= 0;
+                        for (int e = 0; e < dim && ok; ++e)
+                            ok = rlo[e] >= boxes[o].lo[e] && rhi[e] <= boxes[o].hi[e];
+                        if (ok) code = o;
+                    } else if (below || above) {
+                        const int bc = below ? geom->lo_bc[d] : geom->hi_bc[d];
+                        const bool fits = B.hi[d] - B.lo[d] + 1 >= ng;
+                        if (bc == 0 || bc == 2) code = HALO_CLAMP;
+                        else if ((bc == 3 || bc == 4) && fits) code = HALO_MIRROR;
+                        else if (bc == 5 && fits) code = HALO_MIRROR_ALL;
+                    }
+                    t.halo[d][side] = code;
+                }
+        }
+    }
 
     int rc = CMP_OK;
 #define CMP_TRYL(x) do { rc = check_hip(x); if (rc) { cmp_level_destroy(L); return rc; } } while (0)
@@ -696,6 +746,9 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
         fd.lo_bc[d] = geom->lo_bc[d];
         fd.hi_bc[d] = geom->hi_bc[d];
     }
+    int rc2 = check_hip(hipMalloc(&L->d_fd, sizeof(FillDev)));
+    if (!rc2) rc2 = check_hip(hipMemcpy(L->d_fd, &fd, sizeof(FillDev), hipMemcpyHostToDevice));
+    if (rc2) { cmp_level_destroy(L); return rc2; }
     *out = L;
     return CMP_OK;
 }
@@ -707,13 +760,13 @@ static void pf_free(cmp_level* L)
                     L->d_J, L->d_H, L->d_gq, L->d_gq1, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
                     L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Z, L->d_ID, L->d_Bf, L->d_part, L->d_sc,
                     L->d_boxflag,
-                    L->d_pf_tiles, L->d_pf_fill};
+                    L->d_pf_tiles};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
-    void* pins[] = {L->h_pin, L->h_boxflag, L->h_pf_tiles, L->h_pf_fill};
+    void* pins[] = {L->h_pin, L->h_boxflag, L->h_pf_tiles};
     for (void* p : pins)
         if (p) (void)hipHostFree(p);
-    L->d_boxflag = L->d_pf_fill = L->h_boxflag = L->h_pf_fill = nullptr;
+    L->d_boxflag = L->h_boxflag = nullptr;
     L->d_pf_tiles = L->h_pf_tiles = nullptr;
     L->ghost_src.clear();
     L->d_abox = L->d_nbox = nullptr;
@@ -734,7 +787,7 @@ int cmp_level_destroy(cmp_level* L)
     if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
     if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
-    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err};
+    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete L;
@@ -781,17 +834,19 @@ int cmp_level_rk3_stage(cmp_level* L, int stage, const double* Us, const double*
     return CMP_OK;
 }
 
+// The three stages read the cells beyond their boxes where the ghost fill would take them from
+// (the fill fused into the stage kernel's staging), so no stage input has its ghosts written.
 int cmp_level_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, const double* dt_dev,
                        double* cmax, void* stream)
 {
     if (!L || !Un || !U1 || !U2 || !dt_dev) return CMP_ERR_ARG;
-    int r;
-    if ((r = cmp_level_fill_ghosts(L, Un, stream))) return r;
-    if ((r = cmp_level_rk3_stage(L, 0, Un, Un, U1, dt_dev, cmax, stream))) return r;
-    if ((r = cmp_level_fill_ghosts(L, U1, stream))) return r;
-    if ((r = cmp_level_rk3_stage(L, 1, U1, Un, U2, dt_dev, cmax, stream))) return r;
-    if ((r = cmp_level_fill_ghosts(L, U2, stream))) return r;
-    if ((r = cmp_level_rk3_stage(L, 2, U2, Un, Un, dt_dev, cmax, stream))) return r;
+    hipStream_t s = (hipStream_t)stream;
+    double* Us[3] = {Un, U1, U2};
+    double* Uo[3] = {U1, U2, Un};
+    for (int stage = 0; stage < 3; ++stage)
+        CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0,
+                             Us[stage], Un, Uo[stage], dt_dev, (unsigned long long*)cmax, L->d_err, s, nullptr,
+                             L->d_fd));
     return CMP_OK;
 }
 
@@ -1010,8 +1065,6 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     CMP_TRYP(hipHostMalloc((void**)&L->h_boxflag, sizeof(int) * nbox, hipHostMallocDefault));
     CMP_TRYP(hipMalloc(&L->d_pf_tiles, sizeof(TileDev) * std::max(1, L->ntiles)));
     CMP_TRYP(hipHostMalloc((void**)&L->h_pf_tiles, sizeof(TileDev) * std::max(1, L->ntiles), hipHostMallocDefault));
-    CMP_TRYP(hipMalloc(&L->d_pf_fill, sizeof(int) * nbox));
-    CMP_TRYP(hipHostMalloc((void**)&L->h_pf_fill, sizeof(int) * nbox, hipHostMallocDefault));
 #undef CMP_TRYP
     L->ghost_src = ghost_sources(L);
     long long ncell = 0;
@@ -1231,19 +1284,18 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
     return CMP_OK;
 }
 
-// One Phase-Field RK3 sub-step over the given tiles; the ghosts of the `fill` boxes (all boxes when
-// fill is null) are filled before each stage.
+// One Phase-Field RK3 sub-step over the given tiles; cells beyond a tile's box are read where the
+// ghost fill would take them from.
 static int pf_rk3(cmp_level* L, double* Un, double* U1, double* U2, const double* gradq, double pf_dt,
-                  const TileDev* tiles, int ntiles, const int* fill, int nfill, hipStream_t s)
+                  const TileDev* tiles, int ntiles, hipStream_t s)
 {
     CMP_TRY(launch_setd(L->d_pfdt, pf_dt, s));
     double* Us[3] = {Un, U1, U2};
     double* Uo[3] = {U1, U2, Un};
     for (int stage = 0; stage < 3; ++stage) {
-        CMP_TRY(launch_fill(L->P.dim, L->P.nstate, L->fd, L->nghost, Us[stage], L->d_err, s, fill, nfill));
         if (ntiles > 0)
             CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RK, stage, tiles, ntiles, L->d_boxes, nullptr, 0,
-                                 Us[stage], Un, Uo[stage], L->d_pfdt, nullptr, L->d_err, s, gradq));
+                                 Us[stage], Un, Uo[stage], L->d_pfdt, nullptr, L->d_err, s, gradq, L->d_fd));
     }
     return CMP_OK;
 }
@@ -1253,7 +1305,7 @@ int cmp_level_pf_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, cons
 {
     if (!L || !Un || !U1 || !U2 || !gradq) return CMP_ERR_ARG;
     if (!L->pf_on) return CMP_ERR_ARG;
-    return pf_rk3(L, Un, U1, U2, gradq, pf_dt, L->d_tiles, L->ntiles, nullptr, 0, (hipStream_t)stream);
+    return pf_rk3(L, Un, U1, U2, gradq, pf_dt, L->d_tiles, L->ntiles, (hipStream_t)stream);
 }
 
 // The boxes the Phase-Field sub-steps of one step touch, from the box flags of the last Grad_Q:
@@ -1266,7 +1318,7 @@ static int pf_select_boxes(cmp_level* L, hipStream_t s)
     int nfill = 0;
     for (int b = 0; b < L->nbox; ++b) {
         if (!L->h_boxflag[b]) continue;
-        L->h_pf_fill[nfill++] = b;
+        ++nfill;
         need[b] = 1;
         for (int o : L->ghost_src[b]) need[o] = 1;
     }
@@ -1277,7 +1329,6 @@ static int pf_select_boxes(cmp_level* L, hipStream_t s)
     }
     L->pf_nfill = nfill;
     L->pf_ntiles = nt;
-    if (nfill > 0) CMP_TRY(hipMemcpyAsync(L->d_pf_fill, L->h_pf_fill, sizeof(int) * nfill, hipMemcpyHostToDevice, s));
     if (nt > 0) CMP_TRY(hipMemcpyAsync(L->d_pf_tiles, L->h_pf_tiles, sizeof(TileDev) * nt, hipMemcpyHostToDevice, s));
     return CMP_OK;
 }
@@ -1311,7 +1362,7 @@ int cmp_level_pf_evolve(cmp_level* L, double* Un, double* U1, double* U2, const 
         if (pf_time + dt0 > dt - eps) dt0 = dt - pf_time;
         prev = dt0;
         if (L->pf_nfill > 0 &&
-            (r = pf_rk3(L, Un, U1, U2, L->d_gq, dt0, L->d_pf_tiles, L->pf_ntiles, L->d_pf_fill, L->pf_nfill, s)))
+            (r = pf_rk3(L, Un, U1, U2, L->d_gq, dt0, L->d_pf_tiles, L->pf_ntiles, s)))
             return r;
         pf_time += dt0;
         ++n;

@@ -60,7 +60,135 @@ struct StageCtx {
     int tid, cx, cy, cz;
     bool valid;
     const double* __restrict__ gq;   // Grad_Q face buffer (Phase-Field flux only)
+    // fused ghost fill (fd non-null, in device memory): cells outside the box are read where the
+    // ghost fill would take them from (Ul: the level buffer), so the ghost cells of the stage's
+    // input are never read
+    const FillDev* __restrict__ fd;
+    const double* __restrict__ Ul;
+    const BoxDev* __restrict__ boxes;
+    int halo[3][2];                  // the tile's halo sources (HaloSrc), by direction and side
 };
+
+// Where the ghost fill takes cell (ci, cj, ck) of a level from (FillPatchSingleLevel and the physical
+// BCs): the cell itself wrapped on periodic sides; on the others foextrap / first-order extrapolation
+// (codes 0, 2) clamps to the domain, the reflecting codes (3 symmetry, 4 slip wall, 5 no-slip wall)
+// mirror it and flip the normal momentum (all momenta for code 5). si: the source cell's index in the
+// level buffer, ssn: its box's component stride, flip: bit m set = negate momentum m. Returns 0, or
+// CMP_ERR_BC (a code without a fill here) / CMP_ERR_LAYOUT (no box owns the source cell).
+template <int DIM>
+__device__ __forceinline__ int fill_source(const FillDev& fd, int ci, int cj, int ck, long long& si, long long& ssn,
+                                           unsigned& flip)
+{
+    int src[3] = {ci, cj, ck};
+    flip = 0u;
+    int ok = 1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const int dlo = fd.dlo[d], dhi = fd.dhi[d];
+        if (src[d] >= dlo && src[d] <= dhi) continue;
+        if (fd.periodic[d]) {
+            const int per = dhi - dlo + 1;
+            src[d] += (src[d] < dlo) ? per : -per;
+            continue;
+        }
+        const int code = (src[d] < dlo) ? fd.lo_bc[d] : fd.hi_bc[d];
+        if (code == 0 || code == 2) {
+            src[d] = (src[d] < dlo) ? dlo : dhi;
+        } else if (code >= 3 && code <= 5) {
+            src[d] = (src[d] < dlo) ? (2 * dlo - 1 - src[d]) : (2 * dhi + 1 - src[d]);
+            for (int m = 0; m < DIM; ++m)
+                if (m == d || code == 5) flip ^= (1u << m);
+        } else {
+            ok = 0;
+        }
+    }
+    if (!ok) return CMP_ERR_BC;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+        if (src[d] < fd.dlo[d] || src[d] > fd.dhi[d]) return CMP_ERR_LAYOUT;
+    int bi = 0;
+    for (int d = DIM - 1; d >= 0; --d)
+        bi = bi * fd.nb[d] + (src[d] - fd.dlo[d]) / fd.bs;
+    const int own = fd.owner[bi];
+    if (own < 0) return CMP_ERR_LAYOUT;
+    const BoxDev so = fd.boxes[own];
+    si = so.off + fab_index(so, src[0], src[1], src[2]);
+    ssn = so.sn;
+    return 0;
+}
+
+// fill_source for the stage kernel: out of line (only the tiles on box faces take it, and inlined
+// at every read site it would crowd the face computation's registers), the table read from device
+// memory.
+struct GhostSrc {
+    long long si, sn;
+    unsigned flip;
+    int err;
+};
+template <int DIM>
+__device__ __noinline__ GhostSrc ghost_source(const FillDev* __restrict__ fd, int ci, int cj, int ck)
+{
+    GhostSrc g;
+    g.err = fill_source<DIM>(*fd, ci, cj, ck, g.si, g.sn, g.flip);
+    return g;
+}
+
+// A cell outside the tile's box, read as the ghost fill would write it (the box's own stale ghost
+// value when the fill has no source for it; the error is then reported as the fill reports it).
+template <int DIM, int N>
+__device__ __forceinline__ void read_ghost(const StageCtx<DIM>& cx_, int i, int j, int k, long long c, double* Uc,
+                                           int& bad)
+{
+    constexpr int MOM = (N == 4 + DIM) ? 2 : (N - DIM - 1) / 2;
+    const GhostSrc g = ghost_source<DIM>(cx_.fd, i, j, k);
+    if (g.err) {
+        bad = g.err;
+#pragma unroll
+        for (int q = 0; q < N; ++q) Uc[q] = cx_.Ub[c + q * cx_.bx.sn];
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const double v = cx_.Ul[g.si + q * g.sn];
+        Uc[q] = (q >= MOM && q < MOM + DIM && ((g.flip >> (q - MOM)) & 1u)) ? -v : v;
+    }
+}
+
+// A cell beyond the tile's box along D (side 0 low, 1 high) by the tile's halo source: a
+// neighbour box's cell (periodic image along D), the own box's boundary cell (clamp) or mirror
+// image (normal or all momenta negated), or the cell-by-cell lookup.
+template <int DIM, int N, int D>
+__device__ __forceinline__ void read_halo(const StageCtx<DIM>& cx_, int side, int i, int j, int k, long long c,
+                                          double* Uc, int& bad)
+{
+    constexpr int MOM = (N == 4 + DIM) ? 2 : (N - DIM - 1) / 2;
+    const int code = cx_.halo[D][side];
+    int s[3] = {i, j, k};
+    const int dlo = cx_.fd->dlo[D], dhi = cx_.fd->dhi[D];
+    if (code >= 0) {
+        const BoxDev& nb = cx_.boxes[code];
+        if (s[D] < dlo) s[D] += dhi - dlo + 1;
+        else if (s[D] > dhi) s[D] -= dhi - dlo + 1;
+        const double* __restrict__ src = cx_.Ul + nb.off + fab_index(nb, s[0], s[1], s[2]);
+        const long long sn = nb.sn;
+#pragma unroll
+        for (int q = 0; q < N; ++q) Uc[q] = src[q * sn];
+    } else if (code == HALO_LOOKUP) {
+        read_ghost<DIM, N>(cx_, i, j, k, c, Uc, bad);
+    } else {
+        if (code == HALO_CLAMP) s[D] = (s[D] < dlo) ? dlo : dhi;
+        else s[D] = (s[D] < dlo) ? 2 * dlo - 1 - s[D] : 2 * dhi + 1 - s[D];
+        const BoxDev& bx = cx_.bx;
+        const long long cc = fab_index(bx, s[0], s[1], s[2]);
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            const double v = cx_.Ub[cc + q * bx.sn];
+            const bool neg = (code == HALO_MIRROR && q == MOM + D) ||
+                             (code == HALO_MIRROR_ALL && q >= MOM && q < MOM + DIM);
+            Uc[q] = neg ? -v : v;
+        }
+    }
+}
 
 // One direction of the stage kernel (D known at compile time): stage the primitives of the tile
 // plus G cells on both sides along D, compute every face along D once, and accumulate the flux and
@@ -132,8 +260,13 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         const int k = cx_.tlo[2] + z - (D == 2 ? G : 0);
         const long long c = fab_index(bx, i, j, k);
         double Uc[N], Wc[N];
+        const int cd = (D == 0) ? i : ((D == 1) ? j : k);
+        if (cx_.fd && (cd < bx.lo[D] || cd > bx.hi[D])) {
+            read_halo<DIM, N, D>(cx_, cd > bx.hi[D] ? 1 : 0, i, j, k, c, Uc, bad);
+        } else {
 #pragma unroll
-        for (int q = 0; q < N; ++q) Uc[q] = cx_.Ub[c + q * bx.sn];
+            for (int q = 0; q < N; ++q) Uc[q] = cx_.Ub[c + q * bx.sn];
+        }
         if constexpr (NP > 0)
             np_cons2prim<DIM, NP>(kp, Uc, Wc, bad);
         else
@@ -197,6 +330,12 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
                     if (D == 0) { pj += o1; pk += (DIM == 3) ? o2 : 0; }
                     else if (D == 1) { pi += o1; pk += (DIM == 3) ? o2 : 0; }
                     else { pi += o1; pj += o2; }
+                    if (cx_.fd && (pi < bx.lo[0] || pi > bx.hi[0] || pj < bx.lo[1] || pj > bx.hi[1] ||
+                                   (DIM == 3 && (pk < bx.lo[2] || pk > bx.hi[2])))) {
+                        const GhostSrc gs = ghost_source<DIM>(cx_.fd, pi, pj, pk);
+                        if (gs.err == 0) return cx_.Ul[gs.si + (3 + DIM) * gs.sn];
+                        bad = gs.err;
+                    }
                     return cx_.Ub[fab_index(bx, pi, pj, pk) + (3 + DIM) * bx.sn];
                 };
                 auto normal = [&]() -> double { return face_normal_component<DIM>(kp, D, aa); };
@@ -252,7 +391,7 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
              BoxDev sbox, int sntx, int snty, int single,
              const double* __restrict__ Us, const double* Un, double* Uout,
              const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err,
-             const double* __restrict__ gq)
+             const double* __restrict__ gq, const FillDev* __restrict__ fd)
 {
     constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
@@ -270,6 +409,7 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
 
     const int tid = threadIdx.x;
     int tlo[3];
+    int halo[3][2] = {{HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}};
     BoxDev bx;
     if (single) {
         int b = blockIdx.x;
@@ -287,6 +427,13 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
         tlo[0] = td.lo[0];
         tlo[1] = td.lo[1];
         tlo[2] = td.lo[2];
+        if (fd) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                halo[d][0] = td.halo[d][0];
+                halo[d][1] = td.halo[d][1];
+            }
+        }
     }
     int n[3];
     n[0] = min(TX, bx.hi[0] - tlo[0] + 1);
@@ -305,7 +452,8 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
     for (int q = 0; q < N; ++q) acc[q] = 0.0;
 
     StageCtx<DIM> ctx{kp, limiter, bx, Ub, {tlo[0], tlo[1], tlo[2]}, {n[0], n[1], n[2]}, tid, cx, cy, cz,
-                      valid, gq};
+                      valid, gq, fd, Us, boxes,
+                      {{halo[0][0], halo[0][1]}, {halo[1][0], halo[1][1]}, {halo[2][0], halo[2][1]}}};
     dir_pass<DIM, SCHEME, RIEMANN, THINC, 0, NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
     if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1, NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
     if (DIM > 2)
@@ -505,45 +653,14 @@ void k_fill(FillDev fd, const int* __restrict__ list, int box0, double* U, int* 
             }
         }
         const int cell[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
-        int src[3] = {cell[0], cell[1], cell[2]};
-        unsigned flip = 0u;
-        int ok = 1;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            const int dlo = fd.dlo[d], dhi = fd.dhi[d];
-            if (src[d] >= dlo && src[d] <= dhi) continue;
-            if (fd.periodic[d]) {
-                const int per = dhi - dlo + 1;
-                src[d] += (src[d] < dlo) ? per : -per;
-                continue;
-            }
-            const int code = (src[d] < dlo) ? fd.lo_bc[d] : fd.hi_bc[d];
-            if (code == 0 || code == 2) {
-                src[d] = (src[d] < dlo) ? dlo : dhi;
-            } else if (code >= 3 && code <= 5) {
-                src[d] = (src[d] < dlo) ? (2 * dlo - 1 - src[d]) : (2 * dhi + 1 - src[d]);
-                for (int m = 0; m < DIM; ++m)
-                    if (m == d || code == 5) flip ^= (1u << m);
-            } else {
-                ok = 0;
-            }
-        }
-        if (!ok) { atomicMax(err, CMP_ERR_BC); continue; }
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-            if (src[d] < fd.dlo[d] || src[d] > fd.dhi[d]) ok = 0;
-        if (!ok) { atomicMax(err, CMP_ERR_LAYOUT); continue; }
-        int bi = 0;
-        for (int d = DIM - 1; d >= 0; --d)
-            bi = bi * fd.nb[d] + (src[d] - fd.dlo[d]) / fd.bs;
-        const int own = fd.owner[bi];
-        if (own < 0) { atomicMax(err, CMP_ERR_LAYOUT); continue; }
-        const BoxDev so = fd.boxes[own];
-        const long long si = so.off + fab_index(so, src[0], src[1], src[2]);
+        long long si, ssn;
+        unsigned flip;
+        const int e = fill_source<DIM>(fd, cell[0], cell[1], cell[2], si, ssn, flip);
+        if (e) { atomicMax(err, e); continue; }
         const long long di = bx.off + fab_index(bx, cell[0], cell[1], cell[2]);
 #pragma unroll
         for (int c = 0; c < N; ++c) {
-            double v = U[si + c * so.sn];
+            double v = U[si + c * ssn];
             if (MOM >= 0 && c >= MOM && c < MOM + DIM && ((flip >> (c - MOM)) & 1u)) v = -v;
             U[di + c * bx.sn] = v;
         }
@@ -601,7 +718,8 @@ template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP = 0>
 static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, const TileDev* tiles,
                          int ntiles, const BoxDev* boxes, const BoxDev* sbox, int sntiles,
                          const double* Us, const double* Un, double* Uout, const double* dt,
-                         unsigned long long* cmax_bits, int* err, hipStream_t s, const double* gq)
+                         unsigned long long* cmax_bits, int* err, hipStream_t s, const double* gq,
+                         const FillDev* fuse)
 {
     BoxDev sb{};
     int sntx = 1, snty = 1, single = 0, grid = ntiles;
@@ -615,7 +733,7 @@ static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, co
     if (grid <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_stage<DIM, SCHEME, RIEMANN, THINC, NP>), dim3(grid), dim3(NT), 0, s, kp, limiter,
                        mode, stage, tiles, ntiles, boxes, sb, sntx, snty, single, Us, Un, Uout, dt,
-                       cmax_bits, err, gq);
+                       cmax_bits, err, gq, fuse);
     return hipGetLastError();
 }
 
@@ -660,13 +778,14 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
                             const TileDev* tiles, int ntiles, const BoxDev* boxes,
                             const BoxDev* sbox, int sntiles, const double* Us, const double* Un,
                             double* Uout, const double* dt, unsigned long long* cmax_bits, int* err,
-                            hipStream_t s, const double* gq)
+                            hipStream_t s, const double* gq, const FillDev* fuse)
 {
+    if (fuse && sbox) return hipErrorInvalidValue;   // a single box has no level to read ghosts from
     if (lc.riemann == RIEMANN_PF_MIX || lc.riemann == RIEMANN_PF_PHASIC) {
         if (!gq || lc.thinc) return hipErrorInvalidValue;
         const bool mix = lc.riemann == RIEMANN_PF_MIX;
 #define CMP_PF_ARGS kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout, dt, \
-                    cmax_bits, err, s, gq
+                    cmax_bits, err, s, gq, fuse
         if (lc.nphase) {
             // the N-phase model's Phase-Field flux: its schemes (Godunov, MUSCL, WENO5 with the
             // consistent limiter), mixture or phasic form
@@ -698,7 +817,7 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
         // the N-phase model: Godunov, MUSCL, WENO5 (consistent limiter) with HLL or HLLC
         if (lc.thinc || (lc.riemann != 1 && lc.riemann != 2)) return hipErrorInvalidValue;
 #define CMP_NP_ARGS kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout, dt, \
-                    cmax_bits, err, s, nullptr
+                    cmax_bits, err, s, nullptr, fuse
 #define CMP_NP_CASE(SC)                                                                               \
         if (lc.scheme == SC) {                                                                        \
             if (lc.nphase == 3) return lc.riemann == 2 ? stage4<DIM, SC, 2, false, 3>(CMP_NP_ARGS)    \
@@ -714,7 +833,7 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
         return hipErrorInvalidValue;
     }
     CMP_DISPATCH(stage4, kp, lc.limiter, mode, stage, tiles, ntiles, boxes, sbox, sntiles, Us, Un, Uout,
-                 dt, cmax_bits, err, s, nullptr);
+                 dt, cmax_bits, err, s, nullptr, fuse);
     return hipErrorInvalidValue;
 }
 
@@ -777,7 +896,7 @@ template hipError_t launch_stage_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KPa
                                                      const TileDev*, int, const BoxDev*, const BoxDev*,
                                                      int, const double*, const double*, double*,
                                                      const double*, unsigned long long*, int*,
-                                                     hipStream_t, const double*);
+                                                     hipStream_t, const double*, const FillDev*);
 template hipError_t launch_faces_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev&,
                                                      int, const double*, double*, double*, hipStream_t);
 template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(int, const FillDev&, long long, const int*, int, double*, int*,

@@ -18,13 +18,21 @@ struct BoxDev {
     int olo[3];
     int pad2;
     long long osj, osk, osn;
-    // offsets of this box's face arrays in a level face buffer (cmp_face_layout, 1 comp)
+    // offsets of this box's face arrays in a level face buffer (cmp_face_layout with one
+    // component per volume fraction)
     long long goff[3];
 };
+
+// Where the cells beyond a tile's box along direction d on side s (0 low, 1 high) come from when
+// the stage kernel reads them itself (fused ghost fill): a box index (the cells lie in that box,
+// after the periodic wrap along d), or one of the codes below; only a tile that touches its box's
+// face on that side reads there.
+enum HaloSrc { HALO_LOOKUP = -1, HALO_CLAMP = -2, HALO_MIRROR = -3, HALO_MIRROR_ALL = -4 };
 
 struct TileDev {
     int box;
     int lo[3];
+    int halo[3][2];
 };
 
 constexpr int NT = 256;   // threads per workgroup of every kernel here
@@ -76,7 +84,9 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
                         const BoxDev* single_box, int single_ntiles,
                         const double* Us, const double* Un, double* Uout, const double* dt,
                         unsigned long long* cmax_bits, int* err, hipStream_t s,
-                        const double* gradq = nullptr);
+                        const double* gradq = nullptr, const FillDev* fuse = nullptr);
+// fuse: a FillDev in device memory; the stage then reads the cells beyond each tile's box where the
+// ghost fill would take them from, instead of the box's ghost cells
 
 // N = 4 + dim: the flow state (reflecting BCs flip momenta); N = 1 or dim: a Phase-Field field
 // (alpha1 or the normals), filled without sign changes. With `list`, only the ghosts of the nlist

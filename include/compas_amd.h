@@ -186,7 +186,10 @@ int cmp_level_rhs(cmp_level* level, const double* U, double* dUdt, double* cmax,
 int cmp_level_rk3_stage(cmp_level* level, int stage, const double* Us, const double* Un,
                         double* Uout, const double* dt_dev, double* cmax, void* stream);
 
-/* One full RK3 step in place on Un with two scratch level buffers (ghost fills included). */
+/* One full RK3 step in place on Un with two scratch level buffers. The ghost fill is folded into
+ * the stages: cells beyond a box are read where cmp_level_fill_ghosts would take them from, so the
+ * ghost cells of Un, U1 and U2 are neither read nor written (call cmp_level_fill_ghosts to refresh
+ * Un's when the caller reads them). */
 int cmp_level_rk3_step(cmp_level* level, double* Un, double* U1, double* U2,
                        const double* dt_dev, double* cmax, void* stream);
 
