@@ -90,7 +90,9 @@ def cpu_baseline(c, seconds, threads):
 def measure_traffic(workload_name, kernel_prefix, calib_mib=512):
     """HBM bytes per launch of the dominant kernel from two rocprofv3 counter runs of this script
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for
-    a device-to-device copy of calib_mib MiB run first in the same process. The kernel's last
+    an in-place scaling of calib_mib MiB (reads and writes each byte once; a kernel, where torch's
+    device-to-device copy may run on a DMA engine the counters do not see) run first in the same
+    process. The kernel's last
     dispatch is the stage-timing launch the roofline's flop rate is measured on. None when
     rocprofv3 is missing or a run fails."""
     import csv
@@ -121,12 +123,12 @@ def measure_traffic(workload_name, kernel_prefix, calib_mib=512):
             name = row["Kernel_Name"]
             if name.startswith(kernel_prefix):
                 kern.append((int(row["Dispatch_Id"]), float(row["Counter_Value"])))
-            elif "copyBuffer" in name:
+            elif "MulFunctor" in name:
                 copy.append(float(row["Counter_Value"]))
         shutil.rmtree(d, ignore_errors=True)
         if not kern or not copy:
             return None
-        scale = calib_mib * 1024.0 / max(copy)        # the calibration copy is the largest copy
+        scale = calib_mib * 1024.0 / max(copy)        # the calibration pass is the largest product
         got[ctr] = (max(kern)[1] * scale, scale)
     fetch, fs = got["FETCH_SIZE"]
     write, ws = got["WRITE_SIZE"]
@@ -144,8 +146,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
-                    help="run one device-to-device copy of this many MiB first: the reference kernel "
-                         "of known traffic against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
+                    help="scale this many MiB in place first: the reference kernel of known traffic "
+                         "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure the dominant kernel's HBM traffic")
     args = ap.parse_args()
@@ -163,11 +165,10 @@ def main():
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        src.mul_(1.0)
         torch.cuda.synchronize()
-        del src, dst
+        del src
     c = workload(args.workload)
     K = args.steps if args.steps is not None else c["default_steps"]
     W = args.warmup
@@ -202,15 +203,16 @@ def main():
         wall = t1 - t0
         ev_ms = ev0.elapsed_time(ev1)
 
-        # dominant kernel: the fused stage kernel (stage 2 form: reads Us and Un, writes Uout)
-        lev.FillPatch(U1, stream=stream)
+        # dominant kernel: the fused stage kernel as the step runs it (an RK3 step is its three
+        # launches and nothing else; the last launch is the stage-2 form: reads U^(2) and U^n,
+        # writes U^(n+1))
         k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         k0.record(stream)
         for _ in range(args.kernel_reps):
-            lev.rk3_stage(1, U1, Un, U2, dt, cmax, stream=stream)
+            lev.ThirdOrderSSPRK(Un, U1, U2, dt, cmax, stream=stream)
         k1.record(stream)
         stream.synchronize()
-        stage_ms = k0.elapsed_time(k1) / args.kernel_reps
+        stage_ms = k0.elapsed_time(k1) / (3 * args.kernel_reps)
         assert lev.check_errors(stream) == 0
 
         pf_rec = None
@@ -270,8 +272,8 @@ def main():
         rec["phase_field"] = pf_rec
     if not args.no_traffic:
         scheme = {(1, 5): 5, (1, 6): 6, (0, 0): 0}.get((P.id_weno, P.id_limiter), 1)
-        prefix = "void cmp::k_stage<%d, %d, %d, %s>" % (P.dim, scheme, P.id_riemann,
-                                                         "true" if P.id_thinc else "false")
+        prefix = "void cmp::k_stage<%d, %d, %d, %s, %d>" % (P.dim, scheme, P.id_riemann,
+                                                             "true" if P.id_thinc else "false", P.nphase)
         tr = measure_traffic(args.workload, prefix)
         if tr is not None:
             rec["roofline"]["traffic"] = tr["bytes"]
@@ -279,7 +281,7 @@ def main():
                 "fetch_bytes": tr["fetch_bytes"], "write_bytes": tr["write_bytes"],
                 "scale_fetch": tr["scale_fetch"], "scale_write": tr["scale_write"],
                 "algorithmic_bytes": 3 * P.nstate * 8 * cells,
-                "launch": "stage 2 form (reads the stage state with ghosts and U^n, writes U^(2))"}
+                "launch": "stage 2 form (reads U^(2) and U^n, writes U^(n+1); cells beyond a box read from their source boxes)"}
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         rec["cpu_baseline"] = cpu_baseline(c, args.cpu_seconds, threads)
