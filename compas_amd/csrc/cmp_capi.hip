@@ -28,7 +28,7 @@ template <int DIM>
 hipError_t launch_dt_dim(const double*, double*, double*, double, int, double*, int*, int*,
                          hipStream_t);
 template <int DIM>
-hipError_t launch_pf_extract_dim(const PfGeom&, long long, const double*, double*, int, hipStream_t);
+hipError_t launch_pf_extract_dim(const PfGeom&, long long, const double*, double*, double*, int, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_scatter_dim(const PfGeom&, const long long*, const double*, double*, int, hipStream_t);
 template <int DIM>
@@ -39,7 +39,7 @@ hipError_t launch_pf_jfaces_dim(const PfGeom&, const long long*, const double*, 
 template <int DIM>
 hipError_t launch_pf_update_dim(const PfGeom&, long long, double*, const double*, double, hipStream_t);
 template <int DIM>
-hipError_t launch_pf_gradq_dim(const PfGeom&, const long long*, const double*, const double*, double*,
+hipError_t launch_pf_gradq_dim(const PfGeom&, const long long*, const double*, const double*, double, double*,
                                unsigned long long*, int*, hipStream_t);
 template <int DIM>
 hipError_t launch_pf_maxvel_dim(const PfGeom&, long long, const double*, unsigned long long*, int,
@@ -68,10 +68,10 @@ hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, doub
 
 #define CMP_BY_DIM(dim, FN, ...) ((dim) == 2 ? FN<2>(__VA_ARGS__) : FN<3>(__VA_ARGS__))
 
-hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A, int comp,
-                             hipStream_t s)
+hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A, double* A2,
+                             int comp, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_extract_dim, g, maxcells, U, A, comp, s);
+    return CMP_BY_DIM(dim, launch_pf_extract_dim, g, maxcells, U, A, A2, comp, s);
 }
 
 hipError_t launch_pf_scatter(int dim, const PfGeom& g, const long long* maxfaces, const double* src, double* dst,
@@ -100,10 +100,10 @@ hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double
 }
 
 hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* S,
-                           const double* H, double* gq, unsigned long long* gmax_bits, int* boxflag,
-                           hipStream_t s)
+                           const double* H, double hscale, double* gq, unsigned long long* gmax_bits,
+                           int* boxflag, hipStream_t s)
 {
-    return CMP_BY_DIM(dim, launch_pf_gradq_dim, g, maxfaces, S, H, gq, gmax_bits, boxflag, s);
+    return CMP_BY_DIM(dim, launch_pf_gradq_dim, g, maxfaces, S, H, hscale, gq, gmax_bits, boxflag, s);
 }
 
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
@@ -1227,9 +1227,10 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
     long long n = 0;
     for (int ph = 0; ph < npf; ++ph) {
         double* out = (npf == 1) ? gradq : L->d_gq1;
-        CMP_TRY(launch_pf_extract(dim, L->pg, L->maxcells, U, L->d_A0, nph ? nph + dim + 1 + ph : 3 + dim, s));
+        // alpha of the step (A0) and the mechanism's working copy (A, its ghosts filled by each sub-step)
+        CMP_TRY(launch_pf_extract(dim, L->pg, L->maxcells, U, L->d_A0, L->d_A, nph ? nph + dim + 1 + ph : 3 + dim,
+                                  s));
         CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_A0, L->d_err, s));
-        CMP_TRY(hipMemcpyAsync(L->d_A, L->d_A0, sizeof(double) * L->na, hipMemcpyDeviceToDevice, s));
         if (explicit_rc) CMP_TRY(hipMemsetAsync(L->d_H, 0, sizeof(double) * L->nfaces, s));
         double t = 0.0, dt_s = dt_m;
         const double T = dt;
@@ -1260,11 +1261,11 @@ int cmp_level_pf_gradq(cmp_level* L, const double* U, double dt, double* gradq, 
             ++n;
         }
         if (explicit_rc) {
-            CMP_TRY(launch_scale(L->d_H, L->nfaces, 1.0 / dt, s));
             // explicit Grad_Q from the step's alpha (PhaseField_GradQPF, :2315-2403), re-sharpened at
-            // eta_factor * eta
+            // eta_factor * eta, and Jhat = H (1/dt)
             CMP_TRY(launch_pf_map(L->d_A0, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
-            CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, out, L->d_bits + 1, L->d_boxflag, s));
+            CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, L->d_H, 1.0 / dt, out, L->d_bits + 1,
+                                    L->d_boxflag, s));
         } else {
             int r = pf_gradq_implicit(L, dt, eta, out, s);
             if (r) return r;

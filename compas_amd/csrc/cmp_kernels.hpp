@@ -105,9 +105,9 @@ struct PfGeom {
     int periodic[3];
 };
 
-// volume fraction `comp` of the flow state -> an alpha-layout buffer
-hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A, int comp,
-                             hipStream_t s);
+// volume fraction `comp` of the flow state -> an alpha-layout buffer (and a second one, A2, if given)
+hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A, double* A2,
+                             int comp, hipStream_t s);
 // a one-component face buffer (abox offsets) -> component comp of a multi-component one (hbox offsets)
 hipError_t launch_pf_scatter(int dim, const PfGeom& g, const long long* maxfaces, const double* src, double* dst,
                              int comp, hipStream_t s);
@@ -121,10 +121,11 @@ hipError_t launch_pf_jfaces(int dim, const PfGeom& g, const long long* maxfaces,
                             double eta, double dt_m, hipStream_t s);
 hipError_t launch_pf_update(int dim, const PfGeom& g, long long maxcells, double* A, const double* J,
                             double dt_m, hipStream_t s);
-// boxflag (nbox ints, or null): set to 1 for every box with a nonzero Grad_Q on one of its faces
+// Grad_Q from the re-sharpened alpha S and Jhat = hscale * H; boxflag (nbox ints, or null): set to 1
+// for every box with a nonzero Grad_Q on one of its faces
 hipError_t launch_pf_gradq(int dim, const PfGeom& g, const long long* maxfaces, const double* S,
-                           const double* H, double* gq, unsigned long long* gmax_bits, int* boxflag,
-                           hipStream_t s);
+                           const double* H, double hscale, double* gq, unsigned long long* gmax_bits,
+                           int* boxflag, hipStream_t s);
 hipError_t launch_pf_maxvel(int dim, const PfGeom& g, long long maxcells, const double* U,
                             unsigned long long* bits, int nmass, hipStream_t s);
 

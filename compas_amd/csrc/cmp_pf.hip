@@ -110,13 +110,18 @@ __device__ __forceinline__ void block_sum_to(double v, double* part, long long i
 }  // namespace
 
 // One volume fraction (state component comp) of the flow state's valid cells -> the alpha buffer
+// (and into A2 when given: the mechanism's working copy)
 template <int DIM>
-__global__ __launch_bounds__(NT) void k_pf_extract(PfGeom g, const double* __restrict__ U, double* A, int comp)
+__global__ __launch_bounds__(NT) void k_pf_extract(PfGeom g, const double* __restrict__ U, double* A, double* A2,
+                                                   int comp)
 {
     const BoxDev hb = g.hbox[blockIdx.y], ab = g.abox[blockIdx.y];
     int i, j, k;
     if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
-    A[ab.off + fab_index(ab, i, j, k)] = U[hb.off + fab_index(hb, i, j, k) + comp * hb.sn];
+    const long long a = ab.off + fab_index(ab, i, j, k);
+    const double v = U[hb.off + fab_index(hb, i, j, k) + comp * hb.sn];
+    A[a] = v;
+    if (A2) A2[a] = v;
 }
 
 // One phase's Grad_Q (a one-component face buffer, abox offsets) -> component comp of the level's
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(NT) void k_pf_update(PfGeom g, double* A, const dou
 // mode 1) and Jhat, with max |Grad_Q|.
 template <int DIM, int D>
 __global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restrict__ S,
-                                                 const double* __restrict__ H, double* gq,
+                                                 const double* __restrict__ H, double hscale, double* gq,
                                                  unsigned long long* gmax_bits, int* boxflag)
 {
     const BoxDev ab = g.abox[blockIdx.y];
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(NT) void k_pf_gradq(PfGeom g, const double* __restr
         const long long m = c - ((D == 0) ? 1 : ((D == 1) ? ab.sj : ab.sk));
         const double* Sb = S + ab.off;
         const long long f = ab.goff[D] + q;
-        const double r = pf_gradq_sharp(Sb[m], Sb[c], H[f]);
+        const double r = pf_gradq_sharp(Sb[m], Sb[c], H[f] * hscale);
         gq[f] = r;
         v = fabs(r);
         if (__builtin_isnan(r)) v = __longlong_as_double(0x7ff8000000000000LL);
@@ -629,10 +634,10 @@ static dim3 grid_of(long long per_box, int nbox)
 }
 
 template <int DIM>
-hipError_t launch_pf_extract_dim(const PfGeom& g, long long maxcells, const double* U, double* A, int comp,
-                                 hipStream_t s)
+hipError_t launch_pf_extract_dim(const PfGeom& g, long long maxcells, const double* U, double* A, double* A2,
+                                 int comp, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_extract<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, U, A, comp);
+    hipLaunchKernelGGL((k_pf_extract<DIM>), grid_of(maxcells, g.nbox), dim3(NT), 0, s, g, U, A, A2, comp);
     return hipGetLastError();
 }
 
@@ -679,16 +684,16 @@ hipError_t launch_pf_update_dim(const PfGeom& g, long long maxcells, double* A, 
 
 template <int DIM>
 hipError_t launch_pf_gradq_dim(const PfGeom& g, const long long* maxfaces, const double* S,
-                               const double* H, double* gq, unsigned long long* gmax_bits, int* boxflag,
-                               hipStream_t s)
+                               const double* H, double hscale, double* gq, unsigned long long* gmax_bits,
+                               int* boxflag, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_gradq<DIM, 0>), grid_of(maxfaces[0], g.nbox), dim3(NT), 0, s, g, S, H, gq,
+    hipLaunchKernelGGL((k_pf_gradq<DIM, 0>), grid_of(maxfaces[0], g.nbox), dim3(NT), 0, s, g, S, H, hscale, gq,
                        gmax_bits, boxflag);
-    hipLaunchKernelGGL((k_pf_gradq<DIM, 1>), grid_of(maxfaces[1], g.nbox), dim3(NT), 0, s, g, S, H, gq,
+    hipLaunchKernelGGL((k_pf_gradq<DIM, 1>), grid_of(maxfaces[1], g.nbox), dim3(NT), 0, s, g, S, H, hscale, gq,
                        gmax_bits, boxflag);
     if (DIM == 3)
         hipLaunchKernelGGL((k_pf_gradq<DIM, (DIM == 3 ? 2 : 1)>), grid_of(maxfaces[2], g.nbox), dim3(NT),
-                           0, s, g, S, H, gq, gmax_bits, boxflag);
+                           0, s, g, S, H, hscale, gq, gmax_bits, boxflag);
     return hipGetLastError();
 }
 
@@ -819,7 +824,7 @@ hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s)
 #endif
 
 template hipError_t launch_pf_extract_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
-                                                          double*, int, hipStream_t);
+                                                          double*, double*, int, hipStream_t);
 template hipError_t launch_pf_scatter_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
                                                           double*, int, hipStream_t);
 template hipError_t launch_pf_normals_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
@@ -830,8 +835,8 @@ template hipError_t launch_pf_jfaces_dim<CMP_KERNEL_DIM>(const PfGeom&, const lo
 template hipError_t launch_pf_update_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, double*, const double*,
                                                          double, hipStream_t);
 template hipError_t launch_pf_gradq_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*,
-                                                        const double*, double*, unsigned long long*, int*,
-                                                        hipStream_t);
+                                                        const double*, double, double*, unsigned long long*,
+                                                        int*, hipStream_t);
 template hipError_t launch_pf_maxvel_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
                                                          unsigned long long*, int, hipStream_t);
 template hipError_t launch_pf_cac_mech_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*,
