@@ -15,6 +15,13 @@ Workloads (--workload):
                           eta = 0.5 dx)
   advected-interface-2d   256^2 periodic, two ideal gases (gamma 1.4/rho 1 in a circle r = 0.25,
                           gamma 1.6/rho 10 outside), u = v = p = 1, WENO5 + HLLC + RK3, CFL 0.5
+  served-stream           seeded 2D shock-bubble cases (grids 128^2 .. 1024x256, Mach 1.1-3,
+                          density ratio 0.1-10, gammas 1.1-1.7, radius 0.1-0.3) arriving as a
+                          Poisson process at each rate of --rates, 200 RK3 steps each (WENO5 +
+                          HLLC, CFL 0.4), advanced together in one batch of levels; p50/p99 time to
+                          solution per rate and aggregate cell-updates/s, beside the same cases run
+                          one at a time through the single-level path. The flow step only: the
+                          Phase-Field step is not part of the batch.
 Also reported: the dominant kernel's roofline position (measured with HIP events on the stream it
 runs on) and the oracle (CPU restatement, OpenMP) timed on this host on a bounded sample.
 """
@@ -55,6 +62,57 @@ def workload(name):
         c["default_steps"] = 20
         return c
     raise SystemExit("unknown workload %r" % name)
+
+
+def served(args):
+    """The served-stream workload: one JSON line (see the module docstring)."""
+    import numpy as np
+    import oracle
+    from compas_amd import Layout, cases, roofline, serve
+    rates = [float(r) for r in args.rates.split(",") if r]
+    W = args.warmup
+    # warm-up: a few small cases through the same loop (kernels loaded, allocator primed)
+    serve.serve(max(rates), max(2, W), steps=5, seed=999)
+    runs = []
+    for r in rates:
+        res = serve.serve(r, args.cases, steps=args.case_steps, seed=1)
+        res.pop("finals")
+        runs.append(res)
+    alone = serve.one_at_a_time(args.cases, steps=args.case_steps, seed=1)
+    top = runs[-1]
+    P = cases.served_case(0)["params"]
+    achieved = roofline.flops_per_cell_stage(P) * 3 * top["cell_updates_per_s"] / 1e12
+    rec = {
+        "metric": "cell-updates/s (5Eq WENO5-HLLC RK3), % roofline; p50/p99 case latency",
+        "value": top["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.case_steps,
+        "warmup": W,
+        "ms_per_step": top["wall_s"] / max(1, top["ticks"]) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded shock-bubble cases)",
+        "config": {"workload": "served-stream", "cases_per_rate": args.cases, "steps_per_case": args.case_steps,
+                   "grids": [list(g) for g in cases.SERVED_GRIDS], "scheme": "WENO5", "riemann": "HLLC",
+                   "integrator": "TVD-RK3", "cfl": 0.4, "max_grid_size": 64, "phase_field": None,
+                   "value_at_rate": rates[-1]},
+        "latency": [{k: v for k, v in run.items()} for run in runs],
+        "one_at_a_time": alone,
+        "batched_over_one_at_a_time": top["cell_updates_per_s"] / alone["cell_updates_per_s"],
+        "roofline": {"bound": "valu", "kernel": "k_stage_batch (whole request loop: host work and gaps included)",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "flops_per_cell_stage": roofline.flops_per_cell_stage(P)},
+    }
+    if not args.no_cpu_baseline:
+        c = cases.served_case(1 * 100003)
+        c["pf"] = None
+        threads = min(16, os.cpu_count() or 1)
+        rec["cpu_baseline"] = cpu_baseline(c, args.cpu_seconds, threads)
+    print(json.dumps(rec), flush=True)
 
 
 def cpu_baseline(c, seconds, threads):
@@ -148,6 +206,10 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="scale this many MiB in place first: the reference kernel of known traffic "
                          "against which HBM counters (FETCH_SIZE, WRITE_SIZE) are scaled")
+    ap.add_argument("--rates", default="5,50,200",
+                    help="served-stream: arrival rates (cases/s) to run, comma separated")
+    ap.add_argument("--cases", type=int, default=48, help="served-stream: cases per rate")
+    ap.add_argument("--case-steps", type=int, default=200, help="served-stream: RK3 steps per case")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure the dominant kernel's HBM traffic")
     args = ap.parse_args()
@@ -162,6 +224,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.workload == "served-stream":
+        served(args)
+        return
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)

@@ -12,7 +12,7 @@ import numpy as np
 from . import capi
 from .capi import cmp_box, cmp_geometry, cmp_params, check
 
-__all__ = ["make_params", "make_pf_params", "make_geometry", "split_domain", "Layout", "FaceLayout",
+__all__ = ["Batch", "make_params", "make_pf_params", "make_geometry", "split_domain", "Layout", "FaceLayout",
            "Level", "capi", "SCHEMES", "BC"]
 
 # FiniteVolume.Scheme -> (ID_WENO, ID_Limiter, ID_Poly, Num_Grow)
@@ -359,3 +359,71 @@ def rhs_box(params, geom, box, ng, U, dUdt, cmax=None, fluxes=(None, None, None)
                           _ptr(dUdt), _ptr(cmax), _ptr(fluxes[0]), _ptr(fluxes[1]), _ptr(fluxes[2]),
                           _ptr(fluxnc[0]), _ptr(fluxnc[1]), _ptr(fluxnc[2]), _stream(stream)),
           "cmp_rhs_box")
+
+
+class Batch:
+    """Independent levels ("cases") advanced together (cmp_batch_*): each admitted case keeps its
+    state in device memory; step() advances every case by one RK3 step with its own dt, in one
+    launch per stage over the tiles of all cases."""
+
+    def __init__(self, like, capacity_doubles, max_cases):
+        self.lib = capi.load()
+        h = ctypes.c_void_p()
+        check(self.lib.cmp_batch_create(ctypes.byref(like), int(capacity_doubles), int(max_cases),
+                                        ctypes.byref(h)), "cmp_batch_create")
+        self.h = h
+        self.layouts = {}
+
+    def close(self):
+        if self.h:
+            self.lib.cmp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, params, geom, layout, U0, cfl, limiter=1, stream=None):
+        """Admits a case (U0: host level buffer of `layout`); returns its id, or None when the batch
+        has no free slot or arena range for it (CMP_ERR_NOMEM)."""
+        buf = np.ascontiguousarray(U0, dtype=np.float64)
+        cid = ctypes.c_int(-1)
+        rc = self.lib.cmp_batch_admit(self.h, ctypes.byref(params), ctypes.byref(geom), layout.nbox,
+                                      layout.cboxes, layout.ng,
+                                      buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), float(cfl),
+                                      int(limiter), ctypes.byref(cid), _stream(stream))
+        if rc == capi.CMP_ERR_NOMEM:
+            return None
+        check(rc, "cmp_batch_admit")
+        self.layouts[cid.value] = layout
+        return cid.value
+
+    def retire(self, case_id, stream=None):
+        check(self.lib.cmp_batch_retire(self.h, int(case_id), _stream(stream)), "cmp_batch_retire")
+        self.layouts.pop(case_id, None)
+
+    def step(self, stream=None):
+        check(self.lib.cmp_batch_step(self.h, _stream(stream)), "cmp_batch_step")
+
+    def state(self, case_id, stream=None):
+        """(host copy of the case's U^n level buffer, simulated time, steps taken)."""
+        ptr, n = ctypes.c_void_p(), ctypes.c_longlong(0)
+        t, k = ctypes.c_double(0.0), ctypes.c_int(0)
+        check(self.lib.cmp_batch_case_state(self.h, int(case_id), ctypes.byref(ptr), ctypes.byref(n),
+                                            ctypes.byref(t), ctypes.byref(k), _stream(stream)),
+              "cmp_batch_case_state")
+        out = np.empty(n.value, dtype=np.float64)
+        check(self.lib.cmp_batch_case_download(self.h, int(case_id),
+                                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               _stream(stream)), "cmp_batch_case_download")
+        return out, t.value, k.value
+
+    def check_errors(self, case_id, stream=None):
+        return self.lib.cmp_batch_check(self.h, int(case_id), _stream(stream))
+
+    def info(self):
+        n, t, f = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_longlong(0)
+        check(self.lib.cmp_batch_info(self.h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(f)), "cmp_batch_info")
+        return n.value, t.value, f.value

@@ -44,6 +44,7 @@ def build(verbose=False):
         ("cmp_pf.hip", "pf2.o", ["-DCMP_KERNEL_DIM=2"]),
         ("cmp_pf.hip", "pf3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_capi.hip", "capi.o", []),
+        ("cmp_batch.hip", "batch.o", []),
     ]
     procs = []
     for src, obj, extra in units:

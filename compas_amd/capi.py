@@ -17,6 +17,7 @@ CMP_ERR_NAN = 4
 CMP_ERR_COVOLUME = 5
 CMP_ERR_BC = 6
 CMP_ERR_LAYOUT = 7
+CMP_ERR_NOMEM = 8
 
 ERROR_NAMES = {
     CMP_OK: "CMP_OK",
@@ -27,6 +28,7 @@ ERROR_NAMES = {
     CMP_ERR_COVOLUME: "CMP_ERR_COVOLUME",
     CMP_ERR_BC: "CMP_ERR_BC",
     CMP_ERR_LAYOUT: "CMP_ERR_LAYOUT",
+    CMP_ERR_NOMEM: "CMP_ERR_NOMEM",
 }
 
 
@@ -134,6 +136,17 @@ SIGNATURES = {
     "cmp_level_pf_rk3_step": (ctypes.c_int, [vp, dp, dp, dp, dp, ctypes.c_double, vp]),
     "cmp_level_pf_solver_iterations": (ctypes.c_int, [vp, P(ctypes.c_int)]),
     "cmp_level_pf_evolve": (ctypes.c_int, [vp, dp, dp, dp, dp, P(ctypes.c_double), vp]),
+    "cmp_batch_create": (ctypes.c_int, [P(cmp_params), ctypes.c_longlong, ctypes.c_int, P(vp)]),
+    "cmp_batch_destroy": (ctypes.c_int, [vp]),
+    "cmp_batch_admit": (ctypes.c_int, [vp, P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,
+                                       P(ctypes.c_double), ctypes.c_double, ctypes.c_int, P(ctypes.c_int), vp]),
+    "cmp_batch_retire": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+    "cmp_batch_step": (ctypes.c_int, [vp, vp]),
+    "cmp_batch_case_state": (ctypes.c_int, [vp, ctypes.c_int, P(vp), P(ctypes.c_longlong), P(ctypes.c_double),
+                                            P(ctypes.c_int), vp]),
+    "cmp_batch_case_download": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), vp]),
+    "cmp_batch_check": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+    "cmp_batch_info": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_longlong)]),
 }
 
 _lib = None

@@ -272,3 +272,58 @@ def nphase_smooth(dim, n, seed, nphase=3, periodic=True, max_grid_size=None, sch
     boxes = split_domain(shape, max_grid_size or n)
     return dict(params=params, geom=geom, U=U, boxes=boxes, shape=shape, cfl=0.4,
                 name="nphase-smooth")
+
+
+# ---- the served stream: seeded 2D two-phase shock-bubble cases -----------------------------------
+
+SERVED_GRIDS = [(128, 128), (256, 256), (512, 128), (512, 512), (1024, 256)]
+
+
+def shock_bubble_2d(nx, ny, mach, rho_ratio, gamma_bubble, gamma_ambient, radius, max_grid_size=64,
+                    scheme="WENO5", riemann="HLLC"):
+    """2D two-phase shock-bubble: domain [0, nx/ny] x [0, 1] (dx = dy = 1/ny), ambient ideal gas
+    (phase 2: gamma_ambient, rho 1, p 1) at rest, a bubble of phase 1 (gamma_bubble, rho_ratio) of
+    radius `radius` at (x_b, 0.5) with a tanh profile of width dx clipped to [1e-8, 1 - 1e-8], and a
+    planar shock of Mach `mach` at x_b - radius - 0.1 moving in +x with the Rankine-Hugoniot
+    post-shock state behind it (include/PhysicsFunctions.H:15-34). Outflow (foextrap) on every
+    side: the post-shock inflow side holds a uniform state."""
+    width = nx / ny
+    params = make_params(2, gamma=(gamma_bubble, gamma_ambient), scheme=scheme, riemann=riemann, source_term=0)
+    lo_bc, hi_bc = [2, 2], [2, 2]
+    geom = make_geometry((nx, ny), (0.0, 0.0), (width, 1.0), (0, 0), lo_bc=lo_bc, hi_bc=hi_bc)
+    x, y = _centres(geom, (ny, nx))
+    xb = max(radius + 0.25, 0.45 * width)
+    xs = max(xb - radius - 0.1, 0.05)
+    r = np.sqrt((x - xb) ** 2 + (y - 0.5) ** 2)
+    a1 = np.clip(0.5 * (1.0 + np.tanh((radius - r) / (2.0 * geom.dx[0]))), 1e-8, 1.0 - 1e-8)
+    p2, rho2, u2 = rankine_hugoniot(gamma_ambient, 1.0, 1.0, mach)
+    post = x < xs
+    W = np.empty((6, ny, nx))
+    W[0] = a1 * rho_ratio
+    W[1] = (1.0 - a1) * np.where(post, rho2, 1.0)
+    W[2] = np.where(post, u2, 0.0)
+    W[3] = 0.0
+    W[4] = np.where(post, p2, 1.0)
+    W[5] = a1
+    U = prim_to_cons(params, W)
+    boxes = split_domain((nx, ny), max_grid_size)
+    return dict(params=params, geom=geom, U=U, boxes=boxes, shape=(ny, nx), cfl=0.4,
+                name="shock-bubble-2d")
+
+
+def served_case(seed, max_grid_size=64, grids=None):
+    """One case of the served stream, drawn from its seed: grid from SERVED_GRIDS, shock Mach
+    uniform in [1.1, 3.0], bubble/ambient density ratio log-uniform in [0.1, 10], both gammas
+    uniform in [1.1, 1.7], bubble radius uniform in [0.1, 0.3] of the domain height."""
+    rng = np.random.default_rng(seed)
+    grids = grids or SERVED_GRIDS
+    nx, ny = grids[int(rng.integers(len(grids)))]
+    mach = rng.uniform(1.1, 3.0)
+    ratio = float(np.exp(rng.uniform(np.log(0.1), np.log(10.0))))
+    gb, ga = rng.uniform(1.1, 1.7), rng.uniform(1.1, 1.7)
+    radius = rng.uniform(0.1, 0.3)
+    c = shock_bubble_2d(nx, ny, mach, ratio, gb, ga, radius, max_grid_size=max_grid_size)
+    c["seed"] = seed
+    c["draw"] = dict(grid=(nx, ny), mach=mach, rho_ratio=ratio, gamma_bubble=gb, gamma_ambient=ga,
+                     radius=radius)
+    return c

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "cmp_kernels.hpp"
+#include "cmp_level.hpp"
 
 namespace cmp {
 
@@ -57,6 +58,23 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
                                    single_ntiles, Us, Un, Uout, dt, cmax_bits, err, s, gradq, fuse);
     return launch_stage_dim<3>(lc, kp, mode, stage, tiles, ntiles, boxes, single_box, single_ntiles,
                                Us, Un, Uout, dt, cmax_bits, err, s, gradq, fuse);
+}
+
+template <int DIM>
+hipError_t launch_stage_batch_dim(const LaunchCfg&, const KParams*, int, const TileDev*, int, const BoxDev*,
+                                  const double*, const double*, double*, const double*, unsigned long long*, int*,
+                                  const FillDev*, hipStream_t);
+
+hipError_t launch_stage_batch(const LaunchCfg& lc, const KParams* kps, int stage, const TileDev* tiles, int ntiles,
+                              const BoxDev* boxes, const double* Us, const double* Un, double* Uout,
+                              const double* dts, unsigned long long* cmax_bits, int* errs, const FillDev* fds,
+                              hipStream_t s)
+{
+    if (lc.dim == 2)
+        return launch_stage_batch_dim<2>(lc, kps, stage, tiles, ntiles, boxes, Us, Un, Uout, dts, cmax_bits, errs,
+                                         fds, s);
+    return launch_stage_batch_dim<3>(lc, kps, stage, tiles, ntiles, boxes, Us, Un, Uout, dts, cmax_bits, errs, fds,
+                                     s);
 }
 
 hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, double* U, int* err,
@@ -242,91 +260,8 @@ int scheme_ghosts(int scheme) { return scheme >= 5 ? 3 : (scheme == 1 ? 2 : 1); 
 
 using namespace cmp;
 
-struct cmp_level {
-    cmp_params P;
-    cmp_geometry G;
-    int ng;
-    int nbox;
-    std::vector<cmp_box> boxes;
-    KParams kp;
-    LaunchCfg lc;
-    BoxDev* d_boxes = nullptr;
-    FillDev* d_fd = nullptr;        // device copy of fd (the stage kernel's fused ghost reads)
-    TileDev* d_tiles = nullptr;
-    int ntiles = 0;
-    long long* d_gprefix = nullptr;
-    long long nghost = 0;
-    int* d_owner = nullptr;
-    FillDev fd;
-    int* d_err = nullptr;
-    int* d_step = nullptr;
-    hipStream_t own_stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    const void* gkey[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    double gcfl = 0.0;
-    int glimiter = -1;
 
-    // Phase-Field step (cmp_level_pf_setup)
-    bool pf_on = false;
-    cmp_pf_params pf{};
-    LaunchCfg pf_lc{};
-    KParams pf_kp{};
-    BoxDev* d_abox = nullptr;       // alpha1 layout: 1 ghost, 1 comp
-    BoxDev* d_nbox = nullptr;       // normals layout: 1 ghost, dim comps
-    long long* d_gprefix1 = nullptr;
-    long long nghost1 = 0;
-    FillDev fdA{}, fdN{};
-    PfGeom pg{};
-    long long maxcells = 0, maxfaces[3] = {0, 0, 0};
-    long long na = 0, nfaces = 0;
-    double* d_A0 = nullptr;         // alpha1 of the step, PF_bcs ghosts
-    double* d_A = nullptr;          // alpha1 advanced by the mechanism sub-steps
-    double* d_P = nullptr;          // psi of d_A; then the re-sharpened alpha1 of d_A0
-    double* d_N = nullptr;          // cell normals
-    double* d_J = nullptr;          // face J of the current sub-step
-    double* d_H = nullptr;          // face Jhat
-    double* d_gq = nullptr;         // face Grad_Q (cmp_level_pf_evolve), one component per volume fraction
-    double* d_gq1 = nullptr;        // one phase's Grad_Q (N-phase model)
-    unsigned long long* d_bits = nullptr;   // [0] max|u|, [1] max|Grad_Q| as bit patterns
-    double* d_pfdt = nullptr;       // Phase-Field sub-step size read by the stage kernels
-    double* h_pin = nullptr;        // pinned host scalars
-    // CAC mechanism and the implicit RC's solve (alpha layout unless noted)
-    double* d_Mk = nullptr;         // mech of every cell
-    double* d_Wk = nullptr;         // alpha (1 - alpha)
-    double* d_DJ = nullptr;         // Div_Jhat
-    double* d_X = nullptr;          // Q
-    double* d_R = nullptr;          // CG residual
-    double* d_Pv = nullptr;         // CG search direction (its ghosts filled like alpha's)
-    double* d_AP = nullptr;         // operator applied to d_Pv
-    double* d_Z = nullptr;          // preconditioned residual
-    double* d_ID = nullptr;         // inverse diagonal of the operator (Jacobi preconditioner)
-    double* d_Bf = nullptr;         // face coefficients b (face layout)
-    double* d_part = nullptr;       // block partial sums, 2 per block
-    double* d_sc = nullptr;         // CG_NSC device scalars
-    int nblocks = 0;                // blocks of a cell pass over the level
-    long long ncell = 0;            // valid cells of the level
-    int cg_iters = 0;               // iterations of the last implicit solve
-    // Phase-Field sub-steps run on the boxes with a nonzero Grad_Q face and the boxes whose valid
-    // cells fill their ghosts; every other box has a zero Phase-Field flux on all its faces
-    std::vector<std::vector<int>> ghost_src;   // per box: boxes its ghost cells are filled from
-    int* d_boxflag = nullptr;       // per box: a nonzero Grad_Q face (set by the Grad_Q kernels)
-    int* h_boxflag = nullptr;       // pinned copy
-    TileDev* d_pf_tiles = nullptr;  // tiles of the selected boxes
-    TileDev* h_pf_tiles = nullptr;  // pinned staging of the list
-    int pf_ntiles = 0, pf_nfill = 0;
-
-    // host copies of the level tables (tiles box by box, the owner map)
-    std::vector<TileDev> h_tiles;
-    std::vector<int> tile_start;    // first tile of each box, nbox + 1 entries
-    std::vector<int> h_owner;
-    long long bs = 1;
-    int nb[3] = {1, 1, 1};
-};
-
-namespace {
-
-int check_hip(hipError_t e)
+int cmp::check_hip(hipError_t e)
 {
     if (e != hipSuccess) {
         std::fprintf(stderr, "compas_amd: HIP error %d (%s)\n", (int)e, hipGetErrorString(e));
@@ -335,7 +270,7 @@ int check_hip(hipError_t e)
     return CMP_OK;
 }
 
-#define CMP_TRY(x) do { int _r = check_hip(x); if (_r) return _r; } while (0)
+namespace {
 
 // FiniteVolume_Parameter IDs -> kernel scheme. THINC is built on the WENO5 / WENO5B base schemes
 // (the ones the benchmark cases use); THINC on MUSCL/Godunov and the bound-preserving schemes are
@@ -730,6 +665,7 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
     fd.gprefix = L->d_gprefix;
     fd.owner = L->d_owner;
     L->h_tiles = std::move(ht);
+    L->h_boxes = hb;
     L->h_owner = std::move(owner);
     L->bs = bs;
     for (int d = 0; d < 3; ++d) L->nb[d] = nb[d];

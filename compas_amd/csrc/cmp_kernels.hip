@@ -384,57 +384,26 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
 #define CMP_STAGE_ATTR
 #endif
 
+// The stage of one tile (box bx, lowest cell tlo, halo sources halo): staging, faces and flux
+// differences per direction, then the cell update of `mode`; c_max max-accumulated into *cmax_bits,
+// the error word into *err. sWF / sRed: the workgroup's LDS.
 template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP>
-__global__ __launch_bounds__(NT) CMP_STAGE_ATTR
-void k_stage(KParams kp, int limiter, int mode, int stage,
-             const TileDev* __restrict__ tiles, int ntiles, const BoxDev* __restrict__ boxes,
-             BoxDev sbox, int sntx, int snty, int single,
-             const double* __restrict__ Us, const double* Un, double* Uout,
-             const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err,
-             const double* __restrict__ gq, const FillDev* __restrict__ fd)
+__device__ __forceinline__ void stage_tile(const KParams& kp, int limiter, int mode, int stage, const BoxDev& bx,
+                                           const int* tlo, const int (&halo)[3][2], const BoxDev* __restrict__ boxes,
+                                           const double* __restrict__ Us, const double* Un, double* Uout,
+                                           const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err,
+                                           const double* __restrict__ gq, const FillDev* __restrict__ fd,
+                                           double* sWF, double* sRed)
 {
     constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
     using L = Lds<DIM, G>;
     constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
-    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
-
-    // the staged primitives of a direction and, once its faces are computed, their fluxes
-    constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
-    __shared__ double sWF[SWF];
     double* const sW = sWF;
     double* const sF = sWF;
-    __shared__ double sRed[NT / 64];
 
     const int tid = threadIdx.x;
-    int tlo[3];
-    int halo[3][2] = {{HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}};
-    BoxDev bx;
-    if (single) {
-        int b = blockIdx.x;
-        const int tx = b % sntx;
-        b /= sntx;
-        const int ty = b % snty;
-        const int tz = b / snty;
-        bx = sbox;
-        tlo[0] = sbox.lo[0] + tx * TX;
-        tlo[1] = sbox.lo[1] + ty * TY;
-        tlo[2] = sbox.lo[2] + (DIM == 3 ? tz * TZ : 0);
-    } else {
-        const TileDev td = tiles[xcd_tile(blockIdx.x, ntiles)];
-        bx = boxes[td.box];
-        tlo[0] = td.lo[0];
-        tlo[1] = td.lo[1];
-        tlo[2] = td.lo[2];
-        if (fd) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                halo[d][0] = td.halo[d][0];
-                halo[d][1] = td.halo[d][1];
-            }
-        }
-    }
     int n[3];
     n[0] = min(TX, bx.hi[0] - tlo[0] + 1);
     n[1] = min(TY, bx.hi[1] - tlo[1] + 1);
@@ -535,6 +504,84 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
         }
     }
     if (bad && err) atomicMax(err, bad);
+}
+
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP>
+__global__ __launch_bounds__(NT) CMP_STAGE_ATTR
+void k_stage(KParams kp, int limiter, int mode, int stage,
+             const TileDev* __restrict__ tiles, int ntiles, const BoxDev* __restrict__ boxes,
+             BoxDev sbox, int sntx, int snty, int single,
+             const double* __restrict__ Us, const double* Un, double* Uout,
+             const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err,
+             const double* __restrict__ gq, const FillDev* __restrict__ fd)
+{
+    constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
+    constexpr int G = Stencil<SCHEME>::G;
+    using L = Lds<DIM, G>;
+    constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
+    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
+    // the staged primitives of a direction and, once its faces are computed, their fluxes
+    constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
+    __shared__ double sWF[SWF];
+    __shared__ double sRed[NT / 64];
+
+    int tlo[3];
+    int halo[3][2] = {{HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}};
+    BoxDev bx;
+    if (single) {
+        int b = blockIdx.x;
+        const int tx = b % sntx;
+        b /= sntx;
+        const int ty = b % snty;
+        const int tz = b / snty;
+        bx = sbox;
+        tlo[0] = sbox.lo[0] + tx * TX;
+        tlo[1] = sbox.lo[1] + ty * TY;
+        tlo[2] = sbox.lo[2] + (DIM == 3 ? tz * TZ : 0);
+    } else {
+        const TileDev td = tiles[xcd_tile(blockIdx.x, ntiles)];
+        bx = boxes[td.box];
+        tlo[0] = td.lo[0];
+        tlo[1] = td.lo[1];
+        tlo[2] = td.lo[2];
+        if (fd) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                halo[d][0] = td.halo[d][0];
+                halo[d][1] = td.halo[d][1];
+            }
+        }
+    }
+    stage_tile<DIM, SCHEME, RIEMANN, THINC, NP>(kp, limiter, mode, stage, bx, tlo, halo, boxes, Us, Un, Uout, dtp,
+                                                cmax_bits, err, gq, fd, sWF, sRed);
+}
+
+// One RK stage of a batch of independent levels: every tile of every admitted case in one launch.
+// A box's case is its `pad` field; each case has its own parameters, dt, c_max, error word and
+// ghost-fill tables (the halo lookup); the cells beyond a box are read from their source boxes.
+template <int DIM, int SCHEME, int RIEMANN>
+__global__ __launch_bounds__(NT) CMP_STAGE_ATTR
+void k_stage_batch(const KParams* __restrict__ kps, int limiter, int stage, const TileDev* __restrict__ tiles,
+                   int ntiles, const BoxDev* __restrict__ boxes, const double* __restrict__ Us, const double* Un,
+                   double* Uout, const double* __restrict__ dts, unsigned long long* cmax_bits, int* errs,
+                   const FillDev* __restrict__ fds)
+{
+    constexpr int N = 4 + DIM;
+    constexpr int G = Stencil<SCHEME>::G;
+    using L = Lds<DIM, G>;
+    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
+    constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
+    __shared__ double sWF[SWF];
+    __shared__ double sRed[NT / 64];
+
+    const TileDev td = tiles[xcd_tile(blockIdx.x, ntiles)];
+    const BoxDev bx = boxes[td.box];
+    const int cs = bx.pad;
+    const int tlo[3] = {td.lo[0], td.lo[1], td.lo[2]};
+    const int halo[3][2] = {{td.halo[0][0], td.halo[0][1]}, {td.halo[1][0], td.halo[1][1]},
+                            {td.halo[2][0], td.halo[2][1]}};
+    stage_tile<DIM, SCHEME, RIEMANN, false, 0>(kps[cs], limiter, MODE_RK, stage, bx, tlo, halo, boxes, Us, Un, Uout,
+                                               dts + cs, cmax_bits + cs, errs + cs, nullptr, fds + cs, sWF, sRed);
 }
 
 // Face fluxes of one box in the reference's shape (one thread per face, every stencil cell taken
@@ -695,6 +742,39 @@ __global__ void k_compute_dt(double dx0, double dx1, double dx2, double* cmax, d
 }
 
 #if CMP_KERNEL_DIM == 2
+// ComputeDt of every case of a batch (one thread per case; k_compute_dt's rule), then the case's
+// time and step count advance by it and its c_max restarts at zero. Inactive slots are left alone.
+__global__ void k_batch_dt(const BatchSlot* __restrict__ slots, int nslot, double* cmax, double* dt,
+                           double* time, int* steps, int* errs)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nslot || !slots[c].active) return;
+    const BatchSlot sl = slots[c];
+    const double cm = cmax[c];
+    double est = 1.7976931348623157e308;
+    for (int d = 0; d < sl.dim; ++d) {
+        const double e = sl.cfl * sl.dx[d] / cm;
+        est = (e < est) ? e : est;
+    }
+    const double prev = dt[c];
+    double dtn = est;
+    if (sl.limiter && prev > 0.0) dtn = smin(est, 1.1 * prev);
+    if (!(cm > 0.0) || __builtin_isnan(dtn)) atomicMax(errs + c, CMP_ERR_NAN);
+    dt[c] = dtn;
+    time[c] += dtn;
+    steps[c] += 1;
+    cmax[c] = 0.0;
+}
+
+hipError_t launch_batch_dt(const BatchSlot* slots, int nslot, double* cmax, double* dt, double* time, int* steps,
+                           int* errs, hipStream_t s)
+{
+    if (nslot <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_dt, dim3((nslot + 63) / 64), dim3(64), 0, s, slots, nslot, cmax, dt, time, steps,
+                       errs);
+    return hipGetLastError();
+}
+
 // Sets two ints of a device table from kernel arguments (keeps graph-capturable code free of
 // memset nodes and of host-to-device copies).
 __global__ void k_set2(int* p, int a, int b)
@@ -838,6 +918,35 @@ hipError_t launch_stage_dim(const LaunchCfg& lc, const KParams& kp, int mode, in
 }
 
 template <int DIM>
+hipError_t launch_stage_batch_dim(const LaunchCfg& lc, const KParams* kps, int stage, const TileDev* tiles,
+                                  int ntiles, const BoxDev* boxes, const double* Us, const double* Un, double* Uout,
+                                  const double* dts, unsigned long long* cmax_bits, int* errs, const FillDev* fds,
+                                  hipStream_t s)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (lc.thinc || lc.nphase) return hipErrorInvalidValue;
+#define CMP_BATCH_LAUNCH(SC, RI)                                                                        \
+    do {                                                                                                \
+        hipLaunchKernelGGL((k_stage_batch<DIM, SC, RI>), dim3(ntiles), dim3(NT), 0, s, kps, lc.limiter,  \
+                           stage, tiles, ntiles, boxes, Us, Un, Uout, dts, cmax_bits, errs, fds);      \
+        return hipGetLastError();                                                                       \
+    } while (0)
+    if (lc.riemann == 2) {
+        if (lc.scheme == 5) CMP_BATCH_LAUNCH(5, 2);
+        if (lc.scheme == 6) CMP_BATCH_LAUNCH(6, 2);
+        if (lc.scheme == 1) CMP_BATCH_LAUNCH(1, 2);
+        if (lc.scheme == 0) CMP_BATCH_LAUNCH(0, 2);
+    } else if (lc.riemann == 1) {
+        if (lc.scheme == 5) CMP_BATCH_LAUNCH(5, 1);
+        if (lc.scheme == 6) CMP_BATCH_LAUNCH(6, 1);
+        if (lc.scheme == 1) CMP_BATCH_LAUNCH(1, 1);
+        if (lc.scheme == 0) CMP_BATCH_LAUNCH(0, 1);
+    }
+#undef CMP_BATCH_LAUNCH
+    return hipErrorInvalidValue;
+}
+
+template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
                             const double* U, double* flux, double* fluxnc, hipStream_t s)
 {
@@ -892,6 +1001,10 @@ void tile_dims_dim(int* tx, int* ty, int* tz)
 }
 
 template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
+template hipError_t launch_stage_batch_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams*, int, const TileDev*, int,
+                                                           const BoxDev*, const double*, const double*, double*,
+                                                           const double*, unsigned long long*, int*,
+                                                           const FillDev*, hipStream_t);
 template hipError_t launch_stage_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, int, int,
                                                      const TileDev*, int, const BoxDev*, const BoxDev*,
                                                      int, const double*, const double*, double*,

@@ -10,7 +10,7 @@ namespace cmp {
 struct BoxDev {
     int lo[3], hi[3];
     int flo[3];          // lowest stored index of the fab per dim
-    int pad;
+    int pad;             // the box's case in a batch of levels (0 in a single level)
     long long off;       // element offset of the fab in the level buffer
     long long sj, sk, sn;
     // output layout (cmp_rhs_box writes dUdt over the valid box only)
@@ -66,6 +66,16 @@ struct FillDev {
 
 enum StageMode { MODE_RK = 0, MODE_RHS = 1, MODE_CMAX = 2 };
 
+// One case of a batch of levels as the per-case ComputeDt sees it.
+struct BatchSlot {
+    double dx[3];
+    double cfl;
+    int dim;
+    int limiter;
+    int active;
+    int pad;
+};
+
 // Scheme selector: 0 Godunov, 1 MUSCL (limiter from id_limiter), 5 WENO5, 6 WENO5B; THINC on top
 // of WENO5 or WENO5B. riemann: 1 HLL, 2 HLLC for the flow's fluxes; RIEMANN_PF_MIX / RIEMANN_PF_PHASIC
 // select the Phase-Field flux (UD, mixture or phasic formulation), read with a Grad_Q face buffer.
@@ -91,6 +101,15 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 // N = 4 + dim: the flow state (reflecting BCs flip momenta); N = 1 or dim: a Phase-Field field
 // (alpha1 or the normals), filled without sign changes. With `list`, only the ghosts of the nlist
 // boxes it names are filled.
+// One RK stage of every tile of a batch of levels (BoxDev::pad = the box's case; per-case parameters,
+// dt, c_max bits, error word and fill tables indexed by it). Two-phase model, no THINC.
+hipError_t launch_stage_batch(const LaunchCfg& lc, const KParams* kps, int stage, const TileDev* tiles, int ntiles,
+                              const BoxDev* boxes, const double* Us, const double* Un, double* Uout,
+                              const double* dts, unsigned long long* cmax_bits, int* errs, const FillDev* fds,
+                              hipStream_t s);
+// ComputeDt of every active case of a batch; advances each case's time and step count
+hipError_t launch_batch_dt(const BatchSlot* slots, int nslot, double* cmax, double* dt, double* time, int* steps,
+                           int* errs, hipStream_t s);
 hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, double* U, int* err,
                        hipStream_t s, const int* list = nullptr, int nlist = 0);
 

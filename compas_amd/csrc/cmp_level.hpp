@@ -1,0 +1,107 @@
+// cmp_level.hpp - the level object behind the C ABI's opaque cmp_level handle (internal to the
+// library: shared by the level entry points and the batch of levels).
+#pragma once
+
+#include <vector>
+
+#include "../../include/compas_amd.h"
+#include "cmp_kernels.hpp"
+
+namespace cmp {
+
+// HIP status -> CMP_OK / CMP_ERR_HIP (printing the HIP error)
+int check_hip(hipError_t e);
+
+}  // namespace cmp
+
+using cmp::BoxDev;
+using cmp::FillDev;
+using cmp::KParams;
+using cmp::LaunchCfg;
+using cmp::PfGeom;
+using cmp::TileDev;
+
+#define CMP_TRY(x) do { int _r = cmp::check_hip(x); if (_r) return _r; } while (0)
+
+struct cmp_level {
+    cmp_params P;
+    cmp_geometry G;
+    int ng;
+    int nbox;
+    std::vector<cmp_box> boxes;
+    KParams kp;
+    LaunchCfg lc;
+    BoxDev* d_boxes = nullptr;
+    FillDev* d_fd = nullptr;        // device copy of fd (the stage kernel's fused ghost reads)
+    TileDev* d_tiles = nullptr;
+    int ntiles = 0;
+    long long* d_gprefix = nullptr;
+    long long nghost = 0;
+    int* d_owner = nullptr;
+    FillDev fd;
+    int* d_err = nullptr;
+    int* d_step = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const void* gkey[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double gcfl = 0.0;
+    int glimiter = -1;
+
+    // Phase-Field step (cmp_level_pf_setup)
+    bool pf_on = false;
+    cmp_pf_params pf{};
+    LaunchCfg pf_lc{};
+    KParams pf_kp{};
+    BoxDev* d_abox = nullptr;       // alpha1 layout: 1 ghost, 1 comp
+    BoxDev* d_nbox = nullptr;       // normals layout: 1 ghost, dim comps
+    long long* d_gprefix1 = nullptr;
+    long long nghost1 = 0;
+    FillDev fdA{}, fdN{};
+    PfGeom pg{};
+    long long maxcells = 0, maxfaces[3] = {0, 0, 0};
+    long long na = 0, nfaces = 0;
+    double* d_A0 = nullptr;         // alpha1 of the step, PF_bcs ghosts
+    double* d_A = nullptr;          // alpha1 advanced by the mechanism sub-steps
+    double* d_P = nullptr;          // psi of d_A; then the re-sharpened alpha1 of d_A0
+    double* d_N = nullptr;          // cell normals
+    double* d_J = nullptr;          // face J of the current sub-step
+    double* d_H = nullptr;          // face Jhat
+    double* d_gq = nullptr;         // face Grad_Q (cmp_level_pf_evolve), one component per volume fraction
+    double* d_gq1 = nullptr;        // one phase's Grad_Q (N-phase model)
+    unsigned long long* d_bits = nullptr;   // [0] max|u|, [1] max|Grad_Q| as bit patterns
+    double* d_pfdt = nullptr;       // Phase-Field sub-step size read by the stage kernels
+    double* h_pin = nullptr;        // pinned host scalars
+    // CAC mechanism and the implicit RC's solve (alpha layout unless noted)
+    double* d_Mk = nullptr;         // mech of every cell
+    double* d_Wk = nullptr;         // alpha (1 - alpha)
+    double* d_DJ = nullptr;         // Div_Jhat
+    double* d_X = nullptr;          // Q
+    double* d_R = nullptr;          // CG residual
+    double* d_Pv = nullptr;         // CG search direction (its ghosts filled like alpha's)
+    double* d_AP = nullptr;         // operator applied to d_Pv
+    double* d_Z = nullptr;          // preconditioned residual
+    double* d_ID = nullptr;         // inverse diagonal of the operator (Jacobi preconditioner)
+    double* d_Bf = nullptr;         // face coefficients b (face layout)
+    double* d_part = nullptr;       // block partial sums, 2 per block
+    double* d_sc = nullptr;         // CG_NSC device scalars
+    int nblocks = 0;                // blocks of a cell pass over the level
+    long long ncell = 0;            // valid cells of the level
+    int cg_iters = 0;               // iterations of the last implicit solve
+    // Phase-Field sub-steps run on the boxes with a nonzero Grad_Q face and the boxes whose valid
+    // cells fill their ghosts; every other box has a zero Phase-Field flux on all its faces
+    std::vector<std::vector<int>> ghost_src;   // per box: boxes its ghost cells are filled from
+    int* d_boxflag = nullptr;       // per box: a nonzero Grad_Q face (set by the Grad_Q kernels)
+    int* h_boxflag = nullptr;       // pinned copy
+    TileDev* d_pf_tiles = nullptr;  // tiles of the selected boxes
+    TileDev* h_pf_tiles = nullptr;  // pinned staging of the list
+    int pf_ntiles = 0, pf_nfill = 0;
+
+    // host copies of the level tables (tiles box by box, the owner map)
+    std::vector<TileDev> h_tiles;
+    std::vector<BoxDev> h_boxes;    // the box table as uploaded
+    std::vector<int> tile_start;    // first tile of each box, nbox + 1 entries
+    std::vector<int> h_owner;
+    long long bs = 1;
+    int nb[3] = {1, 1, 1};
+};

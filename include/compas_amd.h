@@ -33,6 +33,7 @@ extern "C" {
 #define CMP_ERR_COVOLUME     5  /* NASG covolume bound 1 - b rho <= 0 (EquationOfState_NASG.H:28-32)   */
 #define CMP_ERR_BC           6  /* boundary type without a fill rule here (Inflow/ext_dir)              */
 #define CMP_ERR_LAYOUT       7  /* boxes overlap, leave the domain, or do not cover it                  */
+#define CMP_ERR_NOMEM        8  /* a batch has no free case slot or no free range for a case's state    */
 
 /* Scheme IDs follow FiniteVolume_Parameter::Initialize (include/Operators_FV/FiniteVolume_Parameter.H:71-139):
  *   GODUNOV  id_weno=0 id_limiter=0 id_poly=0  ng>=1
@@ -270,6 +271,53 @@ int cmp_level_pf_rk3_step(cmp_level* level, double* Un, double* U1, double* U2,
  * max|Grad_Q| and the number of sub-steps. Synchronises the stream. */
 int cmp_level_pf_evolve(cmp_level* level, double* Un, double* U1, double* U2, const double* dt_dev,
                         double* info, void* stream);
+
+/* ---- a batch of independent levels (the served-stream form) ---------------------------------
+ * No counterpart in the reference, which runs one case per process (source/main.cpp:21-86): cases of
+ * different grids and parameters are admitted while others run, keep their state in device memory,
+ * and advance together, one launch per RK stage over the tiles of every admitted case. Every case
+ * has its own parameters, geometry, boxes, dt, c_max and error word; each case's result is the one
+ * the single-level entry points give for it alone. Two-phase model without THINC; all cases of a
+ * batch share its dimension, scheme, limiter and Riemann solver. */
+typedef struct cmp_batch cmp_batch;
+
+/* A batch for cases like *like (dim, id_weno/id_limiter, id_riemann), with three device arenas of
+ * `capacity` doubles each (a case's U^n and its two stage buffers sit at one offset in them) and up
+ * to max_cases cases at once. */
+int cmp_batch_create(const cmp_params* like, long long capacity, int max_cases, cmp_batch** out);
+int cmp_batch_destroy(cmp_batch* batch);
+
+/* Admits a case: a free slot and a free arena range for its level buffer (cmp_level_layout of its
+ * boxes with nstate comps and ng ghosts; CMP_ERR_NOMEM when either is missing), its initial state
+ * U0 (host, that layout) and its initial c_max; its steps then use ComputeDt with cfl and limiter
+ * (the first one without the limiter, as cmp_level_advance). Synchronises the stream. */
+int cmp_batch_admit(cmp_batch* batch, const cmp_params* params, const cmp_geometry* geom, int nbox,
+                    const cmp_box* boxes, int ng, const double* U0, double cfl, int limiter, int* case_id,
+                    void* stream);
+
+/* Releases a case's slot and range (after the stream's queued work). */
+int cmp_batch_retire(cmp_batch* batch, int case_id, void* stream);
+
+/* One RK3 step of every admitted case, each with its own dt: one ComputeDt launch for all cases,
+ * then three stage launches over all their tiles (cells beyond a box read from their source boxes,
+ * as in cmp_level_rk3_step). */
+int cmp_batch_step(cmp_batch* batch, void* stream);
+
+/* A case's U^n (device pointer into the arena, its level layout, *len doubles) and, when time or
+ * steps is non-null, its simulated time and step count (synchronises the stream). */
+int cmp_batch_case_state(cmp_batch* batch, int case_id, double** Un, long long* len, double* time,
+                         int* steps, void* stream);
+
+/* Copies a case's U^n (its level layout, the *len of cmp_batch_case_state) into host memory;
+ * synchronises the stream. */
+int cmp_batch_case_download(cmp_batch* batch, int case_id, double* host, void* stream);
+
+/* Reads and clears a case's error word (as cmp_level_check). */
+int cmp_batch_check(cmp_batch* batch, int case_id, void* stream);
+
+/* Admitted cases, tiles of the last rebuilt table (-1 before the next step rebuilds it) and free
+ * arena doubles. */
+int cmp_batch_info(const cmp_batch* batch, int* ncases, int* ntiles, long long* free_doubles);
 
 #ifdef __cplusplus
 }
