@@ -22,6 +22,12 @@ Workloads (--workload):
                           solution per rate and aggregate cell-updates/s, beside the same cases run
                           one at a time through the single-level path. The flow step only: the
                           Phase-Field step is not part of the batch.
+  nphase-uneven-level     the AMR configuration's base level: 512^2 three-phase converging rings
+                          (NPHASE = 3, WENO5 + HLLC, CFL 0.4) on a seeded regrid-like split into
+                          644 boxes of 8..64 cells a side; the batched level step (one launch per
+                          RK stage over the tiles of every box), and the right-hand side of the
+                          level in one launch against the plain per-box form (one cmp_rhs_box
+                          launch per box), both replayed from captured graphs. Flow step only.
 Also reported: the dominant kernel's roofline position (measured with HIP events on the stream it
 runs on) and the oracle (CPU restatement, OpenMP) timed on this host on a bounded sample.
 """
@@ -62,6 +68,96 @@ def workload(name):
         c["default_steps"] = 20
         return c
     raise SystemExit("unknown workload %r" % name)
+
+
+def uneven_level(args):
+    """The nphase-uneven-level workload: one JSON line (see the module docstring)."""
+    import numpy as np
+    import torch
+    from compas_amd import Layout, Level, cases, rhs_box, roofline
+    dev = torch.device("cuda:0")
+    c = cases.nphase_rings_2d(512)
+    P, G = c["params"], c["geom"]
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    cells = int(np.prod(c["shape"]))
+    K = args.steps if args.steps is not None else 100
+    W = args.warmup
+    Un = torch.from_numpy(lay.pack(c["U"])).to(dev)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    cmax = torch.zeros(1, dtype=torch.float64, device=dev)
+    lev = Level(P, G, lay)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        lev.advance(Un, U1, U2, W, c["cfl"], dt, cmax, use_graph=True, resume=False, stream=s)
+        s.synchronize()
+        t0 = time.perf_counter()
+        lev.advance(Un, U1, U2, K, c["cfl"], dt, cmax, use_graph=True, resume=True, stream=s)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+    assert lev.check_errors(s) == 0
+    # the right-hand side of the level: one launch over every tile, against one launch per box
+    nv = [int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in c["boxes"]]
+    dU_level = torch.zeros_like(Un)
+    dU_boxes = torch.zeros(sum(nv) * P.nstate, dtype=torch.float64, device=dev)
+    offs = np.concatenate([[0], np.cumsum(nv)]) * P.nstate
+    graphs = {}
+    for form in ("level", "per-box"):
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            cs = torch.cuda.current_stream()
+            lev.FillPatch(Un, stream=cs)
+            if form == "level":
+                lev.compute_dUdt_FV(dU_level, Un, stream=cs)
+            else:
+                for b, box in enumerate(c["boxes"]):
+                    o = int(lay.cboxes[b].offset)
+                    rhs_box(P, G, box, 3, Un[o:], dU_boxes[int(offs[b]):], stream=cs)
+        graphs[form] = g
+    times = {}
+    for form, g in graphs.items():
+        g.replay()
+        torch.cuda.synchronize()
+        reps = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            g.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        times[form] = e0.elapsed_time(e1) / reps
+    # the two forms give the same right-hand side
+    hl, hb = dU_level.cpu().numpy(), dU_boxes.cpu().numpy()
+    same = all(np.array_equal(hb[int(offs[b]):int(offs[b + 1])],
+                              lay.fab_view(hl, b)[:, 3:-3, 3:-3].ravel())
+               for b in range(len(c["boxes"])))
+    lev.close()
+    flops = roofline.flops_per_cell_stage(P) * cells
+    rec = {
+        "metric": "cell-updates/s (5Eq WENO5-HLLC RK3), % roofline; p50/p99 case latency",
+        "value": cells * K / wall,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded three-phase converging rings)",
+        "config": {"workload": "nphase-uneven-level", "grid": [512, 512], "boxes": len(c["boxes"]),
+                   "box_sizes": "8..64 a side", "nphase": 3, "scheme": "WENO5", "riemann": "HLLC",
+                   "integrator": "TVD-RK3", "cfl": 0.4, "phase_field": None},
+        "rhs_ms": {"level_one_launch": times["level"], "per_box_launches": times["per-box"]},
+        "batched_over_per_box": times["per-box"] / times["level"],
+        "rhs_identical": bool(same),
+        "roofline": {"bound": "valu", "kernel": "k_stage (fill + right-hand side of the level)",
+                     "achieved": flops / (times["level"] * 1e-3) / 1e12, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": flops / (times["level"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                     "traffic": None, "flops_per_cell_stage": roofline.flops_per_cell_stage(P)},
+    }
+    print(json.dumps(rec), flush=True)
 
 
 def served(args):
@@ -226,6 +322,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "served-stream":
         served(args)
+        return
+    if args.workload == "nphase-uneven-level":
+        uneven_level(args)
         return
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8

@@ -327,3 +327,75 @@ def served_case(seed, max_grid_size=64, grids=None):
     c["draw"] = dict(grid=(nx, ny), mach=mach, rho_ratio=ratio, gamma_bubble=gb, gamma_ambient=ga,
                      radius=radius)
     return c
+
+
+# ---- the AMR configuration's base level: three-phase converging rings on uneven boxes -------------
+
+def uneven_boxes(n_cell, lo_size=8, hi_size=64, seed=0):
+    """A seeded split of the domain into boxes of lo_size..hi_size cells a side (multiples of
+    lo_size), the shape a regrid's box list has: recursive cuts at random multiples of lo_size until
+    every box is at most hi_size a side."""
+    rng = np.random.default_rng(seed)
+    dim = len(n_cell)
+    out = []
+    stack = [([0] * dim, [n - 1 for n in n_cell])]
+    while stack:
+        lo, hi = stack.pop()
+        ext = [hi[d] - lo[d] + 1 for d in range(dim)]
+        d = int(np.argmax(ext))
+        if ext[d] <= hi_size and rng.uniform() < 0.5 or ext[d] < 2 * lo_size:
+            if max(ext) <= hi_size:
+                out.append((list(lo), list(hi)))
+                continue
+            d = int(np.argmax(ext))
+        blocks = ext[d] // lo_size
+        cut = lo[d] + lo_size * int(rng.integers(1, blocks))
+        a_hi = list(hi)
+        a_hi[d] = cut - 1
+        b_lo = list(lo)
+        b_lo[d] = cut
+        stack.append((list(lo), a_hi))
+        stack.append((b_lo, list(hi)))
+    return out
+
+
+def nphase_rings_2d(n=512, seed=0, boxes=None, scheme="WENO5", riemann="HLLC"):
+    """The AMR configuration's initial state on its base grid: three ideal gases, (gamma, rho) =
+    (1.4, 1), (1.67, 0.14), (1.09, 5.0), in [0,1]^2 with two concentric annular interfaces at r = 0.3
+    and r = 0.35 around the centre (phase 1 inside, phase 2 in the annulus, phase 3 outside), each
+    with a seeded 8-mode sinusoidal perturbation of amplitude 0.005 and a tanh profile of width dx,
+    and an inward-moving Mach 1.5 shock at r = 0.45 (post-shock state of phase 3 outside it, moving
+    towards the centre). Outflow sides. Boxes: uneven_boxes (8..64 cells a side) unless given."""
+    rng = np.random.default_rng(seed)
+    params = make_params(2, gamma=(1.4, 1.67, 1.09), scheme=scheme, riemann=riemann, nphase=3)
+    geom = make_geometry((n, n), (0.0, 0.0), (1.0, 1.0), (0, 0), lo_bc=[2, 2], hi_bc=[2, 2])
+    x, y = _centres(geom, (n, n))
+    r = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2)
+    th = np.arctan2(y - 0.5, x - 0.5)
+
+    def perturbed(r0):
+        s = np.zeros_like(th)
+        for m in range(1, 9):
+            s += rng.uniform(-1, 1) * np.cos(m * th + rng.uniform(0, 2 * np.pi))
+        return r0 + 0.005 * s / 8.0
+
+    w = 2.0 * geom.dx[0]
+    h1 = 0.5 * (1.0 + np.tanh((perturbed(0.30) - r) / w))
+    h2 = 0.5 * (1.0 + np.tanh((perturbed(0.35) - r) / w))
+    a = [np.clip(h1, 1e-8, 1.0), np.clip(h2 - h1, 1e-8, 1.0), np.clip(1.0 - h2, 1e-8, 1.0)]
+    tot = sum(a)
+    a = [v / tot for v in a]
+    p2, rho2, u2 = rankine_hugoniot(1.09, 1.0, 5.0, 1.5)
+    post = r > 0.45
+    rho = [np.full_like(r, 1.0), np.full_like(r, 0.14), np.where(post, rho2, 5.0)]
+    W = np.empty((9, n, n))
+    for k in range(3):
+        W[k] = a[k] * rho[k]
+        W[6 + k] = a[k]
+    ur = np.where(post, -u2, 0.0)
+    W[3] = ur * np.cos(th)
+    W[4] = ur * np.sin(th)
+    W[5] = np.where(post, p2, 1.0)
+    U = np_prim_to_cons(params, W)
+    boxes = boxes if boxes is not None else uneven_boxes((n, n), seed=seed)
+    return dict(params=params, geom=geom, U=U, boxes=boxes, shape=(n, n), cfl=0.4, name="nphase-rings-2d")
