@@ -297,6 +297,12 @@ class Level:
                                          int(use_graph), int(bool(resume)), _stream(stream)),
               "cmp_level_advance")
 
+    # ErrorEst: tags (torch uint8, one byte per valid cell, box by box) from a ghost-filled U
+    def ErrorEst(self, U, criteria, tags, stream=None):
+        arr = tag_criteria(criteria)
+        check(self.lib.cmp_level_tag(self.h, _ptr(U), arr, len(criteria), _ptr(tags), _stream(stream)),
+              "cmp_level_tag")
+
     def check_errors(self, stream=None):
         """Synchronises and returns the device error word (0 when clean)."""
         return self.lib.cmp_level_check(self.h, _stream(stream))
@@ -344,6 +350,26 @@ class Level:
         check(self.lib.cmp_level_pf_evolve(self.h, _ptr(Un), _ptr(U1), _ptr(U2), _ptr(dt_dev), info,
                                            _stream(stream)), "cmp_level_pf_evolve")
         return list(info)
+
+
+TAG_VARS = {"rho": capi.CMP_TAG_RHO, "P": capi.CMP_TAG_P, "u": capi.CMP_TAG_U, "v": capi.CMP_TAG_V,
+            "w": capi.CMP_TAG_W}
+TAG_KINDS = {"grad": capi.CMP_TAG_GRAD, "value_less": capi.CMP_TAG_LESS, "value_greater": capi.CMP_TAG_GREATER,
+             "value_range": capi.CMP_TAG_RANGE}
+
+
+def tag_criteria(criteria):
+    """[(var, kind, v0[, v1])] -> a ctypes array of cmp_tag_criterion; var is a state component
+    index or one of the derived names of TAG_VARS, kind one of TAG_KINDS (the run.refine.<var>.*
+    keys)."""
+    arr = (capi.cmp_tag_criterion * max(1, len(criteria)))()
+    for q, c in enumerate(criteria):
+        var, kind, v0 = c[0], c[1], c[2]
+        arr[q].var = TAG_VARS[var] if isinstance(var, str) else int(var)
+        arr[q].kind = TAG_KINDS[kind]
+        arr[q].v0 = float(v0)
+        arr[q].v1 = float(c[3]) if len(c) > 3 else 0.0
+    return arr
 
 
 def rhs_box(params, geom, box, ng, U, dUdt, cmax=None, fluxes=(None, None, None),

@@ -99,6 +99,14 @@ class cmp_pf_params(ctypes.Structure):
     ]
 
 
+class cmp_tag_criterion(ctypes.Structure):
+    _fields_ = [("var", ctypes.c_int), ("kind", ctypes.c_int), ("v0", ctypes.c_double), ("v1", ctypes.c_double)]
+
+
+CMP_TAG_RHO, CMP_TAG_P, CMP_TAG_U, CMP_TAG_V, CMP_TAG_W = -1, -2, -3, -4, -5
+CMP_TAG_GRAD, CMP_TAG_LESS, CMP_TAG_GREATER, CMP_TAG_RANGE = 0, 1, 2, 3
+
+
 P = ctypes.POINTER
 vp = ctypes.c_void_p
 dp = ctypes.c_void_p   # device pointers travel as integers (torch data_ptr())
@@ -136,6 +144,7 @@ SIGNATURES = {
     "cmp_level_pf_rk3_step": (ctypes.c_int, [vp, dp, dp, dp, dp, ctypes.c_double, vp]),
     "cmp_level_pf_solver_iterations": (ctypes.c_int, [vp, P(ctypes.c_int)]),
     "cmp_level_pf_evolve": (ctypes.c_int, [vp, dp, dp, dp, dp, P(ctypes.c_double), vp]),
+    "cmp_level_tag": (ctypes.c_int, [vp, dp, P(cmp_tag_criterion), ctypes.c_int, dp, vp]),
     "cmp_batch_create": (ctypes.c_int, [P(cmp_params), ctypes.c_longlong, ctypes.c_int, P(vp)]),
     "cmp_batch_destroy": (ctypes.c_int, [vp]),
     "cmp_batch_admit": (ctypes.c_int, [vp, P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,

@@ -61,6 +61,18 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 }
 
 template <int DIM>
+hipError_t launch_tag_dim(const LaunchCfg&, const KParams&, const BoxDev*, int, long long, const long long*,
+                          const double*, const TagCriteria&, unsigned char*, int*, hipStream_t);
+
+hipError_t launch_tag(const LaunchCfg& lc, const KParams& kp, const BoxDev* boxes, int nbox, long long maxcells,
+                      const long long* toff, const double* U, const TagCriteria& tc, unsigned char* tags, int* err,
+                      hipStream_t s)
+{
+    if (lc.dim == 2) return launch_tag_dim<2>(lc, kp, boxes, nbox, maxcells, toff, U, tc, tags, err, s);
+    return launch_tag_dim<3>(lc, kp, boxes, nbox, maxcells, toff, U, tc, tags, err, s);
+}
+
+template <int DIM>
 hipError_t launch_stage_batch_dim(const LaunchCfg&, const KParams*, int, const TileDev*, int, const BoxDev*,
                                   const double*, const double*, double*, const double*, unsigned long long*, int*,
                                   const FillDev*, hipStream_t);
@@ -723,7 +735,7 @@ int cmp_level_destroy(cmp_level* L)
     if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
     if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
-    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd};
+    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete L;
@@ -1310,6 +1322,43 @@ int cmp_level_pf_evolve(cmp_level* L, double* Un, double* U1, double* U2, const 
         info[4] = gm;
         info[5] = (double)n;
     }
+    return CMP_OK;
+}
+
+int cmp_level_tag(cmp_level* L, const double* U, const cmp_tag_criterion* criteria, int ncriteria,
+                  unsigned char* tags, void* stream)
+{
+    if (!L || !U || !tags || (ncriteria > 0 && !criteria)) return CMP_ERR_ARG;
+    if (ncriteria < 0 || ncriteria > CMP_TAG_MAX_CRITERIA) return CMP_ERR_ARG;
+    TagCriteria tc{};
+    tc.n = ncriteria;
+    for (int q = 0; q < ncriteria; ++q) {
+        const cmp_tag_criterion& c = criteria[q];
+        if (c.var >= L->P.nstate || c.var < CMP_TAG_W || c.kind < CMP_TAG_GRAD || c.kind > CMP_TAG_RANGE)
+            return CMP_ERR_ARG;
+        tc.c[q] = c;
+    }
+    if (L->ng < 1) return CMP_ERR_ARG;   // the gradient reads the face neighbours
+    if (!L->d_toff) {
+        // each box's first tag: the level layout with no ghosts and one component
+        std::vector<cmp_box> b(L->boxes);
+        long long total = 0;
+        int r = cmp_level_layout(L->P.dim, L->nbox, b.data(), 0, 1, &total);
+        if (r) return r;
+        std::vector<long long> off(L->nbox);
+        long long m = 0;
+        for (int q = 0; q < L->nbox; ++q) {
+            off[q] = b[q].offset;
+            long long v = 1;
+            for (int d = 0; d < L->P.dim; ++d) v *= b[q].hi[d] - b[q].lo[d] + 1;
+            m = std::max(m, v);
+        }
+        CMP_TRY(hipMalloc(&L->d_toff, sizeof(long long) * L->nbox));
+        CMP_TRY(hipMemcpy(L->d_toff, off.data(), sizeof(long long) * L->nbox, hipMemcpyHostToDevice));
+        L->tag_maxcells = m;
+    }
+    CMP_TRY(launch_tag(L->lc, L->kp, L->d_boxes, L->nbox, L->tag_maxcells, L->d_toff, U, tc, tags, L->d_err,
+                       (hipStream_t)stream));
     return CMP_OK;
 }
 

@@ -946,6 +946,91 @@ hipError_t launch_stage_batch_dim(const LaunchCfg& lc, const KParams* kps, int s
     return hipErrorInvalidValue;
 }
 
+// ---- refinement tagging ----------------------------------------------------------------------
+// The tagging variable at a cell: a state component, or a derived rho / P / u / v / w
+// (compute_derived_var, include/Physics_Compressible5Eq/ComputeDerivedVar_Compressible5Eq.H:110-160).
+template <int DIM, int NP>
+__device__ __forceinline__ double tag_var(const KParams& kp, const BoxDev& bx, const double* __restrict__ Ub, int i,
+                                          int j, int k, int var, int& bad)
+{
+    constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
+    constexpr int NM = NP ? NP : 2;
+    const long long c = fab_index(bx, i, j, k);
+    if (var >= 0) return Ub[c + var * bx.sn];
+    double Uc[N], W[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) Uc[q] = Ub[c + q * bx.sn];
+    if (var == CMP_TAG_RHO) {
+        double rho = Uc[0];
+#pragma unroll
+        for (int q = 1; q < NM; ++q) rho += Uc[q];
+        return rho;
+    }
+    if constexpr (NP > 0) np_cons2prim<DIM, NP>(kp, Uc, W, bad);
+    else cons2prim<DIM>(kp, Uc, W, bad);
+    if (var == CMP_TAG_P) return W[NM + DIM];
+    const int m = CMP_TAG_U - var;
+    return (m < DIM) ? W[NM + m] : 0.0;
+}
+
+// ErrorEst's criteria loop (source/COMPAS.cpp:839-918; include/Tagging.H:118-201) for every valid
+// cell; one grid row per box, tags box by box (toff: each box's first tag).
+template <int DIM, int NP>
+__global__ __launch_bounds__(NT) void k_tag(KParams kp, const BoxDev* __restrict__ boxes,
+                                            const long long* __restrict__ toff, const double* __restrict__ U,
+                                            TagCriteria tc, unsigned char* tags, int* err)
+{
+    const BoxDev bx = boxes[blockIdx.y];
+    const int n0 = bx.hi[0] - bx.lo[0] + 1, n1 = bx.hi[1] - bx.lo[1] + 1;
+    const int n2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
+    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
+    if (q >= (long long)n0 * n1 * n2) return;
+    const int ii = (int)(q % n0), jj = (int)((q / n0) % n1), kk = (int)(q / ((long long)n0 * n1));
+    const int i = bx.lo[0] + ii, j = bx.lo[1] + jj, k = bx.lo[2] + kk;
+    const double* __restrict__ Ub = U + bx.off;
+    int bad = 0;
+    unsigned char t = 0;
+    for (int m = 0; m < tc.n; ++m) {
+        const cmp_tag_criterion C = tc.c[m];
+        const double s = tag_var<DIM, NP>(kp, bx, Ub, i, j, k, C.var, bad);
+        if (C.kind == CMP_TAG_GRAD) {
+            double ax = fabs(tag_var<DIM, NP>(kp, bx, Ub, i - 1, j, k, C.var, bad) - s);
+            double ay = fabs(tag_var<DIM, NP>(kp, bx, Ub, i, j - 1, k, C.var, bad) - s);
+            ax = smax(ax, fabs(s - tag_var<DIM, NP>(kp, bx, Ub, i + 1, j, k, C.var, bad)));
+            ay = smax(ay, fabs(s - tag_var<DIM, NP>(kp, bx, Ub, i, j + 1, k, C.var, bad)));
+            double az = 0.0;
+            if (DIM == 3) {
+                az = fabs(tag_var<DIM, NP>(kp, bx, Ub, i, j, k - 1, C.var, bad) - s);
+                az = smax(az, fabs(s - tag_var<DIM, NP>(kp, bx, Ub, i, j, k + 1, C.var, bad)));
+            }
+            if (smax(smax(ax, ay), az) >= C.v0) t = 1;
+        } else if (C.kind == CMP_TAG_LESS) {
+            if (s < C.v0) t = 1;
+        } else if (C.kind == CMP_TAG_GREATER) {
+            if (s > C.v0) t = 1;
+        } else if (C.kind == CMP_TAG_RANGE) {
+            if (s > C.v0 && s < C.v1) t = 1;
+        }
+    }
+    tags[toff[blockIdx.y] + q] = t;
+    if (bad && err) atomicMax(err, bad);
+}
+
+template <int DIM>
+hipError_t launch_tag_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev* boxes, int nbox, long long maxcells,
+                          const long long* toff, const double* U, const TagCriteria& tc, unsigned char* tags,
+                          int* err, hipStream_t s)
+{
+    const dim3 grid((unsigned)((maxcells + NT - 1) / NT), (unsigned)nbox);
+    if (lc.nphase == 3)
+        hipLaunchKernelGGL((k_tag<DIM, 3>), grid, dim3(NT), 0, s, kp, boxes, toff, U, tc, tags, err);
+    else if (lc.nphase == 2)
+        hipLaunchKernelGGL((k_tag<DIM, 2>), grid, dim3(NT), 0, s, kp, boxes, toff, U, tc, tags, err);
+    else
+        hipLaunchKernelGGL((k_tag<DIM, 0>), grid, dim3(NT), 0, s, kp, boxes, toff, U, tc, tags, err);
+    return hipGetLastError();
+}
+
 template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
                             const double* U, double* flux, double* fluxnc, hipStream_t s)
@@ -1001,6 +1086,9 @@ void tile_dims_dim(int* tx, int* ty, int* tz)
 }
 
 template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
+template hipError_t launch_tag_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev*, int, long long,
+                                                   const long long*, const double*, const TagCriteria&,
+                                                   unsigned char*, int*, hipStream_t);
 template hipError_t launch_stage_batch_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams*, int, const TileDev*, int,
                                                            const BoxDev*, const double*, const double*, double*,
                                                            const double*, unsigned long long*, int*,

@@ -110,6 +110,14 @@ hipError_t launch_stage_batch(const LaunchCfg& lc, const KParams* kps, int stage
 // ComputeDt of every active case of a batch; advances each case's time and step count
 hipError_t launch_batch_dt(const BatchSlot* slots, int nslot, double* cmax, double* dt, double* time, int* steps,
                            int* errs, hipStream_t s);
+// refinement tags of every valid cell (one grid row per box; the criteria travel as a kernel argument)
+struct TagCriteria {
+    cmp_tag_criterion c[CMP_TAG_MAX_CRITERIA];
+    int n;
+};
+hipError_t launch_tag(const LaunchCfg& lc, const KParams& kp, const BoxDev* boxes, int nbox, long long maxcells,
+                      const long long* toff, const double* U, const TagCriteria& tc, unsigned char* tags, int* err,
+                      hipStream_t s);
 hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, double* U, int* err,
                        hipStream_t s, const int* list = nullptr, int nlist = 0);
 

@@ -100,6 +100,8 @@ struct cmp_level {
     // host copies of the level tables (tiles box by box, the owner map)
     std::vector<TileDev> h_tiles;
     std::vector<BoxDev> h_boxes;    // the box table as uploaded
+    long long* d_toff = nullptr;    // each box's first tag (cmp_level_tag)
+    long long tag_maxcells = 0;
     std::vector<int> tile_start;    // first tile of each box, nbox + 1 entries
     std::vector<int> h_owner;
     long long bs = 1;

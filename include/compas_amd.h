@@ -272,6 +272,36 @@ int cmp_level_pf_rk3_step(cmp_level* level, double* Un, double* U1, double* U2,
 int cmp_level_pf_evolve(cmp_level* level, double* Un, double* U1, double* U2, const double* dt_dev,
                         double* info, void* stream);
 
+/* ---- refinement tagging (ErrorEst, source/COMPAS.cpp:769-932; include/Tagging.H) -------------
+ * One criterion of run.refine.<var>: var >= 0 is a state component; the derived variables of
+ * compute_derived_var (include/Physics_Compressible5Eq/ComputeDerivedVar_Compressible5Eq.H:110-160)
+ * built here are rho (sum of the partial densities), u, v, w and P. kind: state_grad (tag where the
+ * largest one-sided difference to a face neighbour is >= v0, Tagging.H:172-201), state_value_less
+ * (< v0), state_value_greater (> v0), state_value_range (v0 < value < v1). */
+#define CMP_TAG_RHO   (-1)
+#define CMP_TAG_P     (-2)
+#define CMP_TAG_U     (-3)
+#define CMP_TAG_V     (-4)
+#define CMP_TAG_W     (-5)
+#define CMP_TAG_GRAD    0
+#define CMP_TAG_LESS    1
+#define CMP_TAG_GREATER 2
+#define CMP_TAG_RANGE   3
+#define CMP_TAG_MAX_CRITERIA 16
+
+typedef struct cmp_tag_criterion {
+    int var;
+    int kind;
+    double v0, v1;
+} cmp_tag_criterion;
+
+/* Tags of every valid cell of a level (1 = refine, 0 = not): the OR of the criteria, in the
+ * reference's per-variable loop. U: level buffer with its ghosts filled (the gradient reads the
+ * face neighbours). tags: device bytes, one per valid cell, box by box in cmp_level_layout order
+ * with no ghosts and one component (its offsets count bytes here). */
+int cmp_level_tag(cmp_level* level, const double* U, const cmp_tag_criterion* criteria, int ncriteria,
+                  unsigned char* tags, void* stream);
+
 /* ---- a batch of independent levels (the served-stream form) ---------------------------------
  * No counterpart in the reference, which runs one case per process (source/main.cpp:21-86): cases of
  * different grids and parameters are admitted while others run, keep their state in device memory,

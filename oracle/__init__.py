@@ -31,7 +31,7 @@ def load():
         from . import build as _b
         _b.build()
     # the product library's struct mirrors are shared (plain PODs)
-    from compas_amd.capi import cmp_box, cmp_geometry, cmp_params, cmp_pf_params
+    from compas_amd.capi import cmp_box, cmp_geometry, cmp_params, cmp_pf_params, cmp_tag_criterion
     lib = ctypes.CDLL(LIB_PATH)
     D = ctypes.c_double
     PD = ctypes.POINTER(D)
@@ -92,6 +92,9 @@ def load():
         "ref_level_pf_rhs": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, PD]),
         "ref_level_pf_rk3_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, D]),
         "ref_level_max_velocity": (D, [PP, ctypes.c_int, PB, ctypes.c_int, PD]),
+        "ref_level_tag": (ctypes.c_int, [PP, ctypes.c_int, PB, ctypes.c_int, PD,
+                                         ctypes.POINTER(cmp_tag_criterion), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_ubyte)]),
         "ref_level_pf_mechanism": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, D, D, D, D, PD, PD, PI]),
         "ref_pf_cac_mech": (D, [ctypes.c_int, PD, D, PD, PD, PD, PD, PD, D, D, D]),
         "ref_level_pf_gradq_implicit": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, PD, PD, D, PD, PI]),
@@ -441,6 +444,20 @@ def level_pf_rk3_step(params, pf, geom, layout, buf, gradq, pf_dt):
                                       float(pf_dt))
     if rc:
         raise RuntimeError("ref_level_pf_rk3_step failed: %d" % rc)
+
+
+def level_tag(params, layout, buf, criteria):
+    """Tags of every valid cell (bytes, box by box), buf ghost-filled; criteria as
+    compas_amd.tag_criteria takes them."""
+    from compas_amd import tag_criteria
+    arr = tag_criteria(criteria)
+    n = sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in layout.boxes)
+    tags = np.zeros(n, dtype=np.uint8)
+    rc = load().ref_level_tag(ctypes.byref(params), layout.nbox, layout.cboxes, layout.ng, _dptr(buf), arr,
+                              len(criteria), tags.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+    if rc:
+        raise RuntimeError("ref_level_tag failed: %d" % rc)
+    return tags
 
 
 def level_max_velocity(params, layout, buf):

@@ -2456,3 +2456,76 @@ void ref_set_num_threads(int n)
     (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * Refinement tagging: ErrorEst (source/COMPAS.cpp:769-932) with the criteria of include/Tagging.H
+ * (state_grad :172-201, state_value_less / _greater / _range :118-160) on state components or on
+ * the derived rho, u, v, w, P of compute_derived_var
+ * (include/Physics_Compressible5Eq/ComputeDerivedVar_Compressible5Eq.H:110-160; the N-phase model's
+ * …NPhase.H likewise: rho = sum of the partial densities).
+ * ---------------------------------------------------------------------------------------------- */
+static double tag_var(const cmp_params* P, const fab_t* f, const double* U, int i, int j, int k, int var)
+{
+    const long long c = fab_idx(f, i, j, k);
+    if (var >= 0) return U[c + var * f->sn];
+    double Uc[REF_NMAX], W[REF_NMAX];
+    int err = 0;
+    for (int n = 0; n < P->nstate; ++n) Uc[n] = U[c + n * f->sn];
+    const int nm = P->nphase ? P->nphase : 2;
+    if (var == CMP_TAG_RHO) {
+        double rho = Uc[0];
+        for (int q = 1; q < nm; ++q) rho += Uc[q];
+        return rho;
+    }
+    if (P->nphase) ref_np_cons2prim(P, Uc, W, &err);
+    else ref_cons2prim(P, Uc, W, &err);
+    if (var == CMP_TAG_P) return W[nm + P->dim];
+    const int m = CMP_TAG_U - var;          /* u, v, w */
+    return (m < P->dim) ? W[nm + m] : 0.0;
+}
+
+int ref_level_tag(const cmp_params* P, int nbox, const cmp_box* boxes, int ng, const double* U,
+                  const cmp_tag_criterion* crit, int ncrit, unsigned char* tags)
+{
+    const int dim = P->dim;
+    long long toff = 0;
+    for (int b = 0; b < nbox; ++b) {
+        const cmp_box* B = &boxes[b];
+        fab_t f = make_fab(dim, B, ng);
+        const double* UB = U + B->offset;
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = B->hi[d] - B->lo[d] + 1;
+        const int lo2 = (dim == 3) ? B->lo[2] : 0;
+        for (int kk = 0; kk < nv[2]; ++kk)
+            for (int jj = 0; jj < nv[1]; ++jj)
+                for (int ii = 0; ii < nv[0]; ++ii) {
+                    const int i = B->lo[0] + ii, j = B->lo[1] + jj, k = lo2 + kk;
+                    unsigned char t = 0;
+                    for (int q = 0; q < ncrit; ++q) {
+                        const cmp_tag_criterion* C = &crit[q];
+                        const double s = tag_var(P, &f, UB, i, j, k, C->var);
+                        if (C->kind == CMP_TAG_GRAD) {
+                            double ax = fabs(tag_var(P, &f, UB, i - 1, j, k, C->var) - s);
+                            double ay = fabs(tag_var(P, &f, UB, i, j - 1, k, C->var) - s);
+                            ax = smax(ax, fabs(s - tag_var(P, &f, UB, i + 1, j, k, C->var)));
+                            ay = smax(ay, fabs(s - tag_var(P, &f, UB, i, j + 1, k, C->var)));
+                            double az = 0.0;
+                            if (dim == 3) {
+                                az = fabs(tag_var(P, &f, UB, i, j, k - 1, C->var) - s);
+                                az = smax(az, fabs(s - tag_var(P, &f, UB, i, j, k + 1, C->var)));
+                            }
+                            if (smax(smax(ax, ay), az) >= C->v0) t = 1;
+                        } else if (C->kind == CMP_TAG_LESS) {
+                            if (s < C->v0) t = 1;
+                        } else if (C->kind == CMP_TAG_GREATER) {
+                            if (s > C->v0) t = 1;
+                        } else if (C->kind == CMP_TAG_RANGE) {
+                            if (s > C->v0 && s < C->v1) t = 1;
+                        }
+                    }
+                    tags[toff + ii + (long long)nv[0] * (jj + (long long)nv[1] * kk)] = t;
+                }
+        toff += (long long)nv[0] * nv[1] * nv[2];
+    }
+    return CMP_OK;
+}

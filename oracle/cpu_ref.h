@@ -146,6 +146,10 @@ int ref_level_advance_pf(const cmp_params* P, const cmp_pf_params* PF, const cmp
                          int nbox, const cmp_box* boxes, int ng, double* U, int nsteps, double cfl,
                          int limiter, double* dt_hist);
 
+/* refinement tags of every valid cell (cmp_level_tag's layout) */
+int ref_level_tag(const cmp_params* P, int nbox, const cmp_box* boxes, int ng, const double* U,
+                  const cmp_tag_criterion* crit, int ncrit, unsigned char* tags);
+
 /* number of OpenMP threads the level functions use (1 when built without OpenMP) */
 int ref_num_threads(void);
 void ref_set_num_threads(int n);
