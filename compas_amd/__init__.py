@@ -297,6 +297,11 @@ class Level:
                                          int(use_graph), int(bool(resume)), _stream(stream)),
               "cmp_level_advance")
 
+    # AverageDownTo: this (fine) level's Uf onto the coarse level's Uc under the fine boxes
+    def AverageDownTo(self, Uf, coarse, Uc, ratio, stream=None):
+        check(self.lib.cmp_level_average_down(self.h, _ptr(Uf), coarse.h, _ptr(Uc), int(ratio), _stream(stream)),
+              "cmp_level_average_down")
+
     # ErrorEst: tags (torch uint8, one byte per valid cell, box by box) from a ghost-filled U
     def ErrorEst(self, U, criteria, tags, stream=None):
         arr = tag_criteria(criteria)

@@ -61,6 +61,10 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 }
 
 template <int DIM>
+hipError_t launch_avgdown_dim(const BoxDev*, int, long long, const FillDev&, const double*, double*, int, int, double,
+                              int*, hipStream_t);
+
+template <int DIM>
 hipError_t launch_tag_dim(const LaunchCfg&, const KParams&, const BoxDev*, int, long long, const long long*,
                           const double*, const TagCriteria&, unsigned char*, int*, hipStream_t);
 
@@ -1359,6 +1363,35 @@ int cmp_level_tag(cmp_level* L, const double* U, const cmp_tag_criterion* criter
     }
     CMP_TRY(launch_tag(L->lc, L->kp, L->d_boxes, L->nbox, L->tag_maxcells, L->d_toff, U, tc, tags, L->d_err,
                        (hipStream_t)stream));
+    return CMP_OK;
+}
+
+int cmp_level_average_down(const cmp_level* F, const double* Uf, cmp_level* C, double* Uc, int ratio,
+                           void* stream)
+{
+    if (!F || !C || !Uf || !Uc || ratio < 2) return CMP_ERR_ARG;
+    const int dim = F->P.dim;
+    if (C->P.dim != dim || C->P.nstate != F->P.nstate) return CMP_ERR_ARG;
+    for (int d = 0; d < dim; ++d) {
+        if (F->G.domain_lo[d] != C->G.domain_lo[d] * ratio ||
+            F->G.domain_hi[d] + 1 != (C->G.domain_hi[d] + 1) * ratio || C->G.domain_lo[d] < 0)
+            return CMP_ERR_LAYOUT;
+    }
+    long long maxc = 0;
+    for (const cmp_box& b : F->boxes) {
+        long long v = 1;
+        for (int d = 0; d < dim; ++d) {
+            if (b.lo[d] % ratio != 0 || (b.hi[d] + 1) % ratio != 0) return CMP_ERR_LAYOUT;
+            v *= (b.hi[d] - b.lo[d] + 1) / ratio;
+        }
+        maxc = std::max(maxc, v);
+    }
+    double vol = 1.0;
+    for (int d = 0; d < dim; ++d) vol *= ratio;
+    const double volfrac = 1.0 / vol;
+    CMP_TRY((dim == 2 ? launch_avgdown_dim<2> : launch_avgdown_dim<3>)(F->d_boxes, F->nbox, maxc, C->fd, Uf, Uc,
+                                                                        F->P.nstate, ratio, volfrac, C->d_err,
+                                                                        (hipStream_t)stream));
     return CMP_OK;
 }
 

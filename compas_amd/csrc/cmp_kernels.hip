@@ -1031,6 +1031,59 @@ hipError_t launch_tag_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev* 
     return hipGetLastError();
 }
 
+// ---- average_down (AverageDownTo, source/COMPAS.cpp:1550-1562) -----------------------------------
+// Every coarse cell under a fine box takes the mean of its ratio^DIM children (summed k, j, i, times
+// volfrac = 1/ratio^DIM); the coarse cell's box comes from the coarse level's owner map. One grid row
+// per fine box, one thread per coarse cell.
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_avgdown(const BoxDev* __restrict__ fboxes, FillDev cfd,
+                                                const double* __restrict__ Uf, double* Uc, int N, int ratio,
+                                                double volfrac, int* err)
+{
+    const BoxDev fb = fboxes[blockIdx.y];
+    int clo[3] = {0, 0, 0}, ce[3] = {1, 1, 1};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        clo[d] = fb.lo[d] / ratio;
+        ce[d] = (fb.hi[d] + 1) / ratio - clo[d];
+    }
+    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
+    if (q >= (long long)ce[0] * ce[1] * ce[2]) return;
+    const int ic = clo[0] + (int)(q % ce[0]);
+    const int jc = clo[1] + (int)((q / ce[0]) % ce[1]);
+    const int kc = clo[2] + (int)(q / ((long long)ce[0] * ce[1]));
+    const int cc[3] = {ic, jc, kc};
+    int bi = 0;
+    for (int d = DIM - 1; d >= 0; --d) {
+        if (cc[d] < cfd.dlo[d] || cc[d] > cfd.dhi[d]) { atomicMax(err, CMP_ERR_LAYOUT); return; }
+        bi = bi * cfd.nb[d] + (cc[d] - cfd.dlo[d]) / cfd.bs;
+    }
+    const int own = cfd.owner[bi];
+    if (own < 0) { atomicMax(err, CMP_ERR_LAYOUT); return; }
+    const BoxDev cb = cfd.boxes[own];
+    const long long co = cb.off + fab_index(cb, ic, jc, kc);
+    const int rz = (DIM == 3) ? ratio : 1;
+    for (int n = 0; n < N; ++n) {
+        double c = 0.0;
+        for (int kr = 0; kr < rz; ++kr)
+            for (int jr = 0; jr < ratio; ++jr)
+                for (int ir = 0; ir < ratio; ++ir) {
+                    const int i = ic * ratio + ir, j = jc * ratio + jr, k = (DIM == 3) ? kc * ratio + kr : 0;
+                    c += Uf[fb.off + fab_index(fb, i, j, k) + n * fb.sn];
+                }
+        Uc[co + n * cb.sn] = volfrac * c;
+    }
+}
+
+template <int DIM>
+hipError_t launch_avgdown_dim(const BoxDev* fboxes, int nfbox, long long maxcells, const FillDev& cfd,
+                              const double* Uf, double* Uc, int N, int ratio, double volfrac, int* err, hipStream_t s)
+{
+    const dim3 grid((unsigned)((maxcells + NT - 1) / NT), (unsigned)nfbox);
+    hipLaunchKernelGGL((k_avgdown<DIM>), grid, dim3(NT), 0, s, fboxes, cfd, Uf, Uc, N, ratio, volfrac, err);
+    return hipGetLastError();
+}
+
 template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
                             const double* U, double* flux, double* fluxnc, hipStream_t s)
@@ -1086,6 +1139,8 @@ void tile_dims_dim(int* tx, int* ty, int* tz)
 }
 
 template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
+template hipError_t launch_avgdown_dim<CMP_KERNEL_DIM>(const BoxDev*, int, long long, const FillDev&, const double*,
+                                                       double*, int, int, double, int*, hipStream_t);
 template hipError_t launch_tag_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev*, int, long long,
                                                    const long long*, const double*, const TagCriteria&,
                                                    unsigned char*, int*, hipStream_t);

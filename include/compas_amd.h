@@ -302,6 +302,14 @@ typedef struct cmp_tag_criterion {
 int cmp_level_tag(cmp_level* level, const double* U, const cmp_tag_criterion* criteria, int ncriteria,
                   unsigned char* tags, void* stream);
 
+/* AverageDownTo (source/COMPAS.cpp:1550-1562; amrex::average_down): every coarse cell under a fine
+ * box takes the mean of its ratio^dim children (sum over the children, k slowest, times
+ * 1/ratio^dim). The fine level's domain is the coarse one refined by ratio and its boxes are
+ * aligned to ratio; a covered coarse cell outside every coarse box is CMP_ERR_LAYOUT in the coarse
+ * level's error word. Uf (fine level buffer) and Uc (coarse) are device buffers. */
+int cmp_level_average_down(const cmp_level* fine, const double* Uf, cmp_level* coarse, double* Uc, int ratio,
+                           void* stream);
+
 /* ---- a batch of independent levels (the served-stream form) ---------------------------------
  * No counterpart in the reference, which runs one case per process (source/main.cpp:21-86): cases of
  * different grids and parameters are admitted while others run, keep their state in device memory,

@@ -92,6 +92,8 @@ def load():
         "ref_level_pf_rhs": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, PD]),
         "ref_level_pf_rk3_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, D]),
         "ref_level_max_velocity": (D, [PP, ctypes.c_int, PB, ctypes.c_int, PD]),
+        "ref_level_average_down": (ctypes.c_int, [PP, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, PB,
+                                                  ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_tag": (ctypes.c_int, [PP, ctypes.c_int, PB, ctypes.c_int, PD,
                                          ctypes.POINTER(cmp_tag_criterion), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_ubyte)]),
@@ -444,6 +446,14 @@ def level_pf_rk3_step(params, pf, geom, layout, buf, gradq, pf_dt):
                                       float(pf_dt))
     if rc:
         raise RuntimeError("ref_level_pf_rk3_step failed: %d" % rc)
+
+
+def level_average_down(params, flayout, fbuf, clayout, cbuf, ratio):
+    """In place on cbuf: the coarse cells under the fine boxes take their children's mean."""
+    rc = load().ref_level_average_down(ctypes.byref(params), flayout.nbox, flayout.cboxes, flayout.ng, _dptr(fbuf),
+                                       clayout.nbox, clayout.cboxes, clayout.ng, _dptr(cbuf), int(ratio))
+    if rc:
+        raise RuntimeError("ref_level_average_down failed: %d" % rc)
 
 
 def level_tag(params, layout, buf, criteria):

@@ -2529,3 +2529,58 @@ int ref_level_tag(const cmp_params* P, int nbox, const cmp_box* boxes, int ng, c
     }
     return CMP_OK;
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * AverageDownTo (source/COMPAS.cpp:1550-1562): amrex::average_down of the fine level onto the coarse
+ * cells it covers, for cell-centred data on a Cartesian grid: every coarse cell under a fine box
+ * takes the mean of its ratio^dim children, summed k, then j, then i (slowest to fastest) and
+ * multiplied by 1/ratio^dim (AMReX 26.01's amrex_avgdown, not vendored; its published form).
+ * ---------------------------------------------------------------------------------------------- */
+int ref_level_average_down(const cmp_params* P, int nbf, const cmp_box* fboxes, int ngf, const double* Uf,
+                           int nbc, const cmp_box* cboxes, int ngc, double* Uc, int ratio)
+{
+    const int dim = P->dim, N = P->nstate;
+    const int rz = (dim == 3) ? ratio : 1;
+    double vol = 1.0;
+    for (int d = 0; d < dim; ++d) vol *= ratio;
+    const double volfrac = 1.0 / vol;
+    for (int b = 0; b < nbf; ++b) {
+        const cmp_box* F = &fboxes[b];
+        for (int d = 0; d < dim; ++d)
+            if (((F->lo[d] % ratio) + ratio) % ratio != 0 || ((F->hi[d] + 1) % ratio + ratio) % ratio != 0)
+                return CMP_ERR_LAYOUT;
+        fab_t ff = make_fab(dim, F, ngf);
+        int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+        for (int d = 0; d < dim; ++d) {
+            clo[d] = F->lo[d] / ratio;
+            chi[d] = (F->hi[d] + 1) / ratio - 1;
+        }
+        for (int kc = clo[2]; kc <= chi[2]; ++kc)
+            for (int jc = clo[1]; jc <= chi[1]; ++jc)
+                for (int ic = clo[0]; ic <= chi[0]; ++ic) {
+                    int cb = -1;
+                    for (int q = 0; q < nbc && cb < 0; ++q) {
+                        int in = 1;
+                        const int cc[3] = {ic, jc, kc};
+                        for (int d = 0; d < dim; ++d)
+                            if (cc[d] < cboxes[q].lo[d] || cc[d] > cboxes[q].hi[d]) in = 0;
+                        if (in) cb = q;
+                    }
+                    if (cb < 0) return CMP_ERR_LAYOUT;
+                    fab_t fc = make_fab(dim, &cboxes[cb], ngc);
+                    const long long co = cboxes[cb].offset + fab_idx(&fc, ic, jc, kc);
+                    for (int n = 0; n < N; ++n) {
+                        double c = 0.0;
+                        for (int kr = 0; kr < rz; ++kr)
+                            for (int jr = 0; jr < ratio; ++jr)
+                                for (int ir = 0; ir < ratio; ++ir) {
+                                    const int i = ic * ratio + ir, j = jc * ratio + jr;
+                                    const int k = (dim == 3) ? kc * ratio + kr : 0;
+                                    c += Uf[F->offset + fab_idx(&ff, i, j, k) + n * ff.sn];
+                                }
+                        Uc[co + n * fc.sn] = volfrac * c;
+                    }
+                }
+    }
+    return CMP_OK;
+}

@@ -150,6 +150,10 @@ int ref_level_advance_pf(const cmp_params* P, const cmp_pf_params* PF, const cmp
 int ref_level_tag(const cmp_params* P, int nbox, const cmp_box* boxes, int ng, const double* U,
                   const cmp_tag_criterion* crit, int ncrit, unsigned char* tags);
 
+/* average_down of a fine level's boxes onto the coarse level's cells they cover (ratio >= 2) */
+int ref_level_average_down(const cmp_params* P, int nbf, const cmp_box* fboxes, int ngf, const double* Uf,
+                           int nbc, const cmp_box* cboxes, int ngc, double* Uc, int ratio);
+
 /* number of OpenMP threads the level functions use (1 when built without OpenMP) */
 int ref_num_threads(void);
 void ref_set_num_threads(int n);

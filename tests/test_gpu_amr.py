@@ -1,0 +1,41 @@
+"""GPU average_down (cmp_level_average_down through the C ABI) against the oracle: the coarse level
+after averaging a fine level onto it is bit-identical (2D ratio 2 and 4, 3D ratio 2, two-phase and
+three-phase states, fine boxes spread over several coarse boxes); cells outside the fine boxes keep
+their values."""
+import numpy as np
+import pytest
+
+import oracle
+from compas_amd import Layout, Level, make_geometry, make_params, split_domain
+
+pytestmark = pytest.mark.gpu
+
+FINE = {(2, 2): [([4, 6], [19, 13]), ([20, 2], [27, 29]), ([0, 30], [31, 31])],
+        (2, 4): [([4, 8], [19, 23]), ([20, 0], [31, 15])],
+        (3, 2): [([2, 4, 0], [9, 11, 7]), ([10, 0, 8], [17, 7, 15])]}
+
+
+@pytest.mark.parametrize("dim,ratio,nphase", [(2, 2, 0), (2, 4, 0), (3, 2, 0), (2, 2, 3)])
+def test_average_down_matches_oracle(gpu, dim, ratio, nphase):
+    import torch
+    nc = 16 if (dim, ratio) != (2, 4) else 8
+    P = (make_params(dim, gamma=(1.4, 1.67, 1.09), nphase=3) if nphase else make_params(dim, gamma=(1.4, 1.6)))
+    cg = make_geometry((nc,) * dim, (0.0,) * dim, (1.0,) * dim, (1,) * dim)
+    fg = make_geometry((nc * ratio,) * dim, (0.0,) * dim, (1.0,) * dim, (1,) * dim)
+    cl = Layout(dim, split_domain((nc,) * dim, max(4, nc // 2)), 3, P.nstate)
+    fl = Layout(dim, FINE[(dim, ratio)], 3, P.nstate)
+    rng = np.random.default_rng(7)
+    fbuf = rng.uniform(0.1, 2.0, fl.total)
+    cbuf = rng.uniform(-1.0, 0.0, cl.total)
+    ref = cbuf.copy()
+    oracle.level_average_down(P, fl, fbuf, cl, ref, ratio)
+    Uf = torch.from_numpy(fbuf).to(gpu)
+    Uc = torch.from_numpy(cbuf).to(gpu)
+    cf, cc = Level(P, fg, fl), Level(P, cg, cl)
+    cf.AverageDownTo(Uf, cc, Uc, ratio)
+    assert cc.check_errors() == 0
+    got = Uc.cpu().numpy()
+    cf.close()
+    cc.close()
+    assert np.array_equal(got, ref)
+    assert (got != cbuf).sum() > 0
