@@ -1,0 +1,168 @@
+"""Regridding on the host: refinement tags -> the next level's box list (this library's own
+clustering; the reference hands its TagBoxArray to AMReX's Berger-Rigoutsos clustering,
+AmrCore::MakeNewGrids, with amr.n_error_buf, amr.blocking_factor, amr.grid_eff and
+amr.max_grid_size).
+
+`cluster(tags, ...)` works on the tags of one level (a global [(k,) j, i] uint8 array):
+
+1. tags grown by n_error_buf cells, then coarsened by the blocking factor (a block is tagged when
+   any of its cells is);
+2. Berger-Rigoutsos on the blocks: a box whose tagged fraction is below grid_eff is cut, at a
+   plane without tags when one exists (the middlemost), else where the second derivative of the
+   tag signature changes sign most strongly, else in the middle of its longest side;
+3. boxes longer than max_grid_size (in cells) are chopped into near-equal pieces on block lines;
+4. the boxes, in cells of the level, cover every tag; `refine` maps them to the next level.
+
+`regrid(level, U, criteria, ...)` runs the GPU tagging (cmp_level_tag) and clusters its tags.
+"""
+import numpy as np
+
+
+def _grow(tags, n):
+    if n <= 0:
+        return tags.astype(bool)
+    out = tags.astype(bool).copy()
+    for ax in range(tags.ndim):
+        acc = out.copy()
+        for s in range(1, n + 1):
+            acc |= np.roll(out, s, axis=ax) & _not_wrapped(out.shape, ax, s)
+            acc |= np.roll(out, -s, axis=ax) & _not_wrapped(out.shape, ax, -s)
+        out = acc
+    return out
+
+
+def _not_wrapped(shape, ax, s):
+    m = np.ones(shape, dtype=bool)
+    idx = [slice(None)] * len(shape)
+    idx[ax] = slice(0, s) if s > 0 else slice(shape[ax] + s, shape[ax])
+    m[tuple(idx)] = False
+    return m
+
+
+def _coarsen(tags, bf):
+    shp = [-(-n // bf) for n in tags.shape]
+    pad = [(0, s * bf - n) for s, n in zip(shp, tags.shape)]
+    t = np.pad(tags, pad)
+    for ax in range(t.ndim):
+        t = t.reshape(t.shape[:ax] + (shp[ax], bf) + t.shape[ax + 1:]).any(axis=ax + 1)
+    return t
+
+
+def _split(block, lo, hi, eff, out):
+    """Berger-Rigoutsos on the tagged blocks inside [lo, hi] (block indices, numpy axis order)."""
+    sl = tuple(slice(l, h + 1) for l, h in zip(lo, hi))
+    sub = block[sl]
+    if not sub.any():
+        return
+    # shrink to the tags' bounding box
+    nz = [np.nonzero(sub.any(axis=tuple(a for a in range(sub.ndim) if a != ax)))[0] for ax in range(sub.ndim)]
+    hi = [l + int(z[-1]) for l, z in zip(lo, nz)]
+    lo = [l + int(z[0]) for l, z in zip(lo, nz)]
+    sl = tuple(slice(l, h + 1) for l, h in zip(lo, hi))
+    sub = block[sl]
+    if sub.mean() >= eff or all(h == l for l, h in zip(lo, hi)):
+        out.append((lo, hi))
+        return
+    # a plane without tags, the one nearest the middle
+    best = None
+    for ax in range(sub.ndim):
+        sig = sub.sum(axis=tuple(a for a in range(sub.ndim) if a != ax))
+        holes = [c for c in range(1, len(sig) - 1) if sig[c] == 0]
+        for c in holes:
+            score = abs(c - (len(sig) - 1) / 2.0)
+            if best is None or score < best[0]:
+                best = (score, ax, c, True)
+    if best is None:
+        # the strongest sign change of the signature's second difference
+        for ax in range(sub.ndim):
+            sig = sub.sum(axis=tuple(a for a in range(sub.ndim) if a != ax)).astype(float)
+            if len(sig) < 4:
+                continue
+            lap = sig[:-2] - 2 * sig[1:-1] + sig[2:]
+            for c in range(len(lap) - 1):
+                if lap[c] * lap[c + 1] < 0:
+                    jump = abs(lap[c] - lap[c + 1])
+                    if best is None or -jump < best[0]:
+                        best = (-jump, ax, c + 2, False)
+    if best is None:
+        ax = int(np.argmax([h - l for l, h in zip(lo, hi)]))
+        best = (0, ax, (hi[ax] - lo[ax] + 1) // 2, False)
+    _, ax, c, hole = best
+    a_hi = list(hi)
+    b_lo = list(lo)
+    if hole:
+        a_hi[ax] = lo[ax] + c - 1
+        b_lo[ax] = lo[ax] + c + 1
+    else:
+        a_hi[ax] = lo[ax] + c - 1
+        b_lo[ax] = lo[ax] + c
+    _split(block, lo, a_hi, eff, out)
+    _split(block, b_lo, hi, eff, out)
+
+
+def _chop(lo, hi, maxb):
+    """Chop a block box into pieces of at most maxb blocks per side, near-equal sizes."""
+    pieces = [(list(lo), list(hi))]
+    for ax in range(len(lo)):
+        nxt = []
+        for l, h in pieces:
+            n = h[ax] - l[ax] + 1
+            k = -(-n // maxb)
+            base, extra = divmod(n, k)
+            start = l[ax]
+            for q in range(k):
+                m = base + (1 if q < extra else 0)
+                a, b = list(l), list(h)
+                a[ax], b[ax] = start, start + m - 1
+                nxt.append((a, b))
+                start += m
+        pieces = nxt
+    return pieces
+
+
+def cluster(tags, blocking_factor=8, max_grid_size=64, grid_eff=0.7, n_error_buf=1):
+    """Boxes [(lo, hi)] (cell indices, x first, inclusive) covering every tag of `tags`
+    ([(k,) j, i]), aligned to the blocking factor, at most max_grid_size cells a side."""
+    dim = tags.ndim
+    t = _grow(tags, n_error_buf)
+    block = _coarsen(t, blocking_factor)
+    raw = []
+    _split(block, [0] * dim, [s - 1 for s in block.shape], grid_eff, raw)
+    maxb = max(1, max_grid_size // blocking_factor)
+    out = []
+    n_cell = tags.shape
+    for lo, hi in raw:
+        for a, b in _chop(lo, hi, maxb):
+            clo = [a[ax] * blocking_factor for ax in range(dim)]
+            chi = [min((b[ax] + 1) * blocking_factor, n_cell[ax]) - 1 for ax in range(dim)]
+            out.append((list(reversed(clo)), list(reversed(chi))))   # numpy axes -> x first
+    return out
+
+
+def refine(boxes, ratio):
+    return [([l * ratio for l in lo], [(h + 1) * ratio - 1 for h in hi]) for lo, hi in boxes]
+
+
+def tags_to_global(layout, tags, shape):
+    """The one-byte-per-valid-cell tags of cmp_level_tag (box by box) as a global array."""
+    out = np.zeros(shape, dtype=np.uint8)
+    q = 0
+    for lo, hi in layout.boxes:
+        ext = [h - l + 1 for l, h in zip(lo, hi)]
+        n = int(np.prod(ext))
+        sl = tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(len(lo))))
+        out[sl] = tags[q:q + n].reshape(tuple(reversed(ext)))
+        q += n
+    return out
+
+
+def regrid(level, U, criteria, shape, ratio=2, blocking_factor=8, max_grid_size=64, grid_eff=0.7,
+           n_error_buf=1, stream=None):
+    """The next level's boxes (in its cells) from this level's ghost-filled state U (torch tensor
+    on the GPU): GPU tagging with `criteria`, then cluster and refine."""
+    import torch
+    n = sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in level.layout.boxes)
+    tags = torch.zeros(n, dtype=torch.uint8, device=U.device)
+    level.ErrorEst(U, criteria, tags, stream=stream)
+    g = tags_to_global(level.layout, tags.cpu().numpy(), shape)
+    return refine(cluster(g, blocking_factor, max_grid_size, grid_eff, n_error_buf), ratio), g

@@ -39,3 +39,31 @@ def test_tags_match_oracle(gpu, dim, nphase, crit):
     got = tags.cpu().numpy()
     assert np.array_equal(got, ref), (int((got != ref).sum()), ref.sum())
     assert ref.sum() > 0
+
+
+def test_regrid_from_gpu_tags(gpu):
+    """Regrid of the three-phase rings at 128^2: GPU tags on every volume-fraction gradient and on
+    |grad P| (the AMR configuration's criteria), clustered into blocking-factor-aligned boxes of at
+    most 32 cells that cover every tag, refined by 2 for the next level."""
+    import torch
+    from compas_amd import regrid
+    c = cases.nphase_rings_2d(128, boxes=None)
+    P, G = c["params"], c["geom"]
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    oracle.level_fill_ghosts(P, G, lay, buf)
+    crit = [(6, "grad", 0.1), (7, "grad", 0.1), (8, "grad", 0.1), ("P", "grad", 0.1)]
+    lev = Level(P, G, lay)
+    U = torch.from_numpy(buf).to(gpu)
+    boxes, tags = regrid.regrid(lev, U, crit, (128, 128), ratio=2, blocking_factor=8, max_grid_size=32)
+    lev.close()
+    assert np.array_equal(tags.astype(bool), regrid.tags_to_global(lay, oracle.level_tag(P, lay, buf, crit),
+                                                                   (128, 128)).astype(bool))
+    assert tags.sum() > 0 and len(boxes) > 0
+    cover = np.zeros((256, 256), dtype=int)
+    for lo, hi in boxes:
+        assert all(l % 16 == 0 for l in lo) and all(h - l + 1 <= 64 for l, h in zip(lo, hi))
+        cover[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] += 1
+    assert cover.max() == 1
+    fine_tags = np.kron(tags, np.ones((2, 2), dtype=np.uint8)).astype(bool)
+    assert np.all(cover[fine_tags] == 1)

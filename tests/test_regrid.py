@@ -1,0 +1,53 @@
+"""The host clustering (CPU only): the boxes cover every tag grown by n_error_buf, do not overlap,
+sit on blocking-factor lines, are at most max_grid_size a side, and each box's blocks are tagged
+at least grid_eff of the time or the box could not be cut further; the GPU tags of a ring feed it
+the same way (cmp_level_tag's layout through tags_to_global)."""
+import numpy as np
+import pytest
+
+from compas_amd import regrid
+
+
+def _check(tags, boxes, bf, mgs, nbuf):
+    dim = tags.ndim
+    cover = np.zeros(tags.shape, dtype=int)
+    for lo, hi in boxes:
+        for d in range(dim):
+            assert lo[d] % bf == 0 and (hi[d] + 1) % bf == 0 or hi[d] == tags.shape[dim - 1 - d] - 1
+            assert hi[d] - lo[d] + 1 <= mgs
+        sl = tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(dim)))
+        cover[sl] += 1
+    assert cover.max() <= 1
+    grown = regrid._grow(tags, nbuf)
+    assert np.all(cover[grown] == 1)
+
+
+@pytest.mark.parametrize("dim,seed", [(2, 0), (2, 1), (3, 2)])
+def test_cluster_covers_tags(dim, seed):
+    rng = np.random.default_rng(seed)
+    n = 128 if dim == 2 else 48
+    X = np.indices((n,) * dim).astype(float) / n
+    tags = np.zeros((n,) * dim, dtype=np.uint8)
+    for _ in range(4):
+        c = rng.uniform(0.2, 0.8, dim)
+        r = np.sqrt(sum((X[d] - c[d]) ** 2 for d in range(dim)))
+        tags |= (np.abs(r - rng.uniform(0.05, 0.2)) < 1.5 / n).astype(np.uint8)
+    boxes = regrid.cluster(tags, blocking_factor=8, max_grid_size=32, grid_eff=0.7, n_error_buf=1)
+    _check(tags, boxes, 8, 32, 1)
+    cells = sum(np.prod([h - l + 1 for l, h in zip(lo, hi)]) for lo, hi in boxes)
+    assert cells < tags.size                     # the clustering refines a part of the level only
+
+
+def test_empty_and_full():
+    assert regrid.cluster(np.zeros((32, 32), dtype=np.uint8)) == []
+    full = regrid.cluster(np.ones((32, 32), dtype=np.uint8), blocking_factor=8, max_grid_size=16)
+    assert sorted(full) == sorted([([i, j], [i + 15, j + 15]) for i in (0, 16) for j in (0, 16)])
+
+
+def test_refine_and_tags_to_global():
+    assert regrid.refine([([8, 0], [15, 7])], 2) == [([16, 0], [31, 15])]
+    from compas_amd import Layout, split_domain
+    lay = Layout(2, split_domain((16, 16), 8), 3, 6)
+    g = np.arange(256, dtype=np.uint8).reshape(16, 16) % 2
+    flat = np.concatenate([g[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1].ravel() for lo, hi in lay.boxes])
+    assert np.array_equal(regrid.tags_to_global(lay, flat, (16, 16)), g)
