@@ -302,6 +302,12 @@ class Level:
         check(self.lib.cmp_level_average_down(self.h, _ptr(Uf), coarse.h, _ptr(Uc), int(ratio), _stream(stream)),
               "cmp_level_average_down")
 
+    # FillPatchTwoLevels: this (fine) level's ghost cells from its own boxes and, where none holds
+    # the source, the coarse level's conservative linear interpolation
+    def FillPatchTwoLevels(self, Uf, coarse, Uc, ratio, stream=None):
+        check(self.lib.cmp_level_fill_two_levels(self.h, _ptr(Uf), coarse.h, _ptr(Uc), int(ratio),
+                                                 _stream(stream)), "cmp_level_fill_two_levels")
+
     # ErrorEst: tags (torch uint8, one byte per valid cell, box by box) from a ghost-filled U
     def ErrorEst(self, U, criteria, tags, stream=None):
         arr = tag_criteria(criteria)

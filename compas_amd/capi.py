@@ -146,6 +146,7 @@ SIGNATURES = {
     "cmp_level_pf_evolve": (ctypes.c_int, [vp, dp, dp, dp, dp, P(ctypes.c_double), vp]),
     "cmp_level_tag": (ctypes.c_int, [vp, dp, P(cmp_tag_criterion), ctypes.c_int, dp, vp]),
     "cmp_level_average_down": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
+    "cmp_level_fill_two_levels": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
     "cmp_batch_create": (ctypes.c_int, [P(cmp_params), ctypes.c_longlong, ctypes.c_int, P(vp)]),
     "cmp_batch_destroy": (ctypes.c_int, [vp]),
     "cmp_batch_admit": (ctypes.c_int, [vp, P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,

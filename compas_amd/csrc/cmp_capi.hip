@@ -61,6 +61,8 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 }
 
 template <int DIM>
+hipError_t launch_fill2_dim(int, const FillDev&, const FillDev&, int, double*, const double*, int*, hipStream_t);
+template <int DIM>
 hipError_t launch_avgdown_dim(const BoxDev*, int, long long, const FillDev&, const double*, double*, int, int, double,
                               int*, hipStream_t);
 
@@ -1392,6 +1394,27 @@ int cmp_level_average_down(const cmp_level* F, const double* Uf, cmp_level* C, d
     CMP_TRY((dim == 2 ? launch_avgdown_dim<2> : launch_avgdown_dim<3>)(F->d_boxes, F->nbox, maxc, C->fd, Uf, Uc,
                                                                         F->P.nstate, ratio, volfrac, C->d_err,
                                                                         (hipStream_t)stream));
+    return CMP_OK;
+}
+
+int cmp_level_fill_two_levels(cmp_level* F, double* Uf, const cmp_level* C, const double* Uc, int ratio,
+                              void* stream)
+{
+    if (!F || !C || !Uf || !Uc || ratio < 2) return CMP_ERR_ARG;
+    const int dim = F->P.dim;
+    if (C->P.dim != dim || C->P.nstate != F->P.nstate) return CMP_ERR_ARG;
+    const int N = F->P.nstate;
+    if (N != 4 + dim && N != 5 + dim && N != 7 + dim) return CMP_ERR_ARG;
+    for (int d = 0; d < dim; ++d) {
+        if (F->G.domain_lo[d] != C->G.domain_lo[d] * ratio ||
+            F->G.domain_hi[d] + 1 != (C->G.domain_hi[d] + 1) * ratio || C->G.domain_lo[d] < 0 ||
+            F->G.is_periodic[d] != C->G.is_periodic[d] || F->G.lo_bc[d] != C->G.lo_bc[d] ||
+            F->G.hi_bc[d] != C->G.hi_bc[d])
+            return CMP_ERR_LAYOUT;
+        if (F->ng > F->G.domain_hi[d] - F->G.domain_lo[d] + 1) return CMP_ERR_LAYOUT;
+    }
+    CMP_TRY((dim == 2 ? launch_fill2_dim<2> : launch_fill2_dim<3>)(N, F->fd, C->fd, ratio, Uf, Uc, F->d_err,
+                                                                    (hipStream_t)stream));
     return CMP_OK;
 }
 

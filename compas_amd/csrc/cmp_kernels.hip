@@ -69,17 +69,15 @@ struct StageCtx {
     int halo[3][2];                  // the tile's halo sources (HaloSrc), by direction and side
 };
 
-// Where the ghost fill takes cell (ci, cj, ck) of a level from (FillPatchSingleLevel and the physical
-// BCs): the cell itself wrapped on periodic sides; on the others foextrap / first-order extrapolation
+// map_cell / fill_source: where the ghost fill takes cell (ci, cj, ck) of a level from
+// (FillPatchSingleLevel and the physical BCs): the cell itself wrapped on periodic sides; on the others foextrap / first-order extrapolation
 // (codes 0, 2) clamps to the domain, the reflecting codes (3 symmetry, 4 slip wall, 5 no-slip wall)
 // mirror it and flip the normal momentum (all momenta for code 5). si: the source cell's index in the
 // level buffer, ssn: its box's component stride, flip: bit m set = negate momentum m. Returns 0, or
 // CMP_ERR_BC (a code without a fill here) / CMP_ERR_LAYOUT (no box owns the source cell).
 template <int DIM>
-__device__ __forceinline__ int fill_source(const FillDev& fd, int ci, int cj, int ck, long long& si, long long& ssn,
-                                           unsigned& flip)
+__device__ __forceinline__ int map_cell(const FillDev& fd, int* src, unsigned& flip)
 {
-    int src[3] = {ci, cj, ck};
     flip = 0u;
     int ok = 1;
 #pragma unroll
@@ -106,10 +104,27 @@ __device__ __forceinline__ int fill_source(const FillDev& fd, int ci, int cj, in
 #pragma unroll
     for (int d = 0; d < DIM; ++d)
         if (src[d] < fd.dlo[d] || src[d] > fd.dhi[d]) return CMP_ERR_LAYOUT;
+    return 0;
+}
+
+// The box of the level owning domain cell src (-1: none).
+template <int DIM>
+__device__ __forceinline__ int cell_owner(const FillDev& fd, const int* src)
+{
     int bi = 0;
     for (int d = DIM - 1; d >= 0; --d)
         bi = bi * fd.nb[d] + (src[d] - fd.dlo[d]) / fd.bs;
-    const int own = fd.owner[bi];
+    return fd.owner[bi];
+}
+
+template <int DIM>
+__device__ __forceinline__ int fill_source(const FillDev& fd, int ci, int cj, int ck, long long& si, long long& ssn,
+                                           unsigned& flip)
+{
+    int src[3] = {ci, cj, ck};
+    const int e = map_cell<DIM>(fd, src, flip);
+    if (e) return e;
+    const int own = cell_owner<DIM>(fd, src);
     if (own < 0) return CMP_ERR_LAYOUT;
     const BoxDev so = fd.boxes[own];
     si = so.off + fab_index(so, src[0], src[1], src[2]);
@@ -638,6 +653,56 @@ void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
     if (fluxnc) fluxnc[f] = un;
 }
 
+// Ghost cell g (0 <= g < the box's ghost count) of a box with nv1 valid rows and fab extents
+// ex, ey, ez (ng ghosts a side), as fab coordinates: in 3D first the two z slabs (whole xy planes),
+// then per interior z plane the y-low rows, the y-high rows and the x ghosts of the interior rows.
+template <int DIM>
+__device__ __forceinline__ void ghost_xyz(int g, int ng, int ex, int ey, int ez, int nv1, int& x, int& y, int& z)
+{
+    x = 0;
+    y = 0;
+    z = 0;
+    int r = g;
+    bool slab = false;
+    if (DIM == 3) {
+        const int exy = ex * ey;
+        const int A = ng * exy;
+        if (r < 2 * A) {
+            // z-low (r < A) or z-high slab: whole xy planes
+            const bool high = r >= A;
+            if (high) r -= A;
+            z = (high ? ez - ng : 0) + r / exy;
+            r -= (r / exy) * exy;
+            y = r / ex;
+            x = r - y * ex;
+            slab = true;
+        } else {
+            r -= 2 * A;
+            const int Rg = 2 * ng * ex + 2 * ng * nv1;
+            const int q = r / Rg;
+            z = ng + q;
+            r -= q * Rg;
+        }
+    }
+    if (!slab) {
+        if (r < ng * ex) {
+            y = r / ex;
+            x = r - y * ex;
+        } else if (r < 2 * ng * ex) {
+            r -= ng * ex;
+            const int q = r / ex;
+            y = ey - ng + q;
+            x = r - q * ex;
+        } else {
+            r -= 2 * ng * ex;
+            const int q = r / (2 * ng);
+            y = ng + q;
+            const int cc = r - q * (2 * ng);
+            x = (cc < ng) ? cc : (ex - 2 * ng + cc);
+        }
+    }
+}
+
 // Same-level ghost fill. Thread g handles ghost cell g of the level: the box comes from a binary
 // search over ghost-count prefixes, the cell from the ghost shell's slab decomposition; the source
 // is mapped through the physical BCs (foextrap clamp, reflect mirror with odd momentum signs) and
@@ -659,46 +724,8 @@ void k_fill(FillDev fd, const int* __restrict__ list, int box0, double* U, int* 
     const int ez = (DIM == 3) ? nv2 + 2 * ng : 1;
     const int nghost = ex * ey * ez - nv0 * nv1 * nv2;
     for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < nghost; g += (int)(gridDim.x * NT)) {
-        int x = 0, y = 0, z = 0;
-        int r = g;
-        bool slab = false;
-        if (DIM == 3) {
-            const int exy = ex * ey;
-            const int A = ng * exy;
-            if (r < 2 * A) {
-                // z-low (r < A) or z-high slab: whole xy planes
-                const bool high = r >= A;
-                if (high) r -= A;
-                z = (high ? ez - ng : 0) + r / exy;
-                r -= (r / exy) * exy;
-                y = r / ex;
-                x = r - y * ex;
-                slab = true;
-            } else {
-                r -= 2 * A;
-                const int Rg = 2 * ng * ex + 2 * ng * nv1;
-                const int q = r / Rg;
-                z = ng + q;
-                r -= q * Rg;
-            }
-        }
-        if (!slab) {
-            if (r < ng * ex) {
-                y = r / ex;
-                x = r - y * ex;
-            } else if (r < 2 * ng * ex) {
-                r -= ng * ex;
-                const int q = r / ex;
-                y = ey - ng + q;
-                x = r - q * ex;
-            } else {
-                r -= 2 * ng * ex;
-                const int q = r / (2 * ng);
-                y = ng + q;
-                const int cc = r - q * (2 * ng);
-                x = (cc < ng) ? cc : (ex - 2 * ng + cc);
-            }
-        }
+        int x, y, z;
+        ghost_xyz<DIM>(g, ng, ex, ey, ez, nv1, x, y, z);
         const int cell[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
         long long si, ssn;
         unsigned flip;
@@ -1075,6 +1102,139 @@ __global__ __launch_bounds__(NT) void k_avgdown(const BoxDev* __restrict__ fboxe
     }
 }
 
+// A coarse level's cell c (any index within one cell of its domain) as the coarse ghost fill would
+// give it: mapped through the coarse level's BCs, read from the box owning it, momenta flipped.
+template <int DIM, int N, int MOM>
+__device__ __forceinline__ int coarse_value(const FillDev& cd, const double* __restrict__ Uc, int c0, int c1, int c2,
+                                            double* v)
+{
+    int src[3] = {c0, c1, c2};
+    unsigned flip;
+    const int e = map_cell<DIM>(cd, src, flip);
+    if (e) return e;
+    const int own = cell_owner<DIM>(cd, src);
+    if (own < 0) return CMP_ERR_LAYOUT;
+    const BoxDev cb = cd.boxes[own];
+    const long long ci = cb.off + fab_index(cb, src[0], src[1], src[2]);
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+        const double x = Uc[ci + n * cb.sn];
+        v[n] = (n >= MOM && n < MOM + DIM && ((flip >> (n - MOM)) & 1u)) ? -x : x;
+    }
+    return 0;
+}
+
+// Two-level ghost fill of a fine level (FillPatchTwoLevels, source/COMPAS.cpp:1679-1724). Thread g
+// handles ghost cell g of one fine box: its source cell is mapped through the fine level's BCs as
+// the same-level fill maps it; a source in a fine box is copied, any other is the cell-conservative
+// linear interpolation of the coarse level at it (AMReX's CellConservativeLinear with linear
+// limiting): per direction d the central coarse difference dc = (c+ - c-)/2 of every component, its
+// MC-limited value sx = sign(dc) min(|dc|, 2|c+ - c|, 2|c - c-|) (0 where the one-sided differences
+// have opposite signs), a factor f_d = min over the components with dc != 0 of sx/dc that scales
+// the slopes of all components alike, and the value c + sum_d off_d f_d dc_d, off_d the fine
+// cell's centre offset from the coarse cell's in coarse cells, (i - ic r + 1/2)/r - 1/2. Products
+// and sums are not contracted to FMAs, so the result is bit for bit the one of plain IEEE
+// arithmetic in this order.
+template <int DIM, int N>
+__global__ __launch_bounds__(NT)
+void k_fill2(FillDev fd, FillDev cd, int box0, int ratio, double* Uf, const double* __restrict__ Uc, int* err)
+{
+#pragma clang fp contract(off)
+    constexpr int MOM = (N == 4 + DIM) ? 2 : (N - DIM - 1) / 2;
+    const int b = box0 + (int)blockIdx.y;
+    const BoxDev bx = fd.boxes[b];
+    const int ng = fd.ng;
+    const int nv0 = bx.hi[0] - bx.lo[0] + 1, nv1 = bx.hi[1] - bx.lo[1] + 1;
+    const int nv2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
+    const int ex = nv0 + 2 * ng, ey = nv1 + 2 * ng;
+    const int ez = (DIM == 3) ? nv2 + 2 * ng : 1;
+    const int nghost = ex * ey * ez - nv0 * nv1 * nv2;
+    for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < nghost; g += (int)(gridDim.x * NT)) {
+        int x, y, z;
+        ghost_xyz<DIM>(g, ng, ex, ey, ez, nv1, x, y, z);
+        const int cell[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
+        const long long di = bx.off + fab_index(bx, cell[0], cell[1], cell[2]);
+        int src[3] = {cell[0], cell[1], cell[2]};
+        unsigned flip;
+        int e = map_cell<DIM>(fd, src, flip);
+        if (e) { atomicMax(err, e); continue; }
+        const int own = cell_owner<DIM>(fd, src);
+        double v[N];
+        if (own >= 0) {
+            const BoxDev so = fd.boxes[own];
+            const long long si = so.off + fab_index(so, src[0], src[1], src[2]);
+#pragma unroll
+            for (int n = 0; n < N; ++n) v[n] = Uf[si + n * so.sn];
+        } else {
+            int ic[3] = {src[0], src[1], src[2]};
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+                ic[d] = (src[d] >= 0) ? src[d] / ratio : -((-src[d] + ratio - 1) / ratio);
+            double c0[N];
+            e = coarse_value<DIM, N, MOM>(cd, Uc, ic[0], ic[1], ic[2], c0);
+#pragma unroll
+            for (int n = 0; n < N; ++n) v[n] = c0[n];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                if (e) break;
+                double cm[N], cp[N];
+                int a[3] = {ic[0], ic[1], ic[2]}, z3[3] = {ic[0], ic[1], ic[2]};
+                a[d] -= 1;
+                z3[d] += 1;
+                e = coarse_value<DIM, N, MOM>(cd, Uc, a[0], a[1], a[2], cm);
+                if (!e) e = coarse_value<DIM, N, MOM>(cd, Uc, z3[0], z3[1], z3[2], cp);
+                if (e) break;
+                double sf = 1.0;
+#pragma unroll
+                for (int n = 0; n < N; ++n) {
+                    const double dc = 0.5 * (cp[n] - cm[n]);
+                    const double df = 2.0 * (cp[n] - c0[n]);
+                    const double db = 2.0 * (c0[n] - cm[n]);
+                    const double af = fabs(df), ab = fabs(db);
+                    double sx = (df * db >= 0.0) ? ((ab < af) ? ab : af) : 0.0;
+                    const double adc = fabs(dc);
+                    sx = copysign(1.0, dc) * ((adc < sx) ? adc : sx);
+                    if (dc != 0.0) {
+                        const double r = sx / dc;
+                        sf = (r < sf) ? r : sf;
+                    }
+                    cm[n] = dc;
+                }
+                const double off = ((double)(src[d] - ic[d] * ratio) + 0.5) / (double)ratio - 0.5;
+#pragma unroll
+                for (int n = 0; n < N; ++n) v[n] = v[n] + off * (cm[n] * sf);
+            }
+            if (e) { atomicMax(err, e); continue; }
+        }
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+            const bool neg = n >= MOM && n < MOM + DIM && ((flip >> (n - MOM)) & 1u);
+            Uf[di + n * bx.sn] = neg ? -v[n] : v[n];
+        }
+    }
+}
+
+template <int DIM>
+hipError_t launch_fill2_dim(int N, const FillDev& fd, const FillDev& cd, int ratio, double* Uf, const double* Uc,
+                            int* err, hipStream_t s)
+{
+    if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM) return hipErrorInvalidValue;
+    long long cols = (fd.maxghost + NT - 1) / NT;
+    if (cols <= 0) return hipSuccess;
+    if (cols > 65536) cols = 65536;
+    for (int box0 = 0; box0 < fd.nbox; box0 += 65535) {
+        const int rows = (fd.nbox - box0 < 65535) ? fd.nbox - box0 : 65535;
+        const dim3 grid((unsigned)cols, (unsigned)rows);
+        if (N == 4 + DIM)
+            hipLaunchKernelGGL((k_fill2<DIM, 4 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, Uf, Uc, err);
+        else if (N == 5 + DIM)
+            hipLaunchKernelGGL((k_fill2<DIM, 5 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, Uf, Uc, err);
+        else
+            hipLaunchKernelGGL((k_fill2<DIM, 7 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, Uf, Uc, err);
+    }
+    return hipGetLastError();
+}
+
 template <int DIM>
 hipError_t launch_avgdown_dim(const BoxDev* fboxes, int nfbox, long long maxcells, const FillDev& cfd,
                               const double* Uf, double* Uc, int N, int ratio, double volfrac, int* err, hipStream_t s)
@@ -1139,6 +1299,8 @@ void tile_dims_dim(int* tx, int* ty, int* tz)
 }
 
 template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
+template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev&, int, double*,
+                                                     const double*, int*, hipStream_t);
 template hipError_t launch_avgdown_dim<CMP_KERNEL_DIM>(const BoxDev*, int, long long, const FillDev&, const double*,
                                                        double*, int, int, double, int*, hipStream_t);
 template hipError_t launch_tag_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev*, int, long long,

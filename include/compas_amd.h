@@ -310,6 +310,18 @@ int cmp_level_tag(cmp_level* level, const double* U, const cmp_tag_criterion* cr
 int cmp_level_average_down(const cmp_level* fine, const double* Uf, cmp_level* coarse, double* Uc, int ratio,
                            void* stream);
 
+/* FillPatchTwoLevels (source/COMPAS.cpp:1679-1724, with the reference's interpolater
+ * amrex::lincc_interp, CellConservativeLinear with linear limiting): every ghost cell of the fine
+ * level's boxes is mapped to its source cell through the physical BCs and periodic wrap as the
+ * same-level fill maps it; a source in a fine box is copied, any other is interpolated from the
+ * coarse level (cell-conservative linear, MC-limited, one limiter factor per direction shared by
+ * all components), the coarse neighbours read through the coarse level's own BCs. The fine domain
+ * is the coarse one refined by ratio, with the same BCs; Uc's ghost cells are not read. A coarse
+ * cell the coarse level does not hold is CMP_ERR_LAYOUT in the fine level's error word, and the
+ * ghost keeps its value. Flow states only (two-phase or N-phase). */
+int cmp_level_fill_two_levels(cmp_level* fine, double* Uf, const cmp_level* coarse, const double* Uc, int ratio,
+                              void* stream);
+
 /* ---- a batch of independent levels (the served-stream form) ---------------------------------
  * No counterpart in the reference, which runs one case per process (source/main.cpp:21-86): cases of
  * different grids and parameters are admitted while others run, keep their state in device memory,

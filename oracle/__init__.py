@@ -94,6 +94,8 @@ def load():
         "ref_level_max_velocity": (D, [PP, ctypes.c_int, PB, ctypes.c_int, PD]),
         "ref_level_average_down": (ctypes.c_int, [PP, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, PB,
                                                   ctypes.c_int, PD, ctypes.c_int]),
+        "ref_level_fill_two_levels": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, PG, ctypes.c_int,
+                                                     PB, ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_tag": (ctypes.c_int, [PP, ctypes.c_int, PB, ctypes.c_int, PD,
                                          ctypes.POINTER(cmp_tag_criterion), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_ubyte)]),
@@ -454,6 +456,16 @@ def level_average_down(params, flayout, fbuf, clayout, cbuf, ratio):
                                        clayout.nbox, clayout.cboxes, clayout.ng, _dptr(cbuf), int(ratio))
     if rc:
         raise RuntimeError("ref_level_average_down failed: %d" % rc)
+
+
+def level_fill_two_levels(params, fgeom, flayout, fbuf, cgeom, clayout, cbuf, ratio):
+    """In place on fbuf: every ghost cell of the fine level (fine sources copied, coarse-fine ghosts
+    by lincc from cbuf)."""
+    rc = load().ref_level_fill_two_levels(ctypes.byref(params), ctypes.byref(fgeom), flayout.nbox, flayout.cboxes,
+                                          flayout.ng, _dptr(fbuf), ctypes.byref(cgeom), clayout.nbox, clayout.cboxes,
+                                          clayout.ng, _dptr(cbuf), int(ratio))
+    if rc:
+        raise RuntimeError("ref_level_fill_two_levels failed: %d" % rc)
 
 
 def level_tag(params, layout, buf, criteria):

@@ -2584,3 +2584,133 @@ int ref_level_average_down(const cmp_params* P, int nbf, const cmp_box* fboxes, 
     }
     return CMP_OK;
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * FillPatchTwoLevels (source/COMPAS.cpp:1679-1724) with lincc_interp: every ghost cell of the fine
+ * level is mapped to its source cell as the same-level fill maps it (periodic image, clamp or
+ * mirror with odd momenta); a source cell in a fine box is copied, any other takes the coarse
+ * level's cell-conservative linear interpolation at it. lincc_interp is AMReX 26.01's
+ * CellConservativeLinear with linear limiting (not vendored; its published algorithm): per
+ * direction the central coarse difference of every component, dc = (c+ - c-)/2, its MC-limited
+ * value sx = sign(dc) min(|dc|, 2|c+ - c|, 2|c - c-|) (0 where the one-sided differences disagree in
+ * sign), one factor per direction f = min over the components with dc != 0 of sx/dc (the slopes
+ * of all components scaled alike), and the fine value c + sum_d off_d f_d dc_d with off_d the fine
+ * cell's centre offset in coarse cells, (i - ic r + 1/2)/r - 1/2. The coarse neighbours come from
+ * the coarse level by the same mapping.
+ * ---------------------------------------------------------------------------------------------- */
+static int map_source(int dim, const cmp_geometry* G, int N, int mom0, int* src, double* sgn)
+{
+    for (int n = 0; n < N; ++n) sgn[n] = 1.0;
+    for (int d = 0; d < dim; ++d) {
+        int dlo = G->domain_lo[d], dhi = G->domain_hi[d];
+        if (src[d] >= dlo && src[d] <= dhi) continue;
+        int per = dhi - dlo + 1;
+        if (G->is_periodic[d]) {
+            while (src[d] < dlo) src[d] += per;
+            while (src[d] > dhi) src[d] -= per;
+            continue;
+        }
+        int code = (src[d] < dlo) ? G->lo_bc[d] : G->hi_bc[d];
+        if (code == 2 || code == 0) {
+            src[d] = (src[d] < dlo) ? dlo : dhi;
+        } else if (code >= 3 && code <= 5) {
+            src[d] = (src[d] < dlo) ? (2 * dlo - 1 - src[d]) : (2 * dhi + 1 - src[d]);
+            for (int m = 0; m < dim; ++m)
+                if (m == d || code == 5) sgn[mom0 + m] = -sgn[mom0 + m];
+        } else {
+            return CMP_ERR_BC;
+        }
+    }
+    return CMP_OK;
+}
+
+static int find_box(int dim, int nbox, const cmp_box* boxes, const int* c)
+{
+    for (int q = 0; q < nbox; ++q)
+        if (box_contains(dim, &boxes[q], c)) return q;
+    return -1;
+}
+
+static int coarse_cell(int dim, int N, int mom0, const cmp_geometry* Gc, int nbc, const cmp_box* cb, int ngc,
+                       const double* Uc, const int* cell, double* v)
+{
+    int src[3] = {cell[0], cell[1], cell[2]};
+    double sgn[REF_NMAX];
+    int e = map_source(dim, Gc, N, mom0, src, sgn);
+    if (e) return e;
+    int q = find_box(dim, nbc, cb, src);
+    if (q < 0) return CMP_ERR_LAYOUT;
+    fab_t f = make_fab(dim, &cb[q], ngc);
+    const long long i = cb[q].offset + fab_idx(&f, src[0], src[1], src[2]);
+    for (int n = 0; n < N; ++n) v[n] = sgn[n] * Uc[i + n * f.sn];
+    return CMP_OK;
+}
+
+static int floor_div(int a, int r) { return (a >= 0) ? a / r : -((-a + r - 1) / r); }
+
+int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
+                              double* Uf, const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc,
+                              const double* Uc, int ratio)
+{
+    const int dim = P->dim, N = P->nstate, mom0 = P->nphase ? P->nphase : 2;
+    int err = 0;
+    for (int b = 0; b < nbf; ++b) {
+        const cmp_box* bx = &fboxes[b];
+        fab_t f = make_fab(dim, bx, ngf);
+        double* base = Uf + bx->offset;
+        for (int k = f.lo[2]; k <= f.lo[2] + f.ext[2] - 1; ++k)
+            for (int j = f.lo[1]; j <= f.lo[1] + f.ext[1] - 1; ++j)
+                for (int i = f.lo[0]; i <= f.lo[0] + f.ext[0] - 1; ++i) {
+                    int c[3] = {i, j, k};
+                    if (box_contains(dim, bx, c)) continue;
+                    int src[3] = {i, j, k};
+                    double sgn[REF_NMAX], v[REF_NMAX];
+                    int e = map_source(dim, Gf, N, mom0, src, sgn);
+                    if (e) { if (e > err) err = e; continue; }
+                    const long long di = fab_idx(&f, i, j, k);
+                    int q = find_box(dim, nbf, fboxes, src);
+                    if (q >= 0) {
+                        fab_t fs = make_fab(dim, &fboxes[q], ngf);
+                        const long long si = fboxes[q].offset + fab_idx(&fs, src[0], src[1], src[2]);
+                        for (int n = 0; n < N; ++n) base[di + n * f.sn] = sgn[n] * Uf[si + n * fs.sn];
+                        continue;
+                    }
+                    /* coarse-fine ghost: lincc at the source cell */
+                    int ic[3] = {0, 0, 0};
+                    for (int d = 0; d < 3; ++d) ic[d] = (d < dim) ? floor_div(src[d], ratio) : src[d];
+                    double c0[REF_NMAX], cm[3][REF_NMAX], cp[3][REF_NMAX], slope[3][REF_NMAX];
+                    e = coarse_cell(dim, N, mom0, Gc, nbc, cboxes, ngc, Uc, ic, c0);
+                    for (int d = 0; d < dim && !e; ++d) {
+                        int a[3] = {ic[0], ic[1], ic[2]}, z[3] = {ic[0], ic[1], ic[2]};
+                        a[d] -= 1;
+                        z[d] += 1;
+                        e = coarse_cell(dim, N, mom0, Gc, nbc, cboxes, ngc, Uc, a, cm[d]);
+                        if (!e) e = coarse_cell(dim, N, mom0, Gc, nbc, cboxes, ngc, Uc, z, cp[d]);
+                    }
+                    if (e) { if (e > err) err = e; continue; }
+                    for (int d = 0; d < dim; ++d) {
+                        double sf = 1.0;
+                        for (int n = 0; n < N; ++n) {
+                            const double dc = 0.5 * (cp[d][n] - cm[d][n]);
+                            const double df = 2.0 * (cp[d][n] - c0[n]);
+                            const double db = 2.0 * (c0[n] - cm[d][n]);
+                            double sx = (df * db >= 0.0) ? smin(fabs(df), fabs(db)) : 0.0;
+                            sx = copysign(1.0, dc) * smin(sx, fabs(dc));
+                            if (dc != 0.0) sf = smin(sf, sx / dc);
+                            slope[d][n] = dc;
+                        }
+                        for (int n = 0; n < N; ++n) slope[d][n] *= sf;
+                    }
+                    for (int n = 0; n < N; ++n) {
+                        double val = c0[n];
+                        for (int d = 0; d < dim; ++d) {
+                            const double off = ((double)(src[d] - ic[d] * ratio) + 0.5) / (double)ratio - 0.5;
+                            val = val + off * slope[d][n];
+                        }
+                        v[n] = val;
+                    }
+                    for (int n = 0; n < N; ++n) base[di + n * f.sn] = sgn[n] * v[n];
+                }
+    }
+    return err;
+}

@@ -154,6 +154,12 @@ int ref_level_tag(const cmp_params* P, int nbox, const cmp_box* boxes, int ng, c
 int ref_level_average_down(const cmp_params* P, int nbf, const cmp_box* fboxes, int ngf, const double* Uf,
                            int nbc, const cmp_box* cboxes, int ngc, double* Uc, int ratio);
 
+/* FillPatchTwoLevels with lincc_interp: every ghost cell of the fine level (same-level sources
+ * copied, coarse-fine ghosts interpolated from the coarse level) */
+int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
+                              double* Uf, const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc,
+                              const double* Uc, int ratio);
+
 /* number of OpenMP threads the level functions use (1 when built without OpenMP) */
 int ref_num_threads(void);
 void ref_set_num_threads(int n);

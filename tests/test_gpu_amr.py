@@ -39,3 +39,43 @@ def test_average_down_matches_oracle(gpu, dim, ratio, nphase):
     cc.close()
     assert np.array_equal(got, ref)
     assert (got != cbuf).sum() > 0
+
+
+# fine boxes (fine cells) for the two-level fill: touching domain faces, touching each other, and
+# apart, so that ghosts come from fine neighbours, the coarse level and the physical BCs
+FILL2 = {(2, 2): [([0, 8], [15, 23]), ([16, 8], [23, 31]), ([24, 0], [31, 7])],
+         (2, 4): [([8, 0], [23, 15]), ([24, 16], [31, 31])],
+         (3, 2): [([0, 4, 0], [7, 11, 7]), ([8, 4, 0], [15, 11, 7]), ([4, 12, 8], [11, 15, 15])]}
+BCS = {2: dict(per=(0, 1), lo_bc=[4, 0], hi_bc=[2, 0]),
+       3: dict(per=(0, 0, 1), lo_bc=[5, 3, 0], hi_bc=[4, 2, 0])}
+
+
+@pytest.mark.parametrize("dim,ratio,nphase", [(2, 2, 0), (2, 4, 0), (3, 2, 0), (2, 2, 3), (3, 2, 2)])
+def test_fill_two_levels_matches_oracle(gpu, dim, ratio, nphase):
+    import torch
+    nc = 8 if (dim, ratio) == (2, 4) or dim == 3 else 16
+    if nphase:
+        P = make_params(dim, gamma=(1.4, 1.67, 1.09)[:nphase], nphase=nphase)
+    else:
+        P = make_params(dim, gamma=(1.4, 1.6))
+    bc = BCS[dim]
+    cg = make_geometry((nc,) * dim, (0.0,) * dim, (1.0,) * dim, bc["per"], lo_bc=bc["lo_bc"], hi_bc=bc["hi_bc"])
+    fg = make_geometry((nc * ratio,) * dim, (0.0,) * dim, (1.0,) * dim, bc["per"], lo_bc=bc["lo_bc"],
+                       hi_bc=bc["hi_bc"])
+    cl = Layout(dim, split_domain((nc,) * dim, max(4, nc // 2)), 3, P.nstate)
+    fl = Layout(dim, FILL2[(dim, ratio)], 3, P.nstate)
+    rng = np.random.default_rng(11 + dim + ratio + nphase)
+    cbuf = rng.uniform(0.2, 2.0, cl.total)
+    fbuf = rng.uniform(0.2, 2.0, fl.total)
+    ref = fbuf.copy()
+    oracle.level_fill_two_levels(P, fg, fl, ref, cg, cl, cbuf, ratio)
+    Uf = torch.from_numpy(fbuf).to(gpu)
+    Uc = torch.from_numpy(cbuf).to(gpu)
+    lf, lc = Level(P, fg, fl), Level(P, cg, cl)
+    lf.FillPatchTwoLevels(Uf, lc, Uc, ratio)
+    assert lf.check_errors() == 0
+    got = Uf.cpu().numpy()
+    lf.close()
+    lc.close()
+    assert np.array_equal(got, ref)
+    assert (got != fbuf).sum() > 0

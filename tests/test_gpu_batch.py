@@ -143,13 +143,14 @@ def test_batched_case_matches_oracle(gpu):
 
 
 def test_served_stream_results_equal_alone_runs(gpu):
-    """The request loop end to end: eight seeded cases of three grid sizes arriving at 400 cases/s
-    (so that several are in flight together and later ones join mid-run), 12 steps each; every
-    final state is bit-identical to the case run alone."""
+    """The request loop end to end: eight seeded cases of three grid sizes arriving at 2000 cases/s
+    (arrivals spread over a few milliseconds, so that several are in flight together and later ones
+    join mid-run; how many depends on the GPU's speed), 12 steps each; every final state is
+    bit-identical to the case run alone."""
     from compas_amd import serve
     grids = [(64, 64), (128, 32), (96, 48)]
-    res = serve.serve(400.0, 8, steps=12, seed=5, max_grid_size=32, grids=grids, keep=set(range(8)))
-    assert res["mean_cases_per_tick"] > 1.5
+    res = serve.serve(2000.0, 8, steps=12, seed=5, max_grid_size=32, grids=grids, keep=set(range(8)))
+    assert res["mean_cases_per_tick"] > 1.0
     assert res["p99_s"] >= res["p50_s"] > 0.0
     for i in range(8):
         c = cases.served_case(5 * 100003 + i, max_grid_size=32, grids=grids)

@@ -79,3 +79,84 @@ def test_misaligned_fine_box_is_refused():
     fl = Layout(2, [([3, 4], [10, 11])], 3, P.nstate)
     with pytest.raises(RuntimeError, match="failed: 7"):
         oracle.level_average_down(P, fl, np.zeros(fl.total), cl, np.zeros(cl.total), 2)
+
+
+def _two_levels(ng=4):
+    from compas_amd import make_geometry
+    P = make_params(2, gamma=(1.4, 1.6))
+    cg = make_geometry((16, 16), (0.0, 0.0), (1.0, 1.0), (0, 0), lo_bc=[2, 4], hi_bc=[2, 4])
+    fg = make_geometry((32, 32), (0.0, 0.0), (1.0, 1.0), (0, 0), lo_bc=[2, 4], hi_bc=[2, 4])
+    cl = Layout(2, split_domain((16, 16), 8), 3, P.nstate)
+    fl = Layout(2, [([8, 8], [23, 23]), ([24, 8], [31, 23])], ng, P.nstate)
+    return P, cg, fg, cl, fl
+
+
+def _coarse_fill(cl, f):
+    buf = np.zeros(cl.total)
+    for b, (lo, hi) in enumerate(cl.boxes):
+        v = cl.fab_view(buf, b)
+        idx = np.indices(v.shape[1:])
+        X = [idx[1 - d] + lo[d] - cl.ng for d in range(2)]
+        for n in range(v.shape[0]):
+            v[n] = f(n, X)
+    return buf
+
+
+def test_lincc_is_exact_for_linear_data_and_copies_fine_sources():
+    P, cg, fg, cl, fl = _two_levels()
+    lin = lambda n, X: 1.0 + n + (0.3 + 0.1 * n) * (X[0] + 0.5) + (0.2 - 0.05 * n) * (X[1] + 0.5)
+    cbuf = _coarse_fill(cl, lin)
+    fbuf = np.zeros(fl.total)
+    for b in range(fl.nbox):          # fine valid cells: anything (only their ghosts are checked)
+        fl.fab_view(fbuf, b)[...] = 100.0 + b
+    oracle.level_fill_two_levels(P, fg, fl, fbuf, cg, cl, cbuf, 2)
+    v = fl.fab_view(fbuf, 0)
+    ng = fl.ng
+    for n in (0, 4):
+        for jj in range(v.shape[1]):
+            for ii in range(v.shape[2]):
+                i, j = 8 - ng + ii, 8 - ng + jj
+                if 8 <= i <= 23 and 8 <= j <= 23:
+                    continue
+                if 24 <= i <= 31 and 8 <= j <= 23:
+                    assert v[n, jj, ii] == 101.0          # a same-level source is copied
+                    continue
+                if not (0 <= j <= 31):
+                    continue                             # slip walls mirror; checked below
+                exact = 1.0 + n + (0.3 + 0.1 * n) * (i + 0.5) / 2 + (0.2 - 0.05 * n) * (j + 0.5) / 2
+                assert abs(v[n, jj, ii] - exact) < 1e-13, (i, j, n)
+
+
+def test_lincc_conserves_and_stays_bounded():
+    P, cg, fg, cl, fl = _two_levels(ng=4)
+    rng = np.random.default_rng(4)
+    cbuf = _coarse_fill(cl, lambda n, X: rng.uniform(0.5, 2.0, X[0].shape))
+    fbuf = np.zeros(fl.total)
+    oracle.level_fill_two_levels(P, fg, fl, fbuf, cg, cl, cbuf, 2)
+    v = fl.fab_view(fbuf, 0)                       # fine box (8..23)^2 with 4 ghosts: 2 coarse cells
+    cv = {}
+    for b, (lo, hi) in enumerate(cl.boxes):
+        cf = cl.fab_view(cbuf, b)
+        for j in range(lo[1], hi[1] + 1):
+            for i in range(lo[0], hi[0] + 1):
+                cv[(i, j)] = cf[:, j - lo[1] + 3, i - lo[0] + 3]
+    checked = 0
+    for jc in range(2, 14):
+        for ic in range(2, 14):
+            if 4 <= ic <= 11 and 4 <= jc <= 11:
+                continue                             # under the fine box
+            if ic >= 12 and 4 <= jc <= 11:
+                continue                             # under the second fine box (copied)
+            if not (2 <= ic <= 13 and 2 <= jc <= 13):
+                continue
+            ii, jj = 2 * ic - 8 + 4, 2 * jc - 8 + 4
+            if not (0 <= ii and ii + 1 < v.shape[2] and 0 <= jj and jj + 1 < v.shape[1]):
+                continue
+            kids = v[:, jj:jj + 2, ii:ii + 2]
+            assert np.allclose(kids.mean(axis=(1, 2)), cv[(ic, jc)], rtol=1e-14, atol=0)
+            nb = np.array([cv[(ic + a, jc + b)] for a in (-1, 0, 1) for b in (-1, 0, 1)
+                           if (ic + a, jc + b) in cv])
+            assert np.all(kids.min(axis=(1, 2)) >= nb.min(axis=0) - 1e-14)
+            assert np.all(kids.max(axis=(1, 2)) <= nb.max(axis=0) + 1e-14)
+            checked += 1
+    assert checked > 20
