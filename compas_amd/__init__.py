@@ -398,6 +398,63 @@ def rhs_box(params, geom, box, ng, U, dUdt, cmax=None, fluxes=(None, None, None)
           "cmp_rhs_box")
 
 
+class Amr:
+    """A hierarchy of Levels (coarsest first, each the refinement of the previous by ratio) stepped
+    together without subcycling (cmp_amr_*): every RK stage fills every level from the stage state,
+    gives the coarse faces under a finer level the finer level's face means, and averages down.
+    Level buffers (torch CUDA tensors, one per level) stay the caller's."""
+
+    def __init__(self, levels, ratio=2):
+        self.lib = capi.load()
+        self.levels = list(levels)
+        self.ratio = int(ratio)
+        n = len(self.levels)
+        arr = (ctypes.c_void_p * n)(*[lv.h.value for lv in self.levels])
+        h = ctypes.c_void_p()
+        check(self.lib.cmp_amr_create(arr, n, self.ratio, ctypes.byref(h)), "cmp_amr_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.cmp_amr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _arr(ts):
+        return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def cf_cells(self, level):
+        n = ctypes.c_int()
+        check(self.lib.cmp_amr_info(self.h, int(level), ctypes.byref(n)), "cmp_amr_info")
+        return n.value
+
+    def initial_cmax(self, U, cmax, stream=None):
+        check(self.lib.cmp_amr_initial_cmax(self.h, self._arr(U), _ptr(cmax), _stream(stream)),
+              "cmp_amr_initial_cmax")
+
+    def compute_dt(self, cmax, dt_dev, cfl, limiter=1, stream=None):
+        check(self.lib.cmp_amr_compute_dt(self.h, _ptr(cmax), _ptr(dt_dev), float(cfl), int(limiter),
+                                          _stream(stream)), "cmp_amr_compute_dt")
+
+    def rk3_step(self, Un, U1, U2, dt_dev, cmax, stream=None):
+        check(self.lib.cmp_amr_rk3_step(self.h, self._arr(Un), self._arr(U1), self._arr(U2), _ptr(dt_dev),
+                                        _ptr(cmax), _stream(stream)), "cmp_amr_rk3_step")
+
+    def advance(self, Un, U1, U2, nsteps, cfl, dt_dev, cmax, dt_hist=None, limiter=1, stream=None):
+        check(self.lib.cmp_amr_advance(self.h, self._arr(Un), self._arr(U1), self._arr(U2), int(nsteps),
+                                       float(cfl), int(limiter), _ptr(dt_dev), _ptr(cmax), _ptr(dt_hist),
+                                       _stream(stream)), "cmp_amr_advance")
+
+    def check_errors(self, stream=None):
+        return max(lv.check_errors(stream=stream) for lv in self.levels)
+
+
 class Batch:
     """Independent levels ("cases") advanced together (cmp_batch_*): each admitted case keeps its
     state in device memory; step() advances every case by one RK3 step with its own dt, in one

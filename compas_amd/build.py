@@ -45,6 +45,7 @@ def build(verbose=False):
         ("cmp_pf.hip", "pf3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_capi.hip", "capi.o", []),
         ("cmp_batch.hip", "batch.o", []),
+        ("cmp_amr.hip", "amr.o", []),
     ]
     procs = []
     for src, obj, extra in units:

@@ -147,6 +147,15 @@ SIGNATURES = {
     "cmp_level_tag": (ctypes.c_int, [vp, dp, P(cmp_tag_criterion), ctypes.c_int, dp, vp]),
     "cmp_level_average_down": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
     "cmp_level_fill_two_levels": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
+    "cmp_amr_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
+    "cmp_amr_destroy": (ctypes.c_int, [vp]),
+    "cmp_amr_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "cmp_amr_initial_cmax": (ctypes.c_int, [vp, ctypes.POINTER(dp), dp, vp]),
+    "cmp_amr_compute_dt": (ctypes.c_int, [vp, dp, dp, ctypes.c_double, ctypes.c_int, vp]),
+    "cmp_amr_rk3_step": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), dp, dp,
+                                        vp]),
+    "cmp_amr_advance": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int, dp, dp, dp, vp]),
     "cmp_batch_create": (ctypes.c_int, [P(cmp_params), ctypes.c_longlong, ctypes.c_int, P(vp)]),
     "cmp_batch_destroy": (ctypes.c_int, [vp]),
     "cmp_batch_admit": (ctypes.c_int, [vp, P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,

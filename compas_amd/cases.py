@@ -148,6 +148,22 @@ def random_smooth(dim, n, seed, periodic=True, max_grid_size=None, scheme="WENO5
                 name="random-smooth-%dd" % dim)
 
 
+def two_level_smooth(dim, nc, seed, fine_boxes, ratio=2, periodic=True, **kw):
+    """A two-level hierarchy of the seeded smooth state: the fine level's valid cells from
+    random_smooth at nc * ratio cells a side, the coarse level (nc a side) the mean of the fine
+    field over each coarse cell. fine_boxes in fine cells. Returns dict(params, geoms, boxes,
+    fields, shapes, cfl) with the coarse level first."""
+    f = random_smooth(dim, nc * ratio, seed, periodic=periodic, **kw)
+    Uf = f["U"]
+    shp = Uf.shape[1:]
+    Uc = Uf.reshape((Uf.shape[0],) + tuple(x for n in shp for x in (n // ratio, ratio)))
+    Uc = Uc.mean(axis=tuple(2 + 2 * a for a in range(dim)))
+    cg = make_geometry((nc,) * dim, (0.0,) * dim, (1.0,) * dim, (int(periodic),) * dim)
+    return dict(params=f["params"], geoms=[cg, f["geom"]], boxes=[split_domain((nc,) * dim, max(8, nc // 2)),
+                                                                  [(list(lo), list(hi)) for lo, hi in fine_boxes]],
+                fields=[np.ascontiguousarray(Uc), Uf], shapes=[Uc.shape[1:], shp], cfl=f["cfl"])
+
+
 # ---- the N-phase model (cmp_params.nphase = 2 or 3) --------------------------------------------------
 
 def np_prim_to_cons(params, W):

@@ -599,33 +599,19 @@ void k_stage_batch(const KParams* __restrict__ kps, int limiter, int stage, cons
                                                dts + cs, cmax_bits + cs, errs + cs, nullptr, fds + cs, sWF, sRed);
 }
 
-// Face fluxes of one box in the reference's shape (one thread per face, every stencil cell taken
-// through cons->prim by the thread that uses it), for callers that need the face arrays (flux
-// registers, face averaging) and as a second GPU path the fused kernel is checked against.
+// The face flux and u*_n of the face between cells (ci, cj, ck) - e_DIR and (ci, cj, ck) of box bx
+// (Ub: the box's fab, its ghost cells filled), every stencil cell taken through cons->prim here.
 template <int DIM, int SCHEME, int RIEMANN, bool THINC, int DIR>
-__global__ __launch_bounds__(NT)
-void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
-             double* __restrict__ flux, double* __restrict__ fluxnc)
+__device__ __forceinline__ double face_at(const KParams& kp, int limiter, const BoxDev& bx,
+                                          const double* __restrict__ Ub, int ci, int cj, int ck, double* Fl,
+                                          int& bad)
 {
     constexpr int N = 4 + DIM;
     constexpr int IA = 3 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
-    constexpr int dir = DIR;
-    const int fe0 = bx.hi[0] - bx.lo[0] + 1 + (DIR == 0 ? 1 : 0);
-    const int fe1 = bx.hi[1] - bx.lo[1] + 1 + (DIR == 1 ? 1 : 0);
-    const int fe2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + (DIR == 2 ? 1 : 0) : 1;
-    const long long F = (long long)fe0 * fe1 * fe2;
-    const long long f = (long long)blockIdx.x * NT + threadIdx.x;
-    if (f >= F) return;
-    const int x = (int)(f % fe0);
-    const int y = (int)((f / fe0) % fe1);
-    const int z = (int)(f / ((long long)fe0 * fe1));
-    const int ci = bx.lo[0] + x, cj = bx.lo[1] + y, ck = bx.lo[2] + z;
     const long long sd = (DIR == 0) ? 1 : ((DIR == 1) ? bx.sj : bx.sk);
     const long long c0 = fab_index(bx, ci, cj, ck);
-    const double* Ub = U + bx.off;
     double W[2 * G][N];
-    int bad = 0;
 #pragma unroll
     for (int t = 0; t < 2 * G; ++t) {
         double Uc[N];
@@ -644,8 +630,32 @@ void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
         return Ub[fab_index(bx, px, py, pz) + IA * bx.sn];
     };
     auto normal = [&]() -> double { return face_normal_component<DIM>(kp, DIR, aa); };
-    double Fl[N], cm;
-    const double un = face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, dir, st, normal, Fl, cm, bad);
+    double cm;
+    return face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, DIR, st, normal, Fl, cm, bad);
+}
+
+// Face fluxes of one box in the reference's shape (one thread per face, every stencil cell taken
+// through cons->prim by the thread that uses it), for callers that need the face arrays (flux
+// registers, face averaging) and as a second GPU path the fused kernel is checked against.
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int DIR>
+__global__ __launch_bounds__(NT)
+void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
+             double* __restrict__ flux, double* __restrict__ fluxnc)
+{
+    constexpr int N = 4 + DIM;
+    const int fe0 = bx.hi[0] - bx.lo[0] + 1 + (DIR == 0 ? 1 : 0);
+    const int fe1 = bx.hi[1] - bx.lo[1] + 1 + (DIR == 1 ? 1 : 0);
+    const int fe2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + (DIR == 2 ? 1 : 0) : 1;
+    const long long F = (long long)fe0 * fe1 * fe2;
+    const long long f = (long long)blockIdx.x * NT + threadIdx.x;
+    if (f >= F) return;
+    const int x = (int)(f % fe0);
+    const int y = (int)((f / fe0) % fe1);
+    const int z = (int)(f / ((long long)fe0 * fe1));
+    int bad = 0;
+    double Fl[N];
+    const double un = face_at<DIM, SCHEME, RIEMANN, THINC, DIR>(kp, limiter, bx, U + bx.off, bx.lo[0] + x,
+                                                                bx.lo[1] + y, bx.lo[2] + z, Fl, bad);
     if (flux) {
 #pragma unroll
         for (int q = 0; q < N; ++q) flux[f + q * F] = Fl[q];
@@ -799,6 +809,41 @@ hipError_t launch_batch_dt(const BatchSlot* slots, int nslot, double* cmax, doub
     if (nslot <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_batch_dt, dim3((nslot + 63) / 64), dim3(64), 0, s, slots, nslot, cmax, dt, time, steps,
                        errs);
+    return hipGetLastError();
+}
+
+// ComputeDt over the levels of a hierarchy without subcycling (source/COMPAS.cpp:2250-2302 with
+// nsubsteps 1): each level's estimate min_d cfl dx_d / c_max, limited to 1.1 dt_prev when the
+// limiter is on and dt_prev > 0 (a first step passes dt_prev = 0), the smallest over the levels;
+// every level's c_max then restarts at zero. One thread.
+__global__ void k_amr_dt(AmrDt a, double* cmax, double* dt, double cfl, int limiter, double* hist, int step,
+                         int* err)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double prev = *dt;
+    double dt0 = 0.0;
+    int bad = 0;
+    for (int l = 0; l < a.nlev; ++l) {
+        const double cm = cmax[l];
+        if (!(cm > 0.0)) bad = 1;
+        double est = 1.7976931348623157e308;
+        for (int d = 0; d < a.dim; ++d) {
+            const double e = cfl * a.dx[l][d] / cm;
+            est = (e < est) ? e : est;
+        }
+        if (limiter && prev > 0.0) est = smin(est, 1.1 * prev);
+        dt0 = (l == 0) ? est : smin(dt0, est);
+    }
+    if (bad || __builtin_isnan(dt0)) atomicMax(err, CMP_ERR_NAN);
+    *dt = dt0;
+    if (hist) hist[step] = dt0;
+    for (int l = 0; l < a.nlev; ++l) cmax[l] = 0.0;
+}
+
+hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, int limiter, double* hist, int step,
+                         int* err, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(64), 0, s, a, cmax, dt, cfl, limiter, hist, step, err);
     return hipGetLastError();
 }
 
@@ -1214,6 +1259,123 @@ void k_fill2(FillDev fd, FillDev cd, int box0, int ratio, double* Uf, const doub
     }
 }
 
+// The coarse-fine face correction of one RK stage of a hierarchy (StageRHSAllLevels,
+// source/Class_Compressible_PhaseField_StageCoupling.cpp:84-153, with run.do_reflux = 1): a coarse
+// face over the finer level takes the mean of the fine faces over it (amrex::average_down_faces:
+// the lower transverse index fastest, times 1/ratio^(dim-1)), fluxes and u*_n alike, before the
+// divergence. The stage kernel has already written the coarse cell with its own faces; one thread
+// per listed coarse cell adds w dt times the change: -(F_mean - F)/dx for a high face, +(F_mean -
+// F)/dx for a low face, and the nc coefficient of the cell times the change of div u*. Uc and Uf
+// are the stage states with every ghost cell filled; the fine faces are computed from the fine box
+// that holds the covered side.
+template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+__global__ __launch_bounds__(NT)
+void k_cf_correct(KParams kp, int limiter, const CfCell* __restrict__ list, int n, const BoxDev* __restrict__ cboxes,
+                  const double* __restrict__ Uc, FillDev cfd, FillDev ffd, const double* __restrict__ Uf,
+                  int ratio, double* Uout, const double* __restrict__ dtp, int stage, int* err)
+{
+    constexpr int N = 4 + DIM;
+    constexpr int IA = 3 + DIM;
+    const int e = (int)(blockIdx.x * NT + threadIdx.x);
+    if (e >= n) return;
+    const CfCell cc = list[e];
+    const BoxDev cb = cboxes[cc.box];
+    int bad = 0;
+    double dL[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) dL[q] = 0.0;
+    double ddiv = 0.0;
+    auto face_dir = [&](int d, const BoxDev& bx, const double* Ub, int i, int j, int k, double* F) -> double {
+        if (d == 0) return face_at<DIM, SCHEME, RIEMANN, THINC, 0>(kp, limiter, bx, Ub, i, j, k, F, bad);
+        if (d == 1) return face_at<DIM, SCHEME, RIEMANN, THINC, 1>(kp, limiter, bx, Ub, i, j, k, F, bad);
+        return face_at<DIM, SCHEME, RIEMANN, THINC, (DIM == 3 ? 2 : 1)>(kp, limiter, bx, Ub, i, j, k, F, bad);
+    };
+    for (int d = 0; d < DIM; ++d)
+        for (int side = 0; side < 2; ++side) {
+            if (!((cc.mask >> (2 * d + side)) & 1)) continue;
+            int f[3] = {cc.c[0], cc.c[1], cc.c[2]};
+            f[d] += side;
+            double Fc[N];
+            const double uc = face_dir(d, cb, Uc + cb.off, f[0], f[1], f[2], Fc);
+            // the covered coarse cell across the face (periodic image inside the domain)
+            int X[3] = {cc.c[0], cc.c[1], cc.c[2]};
+            X[d] += side ? 1 : -1;
+            const int ncd = cfd.dhi[d] - cfd.dlo[d] + 1;
+            if (X[d] < cfd.dlo[d]) X[d] += ncd;
+            else if (X[d] > cfd.dhi[d]) X[d] -= ncd;
+            int g0[3] = {0, 0, 0};
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) g0[a] = X[a] * ratio;
+            int cov[3] = {g0[0], g0[1], g0[2]};
+            const int fpos = side ? X[d] * ratio : (X[d] + 1) * ratio;
+            cov[d] = side ? fpos : fpos - 1;
+            const int own = cell_owner<DIM>(ffd, cov);
+            if (own < 0) { bad = CMP_ERR_LAYOUT; continue; }
+            const BoxDev fb = ffd.boxes[own];
+            int t1 = -1, t2 = -1;
+            for (int a = 0; a < DIM; ++a)
+                if (a != d) { if (t1 < 0) t1 = a; else t2 = a; }
+            const int r1 = (t1 >= 0) ? ratio : 1, r2 = (t2 >= 0) ? ratio : 1;
+            double sF[N], su = 0.0;
+#pragma unroll
+            for (int q = 0; q < N; ++q) sF[q] = 0.0;
+            for (int o2 = 0; o2 < r2; ++o2)
+                for (int o1 = 0; o1 < r1; ++o1) {
+                    int g[3] = {g0[0], g0[1], g0[2]};
+                    g[d] = fpos;
+                    if (t1 >= 0) g[t1] += o1;
+                    if (t2 >= 0) g[t2] += o2;
+                    double Ff[N];
+                    const double uf = face_dir(d, fb, Uf + fb.off, g[0], g[1], g[2], Ff);
+#pragma unroll
+                    for (int q = 0; q < N; ++q) sF[q] += Ff[q];
+                    su += uf;
+                }
+            const double facInv = 1.0 / (double)(r1 * r2);
+            const double sgn = side ? -1.0 : 1.0;
+            const double idx = kp.inv_dx[d];
+#pragma unroll
+            for (int q = 0; q < N; ++q) dL[q] += sgn * ((sF[q] * facInv - Fc[q]) * idx);
+            ddiv -= sgn * ((su * facInv - uc) * idx);
+        }
+    const long long c = cb.off + fab_index(cb, cc.c[0], cc.c[1], cc.c[2]);
+    double Ucell[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) Ucell[q] = Uc[c + q * cb.sn];
+    dL[IA] += nc_coefficient<DIM>(kp, Ucell, bad) * ddiv;
+    const double dt = *dtp;
+    const double w = (stage == 0) ? dt : ((stage == 1) ? 0.25 * dt : (2.0 / 3.0) * dt);
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const double v = Uout[c + q * cb.sn] + w * dL[q];
+        if (__builtin_isnan(v)) bad = CMP_ERR_NAN;
+        Uout[c + q * cb.sn] = v;
+    }
+    if (bad) atomicMax(err, bad);
+}
+
+template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+static hipError_t cfcorr4(const KParams& kp, int limiter, const CfCell* list, int n, const BoxDev* cboxes,
+                          const double* Uc, const FillDev& cfd, const FillDev& ffd, const double* Uf, int ratio,
+                          double* Uout, const double* dt, int stage, int* err, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cf_correct<DIM, SCHEME, RIEMANN, THINC>), dim3((n + NT - 1) / NT), dim3(NT), 0, s, kp,
+                       limiter, list, n, cboxes, Uc, cfd, ffd, Uf, ratio, Uout, dt, stage, err);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_cf_correct_dim(const LaunchCfg& lc, const KParams& kp, const CfCell* list, int n,
+                                 const BoxDev* cboxes, const double* Uc, const FillDev& cfd, const FillDev& ffd,
+                                 const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
+                                 hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (lc.nphase || lc.riemann == RIEMANN_PF_MIX || lc.riemann == RIEMANN_PF_PHASIC) return hipErrorInvalidValue;
+    CMP_DISPATCH(cfcorr4, kp, lc.limiter, list, n, cboxes, Uc, cfd, ffd, Uf, ratio, Uout, dt, stage, err, s);
+    return hipErrorInvalidValue;
+}
+
 template <int DIM>
 hipError_t launch_fill2_dim(int N, const FillDev& fd, const FillDev& cd, int ratio, double* Uf, const double* Uc,
                             int* err, hipStream_t s)
@@ -1299,6 +1461,10 @@ void tile_dims_dim(int* tx, int* ty, int* tz)
 }
 
 template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
+template hipError_t launch_cf_correct_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const CfCell*, int,
+                                                         const BoxDev*, const double*, const FillDev&,
+                                                         const FillDev&, const double*, int, double*,
+                                                         const double*, int, int*, hipStream_t);
 template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev&, int, double*,
                                                      const double*, int*, hipStream_t);
 template hipError_t launch_avgdown_dim<CMP_KERNEL_DIM>(const BoxDev*, int, long long, const FillDev&, const double*,

@@ -115,6 +115,34 @@ struct TagCriteria {
     cmp_tag_criterion c[CMP_TAG_MAX_CRITERIA];
     int n;
 };
+// A coarse cell next to a finer level: box, cell, and bit 2 d + side set where its face of
+// direction d (side 0 low, 1 high) borders a coarse cell the finer level covers.
+struct CfCell {
+    int box;
+    int c[3];
+    int mask;
+};
+
+// Up to this many levels in a hierarchy.
+constexpr int AMR_MAX_LEVELS = 8;
+
+struct AmrDt {
+    int nlev;
+    int dim;
+    double dx[AMR_MAX_LEVELS][3];
+};
+
+// The coarse-fine face correction of one stage (cmp_amr_rk3_step): every listed coarse cell's
+// stage output gains w dt times the change of its right-hand side when its faces over the finer
+// level take the finer level's face means.
+hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, const CfCell* list, int n,
+                             const BoxDev* cboxes, const double* Uc, const FillDev& cfd, const FillDev& ffd,
+                             const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
+                             hipStream_t s);
+// ComputeDt over the levels of a hierarchy (no subcycling); resets every level's c_max.
+hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, int limiter, double* hist, int step,
+                         int* err, hipStream_t s);
+
 hipError_t launch_tag(const LaunchCfg& lc, const KParams& kp, const BoxDev* boxes, int nbox, long long maxcells,
                       const long long* toff, const double* U, const TagCriteria& tc, unsigned char* tags, int* err,
                       hipStream_t s);

@@ -322,6 +322,40 @@ int cmp_level_average_down(const cmp_level* fine, const double* Uf, cmp_level* c
 int cmp_level_fill_two_levels(cmp_level* fine, double* Uf, const cmp_level* coarse, const double* Uc, int ratio,
                               void* stream);
 
+/* ---- a hierarchy of levels stepped together -------------------------------------------------
+ * The reference's time step without subcycling (run.do_subcycle = 0, run.do_reflux = 1):
+ * timeStepNoSubcycling (source/COMPAS.cpp:2139-2180) -> AdvanceAllLevelsStageCoupled
+ * (source/Class_Compressible_PhaseField_StageCoupling.cpp:156-) and StageRHSAllLevels (:84-153):
+ * every RK stage fills every level's ghost cells from the stage state (level 0 same-level, finer
+ * levels two-level as cmp_level_fill_two_levels), computes every level's faces, gives the coarse
+ * faces under a finer level the mean of the fine faces over them (fluxes and u*_n, finest level
+ * first, AverageDownFacesTo :1575-1583), takes the divergence and the stage combination, and
+ * averages the levels down (AverageDown). ComputeDt over the levels (:2250-2302, nsubsteps 1): the
+ * smallest of the levels' estimates, each limited to 1.1 dt_prev. Two-phase flow states; levels
+ * coarsest first, with the same parameters and ghost width, each the refinement of the previous one
+ * by ratio, boxes aligned to ratio; a third level must not touch the boundary of the second. The
+ * caller's level buffers stay the caller's; every call takes arrays of nlev device pointers. */
+typedef struct cmp_amr cmp_amr;
+
+int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out);
+int cmp_amr_destroy(cmp_amr* amr);
+/* Coarse cells of `level` whose faces border the next finer level (0 for the finest). */
+int cmp_amr_info(const cmp_amr* amr, int level, int* ncf_cells);
+/* CalculateInitialcmax of every level on the filled hierarchy: cmax (device, nlev doubles). Writes
+ * the ghost cells of U. */
+int cmp_amr_initial_cmax(cmp_amr* amr, double* const* U, double* cmax, void* stream);
+/* *dt_dev (device) = ComputeDt over the levels from cmax (device, nlev doubles; reset to 0 after),
+ * with dt_prev = *dt_dev (0: a first step, no limiter). */
+int cmp_amr_compute_dt(cmp_amr* amr, double* cmax, double* dt_dev, double cfl, int limiter, void* stream);
+/* One RK3 step of size *dt_dev of every level, in place on Un[l] (U1[l], U2[l] scratch level
+ * buffers); the levels' c_max accumulate into cmax[l]. */
+int cmp_amr_rk3_step(cmp_amr* amr, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
+                     double* cmax, void* stream);
+/* nsteps steps: initial c_max, then per step ComputeDt and the RK3 step; dt_hist (device, nsteps
+ * doubles, or NULL) receives every step's dt. */
+int cmp_amr_advance(cmp_amr* amr, double* const* Un, double* const* U1, double* const* U2, int nsteps, double cfl,
+                    int limiter, double* dt_dev, double* cmax, double* dt_hist, void* stream);
+
 /* ---- a batch of independent levels (the served-stream form) ---------------------------------
  * No counterpart in the reference, which runs one case per process (source/main.cpp:21-86): cases of
  * different grids and parameters are admitted while others run, keep their state in device memory,

@@ -96,6 +96,11 @@ def load():
                                                   ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_fill_two_levels": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, PG, ctypes.c_int,
                                                      PB, ctypes.c_int, PD, ctypes.c_int]),
+        "ref_amr_rk3_step": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
+                                            ctypes.POINTER(PD), D, ctypes.c_int, PD]),
+        "ref_amr_compute_dt": (D, [ctypes.c_int, PG, PD, D, ctypes.c_int, D]),
+        "ref_amr_advance": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
+                                           ctypes.POINTER(PD), ctypes.c_int, D, ctypes.c_int, ctypes.c_int, PD]),
         "ref_level_tag": (ctypes.c_int, [PP, ctypes.c_int, PB, ctypes.c_int, PD,
                                          ctypes.POINTER(cmp_tag_criterion), ctypes.c_int,
                                          ctypes.POINTER(ctypes.c_ubyte)]),
@@ -566,6 +571,49 @@ def level_advance_pf(params, pf, geom, layout, buf, nsteps, cfl, limiter=1):
                                      float(cfl), int(limiter), _dptr(hist))
     if rc:
         raise RuntimeError("ref_level_advance_pf failed: %d" % rc)
+    return hist[:nsteps]
+
+
+def _amr_args(geoms, layouts, bufs):
+    from compas_amd.capi import cmp_box, cmp_geometry
+    n = len(layouts)
+    if len({l.ng for l in layouts}) != 1:
+        raise ValueError("the levels of a hierarchy share their ghost width")
+    G = (cmp_geometry * n)(*geoms)
+    nb = (ctypes.c_int * n)(*[l.nbox for l in layouts])
+    B = (ctypes.POINTER(cmp_box) * n)(*[ctypes.cast(l.cboxes, ctypes.POINTER(cmp_box)) for l in layouts])
+    U = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(b) for b in bufs])
+    return n, G, nb, B, U
+
+
+def amr_rk3_step(params, geoms, layouts, bufs, dt, ratio):
+    """One stage-coupled RK3 step of all levels (coarsest first), in place on bufs; returns the
+    levels' c_max."""
+    n, G, nb, B, U = _amr_args(geoms, layouts, bufs)
+    cmax = np.zeros(n)
+    rc = load().ref_amr_rk3_step(ctypes.byref(params), n, G, nb, B, layouts[0].ng, U, float(dt), int(ratio),
+                                 _dptr(cmax))
+    if rc:
+        raise RuntimeError("ref_amr_rk3_step failed: %d" % rc)
+    return cmax
+
+
+def amr_compute_dt(geoms, cmax, cfl, limiter, dt_prev):
+    from compas_amd.capi import cmp_geometry
+    n = len(geoms)
+    G = (cmp_geometry * n)(*geoms)
+    c = np.ascontiguousarray(cmax, dtype=np.float64)
+    return load().ref_amr_compute_dt(n, G, _dptr(c), float(cfl), int(limiter), float(dt_prev))
+
+
+def amr_advance(params, geoms, layouts, bufs, nsteps, cfl, ratio, limiter=1):
+    """nsteps steps of the hierarchy in place on bufs; returns the dt of every step."""
+    n, G, nb, B, U = _amr_args(geoms, layouts, bufs)
+    hist = np.zeros(max(1, nsteps))
+    rc = load().ref_amr_advance(ctypes.byref(params), n, G, nb, B, layouts[0].ng, U, int(nsteps), float(cfl),
+                                int(limiter), int(ratio), _dptr(hist))
+    if rc:
+        raise RuntimeError("ref_amr_advance failed: %d" % rc)
     return hist[:nsteps]
 
 

@@ -918,15 +918,14 @@ int ref_face_flux(const cmp_params* P, const cmp_geometry* G, const cmp_box* box
  * FVM_SurfaceIntegral_NC (include/Operators_Compressible5Eq/FVM_SurfaceIntegralNC_Compressible5Eq.H:15-52).
  * ---------------------------------------------------------------------------------------------- */
 
-int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
-                const double* U, double* dUdt, double* cmax, double** flux_out, double** fluxnc_out)
+/* The faces of one box (ComputeFaceFluxes): F[d] (nface x N, x fastest) and FN[d] (u*_n), allocated
+ * here; *cm the c_max of its valid cells. */
+static int box_faces(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
+                     const double* U, double** F, double** FN, double* cm_out)
 {
     const int dim = P->dim, N = P->nstate;
     int nv[3] = {1, 1, 1};
     for (int d = 0; d < dim; ++d) nv[d] = box->hi[d] - box->lo[d] + 1;
-    long long ncell = (long long)nv[0] * nv[1] * nv[2];
-    double* F[3] = {NULL, NULL, NULL};
-    double* FN[3] = {NULL, NULL, NULL};
     int err = 0;
     double cm = 0.0;
     for (int d = 0; d < dim; ++d) {
@@ -959,9 +958,19 @@ int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, 
                 }
         if (e && !err) err = e;
         if (cm < cmd) cm = cmd;
-        if (flux_out && flux_out[d]) memcpy(flux_out[d], F[d], sizeof(double) * nface * N);
-        if (fluxnc_out && fluxnc_out[d]) memcpy(fluxnc_out[d], FN[d], sizeof(double) * nface);
     }
+    *cm_out = cm;
+    return err;
+}
+
+/* FluxDivergence of one box from its faces F, FN: dUdt (valid cells, x fastest, component-major). */
+static int box_divergence(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
+                          const double* U, double* const* F, double* const* FN, double* dUdt)
+{
+    const int dim = P->dim, N = P->nstate;
+    int nv[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) nv[d] = box->hi[d] - box->lo[d] + 1;
+    long long ncell = (long long)nv[0] * nv[1] * nv[2];
     /* divergence, then the non-conservative term, per valid cell */
     fab_t f = make_fab(dim, box, ng);
     long long sf[3][3];   /* face strides per direction */
@@ -1037,6 +1046,25 @@ int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, 
                 dUdt[cell + iA * ncell] += phi1 * (1.0 + K[0]) * divu;
                 if (le > e2) e2 = le;
             }
+    return e2;
+}
+
+int ref_rhs_box(const cmp_params* P, const cmp_geometry* G, const cmp_box* box, int ng,
+                const double* U, double* dUdt, double* cmax, double** flux_out, double** fluxnc_out)
+{
+    const int dim = P->dim, N = P->nstate;
+    double* F[3] = {NULL, NULL, NULL};
+    double* FN[3] = {NULL, NULL, NULL};
+    double cm = 0.0;
+    int err = box_faces(P, G, box, ng, U, F, FN, &cm);
+    for (int d = 0; d < dim; ++d) {
+        int fe[3] = {1, 1, 1};
+        for (int e = 0; e < dim; ++e) fe[e] = box->hi[e] - box->lo[e] + 1 + (e == d ? 1 : 0);
+        const long long nface = (long long)fe[0] * fe[1] * fe[2];
+        if (flux_out && flux_out[d]) memcpy(flux_out[d], F[d], sizeof(double) * nface * N);
+        if (fluxnc_out && fluxnc_out[d]) memcpy(fluxnc_out[d], FN[d], sizeof(double) * nface);
+    }
+    int e2 = box_divergence(P, G, box, ng, U, F, FN, dUdt);
     if (e2 && !err) err = e2;
     for (int d = 0; d < dim; ++d) { free(F[d]); free(FN[d]); }
     if (cmax && (*cmax < cm || isnan(cm))) *cmax = cm;
@@ -2711,6 +2739,258 @@ int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int n
                     }
                     for (int n = 0; n < N; ++n) base[di + n * f.sn] = sgn[n] * v[n];
                 }
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * All levels of a hierarchy, stage by stage, without subcycling: AdvanceAllLevelsStageCoupled
+ * (source/Class_Compressible_PhaseField_StageCoupling.cpp:156-) with run.do_reflux = 1 and
+ * StageRHSAllLevels (:84-153), then AverageDown (timeStepNoSubcycling, source/COMPAS.cpp:2139-2180).
+ * Per stage: every level's stage state takes its ghost cells (level 0 the same-level fill, finer
+ * levels the two-level fill from the next coarser stage state); every box computes its faces; from
+ * the finest level down, the coarse faces under a finer level take the mean of the fine faces over
+ * them (amrex::average_down_faces with periodicity: the face's value goes to every coarse box whose
+ * face box holds the face or a periodic image of it; the fine faces summed with the lower transverse
+ * index fastest, times 1/ratio^(dim-1)), fluxes and u*_n alike; every level then takes its
+ * divergence and the stage combination of ThirdOrderSSPRK; the levels are averaged down (finest
+ * first). ComputeDt over levels (:2250-2302, nsubsteps 1): dt = min over levels of the level's
+ * estimate, each limited to 1.1 dt_prev.
+ * ---------------------------------------------------------------------------------------------- */
+static long long level_total(int dim, int nbox, const cmp_box* boxes, int ng, int N)
+{
+    long long t = 0;
+    for (int b = 0; b < nbox; ++b) t += cmp_fab_size(dim, &boxes[b], ng, N);
+    return t;
+}
+
+static void face_extents(int dim, const cmp_box* b, int d, int* fe)
+{
+    for (int e = 0; e < 3; ++e) fe[e] = 1;
+    for (int e = 0; e < dim; ++e) fe[e] = b->hi[e] - b->lo[e] + 1 + (e == d ? 1 : 0);
+}
+
+/* faces F[lev][b*3+d], FN[lev][b*3+d] */
+static void average_faces(const cmp_params* P, const cmp_geometry* Gc, int nbf, const cmp_box* fb, double** Ff,
+                          double** FNf, int nbc, const cmp_box* cb, double** Fc, double** FNc, int ratio)
+{
+    const int dim = P->dim, N = P->nstate;
+    int rt = 1;
+    for (int e = 1; e < dim; ++e) rt *= ratio;
+    const double facInv = 1.0 / (double)rt;
+    for (int b = 0; b < nbf; ++b)
+        for (int d = 0; d < dim; ++d) {
+            int ffe[3];
+            face_extents(dim, &fb[b], d, ffe);
+            const long long nff = (long long)ffe[0] * ffe[1] * ffe[2];
+            int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+            for (int e = 0; e < dim; ++e) {
+                clo[e] = fb[b].lo[e] / ratio;
+                chi[e] = (fb[b].hi[e] + 1) / ratio - 1 + (e == d ? 1 : 0);
+            }
+            /* transverse directions: t1 the lower (fastest), t2 the higher */
+            int t1 = -1, t2 = -1;
+            for (int e = 0; e < dim; ++e)
+                if (e != d) { if (t1 < 0) t1 = e; else t2 = e; }
+            const int r1 = (t1 >= 0) ? ratio : 1, r2 = (t2 >= 0) ? ratio : 1;
+            for (int K = clo[2]; K <= chi[2]; ++K)
+                for (int J = clo[1]; J <= chi[1]; ++J)
+                    for (int I = clo[0]; I <= chi[0]; ++I) {
+                        double avg[REF_NMAX], avgn = 0.0;
+                        const int C[3] = {I, J, K};
+                        for (int n = 0; n <= N; ++n) {
+                            double c = 0.0;
+                            for (int o2 = 0; o2 < r2; ++o2)
+                                for (int o1 = 0; o1 < r1; ++o1) {
+                                    int fcell[3] = {0, 0, 0};
+                                    for (int e = 0; e < dim; ++e) fcell[e] = C[e] * ratio;
+                                    if (t1 >= 0) fcell[t1] += o1;
+                                    if (t2 >= 0) fcell[t2] += o2;
+                                    long long q = 0;
+                                    for (int e = dim - 1; e >= 0; --e) q = q * ffe[e] + (fcell[e] - fb[b].lo[e]);
+                                    c += (n < N) ? Ff[b * 3 + d][q + n * nff] : FNf[b * 3 + d][q];
+                                }
+                            if (n < N) avg[n] = c * facInv; else avgn = c * facInv;
+                        }
+                        /* the face and its periodic images along d */
+                        int nimg = 1;
+                        int pos[3] = {C[d], C[d], C[d]};
+                        const int ncd = Gc->domain_hi[d] - Gc->domain_lo[d] + 1;
+                        if (Gc->is_periodic[d]) {
+                            if (C[d] == Gc->domain_lo[d]) pos[nimg++] = C[d] + ncd;
+                            else if (C[d] == Gc->domain_hi[d] + 1) pos[nimg++] = C[d] - ncd;
+                        }
+                        for (int m = 0; m < nimg; ++m) {
+                            int F3[3] = {C[0], C[1], C[2]};
+                            F3[d] = pos[m];
+                            for (int q = 0; q < nbc; ++q) {
+                                int cfe[3];
+                                face_extents(dim, &cb[q], d, cfe);
+                                int in = 1;
+                                for (int e = 0; e < dim; ++e)
+                                    if (F3[e] < cb[q].lo[e] || F3[e] > cb[q].lo[e] + cfe[e] - 1) in = 0;
+                                if (!in) continue;
+                                const long long ncf = (long long)cfe[0] * cfe[1] * cfe[2];
+                                long long qq = 0;
+                                for (int e = dim - 1; e >= 0; --e) qq = qq * cfe[e] + (F3[e] - cb[q].lo[e]);
+                                for (int n = 0; n < N; ++n) Fc[q * 3 + d][qq + n * ncf] = avg[n];
+                                FNc[q * 3 + d][qq] = avgn;
+                            }
+                        }
+                    }
+        }
+}
+
+static int amr_fill(const cmp_params* P, int nlev, const cmp_geometry* G, const int* nbox,
+                    const cmp_box* const* boxes, int ng, double** Ub, int ratio)
+{
+    int err = ref_level_fill_ghosts(P, &G[0], nbox[0], boxes[0], ng, Ub[0]);
+    for (int l = 1; l < nlev; ++l) {
+        int e = ref_level_fill_two_levels(P, &G[l], nbox[l], boxes[l], ng, Ub[l], &G[l - 1], nbox[l - 1],
+                                          boxes[l - 1], ng, Ub[l - 1], ratio);
+        if (e && !err) err = e;
+    }
+    return err;
+}
+
+int ref_amr_rk3_step(const cmp_params* P, int nlev, const cmp_geometry* G, const int* nbox,
+                     const cmp_box* const* boxes, int ng, double* const* Un, double dt, int ratio, double* cmax)
+{
+    const int dim = P->dim, N = P->nstate;
+    if (nlev < 1 || nlev > 8) return CMP_ERR_ARG;
+    double* Ub[8];
+    double* Unew[8];
+    double** F[8];
+    double** FN[8];
+    long long tot[8];
+    for (int l = 0; l < nlev; ++l) {
+        tot[l] = level_total(dim, nbox[l], boxes[l], ng, N);
+        Ub[l] = (double*)malloc(sizeof(double) * tot[l]);
+        Unew[l] = (double*)malloc(sizeof(double) * tot[l]);
+        F[l] = (double**)calloc((size_t)nbox[l] * 3, sizeof(double*));
+        FN[l] = (double**)calloc((size_t)nbox[l] * 3, sizeof(double*));
+    }
+    int err = 0, e;
+    const double w2 = 0.25 * dt, w3 = (2.0 / 3.0) * dt;
+    for (int stage = 0; stage < 3; ++stage) {
+        for (int l = 0; l < nlev; ++l) memcpy(Ub[l], stage == 0 ? Un[l] : Unew[l], sizeof(double) * tot[l]);
+        e = amr_fill(P, nlev, G, nbox, boxes, ng, Ub, ratio);
+        if (e && !err) err = e;
+        for (int l = 0; l < nlev; ++l)
+            for (int b = 0; b < nbox[l]; ++b) {
+                double cm = 0.0;
+                e = box_faces(P, &G[l], &boxes[l][b], ng, Ub[l] + boxes[l][b].offset, &F[l][b * 3], &FN[l][b * 3], &cm);
+                if (e && !err) err = e;
+                if (cmax[l] < cm || isnan(cm)) cmax[l] = cm;
+            }
+        for (int l = nlev - 2; l >= 0; --l)
+            average_faces(P, &G[l], nbox[l + 1], boxes[l + 1], F[l + 1], FN[l + 1], nbox[l], boxes[l], F[l], FN[l],
+                          ratio);
+        for (int l = 0; l < nlev; ++l)
+            for (int b = 0; b < nbox[l]; ++b) {
+                const cmp_box* bx = &boxes[l][b];
+                const long long nc = valid_cells(dim, bx);
+                double* L = (double*)malloc(sizeof(double) * nc * N);
+                e = box_divergence(P, &G[l], bx, ng, Ub[l] + bx->offset, F[l] + b * 3, FN[l] + b * 3, L);
+                if (e && !err) err = e;
+                fab_t f = make_fab(dim, bx, ng);
+                int nv[3] = {1, 1, 1};
+                for (int d = 0; d < dim; ++d) nv[d] = bx->hi[d] - bx->lo[d] + 1;
+                const int lo2 = (dim == 3) ? bx->lo[2] : 0;
+                for (int n = 0; n < N; ++n)
+                    for (int kk = 0; kk < nv[2]; ++kk)
+                        for (int jj = 0; jj < nv[1]; ++jj)
+                            for (int ii = 0; ii < nv[0]; ++ii) {
+                                const long long q = bx->offset + fab_idx(&f, bx->lo[0] + ii, bx->lo[1] + jj, lo2 + kk) + n * f.sn;
+                                const double Lq = L[ii + (long long)nv[0] * (jj + (long long)nv[1] * kk) + n * nc];
+                                if (stage == 0) {
+                                    Unew[l][q] = 1.0 * Ub[l][q] + dt * Lq;
+                                } else if (stage == 1) {
+                                    Unew[l][q] = 0.25 * Ub[l][q] + 0.75 * Un[l][q];
+                                    Unew[l][q] += w2 * Lq;
+                                } else {
+                                    Unew[l][q] = (2.0 / 3.0) * Ub[l][q] + (1.0 / 3.0) * Un[l][q];
+                                    Unew[l][q] += w3 * Lq;
+                                }
+                            }
+                free(L);
+            }
+        for (int l = 0; l < nlev; ++l)
+            for (int q = 0; q < nbox[l] * 3; ++q) { free(F[l][q]); free(FN[l][q]); F[l][q] = FN[l][q] = NULL; }
+        for (int l = nlev - 2; l >= 0; --l) {
+            e = ref_level_average_down(P, nbox[l + 1], boxes[l + 1], ng, Unew[l + 1], nbox[l], boxes[l], ng, Unew[l],
+                                       ratio);
+            if (e && !err) err = e;
+        }
+    }
+    for (int l = 0; l < nlev; ++l) {
+        for (int b = 0; b < nbox[l]; ++b) {
+            const cmp_box* bx = &boxes[l][b];
+            fab_t f = make_fab(dim, bx, ng);
+            int nv[3] = {1, 1, 1};
+            for (int d = 0; d < dim; ++d) nv[d] = bx->hi[d] - bx->lo[d] + 1;
+            const int lo2 = (dim == 3) ? bx->lo[2] : 0;
+            for (int n = 0; n < N; ++n)
+                for (int kk = 0; kk < nv[2]; ++kk)
+                    for (int jj = 0; jj < nv[1]; ++jj)
+                        for (int ii = 0; ii < nv[0]; ++ii) {
+                            const long long q = bx->offset + fab_idx(&f, bx->lo[0] + ii, bx->lo[1] + jj, lo2 + kk) + n * f.sn;
+                            Un[l][q] = Unew[l][q];
+                        }
+        }
+        free(Ub[l]); free(Unew[l]); free(F[l]); free(FN[l]);
+    }
+    return err;
+}
+
+double ref_amr_compute_dt(int nlev, const cmp_geometry* G, const double* cmax, double cfl, int limiter,
+                          double dt_prev)
+{
+    double dt0 = 0.0;
+    for (int l = 0; l < nlev; ++l) {
+        double t = ref_est_dt(&G[l], cmax[l], cfl);
+        if (limiter && dt_prev > 0.0) t = smin(t, 1.1 * dt_prev);
+        dt0 = (l == 0) ? t : smin(dt0, t);
+    }
+    return dt0;
+}
+
+/* The initial c_max of every level (CalculateInitialcmax on the filled hierarchy), then nsteps
+ * steps: ComputeDt over levels, one stage-coupled RK3 step. */
+int ref_amr_advance(const cmp_params* P, int nlev, const cmp_geometry* G, const int* nbox,
+                    const cmp_box* const* boxes, int ng, double* const* U, int nsteps, double cfl, int limiter,
+                    int ratio, double* dt_hist)
+{
+    const int dim = P->dim, N = P->nstate;
+    double cmax[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double* Ub[8];
+    int err = 0;
+    if (nlev < 1 || nlev > 8) return CMP_ERR_ARG;
+    for (int l = 0; l < nlev; ++l) {
+        const long long t = level_total(dim, nbox[l], boxes[l], ng, N);
+        Ub[l] = (double*)malloc(sizeof(double) * t);
+        memcpy(Ub[l], U[l], sizeof(double) * t);
+    }
+    err = amr_fill(P, nlev, G, nbox, boxes, ng, Ub, ratio);
+    for (int l = 0; l < nlev && !err; ++l)
+        for (int b = 0; b < nbox[l]; ++b) {
+            double* Fx[3] = {NULL, NULL, NULL};
+            double* FNx[3] = {NULL, NULL, NULL};
+            double cm = 0.0;
+            int e = box_faces(P, &G[l], &boxes[l][b], ng, Ub[l] + boxes[l][b].offset, Fx, FNx, &cm);
+            if (e && !err) err = e;
+            if (cmax[l] < cm || isnan(cm)) cmax[l] = cm;
+            for (int d = 0; d < 3; ++d) { free(Fx[d]); free(FNx[d]); }
+        }
+    for (int l = 0; l < nlev; ++l) free(Ub[l]);
+    double dt = 0.0;
+    for (int s = 0; s < nsteps && !err; ++s) {
+        dt = ref_amr_compute_dt(nlev, G, cmax, cfl, limiter, s == 0 ? 0.0 : dt);
+        if (dt_hist) dt_hist[s] = dt;
+        for (int l = 0; l < nlev; ++l) cmax[l] = 0.0;
+        err = ref_amr_rk3_step(P, nlev, G, nbox, boxes, ng, U, dt, ratio, cmax);
+        for (int l = 0; l < nlev; ++l)
+            if (isnan(cmax[l])) err = CMP_ERR_NAN;
     }
     return err;
 }

@@ -159,6 +159,15 @@ int ref_level_average_down(const cmp_params* P, int nbf, const cmp_box* fboxes, 
 int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
                               double* Uf, const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc,
                               const double* Uc, int ratio);
+/* All levels of a hierarchy, stage by stage without subcycling (stage-coupled, do_reflux = 1),
+ * then averaged down; levels coarsest first, ratio between consecutive levels, ng for all. */
+int ref_amr_rk3_step(const cmp_params* P, int nlev, const cmp_geometry* G, const int* nbox,
+                     const cmp_box* const* boxes, int ng, double* const* Un, double dt, int ratio, double* cmax);
+double ref_amr_compute_dt(int nlev, const cmp_geometry* G, const double* cmax, double cfl, int limiter,
+                          double dt_prev);
+int ref_amr_advance(const cmp_params* P, int nlev, const cmp_geometry* G, const int* nbox,
+                    const cmp_box* const* boxes, int ng, double* const* U, int nsteps, double cfl, int limiter,
+                    int ratio, double* dt_hist);
 
 /* number of OpenMP threads the level functions use (1 when built without OpenMP) */
 int ref_num_threads(void);

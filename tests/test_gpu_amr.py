@@ -79,3 +79,127 @@ def test_fill_two_levels_matches_oracle(gpu, dim, ratio, nphase):
     lc.close()
     assert np.array_equal(got, ref)
     assert (got != fbuf).sum() > 0
+
+
+# ---- the stage-coupled step of a hierarchy (cmp_amr_*) ---------------------------------------------
+
+def _hierarchy(dim, nc, fine_boxes, seed=3, third=None, **kw):
+    from compas_amd import cases
+    h = cases.two_level_smooth(dim, nc, seed, fine_boxes, **kw)
+    P = h["params"]
+    geoms, boxes = list(h["geoms"]), list(h["boxes"])
+    fields = list(h["fields"])
+    if third is not None:
+        f3 = cases.random_smooth(dim, nc * 4, seed, **kw)
+        geoms.append(f3["geom"])
+        boxes.append(third)
+        fields.append(f3["U"])
+    lays = [Layout(dim, b, 3, P.nstate) for b in boxes]
+    bufs = [lay.pack(U) for lay, U in zip(lays, fields)]
+    # the coarser levels under a finer one hold its means, as after a regrid
+    for l in range(len(lays) - 2, -1, -1):
+        oracle.level_average_down(P, lays[l + 1], bufs[l + 1], lays[l], bufs[l], 2)
+    return P, geoms, lays, bufs
+
+
+def _valid_cells(lay, buf):
+    g = lay.ng
+    out = []
+    for b in range(lay.nbox):
+        v = lay.fab_view(buf, b)
+        out.append(v[(slice(None),) + (slice(g, -g),) * lay.dim].reshape(v.shape[0], -1))
+    return np.concatenate(out, axis=1)
+
+
+def _maxrel(lays, got, ref):
+    out = []
+    for lay, a, b in zip(lays, got, ref):
+        x, y = _valid_cells(lay, a), _valid_cells(lay, b)
+        out.append((np.abs(x - y).max(axis=1) / np.abs(y).max(axis=1)).max())
+    return np.array(out)
+
+
+def _assert_close(lays, got, ref, tol, run_oracle, bufs):
+    """Per level, the largest per-variable max difference relative to the variable's largest
+    magnitude is at most tol, or within 4x what the oracle itself moves when its input takes a
+    relative perturbation of 1e-15. (The coarse-fine interpolation limits all components of a cell
+    with one factor, the smallest over the components of sx/dc; at an extremum of a nearly flat
+    component dc is a few ulps, its sign decides whether that factor is 0 or 1, and rounding-level
+    differences of the stage state then move the fine ghost cells, and the step, by more than
+    rounding.)"""
+    err = _maxrel(lays, got, ref)
+    if err.max() <= tol:
+        return
+    rng = np.random.default_rng(1)
+    pert = [b * (1.0 + 1e-15 * rng.standard_normal(b.shape)) for b in bufs]
+    run_oracle(pert)
+    sens = _maxrel(lays, pert, ref)
+    assert np.all(err <= np.maximum(tol, 4.0 * sens)), (err, sens)
+
+
+AMR_CASES = {
+    "2d-centre": (2, 16, [([8, 8], [23, 23])], None),
+    "2d-two-boxes": (2, 16, [([4, 8], [15, 19]), ([16, 4], [27, 23])], None),
+    "2d-periodic-faces": (2, 16, [([0, 8], [7, 23]), ([24, 8], [31, 15])], None),
+    "2d-three-levels": (2, 16, [([8, 8], [23, 23])], [([24, 24], [39, 39])]),
+    "3d-centre": (3, 8, [([4, 4, 4], [11, 11, 11])], None),
+}
+
+
+@pytest.mark.parametrize("name", sorted(AMR_CASES))
+def test_amr_step_matches_oracle(gpu, name):
+    import torch
+    from compas_amd import Amr
+    dim, nc, fine, third = AMR_CASES[name]
+    P, geoms, lays, bufs = _hierarchy(dim, nc, fine, third=third)
+    dt = 1.0e-3
+    ref = [b.copy() for b in bufs]
+    oracle.amr_rk3_step(P, geoms, lays, ref, dt, 2)
+    levels = [Level(P, g, lay) for g, lay in zip(geoms, lays)]
+    amr = Amr(levels, 2)
+    assert amr.cf_cells(0) > 0
+    Un = [torch.from_numpy(b).to(gpu) for b in bufs]
+    U1 = [torch.zeros_like(u) for u in Un]
+    U2 = [torch.zeros_like(u) for u in Un]
+    dtd = torch.full((1,), dt, dtype=torch.float64, device=gpu)
+    cmax = torch.zeros(len(levels), dtype=torch.float64, device=gpu)
+    amr.rk3_step(Un, U1, U2, dtd, cmax)
+    assert amr.check_errors() == 0
+    got = [u.cpu().numpy() for u in Un]
+    amr.close()
+    for lv in levels:
+        lv.close()
+    _assert_close(lays, got, ref, 1e-12, lambda u: oracle.amr_rk3_step(P, geoms, lays, u, dt, 2), bufs)
+    assert np.abs(_valid_cells(lays[0], got[0]) - _valid_cells(lays[0], bufs[0])).max() > 1e-8
+
+
+@pytest.mark.parametrize("name", ["2d-two-boxes", "2d-three-levels"])
+def test_amr_advance_matches_oracle_and_conserves(gpu, name):
+    import torch
+    from compas_amd import Amr
+    dim, nc, fine, third = AMR_CASES[name]
+    P, geoms, lays, bufs = _hierarchy(dim, nc, fine, third=third)
+    nsteps = 20
+    ref = [b.copy() for b in bufs]
+    hist = oracle.amr_advance(P, geoms, lays, ref, nsteps, 0.4, 2)
+    levels = [Level(P, g, lay) for g, lay in zip(geoms, lays)]
+    amr = Amr(levels, 2)
+    Un = [torch.from_numpy(b).to(gpu) for b in bufs]
+    U1 = [torch.zeros_like(u) for u in Un]
+    U2 = [torch.zeros_like(u) for u in Un]
+    dtd = torch.zeros(1, dtype=torch.float64, device=gpu)
+    cmax = torch.zeros(len(levels), dtype=torch.float64, device=gpu)
+    dth = torch.zeros(nsteps, dtype=torch.float64, device=gpu)
+    amr.advance(Un, U1, U2, nsteps, 0.4, dtd, cmax, dth)
+    assert amr.check_errors() == 0
+    got = [u.cpu().numpy() for u in Un]
+    gh = dth.cpu().numpy()
+    amr.close()
+    for lv in levels:
+        lv.close()
+    assert np.all(np.abs(gh - hist) <= 1e-12 * hist), (gh, hist)
+    _assert_close(lays, got, ref, 1e-10, lambda u: oracle.amr_advance(P, geoms, lays, u, nsteps, 0.4, 2), bufs)
+    # composite totals (the coarse level holds the means of the finer ones) conserved
+    c0, c1 = _valid_cells(lays[0], bufs[0]), _valid_cells(lays[0], got[0])
+    for n in range(5):
+        assert abs(c1[n].sum() - c0[n].sum()) <= 1e-13 * np.abs(c0[n]).sum(), n

@@ -160,3 +160,92 @@ def test_lincc_conserves_and_stays_bounded():
             assert np.all(kids.max(axis=(1, 2)) <= nb.max(axis=0) + 1e-14)
             checked += 1
     assert checked > 20
+
+
+# ---- the stage-coupled step of a hierarchy ---------------------------------------------------------
+
+def _hier(dim, fine_boxes, nc=16, seed=3):
+    from compas_amd import cases
+    h = cases.two_level_smooth(dim, nc, seed, fine_boxes)
+    P = h["params"]
+    lays = [Layout(dim, b, 3, P.nstate) for b in h["boxes"]]
+    bufs = [lay.pack(U) for lay, U in zip(lays, h["fields"])]
+    return h, P, lays, bufs
+
+
+def test_amr_step_of_one_level_is_the_level_step():
+    h, P, lays, bufs = _hier(2, [([8, 8], [15, 15])])
+    ref = bufs[0].copy()
+    oracle.level_rk3_step(P, h["geoms"][0], lays[0], ref, 1.0e-3)
+    cm = oracle.amr_rk3_step(P, h["geoms"][:1], lays[:1], bufs[:1], 1.0e-3, 2)
+    assert np.array_equal(lays[0].unpack(bufs[0], h["shapes"][0]), lays[0].unpack(ref, h["shapes"][0]))
+    assert cm[0] > 0
+
+
+def test_amr_step_with_a_covering_fine_level():
+    """A fine level over the whole (periodic) domain takes no coarse data: it steps exactly as the
+    fine level alone, and the coarse level ends as its average."""
+    h, P, lays, bufs = _hier(2, [([0, 0], [15, 31]), ([16, 0], [31, 15]), ([16, 16], [31, 31])])
+    alone = bufs[1].copy()
+    dt = 2.0e-3
+    oracle.amr_rk3_step(P, h["geoms"], lays, bufs, dt, 2)
+    _single_step(P, h["geoms"][1], lays[1], alone, dt)
+    fine = lays[1].unpack(bufs[1], h["shapes"][1])
+    assert np.array_equal(fine, lays[1].unpack(alone, h["shapes"][1]))
+    coarse = lays[0].unpack(bufs[0], h["shapes"][0])
+    avg = fine.reshape(fine.shape[0], 16, 2, 16, 2).sum(axis=(2, 4)) * 0.25
+    assert np.allclose(coarse, avg, rtol=1e-15, atol=0)
+
+
+def _single_step(P, geom, lay, buf, dt):
+    oracle.level_rk3_step(P, geom, lay, buf, dt)
+
+
+@pytest.mark.parametrize("fine", [[([8, 8], [23, 23])], [([4, 8], [15, 19]), ([16, 4], [27, 23])],
+                                  [([0, 8], [7, 23]), ([24, 8], [31, 15])]])
+def test_amr_step_conserves_the_composite_totals(fine):
+    """Face averaging makes the coarse-fine interface conservative: mass of each phase, momentum and
+    energy summed over the coarse level (whose covered cells hold the fine means) stay put to
+    round-off on a periodic domain. The last layout has fine boxes on the domain faces, so the
+    interface crosses the periodic boundary."""
+    h, P, lays, bufs = _hier(2, fine)
+    c0 = lays[0].unpack(bufs[0], h["shapes"][0])
+    oracle.level_average_down(P, lays[1], bufs[1], lays[0], bufs[0], 2)
+    c0 = lays[0].unpack(bufs[0], h["shapes"][0])
+    tot0 = c0.reshape(c0.shape[0], -1).sum(axis=1)
+    for s in range(3):
+        oracle.amr_rk3_step(P, h["geoms"], lays, bufs, 2.0e-3, 2)
+    c1 = lays[0].unpack(bufs[0], h["shapes"][0])
+    tot1 = c1.reshape(c1.shape[0], -1).sum(axis=1)
+    scale = np.abs(c0).reshape(c0.shape[0], -1).sum(axis=1)
+    for n in (0, 1, 2, 3, 4):
+        assert abs(tot1[n] - tot0[n]) <= 1e-13 * scale[n], (n, tot1[n] - tot0[n], scale[n])
+    assert np.abs(c1 - c0).max() > 1e-6
+
+
+def test_amr_step_keeps_a_uniform_state():
+    h, P, lays, bufs = _hier(2, [([8, 8], [23, 23])])
+    for lay, buf in zip(lays, bufs):
+        for b in range(lay.nbox):
+            v = lay.fab_view(buf, b)
+            for n in range(v.shape[0]):
+                v[n] = h["fields"][0][n].mean()
+    ref = [b.copy() for b in bufs]
+    oracle.amr_rk3_step(P, h["geoms"], lays, bufs, 2.0e-3, 2)
+    # (the stage combinations a U + (1 - a) U round: equal to a few ulps, not bit for bit)
+    for lay, a, b in zip(lays, bufs, ref):
+        for q in range(lay.nbox):
+            sl = (slice(None),) + (slice(3, -3),) * 2
+            x, y = lay.fab_view(a, q)[sl], lay.fab_view(b, q)[sl]
+            assert np.all(np.abs(x - y) <= 4e-16 * np.abs(y)), np.abs(x - y).max()
+
+
+def test_amr_compute_dt_is_the_smallest_level_estimate():
+    from compas_amd import make_geometry
+    g0 = make_geometry((16, 16), (0.0, 0.0), (1.0, 1.0), (1, 1))
+    g1 = make_geometry((32, 32), (0.0, 0.0), (1.0, 1.0), (1, 1))
+    cm = np.array([3.0, 2.0])
+    est = [0.4 * (1 / 16) / 3.0, 0.4 * (1 / 32) / 2.0]
+    assert oracle.amr_compute_dt([g0, g1], cm, 0.4, 1, 0.0) == min(est)
+    assert oracle.amr_compute_dt([g0, g1], cm, 0.4, 1, 0.005) == min(min(est), 1.1 * 0.005)
+    assert oracle.amr_compute_dt([g0, g1], cm, 0.4, 0, 0.005) == min(est)
