@@ -28,6 +28,15 @@ Workloads (--workload):
                           RK stage over the tiles of every box), and the right-hand side of the
                           level in one launch against the plain per-box form (one cmp_rhs_box
                           launch per box), both replayed from captured graphs. Flow step only.
+  shock-bubble-amr        the shock-bubble configuration's flow step on its hierarchy: 1024x256 base
+                          grid of [0,4]x[0,1] (boxes of 64) plus one level (ratio 2) from the
+                          GPU tagging (|grad alpha| >= 0.1, |grad p| >= 0.1) clustered into boxes
+                          of at most 64 (blocking factor 8); ambient gamma 1.4 rho 1 p 1, bubble
+                          gamma 1.648 rho 0.138 r 0.25 at (1.5, 0.5), Mach 1.22 shock at x = 1;
+                          WENO5B + HLLC, RK3 at CFL 0.4, levels stepped together without
+                          subcycling (coarse faces under the fine level take its face means), the
+                          hierarchy fixed over the timed steps; beside the uniform 2048x512 grid
+                          it stands for, stepped by the single-level path. Flow step only.
 Also reported: the dominant kernel's roofline position (measured with HIP events on the stream it
 runs on) and the oracle (CPU restatement, OpenMP) timed on this host on a bounded sample.
 """
@@ -156,6 +165,93 @@ def uneven_level(args):
                      "achieved": flops / (times["level"] * 1e-3) / 1e12, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / (times["level"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
                      "traffic": None, "flops_per_cell_stage": roofline.flops_per_cell_stage(P)},
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def shock_bubble_amr(args):
+    """The shock-bubble-amr workload: one JSON line (see the module docstring)."""
+    import numpy as np
+    import torch
+    from compas_amd import Amr, Layout, Level, cases, regrid, roofline
+    dev = torch.device("cuda:0")
+    kw = dict(mach=1.22, rho_ratio=0.138, gamma_bubble=1.648, gamma_ambient=1.4, radius=0.25,
+              max_grid_size=64, scheme="WENO5B", xb=1.5, xs=1.0)
+    c0 = cases.shock_bubble_2d(1024, 256, **kw)
+    P = c0["params"]
+    l0 = Layout(2, c0["boxes"], 3, P.nstate)
+    L0 = Level(P, c0["geom"], l0)
+    U0 = torch.from_numpy(l0.pack(c0["U"])).to(dev)
+    L0.FillPatch(U0)
+    crit = [(5, "grad", 0.1), ("P", "grad", 0.1)]
+    # (blocking factor 8 and max_grid_size 64 in the fine level's cells: 4 and 32 coarse cells)
+    fine_boxes, tags = regrid.regrid(L0, U0, crit, c0["shape"], ratio=2, blocking_factor=4, max_grid_size=32)
+    c1 = cases.shock_bubble_2d(2048, 512, **kw)
+    l1 = Layout(2, fine_boxes, 3, P.nstate)
+    L1 = Level(P, c1["geom"], l1)
+    U1f = torch.from_numpy(l1.pack(c1["U"])).to(dev)
+    L1.AverageDownTo(U1f, L0, U0, 2)
+    amr = Amr([L0, L1], 2)
+    Un = [U0, U1f]
+    S1 = [torch.zeros_like(u) for u in Un]
+    S2 = [torch.zeros_like(u) for u in Un]
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    cmax = torch.zeros(2, dtype=torch.float64, device=dev)
+    cells = [int(np.prod(c0["shape"])), sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in fine_boxes)]
+    K = args.steps if args.steps is not None else 50
+    W = args.warmup
+    amr.advance(Un, S1, S2, W, 0.4, dt, cmax)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    amr.advance(Un, S1, S2, K, 0.4, dt, cmax)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert amr.check_errors() == 0
+    amr.close()
+    # the uniform fine grid the hierarchy stands for, single-level path (graph-replayed steps)
+    lu = Layout(2, c1["boxes"], 3, P.nstate)
+    Lu = Level(P, c1["geom"], lu)
+    Uu = torch.from_numpy(lu.pack(c1["U"])).to(dev)
+    Vu1, Vu2 = torch.zeros_like(Uu), torch.zeros_like(Uu)
+    du = torch.zeros(1, dtype=torch.float64, device=dev)
+    cu = torch.zeros(1, dtype=torch.float64, device=dev)
+    Lu.advance(Uu, Vu1, Vu2, W, 0.4, du, cu, use_graph=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    Lu.advance(Uu, Vu1, Vu2, K, 0.4, du, cu, use_graph=True, resume=True)
+    torch.cuda.synchronize()
+    wall_u = time.perf_counter() - t1
+    assert Lu.check_errors() == 0
+    Lu.close()
+    L0.close()
+    L1.close()
+    total = sum(cells)
+    fps = roofline.flops_per_cell_stage(P)
+    achieved = 3.0 * fps * total * K / wall / 1e12
+    rec = {
+        "metric": "cell-updates/s (5Eq WENO5-HLLC RK3), % roofline; p50/p99 case latency",
+        "value": total * K / wall,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (shock-bubble initial state, hierarchy from the GPU tagging)",
+        "config": {"workload": "shock-bubble-amr", "base_grid": [1024, 256], "levels": 2, "ratio": 2,
+                   "fine_boxes": len(fine_boxes), "cells_per_level": cells,
+                   "fine_coverage": cells[1] / (4.0 * cells[0]), "tagged_cells": int(tags.sum()),
+                   "scheme": "WENO5B", "riemann": "HLLC", "integrator": "TVD-RK3 (levels stage-coupled)",
+                   "cfl": 0.4, "phase_field": None},
+        "uniform_fine_grid": {"grid": [2048, 512], "ms_per_step": wall_u / K * 1e3,
+                              "cell_updates_per_s": 2048 * 512 * K / wall_u},
+        "amr_step_over_uniform_step": (wall / K) / (wall_u / K),
+        "roofline": {"bound": "valu", "kernel": "whole hierarchy step (all kernels)", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": None, "flops_per_cell_stage": fps},
     }
     print(json.dumps(rec), flush=True)
 
@@ -325,6 +421,9 @@ def main():
         return
     if args.workload == "nphase-uneven-level":
         uneven_level(args)
+        return
+    if args.workload == "shock-bubble-amr":
+        shock_bubble_amr(args)
         return
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8

@@ -446,10 +446,11 @@ class Amr:
         check(self.lib.cmp_amr_rk3_step(self.h, self._arr(Un), self._arr(U1), self._arr(U2), _ptr(dt_dev),
                                         _ptr(cmax), _stream(stream)), "cmp_amr_rk3_step")
 
-    def advance(self, Un, U1, U2, nsteps, cfl, dt_dev, cmax, dt_hist=None, limiter=1, stream=None):
+    def advance(self, Un, U1, U2, nsteps, cfl, dt_dev, cmax, dt_hist=None, limiter=1, use_graph=True,
+                stream=None):
         check(self.lib.cmp_amr_advance(self.h, self._arr(Un), self._arr(U1), self._arr(U2), int(nsteps),
                                        float(cfl), int(limiter), _ptr(dt_dev), _ptr(cmax), _ptr(dt_hist),
-                                       _stream(stream)), "cmp_amr_advance")
+                                       int(use_graph), _stream(stream)), "cmp_amr_advance")
 
     def check_errors(self, stream=None):
         return max(lv.check_errors(stream=stream) for lv in self.levels)

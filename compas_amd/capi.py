@@ -155,7 +155,7 @@ SIGNATURES = {
     "cmp_amr_rk3_step": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), dp, dp,
                                         vp]),
     "cmp_amr_advance": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_int,
-                                       ctypes.c_double, ctypes.c_int, dp, dp, dp, vp]),
+                                       ctypes.c_double, ctypes.c_int, dp, dp, dp, ctypes.c_int, vp]),
     "cmp_batch_create": (ctypes.c_int, [P(cmp_params), ctypes.c_longlong, ctypes.c_int, P(vp)]),
     "cmp_batch_destroy": (ctypes.c_int, [vp]),
     "cmp_batch_admit": (ctypes.c_int, [vp, P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,

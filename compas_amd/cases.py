@@ -296,20 +296,23 @@ SERVED_GRIDS = [(128, 128), (256, 256), (512, 128), (512, 512), (1024, 256)]
 
 
 def shock_bubble_2d(nx, ny, mach, rho_ratio, gamma_bubble, gamma_ambient, radius, max_grid_size=64,
-                    scheme="WENO5", riemann="HLLC"):
+                    scheme="WENO5", riemann="HLLC", xb=None, xs=None):
     """2D two-phase shock-bubble: domain [0, nx/ny] x [0, 1] (dx = dy = 1/ny), ambient ideal gas
     (phase 2: gamma_ambient, rho 1, p 1) at rest, a bubble of phase 1 (gamma_bubble, rho_ratio) of
     radius `radius` at (x_b, 0.5) with a tanh profile of width dx clipped to [1e-8, 1 - 1e-8], and a
     planar shock of Mach `mach` at x_b - radius - 0.1 moving in +x with the Rankine-Hugoniot
     post-shock state behind it (include/PhysicsFunctions.H:15-34). Outflow (foextrap) on every
-    side: the post-shock inflow side holds a uniform state."""
+    side: the post-shock inflow side holds a uniform state. xb / xs place the bubble centre and the
+    shock explicitly."""
     width = nx / ny
     params = make_params(2, gamma=(gamma_bubble, gamma_ambient), scheme=scheme, riemann=riemann, source_term=0)
     lo_bc, hi_bc = [2, 2], [2, 2]
     geom = make_geometry((nx, ny), (0.0, 0.0), (width, 1.0), (0, 0), lo_bc=lo_bc, hi_bc=hi_bc)
     x, y = _centres(geom, (ny, nx))
-    xb = max(radius + 0.25, 0.45 * width)
-    xs = max(xb - radius - 0.1, 0.05)
+    if xb is None:
+        xb = max(radius + 0.25, 0.45 * width)
+    if xs is None:
+        xs = max(xb - radius - 0.1, 0.05)
     r = np.sqrt((x - xb) ** 2 + (y - 0.5) ** 2)
     a1 = np.clip(0.5 * (1.0 + np.tanh((radius - r) / (2.0 * geom.dx[0]))), 1e-8, 1.0 - 1e-8)
     p2, rho2, u2 = rankine_hugoniot(gamma_ambient, 1.0, 1.0, mach)

@@ -22,19 +22,19 @@ using namespace cmp;
 
 namespace cmp {
 template <int DIM>
-hipError_t launch_cf_correct_dim(const LaunchCfg&, const KParams&, const CfCell*, int, const BoxDev*, const double*,
-                                 const FillDev&, const FillDev&, const double*, int, double*, const double*, int,
-                                 int*, hipStream_t);
+hipError_t launch_cf_correct_dim(const LaunchCfg&, const KParams&, int, const CfCell*, int, const CfFace*, int,
+                                 double*, const BoxDev*, const double*, const BoxDev*, const double*, int, double*,
+                                 const double*, int, int*, hipStream_t);
 
-hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, const CfCell* list, int n,
-                             const BoxDev* cboxes, const double* Uc, const FillDev& cfd, const FillDev& ffd,
-                             const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
-                             hipStream_t s)
+hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, int part, const CfCell* list, int n,
+                             const CfFace* jobs, int njobs, double* faces, const BoxDev* cboxes, const double* Uc,
+                             const BoxDev* fboxes, const double* Uf, int ratio, double* Uout, const double* dt,
+                             int stage, int* err, hipStream_t s)
 {
-    return lc.dim == 2 ? launch_cf_correct_dim<2>(lc, kp, list, n, cboxes, Uc, cfd, ffd, Uf, ratio, Uout, dt, stage,
-                                                  err, s)
-                       : launch_cf_correct_dim<3>(lc, kp, list, n, cboxes, Uc, cfd, ffd, Uf, ratio, Uout, dt, stage,
-                                                  err, s);
+    return lc.dim == 2 ? launch_cf_correct_dim<2>(lc, kp, part, list, n, jobs, njobs, faces, cboxes, Uc, fboxes, Uf,
+                                                  ratio, Uout, dt, stage, err, s)
+                       : launch_cf_correct_dim<3>(lc, kp, part, list, n, jobs, njobs, faces, cboxes, Uc, fboxes, Uf,
+                                                  ratio, Uout, dt, stage, err, s);
 }
 }  // namespace cmp
 
@@ -43,7 +43,21 @@ struct cmp_amr {
     int ratio = 2;
     std::vector<CfCell*> d_cf;   // per coarse level (all but the finest)
     std::vector<int> ncf;
+    std::vector<CfFace*> d_jobs; // the face jobs of each coarse level's list
+    std::vector<int> njobs;
+    std::vector<double*> d_faces;   // their results, N + 1 doubles a job
+    // the finer levels' work of a stage runs on a second stream beside level 0's (fork / join)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_fill0 = nullptr, ev_join = nullptr;
     AmrDt adt{};
+    int* d_step = nullptr;               // step counter and dt-history capacity of an advance call
+    // the replayed step of cmp_amr_advance, keyed by its buffers and settings
+    hipStream_t own_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<const void*> gkey;
+    double gcfl = 0.0;
+    int glimiter = -1;
 };
 
 namespace {
@@ -61,7 +75,8 @@ int owner_of(const cmp_level* L, const int* c)
 }
 
 // The coarse cells of level C next to level F (its refinement by ratio), with their bordering faces.
-int build_cf(const cmp_level* C, const cmp_level* F, const cmp_level* FF, int ratio, std::vector<CfCell>& out)
+int build_cf(const cmp_level* C, const cmp_level* F, const cmp_level* FF, int ratio, std::vector<CfCell>& out,
+             std::vector<CfFace>& jobs)
 {
     const int dim = C->P.dim;
     auto covered = [&](const int* c) -> bool {
@@ -78,6 +93,7 @@ int build_cf(const cmp_level* C, const cmp_level* F, const cmp_level* FF, int ra
                     const int c[3] = {i, j, k};
                     if (covered(c)) continue;
                     int mask = 0;
+                    const int job0 = (int)jobs.size();
                     for (int d = 0; d < dim; ++d)
                         for (int side = 0; side < 2; ++side) {
                             int x[3] = {i, j, k};
@@ -89,6 +105,29 @@ int build_cf(const cmp_level* C, const cmp_level* F, const cmp_level* FF, int ra
                             }
                             if (!covered(x)) continue;
                             mask |= 1 << (2 * d + side);
+                            // the coarse face, then the fine faces over it (lower transverse index fastest)
+                            CfFace cj{b, 0, d, {i, j, k}};
+                            cj.c[d] += side;
+                            jobs.push_back(cj);
+                            int g0[3] = {0, 0, 0};
+                            for (int a = 0; a < dim; ++a) g0[a] = x[a] * ratio;
+                            const int fpos = side ? x[d] * ratio : (x[d] + 1) * ratio;
+                            int cov[3] = {g0[0], g0[1], g0[2]};
+                            cov[d] = side ? fpos : fpos - 1;
+                            const int fb = owner_of(F, cov);
+                            if (fb < 0) return CMP_ERR_LAYOUT;
+                            int t1 = -1, t2 = -1;
+                            for (int a = 0; a < dim; ++a)
+                                if (a != d) { if (t1 < 0) t1 = a; else t2 = a; }
+                            const int n1 = (t1 >= 0) ? ratio : 1, n2 = (t2 >= 0) ? ratio : 1;
+                            for (int o2 = 0; o2 < n2; ++o2)
+                                for (int o1 = 0; o1 < n1; ++o1) {
+                                    CfFace fj{fb, 1, d, {g0[0], g0[1], g0[2]}};
+                                    fj.c[d] = fpos;
+                                    if (t1 >= 0) fj.c[t1] += o1;
+                                    if (t2 >= 0) fj.c[t2] += o2;
+                                    jobs.push_back(fj);
+                                }
                             // the fine faces used must be the finer level's own (not under a third
                             // level, whose face means would replace them first)
                             if (FF) {
@@ -110,7 +149,7 @@ int build_cf(const cmp_level* C, const cmp_level* F, const cmp_level* FF, int ra
                                     }
                             }
                         }
-                    if (mask) out.push_back(CfCell{b, {i, j, k}, mask});
+                    if (mask) out.push_back(CfCell{b, {i, j, k}, mask, job0});
                 }
     }
     return CMP_OK;
@@ -120,6 +159,18 @@ void amr_free(cmp_amr* A)
 {
     for (CfCell* p : A->d_cf)
         if (p) (void)hipFree(p);
+    for (CfFace* p : A->d_jobs)
+        if (p) (void)hipFree(p);
+    for (double* p : A->d_faces)
+        if (p) (void)hipFree(p);
+    if (A->d_step) (void)hipFree(A->d_step);
+    if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
+    if (A->graph) (void)hipGraphDestroy(A->graph);
+    if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
+    if (A->s2) (void)hipStreamDestroy(A->s2);
+    hipEvent_t evs[] = {A->ev_fork, A->ev_fill0, A->ev_join};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
     delete A;
 }
 
@@ -163,17 +214,33 @@ int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out)
         for (int d = 0; d < 3; ++d) A->adt.dx[l][d] = levels[l]->G.dx[d];
     for (int l = 0; l + 1 < nlev; ++l) {
         std::vector<CfCell> cf;
-        const int rc = build_cf(levels[l], levels[l + 1], (l + 2 < nlev) ? levels[l + 2] : nullptr, ratio, cf);
+        std::vector<CfFace> jobs;
+        const int rc = build_cf(levels[l], levels[l + 1], (l + 2 < nlev) ? levels[l + 2] : nullptr, ratio, cf, jobs);
         if (rc) { amr_free(A); return rc; }
-        CfCell* d = nullptr;
-        if (!cf.empty()) {
-            int r = check_hip(hipMalloc(&d, sizeof(CfCell) * cf.size()));
-            if (!r) r = check_hip(hipMemcpy(d, cf.data(), sizeof(CfCell) * cf.size(), hipMemcpyHostToDevice));
-            if (r) { if (d) (void)hipFree(d); amr_free(A); return r; }
-        }
-        A->d_cf.push_back(d);
+        A->d_cf.push_back(nullptr);
+        A->d_jobs.push_back(nullptr);
+        A->d_faces.push_back(nullptr);
         A->ncf.push_back((int)cf.size());
+        A->njobs.push_back((int)jobs.size());
+        if (!cf.empty()) {
+            int r = check_hip(hipMalloc(&A->d_cf.back(), sizeof(CfCell) * cf.size()));
+            if (!r) r = check_hip(hipMemcpy(A->d_cf.back(), cf.data(), sizeof(CfCell) * cf.size(), hipMemcpyHostToDevice));
+            if (!r) r = check_hip(hipMalloc(&A->d_jobs.back(), sizeof(CfFace) * jobs.size()));
+            if (!r) r = check_hip(hipMemcpy(A->d_jobs.back(), jobs.data(), sizeof(CfFace) * jobs.size(),
+                                            hipMemcpyHostToDevice));
+            if (!r) r = check_hip(hipMalloc(&A->d_faces.back(), sizeof(double) * (size_t)(N + 1) * jobs.size()));
+            if (r) { amr_free(A); return r; }
+        }
     }
+    int r = check_hip(hipMalloc(&A->d_step, sizeof(int) * 2));
+    if (!r) r = check_hip(hipMemset(A->d_step, 0, sizeof(int) * 2));
+    if (!r && nlev > 1) {
+        r = check_hip(hipStreamCreateWithFlags(&A->s2, hipStreamNonBlocking));
+        if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_fork, hipEventDisableTiming));
+        if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_fill0, hipEventDisableTiming));
+        if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_join, hipEventDisableTiming));
+    }
+    if (r) { amr_free(A); return r; }
     *out = A;
     return CMP_OK;
 }
@@ -223,53 +290,126 @@ int cmp_amr_initial_cmax(cmp_amr* A, double* const* U, double* cmax, void* strea
 int cmp_amr_compute_dt(cmp_amr* A, double* cmax, double* dt_dev, double cfl, int limiter, void* stream)
 {
     if (!A || !cmax || !dt_dev) return CMP_ERR_ARG;
-    CMP_TRY(launch_amr_dt(A->adt, cmax, dt_dev, cfl, limiter, nullptr, 0, A->L[0]->d_err, (hipStream_t)stream));
+    CMP_TRY(launch_amr_dt(A->adt, cmax, dt_dev, cfl, limiter, nullptr, A->d_step, A->L[0]->d_err,
+                          (hipStream_t)stream));
     return CMP_OK;
 }
 
+// One stage's work graph: level 0's fill and stage kernel on the caller's stream; on the second
+// stream, each finer level's two-level fill (it reads the coarser stage state's valid cells, which
+// level 0's fill leaves alone) and stage kernel, then the coarse-fine face jobs (after level 0's
+// fill); back on the caller's stream, the corrections (after level 0's stage kernel and the face
+// jobs) and the average-down (after every stage kernel), finest first.
 int cmp_amr_rk3_step(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
                      double* cmax, void* stream)
 {
     if (!A || !Un || !U1 || !U2 || !dt_dev || !cmax) return CMP_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     const int nlev = (int)A->L.size();
+    auto stage_of = [&](int l, int stage, double* const* src, double* const* dst, hipStream_t q) -> int {
+        cmp_level* L = A->L[l];
+        CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src[l],
+                             Un[l], dst[l], dt_dev, (unsigned long long*)(cmax + l), L->d_err, q));
+        return CMP_OK;
+    };
+    auto cf = [&](int l, int part, int stage, double* const* src, double* const* dst, hipStream_t q) -> int {
+        cmp_level* C = A->L[l];
+        cmp_level* F = A->L[l + 1];
+        CMP_TRY(launch_cf_correct(C->lc, C->kp, part, A->d_cf[l], A->ncf[l], A->d_jobs[l], A->njobs[l],
+                                  A->d_faces[l], C->d_boxes, src[l], F->d_boxes, src[l + 1], A->ratio, dst[l], dt_dev,
+                                  stage, C->d_err, q));
+        return CMP_OK;
+    };
     for (int stage = 0; stage < 3; ++stage) {
         double* const* src = (stage == 0) ? Un : ((stage == 1) ? U1 : U2);
         double* const* dst = (stage == 0) ? U1 : ((stage == 1) ? U2 : Un);
-        int r = amr_fill(A, src, s);
-        if (r) return r;
-        for (int l = 0; l < nlev; ++l) {
-            cmp_level* L = A->L[l];
-            CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0,
-                                 src[l], Un[l], dst[l], dt_dev, (unsigned long long*)(cmax + l), L->d_err, s));
+        int r;
+        if (nlev == 1) {
+            if ((r = cmp_level_fill_ghosts(A->L[0], src[0], s))) return r;
+            if ((r = stage_of(0, stage, src, dst, s))) return r;
+            continue;
         }
-        for (int l = 0; l + 1 < nlev; ++l) {
-            cmp_level* C = A->L[l];
-            cmp_level* F = A->L[l + 1];
-            CMP_TRY(launch_cf_correct(C->lc, C->kp, A->d_cf[l], A->ncf[l], C->d_boxes, src[l], C->fd, F->fd,
-                                      src[l + 1], A->ratio, dst[l], dt_dev, stage, C->d_err, s));
+        hipStream_t q = A->s2;
+        CMP_TRY(hipEventRecord(A->ev_fork, s));
+        CMP_TRY(hipStreamWaitEvent(q, A->ev_fork, 0));
+        // level 0
+        if ((r = cmp_level_fill_ghosts(A->L[0], src[0], s))) return r;
+        CMP_TRY(hipEventRecord(A->ev_fill0, s));
+        if ((r = stage_of(0, stage, src, dst, s))) return r;
+        // the finer levels, then the face jobs
+        for (int l = 1; l < nlev; ++l) {
+            if ((r = cmp_level_fill_two_levels(A->L[l], src[l], A->L[l - 1], src[l - 1], A->ratio, q))) return r;
+            if ((r = stage_of(l, stage, src, dst, q))) return r;
         }
-        for (int l = nlev - 2; l >= 0; --l) {
-            r = cmp_level_average_down(A->L[l + 1], dst[l + 1], A->L[l], dst[l], A->ratio, s);
-            if (r) return r;
-        }
+        CMP_TRY(hipStreamWaitEvent(q, A->ev_fill0, 0));
+        for (int l = 0; l + 1 < nlev; ++l)
+            if ((r = cf(l, 1, stage, src, dst, q))) return r;
+        CMP_TRY(hipEventRecord(A->ev_join, q));
+        CMP_TRY(hipStreamWaitEvent(s, A->ev_join, 0));
+        for (int l = 0; l + 1 < nlev; ++l)
+            if ((r = cf(l, 2, stage, src, dst, s))) return r;
+        for (int l = nlev - 2; l >= 0; --l)
+            if ((r = cmp_level_average_down(A->L[l + 1], dst[l + 1], A->L[l], dst[l], A->ratio, s))) return r;
     }
     return CMP_OK;
 }
 
-int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, int nsteps, double cfl,
-                    int limiter, double* dt_dev, double* cmax, double* dt_hist, void* stream)
+static int amr_step(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, double cfl, int limiter,
+                    double* dt_dev, double* cmax, double* dt_hist, hipStream_t s)
 {
-    if (!A || nsteps < 0 || !dt_dev || !cmax) return CMP_ERR_ARG;
+    CMP_TRY(launch_amr_dt(A->adt, cmax, dt_dev, cfl, limiter, dt_hist, A->d_step, A->L[0]->d_err, s));
+    return cmp_amr_rk3_step(A, Un, U1, U2, dt_dev, cmax, s);
+}
+
+int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, int nsteps, double cfl,
+                    int limiter, double* dt_dev, double* cmax, double* dt_hist, int use_graph, void* stream)
+{
+    if (!A || !Un || !U1 || !U2 || nsteps < 0 || !dt_dev || !cmax) return CMP_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
     int r = cmp_amr_initial_cmax(A, Un, cmax, stream);
     if (r) return r;
     CMP_TRY(hipMemsetAsync(dt_dev, 0, sizeof(double), s));
-    for (int step = 0; step < nsteps; ++step) {
-        CMP_TRY(launch_amr_dt(A->adt, cmax, dt_dev, cfl, limiter, dt_hist, step, A->L[0]->d_err, s));
-        r = cmp_amr_rk3_step(A, Un, U1, U2, dt_dev, cmax, stream);
-        if (r) return r;
+    CMP_TRY(launch_set2(A->d_step, 0, dt_hist ? nsteps : 0, s));
+    if (!use_graph || nsteps < 2) {
+        for (int step = 0; step < nsteps; ++step)
+            if ((r = amr_step(A, Un, U1, U2, cfl, limiter, dt_dev, cmax, dt_hist, s))) return r;
+        return CMP_OK;
     }
+    // one step captured and replayed (on a private stream when the caller gave the legacy default
+    // stream, which cannot be captured)
+    hipStream_t cs = s;
+    if (!cs) {
+        if (!A->own_stream) CMP_TRY(hipStreamCreateWithFlags(&A->own_stream, hipStreamNonBlocking));
+        cs = A->own_stream;
+        CMP_TRY(hipStreamSynchronize(nullptr));
+    }
+    const int nlev = (int)A->L.size();
+    std::vector<const void*> key;
+    for (int l = 0; l < nlev; ++l) {
+        key.push_back(Un[l]);
+        key.push_back(U1[l]);
+        key.push_back(U2[l]);
+    }
+    key.push_back(dt_dev);
+    key.push_back(cmax);
+    key.push_back(dt_hist);
+    if (!(A->gexec && A->gkey == key && A->gcfl == cfl && A->glimiter == limiter)) {
+        if (A->gexec) { CMP_TRY(hipGraphExecDestroy(A->gexec)); A->gexec = nullptr; }
+        if (A->graph) { CMP_TRY(hipGraphDestroy(A->graph)); A->graph = nullptr; }
+        CMP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        r = amr_step(A, Un, U1, U2, cfl, limiter, dt_dev, cmax, dt_hist, cs);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cs, &g);
+        if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+        CMP_TRY(e);
+        A->graph = g;
+        CMP_TRY(hipGraphInstantiate(&A->gexec, g, nullptr, nullptr, 0));
+        A->gkey = key;
+        A->gcfl = cfl;
+        A->glimiter = limiter;
+    }
+    for (int step = 0; step < nsteps; ++step) CMP_TRY(hipGraphLaunch(A->gexec, cs));
+    if (cs != s) CMP_TRY(hipStreamSynchronize(cs));
     return CMP_OK;
 }
 

@@ -815,8 +815,9 @@ hipError_t launch_batch_dt(const BatchSlot* slots, int nslot, double* cmax, doub
 // ComputeDt over the levels of a hierarchy without subcycling (source/COMPAS.cpp:2250-2302 with
 // nsubsteps 1): each level's estimate min_d cfl dx_d / c_max, limited to 1.1 dt_prev when the
 // limiter is on and dt_prev > 0 (a first step passes dt_prev = 0), the smallest over the levels;
-// every level's c_max then restarts at zero. One thread.
-__global__ void k_amr_dt(AmrDt a, double* cmax, double* dt, double cfl, int limiter, double* hist, int step,
+// every level's c_max then restarts at zero. stepc[0] counts the steps of the current advance call
+// (hist[stepc[0]] takes the dt while below the capacity stepc[1]). One thread.
+__global__ void k_amr_dt(AmrDt a, double* cmax, double* dt, double cfl, int limiter, double* hist, int* stepc,
                          int* err)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -836,14 +837,16 @@ __global__ void k_amr_dt(AmrDt a, double* cmax, double* dt, double cfl, int limi
     }
     if (bad || __builtin_isnan(dt0)) atomicMax(err, CMP_ERR_NAN);
     *dt = dt0;
-    if (hist) hist[step] = dt0;
+    const int st = stepc[0];
+    if (hist && st >= 0 && st < stepc[1]) hist[st] = dt0;
+    stepc[0] = st + 1;
     for (int l = 0; l < a.nlev; ++l) cmax[l] = 0.0;
 }
 
-hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, int limiter, double* hist, int step,
+hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, int limiter, double* hist, int* stepc,
                          int* err, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(64), 0, s, a, cmax, dt, cfl, limiter, hist, step, err);
+    hipLaunchKernelGGL(k_amr_dt, dim3(1), dim3(64), 0, s, a, cmax, dt, cfl, limiter, hist, stepc, err);
     return hipGetLastError();
 }
 
@@ -1263,20 +1266,46 @@ void k_fill2(FillDev fd, FillDev cd, int box0, int ratio, double* Uf, const doub
 // source/Class_Compressible_PhaseField_StageCoupling.cpp:84-153, with run.do_reflux = 1): a coarse
 // face over the finer level takes the mean of the fine faces over it (amrex::average_down_faces:
 // the lower transverse index fastest, times 1/ratio^(dim-1)), fluxes and u*_n alike, before the
-// divergence. The stage kernel has already written the coarse cell with its own faces; one thread
-// per listed coarse cell adds w dt times the change: -(F_mean - F)/dx for a high face, +(F_mean -
-// F)/dx for a low face, and the nc coefficient of the cell times the change of div u*. Uc and Uf
-// are the stage states with every ghost cell filled; the fine faces are computed from the fine box
-// that holds the covered side.
+// divergence. The stage kernel has already written the coarse cell with its own faces; the
+// correction adds w dt times the change: -(F_mean - F)/dx for a high face, +(F_mean - F)/dx for a
+// low face, and the nc coefficient of the cell times the change of div u*. Uc and Uf are the stage
+// states with every ghost cell filled.
+//
+// k_cf_faces: one face job per thread (a coarse face, or one of the fine faces over it).
 template <int DIM, int SCHEME, int RIEMANN, bool THINC>
-__global__ __launch_bounds__(NT)
-void k_cf_correct(KParams kp, int limiter, const CfCell* __restrict__ list, int n, const BoxDev* __restrict__ cboxes,
-                  const double* __restrict__ Uc, FillDev cfd, FillDev ffd, const double* __restrict__ Uf,
-                  int ratio, double* Uout, const double* __restrict__ dtp, int stage, int* err)
+__global__ __launch_bounds__(64)
+void k_cf_faces(KParams kp, int limiter, const CfFace* __restrict__ jobs, int njobs, const BoxDev* __restrict__ cboxes,
+                const double* __restrict__ Uc, const BoxDev* __restrict__ fboxes, const double* __restrict__ Uf,
+                double* __restrict__ faces, int* err)
+{
+    constexpr int N = 4 + DIM;
+    const int j = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (j >= njobs) return;
+    const CfFace fj = jobs[j];
+    const BoxDev bx = fj.fine ? fboxes[fj.box] : cboxes[fj.box];
+    const double* Ub = (fj.fine ? Uf : Uc) + bx.off;
+    int bad = 0;
+    double F[N], u;
+    if (fj.dir == 0) u = face_at<DIM, SCHEME, RIEMANN, THINC, 0>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
+    else if (fj.dir == 1) u = face_at<DIM, SCHEME, RIEMANN, THINC, 1>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
+    else u = face_at<DIM, SCHEME, RIEMANN, THINC, (DIM == 3 ? 2 : 1)>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
+    double* o = faces + (long long)j * (N + 1);
+#pragma unroll
+    for (int q = 0; q < N; ++q) o[q] = F[q];
+    o[N] = u;
+    if (bad) atomicMax(err, bad);
+}
+
+// k_cf_apply: one thread per listed coarse cell, its face jobs combined in their fixed order.
+template <int DIM>
+__global__ __launch_bounds__(64)
+void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev* __restrict__ cboxes,
+                const double* __restrict__ Uc, const double* __restrict__ faces, int ratio, double* Uout,
+                const double* __restrict__ dtp, int stage, int* err)
 {
     constexpr int N = 4 + DIM;
     constexpr int IA = 3 + DIM;
-    const int e = (int)(blockIdx.x * NT + threadIdx.x);
+    const int e = (int)(blockIdx.x * 64 + threadIdx.x);
     if (e >= n) return;
     const CfCell cc = list[e];
     const BoxDev cb = cboxes[cc.box];
@@ -1285,58 +1314,29 @@ void k_cf_correct(KParams kp, int limiter, const CfCell* __restrict__ list, int 
 #pragma unroll
     for (int q = 0; q < N; ++q) dL[q] = 0.0;
     double ddiv = 0.0;
-    auto face_dir = [&](int d, const BoxDev& bx, const double* Ub, int i, int j, int k, double* F) -> double {
-        if (d == 0) return face_at<DIM, SCHEME, RIEMANN, THINC, 0>(kp, limiter, bx, Ub, i, j, k, F, bad);
-        if (d == 1) return face_at<DIM, SCHEME, RIEMANN, THINC, 1>(kp, limiter, bx, Ub, i, j, k, F, bad);
-        return face_at<DIM, SCHEME, RIEMANN, THINC, (DIM == 3 ? 2 : 1)>(kp, limiter, bx, Ub, i, j, k, F, bad);
-    };
+    int jb = cc.job0;
+    int nsub = 1;
+    for (int a = 1; a < DIM; ++a) nsub *= ratio;
+    const double facInv = 1.0 / (double)nsub;
     for (int d = 0; d < DIM; ++d)
         for (int side = 0; side < 2; ++side) {
             if (!((cc.mask >> (2 * d + side)) & 1)) continue;
-            int f[3] = {cc.c[0], cc.c[1], cc.c[2]};
-            f[d] += side;
-            double Fc[N];
-            const double uc = face_dir(d, cb, Uc + cb.off, f[0], f[1], f[2], Fc);
-            // the covered coarse cell across the face (periodic image inside the domain)
-            int X[3] = {cc.c[0], cc.c[1], cc.c[2]};
-            X[d] += side ? 1 : -1;
-            const int ncd = cfd.dhi[d] - cfd.dlo[d] + 1;
-            if (X[d] < cfd.dlo[d]) X[d] += ncd;
-            else if (X[d] > cfd.dhi[d]) X[d] -= ncd;
-            int g0[3] = {0, 0, 0};
-#pragma unroll
-            for (int a = 0; a < DIM; ++a) g0[a] = X[a] * ratio;
-            int cov[3] = {g0[0], g0[1], g0[2]};
-            const int fpos = side ? X[d] * ratio : (X[d] + 1) * ratio;
-            cov[d] = side ? fpos : fpos - 1;
-            const int own = cell_owner<DIM>(ffd, cov);
-            if (own < 0) { bad = CMP_ERR_LAYOUT; continue; }
-            const BoxDev fb = ffd.boxes[own];
-            int t1 = -1, t2 = -1;
-            for (int a = 0; a < DIM; ++a)
-                if (a != d) { if (t1 < 0) t1 = a; else t2 = a; }
-            const int r1 = (t1 >= 0) ? ratio : 1, r2 = (t2 >= 0) ? ratio : 1;
+            const double* fc = faces + (long long)jb * (N + 1);
             double sF[N], su = 0.0;
 #pragma unroll
             for (int q = 0; q < N; ++q) sF[q] = 0.0;
-            for (int o2 = 0; o2 < r2; ++o2)
-                for (int o1 = 0; o1 < r1; ++o1) {
-                    int g[3] = {g0[0], g0[1], g0[2]};
-                    g[d] = fpos;
-                    if (t1 >= 0) g[t1] += o1;
-                    if (t2 >= 0) g[t2] += o2;
-                    double Ff[N];
-                    const double uf = face_dir(d, fb, Uf + fb.off, g[0], g[1], g[2], Ff);
+            for (int m = 1; m <= nsub; ++m) {
+                const double* ff = faces + (long long)(jb + m) * (N + 1);
 #pragma unroll
-                    for (int q = 0; q < N; ++q) sF[q] += Ff[q];
-                    su += uf;
-                }
-            const double facInv = 1.0 / (double)(r1 * r2);
+                for (int q = 0; q < N; ++q) sF[q] += ff[q];
+                su += ff[N];
+            }
+            jb += 1 + nsub;
             const double sgn = side ? -1.0 : 1.0;
             const double idx = kp.inv_dx[d];
 #pragma unroll
-            for (int q = 0; q < N; ++q) dL[q] += sgn * ((sF[q] * facInv - Fc[q]) * idx);
-            ddiv -= sgn * ((su * facInv - uc) * idx);
+            for (int q = 0; q < N; ++q) dL[q] += sgn * ((sF[q] * facInv - fc[q]) * idx);
+            ddiv -= sgn * ((su * facInv - fc[N]) * idx);
         }
     const long long c = cb.off + fab_index(cb, cc.c[0], cc.c[1], cc.c[2]);
     double Ucell[N];
@@ -1355,24 +1355,30 @@ void k_cf_correct(KParams kp, int limiter, const CfCell* __restrict__ list, int 
 }
 
 template <int DIM, int SCHEME, int RIEMANN, bool THINC>
-static hipError_t cfcorr4(const KParams& kp, int limiter, const CfCell* list, int n, const BoxDev* cboxes,
-                          const double* Uc, const FillDev& cfd, const FillDev& ffd, const double* Uf, int ratio,
-                          double* Uout, const double* dt, int stage, int* err, hipStream_t s)
+static hipError_t cfcorr4(const KParams& kp, int limiter, int part, const CfCell* list, int n, const CfFace* jobs,
+                          int njobs, double* faces, const BoxDev* cboxes, const double* Uc, const BoxDev* fboxes,
+                          const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
+                          hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cf_correct<DIM, SCHEME, RIEMANN, THINC>), dim3((n + NT - 1) / NT), dim3(NT), 0, s, kp,
-                       limiter, list, n, cboxes, Uc, cfd, ffd, Uf, ratio, Uout, dt, stage, err);
+    if (part & 1)
+        hipLaunchKernelGGL((k_cf_faces<DIM, SCHEME, RIEMANN, THINC>), dim3((njobs + 63) / 64), dim3(64), 0, s, kp,
+                           limiter, jobs, njobs, cboxes, Uc, fboxes, Uf, faces, err);
+    if (part & 2)
+        hipLaunchKernelGGL((k_cf_apply<DIM>), dim3((n + 63) / 64), dim3(64), 0, s, kp, list, n, cboxes, Uc, faces,
+                           ratio, Uout, dt, stage, err);
     return hipGetLastError();
 }
 
 template <int DIM>
-hipError_t launch_cf_correct_dim(const LaunchCfg& lc, const KParams& kp, const CfCell* list, int n,
-                                 const BoxDev* cboxes, const double* Uc, const FillDev& cfd, const FillDev& ffd,
-                                 const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
-                                 hipStream_t s)
+hipError_t launch_cf_correct_dim(const LaunchCfg& lc, const KParams& kp, int part, const CfCell* list, int n,
+                                 const CfFace* jobs, int njobs, double* faces, const BoxDev* cboxes,
+                                 const double* Uc, const BoxDev* fboxes, const double* Uf, int ratio, double* Uout,
+                                 const double* dt, int stage, int* err, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     if (lc.nphase || lc.riemann == RIEMANN_PF_MIX || lc.riemann == RIEMANN_PF_PHASIC) return hipErrorInvalidValue;
-    CMP_DISPATCH(cfcorr4, kp, lc.limiter, list, n, cboxes, Uc, cfd, ffd, Uf, ratio, Uout, dt, stage, err, s);
+    CMP_DISPATCH(cfcorr4, kp, lc.limiter, part, list, n, jobs, njobs, faces, cboxes, Uc, fboxes, Uf, ratio, Uout, dt,
+                 stage, err, s);
     return hipErrorInvalidValue;
 }
 
@@ -1461,10 +1467,10 @@ void tile_dims_dim(int* tx, int* ty, int* tz)
 }
 
 template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
-template hipError_t launch_cf_correct_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const CfCell*, int,
-                                                         const BoxDev*, const double*, const FillDev&,
-                                                         const FillDev&, const double*, int, double*,
-                                                         const double*, int, int*, hipStream_t);
+template hipError_t launch_cf_correct_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, int, const CfCell*, int,
+                                                         const CfFace*, int, double*, const BoxDev*,
+                                                         const double*, const BoxDev*, const double*, int,
+                                                         double*, const double*, int, int*, hipStream_t);
 template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev&, int, double*,
                                                      const double*, int*, hipStream_t);
 template hipError_t launch_avgdown_dim<CMP_KERNEL_DIM>(const BoxDev*, int, long long, const FillDev&, const double*,

@@ -115,12 +115,24 @@ struct TagCriteria {
     cmp_tag_criterion c[CMP_TAG_MAX_CRITERIA];
     int n;
 };
-// A coarse cell next to a finer level: box, cell, and bit 2 d + side set where its face of
-// direction d (side 0 low, 1 high) borders a coarse cell the finer level covers.
+// A coarse cell next to a finer level: box, cell, bit 2 d + side set where its face of direction d
+// (side 0 low, 1 high) borders a coarse cell the finer level covers, and the first of its face
+// jobs (per bordering face, in (d, side) order: the coarse face, then the ratio^(dim-1) fine faces
+// over it, the lower transverse index fastest).
 struct CfCell {
     int box;
     int c[3];
     int mask;
+    int job0;
+};
+
+// One face of the coarse-fine correction: the face between cells c - e_dir and c of box `box` of
+// the coarse (fine = 0) or the finer (fine = 1) level.
+struct CfFace {
+    int box;
+    int fine;
+    int dir;
+    int c[3];
 };
 
 // Up to this many levels in a hierarchy.
@@ -135,12 +147,16 @@ struct AmrDt {
 // The coarse-fine face correction of one stage (cmp_amr_rk3_step): every listed coarse cell's
 // stage output gains w dt times the change of its right-hand side when its faces over the finer
 // level take the finer level's face means.
-hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, const CfCell* list, int n,
-                             const BoxDev* cboxes, const double* Uc, const FillDev& cfd, const FillDev& ffd,
+// Two launches: (part & 1) every face job (one thread each) into faces (N + 1 doubles a job: flux,
+// u*_n), (part & 2) one thread per listed cell.
+hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, int part, const CfCell* list, int n,
+                             const CfFace* jobs,
+                             int njobs, double* faces, const BoxDev* cboxes, const double* Uc, const BoxDev* fboxes,
                              const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
                              hipStream_t s);
-// ComputeDt over the levels of a hierarchy (no subcycling); resets every level's c_max.
-hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, int limiter, double* hist, int step,
+// ComputeDt over the levels of a hierarchy (no subcycling); resets every level's c_max. stepc:
+// device step counter and dt-history capacity.
+hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, int limiter, double* hist, int* stepc,
                          int* err, hipStream_t s);
 
 hipError_t launch_tag(const LaunchCfg& lc, const KParams& kp, const BoxDev* boxes, int nbox, long long maxcells,

@@ -352,9 +352,10 @@ int cmp_amr_compute_dt(cmp_amr* amr, double* cmax, double* dt_dev, double cfl, i
 int cmp_amr_rk3_step(cmp_amr* amr, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
                      double* cmax, void* stream);
 /* nsteps steps: initial c_max, then per step ComputeDt and the RK3 step; dt_hist (device, nsteps
- * doubles, or NULL) receives every step's dt. */
+ * doubles, or NULL) receives every step's dt. use_graph: one step captured in a hipGraph (kept
+ * while the buffers and settings stay the same) and replayed. */
 int cmp_amr_advance(cmp_amr* amr, double* const* Un, double* const* U1, double* const* U2, int nsteps, double cfl,
-                    int limiter, double* dt_dev, double* cmax, double* dt_hist, void* stream);
+                    int limiter, double* dt_dev, double* cmax, double* dt_hist, int use_graph, void* stream);
 
 /* ---- a batch of independent levels (the served-stream form) ---------------------------------
  * No counterpart in the reference, which runs one case per process (source/main.cpp:21-86): cases of

@@ -173,8 +173,9 @@ def test_amr_step_matches_oracle(gpu, name):
     assert np.abs(_valid_cells(lays[0], got[0]) - _valid_cells(lays[0], bufs[0])).max() > 1e-8
 
 
-@pytest.mark.parametrize("name", ["2d-two-boxes", "2d-three-levels"])
-def test_amr_advance_matches_oracle_and_conserves(gpu, name):
+@pytest.mark.parametrize("name,use_graph", [("2d-two-boxes", True), ("2d-three-levels", True),
+                                            ("2d-three-levels", False)])
+def test_amr_advance_matches_oracle_and_conserves(gpu, name, use_graph):
     import torch
     from compas_amd import Amr
     dim, nc, fine, third = AMR_CASES[name]
@@ -190,7 +191,7 @@ def test_amr_advance_matches_oracle_and_conserves(gpu, name):
     dtd = torch.zeros(1, dtype=torch.float64, device=gpu)
     cmax = torch.zeros(len(levels), dtype=torch.float64, device=gpu)
     dth = torch.zeros(nsteps, dtype=torch.float64, device=gpu)
-    amr.advance(Un, U1, U2, nsteps, 0.4, dtd, cmax, dth)
+    amr.advance(Un, U1, U2, nsteps, 0.4, dtd, cmax, dth, use_graph=use_graph)
     assert amr.check_errors() == 0
     got = [u.cpu().numpy() for u in Un]
     gh = dth.cpu().numpy()
