@@ -148,12 +148,15 @@ def random_smooth(dim, n, seed, periodic=True, max_grid_size=None, scheme="WENO5
                 name="random-smooth-%dd" % dim)
 
 
-def two_level_smooth(dim, nc, seed, fine_boxes, ratio=2, periodic=True, **kw):
+def two_level_smooth(dim, nc, seed, fine_boxes, ratio=2, periodic=True, nphase=0, **kw):
     """A two-level hierarchy of the seeded smooth state: the fine level's valid cells from
-    random_smooth at nc * ratio cells a side, the coarse level (nc a side) the mean of the fine
-    field over each coarse cell. fine_boxes in fine cells. Returns dict(params, geoms, boxes,
-    fields, shapes, cfl) with the coarse level first."""
-    f = random_smooth(dim, nc * ratio, seed, periodic=periodic, **kw)
+    random_smooth (nphase_smooth for nphase > 0) at nc * ratio cells a side, the coarse level (nc a
+    side) the mean of the fine field over each coarse cell. fine_boxes in fine cells. Returns
+    dict(params, geoms, boxes, fields, shapes, cfl) with the coarse level first."""
+    if nphase:
+        f = nphase_smooth(dim, nc * ratio, seed, nphase=nphase, periodic=periodic, **kw)
+    else:
+        f = random_smooth(dim, nc * ratio, seed, periodic=periodic, **kw)
     Uf = f["U"]
     shp = Uf.shape[1:]
     Uc = Uf.reshape((Uf.shape[0],) + tuple(x for n in shp for x in (n // ratio, ratio)))

@@ -186,7 +186,6 @@ int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out)
     if (!L0) return CMP_ERR_ARG;
     const int dim = L0->P.dim;
     const int N = L0->P.nstate;
-    if (L0->P.nphase) return CMP_ERR_UNSUPPORTED;
     for (int l = 0; l < nlev; ++l) {
         const cmp_level* L = levels[l];
         if (!L || L->P.dim != dim || L->P.nstate != N || L->ng != L0->ng) return CMP_ERR_ARG;

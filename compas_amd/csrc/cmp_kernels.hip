@@ -601,12 +601,13 @@ void k_stage_batch(const KParams* __restrict__ kps, int limiter, int stage, cons
 
 // The face flux and u*_n of the face between cells (ci, cj, ck) - e_DIR and (ci, cj, ck) of box bx
 // (Ub: the box's fab, its ghost cells filled), every stencil cell taken through cons->prim here.
-template <int DIM, int SCHEME, int RIEMANN, bool THINC, int DIR>
+// NP > 0: the N-phase model (no THINC).
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int DIR, int NP = 0>
 __device__ __forceinline__ double face_at(const KParams& kp, int limiter, const BoxDev& bx,
                                           const double* __restrict__ Ub, int ci, int cj, int ck, double* Fl,
                                           int& bad)
 {
-    constexpr int N = 4 + DIM;
+    constexpr int N = (NP > 0) ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
     const long long sd = (DIR == 0) ? 1 : ((DIR == 1) ? bx.sj : bx.sk);
@@ -618,20 +619,25 @@ __device__ __forceinline__ double face_at(const KParams& kp, int limiter, const 
         const long long c = c0 + (t - G) * sd;
 #pragma unroll
         for (int q = 0; q < N; ++q) Uc[q] = Ub[c + q * bx.sn];
-        cons2prim<DIM>(kp, Uc, W[t], bad);
+        if constexpr (NP > 0) np_cons2prim<DIM, NP>(kp, Uc, W[t], bad);
+        else cons2prim<DIM>(kp, Uc, W[t], bad);
     }
     auto st = [&](int t, int q) -> double { return W[t][q]; };
-    auto aa = [&](int od, int o1, int o2) -> double {
-        const int lo = (od == -1) ? 1 : 0;
-        int px = ci - (DIR == 0 ? lo : 0), py = cj - (DIR == 1 ? lo : 0), pz = ck - (DIR == 2 ? lo : 0);
-        if (DIR == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
-        else if (DIR == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
-        else { px += o1; py += o2; }
-        return Ub[fab_index(bx, px, py, pz) + IA * bx.sn];
-    };
-    auto normal = [&]() -> double { return face_normal_component<DIM>(kp, DIR, aa); };
     double cm;
-    return face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, DIR, st, normal, Fl, cm, bad);
+    if constexpr (NP > 0) {
+        return np_face_flux<DIM, SCHEME, RIEMANN, NP>(kp, limiter, DIR, st, Fl, cm, bad);
+    } else {
+        auto aa = [&](int od, int o1, int o2) -> double {
+            const int lo = (od == -1) ? 1 : 0;
+            int px = ci - (DIR == 0 ? lo : 0), py = cj - (DIR == 1 ? lo : 0), pz = ck - (DIR == 2 ? lo : 0);
+            if (DIR == 0) { py += o1; pz += (DIM == 3) ? o2 : 0; }
+            else if (DIR == 1) { px += o1; pz += (DIM == 3) ? o2 : 0; }
+            else { px += o1; py += o2; }
+            return Ub[fab_index(bx, px, py, pz) + IA * bx.sn];
+        };
+        auto normal = [&]() -> double { return face_normal_component<DIM>(kp, DIR, aa); };
+        return face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, DIR, st, normal, Fl, cm, bad);
+    }
 }
 
 // Face fluxes of one box in the reference's shape (one thread per face, every stencil cell taken
@@ -1272,13 +1278,13 @@ void k_fill2(FillDev fd, FillDev cd, int box0, int ratio, double* Uf, const doub
 // states with every ghost cell filled.
 //
 // k_cf_faces: one face job per thread (a coarse face, or one of the fine faces over it).
-template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP>
 __global__ __launch_bounds__(64)
 void k_cf_faces(KParams kp, int limiter, const CfFace* __restrict__ jobs, int njobs, const BoxDev* __restrict__ cboxes,
                 const double* __restrict__ Uc, const BoxDev* __restrict__ fboxes, const double* __restrict__ Uf,
                 double* __restrict__ faces, int* err)
 {
-    constexpr int N = 4 + DIM;
+    constexpr int N = (NP > 0) ? 2 * NP + DIM + 1 : 4 + DIM;
     const int j = (int)(blockIdx.x * 64 + threadIdx.x);
     if (j >= njobs) return;
     const CfFace fj = jobs[j];
@@ -1286,9 +1292,13 @@ void k_cf_faces(KParams kp, int limiter, const CfFace* __restrict__ jobs, int nj
     const double* Ub = (fj.fine ? Uf : Uc) + bx.off;
     int bad = 0;
     double F[N], u;
-    if (fj.dir == 0) u = face_at<DIM, SCHEME, RIEMANN, THINC, 0>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
-    else if (fj.dir == 1) u = face_at<DIM, SCHEME, RIEMANN, THINC, 1>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
-    else u = face_at<DIM, SCHEME, RIEMANN, THINC, (DIM == 3 ? 2 : 1)>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
+    if (fj.dir == 0)
+        u = face_at<DIM, SCHEME, RIEMANN, THINC, 0, NP>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
+    else if (fj.dir == 1)
+        u = face_at<DIM, SCHEME, RIEMANN, THINC, 1, NP>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], F, bad);
+    else
+        u = face_at<DIM, SCHEME, RIEMANN, THINC, (DIM == 3 ? 2 : 1), NP>(kp, limiter, bx, Ub, fj.c[0], fj.c[1],
+                                                                          fj.c[2], F, bad);
     double* o = faces + (long long)j * (N + 1);
 #pragma unroll
     for (int q = 0; q < N; ++q) o[q] = F[q];
@@ -1297,13 +1307,13 @@ void k_cf_faces(KParams kp, int limiter, const CfFace* __restrict__ jobs, int nj
 }
 
 // k_cf_apply: one thread per listed coarse cell, its face jobs combined in their fixed order.
-template <int DIM>
+template <int DIM, int NP>
 __global__ __launch_bounds__(64)
 void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev* __restrict__ cboxes,
                 const double* __restrict__ Uc, const double* __restrict__ faces, int ratio, double* Uout,
                 const double* __restrict__ dtp, int stage, int* err)
 {
-    constexpr int N = 4 + DIM;
+    constexpr int N = (NP > 0) ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
     const int e = (int)(blockIdx.x * 64 + threadIdx.x);
     if (e >= n) return;
@@ -1342,7 +1352,15 @@ void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev
     double Ucell[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) Ucell[q] = Uc[c + q * cb.sn];
-    dL[IA] += nc_coefficient<DIM>(kp, Ucell, bad) * ddiv;
+    if constexpr (NP > 0) {
+        // every alpha_k takes phi_k (1 + K_k) times the change of div u*
+        double f[NP > 0 ? NP : 1];
+        np_nc_coefficients<DIM, (NP > 0 ? NP : 2)>(kp, Ucell, f, bad);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) dL[NpIdx<DIM, (NP > 0 ? NP : 2)>::A + k] += f[k] * ddiv;
+    } else {
+        dL[IA] += nc_coefficient<DIM>(kp, Ucell, bad) * ddiv;
+    }
     const double dt = *dtp;
     const double w = (stage == 0) ? dt : ((stage == 1) ? 0.25 * dt : (2.0 / 3.0) * dt);
 #pragma unroll
@@ -1354,18 +1372,18 @@ void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev
     if (bad) atomicMax(err, bad);
 }
 
-template <int DIM, int SCHEME, int RIEMANN, bool THINC>
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP = 0>
 static hipError_t cfcorr4(const KParams& kp, int limiter, int part, const CfCell* list, int n, const CfFace* jobs,
                           int njobs, double* faces, const BoxDev* cboxes, const double* Uc, const BoxDev* fboxes,
                           const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
                           hipStream_t s)
 {
     if (part & 1)
-        hipLaunchKernelGGL((k_cf_faces<DIM, SCHEME, RIEMANN, THINC>), dim3((njobs + 63) / 64), dim3(64), 0, s, kp,
-                           limiter, jobs, njobs, cboxes, Uc, fboxes, Uf, faces, err);
+        hipLaunchKernelGGL((k_cf_faces<DIM, SCHEME, RIEMANN, THINC, NP>), dim3((njobs + 63) / 64), dim3(64), 0, s,
+                           kp, limiter, jobs, njobs, cboxes, Uc, fboxes, Uf, faces, err);
     if (part & 2)
-        hipLaunchKernelGGL((k_cf_apply<DIM>), dim3((n + 63) / 64), dim3(64), 0, s, kp, list, n, cboxes, Uc, faces,
-                           ratio, Uout, dt, stage, err);
+        hipLaunchKernelGGL((k_cf_apply<DIM, NP>), dim3((n + 63) / 64), dim3(64), 0, s, kp, list, n, cboxes, Uc,
+                           faces, ratio, Uout, dt, stage, err);
     return hipGetLastError();
 }
 
@@ -1376,9 +1394,23 @@ hipError_t launch_cf_correct_dim(const LaunchCfg& lc, const KParams& kp, int par
                                  const double* dt, int stage, int* err, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    if (lc.nphase || lc.riemann == RIEMANN_PF_MIX || lc.riemann == RIEMANN_PF_PHASIC) return hipErrorInvalidValue;
-    CMP_DISPATCH(cfcorr4, kp, lc.limiter, part, list, n, jobs, njobs, faces, cboxes, Uc, fboxes, Uf, ratio, Uout, dt,
-                 stage, err, s);
+    if (lc.riemann == RIEMANN_PF_MIX || lc.riemann == RIEMANN_PF_PHASIC) return hipErrorInvalidValue;
+#define CMP_CF_ARGS kp, lc.limiter, part, list, n, jobs, njobs, faces, cboxes, Uc, fboxes, Uf, ratio, Uout, dt, stage, \
+                    err, s
+    if (lc.nphase) {
+        // the N-phase model's schemes (Godunov, MUSCL, WENO5 with the consistent limiter), HLLC or HLL
+        if (lc.thinc) return hipErrorInvalidValue;
+#define CMP_CF_NP(SC, RS)                                                                                  \
+        if (lc.scheme == SC && lc.riemann == RS) {                                                        \
+            if (lc.nphase == 3) return cfcorr4<DIM, SC, RS, false, 3>(CMP_CF_ARGS);                       \
+            if (lc.nphase == 2) return cfcorr4<DIM, SC, RS, false, 2>(CMP_CF_ARGS);                       \
+        }
+        CMP_CF_NP(5, 2) CMP_CF_NP(5, 1) CMP_CF_NP(1, 2) CMP_CF_NP(1, 1) CMP_CF_NP(0, 2) CMP_CF_NP(0, 1)
+#undef CMP_CF_NP
+        return hipErrorInvalidValue;
+    }
+    CMP_DISPATCH(cfcorr4, CMP_CF_ARGS);
+#undef CMP_CF_ARGS
     return hipErrorInvalidValue;
 }
 

@@ -331,7 +331,7 @@ int cmp_level_fill_two_levels(cmp_level* fine, double* Uf, const cmp_level* coar
  * faces under a finer level the mean of the fine faces over them (fluxes and u*_n, finest level
  * first, AverageDownFacesTo :1575-1583), takes the divergence and the stage combination, and
  * averages the levels down (AverageDown). ComputeDt over the levels (:2250-2302, nsubsteps 1): the
- * smallest of the levels' estimates, each limited to 1.1 dt_prev. Two-phase flow states; levels
+ * smallest of the levels' estimates, each limited to 1.1 dt_prev. Two-phase or N-phase flow states; levels
  * coarsest first, with the same parameters and ghost width, each the refinement of the previous one
  * by ratio, boxes aligned to ratio; a third level must not touch the boundary of the second. The
  * caller's level buffers stay the caller's; every call takes arrays of nlev device pointers. */

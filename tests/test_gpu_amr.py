@@ -90,7 +90,8 @@ def _hierarchy(dim, nc, fine_boxes, seed=3, third=None, **kw):
     geoms, boxes = list(h["geoms"]), list(h["boxes"])
     fields = list(h["fields"])
     if third is not None:
-        f3 = cases.random_smooth(dim, nc * 4, seed, **kw)
+        f3 = (cases.nphase_smooth(dim, nc * 4, seed, nphase=kw["nphase"]) if kw.get("nphase")
+              else cases.random_smooth(dim, nc * 4, seed))
         geoms.append(f3["geom"])
         boxes.append(third)
         fields.append(f3["U"])
@@ -143,6 +144,8 @@ AMR_CASES = {
     "2d-periodic-faces": (2, 16, [([0, 8], [7, 23]), ([24, 8], [31, 15])], None),
     "2d-three-levels": (2, 16, [([8, 8], [23, 23])], [([24, 24], [39, 39])]),
     "3d-centre": (3, 8, [([4, 4, 4], [11, 11, 11])], None),
+    "2d-nphase3": (2, 16, [([4, 8], [15, 19]), ([16, 4], [27, 23])], None, 3),
+    "2d-nphase2-three-levels": (2, 16, [([8, 8], [23, 23])], [([24, 24], [39, 39])], 2),
 }
 
 
@@ -150,8 +153,8 @@ AMR_CASES = {
 def test_amr_step_matches_oracle(gpu, name):
     import torch
     from compas_amd import Amr
-    dim, nc, fine, third = AMR_CASES[name]
-    P, geoms, lays, bufs = _hierarchy(dim, nc, fine, third=third)
+    dim, nc, fine, third, *np_ = AMR_CASES[name]
+    P, geoms, lays, bufs = _hierarchy(dim, nc, fine, third=third, nphase=(np_[0] if np_ else 0))
     dt = 1.0e-3
     ref = [b.copy() for b in bufs]
     oracle.amr_rk3_step(P, geoms, lays, ref, dt, 2)
@@ -174,12 +177,12 @@ def test_amr_step_matches_oracle(gpu, name):
 
 
 @pytest.mark.parametrize("name,use_graph", [("2d-two-boxes", True), ("2d-three-levels", True),
-                                            ("2d-three-levels", False)])
+                                            ("2d-three-levels", False), ("2d-nphase3", True)])
 def test_amr_advance_matches_oracle_and_conserves(gpu, name, use_graph):
     import torch
     from compas_amd import Amr
-    dim, nc, fine, third = AMR_CASES[name]
-    P, geoms, lays, bufs = _hierarchy(dim, nc, fine, third=third)
+    dim, nc, fine, third, *np_ = AMR_CASES[name]
+    P, geoms, lays, bufs = _hierarchy(dim, nc, fine, third=third, nphase=(np_[0] if np_ else 0))
     nsteps = 20
     ref = [b.copy() for b in bufs]
     hist = oracle.amr_advance(P, geoms, lays, ref, nsteps, 0.4, 2)
@@ -202,5 +205,6 @@ def test_amr_advance_matches_oracle_and_conserves(gpu, name, use_graph):
     _assert_close(lays, got, ref, 1e-10, lambda u: oracle.amr_advance(P, geoms, lays, u, nsteps, 0.4, 2), bufs)
     # composite totals (the coarse level holds the means of the finer ones) conserved
     c0, c1 = _valid_cells(lays[0], bufs[0]), _valid_cells(lays[0], got[0])
-    for n in range(5):
+    ncons = P.nstate - (P.nphase if P.nphase else 1)     # masses, momenta, energy
+    for n in range(ncons):
         assert abs(c1[n].sum() - c0[n].sum()) <= 1e-13 * np.abs(c0[n]).sum(), n
