@@ -37,6 +37,15 @@ Workloads (--workload):
                           subcycling (coarse faces under the fine level take its face means), the
                           hierarchy fixed over the timed steps; beside the uniform 2048x512 grid
                           it stands for, stepped by the single-level path. Flow step only.
+  nphase-rings-amr        the AMR configuration's flow step on its hierarchy: the 512^2 three-phase
+                          rings base level (uneven boxes of 8..64) plus three refined levels
+                          (ratio 2), each clustered from the GPU tags of the level below (every
+                          volume fraction's gradient >= 0.1, |grad p| >= 0.1; blocking factor 8,
+                          boxes of at most 64, nested 4 cells inside their parent level), initial
+                          data of each level from the same seeded rings at its resolution;
+                          WENO5 + HLLC, RK3 at CFL 0.4, levels stepped together without
+                          subcycling, hierarchy fixed over the timed steps; beside the uniform
+                          4096^2 grid the finest level stands for. Flow step only.
 Also reported: the dominant kernel's roofline position (measured with HIP events on the stream it
 runs on) and the oracle (CPU restatement, OpenMP) timed on this host on a bounded sample.
 """
@@ -256,6 +265,114 @@ def shock_bubble_amr(args):
     print(json.dumps(rec), flush=True)
 
 
+def nphase_rings_amr(args):
+    """The nphase-rings-amr workload: one JSON line (see the module docstring)."""
+    import numpy as np
+    import torch
+    from compas_amd import Amr, Layout, Level, cases, regrid, roofline
+    dev = torch.device("cuda:0")
+    crit = [(6, "grad", 0.1), (7, "grad", 0.1), (8, "grad", 0.1), ("P", "grad", 0.1)]
+    c = cases.nphase_rings_2d(512)
+    P = c["params"]
+    levels, lays, Us, geoms = [], [], [], []
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    levels.append(Level(P, c["geom"], lay))
+    lays.append(lay)
+    geoms.append(c["geom"])
+    Us.append(torch.from_numpy(lay.pack(c["U"])).to(dev))
+    n = 512
+    tagged = []
+    for lev in range(3):
+        U = Us[lev]
+        if lev == 0:
+            levels[0].FillPatch(U)
+        else:
+            levels[lev].FillPatchTwoLevels(U, levels[lev - 1], Us[lev - 1], 2)
+        # boxes of the next level in this level's cells (blocking factor 8 and 64-cell boxes at the
+        # next level: 4 and 32 here), nested inside this level's own boxes for the finer levels
+        nb, tags = regrid.regrid(levels[lev], U, crit, (n, n), ratio=1, blocking_factor=4, max_grid_size=32)
+        tagged.append(int(tags.sum()))
+        if lev > 0:
+            nb = regrid.nest(nb, lays[lev].boxes, (n, n), blocking_factor=4, margin=4)
+        nb = regrid.refine(nb, 2)
+        n *= 2
+        cf = cases.nphase_rings_2d(n, boxes=nb)
+        lay = Layout(2, nb, 3, P.nstate)
+        levels.append(Level(P, cf["geom"], lay))
+        lays.append(lay)
+        geoms.append(cf["geom"])
+        Us.append(torch.from_numpy(lay.pack(cf["U"])).to(dev))
+        del cf
+    for lev in range(len(levels) - 2, -1, -1):
+        levels[lev + 1].AverageDownTo(Us[lev + 1], levels[lev], Us[lev], 2)
+    amr = Amr(levels, 2)
+    S1 = [torch.zeros_like(u) for u in Us]
+    S2 = [torch.zeros_like(u) for u in Us]
+    dt = torch.zeros(1, dtype=torch.float64, device=dev)
+    cmax = torch.zeros(len(levels), dtype=torch.float64, device=dev)
+    cells = [sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in la.boxes) for la in lays]
+    K = args.steps if args.steps is not None else 50
+    W = args.warmup
+    amr.advance(Us, S1, S2, W, 0.4, dt, cmax)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    amr.advance(Us, S1, S2, K, 0.4, dt, cmax)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    assert amr.check_errors() == 0
+    amr.close()
+    for lv in levels:
+        lv.close()
+    del Us, S1, S2
+    torch.cuda.empty_cache()
+    # the uniform finest grid, single-level path (graph-replayed steps)
+    cu = cases.nphase_rings_2d(n, boxes=cases.split_domain((n, n), 64))
+    lu = Layout(2, cu["boxes"], 3, P.nstate)
+    Lu = Level(P, cu["geom"], lu)
+    Uu = torch.from_numpy(lu.pack(cu["U"])).to(dev)
+    del cu
+    Vu1, Vu2 = torch.zeros_like(Uu), torch.zeros_like(Uu)
+    du = torch.zeros(1, dtype=torch.float64, device=dev)
+    cmu = torch.zeros(1, dtype=torch.float64, device=dev)
+    Ku = max(2, K // 5)
+    Lu.advance(Uu, Vu1, Vu2, 2, 0.4, du, cmu, use_graph=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    Lu.advance(Uu, Vu1, Vu2, Ku, 0.4, du, cmu, use_graph=True, resume=True)
+    torch.cuda.synchronize()
+    wall_u = time.perf_counter() - t1
+    assert Lu.check_errors() == 0
+    Lu.close()
+    total = sum(cells)
+    fps = roofline.flops_per_cell_stage(P)
+    achieved = 3.0 * fps * total * K / wall / 1e12
+    rec = {
+        "metric": "cell-updates/s (5Eq WENO5-HLLC RK3), % roofline; p50/p99 case latency",
+        "value": total * K / wall,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded three-phase converging rings, hierarchy from the GPU tagging)",
+        "config": {"workload": "nphase-rings-amr", "base_grid": [512, 512], "levels": len(levels), "ratio": 2,
+                   "boxes_per_level": [la.nbox for la in lays], "cells_per_level": cells,
+                   "tagged_cells": tagged, "nphase": 3, "scheme": "WENO5", "riemann": "HLLC",
+                   "integrator": "TVD-RK3 (levels stage-coupled)", "cfl": 0.4, "phase_field": None},
+        "uniform_finest_grid": {"grid": [n, n], "ms_per_step": wall_u / Ku * 1e3,
+                                "cell_updates_per_s": n * n * Ku / wall_u},
+        "amr_step_over_uniform_step": (wall / K) / (wall_u / Ku),
+        "roofline": {"bound": "valu", "kernel": "whole hierarchy step (all kernels)", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": None, "flops_per_cell_stage": fps},
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def served(args):
     """The served-stream workload: one JSON line (see the module docstring)."""
     import numpy as np
@@ -424,6 +541,9 @@ def main():
         return
     if args.workload == "shock-bubble-amr":
         shock_bubble_amr(args)
+        return
+    if args.workload == "nphase-rings-amr":
+        nphase_rings_amr(args)
         return
     if args.copy_calibration_mib > 0:
         n = args.copy_calibration_mib * (1 << 20) // 8

@@ -166,3 +166,53 @@ def regrid(level, U, criteria, shape, ratio=2, blocking_factor=8, max_grid_size=
     level.ErrorEst(U, criteria, tags, stream=stream)
     g = tags_to_global(level.layout, tags.cpu().numpy(), shape)
     return refine(cluster(g, blocking_factor, max_grid_size, grid_eff, n_error_buf), ratio), g
+
+
+def nest(boxes, parent_boxes, shape, blocking_factor=8, margin=4):
+    """Proper nesting: the parts of `boxes` (cells of a level, x first) that lie at least `margin`
+    cells inside the union of `parent_boxes` (the same level's own boxes), kept in whole
+    blocking-factor blocks and re-formed into boxes (block runs along x, stacked where the runs of
+    consecutive block rows agree). A level clustered from a non-base level's tags is nested this
+    way before it is refined, so that its ghost cells' coarse sources (the two-level fill) and the
+    faces of the parent's own coarse-fine boundary stay clear of it."""
+    dim = len(shape)
+    cover = np.zeros(shape, dtype=bool)
+    for lo, hi in parent_boxes:
+        cover[tuple(slice(lo[d], hi[d] + 1) for d in reversed(range(dim)))] = True
+    inner = ~_grow(~cover, margin)
+    bf = blocking_factor
+    nbk = [-(-n // bf) for n in shape]
+    keep = np.zeros(nbk, dtype=bool)
+    for lo, hi in boxes:
+        sl = tuple(slice(lo[d] // bf, hi[d] // bf + 1) for d in reversed(range(dim)))
+        keep[sl] = True
+    # a block stays when all its cells are inner
+    pad = [(0, s * bf - n) for s, n in zip(nbk, shape)]
+    t = np.pad(inner, pad)
+    for ax in range(dim):
+        t = t.reshape(t.shape[:ax] + (nbk[ax], bf) + t.shape[ax + 1:]).all(axis=ax + 1)
+    keep &= t
+    if dim != 2:
+        raise NotImplementedError("nest() forms 2D box lists")
+    out = []
+    open_runs = {}          # (i0, i1) -> j0 of the run stack still growing
+    for j in range(nbk[0] + 1):
+        runs = set()
+        if j < nbk[0]:
+            i = 0
+            while i < nbk[1]:
+                if keep[j, i]:
+                    i0 = i
+                    while i < nbk[1] and keep[j, i] and i - i0 < 64 // bf:
+                        i += 1
+                    runs.add((i0, i - 1))
+                else:
+                    i += 1
+        for r in list(open_runs):
+            if r not in runs or j - open_runs[r] >= 64 // bf:
+                j0 = open_runs.pop(r)
+                out.append(([r[0] * bf, j0 * bf], [min((r[1] + 1) * bf, shape[1]) - 1, min(j * bf, shape[0]) - 1]))
+        for r in runs:
+            if r not in open_runs:
+                open_runs[r] = j
+    return out

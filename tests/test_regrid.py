@@ -51,3 +51,29 @@ def test_refine_and_tags_to_global():
     g = np.arange(256, dtype=np.uint8).reshape(16, 16) % 2
     flat = np.concatenate([g[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1].ravel() for lo, hi in lay.boxes])
     assert np.array_equal(regrid.tags_to_global(lay, flat, (16, 16)), g)
+
+
+def test_nest_keeps_whole_blocks_inside_the_parent_with_a_margin():
+    parent = [([32, 32], [95, 95]), ([96, 40], [127, 79])]
+    boxes = [([24, 24], [103, 55]), ([40, 64], [63, 119]), ([96, 48], [119, 71])]
+    out = regrid.nest(boxes, parent, (160, 160), 8, 4)
+    cov = np.zeros((160, 160), bool)
+    for lo, hi in parent:
+        cov[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+    want = np.zeros((160, 160), bool)
+    for lo, hi in boxes:
+        want[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+    m = np.zeros((160, 160), int)
+    for lo, hi in out:
+        assert lo[0] % 8 == 0 and lo[1] % 8 == 0 and (hi[0] + 1) % 8 == 0 and (hi[1] + 1) % 8 == 0
+        assert cov[lo[1] - 4:hi[1] + 5, lo[0] - 4:hi[0] + 5].all(), (lo, hi)
+        m[lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] += 1
+    assert m.max() == 1
+    # nothing requested is dropped that a whole inner block could hold
+    inner = np.zeros((160, 160), bool)
+    for lo, hi in parent:
+        inner[lo[1] + 4:hi[1] - 3, lo[0] + 4:hi[0] - 3] = True
+    blk = (inner & want).reshape(20, 8, 20, 8).all(axis=(1, 3))
+    got = (m > 0).reshape(20, 8, 20, 8).all(axis=(1, 3))
+    assert np.all(got[blk])
+    assert np.all(~(m > 0) | want)
