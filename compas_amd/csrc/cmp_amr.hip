@@ -46,9 +46,11 @@ struct cmp_amr {
     std::vector<CfFace*> d_jobs; // the face jobs of each coarse level's list
     std::vector<int> njobs;
     std::vector<double*> d_faces;   // their results, N + 1 doubles a job
-    // the finer levels' work of a stage runs on a second stream beside level 0's (fork / join)
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_fill0 = nullptr, ev_join = nullptr;
+    // each finer level's work of a stage runs on a stream of its own beside level 0's (fork / join):
+    // q[l - 1] for level l; ev_fill[l] after level l's ghost fill, ev_done[l] after its last work
+    std::vector<hipStream_t> q;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_fill, ev_done;
     AmrDt adt{};
     int* d_step = nullptr;               // step counter and dt-history capacity of an advance call
     // the replayed step of cmp_amr_advance, keyed by its buffers and settings
@@ -167,9 +169,12 @@ void amr_free(cmp_amr* A)
     if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
     if (A->graph) (void)hipGraphDestroy(A->graph);
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
-    if (A->s2) (void)hipStreamDestroy(A->s2);
-    hipEvent_t evs[] = {A->ev_fork, A->ev_fill0, A->ev_join};
-    for (hipEvent_t e : evs)
+    for (hipStream_t x : A->q)
+        if (x) (void)hipStreamDestroy(x);
+    if (A->ev_fork) (void)hipEventDestroy(A->ev_fork);
+    for (hipEvent_t e : A->ev_fill)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : A->ev_done)
         if (e) (void)hipEventDestroy(e);
     delete A;
 }
@@ -234,10 +239,15 @@ int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out)
     int r = check_hip(hipMalloc(&A->d_step, sizeof(int) * 2));
     if (!r) r = check_hip(hipMemset(A->d_step, 0, sizeof(int) * 2));
     if (!r && nlev > 1) {
-        r = check_hip(hipStreamCreateWithFlags(&A->s2, hipStreamNonBlocking));
-        if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_fork, hipEventDisableTiming));
-        if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_fill0, hipEventDisableTiming));
-        if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_join, hipEventDisableTiming));
+        r = check_hip(hipEventCreateWithFlags(&A->ev_fork, hipEventDisableTiming));
+        A->q.assign(nlev - 1, nullptr);
+        A->ev_fill.assign(nlev, nullptr);
+        A->ev_done.assign(nlev, nullptr);
+        for (int l = 0; l < nlev && !r; ++l) {
+            if (l > 0) r = check_hip(hipStreamCreateWithFlags(&A->q[l - 1], hipStreamNonBlocking));
+            if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_fill[l], hipEventDisableTiming));
+            if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_done[l], hipEventDisableTiming));
+        }
     }
     if (r) { amr_free(A); return r; }
     *out = A;
@@ -294,11 +304,12 @@ int cmp_amr_compute_dt(cmp_amr* A, double* cmax, double* dt_dev, double cfl, int
     return CMP_OK;
 }
 
-// One stage's work graph: level 0's fill and stage kernel on the caller's stream; on the second
-// stream, each finer level's two-level fill (it reads the coarser stage state's valid cells, which
-// level 0's fill leaves alone) and stage kernel, then the coarse-fine face jobs (after level 0's
-// fill); back on the caller's stream, the corrections (after level 0's stage kernel and the face
-// jobs) and the average-down (after every stage kernel), finest first.
+// One stage's work graph: level 0's fill and stage kernel on the caller's stream; on a stream per
+// finer level, its two-level fill (it reads the coarser stage state's valid cells, which that
+// level's own fill leaves alone) and stage kernel, then the coarse-fine face jobs between it and the
+// next coarser level (after that level's fill); back on the caller's stream, the corrections (after
+// level 0's stage kernel and every face job) and the average-down (after every stage kernel),
+// finest first.
 int cmp_amr_rk3_step(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
                      double* cmax, void* stream)
 {
@@ -328,23 +339,26 @@ int cmp_amr_rk3_step(cmp_amr* A, double* const* Un, double* const* U1, double* c
             if ((r = stage_of(0, stage, src, dst, s))) return r;
             continue;
         }
-        hipStream_t q = A->s2;
         CMP_TRY(hipEventRecord(A->ev_fork, s));
-        CMP_TRY(hipStreamWaitEvent(q, A->ev_fork, 0));
+        for (int l = 1; l < nlev; ++l) CMP_TRY(hipStreamWaitEvent(A->q[l - 1], A->ev_fork, 0));
         // level 0
         if ((r = cmp_level_fill_ghosts(A->L[0], src[0], s))) return r;
-        CMP_TRY(hipEventRecord(A->ev_fill0, s));
+        CMP_TRY(hipEventRecord(A->ev_fill[0], s));
         if ((r = stage_of(0, stage, src, dst, s))) return r;
-        // the finer levels, then the face jobs
+        // each finer level: fill, stage kernel, then the face jobs with the next coarser level
         for (int l = 1; l < nlev; ++l) {
+            hipStream_t q = A->q[l - 1];
             if ((r = cmp_level_fill_two_levels(A->L[l], src[l], A->L[l - 1], src[l - 1], A->ratio, q))) return r;
-            if ((r = stage_of(l, stage, src, dst, q))) return r;
+            CMP_TRY(hipEventRecord(A->ev_fill[l], q));
         }
-        CMP_TRY(hipStreamWaitEvent(q, A->ev_fill0, 0));
-        for (int l = 0; l + 1 < nlev; ++l)
-            if ((r = cf(l, 1, stage, src, dst, q))) return r;
-        CMP_TRY(hipEventRecord(A->ev_join, q));
-        CMP_TRY(hipStreamWaitEvent(s, A->ev_join, 0));
+        for (int l = 1; l < nlev; ++l) {
+            hipStream_t q = A->q[l - 1];
+            if ((r = stage_of(l, stage, src, dst, q))) return r;
+            CMP_TRY(hipStreamWaitEvent(q, A->ev_fill[l - 1], 0));
+            if ((r = cf(l - 1, 1, stage, src, dst, q))) return r;
+            CMP_TRY(hipEventRecord(A->ev_done[l], q));
+            CMP_TRY(hipStreamWaitEvent(s, A->ev_done[l], 0));
+        }
         for (int l = 0; l + 1 < nlev; ++l)
             if ((r = cf(l, 2, stage, src, dst, s))) return r;
         for (int l = nlev - 2; l >= 0; --l)
