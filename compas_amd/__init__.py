@@ -308,6 +308,12 @@ class Level:
         check(self.lib.cmp_level_fill_two_levels(self.h, _ptr(Uf), coarse.h, _ptr(Uc), int(ratio),
                                                  _stream(stream)), "cmp_level_fill_two_levels")
 
+    # RemakeLevel / MakeNewLevelFromCoarse: every cell of this (new) level from the old level's boxes
+    # (old None: none) where they hold it, else the coarse level's interpolation
+    def FillFrom(self, U, old, Uold, coarse, Uc, ratio, stream=None):
+        check(self.lib.cmp_level_fill_from(self.h, _ptr(U), old.h if old is not None else None, _ptr(Uold),
+                                           coarse.h, _ptr(Uc), int(ratio), _stream(stream)), "cmp_level_fill_from")
+
     # ErrorEst: tags (torch uint8, one byte per valid cell, box by box) from a ghost-filled U
     def ErrorEst(self, U, criteria, tags, stream=None):
         arr = tag_criteria(criteria)

@@ -61,7 +61,8 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 }
 
 template <int DIM>
-hipError_t launch_fill2_dim(int, const FillDev&, const FillDev&, int, double*, const double*, int*, hipStream_t);
+hipError_t launch_fill2_dim(int, const FillDev&, const FillDev*, const FillDev&, int, double*, const double*,
+                            const double*, long long, int, int*, hipStream_t);
 template <int DIM>
 hipError_t launch_avgdown_dim(const BoxDev*, int, long long, const FillDev&, const double*, double*, int, int, double,
                               int*, hipStream_t);
@@ -1397,10 +1398,9 @@ int cmp_level_average_down(const cmp_level* F, const double* Uf, cmp_level* C, d
     return CMP_OK;
 }
 
-int cmp_level_fill_two_levels(cmp_level* F, double* Uf, const cmp_level* C, const double* Uc, int ratio,
-                              void* stream)
+// The checks shared by the two-level fills: C's domain refined by ratio is F's, with the same BCs.
+static int two_level_args(const cmp_level* F, const cmp_level* C, int ratio)
 {
-    if (!F || !C || !Uf || !Uc || ratio < 2) return CMP_ERR_ARG;
     const int dim = F->P.dim;
     if (C->P.dim != dim || C->P.nstate != F->P.nstate) return CMP_ERR_ARG;
     const int N = F->P.nstate;
@@ -1413,8 +1413,43 @@ int cmp_level_fill_two_levels(cmp_level* F, double* Uf, const cmp_level* C, cons
             return CMP_ERR_LAYOUT;
         if (F->ng > F->G.domain_hi[d] - F->G.domain_lo[d] + 1) return CMP_ERR_LAYOUT;
     }
-    CMP_TRY((dim == 2 ? launch_fill2_dim<2> : launch_fill2_dim<3>)(N, F->fd, C->fd, ratio, Uf, Uc, F->d_err,
-                                                                    (hipStream_t)stream));
+    return CMP_OK;
+}
+
+int cmp_level_fill_two_levels(cmp_level* F, double* Uf, const cmp_level* C, const double* Uc, int ratio,
+                              void* stream)
+{
+    if (!F || !C || !Uf || !Uc || ratio < 2) return CMP_ERR_ARG;
+    int r = two_level_args(F, C, ratio);
+    if (r) return r;
+    // the same-level sources are the level's own boxes
+    CMP_TRY((F->P.dim == 2 ? launch_fill2_dim<2> : launch_fill2_dim<3>)(F->P.nstate, F->fd, &F->fd, C->fd, ratio, Uf,
+                                                                        Uf, Uc, 0, 0, F->d_err, (hipStream_t)stream));
+    return CMP_OK;
+}
+
+int cmp_level_fill_from(cmp_level* F, double* Uf, const cmp_level* O, const double* Uo, const cmp_level* C,
+                        const double* Uc, int ratio, void* stream)
+{
+    if (!F || !C || !Uf || !Uc || ratio < 2 || (O && !Uo)) return CMP_ERR_ARG;
+    int r = two_level_args(F, C, ratio);
+    if (r) return r;
+    const int dim = F->P.dim;
+    if (O) {
+        if (O->P.dim != dim || O->P.nstate != F->P.nstate) return CMP_ERR_ARG;
+        for (int d = 0; d < dim; ++d)
+            if (O->G.domain_lo[d] != F->G.domain_lo[d] || O->G.domain_hi[d] != F->G.domain_hi[d]) return CMP_ERR_LAYOUT;
+        if (Uo == Uf) return CMP_ERR_ARG;
+    }
+    long long maxc = 0;
+    for (const cmp_box& b : F->boxes) {
+        long long v = 1;
+        for (int d = 0; d < dim; ++d) v *= b.hi[d] - b.lo[d] + 1 + 2 * F->ng;
+        maxc = std::max(maxc, v);
+    }
+    CMP_TRY((dim == 2 ? launch_fill2_dim<2> : launch_fill2_dim<3>)(F->P.nstate, F->fd, O ? &O->fd : nullptr, C->fd,
+                                                                   ratio, Uf, O ? Uo : Uf, Uc, maxc, 1, F->d_err,
+                                                                   (hipStream_t)stream));
     return CMP_OK;
 }
 

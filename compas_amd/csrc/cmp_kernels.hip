@@ -1189,9 +1189,13 @@ __device__ __forceinline__ int coarse_value(const FillDev& cd, const double* __r
 // cell's centre offset from the coarse cell's in coarse cells, (i - ic r + 1/2)/r - 1/2. Products
 // and sums are not contracted to FMAs, so the result is bit for bit the one of plain IEEE
 // arithmetic in this order.
-template <int DIM, int N>
+// ALL: every cell of the boxes of fd (valid and ghost), the same-level sources taken from the boxes
+// of sfd (has_src 0: none) in Us - the data of a level's new boxes after a regrid; otherwise the
+// ghost cells only, sfd = fd and Us = Uf.
+template <int DIM, int N, bool ALL>
 __global__ __launch_bounds__(NT)
-void k_fill2(FillDev fd, FillDev cd, int box0, int ratio, double* Uf, const double* __restrict__ Uc, int* err)
+void k_fill2(FillDev fd, FillDev sfd, int has_src, FillDev cd, int box0, int ratio, double* Uf,
+             const double* Us, const double* __restrict__ Uc, int* err)
 {
 #pragma clang fp contract(off)
     constexpr int MOM = (N == 4 + DIM) ? 2 : (N - DIM - 1) / 2;
@@ -1202,23 +1206,29 @@ void k_fill2(FillDev fd, FillDev cd, int box0, int ratio, double* Uf, const doub
     const int nv2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
     const int ex = nv0 + 2 * ng, ey = nv1 + 2 * ng;
     const int ez = (DIM == 3) ? nv2 + 2 * ng : 1;
-    const int nghost = ex * ey * ez - nv0 * nv1 * nv2;
-    for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < nghost; g += (int)(gridDim.x * NT)) {
+    const int ncell = ALL ? ex * ey * ez : ex * ey * ez - nv0 * nv1 * nv2;
+    for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < ncell; g += (int)(gridDim.x * NT)) {
         int x, y, z;
-        ghost_xyz<DIM>(g, ng, ex, ey, ez, nv1, x, y, z);
+        if (ALL) {
+            x = g % ex;
+            y = (g / ex) % ey;
+            z = g / (ex * ey);
+        } else {
+            ghost_xyz<DIM>(g, ng, ex, ey, ez, nv1, x, y, z);
+        }
         const int cell[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
         const long long di = bx.off + fab_index(bx, cell[0], cell[1], cell[2]);
         int src[3] = {cell[0], cell[1], cell[2]};
         unsigned flip;
         int e = map_cell<DIM>(fd, src, flip);
         if (e) { atomicMax(err, e); continue; }
-        const int own = cell_owner<DIM>(fd, src);
+        const int own = has_src ? cell_owner<DIM>(sfd, src) : -1;
         double v[N];
         if (own >= 0) {
-            const BoxDev so = fd.boxes[own];
+            const BoxDev so = sfd.boxes[own];
             const long long si = so.off + fab_index(so, src[0], src[1], src[2]);
 #pragma unroll
-            for (int n = 0; n < N; ++n) v[n] = Uf[si + n * so.sn];
+            for (int n = 0; n < N; ++n) v[n] = Us[si + n * so.sn];
         } else {
             int ic[3] = {src[0], src[1], src[2]};
 #pragma unroll
@@ -1415,22 +1425,28 @@ hipError_t launch_cf_correct_dim(const LaunchCfg& lc, const KParams& kp, int par
 }
 
 template <int DIM>
-hipError_t launch_fill2_dim(int N, const FillDev& fd, const FillDev& cd, int ratio, double* Uf, const double* Uc,
-                            int* err, hipStream_t s)
+hipError_t launch_fill2_dim(int N, const FillDev& fd, const FillDev* sfd, const FillDev& cd, int ratio, double* Uf,
+                            const double* Us, const double* Uc, long long maxcells, int all, int* err, hipStream_t s)
 {
     if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM) return hipErrorInvalidValue;
-    long long cols = (fd.maxghost + NT - 1) / NT;
+    long long cols = ((all ? maxcells : fd.maxghost) + NT - 1) / NT;
     if (cols <= 0) return hipSuccess;
     if (cols > 65536) cols = 65536;
+    // sfd null: no same-level source (every cell interpolated from the coarse level)
+    const FillDev& sf = sfd ? *sfd : fd;
+    const int has = sfd ? 1 : 0;
     for (int box0 = 0; box0 < fd.nbox; box0 += 65535) {
         const int rows = (fd.nbox - box0 < 65535) ? fd.nbox - box0 : 65535;
         const dim3 grid((unsigned)cols, (unsigned)rows);
-        if (N == 4 + DIM)
-            hipLaunchKernelGGL((k_fill2<DIM, 4 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, Uf, Uc, err);
-        else if (N == 5 + DIM)
-            hipLaunchKernelGGL((k_fill2<DIM, 5 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, Uf, Uc, err);
-        else
-            hipLaunchKernelGGL((k_fill2<DIM, 7 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, Uf, Uc, err);
+#define CMP_F2(NN)                                                                                              \
+        if (all) hipLaunchKernelGGL((k_fill2<DIM, NN, true>), grid, dim3(NT), 0, s, fd, sf, has, cd, box0, ratio, Uf, \
+                                    Us, Uc, err);                                                               \
+        else hipLaunchKernelGGL((k_fill2<DIM, NN, false>), grid, dim3(NT), 0, s, fd, sf, has, cd, box0, ratio, Uf,    \
+                                Us, Uc, err);
+        if (N == 4 + DIM) { CMP_F2(4 + DIM) }
+        else if (N == 5 + DIM) { CMP_F2(5 + DIM) }
+        else { CMP_F2(7 + DIM) }
+#undef CMP_F2
     }
     return hipGetLastError();
 }
@@ -1503,8 +1519,9 @@ template hipError_t launch_cf_correct_dim<CMP_KERNEL_DIM>(const LaunchCfg&, cons
                                                          const CfFace*, int, double*, const BoxDev*,
                                                          const double*, const BoxDev*, const double*, int,
                                                          double*, const double*, int, int*, hipStream_t);
-template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev&, int, double*,
-                                                     const double*, int*, hipStream_t);
+template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev*, const FillDev&, int,
+                                                     double*, const double*, const double*, long long, int, int*,
+                                                     hipStream_t);
 template hipError_t launch_avgdown_dim<CMP_KERNEL_DIM>(const BoxDev*, int, long long, const FillDev&, const double*,
                                                        double*, int, int, double, int*, hipStream_t);
 template hipError_t launch_tag_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev*, int, long long,

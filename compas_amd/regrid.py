@@ -216,3 +216,17 @@ def nest(boxes, parent_boxes, shape, blocking_factor=8, margin=4):
             if r not in open_runs:
                 open_runs[r] = j
     return out
+
+
+def remake_level(params, geom, new_boxes, old_level, U_old, coarse, Uc, ratio=2, ng=3):
+    """A level on new boxes after a regrid (RemakeLevel; MakeNewLevelFromCoarse when old_level is
+    None): a new Level, its Layout and its state (every cell from the old level's boxes where they
+    hold it, else interpolated from the coarse level, cmp_level_fill_from). Returns
+    (level, layout, U)."""
+    import torch
+    from . import Layout, Level
+    lay = Layout(params.dim, new_boxes, ng, params.nstate)
+    lev = Level(params, geom, lay)
+    U = torch.zeros(lay.total, dtype=torch.float64, device=Uc.device)
+    lev.FillFrom(U, old_level, U_old, coarse, Uc, ratio)
+    return lev, lay, U

@@ -322,6 +322,15 @@ int cmp_level_average_down(const cmp_level* fine, const double* Uf, cmp_level* c
 int cmp_level_fill_two_levels(cmp_level* fine, double* Uf, const cmp_level* coarse, const double* Uc, int ratio,
                               void* stream);
 
+/* A level's data after a regrid (RemakeLevel / MakeNewLevelFromCoarse, source/COMPAS.cpp: the
+ * FillPatch of the new MultiFab): every cell of the new level's boxes, valid and ghost, is mapped
+ * to its source cell as the ghost fill maps it; a source inside a box of the old level (old, Uold;
+ * NULL for a level made from the coarse level alone) is copied, any other is interpolated from the
+ * coarse level as cmp_level_fill_two_levels interpolates. The old level covers the same domain;
+ * U and Uold are distinct device buffers. */
+int cmp_level_fill_from(cmp_level* level, double* U, const cmp_level* old, const double* Uold,
+                        const cmp_level* coarse, const double* Uc, int ratio, void* stream);
+
 /* ---- a hierarchy of levels stepped together -------------------------------------------------
  * The reference's time step without subcycling (run.do_subcycle = 0, run.do_reflux = 1):
  * timeStepNoSubcycling (source/COMPAS.cpp:2139-2180) -> AdvanceAllLevelsStageCoupled

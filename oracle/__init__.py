@@ -96,6 +96,8 @@ def load():
                                                   ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_fill_two_levels": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, PG, ctypes.c_int,
                                                      PB, ctypes.c_int, PD, ctypes.c_int]),
+        "ref_level_fill_from": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, PB,
+                                               ctypes.c_int, PD, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int]),
         "ref_amr_rk3_step": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
                                             ctypes.POINTER(PD), D, ctypes.c_int, PD]),
         "ref_amr_compute_dt": (D, [ctypes.c_int, PG, PD, D, ctypes.c_int, D]),
@@ -471,6 +473,21 @@ def level_fill_two_levels(params, fgeom, flayout, fbuf, cgeom, clayout, cbuf, ra
                                           clayout.ng, _dptr(cbuf), int(ratio))
     if rc:
         raise RuntimeError("ref_level_fill_two_levels failed: %d" % rc)
+
+
+def level_fill_from(params, fgeom, new_layout, new_buf, old_layout, old_buf, cgeom, clayout, cbuf, ratio):
+    """In place on new_buf: every cell of the new boxes from the old boxes of the level (old_layout
+    None: none), else by interpolation from the coarse level."""
+    from compas_amd.capi import cmp_box
+    nbo = old_layout.nbox if old_layout is not None else 0
+    ob = old_layout.cboxes if old_layout is not None else (cmp_box * 1)()
+    ngo = old_layout.ng if old_layout is not None else 0
+    obuf = old_buf if old_buf is not None else np.zeros(1)
+    rc = load().ref_level_fill_from(ctypes.byref(params), ctypes.byref(fgeom), new_layout.nbox, new_layout.cboxes,
+                                    new_layout.ng, _dptr(new_buf), nbo, ob, ngo, _dptr(obuf), ctypes.byref(cgeom),
+                                    clayout.nbox, clayout.cboxes, clayout.ng, _dptr(cbuf), int(ratio))
+    if rc:
+        raise RuntimeError("ref_level_fill_from failed: %d" % rc)
 
 
 def level_tag(params, layout, buf, criteria):

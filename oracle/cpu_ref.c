@@ -2676,9 +2676,13 @@ static int coarse_cell(int dim, int N, int mom0, const cmp_geometry* Gc, int nbc
 
 static int floor_div(int a, int r) { return (a >= 0) ? a / r : -((-a + r - 1) / r); }
 
-int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
-                              double* Uf, const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc,
-                              const double* Uc, int ratio)
+/* The two-level fill of the cells of the boxes dboxes (buffer Ud; all_cells: valid and ghost cells,
+ * else the ghost cells only), the same-level sources taken from the boxes sboxes (buffer Us) of
+ * the same level. */
+static int fill_two_levels_impl(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
+                                double* Uf, int nbs, const cmp_box* sboxes, int ngs, const double* Us, int all_cells,
+                                const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc, const double* Uc,
+                                int ratio)
 {
     const int dim = P->dim, N = P->nstate, mom0 = P->nphase ? P->nphase : 2;
     int err = 0;
@@ -2690,17 +2694,17 @@ int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int n
             for (int j = f.lo[1]; j <= f.lo[1] + f.ext[1] - 1; ++j)
                 for (int i = f.lo[0]; i <= f.lo[0] + f.ext[0] - 1; ++i) {
                     int c[3] = {i, j, k};
-                    if (box_contains(dim, bx, c)) continue;
+                    if (!all_cells && box_contains(dim, bx, c)) continue;
                     int src[3] = {i, j, k};
                     double sgn[REF_NMAX], v[REF_NMAX];
                     int e = map_source(dim, Gf, N, mom0, src, sgn);
                     if (e) { if (e > err) err = e; continue; }
                     const long long di = fab_idx(&f, i, j, k);
-                    int q = find_box(dim, nbf, fboxes, src);
+                    int q = find_box(dim, nbs, sboxes, src);
                     if (q >= 0) {
-                        fab_t fs = make_fab(dim, &fboxes[q], ngf);
-                        const long long si = fboxes[q].offset + fab_idx(&fs, src[0], src[1], src[2]);
-                        for (int n = 0; n < N; ++n) base[di + n * f.sn] = sgn[n] * Uf[si + n * fs.sn];
+                        fab_t fs = make_fab(dim, &sboxes[q], ngs);
+                        const long long si = sboxes[q].offset + fab_idx(&fs, src[0], src[1], src[2]);
+                        for (int n = 0; n < N; ++n) base[di + n * f.sn] = sgn[n] * Us[si + n * fs.sn];
                         continue;
                     }
                     /* coarse-fine ghost: lincc at the source cell */
@@ -2741,6 +2745,24 @@ int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int n
                 }
     }
     return err;
+}
+
+int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
+                              double* Uf, const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc,
+                              const double* Uc, int ratio)
+{
+    return fill_two_levels_impl(P, Gf, nbf, fboxes, ngf, Uf, nbf, fboxes, ngf, Uf, 0, Gc, nbc, cboxes, ngc, Uc, ratio);
+}
+
+/* A level's new boxes after a regrid (RemakeLevel / MakeNewLevelFromCoarse, source/COMPAS.cpp,
+ * FillPatch of the new MultiFab): every cell of the new boxes, valid and ghost, from the old
+ * boxes of the level where one holds its source cell, else from the coarse level by the same
+ * interpolation as the two-level fill. nbo = 0: a level made from the coarse level alone. */
+int ref_level_fill_from(const cmp_params* P, const cmp_geometry* Gf, int nbn, const cmp_box* nboxes, int ngn,
+                        double* Un, int nbo, const cmp_box* oboxes, int ngo, const double* Uo,
+                        const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc, const double* Uc, int ratio)
+{
+    return fill_two_levels_impl(P, Gf, nbn, nboxes, ngn, Un, nbo, oboxes, ngo, Uo, 1, Gc, nbc, cboxes, ngc, Uc, ratio);
 }
 
 /* ------------------------------------------------------------------------------------------------

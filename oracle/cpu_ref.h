@@ -159,6 +159,11 @@ int ref_level_average_down(const cmp_params* P, int nbf, const cmp_box* fboxes, 
 int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
                               double* Uf, const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc,
                               const double* Uc, int ratio);
+/* Every cell (valid and ghost) of a level's new boxes from its old boxes, else from the coarse
+ * level (the regrid's FillPatch of a new level). */
+int ref_level_fill_from(const cmp_params* P, const cmp_geometry* Gf, int nbn, const cmp_box* nboxes, int ngn,
+                        double* Un, int nbo, const cmp_box* oboxes, int ngo, const double* Uo,
+                        const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc, const double* Uc, int ratio);
 /* All levels of a hierarchy, stage by stage without subcycling (stage-coupled, do_reflux = 1),
  * then averaged down; levels coarsest first, ratio between consecutive levels, ng for all. */
 int ref_amr_rk3_step(const cmp_params* P, int nlev, const cmp_geometry* G, const int* nbox,

@@ -208,3 +208,79 @@ def test_amr_advance_matches_oracle_and_conserves(gpu, name, use_graph):
     ncons = P.nstate - (P.nphase if P.nphase else 1)     # masses, momenta, energy
     for n in range(ncons):
         assert abs(c1[n].sum() - c0[n].sum()) <= 1e-13 * np.abs(c0[n]).sum(), n
+
+
+# ---- a regridded level's data (cmp_level_fill_from) --------------------------------------------------
+
+REMAKE = {2: ([([8, 0], [23, 15]), ([24, 16], [31, 31])], [([12, 4], [27, 19]), ([0, 24], [15, 31])]),
+          3: ([([0, 4, 0], [7, 11, 7]), ([8, 4, 0], [15, 11, 7])], [([4, 0, 4], [11, 7, 11]), ([4, 8, 0], [11, 15, 7])])}
+
+
+@pytest.mark.parametrize("dim,nphase,with_old", [(2, 0, True), (2, 0, False), (3, 0, True), (2, 3, True)])
+def test_fill_from_matches_oracle(gpu, dim, nphase, with_old):
+    import torch
+    nc = 8 if dim == 3 else 16
+    if nphase:
+        P = make_params(dim, gamma=(1.4, 1.67, 1.09)[:nphase], nphase=nphase)
+    else:
+        P = make_params(dim, gamma=(1.4, 1.6))
+    bc = BCS[dim]
+    cg = make_geometry((nc,) * dim, (0.0,) * dim, (1.0,) * dim, bc["per"], lo_bc=bc["lo_bc"], hi_bc=bc["hi_bc"])
+    fg = make_geometry((nc * 2,) * dim, (0.0,) * dim, (1.0,) * dim, bc["per"], lo_bc=bc["lo_bc"], hi_bc=bc["hi_bc"])
+    cl = Layout(dim, split_domain((nc,) * dim, max(4, nc // 2)), 3, P.nstate)
+    new_boxes, old_boxes = REMAKE[dim]
+    nl = Layout(dim, new_boxes, 3, P.nstate)
+    ol = Layout(dim, old_boxes, 3, P.nstate)
+    rng = np.random.default_rng(21 + dim + nphase)
+    cbuf = rng.uniform(0.2, 2.0, cl.total)
+    obuf = rng.uniform(0.2, 2.0, ol.total)
+    ref = np.zeros(nl.total)
+    oracle.level_fill_from(P, fg, nl, ref, ol if with_old else None, obuf if with_old else None, cg, cl, cbuf, 2)
+    Un = torch.zeros(nl.total, dtype=torch.float64, device=gpu)
+    Uo = torch.from_numpy(obuf).to(gpu)
+    Uc = torch.from_numpy(cbuf).to(gpu)
+    ln, lo_, lc = Level(P, fg, nl), Level(P, fg, ol), Level(P, cg, cl)
+    ln.FillFrom(Un, lo_ if with_old else None, Uo if with_old else None, lc, Uc, 2)
+    assert ln.check_errors() == 0
+    got = Un.cpu().numpy()
+    for lv in (ln, lo_, lc):
+        lv.close()
+    assert np.array_equal(got, ref)
+
+
+def test_regrid_cycle_keeps_the_hierarchy_conservative(gpu):
+    """Step a two-level hierarchy, move the fine level to new boxes (remake_level), rebuild the
+    hierarchy and step on: the composite totals stay put across the regrid where the new fine
+    level covers what the old one did plus coarse-only cells (its new cells interpolate the coarse
+    level conservatively, its old cells keep their data)."""
+    import torch
+    from compas_amd import Amr, regrid
+    P, geoms, lays, bufs = _hierarchy(2, 16, [([8, 8], [23, 23])])
+    levels = [Level(P, g, lay) for g, lay in zip(geoms, lays)]
+    Un = [torch.from_numpy(b).to(gpu) for b in bufs]
+    dtd = torch.zeros(1, dtype=torch.float64, device=gpu)
+    cmax = torch.zeros(2, dtype=torch.float64, device=gpu)
+
+    def run(levels, Un, n):
+        amr = Amr(levels, 2)
+        S1 = [torch.zeros_like(u) for u in Un]
+        S2 = [torch.zeros_like(u) for u in Un]
+        amr.advance(Un, S1, S2, n, 0.4, dtd, cmax)
+        assert amr.check_errors() == 0
+        amr.close()
+
+    c0 = _valid_cells(lays[0], Un[0].cpu().numpy())
+    run(levels, Un, 5)
+    new_fine, new_lay, Unew = regrid.remake_level(P, geoms[1], [([8, 4], [27, 23])], levels[1], Un[1], levels[0],
+                                                  Un[0], 2)
+    assert new_fine.check_errors() == 0
+    new_fine.AverageDownTo(Unew, levels[0], Un[0], 2)
+    levels[1].close()
+    levels, Un = [levels[0], new_fine], [Un[0], Unew]
+    run(levels, Un, 5)
+    c1 = _valid_cells(lays[0], Un[0].cpu().numpy())
+    for lv in levels:
+        lv.close()
+    for n in range(5):
+        assert abs(c1[n].sum() - c0[n].sum()) <= 1e-13 * np.abs(c0[n]).sum(), n
+    assert np.abs(c1 - c0).max() > 1e-6

@@ -249,3 +249,61 @@ def test_amr_compute_dt_is_the_smallest_level_estimate():
     assert oracle.amr_compute_dt([g0, g1], cm, 0.4, 1, 0.0) == min(est)
     assert oracle.amr_compute_dt([g0, g1], cm, 0.4, 1, 0.005) == min(min(est), 1.1 * 0.005)
     assert oracle.amr_compute_dt([g0, g1], cm, 0.4, 0, 0.005) == min(est)
+
+
+# ---- a regridded level's data: old boxes where they hold the cell, else the coarse interpolation --
+
+def test_fill_from_the_same_boxes_keeps_the_level_and_fills_its_ghosts():
+    P, cg, fg, cl, fl = _two_levels()
+    rng = np.random.default_rng(9)
+    cbuf = rng.uniform(0.5, 2.0, cl.total)
+    old = rng.uniform(0.5, 2.0, fl.total)
+    ref = old.copy()
+    oracle.level_fill_two_levels(P, fg, fl, ref, cg, cl, cbuf, 2)
+    new = np.zeros(fl.total)
+    oracle.level_fill_from(P, fg, fl, new, fl, old, cg, cl, cbuf, 2)
+    assert np.array_equal(new, ref)
+
+
+def test_fill_from_the_coarse_level_alone_is_conservative_and_linear_exact():
+    P, cg, fg, cl, fl = _two_levels()
+    lin = lambda n, X: 1.0 + n + (0.3 + 0.1 * n) * (X[0] + 0.5) + (0.2 - 0.05 * n) * (X[1] + 0.5)
+    cbuf = _coarse_fill(cl, lin)
+    new = np.zeros(fl.total)
+    oracle.level_fill_from(P, fg, fl, new, None, None, cg, cl, cbuf, 2)
+    v = fl.fab_view(new, 0)
+    g = fl.ng
+    for jj in range(v.shape[1]):
+        for ii in range(v.shape[2]):
+            i, j = 8 + ii - g, 8 + jj - g
+            if not (0 <= j <= 31):
+                continue                             # slip walls mirror there
+            exact = 1.0 + (0.3) * (i + 0.5) / 2 + 0.2 * (j + 0.5) / 2
+            assert abs(v[0, jj, ii] - exact) < 1e-13
+    rng = np.random.default_rng(2)
+    cbuf = _coarse_fill(cl, lambda n, X: rng.uniform(0.5, 2.0, X[0].shape))
+    oracle.level_fill_from(P, fg, fl, new, None, None, cg, cl, cbuf, 2)
+    back = cbuf.copy()
+    oracle.level_average_down(P, fl, new, cl, back, 2)
+    assert np.allclose(back, cbuf, rtol=1e-14, atol=0)
+
+
+def test_fill_from_partly_overlapping_old_boxes_copies_where_they_hold_the_cell():
+    P, cg, fg, cl, fl = _two_levels()
+    old_l = Layout(2, [([12, 12], [19, 27])], 3, P.nstate)
+    rng = np.random.default_rng(5)
+    cbuf = rng.uniform(0.5, 2.0, cl.total)
+    old = rng.uniform(0.5, 2.0, old_l.total)
+    new = np.zeros(fl.total)
+    oracle.level_fill_from(P, fg, fl, new, old_l, old, cg, cl, cbuf, 2)
+    alone = np.zeros(fl.total)
+    oracle.level_fill_from(P, fg, fl, alone, None, None, cg, cl, cbuf, 2)
+    v, a, o = fl.fab_view(new, 0), fl.fab_view(alone, 0), old_l.fab_view(old, 0)
+    g = fl.ng
+    for jj in range(v.shape[1]):
+        for ii in range(v.shape[2]):
+            i, j = 8 - g + ii, 8 - g + jj
+            if 12 <= i <= 19 and 12 <= j <= 27:
+                assert np.array_equal(v[:, jj, ii], o[:, j - 12 + 3, i - 12 + 3])
+            elif 0 <= j <= 31:
+                assert np.array_equal(v[:, jj, ii], a[:, jj, ii])
