@@ -297,6 +297,11 @@ class Level:
                                          int(use_graph), int(bool(resume)), _stream(stream)),
               "cmp_level_advance")
 
+    # one step of run.TimeIntegrator: 1 ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, 4 RK4 (U3 for RK4 only)
+    def rk_step(self, integrator, Un, U1, U2, dt_dev, cmax, U3=None, stream=None):
+        check(self.lib.cmp_level_rk_step(self.h, int(integrator), _ptr(Un), _ptr(U1), _ptr(U2), _ptr(U3),
+                                         _ptr(dt_dev), _ptr(cmax), _stream(stream)), "cmp_level_rk_step")
+
     # AverageDownTo: this (fine) level's Uf onto the coarse level's Uc under the fine boxes
     def AverageDownTo(self, Uf, coarse, Uc, ratio, stream=None):
         check(self.lib.cmp_level_average_down(self.h, _ptr(Uf), coarse.h, _ptr(Uc), int(ratio), _stream(stream)),

@@ -805,6 +805,58 @@ int cmp_level_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, const d
     return CMP_OK;
 }
 
+// ForwardEuler, TVD-RK2 and RK4 (run.TimeIntegrator): per stage the filled stage state, its
+// right-hand side into a level buffer, then the reference's LinComb / Saxpy sequence.
+int cmp_level_rk_step(cmp_level* L, int integrator, double* Un, double* U1, double* U2, double* U3,
+                      const double* dt_dev, double* cmax, void* stream)
+{
+    if (integrator == 3) return cmp_level_rk3_step(L, Un, U1, U2, dt_dev, cmax, stream);
+    if (!L || !Un || !U1 || !U2 || !dt_dev || !cmax) return CMP_ERR_ARG;
+    if (integrator != 1 && integrator != 2 && integrator != 4) return CMP_ERR_ARG;
+    if (integrator == 4 && !U3) return CMP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    const int N = L->P.nstate;
+    long long maxc = 0;
+    for (const cmp_box& b : L->boxes) {
+        long long v = 1;
+        for (int d = 0; d < L->P.dim; ++d) v *= b.hi[d] - b.lo[d] + 1;
+        maxc = std::max(maxc, v);
+    }
+    // U1: the stage state (U^(1) of TVD-RK2, U_temp of RK4); U2: the right-hand side; U3: RK4's U^(n+1)
+    double* Lb = U2;
+    auto rhs = [&](double* src) -> int {
+        int r = cmp_level_fill_ghosts(L, src, stream);
+        if (r) return r;
+        CMP_TRY(launch_stage(L->lc, L->kp, MODE_RHS, 0, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src, nullptr,
+                             Lb, nullptr, (unsigned long long*)cmax, L->d_err, s));
+        return CMP_OK;
+    };
+    auto lc = [&](double* out, double p, const double* X, double q, const double* Y, double r, const double* Lr) -> int {
+        CMP_TRY(launch_lincomb(L->d_boxes, L->nbox, maxc, N, out, p, X, q, Y, r, Lr, dt_dev, s));
+        return CMP_OK;
+    };
+    int r;
+    if (integrator == 1) {
+        if ((r = rhs(Un)) || (r = lc(Un, 1.0, Un, 0.0, nullptr, 1.0, Lb))) return r;
+        return CMP_OK;
+    }
+    if (integrator == 2) {
+        if ((r = rhs(Un)) || (r = lc(U1, 1.0, Un, 0.0, nullptr, 1.0, Lb))) return r;
+        if ((r = rhs(U1)) || (r = lc(Un, 0.5, U1, 0.5, Un, 0.5, Lb))) return r;
+        return CMP_OK;
+    }
+    double* Ut = U1;
+    double* Un1 = U3;
+    if ((r = rhs(Un)) || (r = lc(Ut, 1.0, Un, 0.0, nullptr, 0.5, Lb)) || (r = lc(Un1, 1.0, Un, 0.0, nullptr, 1.0 / 6.0, Lb)))
+        return r;
+    if ((r = rhs(Ut)) || (r = lc(Ut, 1.0, Un, 0.0, nullptr, 0.5, Lb)) || (r = lc(Un1, 1.0, Un1, 0.0, nullptr, 1.0 / 3.0, Lb)))
+        return r;
+    if ((r = rhs(Ut)) || (r = lc(Ut, 1.0, Un, 0.0, nullptr, 1.0, Lb)) || (r = lc(Un1, 1.0, Un1, 0.0, nullptr, 1.0 / 3.0, Lb)))
+        return r;
+    if ((r = rhs(Ut)) || (r = lc(Un, 1.0, Un1, 0.0, nullptr, 1.0 / 6.0, Lb))) return r;
+    return CMP_OK;
+}
+
 int cmp_level_initial_cmax(cmp_level* L, double* U, double* cmax, void* stream)
 {
     if (!L || !U || !cmax) return CMP_ERR_ARG;

@@ -856,6 +856,55 @@ hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, i
     return hipGetLastError();
 }
 
+// The stage combinations of the integrators other than TVD-RK3 (AMReX LinComb / Saxpy on the valid
+// cells of a level): out = (p X + q Y) + (r dt) L, a term present where its array is given (no Y:
+// p X + (r dt) L; no L: p X + q Y), in that order and without FMA contraction, as the reference's
+// LinComb (dst = a x + b y) then Saxpy (dst += a y) round. One thread per valid cell and component
+// of one box per grid row.
+__global__ __launch_bounds__(NT)
+void k_lincomb(const BoxDev* __restrict__ boxes, int N, double* out, double p, const double* X, double q,
+               const double* Y, double r, const double* L, const double* __restrict__ dtp)
+{
+#pragma clang fp contract(off)
+    const BoxDev bx = boxes[blockIdx.y];
+    const int n0 = bx.hi[0] - bx.lo[0] + 1, n1 = bx.hi[1] - bx.lo[1] + 1, n2 = bx.hi[2] - bx.lo[2] + 1;
+    const long long ncell = (long long)n0 * n1 * n2;
+    const double rdt = L ? r * (*dtp) : 0.0;
+    for (long long t = (long long)blockIdx.x * NT + threadIdx.x; t < ncell * N; t += (long long)gridDim.x * NT) {
+        const int c = (int)(t / ncell);
+        const long long v = t - (long long)c * ncell;
+        const int i = bx.lo[0] + (int)(v % n0);
+        const int j = bx.lo[1] + (int)((v / n0) % n1);
+        const int k = bx.lo[2] + (int)(v / ((long long)n0 * n1));
+        const long long o = bx.off + fab_index(bx, i, j, k) + c * bx.sn;
+        double w;
+        if (Y && L) {
+            w = p * X[o] + q * Y[o];
+            w = w + rdt * L[o];
+        } else if (Y) {
+            w = p * X[o] + q * Y[o];
+        } else {
+            w = p * X[o] + rdt * L[o];
+        }
+        out[o] = w;
+    }
+}
+
+hipError_t launch_lincomb(const BoxDev* boxes, int nbox, long long maxcells, int N, double* out, double p,
+                          const double* X, double q, const double* Y, double r, const double* L, const double* dt,
+                          hipStream_t s)
+{
+    long long cols = (maxcells * N + NT - 1) / NT;
+    if (cols <= 0 || nbox <= 0) return hipSuccess;
+    if (cols > 4096) cols = 4096;
+    for (int b0 = 0; b0 < nbox; b0 += 65535) {
+        const int rows = (nbox - b0 < 65535) ? nbox - b0 : 65535;
+        hipLaunchKernelGGL(k_lincomb, dim3((unsigned)cols, (unsigned)rows), dim3(NT), 0, s, boxes + b0, N, out, p, X,
+                           q, Y, r, L, dt);
+    }
+    return hipGetLastError();
+}
+
 // Sets two ints of a device table from kernel arguments (keeps graph-capturable code free of
 // memset nodes and of host-to-device copies).
 __global__ void k_set2(int* p, int a, int b)

@@ -154,6 +154,10 @@ hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, int part, c
                              int njobs, double* faces, const BoxDev* cboxes, const double* Uc, const BoxDev* fboxes,
                              const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
                              hipStream_t s);
+// out = (p X + q Y) + (r dt) L on the valid cells of a level (Y, L optional), without contraction.
+hipError_t launch_lincomb(const BoxDev* boxes, int nbox, long long maxcells, int N, double* out, double p,
+                          const double* X, double q, const double* Y, double r, const double* L, const double* dt,
+                          hipStream_t s);
 // ComputeDt over the levels of a hierarchy (no subcycling); resets every level's c_max. stepc:
 // device step counter and dt-history capacity.
 hipError_t launch_amr_dt(const AmrDt& a, double* cmax, double* dt, double cfl, int limiter, double* hist, int* stepc,

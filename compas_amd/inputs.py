@@ -10,7 +10,7 @@ reference's defaults where a key is absent:
 - prob.EOS.EOS, gamma_k / pinf_k / b_k / q_k, or the per-phase lists gamma / pinf / b / q of the
   N-phase model (exec/*/prob/Parm.H; coefficients source/Parm.cpp:122-170, 230-262)
 - geometry.prob_lo / prob_hi / is_periodic, amr.n_cell / max_grid_size / max_level,
-  run.lo_bc / hi_bc / cfl / timestep_change_limiter, max_step, stop_time
+  run.lo_bc / hi_bc / cfl / timestep_change_limiter / TimeIntegrator, max_step, stop_time
   (source/COMPAS.cpp:1186-1546)
 - PhaseField.* (include/Physics_PhaseField/PhaseField_Parameter.H:29-140)
 
@@ -28,6 +28,8 @@ FV_SCHEMES = {
     "WENO3": (1, 3, 2, 2), "WENO3B": (1, 4, 2, 2), "WENO5": (1, 5, 4, 3), "WENO5B": (1, 6, 4, 3),
 }
 RIEMANN = {"LLF": 0, "HLL": 1, "HLLC": 2}
+# run.TimeIntegrator (source/COMPAS.cpp:1306-1311) -> cmp_level_rk_step's integrator
+TIME_INTEGRATORS = {"ForwardEuler": 1, "TVD-RK2": 2, "TVD-RK3": 3, "RK4": 4}
 
 
 def parse(text):
@@ -132,7 +134,11 @@ def case_parameters(t, dim, nphase=0):
     geom = make_geometry(tuple(n_cell), tuple(lo), tuple(hi), tuple(per), lo_bc=lo_bc, hi_bc=hi_bc)
     # time stepping, grids
     # (defaults: include/COMPAS.H:508-613; AMReX's max_grid_size default, 32 in 3D and 128 in 2D)
+    ti = _get(t, "run.TimeIntegrator", "TVD-RK3", str)
+    if ti not in TIME_INTEGRATORS:
+        raise ValueError("run.TimeIntegrator = %r is not an integrator of the reference" % ti)
     run = dict(cfl=_get(t, "run.cfl", 0.4), timestep_change_limiter=_get(t, "run.timestep_change_limiter", 1, int),
+               time_integrator=TIME_INTEGRATORS[ti],
                max_step=_get(t, "max_step", None, int), stop_time=_get(t, "stop_time", None),
                max_grid_size=_get(t, "amr.max_grid_size", 32 if dim == 3 else 128, int),
                max_level=_get(t, "amr.max_level", 0, int), ghost_cells=ng, n_cell=n_cell)

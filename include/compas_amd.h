@@ -215,6 +215,15 @@ int cmp_level_advance(cmp_level* level, double* Un, double* U1, double* U2, int 
                       double cfl, int limiter, double* dt_dev, double* cmax, double* dt_hist,
                       int use_graph, int resume, void* stream);
 
+/* One step of run.TimeIntegrator (source/COMPAS.cpp:1306-1311): integrator 1 ForwardEuler
+ * (source/Class_Compressible_PhaseField.cpp:3713-3800), 2 TVD-RK2 (SecondOrderSSPRK, :3805-3935),
+ * 3 TVD-RK3 (cmp_level_rk3_step), 4 RK4 (FourthOrderRK, :4111-4330): per stage the stage state's
+ * ghost cells filled, its right-hand side, and the reference's LinComb / Saxpy combinations in its
+ * order. In place on Un; U1, U2 scratch level buffers; U3 a fourth one for RK4 (else unused). The
+ * stages' c_max accumulate into *cmax. */
+int cmp_level_rk_step(cmp_level* level, int integrator, double* Un, double* U1, double* U2, double* U3,
+                      const double* dt_dev, double* cmax, void* stream);
+
 /* Reads and clears the level's device error word (CMP_ERR_NAN, CMP_ERR_COVOLUME or CMP_OK).
  * Synchronises the stream. */
 int cmp_level_check(cmp_level* level, void* stream);

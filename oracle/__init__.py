@@ -98,6 +98,7 @@ def load():
                                                      PB, ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_fill_from": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, PB,
                                                ctypes.c_int, PD, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int]),
+        "ref_level_rk_step": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, D, PD, ctypes.c_int]),
         "ref_amr_rk3_step": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
                                             ctypes.POINTER(PD), D, ctypes.c_int, PD]),
         "ref_amr_compute_dt": (D, [ctypes.c_int, PG, PD, D, ctypes.c_int, D]),
@@ -473,6 +474,17 @@ def level_fill_two_levels(params, fgeom, flayout, fbuf, cgeom, clayout, cbuf, ra
                                           clayout.ng, _dptr(cbuf), int(ratio))
     if rc:
         raise RuntimeError("ref_level_fill_two_levels failed: %d" % rc)
+
+
+def level_rk_step(params, geom, layout, buf, dt, integrator):
+    """One step of size dt in place on buf: integrator 1 ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, 4 RK4.
+    Returns the c_max of the step's stages."""
+    cm = np.zeros(1)
+    rc = load().ref_level_rk_step(ctypes.byref(params), ctypes.byref(geom), layout.nbox, layout.cboxes, layout.ng,
+                                  _dptr(buf), float(dt), _dptr(cm), int(integrator))
+    if rc:
+        raise RuntimeError("ref_level_rk_step failed: %d" % rc)
+    return cm[0]
 
 
 def level_fill_from(params, fgeom, new_layout, new_buf, old_layout, old_buf, cgeom, clayout, cbuf, ratio):

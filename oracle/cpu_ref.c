@@ -1277,6 +1277,81 @@ static int rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox, const 
     return err;
 }
 
+/* The other per-level integrators of run.TimeIntegrator (source/COMPAS.cpp:1306-1311):
+ * ForwardEuler (source/Class_Compressible_PhaseField.cpp:3713-3800): U^{n+1} = 1 Ub(U^n) + dt L;
+ * TVD-RK2 / SecondOrderSSPRK (:3805-3935): U1 = 1 Ub(U^n) + dt L1, U^{n+1} = 0.5 Ub(U1) + 0.5 U^n,
+ *   += 0.5 dt L2;
+ * RK4 / FourthOrderRK (:4111-4330): Ut = 1 Ub(U^n) + 0.5 dt L1, Un1 = 1 Ub(U^n) + (1/6) dt L1;
+ *   Ut = 1 U^n + 0.5 dt L2(Ub(Ut)), Un1 += (1/3) dt L2; Ut = 1 U^n + 1 dt L3, Un1 += (1/3) dt L3;
+ *   Un1 += (1/6) dt L4.
+ * (LinComb: dst = a x + b y; Saxpy: dst += a y; every stage's Ub is the filled stage state.)
+ * integrator: 1 ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, 4 RK4. */
+int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
+                      double* Un, double dt, double* cmax, int integrator)
+{
+    if (integrator == 3) return rk3_step(P, G, nbox, boxes, ng, Un, dt, cmax, NULL);
+    if (integrator != 1 && integrator != 2 && integrator != 4) return CMP_ERR_ARG;
+    const int dim = P->dim, N = P->nstate;
+    long long total = 0;
+    for (int b = 0; b < nbox; ++b) total += cmp_fab_size(dim, &boxes[b], ng, N);
+    double* Ub = (double*)malloc(sizeof(double) * total);
+    double* L = (double*)malloc(sizeof(double) * total);
+    double* Ut = (double*)malloc(sizeof(double) * total);
+    double* Un1 = (double*)malloc(sizeof(double) * total);
+    int err = 0, e;
+    const int nst = (integrator == 4) ? 4 : integrator;
+    for (int st = 0; st < nst; ++st) {
+        const double* src = (st == 0) ? Un : ((integrator == 2) ? Un1 : Ut);
+        memcpy(Ub, src, sizeof(double) * total);
+        e = ref_level_fill_ghosts(P, G, nbox, boxes, ng, Ub); if (e && !err) err = e;
+        e = ref_level_rhs(P, G, nbox, boxes, ng, Ub, L, cmax); if (e && !err) err = e;
+        for (int b = 0; b < nbox; ++b) {
+            fab_t f = make_fab(dim, &boxes[b], ng);
+            int nv[3] = {1, 1, 1};
+            for (int d = 0; d < dim; ++d) nv[d] = boxes[b].hi[d] - boxes[b].lo[d] + 1;
+            const int lo2 = (dim == 3) ? boxes[b].lo[2] : 0;
+            for (int n = 0; n < N; ++n)
+                for (int kk = 0; kk < nv[2]; ++kk)
+                    for (int jj = 0; jj < nv[1]; ++jj)
+                        for (int ii = 0; ii < nv[0]; ++ii) {
+                            const long long q = boxes[b].offset + fab_idx(&f, boxes[b].lo[0] + ii, boxes[b].lo[1] + jj, lo2 + kk) + n * f.sn;
+                            if (integrator == 1 || (integrator == 2 && st == 0)) {
+                                Un1[q] = 1.0 * Ub[q] + dt * L[q];
+                            } else if (integrator == 2) {
+                                Un1[q] = 0.5 * Ub[q] + 0.5 * Un[q];
+                                Un1[q] += (0.5 * dt) * L[q];
+                            } else if (st == 0) {
+                                Ut[q] = 1.0 * Ub[q] + (0.5 * dt) * L[q];
+                                Un1[q] = 1.0 * Ub[q] + ((1.0 / 6.0) * dt) * L[q];
+                            } else if (st == 1) {
+                                Ut[q] = 1.0 * Un[q] + (0.5 * dt) * L[q];
+                                Un1[q] += ((1.0 / 3.0) * dt) * L[q];
+                            } else if (st == 2) {
+                                Ut[q] = 1.0 * Un[q] + (1.0 * dt) * L[q];
+                                Un1[q] += ((1.0 / 3.0) * dt) * L[q];
+                            } else {
+                                Un1[q] += ((1.0 / 6.0) * dt) * L[q];
+                            }
+                        }
+        }
+    }
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &boxes[b], ng);
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = boxes[b].hi[d] - boxes[b].lo[d] + 1;
+        const int lo2 = (dim == 3) ? boxes[b].lo[2] : 0;
+        for (int n = 0; n < N; ++n)
+            for (int kk = 0; kk < nv[2]; ++kk)
+                for (int jj = 0; jj < nv[1]; ++jj)
+                    for (int ii = 0; ii < nv[0]; ++ii) {
+                        const long long q = boxes[b].offset + fab_idx(&f, boxes[b].lo[0] + ii, boxes[b].lo[1] + jj, lo2 + kk) + n * f.sn;
+                        Un[q] = Un1[q];
+                    }
+    }
+    free(Ub); free(L); free(Ut); free(Un1);
+    return err;
+}
+
 int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
                        const cmp_box* boxes, int ng, double* Un, double dt, double* cmax)
 {

@@ -80,6 +80,9 @@ int ref_level_rhs(const cmp_params* P, const cmp_geometry* G, int nbox, const cm
                   int ng, const double* U, double* dUdt, double* cmax);
 int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
                        const cmp_box* boxes, int ng, double* Un, double dt, double* cmax);
+/* One step of run.TimeIntegrator: 1 ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, 4 RK4. */
+int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
+                      double* Un, double dt, double* cmax, int integrator);
 int ref_level_initial_cmax(const cmp_params* P, const cmp_geometry* G, int nbox,
                            const cmp_box* boxes, int ng, const double* U, double* cmax);
 double ref_est_dt(const cmp_geometry* G, double cmax, double cfl);

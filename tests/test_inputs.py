@@ -92,3 +92,14 @@ def test_phase_field_keys():
     # the default mechanism is CAC-Adv with the implicit RC
     c = inputs.case_parameters(inputs.parse("amr.n_cell = 8 8\nPhaseField.phase_field = 1\n"), 2)
     assert c["pf"].mechanism == 1 and c["pf"].explicit_rc == 0
+
+
+def test_time_integrator_key():
+    from compas_amd import inputs
+    base = "amr.n_cell = 16 16\n"
+    assert inputs.case_parameters(inputs.parse(base), 2)["run"]["time_integrator"] == 3
+    for name, code in (("ForwardEuler", 1), ("TVD-RK2", 2), ("TVD-RK3", 3), ("RK4", 4)):
+        t = inputs.parse(base + 'run.TimeIntegrator = "%s"\n' % name)
+        assert inputs.case_parameters(t, 2)["run"]["time_integrator"] == code
+    with pytest.raises(ValueError):
+        inputs.case_parameters(inputs.parse(base + "run.TimeIntegrator = RK5\n"), 2)
