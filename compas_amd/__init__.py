@@ -302,6 +302,11 @@ class Level:
         check(self.lib.cmp_level_rk_step(self.h, int(integrator), _ptr(Un), _ptr(U1), _ptr(U2), _ptr(U3),
                                          _ptr(dt_dev), _ptr(cmax), _stream(stream)), "cmp_level_rk_step")
 
+    # one Phase-Field sub-step of pf_dt at a fixed Grad_Q with run.TimeIntegrator
+    def pf_rk_step(self, integrator, Un, U1, U2, gradq, pf_dt, U3=None, stream=None):
+        check(self.lib.cmp_level_pf_rk_step(self.h, int(integrator), _ptr(Un), _ptr(U1), _ptr(U2), _ptr(U3),
+                                            _ptr(gradq), float(pf_dt), _stream(stream)), "cmp_level_pf_rk_step")
+
     # AverageDownTo: this (fine) level's Uf onto the coarse level's Uc under the fine boxes
     def AverageDownTo(self, Uf, coarse, Uc, ratio, stream=None):
         check(self.lib.cmp_level_average_down(self.h, _ptr(Uf), coarse.h, _ptr(Uc), int(ratio), _stream(stream)),

@@ -148,6 +148,7 @@ SIGNATURES = {
     "cmp_level_average_down": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
     "cmp_level_fill_two_levels": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
     "cmp_level_rk_step": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp, dp, dp, dp, vp]),
+    "cmp_level_pf_rk_step": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_double, vp]),
     "cmp_level_fill_from": (ctypes.c_int, [vp, dp, vp, dp, vp, dp, ctypes.c_int, vp]),
     "cmp_amr_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
     "cmp_amr_destroy": (ctypes.c_int, [vp]),

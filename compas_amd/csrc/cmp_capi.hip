@@ -807,11 +807,10 @@ int cmp_level_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, const d
 
 // ForwardEuler, TVD-RK2 and RK4 (run.TimeIntegrator): per stage the filled stage state, its
 // right-hand side into a level buffer, then the reference's LinComb / Saxpy sequence.
-int cmp_level_rk_step(cmp_level* L, int integrator, double* Un, double* U1, double* U2, double* U3,
-                      const double* dt_dev, double* cmax, void* stream)
+// gradq non-null: the Phase-Field right-hand side at that Grad_Q (no c_max).
+static int rk_step_impl(cmp_level* L, int integrator, double* Un, double* U1, double* U2, double* U3,
+                        const double* dt_dev, double* cmax, const double* gradq, void* stream)
 {
-    if (integrator == 3) return cmp_level_rk3_step(L, Un, U1, U2, dt_dev, cmax, stream);
-    if (!L || !Un || !U1 || !U2 || !dt_dev || !cmax) return CMP_ERR_ARG;
     if (integrator != 1 && integrator != 2 && integrator != 4) return CMP_ERR_ARG;
     if (integrator == 4 && !U3) return CMP_ERR_ARG;
     hipStream_t s = (hipStream_t)stream;
@@ -827,8 +826,12 @@ int cmp_level_rk_step(cmp_level* L, int integrator, double* Un, double* U1, doub
     auto rhs = [&](double* src) -> int {
         int r = cmp_level_fill_ghosts(L, src, stream);
         if (r) return r;
-        CMP_TRY(launch_stage(L->lc, L->kp, MODE_RHS, 0, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src, nullptr,
-                             Lb, nullptr, (unsigned long long*)cmax, L->d_err, s));
+        if (gradq)
+            CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RHS, 0, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src,
+                                 nullptr, Lb, nullptr, nullptr, L->d_err, s, gradq));
+        else
+            CMP_TRY(launch_stage(L->lc, L->kp, MODE_RHS, 0, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src,
+                                 nullptr, Lb, nullptr, (unsigned long long*)cmax, L->d_err, s));
         return CMP_OK;
     };
     auto lc = [&](double* out, double p, const double* X, double q, const double* Y, double r, const double* Lr) -> int {
@@ -855,6 +858,14 @@ int cmp_level_rk_step(cmp_level* L, int integrator, double* Un, double* U1, doub
         return r;
     if ((r = rhs(Ut)) || (r = lc(Un, 1.0, Un1, 0.0, nullptr, 1.0 / 6.0, Lb))) return r;
     return CMP_OK;
+}
+
+int cmp_level_rk_step(cmp_level* L, int integrator, double* Un, double* U1, double* U2, double* U3,
+                      const double* dt_dev, double* cmax, void* stream)
+{
+    if (integrator == 3) return cmp_level_rk3_step(L, Un, U1, U2, dt_dev, cmax, stream);
+    if (!L || !Un || !U1 || !U2 || !dt_dev || !cmax) return CMP_ERR_ARG;
+    return rk_step_impl(L, integrator, Un, U1, U2, U3, dt_dev, cmax, nullptr, stream);
 }
 
 int cmp_level_initial_cmax(cmp_level* L, double* U, double* cmax, void* stream)
@@ -1314,6 +1325,16 @@ int cmp_level_pf_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, cons
     if (!L || !Un || !U1 || !U2 || !gradq) return CMP_ERR_ARG;
     if (!L->pf_on) return CMP_ERR_ARG;
     return pf_rk3(L, Un, U1, U2, gradq, pf_dt, L->d_tiles, L->ntiles, (hipStream_t)stream);
+}
+
+int cmp_level_pf_rk_step(cmp_level* L, int integrator, double* Un, double* U1, double* U2, double* U3,
+                         const double* gradq, double pf_dt, void* stream)
+{
+    if (!L || !Un || !U1 || !U2 || !gradq) return CMP_ERR_ARG;
+    if (!L->pf_on) return CMP_ERR_ARG;
+    if (integrator == 3) return pf_rk3(L, Un, U1, U2, gradq, pf_dt, L->d_tiles, L->ntiles, (hipStream_t)stream);
+    CMP_TRY(launch_setd(L->d_pfdt, pf_dt, (hipStream_t)stream));
+    return rk_step_impl(L, integrator, Un, U1, U2, U3, L->d_pfdt, nullptr, gradq, stream);
 }
 
 // The boxes the Phase-Field sub-steps of one step touch, from the box flags of the last Grad_Q:

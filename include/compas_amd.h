@@ -273,6 +273,13 @@ int cmp_level_pf_solver_iterations(const cmp_level* level, int* iterations);
 int cmp_level_pf_rk3_step(cmp_level* level, double* Un, double* U1, double* U2,
                           const double* gradq, double pf_dt, void* stream);
 
+/* One Phase-Field sub-step of pf_dt at a fixed Grad_Q with run.TimeIntegrator (PhaseField_ForwardEuler
+ * :2426-2505, PhaseField_SecondOrderSSPRK :2508-2617, PhaseField_ThirdOrderSSPRK, PhaseField_FourthOrderRK
+ * :2756-): integrator 1..4 as in cmp_level_rk_step, U3 for RK4. cmp_level_pf_evolve's sub-steps are
+ * the TVD-RK3 ones. */
+int cmp_level_pf_rk_step(cmp_level* level, int integrator, double* Un, double* U1, double* U2, double* U3,
+                         const double* gradq, double pf_dt, void* stream);
+
 /* The Phase-Field step that follows a flow step of size dt (PhaseField_Evolve, :425-594):
  * cmp_level_pf_gradq, then sub-steps of PhaseField_ComputeDt (:2037-2098) size,
  * pf.cfl min_d dx_d / (max|Grad_Q| + 1e-14), the last one cut to end at dt.

@@ -99,6 +99,7 @@ def load():
         "ref_level_fill_from": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, PB,
                                                ctypes.c_int, PD, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_rk_step": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, D, PD, ctypes.c_int]),
+        "ref_level_pf_rk_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, D, ctypes.c_int]),
         "ref_amr_rk3_step": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
                                             ctypes.POINTER(PD), D, ctypes.c_int, PD]),
         "ref_amr_compute_dt": (D, [ctypes.c_int, PG, PD, D, ctypes.c_int, D]),
@@ -485,6 +486,14 @@ def level_rk_step(params, geom, layout, buf, dt, integrator):
     if rc:
         raise RuntimeError("ref_level_rk_step failed: %d" % rc)
     return cm[0]
+
+
+def level_pf_rk_step(params, pf, geom, layout, buf, gradq, pf_dt, integrator):
+    """One Phase-Field sub-step in place on buf at the fixed face buffer gradq."""
+    rc = load().ref_level_pf_rk_step(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom), layout.nbox,
+                                     layout.cboxes, layout.ng, _dptr(buf), _dptr(gradq), float(pf_dt), int(integrator))
+    if rc:
+        raise RuntimeError("ref_level_pf_rk_step failed: %d" % rc)
 
 
 def level_fill_from(params, fgeom, new_layout, new_buf, old_layout, old_buf, cgeom, clayout, cbuf, ratio):

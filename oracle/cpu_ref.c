@@ -1286,10 +1286,10 @@ static int rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox, const 
  *   Un1 += (1/6) dt L4.
  * (LinComb: dst = a x + b y; Saxpy: dst += a y; every stage's Ub is the filled stage state.)
  * integrator: 1 ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, 4 RK4. */
-int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
-                      double* Un, double dt, double* cmax, int integrator)
+static int rk_step_any(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
+                       double* Un, double dt, double* cmax, int integrator, const rk3_rhs_t* R)
 {
-    if (integrator == 3) return rk3_step(P, G, nbox, boxes, ng, Un, dt, cmax, NULL);
+    if (integrator == 3) return rk3_step(P, G, nbox, boxes, ng, Un, dt, cmax, R);
     if (integrator != 1 && integrator != 2 && integrator != 4) return CMP_ERR_ARG;
     const int dim = P->dim, N = P->nstate;
     long long total = 0;
@@ -1304,7 +1304,9 @@ int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, cons
         const double* src = (st == 0) ? Un : ((integrator == 2) ? Un1 : Ut);
         memcpy(Ub, src, sizeof(double) * total);
         e = ref_level_fill_ghosts(P, G, nbox, boxes, ng, Ub); if (e && !err) err = e;
-        e = ref_level_rhs(P, G, nbox, boxes, ng, Ub, L, cmax); if (e && !err) err = e;
+        if (R && R->pf) e = ref_level_pf_rhs(P, R->pf, G, nbox, boxes, ng, Ub, R->gradq, L);
+        else            e = ref_level_rhs(P, G, nbox, boxes, ng, Ub, L, cmax);
+        if (e && !err) err = e;
         for (int b = 0; b < nbox; ++b) {
             fab_t f = make_fab(dim, &boxes[b], ng);
             int nv[3] = {1, 1, 1};
@@ -1350,6 +1352,22 @@ int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, cons
     }
     free(Ub); free(L); free(Ut); free(Un1);
     return err;
+}
+
+int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
+                      double* Un, double dt, double* cmax, int integrator)
+{
+    return rk_step_any(P, G, nbox, boxes, ng, Un, dt, cmax, integrator, NULL);
+}
+
+/* The Phase-Field sub-step with run.TimeIntegrator (PhaseField_ForwardEuler :2426-2505,
+ * PhaseField_SecondOrderSSPRK :2508-2617, PhaseField_ThirdOrderSSPRK, PhaseField_FourthOrderRK
+ * :2756-): the stage combinations above with the Phase-Field right-hand side at a fixed Grad_Q. */
+int ref_level_pf_rk_step(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G, int nbox,
+                         const cmp_box* boxes, int ng, double* Un, const double* gradq, double pf_dt, int integrator)
+{
+    rk3_rhs_t R = {PF, gradq};
+    return rk_step_any(P, G, nbox, boxes, ng, Un, pf_dt, NULL, integrator, &R);
 }
 
 int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,

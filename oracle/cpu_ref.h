@@ -83,6 +83,9 @@ int ref_level_rk3_step(const cmp_params* P, const cmp_geometry* G, int nbox,
 /* One step of run.TimeIntegrator: 1 ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, 4 RK4. */
 int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes, int ng,
                       double* Un, double dt, double* cmax, int integrator);
+/* A Phase-Field sub-step of pf_dt at a fixed Grad_Q with run.TimeIntegrator (1..4 as above). */
+int ref_level_pf_rk_step(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G, int nbox,
+                         const cmp_box* boxes, int ng, double* Un, const double* gradq, double pf_dt, int integrator);
 int ref_level_initial_cmax(const cmp_params* P, const cmp_geometry* G, int nbox,
                            const cmp_box* boxes, int ng, const double* U, double* cmax);
 double ref_est_dt(const cmp_geometry* G, double cmax, double cfl);

@@ -323,3 +323,30 @@ def test_pf_evolve_skips_boxes_without_interface(gpu):
     # and the droplet's boxes did change
     centre = (slice(None), slice(28, 36), slice(28, 36), slice(28, 36))
     assert not np.array_equal(got[centre], init[centre])
+
+
+@pytest.mark.parametrize("integrator", [1, 2, 4])
+def test_pf_rk_step_other_integrators_match_oracle(gpu, integrator):
+    """The Phase-Field sub-step with run.TimeIntegrator = ForwardEuler, TVD-RK2, RK4
+    (cmp_level_pf_rk_step) against the oracle's."""
+    import torch
+    c = _interface_case(32, dim=2, width=2.0, max_grid_size=16)
+    P, G = c["params"], c["geom"]
+    pf = make_pf_params(flux_phase2flux=1)
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    flay = FaceLayout(2, c["boxes"])
+    buf = lay.pack(c["U"])
+    gq, gm, _ = oracle.level_pf_gradq_step(P, pf, G, lay, buf, 0.01, flay.total)
+    pf_dt = 0.4 * G.dx[0] / gm
+    Un = _dev(torch, buf, gpu)
+    oracle.level_pf_rk_step(P, pf, G, lay, buf, gq, pf_dt, integrator)
+    ref = lay.unpack(buf, c["shape"])
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    U1, U2, U3 = (torch.zeros_like(Un) for _ in range(3))
+    lev.pf_rk_step(integrator, Un, U1, U2, _dev(torch, gq, gpu), pf_dt, U3=U3)
+    assert lev.check_errors() == 0
+    got = lay.unpack(Un.cpu().numpy(), c["shape"])
+    lev.close()
+    rel = maxrel_per_var(got, ref)
+    assert rel.max() <= TOL_STEP, rel

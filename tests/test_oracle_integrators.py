@@ -69,3 +69,25 @@ def test_temporal_order(integrator, order):
         errs.append(np.abs(_valid(lay, u, shape) - _valid(lay, ref, shape)).max())
     rate = np.log2(errs[0] / errs[1])
     assert rate > order - 0.3, (errs, rate)
+
+
+def test_pf_sub_step_integrators_compose_the_pf_rhs():
+    """ForwardEuler of the Phase-Field sub-step is U + pf_dt L_pf(U) at the fixed Grad_Q, and RK3
+    through the integrator switch is the Phase-Field RK3 sub-step."""
+    from compas_amd import FaceLayout, make_pf_params
+    P, G, lay, U0, shape = _case()
+    pf = make_pf_params(flux_phase2flux=1)
+    flay = FaceLayout(2, lay.boxes)
+    gq, gm, _ = oracle.level_pf_gradq_step(P, pf, G, lay, U0.copy(), 0.01, flay.total)
+    pf_dt = 0.4 * G.dx[0] / gm
+    b = U0.copy()
+    oracle.level_fill_ghosts(P, G, lay, b)
+    Lpf = oracle.level_pf_rhs(P, pf, G, lay, b, gq)
+    fe = U0 + pf_dt * (Lpf[0] if isinstance(Lpf, tuple) else Lpf)
+    got = U0.copy()
+    oracle.level_pf_rk_step(P, pf, G, lay, got, gq, pf_dt, 1)
+    assert np.allclose(_valid(lay, got, shape), _valid(lay, fe, shape), rtol=1e-14, atol=1e-14)
+    a, c = U0.copy(), U0.copy()
+    oracle.level_pf_rk_step(P, pf, G, lay, a, gq, pf_dt, 3)
+    oracle.level_pf_rk3_step(P, pf, G, lay, c, gq, pf_dt)
+    assert np.array_equal(a, c)
