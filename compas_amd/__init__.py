@@ -297,6 +297,10 @@ class Level:
                                          int(use_graph), int(bool(resume)), _stream(stream)),
               "cmp_level_advance")
 
+    # run.TimeIntegrator of advance() and the Phase-Field sub-steps (1..4)
+    def set_integrator(self, integrator):
+        check(self.lib.cmp_level_set_integrator(self.h, int(integrator)), "cmp_level_set_integrator")
+
     # one step of run.TimeIntegrator: 1 ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, 4 RK4 (U3 for RK4 only)
     def rk_step(self, integrator, Un, U1, U2, dt_dev, cmax, U3=None, stream=None):
         check(self.lib.cmp_level_rk_step(self.h, int(integrator), _ptr(Un), _ptr(U1), _ptr(U2), _ptr(U3),

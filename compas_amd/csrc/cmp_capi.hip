@@ -742,7 +742,7 @@ int cmp_level_destroy(cmp_level* L)
     if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
     if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
-    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff};
+    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff, L->d_U3};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete L;
@@ -896,7 +896,21 @@ static int level_step_with_hist(cmp_level* L, double* Un, double* U1, double* U2
 {
     CMP_TRY(launch_compute_dt(L->P.dim, L->G.dx, cmax, dt_dev, cfl, limiter, hist, L->d_step,
                               L->d_err, s));
+    if (L->integrator != 3) return rk_step_impl(L, L->integrator, Un, U1, U2, L->d_U3, dt_dev, cmax, nullptr, s);
     return cmp_level_rk3_step(L, Un, U1, U2, dt_dev, cmax, s);
+}
+
+int cmp_level_set_integrator(cmp_level* L, int integrator)
+{
+    if (!L || integrator < 1 || integrator > 4) return CMP_ERR_ARG;
+    if (integrator == 4 && !L->d_U3) {
+        long long total = 0;
+        for (const cmp_box& b : L->boxes)
+            total = std::max(total, b.offset + cmp_fab_size(L->P.dim, &b, L->ng, L->P.nstate));
+        CMP_TRY(hipMalloc(&L->d_U3, sizeof(double) * (size_t)std::max(1LL, total)));
+    }
+    L->integrator = integrator;
+    return CMP_OK;
 }
 
 int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nsteps, double cfl,
@@ -929,7 +943,7 @@ int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nste
         CMP_TRY(hipStreamSynchronize(nullptr));
     }
     const void* key[6] = {Un, U1, U2, dt_dev, cmax, dt_hist};
-    bool same = L->gexec && L->gcfl == cfl && L->glimiter == limiter;
+    bool same = L->gexec && L->gcfl == cfl && L->glimiter == limiter && L->ginteg == L->integrator;
     for (int q = 0; q < 6 && same; ++q) same = (L->gkey[q] == key[q]);
     if (!same) {
         if (L->gexec) { CMP_TRY(hipGraphExecDestroy(L->gexec)); L->gexec = nullptr; }
@@ -945,6 +959,7 @@ int cmp_level_advance(cmp_level* L, double* Un, double* U1, double* U2, int nste
         for (int q = 0; q < 6; ++q) L->gkey[q] = key[q];
         L->gcfl = cfl;
         L->glimiter = limiter;
+        L->ginteg = L->integrator;
     }
     for (int n = 0; n < nsteps; ++n) {
         CMP_TRY(hipGraphLaunch(L->gexec, cs));
@@ -1391,7 +1406,9 @@ int cmp_level_pf_evolve(cmp_level* L, double* Un, double* U1, double* U2, const 
         if (pf_time + dt0 > dt - eps) dt0 = dt - pf_time;
         prev = dt0;
         if (L->pf_nfill > 0 &&
-            (r = pf_rk3(L, Un, U1, U2, L->d_gq, dt0, L->d_pf_tiles, L->pf_ntiles, s)))
+            (r = (L->integrator == 3)
+                     ? pf_rk3(L, Un, U1, U2, L->d_gq, dt0, L->d_pf_tiles, L->pf_ntiles, s)
+                     : cmp_level_pf_rk_step(L, L->integrator, Un, U1, U2, L->d_U3, L->d_gq, dt0, s)))
             return r;
         pf_time += dt0;
         ++n;

@@ -104,6 +104,9 @@ struct cmp_level {
     long long tag_maxcells = 0;
     std::vector<int> tile_start;    // first tile of each box, nbox + 1 entries
     std::vector<int> h_owner;
+    int integrator = 3;             // run.TimeIntegrator of cmp_level_advance and the Phase-Field sub-steps
+    double* d_U3 = nullptr;         // RK4's fourth level buffer
+    int ginteg = 0;                 // the integrator the captured step was built with
     long long bs = 1;
     int nb[3] = {1, 1, 1};
 };

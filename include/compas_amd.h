@@ -224,6 +224,11 @@ int cmp_level_advance(cmp_level* level, double* Un, double* U1, double* U2, int 
 int cmp_level_rk_step(cmp_level* level, int integrator, double* Un, double* U1, double* U2, double* U3,
                       const double* dt_dev, double* cmax, void* stream);
 
+/* run.TimeIntegrator of the level's cmp_level_advance steps and cmp_level_pf_evolve sub-steps (1
+ * ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, the default, 4 RK4; RK4 allocates a fourth level buffer
+ * here). */
+int cmp_level_set_integrator(cmp_level* level, int integrator);
+
 /* Reads and clears the level's device error word (CMP_ERR_NAN, CMP_ERR_COVOLUME or CMP_OK).
  * Synchronises the stream. */
 int cmp_level_check(cmp_level* level, void* stream);
@@ -275,8 +280,8 @@ int cmp_level_pf_rk3_step(cmp_level* level, double* Un, double* U1, double* U2,
 
 /* One Phase-Field sub-step of pf_dt at a fixed Grad_Q with run.TimeIntegrator (PhaseField_ForwardEuler
  * :2426-2505, PhaseField_SecondOrderSSPRK :2508-2617, PhaseField_ThirdOrderSSPRK, PhaseField_FourthOrderRK
- * :2756-): integrator 1..4 as in cmp_level_rk_step, U3 for RK4. cmp_level_pf_evolve's sub-steps are
- * the TVD-RK3 ones. */
+ * :2756-): integrator 1..4 as in cmp_level_rk_step, U3 for RK4. cmp_level_pf_evolve's sub-steps use
+ * the level's integrator (cmp_level_set_integrator). */
 int cmp_level_pf_rk_step(cmp_level* level, int integrator, double* Un, double* U1, double* U2, double* U3,
                          const double* gradq, double pf_dt, void* stream);
 

@@ -99,6 +99,10 @@ def load():
         "ref_level_fill_from": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, PB,
                                                ctypes.c_int, PD, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_rk_step": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, D, PD, ctypes.c_int]),
+        "ref_level_advance_int": (ctypes.c_int, [PP, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, D,
+                                                 ctypes.c_int, PD, ctypes.c_int]),
+        "ref_level_advance_pf_int": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, ctypes.c_int, D,
+                                                    ctypes.c_int, PD, ctypes.c_int]),
         "ref_level_pf_rk_step": (ctypes.c_int, [PP, PF, PG, ctypes.c_int, PB, ctypes.c_int, PD, PD, D, ctypes.c_int]),
         "ref_amr_rk3_step": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
                                             ctypes.POINTER(PD), D, ctypes.c_int, PD]),
@@ -494,6 +498,24 @@ def level_pf_rk_step(params, pf, geom, layout, buf, gradq, pf_dt, integrator):
                                      layout.cboxes, layout.ng, _dptr(buf), _dptr(gradq), float(pf_dt), int(integrator))
     if rc:
         raise RuntimeError("ref_level_pf_rk_step failed: %d" % rc)
+
+
+def level_advance_int(params, geom, layout, buf, nsteps, cfl, integrator, pf=None, limiter=1):
+    """nsteps steps (with the Phase-Field step when pf is given) with run.TimeIntegrator; returns
+    the dt history."""
+    hist = np.zeros(max(1, nsteps))
+    lib = load()
+    if pf is None:
+        rc = lib.ref_level_advance_int(ctypes.byref(params), ctypes.byref(geom), layout.nbox, layout.cboxes,
+                                       layout.ng, _dptr(buf), int(nsteps), float(cfl), int(limiter), _dptr(hist),
+                                       int(integrator))
+    else:
+        rc = lib.ref_level_advance_pf_int(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom), layout.nbox,
+                                          layout.cboxes, layout.ng, _dptr(buf), int(nsteps), float(cfl), int(limiter),
+                                          _dptr(hist), int(integrator))
+    if rc:
+        raise RuntimeError("ref_level_advance_int failed: %d" % rc)
+    return hist[:nsteps]
 
 
 def level_fill_from(params, fgeom, new_layout, new_buf, old_layout, old_buf, cgeom, clayout, cbuf, ratio):

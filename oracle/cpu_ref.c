@@ -1420,8 +1420,8 @@ double ref_compute_dt(const cmp_geometry* G, double cmax, double cfl, int limite
 
 /* Evolve's step loop for a single level (source/Class_Compressible_PhaseField.cpp:202-263):
  * ComputeDt from the previous step's c_max (the initial state's at step 0), then one RK3 step. */
-int ref_level_advance(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
-                      int ng, double* U, int nsteps, double cfl, int limiter, double* dt_hist)
+int ref_level_advance_int(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                          int ng, double* U, int nsteps, double cfl, int limiter, double* dt_hist, int integrator)
 {
     double cmax = 0.0, dt = 0.0;
     int err = ref_level_initial_cmax(P, G, nbox, boxes, ng, U, &cmax);
@@ -1430,10 +1430,16 @@ int ref_level_advance(const cmp_params* P, const cmp_geometry* G, int nbox, cons
         dt = ref_compute_dt(G, cmax, cfl, limiter, s == 0 ? ref_est_dt(G, cmax, cfl) : dt);
         if (dt_hist) dt_hist[s] = dt;
         cmax = 0.0;   /* c_max reset at stage 0 (:3161-3164) */
-        err = ref_level_rk3_step(P, G, nbox, boxes, ng, U, dt, &cmax);
+        err = ref_level_rk_step(P, G, nbox, boxes, ng, U, dt, &cmax, integrator);
         if (isnan(cmax)) err = CMP_ERR_NAN;
     }
     return err;
+}
+
+int ref_level_advance(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                      int ng, double* U, int nsteps, double cfl, int limiter, double* dt_hist)
+{
+    return ref_level_advance_int(P, G, nbox, boxes, ng, U, nsteps, cfl, limiter, dt_hist, 3);
 }
 
 /* ------------------------------------------------------------------------------------------------
@@ -2475,8 +2481,8 @@ double ref_pf_compute_dt(const cmp_pf_params* PF, const cmp_geometry* G, double 
 
 /* PhaseField_Evolve (:425-594) after a flow step of size dt, in place on U.
  * info[6] (or NULL): max|u|, mobility, eta, mechanism sub-steps, max|Grad_Q|, Phase-Field sub-steps. */
-int ref_level_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
-                        int nbox, const cmp_box* boxes, int ng, double* U, double dt, double* info)
+static int pf_evolve_int(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                         int nbox, const cmp_box* boxes, int ng, double* U, double dt, double* info, int integrator)
 {
     const int dim = P->dim;
     long long tf;
@@ -2489,7 +2495,7 @@ int ref_level_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, const cmp_
     int n = 0;
     for (int step = 0; step < PF->max_step && pf_time < dt && !err; ++step) {
         double pdt = ref_pf_compute_dt(PF, G, gm, pf_time, dt, &prev);
-        err = ref_level_pf_rk3_step(P, PF, G, nbox, boxes, ng, U, Q, pdt);
+        err = ref_level_pf_rk_step(P, PF, G, nbox, boxes, ng, U, Q, pdt, integrator);
         pf_time += pdt;
         ++n;
         if (pf_time >= dt - 1.0e-6 * dt) break;
@@ -2503,11 +2509,17 @@ int ref_level_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, const cmp_
     return err;
 }
 
+int ref_level_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                        int nbox, const cmp_box* boxes, int ng, double* U, double dt, double* info)
+{
+    return pf_evolve_int(P, PF, G, nbox, boxes, ng, U, dt, info, 3);
+}
+
 /* Evolve's step loop with the Phase-Field step (:202-312): ComputeDt, the flow's RK3 step, then
  * PhaseField_Evolve with the same dt. */
-int ref_level_advance_pf(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
-                         int nbox, const cmp_box* boxes, int ng, double* U, int nsteps, double cfl,
-                         int limiter, double* dt_hist)
+int ref_level_advance_pf_int(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                             int nbox, const cmp_box* boxes, int ng, double* U, int nsteps, double cfl,
+                             int limiter, double* dt_hist, int integrator)
 {
     double cmax = 0.0, dt = 0.0;
     int err = ref_level_initial_cmax(P, G, nbox, boxes, ng, U, &cmax);
@@ -2515,11 +2527,18 @@ int ref_level_advance_pf(const cmp_params* P, const cmp_pf_params* PF, const cmp
         dt = ref_compute_dt(G, cmax, cfl, limiter, s == 0 ? ref_est_dt(G, cmax, cfl) : dt);
         if (dt_hist) dt_hist[s] = dt;
         cmax = 0.0;
-        err = ref_level_rk3_step(P, G, nbox, boxes, ng, U, dt, &cmax);
+        err = ref_level_rk_step(P, G, nbox, boxes, ng, U, dt, &cmax, integrator);
         if (isnan(cmax)) err = CMP_ERR_NAN;
-        if (!err) err = ref_level_pf_evolve(P, PF, G, nbox, boxes, ng, U, dt, NULL);
+        if (!err) err = pf_evolve_int(P, PF, G, nbox, boxes, ng, U, dt, NULL, integrator);
     }
     return err;
+}
+
+int ref_level_advance_pf(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                         int nbox, const cmp_box* boxes, int ng, double* U, int nsteps, double cfl,
+                         int limiter, double* dt_hist)
+{
+    return ref_level_advance_pf_int(P, PF, G, nbox, boxes, ng, U, nsteps, cfl, limiter, dt_hist, 3);
 }
 
 /* ------------------------------------------------------------------------------------------------

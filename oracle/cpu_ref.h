@@ -86,6 +86,13 @@ int ref_level_rk_step(const cmp_params* P, const cmp_geometry* G, int nbox, cons
 /* A Phase-Field sub-step of pf_dt at a fixed Grad_Q with run.TimeIntegrator (1..4 as above). */
 int ref_level_pf_rk_step(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G, int nbox,
                          const cmp_box* boxes, int ng, double* Un, const double* gradq, double pf_dt, int integrator);
+/* Evolve's step loop (flow only / with the Phase-Field step) with run.TimeIntegrator for the flow
+ * steps and the Phase-Field sub-steps (1..4 as above). */
+int ref_level_advance_int(const cmp_params* P, const cmp_geometry* G, int nbox, const cmp_box* boxes,
+                          int ng, double* U, int nsteps, double cfl, int limiter, double* dt_hist, int integrator);
+int ref_level_advance_pf_int(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
+                             int nbox, const cmp_box* boxes, int ng, double* U, int nsteps, double cfl,
+                             int limiter, double* dt_hist, int integrator);
 int ref_level_initial_cmax(const cmp_params* P, const cmp_geometry* G, int nbox,
                            const cmp_box* boxes, int ng, const double* U, double* cmax);
 double ref_est_dt(const cmp_geometry* G, double cmax, double cfl);

@@ -32,3 +32,35 @@ def test_rk_step_matches_oracle(gpu, dim, integrator):
     err = np.abs(got - want).reshape(P.nstate, -1).max(axis=1) / np.abs(want).reshape(P.nstate, -1).max(axis=1)
     assert err.max() <= 1e-12, err
     assert cm.item() > 0.0
+
+
+@pytest.mark.parametrize("integrator,with_pf", [(1, False), (2, False), (4, False), (2, True), (4, True)])
+def test_advance_with_the_level_integrator_matches_oracle(gpu, integrator, with_pf):
+    """cmp_level_advance (graph-replayed steps) and, with a Phase-Field step, its sub-steps under
+    cmp_level_set_integrator against the oracle's step loop with the same run.TimeIntegrator."""
+    import torch
+    from compas_amd import make_pf_params
+    c = cases.random_smooth(2, 32, 9, max_grid_size=16)
+    P, G = c["params"], c["geom"]
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    pf = make_pf_params(mechanism="ACDI", explicit_rc=1) if with_pf else None
+    nsteps = 6
+    ref = buf.copy()
+    hist = oracle.level_advance_int(P, G, lay, ref, nsteps, 0.4, integrator, pf=pf)
+    Un = torch.from_numpy(buf).to(gpu)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dt = torch.zeros(1, dtype=torch.float64, device=gpu)
+    cm = torch.zeros(1, dtype=torch.float64, device=gpu)
+    dth = torch.zeros(nsteps, dtype=torch.float64, device=gpu)
+    lev = Level(P, G, lay)
+    if pf is not None:
+        lev.pf_setup(pf)
+    lev.set_integrator(integrator)
+    lev.advance(Un, U1, U2, nsteps, 0.4, dt, cm, dt_hist=dth, use_graph=True)
+    assert lev.check_errors() == 0
+    lev.close()
+    assert np.all(np.abs(dth.cpu().numpy() - hist) <= 1e-12 * hist)
+    got, want = lay.unpack(Un.cpu().numpy(), c["shape"]), lay.unpack(ref, c["shape"])
+    err = np.abs(got - want).reshape(P.nstate, -1).max(axis=1) / np.abs(want).reshape(P.nstate, -1).max(axis=1)
+    assert err.max() <= 1e-11, err
