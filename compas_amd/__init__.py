@@ -297,6 +297,15 @@ class Level:
                                          int(use_graph), int(bool(resume)), _stream(stream)),
               "cmp_level_advance")
 
+    # the level's clock (advanced by every ComputeDt) and stop_time (negative: none)
+    def set_time(self, time, stop_time=-1.0):
+        check(self.lib.cmp_level_set_time(self.h, float(time), float(stop_time)), "cmp_level_set_time")
+
+    def get_time(self, stream=None):
+        t = ctypes.c_double()
+        check(self.lib.cmp_level_get_time(self.h, ctypes.byref(t), _stream(stream)), "cmp_level_get_time")
+        return t.value
+
     # run.TimeIntegrator of advance() and the Phase-Field sub-steps (1..4)
     def set_integrator(self, integrator):
         check(self.lib.cmp_level_set_integrator(self.h, int(integrator)), "cmp_level_set_integrator")

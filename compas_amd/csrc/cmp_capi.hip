@@ -27,7 +27,7 @@ template <int DIM>
 hipError_t launch_fill_dim(int, const FillDev&, long long, const int*, int, double*, int*, hipStream_t);
 template <int DIM>
 hipError_t launch_dt_dim(const double*, double*, double*, double, int, double*, int*, int*,
-                         hipStream_t);
+                         hipStream_t, double*);
 template <int DIM>
 hipError_t launch_pf_extract_dim(const PfGeom&, long long, const double*, double*, double*, int, hipStream_t);
 template <int DIM>
@@ -249,10 +249,10 @@ hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, 
 }
 
 hipError_t launch_compute_dt(int dim, const double* dx, double* cmax, double* dt, double cfl,
-                             int limiter, double* hist, int* step, int* err, hipStream_t s)
+                             int limiter, double* hist, int* step, int* err, hipStream_t s, double* clock)
 {
-    return dim == 2 ? launch_dt_dim<2>(dx, cmax, dt, cfl, limiter, hist, step, err, s)
-                    : launch_dt_dim<3>(dx, cmax, dt, cfl, limiter, hist, step, err, s);
+    return dim == 2 ? launch_dt_dim<2>(dx, cmax, dt, cfl, limiter, hist, step, err, s, clock)
+                    : launch_dt_dim<3>(dx, cmax, dt, cfl, limiter, hist, step, err, s, clock);
 }
 
 hipError_t launch_faces(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
@@ -701,7 +701,12 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
         fd.lo_bc[d] = geom->lo_bc[d];
         fd.hi_bc[d] = geom->hi_bc[d];
     }
-    int rc2 = check_hip(hipMalloc(&L->d_fd, sizeof(FillDev)));
+    // the level's clock: time 0, no stop_time
+    const double clk[2] = {0.0, -1.0};
+    int rc2 = check_hip(hipMalloc(&L->d_clock, sizeof(double) * 2));
+    if (!rc2) rc2 = check_hip(hipMemcpy(L->d_clock, clk, sizeof(double) * 2, hipMemcpyHostToDevice));
+    if (rc2) { cmp_level_destroy(L); return rc2; }
+    rc2 = check_hip(hipMalloc(&L->d_fd, sizeof(FillDev)));
     if (!rc2) rc2 = check_hip(hipMemcpy(L->d_fd, &fd, sizeof(FillDev), hipMemcpyHostToDevice));
     if (rc2) { cmp_level_destroy(L); return rc2; }
     *out = L;
@@ -742,7 +747,8 @@ int cmp_level_destroy(cmp_level* L)
     if (L->gexec) (void)hipGraphExecDestroy(L->gexec);
     if (L->graph) (void)hipGraphDestroy(L->graph);
     if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
-    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff, L->d_U3};
+    void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff, L->d_U3,
+                    L->d_clock};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete L;
@@ -886,7 +892,7 @@ int cmp_level_compute_dt(cmp_level* L, double* cmax, double* dt_dev, double cfl,
     if (!L || !cmax || !dt_dev) return CMP_ERR_ARG;
     // no history is written by this entry point (hist == nullptr)
     CMP_TRY(launch_compute_dt(L->P.dim, L->G.dx, cmax, dt_dev, cfl, limiter, nullptr, L->d_step,
-                              L->d_err, (hipStream_t)stream));
+                              L->d_err, (hipStream_t)stream, L->d_clock));
     return CMP_OK;
 }
 
@@ -895,9 +901,25 @@ static int level_step_with_hist(cmp_level* L, double* Un, double* U1, double* U2
                                 hipStream_t s)
 {
     CMP_TRY(launch_compute_dt(L->P.dim, L->G.dx, cmax, dt_dev, cfl, limiter, hist, L->d_step,
-                              L->d_err, s));
+                              L->d_err, s, L->d_clock));
     if (L->integrator != 3) return rk_step_impl(L, L->integrator, Un, U1, U2, L->d_U3, dt_dev, cmax, nullptr, s);
     return cmp_level_rk3_step(L, Un, U1, U2, dt_dev, cmax, s);
+}
+
+int cmp_level_set_time(cmp_level* L, double time, double stop_time)
+{
+    if (!L) return CMP_ERR_ARG;
+    const double clk[2] = {time, stop_time};
+    CMP_TRY(hipMemcpy(L->d_clock, clk, sizeof(double) * 2, hipMemcpyHostToDevice));
+    return CMP_OK;
+}
+
+int cmp_level_get_time(const cmp_level* L, double* time, void* stream)
+{
+    if (!L || !time) return CMP_ERR_ARG;
+    CMP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    CMP_TRY(hipMemcpy(time, L->d_clock, sizeof(double), hipMemcpyDeviceToHost));
+    return CMP_OK;
 }
 
 int cmp_level_set_integrator(cmp_level* L, int integrator)

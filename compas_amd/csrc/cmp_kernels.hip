@@ -761,8 +761,11 @@ void k_fill(FillDev fd, const int* __restrict__ list, int box0, double* U, int* 
 // source/Class_Compressible_PhaseField.cpp:4473-4499). One thread. step[0] counts the steps of the
 // current advance call, step[1] is the capacity of hist.
 template <int DIM>
+// clock (or null): [0] the level's time, advanced by every dt; [1] stop_time (negative: none), at
+// which the dt is cut so that the step ends on it (source/COMPAS.cpp:2278-2283: when
+// t + dt > stop_time - 1e-3 dt, dt = stop_time - t).
 __global__ void k_compute_dt(double dx0, double dx1, double dx2, double* cmax, double* dt, double cfl,
-                             int limiter, double* hist, int* step, int* err)
+                             int limiter, double* hist, int* step, int* err, double* clock)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double cm = *cmax;
@@ -777,6 +780,14 @@ __global__ void k_compute_dt(double dx0, double dx1, double dx2, double* cmax, d
     double dtn = est;
     if (limiter && prev > 0.0) dtn = smin(est, 1.1 * prev);
     if (!(cm > 0.0) || __builtin_isnan(dtn)) atomicMax(err, CMP_ERR_NAN);
+    if (clock) {
+        const double t = clock[0], stop = clock[1];
+        if (stop >= 0.0) {
+            const double eps = 1.0e-3 * dtn;
+            if (t + dtn > stop - eps) dtn = stop - t;
+        }
+        clock[0] = t + dtn;
+    }
     *dt = dtn;
     const int s = step[0];
     if (hist && s >= 0 && s < step[1]) hist[s] = dtn;
@@ -1548,10 +1559,10 @@ hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, const int
 
 template <int DIM>
 hipError_t launch_dt_dim(const double* dx, double* cmax, double* dt, double cfl, int limiter,
-                         double* hist, int* step, int* err, hipStream_t s)
+                         double* hist, int* step, int* err, hipStream_t s, double* clock)
 {
     hipLaunchKernelGGL((k_compute_dt<DIM>), dim3(1), dim3(64), 0, s, dx[0], dx[1], dx[2], cmax, dt,
-                       cfl, limiter, hist, step, err);
+                       cfl, limiter, hist, step, err, clock);
     return hipGetLastError();
 }
 
@@ -1590,6 +1601,6 @@ template hipError_t launch_faces_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KPa
 template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(int, const FillDev&, long long, const int*, int, double*, int*,
                                                     hipStream_t);
 template hipError_t launch_dt_dim<CMP_KERNEL_DIM>(const double*, double*, double*, double, int,
-                                                  double*, int*, int*, hipStream_t);
+                                                  double*, int*, int*, hipStream_t, double*);
 
 }  // namespace cmp

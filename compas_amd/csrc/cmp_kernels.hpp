@@ -263,7 +263,8 @@ hipError_t launch_setd(double* p, double v, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);
 
 hipError_t launch_compute_dt(int dim, const double* dx, double* cmax, double* dt, double cfl,
-                             int limiter, double* hist, int* step, int* err, hipStream_t s);
+                             int limiter, double* hist, int* step, int* err, hipStream_t s,
+                             double* clock = nullptr);
 
 hipError_t launch_faces(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
                         const double* U, double* flux, double* fluxnc, hipStream_t s);

@@ -107,6 +107,7 @@ struct cmp_level {
     int integrator = 3;             // run.TimeIntegrator of cmp_level_advance and the Phase-Field sub-steps
     double* d_U3 = nullptr;         // RK4's fourth level buffer
     int ginteg = 0;                 // the integrator the captured step was built with
+    double* d_clock = nullptr;      // [0] the level's time (advanced by every ComputeDt), [1] stop_time
     long long bs = 1;
     int nb[3] = {1, 1, 1};
 };

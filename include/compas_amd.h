@@ -224,6 +224,13 @@ int cmp_level_advance(cmp_level* level, double* Un, double* U1, double* U2, int 
 int cmp_level_rk_step(cmp_level* level, int integrator, double* Un, double* U1, double* U2, double* U3,
                       const double* dt_dev, double* cmax, void* stream);
 
+/* The level's clock: every ComputeDt (cmp_level_compute_dt, cmp_level_advance) advances its time by
+ * the dt it sets, and with stop_time >= 0 cuts that dt so that the step ends on stop_time
+ * (source/COMPAS.cpp:2278-2283: t + dt > stop_time - 1e-3 dt gives dt = stop_time - t). A new level
+ * has time 0 and no stop_time (negative). */
+int cmp_level_set_time(cmp_level* level, double time, double stop_time);
+int cmp_level_get_time(const cmp_level* level, double* time, void* stream);
+
 /* run.TimeIntegrator of the level's cmp_level_advance steps and cmp_level_pf_evolve sub-steps (1
  * ForwardEuler, 2 TVD-RK2, 3 TVD-RK3, the default, 4 RK4; RK4 allocates a fourth level buffer
  * here). */

@@ -64,3 +64,41 @@ def test_advance_with_the_level_integrator_matches_oracle(gpu, integrator, with_
     got, want = lay.unpack(Un.cpu().numpy(), c["shape"]), lay.unpack(ref, c["shape"])
     err = np.abs(got - want).reshape(P.nstate, -1).max(axis=1) / np.abs(want).reshape(P.nstate, -1).max(axis=1)
     assert err.max() <= 1e-11, err
+
+
+def test_level_clock_and_stop_time(gpu):
+    """The level's time advances by every dt; with stop_time the dt that would pass it (beyond
+    stop_time - 1e-3 dt) is cut to end on it, and the next step's limiter sees the cut dt."""
+    import torch
+    c = cases.random_smooth(2, 32, 9, max_grid_size=16)
+    P, G = c["params"], c["geom"]
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    buf = lay.pack(c["U"])
+    ref = buf.copy()
+    hist = oracle.level_advance(P, G, lay, ref, 4, 0.4)
+
+    def run(stop):
+        Un = torch.from_numpy(buf).to(gpu)
+        U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+        dt = torch.zeros(1, dtype=torch.float64, device=gpu)
+        cm = torch.zeros(1, dtype=torch.float64, device=gpu)
+        dth = torch.zeros(4, dtype=torch.float64, device=gpu)
+        lev = Level(P, G, lay)
+        lev.set_time(0.0, stop)
+        lev.advance(Un, U1, U2, 4, 0.4, dt, cm, dt_hist=dth, use_graph=True)
+        t = lev.get_time()
+        assert lev.check_errors() == 0
+        lev.close()
+        return dth.cpu().numpy(), t
+
+    h, t = run(-1.0)
+    assert np.all(np.abs(h - hist) <= 1e-12 * hist)
+    assert t == ((h[0] + h[1]) + h[2]) + h[3]
+    stop = h[0] + h[1] + 0.5 * h[2]
+    h2, t2 = run(stop)
+    assert np.array_equal(h2[:2], h[:2])
+    assert h2[2] == stop - (h2[0] + h2[1])
+    assert t2 == ((h2[0] + h2[1]) + h2[2]) + h2[3]
+    assert abs(((h2[0] + h2[1]) + h2[2]) - stop) <= 1e-15 * stop
+    # past stop_time every further dt is cut to stop_time - t (zero-length steps)
+    assert h2[3] == stop - ((h2[0] + h2[1]) + h2[2])
