@@ -10,8 +10,8 @@
  * Operation order follows the cited reference lines; build with -ffp-contract=off (no FMA),
  * like x86-64 g++ builds the reference. std::min/std::max semantics (not fmin/fmax) are kept.
  *
- * Parity pinning of each function (what checks it independently) is listed in tests/ and
- * DESIGN.md; every function here has at least one such check.
+ * The checks that pin each function independently (golden vectors, closed forms, invariants,
+ * brute force) are the tests under tests/ named for it; every function here has at least one.
  */
 #include "cpu_ref.h"
 
