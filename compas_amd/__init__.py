@@ -298,8 +298,9 @@ class Level:
               "cmp_level_advance")
 
     # the level's clock (advanced by every ComputeDt) and stop_time (negative: none)
-    def set_time(self, time, stop_time=-1.0):
-        check(self.lib.cmp_level_set_time(self.h, float(time), float(stop_time)), "cmp_level_set_time")
+    def set_time(self, time, stop_time=-1.0, stream=None):
+        check(self.lib.cmp_level_set_time(self.h, float(time), float(stop_time), _stream(stream)),
+              "cmp_level_set_time")
 
     def get_time(self, stream=None):
         t = ctypes.c_double()

@@ -148,7 +148,7 @@ SIGNATURES = {
     "cmp_level_average_down": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
     "cmp_level_fill_two_levels": (ctypes.c_int, [vp, dp, vp, dp, ctypes.c_int, vp]),
     "cmp_level_set_integrator": (ctypes.c_int, [vp, ctypes.c_int]),
-    "cmp_level_set_time": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double]),
+    "cmp_level_set_time": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double, vp]),
     "cmp_level_get_time": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_double), vp]),
     "cmp_level_rk_step": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp, dp, dp, dp, vp]),
     "cmp_level_pf_rk_step": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp, dp, dp, ctypes.c_double, vp]),

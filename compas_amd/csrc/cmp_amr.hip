@@ -197,6 +197,9 @@ int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out)
         if (L->P.id_weno != L0->P.id_weno || L->P.id_limiter != L0->P.id_limiter ||
             L->P.id_riemann != L0->P.id_riemann || L->P.id_thinc != L0->P.id_thinc)
             return CMP_ERR_ARG;
+        // the hierarchy step is the stage-coupled TVD-RK3 without a stop_time cut; a level set to
+        // another integrator or given a stop_time would silently be stepped otherwise
+        if (L->integrator != 3 || L->stop_time >= 0.0) return CMP_ERR_UNSUPPORTED;
         if (l == 0) continue;
         const cmp_level* C = levels[l - 1];
         for (int d = 0; d < dim; ++d) {

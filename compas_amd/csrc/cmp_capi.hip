@@ -906,11 +906,18 @@ static int level_step_with_hist(cmp_level* L, double* Un, double* U1, double* U2
     return cmp_level_rk3_step(L, Un, U1, U2, dt_dev, cmax, s);
 }
 
-int cmp_level_set_time(cmp_level* L, double time, double stop_time)
+int cmp_level_set_time(cmp_level* L, double time, double stop_time, void* stream)
 {
     if (!L) return CMP_ERR_ARG;
-    const double clk[2] = {time, stop_time};
-    CMP_TRY(hipMemcpy(L->d_clock, clk, sizeof(double) * 2, hipMemcpyHostToDevice));
+    hipStream_t s = (hipStream_t)stream;
+    // in the order of the stream's queued work (a queued ComputeDt advances the clock)
+    double* clk = L->h_clock;
+    CMP_TRY(hipStreamSynchronize(s));
+    clk[0] = time;
+    clk[1] = stop_time;
+    CMP_TRY(hipMemcpyAsync(L->d_clock, clk, sizeof(double) * 2, hipMemcpyHostToDevice, s));
+    CMP_TRY(hipStreamSynchronize(s));
+    L->stop_time = stop_time;
     return CMP_OK;
 }
 
@@ -1409,6 +1416,12 @@ int cmp_level_pf_evolve(cmp_level* L, double* Un, double* U1, double* U2, const 
     CMP_TRY(hipStreamSynchronize(s));
     const double dt = L->h_pin[2];
     double gm = 0.0, inf4[4] = {0.0, 0.0, 0.0, 0.0};
+    if (dt == 0.0) {
+        // a zero step (the level's clock past its stop_time) has no Phase-Field step either
+        if (info)
+            for (int q = 0; q < 6; ++q) info[q] = 0.0;
+        return CMP_OK;
+    }
     int r = cmp_level_pf_gradq(L, Un, dt, L->d_gq, &gm, inf4, stream);
     if (r) return r;
     if ((r = pf_select_boxes(L, s))) return r;

@@ -539,6 +539,8 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
     constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
     __shared__ double sWF[SWF];
     __shared__ double sRed[NT / 64];
+    // a step of size zero (a level clock past its stop_time) leaves every state as it is
+    if (mode == MODE_RK && *dtp == 0.0) return;
 
     int tlo[3];
     int halo[3][2] = {{HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}};
@@ -763,11 +765,20 @@ void k_fill(FillDev fd, const int* __restrict__ list, int box0, double* U, int* 
 template <int DIM>
 // clock (or null): [0] the level's time, advanced by every dt; [1] stop_time (negative: none), at
 // which the dt is cut so that the step ends on it (source/COMPAS.cpp:2278-2283: when
-// t + dt > stop_time - 1e-3 dt, dt = stop_time - t).
+// t + dt > stop_time - 1e-3 dt, dt = stop_time - t). Once the time has reached stop_time the
+// reference's step loop ends (Evolve's `cur_time < stop_time`); here every later step has dt = 0,
+// its stage kernels return without writing, and c_max keeps the last real step's value.
 __global__ void k_compute_dt(double dx0, double dx1, double dx2, double* cmax, double* dt, double cfl,
                              int limiter, double* hist, int* step, int* err, double* clock)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (clock && clock[1] >= 0.0 && clock[0] >= clock[1]) {
+        *dt = 0.0;
+        const int s0 = step[0];
+        if (hist && s0 >= 0 && s0 < step[1]) hist[s0] = 0.0;
+        step[0] = s0 + 1;
+        return;
+    }
     const double cm = *cmax;
     const double dxs[3] = {dx0, dx1, dx2};
     double est = 1.7976931348623157e308;
@@ -785,6 +796,7 @@ __global__ void k_compute_dt(double dx0, double dx1, double dx2, double* cmax, d
         if (stop >= 0.0) {
             const double eps = 1.0e-3 * dtn;
             if (t + dtn > stop - eps) dtn = stop - t;
+            if (!(dtn > 0.0)) dtn = 0.0;
         }
         clock[0] = t + dtn;
     }
@@ -880,6 +892,8 @@ void k_lincomb(const BoxDev* __restrict__ boxes, int N, double* out, double p, c
     const BoxDev bx = boxes[blockIdx.y];
     const int n0 = bx.hi[0] - bx.lo[0] + 1, n1 = bx.hi[1] - bx.lo[1] + 1, n2 = bx.hi[2] - bx.lo[2] + 1;
     const long long ncell = (long long)n0 * n1 * n2;
+    // a step of size zero (a level clock past its stop_time) leaves every state as it is
+    if (L && *dtp == 0.0) return;
     const double rdt = L ? r * (*dtp) : 0.0;
     for (long long t = (long long)blockIdx.x * NT + threadIdx.x; t < ncell * N; t += (long long)gridDim.x * NT) {
         const int c = (int)(t / ncell);

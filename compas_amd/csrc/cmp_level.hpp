@@ -108,6 +108,8 @@ struct cmp_level {
     double* d_U3 = nullptr;         // RK4's fourth level buffer
     int ginteg = 0;                 // the integrator the captured step was built with
     double* d_clock = nullptr;      // [0] the level's time (advanced by every ComputeDt), [1] stop_time
+    double h_clock[2] = {0.0, -1.0};  // host staging of cmp_level_set_time
+    double stop_time = -1.0;        // the last stop_time given to cmp_level_set_time
     long long bs = 1;
     int nb[3] = {1, 1, 1};
 };

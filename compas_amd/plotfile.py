@@ -9,12 +9,16 @@
   little-endian doubles, component by component, x fastest).
 - `write_checkpoint` / `read_checkpoint`: the reference's checkpoint (WriteCheckpointFile,
   :2515-2598): `Header` ("Checkpoint file for COMPAS", finest level, steps, dt and time per level,
-  each level's box list), and per level the state with its ghost cells in `Level_<l>/Cell_H` +
-  `Cell_D_00000` without per-box headers (VisMF's NoFabHeader form). A checkpoint read back gives
-  the level buffer bit for bit, which is what a restart needs.
+  each level's box list), and per level the state with its ghost cells written by VisMF::Write to
+  amrex::MultiFabFileFullPrefix(lev, chk, "Level_", "") (:2594; read back at :2744), i.e. the
+  files `Level_<l>/_H` and `Level_<l>/_D_00000`, in VisMF's NoFabHeader_v1 form (:2590): header
+  version 2, the valid boxes with the ghost width on its own line, no per-box FAB headers and no
+  min/max, and the real format last. A checkpoint read back gives the level buffer bit for bit,
+  which is what a restart needs.
 
 AMReX itself (26.01, which defines these formats) is not vendored with the reference; the layouts
-here follow its published format and are checked by reading them back.
+here follow its published format and are checked by reading them back and against the file names
+and header fields the reference's writer implies. Whether AMReX reads them is unpinned.
 """
 import os
 
@@ -81,6 +85,9 @@ def write_plotfile(path, levels, varnames, time=0.0, steps=None, ref_ratios=()):
 
 
 def _write_multifab(dirname, prefix, lay, buf, comps, ng, fab_headers):
+    """VisMF::Write of a level: fab_headers -> Version_v1 (per-box FAB header lines, min/max),
+    else NoFabHeader_v1 (version 2: raw data, the real format after the FabOnDisk lines). The box
+    list is the valid boxes; ng is the ghost width written with each fab."""
     os.makedirs(dirname, exist_ok=True)
     dim = lay.dim
     comps = list(range(lay.ncomp)) if comps is None else list(comps)
@@ -98,14 +105,16 @@ def _write_multifab(dirname, prefix, lay, buf, comps, ng, fab_headers):
             f.write(np.ascontiguousarray(v, dtype="<f8").tobytes())
             mins.append(v.reshape(len(comps), -1).min(axis=1))
             maxs.append(v.reshape(len(comps), -1).max(axis=1))
-    lines = ["1" if fab_headers else "2", "1", str(len(comps)), str(ng), _boxarray_str(lay.boxes, dim, ng),
+    lines = ["1" if fab_headers else "2", "1", str(len(comps)), str(ng), _boxarray_str(lay.boxes, dim),
              str(lay.nbox)]
     lines += ["FabOnDisk: %s %d" % (data_name, o) for o in offsets]
+    lines.append("")
     if fab_headers:
-        lines.append("")
         for arr in (mins, maxs):
             lines.append("%d,%d" % (lay.nbox, len(comps)))
             lines += ["".join("%s," % repr(float(x)) for x in row) for row in arr]
+    else:
+        lines.append("(%s)" % LITTLE_ENDIAN_REAL)
     with open(os.path.join(dirname, prefix + "_H"), "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -125,7 +134,7 @@ def write_checkpoint(path, levels, steps, dts, times):
     with open(os.path.join(path, "Header"), "w") as f:
         f.write("\n".join(lines) + "\n")
     for lev, (lay, buf) in enumerate(levels):
-        _write_multifab(os.path.join(path, "Level_%d" % lev), "Cell", lay, buf, None, ng=lay.ng, fab_headers=False)
+        _write_multifab(os.path.join(path, "Level_%d" % lev), "", lay, buf, None, ng=lay.ng, fab_headers=False)
 
 
 def _parse_boxarray(tokens_text):
@@ -155,17 +164,19 @@ def read_checkpoint(path, layouts):
     bufs = []
     for lev in range(nlev):
         lay = layouts[lev]
-        with open(os.path.join(path, "Level_%d" % lev, "Cell_H")) as f:
+        with open(os.path.join(path, "Level_%d" % lev, "_H")) as f:
             hdr = f.read()
+        hl = hdr.split("\n")
+        if int(hl[0]) != 2 or int(hl[3]) != lay.ng:
+            raise ValueError("checkpoint level %d: header version %s, ghost width %s" % (lev, hl[0], hl[3]))
         start = hdr.index("(")
         ba = hdr[start:hdr.index("\n)", start) + 2]      # the box list ends with a line ")"
         boxes = [(lo, hi) for lo, hi in _parse_boxarray(ba)]
-        want = [([l - lay.ng for l in lo], [h + lay.ng for h in hi]) for lo, hi in lay.boxes]
-        if [(list(a), list(b)) for a, b in boxes] != [(list(a), list(b)) for a, b in want]:
+        if [(list(a), list(b)) for a, b in boxes] != [(list(a), list(b)) for a, b in lay.boxes]:
             raise ValueError("checkpoint level %d has other boxes than the layout" % lev)
         offs = [int(l.split()[2]) for l in hdr.splitlines() if l.startswith("FabOnDisk:")]
         buf = np.zeros(lay.total)
-        raw = np.fromfile(os.path.join(path, "Level_%d" % lev, "Cell_D_00000"), dtype="<f8")
+        raw = np.fromfile(os.path.join(path, "Level_%d" % lev, "_D_00000"), dtype="<f8")
         for b in range(lay.nbox):
             v = lay.fab_view(buf, b)
             start = offs[b] // 8

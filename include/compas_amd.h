@@ -226,9 +226,14 @@ int cmp_level_rk_step(cmp_level* level, int integrator, double* Un, double* U1, 
 
 /* The level's clock: every ComputeDt (cmp_level_compute_dt, cmp_level_advance) advances its time by
  * the dt it sets, and with stop_time >= 0 cuts that dt so that the step ends on stop_time
- * (source/COMPAS.cpp:2278-2283: t + dt > stop_time - 1e-3 dt gives dt = stop_time - t). A new level
- * has time 0 and no stop_time (negative). */
-int cmp_level_set_time(cmp_level* level, double time, double stop_time);
+ * (source/COMPAS.cpp:2278-2283: t + dt > stop_time - 1e-3 dt gives dt = stop_time - t; the cut
+ * follows the default run.do_subcycle = 1, where it can lengthen the step by up to 1e-3 dt). Once
+ * the time has reached stop_time, where the reference's step loop ends, every further step has
+ * dt = 0 and leaves the state and c_max as they are (no stage, no Phase-Field step). A new level has
+ * time 0 and no stop_time (negative). cmp_level_set_time waits for the stream's queued work, then
+ * writes the clock in the stream's order; a level with a stop_time cannot join a hierarchy
+ * (cmp_amr_create). */
+int cmp_level_set_time(cmp_level* level, double time, double stop_time, void* stream);
 int cmp_level_get_time(const cmp_level* level, double* time, void* stream);
 
 /* run.TimeIntegrator of the level's cmp_level_advance steps and cmp_level_pf_evolve sub-steps (1

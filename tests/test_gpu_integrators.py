@@ -102,3 +102,60 @@ def test_level_clock_and_stop_time(gpu):
     assert abs(((h2[0] + h2[1]) + h2[2]) - stop) <= 1e-15 * stop
     # past stop_time every further dt is cut to stop_time - t (zero-length steps)
     assert h2[3] == stop - ((h2[0] + h2[1]) + h2[2])
+
+
+def _clock_run(gpu, c, lay, nsteps, stop, pf=None, use_graph=True):
+    import torch
+    P, G = c["params"], c["geom"]
+    Un = torch.from_numpy(lay.pack(c["U"])).to(gpu)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dt = torch.zeros(1, dtype=torch.float64, device=gpu)
+    cm = torch.zeros(1, dtype=torch.float64, device=gpu)
+    dth = torch.zeros(nsteps, dtype=torch.float64, device=gpu)
+    lev = Level(P, G, lay)
+    if pf is not None:
+        lev.pf_setup(pf)
+    lev.set_time(0.0, stop)
+    lev.advance(Un, U1, U2, nsteps, c["cfl"], dt, cm, dt_hist=dth, use_graph=use_graph)
+    t = lev.get_time()
+    assert lev.check_errors() == 0
+    lev.close()
+    return Un.cpu().numpy(), dth.cpu().numpy(), t
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_stop_time_inside_a_run_with_the_phase_field_step(gpu, use_graph):
+    """stop_time half-way through the third step of a five-step run with the Phase-Field step
+    (ACDI, explicit RC): the third step is cut to end on stop_time, the two steps after it have
+    dt = 0 and change nothing (no flow stage, no Phase-Field step), the run reports no error, and its
+    state equals, bit for bit, a run of three steps to the same stop_time."""
+    from compas_amd import make_pf_params
+    c = cases.advected_interface_2d(n=32, max_grid_size=16)
+    lay = Layout(2, c["boxes"], 3, 6)
+    pf = make_pf_params(eta_multiplier=0.5, cfl=0.3)
+    _, h, _ = _clock_run(gpu, c, lay, 3, -1.0, pf, use_graph)
+    stop = (h[0] + h[1]) + 0.5 * h[2]
+    U5, h5, t5 = _clock_run(gpu, c, lay, 5, stop, pf, use_graph)
+    U3, h3, t3 = _clock_run(gpu, c, lay, 3, stop, pf, use_graph)
+    assert np.array_equal(h5[:2], h[:2]) and h5[2] == stop - (h5[0] + h5[1])
+    assert h5[3] == 0.0 and h5[4] == 0.0
+    assert t5 == stop and t3 == stop
+    assert np.array_equal(U5, U3)
+    # the cut step moved the state
+    U2, _, _ = _clock_run(gpu, c, lay, 2, -1.0, pf, use_graph)
+    assert not np.array_equal(U3, U2)
+
+
+def test_stop_time_cut_can_lengthen_a_step(gpu):
+    """With the default run.do_subcycle = 1, ComputeDt's cut (t + dt > stop_time - 1e-3 dt gives
+    dt = stop_time - t, source/COMPAS.cpp:2278-2283) also applies when stop_time lies up to 1e-3 dt
+    beyond t + dt: the step is lengthened to end on stop_time."""
+    c = cases.random_smooth(2, 32, 9, max_grid_size=16)
+    c["cfl"] = 0.4
+    lay = Layout(2, c["boxes"], 3, c["params"].nstate)
+    _, h, _ = _clock_run(gpu, c, lay, 3, -1.0)
+    stop = (h[0] + h[1]) + h[2] * (1.0 + 5.0e-4)
+    _, h2, t2 = _clock_run(gpu, c, lay, 3, stop)
+    assert np.array_equal(h2[:2], h[:2])
+    assert h2[2] == stop - (h2[0] + h2[1]) and h2[2] > h[2]
+    assert t2 == stop

@@ -55,3 +55,30 @@ def test_checkpoint_round_trip(tmp_path):
     assert steps == [20] and dts == [1.5e-3] and times == [0.03]
     assert np.array_equal(bufs[0], buf)
     assert open(os.path.join(path, "Header")).readline() == "Checkpoint file for COMPAS\n"
+
+
+def test_checkpoint_files_follow_the_reference_writer(tmp_path):
+    """WriteCheckpointFile (source/COMPAS.cpp:2586-2595) writes each level with
+    VisMF::Write(dof_new[lev], MultiFabFileFullPrefix(lev, chk, "Level_", "")), i.e. the files
+    Level_<l>/_H and Level_<l>/_D_00000, in the NoFabHeader_v1 form: header version 2, one file,
+    the component count, the ghost width on its own line, the VALID boxes, the fab offsets, and the
+    real format; the data file holds every fab (ghost cells included) with no FAB header lines."""
+    c, lay, buf = _case()
+    path = str(tmp_path / "chk00030")
+    plotfile.write_checkpoint(path, [(lay, buf)], steps=[30], dts=[1e-3], times=[0.5])
+    assert sorted(os.listdir(os.path.join(path, "Level_0"))) == ["_D_00000", "_H"]
+    h = open(os.path.join(path, "Level_0", "_H")).read().split("\n")
+    assert h[0] == "2" and h[1] == "1" and int(h[2]) == 6 and int(h[3]) == 3
+    assert h[4] == "(%d 0" % lay.nbox
+    for b, (lo, hi) in enumerate(lay.boxes):
+        assert h[5 + b] == "((%d,%d) (%d,%d) (0,0))" % (lo[0], lo[1], hi[0], hi[1])
+    assert h[5 + lay.nbox] == ")"
+    assert int(h[6 + lay.nbox]) == lay.nbox
+    fod = h[7 + lay.nbox:7 + 2 * lay.nbox]
+    sizes = [lay.fab_view(buf, b).size * 8 for b in range(lay.nbox)]
+    assert [int(x.split()[2]) for x in fod] == list(np.cumsum([0] + sizes[:-1]))
+    assert all(x.startswith("FabOnDisk: _D_00000 ") for x in fod)
+    assert h[-2] == "((8, (64 11 52 0 1 12 0 1023)),(8, (1 2 3 4 5 6 7 8)))" and h[-1] == ""
+    raw = open(os.path.join(path, "Level_0", "_D_00000"), "rb").read()
+    assert len(raw) == sum(sizes)
+    assert np.array_equal(np.frombuffer(raw, dtype="<f8"), buf)
