@@ -18,6 +18,9 @@ CMP_ERR_COVOLUME = 5
 CMP_ERR_BC = 6
 CMP_ERR_LAYOUT = 7
 CMP_ERR_NOMEM = 8
+CMP_ERR_SCRATCH = 9
+CMP_NOALLOC_MAX_BOXES = 4096
+CMP_NOALLOC_MAX_TILES = 65536
 
 ERROR_NAMES = {
     CMP_OK: "CMP_OK",
@@ -29,6 +32,7 @@ ERROR_NAMES = {
     CMP_ERR_BC: "CMP_ERR_BC",
     CMP_ERR_LAYOUT: "CMP_ERR_LAYOUT",
     CMP_ERR_NOMEM: "CMP_ERR_NOMEM",
+    CMP_ERR_SCRATCH: "CMP_ERR_SCRATCH",
 }
 
 
@@ -173,6 +177,8 @@ SIGNATURES = {
     "cmp_batch_case_download": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), vp]),
     "cmp_batch_check": (ctypes.c_int, [vp, ctypes.c_int, vp]),
     "cmp_batch_info": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_longlong)]),
+    "cmp_level_rhs_noalloc": (ctypes.c_int, [P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,
+                                             dp, dp, dp, dp, vp]),
 }
 
 _lib = None

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -1586,6 +1587,113 @@ int cmp_level_check(cmp_level* L, void* stream)
     CMP_TRY(hipMemcpy(&h, L->d_err, sizeof(int), hipMemcpyDeviceToHost));
     CMP_TRY(hipMemset(L->d_err, 0, sizeof(int)));
     return h;
+}
+
+}  // extern "C"
+
+// ---- the allocation-free right-hand side of a box list ------------------------------------------
+// cmp_level_rhs_noalloc keeps the device tables cmp_level_create would allocate (the box table,
+// then the tile work-list) in this file-scope array, which each GPU holds once for the life of the
+// process. In host code the array's name is not a device address: every host use goes through the
+// per-device address that scratch_base() looks up with hipGetSymbolAddress.
+namespace {
+constexpr size_t NOALLOC_BOX_BYTES = ((sizeof(BoxDev) * CMP_NOALLOC_MAX_BOXES + 255) / 256) * 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_BOX_BYTES + sizeof(TileDev) * CMP_NOALLOC_MAX_TILES;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch`, looked up on the first call made with that device current.
+int scratch_base(void** out)
+{
+    int dev = -1;
+    CMP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return CMP_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        CMP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return CMP_ERR_HIP;
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return CMP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int cmp_level_rhs_noalloc(const cmp_params* params, const cmp_geometry* geom, int nbox, const cmp_box* boxes,
+                          int ng, const double* U, double* dUdt, double* cmax, int* err, void* stream)
+{
+    LaunchCfg lc;
+    KParams kp;
+    int r = validate(params, geom, ng, &lc, &kp);
+    if (r) return r;
+    if (nbox <= 0 || !boxes || !U || !dUdt) return CMP_ERR_ARG;
+    if (nbox > CMP_NOALLOC_MAX_BOXES) return CMP_ERR_SCRATCH;
+    const int dim = params->dim;
+    int tx, ty, tz;
+    tile_dims(dim, &tx, &ty, &tz);
+    const int te[3] = {tx, ty, dim == 3 ? tz : 1};
+    // the checks of cmp_level_create: boxes inside the domain, offsets of cmp_level_layout
+    long long expect = 0, ntiles = 0;
+    for (int b = 0; b < nbox; ++b) {
+        long long nt = 1;
+        for (int d = 0; d < dim; ++d) {
+            if (boxes[b].lo[d] < geom->domain_lo[d] || boxes[b].hi[d] > geom->domain_hi[d] ||
+                boxes[b].hi[d] < boxes[b].lo[d])
+                return CMP_ERR_LAYOUT;
+            nt *= (boxes[b].hi[d] - boxes[b].lo[d] + te[d]) / te[d];
+        }
+        if (boxes[b].offset != expect) return CMP_ERR_LAYOUT;
+        if (cmp_fab_size(dim, &boxes[b], ng, 1) >= (1LL << 31)) return CMP_ERR_UNSUPPORTED;
+        expect += cmp_fab_size(dim, &boxes[b], ng, params->nstate);
+        ntiles += nt;
+    }
+    if (ntiles > CMP_NOALLOC_MAX_TILES) return CMP_ERR_SCRATCH;
+    void* base = nullptr;
+    if ((r = scratch_base(&base))) return r;
+    BoxDev* d_boxes = (BoxDev*)base;
+    TileDev* d_tiles = (TileDev*)((unsigned char*)base + NOALLOC_BOX_BYTES);
+    hipStream_t s = (hipStream_t)stream;
+    // the tables go up in fixed-size pieces from the stack, in stream order behind the previous
+    // call's kernels (a copy from pageable host memory is staged before hipMemcpyAsync returns)
+    constexpr int CH = 64;
+    BoxDev hb[CH];
+    for (int b0 = 0; b0 < nbox; b0 += CH) {
+        const int n = std::min(CH, nbox - b0);
+        for (int q = 0; q < n; ++q) hb[q] = make_boxdev(dim, boxes[b0 + q], ng);
+        CMP_TRY(hipMemcpyAsync(d_boxes + b0, hb, sizeof(BoxDev) * n, hipMemcpyHostToDevice, s));
+    }
+    // tiles box by box, in cmp_level_create's order
+    TileDev ht[4 * CH];
+    int nh = 0;
+    long long t0 = 0;
+    for (int b = 0; b < nbox; ++b) {
+        const cmp_box& B = boxes[b];
+        const int klo = (dim == 3) ? B.lo[2] : 0, khi = (dim == 3) ? B.hi[2] : 0;
+        for (int k = klo; k <= khi; k += te[2])
+            for (int j = B.lo[1]; j <= B.hi[1]; j += ty)
+                for (int i = B.lo[0]; i <= B.hi[0]; i += tx) {
+                    ht[nh++] = TileDev{b, {i, j, k}, {{HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP},
+                                                      {HALO_LOOKUP, HALO_LOOKUP}}};
+                    if (nh == 4 * CH) {
+                        CMP_TRY(hipMemcpyAsync(d_tiles + t0, ht, sizeof(TileDev) * nh, hipMemcpyHostToDevice, s));
+                        t0 += nh;
+                        nh = 0;
+                    }
+                }
+    }
+    if (nh > 0) CMP_TRY(hipMemcpyAsync(d_tiles + t0, ht, sizeof(TileDev) * nh, hipMemcpyHostToDevice, s));
+    CMP_TRY(launch_stage(lc, kp, MODE_RHS, 0, d_tiles, (int)ntiles, d_boxes, nullptr, 0, U, nullptr, dUdt, nullptr,
+                         (unsigned long long*)cmax, err, s));
+    return CMP_OK;
 }
 
 }  // extern "C"

@@ -34,6 +34,7 @@ extern "C" {
 #define CMP_ERR_BC           6  /* boundary type without a fill rule here (Inflow/ext_dir)              */
 #define CMP_ERR_LAYOUT       7  /* boxes overlap, leave the domain, or do not cover it                  */
 #define CMP_ERR_NOMEM        8  /* a batch has no free case slot or no free range for a case's state    */
+#define CMP_ERR_SCRATCH      9  /* larger than the fixed device scratch of an allocation-free entry point */
 
 /* Scheme IDs follow FiniteVolume_Parameter::Initialize (include/Operators_FV/FiniteVolume_Parameter.H:71-139):
  *   GODUNOV  id_weno=0 id_limiter=0 id_poly=0  ng>=1
@@ -445,6 +446,23 @@ int cmp_batch_check(cmp_batch* batch, int case_id, void* stream);
 /* Admitted cases, tiles of the last rebuilt table (-1 before the next step rebuilds it) and free
  * arena doubles. */
 int cmp_batch_info(const cmp_batch* batch, int* ncases, int* ntiles, long long* free_doubles);
+
+/* ---- allocation-free entry point -----------------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The same right-hand side as
+ * cmp_level_create + cmp_level_rhs (every box of a level, U's ghost cells already filled, dUdt a
+ * level buffer laid out like U, *cmax max-accumulated when cmax is non-null, the device error word
+ * *err max-accumulated when err is non-null), with the box table and tile work-list that
+ * cmp_level_create allocates kept instead in a fixed device array of the library, one per GPU,
+ * reused by every call: no hipMalloc, hipFree or other allocating routine is called. Limits:
+ * nbox <= CMP_NOALLOC_MAX_BOXES and at most CMP_NOALLOC_MAX_TILES tiles (16x15 cells in 2D, 8x8x3
+ * in 3D, per box), else CMP_ERR_SCRATCH before any GPU work. The tables are written in stream order,
+ * so calls on one stream follow each other; calls that overlap on one GPU (other streams, other
+ * threads) race on the array. */
+#define CMP_NOALLOC_MAX_BOXES 4096
+#define CMP_NOALLOC_MAX_TILES 65536
+int cmp_level_rhs_noalloc(const cmp_params* params, const cmp_geometry* geom, int nbox, const cmp_box* boxes,
+                          int ng, const double* U, double* dUdt, double* cmax, int* err, void* stream);
 
 #ifdef __cplusplus
 }
