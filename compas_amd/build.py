@@ -43,6 +43,8 @@ def build(verbose=False):
         ("cmp_kernels.hip", "k3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_pf.hip", "pf2.o", ["-DCMP_KERNEL_DIM=2"]),
         ("cmp_pf.hip", "pf3.o", ["-DCMP_KERNEL_DIM=3"]),
+        ("cmp_mg.hip", "mg2.o", ["-DCMP_KERNEL_DIM=2"]),
+        ("cmp_mg.hip", "mg3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_capi.hip", "capi.o", []),
         ("cmp_batch.hip", "batch.o", []),
         ("cmp_amr.hip", "amr.o", []),

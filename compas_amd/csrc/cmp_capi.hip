@@ -738,7 +738,64 @@ static void pf_free(cmp_level* L)
     L->d_part = L->d_sc = nullptr;
     L->d_bits = nullptr;
     L->h_pin = nullptr;
+    for (auto& V : L->mg) {
+        void* mb[] = {V.x[0], V.x[1], V.r, V.id, V.b[0], V.b[1], V.b[2]};
+        for (void* p : mb)
+            if (p) (void)hipFree(p);
+    }
+    L->mg.clear();
     L->pf_on = false;
+}
+
+// The dense multigrid grids of a level that covers its whole domain: its own resolution, then halved
+// in every active direction while every active extent is even; none (Jacobi preconditioning) when
+// the level leaves part of its domain uncovered or its coarsest grid would exceed 1024 cells.
+static int mg_setup(cmp_level* L)
+{
+    const int dim = L->P.dim;
+    long long dom = 1;
+    for (int d = 0; d < dim; ++d) dom *= L->G.domain_hi[d] - L->G.domain_lo[d] + 1;
+    if (dom != L->ncell) return CMP_OK;
+    cmp::MgGrid G{};
+    for (int d = 0; d < 3; ++d) {
+        G.n[d] = (d < dim) ? L->G.domain_hi[d] - L->G.domain_lo[d] + 1 : 1;
+        G.periodic[d] = (d < dim) ? L->G.is_periodic[d] : 1;
+        G.ih2[d] = (d < dim) ? 1.0 / (L->G.dx[d] * L->G.dx[d]) : 0.0;
+    }
+    std::vector<cmp::MgGrid> grids;
+    for (;;) {
+        G.nc = (long long)G.n[0] * G.n[1] * G.n[2];
+        grids.push_back(G);
+        bool even = true;
+        for (int d = 0; d < dim; ++d) even = even && (G.n[d] % 2 == 0);
+        if (!even) break;
+        for (int d = 0; d < dim; ++d) {
+            G.n[d] /= 2;
+            G.ih2[d] *= 0.25;
+        }
+    }
+    const cmp::MgGrid& B = grids.back();
+    if (B.nc > 1024) return CMP_OK;
+    int mx = 1;
+    for (int d = 0; d < dim; ++d) mx = std::max(mx, B.n[d]);
+    L->mg_bottom_sweeps = std::min(256, std::max(8, 2 * mx * mx));
+    L->mg_omega = (dim == 3) ? 6.0 / 7.0 : 0.8;
+    L->mg.resize(grids.size());
+    for (size_t m = 0; m < grids.size(); ++m) {
+        auto& V = L->mg[m];
+        V.G = grids[m];
+        const size_t nc = (size_t)V.G.nc;
+        CMP_TRY(hipMalloc(&V.x[0], sizeof(double) * nc));
+        CMP_TRY(hipMalloc(&V.x[1], sizeof(double) * nc));
+        CMP_TRY(hipMalloc(&V.r, sizeof(double) * nc));
+        CMP_TRY(hipMalloc(&V.id, sizeof(double) * nc));
+        for (int d = 0; d < dim; ++d) {
+            size_t nf = 1;
+            for (int e = 0; e < dim; ++e) nf *= (size_t)(V.G.n[e] + (e == d));
+            CMP_TRY(hipMalloc(&V.b[d], sizeof(double) * nf));
+        }
+    }
+    return CMP_OK;
 }
 
 int cmp_level_destroy(cmp_level* L)
@@ -1172,6 +1229,10 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     L->pf_kp = L->kp;
     L->pf_kp.thinc = 0;
     for (int k = 0; k < CMP_MAX_PHASES; ++k) L->pf_kp.pf_pmin[k] = pf->p_min[k];
+    if (pf->explicit_rc == 0) {
+        rc = mg_setup(L);
+        if (rc) { pf_free(L); return rc; }
+    }
     L->pf_on = true;
     return CMP_OK;
 }
@@ -1190,6 +1251,118 @@ static inline double bits_to_double(const void* p)
     double v;
     std::memcpy(&v, p, sizeof(double));
     return v;
+}
+
+static int mg_launch(int dim, int op, const cmp::MgArgs& a, hipStream_t s)
+{
+    CMP_TRY(dim == 2 ? cmp::launch_mg_dim<2>(op, a, s) : cmp::launch_mg_dim<3>(op, a, s));
+    return CMP_OK;
+}
+
+static cmp::MgArgs mg_args(const cmp_level* L, const double* sc)
+{
+    cmp::MgArgs a{};
+    a.g = L->pg;
+    a.maxcells = L->maxcells;
+    for (int d = 0; d < 3; ++d) a.maxfaces[d] = L->maxfaces[d];
+    a.sc = sc;
+    a.omega = L->mg_omega;
+    return a;
+}
+
+// Face coefficients of every multigrid grid (the level's b gathered, then restricted) and their
+// inverse diagonals; once per solve.
+static int mg_prepare(cmp_level* L, hipStream_t s)
+{
+    const int dim = L->P.dim;
+    cmp::MgArgs a = mg_args(L, nullptr);
+    a.src = L->d_Bf;
+    for (int d = 0; d < 3; ++d) a.b[d] = L->mg[0].b[d];
+    a.G = L->mg[0].G;
+    int r = mg_launch(dim, cmp::MG_GATHER_B, a, s);
+    for (size_t m = 0; !r && m + 1 < L->mg.size(); ++m) {
+        for (int d = 0; d < 3; ++d) {
+            a.b[d] = L->mg[m].b[d];
+            a.bc[d] = L->mg[m + 1].b[d];
+        }
+        a.G = L->mg[m].G;
+        a.Gc = L->mg[m + 1].G;
+        r = mg_launch(dim, cmp::MG_RESTRICT_B, a, s);
+    }
+    for (size_t m = 0; !r && m < L->mg.size(); ++m) {
+        for (int d = 0; d < 3; ++d) a.b[d] = L->mg[m].b[d];
+        a.G = L->mg[m].G;
+        a.dst = L->mg[m].id;
+        r = mg_launch(dim, cmp::MG_DIAG, a, s);
+    }
+    return r;
+}
+
+// One V-cycle on grid m from its residual r, the result left in x[cur]: two damped Jacobi sweeps
+// (the first from zero), the residual restricted, the coarser grid's V-cycle prolonged and added,
+// two more sweeps; the coarsest grid takes its fixed number of sweeps in one workgroup.
+static int mg_vcycle(cmp_level* L, const double* sc, size_t m, hipStream_t s)
+{
+    const int dim = L->P.dim;
+    auto& V = L->mg[m];
+    cmp::MgArgs a = mg_args(L, sc);
+    for (int d = 0; d < 3; ++d) a.b[d] = V.b[d];
+    a.G = V.G;
+    a.r = V.r;
+    a.id = V.id;
+    int r;
+    if (m + 1 == L->mg.size()) {
+        a.dst = V.x[0];
+        a.sweeps = L->mg_bottom_sweeps;
+        V.cur = 0;
+        return mg_launch(dim, cmp::MG_BOTTOM, a, s);
+    }
+    a.zero_x = 1;
+    a.dst = V.x[0];
+    if ((r = mg_launch(dim, cmp::MG_SMOOTH, a, s))) return r;
+    a.zero_x = 0;
+    a.src = V.x[0];
+    a.dst = V.x[1];
+    if ((r = mg_launch(dim, cmp::MG_SMOOTH, a, s))) return r;
+    auto& C = L->mg[m + 1];
+    a.src = V.x[1];
+    a.dst = C.r;
+    a.Gc = C.G;
+    if ((r = mg_launch(dim, cmp::MG_RESID_RESTRICT, a, s))) return r;
+    if ((r = mg_vcycle(L, sc, m + 1, s))) return r;
+    a.src = C.x[C.cur];
+    a.dst = V.x[1];
+    if ((r = mg_launch(dim, cmp::MG_PROLONG, a, s))) return r;
+    a.src = V.x[1];
+    a.dst = V.x[0];
+    if ((r = mg_launch(dim, cmp::MG_SMOOTH, a, s))) return r;
+    a.src = V.x[0];
+    a.dst = V.x[1];
+    if ((r = mg_launch(dim, cmp::MG_SMOOTH, a, s))) return r;
+    V.cur = 1;
+    return CMP_OK;
+}
+
+// z = V(r) for the CG's residual (level buffers R -> Z), then r.z block partials into the second
+// slot of d_part; with P non-null also p = z (the first iteration).
+static int mg_precondition(cmp_level* L, const double* sc, double* P, hipStream_t s)
+{
+    const int dim = L->P.dim;
+    cmp::MgArgs a = mg_args(L, sc);
+    a.G = L->mg[0].G;
+    a.src = L->d_R;
+    a.dst = L->mg[0].r;
+    int r = mg_launch(dim, cmp::MG_GATHER_R, a, s);
+    if (!r) r = mg_vcycle(L, sc, 0, s);
+    if (r) return r;
+    a.src = L->mg[0].x[L->mg[0].cur];
+    a.dst = L->d_Z;
+    if ((r = mg_launch(dim, cmp::MG_SCATTER, a, s))) return r;
+    a.src = L->d_R;
+    a.r = L->d_Z;
+    a.dst = P;
+    a.part = L->d_part;
+    return mg_launch(dim, cmp::MG_CG_RZ, a, s);
 }
 
 // The implicit Grad_Q (PhaseField_GradQPF with Div_J, source/Class_Compressible_PhaseField.cpp:2133-2310):
@@ -1213,18 +1386,29 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
     CMP_TRY(launch_cg_idiag(dim, L->pg, L->maxcells, L->d_Bf, L->d_ID, s));
     CMP_TRY(launch_cg_init(dim, L->pg, L->maxcells, L->d_DJ, sc + CG_SUM, (double)L->ncell, L->d_ID, L->d_X,
                            L->d_R, L->d_Z, L->d_Pv, L->d_part, s));
+    const bool mg = !L->mg.empty();
+    if (mg) {
+        // the V-cycle's face coefficients and diagonals for this b, then z = V(r), p = z
+        int r = mg_prepare(L, s);
+        if (!r) r = mg_precondition(L, nullptr, L->d_Pv, s);
+        if (r) return r;
+    }
     CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RR, s));
     CMP_TRY(launch_cg_start(sc, s));
     // the host looks at the stop flag every `chunk` iterations; iterations queued past the stop
     // return at once
-    const int chunk = 16;
+    const int chunk = mg ? 8 : 16;
     for (long long queued = 0;; queued += chunk) {
         for (int q = 0; q < chunk; ++q) {
             CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_Pv, L->d_err, s));
             CMP_TRY(launch_cg_ap(dim, L->pg, L->maxcells, sc, L->d_Bf, L->d_Pv, L->d_AP, L->d_part, s));
             CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_PAP, s));
-            CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, L->d_ID, L->d_X, L->d_R, L->d_Z, L->d_Pv, L->d_AP,
-                                 L->d_part, s));
+            CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, mg ? nullptr : L->d_ID, L->d_X, L->d_R, L->d_Z,
+                                 L->d_Pv, L->d_AP, L->d_part, s));
+            if (mg) {
+                int r = mg_precondition(L, sc, nullptr, s);
+                if (r) return r;
+            }
             CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RRNEW, s));
             CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s));
             CMP_TRY(launch_cg_end(sc, maxit, s));

@@ -180,6 +180,49 @@ struct PfGeom {
     int periodic[3];
 };
 
+// A dense domain-shaped grid of the multigrid preconditioner (cmp_mg.hip): extents n (1 in inactive
+// dimensions), cells i fastest; face arrays of direction d have extents n + e_d.
+struct MgGrid {
+    int n[3];
+    int periodic[3];
+    double ih2[3];      // 1 / h_d^2
+    long long nc;       // cells
+};
+
+enum MgOp {
+    MG_GATHER_R, MG_SCATTER, MG_GATHER_B, MG_RESTRICT_B, MG_DIAG, MG_SMOOTH, MG_RESID_RESTRICT, MG_PROLONG,
+    MG_BOTTOM, MG_CG_RZ
+};
+
+// Arguments of one multigrid operation (the fields an operation does not use are ignored):
+//   GATHER_R: src (box layout) -> dst (dense G); SCATTER: src (dense G) -> dst (box layout);
+//   GATHER_B: src (box face buffer) -> b[d] (dense faces); RESTRICT_B: b[d] (G) -> bc[d] (Gc);
+//   DIAG: b -> dst = 1/diag; SMOOTH: dst = src + omega id (r - A src) (zero_x: src = 0);
+//   RESID_RESTRICT: dst (Gc) = mean of the children of r - A src (G); PROLONG: dst (G) += src (Gc);
+//   BOTTOM: dst = `sweeps` Jacobi sweeps from 0 on r (G at most 1024 cells);
+//   CG_RZ: r.z partials of src (R) and r (Z) into part's second slot, dst (P) = Z when non-null.
+struct MgArgs {
+    PfGeom g;
+    long long maxcells;
+    long long maxfaces[3];
+    const double* sc;
+    const double* src;
+    double* dst;
+    const double* r;
+    const double* id;
+    double* b[3];
+    double* bc[3];
+    double* part;
+    MgGrid G, Gc;
+    double omega;
+    int zero_x;
+    int sweeps;
+};
+
+template <int DIM>
+hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s);
+
+
 // volume fraction `comp` of the flow state -> an alpha-layout buffer (and a second one, A2, if given)
 hipError_t launch_pf_extract(int dim, const PfGeom& g, long long maxcells, const double* U, double* A, double* A2,
                              int comp, hipStream_t s);

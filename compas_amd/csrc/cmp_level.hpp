@@ -107,6 +107,20 @@ struct cmp_level {
     int integrator = 3;             // run.TimeIntegrator of cmp_level_advance and the Phase-Field sub-steps
     double* d_U3 = nullptr;         // RK4's fourth level buffer
     int ginteg = 0;                 // the integrator the captured step was built with
+    // multigrid preconditioner of the implicit Grad_Q solve (a level that covers its domain): dense
+    // grids from the level's resolution down, each with its iterate pair, residual, inverse diagonal
+    // and face coefficients
+    struct MgLev {
+        cmp::MgGrid G;
+        double* x[2] = {nullptr, nullptr};
+        double* r = nullptr;
+        double* id = nullptr;
+        double* b[3] = {nullptr, nullptr, nullptr};
+        int cur = 0;
+    };
+    std::vector<MgLev> mg;
+    int mg_bottom_sweeps = 0;
+    double mg_omega = 0.8;
     double* d_clock = nullptr;      // [0] the level's time (advanced by every ComputeDt), [1] stop_time
     double h_clock[2] = {0.0, -1.0};  // host staging of cmp_level_set_time
     double stop_time = -1.0;        // the last stop_time given to cmp_level_set_time

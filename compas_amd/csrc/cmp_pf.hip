@@ -475,11 +475,14 @@ __global__ __launch_bounds__(NT) void k_cg_xr(PfGeom g, const double* __restrict
         const double al = sc[CG_RZ] / sc[CG_PAP];
         X[c] += al * P[c];
         const double r = R[c] - al * AP[c];
-        const double z = r * ID[c];
         R[c] = r;
-        Z[c] = z;
         v = r * r;
-        w = r * z;
+        // ID null: another preconditioner computes z and r.z after this kernel
+        if (ID) {
+            const double z = r * ID[c];
+            Z[c] = z;
+            w = r * z;
+        }
     }
     const long long blk = (long long)blockIdx.y * gridDim.x + blockIdx.x;
     block_sum_to(v, part, blk);

@@ -2158,8 +2158,9 @@ int ref_level_pf_mechanism(const cmp_params* P, const cmp_pf_params* PF, const c
  * The reference's solver is AMReX 26.01's geometric multigrid (MLMG, not vendored in the
  * reference) stopped at a relative residual of 1e-10 (LinearSystem_Parameter.H:31). Any converged
  * solve of the same discrete system gives the same Grad_Q to within that tolerance; this restates
- * the system and solves it by conjugate gradients on the zero-mean subspace down to the rounding
- * floor (relative residual 1e-14, or no further decrease), which the GPU solve also reaches.
+ * the system and solves it by Jacobi-preconditioned conjugate gradients on the zero-mean subspace
+ * down to the rounding floor (r.r <= 1e-28 of the initial r.r, or 400 iterations without a new
+ * minimum of r.r), which the GPU solve also reaches.
  * alpha: 1 ghost, filled; divjhat: alpha layout; gradq: face buffer. *iters: CG iterations. */
 int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, const cmp_geometry* G,
                                 int nbox, const cmp_box* boxes, const double* alpha,
@@ -2201,6 +2202,33 @@ int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, co
     double* R = (double*)calloc((size_t)ta, sizeof(double));
     double* Pv = (double*)calloc((size_t)ta, sizeof(double));
     double* AP = (double*)calloc((size_t)ta, sizeof(double));
+    double* ID = (double*)calloc((size_t)ta, sizeof(double));
+    /* Jacobi preconditioner: 1 / sum of b / dx^2 over the faces that carry a flux */
+    for (int b = 0; b < nbox; ++b) {
+        fab_t f = make_fab(dim, &ba[b], 1);
+        int nv[3] = {1, 1, 1};
+        for (int d = 0; d < dim; ++d) nv[d] = ba[b].hi[d] - ba[b].lo[d] + 1;
+        int lo2 = (dim == 3) ? ba[b].lo[2] : 0;
+        for (int kk = 0; kk < nv[2]; ++kk)
+            for (int jj = 0; jj < nv[1]; ++jj)
+                for (int ii = 0; ii < nv[0]; ++ii) {
+                    int c3[3] = {ba[b].lo[0] + ii, ba[b].lo[1] + jj, lo2 + kk};
+                    int idx[3] = {ii, jj, kk};
+                    double sd = 0.0;
+                    for (int d = 0; d < dim; ++d) {
+                        int fe[3] = {nv[0], nv[1], nv[2]};
+                        fe[d] += 1;
+                        long long qm = idx[0] + (long long)fe[0] * (idx[1] + (long long)fe[1] * idx[2]);
+                        long long qp = qm + (d == 0 ? 1 : (d == 1 ? fe[0] : (long long)fe[0] * fe[1]));
+                        int lo_dom = (c3[d] == G->domain_lo[d]) && !G->is_periodic[d];
+                        int hi_dom = (c3[d] == G->domain_hi[d]) && !G->is_periodic[d];
+                        double h2 = G->dx[d] * G->dx[d];
+                        if (!hi_dom) sd += Bf[foff[b * dim + d] + qp] / h2;
+                        if (!lo_dom) sd += Bf[foff[b * dim + d] + qm] / h2;
+                    }
+                    ID[ba[b].offset + fab_idx(&f, c3[0], c3[1], c3[2])] = (sd > 0.0) ? 1.0 / sd : 0.0;
+                }
+    }
     long long ncell = 0;
     double mean = 0.0;
     for (int b = 0; b < nbox; ++b) {
@@ -2214,8 +2242,9 @@ int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, co
                 }
     }
     mean /= (double)ncell;
-    /* the operator is -div(b grad .): symmetric positive semi-definite; r = -(rhs - mean) */
-    double rr = 0.0, b2 = 0.0;
+    /* the operator is -div(b grad .): symmetric positive semi-definite; r = -(rhs - mean); Jacobi-
+     * preconditioned conjugate gradients */
+    double rr = 0.0, b2 = 0.0, rz = 0.0;
     for (int b = 0; b < nbox; ++b) {
         fab_t f = make_fab(dim, &ba[b], 1);
         int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
@@ -2224,8 +2253,9 @@ int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, co
                 for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
                     long long c = ba[b].offset + fab_idx(&f, i, j, k);
                     R[c] = -(divjhat[c] - mean);
-                    Pv[c] = R[c];
+                    Pv[c] = ID[c] * R[c];
                     rr += R[c] * R[c];
+                    rz += R[c] * Pv[c];
                 }
     }
     b2 = rr;
@@ -2270,7 +2300,7 @@ int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, co
                     }
         }
         if (!(pAp > 0.0)) break;
-        double al = rr / pAp, rr2 = 0.0;
+        double al = rz / pAp, rr2 = 0.0, rz2 = 0.0;
         for (int b = 0; b < nbox; ++b) {
             fab_t f = make_fab(dim, &ba[b], 1);
             int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
@@ -2281,10 +2311,12 @@ int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, co
                         X[c] += al * Pv[c];
                         R[c] -= al * AP[c];
                         rr2 += R[c] * R[c];
+                        rz2 += R[c] * ID[c] * R[c];
                     }
         }
-        double be = rr2 / rr;
+        double be = rz2 / rz;
         rr = rr2;
+        rz = rz2;
         for (int b = 0; b < nbox; ++b) {
             fab_t f = make_fab(dim, &ba[b], 1);
             int lo2 = (dim == 3) ? ba[b].lo[2] : 0, hi2 = (dim == 3) ? ba[b].hi[2] : 0;
@@ -2292,12 +2324,13 @@ int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, co
                 for (int j = ba[b].lo[1]; j <= ba[b].hi[1]; ++j)
                     for (int i = ba[b].lo[0]; i <= ba[b].hi[0]; ++i) {
                         long long c = ba[b].offset + fab_idx(&f, i, j, k);
-                        Pv[c] = R[c] + be * Pv[c];
+                        Pv[c] = ID[c] * R[c] + be * Pv[c];
                     }
         }
         ++it;
+        /* the rounding floor: no new minimum of r.r over a window long enough for CG's plateaus */
         if (rr < best) { best = rr; stall = 0; }
-        else if (++stall > 50) break;   /* rounding floor reached */
+        else if (++stall > 400) break;
     }
     /* Grad_Q on every face */
     int e = ref_level_fill_pf(dim, G, nbox, ba, 1, 1, X);
@@ -2325,7 +2358,7 @@ int ref_level_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, co
         }
     }
     if (iters) *iters = it;
-    free(Bf); free(X); free(R); free(Pv); free(AP); free(foff); free(ba);
+    free(Bf); free(X); free(R); free(Pv); free(AP); free(ID); free(foff); free(ba);
     return err;
 }
 

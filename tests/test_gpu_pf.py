@@ -350,3 +350,42 @@ def test_pf_rk_step_other_integrators_match_oracle(gpu, integrator):
     lev.close()
     rel = maxrel_per_var(got, ref)
     assert rel.max() <= TOL_STEP, rel
+
+
+@pytest.mark.parametrize("dim,sizes", [(2, (64, 128, 256, 512)), (3, (16, 32, 64))])
+def test_pf_implicit_multigrid_iterations_stay_flat(gpu, dim, sizes):
+    """The implicit Grad_Q's CG is preconditioned by a multigrid V-cycle when the level covers its
+    domain: its iteration count to the rounding floor stays nearly flat as the grid is refined (Jacobi
+    preconditioning needs O(n) iterations on an n-cell-wide level), and the result is the oracle's."""
+    import time
+    import torch
+    its = []
+    for n in sizes:
+        c = _interface_case(n, dim=dim, width=2.0, periodic=False, max_grid_size=max(16, n // 4))
+        P, G = c["params"], c["geom"]
+        pf = make_pf_params(mechanism="CAC-Adv", explicit_rc=0, eta_multiplier=0.5)
+        lay = Layout(dim, c["boxes"], 3, P.nstate)
+        flay = FaceLayout(dim, c["boxes"])
+        buf = lay.pack(c["U"])
+        lev = Level(P, G, lay)
+        lev.pf_setup(pf)
+        U = _dev(torch, buf, gpu)
+        gq = torch.zeros(flay.total, dtype=torch.float64, device=gpu)
+        dt = 0.3 / n
+        lev.PhaseField_GradQPF(U, dt, gq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lev.PhaseField_GradQPF(U, dt, gq)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        assert lev.check_errors() == 0
+        it = lev.solver_iterations()
+        its.append(it)
+        print("implicit Grad_Q %dD n=%d: %d CG iterations, %.2f ms per Grad_Q" % (dim, n, it, 1e3 * el), flush=True)
+        if n == sizes[0]:
+            ref, _, _ = oracle.level_pf_gradq_step(P, pf, G, lay, buf, dt, flay.total)
+            got = gq.cpu().numpy()
+            assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(np.abs(ref))
+        lev.close()
+    assert max(its) <= 45, its
+    assert max(its) <= 2.5 * min(its), its

@@ -127,12 +127,12 @@ def test_implicit_solve_reproduces_a_manufactured_solution():
     assert np.max(np.abs(gy[1:, :] - ey)) < 1e-10 * np.max(np.abs(ey))
 
 
-@pytest.mark.parametrize("periodic", [True, False])
-def test_implicit_solution_satisfies_its_equation(periodic):
+@pytest.mark.parametrize("periodic,n", [(True, 24), (False, 24), (False, 64)])
+def test_implicit_solution_satisfies_its_equation(periodic, n):
     """Variable b from an interface; the output faces satisfy div(b Grad_Q) = rhs - mean(rhs)
-    and, on Neumann sides, Grad_Q = 0 on the domain faces."""
-    n = 24
-    c = _interface_case(n, width=2.0, periodic=periodic, max_grid_size=12)
+    and, on Neumann sides, Grad_Q = 0 on the domain faces. (At 64^2 with Neumann sides, plain CG
+    stopped by a 50-iteration stall window ended at a 0.38 relative residual.)"""
+    c = _interface_case(n, width=2.0, periodic=periodic, max_grid_size=12 if n == 24 else 16)
     P, G = c["params"], c["geom"]
     pf = make_pf_params(mechanism="CAC-Adv", explicit_rc=0, eta_factor=2.0)
     alay = Layout(2, c["boxes"], 1, 1)
