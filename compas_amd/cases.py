@@ -421,3 +421,50 @@ def nphase_rings_2d(n=512, seed=0, boxes=None, scheme="WENO5", riemann="HLLC"):
     U = np_prim_to_cons(params, W)
     boxes = boxes if boxes is not None else uneven_boxes((n, n), seed=seed)
     return dict(params=params, geom=geom, U=U, boxes=boxes, shape=(n, n), cfl=0.4, name="nphase-rings-2d")
+
+
+def interface_hierarchy(dim, nc, fine_boxes, ratio=2, periodic=True, width=2.0, eos=None, nphase=0,
+                        velocity=(1.0, 0.5, -0.25), max_grid_size=None):
+    """A circle (sphere) of phase 1 of radius 0.25 about the domain centre in phase 2, uniform u and p,
+    tanh profile of width*dx at the finest level, on a hierarchy: level 0 nc cells a side, every
+    finer level the refinement of the one before by ratio with the boxes fine_boxes[l - 1] (in that
+    level's cells). The finest level's field is evaluated at its cell centres and every coarser
+    level's is the mean of it over each cell (the state AverageDown leaves). eos: dict(gamma, pinf);
+    nphase 3: phase 2 split 65/35 into two phases of its material. Returns dict(params, geoms, boxes,
+    fields, shapes) with the coarsest level first."""
+    eos = eos or dict(gamma=(1.4, 1.6), pinf=(0.0, 0.0))
+    nlev = 1 + len(fine_boxes)
+    nf = nc * ratio ** (nlev - 1)
+    P = make_params(dim, **eos)
+    shape_f = (nf,) * dim
+    Gf = make_geometry(shape_f, (0.0,) * dim, (1.0,) * dim, (int(periodic),) * dim)
+    X = _centres(Gf, shape_f)
+    r = np.sqrt(sum((X[d] - 0.5) ** 2 for d in range(dim)))
+    a1 = np.clip(0.5 * (1.0 + np.tanh((0.25 - r) / (width * Gf.dx[0]))), 1e-8, 1 - 1e-8)
+    sg = eos["pinf"][0] > 0
+    W = np.empty((P.nstate,) + shape_f)
+    W[0] = a1 * (1000.0 if sg else 1.0)
+    W[1] = (1 - a1) * (1.2 if sg else 10.0)
+    for m in range(dim):
+        W[2 + m] = velocity[m]
+    W[2 + dim] = 1.0e5 if sg else 1.0
+    W[3 + dim] = a1
+    U = prim_to_cons(P, W)
+    c = dict(params=P, U=U)
+    if nphase:
+        c = nphase_from_two_phase(c, nphase=nphase)
+    P, U = c["params"], c["U"]
+    fields, shapes, geoms, boxes = [], [], [], []
+    for l in range(nlev):
+        n = nc * ratio ** l
+        f = ratio ** (nlev - 1 - l)
+        Ul = U.reshape((U.shape[0],) + tuple(x for s in shape_f for x in (s // f, f)))
+        Ul = Ul.mean(axis=tuple(2 + 2 * a for a in range(dim))) if f > 1 else U
+        fields.append(np.ascontiguousarray(Ul))
+        shapes.append(Ul.shape[1:])
+        geoms.append(make_geometry((n,) * dim, (0.0,) * dim, (1.0,) * dim, (int(periodic),) * dim))
+        if l == 0:
+            boxes.append(split_domain((nc,) * dim, max_grid_size or max(8, nc // 2)))
+        else:
+            boxes.append([(list(lo), list(hi)) for lo, hi in fine_boxes[l - 1]])
+    return dict(params=P, geoms=geoms, boxes=boxes, fields=fields, shapes=shapes)

@@ -107,6 +107,15 @@ def load():
         "ref_amr_rk3_step": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
                                             ctypes.POINTER(PD), D, ctypes.c_int, PD]),
         "ref_amr_compute_dt": (D, [ctypes.c_int, PG, PD, D, ctypes.c_int, D]),
+        "ref_amr_pf_gradq_step": (ctypes.c_int, [PP, PF, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
+                                                 ctypes.POINTER(PD), D, ctypes.c_int, ctypes.POINTER(PD), PD]),
+        "ref_amr_pf_gradq_implicit": (ctypes.c_int, [PP, PF, ctypes.c_int, PG, PI, ctypes.POINTER(PB),
+                                                     ctypes.POINTER(PD), ctypes.POINTER(PD), D, ctypes.c_int,
+                                                     ctypes.POINTER(PD), ctypes.POINTER(PD), PI]),
+        "ref_amr_pf_evolve": (ctypes.c_int, [PP, PF, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
+                                             ctypes.POINTER(PD), D, ctypes.c_int, PD]),
+        "ref_amr_advance_pf": (ctypes.c_int, [PP, PF, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
+                                              ctypes.POINTER(PD), ctypes.c_int, D, ctypes.c_int, ctypes.c_int, PD]),
         "ref_amr_advance": (ctypes.c_int, [PP, ctypes.c_int, PG, PI, ctypes.POINTER(PB), ctypes.c_int,
                                            ctypes.POINTER(PD), ctypes.c_int, D, ctypes.c_int, ctypes.c_int, PD]),
         "ref_level_tag": (ctypes.c_int, [PP, ctypes.c_int, PB, ctypes.c_int, PD,
@@ -674,6 +683,65 @@ def amr_advance(params, geoms, layouts, bufs, nsteps, cfl, ratio, limiter=1):
                                 int(limiter), int(ratio), _dptr(hist))
     if rc:
         raise RuntimeError("ref_amr_advance failed: %d" % rc)
+    return hist[:nsteps]
+
+
+def amr_pf_gradq_step(params, pf, geoms, layouts, bufs, dt, ratio):
+    """Grad_Q of every level of a hierarchy for a flow step of size dt (PhaseField_Evolve's
+    mechanism and Grad_Q over the levels): ([face buffer per level, cmp_face_layout with one
+    component per volume fraction], [max|u|, mobility, eta, mechanism sub-steps, CG iterations])."""
+    from compas_amd import FaceLayout
+    n, G, nb, B, U = _amr_args(geoms, layouts, bufs)
+    npf = params.nphase if params.nphase else 1
+    gqs = [np.zeros(FaceLayout(params.dim, l.boxes, npf).total) for l in layouts]
+    GQ = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(g) for g in gqs])
+    info = np.zeros(5)
+    rc = load().ref_amr_pf_gradq_step(ctypes.byref(params), ctypes.byref(pf), n, G, nb, B, layouts[0].ng, U,
+                                      float(dt), int(ratio), GQ, _dptr(info))
+    if rc:
+        raise RuntimeError("ref_amr_pf_gradq_step failed: %d" % rc)
+    return gqs, list(info)
+
+
+def amr_pf_gradq_implicit(params, pf, geoms, alayouts, alphas, rhss, eta, ratio):
+    """The composite implicit solve of a two-phase hierarchy: alphas and rhss are 1-comp, 1-ghost
+    level buffers (alpha's ghosts are filled in place); returns ([Grad_Q per level], [b per level],
+    CG iterations)."""
+    from compas_amd import FaceLayout
+    n, G, nb, B, A = _amr_args(geoms, alayouts, alphas)
+    R = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(r) for r in rhss])
+    gqs = [np.zeros(FaceLayout(params.dim, l.boxes).total) for l in alayouts]
+    bfs = [np.zeros(FaceLayout(params.dim, l.boxes).total) for l in alayouts]
+    GQ = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(g) for g in gqs])
+    BF = (ctypes.POINTER(ctypes.c_double) * n)(*[_dptr(g) for g in bfs])
+    it = ctypes.c_int(0)
+    rc = load().ref_amr_pf_gradq_implicit(ctypes.byref(params), ctypes.byref(pf), n, G, nb, B, A, R, float(eta),
+                                          int(ratio), GQ, BF, ctypes.byref(it))
+    if rc:
+        raise RuntimeError("ref_amr_pf_gradq_implicit failed: %d" % rc)
+    return gqs, bfs, it.value
+
+
+def amr_pf_evolve(params, pf, geoms, layouts, bufs, dt, ratio):
+    """The Phase-Field step of a hierarchy, in place on bufs; returns [max|u|, mobility, eta, mechanism
+    sub-steps, max|Grad_Q|, Phase-Field sub-steps, CG iterations]."""
+    n, G, nb, B, U = _amr_args(geoms, layouts, bufs)
+    info = np.zeros(7)
+    rc = load().ref_amr_pf_evolve(ctypes.byref(params), ctypes.byref(pf), n, G, nb, B, layouts[0].ng, U, float(dt),
+                                  int(ratio), _dptr(info))
+    if rc:
+        raise RuntimeError("ref_amr_pf_evolve failed: %d" % rc)
+    return list(info)
+
+
+def amr_advance_pf(params, pf, geoms, layouts, bufs, nsteps, cfl, ratio, limiter=1):
+    """nsteps flow + Phase-Field steps of a hierarchy in place on bufs; returns every step's dt."""
+    n, G, nb, B, U = _amr_args(geoms, layouts, bufs)
+    hist = np.zeros(max(1, nsteps))
+    rc = load().ref_amr_advance_pf(ctypes.byref(params), ctypes.byref(pf), n, G, nb, B, layouts[0].ng, U,
+                                   int(nsteps), float(cfl), int(limiter), int(ratio), _dptr(hist))
+    if rc:
+        raise RuntimeError("ref_amr_advance_pf failed: %d" % rc)
     return hist[:nsteps]
 
 

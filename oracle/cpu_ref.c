@@ -12,6 +12,10 @@
  *
  * The checks that pin each function independently (golden vectors, closed forms, invariants,
  * brute force) are the tests under tests/ named for it; every function here has at least one.
+ * One part is parity unpinned against the reference: the composite implicit Grad_Q solve of a
+ * hierarchy uses its own two-point coarse-fine flux where the reference's MLMG (AMReX, not
+ * vendored) uses a higher-order coarse-fine stencil; it is pinned against its own equation,
+ * symmetry and the single-level solve, not against the reference's numbers at the interfaces.
  */
 #include "cpu_ref.h"
 
@@ -2824,12 +2828,11 @@ static int floor_div(int a, int r) { return (a >= 0) ? a / r : -((-a + r - 1) / 
 /* The two-level fill of the cells of the boxes dboxes (buffer Ud; all_cells: valid and ghost cells,
  * else the ghost cells only), the same-level sources taken from the boxes sboxes (buffer Us) of
  * the same level. */
-static int fill_two_levels_impl(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
-                                double* Uf, int nbs, const cmp_box* sboxes, int ngs, const double* Us, int all_cells,
-                                const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc, const double* Uc,
-                                int ratio)
+static int fill_two_levels_nc(int dim, int N, int mom0, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes,
+                              int ngf, double* Uf, int nbs, const cmp_box* sboxes, int ngs, const double* Us,
+                              int all_cells, const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc,
+                              const double* Uc, int ratio)
 {
-    const int dim = P->dim, N = P->nstate, mom0 = P->nphase ? P->nphase : 2;
     int err = 0;
     for (int b = 0; b < nbf; ++b) {
         const cmp_box* bx = &fboxes[b];
@@ -2890,6 +2893,15 @@ static int fill_two_levels_impl(const cmp_params* P, const cmp_geometry* Gf, int
                 }
     }
     return err;
+}
+
+static int fill_two_levels_impl(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
+                                double* Uf, int nbs, const cmp_box* sboxes, int ngs, const double* Us, int all_cells,
+                                const cmp_geometry* Gc, int nbc, const cmp_box* cboxes, int ngc, const double* Uc,
+                                int ratio)
+{
+    return fill_two_levels_nc(P->dim, P->nstate, P->nphase ? P->nphase : 2, Gf, nbf, fboxes, ngf, Uf, nbs, sboxes, ngs,
+                              Us, all_cells, Gc, nbc, cboxes, ngc, Uc, ratio);
 }
 
 int ref_level_fill_two_levels(const cmp_params* P, const cmp_geometry* Gf, int nbf, const cmp_box* fboxes, int ngf,
@@ -3158,6 +3170,1146 @@ int ref_amr_advance(const cmp_params* P, int nlev, const cmp_geometry* G, const 
         err = ref_amr_rk3_step(P, nlev, G, nbox, boxes, ng, U, dt, ratio, cmax);
         for (int l = 0; l < nlev; ++l)
             if (isnan(cmax[l])) err = CMP_ERR_NAN;
+    }
+    return err;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * The Phase-Field step of a hierarchy without subcycling (run.do_subcycle = 0): PhaseField_Evolve
+ * (source/Class_Compressible_PhaseField.cpp:425-594) over finest_level + 1 levels.
+ *   - MaxVelocity over every level (:3577-3590); eta from the finest cell size (:433-443); mobility
+ *     0.5 eta max|u|.
+ *   - PFBorder: the step's volume fractions with their ghost cells, every phase in one FillPatch
+ *     (:485-493): level 0 same-level, finer levels two-level (the coarse-fine ghosts by the
+ *     interpolation of the flow's two-level fill, its limiter shared by the phases); PF_bcs int_dir
+ *     on periodic sides, foextrap elsewhere.
+ *   - PhaseFieldMechanism_Evolve (:597-709): sub-steps of dt_m = 0.9/(2 M Lambda)/(sum_d 1/dx_d^2 +
+ *     1/eta^2) with the finest level's dx, every level advanced by each sub-step after the filled
+ *     sub-step state (FillPatch of PF_new, all phases):
+ *       CAC (PhaseField_Mechanism :1007-1152): per phase the cell normals of alpha (PhaseField_Normal
+ *       :1958-2032, valid cells, then their own FillPatch), mech and W per level, AverageDown of both
+ *       (the coarse cells under a finer level take the mean of their children), their sums over
+ *       level 0, Div_J = mech + W L at every level;
+ *       ACDI (:1155-1233, PhaseField_ACDI :1495-1644): psi of the valid cells and its FillPatch, per
+ *       phase the normals of psi and J on every face of every level, then AverageDownFaces(J), Div_J
+ *       = div J; Jhat += dt_m J after the averaging (explicit RC);
+ *       alpha += dt_m Div_J on every level's valid cells.
+ *   - Grad_Q (:2133-2403): explicit RC per level from Jhat/dt and PFBorder; implicit RC per phase:
+ *     b = W(phi_f) + 1e-3 on every face of every level from PFBorder, AverageDownFaces(b), the
+ *     composite solve below, AverageDownFaces(Grad_Q).
+ *   - Phase-Field sub-steps: PhaseField_ComputeDt (:2037-2098) over the levels with the no-subcycling
+ *     minimum, each sub-step the stage-coupled TVD-RK3 of the Phase-Field right-hand side over the
+ *     levels (PhaseField_AdvanceAllLevels, Class_Compressible_PhaseField_StageCoupling.cpp:43-51, the
+ *     coarse faces under a finer level taking the mean of its faces before the divergence) and
+ *     AverageDown (PhaseField_TimeStepNoSubcycling :3010-3030).
+ *
+ * The composite solve (the reference calls AMReX's MLMG with composite_solve = 1,
+ * include/Operators_LinearSystem/SolveABecLaplacianAllLevels_HomBC.H:11-62, MLMG not vendored):
+ * the unknowns are Q on the cells no finer level covers; every face between two such cells carries
+ * the flux b (Q_n - Q_c) / h; a coarse-fine face carries, per fine face on it, b_fine (Q_coarse -
+ * Q_fine) / ((dx_fine + dx_coarse) / 2) (the fine ghost value linear between the two cell centres
+ * along the normal), which is the coarse cell's flux through that face too (conservative and
+ * symmetric). The volume-weighted system is solved by Jacobi-preconditioned conjugate gradients on
+ * the volume-weighted zero-mean subspace to the rounding floor; covered cells then take the mean of
+ * their children; Grad_Q is (Q_c - Q_m)/h on a level's faces between two of its cells, the two-point
+ * difference on its coarse-fine faces, zero on Neumann domain faces. MLMG's coarse-fine stencil
+ * (LinOp_maxorder 3, LinearSystem_Parameter.H:18) differs from this two-point flux at the
+ * interfaces: parity of the hierarchy's implicit Grad_Q with the reference is unpinned there.
+ * ---------------------------------------------------------------------------------------------- */
+#define AMR_NLEV_MAX 8
+
+typedef struct {
+    int dim, nlev, ratio;
+    cmp_geometry PG[AMR_NLEV_MAX];   /* PF_bcs: int_dir on periodic sides, foextrap elsewhere */
+    int nbox[AMR_NLEV_MAX];
+    const cmp_box* boxes[AMR_NLEV_MAX];
+    cmp_box* a1[AMR_NLEV_MAX];       /* 1 ghost, 1 comp */
+    cmp_box* ap[AMR_NLEV_MAX];       /* 1 ghost, npf comps */
+    cmp_box* an[AMR_NLEV_MAX];       /* 1 ghost, dim comps */
+    long long n1[AMR_NLEV_MAX], np[AMR_NLEV_MAX], nn[AMR_NLEV_MAX];
+    long long* f1[AMR_NLEV_MAX];     /* face offsets, 1 comp */
+    long long* fp[AMR_NLEV_MAX];     /* face offsets, npf comps */
+    long long nf1[AMR_NLEV_MAX], nfp[AMR_NLEV_MAX];
+    int npf;
+} hier_t;
+
+static void hier_init(hier_t* H, const cmp_params* P, int nlev, const cmp_geometry* G, const int* nbox,
+                      const cmp_box* const* boxes, int ratio)
+{
+    memset(H, 0, sizeof(*H));
+    H->dim = P->dim;
+    H->nlev = nlev;
+    H->ratio = ratio;
+    H->npf = P->nphase ? P->nphase : 1;
+    for (int l = 0; l < nlev; ++l) {
+        H->PG[l] = G[l];
+        for (int d = 0; d < 3; ++d)
+            if (!H->PG[l].is_periodic[d]) { H->PG[l].lo_bc[d] = 2; H->PG[l].hi_bc[d] = 2; }
+        H->nbox[l] = nbox[l];
+        H->boxes[l] = boxes[l];
+        H->a1[l] = pf_boxes(P->dim, nbox[l], boxes[l], 1, 1, &H->n1[l]);
+        H->ap[l] = pf_boxes(P->dim, nbox[l], boxes[l], 1, H->npf, &H->np[l]);
+        H->an[l] = pf_boxes(P->dim, nbox[l], boxes[l], 1, P->dim, &H->nn[l]);
+        H->f1[l] = (long long*)malloc(sizeof(long long) * (size_t)(nbox[l] * P->dim + 1));
+        H->fp[l] = (long long*)malloc(sizeof(long long) * (size_t)(nbox[l] * P->dim + 1));
+        cmp_face_layout(P->dim, nbox[l], boxes[l], 1, H->f1[l], &H->nf1[l]);
+        cmp_face_layout(P->dim, nbox[l], boxes[l], H->npf, H->fp[l], &H->nfp[l]);
+    }
+}
+
+static void hier_free(hier_t* H)
+{
+    for (int l = 0; l < H->nlev; ++l) {
+        free(H->a1[l]); free(H->ap[l]); free(H->an[l]); free(H->f1[l]); free(H->fp[l]);
+    }
+}
+
+/* FillPatch of an nc-component Phase-Field field (1 ghost) on every level; lay: its layouts */
+static int hier_fill(const hier_t* H, cmp_box* const* lay, int nc, double* const* F)
+{
+    int err = ref_level_fill_pf(H->dim, &H->PG[0], H->nbox[0], lay[0], 1, nc, F[0]);
+    for (int l = 1; l < H->nlev; ++l) {
+        int e = fill_two_levels_nc(H->dim, nc, 1000, &H->PG[l], H->nbox[l], lay[l], 1, F[l], H->nbox[l], lay[l], 1,
+                                   F[l], 0, &H->PG[l - 1], H->nbox[l - 1], lay[l - 1], 1, F[l - 1], H->ratio);
+        if (e && !err) err = e;
+    }
+    return err;
+}
+
+/* AverageDown of an nc-component cell field (lay: 1-ghost layouts), finest level first */
+static int hier_avgdown(const hier_t* H, const cmp_params* P, cmp_box* const* lay, int nc, double* const* F)
+{
+    cmp_params Pn = *P;
+    Pn.nstate = nc;
+    int err = 0;
+    for (int l = H->nlev - 2; l >= 0; --l) {
+        int e = ref_level_average_down(&Pn, H->nbox[l + 1], lay[l + 1], 1, F[l + 1], H->nbox[l], lay[l], 1, F[l],
+                                       H->ratio);
+        if (e && !err) err = e;
+    }
+    return err;
+}
+
+/* AverageDownFaces of an nc-component face buffer (face offsets foff per level), finest level
+ * first: every coarse face of the coarsened fine boxes (and its periodic images) takes the mean of
+ * the fine faces on it, the lower transverse index fastest, times 1/ratio^(dim-1). */
+static void hier_avgdown_faces(const hier_t* H, long long* const* foff, int nc, double* const* Fb)
+{
+    const int dim = H->dim, ratio = H->ratio;
+    int rt = 1;
+    for (int e = 1; e < dim; ++e) rt *= ratio;
+    const double facInv = 1.0 / (double)rt;
+    for (int l = H->nlev - 2; l >= 0; --l) {
+        const cmp_geometry* Gc = &H->PG[l];
+        const int nbf = H->nbox[l + 1], nbc = H->nbox[l];
+        const cmp_box* fb = H->boxes[l + 1];
+        const cmp_box* cb = H->boxes[l];
+        for (int b = 0; b < nbf; ++b)
+            for (int d = 0; d < dim; ++d) {
+                int ffe[3];
+                face_extents(dim, &fb[b], d, ffe);
+                const long long nff = (long long)ffe[0] * ffe[1] * ffe[2];
+                const double* Ff = Fb[l + 1] + foff[l + 1][b * dim + d];
+                int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+                for (int e = 0; e < dim; ++e) {
+                    clo[e] = fb[b].lo[e] / ratio;
+                    chi[e] = (fb[b].hi[e] + 1) / ratio - 1 + (e == d ? 1 : 0);
+                }
+                int t1 = -1, t2 = -1;
+                for (int e = 0; e < dim; ++e)
+                    if (e != d) { if (t1 < 0) t1 = e; else t2 = e; }
+                const int r1 = (t1 >= 0) ? ratio : 1, r2 = (t2 >= 0) ? ratio : 1;
+                for (int K = clo[2]; K <= chi[2]; ++K)
+                    for (int J = clo[1]; J <= chi[1]; ++J)
+                        for (int I = clo[0]; I <= chi[0]; ++I) {
+                            const int C[3] = {I, J, K};
+                            double avg[REF_NMAX];
+                            for (int n = 0; n < nc; ++n) {
+                                double c = 0.0;
+                                for (int o2 = 0; o2 < r2; ++o2)
+                                    for (int o1 = 0; o1 < r1; ++o1) {
+                                        int fcell[3] = {0, 0, 0};
+                                        for (int e = 0; e < dim; ++e) fcell[e] = C[e] * ratio;
+                                        if (t1 >= 0) fcell[t1] += o1;
+                                        if (t2 >= 0) fcell[t2] += o2;
+                                        long long q = 0;
+                                        for (int e = dim - 1; e >= 0; --e) q = q * ffe[e] + (fcell[e] - fb[b].lo[e]);
+                                        c += Ff[q + n * nff];
+                                    }
+                                avg[n] = c * facInv;
+                            }
+                            int nimg = 1;
+                            int pos[3] = {C[d], C[d], C[d]};
+                            const int ncd = Gc->domain_hi[d] - Gc->domain_lo[d] + 1;
+                            if (Gc->is_periodic[d]) {
+                                if (C[d] == Gc->domain_lo[d]) pos[nimg++] = C[d] + ncd;
+                                else if (C[d] == Gc->domain_hi[d] + 1) pos[nimg++] = C[d] - ncd;
+                            }
+                            for (int m = 0; m < nimg; ++m) {
+                                int F3[3] = {C[0], C[1], C[2]};
+                                F3[d] = pos[m];
+                                for (int q = 0; q < nbc; ++q) {
+                                    int cfe[3];
+                                    face_extents(dim, &cb[q], d, cfe);
+                                    int in = 1;
+                                    for (int e = 0; e < dim; ++e)
+                                        if (F3[e] < cb[q].lo[e] || F3[e] > cb[q].lo[e] + cfe[e] - 1) in = 0;
+                                    if (!in) continue;
+                                    const long long ncf = (long long)cfe[0] * cfe[1] * cfe[2];
+                                    long long qq = 0;
+                                    for (int e = dim - 1; e >= 0; --e) qq = qq * cfe[e] + (F3[e] - cb[q].lo[e]);
+                                    double* Fc = Fb[l] + foff[l][q * dim + d];
+                                    for (int n = 0; n < nc; ++n) Fc[qq + n * ncf] = avg[n];
+                                }
+                            }
+                        }
+            }
+    }
+}
+
+/* The box of level l holding cell c, after the periodic wrap (-1: none; *outside = 1 when c lies
+ * beyond a non-periodic domain face). c is wrapped in place. */
+static int hier_owner(const hier_t* H, int l, int* c, int* outside)
+{
+    *outside = 0;
+    const cmp_geometry* G = &H->PG[l];
+    for (int d = 0; d < H->dim; ++d) {
+        const int lo = G->domain_lo[d], hi = G->domain_hi[d], n = hi - lo + 1;
+        if (c[d] < lo || c[d] > hi) {
+            if (!G->is_periodic[d]) { *outside = 1; return -1; }
+            c[d] = (c[d] < lo) ? c[d] + n : c[d] - n;
+        }
+    }
+    return find_box(H->dim, H->nbox[l], H->boxes[l], c);
+}
+
+static int hier_covered(const hier_t* H, int l, const int* c)
+{
+    if (l + 1 >= H->nlev) return 0;
+    int f[3] = {0, 0, 0};
+    for (int d = 0; d < H->dim; ++d) f[d] = c[d] * H->ratio;
+    return find_box(H->dim, H->nbox[l + 1], H->boxes[l + 1], f) >= 0;
+}
+
+/* index (1-comp, 1-ghost layout) of cell c of box b of level l */
+static long long hier_idx(const hier_t* H, int l, int b, const int* c)
+{
+    fab_t f = make_fab(H->dim, &H->a1[l][b], 1);
+    return H->a1[l][b].offset + fab_idx(&f, c[0], c[1], c[2]);
+}
+
+/* the face index (1-comp face buffer) of the face of box b of level l between cell c - e_d and c */
+static long long hier_face(const hier_t* H, int l, int b, int d, const int* c)
+{
+    int fe[3];
+    face_extents(H->dim, &H->boxes[l][b], d, fe);
+    long long q = 0;
+    for (int e = H->dim - 1; e >= 0; --e) q = q * fe[e] + (c[e] - H->boxes[l][b].lo[e]);
+    return H->f1[l][b * H->dim + d] + q;
+}
+
+/* One link of composite cell (l, b, c) across its side s of direction d: up to ratio^(dim-1)
+ * neighbours (level, box, cell) with their transmissibilities b area / distance. Returns the count. */
+typedef struct { int l, b, c[3]; double T; } link_t;
+
+static int hier_links(const hier_t* H, double* const* Bf, int l, int b, const int* c, int d, int s, link_t* out)
+{
+    const int dim = H->dim, ratio = H->ratio;
+    const cmp_geometry* G = &H->PG[l];
+    int x[3] = {c[0], c[1], c[2]};
+    x[d] += s ? 1 : -1;
+    int outside;
+    const int nb = hier_owner(H, l, x, &outside);
+    if (outside) return 0;
+    double area = 1.0;
+    for (int e = 0; e < dim; ++e)
+        if (e != d) area *= G->dx[e];
+    if (nb >= 0 && !hier_covered(H, l, x)) {
+        int fc[3] = {c[0], c[1], c[2]};
+        fc[d] += s;
+        out[0].l = l; out[0].b = nb;
+        for (int e = 0; e < 3; ++e) out[0].c[e] = x[e];
+        out[0].T = Bf[l][hier_face(H, l, b, d, fc)] * area / G->dx[d];
+        return 1;
+    }
+    if (nb >= 0) {
+        /* the finer level covers x: the fine cells next to this face, each with its fine face */
+        const cmp_geometry* Gf = &H->PG[l + 1];
+        double farea = 1.0;
+        for (int e = 0; e < dim; ++e)
+            if (e != d) farea *= Gf->dx[e];
+        const double dist = 0.5 * (Gf->dx[d] + G->dx[d]);
+        int t1 = -1, t2 = -1;
+        for (int e = 0; e < dim; ++e)
+            if (e != d) { if (t1 < 0) t1 = e; else t2 = e; }
+        const int r1 = (t1 >= 0) ? ratio : 1, r2 = (t2 >= 0) ? ratio : 1;
+        int n = 0;
+        for (int o2 = 0; o2 < r2; ++o2)
+            for (int o1 = 0; o1 < r1; ++o1) {
+                int f[3] = {0, 0, 0}, ff[3];
+                for (int e = 0; e < dim; ++e) f[e] = x[e] * ratio;
+                f[d] = s ? x[d] * ratio : (x[d] + 1) * ratio - 1;
+                if (t1 >= 0) f[t1] = c[t1] * ratio + o1;
+                if (t2 >= 0) f[t2] = c[t2] * ratio + o2;
+                for (int e = 0; e < 3; ++e) ff[e] = f[e];
+                ff[d] += s ? 0 : 1;       /* the fine face between f and the coarse side */
+                const int fb = find_box(dim, H->nbox[l + 1], H->boxes[l + 1], f);
+                out[n].l = l + 1; out[n].b = fb;
+                for (int e = 0; e < 3; ++e) out[n].c[e] = f[e];
+                out[n].T = Bf[l + 1][hier_face(H, l + 1, fb, d, ff)] * farea / dist;
+                ++n;
+            }
+        return n;
+    }
+    /* x is not on level l: the coarse cell holding it, through this box's face */
+    const cmp_geometry* Gc = &H->PG[l - 1];
+    int p[3] = {0, 0, 0};
+    for (int e = 0; e < dim; ++e) p[e] = floor_div(x[e], ratio);
+    int po;
+    const int pb = hier_owner(H, l - 1, p, &po);
+    int fc[3] = {c[0], c[1], c[2]};
+    fc[d] += s;
+    out[0].l = l - 1; out[0].b = pb;
+    for (int e = 0; e < 3; ++e) out[0].c[e] = p[e];
+    out[0].T = Bf[l][hier_face(H, l, b, d, fc)] * area / (0.5 * (G->dx[d] + Gc->dx[d]));
+    return 1;
+}
+
+/* y = A x (volume weighted, A = -div(b grad .)) on the composite cells; diag (or NULL): its diagonal */
+static void hier_apply(const hier_t* H, double* const* Bf, double* const* X, double* const* Y, double* const* Dg)
+{
+    const int dim = H->dim;
+    for (int l = 0; l < H->nlev; ++l)
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            const cmp_box* B = &H->boxes[l][b];
+            const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                    for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                        const int c[3] = {i, j, k};
+                        const long long ci = hier_idx(H, l, b, c);
+                        if (hier_covered(H, l, c)) { if (Y) Y[l][ci] = 0.0; if (Dg) Dg[l][ci] = 0.0; continue; }
+                        double acc = 0.0, dsum = 0.0;
+                        for (int d = 0; d < dim; ++d)
+                            for (int s = 0; s < 2; ++s) {
+                                link_t lk[4];
+                                const int n = hier_links(H, Bf, l, b, c, d, s, lk);
+                                for (int q = 0; q < n; ++q) {
+                                    dsum += lk[q].T;
+                                    if (X) acc += lk[q].T * (X[l][ci] - X[lk[q].l][hier_idx(H, lk[q].l, lk[q].b, lk[q].c)]);
+                                }
+                            }
+                        if (Y) Y[l][ci] = acc;
+                        if (Dg) Dg[l][ci] = dsum;
+                    }
+        }
+}
+
+static double hier_vol(const hier_t* H, int l)
+{
+    double v = 1.0;
+    for (int d = 0; d < H->dim; ++d) v *= H->PG[l].dx[d];
+    return v;
+}
+
+/* sum over composite cells of w(l) a b (a NULL: of w(l) b) */
+static double hier_dot(const hier_t* H, double* const* A, double* const* B, int weighted)
+{
+    const int dim = H->dim;
+    double s = 0.0;
+    for (int l = 0; l < H->nlev; ++l) {
+        const double w = weighted ? hier_vol(H, l) : 1.0;
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            const cmp_box* Bx = &H->boxes[l][b];
+            const int lo2 = (dim == 3) ? Bx->lo[2] : 0, hi2 = (dim == 3) ? Bx->hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = Bx->lo[1]; j <= Bx->hi[1]; ++j)
+                    for (int i = Bx->lo[0]; i <= Bx->hi[0]; ++i) {
+                        const int c[3] = {i, j, k};
+                        if (hier_covered(H, l, c)) continue;
+                        const long long ci = hier_idx(H, l, b, c);
+                        s += w * (A ? A[l][ci] : 1.0) * B[l][ci];
+                    }
+        }
+    }
+    return s;
+}
+
+/* The composite solve of div(b grad Q) = rhs (rhs and Q: 1-comp 1-ghost layouts of every level;
+ * Bf: b on every face of every level, the coarse faces under a finer level averaged down), then
+ * Grad_Q on every face of every level (averaged down). *iters: CG iterations. */
+static int hier_solve(const hier_t* H, const cmp_params* P, double* const* Bf, double* const* rhs, double* const* Gq,
+                      int* iters)
+{
+    const int dim = H->dim, nlev = H->nlev;
+    double *X[AMR_NLEV_MAX] = {NULL}, *R[AMR_NLEV_MAX] = {NULL}, *Pv[AMR_NLEV_MAX] = {NULL};
+    double *AP[AMR_NLEV_MAX] = {NULL}, *ID[AMR_NLEV_MAX] = {NULL};
+    for (int l = 0; l < nlev; ++l) {
+        X[l] = (double*)calloc((size_t)H->n1[l], sizeof(double));
+        R[l] = (double*)calloc((size_t)H->n1[l], sizeof(double));
+        Pv[l] = (double*)calloc((size_t)H->n1[l], sizeof(double));
+        AP[l] = (double*)calloc((size_t)H->n1[l], sizeof(double));
+        ID[l] = (double*)calloc((size_t)H->n1[l], sizeof(double));
+    }
+    hier_apply(H, Bf, NULL, NULL, ID);
+    /* volume-weighted mean of rhs; r = V (-(rhs - mean)), the composite cells only */
+    double sv = 0.0, srhs = 0.0;
+    for (int l = 0; l < nlev; ++l) {
+        const double v = hier_vol(H, l);
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            const cmp_box* Bx = &H->boxes[l][b];
+            const int lo2 = (dim == 3) ? Bx->lo[2] : 0, hi2 = (dim == 3) ? Bx->hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = Bx->lo[1]; j <= Bx->hi[1]; ++j)
+                    for (int i = Bx->lo[0]; i <= Bx->hi[0]; ++i) {
+                        const int c[3] = {i, j, k};
+                        if (hier_covered(H, l, c)) continue;
+                        sv += v;
+                        srhs += v * rhs[l][hier_idx(H, l, b, c)];
+                    }
+        }
+    }
+    const double mean = srhs / sv;
+    double rr = 0.0, rz = 0.0;
+    for (int l = 0; l < nlev; ++l) {
+        const double v = hier_vol(H, l);
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            const cmp_box* Bx = &H->boxes[l][b];
+            const int lo2 = (dim == 3) ? Bx->lo[2] : 0, hi2 = (dim == 3) ? Bx->hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = Bx->lo[1]; j <= Bx->hi[1]; ++j)
+                    for (int i = Bx->lo[0]; i <= Bx->hi[0]; ++i) {
+                        const int c[3] = {i, j, k};
+                        if (hier_covered(H, l, c)) continue;
+                        const long long ci = hier_idx(H, l, b, c);
+                        const double dg = ID[l][ci];
+                        ID[l][ci] = (dg > 0.0) ? 1.0 / dg : 0.0;
+                        R[l][ci] = -(rhs[l][ci] - mean) * v;
+                        Pv[l][ci] = ID[l][ci] * R[l][ci];
+                        rr += R[l][ci] * R[l][ci];
+                        rz += R[l][ci] * Pv[l][ci];
+                    }
+        }
+    }
+    const double b2 = rr;
+    double best = rr;
+    int it = 0, stall = 0;
+    while (rr > 1.0e-28 * b2 && it < 100000 && b2 > 0.0) {
+        hier_apply(H, Bf, Pv, AP, NULL);
+        const double pAp = hier_dot(H, Pv, AP, 0);
+        if (!(pAp > 0.0)) break;
+        const double al = rz / pAp;
+        double rr2 = 0.0, rz2 = 0.0;
+        for (int l = 0; l < nlev; ++l)
+            for (long long q = 0; q < H->n1[l]; ++q) {
+                X[l][q] += al * Pv[l][q];
+                R[l][q] -= al * AP[l][q];
+            }
+        rr2 = hier_dot(H, R, R, 0);
+        for (int l = 0; l < nlev; ++l)
+            for (long long q = 0; q < H->n1[l]; ++q) AP[l][q] = ID[l][q] * R[l][q];
+        rz2 = hier_dot(H, R, AP, 0);
+        const double be = rz2 / rz;
+        for (int l = 0; l < nlev; ++l)
+            for (long long q = 0; q < H->n1[l]; ++q) Pv[l][q] = AP[l][q] + be * Pv[l][q];
+        rr = rr2;
+        rz = rz2;
+        ++it;
+        if (rr < best) { best = rr; stall = 0; }
+        else if (++stall > 400) break;
+    }
+    /* covered cells: the mean of their children (finest first) */
+    int err = hier_avgdown(H, P, H->a1, 1, X);
+    /* Grad_Q on every face of every level */
+    for (int l = 0; l < nlev; ++l) {
+        const cmp_geometry* G = &H->PG[l];
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            const cmp_box* Bx = &H->boxes[l][b];
+            for (int d = 0; d < dim; ++d) {
+                int fe[3];
+                face_extents(dim, Bx, d, fe);
+                const int lo2 = (dim == 3) ? Bx->lo[2] : 0;
+                for (int kk = 0; kk < fe[2]; ++kk)
+                    for (int jj = 0; jj < fe[1]; ++jj)
+                        for (int ii = 0; ii < fe[0]; ++ii) {
+                            int c[3] = {Bx->lo[0] + ii, Bx->lo[1] + jj, lo2 + kk};
+                            const long long fi = hier_face(H, l, b, d, c);
+                            const int cd = c[d];
+                            if (!G->is_periodic[d] && (cd == G->domain_lo[d] || cd == G->domain_hi[d] + 1)) {
+                                Gq[l][fi] = 0.0;
+                                continue;
+                            }
+                            int m[3] = {c[0], c[1], c[2]};
+                            m[d] -= 1;
+                            int oc, om;
+                            int cc[3] = {c[0], c[1], c[2]};
+                            const int bc = hier_owner(H, l, cc, &oc);
+                            const int bm = hier_owner(H, l, m, &om);
+                            double v;
+                            if (bc >= 0 && bm >= 0) {
+                                v = (X[l][hier_idx(H, l, bc, cc)] - X[l][hier_idx(H, l, bm, m)]) / G->dx[d];
+                            } else {
+                                /* a coarse-fine face of this box: the coarse cell beyond it */
+                                const int in_c = (bc >= 0);
+                                int o[3] = {in_c ? m[0] : cc[0], in_c ? m[1] : cc[1], in_c ? m[2] : cc[2]};
+                                int p[3] = {0, 0, 0}, po;
+                                for (int e = 0; e < dim; ++e) p[e] = floor_div(o[e], H->ratio);
+                                const int pb = hier_owner(H, l - 1, p, &po);
+                                const double xp = X[l - 1][hier_idx(H, l - 1, pb, p)];
+                                const double dist = 0.5 * (G->dx[d] + H->PG[l - 1].dx[d]);
+                                v = in_c ? (X[l][hier_idx(H, l, bc, cc)] - xp) / dist
+                                         : (xp - X[l][hier_idx(H, l, bm, m)]) / dist;
+                            }
+                            Gq[l][fi] = v;
+                        }
+            }
+        }
+    }
+    hier_avgdown_faces(H, H->f1, 1, Gq);
+    if (iters) *iters = it;
+    for (int l = 0; l < nlev; ++l) { free(X[l]); free(R[l]); free(Pv[l]); free(AP[l]); free(ID[l]); }
+    return err;
+}
+
+/* the cell normals (component d of every valid cell, then the dim-component field's own FillPatch)
+ * of component ph of the npf-component field F (central differences, 0 where the gradient is 0;
+ * FDM_Normal_CenterFromCenter_D.H:12-63) on every level */
+static int hier_normals(const hier_t* H, double* const* F, int ph, double* const* Nrm)
+{
+    const int dim = H->dim;
+    for (int l = 0; l < H->nlev; ++l) {
+        const cmp_geometry* G = &H->PG[l];
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            fab_t fa = make_fab(dim, &H->ap[l][b], 1);
+            fab_t fn = make_fab(dim, &H->an[l][b], 1);
+            const double* Ab = F[l] + H->ap[l][b].offset + ph * fa.sn;
+            double* Nb = Nrm[l] + H->an[l][b].offset;
+            const cmp_box* B = &H->boxes[l][b];
+            const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+            for (int k = lo2; k <= hi2; ++k)
+                for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                    for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                        const int c3[3] = {i, j, k};
+                        const long long c = fab_idx(&fa, i, j, k);
+                        double g[3], sum = 0.0;
+                        for (int d = 0; d < dim; ++d) {
+                            int m3[3] = {i, j, k}, p3[3] = {i, j, k};
+                            m3[d] -= 1; p3[d] += 1;
+                            const double gm = (Ab[c] - Ab[fab_idx(&fa, m3[0], m3[1], m3[2])]) / G->dx[d];
+                            const double gp = (Ab[fab_idx(&fa, p3[0], p3[1], p3[2])] - Ab[c]) / G->dx[d];
+                            g[d] = 0.5 * gm + 0.5 * gp;
+                            sum += g[d] * g[d];
+                        }
+                        (void)c3;
+                        const long long cn = fab_idx(&fn, i, j, k);
+                        for (int d = 0; d < dim; ++d) Nb[cn + d * fn.sn] = (sum == 0.0) ? 0.0 : g[d] / sqrt(sum);
+                    }
+        }
+    }
+    return hier_fill(H, H->an, dim, Nrm);
+}
+
+/* the step's volume fractions (every phase) of the flow states U (ng ghosts) into A0 (npf comps,
+ * 1 ghost), filled */
+static int hier_extract(const hier_t* H, const cmp_params* P, int ng, double* const* U, double* const* A0)
+{
+    const int dim = H->dim;
+    for (int l = 0; l < H->nlev; ++l)
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            const cmp_box* B = &H->boxes[l][b];
+            fab_t fu = make_fab(dim, B, ng);
+            fab_t fa = make_fab(dim, &H->ap[l][b], 1);
+            const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+            for (int ph = 0; ph < H->npf; ++ph) {
+                const int comp = P->nphase ? P->nphase + dim + 1 + ph : 3 + dim;
+                for (int k = lo2; k <= hi2; ++k)
+                    for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                        for (int i = B->lo[0]; i <= B->hi[0]; ++i)
+                            A0[l][H->ap[l][b].offset + fab_idx(&fa, i, j, k) + ph * fa.sn] =
+                                U[l][B->offset + fab_idx(&fu, i, j, k) + comp * fu.sn];
+            }
+        }
+    return hier_fill(H, H->ap, H->npf, A0);
+}
+
+/* b = W(phi_f) + 1e-3 on every face of every level (phi_f the face mean of phase ph of the npf-comp
+ * field A, its ghosts filled, re-sharpened through the mappings at eta and eta_factor eta;
+ * :2205-2274), then AverageDownFaces(b) */
+static void hier_bcoef(const hier_t* H, const cmp_pf_params* PF, double* const* A, int ph, double eta, double* const* Bf)
+{
+    const int dim = H->dim;
+    for (int l = 0; l < H->nlev; ++l)
+        for (int b = 0; b < H->nbox[l]; ++b) {
+            fab_t fa = make_fab(dim, &H->ap[l][b], 1);
+            const cmp_box* B = &H->boxes[l][b];
+            const int lo2 = (dim == 3) ? B->lo[2] : 0;
+            const double* Ab = A[l] + H->ap[l][b].offset + ph * fa.sn;
+            for (int d = 0; d < dim; ++d) {
+                int fe[3];
+                face_extents(dim, B, d, fe);
+                for (int kk = 0; kk < fe[2]; ++kk)
+                    for (int jj = 0; jj < fe[1]; ++jj)
+                        for (int ii = 0; ii < fe[0]; ++ii) {
+                            int c3[3] = {B->lo[0] + ii, B->lo[1] + jj, lo2 + kk};
+                            int m3[3] = {c3[0], c3[1], c3[2]};
+                            m3[d] -= 1;
+                            const double c = ref_pf_map_forward(
+                                ref_pf_map_backward(Ab[fab_idx(&fa, c3[0], c3[1], c3[2])], eta), PF->eta_factor * eta, 0);
+                            const double m = ref_pf_map_forward(
+                                ref_pf_map_backward(Ab[fab_idx(&fa, m3[0], m3[1], m3[2])], eta), PF->eta_factor * eta, 0);
+                            const double pf = 0.5 * m + 0.5 * c;
+                            const long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                            Bf[l][H->f1[l][b * dim + d] + q] = pf * (1.0 - pf) + 1.0e-3;
+                        }
+            }
+        }
+    hier_avgdown_faces(H, H->f1, 1, Bf);
+}
+
+/* The composite implicit solve alone, for a two-phase alpha (1 comp, 1 ghost, its ghosts filled
+ * here) and a right-hand side (1 comp, 1 ghost layout) on every level: Grad_Q (1 comp face buffers)
+ * and the face coefficients b it used (averaged down). */
+int ref_amr_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G,
+                              const int* nbox, const cmp_box* const* boxes, double* const* alpha,
+                              double* const* rhs, double eta, int ratio, double* const* gq, double* const* bf,
+                              int* iters)
+{
+    if (nlev < 1 || nlev > AMR_NLEV_MAX || P->nphase) return CMP_ERR_ARG;
+    hier_t H;
+    hier_init(&H, P, nlev, G, nbox, boxes, ratio);
+    int err = hier_fill(&H, H.ap, 1, alpha);
+    hier_bcoef(&H, PF, alpha, 0, eta, bf);
+    int e = hier_solve(&H, P, bf, rhs, gq, iters);
+    if (e && !err) err = e;
+    hier_free(&H);
+    return err;
+}
+
+/* Grad_Q of every level (face buffers gq[l] of cmp_face_layout with npf comps) for a flow step of
+ * size dt from the hierarchy's states U; info: max|u|, mobility, eta, mechanism sub-steps, CG
+ * iterations of the last implicit solve. */
+int ref_amr_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G,
+                          const int* nbox, const cmp_box* const* boxes, int ng, double* const* U, double dt,
+                          int ratio, double* const* gq, double* info)
+{
+    if (nlev < 1 || nlev > AMR_NLEV_MAX) return CMP_ERR_ARG;
+    const int dim = P->dim;
+    hier_t H;
+    hier_init(&H, P, nlev, G, nbox, boxes, ratio);
+    const int npf = H.npf;
+    const int cac = (PF->mechanism == CMP_PF_CAC);
+    const int explicit_rc = (PF->explicit_rc == 1);
+    int err = 0;
+    /* MaxVelocity over the levels; Lambda, eta (finest cell size), mobility */
+    double maxv = 0.0;
+    for (int l = 0; l < nlev; ++l) {
+        const double v = ref_level_max_velocity(P, nbox[l], boxes[l], ng, U[l]);
+        if (maxv < v || isnan(v)) maxv = v;
+    }
+    double lambda, eta, mob;
+    ref_pf_coefficients(PF, &G[nlev - 1], maxv, &lambda, &eta, &mob);
+    double *A0[AMR_NLEV_MAX], *A[AMR_NLEV_MAX], *Ps[AMR_NLEV_MAX], *Nrm[AMR_NLEV_MAX], *J[AMR_NLEV_MAX];
+    double *Jh[AMR_NLEV_MAX], *Mk[AMR_NLEV_MAX], *Wk[AMR_NLEV_MAX];
+    for (int l = 0; l < nlev; ++l) {
+        A0[l] = (double*)calloc((size_t)H.np[l], sizeof(double));
+        A[l] = (double*)calloc((size_t)H.np[l], sizeof(double));
+        Ps[l] = (double*)calloc((size_t)H.np[l], sizeof(double));
+        Nrm[l] = (double*)calloc((size_t)H.nn[l], sizeof(double));
+        J[l] = (double*)calloc((size_t)H.nfp[l], sizeof(double));
+        Jh[l] = (double*)calloc((size_t)H.nfp[l], sizeof(double));
+        Mk[l] = (double*)calloc((size_t)H.n1[l], sizeof(double));
+        Wk[l] = (double*)calloc((size_t)H.n1[l], sizeof(double));
+    }
+    int e = hier_extract(&H, P, ng, U, A0);
+    if (e && !err) err = e;
+    for (int l = 0; l < nlev; ++l) memcpy(A[l], A0[l], sizeof(double) * (size_t)H.np[l]);
+    /* PhaseFieldMechanism_Evolve */
+    const double* dxf = G[nlev - 1].dx;
+    double sdx = 1.0 / dxf[0] / dxf[0];
+    for (int d = 1; d < dim; ++d) sdx = sdx + 1.0 / dxf[d] / dxf[d];
+    double dt_m = 0.9 / (2.0 * mob * lambda) / (sdx + 1.0 / eta / eta);
+    double t = 0.0;
+    const double T = dt;
+    int nsub = 0;
+    while (t < T) {
+        if (t + dt_m > T) dt_m = T - t;
+        e = hier_fill(&H, H.ap, npf, A);
+        if (e && !err) err = e;
+        if (cac) {
+            for (int ph = 0; ph < npf; ++ph) {
+                e = hier_normals(&H, A, ph, Nrm);
+                if (e && !err) err = e;
+                for (int l = 0; l < nlev; ++l)
+                    for (int b = 0; b < nbox[l]; ++b) {
+                        fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                        fab_t fn = make_fab(dim, &H.an[l][b], 1);
+                        fab_t f1 = make_fab(dim, &H.a1[l][b], 1);
+                        const double* Ab = A[l] + H.ap[l][b].offset + ph * fa.sn;
+                        const double* Nb = Nrm[l] + H.an[l][b].offset;
+                        const cmp_box* B = &boxes[l][b];
+                        const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+                        for (int k = lo2; k <= hi2; ++k)
+                            for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                                for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                                    const long long c = fab_idx(&fa, i, j, k);
+                                    double fm[3], fp[3], nc[3], nm[3], np_[3];
+                                    for (int d = 0; d < dim; ++d) {
+                                        int m3[3] = {i, j, k}, p3[3] = {i, j, k};
+                                        m3[d] -= 1; p3[d] += 1;
+                                        fm[d] = Ab[fab_idx(&fa, m3[0], m3[1], m3[2])];
+                                        fp[d] = Ab[fab_idx(&fa, p3[0], p3[1], p3[2])];
+                                        nc[d] = Nb[fab_idx(&fn, i, j, k) + d * fn.sn];
+                                        nm[d] = Nb[fab_idx(&fn, m3[0], m3[1], m3[2]) + d * fn.sn];
+                                        np_[d] = Nb[fab_idx(&fn, p3[0], p3[1], p3[2]) + d * fn.sn];
+                                    }
+                                    const long long c1 = H.a1[l][b].offset + fab_idx(&f1, i, j, k);
+                                    Mk[l][c1] = ref_pf_cac_mech(dim, G[l].dx, Ab[c], fm, fp, nc, nm, np_, mob, lambda,
+                                                                eta);
+                                    Wk[l][c1] = Ab[c] * (1.0 - Ab[c]);
+                                }
+                    }
+                e = hier_avgdown(&H, P, H.a1, 1, Mk);
+                if (e && !err) err = e;
+                e = hier_avgdown(&H, P, H.a1, 1, Wk);
+                if (e && !err) err = e;
+                double sm = 0.0, sw = 0.0;
+                for (int b = 0; b < nbox[0]; ++b) {
+                    fab_t f1 = make_fab(dim, &H.a1[0][b], 1);
+                    const cmp_box* B = &boxes[0][b];
+                    const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+                    for (int k = lo2; k <= hi2; ++k)
+                        for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                            for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                                const long long c1 = H.a1[0][b].offset + fab_idx(&f1, i, j, k);
+                                sm += Mk[0][c1];
+                                sw += Wk[0][c1];
+                            }
+                }
+                const double L = (sw == 0.0) ? 0.0 : -sm / sw;
+                /* Div_J of this phase; alpha advances after every phase's Div_J (LinComb) */
+                for (int l = 0; l < nlev; ++l)
+                    for (long long q = 0; q < H.n1[l]; ++q) Mk[l][q] = Mk[l][q] + Wk[l][q] * L;
+                for (int l = 0; l < nlev; ++l)
+                    for (int b = 0; b < nbox[l]; ++b) {
+                        fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                        fab_t f1 = make_fab(dim, &H.a1[l][b], 1);
+                        const cmp_box* B = &boxes[l][b];
+                        const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+                        for (int k = lo2; k <= hi2; ++k)
+                            for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                                for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                                    /* the phase's Div_J into Ps (a holding field of npf comps) */
+                                    Ps[l][H.ap[l][b].offset + fab_idx(&fa, i, j, k) + ph * fa.sn] =
+                                        Mk[l][H.a1[l][b].offset + fab_idx(&f1, i, j, k)];
+                                }
+                    }
+            }
+            for (int l = 0; l < nlev; ++l)
+                for (int b = 0; b < nbox[l]; ++b) {
+                    fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                    const cmp_box* B = &boxes[l][b];
+                    const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+                    for (int ph = 0; ph < npf; ++ph)
+                        for (int k = lo2; k <= hi2; ++k)
+                            for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                                for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                                    const long long c = H.ap[l][b].offset + fab_idx(&fa, i, j, k) + ph * fa.sn;
+                                    A[l][c] = 1.0 * A[l][c] + dt_m * Ps[l][c];
+                                }
+                }
+        } else {
+            /* psi of the valid cells, then its own FillPatch (LSBorder) */
+            for (int l = 0; l < nlev; ++l)
+                for (int b = 0; b < nbox[l]; ++b) {
+                    fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                    const cmp_box* B = &boxes[l][b];
+                    const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+                    for (int ph = 0; ph < npf; ++ph)
+                        for (int k = lo2; k <= hi2; ++k)
+                            for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                                for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                                    const long long c = H.ap[l][b].offset + fab_idx(&fa, i, j, k) + ph * fa.sn;
+                                    Ps[l][c] = ref_pf_map_backward(A[l][c], eta);
+                                }
+                }
+            e = hier_fill(&H, H.ap, npf, Ps);
+            if (e && !err) err = e;
+            for (int ph = 0; ph < npf; ++ph) {
+                e = hier_normals(&H, Ps, ph, Nrm);
+                if (e && !err) err = e;
+                for (int l = 0; l < nlev; ++l)
+                    for (int b = 0; b < nbox[l]; ++b) {
+                        fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                        fab_t fn = make_fab(dim, &H.an[l][b], 1);
+                        const double* Ab = A[l] + H.ap[l][b].offset + ph * fa.sn;
+                        const double* Nb = Nrm[l] + H.an[l][b].offset;
+                        const cmp_box* B = &boxes[l][b];
+                        const int lo2 = (dim == 3) ? B->lo[2] : 0;
+                        for (int d = 0; d < dim; ++d) {
+                            int fe[3];
+                            face_extents(dim, B, d, fe);
+                            const long long nface = (long long)fe[0] * fe[1] * fe[2];
+                            double* Jd = J[l] + H.fp[l][b * dim + d] + ph * nface;
+                            for (int kk = 0; kk < fe[2]; ++kk)
+                                for (int jj = 0; jj < fe[1]; ++jj)
+                                    for (int ii = 0; ii < fe[0]; ++ii) {
+                                        int c3[3] = {B->lo[0] + ii, B->lo[1] + jj, lo2 + kk};
+                                        int m3[3] = {c3[0], c3[1], c3[2]};
+                                        m3[d] -= 1;
+                                        const long long c = fab_idx(&fa, c3[0], c3[1], c3[2]);
+                                        const long long m = fab_idx(&fa, m3[0], m3[1], m3[2]);
+                                        const long long cn = fab_idx(&fn, c3[0], c3[1], c3[2]);
+                                        const long long mn = fab_idx(&fn, m3[0], m3[1], m3[2]);
+                                        const double mob_face = 0.5 * mob + 0.5 * mob;
+                                        const long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                                        Jd[q] = ref_pf_acdi_flux(Ab[m], Ab[c], Nb[mn + d * fn.sn], Nb[cn + d * fn.sn],
+                                                                 mob_face, eta, G[l].dx[d]);
+                                    }
+                        }
+                    }
+            }
+            hier_avgdown_faces(&H, H.fp, npf, J);
+            for (int l = 0; l < nlev; ++l) {
+                if (explicit_rc)
+                    for (long long q = 0; q < H.nfp[l]; ++q) Jh[l][q] += dt_m * J[l][q];
+                for (int b = 0; b < nbox[l]; ++b) {
+                    fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                    const cmp_box* B = &boxes[l][b];
+                    int nv[3] = {1, 1, 1};
+                    for (int d = 0; d < dim; ++d) nv[d] = B->hi[d] - B->lo[d] + 1;
+                    const int lo2 = (dim == 3) ? B->lo[2] : 0;
+                    for (int ph = 0; ph < npf; ++ph)
+                        for (int kk = 0; kk < nv[2]; ++kk)
+                            for (int jj = 0; jj < nv[1]; ++jj)
+                                for (int ii = 0; ii < nv[0]; ++ii) {
+                                    const int idx[3] = {ii, jj, kk};
+                                    double div = 0.0;
+                                    for (int d = 0; d < dim; ++d) {
+                                        int fe[3] = {nv[0], nv[1], nv[2]};
+                                        fe[d] += 1;
+                                        const long long nface = (long long)fe[0] * fe[1] * fe[2];
+                                        const double* Jd = J[l] + H.fp[l][b * dim + d] + ph * nface;
+                                        const long long qm = idx[0] + (long long)fe[0] * (idx[1] + (long long)fe[1] * idx[2]);
+                                        const long long qp = qm + (d == 0 ? 1 : (d == 1 ? fe[0] : (long long)fe[0] * fe[1]));
+                                        const double term = (Jd[qp] - Jd[qm]) / G[l].dx[d];
+                                        div = (d == 0) ? term : div + term;
+                                    }
+                                    const long long c = H.ap[l][b].offset +
+                                                        fab_idx(&fa, B->lo[0] + ii, B->lo[1] + jj, lo2 + kk) + ph * fa.sn;
+                                    A[l][c] = 1.0 * A[l][c] + dt_m * div;
+                                }
+                }
+            }
+        }
+        t += dt_m;
+        ++nsub;
+    }
+    /* Grad_Q */
+    int iters = 0;
+    if (explicit_rc) {
+        const double sc = 1.0 / dt;
+        for (int l = 0; l < nlev; ++l) {
+            for (long long q = 0; q < H.nfp[l]; ++q) Jh[l][q] *= sc;
+            for (int b = 0; b < nbox[l]; ++b) {
+                fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                const cmp_box* B = &boxes[l][b];
+                const int lo2 = (dim == 3) ? B->lo[2] : 0;
+                for (int d = 0; d < dim; ++d) {
+                    int fe[3];
+                    face_extents(dim, B, d, fe);
+                    const long long nface = (long long)fe[0] * fe[1] * fe[2];
+                    for (int ph = 0; ph < npf; ++ph) {
+                        const double* Ab = A0[l] + H.ap[l][b].offset + ph * fa.sn;
+                        const double* Hd = Jh[l] + H.fp[l][b * dim + d] + ph * nface;
+                        double* Gd = gq[l] + H.fp[l][b * dim + d] + ph * nface;
+                        for (int kk = 0; kk < fe[2]; ++kk)
+                            for (int jj = 0; jj < fe[1]; ++jj)
+                                for (int ii = 0; ii < fe[0]; ++ii) {
+                                    int c3[3] = {B->lo[0] + ii, B->lo[1] + jj, lo2 + kk};
+                                    int m3[3] = {c3[0], c3[1], c3[2]};
+                                    m3[d] -= 1;
+                                    const long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                                    Gd[q] = ref_pf_gradq_face(Ab[fab_idx(&fa, m3[0], m3[1], m3[2])],
+                                                              Ab[fab_idx(&fa, c3[0], c3[1], c3[2])], Hd[q], eta,
+                                                              PF->eta_factor);
+                                }
+                    }
+                }
+            }
+        }
+    } else {
+        double *Bf[AMR_NLEV_MAX], *Dj[AMR_NLEV_MAX], *Gx[AMR_NLEV_MAX];
+        for (int l = 0; l < nlev; ++l) {
+            Bf[l] = (double*)calloc((size_t)H.nf1[l], sizeof(double));
+            Gx[l] = (double*)calloc((size_t)H.nf1[l], sizeof(double));
+            Dj[l] = (double*)calloc((size_t)H.n1[l], sizeof(double));
+        }
+        for (int ph = 0; ph < npf; ++ph) {
+            for (int l = 0; l < nlev; ++l)
+                for (int b = 0; b < nbox[l]; ++b) {
+                    fab_t fa = make_fab(dim, &H.ap[l][b], 1);
+                    fab_t f1 = make_fab(dim, &H.a1[l][b], 1);
+                    const cmp_box* B = &boxes[l][b];
+                    const int lo2 = (dim == 3) ? B->lo[2] : 0, hi2 = (dim == 3) ? B->hi[2] : 0;
+                    const double* Ab = A0[l] + H.ap[l][b].offset + ph * fa.sn;
+                    /* Div_Jhat = LinComb(1/dt, PF_new, -1/dt, PF) (:703) */
+                    for (int k = lo2; k <= hi2; ++k)
+                        for (int j = B->lo[1]; j <= B->hi[1]; ++j)
+                            for (int i = B->lo[0]; i <= B->hi[0]; ++i) {
+                                const long long c = H.ap[l][b].offset + fab_idx(&fa, i, j, k) + ph * fa.sn;
+                                Dj[l][H.a1[l][b].offset + fab_idx(&f1, i, j, k)] =
+                                    (1.0 / dt) * A[l][c] + (-1.0 / dt) * A0[l][c];
+                            }
+                    (void)Ab;
+                }
+            hier_bcoef(&H, PF, A0, ph, eta, Bf);
+            e = hier_solve(&H, P, Bf, Dj, Gx, &iters);
+            if (e && !err) err = e;
+            /* phase ph's component of Grad_Q */
+            for (int l = 0; l < nlev; ++l)
+                for (int b = 0; b < nbox[l]; ++b)
+                    for (int d = 0; d < dim; ++d) {
+                        int fe[3];
+                        face_extents(dim, &boxes[l][b], d, fe);
+                        const long long nface = (long long)fe[0] * fe[1] * fe[2];
+                        for (long long q = 0; q < nface; ++q)
+                            gq[l][H.fp[l][b * dim + d] + ph * nface + q] = Gx[l][H.f1[l][b * dim + d] + q];
+                    }
+        }
+        for (int l = 0; l < nlev; ++l) { free(Bf[l]); free(Gx[l]); free(Dj[l]); }
+    }
+    if (info) {
+        info[0] = maxv;
+        info[1] = mob;
+        info[2] = eta;
+        info[3] = (double)nsub;
+        info[4] = (double)iters;
+    }
+    for (int l = 0; l < nlev; ++l) {
+        free(A0[l]); free(A[l]); free(Ps[l]); free(Nrm[l]); free(J[l]); free(Jh[l]); free(Mk[l]); free(Wk[l]);
+    }
+    hier_free(&H);
+    return err;
+}
+
+/* One stage-coupled TVD-RK3 sub-step of the Phase-Field right-hand side over the levels
+ * (PhaseField_AdvanceAllLevels -> AdvanceAllLevelsStageCoupled with Grad_Q,
+ * Class_Compressible_PhaseField_StageCoupling.cpp:43-51, 84-153; PhaseField_ComputeFaceFluxes stores
+ * -fhat, :840): per stage the flow fill of every level, the faces of every box, the coarse faces
+ * under a finer level averaged down, -div F and the stage combination, AverageDown. gq: per level a
+ * face buffer with npf comps. */
+static int amr_pf_rk3_step(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G,
+                           const int* nbox, const cmp_box* const* boxes, int ng, double* const* Un,
+                           double* const* gq, double dt, int ratio)
+{
+    const int dim = P->dim, N = P->nstate, npf = P->nphase ? P->nphase : 1;
+    double *Ub[AMR_NLEV_MAX], *Unew[AMR_NLEV_MAX];
+    double **F[AMR_NLEV_MAX], **FN[AMR_NLEV_MAX];
+    long long tot[AMR_NLEV_MAX];
+    long long* foff[AMR_NLEV_MAX];
+    for (int l = 0; l < nlev; ++l) {
+        tot[l] = level_total(dim, nbox[l], boxes[l], ng, N);
+        Ub[l] = (double*)malloc(sizeof(double) * tot[l]);
+        Unew[l] = (double*)malloc(sizeof(double) * tot[l]);
+        F[l] = (double**)calloc((size_t)nbox[l] * 3, sizeof(double*));
+        FN[l] = (double**)calloc((size_t)nbox[l] * 3, sizeof(double*));
+        foff[l] = (long long*)malloc(sizeof(long long) * (size_t)(nbox[l] * dim + 1));
+        cmp_face_layout(dim, nbox[l], boxes[l], npf, foff[l], NULL);
+    }
+    int err = 0, e;
+    const double w2 = 0.25 * dt, w3 = (2.0 / 3.0) * dt;
+    for (int stage = 0; stage < 3; ++stage) {
+        for (int l = 0; l < nlev; ++l) memcpy(Ub[l], stage == 0 ? Un[l] : Unew[l], sizeof(double) * tot[l]);
+        e = amr_fill(P, nlev, G, nbox, boxes, ng, Ub, ratio);
+        if (e && !err) err = e;
+        for (int l = 0; l < nlev; ++l)
+            for (int b = 0; b < nbox[l]; ++b) {
+                const cmp_box* bx = &boxes[l][b];
+                for (int d = 0; d < dim; ++d) {
+                    int fe[3];
+                    face_extents(dim, bx, d, fe);
+                    const long long nface = (long long)fe[0] * fe[1] * fe[2];
+                    F[l][b * 3 + d] = (double*)malloc(sizeof(double) * nface * N);
+                    FN[l][b * 3 + d] = (double*)calloc((size_t)nface, sizeof(double));
+                    const double* gd = gq[l] + foff[l][b * dim + d];
+                    const int lo2 = (dim == 3) ? bx->lo[2] : 0;
+                    for (int kk = 0; kk < fe[2]; ++kk)
+                        for (int jj = 0; jj < fe[1]; ++jj)
+                            for (int ii = 0; ii < fe[0]; ++ii) {
+                                const long long q = ii + (long long)fe[0] * (jj + (long long)fe[1] * kk);
+                                double fh[REF_NMAX], c, gv[CMP_MAX_PHASES];
+                                for (int m = 0; m < npf; ++m) gv[m] = gd[q + m * nface];
+                                e = ref_pf_face_flux(P, PF, &G[l], bx, ng, Ub[l] + bx->offset, bx->lo[0] + ii,
+                                                     bx->lo[1] + jj, lo2 + kk, d, gv, fh, &c);
+                                if (e && !err) err = e;
+                                for (int n = 0; n < N; ++n) F[l][b * 3 + d][q + n * nface] = -fh[n];
+                            }
+                }
+            }
+        for (int l = nlev - 2; l >= 0; --l)
+            average_faces(P, &G[l], nbox[l + 1], boxes[l + 1], F[l + 1], FN[l + 1], nbox[l], boxes[l], F[l], FN[l],
+                          ratio);
+        for (int l = 0; l < nlev; ++l)
+            for (int b = 0; b < nbox[l]; ++b) {
+                const cmp_box* bx = &boxes[l][b];
+                fab_t f = make_fab(dim, bx, ng);
+                int nv[3] = {1, 1, 1};
+                for (int d = 0; d < dim; ++d) nv[d] = bx->hi[d] - bx->lo[d] + 1;
+                const int lo2 = (dim == 3) ? bx->lo[2] : 0;
+                for (int n = 0; n < N; ++n)
+                    for (int kk = 0; kk < nv[2]; ++kk)
+                        for (int jj = 0; jj < nv[1]; ++jj)
+                            for (int ii = 0; ii < nv[0]; ++ii) {
+                                const int idx[3] = {ii, jj, kk};
+                                double div = 0.0;
+                                for (int d = 0; d < dim; ++d) {
+                                    int fe[3] = {nv[0], nv[1], nv[2]};
+                                    fe[d] += 1;
+                                    const long long nface = (long long)fe[0] * fe[1] * fe[2];
+                                    const long long qm = idx[0] + (long long)fe[0] * (idx[1] + (long long)fe[1] * idx[2]);
+                                    const long long qp = qm + (d == 0 ? 1 : (d == 1 ? fe[0] : (long long)fe[0] * fe[1]));
+                                    const double* Fd = F[l][b * 3 + d];
+                                    const double term = (Fd[qp + n * nface] - Fd[qm + n * nface]) / G[l].dx[d];
+                                    div = (d == 0) ? term : div + term;
+                                }
+                                const double Lq = -div;
+                                const long long q = bx->offset + fab_idx(&f, bx->lo[0] + ii, bx->lo[1] + jj, lo2 + kk) + n * f.sn;
+                                if (stage == 0) {
+                                    Unew[l][q] = 1.0 * Ub[l][q] + dt * Lq;
+                                } else if (stage == 1) {
+                                    Unew[l][q] = 0.25 * Ub[l][q] + 0.75 * Un[l][q];
+                                    Unew[l][q] += w2 * Lq;
+                                } else {
+                                    Unew[l][q] = (2.0 / 3.0) * Ub[l][q] + (1.0 / 3.0) * Un[l][q];
+                                    Unew[l][q] += w3 * Lq;
+                                }
+                            }
+            }
+        for (int l = 0; l < nlev; ++l)
+            for (int q = 0; q < nbox[l] * 3; ++q) { free(F[l][q]); free(FN[l][q]); F[l][q] = FN[l][q] = NULL; }
+        for (int l = nlev - 2; l >= 0; --l) {
+            e = ref_level_average_down(P, nbox[l + 1], boxes[l + 1], ng, Unew[l + 1], nbox[l], boxes[l], ng, Unew[l],
+                                       ratio);
+            if (e && !err) err = e;
+        }
+    }
+    for (int l = 0; l < nlev; ++l) {
+        for (int b = 0; b < nbox[l]; ++b) {
+            const cmp_box* bx = &boxes[l][b];
+            fab_t f = make_fab(dim, bx, ng);
+            int nv[3] = {1, 1, 1};
+            for (int d = 0; d < dim; ++d) nv[d] = bx->hi[d] - bx->lo[d] + 1;
+            const int lo2 = (dim == 3) ? bx->lo[2] : 0;
+            for (int n = 0; n < N; ++n)
+                for (int kk = 0; kk < nv[2]; ++kk)
+                    for (int jj = 0; jj < nv[1]; ++jj)
+                        for (int ii = 0; ii < nv[0]; ++ii) {
+                            const long long q = bx->offset + fab_idx(&f, bx->lo[0] + ii, bx->lo[1] + jj, lo2 + kk) + n * f.sn;
+                            Un[l][q] = Unew[l][q];
+                        }
+        }
+        free(Ub[l]); free(Unew[l]); free(F[l]); free(FN[l]); free(foff[l]);
+    }
+    return err;
+}
+
+/* The Phase-Field step of a hierarchy after a flow step of size dt (see above), in place on U.
+ * info (7 doubles): max|u|, mobility, eta, mechanism sub-steps, max |Grad_Q| over the levels,
+ * Phase-Field sub-steps, CG iterations of the last implicit solve. */
+int ref_amr_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G, const int* nbox,
+                      const cmp_box* const* boxes, int ng, double* const* U, double dt, int ratio, double* info)
+{
+    if (nlev < 1 || nlev > AMR_NLEV_MAX) return CMP_ERR_ARG;
+    const int dim = P->dim, npf = P->nphase ? P->nphase : 1;
+    double* gq[AMR_NLEV_MAX] = {NULL};
+    long long nf[AMR_NLEV_MAX];
+    for (int l = 0; l < nlev; ++l) {
+        cmp_face_layout(dim, nbox[l], boxes[l], npf, NULL, &nf[l]);
+        gq[l] = (double*)calloc((size_t)nf[l], sizeof(double));
+    }
+    double inf5[5] = {0, 0, 0, 0, 0};
+    int err = ref_amr_pf_gradq_step(P, PF, nlev, G, nbox, boxes, ng, U, dt, ratio, gq, inf5);
+    double gm[AMR_NLEV_MAX], gmax = 0.0;
+    for (int l = 0; l < nlev; ++l) {
+        gm[l] = 0.0;
+        for (long long q = 0; q < nf[l]; ++q) gm[l] = smax(gm[l], fabs(gq[l][q]));
+        gmax = smax(gmax, gm[l]);
+    }
+    /* sub-steps: PhaseField_ComputeDt over the levels (nsubsteps 1, run.do_subcycle = 0) */
+    double prev[AMR_NLEV_MAX];
+    double pf_time = 0.0;
+    int n = 0;
+    for (int step = 0; step < PF->max_step && pf_time < dt && !err; ++step) {
+        double tmp[AMR_NLEV_MAX] = {0.0};
+        for (int l = 0; l < nlev; ++l) {
+            double dxm = G[l].dx[0];
+            for (int d = 1; d < dim; ++d) dxm = smin(dxm, G[l].dx[d]);
+            tmp[l] = PF->cfl * dxm / (gm[l] + 1.0e-14);
+            if (pf_time == 0.0) prev[l] = tmp[l];
+        }
+        double dt0 = tmp[0];
+        for (int l = 0; l < nlev; ++l) {
+            if (PF->change_limiter && prev[l] > 0.0) tmp[l] = smin(tmp[l], 1.1 * prev[l]);
+            dt0 = smin(dt0, tmp[l]);
+        }
+        const double eps = 1.0e-3 * dt0;
+        if (pf_time + dt0 > dt - eps) dt0 = dt - pf_time;
+        double dmin = dt0;
+        for (int l = 0; l < nlev; ++l) dmin = smin(dmin, tmp[l]);
+        for (int l = 0; l < nlev; ++l) prev[l] = dmin;
+        err = amr_pf_rk3_step(P, PF, nlev, G, nbox, boxes, ng, U, gq, dmin, ratio);
+        pf_time += dmin;
+        ++n;
+        if (pf_time >= dt - 1.0e-6 * dt) break;
+    }
+    if (info) {
+        for (int q = 0; q < 4; ++q) info[q] = inf5[q];
+        info[4] = gmax;
+        info[5] = (double)n;
+        info[6] = inf5[4];
+    }
+    for (int l = 0; l < nlev; ++l) free(gq[l]);
+    return err;
+}
+
+/* nsteps steps of a hierarchy with the Phase-Field step after each flow step (Evolve with
+ * timeStepNoSubcycling then PhaseField_Evolve): the initial c_max, then per step ComputeDt over
+ * levels, the stage-coupled RK3 step, the Phase-Field step of that dt. */
+int ref_amr_advance_pf(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G, const int* nbox,
+                       const cmp_box* const* boxes, int ng, double* const* U, int nsteps, double cfl, int limiter,
+                       int ratio, double* dt_hist)
+{
+    const int dim = P->dim, N = P->nstate;
+    double cmax[AMR_NLEV_MAX] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double* Ub[AMR_NLEV_MAX];
+    int err = 0;
+    if (nlev < 1 || nlev > AMR_NLEV_MAX) return CMP_ERR_ARG;
+    for (int l = 0; l < nlev; ++l) {
+        const long long t = level_total(dim, nbox[l], boxes[l], ng, N);
+        Ub[l] = (double*)malloc(sizeof(double) * t);
+        memcpy(Ub[l], U[l], sizeof(double) * t);
+    }
+    err = amr_fill(P, nlev, G, nbox, boxes, ng, Ub, ratio);
+    for (int l = 0; l < nlev && !err; ++l)
+        for (int b = 0; b < nbox[l]; ++b) {
+            double* Fx[3] = {NULL, NULL, NULL};
+            double* FNx[3] = {NULL, NULL, NULL};
+            double cm = 0.0;
+            int e = box_faces(P, &G[l], &boxes[l][b], ng, Ub[l] + boxes[l][b].offset, Fx, FNx, &cm);
+            if (e && !err) err = e;
+            if (cmax[l] < cm || isnan(cm)) cmax[l] = cm;
+            for (int d = 0; d < 3; ++d) { free(Fx[d]); free(FNx[d]); }
+        }
+    for (int l = 0; l < nlev; ++l) free(Ub[l]);
+    double dt = 0.0;
+    for (int s = 0; s < nsteps && !err; ++s) {
+        dt = ref_amr_compute_dt(nlev, G, cmax, cfl, limiter, s == 0 ? 0.0 : dt);
+        if (dt_hist) dt_hist[s] = dt;
+        for (int l = 0; l < nlev; ++l) cmax[l] = 0.0;
+        err = ref_amr_rk3_step(P, nlev, G, nbox, boxes, ng, U, dt, ratio, cmax);
+        for (int l = 0; l < nlev; ++l)
+            if (isnan(cmax[l])) err = CMP_ERR_NAN;
+        if (!err) err = ref_amr_pf_evolve(P, PF, nlev, G, nbox, boxes, ng, U, dt, ratio, NULL);
     }
     return err;
 }

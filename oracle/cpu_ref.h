@@ -188,6 +188,20 @@ int ref_amr_advance(const cmp_params* P, int nlev, const cmp_geometry* G, const 
                     int ratio, double* dt_hist);
 
 /* number of OpenMP threads the level functions use (1 when built without OpenMP) */
+/* The Phase-Field step of a hierarchy without subcycling (PhaseField_Evolve over finest_level + 1
+ * levels, with the composite implicit solve), its Grad_Q alone, and the flow + Phase-Field loop. */
+int ref_amr_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G,
+                          const int* nbox, const cmp_box* const* boxes, int ng, double* const* U, double dt,
+                          int ratio, double* const* gq, double* info);
+int ref_amr_pf_gradq_implicit(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G,
+                              const int* nbox, const cmp_box* const* boxes, double* const* alpha,
+                              double* const* rhs, double eta, int ratio, double* const* gq, double* const* bf,
+                              int* iters);
+int ref_amr_pf_evolve(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G, const int* nbox,
+                      const cmp_box* const* boxes, int ng, double* const* U, double dt, int ratio, double* info);
+int ref_amr_advance_pf(const cmp_params* P, const cmp_pf_params* PF, int nlev, const cmp_geometry* G, const int* nbox,
+                       const cmp_box* const* boxes, int ng, double* const* U, int nsteps, double cfl, int limiter,
+                       int ratio, double* dt_hist);
 int ref_num_threads(void);
 void ref_set_num_threads(int n);
 
