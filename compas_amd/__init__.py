@@ -482,6 +482,33 @@ class Amr:
                                        float(cfl), int(limiter), _ptr(dt_dev), _ptr(cmax), _ptr(dt_hist),
                                        int(use_graph), _stream(stream)), "cmp_amr_advance")
 
+    # the Phase-Field step of the hierarchy (PhaseField_Evolve over the levels)
+    def pf_setup(self, pf):
+        check(self.lib.cmp_amr_pf_setup(self.h, ctypes.byref(pf)), "cmp_amr_pf_setup")
+
+    def PhaseField_GradQPF(self, U, dt, gradq=None, stream=None):
+        """Grad_Q of every level (into the torch face buffers gradq[l] when given); returns
+        (max|Grad_Q| per level, [max|u|, mobility, eta, mechanism sub-steps, CG iterations])."""
+        n = len(self.levels)
+        gm = (ctypes.c_double * n)()
+        info = (ctypes.c_double * 5)()
+        g = self._arr(gradq) if gradq is not None else None
+        check(self.lib.cmp_amr_pf_gradq(self.h, self._arr(U), float(dt), g, gm, info, _stream(stream)),
+              "cmp_amr_pf_gradq")
+        return list(gm), list(info)
+
+    def PhaseField_Evolve(self, Un, U1, U2, dt_dev, stream=None):
+        """[max|u|, mobility, eta, mechanism sub-steps, max|Grad_Q|, Phase-Field sub-steps, CG iterations]"""
+        info = (ctypes.c_double * 7)()
+        check(self.lib.cmp_amr_pf_evolve(self.h, self._arr(Un), self._arr(U1), self._arr(U2), _ptr(dt_dev), info,
+                                         _stream(stream)), "cmp_amr_pf_evolve")
+        return list(info)
+
+    def solver_iterations(self):
+        it = ctypes.c_int()
+        check(self.lib.cmp_amr_pf_solver_iterations(self.h, ctypes.byref(it)), "cmp_amr_pf_solver_iterations")
+        return it.value
+
     def check_errors(self, stream=None):
         return max(lv.check_errors(stream=stream) for lv in self.levels)
 

@@ -45,6 +45,8 @@ def build(verbose=False):
         ("cmp_pf.hip", "pf3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_mg.hip", "mg2.o", ["-DCMP_KERNEL_DIM=2"]),
         ("cmp_mg.hip", "mg3.o", ["-DCMP_KERNEL_DIM=3"]),
+        ("cmp_amr_pf.hip", "ap2.o", ["-DCMP_KERNEL_DIM=2"]),
+        ("cmp_amr_pf.hip", "ap3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_capi.hip", "capi.o", []),
         ("cmp_batch.hip", "batch.o", []),
         ("cmp_amr.hip", "amr.o", []),

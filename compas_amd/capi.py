@@ -166,6 +166,12 @@ SIGNATURES = {
                                         vp]),
     "cmp_amr_advance": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_int,
                                        ctypes.c_double, ctypes.c_int, dp, dp, dp, ctypes.c_int, vp]),
+    "cmp_amr_pf_setup": (ctypes.c_int, [vp, P(cmp_pf_params)]),
+    "cmp_amr_pf_gradq": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.c_double, ctypes.POINTER(dp),
+                                        P(ctypes.c_double), P(ctypes.c_double), vp]),
+    "cmp_amr_pf_evolve": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), dp,
+                                         P(ctypes.c_double), vp]),
+    "cmp_amr_pf_solver_iterations": (ctypes.c_int, [vp, P(ctypes.c_int)]),
     "cmp_batch_create": (ctypes.c_int, [P(cmp_params), ctypes.c_longlong, ctypes.c_int, P(vp)]),
     "cmp_batch_destroy": (ctypes.c_int, [vp]),
     "cmp_batch_admit": (ctypes.c_int, [vp, P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,

@@ -424,14 +424,17 @@ def nphase_rings_2d(n=512, seed=0, boxes=None, scheme="WENO5", riemann="HLLC"):
 
 
 def interface_hierarchy(dim, nc, fine_boxes, ratio=2, periodic=True, width=2.0, eos=None, nphase=0,
-                        velocity=(1.0, 0.5, -0.25), max_grid_size=None):
+                        velocity=(1.0, 0.5, -0.25), max_grid_size=None, rings=False):
     """A circle (sphere) of phase 1 of radius 0.25 about the domain centre in phase 2, uniform u and p,
     tanh profile of width*dx at the finest level, on a hierarchy: level 0 nc cells a side, every
     finer level the refinement of the one before by ratio with the boxes fine_boxes[l - 1] (in that
     level's cells). The finest level's field is evaluated at its cell centres and every coarser
     level's is the mean of it over each cell (the state AverageDown leaves). eos: dict(gamma, pinf);
-    nphase 3: phase 2 split 65/35 into two phases of its material. Returns dict(params, geoms, boxes,
-    fields, shapes) with the coarsest level first."""
+    nphase 3: phase 2 split 65/35 into two phases of its material (every phase fraction of the outer
+    region lies inside (0, 1), where the sharpening normal is set by rounding), or with rings=True
+    three ideal gases with separate interfaces: a disk of radius 0.2 (phase 1) inside a ring to
+    radius 0.32 (phase 3) inside phase 2. Returns dict(params, geoms, boxes, fields, shapes) with the
+    coarsest level first."""
     eos = eos or dict(gamma=(1.4, 1.6), pinf=(0.0, 0.0))
     nlev = 1 + len(fine_boxes)
     nf = nc * ratio ** (nlev - 1)
@@ -451,7 +454,21 @@ def interface_hierarchy(dim, nc, fine_boxes, ratio=2, periodic=True, width=2.0, 
     W[3 + dim] = a1
     U = prim_to_cons(P, W)
     c = dict(params=P, U=U)
-    if nphase:
+    if nphase == 3 and rings:
+        P = make_params(dim, gamma=(1.4, 1.67, 1.09), pinf=(0.0, 0.0, 0.0), nphase=3)
+        h1 = 0.5 * (1 + np.tanh((0.2 - r) / (width * Gf.dx[0])))
+        h2 = 0.5 * (1 + np.tanh((0.32 - r) / (width * Gf.dx[0])))
+        a = [np.clip(v, 1e-8, 1.0) for v in (h1, 1.0 - h2, h2 - h1)]
+        tot = sum(a)
+        W = np.empty((P.nstate,) + shape_f)
+        for k, rho in enumerate((1.0, 0.14, 5.0)):
+            W[k] = a[k] / tot * rho
+            W[4 + dim + k] = a[k] / tot
+        for m in range(dim):
+            W[3 + m] = velocity[m]
+        W[3 + dim] = 1.0
+        c = dict(params=P, U=np_prim_to_cons(P, W))
+    elif nphase:
         c = nphase_from_two_phase(c, nphase=nphase)
     P, U = c["params"], c["U"]
     fields, shapes, geoms, boxes = [], [], [], []

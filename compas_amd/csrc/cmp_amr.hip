@@ -13,6 +13,7 @@
 // The step is the reference's to rounding: the reference replaces the coarse faces before its
 // divergence, this library adds the difference after the stage kernel's update.
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "cmp_kernels.hpp"
@@ -38,6 +39,41 @@ hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, int part, c
 }
 }  // namespace cmp
 
+namespace cmp {
+template <int DIM>
+hipError_t launch_cf_correct_pf_dim(const LaunchCfg&, const KParams&, int, const CfCell*, int, const CfFace*, int,
+                                    double*, const BoxDev*, const double*, const double*, const BoxDev*,
+                                    const double*, const double*, int, double*, const double*, int, int*,
+                                    hipStream_t);
+template <int DIM>
+hipError_t launch_fill2_dim(int, const FillDev&, const FillDev*, const FillDev&, int, double*, const double*,
+                            const double*, long long, int, int*, hipStream_t);
+template <int DIM>
+hipError_t launch_avgdown_dim(const BoxDev*, int, long long, const FillDev&, const double*, double*, int, int, double,
+                              int*, hipStream_t);
+}  // namespace cmp
+
+// The Phase-Field step of a hierarchy: per level the phase-major fields of the mechanism (component
+// k of a phase-major field is k times the level's one-component alpha-layout length further on) and
+// the composite implicit solve's tables.
+struct AmrPf {
+    bool on = false;
+    cmp_pf_params pf{};
+    int npf = 1;
+    std::vector<BoxDev*> d_pbox;          // phase-major alpha layout (npf > 1; else the level's d_abox)
+    std::vector<FillDev> fdP;
+    std::vector<double*> A0, A, Ps, H, S;
+    std::vector<int*> d_mask;             // composite masks (implicit RC)
+    CompCell* d_cells = nullptr;
+    CompLink* d_links = nullptr;
+    int ncells = 0, nlinks = 0;
+    double vol_total = 0.0;
+    double* d_lsum = nullptr;             // per-level sums, 2 per level
+    unsigned long long* d_bits = nullptr; // [0] max|u|, [1 + l] max|Grad_Q| of level l
+    double* h_pin = nullptr;              // pinned host copies
+    int cg_iters = 0;
+};
+
 struct cmp_amr {
     std::vector<cmp_level*> L;
     int ratio = 2;
@@ -60,6 +96,7 @@ struct cmp_amr {
     std::vector<const void*> gkey;
     double gcfl = 0.0;
     int glimiter = -1;
+    AmrPf pf;
 };
 
 namespace {
@@ -157,8 +194,27 @@ int build_cf(const cmp_level* C, const cmp_level* F, const cmp_level* FF, int ra
     return CMP_OK;
 }
 
+void amr_pf_free(cmp_amr* A)
+{
+    AmrPf& P = A->pf;
+    for (size_t l = 0; l < P.d_pbox.size(); ++l)
+        if (P.d_pbox[l] && P.d_pbox[l] != A->L[l]->d_abox) (void)hipFree(P.d_pbox[l]);
+    for (auto* v : {&P.A0, &P.A, &P.Ps, &P.H, &P.S})
+        for (double* p : *v)
+            if (p) (void)hipFree(p);
+    for (int* p : P.d_mask)
+        if (p) (void)hipFree(p);
+    if (P.d_cells) (void)hipFree(P.d_cells);
+    if (P.d_links) (void)hipFree(P.d_links);
+    if (P.d_lsum) (void)hipFree(P.d_lsum);
+    if (P.d_bits) (void)hipFree(P.d_bits);
+    if (P.h_pin) (void)hipHostFree(P.h_pin);
+    P = AmrPf();
+}
+
 void amr_free(cmp_amr* A)
 {
+    amr_pf_free(A);
     for (CfCell* p : A->d_cf)
         if (p) (void)hipFree(p);
     for (CfFace* p : A->d_jobs)
@@ -313,24 +369,34 @@ int cmp_amr_compute_dt(cmp_amr* A, double* cmax, double* dt_dev, double cfl, int
 // next coarser level (after that level's fill); back on the caller's stream, the corrections (after
 // level 0's stage kernel and every face job) and the average-down (after every stage kernel),
 // finest first.
-int cmp_amr_rk3_step(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
-                     double* cmax, void* stream)
+// The stages of one TVD-RK3 step of the hierarchy: the flow's (pf false: c_max accumulated into
+// cmax[l]) or the Phase-Field sub-step's (pf true: every level's right-hand side the Phase-Field
+// flux at its Grad_Q, the levels' d_gq; no c_max).
+static int amr_stages(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
+                      double* cmax, bool pf, hipStream_t s)
 {
-    if (!A || !Un || !U1 || !U2 || !dt_dev || !cmax) return CMP_ERR_ARG;
-    hipStream_t s = (hipStream_t)stream;
     const int nlev = (int)A->L.size();
     auto stage_of = [&](int l, int stage, double* const* src, double* const* dst, hipStream_t q) -> int {
         cmp_level* L = A->L[l];
-        CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src[l],
-                             Un[l], dst[l], dt_dev, (unsigned long long*)(cmax + l), L->d_err, q));
+        if (pf)
+            CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0,
+                                 src[l], Un[l], dst[l], dt_dev, nullptr, L->d_err, q, L->d_gq));
+        else
+            CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src[l],
+                                 Un[l], dst[l], dt_dev, (unsigned long long*)(cmax + l), L->d_err, q));
         return CMP_OK;
     };
     auto cf = [&](int l, int part, int stage, double* const* src, double* const* dst, hipStream_t q) -> int {
         cmp_level* C = A->L[l];
         cmp_level* F = A->L[l + 1];
-        CMP_TRY(launch_cf_correct(C->lc, C->kp, part, A->d_cf[l], A->ncf[l], A->d_jobs[l], A->njobs[l],
-                                  A->d_faces[l], C->d_boxes, src[l], F->d_boxes, src[l + 1], A->ratio, dst[l], dt_dev,
-                                  stage, C->d_err, q));
+        if (pf)
+            CMP_TRY((C->P.dim == 2 ? launch_cf_correct_pf_dim<2> : launch_cf_correct_pf_dim<3>)(
+                C->pf_lc, C->pf_kp, part, A->d_cf[l], A->ncf[l], A->d_jobs[l], A->njobs[l], A->d_faces[l], C->d_boxes,
+                src[l], C->d_gq, F->d_boxes, src[l + 1], F->d_gq, A->ratio, dst[l], dt_dev, stage, C->d_err, q));
+        else
+            CMP_TRY(launch_cf_correct(C->lc, C->kp, part, A->d_cf[l], A->ncf[l], A->d_jobs[l], A->njobs[l],
+                                      A->d_faces[l], C->d_boxes, src[l], F->d_boxes, src[l + 1], A->ratio, dst[l],
+                                      dt_dev, stage, C->d_err, q));
         return CMP_OK;
     };
     for (int stage = 0; stage < 3; ++stage) {
@@ -370,6 +436,13 @@ int cmp_amr_rk3_step(cmp_amr* A, double* const* Un, double* const* U1, double* c
     return CMP_OK;
 }
 
+int cmp_amr_rk3_step(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
+                     double* cmax, void* stream)
+{
+    if (!A || !Un || !U1 || !U2 || !dt_dev || !cmax) return CMP_ERR_ARG;
+    return amr_stages(A, Un, U1, U2, dt_dev, cmax, false, (hipStream_t)stream);
+}
+
 static int amr_step(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, double cfl, int limiter,
                     double* dt_dev, double* cmax, double* dt_hist, hipStream_t s)
 {
@@ -387,8 +460,10 @@ int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* co
     CMP_TRY(hipMemsetAsync(dt_dev, 0, sizeof(double), s));
     CMP_TRY(launch_set2(A->d_step, 0, dt_hist ? nsteps : 0, s));
     if (!use_graph || nsteps < 2) {
-        for (int step = 0; step < nsteps; ++step)
+        for (int step = 0; step < nsteps; ++step) {
             if ((r = amr_step(A, Un, U1, U2, cfl, limiter, dt_dev, cmax, dt_hist, s))) return r;
+            if (A->pf.on && (r = cmp_amr_pf_evolve(A, Un, U1, U2, dt_dev, nullptr, s))) return r;
+        }
         return CMP_OK;
     }
     // one step captured and replayed (on a private stream when the caller gave the legacy default
@@ -424,8 +499,771 @@ int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* co
         A->gcfl = cfl;
         A->glimiter = limiter;
     }
-    for (int step = 0; step < nsteps; ++step) CMP_TRY(hipGraphLaunch(A->gexec, cs));
+    for (int step = 0; step < nsteps; ++step) {
+        CMP_TRY(hipGraphLaunch(A->gexec, cs));
+        if (A->pf.on && (r = cmp_amr_pf_evolve(A, Un, U1, U2, dt_dev, nullptr, cs))) return r;
+    }
     if (cs != s) CMP_TRY(hipStreamSynchronize(cs));
+    return CMP_OK;
+}
+
+
+}  // extern "C"
+
+// ---- the Phase-Field step of a hierarchy -----------------------------------------------------------
+// PhaseField_Evolve over finest_level + 1 levels without subcycling (source/Class_Compressible_PhaseField.cpp:
+// 425-594): MaxVelocity over the levels, eta from the finest cell size, the mechanism sub-steps of
+// PhaseFieldMechanism_Evolve (:597-709) on every level with the finest level's dt_m (the sub-step
+// state filled on every level, all phases at once; CAC's sums over the averaged-down level 0; ACDI's
+// J averaged down before its divergence and before Jhat takes it), Grad_Q per level (explicit RC) or
+// from the composite implicit solve, then PhaseField_ComputeDt over the levels and stage-coupled
+// Phase-Field RK3 sub-steps with AverageDown.
+
+namespace {
+
+// valid cells (host): the index of cell c of box b of a level in its alpha layout (1 ghost, 1 comp)
+long long aidx(const cmp_level* L, int b, const int* c)
+{
+    const cmp_box& B = L->boxes[b];
+    const int dim = L->P.dim;
+    long long off = 0;
+    for (int q = 0; q < b; ++q) {
+        long long f = 1;
+        for (int d = 0; d < dim; ++d) f *= L->boxes[q].hi[d] - L->boxes[q].lo[d] + 3;
+        off += f;
+    }
+    long long idx = 0, st = 1;
+    for (int d = 0; d < dim; ++d) {
+        idx += (long long)(c[d] - B.lo[d] + 1) * st;
+        st *= B.hi[d] - B.lo[d] + 3;
+    }
+    return off + idx;
+}
+
+// offset of box b's direction-d block in a level's one-component face buffer, and the face index of
+// the face at the low side of cell c along d inside it
+long long aface(const cmp_level* L, const std::vector<long long>& foff, int b, int d, const int* c)
+{
+    const cmp_box& B = L->boxes[b];
+    const int dim = L->P.dim;
+    long long idx = 0, st = 1;
+    for (int e = 0; e < dim; ++e) {
+        idx += (long long)(c[e] - B.lo[e]) * st;
+        st *= B.hi[e] - B.lo[e] + 1 + (e == d ? 1 : 0);
+    }
+    return foff[(size_t)b * dim + d] + idx;
+}
+
+// periodic wrap of c into level L's domain; false when c lies beyond a non-periodic face
+bool wrap_cell(const cmp_level* L, int* c)
+{
+    for (int d = 0; d < L->P.dim; ++d) {
+        const int lo = L->G.domain_lo[d], hi = L->G.domain_hi[d], n = hi - lo + 1;
+        if (c[d] < lo || c[d] > hi) {
+            if (!L->G.is_periodic[d]) return false;
+            c[d] = (c[d] < lo) ? c[d] + n : c[d] - n;
+        }
+    }
+    return true;
+}
+
+int floor_div_h(int a, int r) { return (a >= 0) ? a / r : -((-a + r - 1) / r); }
+
+// The composite masks of every level and the link lists of the cells with a coarse-fine face.
+int amr_comp_build(cmp_amr* A)
+{
+    AmrPf& P = A->pf;
+    const int nlev = (int)A->L.size();
+    const int ratio = A->ratio;
+    const int dim = A->L[0]->P.dim;
+    std::vector<CompCell> cells;
+    std::vector<CompLink> links;
+    std::vector<std::vector<long long>> foff(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        const cmp_level* L = A->L[l];
+        foff[l].resize((size_t)L->nbox * dim);
+        cmp_face_layout(dim, L->nbox, L->boxes.data(), 1, foff[l].data(), nullptr);
+    }
+    auto geo_of = [&](int l, int d) {
+        double a = 1.0;
+        for (int e = 0; e < dim; ++e)
+            if (e != d) a *= A->L[l]->G.dx[e];
+        return a;
+    };
+    double vol_total = 0.0;
+    P.d_mask.assign(nlev, nullptr);
+    for (int l = 0; l < nlev; ++l) {
+        const cmp_level* L = A->L[l];
+        std::vector<int> mask((size_t)L->na, COMP_REGULAR);
+        double vol = 1.0;
+        for (int d = 0; d < dim; ++d) vol *= L->G.dx[d];
+        for (int b = 0; b < L->nbox; ++b) {
+            const cmp_box& B = L->boxes[b];
+            const int klo = (dim == 3) ? B.lo[2] : 0, khi = (dim == 3) ? B.hi[2] : 0;
+            for (int k = klo; k <= khi; ++k)
+                for (int j = B.lo[1]; j <= B.hi[1]; ++j)
+                    for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
+                        const int c[3] = {i, j, k};
+                        const long long ci = aidx(L, b, c);
+                        int f[3] = {0, 0, 0};
+                        for (int d = 0; d < dim; ++d) f[d] = c[d] * ratio;
+                        if (l + 1 < nlev && owner_of(A->L[l + 1], f) >= 0) {
+                            mask[ci] = COMP_COVERED;
+                            continue;
+                        }
+                        vol_total += vol;
+                        std::vector<CompLink> lk;
+                        bool special = false;
+                        for (int d = 0; d < dim; ++d)
+                            for (int side = 0; side < 2; ++side) {
+                                int x[3] = {i, j, k};
+                                x[d] += side ? 1 : -1;
+                                if (!wrap_cell(L, x)) continue;   // Neumann domain face
+                                int fc[3] = {i, j, k};
+                                fc[d] += side;
+                                const int nb = owner_of(L, x);
+                                int fx[3] = {0, 0, 0};
+                                for (int e = 0; e < dim; ++e) fx[e] = x[e] * ratio;
+                                const bool xcov = nb >= 0 && l + 1 < nlev && owner_of(A->L[l + 1], fx) >= 0;
+                                if (nb >= 0 && !xcov) {
+                                    lk.push_back(CompLink{l, l, aidx(L, nb, x), aface(L, foff[l], b, d, fc),
+                                                          geo_of(l, d) / L->G.dx[d]});
+                                    continue;
+                                }
+                                special = true;
+                                if (nb >= 0) {
+                                    // the fine cells next to this face, each with its fine face
+                                    const cmp_level* F = A->L[l + 1];
+                                    const double dist = 0.5 * (F->G.dx[d] + L->G.dx[d]);
+                                    int t1 = -1, t2 = -1;
+                                    for (int e = 0; e < dim; ++e)
+                                        if (e != d) { if (t1 < 0) t1 = e; else t2 = e; }
+                                    const int r1 = (t1 >= 0) ? ratio : 1, r2 = (t2 >= 0) ? ratio : 1;
+                                    for (int o2 = 0; o2 < r2; ++o2)
+                                        for (int o1 = 0; o1 < r1; ++o1) {
+                                            int g[3] = {0, 0, 0};
+                                            for (int e = 0; e < dim; ++e) g[e] = x[e] * ratio;
+                                            g[d] = side ? x[d] * ratio : (x[d] + 1) * ratio - 1;
+                                            if (t1 >= 0) g[t1] = c[t1] * ratio + o1;
+                                            if (t2 >= 0) g[t2] = c[t2] * ratio + o2;
+                                            int gf[3] = {g[0], g[1], g[2]};
+                                            gf[d] += side ? 0 : 1;
+                                            const int fb = owner_of(F, g);
+                                            if (fb < 0) return CMP_ERR_LAYOUT;
+                                            lk.push_back(CompLink{l + 1, l + 1, aidx(F, fb, g), aface(F, foff[l + 1], fb, d, gf),
+                                                                  geo_of(l + 1, d) / dist});
+                                        }
+                                } else {
+                                    // x is not on this level: the coarse cell holding it
+                                    if (l == 0) return CMP_ERR_LAYOUT;
+                                    const cmp_level* C = A->L[l - 1];
+                                    int p[3] = {0, 0, 0};
+                                    for (int e = 0; e < dim; ++e) p[e] = floor_div_h(x[e], ratio);
+                                    if (!wrap_cell(C, p)) return CMP_ERR_LAYOUT;
+                                    const int pb = owner_of(C, p);
+                                    if (pb < 0) return CMP_ERR_LAYOUT;
+                                    lk.push_back(CompLink{l - 1, l, aidx(C, pb, p), aface(L, foff[l], b, d, fc),
+                                                          geo_of(l, d) / (0.5 * (L->G.dx[d] + C->G.dx[d]))});
+                                }
+                            }
+                        if (special) {
+                            mask[ci] = (int)cells.size();
+                            cells.push_back(CompCell{(int)lk.size(), (int)links.size()});
+                            links.insert(links.end(), lk.begin(), lk.end());
+                        }
+                    }
+        }
+        CMP_TRY(hipMalloc(&P.d_mask[l], sizeof(int) * mask.size()));
+        CMP_TRY(hipMemcpy(P.d_mask[l], mask.data(), sizeof(int) * mask.size(), hipMemcpyHostToDevice));
+    }
+    P.ncells = (int)cells.size();
+    P.nlinks = (int)links.size();
+    CMP_TRY(hipMalloc(&P.d_cells, sizeof(CompCell) * std::max<size_t>(1, cells.size())));
+    CMP_TRY(hipMalloc(&P.d_links, sizeof(CompLink) * std::max<size_t>(1, links.size())));
+    if (!cells.empty())
+        CMP_TRY(hipMemcpy(P.d_cells, cells.data(), sizeof(CompCell) * cells.size(), hipMemcpyHostToDevice));
+    if (!links.empty())
+        CMP_TRY(hipMemcpy(P.d_links, links.data(), sizeof(CompLink) * links.size(), hipMemcpyHostToDevice));
+    P.vol_total = vol_total;
+    return CMP_OK;
+}
+
+CompArgs comp_args(const cmp_amr* A, bool solution)
+{
+    const AmrPf& P = A->pf;
+    CompArgs ca{};
+    ca.nlev = (int)A->L.size();
+    ca.ratio = A->ratio;
+    ca.cells = P.d_cells;
+    ca.links = P.d_links;
+    for (int l = 0; l < ca.nlev; ++l) {
+        const cmp_level* L = A->L[l];
+        CompLev& c = ca.L[l];
+        c.abox = L->d_abox;
+        c.fd = L->fdA;
+        c.mask = P.d_mask[l];
+        c.v = solution ? L->d_X : L->d_Pv;
+        c.bf = L->d_Bf;
+        c.vol = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            c.dx[d] = L->pg.dx[d];
+            if (d < L->P.dim) c.vol *= L->G.dx[d];
+        }
+        for (int d = 0; d < 3; ++d) {
+            double a = 1.0;
+            for (int e = 0; e < L->P.dim; ++e)
+                if (e != d) a *= L->G.dx[e];
+            c.area[d] = a;
+        }
+    }
+    return ca;
+}
+
+int ap_launch(int dim, int op, const AmrPfArgs& a, hipStream_t s)
+{
+    CMP_TRY(dim == 2 ? launch_amrpf_dim<2>(op, a, s) : launch_amrpf_dim<3>(op, a, s));
+    return CMP_OK;
+}
+
+AmrPfArgs level_args(const cmp_level* L, int l, const CompArgs* ca)
+{
+    AmrPfArgs a{};
+    a.nbox = L->nbox;
+    a.maxcells = L->maxcells;
+    for (int d = 0; d < 3; ++d) a.maxfaces[d] = L->maxfaces[d];
+    a.ca = ca;
+    a.lev = l;
+    a.part = L->d_part;
+    return a;
+}
+
+// AverageDownFaces of one-component face buffers F[l] (finest level first)
+int amr_avgdown_faces(cmp_amr* A, double* const* F, hipStream_t s)
+{
+    const int nlev = (int)A->L.size();
+    for (int l = nlev - 2; l >= 0; --l) {
+        const cmp_level* Fl = A->L[l + 1];
+        const cmp_level* C = A->L[l];
+        AmrPfArgs a{};
+        a.fboxes = Fl->d_abox;
+        a.cfd = C->fdA;
+        a.nbox = Fl->nbox;
+        // coarse faces of a coarsened fine box: at most (n/ratio + 1) along d
+        for (int d = 0; d < Fl->P.dim; ++d) {
+            long long m = 0;
+            for (const cmp_box& b : Fl->boxes) {
+                long long v = 1;
+                for (int e = 0; e < Fl->P.dim; ++e) v *= (b.hi[e] - b.lo[e] + 1) / A->ratio + (e == d ? 1 : 0);
+                m = std::max(m, v);
+            }
+            a.maxfaces[d] = m;
+        }
+        a.ratio = A->ratio;
+        a.src = F[l + 1];
+        a.dst = F[l];
+        int r = ap_launch(C->P.dim, AP_AVGDOWN_FACES, a, s);
+        if (r) return r;
+    }
+    return CMP_OK;
+}
+
+// AverageDown of one-component alpha-layout cell fields X[l] (finest level first)
+int amr_avgdown_cells(cmp_amr* A, double* const* X, hipStream_t s)
+{
+    const int nlev = (int)A->L.size();
+    for (int l = nlev - 2; l >= 0; --l) {
+        cmp_level* F = A->L[l + 1];
+        cmp_level* C = A->L[l];
+        const int dim = F->P.dim;
+        long long maxc = 0;
+        for (const cmp_box& b : F->boxes) {
+            long long v = 1;
+            for (int d = 0; d < dim; ++d) v *= (b.hi[d] - b.lo[d] + 1) / A->ratio;
+            maxc = std::max(maxc, v);
+        }
+        double vol = 1.0;
+        for (int d = 0; d < dim; ++d) vol *= A->ratio;
+        CMP_TRY((dim == 2 ? launch_avgdown_dim<2> : launch_avgdown_dim<3>)(F->d_abox, F->nbox, maxc, C->fdA, X[l + 1],
+                                                                            X[l], 1, A->ratio, 1.0 / vol, C->d_err, s));
+    }
+    return CMP_OK;
+}
+
+// FillPatch of an nc-component Phase-Field field of every level (1 ghost): level 0 same-level, finer
+// levels two-level. F[l]: the field; fd[l]: its tables (phase-major alpha layout or the normals').
+int amr_fill_pf(cmp_amr* A, int nc, double* const* F, const std::vector<FillDev>& fd, hipStream_t s)
+{
+    const int nlev = (int)A->L.size();
+    cmp_level* L0 = A->L[0];
+    CMP_TRY(launch_fill(L0->P.dim, nc, fd[0], L0->nghost1, F[0], L0->d_err, s));
+    for (int l = 1; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        CMP_TRY((L->P.dim == 2 ? launch_fill2_dim<2> : launch_fill2_dim<3>)(nc, fd[l], &fd[l], fd[l - 1], A->ratio,
+                                                                            F[l], F[l], F[l - 1], 0, 0, L->d_err, s));
+    }
+    return CMP_OK;
+}
+
+// The composite implicit solve of one phase: rhs in every level's d_DJ, b in d_Bf (averaged down);
+// Grad_Q into out[l] (one-component face buffers), averaged down.
+int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
+{
+    AmrPf& P = A->pf;
+    const int nlev = (int)A->L.size();
+    cmp_level* L0 = A->L[0];
+    const int dim = L0->P.dim;
+    double* sc = L0->d_sc;
+    const CompArgs caP = comp_args(A, false), caX = comp_args(A, true);
+    const bool mg = !L0->mg.empty();
+    int r;
+    auto sum_levels = [&](int nv, double* dst, const double* scx) -> int {
+        for (int l = 0; l < nlev; ++l)
+            CMP_TRY(launch_reduce_sums(A->L[l]->d_part, A->L[l]->nblocks, nv, P.d_lsum + nv * l, s));
+        AmrPfArgs a{};
+        a.src = P.d_lsum;
+        a.nlev = nlev;
+        a.nvals = nv;
+        a.dst = dst;
+        a.sc = scx;
+        return ap_launch(dim, AP_SUM_LEVELS, a, s);
+    };
+    // z = R ID (levels above 0, Jacobi) + the level-0 V-cycle of the restricted R/V at every cell's
+    // level-0 ancestor; r.z partials into each level's second partial slot (P = Z when init)
+    auto precondition = [&](const double* scx, bool init) -> int {
+        for (int l = 0; l < nlev; ++l) {
+            AmrPfArgs a = level_args(A->L[l], l, &caP);
+            a.sc = scx;
+            a.src = A->L[l]->d_R;
+            a.dst = P.S[l];
+            if ((r = ap_launch(dim, AP_COMP_S, a, s))) return r;
+        }
+        if ((r = amr_avgdown_cells(A, P.S.data(), s))) return r;
+        const double* x0 = nullptr;
+        if (mg) {
+            cmp::MgArgs m{};
+            m.g = L0->pg;
+            m.maxcells = L0->maxcells;
+            m.sc = scx;
+            m.src = P.S[0];
+            m.dst = L0->mg[0].r;
+            m.G = L0->mg[0].G;
+            CMP_TRY(dim == 2 ? launch_mg_dim<2>(MG_GATHER_R, m, s) : launch_mg_dim<3>(MG_GATHER_R, m, s));
+            if ((r = mg_vcycle(L0, scx, 0, s))) return r;
+            x0 = L0->mg[0].x[L0->mg[0].cur];
+        }
+        for (int l = 0; l < nlev; ++l) {
+            cmp_level* L = A->L[l];
+            AmrPfArgs a = level_args(L, l, &caP);
+            a.sc = scx;
+            a.src = L->d_R;
+            a.src2 = L->d_ID;
+            a.src3 = x0;
+            if (mg) a.G0 = L0->mg[0].G;
+            for (int d = 0; d < 3; ++d) a.dlo0[d] = (d < dim) ? L0->G.domain_lo[d] : 0;
+            a.dst = L->d_Z;
+            if ((r = ap_launch(dim, AP_COMP_Z, a, s))) return r;
+            cmp::MgArgs m{};
+            m.g = L->pg;
+            m.maxcells = L->maxcells;
+            m.sc = scx;
+            m.src = L->d_R;
+            m.r = L->d_Z;
+            m.dst = init ? L->d_Pv : nullptr;
+            m.part = L->d_part;
+            CMP_TRY(dim == 2 ? launch_mg_dim<2>(MG_CG_RZ, m, s) : launch_mg_dim<3>(MG_CG_RZ, m, s));
+        }
+        return CMP_OK;
+    };
+    // Jacobi diagonal; the rhs mean over the composite cells; r, x
+    for (int l = 0; l < nlev; ++l) {
+        AmrPfArgs a = level_args(A->L[l], l, &caP);
+        a.dst = A->L[l]->d_ID;
+        if ((r = ap_launch(dim, AP_COMP_DIAG, a, s))) return r;
+        a.src = A->L[l]->d_DJ;
+        a.mean_sum = nullptr;
+        if ((r = ap_launch(dim, AP_COMP_INIT, a, s))) return r;
+    }
+    if ((r = sum_levels(1, sc + CG_SUM, nullptr))) return r;
+    for (int l = 0; l < nlev; ++l) {
+        AmrPfArgs a = level_args(A->L[l], l, &caP);
+        a.src = A->L[l]->d_DJ;
+        a.mean_sum = sc + CG_SUM;
+        a.a = P.vol_total;
+        a.dst = A->L[l]->d_X;
+        a.dst2 = A->L[l]->d_R;
+        if ((r = ap_launch(dim, AP_COMP_INIT, a, s))) return r;
+    }
+    if (mg && (r = mg_prepare(L0, s))) return r;
+    if ((r = precondition(nullptr, true))) return r;
+    if ((r = sum_levels(2, sc + CG_RR, nullptr))) return r;
+    CMP_TRY(launch_cg_start(sc, s));
+    const int maxit = 100000;
+    const int chunk = 8;
+    for (long long queued = 0;; queued += chunk) {
+        for (int q = 0; q < chunk; ++q) {
+            // the search direction's ghost cells for the regular stencil (a finer level's coarse-fine
+            // ghosts are interpolated but never read: those cells take their link lists)
+            {
+                std::vector<double*> Pv(nlev);
+                std::vector<FillDev> fda(nlev);
+                for (int l = 0; l < nlev; ++l) {
+                    Pv[l] = A->L[l]->d_Pv;
+                    fda[l] = A->L[l]->fdA;
+                }
+                if ((r = amr_fill_pf(A, 1, Pv.data(), fda, s))) return r;
+            }
+            for (int l = 0; l < nlev; ++l) {
+                AmrPfArgs a = level_args(A->L[l], l, &caP);
+                a.sc = sc;
+                a.dst = A->L[l]->d_AP;
+                if ((r = ap_launch(dim, AP_COMP_AP, a, s))) return r;
+            }
+            if ((r = sum_levels(1, sc + CG_PAP, sc))) return r;
+            for (int l = 0; l < nlev; ++l) {
+                cmp_level* L = A->L[l];
+                CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, nullptr, L->d_X, L->d_R, L->d_Z, L->d_Pv, L->d_AP,
+                                     L->d_part, s));
+            }
+            if ((r = precondition(sc, false))) return r;
+            if ((r = sum_levels(2, sc + CG_RRNEW, sc))) return r;
+            for (int l = 0; l < nlev; ++l) {
+                cmp_level* L = A->L[l];
+                CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s));
+            }
+            CMP_TRY(launch_cg_end(sc, maxit, s));
+        }
+        CMP_TRY(hipMemcpyAsync(P.h_pin + 0, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
+        CMP_TRY(hipMemcpyAsync(P.h_pin + 1, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, s));
+        CMP_TRY(hipStreamSynchronize(s));
+        if (P.h_pin[0] != 0.0) break;
+        if (queued > maxit) return CMP_ERR_NAN;
+    }
+    P.cg_iters = (int)P.h_pin[1];
+    // covered cells take the mean of their children; Grad_Q of every level, averaged down
+    std::vector<double*> X(nlev);
+    for (int l = 0; l < nlev; ++l) X[l] = A->L[l]->d_X;
+    if ((r = amr_avgdown_cells(A, X.data(), s))) return r;
+    for (int l = 0; l < nlev; ++l) {
+        AmrPfArgs a = level_args(A->L[l], l, &caX);
+        a.dst = out[l];
+        if ((r = ap_launch(dim, AP_COMP_GRAD, a, s))) return r;
+    }
+    return amr_avgdown_faces(A, out, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cmp_amr_pf_setup(cmp_amr* A, const cmp_pf_params* pf)
+{
+    if (!A || !pf) return CMP_ERR_ARG;
+    amr_pf_free(A);
+    const int nlev = (int)A->L.size();
+    for (int l = 0; l < nlev; ++l) {
+        int r = cmp_level_pf_setup(A->L[l], pf);
+        if (r) return r;
+    }
+    AmrPf& P = A->pf;
+    P.pf = *pf;
+    const int nph = A->L[0]->P.nphase;
+    P.npf = nph ? nph : 1;
+    const bool implicit = (pf->explicit_rc == 0);
+    int r = CMP_OK;
+#define CMP_TRYA(x) do { r = cmp::check_hip(x); if (r) { amr_pf_free(A); return r; } } while (0)
+    for (int l = 0; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        BoxDev* pb = L->d_abox;
+        if (P.npf > 1) {
+            // the phase-major table: the alpha layout with the component stride of the whole level buffer
+            std::vector<BoxDev> hb(L->nbox);
+            CMP_TRYA(hipMemcpy(hb.data(), L->d_abox, sizeof(BoxDev) * L->nbox, hipMemcpyDeviceToHost));
+            for (BoxDev& b : hb) b.sn = L->na;
+            CMP_TRYA(hipMalloc(&pb, sizeof(BoxDev) * L->nbox));
+            CMP_TRYA(hipMemcpy(pb, hb.data(), sizeof(BoxDev) * L->nbox, hipMemcpyHostToDevice));
+        }
+        P.d_pbox.push_back(pb);
+        FillDev fd = L->fdA;
+        fd.boxes = pb;
+        P.fdP.push_back(fd);
+        double *a0 = nullptr, *a = nullptr, *ps = nullptr, *h = nullptr, *sv = nullptr;
+        CMP_TRYA(hipMalloc(&a0, sizeof(double) * L->na * P.npf));
+        P.A0.push_back(a0);
+        CMP_TRYA(hipMalloc(&a, sizeof(double) * L->na * P.npf));
+        P.A.push_back(a);
+        CMP_TRYA(hipMalloc(&ps, sizeof(double) * L->na * P.npf));
+        P.Ps.push_back(ps);
+        CMP_TRYA(hipMalloc(&h, sizeof(double) * L->nfaces * P.npf));
+        P.H.push_back(h);
+        if (implicit) {
+            CMP_TRYA(hipMalloc(&sv, sizeof(double) * L->na));
+            CMP_TRYA(hipMemset(sv, 0, sizeof(double) * L->na));
+        }
+        P.S.push_back(sv);
+        // the unfilled ghost cells stay defined
+        CMP_TRYA(hipMemset(a0, 0, sizeof(double) * L->na * P.npf));
+        CMP_TRYA(hipMemset(a, 0, sizeof(double) * L->na * P.npf));
+        CMP_TRYA(hipMemset(ps, 0, sizeof(double) * L->na * P.npf));
+    }
+    CMP_TRYA(hipMalloc(&P.d_lsum, sizeof(double) * 2 * AMR_MAX_LEVELS));
+    CMP_TRYA(hipMalloc(&P.d_bits, sizeof(unsigned long long) * (1 + AMR_MAX_LEVELS)));
+    CMP_TRYA(hipHostMalloc((void**)&P.h_pin, sizeof(double) * (4 + AMR_MAX_LEVELS), hipHostMallocDefault));
+#undef CMP_TRYA
+    if (implicit && (r = amr_comp_build(A))) { amr_pf_free(A); return r; }
+    P.on = true;
+    return CMP_OK;
+}
+
+int cmp_amr_pf_solver_iterations(const cmp_amr* A, int* iterations)
+{
+    if (!A || !iterations) return CMP_ERR_ARG;
+    *iterations = A->pf.cg_iters;
+    return CMP_OK;
+}
+
+// Grad_Q of every level into the levels' own face buffers (and the caller's, when given).
+int cmp_amr_pf_gradq(cmp_amr* A, double* const* U, double dt, double* const* gradq, double* gradq_max, double* info,
+                     void* stream)
+{
+    if (!A || !U || !A->pf.on) return CMP_ERR_ARG;
+    if (!(dt > 0.0)) return CMP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    AmrPf& P = A->pf;
+    const cmp_pf_params& pf = P.pf;
+    const int nlev = (int)A->L.size();
+    cmp_level* L0 = A->L[0];
+    cmp_level* Lf = A->L[nlev - 1];
+    const int dim = L0->P.dim;
+    const int nph = L0->P.nphase;
+    const int npf = P.npf;
+    const bool cac = (pf.mechanism == CMP_PF_CAC);
+    const bool explicit_rc = (pf.explicit_rc == 1);
+    int r;
+    CMP_TRY(launch_zero_u64(P.d_bits, 1 + nlev, s));
+    // MaxVelocity over the levels
+    for (int l = 0; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        CMP_TRY(launch_pf_maxvel(dim, L->pg, L->maxcells, U[l], P.d_bits, nph ? nph : 2, s));
+    }
+    CMP_TRY(hipMemcpyAsync(P.h_pin, P.d_bits, sizeof(double), hipMemcpyDeviceToHost, s));
+    CMP_TRY(hipStreamSynchronize(s));
+    double maxv;
+    std::memcpy(&maxv, P.h_pin, sizeof(double));
+    if (__builtin_isnan(maxv)) return CMP_ERR_NAN;
+    double lambda = pf.lambda;
+    if (lambda <= 0.0) lambda = 1.0;
+    double eta = pf.eta;
+    if (eta <= 0.0) {
+        double m = Lf->G.dx[0];
+        for (int d = 1; d < dim; ++d) m = (m < Lf->G.dx[d]) ? Lf->G.dx[d] : m;
+        eta = pf.eta_multiplier * m;
+    }
+    double mob = 1.0;
+    mob *= 0.5 * eta * maxv;
+    double sdx = 1.0 / Lf->G.dx[0] / Lf->G.dx[0];
+    for (int d = 1; d < dim; ++d) sdx = sdx + 1.0 / Lf->G.dx[d] / Lf->G.dx[d];
+    double dt_m = 0.9 / (2.0 * mob * lambda) / (sdx + 1.0 / eta / eta);
+    const double mob_face = 0.5 * mob + 0.5 * mob;
+    // the step's volume fractions (PFBorder) and the mechanism's copy
+    for (int l = 0; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        for (int ph = 0; ph < npf; ++ph)
+            CMP_TRY(launch_pf_extract(dim, L->pg, L->maxcells, U[l], P.A0[l] + ph * L->na, nullptr,
+                                      nph ? nph + dim + 1 + ph : 3 + dim, s));
+    }
+    if ((r = amr_fill_pf(A, npf, P.A0.data(), P.fdP, s))) return r;
+    for (int l = 0; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        CMP_TRY(hipMemcpyAsync(P.A[l], P.A0[l], sizeof(double) * L->na * npf, hipMemcpyDeviceToDevice, s));
+        if (explicit_rc)
+            CMP_TRY(hipMemsetAsync(P.H[l], 0, sizeof(double) * L->nfaces * npf, s));
+    }
+    std::vector<double*> Jl(nlev), Nl(nlev), Mk(nlev), Wk(nlev);
+    std::vector<FillDev> fdN(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        Jl[l] = A->L[l]->d_J;
+        Nl[l] = A->L[l]->d_N;
+        Mk[l] = A->L[l]->d_Mk;
+        Wk[l] = A->L[l]->d_Wk;
+        fdN[l] = A->L[l]->fdN;
+    }
+    double t = 0.0;
+    const double T = dt;
+    long long n = 0;
+    while (t < T) {
+        if (t + dt_m > T) dt_m = T - t;
+        if (!(dt_m > 0.0) || t + dt_m == t || n >= 10000000) return CMP_ERR_NAN;
+        if ((r = amr_fill_pf(A, npf, P.A.data(), P.fdP, s))) return r;
+        if (cac) {
+            for (int ph = 0; ph < npf; ++ph) {
+                for (int l = 0; l < nlev; ++l) {
+                    cmp_level* L = A->L[l];
+                    CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, P.A[l] + ph * L->na, L->d_N, s));
+                }
+                if ((r = amr_fill_pf(A, dim, Nl.data(), fdN, s))) return r;
+                for (int l = 0; l < nlev; ++l) {
+                    cmp_level* L = A->L[l];
+                    CMP_TRY(launch_pf_cac_mech(dim, L->pg, L->maxcells, P.A[l] + ph * L->na, L->d_N, L->d_Mk,
+                                               L->d_Wk, mob, lambda, eta, L->d_part, s));
+                }
+                if ((r = amr_avgdown_cells(A, Mk.data(), s))) return r;
+                if ((r = amr_avgdown_cells(A, Wk.data(), s))) return r;
+                CMP_TRY(launch_cell_sum(dim, L0->pg, L0->maxcells, L0->d_Mk, L0->d_part, s));
+                CMP_TRY(launch_cell_sum(dim, L0->pg, L0->maxcells, L0->d_Wk, L0->d_part + L0->nblocks, s));
+                CMP_TRY(launch_reduce_sums(L0->d_part, L0->nblocks, 2, L0->d_sc + CG_SUM, s));
+                for (int l = 0; l < nlev; ++l) {
+                    cmp_level* L = A->L[l];
+                    CMP_TRY(launch_pf_cac_update(dim, L->pg, L->maxcells, P.A[l] + ph * L->na, L->d_Mk, L->d_Wk,
+                                                 L0->d_sc + CG_SUM, dt_m, s));
+                }
+            }
+        } else {
+            for (int l = 0; l < nlev; ++l) {
+                cmp_level* L = A->L[l];
+                CMP_TRY(launch_pf_map(P.A[l], P.Ps[l], L->na * npf, eta, eta, 0, s));
+            }
+            if ((r = amr_fill_pf(A, npf, P.Ps.data(), P.fdP, s))) return r;
+            for (int ph = 0; ph < npf; ++ph) {
+                for (int l = 0; l < nlev; ++l) {
+                    cmp_level* L = A->L[l];
+                    CMP_TRY(launch_pf_normals(dim, L->pg, L->maxcells, P.Ps[l] + ph * L->na, L->d_N, s));
+                }
+                if ((r = amr_fill_pf(A, dim, Nl.data(), fdN, s))) return r;
+                for (int l = 0; l < nlev; ++l) {
+                    cmp_level* L = A->L[l];
+                    // the face psi from alpha of the two cells (FDM_PhaseFieldFlux_ACDI_D maps alpha)
+                    CMP_TRY(launch_pf_map(P.A[l] + ph * L->na, L->d_P, L->na, eta, eta, 0, s));
+                    CMP_TRY(launch_pf_jfaces(dim, L->pg, L->maxfaces, P.A[l] + ph * L->na, L->d_P, L->d_N, L->d_J,
+                                             nullptr, mob_face, eta, dt_m, s));
+                }
+                if ((r = amr_avgdown_faces(A, Jl.data(), s))) return r;
+                for (int l = 0; l < nlev; ++l) {
+                    cmp_level* L = A->L[l];
+                    if (explicit_rc) {
+                        AmrPfArgs a{};
+                        a.n = L->nfaces;
+                        a.a = dt_m;
+                        a.src = L->d_J;
+                        a.dst = P.H[l] + ph * L->nfaces;
+                        if ((r = ap_launch(dim, AP_AXPY, a, s))) return r;
+                    }
+                    CMP_TRY(launch_pf_update(dim, L->pg, L->maxcells, P.A[l] + ph * L->na, L->d_J, dt_m, s));
+                }
+            }
+        }
+        t += dt_m;
+        ++n;
+    }
+    // Grad_Q of every level
+    for (int ph = 0; ph < npf; ++ph) {
+        std::vector<double*> out(nlev);
+        for (int l = 0; l < nlev; ++l) out[l] = (npf == 1) ? A->L[l]->d_gq : A->L[l]->d_gq1;
+        if (explicit_rc) {
+            for (int l = 0; l < nlev; ++l) {
+                cmp_level* L = A->L[l];
+                CMP_TRY(launch_pf_map(P.A0[l] + ph * L->na, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
+                CMP_TRY(launch_pf_gradq(dim, L->pg, L->maxfaces, L->d_P, P.H[l] + ph * L->nfaces, 1.0 / dt, out[l],
+                                        L->d_bits + 1, nullptr, s));
+            }
+        } else {
+            std::vector<double*> Bf(nlev);
+            for (int l = 0; l < nlev; ++l) {
+                cmp_level* L = A->L[l];
+                Bf[l] = L->d_Bf;
+                CMP_TRY(launch_pf_divjhat(dim, L->pg, L->maxcells, P.A[l] + ph * L->na, P.A0[l] + ph * L->na, L->d_DJ,
+                                          dt, s));
+                CMP_TRY(launch_pf_map(P.A0[l] + ph * L->na, L->d_P, L->na, eta, pf.eta_factor * eta, 1, s));
+                CMP_TRY(launch_pf_bcoef(dim, L->pg, L->maxfaces, L->d_P, L->d_Bf, s));
+            }
+            if ((r = amr_avgdown_faces(A, Bf.data(), s))) return r;
+            if ((r = amr_comp_solve(A, out.data(), s))) return r;
+        }
+        if (npf > 1)
+            for (int l = 0; l < nlev; ++l) {
+                cmp_level* L = A->L[l];
+                CMP_TRY(launch_pf_scatter(dim, L->pg, L->maxfaces, out[l], L->d_gq, ph, s));
+            }
+    }
+    // max |Grad_Q| of every level, every box flagged for the sub-steps
+    for (int l = 0; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        AmrPfArgs a{};
+        a.n = L->nfaces * npf;
+        a.src = L->d_gq;
+        a.bits = P.d_bits + 1 + l;
+        if ((r = ap_launch(dim, AP_MAXABS, a, s))) return r;
+        if (gradq && gradq[l])
+            CMP_TRY(hipMemcpyAsync(gradq[l], L->d_gq, sizeof(double) * L->nfaces * npf, hipMemcpyDeviceToDevice, s));
+    }
+    CMP_TRY(hipMemcpyAsync(P.h_pin + 4, P.d_bits + 1, sizeof(double) * nlev, hipMemcpyDeviceToHost, s));
+    CMP_TRY(hipStreamSynchronize(s));
+    if (gradq_max)
+        for (int l = 0; l < nlev; ++l) std::memcpy(&gradq_max[l], P.h_pin + 4 + l, sizeof(double));
+    if (info) {
+        info[0] = maxv;
+        info[1] = mob;
+        info[2] = eta;
+        info[3] = (double)n;
+        info[4] = explicit_rc ? 0.0 : (double)P.cg_iters;
+    }
+    return CMP_OK;
+}
+
+int cmp_amr_pf_evolve(cmp_amr* A, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
+                      double* info, void* stream)
+{
+    if (!A || !Un || !U1 || !U2 || !dt_dev || !A->pf.on) return CMP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    AmrPf& P = A->pf;
+    const int nlev = (int)A->L.size();
+    cmp_level* L0 = A->L[0];
+    CMP_TRY(hipMemcpyAsync(P.h_pin + 2, dt_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+    CMP_TRY(hipStreamSynchronize(s));
+    const double dt = P.h_pin[2];
+    if (dt == 0.0) {
+        if (info)
+            for (int q = 0; q < 7; ++q) info[q] = 0.0;
+        return CMP_OK;
+    }
+    double gm[AMR_MAX_LEVELS], inf5[5];
+    int r = cmp_amr_pf_gradq(A, Un, dt, nullptr, gm, inf5, stream);
+    if (r) return r;
+    // sub-steps: PhaseField_ComputeDt over the levels (nsubsteps 1, run.do_subcycle = 0, :2037-2098)
+    const int dim = L0->P.dim;
+    double prev[AMR_MAX_LEVELS] = {0.0};
+    double pf_time = 0.0, gmax = 0.0;
+    for (int l = 0; l < nlev; ++l) gmax = (gm[l] > gmax) ? gm[l] : gmax;
+    int n = 0;
+    for (int step = 0; step < P.pf.max_step && pf_time < dt; ++step) {
+        double tmp[AMR_MAX_LEVELS];
+        for (int l = 0; l < nlev; ++l) {
+            double dxm = A->L[l]->G.dx[0];
+            for (int d = 1; d < dim; ++d) dxm = (A->L[l]->G.dx[d] < dxm) ? A->L[l]->G.dx[d] : dxm;
+            tmp[l] = P.pf.cfl * dxm / (gm[l] + 1.0e-14);
+            if (pf_time == 0.0) prev[l] = tmp[l];
+        }
+        double dt0 = tmp[0];
+        for (int l = 0; l < nlev; ++l) {
+            if (P.pf.change_limiter && prev[l] > 0.0) tmp[l] = (1.1 * prev[l] < tmp[l]) ? 1.1 * prev[l] : tmp[l];
+            dt0 = (tmp[l] < dt0) ? tmp[l] : dt0;
+        }
+        const double eps = 1.0e-3 * dt0;
+        if (pf_time + dt0 > dt - eps) dt0 = dt - pf_time;
+        double dmin = dt0;
+        for (int l = 0; l < nlev; ++l) dmin = (tmp[l] < dmin) ? tmp[l] : dmin;
+        for (int l = 0; l < nlev; ++l) prev[l] = dmin;
+        CMP_TRY(launch_setd(L0->d_pfdt, dmin, s));
+        if ((r = amr_stages(A, Un, U1, U2, L0->d_pfdt, nullptr, true, s))) return r;
+        pf_time += dmin;
+        ++n;
+        if (pf_time >= dt - 1.0e-6 * dt) break;
+    }
+    if (info) {
+        for (int q = 0; q < 4; ++q) info[q] = inf5[q];
+        info[4] = gmax;
+        info[5] = (double)n;
+        info[6] = inf5[4];
+    }
     return CMP_OK;
 }
 

@@ -1253,6 +1253,11 @@ static inline double bits_to_double(const void* p)
     return v;
 }
 
+}  // extern "C"
+
+using cmp::mg_prepare;
+using cmp::mg_vcycle;
+
 static int mg_launch(int dim, int op, const cmp::MgArgs& a, hipStream_t s)
 {
     CMP_TRY(dim == 2 ? cmp::launch_mg_dim<2>(op, a, s) : cmp::launch_mg_dim<3>(op, a, s));
@@ -1272,7 +1277,7 @@ static cmp::MgArgs mg_args(const cmp_level* L, const double* sc)
 
 // Face coefficients of every multigrid grid (the level's b gathered, then restricted) and their
 // inverse diagonals; once per solve.
-static int mg_prepare(cmp_level* L, hipStream_t s)
+int cmp::mg_prepare(cmp_level* L, hipStream_t s)
 {
     const int dim = L->P.dim;
     cmp::MgArgs a = mg_args(L, nullptr);
@@ -1301,7 +1306,7 @@ static int mg_prepare(cmp_level* L, hipStream_t s)
 // One V-cycle on grid m from its residual r, the result left in x[cur]: two damped Jacobi sweeps
 // (the first from zero), the residual restricted, the coarser grid's V-cycle prolonged and added,
 // two more sweeps; the coarsest grid takes its fixed number of sweeps in one workgroup.
-static int mg_vcycle(cmp_level* L, const double* sc, size_t m, hipStream_t s)
+int cmp::mg_vcycle(cmp_level* L, const double* sc, size_t m, hipStream_t s)
 {
     const int dim = L->P.dim;
     auto& V = L->mg[m];
@@ -1365,14 +1370,16 @@ static int mg_precondition(cmp_level* L, const double* sc, double* P, hipStream_
     return mg_launch(dim, cmp::MG_CG_RZ, a, s);
 }
 
+extern "C" {
+
 // The implicit Grad_Q (PhaseField_GradQPF with Div_J, source/Class_Compressible_PhaseField.cpp:2133-2310):
 // Div_Jhat = (alpha_new - alpha)/dt (:703); b = W(phi_f) + 1e-3 on the faces from the re-sharpened
 // alpha; Q solves div(b grad Q) = Div_Jhat with periodic / homogeneous Neumann sides; Grad_Q = grad Q
 // on the faces, zero on Neumann domain faces. The reference hands the system to AMReX's MLMG (relative
-// residual 1e-10); here conjugate gradients with a Jacobi (diagonal) preconditioner on the zero-mean
-// subspace run to the rounding floor (r.r <= 1e-28 of the initial r.r, or 50 iterations without a
-// new minimum), each iteration five passes over the level and two fixed-order reductions, the
-// stopping test on the device.
+// residual 1e-10); here conjugate gradients on the zero-mean subspace, preconditioned by a multigrid
+// V-cycle when the level covers its domain (else by the Jacobi diagonal), run to the rounding floor
+// (r.r <= 1e-28 of the initial r.r, or 400 iterations without a new minimum), with fixed-order
+// reductions and the stopping test on the device.
 static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq, hipStream_t s)
 {
     const int dim = L->P.dim;

@@ -1272,7 +1272,8 @@ void k_fill2(FillDev fd, FillDev sfd, int has_src, FillDev cd, int box0, int rat
              const double* Us, const double* __restrict__ Uc, int* err)
 {
 #pragma clang fp contract(off)
-    constexpr int MOM = (N == 4 + DIM) ? 2 : (N - DIM - 1) / 2;
+    // first momentum component of a flow state; a Phase-Field field (1 to 3 comps) has none
+    constexpr int MOM = (N == 4 + DIM) ? 2 : ((N > 3) ? (N - DIM - 1) / 2 : -100);
     const int b = box0 + (int)blockIdx.y;
     const BoxDev bx = fd.boxes[b];
     const int ng = fd.ng;
@@ -1390,8 +1391,9 @@ void k_cf_faces(KParams kp, int limiter, const CfFace* __restrict__ jobs, int nj
     if (bad) atomicMax(err, bad);
 }
 
-// k_cf_apply: one thread per listed coarse cell, its face jobs combined in their fixed order.
-template <int DIM, int NP>
+// k_cf_apply: one thread per listed coarse cell, its face jobs combined in their fixed order. PF: a
+// Phase-Field stage (faces hold -fhat; no non-conservative term).
+template <int DIM, int NP, bool PF = false>
 __global__ __launch_bounds__(64)
 void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev* __restrict__ cboxes,
                 const double* __restrict__ Uc, const double* __restrict__ faces, int ratio, double* Uout,
@@ -1436,7 +1438,9 @@ void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev
     double Ucell[N];
 #pragma unroll
     for (int q = 0; q < N; ++q) Ucell[q] = Uc[c + q * cb.sn];
-    if constexpr (NP > 0) {
+    if constexpr (PF) {
+        (void)ddiv;
+    } else if constexpr (NP > 0) {
         // every alpha_k takes phi_k (1 + K_k) times the change of div u*
         double f[NP > 0 ? NP : 1];
         np_nc_coefficients<DIM, (NP > 0 ? NP : 2)>(kp, Ucell, f, bad);
@@ -1454,6 +1458,123 @@ void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev
         Uout[c + q * cb.sn] = v;
     }
     if (bad) atomicMax(err, bad);
+}
+
+// The Phase-Field flux of the face between cells c - e_DIR and c of box bx (Ub its fab, ghosts
+// filled) at the Grad_Q values g of that face, every stencil cell taken through cons->prim here.
+template <int DIM, int SCHEME, int FORM, int DIR, int NP>
+__device__ __forceinline__ void face_at_pf(const KParams& kp, int limiter, const BoxDev& bx,
+                                           const double* __restrict__ Ub, int ci, int cj, int ck, const double* g,
+                                           double* Fl, int& bad)
+{
+    constexpr int N = (NP > 0) ? 2 * NP + DIM + 1 : 4 + DIM;
+    constexpr int G = Stencil<SCHEME>::G;
+    const long long sd = (DIR == 0) ? 1 : ((DIR == 1) ? bx.sj : bx.sk);
+    const long long c0 = fab_index(bx, ci, cj, ck);
+    double W[2 * G][N];
+#pragma unroll
+    for (int t = 0; t < 2 * G; ++t) {
+        double Uc[N];
+        const long long c = c0 + (t - G) * sd;
+#pragma unroll
+        for (int q = 0; q < N; ++q) Uc[q] = Ub[c + q * bx.sn];
+        if constexpr (NP > 0) np_cons2prim<DIM, NP>(kp, Uc, W[t], bad);
+        else cons2prim<DIM>(kp, Uc, W[t], bad);
+    }
+    auto st = [&](int t, int q) -> double { return W[t][q]; };
+    if constexpr (NP > 0)
+        np_pf_face_flux<DIM, SCHEME, FORM, (NP > 0 ? NP : 2)>(kp, limiter, DIR, st, g, Fl, bad);
+    else
+        pf_face_flux<DIM, SCHEME, FORM>(kp, limiter, DIR, st, g[0], Fl, bad);
+}
+
+// The coarse-fine face jobs of a Phase-Field stage (PhaseField_AdvanceAllLevels, the Grad_Q form of
+// StageRHSAllLevels): the Phase-Field flux at the face's Grad_Q (each box's face block of cmp_face_layout
+// with one component per volume fraction), stored as -fhat as PhaseField_ComputeFaceFluxes stores it
+// (source/Class_Compressible_PhaseField.cpp:840), so that k_cf_apply's -div F is the stage's +div fhat;
+// a face with Grad_Q = 0 has the flux 0.
+template <int DIM, int SCHEME, int FORM, int NP>
+__global__ __launch_bounds__(64)
+void k_cf_faces_pf(KParams kp, int limiter, const CfFace* __restrict__ jobs, int njobs, const BoxDev* __restrict__ cboxes,
+                   const double* __restrict__ Uc, const double* __restrict__ gqc, const BoxDev* __restrict__ fboxes,
+                   const double* __restrict__ Uf, const double* __restrict__ gqf, double* __restrict__ faces, int* err)
+{
+    constexpr int N = (NP > 0) ? 2 * NP + DIM + 1 : 4 + DIM;
+    constexpr int NG = (NP > 0) ? NP : 1;
+    const int j = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (j >= njobs) return;
+    const CfFace fj = jobs[j];
+    const BoxDev bx = fj.fine ? fboxes[fj.box] : cboxes[fj.box];
+    const double* Ub = (fj.fine ? Uf : Uc) + bx.off;
+    const double* gq = fj.fine ? gqf : gqc;
+    const int d = fj.dir;
+    const int n0 = bx.hi[0] - bx.lo[0] + 1 + (d == 0), n1 = bx.hi[1] - bx.lo[1] + 1 + (d == 1);
+    const int n2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + (d == 2) : 1;
+    const long long nface = (long long)n0 * n1 * n2;
+    const long long q = (long long)(fj.c[0] - bx.lo[0]) +
+                        (long long)n0 * ((fj.c[1] - bx.lo[1]) + (long long)n1 * ((DIM == 3) ? fj.c[2] - bx.lo[2] : 0));
+    double g[NG];
+    bool any = false;
+#pragma unroll
+    for (int m = 0; m < NG; ++m) {
+        g[m] = gq[bx.goff[d] + q + m * nface];
+        any = any || (g[m] != 0.0);
+    }
+    int bad = 0;
+    double F[N];
+#pragma unroll
+    for (int n = 0; n < N; ++n) F[n] = 0.0;
+    if (any) {
+        if (d == 0)
+            face_at_pf<DIM, SCHEME, FORM, 0, NP>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], g, F, bad);
+        else if (d == 1)
+            face_at_pf<DIM, SCHEME, FORM, 1, NP>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], g, F, bad);
+        else
+            face_at_pf<DIM, SCHEME, FORM, (DIM == 3 ? 2 : 1), NP>(kp, limiter, bx, Ub, fj.c[0], fj.c[1], fj.c[2], g,
+                                                                  F, bad);
+    }
+    double* o = faces + (long long)j * (N + 1);
+#pragma unroll
+    for (int n = 0; n < N; ++n) o[n] = -F[n];
+    o[N] = 0.0;
+    if (bad) atomicMax(err, bad);
+}
+
+template <int DIM, int SCHEME, int FORM, int NP>
+static hipError_t cfcorr_pf(const KParams& kp, int limiter, int part, const CfCell* list, int n, const CfFace* jobs,
+                            int njobs, double* faces, const BoxDev* cboxes, const double* Uc, const double* gqc,
+                            const BoxDev* fboxes, const double* Uf, const double* gqf, int ratio, double* Uout,
+                            const double* dt, int stage, int* err, hipStream_t s)
+{
+    if (part & 1)
+        hipLaunchKernelGGL((k_cf_faces_pf<DIM, SCHEME, FORM, NP>), dim3((njobs + 63) / 64), dim3(64), 0, s, kp, limiter,
+                           jobs, njobs, cboxes, Uc, gqc, fboxes, Uf, gqf, faces, err);
+    if (part & 2)
+        hipLaunchKernelGGL((k_cf_apply<DIM, NP, true>), dim3((n + 63) / 64), dim3(64), 0, s, kp, list, n, cboxes, Uc,
+                           faces, ratio, Uout, dt, stage, err);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_cf_correct_pf_dim(const LaunchCfg& lc, const KParams& kp, int part, const CfCell* list, int n,
+                                    const CfFace* jobs, int njobs, double* faces, const BoxDev* cboxes,
+                                    const double* Uc, const double* gqc, const BoxDev* fboxes, const double* Uf,
+                                    const double* gqf, int ratio, double* Uout, const double* dt, int stage, int* err,
+                                    hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (lc.riemann != RIEMANN_PF_MIX && lc.riemann != RIEMANN_PF_PHASIC) return hipErrorInvalidValue;
+    const int form = lc.riemann - RIEMANN_PF_MIX;
+#define CMP_CFPF_ARGS kp, lc.limiter, part, list, n, jobs, njobs, faces, cboxes, Uc, gqc, fboxes, Uf, gqf, ratio, Uout, \
+                      dt, stage, err, s
+#define CMP_CFPF(SC, NPV)                                                                                       \
+    if (lc.scheme == SC) return form ? cfcorr_pf<DIM, SC, 1, NPV>(CMP_CFPF_ARGS) : cfcorr_pf<DIM, SC, 0, NPV>(CMP_CFPF_ARGS);
+    if (lc.nphase == 3) { CMP_CFPF(5, 3) CMP_CFPF(1, 3) CMP_CFPF(0, 3) }
+    else if (lc.nphase == 2) { CMP_CFPF(5, 2) CMP_CFPF(1, 2) CMP_CFPF(0, 2) }
+    else { CMP_CFPF(5, 0) CMP_CFPF(6, 0) CMP_CFPF(1, 0) CMP_CFPF(0, 0) }
+#undef CMP_CFPF
+#undef CMP_CFPF_ARGS
+    return hipErrorInvalidValue;
 }
 
 template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP = 0>
@@ -1502,7 +1623,7 @@ template <int DIM>
 hipError_t launch_fill2_dim(int N, const FillDev& fd, const FillDev* sfd, const FillDev& cd, int ratio, double* Uf,
                             const double* Us, const double* Uc, long long maxcells, int all, int* err, hipStream_t s)
 {
-    if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM) return hipErrorInvalidValue;
+    if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM && (N < 1 || N > 3)) return hipErrorInvalidValue;
     long long cols = ((all ? maxcells : fd.maxghost) + NT - 1) / NT;
     if (cols <= 0) return hipSuccess;
     if (cols > 65536) cols = 65536;
@@ -1519,7 +1640,10 @@ hipError_t launch_fill2_dim(int N, const FillDev& fd, const FillDev* sfd, const 
                                 Us, Uc, err);
         if (N == 4 + DIM) { CMP_F2(4 + DIM) }
         else if (N == 5 + DIM) { CMP_F2(5 + DIM) }
-        else { CMP_F2(7 + DIM) }
+        else if (N == 7 + DIM) { CMP_F2(7 + DIM) }
+        else if (N == 3) { CMP_F2(3) }
+        else if (N == 2) { CMP_F2(2) }
+        else { CMP_F2(1) }
 #undef CMP_F2
     }
     return hipGetLastError();
@@ -1549,7 +1673,7 @@ hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, const int
                            hipStream_t s)
 {
     if (nghost <= 0) return hipSuccess;
-    if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM && N != DIM && N != 1) return hipErrorInvalidValue;
+    if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM && (N < 1 || N > 3)) return hipErrorInvalidValue;
     // grid rows are boxes (at most 65535 per launch); grid columns cover the largest box's ghosts
     long long cols = (fd.maxghost + NT - 1) / NT;
     if (cols > 65536) cols = 65536;
@@ -1563,8 +1687,11 @@ hipError_t launch_fill_dim(int N, const FillDev& fd, long long nghost, const int
             hipLaunchKernelGGL((k_fill<DIM, 5 + DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
         else if (N == 7 + DIM)
             hipLaunchKernelGGL((k_fill<DIM, 7 + DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
-        else if (N == DIM)
-            hipLaunchKernelGGL((k_fill<DIM, DIM>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
+        // a Phase-Field field: 1 to 3 components (volume fractions, or the cell normals)
+        else if (N == 3)
+            hipLaunchKernelGGL((k_fill<DIM, 3>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
+        else if (N == 2)
+            hipLaunchKernelGGL((k_fill<DIM, 2>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
         else
             hipLaunchKernelGGL((k_fill<DIM, 1>), grid, dim3(NT), 0, s, fd, list, box0, U, err);
     }
@@ -1593,6 +1720,10 @@ template hipError_t launch_cf_correct_dim<CMP_KERNEL_DIM>(const LaunchCfg&, cons
                                                          const CfFace*, int, double*, const BoxDev*,
                                                          const double*, const BoxDev*, const double*, int,
                                                          double*, const double*, int, int*, hipStream_t);
+template hipError_t launch_cf_correct_pf_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, int, const CfCell*, int,
+                                                            const CfFace*, int, double*, const BoxDev*, const double*,
+                                                            const double*, const BoxDev*, const double*, const double*,
+                                                            int, double*, const double*, int, int*, hipStream_t);
 template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev*, const FillDev&, int,
                                                      double*, const double*, const double*, long long, int, int*,
                                                      hipStream_t);

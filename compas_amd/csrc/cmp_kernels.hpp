@@ -138,6 +138,8 @@ struct CfFace {
 // Up to this many levels in a hierarchy.
 constexpr int AMR_MAX_LEVELS = 8;
 
+
+
 struct AmrDt {
     int nlev;
     int dim;
@@ -221,6 +223,83 @@ struct MgArgs {
 
 template <int DIM>
 hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s);
+
+// ---- the composite implicit Grad_Q of a hierarchy (cmp_amr_pf.hip) ----------------------------------
+// mask of a level's cell (alpha layout): covered by the next finer level, a regular cell (five- or
+// seven-point stencil over the same-level ghosts), or >= 0: the index of its CompCell
+constexpr int COMP_COVERED = -2;
+constexpr int COMP_REGULAR = -1;
+
+// one link of a cell with a coarse-fine face: the neighbour (level, index in its alpha layout), the
+// face whose b it takes (level, index in that level's one-component face buffer) and area / distance
+struct CompLink {
+    int lev;
+    int flev;
+    long long idx;
+    long long face;
+    double geo;
+};
+struct CompCell {
+    int nlink;
+    int link0;
+};
+struct CompLev {
+    const BoxDev* abox;     // alpha layout (1 ghost, 1 comp), goff = one-component face offsets
+    FillDev fd;             // owner map and BCs of that layout
+    const int* mask;
+    double* v;              // the vector the operator reads (or the solution, for the gradient)
+    const double* bf;       // b on the level's faces
+    double dx[3];
+    double area[3];         // face areas per direction
+    double vol;             // cell volume
+};
+struct CompArgs {
+    int nlev;
+    int ratio;
+    CompLev L[AMR_MAX_LEVELS];
+    const CompCell* cells;
+    const CompLink* links;
+};
+
+enum AmrPfOp {
+    AP_AVGDOWN_FACES, AP_AXPY, AP_MAXABS, AP_SUM_LEVELS, AP_COMP_AP, AP_COMP_DIAG, AP_COMP_INIT, AP_COMP_S,
+    AP_COMP_Z, AP_COMP_GRAD
+};
+
+// Arguments of one hierarchy Phase-Field operation (unused fields ignored):
+//   AVGDOWN_FACES: fboxes (fine alpha layout, nbox boxes, maxfaces) / cfd (coarse) src -> dst;
+//   AXPY: dst += a src (n); MAXABS: bits = max(bits, max |src|) (n); SUM_LEVELS: dst[v] = sum_l src[l nvals + v];
+//   COMP_AP / COMP_DIAG: level lev of *ca: dst = A v (partials of v.Av into part) / dst = 1/diag;
+//   COMP_INIT: mean_sum null: partials of V rhs (src); else dst = X = 0, dst2 = R, partials of r.r;
+//   COMP_S: dst = src / V; COMP_Z: dst = src (R) src2 (ID) [levels > 0] + src3 (x0 dense, G0) at the
+//   ancestor; COMP_GRAD: dst = Grad_Q of the level's faces from ca->L[lev].v.
+struct AmrPfArgs {
+    const BoxDev* fboxes;
+    FillDev cfd;
+    int nbox;
+    long long maxcells;
+    long long maxfaces[3];
+    int ratio;
+    const double* src;
+    const double* src2;
+    const double* src3;
+    double* dst;
+    double* dst2;
+    long long n;
+    double a;
+    unsigned long long* bits;
+    int nlev, nvals;
+    const double* sc;
+    const CompArgs* ca;
+    int lev;
+    double* part;
+    const double* mean_sum;
+    MgGrid G0;
+    int dlo0[3];
+};
+
+template <int DIM>
+hipError_t launch_amrpf_dim(int op, const AmrPfArgs& p, hipStream_t s);
 
 
 // volume fraction `comp` of the flow state -> an alpha-layout buffer (and a second one, A2, if given)

@@ -23,6 +23,15 @@ using cmp::TileDev;
 
 #define CMP_TRY(x) do { int _r = cmp::check_hip(x); if (_r) return _r; } while (0)
 
+struct cmp_level;
+namespace cmp {
+// the multigrid preconditioner of a level that covers its domain (cmp_capi.hip): face coefficients
+// of every grid from the level's d_Bf, and one V-cycle from grid m's residual (its result in
+// mg[m].x[mg[m].cur])
+int mg_prepare(cmp_level* L, hipStream_t s);
+int mg_vcycle(cmp_level* L, const double* sc, size_t m, hipStream_t s);
+}  // namespace cmp
+
 struct cmp_level {
     cmp_params P;
     cmp_geometry G;

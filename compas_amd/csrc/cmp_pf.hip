@@ -769,7 +769,8 @@ __global__ __launch_bounds__(NT) void k_reduce_sums(const double* __restrict__ p
 // CG bookkeeping on the device, so that the iteration stops where the host loop of the solve's
 // restatement stops without a host round trip per iteration: start after r.r of the initial
 // residual landed in sc[CG_RR]; end after each iteration (pAp <= 0 stops without counting it;
-// r.r <= 1e-28 b.b, 50 iterations without a new minimum or maxit stop after it).
+// r.r <= 1e-28 b.b, 400 iterations without a new minimum of r.r (CG's residual norm is not monotone
+// and can plateau for tens of iterations before it falls again) or maxit stop after it).
 __global__ void k_cg_start(double* sc)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -801,7 +802,7 @@ __global__ void k_cg_end(double* sc, double maxit)
     } else {
         const double st = sc[CG_STALL] + 1.0;
         sc[CG_STALL] = st;
-        done = st > 50.0;
+        done = st > 400.0;
     }
     if (!(rr > 1.0e-28 * sc[CG_B2]) || !(it < maxit)) done = true;
     if (done) sc[CG_DONE] = 1.0;

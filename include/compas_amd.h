@@ -400,6 +400,44 @@ int cmp_amr_rk3_step(cmp_amr* amr, double* const* Un, double* const* U1, double*
 int cmp_amr_advance(cmp_amr* amr, double* const* Un, double* const* U1, double* const* U2, int nsteps, double cfl,
                     int limiter, double* dt_dev, double* cmax, double* dt_hist, int use_graph, void* stream);
 
+/* The Phase-Field step of the hierarchy (PhaseField_Evolve over finest_level + 1 levels without
+ * subcycling, source/Class_Compressible_PhaseField.cpp:425-594; the Phase-Field sub-steps by
+ * PhaseField_TimeStepNoSubcycling -> PhaseField_AdvanceAllLevels,
+ * source/Class_Compressible_PhaseField_StageCoupling.cpp:43-51, and AverageDown). cmp_amr_pf_setup
+ * calls cmp_level_pf_setup on every level and builds the hierarchy's own work buffers; after it,
+ * cmp_amr_advance ends every step with cmp_amr_pf_evolve. The mechanism sub-steps (dt_m from the
+ * finest cell size) advance every level with the sub-step state filled on every level (all phases
+ * at once, the coarse-fine ghosts interpolated as the two-level fill interpolates); CAC takes its two
+ * sums over level 0 after the levels' mech and W are averaged down; ACDI's J is averaged down
+ * (AverageDownFaces) before its divergence and before Jhat takes it. With the explicit RC every level
+ * has its Grad_Q from its Jhat; with the implicit RC Grad_Q comes from one composite solve over the
+ * cells no finer level covers: a face between two such cells of one level carries
+ * b (Q_n - Q_c) / dx, a coarse-fine face per fine face on it b_fine (Q_coarse - Q_fine) /
+ * ((dx_fine + dx_coarse) / 2) (a two-point flux where the reference's MLMG, composite_solve = 1,
+ * LinearSystem_Parameter.H:16, uses its own higher-order coarse-fine stencil), solved by conjugate
+ * gradients preconditioned by Jacobi on the finer levels plus a multigrid V-cycle of level 0 at every
+ * cell's level-0 ancestor, to the rounding floor; covered cells then take the mean of their children
+ * and the coarse faces under a finer level the mean of its faces. */
+int cmp_amr_pf_setup(cmp_amr* amr, const cmp_pf_params* pf);
+/* Grad_Q of every level for a flow step of size dt into every level's own Grad_Q buffer, copied into
+ * gradq[l] (device face buffers of cmp_face_layout with one component per volume fraction) where
+ * gradq and gradq[l] are non-null; gradq_max (host, nlev doubles, or NULL): max |Grad_Q| per level;
+ * info (host, 5 doubles, or NULL): max|u| over the levels, mobility, eta, mechanism sub-steps, CG
+ * iterations of the last implicit solve. Synchronises the stream. */
+int cmp_amr_pf_gradq(cmp_amr* amr, double* const* U, double dt, double* const* gradq, double* gradq_max,
+                     double* info, void* stream);
+/* The Phase-Field step after a flow step of size *dt_dev (nothing when it is 0): Grad_Q, then
+ * sub-steps of PhaseField_ComputeDt over the levels (the smallest of the levels' estimates, limited
+ * to 1.1 the previous sub-step, the last one cut to end at dt), each a stage-coupled TVD-RK3 step of
+ * the Phase-Field right-hand side of every level (U1, U2 scratch level buffers) with AverageDown.
+ * info (host, 7 doubles, or NULL): the 5 of cmp_amr_pf_gradq in order max|u|, mobility, eta,
+ * mechanism sub-steps, then max|Grad_Q| over the levels, the number of sub-steps, CG iterations.
+ * Synchronises the stream. */
+int cmp_amr_pf_evolve(cmp_amr* amr, double* const* Un, double* const* U1, double* const* U2, const double* dt_dev,
+                      double* info, void* stream);
+/* Iterations of the last composite implicit solve (0 with the explicit RC). */
+int cmp_amr_pf_solver_iterations(const cmp_amr* amr, int* iterations);
+
 /* ---- a batch of independent levels (the served-stream form) ---------------------------------
  * No counterpart in the reference, which runs one case per process (source/main.cpp:21-86): cases of
  * different grids and parameters are admitted while others run, keep their state in device memory,
