@@ -201,6 +201,9 @@ def shock_bubble_amr(args):
     U1f = torch.from_numpy(l1.pack(c1["U"])).to(dev)
     L1.AverageDownTo(U1f, L0, U0, 2)
     amr = Amr([L0, L1], 2)
+    pf = _amr_pf(args)
+    if pf is not None:
+        amr.pf_setup(pf)
     Un = [U0, U1f]
     S1 = [torch.zeros_like(u) for u in Un]
     S2 = [torch.zeros_like(u) for u in Un]
@@ -216,11 +219,14 @@ def shock_bubble_amr(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     assert amr.check_errors() == 0
+    pf_rec = _amr_pf_split(amr, Un, S1, S2, dt, pf)
     amr.close()
     # the uniform fine grid the hierarchy stands for, single-level path (graph-replayed steps)
     lu = Layout(2, c1["boxes"], 3, P.nstate)
     Lu = Level(P, c1["geom"], lu)
     Uu = torch.from_numpy(lu.pack(c1["U"])).to(dev)
+    if pf is not None:
+        Lu.pf_setup(pf)
     Vu1, Vu2 = torch.zeros_like(Uu), torch.zeros_like(Uu)
     du = torch.zeros(1, dtype=torch.float64, device=dev)
     cu = torch.zeros(1, dtype=torch.float64, device=dev)
@@ -254,7 +260,7 @@ def shock_bubble_amr(args):
                    "fine_boxes": len(fine_boxes), "cells_per_level": cells,
                    "fine_coverage": cells[1] / (4.0 * cells[0]), "tagged_cells": int(tags.sum()),
                    "scheme": "WENO5B", "riemann": "HLLC", "integrator": "TVD-RK3 (levels stage-coupled)",
-                   "cfl": 0.4, "phase_field": None},
+                   "cfl": 0.4, "phase_field": _amr_pf_config(pf)},
         "uniform_fine_grid": {"grid": [2048, 512], "ms_per_step": wall_u / K * 1e3,
                               "cell_updates_per_s": 2048 * 512 * K / wall_u},
         "amr_step_over_uniform_step": (wall / K) / (wall_u / K),
@@ -262,7 +268,40 @@ def shock_bubble_amr(args):
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": None, "flops_per_cell_stage": fps},
     }
+    if pf_rec is not None:
+        rec["phase_field"] = pf_rec
     print(json.dumps(rec), flush=True)
+
+
+def _amr_pf(args):
+    """The hierarchy workloads' interface regularization: CAC-Adv with the implicit RC (CAC requires
+    it), Phase-Field CFL 0.3, eta = 0.5 finest cells; None with --no-pf."""
+    if args.no_pf:
+        return None
+    from compas_amd import make_pf_params
+    return make_pf_params(mechanism="CAC-Adv", explicit_rc=0, cfl=0.3, eta_multiplier=0.5)
+
+
+def _amr_pf_config(pf):
+    if pf is None:
+        return None
+    return {"mechanism": "CAC-Adv", "explicit_rc": 0, "flux": "UD mixture", "cfl": 0.3, "eta_multiplier": 0.5,
+            "grad_q": "composite implicit solve over the levels (CG, Jacobi + level-0 multigrid V-cycle)"}
+
+
+def _amr_pf_split(amr, Un, S1, S2, dt, pf):
+    """One more Phase-Field step of the hierarchy after the timed steps, timed alone (it is inside
+    every timed step too): its wall time, sub-step counts and composite CG iterations."""
+    if pf is None:
+        return None
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    info = amr.PhaseField_Evolve(Un, S1, S2, dt)
+    ms = (time.perf_counter() - t) * 1e3
+    assert amr.check_errors() == 0
+    return {"ms_per_pf_step": ms, "mechanism_substeps": info[3], "pf_rk3_substeps": info[5],
+            "max_grad_q": info[4], "composite_cg_iterations": info[6]}
 
 
 def nphase_rings_amr(args):
@@ -306,6 +345,9 @@ def nphase_rings_amr(args):
     for lev in range(len(levels) - 2, -1, -1):
         levels[lev + 1].AverageDownTo(Us[lev + 1], levels[lev], Us[lev], 2)
     amr = Amr(levels, 2)
+    pf = _amr_pf(args)
+    if pf is not None:
+        amr.pf_setup(pf)
     S1 = [torch.zeros_like(u) for u in Us]
     S2 = [torch.zeros_like(u) for u in Us]
     dt = torch.zeros(1, dtype=torch.float64, device=dev)
@@ -320,6 +362,7 @@ def nphase_rings_amr(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     assert amr.check_errors() == 0
+    pf_rec = _amr_pf_split(amr, Us, S1, S2, dt, pf)
     amr.close()
     for lv in levels:
         lv.close()
@@ -331,6 +374,8 @@ def nphase_rings_amr(args):
     Lu = Level(P, cu["geom"], lu)
     Uu = torch.from_numpy(lu.pack(cu["U"])).to(dev)
     del cu
+    if pf is not None:
+        Lu.pf_setup(pf)
     Vu1, Vu2 = torch.zeros_like(Uu), torch.zeros_like(Uu)
     du = torch.zeros(1, dtype=torch.float64, device=dev)
     cmu = torch.zeros(1, dtype=torch.float64, device=dev)
@@ -362,7 +407,8 @@ def nphase_rings_amr(args):
         "config": {"workload": "nphase-rings-amr", "base_grid": [512, 512], "levels": len(levels), "ratio": 2,
                    "boxes_per_level": [la.nbox for la in lays], "cells_per_level": cells,
                    "tagged_cells": tagged, "nphase": 3, "scheme": "WENO5", "riemann": "HLLC",
-                   "integrator": "TVD-RK3 (levels stage-coupled)", "cfl": 0.4, "phase_field": None},
+                   "integrator": "TVD-RK3 (levels stage-coupled)", "cfl": 0.4,
+                   "phase_field": _amr_pf_config(pf)},
         "uniform_finest_grid": {"grid": [n, n], "ms_per_step": wall_u / Ku * 1e3,
                                 "cell_updates_per_s": n * n * Ku / wall_u},
         "amr_step_over_uniform_step": (wall / K) / (wall_u / Ku),
@@ -370,6 +416,8 @@ def nphase_rings_amr(args):
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": None, "flops_per_cell_stage": fps},
     }
+    if pf_rec is not None:
+        rec["phase_field"] = pf_rec
     print(json.dumps(rec), flush=True)
 
 
@@ -519,6 +567,8 @@ def main():
                     help="served-stream: arrival rates (cases/s) to run, comma separated")
     ap.add_argument("--cases", type=int, default=48, help="served-stream: cases per rate")
     ap.add_argument("--case-steps", type=int, default=200, help="served-stream: RK3 steps per case")
+    ap.add_argument("--no-pf", action="store_true",
+                    help="shock-bubble-amr, nphase-rings-amr: the flow step alone, without the CAC Phase-Field step")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure the dominant kernel's HBM traffic")
     args = ap.parse_args()

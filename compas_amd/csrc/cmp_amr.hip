@@ -68,6 +68,7 @@ struct AmrPf {
     CompLink* d_links = nullptr;
     int ncells = 0, nlinks = 0;
     double vol_total = 0.0;
+    long long ncomp = 0;                  // unknowns of the composite solve (cells no finer level covers)
     double* d_lsum = nullptr;             // per-level sums, 2 per level
     unsigned long long* d_bits = nullptr; // [0] max|u|, [1 + l] max|Grad_Q| of level l
     double* h_pin = nullptr;              // pinned host copies
@@ -591,6 +592,7 @@ int amr_comp_build(cmp_amr* A)
         return a;
     };
     double vol_total = 0.0;
+    long long ncomp = 0;
     P.d_mask.assign(nlev, nullptr);
     for (int l = 0; l < nlev; ++l) {
         const cmp_level* L = A->L[l];
@@ -612,6 +614,7 @@ int amr_comp_build(cmp_amr* A)
                             continue;
                         }
                         vol_total += vol;
+                        ++ncomp;
                         std::vector<CompLink> lk;
                         bool special = false;
                         for (int d = 0; d < dim; ++d)
@@ -685,6 +688,7 @@ int amr_comp_build(cmp_amr* A)
     if (!links.empty())
         CMP_TRY(hipMemcpy(P.d_links, links.data(), sizeof(CompLink) * links.size(), hipMemcpyHostToDevice));
     P.vol_total = vol_total;
+    P.ncomp = ncomp;
     return CMP_OK;
 }
 
@@ -815,10 +819,16 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
     double* sc = L0->d_sc;
     const CompArgs caP = comp_args(A, false), caX = comp_args(A, true);
     const bool mg = !L0->mg.empty();
+    // with the level-0 V-cycle the preconditioned residual is projected off the constant vector over
+    // the composite cells (the operator's null space)
+    const double inv_n = (mg && P.ncomp > 0) ? 1.0 / (double)P.ncomp : 0.0;
     int r;
-    auto sum_levels = [&](int nv, double* dst, const double* scx) -> int {
-        for (int l = 0; l < nlev; ++l)
-            CMP_TRY(launch_reduce_sums(A->L[l]->d_part, A->L[l]->nblocks, nv, P.d_lsum + nv * l, s));
+    // the level sums of nv partial slots from slot s0 on
+    auto sum_levels = [&](int nv, double* dst, const double* scx, int s0 = 0) -> int {
+        for (int l = 0; l < nlev; ++l) {
+            const cmp_level* L = A->L[l];
+            CMP_TRY(launch_reduce_sums(L->d_part + (long long)s0 * L->nblocks, L->nblocks, nv, P.d_lsum + nv * l, s));
+        }
         AmrPfArgs a{};
         a.src = P.d_lsum;
         a.nlev = nlev;
@@ -828,8 +838,8 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
         return ap_launch(dim, AP_SUM_LEVELS, a, s);
     };
     // z = R ID (levels above 0, Jacobi) + the level-0 V-cycle of the restricted R/V at every cell's
-    // level-0 ancestor; r.z partials into each level's second partial slot (P = Z when init)
-    auto precondition = [&](const double* scx, bool init) -> int {
+    // level-0 ancestor; partials of r.z, sum z and sum r into each level's second to fourth slots
+    auto precondition = [&](const double* scx) -> int {
         for (int l = 0; l < nlev; ++l) {
             AmrPfArgs a = level_args(A->L[l], l, &caP);
             a.sc = scx;
@@ -868,7 +878,6 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
             m.sc = scx;
             m.src = L->d_R;
             m.r = L->d_Z;
-            m.dst = init ? L->d_Pv : nullptr;
             m.part = L->d_part;
             CMP_TRY(dim == 2 ? launch_mg_dim<2>(MG_CG_RZ, m, s) : launch_mg_dim<3>(MG_CG_RZ, m, s));
         }
@@ -894,9 +903,14 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
         if ((r = ap_launch(dim, AP_COMP_INIT, a, s))) return r;
     }
     if (mg && (r = mg_prepare(L0, s))) return r;
-    if ((r = precondition(nullptr, true))) return r;
+    if ((r = precondition(nullptr))) return r;
     if ((r = sum_levels(2, sc + CG_RR, nullptr))) return r;
-    CMP_TRY(launch_cg_start(sc, s));
+    if ((r = sum_levels(2, sc + CG_ZS, nullptr, 2))) return r;
+    CMP_TRY(launch_cg_start(sc, s, inv_n));
+    for (int l = 0; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s, inv_n, P.d_mask[l], 1));
+    }
     const int maxit = 100000;
     const int chunk = 8;
     for (long long queued = 0;; queued += chunk) {
@@ -924,13 +938,14 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
                 CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, nullptr, L->d_X, L->d_R, L->d_Z, L->d_Pv, L->d_AP,
                                      L->d_part, s));
             }
-            if ((r = precondition(sc, false))) return r;
+            if ((r = precondition(sc))) return r;
             if ((r = sum_levels(2, sc + CG_RRNEW, sc))) return r;
+            if (mg && (r = sum_levels(2, sc + CG_ZS, sc, 2))) return r;
             for (int l = 0; l < nlev; ++l) {
                 cmp_level* L = A->L[l];
-                CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s));
+                CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s, inv_n, P.d_mask[l]));
             }
-            CMP_TRY(launch_cg_end(sc, maxit, s));
+            CMP_TRY(launch_cg_end(sc, maxit, s, inv_n));
         }
         CMP_TRY(hipMemcpyAsync(P.h_pin + 0, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
         CMP_TRY(hipMemcpyAsync(P.h_pin + 1, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, s));

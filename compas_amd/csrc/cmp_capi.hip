@@ -175,7 +175,8 @@ template <int DIM>
 hipError_t launch_cg_xr_dim(const PfGeom&, long long, const double*, const double*, double*, double*, double*,
                             const double*, const double*, double*, hipStream_t);
 template <int DIM>
-hipError_t launch_cg_p_dim(const PfGeom&, long long, const double*, const double*, double*, hipStream_t);
+hipError_t launch_cg_p_dim(const PfGeom&, long long, const double*, const double*, double*, double, const int*, int,
+                           hipStream_t);
 template <int DIM>
 hipError_t launch_pf_gradx_dim(const PfGeom&, const long long*, const double*, double*, unsigned long long*,
                                int*, hipStream_t);
@@ -238,9 +239,9 @@ hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const doub
 }
 
 hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Z,
-                       double* P, hipStream_t s)
+                       double* P, hipStream_t s, double inv_n, const int* mask, int first)
 {
-    return CMP_BY_DIM(dim, launch_cg_p_dim, g, maxcells, sc, Z, P, s);
+    return CMP_BY_DIM(dim, launch_cg_p_dim, g, maxcells, sc, Z, P, inv_n, mask, first, s);
 }
 
 hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X, double* gq,
@@ -1166,8 +1167,8 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     L->nblocks = (int)nblk;
     CMP_TRYP(hipMalloc(&L->d_sc, sizeof(double) * CG_NSC));
     CMP_TRYP(hipMemset(L->d_sc, 0, sizeof(double) * CG_NSC));
-    CMP_TRYP(hipMalloc(&L->d_part, sizeof(double) * 2 * nblk));
-    CMP_TRYP(hipMemset(L->d_part, 0, sizeof(double) * 2 * nblk));
+    CMP_TRYP(hipMalloc(&L->d_part, sizeof(double) * 4 * nblk));
+    CMP_TRYP(hipMemset(L->d_part, 0, sizeof(double) * 4 * nblk));
     if (pf->mechanism == CMP_PF_CAC) {
         CMP_TRYP(hipMalloc(&L->d_Mk, sizeof(double) * na));
         CMP_TRYP(hipMalloc(&L->d_Wk, sizeof(double) * na));
@@ -1348,9 +1349,9 @@ int cmp::mg_vcycle(cmp_level* L, const double* sc, size_t m, hipStream_t s)
     return CMP_OK;
 }
 
-// z = V(r) for the CG's residual (level buffers R -> Z), then r.z block partials into the second
-// slot of d_part; with P non-null also p = z (the first iteration).
-static int mg_precondition(cmp_level* L, const double* sc, double* P, hipStream_t s)
+// z = V(r) for the CG's residual (level buffers R -> Z), then the block partials of r.z, sum z and
+// sum r into the second to fourth slots of d_part.
+static int mg_precondition(cmp_level* L, const double* sc, hipStream_t s)
 {
     const int dim = L->P.dim;
     cmp::MgArgs a = mg_args(L, sc);
@@ -1365,7 +1366,6 @@ static int mg_precondition(cmp_level* L, const double* sc, double* P, hipStream_
     if ((r = mg_launch(dim, cmp::MG_SCATTER, a, s))) return r;
     a.src = L->d_R;
     a.r = L->d_Z;
-    a.dst = P;
     a.part = L->d_part;
     return mg_launch(dim, cmp::MG_CG_RZ, a, s);
 }
@@ -1379,7 +1379,8 @@ extern "C" {
 // residual 1e-10); here conjugate gradients on the zero-mean subspace, preconditioned by a multigrid
 // V-cycle when the level covers its domain (else by the Jacobi diagonal), run to the rounding floor
 // (r.r <= 1e-28 of the initial r.r, or 400 iterations without a new minimum), with fixed-order
-// reductions and the stopping test on the device.
+// reductions and the stopping test on the device. With the V-cycle the preconditioned residual is
+// projected off the constant vector (the operator's null space) before it enters the direction.
 static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq, hipStream_t s)
 {
     const int dim = L->P.dim;
@@ -1394,14 +1395,22 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
     CMP_TRY(launch_cg_init(dim, L->pg, L->maxcells, L->d_DJ, sc + CG_SUM, (double)L->ncell, L->d_ID, L->d_X,
                            L->d_R, L->d_Z, L->d_Pv, L->d_part, s));
     const bool mg = !L->mg.empty();
+    const double inv_n = mg ? 1.0 / (double)L->ncell : 0.0;
     if (mg) {
-        // the V-cycle's face coefficients and diagonals for this b, then z = V(r), p = z
+        // the V-cycle's face coefficients and diagonals for this b, then z = V(r)
         int r = mg_prepare(L, s);
-        if (!r) r = mg_precondition(L, nullptr, L->d_Pv, s);
+        if (!r) r = mg_precondition(L, nullptr, s);
         if (r) return r;
     }
     CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RR, s));
-    CMP_TRY(launch_cg_start(sc, s));
+    if (mg) {
+        // p = z - mean(z)
+        CMP_TRY(launch_reduce_sums(L->d_part + 2 * (long long)L->nblocks, L->nblocks, 2, sc + CG_ZS, s));
+        CMP_TRY(launch_cg_start(sc, s, inv_n));
+        CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s, inv_n, nullptr, 1));
+    } else {
+        CMP_TRY(launch_cg_start(sc, s));
+    }
     // the host looks at the stop flag every `chunk` iterations; iterations queued past the stop
     // return at once
     const int chunk = mg ? 8 : 16;
@@ -1413,12 +1422,14 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
             CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, mg ? nullptr : L->d_ID, L->d_X, L->d_R, L->d_Z,
                                  L->d_Pv, L->d_AP, L->d_part, s));
             if (mg) {
-                int r = mg_precondition(L, sc, nullptr, s);
+                int r = mg_precondition(L, sc, s);
                 if (r) return r;
             }
             CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RRNEW, s));
-            CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s));
-            CMP_TRY(launch_cg_end(sc, maxit, s));
+            if (mg)
+                CMP_TRY(launch_reduce_sums(L->d_part + 2 * (long long)L->nblocks, L->nblocks, 2, sc + CG_ZS, s));
+            CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s, inv_n));
+            CMP_TRY(launch_cg_end(sc, maxit, s, inv_n));
         }
         CMP_TRY(hipMemcpyAsync(L->h_pin + 2, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
         CMP_TRY(hipMemcpyAsync(L->h_pin + 3, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, s));

@@ -362,8 +362,16 @@ enum {
     CG_STALL = 8,   // iterations since the last new minimum
     CG_IT = 9,      // iterations done
     CG_SUM = 10,    // two slots for level sums (CAC's sum mech, sum W; the mean of the rhs)
+    CG_ZS = 12,     // sum of z over the unknowns (the preconditioned residual's constant part)
+    CG_RS = 13,     // sum of r over the unknowns; the pair is reduced together
     CG_NSC = 16
 };
+// r.z of the residual against the projected z - mean(z) when inv_n = 1/(number of unknowns) > 0:
+// r.z - (sum z)(sum r)/n; the plain r.z when inv_n is 0
+__host__ __device__ inline double cg_rz_projected(const double* sc, int slot, double inv_n)
+{
+    return (inv_n > 0.0) ? sc[slot] - sc[CG_ZS] * inv_n * sc[CG_RS] : sc[slot];
+}
 // ap = -div(b grad p) (p ghosts filled; Neumann domain faces carry no flux), partial of p.ap
 hipError_t launch_cg_ap(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Bf,
                         const double* P, double* AP, double* part, hipStream_t s);
@@ -371,11 +379,13 @@ hipError_t launch_cg_ap(int dim, const PfGeom& g, long long maxcells, const doub
 hipError_t launch_cg_xr(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* ID,
                         double* X, double* R, double* Z, const double* P, const double* AP, double* part,
                         hipStream_t s);
-// be = rz_new/rz; p = z + be p
+// be = rz_new/rz; p = z' + be p, z' = z - sum(z) inv_n on every cell not masked as covered (z' = z
+// with inv_n 0; mask null: no covered cells); first: p = z' (the first direction, after cg_start)
 hipError_t launch_cg_p(int dim, const PfGeom& g, long long maxcells, const double* sc, const double* Z,
-                       double* P, hipStream_t s);
-hipError_t launch_cg_start(double* sc, hipStream_t s);
-hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s);
+                       double* P, hipStream_t s, double inv_n = 0.0, const int* mask = nullptr, int first = 0);
+// the initial r.r and r.z (projected with inv_n > 0) start the stopping test
+hipError_t launch_cg_start(double* sc, hipStream_t s, double inv_n = 0.0);
+hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s, double inv_n = 0.0);
 // face gradient of x (ghosts filled), zero on Neumann domain faces, with max |.|
 hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X,
                            double* gq, unsigned long long* gmax_bits, int* boxflag, hipStream_t s);

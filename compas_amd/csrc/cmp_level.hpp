@@ -92,7 +92,7 @@ struct cmp_level {
     double* d_Z = nullptr;          // preconditioned residual
     double* d_ID = nullptr;         // inverse diagonal of the operator (Jacobi preconditioner)
     double* d_Bf = nullptr;         // face coefficients b (face layout)
-    double* d_part = nullptr;       // block partial sums, 2 per block
+    double* d_part = nullptr;       // block partial sums, 4 per block
     double* d_sc = nullptr;         // CG_NSC device scalars
     int nblocks = 0;                // blocks of a cell pass over the level
     long long ncell = 0;            // valid cells of the level

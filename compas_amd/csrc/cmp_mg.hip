@@ -257,34 +257,41 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const double* __restrict__ s
     if (on) x[q] = xs[cur][q];
 }
 
-// r.z partial sums of one CG iteration (the second partial slot), and P = Z when P is given
+// the partial sums of one CG iteration's r.z, sum z and sum r (the second, third and fourth partial
+// slots; the first holds r.r)
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_cg_rz(PfGeom g, const double* __restrict__ sc, const double* __restrict__ R,
-                                              const double* __restrict__ Z, double* P, double* part)
+                                              const double* __restrict__ Z, double* part)
 {
     if (sc && sc[CG_DONE] != 0.0) return;
     const BoxDev ab = g.abox[blockIdx.y];
     const int n0 = ab.hi[0] - ab.lo[0] + 1, n1 = ab.hi[1] - ab.lo[1] + 1;
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 : 1;
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
-    double w = 0.0;
+    double w[3] = {0.0, 0.0, 0.0};
     if (q < (long long)n0 * n1 * n2) {
         const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
         const int k = ab.lo[2] + (int)(q / ((long long)n0 * n1));
         const long long c = ab.off + fab_index(ab, i, j, k);
-        w = R[c] * Z[c];
-        if (P) P[c] = Z[c];
+        const double r = R[c], z = Z[c];
+        w[0] = r * z;
+        w[1] = z;
+        w[2] = r;
     }
-    // block sum into the second slot (the first holds r.r)
-    __shared__ double red[NT / 64];
-    w = mg_wave_sum(w);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = red[0];
+    __shared__ double red[3][NT / 64];
 #pragma unroll
-        for (int v = 1; v < NT / 64; ++v) s += red[v];
-        part[(long long)gridDim.x * gridDim.y + (long long)blockIdx.y * gridDim.x + blockIdx.x] = s;
+    for (int m = 0; m < 3; ++m) {
+        const double t = mg_wave_sum(w[m]);
+        if ((threadIdx.x & 63) == 0) red[m][threadIdx.x >> 6] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int m = threadIdx.x;
+        double s = red[m][0];
+#pragma unroll
+        for (int v = 1; v < NT / 64; ++v) s += red[m][v];
+        const long long nb = (long long)gridDim.x * gridDim.y;
+        part[(m + 1) * nb + (long long)blockIdx.y * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -354,7 +361,7 @@ hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s)
         break;
     case MG_CG_RZ:
         hipLaunchKernelGGL((k_cg_rz<DIM>), box_grid(a.maxcells, a.g.nbox), dim3(NT), 0, s, a.g, a.sc, a.src, a.r,
-                           a.dst, a.part);
+                           a.part);
         break;
     default:
         return hipErrorInvalidValue;

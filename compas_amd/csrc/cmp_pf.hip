@@ -489,17 +489,29 @@ __global__ __launch_bounds__(NT) void k_cg_xr(PfGeom g, const double* __restrict
     block_sum_to(w, part, (long long)gridDim.x * gridDim.y + blk);
 }
 
+// With inv_n > 0 the search directions stay orthogonal to the constant vector, the null space of a
+// Neumann / periodic operator: a multigrid preconditioner's coarsest grids leave the constant part of
+// a residual's rounding unreduced and return it amplified, and once the residual is at the rounding
+// floor that part would dominate p, p.Ap would fall to rounding and the step alpha = r.z / p.Ap
+// would throw the iterate off.
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_cg_p(PfGeom g, const double* __restrict__ sc,
-                                             const double* __restrict__ Z, double* P)
+                                             const double* __restrict__ Z, double* P, double inv_n,
+                                             const int* __restrict__ mask, int first)
 {
-    if (sc[CG_DONE] != 0.0 || !(sc[CG_PAP] > 0.0)) return;
+    if (sc[CG_DONE] != 0.0 || (!first && !(sc[CG_PAP] > 0.0))) return;
     const BoxDev ab = g.abox[blockIdx.y];
     int i, j, k;
     if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
     const long long c = ab.off + fab_index(ab, i, j, k);
-    const double be = sc[CG_RZNEW] / sc[CG_RZ];
-    P[c] = Z[c] + be * P[c];
+    const bool covered = mask && mask[c] == COMP_COVERED;
+    const double zc = (inv_n > 0.0 && !covered) ? Z[c] - sc[CG_ZS] * inv_n : Z[c];
+    if (first) {
+        P[c] = zc;
+    } else {
+        const double be = cg_rz_projected(sc, CG_RZNEW, inv_n) / sc[CG_RZ];
+        P[c] = zc + be * P[c];
+    }
 }
 
 // Grad_Q = (x_c - x_m)/dx on every face of direction D; zero on Neumann domain faces
@@ -611,9 +623,10 @@ hipError_t launch_cg_xr_dim(const PfGeom& g, long long maxcells, const double* s
 
 template <int DIM>
 hipError_t launch_cg_p_dim(const PfGeom& g, long long maxcells, const double* sc, const double* Z, double* P,
-                           hipStream_t s)
+                           double inv_n, const int* mask, int first, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_p<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, Z, P);
+    hipLaunchKernelGGL((k_cg_p<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, Z, P, inv_n, mask,
+                       first);
     return hipGetLastError();
 }
 
@@ -771,9 +784,10 @@ __global__ __launch_bounds__(NT) void k_reduce_sums(const double* __restrict__ p
 // residual landed in sc[CG_RR]; end after each iteration (pAp <= 0 stops without counting it;
 // r.r <= 1e-28 b.b, 400 iterations without a new minimum of r.r (CG's residual norm is not monotone
 // and can plateau for tens of iterations before it falls again) or maxit stop after it).
-__global__ void k_cg_start(double* sc)
+__global__ void k_cg_start(double* sc, double inv_n)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    sc[CG_RZ] = cg_rz_projected(sc, CG_RZ, inv_n);
     const double rr = sc[CG_RR];
     sc[CG_B2] = rr;
     sc[CG_BEST] = rr;
@@ -782,7 +796,7 @@ __global__ void k_cg_start(double* sc)
     sc[CG_DONE] = (rr > 0.0 && sc[CG_RZ] > 0.0) ? 0.0 : 1.0;   // a zero (or NaN) right-hand side: x = 0
 }
 
-__global__ void k_cg_end(double* sc, double maxit)
+__global__ void k_cg_end(double* sc, double maxit, double inv_n)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     if (sc[CG_DONE] != 0.0) return;
@@ -792,7 +806,7 @@ __global__ void k_cg_end(double* sc, double maxit)
     }
     const double rr = sc[CG_RRNEW];
     sc[CG_RR] = rr;
-    sc[CG_RZ] = sc[CG_RZNEW];
+    sc[CG_RZ] = cg_rz_projected(sc, CG_RZNEW, inv_n);
     const double it = sc[CG_IT] + 1.0;
     sc[CG_IT] = it;
     bool done = false;
@@ -814,15 +828,15 @@ hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double
     return hipGetLastError();
 }
 
-hipError_t launch_cg_start(double* sc, hipStream_t s)
+hipError_t launch_cg_start(double* sc, hipStream_t s, double inv_n)
 {
-    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s, sc);
+    hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(64), 0, s, sc, inv_n);
     return hipGetLastError();
 }
 
-hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s)
+hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s, double inv_n)
 {
-    hipLaunchKernelGGL(k_cg_end, dim3(1), dim3(64), 0, s, sc, (double)maxit);
+    hipLaunchKernelGGL(k_cg_end, dim3(1), dim3(64), 0, s, sc, (double)maxit, inv_n);
     return hipGetLastError();
 }
 #endif
@@ -865,7 +879,7 @@ template hipError_t launch_cg_xr_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, c
                                                      double*, double*, double*, const double*, const double*,
                                                      double*, hipStream_t);
 template hipError_t launch_cg_p_dim<CMP_KERNEL_DIM>(const PfGeom&, long long, const double*, const double*,
-                                                    double*, hipStream_t);
+                                                   double*, double, const int*, int, hipStream_t);
 template hipError_t launch_pf_gradx_dim<CMP_KERNEL_DIM>(const PfGeom&, const long long*, const double*, double*,
                                                         unsigned long long*, int*, hipStream_t);
 

@@ -389,3 +389,73 @@ def test_pf_implicit_multigrid_iterations_stay_flat(gpu, dim, sizes):
         lev.close()
     assert max(its) <= 45, its
     assert max(its) <= 2.5 * min(its), its
+
+
+def _faces_of_component(flay, buf, shape, k):
+    """The direction-0 and direction-1 faces of component k of a 2D face buffer as global arrays."""
+    out = []
+    for d in range(2):
+        fs = list(shape)
+        fs[1 - d] += 1
+        g = np.full(fs, np.nan)
+        for b, (lo, hi) in enumerate(flay.boxes):
+            sl = tuple(slice(lo[e], hi[e] + 1 + (e == d)) for e in reversed(range(2)))
+            g[sl] = flay.face_view(buf, b, d)[k]
+        out.append(g)
+    return out
+
+
+def test_pf_implicit_multigrid_three_phase_rings_1024(gpu):
+    """The multigrid-preconditioned CG of the implicit Grad_Q on the three-phase rings at 1024^2 with
+    Neumann sides, in a process that has created and freed another level first. Each phase's Grad_Q
+    must satisfy div(b Grad_Q) = Div_Jhat - mean with Div_Jhat from the oracle's mechanism and b
+    evaluated independently (the residual of a converged solve is at the rounding floor; a wrong one
+    is O(1)). Before the preconditioned residual was projected off the constant vector, two of the
+    three phases ended 400 iterations past a blow-up with max |Grad_Q| of 12 and 64 against 1.84."""
+    import math
+    import torch
+    small = cases.nphase_rings_2d(512, boxes=cases.split_domain((512, 512), 64))
+    pf = make_pf_params(mechanism="CAC-Adv", explicit_rc=0, cfl=0.3, eta_multiplier=0.5)
+    lay = Layout(2, small["boxes"], 3, small["params"].nstate)
+    lev = Level(small["params"], small["geom"], lay)
+    lev.pf_setup(pf)
+    gq = torch.zeros(FaceLayout(2, small["boxes"], 3).total, dtype=torch.float64, device=gpu)
+    lev.PhaseField_GradQPF(_dev(torch, lay.pack(small["U"]), gpu), 2.262e-4, gq)
+    assert lev.check_errors() == 0
+    lev.close()
+    del small, gq
+
+    n = 1024
+    c = cases.nphase_rings_2d(n, boxes=cases.split_domain((n, n), 64))
+    P, G = c["params"], c["geom"]
+    lay = Layout(2, c["boxes"], 3, P.nstate)
+    flay = FaceLayout(2, c["boxes"], 3)
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    dt = 1.131e-4
+    gq = torch.zeros(flay.total, dtype=torch.float64, device=gpu)
+    gm, info = lev.PhaseField_GradQPF(_dev(torch, lay.pack(c["U"]), gpu), dt, gq)
+    assert lev.check_errors() == 0
+    its = lev.solver_iterations()
+    lev.close()
+    got = gq.cpu().numpy()
+    print("rings 1024^2: max |Grad_Q| %.4f, last phase %d CG iterations" % (gm, its), flush=True)
+    assert its <= 30
+    mob, eta = info[1], info[2]
+    alay = Layout(2, c["boxes"], 1, 1)
+    for k in range(3):
+        a = alay.pack(c["U"][6 + k:7 + k])
+        oracle.level_fill_pf(G, alay, a)
+        dj, _ = oracle.level_pf_mechanism(P, pf, G, alay, a, dt, mob, eta, 1.0, 0, implicit=True)
+        rhs = alay.unpack(dj, (n, n))[0]
+        gx, gy = _faces_of_component(flay, got, (n, n), k)
+        A = np.pad(np.clip(c["U"][6 + k], 1e-12, 1 - 1e-12), 1, mode="edge")
+        psi = eta / math.sqrt(2.0) * np.log(A / (1 - A))
+        S = 0.5 + 0.5 * np.tanh(psi / (math.sqrt(2.0) * pf.eta_factor * eta))
+        fx = 0.5 * S[1:-1, :-1] + 0.5 * S[1:-1, 1:]
+        fy = 0.5 * S[:-1, 1:-1] + 0.5 * S[1:, 1:-1]
+        bx, by = fx * (1 - fx) + 1e-3, fy * (1 - fy) + 1e-3
+        div = (bx[:, 1:] * gx[:, 1:] - bx[:, :-1] * gx[:, :-1]) * n + (by[1:, :] * gy[1:, :] - by[:-1, :] * gy[:-1, :]) * n
+        target = rhs - rhs.mean()
+        rel = np.max(np.abs(div - target)) / np.max(np.abs(target))
+        assert rel < 1e-8, (k, rel, np.max(np.abs(gx)))
