@@ -428,6 +428,31 @@ def rhs_box(params, geom, box, ng, U, dUdt, cmax=None, fluxes=(None, None, None)
           "cmp_rhs_box")
 
 
+def cmax_box(params, geom, box, ng, U, cmax_cells, stream=None):
+    """Per-cell c_max of one box through cmp_cmax_box (torch CUDA tensors)."""
+    lib = capi.load()
+    b = cmp_box()
+    lo, hi = box
+    for d in range(3):
+        b.lo[d] = lo[d] if d < len(lo) else 0
+        b.hi[d] = hi[d] if d < len(hi) else 0
+    check(lib.cmp_cmax_box(ctypes.byref(params), ctypes.byref(geom), ctypes.byref(b), ng, _ptr(U), _ptr(cmax_cells),
+                           _stream(stream)), "cmp_cmax_box")
+
+
+def pf_rhs_box(params, pf, geom, box, ng, U, gradq, dUdt, stream=None):
+    """Per-box PhaseField_compute_dUdt_FV through cmp_pf_rhs_box (torch CUDA tensors; gradq in the
+    box's own cmp_face_layout, one component per volume fraction)."""
+    lib = capi.load()
+    b = cmp_box()
+    lo, hi = box
+    for d in range(3):
+        b.lo[d] = lo[d] if d < len(lo) else 0
+        b.hi[d] = hi[d] if d < len(hi) else 0
+    check(lib.cmp_pf_rhs_box(ctypes.byref(params), ctypes.byref(pf), ctypes.byref(geom), ctypes.byref(b), ng,
+                             _ptr(U), _ptr(gradq), _ptr(dUdt), _stream(stream)), "cmp_pf_rhs_box")
+
+
 class Amr:
     """A hierarchy of Levels (coarsest first, each the refinement of the previous by ratio) stepped
     together without subcycling (cmp_amr_*): every RK stage fills every level from the stage state,

@@ -19,6 +19,9 @@ CMP_ERR_BC = 6
 CMP_ERR_LAYOUT = 7
 CMP_ERR_NOMEM = 8
 CMP_ERR_SCRATCH = 9
+CMP_ERR_DENSITY = 10
+CMP_ERR_PRESSURE = 11
+CMP_ERR_ALPHA = 12
 CMP_NOALLOC_MAX_BOXES = 4096
 CMP_NOALLOC_MAX_TILES = 65536
 
@@ -33,6 +36,9 @@ ERROR_NAMES = {
     CMP_ERR_LAYOUT: "CMP_ERR_LAYOUT",
     CMP_ERR_NOMEM: "CMP_ERR_NOMEM",
     CMP_ERR_SCRATCH: "CMP_ERR_SCRATCH",
+    CMP_ERR_DENSITY: "CMP_ERR_DENSITY",
+    CMP_ERR_PRESSURE: "CMP_ERR_PRESSURE",
+    CMP_ERR_ALPHA: "CMP_ERR_ALPHA",
 }
 
 
@@ -124,6 +130,9 @@ SIGNATURES = {
                                         ctypes.c_int, P(ctypes.c_longlong)]),
     "cmp_rhs_box": (ctypes.c_int, [P(cmp_params), P(cmp_geometry), P(cmp_box), ctypes.c_int,
                                    dp, dp, dp, dp, dp, dp, dp, dp, dp, vp]),
+    "cmp_cmax_box": (ctypes.c_int, [P(cmp_params), P(cmp_geometry), P(cmp_box), ctypes.c_int, dp, dp, vp]),
+    "cmp_pf_rhs_box": (ctypes.c_int, [P(cmp_params), P(cmp_pf_params), P(cmp_geometry), P(cmp_box), ctypes.c_int,
+                                      dp, dp, dp, vp]),
     "cmp_level_create": (ctypes.c_int, [P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box),
                                         ctypes.c_int, P(vp)]),
     "cmp_level_destroy": (ctypes.c_int, [vp]),

@@ -23,7 +23,7 @@ hipError_t launch_stage_dim(const LaunchCfg&, const KParams&, int, int, const Ti
                             const double*, const FillDev*);
 template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg&, const KParams&, const BoxDev&, int, const double*,
-                            double*, double*, hipStream_t);
+                            double*, double*, double*, hipStream_t);
 template <int DIM>
 hipError_t launch_fill_dim(int, const FillDev&, long long, const int*, int, double*, int*, hipStream_t);
 template <int DIM>
@@ -258,10 +258,10 @@ hipError_t launch_compute_dt(int dim, const double* dx, double* cmax, double* dt
 }
 
 hipError_t launch_faces(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
-                        const double* U, double* flux, double* fluxnc, hipStream_t s)
+                        const double* U, double* flux, double* fluxnc, double* cmc, hipStream_t s)
 {
-    return lc.dim == 2 ? launch_faces_dim<2>(lc, kp, box, dir, U, flux, fluxnc, s)
-                       : launch_faces_dim<3>(lc, kp, box, dir, U, flux, fluxnc, s);
+    return lc.dim == 2 ? launch_faces_dim<2>(lc, kp, box, dir, U, flux, fluxnc, cmc, s)
+                       : launch_faces_dim<3>(lc, kp, box, dir, U, flux, fluxnc, cmc, s);
 }
 
 template <int DIM>
@@ -503,8 +503,70 @@ int cmp_rhs_box(const cmp_params* params, const cmp_geometry* geom, const cmp_bo
     for (int d = 0; d < dim; ++d) {
         if (!fx[d] && !fn[d]) continue;
         if (lc.nphase) return CMP_ERR_UNSUPPORTED;   // face arrays of the N-phase model are not built
-        CMP_TRY(launch_faces(lc, kp, bd, d, U, fx[d], fn[d], s));
+        CMP_TRY(launch_faces(lc, kp, bd, d, U, fx[d], fn[d], nullptr, s));
     }
+    return CMP_OK;
+}
+
+int cmp_cmax_box(const cmp_params* params, const cmp_geometry* geom, const cmp_box* box, int ng, const double* U,
+                 double* cmax_cells, void* stream)
+{
+    LaunchCfg lc;
+    KParams kp;
+    int r = validate(params, geom, ng, &lc, &kp);
+    if (r) return r;
+    if (!box || !U || !cmax_cells) return CMP_ERR_ARG;
+    if (lc.nphase) return CMP_ERR_UNSUPPORTED;
+    const int dim = params->dim;
+    for (int d = 0; d < dim; ++d)
+        if (box->hi[d] < box->lo[d]) return CMP_ERR_ARG;
+    cmp_box b0 = *box;
+    b0.offset = 0;
+    const BoxDev bd = make_boxdev(dim, b0, ng);
+    // direction 0 writes every cell, the later directions max-combine, in stream order
+    for (int d = 0; d < dim; ++d)
+        CMP_TRY(launch_faces(lc, kp, bd, d, U, nullptr, nullptr, cmax_cells, (hipStream_t)stream));
+    return CMP_OK;
+}
+
+int cmp_pf_rhs_box(const cmp_params* params, const cmp_pf_params* pf, const cmp_geometry* geom, const cmp_box* box,
+                   int ng, const double* U, const double* gradq, double* dUdt, void* stream)
+{
+    LaunchCfg lc;
+    KParams kp;
+    int r = validate(params, geom, ng, &lc, &kp);
+    if (r) return r;
+    if (!pf || !box || !U || !gradq || !dUdt) return CMP_ERR_ARG;
+    if (pf->id_flux != 0) return CMP_ERR_UNSUPPORTED;
+    if (pf->flux_phase2flux != 0 && pf->flux_phase2flux != 1) return CMP_ERR_ARG;
+    const int dim = params->dim;
+    for (int d = 0; d < dim; ++d)
+        if (box->hi[d] < box->lo[d]) return CMP_ERR_ARG;
+    // the Phase-Field form of the stage kernel (as cmp_level_pf_setup configures it)
+    lc.thinc = 0;
+    lc.riemann = (pf->flux_phase2flux == 1) ? RIEMANN_PF_MIX : RIEMANN_PF_PHASIC;
+    kp.thinc = 0;
+    for (int k = 0; k < CMP_MAX_PHASES; ++k) kp.pf_pmin[k] = pf->p_min[k];
+    cmp_box b0 = *box;
+    b0.offset = 0;
+    BoxDev bd = make_boxdev(dim, b0, ng);
+    long long foff[3] = {0, 0, 0};
+    face_offsets(dim, 1, &b0, params->nphase ? params->nphase : 1, foff);
+    for (int d = 0; d < dim; ++d) bd.goff[d] = foff[d];
+    // output: valid box only, comp-major
+    bd.ooff = 0;
+    long long nv[3] = {1, 1, 1};
+    for (int d = 0; d < dim; ++d) nv[d] = box->hi[d] - box->lo[d] + 1;
+    for (int m = 0; m < 3; ++m) bd.olo[m] = bd.lo[m];
+    bd.osj = nv[0];
+    bd.osk = nv[0] * nv[1];
+    bd.osn = nv[0] * nv[1] * nv[2];
+    int tx, ty, tz;
+    tile_dims(dim, &tx, &ty, &tz);
+    const int ntx = (int)((nv[0] + tx - 1) / tx), nty = (int)((nv[1] + ty - 1) / ty);
+    const int ntz = (dim == 3) ? (int)((nv[2] + tz - 1) / tz) : 1;
+    CMP_TRY(launch_stage(lc, kp, MODE_RHS, 0, nullptr, 0, nullptr, &bd, ntx * nty * ntz, U, nullptr, dUdt, nullptr,
+                         nullptr, nullptr, (hipStream_t)stream, gradq));
     return CMP_OK;
 }
 

@@ -32,6 +32,14 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }  // std::max
 __device__ __forceinline__ double dsign(double x) { return (x > 0.0) ? 1.0 : ((x < 0.0) ? -1.0 : 0.0); }
 
+// c = sqrt(c^2); c^2 <= 0 (p + p_inf <= 0 in the mixture or a phase) sets bad, where the reference's
+// sound speed would be NaN or zero.
+__device__ __forceinline__ double checked_sqrt(double c2, int& bad)
+{
+    if (c2 <= 0.0) bad = CMP_ERR_PRESSURE;
+    return sqrt(c2);
+}
+
 // NASG_Phi (include/EquationOfState_NASG.H:34-41); the covolume violation sets bad.
 __device__ __forceinline__ double nasg_phi(double alpha, double alpha_rho, double b, int& bad)
 {
@@ -65,6 +73,7 @@ __device__ __forceinline__ void cons2prim(const KParams& kp, const double* U, do
 {
     const double a1r1 = U[0], a2r2 = U[1];
     const double rho = a1r1 + a2r2;
+    if (rho <= 0.0) bad = CMP_ERR_DENSITY;
     double AP, BP, rho_D;
     mix_AB(kp, U[3 + DIM], a1r1, a2r2, AP, BP, rho_D, bad);
     const double inv = 1.0 / (rho * AP);
@@ -113,7 +122,7 @@ __device__ __forceinline__ void prim2cons(const KParams& kp, const double* W, do
 __device__ __forceinline__ double sound_speed(const KParams& kp, double p, double a1, double a1r1,
                                               double a2r2, double rho, double AP, double BP, int& bad)
 {
-    if (kp.source_term == 0 && !kp.nasg) return sqrt(((AP + 1.0) * p + BP) / (AP * rho));
+    if (kp.source_term == 0 && !kp.nasg) return checked_sqrt(((AP + 1.0) * p + BP) / (AP * rho), bad);
     const double g1 = (kp.A[0] + 1.0) * p + kp.B[0];
     const double g2 = (kp.A[1] + 1.0) * p + kp.B[1];
     const double a2 = 1.0 - a1;
@@ -122,9 +131,9 @@ __device__ __forceinline__ double sound_speed(const KParams& kp, double p, doubl
         phi1 = nasg_phi(a1, a1r1, kp.C[0], bad);
         phi2 = nasg_phi(a2, a2r2, kp.C[1], bad);
     }
-    if (kp.source_term == 0) return sqrt((a1 * g1 + a2 * g2) / (AP * rho));
+    if (kp.source_term == 0) return checked_sqrt((a1 * g1 + a2 * g2) / (AP * rho), bad);
     // Wood: rho c^2 = 1/(phi1 A1/g1 + phi2 A2/g2)
-    return sqrt(g1 * g2 / ((phi1 * kp.A[0] * g2 + phi2 * kp.A[1] * g1) * rho));
+    return checked_sqrt(g1 * g2 / ((phi1 * kp.A[0] * g2 + phi2 * kp.A[1] * g1) * rho), bad);
 }
 
 // Kapila coefficient phi1 (1 + K1) of the alpha1 equation's div(u) term
@@ -612,6 +621,7 @@ __device__ __forceinline__ void np_cons2prim(const KParams& kp, const double* U,
     double rho = U[0];
 #pragma unroll
     for (int k = 1; k < NP; ++k) rho += U[k];
+    if (rho <= 0.0) bad = CMP_ERR_DENSITY;
     double AP, BP, rho_D;
     np_mix<NP>(kp, U + I::A, U, AP, BP, rho_D, bad);
     const double inv = 1.0 / (rho * AP);
@@ -668,7 +678,7 @@ __device__ __forceinline__ double np_sound_speed(const KParams& kp, double p, co
         double s = alpha[0] * ((kp.A[0] + 1.0) * p + kp.B[0]);
 #pragma unroll
         for (int k = 1; k < NP; ++k) s += alpha[k] * ((kp.A[k] + 1.0) * p + kp.B[k]);
-        return sqrt(s / (AP * rho));
+        return checked_sqrt(s / (AP * rho), bad);
     }
     double inv = 0.0;
 #pragma unroll
@@ -676,7 +686,7 @@ __device__ __forceinline__ double np_sound_speed(const KParams& kp, double p, co
         const double phi = kp.nasg ? nasg_phi(alpha[k], arho[k], kp.C[k], bad) : alpha[k];
         inv += phi * kp.A[k] / ((kp.A[k] + 1.0) * p + kp.B[k]);
     }
-    return sqrt(1.0 / (inv * rho));
+    return checked_sqrt(1.0 / (inv * rho), bad);
 }
 
 // Kapila coefficients phi_k (1 + K_k) of the alpha_k equations (FVM_SurfaceIntegral_NC,

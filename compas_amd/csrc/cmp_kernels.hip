@@ -496,6 +496,12 @@ __device__ __forceinline__ void stage_tile(const KParams& kp, int limiter, int m
                     Ob[c + q * bx.sn] = v;
                 }
             }
+            if constexpr (SCHEME == 6 && NP == 0 && RIEMANN < RIEMANN_PF_MIX) {
+                // the bounded limiter keeps alpha1 in [0, 1]; a stage state beyond it (by more than
+                // rounding) is reported
+                const double an = Ob[c + IA * bx.sn];
+                if (an < -1.0e-12 || an > 1.0 + 1.0e-12) bad = CMP_ERR_ALPHA;
+            }
         } else if (mode == MODE_RHS) {
             const long long o = bx.ooff + (long long)(i - bx.olo[0]) + bx.osj * (j - bx.olo[1]) + bx.osk * (k - bx.olo[2]);
 #pragma unroll
@@ -607,7 +613,7 @@ void k_stage_batch(const KParams* __restrict__ kps, int limiter, int stage, cons
 template <int DIM, int SCHEME, int RIEMANN, bool THINC, int DIR, int NP = 0>
 __device__ __forceinline__ double face_at(const KParams& kp, int limiter, const BoxDev& bx,
                                           const double* __restrict__ Ub, int ci, int cj, int ck, double* Fl,
-                                          int& bad)
+                                          int& bad, double* cm_out = nullptr)
 {
     constexpr int N = (NP > 0) ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
@@ -627,7 +633,9 @@ __device__ __forceinline__ double face_at(const KParams& kp, int limiter, const 
     auto st = [&](int t, int q) -> double { return W[t][q]; };
     double cm;
     if constexpr (NP > 0) {
-        return np_face_flux<DIM, SCHEME, RIEMANN, NP>(kp, limiter, DIR, st, Fl, cm, bad);
+        const double un = np_face_flux<DIM, SCHEME, RIEMANN, NP>(kp, limiter, DIR, st, Fl, cm, bad);
+        if (cm_out) *cm_out = cm;
+        return un;
     } else {
         auto aa = [&](int od, int o1, int o2) -> double {
             const int lo = (od == -1) ? 1 : 0;
@@ -638,17 +646,21 @@ __device__ __forceinline__ double face_at(const KParams& kp, int limiter, const 
             return Ub[fab_index(bx, px, py, pz) + IA * bx.sn];
         };
         auto normal = [&]() -> double { return face_normal_component<DIM>(kp, DIR, aa); };
-        return face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, DIR, st, normal, Fl, cm, bad);
+        const double un = face_flux<DIM, SCHEME, RIEMANN, THINC>(kp, limiter, DIR, st, normal, Fl, cm, bad);
+        if (cm_out) *cm_out = cm;
+        return un;
     }
 }
 
 // Face fluxes of one box in the reference's shape (one thread per face, every stencil cell taken
 // through cons->prim by the thread that uses it), for callers that need the face arrays (flux
-// registers, face averaging) and as a second GPU path the fused kernel is checked against.
+// registers, face averaging) and as a second GPU path the fused kernel is checked against. cmc
+// (valid cells of the box): the wave speed of the face that is the low face of a cell along DIR,
+// written by the DIR 0 launch and max-combined by the later ones (the reference's per-cell c_max).
 template <int DIM, int SCHEME, int RIEMANN, bool THINC, int DIR>
 __global__ __launch_bounds__(NT)
 void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
-             double* __restrict__ flux, double* __restrict__ fluxnc)
+             double* __restrict__ flux, double* __restrict__ fluxnc, double* __restrict__ cmc)
 {
     constexpr int N = 4 + DIM;
     const int fe0 = bx.hi[0] - bx.lo[0] + 1 + (DIR == 0 ? 1 : 0);
@@ -662,13 +674,23 @@ void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
     const int z = (int)(f / ((long long)fe0 * fe1));
     int bad = 0;
     double Fl[N];
+    double cm = 0.0;
     const double un = face_at<DIM, SCHEME, RIEMANN, THINC, DIR>(kp, limiter, bx, U + bx.off, bx.lo[0] + x,
-                                                                bx.lo[1] + y, bx.lo[2] + z, Fl, bad);
+                                                                bx.lo[1] + y, bx.lo[2] + z, Fl, bad, &cm);
     if (flux) {
 #pragma unroll
         for (int q = 0; q < N; ++q) flux[f + q * F] = Fl[q];
     }
     if (fluxnc) fluxnc[f] = un;
+    if (cmc) {
+        const int n0 = bx.hi[0] - bx.lo[0] + 1, n1 = bx.hi[1] - bx.lo[1] + 1;
+        const int co = (DIR == 0) ? x : ((DIR == 1) ? y : z);
+        const int nd = (DIR == 0) ? n0 : ((DIR == 1) ? n1 : bx.hi[2] - bx.lo[2] + 1);
+        if (co < nd) {
+            const long long c = (long long)x + (long long)n0 * (y + (long long)n1 * z);
+            cmc[c] = (DIR == 0) ? cm : ((cmc[c] < cm) ? cm : cmc[c]);
+        }
+    }
 }
 
 // Ghost cell g (0 <= g < the box's ghost count) of a box with nv1 valid rows and fab extents
@@ -974,20 +996,20 @@ static hipError_t stage4(const KParams& kp, int limiter, int mode, int stage, co
 
 template <int DIM, int SCHEME, int RIEMANN, bool THINC>
 static hipError_t faces4(const KParams& kp, int limiter, const BoxDev& box, int dir, const double* U,
-                         double* flux, double* fluxnc, hipStream_t s)
+                         double* flux, double* fluxnc, double* cmc, hipStream_t s)
 {
     long long F = 1;
     for (int m = 0; m < DIM; ++m) F *= (box.hi[m] - box.lo[m] + 1 + (m == dir ? 1 : 0));
     const int grid = (int)((F + NT - 1) / NT);
     if (dir == 0)
         hipLaunchKernelGGL((k_faces<DIM, SCHEME, RIEMANN, THINC, 0>), dim3(grid), dim3(NT), 0, s, kp,
-                           limiter, box, U, flux, fluxnc);
+                           limiter, box, U, flux, fluxnc, cmc);
     else if (dir == 1)
         hipLaunchKernelGGL((k_faces<DIM, SCHEME, RIEMANN, THINC, 1>), dim3(grid), dim3(NT), 0, s, kp,
-                           limiter, box, U, flux, fluxnc);
+                           limiter, box, U, flux, fluxnc, cmc);
     else
         hipLaunchKernelGGL((k_faces<DIM, SCHEME, RIEMANN, THINC, (DIM == 3 ? 2 : 1)>), dim3(grid),
-                           dim3(NT), 0, s, kp, limiter, box, U, flux, fluxnc);
+                           dim3(NT), 0, s, kp, limiter, box, U, flux, fluxnc, cmc);
     return hipGetLastError();
 }
 
@@ -1660,10 +1682,10 @@ hipError_t launch_avgdown_dim(const BoxDev* fboxes, int nfbox, long long maxcell
 
 template <int DIM>
 hipError_t launch_faces_dim(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
-                            const double* U, double* flux, double* fluxnc, hipStream_t s)
+                            const double* U, double* flux, double* fluxnc, double* cmc, hipStream_t s)
 {
     if (lc.nphase) return hipErrorInvalidValue;   // face arrays of the N-phase model are not built
-    CMP_DISPATCH(faces4, kp, lc.limiter, box, dir, U, flux, fluxnc, s);
+    CMP_DISPATCH(faces4, kp, lc.limiter, box, dir, U, flux, fluxnc, cmc, s);
     return hipErrorInvalidValue;
 }
 
@@ -1742,7 +1764,7 @@ template hipError_t launch_stage_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KPa
                                                      const double*, unsigned long long*, int*,
                                                      hipStream_t, const double*, const FillDev*);
 template hipError_t launch_faces_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev&,
-                                                     int, const double*, double*, double*, hipStream_t);
+                                                     int, const double*, double*, double*, double*, hipStream_t);
 template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(int, const FillDev&, long long, const int*, int, double*, int*,
                                                     hipStream_t);
 template hipError_t launch_dt_dim<CMP_KERNEL_DIM>(const double*, double*, double*, double, int,

@@ -399,7 +399,7 @@ hipError_t launch_compute_dt(int dim, const double* dx, double* cmax, double* dt
                              double* clock = nullptr);
 
 hipError_t launch_faces(const LaunchCfg& lc, const KParams& kp, const BoxDev& box, int dir,
-                        const double* U, double* flux, double* fluxnc, hipStream_t s);
+                        const double* U, double* flux, double* fluxnc, double* cmc, hipStream_t s);
 
 hipError_t launch_set2(int* p, int a, int b, hipStream_t s);
 

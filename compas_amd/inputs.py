@@ -9,13 +9,14 @@ reference's defaults where a key is absent:
   Physics.RiemannSolver (source/Class_Compressible_PhaseField.cpp:91-111)
 - prob.EOS.EOS, gamma_k / pinf_k / b_k / q_k, or the per-phase lists gamma / pinf / b / q of the
   N-phase model (exec/*/prob/Parm.H; coefficients source/Parm.cpp:122-170, 230-262)
-- geometry.prob_lo / prob_hi / is_periodic, amr.n_cell / max_grid_size / max_level,
-  run.lo_bc / hi_bc / cfl / timestep_change_limiter / TimeIntegrator, max_step, stop_time
-  (source/COMPAS.cpp:1186-1546)
+- geometry.prob_lo / prob_hi / is_periodic, amr.n_cell / max_grid_size / max_level, and with
+  max_level > 0 amr.ref_ratio / regrid_int / n_error_buf / grid_eff / blocking_factor and
+  run.do_subcycle (the hierarchy runs without subcycling), run.lo_bc / hi_bc / cfl /
+  timestep_change_limiter / TimeIntegrator, max_step, stop_time (source/COMPAS.cpp:1186-1546)
 - PhaseField.* (include/Physics_PhaseField/PhaseField_Parameter.H:29-140)
 
 Settings the reference accepts but this library does not run are listed in "unsupported" (never
-silently dropped); settings it does not use (output, AMR regridding) are ignored.
+silently dropped); settings it does not use (output, plotting) are ignored.
 """
 import shlex
 
@@ -142,8 +143,19 @@ def case_parameters(t, dim, nphase=0):
                max_step=_get(t, "max_step", None, int), stop_time=_get(t, "stop_time", None),
                max_grid_size=_get(t, "amr.max_grid_size", 32 if dim == 3 else 128, int),
                max_level=_get(t, "amr.max_level", 0, int), ghost_cells=ng, n_cell=n_cell)
+    # a hierarchy (cmp_amr_*: levels stepped together without subcycling, COMPAS.cpp:2287-2300;
+    # regrid by the caller every regrid_int steps); COMPAS.H:533 defaults to subcycling
     if run["max_level"] > 0:
-        unsupported.append("amr.max_level = %d (this library advances single levels)" % run["max_level"])
+        ratios = _arr(t, "amr.ref_ratio", [2] * run["max_level"], int)[:run["max_level"]]
+        run.update(ref_ratio=ratios, regrid_int=_get(t, "amr.regrid_int", 1, int),
+                   n_error_buf=_get(t, "amr.n_error_buf", 1, int), grid_eff=_get(t, "amr.grid_eff", 0.7),
+                   blocking_factor=_get(t, "amr.blocking_factor_x", _get(t, "amr.blocking_factor", 8, int), int),
+                   do_subcycle=_get(t, "run.do_subcycle", 1, int))
+        if run["do_subcycle"]:
+            unsupported.append("run.do_subcycle = 1 (the hierarchy is stepped without subcycling, "
+                               "as with run.do_subcycle = 0)")
+        if len(set(ratios)) > 1 or ratios[0] not in (2, 4):
+            unsupported.append("amr.ref_ratio = %s (one ratio, 2 or 4, for every level)" % ratios)
     # phase field
     pf = None
     if _get(t, "PhaseField.phase_field", 0, int):

@@ -35,6 +35,11 @@ extern "C" {
 #define CMP_ERR_LAYOUT       7  /* boxes overlap, leave the domain, or do not cover it                  */
 #define CMP_ERR_NOMEM        8  /* a batch has no free case slot or no free range for a case's state    */
 #define CMP_ERR_SCRATCH      9  /* larger than the fixed device scratch of an allocation-free entry point */
+#define CMP_ERR_DENSITY     10  /* a cell's density sum_k alpha_k rho_k <= 0                              */
+#define CMP_ERR_PRESSURE    11  /* c^2 <= 0: p + p_inf <= 0 in the mixture or a phase (the reference's
+                                 * sound speed would be NaN or zero)                                     */
+#define CMP_ERR_ALPHA       12  /* alpha1 of a stage state outside [0, 1] by more than 1e-12 with the
+                                 * bounded limiter (WENO5B) on                                           */
 
 /* Scheme IDs follow FiniteVolume_Parameter::Initialize (include/Operators_FV/FiniteVolume_Parameter.H:71-139):
  *   GODUNOV  id_weno=0 id_limiter=0 id_poly=0  ng>=1
@@ -160,6 +165,28 @@ int cmp_rhs_box(const cmp_params* params, const cmp_geometry* geom, const cmp_bo
                 double* fluxnc_x, double* fluxnc_y, double* fluxnc_z,
                 void* stream);
 
+/* The per-cell wave speed of one box from its ghost-filled state, the reference's c_max MultiFab
+ * (ComputeFaceFluxes, source/Class_Compressible_PhaseField.cpp:3152-3254: c_max(i,j,k) = the largest
+ * wave speed over the faces that are the low faces of cell (i,j,k), reduced over the valid cells by
+ * ComputeDt :4490). cmax_cells: device, valid box (no ghosts), one double per cell, overwritten.
+ * Its maximum is what cmp_rhs_box max-accumulates into its cmax. Two-phase model
+ * (CMP_ERR_UNSUPPORTED for nphase > 0, whose face arrays are not built). */
+int cmp_cmax_box(const cmp_params* params, const cmp_geometry* geom, const cmp_box* box, int ng, const double* U,
+                 double* cmax_cells, void* stream);
+
+/* Phase-Field right-hand side of one box from its ghost-filled state, i.e. the per-box body of
+ * PhaseField_compute_dUdt_FV (source/Class_Compressible_PhaseField.cpp:951-1003): the faces of
+ * PhaseField_ComputeFaceFluxes (:771-852) with FVM_Conservative2FluxPhaseField_K, their -fhat and
+ * the conservative surface integral of PhaseField_FluxDivergence (:859-948); the same arithmetic as
+ * cmp_level_pf_rhs on that box.
+ *   pf      the flux form (flux_phase2flux 1 mixture, 0 phasic) and the phasic p_min; id_flux 0
+ *   U       device, fab of `box` with ng ghosts, nstate comps (ghosts already filled)
+ *   gradq   device, the box's Grad_Q faces in the layout of cmp_face_layout for this box alone
+ *           (one component per volume fraction)
+ *   dUdt    device, valid box (no ghosts), nstate comps; overwritten */
+int cmp_pf_rhs_box(const cmp_params* params, const cmp_pf_params* pf, const cmp_geometry* geom, const cmp_box* box,
+                   int ng, const double* U, const double* gradq, double* dUdt, void* stream);
+
 /* ---- one level (all boxes of a level advance together) ---------------------------------------- */
 
 typedef struct cmp_level cmp_level;
@@ -242,7 +269,8 @@ int cmp_level_get_time(const cmp_level* level, double* time, void* stream);
  * here). */
 int cmp_level_set_integrator(cmp_level* level, int integrator);
 
-/* Reads and clears the level's device error word (CMP_ERR_NAN, CMP_ERR_COVOLUME or CMP_OK).
+/* Reads and clears the level's device error word (CMP_OK, or the largest of CMP_ERR_NAN, CMP_ERR_COVOLUME,
+ * CMP_ERR_DENSITY, CMP_ERR_PRESSURE, CMP_ERR_ALPHA raised since the last read).
  * Synchronises the stream. */
 int cmp_level_check(cmp_level* level, void* stream);
 

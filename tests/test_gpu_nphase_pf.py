@@ -186,3 +186,46 @@ def test_nphase_advance_with_pf_matches_oracle(gpu):
     for q in range(P.nstate):
         assert abs(got[q].sum() - c["U"][q].sum()) <= 1e-12 * np.abs(c["U"][q]).sum()
     assert np.max(np.abs(got[6:9].sum(axis=0) - 1.0)) <= 1e-12
+
+
+@pytest.mark.parametrize("dim,nphase,scheme,form", [(2, 0, "WENO5", 1), (2, 0, "WENO5B", 0), (3, 0, "WENO5", 1),
+                                                    (2, 3, "WENO5", 0), (2, 3, "MUSCL", 1), (3, 3, "WENO5", 1)])
+def test_pf_rhs_box_equals_level_form(gpu, dim, nphase, scheme, form):
+    """cmp_pf_rhs_box (one box, its fab's ghosts filled, its own Grad_Q face block) gives every box
+    the same right-hand side bit for bit as cmp_level_pf_rhs over the level, which the oracle pins;
+    the box with a zero Grad_Q third gets exact zeros there."""
+    import torch
+    from compas_amd import pf_rhs_box
+    n = 24 if dim == 2 else 12
+    if nphase:
+        c = cases.nphase_smooth(dim, n, seed=41 + dim, nphase=nphase, scheme=scheme, max_grid_size=n // 2)
+        pf = make_pf_params(flux_phase2flux=form, p_min=(-10.0,) * nphase)
+    else:
+        c = cases.random_smooth(dim, n, seed=43 + dim, scheme=scheme, max_grid_size=n // 2)
+        pf = make_pf_params(flux_phase2flux=form, p_min=(-1.0e3, -1.0e3))
+    P, G = c["params"], c["geom"]
+    ng = scheme_ghosts(scheme)
+    lay = Layout(dim, c["boxes"], ng, P.nstate)
+    flay = FaceLayout(dim, c["boxes"], nphase or 1)
+    buf = lay.pack(c["U"])
+    oracle.level_fill_ghosts(P, G, lay, buf)
+    gq = _phase_faces(flay, c["shape"], np.random.default_rng(17))
+    lev = Level(P, G, lay)
+    lev.pf_setup(pf)
+    dU = torch.zeros(lay.total, dtype=torch.float64, device=gpu)
+    lev.PhaseField_compute_dUdt_FV(dU, _dev(torch, buf, gpu), _dev(torch, gq, gpu))
+    assert lev.check_errors() == 0
+    lev.close()
+    lvl = dU.cpu().numpy()
+    for b, (lo, hi) in enumerate(c["boxes"]):
+        fab = lay.fab_view(buf, b)
+        g0 = flay.offsets[b][0]
+        g1 = flay.offsets[b + 1][0] if b + 1 < len(c["boxes"]) else flay.total
+        out = torch.full((P.nstate * int(np.prod([hi[d] - lo[d] + 1 for d in range(dim)])),), float("nan"),
+                         dtype=torch.float64, device=gpu)
+        pf_rhs_box(P, pf, G, (lo, hi), ng, _dev(torch, fab.ravel(), gpu), _dev(torch, gq[g0:g1], gpu), out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy().reshape((P.nstate,) + tuple(hi[d] - lo[d] + 1 for d in reversed(range(dim))))
+        sl = (slice(None),) + tuple(slice(ng, ng + hi[d] - lo[d] + 1) for d in reversed(range(dim)))
+        want = lay.fab_view(lvl, b)[sl]
+        assert np.array_equal(got, want), (b, np.max(np.abs(got - want)))

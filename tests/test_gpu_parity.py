@@ -280,3 +280,84 @@ def test_rk3_steps_with_staggered_boxes_and_walls_match_oracle(gpu, dim, bc, thi
         assert np.max(np.abs(hist - hist_ref) / hist_ref) <= 1e-13
         rel = maxrel_per_var(got, ref)
         assert rel.max() <= tol, (nsteps, rel)
+
+
+@pytest.mark.parametrize("dim,scheme,source_term", [(2, "WENO5", 0), (2, "WENO5B", 1), (3, "WENO5", 0)])
+def test_cmax_box_per_cell(gpu, dim, scheme, source_term):
+    """cmp_cmax_box: on a uniform state every cell's c_max is |u| + c with the oracle's sound speed
+    (closed form); on a smooth state its maximum is cmp_rhs_box's c_max and the oracle's to 1e-14."""
+    import torch
+    from compas_amd import cmax_box, make_params
+    n = 16 if dim == 2 else 8
+    c = cases.random_smooth(dim, n, seed=11 + dim, scheme=scheme, source_term=source_term)
+    P, G = c["params"], c["geom"]
+    ng = scheme_ghosts(scheme)
+    lay = Layout(dim, c["boxes"], ng, P.nstate)
+    box = c["boxes"][0]
+    ncell = int(np.prod([box[1][d] - box[0][d] + 1 for d in range(dim)]))
+    # uniform state
+    vel = (1.0, 0.5, -0.25)[:dim]
+    W = np.empty((P.nstate,) + c["shape"])
+    W[0], W[1] = 0.4 * 1.0, 0.6 * 10.0
+    for m in range(dim):
+        W[2 + m] = vel[m]
+    W[2 + dim], W[3 + dim] = 1.0, 0.4
+    buf = lay.pack(cases.prim_to_cons(P, W))
+    _fill_cpu(c, lay, buf)
+    cc = torch.full((ncell,), float("nan"), dtype=torch.float64, device=gpu)
+    cmax_box(P, G, box, ng, _dev(torch, lay.fab_view(buf, 0).ravel(), gpu), cc)
+    torch.cuda.synchronize()
+    want = np.sqrt(sum(v * v for v in vel)) + oracle.sound_speed(P, 1.0, 0.4, 0.4, 6.0, source_term)
+    assert np.max(np.abs(cc.cpu().numpy() - want)) <= 1e-14 * want
+    # smooth state
+    buf = lay.pack(c["U"])
+    _fill_cpu(c, lay, buf)
+    fab = lay.fab_view(buf, 0).copy()
+    _, ref_cmax = oracle.rhs_box(P, G, box, ng, fab)[:2]
+    U = _dev(torch, fab.ravel(), gpu)
+    cmax_box(P, G, box, ng, U, cc)
+    dUdt = torch.zeros(ncell * P.nstate, dtype=torch.float64, device=gpu)
+    cm = torch.zeros(1, dtype=torch.float64, device=gpu)
+    rhs_box(P, G, box, ng, U, dUdt, cm)
+    torch.cuda.synchronize()
+    got = cc.cpu().numpy()
+    assert np.all(np.isfinite(got)) and np.all(got > 0.0)
+    # the per-face kernel and the fused stage kernel evaluate the reconstruction in other orders
+    assert abs(got.max() - cm.item()) <= 1e-14 * cm.item()
+    assert abs(got.max() - ref_cmax) <= 1e-14 * ref_cmax
+
+
+@pytest.mark.parametrize("what,code", [("density", 10), ("pressure", 11), ("alpha", 12), ("none", 0)])
+def test_device_error_codes(gpu, what, code):
+    """The stage kernel's error word: a cell of zero density (CMP_ERR_DENSITY), p < 0 in an ideal gas
+    (c^2 < 0, CMP_ERR_PRESSURE), alpha1 = 1.1 with WENO5B (CMP_ERR_ALPHA); the word keeps the largest
+    code and a valid state raises none. Every kernel ends normally: the conditions are arithmetic.
+    (First-order Godunov faces for the first two, so that no reconstructed face state of a neighbour
+    raises another code.)"""
+    import torch
+    from compas_amd import capi
+    assert (capi.CMP_ERR_DENSITY, capi.CMP_ERR_PRESSURE, capi.CMP_ERR_ALPHA) == (10, 11, 12)
+    n = 16
+    scheme = "WENO5B" if what in ("alpha", "none") else "GODUNOV"
+    c = cases.random_smooth(2, n, seed=3, scheme=scheme, max_grid_size=8)
+    P, G = c["params"], c["geom"]
+    W = np.empty((P.nstate, n, n))
+    W[0], W[1], W[2], W[3], W[4], W[5] = 0.5, 5.0, 1.0, 0.5, 1.0, 0.5
+    if what == "pressure":
+        W[4][4:8, 4:8] = -0.5
+    if what == "alpha":
+        W[5][4:8, 4:8] = 1.1
+    U = cases.prim_to_cons(P, W)
+    if what == "density":
+        U[0][5, 5] = U[1][5, 5] = 0.0
+    lay = Layout(2, c["boxes"], scheme_ghosts(scheme), P.nstate)
+    lev = Level(P, G, lay)
+    Un = _dev(torch, lay.pack(U), gpu)
+    U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+    dt = torch.full((1,), 1e-3, dtype=torch.float64, device=gpu)
+    cm = torch.zeros(1, dtype=torch.float64, device=gpu)
+    lev.FillPatch(Un)
+    lev.ThirdOrderSSPRK(Un, U1, U2, dt, cm)
+    err = lev.check_errors()
+    lev.close()
+    assert err == code

@@ -35,7 +35,11 @@ def test_shock_bubble_air_helium():
     assert list(G.lo_bc[:2]) == [2, 4] and list(G.hi_bc[:2]) == [2, 4]
     assert run["cfl"] == 0.3 and run["timestep_change_limiter"] == 0 and run["max_level"] == 6
     assert c["pf"] is None
-    assert any("max_level" in u for u in c["unsupported"])
+    # a six-level hierarchy is run without subcycling: only the subcycling setting is flagged
+    assert not any("max_level" in u for u in c["unsupported"])
+    assert any("do_subcycle" in u for u in c["unsupported"])
+    assert run["ref_ratio"] == [2] * 6 and run["regrid_int"] == 1 and run["n_error_buf"] == 1
+    assert run["blocking_factor"] == 8 and run["max_grid_size"] == 8
     assert not any("THINC" in u for u in c["unsupported"])     # THINC options count only with THINC = 1
 
 
