@@ -12,12 +12,13 @@ import pytest
 
 import oracle
 from compas_amd import Layout, Level, cases, rhs_box, scheme_ghosts
-from tests._util import l1rel_per_var, maxrel_per_var
+from tests._util import cellrel_per_var, l1rel_per_var, maxrel_per_var
 
 pytestmark = pytest.mark.gpu
 
 TOL_STEP = 1e-12
 TOL_L1_100 = 1e-9
+TOL_CELL = 1e-9        # per-cell relative difference after one step, cells floored at 1e-3 of scale
 
 
 def _dev(torch, a, device):
@@ -208,6 +209,11 @@ def test_water_air_3d_small_matches_oracle(gpu, thinc):
     ref = lay.unpack(buf, c["shape"])
     got, _ = _gpu_advance(c, lay, 1, use_graph=False)
     assert maxrel_per_var(got, ref).max() <= TOL_STEP
+    # every cell against its own value (cells below 1e-3 of a variable's scale against that floor:
+    # the transverse momenta start at exactly 0 and carry 1e-13 of their scale of rounding)
+    cr = cellrel_per_var(got, ref, 1e-3)
+    print("water/air thinc=%d one step: per-cell relative %s" % (thinc, np.array2string(cr, precision=2)))
+    assert cr.max() <= TOL_CELL, cr
     buf = lay.pack(c["U"])
     oracle.level_advance(c["params"], c["geom"], lay, buf, 20, c["cfl"])
     ref = lay.unpack(buf, c["shape"])
@@ -361,3 +367,57 @@ def test_device_error_codes(gpu, what, code):
     err = lev.check_errors()
     lev.close()
     assert err == code
+
+
+def test_shock_bubble_weno5b_100_steps(gpu):
+    """The shock-bubble configuration's physics (Mach 1.22 air shock on a helium-like bubble,
+    gamma 1.648 / rho 0.138, WENO5B + HLLC, CFL 0.4, outflow) at 128 x 32 on 32-cell boxes for 100
+    steps: L1 <= 1e-9 and per-cell relative <= 1e-7 (floor 1e-3 of scale) against the oracle, dt
+    history to 1e-10, and with the bounded limiter alpha1 stays in [0, 1] on both sides with the same
+    extremes (1e-10). The oracle's own trajectory moves by 1.3e-10 in L1 and 4.8e-11 in dt when its
+    initial state is perturbed by 1e-16 (rounding), which sets these bars."""
+    kw = dict(mach=1.22, rho_ratio=0.138, gamma_bubble=1.648, gamma_ambient=1.4, radius=0.25,
+              max_grid_size=32, scheme="WENO5B", xb=1.5, xs=1.0)
+    c = cases.shock_bubble_2d(128, 32, **kw)
+    lay = Layout(2, c["boxes"], 3, 6)
+    buf = lay.pack(c["U"])
+    hist_ref = oracle.level_advance(c["params"], c["geom"], lay, buf, 100, 0.4)
+    ref = lay.unpack(buf, c["shape"])
+    c["cfl"] = 0.4
+    got, hist = _gpu_advance(c, lay, 100)
+    l1 = l1rel_per_var(got, ref)
+    cr = cellrel_per_var(got, ref, 1e-3)
+    dth = np.max(np.abs(hist - hist_ref) / hist_ref)
+    print("shock-bubble WENO5B 100 steps: L1 %s per-cell %s dt history %.2e alpha [%.3e, %.15f]" % (
+        np.array2string(l1, precision=2), np.array2string(cr, precision=2), dth, got[5].min(), got[5].max()))
+    assert dth <= 1e-10
+    assert l1.max() <= TOL_L1_100, l1
+    assert cr.max() <= 1e-7, cr
+    for a in (got[5], ref[5]):
+        assert a.min() >= 0.0 and a.max() <= 1.0
+    assert abs(got[5].min() - ref[5].min()) <= 1e-10 and abs(got[5].max() - ref[5].max()) <= 1e-10
+
+
+def test_advected_interface_256_100_steps(gpu):
+    """The advected-interface configuration at its real size: 256^2 periodic, WENO5 + HLLC + RK3 at
+    CFL 0.5, 100 steps on 64-cell boxes: L1 <= 5e-9 per variable (<= 1e-9 but for alpha1 rho1, at
+    2.4e-9), dt history to 1e-12, totals conserved to 1e-13, u and p uniform to 1e-12. At this size
+    the configuration amplifies rounding: the oracle's own 100-step result moves by 3.1e-9 in L1 of
+    alpha1 rho1 when its initial state is perturbed by 1e-16 (test_oracle_level.py pins that), so a
+    1e-9 bar would hold no implementation that rounds differently from the reference."""
+    c = cases.advected_interface_2d(n=256, max_grid_size=64)
+    lay = Layout(2, c["boxes"], 3, 6)
+    buf = lay.pack(c["U"])
+    hist_ref = oracle.level_advance(c["params"], c["geom"], lay, buf, 100, c["cfl"])
+    ref = lay.unpack(buf, c["shape"])
+    got, hist = _gpu_advance(c, lay, 100)
+    l1 = l1rel_per_var(got, ref)
+    print("advected interface 256^2, 100 steps: L1 %s" % np.array2string(l1, precision=2))
+    assert l1.max() <= 5e-9, l1
+    assert l1[1:].max() <= TOL_L1_100, l1
+    assert np.max(np.abs(hist - hist_ref) / hist_ref) <= 1e-12
+    U0 = c["U"]
+    for n in range(5):
+        assert abs(got[n].sum() - U0[n].sum()) <= 1e-13 * np.abs(U0[n]).sum()
+    rho = got[0] + got[1]
+    assert np.max(np.abs(got[2] / rho - 1.0)) < 1e-12 and np.max(np.abs(got[3] / rho - 1.0)) < 1e-12
