@@ -175,6 +175,11 @@ def uneven_level(args):
                      "unit": "TFLOP/s", "frac": flops / (times["level"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
                      "traffic": None, "flops_per_cell_stage": roofline.flops_per_cell_stage(P)},
     }
+    if not args.no_cpu_baseline:
+        cs = cases.nphase_rings_2d(128)
+        cs["name"] = "the rings at 128^2 (uneven boxes)"
+        c["name"] = "the rings base level at 512^2 (644 uneven boxes)"
+        rec["cpu_baseline"] = cpu_baseline(level_sample(c, 1), level_sample(cs, 1))
     print(json.dumps(rec), flush=True)
 
 
@@ -205,6 +210,7 @@ def shock_bubble_amr(args):
     if pf is not None:
         amr.pf_setup(pf)
     Un = [U0, U1f]
+    init = [u.cpu().numpy().copy() for u in Un]   # the hierarchy's initial state, for the CPU baseline
     S1 = [torch.zeros_like(u) for u in Un]
     S2 = [torch.zeros_like(u) for u in Un]
     dt = torch.zeros(1, dtype=torch.float64, device=dev)
@@ -270,6 +276,14 @@ def shock_bubble_amr(args):
     }
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
+    if not args.no_cpu_baseline:
+        cs = cases.shock_bubble_2d(256, 64, **kw)
+        cs["name"] = "the same shock-bubble state at 256x64 (one level)"
+        rec["cpu_baseline"] = cpu_baseline(
+            amr_sample(P, [c0["geom"], c1["geom"]], [l0, l1], init, 1, 0.4, "the shock-bubble hierarchy"),
+            level_sample(cs, 1))
+        rec["cpu_baseline"]["note"] = ("flow steps only: the oracle's composite implicit Grad_Q is Jacobi-preconditioned "
+                                       "CG, not the reference's MLMG, so its time says nothing about the reference")
     print(json.dumps(rec), flush=True)
 
 
@@ -344,6 +358,7 @@ def nphase_rings_amr(args):
         del cf
     for lev in range(len(levels) - 2, -1, -1):
         levels[lev + 1].AverageDownTo(Us[lev + 1], levels[lev], Us[lev], 2)
+    init = [u.cpu().numpy().copy() for u in Us] if not args.no_cpu_baseline else None
     amr = Amr(levels, 2)
     pf = _amr_pf(args)
     if pf is not None:
@@ -418,6 +433,13 @@ def nphase_rings_amr(args):
     }
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
+    if init is not None:
+        cs = cases.nphase_rings_2d(128)
+        cs["name"] = "the rings at 128^2 (one level, uneven boxes)"
+        rec["cpu_baseline"] = cpu_baseline(amr_sample(P, geoms, lays, init, 1, 0.4, "the rings hierarchy"),
+                                           level_sample(cs, 1))
+        rec["cpu_baseline"]["note"] = ("flow steps only: the oracle's composite implicit Grad_Q is Jacobi-preconditioned "
+                                       "CG, not the reference's MLMG, so its time says nothing about the reference")
     print(json.dumps(rec), flush=True)
 
 
@@ -465,41 +487,152 @@ def served(args):
                      "flops_per_cell_stage": roofline.flops_per_cell_stage(P)},
     }
     if not args.no_cpu_baseline:
-        c = cases.served_case(1 * 100003)
-        c["pf"] = None
-        threads = min(16, os.cpu_count() or 1)
-        rec["cpu_baseline"] = cpu_baseline(c, args.cpu_seconds, threads)
+        rec["cpu_baseline"] = served_cpu_baseline(args.case_steps)
     print(json.dumps(rec), flush=True)
 
 
-def cpu_baseline(c, seconds, threads):
-    """The oracle (reference-structure CPU restatement) on the same initial state, timed on this
-    host: as many full time steps (flow RK3 step, then the Phase-Field step where the workload has
-    one) as fit in about `seconds`, at least one."""
+def served_cpu_baseline(case_steps, ncases=4, sample_steps=5):
+    """The served stream's cases on the CPU one at a time at all cores (as a CPU server would run
+    them): per case the rate of `sample_steps` steps (1 warm-up step first) and its time to
+    solution for `case_steps` steps extrapolated from it; p50 / p99 over `ncases` cases of the
+    stream's seed sequence, aggregate cell-updates/s; plus the cpu_baseline record of the first case
+    (all cores and 1 thread, median of 5)."""
+    import numpy as np
+    import oracle
+    from compas_amd import cases
+    oracle.set_num_threads(_threads())
+    tts, upd, tsum = [], 0, 0.0
+    for k in range(ncases):
+        c = cases.served_case((k + 1) * 100003)
+        smp = level_sample(c, sample_steps)
+        el, _ = _time_runs(smp["reset"], smp["run"], 1)
+        tts.append(el / sample_steps * case_steps)
+        upd += smp["updates"]
+        tsum += el
+    c0 = cases.served_case(100003)
+    c0["name"] = "served case 1 (%dx%d)" % (c0["shape"][1], c0["shape"][0])
+    rec = cpu_baseline(level_sample(c0, 5), level_sample(c0, 1), native=False)
+    rec["served_one_at_a_time"] = {
+        "cases": ncases, "steps_per_case": case_steps, "cores": _threads(),
+        "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
+        "cell_updates_per_s": upd / tsum,
+        "note": "time to solution of %d steps extrapolated from %d timed steps per case" % (case_steps, sample_steps)}
+    return rec
+
+
+def _host():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def _threads():
+    """All host cores this process may use (OMP_NUM_THREADS caps it where the host sets it)."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def _time_runs(reset, run, runs):
+    """1 warm-up run, then `runs` timed runs; reset() (untimed) before each. Returns the median time
+    and the list."""
+    import statistics
+    reset()
+    run()
+    ts = []
+    for _ in range(runs):
+        reset()
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), ts
+
+
+def level_sample(c, steps, pf=None):
+    """A CPU sample: `steps` full time steps of a single level (RK3 flow step with ComputeDt, then
+    the Phase-Field step when pf is given) from the case's initial state."""
     import numpy as np
     import oracle
     from compas_amd import Layout
-    oracle.set_num_threads(threads)
     P, G = c["params"], c["geom"]
-    lay = Layout(P.dim, c["boxes"], 3, P.nstate)
-    buf = lay.pack(c["U"])
-    cells = int(np.prod(c["shape"]))
-    cmax = oracle.level_initial_cmax(P, G, lay, buf)
-    dt = oracle.compute_dt(G, cmax, c["cfl"], 1, 0.0)
-    t0 = time.perf_counter()
-    steps = 0
-    while True:
-        oracle.level_rk3_step(P, G, lay, buf, dt)
-        if c["pf"] is not None:
-            oracle.level_pf_evolve(P, c["pf"], G, lay, buf, dt)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or steps >= 200:
-            break
-    what = "flow + Phase-Field steps" if c["pf"] is not None else "RK3 steps"
-    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": oracle.num_threads(),
-            "kind": "port",
-            "sample": "%d %s of the %s workload (%d cells) in %.1f s" % (steps, what, c["name"], cells, el)}
+    lay = Layout(P.dim, c["boxes"], c.get("ng", 3), P.nstate)
+    buf0 = lay.pack(c["U"])
+    st = {}
+
+    def reset():
+        st["b"] = buf0.copy()
+
+    def run():
+        if pf is None:
+            oracle.level_advance(P, G, lay, st["b"], steps, c["cfl"])
+        else:
+            oracle.level_advance_pf(P, pf, G, lay, st["b"], steps, c["cfl"])
+    cells = sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in c["boxes"])
+    return dict(reset=reset, run=run, updates=cells * steps,
+                what="%d %s of %s (%d cells)" % (steps, "flow + Phase-Field steps" if pf is not None else "RK3 steps",
+                                                 c.get("name", "the case"), cells))
+
+
+def amr_sample(P, geoms, lays, bufs, steps, cfl, name):
+    """A CPU sample: `steps` flow steps of a hierarchy stepped together without subcycling."""
+    import numpy as np
+    import oracle
+    st = {}
+
+    def reset():
+        st["b"] = [b.copy() for b in bufs]
+
+    def run():
+        oracle.amr_advance(P, geoms, lays, st["b"], steps, cfl, 2)
+    cells = sum(sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in la.boxes) for la in lays)
+    return dict(reset=reset, run=run, updates=cells * steps,
+                what="%d hierarchy flow steps of %s (%d levels, %d cells)" % (steps, name, len(lays), cells))
+
+
+def cpu_baseline(sample_all, sample_one, runs=5, native=True):
+    """The oracle (reference-structure CPU restatement, gcc -O3 -ffp-contract=off -fopenmp, no
+    fast-math) timed on this host: 1 warm-up run, then the median of `runs` runs of sample_all at all
+    host cores and of sample_one at 1 thread; with native, sample_all again with a -march=native
+    build of the same source compiled here. Samples come from level_sample / amr_sample."""
+    import oracle
+    import tempfile
+    from oracle import build as obuild
+    nall = _threads()
+    oracle.set_num_threads(nall)
+    med, ts = _time_runs(sample_all["reset"], sample_all["run"], runs)
+    rec = {"value": sample_all["updates"] / med, "unit": "cell-updates/s", "cores": nall, "kind": "port",
+           "sample": sample_all["what"] + ", median of %d runs after 1 warm-up" % runs,
+           "runs_s": ts, "build": "gcc " + " ".join(obuild.CFLAGS)}
+    rec.update(_host())
+    oracle.set_num_threads(1)
+    med1, ts1 = _time_runs(sample_one["reset"], sample_one["run"], runs)
+    rec["one_thread"] = {"value": sample_one["updates"] / med1, "unit": "cell-updates/s", "cores": 1,
+                         "sample": sample_one["what"] + ", median of %d runs after 1 warm-up" % runs, "runs_s": ts1}
+    oracle.set_num_threads(nall)
+    if native:
+        try:
+            with tempfile.TemporaryDirectory() as d:
+                path = obuild.build_variant(os.path.join(d, "libcpu_ref_native.so"), ["-march=native"])
+                oracle.use_library(path)
+                oracle.set_num_threads(nall)
+                medn, tsn = _time_runs(sample_all["reset"], sample_all["run"], runs)
+                rec["march_native"] = {"value": sample_all["updates"] / medn, "unit": "cell-updates/s",
+                                       "cores": nall, "runs_s": tsn,
+                                       "build": "gcc " + " ".join(obuild.CFLAGS + ["-march=native"])}
+        except Exception as e:   # a host without a compiler: the variant is reported as missing
+            rec["march_native"] = {"error": str(e)[:200]}
+        finally:
+            oracle.use_library(None)
+    return rec
 
 
 def measure_traffic(workload_name, kernel_prefix, calib_mib=512):
@@ -557,7 +690,6 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="water-air-droplet-3d")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -716,8 +848,18 @@ def main():
                 "algorithmic_bytes": 3 * P.nstate * 8 * cells,
                 "launch": "stage 2 form (reads U^(2) and U^n, writes U^(n+1); cells beyond a box read from their source boxes)"}
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        rec["cpu_baseline"] = cpu_baseline(c, args.cpu_seconds, threads)
+        from compas_amd import cases
+        if args.workload == "water-air-droplet-3d":
+            # the same case and boxes at 128^3 (all cores) and 64^3 (1 thread): the per-cell work of
+            # the flow and Phase-Field steps, bounded to seconds
+            ca = cases.water_air_droplet_3d(n=128, box=32)
+            ca["name"] = "the water-air droplet at 128^3 in 32^3 boxes"
+            c1 = cases.water_air_droplet_3d(n=64, box=32)
+            c1["name"] = "the water-air droplet at 64^3 in 32^3 boxes"
+            rec["cpu_baseline"] = cpu_baseline(level_sample(ca, 1, c["pf"]), level_sample(c1, 1, c["pf"]))
+        else:
+            ca = dict(c, name="the advected interface at 256^2")
+            rec["cpu_baseline"] = cpu_baseline(level_sample(ca, 10), level_sample(ca, 2))
     lev.close()
     print(json.dumps(rec), flush=True)
 

@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libcpu_ref.so")
 
 _lib = None
+_lib_path = None     # another build of the same source (the bench's -march=native baseline), or None
 
 
 def _dptr(a):
@@ -23,16 +24,23 @@ def _dptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def use_library(path=None):
+    """Binds another build of oracle/cpu_ref.c (path), or the default one (None), from the next call
+    on. The bench's CPU baseline times a -march=native build this way."""
+    global _lib, _lib_path
+    _lib, _lib_path = None, path
+
+
 def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    if _lib_path is None and not os.path.exists(LIB_PATH):
         from . import build as _b
         _b.build()
     # the product library's struct mirrors are shared (plain PODs)
     from compas_amd.capi import cmp_box, cmp_geometry, cmp_params, cmp_pf_params, cmp_tag_criterion
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(_lib_path or LIB_PATH)
     D = ctypes.c_double
     PD = ctypes.POINTER(D)
     PI = ctypes.POINTER(ctypes.c_int)

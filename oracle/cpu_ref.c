@@ -2792,8 +2792,10 @@ static int map_source(int dim, const cmp_geometry* G, int N, int mom0, int* src,
             src[d] = (src[d] < dlo) ? dlo : dhi;
         } else if (code >= 3 && code <= 5) {
             src[d] = (src[d] < dlo) ? (2 * dlo - 1 - src[d]) : (2 * dhi + 1 - src[d]);
-            for (int m = 0; m < dim; ++m)
-                if (m == d || code == 5) sgn[mom0 + m] = -sgn[mom0 + m];
+            /* a field without momenta (mom0 outside [0, N - dim]) has no sign to flip */
+            if (mom0 >= 0 && mom0 + dim <= N)
+                for (int m = 0; m < dim; ++m)
+                    if (m == d || code == 5) sgn[mom0 + m] = -sgn[mom0 + m];
         } else {
             return CMP_ERR_BC;
         }
@@ -3808,8 +3810,9 @@ int ref_amr_pf_gradq_step(const cmp_params* P, const cmp_pf_params* PF, int nlev
     }
     double lambda, eta, mob;
     ref_pf_coefficients(PF, &G[nlev - 1], maxv, &lambda, &eta, &mob);
-    double *A0[AMR_NLEV_MAX], *A[AMR_NLEV_MAX], *Ps[AMR_NLEV_MAX], *Nrm[AMR_NLEV_MAX], *J[AMR_NLEV_MAX];
-    double *Jh[AMR_NLEV_MAX], *Mk[AMR_NLEV_MAX], *Wk[AMR_NLEV_MAX];
+    double *A0[AMR_NLEV_MAX] = {NULL}, *A[AMR_NLEV_MAX] = {NULL}, *Ps[AMR_NLEV_MAX] = {NULL};
+    double *Nrm[AMR_NLEV_MAX] = {NULL}, *J[AMR_NLEV_MAX] = {NULL};
+    double *Jh[AMR_NLEV_MAX] = {NULL}, *Mk[AMR_NLEV_MAX] = {NULL}, *Wk[AMR_NLEV_MAX] = {NULL};
     for (int l = 0; l < nlev; ++l) {
         A0[l] = (double*)calloc((size_t)H.np[l], sizeof(double));
         A[l] = (double*)calloc((size_t)H.np[l], sizeof(double));
