@@ -296,11 +296,11 @@ def _amr_pf(args):
     return make_pf_params(mechanism="CAC-Adv", explicit_rc=0, cfl=0.3, eta_multiplier=0.5)
 
 
-def _amr_pf_config(pf):
+def _amr_pf_config(pf, grad_q="composite implicit solve over the levels (CG, Jacobi + level-0 multigrid V-cycle)"):
     if pf is None:
         return None
     return {"mechanism": "CAC-Adv", "explicit_rc": 0, "flux": "UD mixture", "cfl": 0.3, "eta_multiplier": 0.5,
-            "grad_q": "composite implicit solve over the levels (CG, Jacobi + level-0 multigrid V-cycle)"}
+            "grad_q": grad_q}
 
 
 def _amr_pf_split(amr, Un, S1, S2, dt, pf):
@@ -450,14 +450,15 @@ def served(args):
     from compas_amd import Layout, cases, roofline, serve
     rates = [float(r) for r in args.rates.split(",") if r]
     W = args.warmup
+    pf = _amr_pf(args)   # CAC-Adv, implicit RC (the served configuration's regularization)
     # warm-up: a few small cases through the same loop (kernels loaded, allocator primed)
-    serve.serve(max(rates), max(2, W), steps=5, seed=999)
+    serve.serve(max(rates), max(2, W), steps=5, seed=999, pf=pf)
     runs = []
     for r in rates:
-        res = serve.serve(r, args.cases, steps=args.case_steps, seed=1)
+        res = serve.serve(r, args.cases, steps=args.case_steps, seed=1, pf=pf)
         res.pop("finals")
         runs.append(res)
-    alone = serve.one_at_a_time(args.cases, steps=args.case_steps, seed=1)
+    alone = serve.one_at_a_time(args.cases, steps=args.case_steps, seed=1, pf=pf)
     top = runs[-1]
     P = cases.served_case(0)["params"]
     achieved = roofline.flops_per_cell_stage(P) * 3 * top["cell_updates_per_s"] / 1e12
@@ -476,7 +477,9 @@ def served(args):
         "data": "synthetic (seeded shock-bubble cases)",
         "config": {"workload": "served-stream", "cases_per_rate": args.cases, "steps_per_case": args.case_steps,
                    "grids": [list(g) for g in cases.SERVED_GRIDS], "scheme": "WENO5", "riemann": "HLLC",
-                   "integrator": "TVD-RK3", "cfl": 0.4, "max_grid_size": 64, "phase_field": None,
+                   "integrator": "TVD-RK3", "cfl": 0.4, "max_grid_size": 64, "phase_field": _amr_pf_config(
+                       pf, "implicit solve per case (CG preconditioned by the case's multigrid V-cycle); the "
+                           "cases' Phase-Field steps concurrent, each on its own stream"),
                    "value_at_rate": rates[-1]},
         "latency": [{k: v for k, v in run.items()} for run in runs],
         "one_at_a_time": alone,
@@ -700,7 +703,7 @@ def main():
     ap.add_argument("--cases", type=int, default=48, help="served-stream: cases per rate")
     ap.add_argument("--case-steps", type=int, default=200, help="served-stream: RK3 steps per case")
     ap.add_argument("--no-pf", action="store_true",
-                    help="shock-bubble-amr, nphase-rings-amr: the flow step alone, without the CAC Phase-Field step")
+                    help="shock-bubble-amr, nphase-rings-amr, served-stream: the flow step alone, without the CAC Phase-Field step")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure the dominant kernel's HBM traffic")
     args = ap.parse_args()

@@ -549,6 +549,7 @@ class Batch:
         check(self.lib.cmp_batch_create(ctypes.byref(like), int(capacity_doubles), int(max_cases),
                                         ctypes.byref(h)), "cmp_batch_create")
         self.h = h
+        self.max_cases = int(max_cases)
         self.layouts = {}
 
     def close(self):
@@ -583,6 +584,17 @@ class Batch:
 
     def step(self, stream=None):
         check(self.lib.cmp_batch_step(self.h, _stream(stream)), "cmp_batch_step")
+
+    def pf_setup(self, pf):
+        """The Phase-Field step for every admitted and every later case (cmp_batch_pf_setup)."""
+        check(self.lib.cmp_batch_pf_setup(self.h, ctypes.byref(pf)), "cmp_batch_pf_setup")
+
+    def pf_step(self, nthreads=0, stream=None):
+        """The Phase-Field step of every admitted case after step(); returns {case id: info} with
+        cmp_level_pf_evolve's six numbers per case."""
+        info = (ctypes.c_double * (6 * self.max_cases))()
+        check(self.lib.cmp_batch_pf_step(self.h, int(nthreads), info, _stream(stream)), "cmp_batch_pf_step")
+        return info
 
     def state(self, case_id, stream=None):
         """(host copy of the case's U^n level buffer, simulated time, steps taken)."""

@@ -191,6 +191,8 @@ SIGNATURES = {
                                             P(ctypes.c_int), vp]),
     "cmp_batch_case_download": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), vp]),
     "cmp_batch_check": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+    "cmp_batch_pf_setup": (ctypes.c_int, [vp, P(cmp_pf_params)]),
+    "cmp_batch_pf_step": (ctypes.c_int, [vp, ctypes.c_int, dp, vp]),
     "cmp_batch_info": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_longlong)]),
     "cmp_level_rhs_noalloc": (ctypes.c_int, [P(cmp_params), P(cmp_geometry), ctypes.c_int, P(cmp_box), ctypes.c_int,
                                              dp, dp, dp, dp, vp]),

@@ -5,8 +5,10 @@
 // parameters, geometry, boxes, dt, c_max and error word, so a case's result does not depend on what
 // else is in the batch (the stage kernel computes every tile from its own case's data only).
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "cmp_kernels.hpp"
@@ -22,6 +24,7 @@ struct BatchCase {
     long long len = 0;
     double cfl = 0.0;
     int limiter = 1;
+    hipStream_t pf_stream = nullptr;   // the stream of its Phase-Field step (with the Phase-Field step on)
 };
 
 }  // namespace
@@ -52,14 +55,21 @@ struct cmp_batch {
     std::vector<KParams> h_kps;
     std::vector<FillDev> h_fds;
     std::vector<BatchSlot> h_slots;
+    // the Phase-Field step of every case (cmp_batch_pf_setup): each case's level carries its buffers
+    bool pf_on = false;
+    cmp_pf_params pf{};
+    hipEvent_t ev_flow = nullptr;       // the flow step queued on the batch stream
 };
 
 namespace {
 
 void batch_free(cmp_batch* B)
 {
-    for (BatchCase& c : B->cases)
+    for (BatchCase& c : B->cases) {
         if (c.L) cmp_level_destroy(c.L);
+        if (c.pf_stream) (void)hipStreamDestroy(c.pf_stream);
+    }
+    if (B->ev_flow) (void)hipEventDestroy(B->ev_flow);
     void* bufs[] = {B->d_arena[0], B->d_arena[1], B->d_arena[2], B->d_kps, B->d_fds, B->d_slots, B->d_cmax,
                     B->d_dt, B->d_time, B->d_steps, B->d_err, B->d_boxes, B->d_tiles};
     for (void* p : bufs)
@@ -262,6 +272,7 @@ int cmp_batch_admit(cmp_batch* B, const cmp_params* P, const cmp_geometry* G, in
     }
     // the level's own error word holds what its initial c_max found
     int e = cmp_level_check(L, stream);
+    if (!e && B->pf_on) e = cmp_level_pf_setup(L, &B->pf);
     if (e) {
         cmp_level_destroy(L);
         arena_give(B, off, len);
@@ -354,7 +365,65 @@ int cmp_batch_check(cmp_batch* B, int case_id, void* stream)
     CMP_TRY(hipMemcpyAsync(&h, B->d_err + case_id, sizeof(int), hipMemcpyDeviceToHost, s));
     CMP_TRY(hipStreamSynchronize(s));
     if (h) CMP_TRY(hipMemsetAsync(B->d_err + case_id, 0, sizeof(int), s));
-    return h;
+    // the case's level word: its Phase-Field step's kernels
+    const int l = cmp_level_check(B->cases[case_id].L, stream);
+    return std::max(h, l);
+}
+
+int cmp_batch_pf_setup(cmp_batch* B, const cmp_pf_params* pf)
+{
+    if (!B || !pf) return CMP_ERR_ARG;
+    if (pf->mechanism == CMP_PF_CAC && pf->explicit_rc == 1) return CMP_ERR_UNSUPPORTED;
+    if (!B->ev_flow) CMP_TRY(hipEventCreateWithFlags(&B->ev_flow, hipEventDisableTiming));
+    for (BatchCase& c : B->cases) {
+        if (!c.L) continue;
+        int r = cmp_level_pf_setup(c.L, pf);
+        if (r) return r;
+    }
+    B->pf = *pf;
+    B->pf_on = true;
+    return CMP_OK;
+}
+
+int cmp_batch_pf_step(cmp_batch* B, int nthreads, double* info, void* stream)
+{
+    if (!B || !B->pf_on) return CMP_ERR_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int> slots;
+    for (int q = 0; q < B->max_cases; ++q)
+        if (B->cases[q].L) slots.push_back(q);
+    if (slots.empty()) return CMP_OK;
+    int dev = 0;
+    CMP_TRY(hipGetDevice(&dev));
+    for (int q : slots)
+        if (!B->cases[q].pf_stream) CMP_TRY(hipStreamCreateWithFlags(&B->cases[q].pf_stream, hipStreamNonBlocking));
+    // every case's Phase-Field step starts after the flow step the batch stream holds
+    CMP_TRY(hipEventRecord(B->ev_flow, s));
+    std::atomic<int> next{0}, err{0};
+    auto work = [&]() {
+        if (hipSetDevice(dev) != hipSuccess) { err.store(CMP_ERR_HIP); return; }
+        for (int k = next.fetch_add(1); k < (int)slots.size(); k = next.fetch_add(1)) {
+            const int q = slots[k];
+            BatchCase& c = B->cases[q];
+            double inf6[6] = {0, 0, 0, 0, 0, 0};
+            int r = check_hip(hipStreamWaitEvent(c.pf_stream, B->ev_flow, 0));
+            if (!r) r = cmp_level_pf_evolve(c.L, B->d_arena[0] + c.off, B->d_arena[1] + c.off,
+                                            B->d_arena[2] + c.off, B->d_dt + q, inf6, c.pf_stream);
+            if (!r) r = check_hip(hipStreamSynchronize(c.pf_stream));
+            if (info)
+                for (int m = 0; m < 6; ++m) info[6 * q + m] = inf6[m];
+            if (r) {
+                int z = 0;
+                err.compare_exchange_strong(z, r);
+            }
+        }
+    };
+    const int nw = std::max(1, std::min(nthreads > 0 ? nthreads : 8, (int)slots.size()));
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nw; ++w) pool.emplace_back(work);
+    work();
+    for (std::thread& t : pool) t.join();
+    return err.load();
 }
 
 int cmp_batch_info(const cmp_batch* B, int* ncases, int* ntiles, long long* free_doubles)

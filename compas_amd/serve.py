@@ -5,8 +5,9 @@ over the tiles of all of them). A case leaves the batch after its last step; its
 is the time from its arrival to the end of that step.
 
 This path has no counterpart in the reference, which runs one case per process
-(source/main.cpp:21-86). The flow step is batched; the Phase-Field step of these cases is not part
-of the batch (run a case's level through cmp_level_pf_evolve for it).
+(source/main.cpp:21-86). The flow step is batched; with `pf` every tick also runs the Phase-Field
+step of every admitted case (cmp_batch_pf_step: the cases concurrently, each on its own stream), as
+each case's own run would after its flow step.
 """
 import time
 
@@ -26,7 +27,7 @@ def _prepare(seeds, max_grid_size, grids):
 
 
 def serve(rate, n_cases, steps=200, budget_bytes=16 << 30, max_cases=256, seed=0, max_grid_size=64,
-          grids=None, stream=None, keep=None):
+          grids=None, stream=None, keep=None, pf=None, pf_threads=0):
     """Runs n_cases seeded cases arriving at `rate` cases/s through one batch whose three arenas take
     budget_bytes of device memory. Returns a dict: p50/p99/max time to solution (s), aggregate
     cell-updates/s over the run (first arrival to last completion), cells and cell-updates, ticks,
@@ -38,6 +39,8 @@ def serve(rate, n_cases, steps=200, budget_bytes=16 << 30, max_cases=256, seed=0
     arrivals -= arrivals[0]
     like = reqs[0]["case"]["params"]
     batch = Batch(like, budget_bytes // (3 * 8), max_cases)
+    if pf is not None:
+        batch.pf_setup(pf)
     s = stream or torch.cuda.current_stream()
     pending = list(range(n_cases))
     active = {}                      # case id -> (request index, steps done)
@@ -60,6 +63,8 @@ def serve(rate, n_cases, steps=200, budget_bytes=16 << 30, max_cases=256, seed=0
             time.sleep(max(0.0, arrivals[pending[0]] - (time.perf_counter() - t0)))
             continue
         batch.step(stream=s)
+        if pf is not None:
+            batch.pf_step(pf_threads, stream=s)
         s.synchronize()
         ticks += 1
         occupancy += len(active)
@@ -87,9 +92,9 @@ def serve(rate, n_cases, steps=200, budget_bytes=16 << 30, max_cases=256, seed=0
                 ticks=ticks, mean_cases_per_tick=occupancy / max(1, ticks), finals=finals)
 
 
-def one_at_a_time(n_cases, steps=200, seed=0, max_grid_size=64, grids=None):
+def one_at_a_time(n_cases, steps=200, seed=0, max_grid_size=64, grids=None, pf=None):
     """The same cases run one after another, each alone through the single-level path (a captured
-    step replayed): cell-updates/s of that form."""
+    step replayed, then the Phase-Field step with `pf`): cell-updates/s of that form."""
     import torch
     reqs = _prepare([seed * 100003 + i for i in range(n_cases)], max_grid_size, grids)
     dev = torch.device("cuda:0")
@@ -103,6 +108,8 @@ def one_at_a_time(n_cases, steps=200, seed=0, max_grid_size=64, grids=None):
         dt = torch.zeros(1, dtype=torch.float64, device=dev)
         cm = torch.zeros(1, dtype=torch.float64, device=dev)
         lev = Level(c["params"], c["geom"], r["layout"])
+        if pf is not None:
+            lev.pf_setup(pf)
         lev.advance(Un, U1, U2, steps, c["cfl"], dt, cm, use_graph=True)
         torch.cuda.synchronize()
         if lev.check_errors():

@@ -506,8 +506,22 @@ int cmp_batch_case_state(cmp_batch* batch, int case_id, double** Un, long long* 
  * synchronises the stream. */
 int cmp_batch_case_download(cmp_batch* batch, int case_id, double* host, void* stream);
 
-/* Reads and clears a case's error word (as cmp_level_check). */
+/* Reads and clears a case's error word (as cmp_level_check), its flow step's and its Phase-Field
+ * step's together. */
 int cmp_batch_check(cmp_batch* batch, int case_id, void* stream);
+
+/* The Phase-Field step of the served cases (PhaseField_Evolve after every flow step, as each case's
+ * own run has it, source/Class_Compressible_PhaseField.cpp:425-594): every admitted case, and every
+ * case admitted later, gets the Phase-Field buffers of cmp_level_pf_setup with these parameters
+ * (device memory outside the arenas). CAC with the explicit RC is CMP_ERR_UNSUPPORTED. */
+int cmp_batch_pf_setup(cmp_batch* batch, const cmp_pf_params* pf);
+/* After cmp_batch_step on `stream`: the Phase-Field step of every admitted case with the dt of its
+ * flow step, the cases concurrently, each on a stream of its own driven by one of `nthreads` host
+ * workers (0: 8), since a case's sub-step counts and solver iterations follow from its own numbers
+ * (cmp_level_pf_evolve per case, so a case's result does not depend on what else is in the batch).
+ * info (host, 6 doubles per case slot, or NULL): cmp_level_pf_evolve's info of each admitted case.
+ * Returns when every case's step is done. */
+int cmp_batch_pf_step(cmp_batch* batch, int nthreads, double* info, void* stream);
 
 /* Admitted cases, tiles of the last rebuilt table (-1 before the next step rebuilds it) and free
  * arena doubles. */

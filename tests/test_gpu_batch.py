@@ -21,7 +21,7 @@ from tests.test_gpu_parity import _staggered_boxes
 pytestmark = pytest.mark.gpu
 
 
-def _alone(c, nsteps):
+def _alone(c, nsteps, pf=None):
     import torch
     dev = torch.device("cuda:0")
     P = c["params"]
@@ -32,6 +32,8 @@ def _alone(c, nsteps):
     cm = torch.zeros(1, dtype=torch.float64, device=dev)
     hist = torch.zeros(max(1, nsteps), dtype=torch.float64, device=dev)
     lev = Level(P, c["geom"], lay)
+    if pf is not None:
+        lev.pf_setup(pf)
     lev.advance(Un, U1, U2, nsteps, c["cfl"], dt, cm, hist, use_graph=False)
     assert lev.check_errors() == 0
     lev.close()
@@ -157,3 +159,24 @@ def test_served_stream_results_equal_alone_runs(gpu):
         lay = Layout(2, c["boxes"], 3, 6)
         ref, _ = _alone(c, 12)
         assert np.array_equal(lay.unpack(res["finals"][i], c["shape"]), ref), i
+
+
+@pytest.mark.parametrize("mech,rc", [("CAC-Adv", 0), ("ACDI", 1)])
+def test_served_stream_with_phase_field_equals_alone_runs(gpu, mech, rc):
+    """The request loop with the Phase-Field step of every case after its flow step (the cases'
+    steps concurrent on their own streams): six seeded cases of three grid sizes arriving at 2000
+    cases/s, 6 steps each; every final state is bit-identical to the case run alone through the
+    single-level path with the same Phase-Field step, and the interface has moved."""
+    from compas_amd import make_pf_params, serve
+    pf = make_pf_params(mechanism=mech, explicit_rc=rc, cfl=0.3, eta_multiplier=0.5)
+    grids = [(64, 64), (128, 32), (96, 48)]
+    res = serve.serve(2000.0, 6, steps=6, seed=9, max_grid_size=32, grids=grids, keep=set(range(6)), pf=pf)
+    assert res["p99_s"] >= res["p50_s"] > 0.0
+    for i in range(6):
+        c = cases.served_case(9 * 100003 + i, max_grid_size=32, grids=grids)
+        lay = Layout(2, c["boxes"], 3, 6)
+        ref, _ = _alone(c, 6, pf)
+        flow, _ = _alone(c, 6)
+        got = lay.unpack(res["finals"][i], c["shape"])
+        assert np.array_equal(got, ref), (i, np.max(np.abs(got - ref)))
+        assert np.max(np.abs(ref[5] - flow[5])) > 1e-9, i
