@@ -459,6 +459,10 @@ def served(args):
         res.pop("finals")
         runs.append(res)
     alone = serve.one_at_a_time(args.cases, steps=args.case_steps, seed=1, pf=pf)
+    # the plain per-box GPU form (flow step only, against the batched flow step below)
+    plain = serve.plain_per_box(args.cases, steps=3, seed=1)
+    flow = serve.serve(rates[-1], args.cases, steps=args.case_steps, seed=1)
+    flow.pop("finals")
     top = runs[-1]
     P = cases.served_case(0)["params"]
     achieved = roofline.flops_per_cell_stage(P) * 3 * top["cell_updates_per_s"] / 1e12
@@ -484,6 +488,10 @@ def served(args):
         "latency": [{k: v for k, v in run.items()} for run in runs],
         "one_at_a_time": alone,
         "batched_over_one_at_a_time": top["cell_updates_per_s"] / alone["cell_updates_per_s"],
+        "plain_per_box": dict(plain, form="one case at a time: per RK3 stage a level ghost fill, one cmp_rhs_box "
+                                          "launch per box and the stage combination of each box; flow step only"),
+        "batched_flow_only": {k: flow[k] for k in ("p50_s", "p99_s", "cell_updates_per_s", "wall_s", "ticks")},
+        "batched_flow_over_plain_per_box": flow["cell_updates_per_s"] / plain["cell_updates_per_s"],
         "roofline": {"bound": "valu", "kernel": "k_stage_batch (whole request loop: host work and gaps included)",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,

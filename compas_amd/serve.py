@@ -118,3 +118,60 @@ def one_at_a_time(n_cases, steps=200, seed=0, max_grid_size=64, grids=None, pf=N
         total += r["cells"] * steps
     wall = time.perf_counter() - t0
     return dict(cases=n_cases, steps=steps, cell_updates=total, wall_s=wall, cell_updates_per_s=total / wall)
+
+
+def plain_per_box(n_cases, steps=3, seed=0, max_grid_size=64, grids=None):
+    """The same cases one at a time in the plain per-box form a direct port of the reference's box
+    loop takes (compute_dUdt_FV per box: source/Class_Compressible_PhaseField.cpp:3473-3537, then
+    the RK3 LinComb per box): per RK3 stage a ghost fill of the level, one cmp_rhs_box launch per box
+    and the stage's combination of that box's valid cells; ComputeDt from the boxes' c_max on the
+    host each step. Flow step only; cell-updates/s over `steps` steps of every case."""
+    import torch
+    from . import rhs_box
+    reqs = _prepare([seed * 100003 + i for i in range(n_cases)], max_grid_size, grids)
+    dev = torch.device("cuda:0")
+    total = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in reqs:
+        c, lay = r["case"], r["layout"]
+        P, G = c["params"], c["geom"]
+        N, ng = P.nstate, lay.ng
+        lev = Level(P, G, lay)
+        Un = torch.from_numpy(r["U0"]).to(dev)
+        U1, U2 = torch.zeros_like(Un), torch.zeros_like(Un)
+        cm = torch.zeros(1, dtype=torch.float64, device=dev)
+        views = []
+        for b, (lo, hi) in enumerate(lay.boxes):
+            off = lay.cboxes[b].offset
+            shp = lay.fab_shape(b)
+            size = int(np.prod(shp))
+            nv = (hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+            sl = (slice(None), slice(ng, ng + nv[0]), slice(ng, ng + nv[1]))
+            dU = torch.zeros((N,) + nv, dtype=torch.float64, device=dev)
+            views.append((off, size, shp, sl, dU, (lo, hi)))
+        dx = min(G.dx[0], G.dx[1])
+        lev.FillPatch(Un)
+        lev.CalculateInitialcmax(Un, cm)
+        for _ in range(steps):
+            dt = c["cfl"] * dx / float(cm.item())
+            cm.zero_()
+            for stage, (Us, Uo) in enumerate(((Un, U1), (U1, U2), (U2, Un))):
+                lev.FillPatch(Us)
+                a, b, w = ((0.0, 1.0, 1.0), (0.25, 0.75, 0.25), (2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0))[stage]
+                for off, size, shp, sl, dU, box in views:
+                    rhs_box(P, G, box, ng, Us[off:off + size], dU, cm if stage == 0 else None)
+                    us = Us[off:off + size].view(shp)[sl]
+                    un = Un[off:off + size].view(shp)[sl]
+                    uo = Uo[off:off + size].view(shp)[sl]
+                    if stage == 0:
+                        uo.copy_(us + dt * dU)
+                    else:
+                        uo.copy_((a * us + b * un) + (w * dt) * dU)
+            total += r["cells"]
+        torch.cuda.synchronize()
+        if lev.check_errors():
+            raise RuntimeError("case ended with a device error")
+        lev.close()
+    wall = time.perf_counter() - t0
+    return dict(cases=n_cases, steps=steps, cell_updates=total, wall_s=wall, cell_updates_per_s=total / wall)
