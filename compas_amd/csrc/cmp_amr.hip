@@ -62,7 +62,7 @@ struct AmrPf {
     int npf = 1;
     std::vector<BoxDev*> d_pbox;          // phase-major alpha layout (npf > 1; else the level's d_abox)
     std::vector<FillDev> fdP;
-    std::vector<double*> A0, A, Ps, H, S;
+    std::vector<double*> A0, A, Ps, H, S, IDL;   // IDL: the composite preconditioner's level diagonals
     std::vector<int*> d_mask;             // composite masks (implicit RC)
     CompCell* d_cells = nullptr;
     CompLink* d_links = nullptr;
@@ -73,6 +73,7 @@ struct AmrPf {
     unsigned long long* d_bits = nullptr; // [0] max|u|, [1 + l] max|Grad_Q| of level l
     double* h_pin = nullptr;              // pinned host copies
     int cg_iters = 0;
+    hipGraphExec_t cg_exec = nullptr;     // a chunk of the composite CG's iterations, captured once
 };
 
 struct cmp_amr {
@@ -200,7 +201,7 @@ void amr_pf_free(cmp_amr* A)
     AmrPf& P = A->pf;
     for (size_t l = 0; l < P.d_pbox.size(); ++l)
         if (P.d_pbox[l] && P.d_pbox[l] != A->L[l]->d_abox) (void)hipFree(P.d_pbox[l]);
-    for (auto* v : {&P.A0, &P.A, &P.Ps, &P.H, &P.S})
+    for (auto* v : {&P.A0, &P.A, &P.Ps, &P.H, &P.S, &P.IDL})
         for (double* p : *v)
             if (p) (void)hipFree(p);
     for (int* p : P.d_mask)
@@ -210,6 +211,7 @@ void amr_pf_free(cmp_amr* A)
     if (P.d_lsum) (void)hipFree(P.d_lsum);
     if (P.d_bits) (void)hipFree(P.d_bits);
     if (P.h_pin) (void)hipHostFree(P.h_pin);
+    if (P.cg_exec) (void)hipGraphExecDestroy(P.cg_exec);
     P = AmrPf();
 }
 
@@ -708,6 +710,8 @@ CompArgs comp_args(const cmp_amr* A, bool solution)
         c.mask = P.d_mask[l];
         c.v = solution ? L->d_X : L->d_Pv;
         c.bf = L->d_Bf;
+        c.s = (l < (int)P.S.size()) ? P.S[l] : nullptr;
+        c.idl = (l < (int)P.IDL.size()) ? P.IDL[l] : nullptr;
         c.vol = 1.0;
         for (int d = 0; d < 3; ++d) {
             c.dx[d] = L->pg.dx[d];
@@ -810,8 +814,11 @@ int amr_fill_pf(cmp_amr* A, int nc, double* const* F, const std::vector<FillDev>
 
 // The composite implicit solve of one phase: rhs in every level's d_DJ, b in d_Bf (averaged down);
 // Grad_Q into out[l] (one-component face buffers), averaged down.
-int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
+int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s0)
 {
+    // s: the stream every launch below goes to (the caller's; the capture stream while a chunk of
+    // CG iterations is captured)
+    hipStream_t s = s0;
     AmrPf& P = A->pf;
     const int nlev = (int)A->L.size();
     cmp_level* L0 = A->L[0];
@@ -823,11 +830,12 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
     // the composite cells (the operator's null space)
     const double inv_n = (mg && P.ncomp > 0) ? 1.0 / (double)P.ncomp : 0.0;
     int r;
-    // the level sums of nv partial slots from slot s0 on
-    auto sum_levels = [&](int nv, double* dst, const double* scx, int s0 = 0) -> int {
+    // the level sums of nv partial slots from slot slot0 on
+    auto sum_levels = [&](int nv, double* dst, const double* scx, int slot0 = 0) -> int {
         for (int l = 0; l < nlev; ++l) {
             const cmp_level* L = A->L[l];
-            CMP_TRY(launch_reduce_sums(L->d_part + (long long)s0 * L->nblocks, L->nblocks, nv, P.d_lsum + nv * l, s));
+            CMP_TRY(launch_reduce_sums(L->d_part + (long long)slot0 * L->nblocks, L->nblocks, nv, P.d_lsum + nv * l,
+                                       s, scx));
         }
         AmrPfArgs a{};
         a.src = P.d_lsum;
@@ -883,6 +891,11 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
         }
         return CMP_OK;
     };
+    // every level's own operator diagonal for the preconditioner's intermediate levels
+    for (int l = 1; l + 1 < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        CMP_TRY(launch_cg_idiag(dim, L->pg, L->maxcells, L->d_Bf, P.IDL[l], s));
+    }
     // Jacobi diagonal; the rhs mean over the composite cells; r, x
     for (int l = 0; l < nlev; ++l) {
         AmrPfArgs a = level_args(A->L[l], l, &caP);
@@ -913,7 +926,9 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
     }
     const int maxit = 100000;
     const int chunk = 8;
-    for (long long queued = 0;; queued += chunk) {
+    // a chunk of iterations depends only on the hierarchy's buffers: captured once into a graph and
+    // replayed (one launch instead of about a hundred per iteration)
+    auto enqueue_chunk = [&]() -> int {
         for (int q = 0; q < chunk; ++q) {
             // the search direction's ghost cells for the regular stencil (a finer level's coarse-fine
             // ghosts are interpolated but never read: those cells take their link lists)
@@ -947,9 +962,32 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s)
             }
             CMP_TRY(launch_cg_end(sc, maxit, s, inv_n));
         }
-        CMP_TRY(hipMemcpyAsync(P.h_pin + 0, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
-        CMP_TRY(hipMemcpyAsync(P.h_pin + 1, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, s));
-        CMP_TRY(hipStreamSynchronize(s));
+        return CMP_OK;
+    };
+    hipStream_t cs = s0;
+    if (!cs) {
+        if (!A->own_stream) CMP_TRY(hipStreamCreateWithFlags(&A->own_stream, hipStreamNonBlocking));
+        cs = A->own_stream;
+        CMP_TRY(hipStreamSynchronize(nullptr));
+    }
+    if (!P.cg_exec) {
+        CMP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        s = cs;
+        const int rc = enqueue_chunk();
+        s = s0;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        CMP_TRY(e);
+        const hipError_t ei = hipGraphInstantiate(&P.cg_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CMP_TRY(ei);
+    }
+    for (long long queued = 0;; queued += chunk) {
+        CMP_TRY(hipGraphLaunch(P.cg_exec, cs));
+        CMP_TRY(hipMemcpyAsync(P.h_pin + 0, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, cs));
+        CMP_TRY(hipMemcpyAsync(P.h_pin + 1, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, cs));
+        CMP_TRY(hipStreamSynchronize(cs));
         if (P.h_pin[0] != 0.0) break;
         if (queued > maxit) return CMP_ERR_NAN;
     }
@@ -1001,7 +1039,7 @@ int cmp_amr_pf_setup(cmp_amr* A, const cmp_pf_params* pf)
         FillDev fd = L->fdA;
         fd.boxes = pb;
         P.fdP.push_back(fd);
-        double *a0 = nullptr, *a = nullptr, *ps = nullptr, *h = nullptr, *sv = nullptr;
+        double *a0 = nullptr, *a = nullptr, *ps = nullptr, *h = nullptr, *sv = nullptr, *idl = nullptr;
         CMP_TRYA(hipMalloc(&a0, sizeof(double) * L->na * P.npf));
         P.A0.push_back(a0);
         CMP_TRYA(hipMalloc(&a, sizeof(double) * L->na * P.npf));
@@ -1013,8 +1051,11 @@ int cmp_amr_pf_setup(cmp_amr* A, const cmp_pf_params* pf)
         if (implicit) {
             CMP_TRYA(hipMalloc(&sv, sizeof(double) * L->na));
             CMP_TRYA(hipMemset(sv, 0, sizeof(double) * L->na));
+            CMP_TRYA(hipMalloc(&idl, sizeof(double) * L->na));
+            CMP_TRYA(hipMemset(idl, 0, sizeof(double) * L->na));
         }
         P.S.push_back(sv);
+        P.IDL.push_back(idl);
         // the unfilled ghost cells stay defined
         CMP_TRYA(hipMemset(a0, 0, sizeof(double) * L->na * P.npf));
         CMP_TRYA(hipMemset(a, 0, sizeof(double) * L->na * P.npf));

@@ -307,7 +307,21 @@ __global__ __launch_bounds__(NT) void k_comp_z(CompArgs a, int l, const double* 
     const int a2 = (DIM == 3) ? ap_floor_div(k, f) - dlo2 : 0;
     // without a multigrid level 0 (x0 null) every level takes the Jacobi term alone
     const double z0 = x0 ? x0[(long long)a0 + (long long)G0.n[0] * (a1 + (long long)G0.n[1] * a2)] : 0.0;
-    Z[c] = ((l > 0 || !x0) ? R[c] * ID[c] : 0.0) + z0;
+    // every level between 0 and l: its averaged-down residual per volume over its own diagonal at the
+    // cell's ancestor there (an additive multilevel term per level: symmetric, as the sum of P D^-1 P^T)
+    double zm = 0.0;
+    int fm = 1;
+    for (int m = l - 1; m >= 1; --m) {
+        fm *= a.ratio;
+        const CompLev& M = a.L[m];
+        int q[3] = {ap_floor_div(i, fm), ap_floor_div(j, fm), (DIM == 3) ? ap_floor_div(k, fm) : 0};
+        const int own = ap_owner<DIM>(M.fd, q);
+        if (own < 0) continue;
+        const BoxDev mb = M.abox[own];
+        const long long mc = mb.off + fab_index(mb, q[0], q[1], q[2]);
+        zm += M.s[mc] * M.idl[mc];
+    }
+    Z[c] = ((l > 0 || !x0) ? R[c] * ID[c] : 0.0) + zm + z0;
 }
 
 // Grad_Q on every face of direction D of level l from the composite solution X (covered cells

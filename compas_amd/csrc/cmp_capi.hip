@@ -780,6 +780,8 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
 // Releases what cmp_level_pf_setup allocated (errors of the frees are not actionable here).
 static void pf_free(cmp_level* L)
 {
+    if (L->cg_exec) (void)hipGraphExecDestroy(L->cg_exec);
+    L->cg_exec = nullptr;
     void* bufs[] = {L->d_abox, L->d_nbox, L->d_gprefix1, L->d_A0, L->d_A, L->d_P, L->d_N,
                     L->d_J, L->d_H, L->d_gq, L->d_gq1, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
                     L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Z, L->d_ID, L->d_Bf, L->d_part, L->d_sc,
@@ -1373,6 +1375,26 @@ int cmp::mg_vcycle(cmp_level* L, const double* sc, size_t m, hipStream_t s)
 {
     const int dim = L->P.dim;
     auto& V = L->mg[m];
+    if (V.G.nc <= cmp::MG_TAIL_MAX_CELLS && L->mg.size() - m <= (size_t)cmp::MG_TAIL_MAX_LEVELS) {
+        // this grid and every coarser one in a single workgroup (one launch instead of six a grid)
+        cmp::MgTail t{};
+        t.nlev = (int)(L->mg.size() - m);
+        t.sweeps = L->mg_bottom_sweeps;
+        t.omega = L->mg_omega;
+        t.sc = sc;
+        for (int q = 0; q < t.nlev; ++q) {
+            const auto& W = L->mg[m + q];
+            t.G[q] = W.G;
+            t.x0[q] = W.x[0];
+            t.x1[q] = W.x[1];
+            t.r[q] = W.r;
+            t.id[q] = W.id;
+            for (int d = 0; d < 3; ++d) t.b[q][d] = W.b[d];
+        }
+        CMP_TRY(dim == 2 ? cmp::launch_mg_tail_dim<2>(t, s) : cmp::launch_mg_tail_dim<3>(t, s));
+        V.cur = (t.nlev == 1) ? 0 : 1;
+        return CMP_OK;
+    }
     cmp::MgArgs a = mg_args(L, sc);
     for (int d = 0; d < 3; ++d) a.b[d] = V.b[d];
     a.G = V.G;
@@ -1474,28 +1496,52 @@ static int pf_gradq_implicit(cmp_level* L, double dt, double eta, double* gradq,
         CMP_TRY(launch_cg_start(sc, s));
     }
     // the host looks at the stop flag every `chunk` iterations; iterations queued past the stop
-    // return at once
+    // return at once. A chunk's launches depend only on the level's buffers, so it is captured once
+    // into a graph and replayed (one launch instead of 60-80 per iteration).
     const int chunk = mg ? 8 : 16;
-    for (long long queued = 0;; queued += chunk) {
+    auto enqueue_chunk = [&](hipStream_t cs) -> int {
         for (int q = 0; q < chunk; ++q) {
-            CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_Pv, L->d_err, s));
-            CMP_TRY(launch_cg_ap(dim, L->pg, L->maxcells, sc, L->d_Bf, L->d_Pv, L->d_AP, L->d_part, s));
-            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_PAP, s));
+            CMP_TRY(launch_fill(dim, 1, L->fdA, L->nghost1, L->d_Pv, L->d_err, cs));
+            CMP_TRY(launch_cg_ap(dim, L->pg, L->maxcells, sc, L->d_Bf, L->d_Pv, L->d_AP, L->d_part, cs));
+            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 1, sc + CG_PAP, cs, sc));
             CMP_TRY(launch_cg_xr(dim, L->pg, L->maxcells, sc, mg ? nullptr : L->d_ID, L->d_X, L->d_R, L->d_Z,
-                                 L->d_Pv, L->d_AP, L->d_part, s));
+                                 L->d_Pv, L->d_AP, L->d_part, cs));
             if (mg) {
-                int r = mg_precondition(L, sc, s);
+                int r = mg_precondition(L, sc, cs);
                 if (r) return r;
             }
-            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RRNEW, s));
+            CMP_TRY(launch_reduce_sums(L->d_part, L->nblocks, 2, sc + CG_RRNEW, cs, sc));
             if (mg)
-                CMP_TRY(launch_reduce_sums(L->d_part + 2 * (long long)L->nblocks, L->nblocks, 2, sc + CG_ZS, s));
-            CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, s, inv_n));
-            CMP_TRY(launch_cg_end(sc, maxit, s, inv_n));
+                CMP_TRY(launch_reduce_sums(L->d_part + 2 * (long long)L->nblocks, L->nblocks, 2, sc + CG_ZS, cs,
+                                           sc));
+            CMP_TRY(launch_cg_p(dim, L->pg, L->maxcells, sc, L->d_Z, L->d_Pv, cs, inv_n));
+            CMP_TRY(launch_cg_end(sc, maxit, cs, inv_n));
         }
-        CMP_TRY(hipMemcpyAsync(L->h_pin + 2, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, s));
-        CMP_TRY(hipMemcpyAsync(L->h_pin + 3, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, s));
-        CMP_TRY(hipStreamSynchronize(s));
+        return CMP_OK;
+    };
+    // captured on a private stream when the caller gave the legacy default stream
+    hipStream_t cs = s;
+    if (!cs) {
+        if (!L->own_stream) CMP_TRY(hipStreamCreateWithFlags(&L->own_stream, hipStreamNonBlocking));
+        cs = L->own_stream;
+        CMP_TRY(hipStreamSynchronize(nullptr));
+    }
+    if (!L->cg_exec) {
+        CMP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int r = enqueue_chunk(cs);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        if (r) { if (g) (void)hipGraphDestroy(g); return r; }
+        CMP_TRY(e);
+        const hipError_t ei = hipGraphInstantiate(&L->cg_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CMP_TRY(ei);
+    }
+    for (long long queued = 0;; queued += chunk) {
+        CMP_TRY(hipGraphLaunch(L->cg_exec, cs));
+        CMP_TRY(hipMemcpyAsync(L->h_pin + 2, sc + CG_DONE, sizeof(double), hipMemcpyDeviceToHost, cs));
+        CMP_TRY(hipMemcpyAsync(L->h_pin + 3, sc + CG_IT, sizeof(double), hipMemcpyDeviceToHost, cs));
+        CMP_TRY(hipStreamSynchronize(cs));
         if (L->h_pin[2] != 0.0) break;
         if (queued > maxit) return CMP_ERR_NAN;   // the device stop test failed to fire
     }

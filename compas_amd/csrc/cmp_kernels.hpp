@@ -224,6 +224,27 @@ struct MgArgs {
 template <int DIM>
 hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s);
 
+// The coarse end of a V-cycle in one workgroup: grids 0..nlev-1 of the tail (the coarsest last),
+// the residual of grid 0 ready in r[0]; the same operations in the same order as the per-grid
+// launches (two damped Jacobi sweeps, residual restriction, the coarser grids, prolongation, two
+// sweeps; the last grid `sweeps` sweeps from zero), the result in x1[0] (x0[0] when nlev is 1).
+constexpr int MG_TAIL_MAX_LEVELS = 16;
+constexpr long long MG_TAIL_MAX_CELLS = 4096;   // grids at most this size go to the tail
+struct MgTail {
+    int nlev;
+    int sweeps;
+    double omega;
+    const double* sc;
+    MgGrid G[MG_TAIL_MAX_LEVELS];
+    double* x0[MG_TAIL_MAX_LEVELS];
+    double* x1[MG_TAIL_MAX_LEVELS];
+    double* r[MG_TAIL_MAX_LEVELS];
+    const double* id[MG_TAIL_MAX_LEVELS];
+    const double* b[MG_TAIL_MAX_LEVELS][3];
+};
+template <int DIM>
+hipError_t launch_mg_tail_dim(const MgTail& t, hipStream_t s);
+
 // ---- the composite implicit Grad_Q of a hierarchy (cmp_amr_pf.hip) ----------------------------------
 // mask of a level's cell (alpha layout): covered by the next finer level, a regular cell (five- or
 // seven-point stencil over the same-level ghosts), or >= 0: the index of its CompCell
@@ -252,6 +273,9 @@ struct CompLev {
     double dx[3];
     double area[3];         // face areas per direction
     double vol;             // cell volume
+    const double* s;        // preconditioner: the residual per volume, covered cells the mean of
+                            // their children
+    const double* idl;      // preconditioner: 1 / the diagonal of the level's own operator, every cell
 };
 struct CompArgs {
     int nlev;
@@ -389,7 +413,10 @@ hipError_t launch_cg_end(double* sc, int maxit, hipStream_t s, double inv_n = 0.
 // face gradient of x (ghosts filled), zero on Neumann domain faces, with max |.|
 hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, const double* X,
                            double* gq, unsigned long long* gmax_bits, int* boxflag, hipStream_t s);
-hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s);
+// sums[v] = the sum of the nblocks partials of value v (fixed order); sc non-null: nothing once the
+// CG's stop flag sc[CG_DONE] is set
+hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s,
+                              const double* sc = nullptr);
 hipError_t launch_scale(double* p, long long n, double s_, hipStream_t s);
 hipError_t launch_setd(double* p, double v, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);

@@ -53,6 +53,7 @@ struct cmp_level {
     hipStream_t own_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    hipGraphExec_t cg_exec = nullptr;   // a chunk of the implicit Grad_Q's CG iterations (its buffers)
     const void* gkey[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double gcfl = 0.0;
     int glimiter = -1;

@@ -257,6 +257,87 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const double* __restrict__ s
     if (on) x[q] = xs[cur][q];
 }
 
+// The coarse end of the V-cycle (MgTail) in one workgroup of 1024 threads, every operation a
+// strided loop over the grid's cells followed by a barrier.
+constexpr int NTAIL = 1024;
+template <int DIM>
+__global__ __launch_bounds__(NTAIL) void k_mg_tail(MgTail t)
+{
+    if (t.sc && t.sc[CG_DONE] != 0.0) return;
+    const int tid = threadIdx.x;
+    const double om = t.omega;
+    const int last = t.nlev - 1;
+    // y = x + omega id (r - A x) on grid m (zero_x: y = omega id r)
+    auto smooth = [&](int m, const double* x, double* y, bool zero_x) {
+        const MgGrid& G = t.G[m];
+        const double* r = t.r[m];
+        const double* id = t.id[m];
+        for (long long q = tid; q < G.nc; q += NTAIL) {
+            if (zero_x) {
+                y[q] = om * id[q] * r[q];
+            } else {
+                int i, j, k;
+                mg_cell<DIM>(G, q, i, j, k);
+                y[q] = x[q] + om * id[q] * (r[q] - mg_apply<DIM>(G, x, t.b[m], i, j, k));
+            }
+        }
+        __syncthreads();
+    };
+    for (int m = 0; m < last; ++m) {
+        smooth(m, nullptr, t.x0[m], true);
+        smooth(m, t.x0[m], t.x1[m], false);
+        // r of grid m + 1: the mean of the children of r - A x1
+        const MgGrid& Gf = t.G[m];
+        const MgGrid& Gc = t.G[m + 1];
+        constexpr int NCH = (DIM == 3) ? 8 : 4;
+        for (long long q = tid; q < Gc.nc; q += NTAIL) {
+            int I, J, K;
+            mg_cell<DIM>(Gc, q, I, J, K);
+            double sum = 0.0;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = (DIM == 3) ? 2 * K + (ch >> 2) : 0;
+                const long long c = mg_idx(Gf, i, j, k);
+                sum += t.r[m][c] - mg_apply<DIM>(Gf, t.x1[m], t.b[m], i, j, k);
+            }
+            t.r[m + 1][q] = sum * (1.0 / (double)NCH);
+        }
+        __syncthreads();
+    }
+    // the coarsest grid: `sweeps` damped Jacobi sweeps from zero, alternating x0 / x1, the result
+    // copied to x0
+    {
+        const MgGrid& G = t.G[last];
+        double* xa = t.x0[last];
+        double* xb = t.x1[last];
+        smooth(last, nullptr, xa, true);
+        for (int sw = 1; sw < t.sweeps; ++sw) {
+            smooth(last, xa, xb, false);
+            double* tmp = xa;
+            xa = xb;
+            xb = tmp;
+        }
+        if (xa != t.x0[last]) {
+            for (long long q = tid; q < G.nc; q += NTAIL) t.x0[last][q] = xa[q];
+            __syncthreads();
+        }
+    }
+    for (int m = last - 1; m >= 0; --m) {
+        // x1 += the coarser grid's result at the parent cell, then two sweeps
+        const MgGrid& Gf = t.G[m];
+        const MgGrid& Gc = t.G[m + 1];
+        const double* xc = (m + 1 == last) ? t.x0[m + 1] : t.x1[m + 1];
+        for (long long q = tid; q < Gf.nc; q += NTAIL) {
+            int i, j, k;
+            mg_cell<DIM>(Gf, q, i, j, k);
+            t.x1[m][q] += xc[mg_idx(Gc, i >> 1, j >> 1, (DIM == 3) ? (k >> 1) : 0)];
+        }
+        __syncthreads();
+        smooth(m, t.x1[m], t.x0[m], false);
+        smooth(m, t.x0[m], t.x1[m], false);
+    }
+}
+
 // the partial sums of one CG iteration's r.z, sum z and sum r (the second, third and fourth partial
 // slots; the first holds r.r)
 template <int DIM>
@@ -370,5 +451,15 @@ hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s)
 }
 
 template hipError_t launch_mg_dim<CMP_KERNEL_DIM>(int, const MgArgs&, hipStream_t);
+
+template <int DIM>
+hipError_t launch_mg_tail_dim(const MgTail& t, hipStream_t s)
+{
+    if (t.nlev < 1 || t.nlev > MG_TAIL_MAX_LEVELS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_mg_tail<DIM>), dim3(1), dim3(NTAIL), 0, s, t);
+    return hipGetLastError();
+}
+
+template hipError_t launch_mg_tail_dim<CMP_KERNEL_DIM>(const MgTail&, hipStream_t);
 
 }  // namespace cmp

@@ -760,21 +760,37 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s)
     return hipGetLastError();
 }
 
-// sums[v] = sum_b part[v * nblocks + b], one workgroup, a fixed order of additions
-__global__ __launch_bounds__(NT) void k_reduce_sums(const double* __restrict__ part, int nblocks, int nvals,
-                                                    double* sums)
+// sums[v] = sum_b part[v * nblocks + b], one workgroup of 1024 threads, a fixed order of additions
+// (each thread four interleaved running sums, then the wave and workgroup sums in index order).
+// sc non-null: nothing once the solve's stop flag is set (the queued iterations past the stop).
+constexpr int NRED = 1024;
+__global__ __launch_bounds__(NRED) void k_reduce_sums(const double* __restrict__ part, int nblocks, int nvals,
+                                                      double* sums, const double* __restrict__ sc)
 {
-    __shared__ double red[NT];
+    if (sc && sc[CG_DONE] != 0.0) return;
+    __shared__ double red[NRED / 64];
     for (int v = 0; v < nvals; ++v) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += NT) s += part[(long long)v * nblocks + b];
-        red[threadIdx.x] = s;
-        __syncthreads();
-        for (int w = NT / 2; w > 0; w >>= 1) {
-            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+        const double* __restrict__ p = part + (long long)v * nblocks;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int b = threadIdx.x;
+        for (; b + 3 * NRED < nblocks; b += 4 * NRED) {
+            a0 += p[b];
+            a1 += p[b + NRED];
+            a2 += p[b + 2 * NRED];
+            a3 += p[b + 3 * NRED];
         }
-        if (threadIdx.x == 0) sums[v] = red[0];
+        for (; b < nblocks; b += NRED) a0 += p[b];
+        double t = (a0 + a1) + (a2 + a3);
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double u = red[0];
+#pragma unroll
+            for (int w = 1; w < NRED / 64; ++w) u += red[w];
+            sums[v] = u;
+        }
         __syncthreads();
     }
 }
@@ -822,9 +838,10 @@ __global__ void k_cg_end(double* sc, double maxit, double inv_n)
     if (done) sc[CG_DONE] = 1.0;
 }
 
-hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s)
+hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s,
+                              const double* sc)
 {
-    hipLaunchKernelGGL(k_reduce_sums, dim3(1), dim3(NT), 0, s, part, nblocks, nvals, sums);
+    hipLaunchKernelGGL(k_reduce_sums, dim3(1), dim3(NRED), 0, s, part, nblocks, nvals, sums, sc);
     return hipGetLastError();
 }
 
