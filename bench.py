@@ -20,8 +20,9 @@ Workloads (--workload):
                           Poisson process at each rate of --rates, 200 RK3 steps each (WENO5 +
                           HLLC, CFL 0.4), advanced together in one batch of levels; p50/p99 time to
                           solution per rate and aggregate cell-updates/s, beside the same cases run
-                          one at a time through the single-level path. The flow step only: the
-                          Phase-Field step is not part of the batch.
+                          one at a time through the single-level path, and the plain per-box
+                          form; every tick also runs each case's CAC Phase-Field step (the cases
+                          concurrently; --no-pf: flow only).
   nphase-uneven-level     the AMR configuration's base level: 512^2 three-phase converging rings
                           (NPHASE = 3, WENO5 + HLLC, CFL 0.4) on a seeded regrid-like split into
                           644 boxes of 8..64 cells a side; the batched level step (one launch per
@@ -34,9 +35,12 @@ Workloads (--workload):
                           of at most 64 (blocking factor 8); ambient gamma 1.4 rho 1 p 1, bubble
                           gamma 1.648 rho 0.138 r 0.25 at (1.5, 0.5), Mach 1.22 shock at x = 1;
                           WENO5B + HLLC, RK3 at CFL 0.4, levels stepped together without
-                          subcycling (coarse faces under the fine level take its face means), the
-                          hierarchy fixed over the timed steps; beside the uniform 2048x512 grid
-                          it stands for, stepped by the single-level path. Flow step only.
+                          subcycling (coarse faces under the fine level take its face means), a
+                          regrid every --regrid-int steps inside the timed run (GPU tags, host
+                          clustering, the fine level remade, graph rebuilt); each step followed by
+                          the CAC Phase-Field step (composite implicit Grad_Q; --no-pf: flow only);
+                          beside the uniform 2048x512 grid it stands for, stepped by the
+                          single-level path with the same Phase-Field step.
   nphase-rings-amr        the AMR configuration's flow step on its hierarchy: the 512^2 three-phase
                           rings base level (uneven boxes of 8..64) plus three refined levels
                           (ratio 2), each clustered from the GPU tags of the level below (every
@@ -44,8 +48,9 @@ Workloads (--workload):
                           boxes of at most 64, nested 4 cells inside their parent level), initial
                           data of each level from the same seeded rings at its resolution;
                           WENO5 + HLLC, RK3 at CFL 0.4, levels stepped together without
-                          subcycling, hierarchy fixed over the timed steps; beside the uniform
-                          4096^2 grid the finest level stands for. Flow step only.
+                          subcycling, a regrid every --regrid-int steps inside the timed run, the
+                          CAC Phase-Field step after each step (--no-pf: flow only); beside the
+                          uniform 4096^2 grid the finest level stands for.
 Also reported: the dominant kernel's roofline position (measured with HIP events on the stream it
 runs on) and the oracle (CPU restatement, OpenMP) timed on this host on a bounded sample.
 """
@@ -187,7 +192,8 @@ def shock_bubble_amr(args):
     """The shock-bubble-amr workload: one JSON line (see the module docstring)."""
     import numpy as np
     import torch
-    from compas_amd import Amr, Layout, Level, cases, regrid, roofline
+    from compas_amd import Layout, Level, cases, regrid, roofline
+    from compas_amd.amr_run import RegridHierarchy
     dev = torch.device("cuda:0")
     kw = dict(mach=1.22, rho_ratio=0.138, gamma_bubble=1.648, gamma_ambient=1.4, radius=0.25,
               max_grid_size=64, scheme="WENO5B", xb=1.5, xs=1.0)
@@ -205,28 +211,28 @@ def shock_bubble_amr(args):
     L1 = Level(P, c1["geom"], l1)
     U1f = torch.from_numpy(l1.pack(c1["U"])).to(dev)
     L1.AverageDownTo(U1f, L0, U0, 2)
-    amr = Amr([L0, L1], 2)
     pf = _amr_pf(args)
-    if pf is not None:
-        amr.pf_setup(pf)
-    Un = [U0, U1f]
-    init = [u.cpu().numpy().copy() for u in Un]   # the hierarchy's initial state, for the CPU baseline
-    S1 = [torch.zeros_like(u) for u in Un]
-    S2 = [torch.zeros_like(u) for u in Un]
-    dt = torch.zeros(1, dtype=torch.float64, device=dev)
-    cmax = torch.zeros(2, dtype=torch.float64, device=dev)
-    cells = [int(np.prod(c0["shape"])), sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in fine_boxes)]
+    init = [U0.cpu().numpy().copy(), U1f.cpu().numpy().copy()]   # the initial hierarchy, for the CPU baseline
+    h = RegridHierarchy(P, [c0["geom"], c1["geom"]], [L0, L1], [U0, U1f], 2, criteria=crit, blocking_factor=8,
+                        max_grid_size=64, pf=pf)
+    cells = h.cells()
     K = args.steps if args.steps is not None else 50
     W = args.warmup
-    amr.advance(Un, S1, S2, W, 0.4, dt, cmax)
+    h.advance(W, 0.4, args.regrid_int)
     torch.cuda.synchronize()
+    u0 = h.updates
+    st0 = dict(h.stats)
     t0 = time.perf_counter()
-    amr.advance(Un, S1, S2, K, 0.4, dt, cmax)
+    h.advance(K, 0.4, args.regrid_int)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    assert amr.check_errors() == 0
-    pf_rec = _amr_pf_split(amr, Un, S1, S2, dt, pf)
-    amr.close()
+    updates = h.updates - u0
+    assert h.check_errors() == 0
+    regrid_rec = {"regrid_int": args.regrid_int, "regrids_in_timed_steps": h.stats["regrids"] - st0["regrids"],
+                  "tag_and_cluster_s": h.stats["tag_and_cluster_s"] - st0["tag_and_cluster_s"],
+                  "remake_and_rebuild_s": h.stats["remake_s"] - st0["remake_s"], "cells_per_level_at_end": h.cells()}
+    pf_rec = _amr_pf_split(h.amr, h.Us, h.S1, h.S2, h.dt, pf)
+    h.close()
     # the uniform fine grid the hierarchy stands for, single-level path (graph-replayed steps)
     lu = Layout(2, c1["boxes"], 3, P.nstate)
     Lu = Level(P, c1["geom"], lu)
@@ -244,14 +250,11 @@ def shock_bubble_amr(args):
     wall_u = time.perf_counter() - t1
     assert Lu.check_errors() == 0
     Lu.close()
-    L0.close()
-    L1.close()
-    total = sum(cells)
     fps = roofline.flops_per_cell_stage(P)
-    achieved = 3.0 * fps * total * K / wall / 1e12
+    achieved = 3.0 * fps * updates / wall / 1e12
     rec = {
         "metric": "cell-updates/s (5Eq WENO5-HLLC RK3), % roofline; p50/p99 case latency",
-        "value": total * K / wall,
+        "value": updates / wall,
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": K,
@@ -274,6 +277,7 @@ def shock_bubble_amr(args):
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": None, "flops_per_cell_stage": fps},
     }
+    rec["regrid"] = regrid_rec
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
     if not args.no_cpu_baseline:
@@ -322,7 +326,8 @@ def nphase_rings_amr(args):
     """The nphase-rings-amr workload: one JSON line (see the module docstring)."""
     import numpy as np
     import torch
-    from compas_amd import Amr, Layout, Level, cases, regrid, roofline
+    from compas_amd import Layout, Level, cases, regrid, roofline
+    from compas_amd.amr_run import RegridHierarchy
     dev = torch.device("cuda:0")
     crit = [(6, "grad", 0.1), (7, "grad", 0.1), (8, "grad", 0.1), ("P", "grad", 0.1)]
     c = cases.nphase_rings_2d(512)
@@ -359,29 +364,28 @@ def nphase_rings_amr(args):
     for lev in range(len(levels) - 2, -1, -1):
         levels[lev + 1].AverageDownTo(Us[lev + 1], levels[lev], Us[lev], 2)
     init = [u.cpu().numpy().copy() for u in Us] if not args.no_cpu_baseline else None
-    amr = Amr(levels, 2)
     pf = _amr_pf(args)
-    if pf is not None:
-        amr.pf_setup(pf)
-    S1 = [torch.zeros_like(u) for u in Us]
-    S2 = [torch.zeros_like(u) for u in Us]
-    dt = torch.zeros(1, dtype=torch.float64, device=dev)
-    cmax = torch.zeros(len(levels), dtype=torch.float64, device=dev)
-    cells = [sum(int(np.prod([h - l + 1 for l, h in zip(lo, hi)])) for lo, hi in la.boxes) for la in lays]
+    h = RegridHierarchy(P, geoms, levels, Us, 2, criteria=crit, blocking_factor=8, max_grid_size=64, nest_margin=4,
+                        pf=pf)
+    cells = h.cells()
     K = args.steps if args.steps is not None else 50
     W = args.warmup
-    amr.advance(Us, S1, S2, W, 0.4, dt, cmax)
+    h.advance(W, 0.4, args.regrid_int)
     torch.cuda.synchronize()
+    u0 = h.updates
+    st0 = dict(h.stats)
     t0 = time.perf_counter()
-    amr.advance(Us, S1, S2, K, 0.4, dt, cmax)
+    h.advance(K, 0.4, args.regrid_int)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    assert amr.check_errors() == 0
-    pf_rec = _amr_pf_split(amr, Us, S1, S2, dt, pf)
-    amr.close()
-    for lv in levels:
-        lv.close()
-    del Us, S1, S2
+    updates = h.updates - u0
+    assert h.check_errors() == 0
+    regrid_rec = {"regrid_int": args.regrid_int, "regrids_in_timed_steps": h.stats["regrids"] - st0["regrids"],
+                  "tag_and_cluster_s": h.stats["tag_and_cluster_s"] - st0["tag_and_cluster_s"],
+                  "remake_and_rebuild_s": h.stats["remake_s"] - st0["remake_s"], "cells_per_level_at_end": h.cells()}
+    pf_rec = _amr_pf_split(h.amr, h.Us, h.S1, h.S2, h.dt, pf)
+    h.close()
+    del Us
     torch.cuda.empty_cache()
     # the uniform finest grid, single-level path (graph-replayed steps)
     cu = cases.nphase_rings_2d(n, boxes=cases.split_domain((n, n), 64))
@@ -403,12 +407,11 @@ def nphase_rings_amr(args):
     wall_u = time.perf_counter() - t1
     assert Lu.check_errors() == 0
     Lu.close()
-    total = sum(cells)
     fps = roofline.flops_per_cell_stage(P)
-    achieved = 3.0 * fps * total * K / wall / 1e12
+    achieved = 3.0 * fps * updates / wall / 1e12
     rec = {
         "metric": "cell-updates/s (5Eq WENO5-HLLC RK3), % roofline; p50/p99 case latency",
-        "value": total * K / wall,
+        "value": updates / wall,
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": K,
@@ -431,6 +434,7 @@ def nphase_rings_amr(args):
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                      "traffic": None, "flops_per_cell_stage": fps},
     }
+    rec["regrid"] = regrid_rec
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
     if init is not None:
@@ -712,6 +716,9 @@ def main():
     ap.add_argument("--case-steps", type=int, default=200, help="served-stream: RK3 steps per case")
     ap.add_argument("--no-pf", action="store_true",
                     help="shock-bubble-amr, nphase-rings-amr, served-stream: the flow step alone, without the CAC Phase-Field step")
+    ap.add_argument("--regrid-int", type=int, default=10,
+                    help="shock-bubble-amr, nphase-rings-amr: regrid every this many steps inside the timed run "
+                         "(amr.regrid_int; 0: the hierarchy fixed)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure the dominant kernel's HBM traffic")
     args = ap.parse_args()

@@ -120,9 +120,11 @@ def _chop(lo, hi, maxb):
     return pieces
 
 
-def cluster(tags, blocking_factor=8, max_grid_size=64, grid_eff=0.7, n_error_buf=1):
+def cluster(tags, blocking_factor=8, max_grid_size=64, grid_eff=0.7, n_error_buf=1, scale=1, n_cell=None):
     """Boxes [(lo, hi)] (cell indices, x first, inclusive) covering every tag of `tags`
-    ([(k,) j, i]), aligned to the blocking factor, at most max_grid_size cells a side."""
+    ([(k,) j, i]), aligned to the blocking factor, at most max_grid_size cells a side. scale /
+    n_cell: the tags stand for blocks of `scale` cells of a grid n_cell ([(k,) j, i]) in size, and
+    the boxes are returned in those cells."""
     dim = tags.ndim
     t = _grow(tags, n_error_buf)
     block = _coarsen(t, blocking_factor)
@@ -130,11 +132,12 @@ def cluster(tags, blocking_factor=8, max_grid_size=64, grid_eff=0.7, n_error_buf
     _split(block, [0] * dim, [s - 1 for s in block.shape], grid_eff, raw)
     maxb = max(1, max_grid_size // blocking_factor)
     out = []
-    n_cell = tags.shape
+    f = blocking_factor * scale
+    n_cell = tags.shape if n_cell is None else tuple(n_cell)
     for lo, hi in raw:
         for a, b in _chop(lo, hi, maxb):
-            clo = [a[ax] * blocking_factor for ax in range(dim)]
-            chi = [min((b[ax] + 1) * blocking_factor, n_cell[ax]) - 1 for ax in range(dim)]
+            clo = [a[ax] * f for ax in range(dim)]
+            chi = [min((b[ax] + 1) * f, n_cell[ax]) - 1 for ax in range(dim)]
             out.append((list(reversed(clo)), list(reversed(chi))))   # numpy axes -> x first
     return out
 
@@ -193,7 +196,10 @@ def nest(boxes, parent_boxes, shape, blocking_factor=8, margin=4):
         t = t.reshape(t.shape[:ax] + (nbk[ax], bf) + t.shape[ax + 1:]).all(axis=ax + 1)
     keep &= t
     if dim != 2:
-        raise NotImplementedError("nest() forms 2D box lists")
+        # the kept blocks as boxes: Berger-Rigoutsos on them with efficiency 1 (every box entirely of
+        # kept blocks), at most 64 cells a side
+        return cluster(keep.astype(np.uint8), blocking_factor=1, max_grid_size=max(1, 64 // bf), grid_eff=1.0,
+                       n_error_buf=0, scale=bf, n_cell=shape)
     out = []
     open_runs = {}          # (i0, i1) -> j0 of the run stack still growing
     for j in range(nbk[0] + 1):

@@ -409,6 +409,8 @@ int cmp_level_fill_from(cmp_level* level, double* U, const cmp_level* old, const
 typedef struct cmp_amr cmp_amr;
 
 int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out);
+/* The hierarchy refers to its levels until it is destroyed: destroy it before any of them (a regrid
+ * destroys the hierarchy, remakes the levels, then creates a new one). */
 int cmp_amr_destroy(cmp_amr* amr);
 /* Coarse cells of `level` whose faces border the next finer level (0 for the finest). */
 int cmp_amr_info(const cmp_amr* amr, int level, int* ncf_cells);

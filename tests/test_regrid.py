@@ -77,3 +77,30 @@ def test_nest_keeps_whole_blocks_inside_the_parent_with_a_margin():
     got = (m > 0).reshape(20, 8, 20, 8).all(axis=(1, 3))
     assert np.all(got[blk])
     assert np.all(~(m > 0) | want)
+
+
+def test_nest_in_3d_keeps_whole_blocks_inside_the_parent_with_a_margin():
+    """3D nesting: boxes of whole 4-cell blocks, every box 4 cells inside the parent union, no
+    overlap, at most 64 cells a side, and every requested block that lies inside kept."""
+    parent = [([8, 8, 8], [39, 39, 31]), ([40, 8, 8], [55, 23, 31])]
+    boxes = [([4, 4, 4], [47, 19, 27]), ([12, 20, 12], [27, 43, 23])]
+    n = 64
+    out = regrid.nest(boxes, parent, (n, n, n), 4, 4)
+    cov = np.zeros((n, n, n), bool)
+    for lo, hi in parent:
+        cov[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+    want = np.zeros((n, n, n), bool)
+    for lo, hi in boxes:
+        want[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] = True
+    m = np.zeros((n, n, n), int)
+    assert out
+    for lo, hi in out:
+        assert all(lo[d] % 4 == 0 and (hi[d] + 1) % 4 == 0 and hi[d] - lo[d] + 1 <= 64 for d in range(3))
+        assert cov[lo[2] - 4:hi[2] + 5, lo[1] - 4:hi[1] + 5, lo[0] - 4:hi[0] + 5].all(), (lo, hi)
+        m[lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1] += 1
+    assert m.max() == 1
+    inner = ~regrid._grow(~cov, 4)
+    blk = (inner & want).reshape(16, 4, 16, 4, 16, 4).all(axis=(1, 3, 5))
+    got = (m > 0).reshape(16, 4, 16, 4, 16, 4).all(axis=(1, 3, 5))
+    assert np.all(got[blk])
+    assert np.all(~(m > 0) | want)
