@@ -110,6 +110,7 @@ def uneven_level(args):
     dt = torch.zeros(1, dtype=torch.float64, device=dev)
     cmax = torch.zeros(1, dtype=torch.float64, device=dev)
     lev = Level(P, G, lay)
+    tune_s = tune_level(lev, Un, args)
     s = torch.cuda.Stream()
     with torch.cuda.stream(s):
         lev.advance(Un, U1, U2, W, c["cfl"], dt, cmax, use_graph=True, resume=False, stream=s)
@@ -154,6 +155,7 @@ def uneven_level(args):
     same = all(np.array_equal(hb[int(offs[b]):int(offs[b + 1])],
                               lay.fab_view(hl, b)[:, 3:-3, 3:-3].ravel())
                for b in range(len(c["boxes"])))
+    tiles_rec = tiles_record([lev], tune_s)
     lev.close()
     flops = roofline.flops_per_cell_stage(P) * cells
     rec = {
@@ -175,6 +177,7 @@ def uneven_level(args):
         "rhs_ms": {"level_one_launch": times["level"], "per_box_launches": times["per-box"]},
         "batched_over_per_box": times["per-box"] / times["level"],
         "rhs_identical": bool(same),
+        "tiles": tiles_rec,
         "roofline": {"bound": "valu", "kernel": "k_stage (fill + right-hand side of the level)",
                      "achieved": flops / (times["level"] * 1e-3) / 1e12, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": flops / (times["level"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
@@ -214,7 +217,8 @@ def shock_bubble_amr(args):
     pf = _amr_pf(args)
     init = [U0.cpu().numpy().copy(), U1f.cpu().numpy().copy()]   # the initial hierarchy, for the CPU baseline
     h = RegridHierarchy(P, [c0["geom"], c1["geom"]], [L0, L1], [U0, U1f], 2, criteria=crit, blocking_factor=8,
-                        max_grid_size=64, pf=pf)
+                        max_grid_size=64, pf=pf, tiles=None if args.tiles == "fixed" else args.tiles,
+                        tile_shapes=_bf_shapes())
     cells = h.cells()
     K = args.steps if args.steps is not None else 50
     W = args.warmup
@@ -230,13 +234,17 @@ def shock_bubble_amr(args):
     assert h.check_errors() == 0
     regrid_rec = {"regrid_int": args.regrid_int, "regrids_in_timed_steps": h.stats["regrids"] - st0["regrids"],
                   "tag_and_cluster_s": h.stats["tag_and_cluster_s"] - st0["tag_and_cluster_s"],
-                  "remake_and_rebuild_s": h.stats["remake_s"] - st0["remake_s"], "cells_per_level_at_end": h.cells()}
+                  "remake_and_rebuild_s": h.stats["remake_s"] - st0["remake_s"],
+                  "tile_tuning_s": h.stats["tune_s"] - st0["tune_s"], "cells_per_level_at_end": h.cells()}
+    tiles_rec = tiles_record(h.levels, st0["tune_s"], h.amr)
     pf_rec = _amr_pf_split(h.amr, h.Us, h.S1, h.S2, h.dt, pf)
     h.close()
     # the uniform fine grid the hierarchy stands for, single-level path (graph-replayed steps)
     lu = Layout(2, c1["boxes"], 3, P.nstate)
     Lu = Level(P, c1["geom"], lu)
     Uu = torch.from_numpy(lu.pack(c1["U"])).to(dev)
+    tune_level(Lu, Uu, args)
+    uniform_tiles = tiles_record([Lu])["levels"][0]
     if pf is not None:
         Lu.pf_setup(pf)
     Vu1, Vu2 = torch.zeros_like(Uu), torch.zeros_like(Uu)
@@ -278,6 +286,8 @@ def shock_bubble_amr(args):
                      "traffic": None, "flops_per_cell_stage": fps},
     }
     rec["regrid"] = regrid_rec
+    rec["tiles"] = tiles_rec
+    rec["uniform_fine_grid"]["tiles"] = {k: uniform_tiles[k] for k in ("work_list_kernel", "waves_per_simd")}
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
     if not args.no_cpu_baseline:
@@ -289,6 +299,71 @@ def shock_bubble_amr(args):
         rec["cpu_baseline"]["note"] = ("flow steps only: the oracle's composite implicit Grad_Q is Jacobi-preconditioned "
                                        "CG, not the reference's MLMG, so its time says nothing about the reference")
     print(json.dumps(rec), flush=True)
+
+
+def _tiles_force(args):
+    return {"auto": 0, "list": 1, "fixed": -1}[args.tiles]
+
+
+def _bf_shapes(bf=8, mgs=64):
+    """Every box shape a regrid with blocking factor bf and max_grid_size mgs can make (2D)."""
+    return [(i, j) for i in range(bf, mgs + 1, bf) for j in range(bf, mgs + 1, bf)]
+
+
+def tiles_record(levels, tune_s=None, amr=None):
+    """The tile choice of every level for the bench line: work-list kernel on, its occupancy target,
+    and per box shape (the 8 holding the most cells) the chosen tile and the swept times (ns per
+    cell) of the fixed kernel and of that tile; the swept sums over the level's cells."""
+    import numpy as np
+    out = []
+    for lv in levels:
+        on, waves, ext = lv.tile_shapes()
+        cells, chosen = {}, {}
+        for (lo, hi), e in zip(lv.layout.boxes, ext):
+            n = tuple(h - l + 1 for l, h in zip(lo, hi))
+            cells[n] = cells.get(n, 0) + int(np.prod(n))
+            chosen[n] = tuple(e[:len(n)])
+        t_fixed = t_chosen = 0.0
+        top = {}
+        swept = True
+        for n in sorted(cells, key=lambda k: -cells[k]):
+            t = lv.tile_tuning(n)
+            if t is None:
+                swept = False
+                continue
+            ns = t["fixed"]
+            if on:
+                w = 1 if waves == 3 else 0
+                ns = min((c[1 + w] for c in t["cand"] if c[0][:len(n)] == chosen[n]), default=t["fixed"])
+            t_fixed += cells[n] * t["fixed"]
+            t_chosen += cells[n] * ns
+            if len(top) < 8:
+                top["x".join(map(str, n))] = {"tile": "x".join(map(str, chosen[n])),
+                                              "ns_per_cell_fixed": round(t["fixed"], 4),
+                                              "ns_per_cell_tile": round(ns, 4)}
+        out.append({"work_list_kernel": on, "waves_per_simd": waves if on else None,
+                    "box_shapes": len(cells), "swept": swept, "by_box_shape": top,
+                    "swept_stage_ms_fixed": t_fixed * 1e-6 if swept else None,
+                    "swept_stage_ms_chosen": t_chosen * 1e-6 if swept else None})
+    rec = {"levels": out}
+    if tune_s is not None:
+        rec["tune_s"] = tune_s
+    if amr is not None:
+        on, waves = amr.tile_info()
+        rec["hierarchy_one_launch"] = on
+        rec["hierarchy_waves_per_simd"] = waves if on else None
+    return rec
+
+
+def tune_level(lev, U, args, extra=()):
+    """The tile autotuner on a level before the timed steps; seconds it took."""
+    import torch
+    if args.tiles == "fixed":
+        return 0.0
+    t = time.perf_counter()
+    lev.tune_tiles(U, extra_shapes=extra, force=_tiles_force(args))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t
 
 
 def _amr_pf(args):
@@ -366,7 +441,7 @@ def nphase_rings_amr(args):
     init = [u.cpu().numpy().copy() for u in Us] if not args.no_cpu_baseline else None
     pf = _amr_pf(args)
     h = RegridHierarchy(P, geoms, levels, Us, 2, criteria=crit, blocking_factor=8, max_grid_size=64, nest_margin=4,
-                        pf=pf)
+                        pf=pf, tiles=None if args.tiles == "fixed" else args.tiles, tile_shapes=_bf_shapes())
     cells = h.cells()
     K = args.steps if args.steps is not None else 50
     W = args.warmup
@@ -382,7 +457,9 @@ def nphase_rings_amr(args):
     assert h.check_errors() == 0
     regrid_rec = {"regrid_int": args.regrid_int, "regrids_in_timed_steps": h.stats["regrids"] - st0["regrids"],
                   "tag_and_cluster_s": h.stats["tag_and_cluster_s"] - st0["tag_and_cluster_s"],
-                  "remake_and_rebuild_s": h.stats["remake_s"] - st0["remake_s"], "cells_per_level_at_end": h.cells()}
+                  "remake_and_rebuild_s": h.stats["remake_s"] - st0["remake_s"],
+                  "tile_tuning_s": h.stats["tune_s"] - st0["tune_s"], "cells_per_level_at_end": h.cells()}
+    tiles_rec = tiles_record(h.levels, st0["tune_s"], h.amr)
     pf_rec = _amr_pf_split(h.amr, h.Us, h.S1, h.S2, h.dt, pf)
     h.close()
     del Us
@@ -393,6 +470,8 @@ def nphase_rings_amr(args):
     Lu = Level(P, cu["geom"], lu)
     Uu = torch.from_numpy(lu.pack(cu["U"])).to(dev)
     del cu
+    tune_level(Lu, Uu, args)
+    uniform_tiles = tiles_record([Lu])["levels"][0]
     if pf is not None:
         Lu.pf_setup(pf)
     Vu1, Vu2 = torch.zeros_like(Uu), torch.zeros_like(Uu)
@@ -435,6 +514,8 @@ def nphase_rings_amr(args):
                      "traffic": None, "flops_per_cell_stage": fps},
     }
     rec["regrid"] = regrid_rec
+    rec["tiles"] = tiles_rec
+    rec["uniform_finest_grid"]["tiles"] = {k: uniform_tiles[k] for k in ("work_list_kernel", "waves_per_simd")}
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
     if init is not None:
@@ -650,7 +731,7 @@ def cpu_baseline(sample_all, sample_one, runs=5, native=True):
     return rec
 
 
-def measure_traffic(workload_name, kernel_prefix, calib_mib=512):
+def measure_traffic(workload_name, kernel_prefix, calib_mib=512, extra=()):
     """HBM bytes per launch of the dominant kernel from two rocprofv3 counter runs of this script
     (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for
     an in-place scaling of calib_mib MiB (reads and writes each byte once; a kernel, where torch's
@@ -668,7 +749,7 @@ def measure_traffic(workload_name, kernel_prefix, calib_mib=512):
         return None
     child = [sys.executable, os.path.abspath(__file__), "--workload", workload_name, "--steps", "1",
              "--warmup", "1", "--no-cpu-baseline", "--kernel-reps", "2",
-             "--copy-calibration-mib", str(calib_mib), "--no-traffic"]
+             "--copy-calibration-mib", str(calib_mib), "--no-traffic"] + list(extra)
     env = dict(os.environ, TMPDIR="/tmp")
     got = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -719,6 +800,12 @@ def main():
     ap.add_argument("--regrid-int", type=int, default=10,
                     help="shock-bubble-amr, nphase-rings-amr: regrid every this many steps inside the timed run "
                          "(amr.regrid_int; 0: the hierarchy fixed)")
+    ap.add_argument("--tiles", choices=("auto", "fixed", "list"), default="auto",
+                    help="the tile autotuner before the timed steps: auto (the swept times choose between the "
+                         "fixed-shape kernel and the persistent work-list kernel with per-box-shape tiles), "
+                         "fixed, or list (the work-list kernel with the swept shapes)")
+    ap.add_argument("--tile-cache", default=None,
+                    help="load the tile autotuner's cache from this file first (cmp_tile_cache_load)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter runs that measure the dominant kernel's HBM traffic")
     args = ap.parse_args()
@@ -733,6 +820,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
+    if args.tile_cache and os.path.exists(args.tile_cache):
+        from compas_amd import tile_cache_load
+        tile_cache_load(args.tile_cache)
     if args.workload == "served-stream":
         served(args)
         return
@@ -763,6 +853,8 @@ def main():
     dt = torch.zeros(1, dtype=torch.float64, device=dev)
     cmax = torch.zeros(1, dtype=torch.float64, device=dev)
     lev = Level(P, G, lay)
+    tune_s = tune_level(lev, Un, args)
+    tiles_on, tiles_waves, _ = lev.tile_shapes()
     if c["pf"] is not None:
         lev.pf_setup(c["pf"])
     stream = torch.cuda.Stream()
@@ -838,8 +930,9 @@ def main():
         "config": c["config"],
         "roofline": {
             "bound": "valu",
-            "kernel": "k_stage (fused %s + %s + divergence + RK3 stage)" % (c["config"]["scheme"],
-                                                                            c["config"]["riemann"]),
+            "kernel": ("k_stage_list (persistent work-list, the autotuner's tile per box shape, %d waves/SIMD; "
+                       % tiles_waves if tiles_on else "k_stage (")
+                      + "fused %s + %s + divergence + RK3 stage)" % (c["config"]["scheme"], c["config"]["riemann"]),
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -851,13 +944,21 @@ def main():
         },
         "event_ms_per_step": ev_ms / K,
     }
+    rec["tiles"] = tiles_record([lev], tune_s)
     if pf_rec is not None:
         rec["phase_field"] = pf_rec
     if not args.no_traffic:
         scheme = {(1, 5): 5, (1, 6): 6, (0, 0): 0}.get((P.id_weno, P.id_limiter), 1)
-        prefix = "void cmp::k_stage<%d, %d, %d, %s, %d>" % (P.dim, scheme, P.id_riemann,
-                                                             "true" if P.id_thinc else "false", P.nphase)
-        tr = measure_traffic(args.workload, prefix)
+        targs = "%d, %d, %d, %s, %d" % (P.dim, scheme, P.id_riemann, "true" if P.id_thinc else "false", P.nphase)
+        prefix = ("void cmp::k_stage_list<%s, %d>" % (targs, tiles_waves) if tiles_on
+                  else "void cmp::k_stage<%s>" % targs)
+        # the counter runs take this run's tile choice from its cache (no sweep of their own)
+        import tempfile
+        from compas_amd import tile_cache_save
+        cache = os.path.join(tempfile.mkdtemp(prefix="cmp_tiles_", dir="/tmp"), "tiles.txt")
+        tile_cache_save(cache)
+        tr = measure_traffic(args.workload, prefix,
+                             extra=["--tiles", "list" if tiles_on else "fixed", "--tile-cache", cache])
         if tr is not None:
             rec["roofline"]["traffic"] = tr["bytes"]
             rec["roofline"]["traffic_detail"] = {

@@ -243,6 +243,19 @@ def _stream(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+def tile_cache_save(path):
+    """The tile autotuner's cache as a text file."""
+    check(capi.load().cmp_tile_cache_save(str(path).encode()), "cmp_tile_cache_save")
+
+
+def tile_cache_load(path):
+    check(capi.load().cmp_tile_cache_load(str(path).encode()), "cmp_tile_cache_load")
+
+
+def tile_cache_clear():
+    check(capi.load().cmp_tile_cache_clear(), "cmp_tile_cache_clear")
+
+
 class Level:
     """One level's boxes on the GPU. Buffers are torch CUDA tensors owned by the caller."""
 
@@ -353,6 +366,45 @@ class Level:
         check(self.lib.cmp_level_info(self.h, ctypes.byref(n), ctypes.byref(x), ctypes.byref(y),
                                       ctypes.byref(z)), "cmp_level_info")
         return {"ntiles": n.value, "tile": (x.value, y.value, z.value)}
+
+    # ---- the persistent work-list stage kernel and the tile autotuner ------------------------------
+    def set_tiles(self, ext, waves=2):
+        """The TVD-RK3 flow stage through the work-list kernel, box b in tiles of ext[b] (3 ints per
+        box; one triple for every box), occupancy target `waves`; ext None: the fixed-shape kernel."""
+        if ext is None:
+            check(self.lib.cmp_level_set_tiles(self.h, None, 0), "cmp_level_set_tiles")
+            return
+        e = [int(v) for v in ext] if len(ext) != 3 else [int(v) for v in ext] * self.layout.nbox
+        arr = (ctypes.c_int * (3 * self.layout.nbox))(*e)
+        check(self.lib.cmp_level_set_tiles(self.h, arr, int(waves)), "cmp_level_set_tiles")
+
+    def tile_shapes(self):
+        """(work-list kernel on, its occupancy target, [tile extents of each box])"""
+        on, w = ctypes.c_int(), ctypes.c_int()
+        arr = (ctypes.c_int * (3 * self.layout.nbox))()
+        check(self.lib.cmp_level_tile_shapes(self.h, ctypes.byref(on), ctypes.byref(w), arr),
+              "cmp_level_tile_shapes")
+        return bool(on.value), w.value, [tuple(arr[3 * b:3 * b + 3]) for b in range(self.layout.nbox)]
+
+    def tune_tiles(self, U, extra_shapes=(), force=0, stream=None):
+        """The measured tile-shape sweep for this level's box shapes (and extra_shapes), then the
+        level's choice (force 1 / -1: work-list kernel always / never)."""
+        ex = [int(v) for sh in extra_shapes for v in (list(sh) + [1, 1])[:3]]
+        arr = (ctypes.c_int * max(1, len(ex)))(*ex)
+        check(self.lib.cmp_level_tune_tiles(self.h, _ptr(U), len(ex) // 3, arr if ex else None, int(force),
+                                            _stream(stream)), "cmp_level_tune_tiles")
+
+    def tile_tuning(self, shape):
+        """The cached sweep of one box shape: {"fixed": ns per cell, "cand": [(ext, ns at 2 waves,
+        ns at 3 waves)]}, or None when not swept."""
+        n = (ctypes.c_int * 3)(*(list(shape) + [1, 1])[:3])
+        nf, nc = ctypes.c_double(), ctypes.c_int()
+        ce = (ctypes.c_int * 36)()
+        cn = (ctypes.c_double * 24)()
+        if self.lib.cmp_tile_tuning(self.h, n, ctypes.byref(nf), ctypes.byref(nc), ce, cn) != 0:
+            return None
+        return {"fixed": nf.value,
+                "cand": [(tuple(ce[3 * k:3 * k + 3]), cn[2 * k], cn[2 * k + 1]) for k in range(nc.value)]}
 
     # ---- Phase-Field regularization ----------------------------------------------------------
     def pf_setup(self, pf):
@@ -483,6 +535,19 @@ class Amr:
     @staticmethod
     def _arr(ts):
         return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+    def tiles(self, force=0):
+        """Every level's flow stage in one work-list launch (force 1 on, -1 off, 0 by the tuned times);
+        returns (on, occupancy target)."""
+        on, w = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.cmp_amr_tiles(self.h, int(force), ctypes.byref(on), ctypes.byref(w)), "cmp_amr_tiles")
+        return bool(on.value), w.value
+
+    def tile_info(self):
+        """(one work-list launch for every level, its occupancy target), changing nothing."""
+        on, w = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.cmp_amr_tile_info(self.h, ctypes.byref(on), ctypes.byref(w)), "cmp_amr_tile_info")
+        return bool(on.value), w.value
 
     def cf_cells(self, level):
         n = ctypes.c_int()

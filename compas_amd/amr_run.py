@@ -5,7 +5,10 @@ again from the GPU tags of the level below: the tags are clustered on the host (
 nested inside that level's boxes, refined, and the new level takes its old data where its old boxes
 hold it and the coarse level's conservative interpolation elsewhere (regrid.remake_level); then the
 levels are averaged down and the hierarchy (cmp_amr, its step graph and, with the Phase-Field step
-on, its Phase-Field buffers) is built again.
+on, its Phase-Field buffers) is built again. With `tiles`, every level takes the tile autotuner's
+choice before the hierarchy is built (cmp_level_tune_tiles: box shapes not swept yet are swept
+then, so a run that pre-sweeps the shapes its regrids can make, `tile_shapes`, sweeps nothing
+later), and the hierarchy its single work-list launch when the swept times favour it.
 """
 import time
 
@@ -20,14 +23,17 @@ class RegridHierarchy:
     level being made; nest_margin in the cells of the level below it."""
 
     def __init__(self, params, geoms, levels, Us, ratio=2, criteria=None, blocking_factor=8, max_grid_size=64,
-                 grid_eff=0.7, n_error_buf=1, nest_margin=4, pf=None):
+                 grid_eff=0.7, n_error_buf=1, nest_margin=4, pf=None, tiles=None, tile_shapes=()):
         import torch
         self.P, self.geoms, self.levels, self.Us = params, list(geoms), list(levels), list(Us)
         self.ratio, self.criteria = ratio, criteria
         self.bf, self.mgs, self.eff, self.nbuf, self.margin = blocking_factor, max_grid_size, grid_eff, n_error_buf, nest_margin
         self.pf = pf
+        # tile autotuner: None / "fixed" off, "auto" by the swept times, "list" the work-list kernel
+        self.tiles = tiles
+        self.tile_shapes = list(tile_shapes)
         self.amr = None
-        self.stats = dict(regrids=0, tag_and_cluster_s=0.0, remake_s=0.0)
+        self.stats = dict(regrids=0, tag_and_cluster_s=0.0, remake_s=0.0, tune_s=0.0)
         self.updates = 0     # cell-updates of every step advanced (the cells of the hierarchy at that step)
         self.dt = torch.zeros(1, dtype=torch.float64, device=self.Us[0].device)
         self._build()
@@ -36,7 +42,15 @@ class RegridHierarchy:
         import torch
         if self.amr is not None:
             self.amr.close()
+        force = {"auto": 0, "list": 1}.get(self.tiles)
+        if force is not None:
+            t = time.perf_counter()
+            for lv, U in zip(self.levels, self.Us):
+                lv.tune_tiles(U, extra_shapes=self.tile_shapes, force=force)
+            self.stats["tune_s"] += time.perf_counter() - t
         self.amr = Amr(self.levels, self.ratio)
+        if force == 1:
+            self.amr.tiles(1)
         if self.pf is not None:
             self.amr.pf_setup(self.pf)
         self.S1 = [torch.zeros_like(u) for u in self.Us]

@@ -41,6 +41,8 @@ def build(verbose=False):
     units = [
         ("cmp_kernels.hip", "k2.o", ["-DCMP_KERNEL_DIM=2"]),
         ("cmp_kernels.hip", "k3.o", ["-DCMP_KERNEL_DIM=3"]),
+        ("cmp_kernels.hip", "l2.o", ["-DCMP_KERNEL_DIM=2", "-DCMP_LIST_UNIT"]),
+        ("cmp_kernels.hip", "l3.o", ["-DCMP_KERNEL_DIM=3", "-DCMP_LIST_UNIT"]),
         ("cmp_pf.hip", "pf2.o", ["-DCMP_KERNEL_DIM=2"]),
         ("cmp_pf.hip", "pf3.o", ["-DCMP_KERNEL_DIM=3"]),
         ("cmp_mg.hip", "mg2.o", ["-DCMP_KERNEL_DIM=2"]),
@@ -50,6 +52,7 @@ def build(verbose=False):
         ("cmp_capi.hip", "capi.o", []),
         ("cmp_batch.hip", "batch.o", []),
         ("cmp_amr.hip", "amr.o", []),
+        ("cmp_tune.hip", "tune.o", []),
     ]
     procs = []
     for src, obj, extra in units:

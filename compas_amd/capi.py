@@ -147,6 +147,14 @@ SIGNATURES = {
     "cmp_level_check": (ctypes.c_int, [vp, vp]),
     "cmp_level_info": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                       P(ctypes.c_int)]),
+    "cmp_level_set_tiles": (ctypes.c_int, [vp, P(ctypes.c_int), ctypes.c_int]),
+    "cmp_level_tile_shapes": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]),
+    "cmp_level_tune_tiles": (ctypes.c_int, [vp, dp, ctypes.c_int, P(ctypes.c_int), ctypes.c_int, vp]),
+    "cmp_tile_tuning": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_int),
+                                       P(ctypes.c_double)]),
+    "cmp_tile_cache_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "cmp_tile_cache_load": (ctypes.c_int, [ctypes.c_char_p]),
+    "cmp_tile_cache_clear": (ctypes.c_int, []),
     "cmp_face_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(cmp_box), ctypes.c_int,
                                        P(ctypes.c_longlong), P(ctypes.c_longlong)]),
     "cmp_abi_pf_size": (ctypes.c_int, [P(ctypes.c_int)]),
@@ -169,6 +177,8 @@ SIGNATURES = {
     "cmp_amr_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]),
     "cmp_amr_destroy": (ctypes.c_int, [vp]),
     "cmp_amr_info": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "cmp_amr_tiles": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int)]),
+    "cmp_amr_tile_info": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int)]),
     "cmp_amr_initial_cmax": (ctypes.c_int, [vp, ctypes.POINTER(dp), dp, vp]),
     "cmp_amr_compute_dt": (ctypes.c_int, [vp, dp, dp, ctypes.c_double, ctypes.c_int, vp]),
     "cmp_amr_rk3_step": (ctypes.c_int, [vp, ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), dp, dp,

@@ -99,6 +99,13 @@ struct cmp_amr {
     double gcfl = 0.0;
     int glimiter = -1;
     AmrPf pf;
+    // the flow stage of every level in one launch of the persistent work-list kernel (the levels'
+    // tuned tile shapes; ghost cells read as filled by the level and two-level fills)
+    bool use_list = false;
+    int list_waves = 2, list_grid = 0, nltiles = 0;
+    ListTile* d_ltiles = nullptr;
+    ListLevel* d_llev = nullptr;
+    int* d_lctr = nullptr;
 };
 
 namespace {
@@ -225,6 +232,9 @@ void amr_free(cmp_amr* A)
     for (double* p : A->d_faces)
         if (p) (void)hipFree(p);
     if (A->d_step) (void)hipFree(A->d_step);
+    void* lb[] = {A->d_ltiles, A->d_llev, A->d_lctr};
+    for (void* p : lb)
+        if (p) (void)hipFree(p);
     if (A->gexec) (void)hipGraphExecDestroy(A->gexec);
     if (A->graph) (void)hipGraphDestroy(A->graph);
     if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
@@ -236,6 +246,63 @@ void amr_free(cmp_amr* A)
     for (hipEvent_t e : A->ev_done)
         if (e) (void)hipEventDestroy(e);
     delete A;
+}
+
+// The hierarchy's work-list from the levels' cached tile sweeps (force: 1 always, -1 never, 0 when
+// the summed times over every level's boxes favour it): the occupancy target of least summed time,
+// every level's tiles with their tuned shapes in one list, one table entry per level.
+int amr_tiles(cmp_amr* A, int force)
+{
+    const int nlev = (int)A->L.size();
+    A->use_list = false;
+    A->gkey.clear();   // a captured step is captured again
+    if (force < 0 || nlev > LIST_MAXLEV) return force > 0 ? CMP_ERR_UNSUPPORTED : CMP_OK;
+    const cmp_level* L0 = A->L[0];
+    if (L0->lc.riemann != 1 && L0->lc.riemann != 2) return force > 0 ? CMP_ERR_UNSUPPORTED : CMP_OK;
+    std::vector<std::vector<int>> ext[2];
+    double t_list[2] = {0.0, 0.0}, t_fixed = 0.0;
+    for (int l = 0; l < nlev; ++l) {
+        std::vector<int> e[2];
+        double tl[2], tf;
+        const int r = level_tuned(A->L[l], e, tl, &tf);
+        if (r) return (force > 0) ? r : CMP_OK;
+        for (int w = 0; w < 2; ++w) {
+            t_list[w] += tl[w];
+            ext[w].push_back(std::move(e[w]));
+        }
+        t_fixed += tf;
+    }
+    const int w = (t_list[1] < t_list[0]) ? 1 : 0;
+    if (force == 0 && !(t_list[w] < t_fixed)) return CMP_OK;
+    std::vector<ListTile> ht;
+    std::vector<ListLevel> lv(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        cmp_level* L = A->L[l];
+        // the level's own list choice is left as it was: its box_ext is borrowed for the build
+        std::vector<int> keep = std::move(L->box_ext);
+        L->box_ext = std::move(ext[w][l]);
+        level_list_tiles(L, l, ht);
+        L->box_ext = std::move(keep);
+        lv[l] = ListLevel{L->kp, L->d_boxes, nullptr, L->d_err};
+    }
+    for (void* p : {(void*)A->d_ltiles, (void*)A->d_llev})
+        if (p) CMP_TRY(hipFree(p));
+    A->d_ltiles = nullptr;
+    A->d_llev = nullptr;
+    CMP_TRY(hipMalloc(&A->d_ltiles, sizeof(ListTile) * std::max<size_t>(1, ht.size())));
+    CMP_TRY(hipMemcpy(A->d_ltiles, ht.data(), sizeof(ListTile) * ht.size(), hipMemcpyHostToDevice));
+    CMP_TRY(hipMalloc(&A->d_llev, sizeof(ListLevel) * nlev));
+    CMP_TRY(hipMemcpy(A->d_llev, lv.data(), sizeof(ListLevel) * nlev, hipMemcpyHostToDevice));
+    if (!A->d_lctr) {
+        CMP_TRY(hipMalloc(&A->d_lctr, sizeof(int) * LIST_NCTR));
+        CMP_TRY(hipMemset(A->d_lctr, 0, sizeof(int) * LIST_NCTR));
+    }
+    A->list_waves = (w == 1) ? 3 : 2;
+    A->list_grid = list_capacity(L0->lc, A->list_waves);
+    if (A->list_grid <= 0) return CMP_ERR_UNSUPPORTED;
+    A->nltiles = (int)ht.size();
+    A->use_list = true;
+    return CMP_OK;
 }
 
 }  // namespace
@@ -311,8 +378,27 @@ int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out)
             if (!r) r = check_hip(hipEventCreateWithFlags(&A->ev_done[l], hipEventDisableTiming));
         }
     }
+    if (!r) r = amr_tiles(A, 0);
     if (r) { amr_free(A); return r; }
     *out = A;
+    return CMP_OK;
+}
+
+int cmp_amr_tiles(cmp_amr* A, int force, int* use_list, int* waves)
+{
+    if (!A || force < -1 || force > 1) return CMP_ERR_ARG;
+    const int r = amr_tiles(A, force);
+    if (r) return r;
+    if (use_list) *use_list = A->use_list ? 1 : 0;
+    if (waves) *waves = A->use_list ? A->list_waves : 0;
+    return CMP_OK;
+}
+
+int cmp_amr_tile_info(const cmp_amr* A, int* use_list, int* waves)
+{
+    if (!A) return CMP_ERR_ARG;
+    if (use_list) *use_list = A->use_list ? 1 : 0;
+    if (waves) *waves = A->use_list ? A->list_waves : 0;
     return CMP_OK;
 }
 
@@ -371,7 +457,8 @@ int cmp_amr_compute_dt(cmp_amr* A, double* cmax, double* dt_dev, double cfl, int
 // level's own fill leaves alone) and stage kernel, then the coarse-fine face jobs between it and the
 // next coarser level (after that level's fill); back on the caller's stream, the corrections (after
 // level 0's stage kernel and every face job) and the average-down (after every stage kernel),
-// finest first.
+// finest first. With the work-list kernel (A->use_list) every level's stage is one launch on the
+// caller's stream after all the fills, and the face jobs run beside it on the finer levels' streams.
 // The stages of one TVD-RK3 step of the hierarchy: the flow's (pf false: c_max accumulated into
 // cmax[l]) or the Phase-Field sub-step's (pf true: every level's right-hand side the Phase-Field
 // flux at its Grad_Q, the levels' d_gq; no c_max).
@@ -413,6 +500,39 @@ static int amr_stages(cmp_amr* A, double* const* Un, double* const* U1, double* 
         }
         CMP_TRY(hipEventRecord(A->ev_fork, s));
         for (int l = 1; l < nlev; ++l) CMP_TRY(hipStreamWaitEvent(A->q[l - 1], A->ev_fork, 0));
+        if (!pf && A->use_list) {
+            // every level's fill (the finer ones on their streams), then the face jobs on those
+            // streams beside one work-list launch for every level's tiles on the caller's stream
+            if ((r = cmp_level_fill_ghosts(A->L[0], src[0], s))) return r;
+            CMP_TRY(hipEventRecord(A->ev_fill[0], s));
+            for (int l = 1; l < nlev; ++l) {
+                hipStream_t q = A->q[l - 1];
+                if ((r = cmp_level_fill_two_levels(A->L[l], src[l], A->L[l - 1], src[l - 1], A->ratio, q))) return r;
+                CMP_TRY(hipEventRecord(A->ev_fill[l], q));
+            }
+            for (int l = 1; l < nlev; ++l) {
+                hipStream_t q = A->q[l - 1];
+                CMP_TRY(hipStreamWaitEvent(q, A->ev_fill[l - 1], 0));
+                if ((r = cf(l - 1, 1, stage, src, dst, q))) return r;
+                CMP_TRY(hipEventRecord(A->ev_done[l], q));
+                CMP_TRY(hipStreamWaitEvent(s, A->ev_fill[l], 0));
+            }
+            ListCall c{};
+            for (int l = 0; l < nlev; ++l) {
+                c.Us[l] = src[l];
+                c.Un[l] = Un[l];
+                c.Uout[l] = dst[l];
+                c.cmax[l] = (unsigned long long*)(cmax + l);
+            }
+            CMP_TRY(launch_stage_list(A->L[0]->lc, A->list_waves, A->d_llev, A->d_ltiles, A->nltiles, A->d_lctr, c,
+                                      MODE_RK, stage, dt_dev, A->list_grid, s));
+            for (int l = 1; l < nlev; ++l) CMP_TRY(hipStreamWaitEvent(s, A->ev_done[l], 0));
+            for (int l = 0; l + 1 < nlev; ++l)
+                if ((r = cf(l, 2, stage, src, dst, s))) return r;
+            for (int l = nlev - 2; l >= 0; --l)
+                if ((r = cmp_level_average_down(A->L[l + 1], dst[l + 1], A->L[l], dst[l], A->ratio, s))) return r;
+            continue;
+        }
         // level 0
         if ((r = cmp_level_fill_ghosts(A->L[0], src[0], s))) return r;
         CMP_TRY(hipEventRecord(A->ev_fill[0], s));

@@ -570,6 +570,57 @@ int cmp_pf_rhs_box(const cmp_params* params, const cmp_pf_params* pf, const cmp_
     return CMP_OK;
 }
 
+}  // extern "C"
+
+// The halo sources of tile [tlo, thi] of box b on its box's faces (the ghost fill's rules, resolved
+// per tile side): the ng cells beyond the face, across the tile's transverse extent, lie in one box
+// (after the periodic wrap), or beyond a non-periodic domain face whose BC clamps (codes 0, 2) or
+// mirrors (3, 4: normal momentum negated; 5: all momenta) within the tile's own box; anything else
+// is looked up cell by cell. Sides that are not on the box's face keep their entry.
+void cmp::tile_halo(int dim, const cmp_geometry* geom, const cmp_box* boxes, int b, const int* tlo, const int* thi,
+                    int ng, const std::vector<int>& owner, const int* nb, long long bs, int (*halo)[2])
+{
+    auto owner_of = [&](const int* c) -> int {
+        long long q = 0;
+        for (int d = dim - 1; d >= 0; --d) q = q * nb[d] + (c[d] - geom->domain_lo[d]) / bs;
+        return owner[q];
+    };
+    const cmp_box& B = boxes[b];
+    for (int d = 0; d < dim; ++d)
+        for (int side = 0; side < 2; ++side) {
+            if ((side == 0 && tlo[d] != B.lo[d]) || (side == 1 && thi[d] != B.hi[d])) continue;
+            const int dlo = geom->domain_lo[d], dhi = geom->domain_hi[d];
+            int rlo[3], rhi[3];
+            for (int e = 0; e < 3; ++e) {
+                rlo[e] = (e < dim) ? tlo[e] : 0;
+                rhi[e] = (e < dim) ? thi[e] : 0;
+            }
+            if (side == 0) { rlo[d] = B.lo[d] - ng; rhi[d] = B.lo[d] - 1; }
+            else { rlo[d] = B.hi[d] + 1; rhi[d] = B.hi[d] + ng; }
+            const bool below = rhi[d] < dlo, above = rlo[d] > dhi;
+            const bool inside = rlo[d] >= dlo && rhi[d] <= dhi;
+            int code = HALO_LOOKUP;
+            if (inside || ((below || above) && geom->is_periodic[d])) {
+                if (below) { rlo[d] += dhi - dlo + 1; rhi[d] += dhi - dlo + 1; }
+                if (above) { rlo[d] -= dhi - dlo + 1; rhi[d] -= dhi - dlo + 1; }
+                const int o = owner_of(rlo);
+                bool ok = o >= 0;
+                for (int e = 0; e < dim && ok; ++e)
+                    ok = rlo[e] >= boxes[o].lo[e] && rhi[e] <= boxes[o].hi[e];
+                if (ok) code = o;
+            } else if (below || above) {
+                const int bc = below ? geom->lo_bc[d] : geom->hi_bc[d];
+                const bool fits = B.hi[d] - B.lo[d] + 1 >= ng;
+                if (bc == 0 || bc == 2) code = HALO_CLAMP;
+                else if ((bc == 3 || bc == 4) && fits) code = HALO_MIRROR;
+                else if (bc == 5 && fits) code = HALO_MIRROR_ALL;
+            }
+            halo[d][side] = code;
+        }
+}
+
+extern "C" {
+
 int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbox,
                      const cmp_box* boxes, int ng, cmp_level** out)
 {
@@ -677,54 +728,13 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
                     owner[q] = b;
                 }
     }
-    // halo sources of the tiles on box faces (the ghost fill's rules, resolved per tile side): the
-    // ng cells beyond the face, across the tile's transverse extent, lie in one box (after the
-    // periodic wrap), or beyond a non-periodic domain face whose BC clamps (codes 0, 2) or
-    // mirrors (3, 4: normal momentum negated; 5: all momenta) within the tile's own box; anything
-    // else is looked up cell by cell
-    {
-        auto owner_of = [&](const int* c) -> int {
-            long long q = 0;
-            for (int d = dim - 1; d >= 0; --d) q = q * nb[d] + (c[d] - geom->domain_lo[d]) / bs;
-            return owner[q];
-        };
-        for (TileDev& t : ht) {
-            const cmp_box& B = boxes[t.box];
-            int tlo[3] = {0, 0, 0}, thi[3] = {0, 0, 0};
-            const int te[3] = {tx, ty, tz};
-            for (int d = 0; d < dim; ++d) {
-                tlo[d] = t.lo[d];
-                thi[d] = std::min(t.lo[d] + te[d] - 1, B.hi[d]);
-            }
-            for (int d = 0; d < dim; ++d)
-                for (int side = 0; side < 2; ++side) {
-                    if ((side == 0 && tlo[d] != B.lo[d]) || (side == 1 && thi[d] != B.hi[d])) continue;
-                    const int dlo = geom->domain_lo[d], dhi = geom->domain_hi[d];
-                    int rlo[3], rhi[3];
-                    for (int e = 0; e < 3; ++e) { rlo[e] = tlo[e]; rhi[e] = thi[e]; }
-                    if (side == 0) { rlo[d] = B.lo[d] - ng; rhi[d] = B.lo[d] - 1; }
-                    else { rlo[d] = B.hi[d] + 1; rhi[d] = B.hi[d] + ng; }
-                    const bool below = rhi[d] < dlo, above = rlo[d] > dhi;
-                    const bool inside = rlo[d] >= dlo && rhi[d] <= dhi;
-                    int code = HALO_LOOKUP;
-                    if (inside || ((below || above) && geom->is_periodic[d])) {
-                        if (below) { rlo[d] += dhi - dlo + 1; rhi[d] += dhi - dlo + 1; }
-                        if (above) { rlo[d] -= dhi - dlo + 1; rhi[d] -= dhi - dlo + 1; }
-                        const int o = owner_of(rlo);
-                        bool ok = o >= 0;
-                        for (int e = 0; e < dim && ok; ++e)
-                            ok = rlo[e] >= boxes[o].lo[e] && rhi[e] <= boxes[o].hi[e];
-                        if (ok) code = o;
-                    } else if (below || above) {
-                        const int bc = below ? geom->lo_bc[d] : geom->hi_bc[d];
-                        const bool fits = B.hi[d] - B.lo[d] + 1 >= ng;
-                        if (bc == 0 || bc == 2) code = HALO_CLAMP;
-                        else if ((bc == 3 || bc == 4) && fits) code = HALO_MIRROR;
-                        else if (bc == 5 && fits) code = HALO_MIRROR_ALL;
-                    }
-                    t.halo[d][side] = code;
-                }
-        }
+    // halo sources of the tiles on box faces
+    for (TileDev& t : ht) {
+        const cmp_box& B = boxes[t.box];
+        int thi[3] = {0, 0, 0};
+        const int te[3] = {tx, ty, tz};
+        for (int d = 0; d < dim; ++d) thi[d] = std::min(t.lo[d] + te[d] - 1, B.hi[d]);
+        tile_halo(dim, geom, boxes, t.box, t.lo, thi, ng, owner, nb, bs, t.halo);
     }
 
     int rc = CMP_OK;
@@ -871,7 +881,7 @@ int cmp_level_destroy(cmp_level* L)
     if (L->graph) (void)hipGraphDestroy(L->graph);
     if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
     void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff, L->d_U3,
-                    L->d_clock};
+                    L->d_clock, L->d_ltiles, L->d_llev, L->d_lctr};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete L;
@@ -927,10 +937,16 @@ int cmp_level_rk3_step(cmp_level* L, double* Un, double* U1, double* U2, const d
     hipStream_t s = (hipStream_t)stream;
     double* Us[3] = {Un, U1, U2};
     double* Uo[3] = {U1, U2, Un};
-    for (int stage = 0; stage < 3; ++stage)
+    for (int stage = 0; stage < 3; ++stage) {
+        if (L->use_list) {
+            const int r = level_list_stage(L, MODE_RK, stage, Us[stage], Un, Uo[stage], dt_dev, cmax, s);
+            if (r) return r;
+            continue;
+        }
         CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0,
                              Us[stage], Un, Uo[stage], dt_dev, (unsigned long long*)cmax, L->d_err, s, nullptr,
                              L->d_fd));
+    }
     return CMP_OK;
 }
 

@@ -40,6 +40,73 @@ struct Lds {
                                       DIM == 3 ? TX * TY * (TZ + 1) : 0);
 };
 
+// The tile shape the stage code is instantiated for. ShapeFixed: k_stage's one shape, whose
+// extents fold into the index arithmetic as constants. ShapeRT: a shape read per tile at run time
+// (k_stage_list), any TX x TY x TZ of at most NT cells whose staged region along every direction
+// fits LdsRT::RMAX cells and whose faces along every direction fit NT + LdsRT::XS (the faces beyond
+// NT take a second pass and LDS slots of their own); the host checks a shape against these bounds
+// (tile_shape_ok) before it enters a work-list.
+template <int DIM>
+struct ShapeFixed {
+    static constexpr bool RT = false;
+    __device__ __forceinline__ int t(int d) const
+    {
+        return d == 0 ? Tile<DIM>::X : (d == 1 ? Tile<DIM>::Y : Tile<DIM>::Z);
+    }
+};
+struct ShapeRT {
+    static constexpr bool RT = true;
+    int e[3];
+    __device__ __forceinline__ int t(int d) const { return e[d]; }
+};
+
+template <int DIM, int G>
+struct LdsRT {
+    // 2D: 32 cells across times a pencil of 8 + halo (32 x 8, 8 x 32, 16 x 16, ...); 3D: 64 cells
+    // across times a pencil of 4 + halo (8 x 8 x 4, 16 x 4 x 4, ...)
+    static constexpr int RMAX = (DIM == 2) ? 32 * (8 + 2 * G) : 64 * (4 + 2 * G);
+    static constexpr int XS = (DIM == 2) ? 32 : 64;
+};
+
+// LDS layout of a shape policy: RMAX cells of staged primitives per component, the first FS faces'
+// fluxes over them, XS more faces' fluxes after them.
+template <int DIM, int G, class SH> struct LdsOf;
+template <int DIM, int G>
+struct LdsOf<DIM, G, ShapeFixed<DIM>> {
+    static constexpr int RMAX = Lds<DIM, G>::RMAX, FS = Lds<DIM, G>::FMAX, XS = 0;
+};
+template <int DIM, int G>
+struct LdsOf<DIM, G, ShapeRT> {
+    static constexpr int RMAX = LdsRT<DIM, G>::RMAX, FS = NT, XS = LdsRT<DIM, G>::XS;
+};
+// doubles of a stage workgroup's LDS for N components: the staged region (or the main flux slots,
+// N fluxes and u*_n each), then the extra flux slots
+template <int N, class LO>
+constexpr int lds_doubles()
+{
+    return ((N * LO::RMAX > (N + 1) * LO::FS) ? N * LO::RMAX : (N + 1) * LO::FS) + (N + 1) * LO::XS;
+}
+
+// (x, y, z) of index r in a block of e0 x e1 x * positions. Run-time extents: the quotients by a
+// float reciprocal (rcp0 = 1/e0, rcp01 = 1/(e0 e1)): the exact quotient (r + 1/2)/e lies at least
+// 1/(2e) from an integer and the float product errs by less than 3e-7 of it, so truncation gives
+// the integer quotient for every r below 10^6.
+template <class SH>
+__device__ __forceinline__ void split3(int r, int e0, int e1, float rcp0, float rcp01, int& x, int& y, int& z)
+{
+    if constexpr (SH::RT) {
+        const float rh = (float)r + 0.5f;
+        const int q = (int)(rh * rcp0);
+        z = (int)(rh * rcp01);
+        x = r - q * e0;
+        y = q - z * e1;
+    } else {
+        x = r % e0;
+        y = (r / e0) % e1;
+        z = r / (e0 * e1);
+    }
+}
+
 // Maps a workgroup to a tile so that the workgroups one XCD receives (blocks b, b+8, b+16, ...
 // under round-robin dispatch) hold consecutive tiles, whose halos overlap in that XCD's L2.
 __device__ __forceinline__ int xcd_tile(int b, int nb)
@@ -207,30 +274,39 @@ __device__ __forceinline__ void read_halo(const StageCtx<DIM>& cx_, int side, in
 
 // One direction of the stage kernel (D known at compile time): stage the primitives of the tile
 // plus G cells on both sides along D, compute every face along D once, and accumulate the flux and
-// u*_n differences of the thread's cell.
-template <int DIM, int SCHEME, int RIEMANN, bool THINC, int D, int NP>
-__device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, double* sF, double* acc,
-                                         double& divu, double& cmax_loc, int& bad)
+// u*_n differences of the thread's cell. sh: the tile shape; sW / sF: the staged primitives and,
+// over them, the fluxes of the first NT faces; sX: the fluxes of the faces beyond NT.
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int D, int NP, class SH>
+__device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, const SH& sh, double* sW, double* sF, double* sX,
+                                         double* acc, double& divu, double& cmax_loc, int& bad)
 {
     constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
-    using L = Lds<DIM, G>;
-    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
+    using LO = LdsOf<DIM, G, SH>;
+    constexpr int RMAX = LO::RMAX, FS = LO::FS, XS = LO::XS;
     const KParams& kp = cx_.kp;
     const BoxDev& bx = cx_.bx;
     const int tid = cx_.tid;
     // Region (tile plus G cells on both sides along D) and face index spaces have the full tile's
-    // extents, known at compile time (divisions by constants, constant LDS strides); a tile clipped
-    // by its box leaves the positions beyond its cells unused.
-    constexpr int E0 = L::TX + (D == 0 ? 2 * G : 0);
-    constexpr int E1 = L::TY + (D == 1 ? 2 * G : 0);
-    constexpr int E2 = L::TZ + (D == 2 ? 2 * G : 0);
-    constexpr int R = E0 * E1 * E2;
-    constexpr int F0 = L::TX + (D == 0 ? 1 : 0);
-    constexpr int F1 = L::TY + (D == 1 ? 1 : 0);
-    constexpr int F2 = L::TZ + (D == 2 ? 1 : 0);
-    constexpr int F = F0 * F1 * F2;
-    static_assert(R <= RMAX && F <= FMAX, "LDS regions");
+    // extents (for the fixed shape known at compile time: divisions by constants, constant LDS
+    // strides); a tile clipped by its box leaves the positions beyond its cells unused.
+    const int E0 = sh.t(0) + (D == 0 ? 2 * G : 0);
+    const int E1 = sh.t(1) + (D == 1 ? 2 * G : 0);
+    const int E2 = sh.t(2) + (D == 2 ? 2 * G : 0);
+    const int R = E0 * E1 * E2;
+    const int F0 = sh.t(0) + (D == 0 ? 1 : 0);
+    const int F1 = sh.t(1) + (D == 1 ? 1 : 0);
+    const int F2 = sh.t(2) + (D == 2 ? 1 : 0);
+    const int F = F0 * F1 * F2;
+    float rE0 = 0.f, rE01 = 0.f, rF0 = 0.f, rF01 = 0.f;
+    if constexpr (SH::RT) {
+        rE0 = __builtin_amdgcn_rcpf((float)E0);
+        rE01 = __builtin_amdgcn_rcpf((float)(E0 * E1));
+        rF0 = __builtin_amdgcn_rcpf((float)F0);
+        rF01 = __builtin_amdgcn_rcpf((float)(F0 * F1));
+    } else {
+        static_assert(SH::RT || XS == 0, "a fixed shape has no second face pass");
+    }
     // extents actually used by this tile
     const int e0 = cx_.n[0] + (D == 0 ? 2 * G : 0);
     const int e1 = cx_.n[1] + (D == 1 ? 2 * G : 0);
@@ -257,7 +333,8 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     if constexpr (RIEMANN >= RIEMANN_PF_MIX) {
         int nz = 0;
         for (int f = tid; f < F; f += NT) {
-            const int x = f % F0, y = (f / F0) % F1, z = f / (F0 * F1);
+            int x, y, z;
+            split3<SH>(f, F0, F1, rF0, rF01, x, y, z);
             if (x < f0 && y < f1 && z < f2)
 #pragma unroll
                 for (int m = 0; m < NG; ++m) nz |= (gface(x, y, z, m) != 0.0);
@@ -266,9 +343,8 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
     }
     // 1. primitives of the region, staged in LDS
     for (int r = tid; r < R; r += NT) {
-        const int x = r % E0;
-        const int y = (r / E0) % E1;
-        const int z = r / (E0 * E1);
+        int x, y, z;
+        split3<SH>(r, E0, E1, rE0, rE01, x, y, z);
         if (x >= e0 || y >= e1 || z >= e2) continue;
         const int i = cx_.tlo[0] + x - (D == 0 ? G : 0);
         const int j = cx_.tlo[1] + y - (D == 1 ? G : 0);
@@ -290,17 +366,23 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
         for (int q = 0; q < N; ++q) sW[q * RMAX + r] = Wc[q];
     }
     __syncthreads();
-    // 2. faces along D: one face per thread (F <= NT for every tile shape). The face is computed
-    // into registers while the staged primitives are read; after a barrier the fluxes are written
-    // over the staged region, whose LDS the flux slots share.
-    static_assert(F <= NT, "one face per thread");
-    constexpr int rs = (D == 0) ? 1 : ((D == 1) ? E0 : E0 * E1);
-    const int x = tid % F0;
-    const int y = (tid / F0) % F1;
-    const int z = tid / (F0 * F1);
-    const bool has_face = (tid < F) && (x < f0) && (y < f1) && (z < f2);
+    // 2. faces along D: one face per thread and pass (the fixed shape has at most NT faces per
+    // direction, a run-time shape at most NT + XS). A face of the second pass (index NT + tid) is
+    // computed first and its flux stored in its own slots right away; the face of the first pass is
+    // computed into registers while the staged primitives are read, and after a barrier its flux is
+    // written over the staged region, whose LDS the first NT flux slots share.
+    const int rs = (D == 0) ? 1 : ((D == 1) ? E0 : E0 * E1);
+    const int npass = SH::RT ? (F > NT ? 2 : 1) : 1;
     double Fl[N];
     double un = 0.0;
+    bool has_face = false;
+#pragma unroll 1
+    for (int pass = npass - 1; pass >= 0; --pass) {
+    const int fi = pass * NT + tid;
+    int x, y, z;
+    split3<SH>(fi, F0, F1, rF0, rF01, x, y, z);
+    has_face = (fi < F) && (x < f0) && (y < f1) && (z < f2);
+    un = 0.0;
     if (has_face) {
         const int a = (D == 0) ? x : ((D == 1) ? y : z);
         const int base = x + E0 * (y + E1 * z);
@@ -364,25 +446,39 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
             }
         }
     }
+    if (pass > 0 && has_face) {
+        if constexpr (XS > 0) {
+#pragma unroll
+            for (int q = 0; q < N; ++q) sX[q * XS + tid] = Fl[q];
+            if constexpr (RIEMANN < RIEMANN_PF_MIX) sX[N * XS + tid] = un;
+        }
+    }
+    }
     __syncthreads();
     if (has_face) {
 #pragma unroll
-        for (int q = 0; q < N; ++q) sF[q * FMAX + tid] = Fl[q];
-        if constexpr (RIEMANN < RIEMANN_PF_MIX) sF[N * FMAX + tid] = un;
+        for (int q = 0; q < N; ++q) sF[q * FS + tid] = Fl[q];
+        if constexpr (RIEMANN < RIEMANN_PF_MIX) sF[N * FS + tid] = un;
     }
     __syncthreads();
     // 3. flux differences of this direction
+    auto flux = [&](int q, int f) -> double {
+        if constexpr (XS > 0)
+            return (f < NT) ? sF[q * FS + f] : sX[q * XS + (f - NT)];
+        else
+            return sF[q * FS + f];
+    };
     if (cx_.valid) {
         const int fm = cx_.cx + F0 * (cx_.cy + F1 * cx_.cz);
         const int fp = fm + ((D == 0) ? 1 : ((D == 1) ? F0 : F0 * F1));
         const double idx = kp.inv_dx[D];
 #pragma unroll
         for (int q = 0; q < N; ++q) {
-            const double t = (sF[q * FMAX + fp] - sF[q * FMAX + fm]) * idx;
+            const double t = (flux(q, fp) - flux(q, fm)) * idx;
             acc[q] = (D == 0) ? t : acc[q] + t;
         }
         if constexpr (RIEMANN < RIEMANN_PF_MIX) {
-            const double t = (sF[N * FMAX + fp] - sF[N * FMAX + fm]) * idx;
+            const double t = (flux(N, fp) - flux(N, fm)) * idx;
             divu = (D == 0) ? t : divu + t;
         }
     }
@@ -399,32 +495,39 @@ __device__ __forceinline__ void dir_pass(const StageCtx<DIM>& cx_, double* sW, d
 #define CMP_STAGE_ATTR
 #endif
 
-// The stage of one tile (box bx, lowest cell tlo, halo sources halo): staging, faces and flux
-// differences per direction, then the cell update of `mode`; c_max max-accumulated into *cmax_bits,
-// the error word into *err. sWF / sRed: the workgroup's LDS.
-template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP>
+// The stage of one tile (box bx, lowest cell tlo, shape sh, halo sources halo): staging, faces and
+// flux differences per direction, then the cell update of `mode`; c_max max-accumulated into
+// *cmax_bits, the error word into *err. sWF / sRed: the workgroup's LDS (sWF: lds_doubles of the
+// shape policy's layout).
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP, class SH>
 __device__ __forceinline__ void stage_tile(const KParams& kp, int limiter, int mode, int stage, const BoxDev& bx,
-                                           const int* tlo, const int (&halo)[3][2], const BoxDev* __restrict__ boxes,
+                                           const int* tlo, const SH& sh, const int (&halo)[3][2],
+                                           const BoxDev* __restrict__ boxes,
                                            const double* __restrict__ Us, const double* Un, double* Uout,
                                            const double* __restrict__ dtp, unsigned long long* cmax_bits, int* err,
                                            const double* __restrict__ gq, const FillDev* __restrict__ fd,
-                                           double* sWF, double* sRed)
+                                           double* sWF, double* sRed, int tid = -1)
 {
     constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
-    using L = Lds<DIM, G>;
-    constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
+    using LO = LdsOf<DIM, G, SH>;
     double* const sW = sWF;
     double* const sF = sWF;
+    double* const sX = sWF + (lds_doubles<N, LO>() - (N + 1) * LO::XS);
 
-    const int tid = threadIdx.x;
+    if (tid < 0) tid = threadIdx.x;
+    const int TX = sh.t(0), TY = sh.t(1), TZ = sh.t(2);
     int n[3];
     n[0] = min(TX, bx.hi[0] - tlo[0] + 1);
     n[1] = min(TY, bx.hi[1] - tlo[1] + 1);
     n[2] = (DIM == 3) ? min(TZ, bx.hi[2] - tlo[2] + 1) : 1;
 
-    const int cx = tid % TX, cy = (tid / TX) % TY, cz = tid / (TX * TY);
+    int cx, cy, cz;
+    if constexpr (SH::RT)
+        split3<SH>(tid, TX, TY, __builtin_amdgcn_rcpf((float)TX), __builtin_amdgcn_rcpf((float)(TX * TY)), cx, cy, cz);
+    else
+        split3<SH>(tid, TX, TY, 0.f, 0.f, cx, cy, cz);
     const bool valid = (cx < n[0]) && (cy < n[1]) && (cz < n[2]);
     const double* __restrict__ Ub = Us + bx.off;
 
@@ -438,10 +541,10 @@ __device__ __forceinline__ void stage_tile(const KParams& kp, int limiter, int m
     StageCtx<DIM> ctx{kp, limiter, bx, Ub, {tlo[0], tlo[1], tlo[2]}, {n[0], n[1], n[2]}, tid, cx, cy, cz,
                       valid, gq, fd, Us, boxes,
                       {{halo[0][0], halo[0][1]}, {halo[1][0], halo[1][1]}, {halo[2][0], halo[2][1]}}};
-    dir_pass<DIM, SCHEME, RIEMANN, THINC, 0, NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
-    if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1, NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
+    dir_pass<DIM, SCHEME, RIEMANN, THINC, 0, NP>(ctx, sh, sW, sF, sX, acc, divu, cmax_loc, bad);
+    if (DIM > 1) dir_pass<DIM, SCHEME, RIEMANN, THINC, 1, NP>(ctx, sh, sW, sF, sX, acc, divu, cmax_loc, bad);
     if (DIM > 2)
-        dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1), NP>(ctx, sW, sF, acc, divu, cmax_loc, bad);
+        dir_pass<DIM, SCHEME, RIEMANN, THINC, (DIM > 2 ? 2 : 1), NP>(ctx, sh, sW, sF, sX, acc, divu, cmax_loc, bad);
 
     if (valid) {
         const int i = tlo[0] + cx, j = tlo[1] + cy, k = tlo[2] + cz;
@@ -540,10 +643,8 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
     constexpr int G = Stencil<SCHEME>::G;
     using L = Lds<DIM, G>;
     constexpr int TX = L::TX, TY = L::TY, TZ = L::TZ;
-    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
     // the staged primitives of a direction and, once its faces are computed, their fluxes
-    constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
-    __shared__ double sWF[SWF];
+    __shared__ double sWF[lds_doubles<N, LdsOf<DIM, G, ShapeFixed<DIM>>>()];
     __shared__ double sRed[NT / 64];
     // a step of size zero (a level clock past its stop_time) leaves every state as it is
     if (mode == MODE_RK && *dtp == 0.0) return;
@@ -575,8 +676,8 @@ void k_stage(KParams kp, int limiter, int mode, int stage,
             }
         }
     }
-    stage_tile<DIM, SCHEME, RIEMANN, THINC, NP>(kp, limiter, mode, stage, bx, tlo, halo, boxes, Us, Un, Uout, dtp,
-                                                cmax_bits, err, gq, fd, sWF, sRed);
+    stage_tile<DIM, SCHEME, RIEMANN, THINC, NP>(kp, limiter, mode, stage, bx, tlo, ShapeFixed<DIM>{}, halo, boxes, Us,
+                                                Un, Uout, dtp, cmax_bits, err, gq, fd, sWF, sRed);
 }
 
 // One RK stage of a batch of independent levels: every tile of every admitted case in one launch.
@@ -591,10 +692,7 @@ void k_stage_batch(const KParams* __restrict__ kps, int limiter, int stage, cons
 {
     constexpr int N = 4 + DIM;
     constexpr int G = Stencil<SCHEME>::G;
-    using L = Lds<DIM, G>;
-    constexpr int RMAX = L::RMAX, FMAX = L::FMAX;
-    constexpr int SWF = (N * RMAX > (N + 1) * FMAX) ? N * RMAX : (N + 1) * FMAX;
-    __shared__ double sWF[SWF];
+    __shared__ double sWF[lds_doubles<N, LdsOf<DIM, G, ShapeFixed<DIM>>>()];
     __shared__ double sRed[NT / 64];
 
     const TileDev td = tiles[xcd_tile(blockIdx.x, ntiles)];
@@ -603,10 +701,12 @@ void k_stage_batch(const KParams* __restrict__ kps, int limiter, int stage, cons
     const int tlo[3] = {td.lo[0], td.lo[1], td.lo[2]};
     const int halo[3][2] = {{td.halo[0][0], td.halo[0][1]}, {td.halo[1][0], td.halo[1][1]},
                             {td.halo[2][0], td.halo[2][1]}};
-    stage_tile<DIM, SCHEME, RIEMANN, false, 0>(kps[cs], limiter, MODE_RK, stage, bx, tlo, halo, boxes, Us, Un, Uout,
-                                               dts + cs, cmax_bits + cs, errs + cs, nullptr, fds + cs, sWF, sRed);
+    stage_tile<DIM, SCHEME, RIEMANN, false, 0>(kps[cs], limiter, MODE_RK, stage, bx, tlo, ShapeFixed<DIM>{}, halo,
+                                               boxes, Us, Un, Uout, dts + cs, cmax_bits + cs, errs + cs, nullptr,
+                                               fds + cs, sWF, sRed);
 }
 
+#ifndef CMP_LIST_UNIT
 // The face flux and u*_n of the face between cells (ci, cj, ck) - e_DIR and (ci, cj, ck) of box bx
 // (Ub: the box's fab, its ghost cells filled), every stencil cell taken through cons->prim here.
 // NP > 0: the N-phase model (no THINC).
@@ -1769,5 +1869,175 @@ template hipError_t launch_fill_dim<CMP_KERNEL_DIM>(int, const FillDev&, long lo
                                                     hipStream_t);
 template hipError_t launch_dt_dim<CMP_KERNEL_DIM>(const double*, double*, double*, double, int,
                                                   double*, int*, int*, hipStream_t, double*);
+#else   // CMP_LIST_UNIT
+
+// ---- persistent work-list stage kernel ------------------------------------------------------------
+// A fixed grid of workgroups (as many as are resident at once) walks one list of tiles, each with
+// its own level and shape (the tile shape the autotuner chose for its box's shape); the tiles of
+// every level of a hierarchy go into one launch. The list is cut into 8 consecutive chunks, one per
+// XCD: workgroup b starts on chunk b mod 8 (workgroups are dispatched round-robin over the XCDs, so
+// a chunk's tiles, whose halos overlap, meet in one XCD's L2), takes its tiles one at a time from
+// the chunk's device counter and, once the chunk is exhausted, moves on to the following chunks.
+// Every workgroup leaves after it has found all 8 chunks exhausted; the last one to leave sets the
+// counters back to zero for the next launch. W: the resident waves per SIMD the register allocation
+// must leave room for (2, or 3 at the price of some register spills: the autotuner measures both).
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP, int W>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(W)))
+void k_stage_list(const ListLevel* __restrict__ levs, const ListTile* __restrict__ tiles, int ntiles, int* ctr,
+                  ListCall call, int limiter, int mode, int stage, const double* __restrict__ dtp)
+{
+    constexpr int N = NP ? 2 * NP + DIM + 1 : 4 + DIM;
+    constexpr int G = Stencil<SCHEME>::G;
+    __shared__ double sWF[lds_doubles<N, LdsOf<DIM, G, ShapeRT>>()];
+    __shared__ double sRed[NT / 64];
+    __shared__ int sT;
+    // a step of size zero leaves every state as it is (and the counters untouched)
+    if (mode == MODE_RK && *dtp == 0.0) return;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x & 7;
+    int tried = 0;   // chunks found exhausted (thread 0)
+    for (;;) {
+        if (tid == 0) {
+            int t = -1;
+            while (tried < 8) {
+                const int c = (x0 + tried) & 7;
+                const int lo = (int)(((long long)ntiles * c) >> 3);
+                const int hi = (int)(((long long)ntiles * (c + 1)) >> 3);
+                if (lo < hi) {
+                    const int k = atomicAdd(ctr + c, 1);
+                    if (k < hi - lo) {
+                        t = lo + k;
+                        break;
+                    }
+                }
+                ++tried;
+            }
+            sT = t;
+        }
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(sT);
+        __syncthreads();
+        if (t < 0) break;
+        const ListTile td = tiles[t];
+        const int l = td.lev;
+        const ListLevel& lv = levs[l];
+        const BoxDev bx = lv.boxes[td.box];
+        ShapeRT sh;
+        sh.e[0] = td.ext[0];
+        sh.e[1] = td.ext[1];
+        sh.e[2] = td.ext[2];
+        const int tlo[3] = {td.lo[0], td.lo[1], td.lo[2]};
+        int halo[3][2] = {{HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}, {HALO_LOOKUP, HALO_LOOKUP}};
+        if (lv.fd) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                halo[d][0] = td.halo[d][0];
+                halo[d][1] = td.halo[d][1];
+            }
+        }
+        stage_tile<DIM, SCHEME, RIEMANN, THINC, NP>(lv.kp, limiter, mode, stage, bx, tlo, sh, halo, lv.boxes,
+                                                    call.Us[l], call.Un[l], call.Uout[l], dtp, call.cmax[l], lv.err,
+                                                    nullptr, lv.fd, sWF, sRed);
+    }
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(ctr + 8, 1) == (int)gridDim.x - 1) {
+            // every other workgroup has left: no counter is taken from any more
+#pragma unroll
+            for (int c = 0; c < LIST_NCTR; ++c) atomicExch(ctr + c, 0);
+        }
+    }
+}
+
+template <int DIM, int SCHEME, int RIEMANN, bool THINC, int NP = 0>
+static hipError_t list4(int waves, int query, int* cap, const ListLevel* levs, const ListTile* tiles, int ntiles,
+                        int* ctr, const ListCall& call, int limiter, int mode, int stage, const double* dt, int grid,
+                        hipStream_t s)
+{
+    if (waves != 2 && waves != 3) return hipErrorInvalidValue;
+    auto kern = (waves == 3) ? k_stage_list<DIM, SCHEME, RIEMANN, THINC, NP, 3>
+                             : k_stage_list<DIM, SCHEME, RIEMANN, THINC, NP, 2>;
+    if (query) {
+        int per_cu = 0, dev = 0, cus = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, NT, 0);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        *cap = (e == hipSuccess) ? per_cu * cus : 0;
+        return e;
+    }
+    if (ntiles <= 0) return hipSuccess;
+    if (grid <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kern, dim3(grid < ntiles ? grid : ntiles), dim3(NT), 0, s, levs, tiles, ntiles, ctr, call,
+                       limiter, mode, stage, dt);
+    return hipGetLastError();
+}
+
+// query != 0: the resident-workgroup capacity of lc's kernel into *cap, nothing launched
+template <int DIM>
+hipError_t launch_stage_list_dim(const LaunchCfg& lc, int waves, int query, int* cap, const ListLevel* levs,
+                                 const ListTile* tiles, int ntiles, int* ctr, const ListCall& call, int mode,
+                                 int stage, const double* dt, int grid, hipStream_t s)
+{
+    if (lc.riemann != 1 && lc.riemann != 2) return hipErrorInvalidValue;
+#define CMP_LIST_ARGS waves, query, cap, levs, tiles, ntiles, ctr, call, lc.limiter, mode, stage, dt, grid, s
+    if (lc.nphase) {
+        if (lc.thinc) return hipErrorInvalidValue;
+#define CMP_NPL_CASE(SC)                                                                                \
+        if (lc.scheme == SC) {                                                                          \
+            if (lc.nphase == 3) return lc.riemann == 2 ? list4<DIM, SC, 2, false, 3>(CMP_LIST_ARGS)     \
+                                                       : list4<DIM, SC, 1, false, 3>(CMP_LIST_ARGS);    \
+            if (lc.nphase == 2) return lc.riemann == 2 ? list4<DIM, SC, 2, false, 2>(CMP_LIST_ARGS)     \
+                                                       : list4<DIM, SC, 1, false, 2>(CMP_LIST_ARGS);    \
+        }
+        CMP_NPL_CASE(5)
+        CMP_NPL_CASE(1)
+        CMP_NPL_CASE(0)
+#undef CMP_NPL_CASE
+        return hipErrorInvalidValue;
+    }
+    const bool th = lc.thinc != 0;
+    if (lc.riemann == 2) {
+        if (lc.scheme == 5) return th ? list4<DIM, 5, 2, true>(CMP_LIST_ARGS) : list4<DIM, 5, 2, false>(CMP_LIST_ARGS);
+        if (lc.scheme == 6) return th ? list4<DIM, 6, 2, true>(CMP_LIST_ARGS) : list4<DIM, 6, 2, false>(CMP_LIST_ARGS);
+        if (lc.scheme == 1 && !th) return list4<DIM, 1, 2, false>(CMP_LIST_ARGS);
+        if (lc.scheme == 0 && !th) return list4<DIM, 0, 2, false>(CMP_LIST_ARGS);
+    } else {
+        if (lc.scheme == 5) return th ? list4<DIM, 5, 1, true>(CMP_LIST_ARGS) : list4<DIM, 5, 1, false>(CMP_LIST_ARGS);
+        if (lc.scheme == 6) return th ? list4<DIM, 6, 1, true>(CMP_LIST_ARGS) : list4<DIM, 6, 1, false>(CMP_LIST_ARGS);
+        if (lc.scheme == 1 && !th) return list4<DIM, 1, 1, false>(CMP_LIST_ARGS);
+        if (lc.scheme == 0 && !th) return list4<DIM, 0, 1, false>(CMP_LIST_ARGS);
+    }
+#undef CMP_LIST_ARGS
+    return hipErrorInvalidValue;
+}
+
+// Whether ext fits k_stage_list's layout: at most NT cells, every direction's staged region within
+// LdsRT::RMAX cells and its faces within NT + LdsRT::XS.
+template <int DIM>
+bool tile_shape_ok_dim(const LaunchCfg& lc, const int* ext)
+{
+    const int G = (lc.scheme >= 5) ? 3 : (lc.scheme == 1 ? 2 : 1);
+    const int rmax = (DIM == 2) ? 32 * (8 + 2 * G) : 64 * (4 + 2 * G);
+    const int xs = (DIM == 2) ? 32 : 64;
+    const int e[3] = {ext[0], ext[1], DIM == 3 ? ext[2] : 1};
+    for (int d = 0; d < 3; ++d)
+        if (e[d] < 1 || e[d] > 255) return false;
+    if (e[0] * e[1] * e[2] > NT) return false;
+    for (int d = 0; d < DIM; ++d) {
+        int r = 1, f = 1;
+        for (int m = 0; m < 3; ++m) {
+            r *= e[m] + (m == d ? 2 * G : 0);
+            f *= e[m] + (m == d ? 1 : 0);
+        }
+        if (r > rmax || f > NT + xs) return false;
+    }
+    return true;
+}
+
+template hipError_t launch_stage_list_dim<CMP_KERNEL_DIM>(const LaunchCfg&, int, int, int*, const ListLevel*,
+                                                          const ListTile*, int, int*, const ListCall&, int, int,
+                                                          const double*, int, hipStream_t);
+template bool tile_shape_ok_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const int*);
+#endif  // CMP_LIST_UNIT
 
 }  // namespace cmp

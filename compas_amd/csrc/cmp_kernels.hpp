@@ -98,6 +98,47 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 // fuse: a FillDev in device memory; the stage then reads the cells beyond each tile's box where the
 // ghost fill would take them from, instead of the box's ghost cells
 
+// ---- persistent work-list stage kernel (k_stage_list) ----
+// A tile of a work-list that may span several levels, each tile with its own shape: level (index
+// into the launch's level table), box, lowest cell, extents, halo sources as in TileDev.
+struct ListTile {
+    int lev;
+    int box;
+    int lo[3];
+    int ext[3];
+    int halo[3][2];
+};
+// What the work-list kernel needs of one level: parameters, box table, fused ghost-fill table in
+// device memory (null: the stage input's ghost cells are read) and error word.
+struct ListLevel {
+    KParams kp;
+    const BoxDev* boxes;
+    const FillDev* fd;
+    int* err;
+};
+constexpr int LIST_MAXLEV = 8;
+// The per-launch state pointers of the work-list kernel, by level.
+struct ListCall {
+    const double* Us[LIST_MAXLEV];
+    const double* Un[LIST_MAXLEV];
+    double* Uout[LIST_MAXLEV];
+    unsigned long long* cmax[LIST_MAXLEV];
+};
+// ints of a work-list's device counters: one per XCD chunk of the list, then the workgroups done
+constexpr int LIST_NCTR = 9;
+// The flow stage of every tile of a work-list (two-phase or N-phase model, not the Phase-Field
+// flux). waves: the kernel's occupancy target (2 or 3 waves per SIMD); grid: workgroups of the
+// persistent launch (list_capacity); ctr: LIST_NCTR ints, zero before the first launch and left
+// zero by every launch.
+hipError_t launch_stage_list(const LaunchCfg& lc, int waves, const ListLevel* levs, const ListTile* tiles, int ntiles,
+                             int* ctr, const ListCall& call, int mode, int stage, const double* dt, int grid,
+                             hipStream_t s);
+// Workgroups of the work-list kernel of lc (occupancy target waves) resident on the device at once
+// (0: no such kernel).
+int list_capacity(const LaunchCfg& lc, int waves);
+// Whether a tile shape fits the work-list kernel's LDS and thread layout for lc.
+bool tile_shape_ok(const LaunchCfg& lc, const int* ext);
+
 // N = 4 + dim: the flow state (reflecting BCs flip momenta); N = 1 or dim: a Phase-Field field
 // (alpha1 or the normals), filled without sign changes. With `list`, only the ghosts of the nlist
 // boxes it names are filled.
@@ -145,6 +186,7 @@ struct AmrDt {
     int dim;
     double dx[AMR_MAX_LEVELS][3];
 };
+
 
 // The coarse-fine face correction of one stage (cmp_amr_rk3_step): every listed coarse cell's
 // stage output gains w dt times the change of its right-hand side when its faces over the finer

@@ -12,6 +12,24 @@ namespace cmp {
 // HIP status -> CMP_OK / CMP_ERR_HIP (printing the HIP error)
 int check_hip(hipError_t e);
 
+// The halo sources of tile [tlo, thi] of box b on its box's faces (cmp_capi.hip); owner, nb, bs:
+// the level's owner map of bs^dim blocks.
+void tile_halo(int dim, const cmp_geometry* geom, const cmp_box* boxes, int b, const int* tlo, const int* thi,
+               int ng, const std::vector<int>& owner, const int* nb, long long bs, int (*halo)[2]);
+
+}  // namespace cmp
+struct cmp_level;
+namespace cmp {
+// The work-list form of a level's flow stage (cmp_tune.hip): one stage through the level's list
+// (fused ghost reads), and the tiles of every box of a level by its box_ext with level index lev.
+int level_list_stage(cmp_level* L, int mode, int stage, const double* Us, const double* Un, double* Uout,
+                     const double* dt, double* cmax, hipStream_t s);
+void level_list_tiles(const cmp_level* L, int lev, std::vector<ListTile>& out);
+// The tuned choices of L from the autotuner's cache: per occupancy target (2, 3 waves per SIMD) each
+// box's tile extents and the level's summed time, and the fixed kernel's summed time;
+// CMP_ERR_UNSUPPORTED when a box shape of L has no cached sweep.
+int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, double* t_fixed);
+
 }  // namespace cmp
 
 using cmp::BoxDev;
@@ -136,4 +154,16 @@ struct cmp_level {
     double stop_time = -1.0;        // the last stop_time given to cmp_level_set_time
     long long bs = 1;
     int nb[3] = {1, 1, 1};
+
+    // The flow stage as the persistent work-list kernel (cmp_level_set_tiles, cmp_level_tune_tiles):
+    // per box its tile extents, the tiles, the one-level table (fused ghost reads) and the list's
+    // device counters; use_list routes cmp_level_rk3_step and cmp_level_advance through it.
+    bool use_list = false;
+    std::vector<int> box_ext;       // 3 ints per box
+    std::vector<cmp::ListTile> h_ltiles;
+    cmp::ListTile* d_ltiles = nullptr;
+    cmp::ListLevel* d_llev = nullptr;
+    int* d_lctr = nullptr;
+    int list_grid = 0;
+    int list_waves = 2;             // the work-list kernel's occupancy target (waves per SIMD)
 };

@@ -277,6 +277,43 @@ int cmp_level_check(cmp_level* level, void* stream);
 /* Number of tiles in the level work-list and the tile shape used (for reporting). */
 int cmp_level_info(const cmp_level* level, int* ntiles, int* tile_x, int* tile_y, int* tile_z);
 
+/* ---- persistent work-list stage kernel and tile autotuner ---------------------------------------
+ * These replace the per-box tiling of ComputeFaceFluxes' MFIter loop
+ * (source/Class_Compressible_PhaseField.cpp:3152-3254; tiles of amrex::TilingIfNotGPU) with one
+ * persistent launch per RK stage over a list of tiles, each box with a tile shape of its own. The
+ * reference has no counterpart of the measured shape sweep.
+ *
+ * cmp_level_set_tiles: the flow stage of cmp_level_rk3_step / cmp_level_advance (TVD-RK3) through
+ * the work-list kernel, box b cut into tiles of ext[3b..3b+2] cells (clipped to the box; at most 256
+ * cells, CMP_ERR_ARG for a shape the kernel's LDS layout does not hold), the kernel built for
+ * `waves` (2 or 3) resident waves per SIMD; ext NULL: back to the fixed-shape kernel. Two-phase and
+ * N-phase flow (not the Phase-Field flux). The results are those of the fixed-shape kernel bit for
+ * bit.
+ * cmp_level_tile_shapes: *use_list 1 when the work-list kernel is on, *waves its occupancy target
+ * (0 when off); ext (3 ints per box, or NULL): each box's tile extents. */
+int cmp_level_set_tiles(cmp_level* level, const int* ext, int waves);
+int cmp_level_tile_shapes(const cmp_level* level, int* use_list, int* waves, int* ext);
+
+/* The tile autotuner: for every box shape of the level (and the nextra shapes of `extra`, 3 ints
+ * each) not yet in this process's cache for the level's scheme (scheme, limiter, Riemann solver,
+ * THINC, nphase, ghost cells), a synthetic periodic level of copies of that box, its cells sampled
+ * from the level's largest box in U, takes one RK stage per candidate tile shape with the work-list
+ * kernel at both occupancy targets and one with the fixed-shape kernel (HIP events, least of 3); the
+ * times per cell are cached. Then the level takes the occupancy target of least summed time, per box
+ * its candidate of least time, and the work-list kernel when that sum is below the fixed kernel's
+ * (force 1: always, -1: never). A box thinner than the ghost width is not swept, and a level holding
+ * one keeps the fixed kernel. Allocates device memory while it runs; synchronises. */
+int cmp_level_tune_tiles(cmp_level* level, const double* U, int nextra, const int* extra, int force, void* stream);
+/* The cached sweep of one box shape (n[3]) for the level's scheme: the fixed-shape kernel's ns per
+ * cell, the candidates' extents (3 ints each) and their ns per cell at 2 and 3 waves per SIMD
+ * (2 doubles each; arrays of at least 12 candidates, or NULL). CMP_ERR_ARG when not cached. */
+int cmp_tile_tuning(const cmp_level* level, const int* n, double* ns_fixed, int* ncand, int* cand_ext,
+                    double* cand_ns);
+/* The cache as a text file (one line per box shape and scheme) and back; clear empties it. */
+int cmp_tile_cache_save(const char* path);
+int cmp_tile_cache_load(const char* path);
+int cmp_tile_cache_clear(void);
+
 /* ---- phase-field interface regularization of a level ------------------------------------------
  * Grad_Q face buffers below follow cmp_face_layout with one component per volume fraction: one for
  * the two-phase model (NPHASE = 1), nphase for the N-phase model (component k is phase k's Grad_Q,
@@ -414,6 +451,15 @@ int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out)
 int cmp_amr_destroy(cmp_amr* amr);
 /* Coarse cells of `level` whose faces border the next finer level (0 for the finest). */
 int cmp_amr_info(const cmp_amr* amr, int level, int* ncf_cells);
+/* The flow stage of every level in one launch of the persistent work-list kernel, each level's boxes
+ * cut into the tile shapes of the autotuner's cache (cmp_level_tune_tiles on every level first;
+ * its cache holds the sweeps): force 1 on (CMP_ERR_UNSUPPORTED when a box shape has no sweep), -1
+ * off, 0 on when the swept times summed over every level's boxes favour it, which cmp_amr_create
+ * also applies. *use_list, *waves (or NULL): the outcome and the kernel's occupancy target. The
+ * stage results are those of the per-level fixed-shape kernels bit for bit. */
+int cmp_amr_tiles(cmp_amr* amr, int force, int* use_list, int* waves);
+/* The hierarchy's current choice (as cmp_amr_tiles reports it), changing nothing. */
+int cmp_amr_tile_info(const cmp_amr* amr, int* use_list, int* waves);
 /* CalculateInitialcmax of every level on the filled hierarchy: cmax (device, nlev doubles). Writes
  * the ghost cells of U. */
 int cmp_amr_initial_cmax(cmp_amr* amr, double* const* U, double* cmax, void* stream);

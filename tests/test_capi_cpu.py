@@ -87,3 +87,26 @@ def test_argument_errors_are_returned_not_raised():
                                 ctypes.byref(h)) == capi.CMP_ERR_ARG
     assert lib.cmp_level_destroy(None) == 0
     assert lib.cmp_level_check(None, None) == capi.CMP_ERR_ARG
+
+
+def test_tile_cache_file_round_trip(tmp_path):
+    """The tile autotuner's cache as a text file (host code only): a file of two swept box shapes
+    loads, saves back to the same entries, and a truncated file is refused with CMP_ERR_ARG."""
+    lib = capi.load()
+    src = tmp_path / "in.txt"
+    src.write_text("2 5 5 2 0 3 3 16 16 1 0.47 2  16 15 1 0.471 0.52  16 16 1 0.444 0.46\n"
+                   "3 5 5 2 1 0 3 32 32 32 0.2076 1  8 8 3 0.21 0.25\n")
+    assert lib.cmp_tile_cache_clear() == 0
+    assert lib.cmp_tile_cache_load(str(src).encode()) == 0
+    out = tmp_path / "out.txt"
+    assert lib.cmp_tile_cache_save(str(out).encode()) == 0
+
+    def parse(p):
+        return sorted(tuple(float(v) for v in line.split()) for line in p.read_text().splitlines() if line.strip())
+
+    assert parse(out) == parse(src)
+    bad = tmp_path / "bad.txt"
+    bad.write_text("2 5 5 2 0 3 3 16 16 1 0.47 2  16 15 1 0.471\n")
+    assert lib.cmp_tile_cache_load(str(bad).encode()) == capi.CMP_ERR_ARG
+    assert lib.cmp_tile_cache_load(str(tmp_path / "missing.txt").encode()) == capi.CMP_ERR_ARG
+    assert lib.cmp_tile_cache_clear() == 0
