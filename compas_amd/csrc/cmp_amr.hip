@@ -826,6 +826,8 @@ CompArgs comp_args(const cmp_amr* A, bool solution)
         const cmp_level* L = A->L[l];
         CompLev& c = ca.L[l];
         c.abox = L->d_abox;
+        c.cmap = L->pg.cmap;
+        c.ncmap = L->pg.ncmap;
         c.fd = L->fdA;
         c.mask = P.d_mask[l];
         c.v = solution ? L->d_X : L->d_Pv;

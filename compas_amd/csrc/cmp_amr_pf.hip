@@ -224,10 +224,11 @@ __global__ __launch_bounds__(NT) void k_comp_ap(CompArgs a, int l, const double*
 {
     if (!diag && sc[CG_DONE] != 0.0) return;
     const CompLev& L = a.L[l];
-    const BoxDev ab = L.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = L.abox[cell_block(L.cmap, q_)];
     int i, j, k;
     double v = 0.0;
-    if (ap_cell<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+    if (ap_cell<DIM>(ab, q_, i, j, k)) {
         const long long c = ab.off + fab_index(ab, i, j, k);
         const int mask = L.mask[c];
         if (mask == COMP_COVERED) {
@@ -253,10 +254,11 @@ __global__ __launch_bounds__(NT) void k_comp_init(CompArgs a, int l, const doubl
                                                   double* R, double* part)
 {
     const CompLev& L = a.L[l];
-    const BoxDev ab = L.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = L.abox[cell_block(L.cmap, q_)];
     int i, j, k;
     double v = 0.0;
-    if (ap_cell<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+    if (ap_cell<DIM>(ab, q_, i, j, k)) {
         const long long c = ab.off + fab_index(ab, i, j, k);
         const bool cov = L.mask[c] == COMP_COVERED;
         if (!mean_sum) {
@@ -279,9 +281,10 @@ __global__ __launch_bounds__(NT) void k_comp_s(CompArgs a, int l, const double* 
 {
     if (sc && sc[CG_DONE] != 0.0) return;
     const CompLev& L = a.L[l];
-    const BoxDev ab = L.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = L.abox[cell_block(L.cmap, q_)];
     int i, j, k;
-    if (!ap_cell<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    if (!ap_cell<DIM>(ab, q_, i, j, k)) return;
     const long long c = ab.off + fab_index(ab, i, j, k);
     S[c] = R[c] * (1.0 / L.vol);
 }
@@ -296,9 +299,10 @@ __global__ __launch_bounds__(NT) void k_comp_z(CompArgs a, int l, const double* 
 {
     if (sc && sc[CG_DONE] != 0.0) return;
     const CompLev& L = a.L[l];
-    const BoxDev ab = L.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = L.abox[cell_block(L.cmap, q_)];
     int i, j, k;
-    if (!ap_cell<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    if (!ap_cell<DIM>(ab, q_, i, j, k)) return;
     const long long c = ab.off + fab_index(ab, i, j, k);
     if (L.mask[c] == COMP_COVERED) { Z[c] = 0.0; return; }
     int f = 1;
@@ -383,6 +387,12 @@ dim3 box_grid2(long long maxn, int nbox)
     if (b < 1) b = 1;
     return dim3((unsigned)b, (unsigned)nbox);
 }
+// a launch over the cells of level p.lev of the composite solve (its compact block list)
+dim3 comp_grid(const AmrPfArgs& p)
+{
+    const CompLev& L = p.ca->L[p.lev];
+    return cell_grid(L.cmap, L.ncmap, p.maxcells, p.nbox);
+}
 dim3 flat_grid(long long n)
 {
     long long b = (n + NT - 1) / NT;
@@ -411,23 +421,23 @@ hipError_t launch_amrpf_dim(int op, const AmrPfArgs& p, hipStream_t s)
         hipLaunchKernelGGL(k_sum_levels, dim3(1), dim3(64), 0, s, p.src, p.nlev, p.nvals, p.dst, p.sc);
         break;
     case AP_COMP_AP:
-        hipLaunchKernelGGL((k_comp_ap<DIM>), box_grid2(p.maxcells, p.nbox), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
+        hipLaunchKernelGGL((k_comp_ap<DIM>), comp_grid(p), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
                            p.dst, p.part, 0);
         break;
     case AP_COMP_DIAG:
-        hipLaunchKernelGGL((k_comp_ap<DIM>), box_grid2(p.maxcells, p.nbox), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
+        hipLaunchKernelGGL((k_comp_ap<DIM>), comp_grid(p), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
                            p.dst, p.part, 1);
         break;
     case AP_COMP_INIT:
-        hipLaunchKernelGGL((k_comp_init<DIM>), box_grid2(p.maxcells, p.nbox), dim3(NT), 0, s, *p.ca, p.lev, p.src,
+        hipLaunchKernelGGL((k_comp_init<DIM>), comp_grid(p), dim3(NT), 0, s, *p.ca, p.lev, p.src,
                            p.mean_sum, p.a, p.dst, p.dst2, p.part);
         break;
     case AP_COMP_S:
-        hipLaunchKernelGGL((k_comp_s<DIM>), box_grid2(p.maxcells, p.nbox), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
+        hipLaunchKernelGGL((k_comp_s<DIM>), comp_grid(p), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
                            p.src, p.dst);
         break;
     case AP_COMP_Z:
-        hipLaunchKernelGGL((k_comp_z<DIM>), box_grid2(p.maxcells, p.nbox), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
+        hipLaunchKernelGGL((k_comp_z<DIM>), comp_grid(p), dim3(NT), 0, s, *p.ca, p.lev, p.sc,
                            p.src, p.src2, p.src3, p.G0, p.dlo0[0], p.dlo0[1], p.dlo0[2], p.dst);
         break;
     case AP_COMP_GRAD:

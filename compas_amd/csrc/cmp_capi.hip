@@ -796,13 +796,16 @@ static void pf_free(cmp_level* L)
                     L->d_J, L->d_H, L->d_gq, L->d_gq1, L->d_bits, L->d_pfdt, L->d_Mk, L->d_Wk, L->d_DJ,
                     L->d_X, L->d_R, L->d_Pv, L->d_AP, L->d_Z, L->d_ID, L->d_Bf, L->d_part, L->d_sc,
                     L->d_boxflag,
-                    L->d_pf_tiles};
+                    L->d_pf_tiles, L->d_cmap};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     void* pins[] = {L->h_pin, L->h_boxflag, L->h_pf_tiles};
     for (void* p : pins)
         if (p) (void)hipHostFree(p);
     L->d_boxflag = L->h_boxflag = nullptr;
+    L->d_cmap = nullptr;
+    L->pg.cmap = nullptr;
+    L->pg.ncmap = 0;
     L->d_pf_tiles = L->h_pf_tiles = nullptr;
     L->ghost_src.clear();
     L->d_abox = L->d_nbox = nullptr;
@@ -1242,9 +1245,20 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     CMP_TRYP(hipHostMalloc((void**)&L->h_pin, sizeof(double) * 4, hipHostMallocDefault));
     // the unfilled ghost cells of the normals are never read, but keep them defined
     CMP_TRYP(hipMemset(L->d_N, 0, sizeof(double) * nn));
-    const long long nblk = ((maxcells + NT - 1) / NT) * nbox;
-    if (nblk > (1LL << 30)) { pf_free(L); return CMP_ERR_UNSUPPORTED; }
+    // the compact block list of the level's cells: every cell kernel of the Phase-Field step (and the
+    // composite solve) launches one workgroup per listed block, and its block partial sums have
+    // that many slots
+    std::vector<cmp::BoxBlock> cmap;
+    for (int b = 0; b < nbox; ++b) {
+        long long cells = 1;
+        for (int d = 0; d < dim; ++d) cells *= L->boxes[b].hi[d] - L->boxes[b].lo[d] + 1;
+        for (long long k = 0; k < (cells + NT - 1) / NT; ++k) cmap.push_back(cmp::BoxBlock{b, (int)k});
+    }
+    const long long nblk = (long long)cmap.size();
+    if (nblk < 1 || nblk > (1LL << 30)) { pf_free(L); return CMP_ERR_UNSUPPORTED; }
     L->nblocks = (int)nblk;
+    CMP_TRYP(hipMalloc(&L->d_cmap, sizeof(cmp::BoxBlock) * cmap.size()));
+    CMP_TRYP(hipMemcpy(L->d_cmap, cmap.data(), sizeof(cmp::BoxBlock) * cmap.size(), hipMemcpyHostToDevice));
     CMP_TRYP(hipMalloc(&L->d_sc, sizeof(double) * CG_NSC));
     CMP_TRYP(hipMemset(L->d_sc, 0, sizeof(double) * CG_NSC));
     CMP_TRYP(hipMalloc(&L->d_part, sizeof(double) * 4 * nblk));
@@ -1297,6 +1311,8 @@ int cmp_level_pf_setup(cmp_level* L, const cmp_pf_params* pf)
     L->pg.hbox = L->d_boxes;
     L->pg.abox = L->d_abox;
     L->pg.nbox = nbox;
+    L->pg.cmap = L->d_cmap;
+    L->pg.ncmap = L->nblocks;
     for (int d = 0; d < 3; ++d) {
         L->pg.dx[d] = (d < dim) ? L->G.dx[d] : 1.0;
         L->pg.dlo[d] = (d < dim) ? L->G.domain_lo[d] : 0;

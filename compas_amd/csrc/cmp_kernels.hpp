@@ -215,6 +215,14 @@ hipError_t launch_fill(int dim, int N, const FillDev& fd, long long nghost, doub
 
 // ---- Phase-Field mechanism kernels (grid.y = box) ----
 // PfGeom: per-level constants of the mechanism kernels.
+// One workgroup of a compact launch over the cells of a level's boxes: the box and the block of NT
+// cells inside it. Only blocks that hold cells are listed, so a level of many small boxes launches no
+// empty workgroups (a launch sized by its largest box would, for every smaller box).
+struct BoxBlock {
+    int box;
+    int blk;
+};
+
 struct PfGeom {
     const BoxDev* hbox;    // flow-state layout (ng ghosts, nstate comps)
     const BoxDev* abox;    // alpha layout (1 ghost, 1 comp); normals use DIM * off with the same strides
@@ -222,7 +230,29 @@ struct PfGeom {
     double dx[3];
     int dlo[3], dhi[3];    // domain index range
     int periodic[3];
+    const BoxBlock* cmap;  // the compact block list of the level's cells (null: one grid row per box)
+    int ncmap;
 };
+
+// The box and cell index of this thread in a launch over a level's cells: with a compact block list,
+// workgroup x is its entry x; without, workgroup (x, y) is block x of box y (cell_grid sets the grid
+// to match). Block sums index their partials by blockIdx.y * gridDim.x + blockIdx.x in both forms.
+__device__ __forceinline__ int cell_block(const BoxBlock* map, long long& q)
+{
+    if (map) {
+        const BoxBlock bb = map[blockIdx.x];
+        q = (long long)bb.blk * NT + threadIdx.x;
+        return bb.box;
+    }
+    q = (long long)blockIdx.x * NT + threadIdx.x;
+    return (int)blockIdx.y;
+}
+inline dim3 cell_grid(const BoxBlock* map, int nmap, long long maxcells, int nbox)
+{
+    if (map) return dim3((unsigned)(nmap > 0 ? nmap : 1));
+    const long long b = (maxcells + NT - 1) / NT;
+    return dim3((unsigned)(b > 0 ? b : 1), (unsigned)nbox);
+}
 
 // A dense domain-shaped grid of the multigrid preconditioner (cmp_mg.hip): extents n (1 in inactive
 // dimensions), cells i fastest; face arrays of direction d have extents n + e_d.
@@ -308,6 +338,8 @@ struct CompCell {
 };
 struct CompLev {
     const BoxDev* abox;     // alpha layout (1 ghost, 1 comp), goff = one-component face offsets
+    const BoxBlock* cmap;   // the compact block list of its cells (PfGeom::cmap)
+    int ncmap;
     FillDev fd;             // owner map and BCs of that layout
     const int* mask;
     double* v;              // the vector the operator reads (or the solution, for the gradient)

@@ -122,6 +122,7 @@ struct cmp_level {
     int* d_boxflag = nullptr;       // per box: a nonzero Grad_Q face (set by the Grad_Q kernels)
     int* h_boxflag = nullptr;       // pinned copy
     TileDev* d_pf_tiles = nullptr;  // tiles of the selected boxes
+    cmp::BoxBlock* d_cmap = nullptr;  // compact block list of the level's cells (PfGeom::cmap)
     TileDev* h_pf_tiles = nullptr;  // pinned staging of the list
     int pf_ntiles = 0, pf_nfill = 0;
 

@@ -82,10 +82,10 @@ __global__ __launch_bounds__(NT) void k_mg_gather_r(PfGeom g, const double* __re
                                                     const double* __restrict__ R, double* r0, MgGrid G)
 {
     if (sc && sc[CG_DONE] != 0.0) return;
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q)];
     const int n0 = ab.hi[0] - ab.lo[0] + 1, n1 = ab.hi[1] - ab.lo[1] + 1;
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 : 1;
-    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     if (q >= (long long)n0 * n1 * n2) return;
     const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
     const int k = ab.lo[2] + (int)(q / ((long long)n0 * n1));
@@ -97,10 +97,10 @@ __global__ __launch_bounds__(NT) void k_mg_scatter(PfGeom g, const double* __res
                                                    const double* __restrict__ x0, double* Z, MgGrid G)
 {
     if (sc && sc[CG_DONE] != 0.0) return;
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q)];
     const int n0 = ab.hi[0] - ab.lo[0] + 1, n1 = ab.hi[1] - ab.lo[1] + 1;
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 : 1;
-    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     if (q >= (long long)n0 * n1 * n2) return;
     const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
     const int k = ab.lo[2] + (int)(q / ((long long)n0 * n1));
@@ -345,10 +345,10 @@ __global__ __launch_bounds__(NT) void k_cg_rz(PfGeom g, const double* __restrict
                                               const double* __restrict__ Z, double* part)
 {
     if (sc && sc[CG_DONE] != 0.0) return;
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q)];
     const int n0 = ab.hi[0] - ab.lo[0] + 1, n1 = ab.hi[1] - ab.lo[1] + 1;
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 : 1;
-    const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     double w[3] = {0.0, 0.0, 0.0};
     if (q < (long long)n0 * n1 * n2) {
         const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
@@ -398,11 +398,11 @@ hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s)
 {
     switch (op) {
     case MG_GATHER_R:
-        hipLaunchKernelGGL((k_mg_gather_r<DIM>), box_grid(a.maxcells, a.g.nbox), dim3(NT), 0, s, a.g, a.sc, a.src,
+        hipLaunchKernelGGL((k_mg_gather_r<DIM>), cell_grid(a.g.cmap, a.g.ncmap, a.maxcells, a.g.nbox), dim3(NT), 0, s, a.g, a.sc, a.src,
                            a.dst, a.G);
         break;
     case MG_SCATTER:
-        hipLaunchKernelGGL((k_mg_scatter<DIM>), box_grid(a.maxcells, a.g.nbox), dim3(NT), 0, s, a.g, a.sc, a.src,
+        hipLaunchKernelGGL((k_mg_scatter<DIM>), cell_grid(a.g.cmap, a.g.ncmap, a.maxcells, a.g.nbox), dim3(NT), 0, s, a.g, a.sc, a.src,
                            a.dst, a.G);
         break;
     case MG_GATHER_B:
@@ -441,7 +441,7 @@ hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s)
                            a.id, a.G, a.omega, a.sweeps);
         break;
     case MG_CG_RZ:
-        hipLaunchKernelGGL((k_cg_rz<DIM>), box_grid(a.maxcells, a.g.nbox), dim3(NT), 0, s, a.g, a.sc, a.src, a.r,
+        hipLaunchKernelGGL((k_cg_rz<DIM>), cell_grid(a.g.cmap, a.g.ncmap, a.maxcells, a.g.nbox), dim3(NT), 0, s, a.g, a.sc, a.src, a.r,
                            a.part);
         break;
     default:

@@ -297,10 +297,11 @@ __global__ __launch_bounds__(NT) void k_pf_cac_mech(PfGeom g, const double* __re
                                                     const double* __restrict__ Nrm, double* Mk, double* Wk,
                                                     double mob, double lambda, double eta, double* part)
 {
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
     double mk = 0.0, wk = 0.0;
-    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+    if (cell_of<DIM>(ab, q_, i, j, k)) {
         const double* Ab = A + ab.off;
         const double* Nb = Nrm + (long long)DIM * ab.off;
         const long long c = fab_index(ab, i, j, k);
@@ -332,9 +333,10 @@ __global__ __launch_bounds__(NT) void k_pf_cac_update(PfGeom g, double* A, const
                                                       const double* __restrict__ Wk,
                                                       const double* __restrict__ sums, double dt_m)
 {
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
-    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    if (!cell_of<DIM>(ab, q_, i, j, k)) return;
     const double sw = sums[1];
     const double L = (sw == 0.0) ? 0.0 : -sums[0] / sw;
     const long long c = ab.off + fab_index(ab, i, j, k);
@@ -348,9 +350,10 @@ __global__ __launch_bounds__(NT) void k_pf_divjhat(PfGeom g, const double* __res
                                                    const double* __restrict__ A0, double* DJ, double a,
                                                    double b)
 {
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
-    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    if (!cell_of<DIM>(ab, q_, i, j, k)) return;
     const long long c = ab.off + fab_index(ab, i, j, k);
     DJ[c] = a * A[c] + b * A0[c];
 }
@@ -376,9 +379,10 @@ __global__ __launch_bounds__(NT) void k_pf_bcoef(PfGeom g, const double* __restr
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_cg_idiag(PfGeom g, const double* __restrict__ Bf, double* ID)
 {
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
-    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    if (!cell_of<DIM>(ab, q_, i, j, k)) return;
     const int co[3] = {i, j, k};
     double dsum = 0.0;
 #pragma unroll
@@ -398,10 +402,11 @@ __global__ __launch_bounds__(NT) void k_cg_init(PfGeom g, const double* __restri
                                                 const double* __restrict__ ID, double* X, double* R,
                                                 double* Z, double* P, double* part)
 {
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
     double v = 0.0, w = 0.0;
-    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+    if (cell_of<DIM>(ab, q_, i, j, k)) {
         const long long c = ab.off + fab_index(ab, i, j, k);
         const double mean = mean_sum[0] / ncell;
         const double r = -(rhs[c] - mean);
@@ -422,10 +427,11 @@ __global__ __launch_bounds__(NT) void k_cg_init(PfGeom g, const double* __restri
 template <int DIM>
 __global__ __launch_bounds__(NT) void k_cell_sum(PfGeom g, const double* __restrict__ F, double* part)
 {
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
     double v = 0.0;
-    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) v = F[ab.off + fab_index(ab, i, j, k)];
+    if (cell_of<DIM>(ab, q_, i, j, k)) v = F[ab.off + fab_index(ab, i, j, k)];
     block_sum_to(v, part, (long long)blockIdx.y * gridDim.x + blockIdx.x);
 }
 
@@ -435,10 +441,11 @@ __global__ __launch_bounds__(NT) void k_cg_ap(PfGeom g, const double* __restrict
                                               double* AP, double* part)
 {
     if (sc[CG_DONE] != 0.0) return;   // uniform over the grid
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
     double v = 0.0;
-    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+    if (cell_of<DIM>(ab, q_, i, j, k)) {
         const double* Pb = P + ab.off;
         const long long c = fab_index(ab, i, j, k);
         const long long st[3] = {1, ab.sj, ab.sk};
@@ -467,10 +474,11 @@ __global__ __launch_bounds__(NT) void k_cg_xr(PfGeom g, const double* __restrict
                                               double* part)
 {
     if (sc[CG_DONE] != 0.0 || !(sc[CG_PAP] > 0.0)) return;
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
     double v = 0.0, w = 0.0;
-    if (cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) {
+    if (cell_of<DIM>(ab, q_, i, j, k)) {
         const long long c = ab.off + fab_index(ab, i, j, k);
         const double al = sc[CG_RZ] / sc[CG_PAP];
         X[c] += al * P[c];
@@ -500,9 +508,10 @@ __global__ __launch_bounds__(NT) void k_cg_p(PfGeom g, const double* __restrict_
                                              const int* __restrict__ mask, int first)
 {
     if (sc[CG_DONE] != 0.0 || (!first && !(sc[CG_PAP] > 0.0))) return;
-    const BoxDev ab = g.abox[blockIdx.y];
+    long long q_;
+    const BoxDev ab = g.abox[cell_block(g.cmap, q_)];
     int i, j, k;
-    if (!cell_of<DIM>(ab, (long long)blockIdx.x * NT + threadIdx.x, i, j, k)) return;
+    if (!cell_of<DIM>(ab, q_, i, j, k)) return;
     const long long c = ab.off + fab_index(ab, i, j, k);
     const bool covered = mask && mask[c] == COMP_COVERED;
     const double zc = (inv_n > 0.0 && !covered) ? Z[c] - sc[CG_ZS] * inv_n : Z[c];
@@ -545,7 +554,7 @@ hipError_t launch_pf_cac_mech_dim(const PfGeom& g, long long maxcells, const dou
                                   double* Mk, double* Wk, double mob, double lambda, double eta,
                                   double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_cac_mech<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, A, Nrm, Mk,
+    hipLaunchKernelGGL((k_pf_cac_mech<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, A, Nrm, Mk,
                        Wk, mob, lambda, eta, part);
     return hipGetLastError();
 }
@@ -554,7 +563,7 @@ template <int DIM>
 hipError_t launch_pf_cac_update_dim(const PfGeom& g, long long maxcells, double* A, const double* Mk,
                                     const double* Wk, const double* sums, double dt_m, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_cac_update<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, A, Mk, Wk,
+    hipLaunchKernelGGL((k_pf_cac_update<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, A, Mk, Wk,
                        sums, dt_m);
     return hipGetLastError();
 }
@@ -563,7 +572,7 @@ template <int DIM>
 hipError_t launch_pf_divjhat_dim(const PfGeom& g, long long maxcells, const double* A, const double* A0,
                                  double* DJ, double dt, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_pf_divjhat<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, A, A0, DJ,
+    hipLaunchKernelGGL((k_pf_divjhat<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, A, A0, DJ,
                        1.0 / dt, -1.0 / dt);
     return hipGetLastError();
 }
@@ -583,14 +592,14 @@ hipError_t launch_pf_bcoef_dim(const PfGeom& g, const long long* maxfaces, const
 template <int DIM>
 hipError_t launch_cell_sum_dim(const PfGeom& g, long long maxcells, const double* F, double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cell_sum<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, F, part);
+    hipLaunchKernelGGL((k_cell_sum<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, F, part);
     return hipGetLastError();
 }
 
 template <int DIM>
 hipError_t launch_cg_idiag_dim(const PfGeom& g, long long maxcells, const double* Bf, double* ID, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_idiag<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, Bf, ID);
+    hipLaunchKernelGGL((k_cg_idiag<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, Bf, ID);
     return hipGetLastError();
 }
 
@@ -599,7 +608,7 @@ hipError_t launch_cg_init_dim(const PfGeom& g, long long maxcells, const double*
                               double ncell, const double* ID, double* X, double* R, double* Z, double* P,
                               double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_init<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, rhs, mean_sum,
+    hipLaunchKernelGGL((k_cg_init<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, rhs, mean_sum,
                        ncell, ID, X, R, Z, P, part);
     return hipGetLastError();
 }
@@ -608,7 +617,7 @@ template <int DIM>
 hipError_t launch_cg_ap_dim(const PfGeom& g, long long maxcells, const double* sc, const double* Bf,
                             const double* P, double* AP, double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_ap<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, Bf, P, AP, part);
+    hipLaunchKernelGGL((k_cg_ap<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, sc, Bf, P, AP, part);
     return hipGetLastError();
 }
 
@@ -616,7 +625,7 @@ template <int DIM>
 hipError_t launch_cg_xr_dim(const PfGeom& g, long long maxcells, const double* sc, const double* ID, double* X,
                             double* R, double* Z, const double* P, const double* AP, double* part, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_xr<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, ID, X, R, Z, P, AP,
+    hipLaunchKernelGGL((k_cg_xr<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, sc, ID, X, R, Z, P, AP,
                        part);
     return hipGetLastError();
 }
@@ -625,7 +634,7 @@ template <int DIM>
 hipError_t launch_cg_p_dim(const PfGeom& g, long long maxcells, const double* sc, const double* Z, double* P,
                            double inv_n, const int* mask, int first, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg_p<DIM>), grid_of_cells(maxcells, g.nbox), dim3(NT), 0, s, g, sc, Z, P, inv_n, mask,
+    hipLaunchKernelGGL((k_cg_p<DIM>), cell_grid(g.cmap, g.ncmap, maxcells, g.nbox), dim3(NT), 0, s, g, sc, Z, P, inv_n, mask,
                        first);
     return hipGetLastError();
 }
