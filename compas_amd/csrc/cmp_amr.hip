@@ -952,20 +952,17 @@ int amr_comp_solve(cmp_amr* A, double* const* out, hipStream_t s0)
     // the composite cells (the operator's null space)
     const double inv_n = (mg && P.ncomp > 0) ? 1.0 / (double)P.ncomp : 0.0;
     int r;
-    // the level sums of nv partial slots from slot slot0 on
+    // the level sums of nv partial slots from slot slot0 on, and their sum over the levels (one launch)
     auto sum_levels = [&](int nv, double* dst, const double* scx, int slot0 = 0) -> int {
+        LevelParts lp{};
+        lp.nlev = nlev;
         for (int l = 0; l < nlev; ++l) {
             const cmp_level* L = A->L[l];
-            CMP_TRY(launch_reduce_sums(L->d_part + (long long)slot0 * L->nblocks, L->nblocks, nv, P.d_lsum + nv * l,
-                                       s, scx));
+            lp.part[l] = L->d_part + (long long)slot0 * L->nblocks;
+            lp.nblocks[l] = L->nblocks;
         }
-        AmrPfArgs a{};
-        a.src = P.d_lsum;
-        a.nlev = nlev;
-        a.nvals = nv;
-        a.dst = dst;
-        a.sc = scx;
-        return ap_launch(dim, AP_SUM_LEVELS, a, s);
+        CMP_TRY(launch_reduce_levels(lp, nv, P.d_lsum, dst, s, scx));
+        return CMP_OK;
     };
     // z = R ID (levels above 0, Jacobi) + the level-0 V-cycle of the restricted R/V at every cell's
     // level-0 ancestor; partials of r.z, sum z and sum r into each level's second to fourth slots

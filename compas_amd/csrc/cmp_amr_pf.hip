@@ -202,18 +202,6 @@ __global__ __launch_bounds__(NT) void k_maxabs(long long n, const double* __rest
     }
 }
 
-// sums[v] = sum over the levels of per[l * nvals + v]
-__global__ void k_sum_levels(const double* __restrict__ per, int nlev, int nvals, double* sums, const double* sc)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (sc && sc[CG_DONE] != 0.0) return;
-    for (int v = 0; v < nvals; ++v) {
-        double s = per[v];
-        for (int l = 1; l < nlev; ++l) s += per[l * nvals + v];
-        sums[v] = s;
-    }
-}
-
 }  // namespace
 
 // the composite operator on level l's valid cells: ap = A p (0 on covered cells), partials of p.ap;
@@ -416,9 +404,6 @@ hipError_t launch_amrpf_dim(int op, const AmrPfArgs& p, hipStream_t s)
         break;
     case AP_MAXABS:
         hipLaunchKernelGGL(k_maxabs, flat_grid(p.n), dim3(NT), 0, s, p.n, p.src, p.bits);
-        break;
-    case AP_SUM_LEVELS:
-        hipLaunchKernelGGL(k_sum_levels, dim3(1), dim3(64), 0, s, p.src, p.nlev, p.nvals, p.dst, p.sc);
         break;
     case AP_COMP_AP:
         hipLaunchKernelGGL((k_comp_ap<DIM>), comp_grid(p), dim3(NT), 0, s, *p.ca, p.lev, p.sc,

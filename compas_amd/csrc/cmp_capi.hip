@@ -1423,6 +1423,37 @@ int cmp::mg_vcycle(cmp_level* L, const double* sc, size_t m, hipStream_t s)
             t.id[q] = W.id;
             for (int d = 0; d < 3; ++d) t.b[q][d] = W.b[d];
         }
+        // the coarsest grids that fit the tail's LDS together (never the first, whose result the
+        // caller reads from device memory)
+        auto need = [&](int q) -> long long {
+            long long n = 4 * t.G[q].nc;
+            for (int d = 0; d < dim; ++d) {
+                long long nf = 1;
+                for (int e = 0; e < dim; ++e) nf *= t.G[q].n[e] + (e == d);
+                n += nf;
+            }
+            return n;
+        };
+        t.lds_from = t.nlev;
+        long long used = 0;
+        for (int q = t.nlev - 1; q >= 1 && used + need(q) <= cmp::MG_TAIL_LDS; --q) {
+            used += need(q);
+            t.lds_from = q;
+        }
+        long long off = 0;
+        for (int q = t.lds_from; q < t.nlev; ++q) {
+            const long long nc = t.G[q].nc;
+            for (int a = 0; a < 4; ++a) t.loff[q][a] = (int)(off + a * nc);
+            off += 4 * nc;
+            for (int d = 0; d < 3; ++d) {
+                t.loff[q][4 + d] = (int)off;
+                if (d < dim) {
+                    long long nf = 1;
+                    for (int e = 0; e < dim; ++e) nf *= t.G[q].n[e] + (e == d);
+                    off += nf;
+                }
+            }
+        }
         CMP_TRY(dim == 2 ? cmp::launch_mg_tail_dim<2>(t, s) : cmp::launch_mg_tail_dim<3>(t, s));
         V.cur = (t.nlev == 1) ? 0 : 1;
         return CMP_OK;

@@ -300,8 +300,12 @@ hipError_t launch_mg_dim(int op, const MgArgs& a, hipStream_t s);
 // the residual of grid 0 ready in r[0]; the same operations in the same order as the per-grid
 // launches (two damped Jacobi sweeps, residual restriction, the coarser grids, prolongation, two
 // sweeps; the last grid `sweeps` sweeps from zero), the result in x1[0] (x0[0] when nlev is 1).
+// Grids lds_from..nlev-1 (lds_from >= 1) live in the workgroup's LDS for the whole tail (their
+// iterates, residual, inverse diagonal and face coefficients at the double offsets loff[m][0..6]:
+// x0, x1, r, id, b0, b1, b2; id and b copied in first), the others in device memory.
 constexpr int MG_TAIL_MAX_LEVELS = 16;
 constexpr long long MG_TAIL_MAX_CELLS = 4096;   // grids at most this size go to the tail
+constexpr int MG_TAIL_LDS = 9216;               // doubles of LDS for the tail's coarsest grids
 struct MgTail {
     int nlev;
     int sweeps;
@@ -313,6 +317,8 @@ struct MgTail {
     double* r[MG_TAIL_MAX_LEVELS];
     const double* id[MG_TAIL_MAX_LEVELS];
     const double* b[MG_TAIL_MAX_LEVELS][3];
+    int lds_from;
+    int loff[MG_TAIL_MAX_LEVELS][7];
 };
 template <int DIM>
 hipError_t launch_mg_tail_dim(const MgTail& t, hipStream_t s);
@@ -360,7 +366,7 @@ struct CompArgs {
 };
 
 enum AmrPfOp {
-    AP_AVGDOWN_FACES, AP_AXPY, AP_MAXABS, AP_SUM_LEVELS, AP_COMP_AP, AP_COMP_DIAG, AP_COMP_INIT, AP_COMP_S,
+    AP_AVGDOWN_FACES, AP_AXPY, AP_MAXABS, AP_COMP_AP, AP_COMP_DIAG, AP_COMP_INIT, AP_COMP_S,
     AP_COMP_Z, AP_COMP_GRAD
 };
 
@@ -491,6 +497,16 @@ hipError_t launch_pf_gradx(int dim, const PfGeom& g, const long long* maxfaces, 
 // CG's stop flag sc[CG_DONE] is set
 hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s,
                               const double* sc = nullptr);
+// The partial sums of every level of a composite solve in one launch: per level the same fixed-order
+// reduction as launch_reduce_sums (into per[l * nvals + v]), then sums[v] = the levels' values added
+// in level order.
+struct LevelParts {
+    const double* part[AMR_MAX_LEVELS];
+    int nblocks[AMR_MAX_LEVELS];
+    int nlev;
+};
+hipError_t launch_reduce_levels(const LevelParts& lp, int nvals, double* per, double* sums, hipStream_t s,
+                                const double* sc = nullptr);
 hipError_t launch_scale(double* p, long long n, double s_, hipStream_t s);
 hipError_t launch_setd(double* p, double v, hipStream_t s);
 hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s);

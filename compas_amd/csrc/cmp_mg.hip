@@ -258,37 +258,62 @@ __global__ __launch_bounds__(1024) void k_mg_bottom(const double* __restrict__ s
 }
 
 // The coarse end of the V-cycle (MgTail) in one workgroup of 1024 threads, every operation a
-// strided loop over the grid's cells followed by a barrier.
+// strided loop over the grid's cells followed by a barrier. The coarsest grids (from t.lds_from on)
+// are kept in LDS, so that their many short phases wait on LDS rather than on device memory; the
+// arithmetic is the same wherever a grid lives.
 constexpr int NTAIL = 1024;
 template <int DIM>
 __global__ __launch_bounds__(NTAIL) void k_mg_tail(MgTail t)
 {
     if (t.sc && t.sc[CG_DONE] != 0.0) return;
+    __shared__ double lds[MG_TAIL_LDS];
     const int tid = threadIdx.x;
     const double om = t.omega;
     const int last = t.nlev - 1;
+    auto in_lds = [&](int m) { return m >= t.lds_from; };
+    auto X0 = [&](int m) -> double* { return in_lds(m) ? lds + t.loff[m][0] : t.x0[m]; };
+    auto X1 = [&](int m) -> double* { return in_lds(m) ? lds + t.loff[m][1] : t.x1[m]; };
+    auto Rr = [&](int m) -> double* { return in_lds(m) ? lds + t.loff[m][2] : t.r[m]; };
+    auto ID = [&](int m) -> const double* { return in_lds(m) ? lds + t.loff[m][3] : t.id[m]; };
+    auto Bf = [&](int m, int d) -> const double* { return in_lds(m) ? lds + t.loff[m][4 + d] : t.b[m][d]; };
+    // the LDS grids' inverse diagonals and face coefficients
+    for (int m = t.lds_from; m < t.nlev; ++m) {
+        const MgGrid& G = t.G[m];
+        for (long long q = tid; q < G.nc; q += NTAIL) lds[t.loff[m][3] + q] = t.id[m][q];
+        for (int d = 0; d < DIM; ++d) {
+            long long nf = 1;
+            for (int e = 0; e < DIM; ++e) nf *= G.n[e] + (e == d);
+            for (long long q = tid; q < nf; q += NTAIL) lds[t.loff[m][4 + d] + q] = t.b[m][d][q];
+        }
+    }
+    __syncthreads();
     // y = x + omega id (r - A x) on grid m (zero_x: y = omega id r)
     auto smooth = [&](int m, const double* x, double* y, bool zero_x) {
         const MgGrid& G = t.G[m];
-        const double* r = t.r[m];
-        const double* id = t.id[m];
+        const double* r = Rr(m);
+        const double* id = ID(m);
+        const double* bb[3] = {Bf(m, 0), Bf(m, 1), Bf(m, DIM == 3 ? 2 : 1)};
         for (long long q = tid; q < G.nc; q += NTAIL) {
             if (zero_x) {
                 y[q] = om * id[q] * r[q];
             } else {
                 int i, j, k;
                 mg_cell<DIM>(G, q, i, j, k);
-                y[q] = x[q] + om * id[q] * (r[q] - mg_apply<DIM>(G, x, t.b[m], i, j, k));
+                y[q] = x[q] + om * id[q] * (r[q] - mg_apply<DIM>(G, x, bb, i, j, k));
             }
         }
         __syncthreads();
     };
     for (int m = 0; m < last; ++m) {
-        smooth(m, nullptr, t.x0[m], true);
-        smooth(m, t.x0[m], t.x1[m], false);
+        smooth(m, nullptr, X0(m), true);
+        smooth(m, X0(m), X1(m), false);
         // r of grid m + 1: the mean of the children of r - A x1
         const MgGrid& Gf = t.G[m];
         const MgGrid& Gc = t.G[m + 1];
+        const double* rf = Rr(m);
+        const double* x1 = X1(m);
+        const double* bb[3] = {Bf(m, 0), Bf(m, 1), Bf(m, DIM == 3 ? 2 : 1)};
+        double* rc = Rr(m + 1);
         constexpr int NCH = (DIM == 3) ? 8 : 4;
         for (long long q = tid; q < Gc.nc; q += NTAIL) {
             int I, J, K;
@@ -298,9 +323,9 @@ __global__ __launch_bounds__(NTAIL) void k_mg_tail(MgTail t)
             for (int ch = 0; ch < NCH; ++ch) {
                 const int i = 2 * I + (ch & 1), j = 2 * J + ((ch >> 1) & 1), k = (DIM == 3) ? 2 * K + (ch >> 2) : 0;
                 const long long c = mg_idx(Gf, i, j, k);
-                sum += t.r[m][c] - mg_apply<DIM>(Gf, t.x1[m], t.b[m], i, j, k);
+                sum += rf[c] - mg_apply<DIM>(Gf, x1, bb, i, j, k);
             }
-            t.r[m + 1][q] = sum * (1.0 / (double)NCH);
+            rc[q] = sum * (1.0 / (double)NCH);
         }
         __syncthreads();
     }
@@ -308,8 +333,8 @@ __global__ __launch_bounds__(NTAIL) void k_mg_tail(MgTail t)
     // copied to x0
     {
         const MgGrid& G = t.G[last];
-        double* xa = t.x0[last];
-        double* xb = t.x1[last];
+        double* xa = X0(last);
+        double* xb = X1(last);
         smooth(last, nullptr, xa, true);
         for (int sw = 1; sw < t.sweeps; ++sw) {
             smooth(last, xa, xb, false);
@@ -317,8 +342,9 @@ __global__ __launch_bounds__(NTAIL) void k_mg_tail(MgTail t)
             xa = xb;
             xb = tmp;
         }
-        if (xa != t.x0[last]) {
-            for (long long q = tid; q < G.nc; q += NTAIL) t.x0[last][q] = xa[q];
+        if (xa != X0(last)) {
+            double* x0l = X0(last);
+            for (long long q = tid; q < G.nc; q += NTAIL) x0l[q] = xa[q];
             __syncthreads();
         }
     }
@@ -326,15 +352,16 @@ __global__ __launch_bounds__(NTAIL) void k_mg_tail(MgTail t)
         // x1 += the coarser grid's result at the parent cell, then two sweeps
         const MgGrid& Gf = t.G[m];
         const MgGrid& Gc = t.G[m + 1];
-        const double* xc = (m + 1 == last) ? t.x0[m + 1] : t.x1[m + 1];
+        const double* xc = (m + 1 == last) ? X0(m + 1) : X1(m + 1);
+        double* x1 = X1(m);
         for (long long q = tid; q < Gf.nc; q += NTAIL) {
             int i, j, k;
             mg_cell<DIM>(Gf, q, i, j, k);
-            t.x1[m][q] += xc[mg_idx(Gc, i >> 1, j >> 1, (DIM == 3) ? (k >> 1) : 0)];
+            x1[q] += xc[mg_idx(Gc, i >> 1, j >> 1, (DIM == 3) ? (k >> 1) : 0)];
         }
         __syncthreads();
-        smooth(m, t.x1[m], t.x0[m], false);
-        smooth(m, t.x0[m], t.x1[m], false);
+        smooth(m, X1(m), X0(m), false);
+        smooth(m, X0(m), X1(m), false);
     }
 }
 

@@ -773,35 +773,63 @@ hipError_t launch_zero_u64(unsigned long long* p, int n, hipStream_t s)
 // (each thread four interleaved running sums, then the wave and workgroup sums in index order).
 // sc non-null: nothing once the solve's stop flag is set (the queued iterations past the stop).
 constexpr int NRED = 1024;
+// The fixed-order sum of nblocks partials p[0..nblocks) by the NRED threads of one workgroup: thread
+// t adds p[t], p[t + NRED], ... in four interleaved accumulators, then the waves and the workgroup
+// add their values in a fixed tree; the result is returned to thread 0.
+__device__ __forceinline__ double reduce_block_parts(const double* __restrict__ p, int nblocks, double* red)
+{
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int b = threadIdx.x;
+    for (; b + 3 * NRED < nblocks; b += 4 * NRED) {
+        a0 += p[b];
+        a1 += p[b + NRED];
+        a2 += p[b + 2 * NRED];
+        a3 += p[b + 3 * NRED];
+    }
+    for (; b < nblocks; b += NRED) a0 += p[b];
+    double t = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    double u = 0.0;
+    if (threadIdx.x == 0) {
+        u = red[0];
+#pragma unroll
+        for (int w = 1; w < NRED / 64; ++w) u += red[w];
+    }
+    __syncthreads();
+    return u;
+}
+
 __global__ __launch_bounds__(NRED) void k_reduce_sums(const double* __restrict__ part, int nblocks, int nvals,
                                                       double* sums, const double* __restrict__ sc)
 {
     if (sc && sc[CG_DONE] != 0.0) return;
     __shared__ double red[NRED / 64];
     for (int v = 0; v < nvals; ++v) {
-        const double* __restrict__ p = part + (long long)v * nblocks;
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int b = threadIdx.x;
-        for (; b + 3 * NRED < nblocks; b += 4 * NRED) {
-            a0 += p[b];
-            a1 += p[b + NRED];
-            a2 += p[b + 2 * NRED];
-            a3 += p[b + 3 * NRED];
-        }
-        for (; b < nblocks; b += NRED) a0 += p[b];
-        double t = (a0 + a1) + (a2 + a3);
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = t;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double u = red[0];
-#pragma unroll
-            for (int w = 1; w < NRED / 64; ++w) u += red[w];
-            sums[v] = u;
-        }
-        __syncthreads();
+        const double u = reduce_block_parts(part + (long long)v * nblocks, nblocks, red);
+        if (threadIdx.x == 0) sums[v] = u;
     }
+}
+
+// every level's sums (per[l * nvals + v], as k_reduce_sums leaves them) and their sum in level order
+__global__ __launch_bounds__(NRED) void k_reduce_levels(LevelParts lp, int nvals, double* per, double* sums,
+                                                        const double* __restrict__ sc)
+{
+    if (sc && sc[CG_DONE] != 0.0) return;
+    __shared__ double red[NRED / 64];
+    for (int l = 0; l < lp.nlev; ++l)
+        for (int v = 0; v < nvals; ++v) {
+            const double u = reduce_block_parts(lp.part[l] + (long long)v * lp.nblocks[l], lp.nblocks[l], red);
+            if (threadIdx.x == 0) per[l * nvals + v] = u;
+        }
+    if (threadIdx.x == 0)
+        for (int v = 0; v < nvals; ++v) {
+            double t = per[v];
+            for (int l = 1; l < lp.nlev; ++l) t += per[l * nvals + v];
+            sums[v] = t;
+        }
 }
 
 // CG bookkeeping on the device, so that the iteration stops where the host loop of the solve's
@@ -845,6 +873,14 @@ __global__ void k_cg_end(double* sc, double maxit, double inv_n)
     }
     if (!(rr > 1.0e-28 * sc[CG_B2]) || !(it < maxit)) done = true;
     if (done) sc[CG_DONE] = 1.0;
+}
+
+hipError_t launch_reduce_levels(const LevelParts& lp, int nvals, double* per, double* sums, hipStream_t s,
+                                const double* sc)
+{
+    if (lp.nlev < 1 || lp.nlev > AMR_MAX_LEVELS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_reduce_levels, dim3(1), dim3(NRED), 0, s, lp, nvals, per, sums, sc);
+    return hipGetLastError();
 }
 
 hipError_t launch_reduce_sums(const double* part, int nblocks, int nvals, double* sums, hipStream_t s,
