@@ -51,9 +51,10 @@ __device__ __forceinline__ bool ap_cell(const BoxDev& bx, long long q, int& i, i
     const int n0 = bx.hi[0] - bx.lo[0] + 1, n1 = bx.hi[1] - bx.lo[1] + 1;
     const int n2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
     if (q >= (long long)n0 * n1 * n2) return false;
-    i = bx.lo[0] + (int)(q % n0);
-    j = bx.lo[1] + (int)((q / n0) % n1);
-    k = bx.lo[2] + (int)(q / ((long long)n0 * n1));
+    pos3(q, n0, n1, i, j, k);
+    i += bx.lo[0];
+    j += bx.lo[1];
+    k += bx.lo[2];
     return true;
 }
 
@@ -143,8 +144,11 @@ __global__ __launch_bounds__(NT) void k_avgdown_faces(const BoxDev* __restrict__
     }
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     if (q >= (long long)ce[0] * ce[1] * ce[2]) return;
-    const int C[3] = {clo[0] + (int)(q % ce[0]), clo[1] + (int)((q / ce[0]) % ce[1]),
-                      clo[2] + (int)(q / ((long long)ce[0] * ce[1]))};
+    int C[3];
+    pos3(q, ce[0], ce[1], C[0], C[1], C[2]);
+    C[0] += clo[0];
+    C[1] += clo[1];
+    C[2] += clo[2];
     const int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
     const int r1 = ratio, r2 = (DIM == 3) ? ratio : 1;
     double s = 0.0;
@@ -328,7 +332,11 @@ __global__ __launch_bounds__(NT) void k_comp_grad(CompArgs a, int l, double* gq)
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 + (D == 2) : 1;
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     if (q >= (long long)n0 * n1 * n2) return;
-    int c[3] = {ab.lo[0] + (int)(q % n0), ab.lo[1] + (int)((q / n0) % n1), ab.lo[2] + (int)(q / ((long long)n0 * n1))};
+    int c[3];
+    pos3(q, n0, n1, c[0], c[1], c[2]);
+    c[0] += ab.lo[0];
+    c[1] += ab.lo[1];
+    c[2] += ab.lo[2];
     if (DIM == 2) c[2] = ab.lo[2];
     const long long fi = ab.goff[D] + q;
     if (!L.fd.periodic[D] && (c[D] == L.fd.dlo[D] || c[D] == L.fd.dhi[D] + 1)) { gq[fi] = 0.0; return; }

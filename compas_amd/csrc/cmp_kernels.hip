@@ -769,9 +769,8 @@ void k_faces(KParams kp, int limiter, BoxDev bx, const double* __restrict__ U,
     const long long F = (long long)fe0 * fe1 * fe2;
     const long long f = (long long)blockIdx.x * NT + threadIdx.x;
     if (f >= F) return;
-    const int x = (int)(f % fe0);
-    const int y = (int)((f / fe0) % fe1);
-    const int z = (int)(f / ((long long)fe0 * fe1));
+    int x, y, z;
+    pos3(f, fe0, fe1, x, y, z);
     int bad = 0;
     double Fl[N];
     double cm = 0.0;
@@ -1020,9 +1019,11 @@ void k_lincomb(const BoxDev* __restrict__ boxes, int N, double* out, double p, c
     for (long long t = (long long)blockIdx.x * NT + threadIdx.x; t < ncell * N; t += (long long)gridDim.x * NT) {
         const int c = (int)(t / ncell);
         const long long v = t - (long long)c * ncell;
-        const int i = bx.lo[0] + (int)(v % n0);
-        const int j = bx.lo[1] + (int)((v / n0) % n1);
-        const int k = bx.lo[2] + (int)(v / ((long long)n0 * n1));
+        int i, j, k;
+        pos3(v, n0, n1, i, j, k);
+        i += bx.lo[0];
+        j += bx.lo[1];
+        k += bx.lo[2];
         const long long o = bx.off + fab_index(bx, i, j, k) + c * bx.sn;
         double w;
         if (Y && L) {
@@ -1262,7 +1263,8 @@ __global__ __launch_bounds__(NT) void k_tag(KParams kp, const BoxDev* __restrict
     const int n2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     if (q >= (long long)n0 * n1 * n2) return;
-    const int ii = (int)(q % n0), jj = (int)((q / n0) % n1), kk = (int)(q / ((long long)n0 * n1));
+    int ii, jj, kk;
+    pos3(q, n0, n1, ii, jj, kk);
     const int i = bx.lo[0] + ii, j = bx.lo[1] + jj, k = bx.lo[2] + kk;
     const double* __restrict__ Ub = U + bx.off;
     int bad = 0;
@@ -1326,9 +1328,11 @@ __global__ __launch_bounds__(NT) void k_avgdown(const BoxDev* __restrict__ fboxe
     }
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     if (q >= (long long)ce[0] * ce[1] * ce[2]) return;
-    const int ic = clo[0] + (int)(q % ce[0]);
-    const int jc = clo[1] + (int)((q / ce[0]) % ce[1]);
-    const int kc = clo[2] + (int)(q / ((long long)ce[0] * ce[1]));
+    int ic, jc, kc;
+    pos3(q, ce[0], ce[1], ic, jc, kc);
+    ic += clo[0];
+    jc += clo[1];
+    kc += clo[2];
     const int cc[3] = {ic, jc, kc};
     int bi = 0;
     for (int d = DIM - 1; d >= 0; --d) {

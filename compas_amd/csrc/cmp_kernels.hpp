@@ -37,6 +37,18 @@ struct TileDev {
 
 constexpr int NT = 256;   // threads per workgroup of every kernel here
 
+// (x, y, z) of position p < n0 n1 n2 of a block of n0 x n1 x n2 positions, x fastest; 32-bit
+// arithmetic (every box's and dense grid's position count is below 2^31, which level creation
+// checks), where 64-bit division would cost an emulated sequence per thread
+__device__ __forceinline__ void pos3(long long p, int n0, int n1, int& x, int& y, int& z)
+{
+    const int q = (int)p;
+    const int a = q / n0;
+    x = q - a * n0;
+    z = a / n1;
+    y = a - z * n1;
+}
+
 __device__ __forceinline__ long long fab_index(const BoxDev& bx, int i, int j, int k)
 {
     return (long long)(i - bx.flo[0]) + bx.sj * (j - bx.flo[1]) + bx.sk * (k - bx.flo[2]);

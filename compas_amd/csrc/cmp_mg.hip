@@ -32,9 +32,8 @@ __device__ __forceinline__ double mg_wave_sum(double v)
 template <int DIM>
 __device__ __forceinline__ void mg_cell(const MgGrid& G, long long q, int& i, int& j, int& k)
 {
-    i = (int)(q % G.n[0]);
-    j = (int)((q / G.n[0]) % G.n[1]);
-    k = (DIM == 3) ? (int)(q / ((long long)G.n[0] * G.n[1])) : 0;
+    pos3(q, G.n[0], G.n[1], i, j, k);
+    if (DIM != 3) k = 0;
 }
 
 __device__ __forceinline__ long long mg_idx(const MgGrid& G, int i, int j, int k)
@@ -87,8 +86,11 @@ __global__ __launch_bounds__(NT) void k_mg_gather_r(PfGeom g, const double* __re
     const int n0 = ab.hi[0] - ab.lo[0] + 1, n1 = ab.hi[1] - ab.lo[1] + 1;
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 : 1;
     if (q >= (long long)n0 * n1 * n2) return;
-    const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
-    const int k = ab.lo[2] + (int)(q / ((long long)n0 * n1));
+    int i, j, k;
+    pos3(q, n0, n1, i, j, k);
+    i += ab.lo[0];
+    j += ab.lo[1];
+    k += ab.lo[2];
     r0[mg_idx(G, i - g.dlo[0], j - g.dlo[1], (DIM == 3) ? k - g.dlo[2] : 0)] = R[ab.off + fab_index(ab, i, j, k)];
 }
 
@@ -102,8 +104,11 @@ __global__ __launch_bounds__(NT) void k_mg_scatter(PfGeom g, const double* __res
     const int n0 = ab.hi[0] - ab.lo[0] + 1, n1 = ab.hi[1] - ab.lo[1] + 1;
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 : 1;
     if (q >= (long long)n0 * n1 * n2) return;
-    const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
-    const int k = ab.lo[2] + (int)(q / ((long long)n0 * n1));
+    int i, j, k;
+    pos3(q, n0, n1, i, j, k);
+    i += ab.lo[0];
+    j += ab.lo[1];
+    k += ab.lo[2];
     Z[ab.off + fab_index(ab, i, j, k)] = x0[mg_idx(G, i - g.dlo[0], j - g.dlo[1], (DIM == 3) ? k - g.dlo[2] : 0)];
 }
 
@@ -116,8 +121,11 @@ __global__ __launch_bounds__(NT) void k_mg_gather_b(PfGeom g, const double* __re
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 + (D == 2) : 1;
     const long long q = (long long)blockIdx.x * NT + threadIdx.x;
     if (q >= (long long)n0 * n1 * n2) return;
-    const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
-    const int k = ab.lo[2] + (int)(q / ((long long)n0 * n1));
+    int i, j, k;
+    pos3(q, n0, n1, i, j, k);
+    i += ab.lo[0];
+    j += ab.lo[1];
+    k += ab.lo[2];
     b0[mg_face(G, D, i - g.dlo[0], j - g.dlo[1], (DIM == 3) ? k - g.dlo[2] : 0)] = Bf[ab.goff[D] + q];
 }
 
@@ -133,7 +141,9 @@ __global__ __launch_bounds__(NT) void k_mg_restrict_b(const double* __restrict__
     const int t2 = (d == 2) ? 1 : 2;
     constexpr int NS = (DIM == 3) ? 4 : 2;
     for (long long q = (long long)blockIdx.x * NT + threadIdx.x; q < nf; q += (long long)gridDim.x * NT) {
-        const int co[3] = {(int)(q % f0), (int)((q / f0) % f1), (DIM == 3) ? (int)(q / ((long long)f0 * f1)) : 0};
+        int co[3];
+        pos3(q, f0, f1, co[0], co[1], co[2]);
+        if (DIM != 3) co[2] = 0;
         double s = 0.0;
 #pragma unroll
         for (int m = 0; m < NS; ++m) {
@@ -378,8 +388,11 @@ __global__ __launch_bounds__(NT) void k_cg_rz(PfGeom g, const double* __restrict
     const int n2 = (DIM == 3) ? ab.hi[2] - ab.lo[2] + 1 : 1;
     double w[3] = {0.0, 0.0, 0.0};
     if (q < (long long)n0 * n1 * n2) {
-        const int i = ab.lo[0] + (int)(q % n0), j = ab.lo[1] + (int)((q / n0) % n1);
-        const int k = ab.lo[2] + (int)(q / ((long long)n0 * n1));
+        int i, j, k;
+        pos3(q, n0, n1, i, j, k);
+        i += ab.lo[0];
+        j += ab.lo[1];
+        k += ab.lo[2];
         const long long c = ab.off + fab_index(ab, i, j, k);
         const double r = R[c], z = Z[c];
         w[0] = r * z;

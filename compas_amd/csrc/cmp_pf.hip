@@ -24,9 +24,10 @@ __device__ __forceinline__ bool cell_of(const BoxDev& bx, long long q, int& i, i
     const int n0 = bx.hi[0] - bx.lo[0] + 1, n1 = bx.hi[1] - bx.lo[1] + 1;
     const int n2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
     if (q >= (long long)n0 * n1 * n2) return false;
-    i = bx.lo[0] + (int)(q % n0);
-    j = bx.lo[1] + (int)((q / n0) % n1);
-    k = bx.lo[2] + (int)(q / ((long long)n0 * n1));
+    pos3(q, n0, n1, i, j, k);
+    i += bx.lo[0];
+    j += bx.lo[1];
+    k += bx.lo[2];
     return true;
 }
 
@@ -37,9 +38,10 @@ __device__ __forceinline__ bool face_of(const BoxDev& bx, long long q, int& i, i
     const int n0 = bx.hi[0] - bx.lo[0] + 1 + (D == 0), n1 = bx.hi[1] - bx.lo[1] + 1 + (D == 1);
     const int n2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 + (D == 2) : 1;
     if (q >= (long long)n0 * n1 * n2) return false;
-    i = bx.lo[0] + (int)(q % n0);
-    j = bx.lo[1] + (int)((q / n0) % n1);
-    k = bx.lo[2] + (int)(q / ((long long)n0 * n1));
+    pos3(q, n0, n1, i, j, k);
+    i += bx.lo[0];
+    j += bx.lo[1];
+    k += bx.lo[2];
     return true;
 }
 

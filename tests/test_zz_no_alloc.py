@@ -68,10 +68,15 @@ def test_noalloc_rhs_rejects_what_does_not_fit(gpu):
     rc = lib.cmp_level_rhs_noalloc(ctypes.byref(P), ctypes.byref(G), n, boxes, 3, t.data_ptr(), t.data_ptr(),
                                    None, None, None)
     assert rc == capi.CMP_ERR_SCRATCH
-    # more tiles than the work-list holds: a 4096^2 domain in 512^2 boxes (16x15-cell tiles)
+    # more tiles than the work-list holds: an 8192^2 domain in 512^2 boxes. A tile holds at most 256
+    # cells whatever its shape, so the level has at least 8192^2 / 256 tiles; the call below must come
+    # back before any GPU work (the state buffer is 8 doubles), so the count is checked first.
     from compas_amd import make_geometry, split_domain
-    G2 = make_geometry((4096, 4096), (0, 0), (1, 1), (1, 1))
-    lay_boxes = split_domain((4096, 4096), 512)
+    N = 8192
+    assert N * N // 256 > capi.CMP_NOALLOC_MAX_TILES
+    G2 = make_geometry((N, N), (0, 0), (1, 1), (1, 1))
+    lay_boxes = split_domain((N, N), 512)
+    assert len(lay_boxes) <= capi.CMP_NOALLOC_MAX_BOXES
     cb = (capi.cmp_box * len(lay_boxes))()
     off = 0
     for b, (lo, hi) in enumerate(lay_boxes):
