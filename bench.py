@@ -312,17 +312,20 @@ def _bf_shapes(bf=8, mgs=64):
 
 def tiles_record(levels, tune_s=None, amr=None):
     """The tile choice of every level for the bench line: work-list kernel on, its occupancy target,
-    and per box shape (the 8 holding the most cells) the chosen tile and the swept times (ns per
-    cell) of the fixed kernel and of that tile; the swept sums over the level's cells."""
+    how many boxes each kernel steps, and per box shape (the 8 holding the most cells) the chosen tile
+    ("fixed" for a box the fixed-shape kernel keeps) and the swept times (ns per cell) of the fixed
+    kernel and of that tile; the swept sums over the level's cells."""
     import numpy as np
     out = []
     for lv in levels:
         on, waves, ext = lv.tile_shapes()
         cells, chosen = {}, {}
+        nfixed = 0
         for (lo, hi), e in zip(lv.layout.boxes, ext):
             n = tuple(h - l + 1 for l, h in zip(lo, hi))
             cells[n] = cells.get(n, 0) + int(np.prod(n))
             chosen[n] = tuple(e[:len(n)])
+            nfixed += int(not on or e[0] == 0)
         t_fixed = t_chosen = 0.0
         top = {}
         swept = True
@@ -332,16 +335,18 @@ def tiles_record(levels, tune_s=None, amr=None):
                 swept = False
                 continue
             ns = t["fixed"]
-            if on:
+            if on and chosen[n][0] > 0:
                 w = 1 if waves == 3 else 0
                 ns = min((c[1 + w] for c in t["cand"] if c[0][:len(n)] == chosen[n]), default=t["fixed"])
             t_fixed += cells[n] * t["fixed"]
             t_chosen += cells[n] * ns
             if len(top) < 8:
-                top["x".join(map(str, n))] = {"tile": "x".join(map(str, chosen[n])),
+                tile = "x".join(map(str, chosen[n])) if on and chosen[n][0] > 0 else "fixed"
+                top["x".join(map(str, n))] = {"tile": tile,
                                               "ns_per_cell_fixed": round(t["fixed"], 4),
                                               "ns_per_cell_tile": round(ns, 4)}
         out.append({"work_list_kernel": on, "waves_per_simd": waves if on else None,
+                    "boxes_fixed_kernel": nfixed, "boxes_work_list": len(ext) - nfixed,
                     "box_shapes": len(cells), "swept": swept, "by_box_shape": top,
                     "swept_stage_ms_fixed": t_fixed * 1e-6 if swept else None,
                     "swept_stage_ms_chosen": t_chosen * 1e-6 if swept else None})

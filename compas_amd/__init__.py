@@ -370,7 +370,8 @@ class Level:
     # ---- the persistent work-list stage kernel and the tile autotuner ------------------------------
     def set_tiles(self, ext, waves=2):
         """The TVD-RK3 flow stage through the work-list kernel, box b in tiles of ext[b] (3 ints per
-        box; one triple for every box), occupancy target `waves`; ext None: the fixed-shape kernel."""
+        box; one triple for every box; 0 0 0: the box stays with the fixed-shape kernel), occupancy
+        target `waves`; ext None: the fixed-shape kernel."""
         if ext is None:
             check(self.lib.cmp_level_set_tiles(self.h, None, 0), "cmp_level_set_tiles")
             return
@@ -379,7 +380,8 @@ class Level:
         check(self.lib.cmp_level_set_tiles(self.h, arr, int(waves)), "cmp_level_set_tiles")
 
     def tile_shapes(self):
-        """(work-list kernel on, its occupancy target, [tile extents of each box])"""
+        """(work-list kernel on, its occupancy target, [tile extents of each box]; (0, 0, 0) for a
+        box the fixed-shape kernel keeps in a level that mixes the two)"""
         on, w = ctypes.c_int(), ctypes.c_int()
         arr = (ctypes.c_int * (3 * self.layout.nbox))()
         check(self.lib.cmp_level_tile_shapes(self.h, ctypes.byref(on), ctypes.byref(w), arr),

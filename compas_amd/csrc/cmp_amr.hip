@@ -27,6 +27,17 @@ hipError_t launch_cf_correct_dim(const LaunchCfg&, const KParams&, int, const Cf
                                  double*, const BoxDev*, const double*, const BoxDev*, const double*, int, double*,
                                  const double*, int, int*, hipStream_t);
 
+template <int DIM>
+hipError_t launch_cf_apply_all_dim(const LaunchCfg&, const CfPairDev*, int, int, const CfStatePtrs&, int, const double*,
+                                   int, hipStream_t);
+
+hipError_t launch_cf_apply_all(const LaunchCfg& lc, const CfPairDev* pairs, int npairs, int ntotal,
+                               const CfStatePtrs& st, int ratio, const double* dt, int stage, hipStream_t s)
+{
+    return lc.dim == 2 ? launch_cf_apply_all_dim<2>(lc, pairs, npairs, ntotal, st, ratio, dt, stage, s)
+                       : launch_cf_apply_all_dim<3>(lc, pairs, npairs, ntotal, st, ratio, dt, stage, s);
+}
+
 hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, int part, const CfCell* list, int n,
                              const CfFace* jobs, int njobs, double* faces, const BoxDev* cboxes, const double* Uc,
                              const BoxDev* fboxes, const double* Uf, int ratio, double* Uout, const double* dt,
@@ -84,6 +95,11 @@ struct cmp_amr {
     std::vector<CfFace*> d_jobs; // the face jobs of each coarse level's list
     std::vector<int> njobs;
     std::vector<double*> d_faces;   // their results, N + 1 doubles a job
+    // the flow corrections of every pair in one launch: the pairs with listed cells (device table),
+    // each pair's coarse level, and the cells in all
+    CfPairDev* d_cfpairs = nullptr;
+    std::vector<int> cfpair_level;
+    int ncftotal = 0;
     // each finer level's work of a stage runs on a stream of its own beside level 0's (fork / join):
     // q[l - 1] for level l; ev_fill[l] after level l's ghost fill, ev_done[l] after its last work
     std::vector<hipStream_t> q;
@@ -232,6 +248,7 @@ void amr_free(cmp_amr* A)
     for (double* p : A->d_faces)
         if (p) (void)hipFree(p);
     if (A->d_step) (void)hipFree(A->d_step);
+    if (A->d_cfpairs) (void)hipFree(A->d_cfpairs);
     void* lb[] = {A->d_ltiles, A->d_llev, A->d_lctr};
     for (void* p : lb)
         if (p) (void)hipFree(p);
@@ -365,7 +382,23 @@ int cmp_amr_create(cmp_level* const* levels, int nlev, int ratio, cmp_amr** out)
             if (r) { amr_free(A); return r; }
         }
     }
-    int r = check_hip(hipMalloc(&A->d_step, sizeof(int) * 2));
+    int r = CMP_OK;
+    {
+        std::vector<CfPairDev> pairs;
+        for (int l = 0; l + 1 < nlev; ++l) {
+            if (A->ncf[l] <= 0) continue;
+            const cmp_level* C = levels[l];
+            pairs.push_back(CfPairDev{C->kp, A->d_cf[l], A->ncf[l], A->ncftotal, C->d_boxes, A->d_faces[l], C->d_err});
+            A->cfpair_level.push_back(l);
+            A->ncftotal += A->ncf[l];
+        }
+        if (!pairs.empty() && pairs.size() <= (size_t)LIST_MAXLEV) {
+            r = check_hip(hipMalloc(&A->d_cfpairs, sizeof(CfPairDev) * pairs.size()));
+            if (!r) r = check_hip(hipMemcpy(A->d_cfpairs, pairs.data(), sizeof(CfPairDev) * pairs.size(),
+                                            hipMemcpyHostToDevice));
+        }
+    }
+    if (!r) r = check_hip(hipMalloc(&A->d_step, sizeof(int) * 2));
     if (!r) r = check_hip(hipMemset(A->d_step, 0, sizeof(int) * 2));
     if (!r && nlev > 1) {
         r = check_hip(hipEventCreateWithFlags(&A->ev_fork, hipEventDisableTiming));
@@ -471,6 +504,8 @@ static int amr_stages(cmp_amr* A, double* const* Un, double* const* U1, double* 
         if (pf)
             CMP_TRY(launch_stage(L->pf_lc, L->pf_kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0,
                                  src[l], Un[l], dst[l], dt_dev, nullptr, L->d_err, q, L->d_gq));
+        else if (L->use_list)   // the level's tuned tiles (work-list kernel, or both kernels)
+            return level_list_stage(L, MODE_RK, stage, src[l], Un[l], dst[l], dt_dev, cmax + l, q, false);
         else
             CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, stage, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, src[l],
                                  Un[l], dst[l], dt_dev, (unsigned long long*)(cmax + l), L->d_err, q));
@@ -487,6 +522,24 @@ static int amr_stages(cmp_amr* A, double* const* Un, double* const* U1, double* 
             CMP_TRY(launch_cf_correct(C->lc, C->kp, part, A->d_cf[l], A->ncf[l], A->d_jobs[l], A->njobs[l],
                                       A->d_faces[l], C->d_boxes, src[l], F->d_boxes, src[l + 1], A->ratio, dst[l],
                                       dt_dev, stage, C->d_err, q));
+        return CMP_OK;
+    };
+    // every pair's cell corrections on the caller's stream: the flow's in one launch
+    auto cf_cells = [&](int stage, double* const* src, double* const* dst) -> int {
+        if (pf || !A->d_cfpairs) {
+            for (int l = 0; l + 1 < nlev; ++l) {
+                const int r = cf(l, 2, stage, src, dst, s);
+                if (r) return r;
+            }
+            return CMP_OK;
+        }
+        CfStatePtrs st{};
+        for (size_t k = 0; k < A->cfpair_level.size(); ++k) {
+            st.Uc[k] = src[A->cfpair_level[k]];
+            st.Uout[k] = dst[A->cfpair_level[k]];
+        }
+        CMP_TRY(launch_cf_apply_all(A->L[0]->lc, A->d_cfpairs, (int)A->cfpair_level.size(), A->ncftotal, st, A->ratio,
+                                    dt_dev, stage, s));
         return CMP_OK;
     };
     for (int stage = 0; stage < 3; ++stage) {
@@ -527,8 +580,7 @@ static int amr_stages(cmp_amr* A, double* const* Un, double* const* U1, double* 
             CMP_TRY(launch_stage_list(A->L[0]->lc, A->list_waves, A->d_llev, A->d_ltiles, A->nltiles, A->d_lctr, c,
                                       MODE_RK, stage, dt_dev, A->list_grid, s));
             for (int l = 1; l < nlev; ++l) CMP_TRY(hipStreamWaitEvent(s, A->ev_done[l], 0));
-            for (int l = 0; l + 1 < nlev; ++l)
-                if ((r = cf(l, 2, stage, src, dst, s))) return r;
+            if ((r = cf_cells(stage, src, dst))) return r;
             for (int l = nlev - 2; l >= 0; --l)
                 if ((r = cmp_level_average_down(A->L[l + 1], dst[l + 1], A->L[l], dst[l], A->ratio, s))) return r;
             continue;
@@ -551,8 +603,7 @@ static int amr_stages(cmp_amr* A, double* const* Un, double* const* U1, double* 
             CMP_TRY(hipEventRecord(A->ev_done[l], q));
             CMP_TRY(hipStreamWaitEvent(s, A->ev_done[l], 0));
         }
-        for (int l = 0; l + 1 < nlev; ++l)
-            if ((r = cf(l, 2, stage, src, dst, s))) return r;
+        if ((r = cf_cells(stage, src, dst))) return r;
         for (int l = nlev - 2; l >= 0; --l)
             if ((r = cmp_level_average_down(A->L[l + 1], dst[l + 1], A->L[l], dst[l], A->ratio, s))) return r;
     }
@@ -598,6 +649,9 @@ int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* co
         CMP_TRY(hipStreamSynchronize(nullptr));
     }
     const int nlev = (int)A->L.size();
+    // the finer levels' two-level fill plans (they allocate, so before the capture)
+    for (int l = 1; l < nlev; ++l)
+        if ((r = level_fill2_plan(A->L[l], A->L[l - 1], A->ratio, cs))) return r;
     std::vector<const void*> key;
     for (int l = 0; l < nlev; ++l) {
         key.push_back(Un[l]);
@@ -607,6 +661,13 @@ int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* co
     key.push_back(dt_dev);
     key.push_back(cmax);
     key.push_back(dt_hist);
+    // the levels' tile choices (cmp_level_set_tiles, cmp_level_tune_tiles) and fill plans are baked
+    // into the graph
+    for (int l = 0; l < nlev; ++l) {
+        key.push_back((const void*)(uintptr_t)A->L[l]->tiles_gen);
+        key.push_back(A->L[l]->d_f2plan);
+        key.push_back((const void*)(uintptr_t)A->L[l]->f2_uid);
+    }
     if (!(A->gexec && A->gkey == key && A->gcfl == cfl && A->glimiter == limiter)) {
         if (A->gexec) { CMP_TRY(hipGraphExecDestroy(A->gexec)); A->gexec = nullptr; }
         if (A->graph) { CMP_TRY(hipGraphDestroy(A->graph)); A->graph = nullptr; }

@@ -5,6 +5,7 @@
 // block->box owner map the ghost fill uses. State buffers are the caller's, as in the reference,
 // where every kernel receives non-owning Array4 views of caller-owned MultiFabs.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -64,6 +65,11 @@ hipError_t launch_stage(const LaunchCfg& lc, const KParams& kp, int mode, int st
 template <int DIM>
 hipError_t launch_fill2_dim(int, const FillDev&, const FillDev*, const FillDev&, int, double*, const double*,
                             const double*, long long, int, int*, hipStream_t);
+template <int DIM>
+hipError_t launch_fill2_plan_dim(const FillDev&, const FillDev&, int, void*, hipStream_t);
+template <int DIM>
+hipError_t launch_fill2_apply_dim(int, const FillDev&, const FillDev&, int, const void*, double*, const double*, int*,
+                                  hipStream_t);
 template <int DIM>
 hipError_t launch_avgdown_dim(const BoxDev*, int, long long, const FillDev&, const double*, double*, int, int, double,
                               int*, hipStream_t);
@@ -652,6 +658,8 @@ int cmp_level_create(const cmp_params* params, const cmp_geometry* geom, int nbo
         if (geom->domain_hi[d] - geom->domain_lo[d] + 1 < ng && geom->is_periodic[d]) return CMP_ERR_LAYOUT;
 
     cmp_level* L = new cmp_level();
+    static std::atomic<unsigned long long> next_uid{1};
+    L->uid = next_uid++;
     L->P = *params;
     L->G = *geom;
     L->ng = ng;
@@ -884,7 +892,7 @@ int cmp_level_destroy(cmp_level* L)
     if (L->graph) (void)hipGraphDestroy(L->graph);
     if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
     void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff, L->d_U3,
-                    L->d_clock, L->d_ltiles, L->d_llev, L->d_lctr};
+                    L->d_clock, L->d_ltiles, L->d_llev, L->d_lctr, L->d_ftiles, L->d_f2plan};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete L;
@@ -1915,15 +1923,52 @@ static int two_level_args(const cmp_level* F, const cmp_level* C, int ratio)
     return CMP_OK;
 }
 
+}  // extern "C"
+
+// The gather plan of F's two-level fill from C at ratio (k_fill2_plan), built when F has none for that
+// pair; allocates device memory (not while the stream is captured).
+int cmp::level_fill2_plan(cmp_level* F, const cmp_level* C, int ratio, hipStream_t s)
+{
+    if (F->d_f2plan && F->f2_uid == C->uid && F->f2_ratio == ratio) return CMP_OK;
+    if (ratio > 255) return CMP_ERR_UNSUPPORTED;
+    if (F->d_f2plan) {
+        CMP_TRY(hipFree(F->d_f2plan));
+        F->d_f2plan = nullptr;
+    }
+    F->f2_uid = 0;
+    F->f2_ratio = 0;
+    const int dim = F->P.dim;
+    const size_t bytes = (dim == 2 ? sizeof(F2Cell<2>) : sizeof(F2Cell<3>)) * (size_t)std::max(1LL, F->nghost);
+    CMP_TRY(hipMalloc(&F->d_f2plan, bytes));
+    CMP_TRY((dim == 2 ? launch_fill2_plan_dim<2> : launch_fill2_plan_dim<3>)(F->fd, C->fd, ratio, F->d_f2plan, s));
+    F->f2_uid = C->uid;
+    F->f2_ratio = ratio;
+    return CMP_OK;
+}
+
+extern "C" {
+
 int cmp_level_fill_two_levels(cmp_level* F, double* Uf, const cmp_level* C, const double* Uc, int ratio,
                               void* stream)
 {
     if (!F || !C || !Uf || !Uc || ratio < 2) return CMP_ERR_ARG;
     int r = two_level_args(F, C, ratio);
     if (r) return r;
-    // the same-level sources are the level's own boxes
-    CMP_TRY((F->P.dim == 2 ? launch_fill2_dim<2> : launch_fill2_dim<3>)(F->P.nstate, F->fd, &F->fd, C->fd, ratio, Uf,
-                                                                        Uf, Uc, 0, 0, F->d_err, (hipStream_t)stream));
+    hipStream_t s = (hipStream_t)stream;
+    const int dim = F->P.dim;
+    if (!(F->d_f2plan && F->f2_uid == C->uid && F->f2_ratio == ratio)) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        CMP_TRY(hipStreamIsCapturing(s, &cs));
+        if (cs == hipStreamCaptureStatusNone && (r = level_fill2_plan(F, C, ratio, s))) return r;
+    }
+    if (F->d_f2plan && F->f2_uid == C->uid && F->f2_ratio == ratio) {
+        CMP_TRY((dim == 2 ? launch_fill2_apply_dim<2> : launch_fill2_apply_dim<3>)(F->P.nstate, F->fd, C->fd, ratio,
+                                                                                 F->d_f2plan, Uf, Uc, F->d_err, s));
+        return CMP_OK;
+    }
+    // under capture without a plan: the search form (the same-level sources are the level's own boxes)
+    CMP_TRY((dim == 2 ? launch_fill2_dim<2> : launch_fill2_dim<3>)(F->P.nstate, F->fd, &F->fd, C->fd, ratio, Uf, Uf, Uc,
+                                                                   0, 0, F->d_err, s));
     return CMP_OK;
 }
 

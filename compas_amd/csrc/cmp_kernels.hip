@@ -1392,6 +1392,31 @@ __device__ __forceinline__ int coarse_value(const FillDev& cd, const double* __r
 // ALL: every cell of the boxes of fd (valid and ghost), the same-level sources taken from the boxes
 // of sfd (has_src 0: none) in Us - the data of a level's new boxes after a regrid; otherwise the
 // ghost cells only, sfd = fd and Us = Uf.
+// One direction of that interpolation: the central differences dc of every component (returned in
+// cm, which holds the minus neighbour on entry) and the factor sf that scales them all.
+template <int N>
+__device__ __forceinline__ double lin_slopes(const double* c0, double* cm, const double* cp)
+{
+#pragma clang fp contract(off)
+    double sf = 1.0;
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+        const double dc = 0.5 * (cp[n] - cm[n]);
+        const double df = 2.0 * (cp[n] - c0[n]);
+        const double db = 2.0 * (c0[n] - cm[n]);
+        const double af = fabs(df), ab = fabs(db);
+        double sx = (df * db >= 0.0) ? ((ab < af) ? ab : af) : 0.0;
+        const double adc = fabs(dc);
+        sx = copysign(1.0, dc) * ((adc < sx) ? adc : sx);
+        if (dc != 0.0) {
+            const double r = sx / dc;
+            sf = (r < sf) ? r : sf;
+        }
+        cm[n] = dc;
+    }
+    return sf;
+}
+
 template <int DIM, int N, bool ALL>
 __global__ __launch_bounds__(NT)
 void k_fill2(FillDev fd, FillDev sfd, int has_src, FillDev cd, int box0, int ratio, double* Uf,
@@ -1449,22 +1474,7 @@ void k_fill2(FillDev fd, FillDev sfd, int has_src, FillDev cd, int box0, int rat
                 e = coarse_value<DIM, N, MOM>(cd, Uc, a[0], a[1], a[2], cm);
                 if (!e) e = coarse_value<DIM, N, MOM>(cd, Uc, z3[0], z3[1], z3[2], cp);
                 if (e) break;
-                double sf = 1.0;
-#pragma unroll
-                for (int n = 0; n < N; ++n) {
-                    const double dc = 0.5 * (cp[n] - cm[n]);
-                    const double df = 2.0 * (cp[n] - c0[n]);
-                    const double db = 2.0 * (c0[n] - cm[n]);
-                    const double af = fabs(df), ab = fabs(db);
-                    double sx = (df * db >= 0.0) ? ((ab < af) ? ab : af) : 0.0;
-                    const double adc = fabs(dc);
-                    sx = copysign(1.0, dc) * ((adc < sx) ? adc : sx);
-                    if (dc != 0.0) {
-                        const double r = sx / dc;
-                        sf = (r < sf) ? r : sf;
-                    }
-                    cm[n] = dc;
-                }
+                const double sf = lin_slopes<N>(c0, cm, cp);
                 const double off = ((double)(src[d] - ic[d] * ratio) + 0.5) / (double)ratio - 0.5;
 #pragma unroll
                 for (int n = 0; n < N; ++n) v[n] = v[n] + off * (cm[n] * sf);
@@ -1474,6 +1484,138 @@ void k_fill2(FillDev fd, FillDev sfd, int has_src, FillDev cd, int box0, int rat
 #pragma unroll
         for (int n = 0; n < N; ++n) {
             const bool neg = n >= MOM && n < MOM + DIM && ((flip >> (n - MOM)) & 1u);
+            Uf[di + n * bx.sn] = neg ? -v[n] : v[n];
+        }
+    }
+}
+
+// The two-level ghost fill of a fine level as a gather: k_fill2_plan finds, once for a pair of
+// levels, what k_fill2 (ghost cells only, the same-level sources the level's own boxes) finds per
+// call for each ghost cell - the BC map and momentum flips of the cell, its same-level source, or
+// the coarse cell under it and that cell's neighbours along each direction (each mapped through the
+// coarse level's BCs, with its flips and owning box) - and k_fill2_apply reads those sources and
+// does k_fill2's arithmetic, so its values are k_fill2's bit for bit. Plan entry g of box b at
+// fd.gprefix[b] + g. Slot 0: the same-level source or the coarse centre; slots 1 + 2d and 2 + 2d the
+// coarse neighbours below and above along d.
+template <int DIM>
+__device__ __forceinline__ int f2_source(const FillDev& fd, int* c, int& box, int& cell, unsigned& flip)
+{
+    const int e = map_cell<DIM>(fd, c, flip);
+    if (e) return e;
+    box = cell_owner<DIM>(fd, c);
+    if (box < 0) return CMP_ERR_LAYOUT;
+    cell = (int)fab_index(fd.boxes[box], c[0], c[1], c[2]);
+    return 0;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT)
+void k_fill2_plan(FillDev fd, FillDev cd, int box0, int ratio, F2Cell<DIM>* __restrict__ plan)
+{
+    const int b = box0 + (int)blockIdx.y;
+    const BoxDev bx = fd.boxes[b];
+    const int ng = fd.ng;
+    const int nv0 = bx.hi[0] - bx.lo[0] + 1, nv1 = bx.hi[1] - bx.lo[1] + 1;
+    const int nv2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
+    const int ex = nv0 + 2 * ng, ey = nv1 + 2 * ng;
+    const int ez = (DIM == 3) ? nv2 + 2 * ng : 1;
+    const int ncell = ex * ey * ez - nv0 * nv1 * nv2;
+    for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < ncell; g += (int)(gridDim.x * NT)) {
+        int x, y, z;
+        ghost_xyz<DIM>(g, ng, ex, ey, ez, nv1, x, y, z);
+        F2Cell<DIM> p;
+        for (int k = 0; k < F2Cell<DIM>::NS; ++k) p.box[k] = p.cell[k] = 0;
+        p.kind = 0;
+        p.err = 0;
+        int src[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
+        unsigned flip;
+        int e = map_cell<DIM>(fd, src, flip);
+        p.flip = flip & 7u;
+        if (!e) {
+            const int own = cell_owner<DIM>(fd, src);
+            if (own >= 0) {
+                p.box[0] = own;
+                p.cell[0] = (int)fab_index(fd.boxes[own], src[0], src[1], src[2]);
+            } else {
+                int ic[3] = {src[0], src[1], src[2]};
+                p.kind = 1;
+                for (int d = 0; d < DIM; ++d) {
+                    ic[d] = (src[d] >= 0) ? src[d] / ratio : -((-src[d] + ratio - 1) / ratio);
+                    p.kind |= (src[d] - ic[d] * ratio) << (8 + 8 * d);
+                }
+                for (int k = 0; k < 1 + 2 * DIM && !e; ++k) {
+                    int c[3] = {ic[0], ic[1], ic[2]};
+                    if (k > 0) c[(k - 1) / 2] += ((k - 1) % 2) ? 1 : -1;
+                    unsigned fl;
+                    e = f2_source<DIM>(cd, c, p.box[k], p.cell[k], fl);
+                    p.flip |= (fl & 7u) << (3 * (k + 1));
+                }
+            }
+        }
+        p.err = e;
+        plan[fd.gprefix[b] + g] = p;
+    }
+}
+
+template <int DIM, int N>
+__global__ __launch_bounds__(NT)
+void k_fill2_apply(FillDev fd, FillDev cd, int box0, int ratio, const F2Cell<DIM>* __restrict__ plan, double* Uf,
+                   const double* __restrict__ Uc, int* err)
+{
+#pragma clang fp contract(off)
+    constexpr int MOM = (N == 4 + DIM) ? 2 : ((N > 3) ? (N - DIM - 1) / 2 : -100);
+    const int b = box0 + (int)blockIdx.y;
+    const BoxDev bx = fd.boxes[b];
+    const int ng = fd.ng;
+    const int nv0 = bx.hi[0] - bx.lo[0] + 1, nv1 = bx.hi[1] - bx.lo[1] + 1;
+    const int nv2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
+    const int ex = nv0 + 2 * ng, ey = nv1 + 2 * ng;
+    const int ez = (DIM == 3) ? nv2 + 2 * ng : 1;
+    const int ncell = ex * ey * ez - nv0 * nv1 * nv2;
+    const F2Cell<DIM>* __restrict__ pb = plan + fd.gprefix[b];
+    for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < ncell; g += (int)(gridDim.x * NT)) {
+        const F2Cell<DIM> p = pb[g];
+        if (p.err) { atomicMax(err, p.err); continue; }
+        int x, y, z;
+        ghost_xyz<DIM>(g, ng, ex, ey, ez, nv1, x, y, z);
+        const long long di = bx.off + fab_index(bx, bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z);
+        // the N components of source k of the entry (momenta flipped by its bits)
+        auto load = [&](const FillDev& f, const double* __restrict__ U, int k, double* v) {
+            const BoxDev so = f.boxes[p.box[k]];
+            const long long si = so.off + p.cell[k];
+            const unsigned fl = (p.flip >> (3 * (k + 1))) & 7u;
+#pragma unroll
+            for (int n = 0; n < N; ++n) {
+                const double w = U[si + n * so.sn];
+                v[n] = (n >= MOM && n < MOM + DIM && ((fl >> (n - MOM)) & 1u)) ? -w : w;
+            }
+        };
+        double v[N];
+        if (!(p.kind & 1)) {
+            const BoxDev so = fd.boxes[p.box[0]];
+            const long long si = so.off + p.cell[0];
+#pragma unroll
+            for (int n = 0; n < N; ++n) v[n] = Uf[si + n * so.sn];
+        } else {
+            double c0[N];
+            load(cd, Uc, 0, c0);
+#pragma unroll
+            for (int n = 0; n < N; ++n) v[n] = c0[n];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                double cm[N], cp[N];
+                load(cd, Uc, 1 + 2 * d, cm);
+                load(cd, Uc, 2 + 2 * d, cp);
+                const double sf = lin_slopes<N>(c0, cm, cp);
+                const int rem = (p.kind >> (8 + 8 * d)) & 255;
+                const double off = ((double)rem + 0.5) / (double)ratio - 0.5;
+#pragma unroll
+                for (int n = 0; n < N; ++n) v[n] = v[n] + off * (cm[n] * sf);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < N; ++n) {
+            const bool neg = n >= MOM && n < MOM + DIM && ((p.flip >> (n - MOM)) & 1u);
             Uf[di + n * bx.sn] = neg ? -v[n] : v[n];
         }
     }
@@ -1517,19 +1659,16 @@ void k_cf_faces(KParams kp, int limiter, const CfFace* __restrict__ jobs, int nj
     if (bad) atomicMax(err, bad);
 }
 
-// k_cf_apply: one thread per listed coarse cell, its face jobs combined in their fixed order. PF: a
+// The correction of one listed coarse cell, its face jobs combined in their fixed order. PF: a
 // Phase-Field stage (faces hold -fhat; no non-conservative term).
-template <int DIM, int NP, bool PF = false>
-__global__ __launch_bounds__(64)
-void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev* __restrict__ cboxes,
-                const double* __restrict__ Uc, const double* __restrict__ faces, int ratio, double* Uout,
-                const double* __restrict__ dtp, int stage, int* err)
+template <int DIM, int NP, bool PF>
+__device__ __forceinline__ void cf_apply_cell(const KParams& kp, const CfCell& cc, const BoxDev* __restrict__ cboxes,
+                                              const double* __restrict__ Uc, const double* __restrict__ faces,
+                                              int ratio, double* Uout, const double* __restrict__ dtp, int stage,
+                                              int* err)
 {
     constexpr int N = (NP > 0) ? 2 * NP + DIM + 1 : 4 + DIM;
     constexpr int IA = 3 + DIM;
-    const int e = (int)(blockIdx.x * 64 + threadIdx.x);
-    if (e >= n) return;
-    const CfCell cc = list[e];
     const BoxDev cb = cboxes[cc.box];
     int bad = 0;
     double dL[N];
@@ -1584,6 +1723,50 @@ void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev
         Uout[c + q * cb.sn] = v;
     }
     if (bad) atomicMax(err, bad);
+}
+
+// k_cf_apply: one thread per listed coarse cell of one pair of levels.
+template <int DIM, int NP, bool PF = false>
+__global__ __launch_bounds__(64)
+void k_cf_apply(KParams kp, const CfCell* __restrict__ list, int n, const BoxDev* __restrict__ cboxes,
+                const double* __restrict__ Uc, const double* __restrict__ faces, int ratio, double* Uout,
+                const double* __restrict__ dtp, int stage, int* err)
+{
+    const int e = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (e >= n) return;
+    cf_apply_cell<DIM, NP, PF>(kp, list[e], cboxes, Uc, faces, ratio, Uout, dtp, stage, err);
+}
+
+// k_cf_apply_all: the flow corrections of every pair of levels of a hierarchy in one launch (they
+// write different levels), thread e taking cell e - start of the last pair that starts at or before e.
+template <int DIM, int NP>
+__global__ __launch_bounds__(64)
+void k_cf_apply_all(const CfPairDev* __restrict__ pairs, int npairs, int ntotal, CfStatePtrs st, int ratio,
+                    const double* __restrict__ dtp, int stage)
+{
+    const int e = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (e >= ntotal) return;
+    int k = npairs - 1;
+    while (k > 0 && pairs[k].start > e) --k;
+    const CfPairDev& P = pairs[k];
+    cf_apply_cell<DIM, NP, false>(P.kp, P.list[e - P.start], P.cboxes, st.Uc[k], P.faces, ratio, st.Uout[k], dtp,
+                                  stage, P.err);
+}
+
+template <int DIM>
+hipError_t launch_cf_apply_all_dim(const LaunchCfg& lc, const CfPairDev* pairs, int npairs, int ntotal,
+                                   const CfStatePtrs& st, int ratio, const double* dt, int stage, hipStream_t s)
+{
+    if (ntotal <= 0 || npairs <= 0) return hipSuccess;
+    if (lc.riemann == RIEMANN_PF_MIX || lc.riemann == RIEMANN_PF_PHASIC) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((ntotal + 63) / 64));
+    if (lc.nphase == 3)
+        hipLaunchKernelGGL((k_cf_apply_all<DIM, 3>), grid, dim3(64), 0, s, pairs, npairs, ntotal, st, ratio, dt, stage);
+    else if (lc.nphase == 2)
+        hipLaunchKernelGGL((k_cf_apply_all<DIM, 2>), grid, dim3(64), 0, s, pairs, npairs, ntotal, st, ratio, dt, stage);
+    else
+        hipLaunchKernelGGL((k_cf_apply_all<DIM, 0>), grid, dim3(64), 0, s, pairs, npairs, ntotal, st, ratio, dt, stage);
+    return hipGetLastError();
 }
 
 // The Phase-Field flux of the face between cells c - e_DIR and c of box bx (Ub its fab, ghosts
@@ -1746,6 +1929,42 @@ hipError_t launch_cf_correct_dim(const LaunchCfg& lc, const KParams& kp, int par
 }
 
 template <int DIM>
+hipError_t launch_fill2_plan_dim(const FillDev& fd, const FillDev& cd, int ratio, void* plan, hipStream_t s)
+{
+    long long cols = (fd.maxghost + NT - 1) / NT;
+    if (cols > 65536) cols = 65536;
+    if (cols <= 0) return hipSuccess;
+    for (int box0 = 0; box0 < fd.nbox; box0 += 65535) {
+        const int rows = (fd.nbox - box0 < 65535) ? fd.nbox - box0 : 65535;
+        hipLaunchKernelGGL((k_fill2_plan<DIM>), dim3((unsigned)cols, (unsigned)rows), dim3(NT), 0, s, fd, cd, box0, ratio,
+                           (F2Cell<DIM>*)plan);
+    }
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_fill2_apply_dim(int N, const FillDev& fd, const FillDev& cd, int ratio, const void* plan, double* Uf,
+                                  const double* Uc, int* err, hipStream_t s)
+{
+    if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM) return hipErrorInvalidValue;
+    long long cols = (fd.maxghost + NT - 1) / NT;
+    if (cols > 65536) cols = 65536;
+    if (cols <= 0) return hipSuccess;
+    const F2Cell<DIM>* pl = (const F2Cell<DIM>*)plan;
+    for (int box0 = 0; box0 < fd.nbox; box0 += 65535) {
+        const int rows = (fd.nbox - box0 < 65535) ? fd.nbox - box0 : 65535;
+        const dim3 grid((unsigned)cols, (unsigned)rows);
+        if (N == 4 + DIM)
+            hipLaunchKernelGGL((k_fill2_apply<DIM, 4 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, pl, Uf, Uc, err);
+        else if (N == 5 + DIM)
+            hipLaunchKernelGGL((k_fill2_apply<DIM, 5 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, pl, Uf, Uc, err);
+        else
+            hipLaunchKernelGGL((k_fill2_apply<DIM, 7 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, pl, Uf, Uc, err);
+    }
+    return hipGetLastError();
+}
+
+template <int DIM>
 hipError_t launch_fill2_dim(int N, const FillDev& fd, const FillDev* sfd, const FillDev& cd, int ratio, double* Uf,
                             const double* Us, const double* Uc, long long maxcells, int all, int* err, hipStream_t s)
 {
@@ -1842,6 +2061,8 @@ void tile_dims_dim(int* tx, int* ty, int* tz)
 }
 
 template void tile_dims_dim<CMP_KERNEL_DIM>(int*, int*, int*);
+template hipError_t launch_cf_apply_all_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const CfPairDev*, int, int,
+                                                           const CfStatePtrs&, int, const double*, int, hipStream_t);
 template hipError_t launch_cf_correct_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, int, const CfCell*, int,
                                                          const CfFace*, int, double*, const BoxDev*,
                                                          const double*, const BoxDev*, const double*, int,
@@ -1853,6 +2074,9 @@ template hipError_t launch_cf_correct_pf_dim<CMP_KERNEL_DIM>(const LaunchCfg&, c
 template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev*, const FillDev&, int,
                                                      double*, const double*, const double*, long long, int, int*,
                                                      hipStream_t);
+template hipError_t launch_fill2_plan_dim<CMP_KERNEL_DIM>(const FillDev&, const FillDev&, int, void*, hipStream_t);
+template hipError_t launch_fill2_apply_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev&, int, const void*, double*,
+                                                           const double*, int*, hipStream_t);
 template hipError_t launch_avgdown_dim<CMP_KERNEL_DIM>(const BoxDev*, int, long long, const FillDev&, const double*,
                                                        double*, int, int, double, int*, hipStream_t);
 template hipError_t launch_tag_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev*, int, long long,

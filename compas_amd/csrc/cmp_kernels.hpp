@@ -76,6 +76,20 @@ struct FillDev {
     int lo_bc[3], hi_bc[3];
 };
 
+// The sources of one ghost cell of a fine level's two-level fill, found once for a pair of levels
+// (k_fill2_plan): slot 0 the same-level source or the coarse cell under it, slots 1 + 2d / 2 + 2d the
+// coarse neighbours below / above along d, each as its level's box index and the cell's offset in
+// that box's fab; flip bits 0-2 the ghost cell's momentum flips, bits 3(k + 1).. those of slot k.
+template <int DIM>
+struct F2Cell {
+    static constexpr int NS = 1 + 2 * DIM;
+    int box[NS];
+    int cell[NS];
+    unsigned flip;
+    int kind;   // bit 0: interpolated from the coarse level; bits 8 + 8d: offset in the coarse cell along d
+    int err;    // nonzero: the fill's error code for this cell (nothing written)
+};
+
 enum StageMode { MODE_RK = 0, MODE_RHS = 1, MODE_CMAX = 2 };
 
 // One case of a batch of levels as the per-case ComputeDt sees it.
@@ -210,6 +224,26 @@ hipError_t launch_cf_correct(const LaunchCfg& lc, const KParams& kp, int part, c
                              int njobs, double* faces, const BoxDev* cboxes, const double* Uc, const BoxDev* fboxes,
                              const double* Uf, int ratio, double* Uout, const double* dt, int stage, int* err,
                              hipStream_t s);
+// One pair of levels of a hierarchy in the fused flow correction (launch_cf_apply_all): the coarse
+// level's parameters, its listed cells (start: the pair's first thread of the launch), box table and
+// error word, and the pair's face values.
+struct CfPairDev {
+    KParams kp;
+    const CfCell* list;
+    int n, start;
+    const BoxDev* cboxes;
+    const double* faces;
+    int* err;
+};
+// Per pair of the launch, the coarse level's stage state and stage output.
+struct CfStatePtrs {
+    const double* Uc[LIST_MAXLEV];
+    double* Uout[LIST_MAXLEV];
+};
+// The cell part (part 2) of launch_cf_correct for every pair of levels in one launch: ntotal threads,
+// pairs[] in device memory in order of start. Flow stages only (not the Phase-Field flux).
+hipError_t launch_cf_apply_all(const LaunchCfg& lc, const CfPairDev* pairs, int npairs, int ntotal,
+                               const CfStatePtrs& st, int ratio, const double* dt, int stage, hipStream_t s);
 // out = (p X + q Y) + (r dt) L on the valid cells of a level (Y, L optional), without contraction.
 hipError_t launch_lincomb(const BoxDev* boxes, int nbox, long long maxcells, int N, double* out, double p,
                           const double* X, double q, const double* Y, double r, const double* L, const double* dt,

@@ -21,14 +21,21 @@ void tile_halo(int dim, const cmp_geometry* geom, const cmp_box* boxes, int b, c
 struct cmp_level;
 namespace cmp {
 // The work-list form of a level's flow stage (cmp_tune.hip): one stage through the level's list
-// (fused ghost reads), and the tiles of every box of a level by its box_ext with level index lev.
+// (fused ghost reads, or the filled ghost cells when fused is false), and the tiles of every box of a
+// level by its box_ext with level index lev.
 int level_list_stage(cmp_level* L, int mode, int stage, const double* Us, const double* Un, double* Uout,
-                     const double* dt, double* cmax, hipStream_t s);
+                     const double* dt, double* cmax, hipStream_t s, bool fused = true);
 void level_list_tiles(const cmp_level* L, int lev, std::vector<ListTile>& out);
 // The tuned choices of L from the autotuner's cache: per occupancy target (2, 3 waves per SIMD) each
-// box's tile extents and the level's summed time, and the fixed kernel's summed time;
-// CMP_ERR_UNSUPPORTED when a box shape of L has no cached sweep.
-int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, double* t_fixed);
+// box's tile extents and the level's summed time, and the fixed kernel's summed time; with ext_mixed
+// and t_mixed, per occupancy target the per-box choice of the faster kernel (0 0 0 for a box the
+// fixed kernel keeps) and its summed time. CMP_ERR_UNSUPPORTED when a box shape of L has no cached
+// sweep.
+int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, double* t_fixed,
+                std::vector<int>* ext_mixed = nullptr, double* t_mixed = nullptr);
+// The gather plan of fine level F's two-level fill from coarse level C at ratio, built when F holds
+// none for that pair (cmp_capi.hip); allocates device memory, so not under stream capture.
+int level_fill2_plan(cmp_level* F, const cmp_level* C, int ratio, hipStream_t s);
 
 }  // namespace cmp
 
@@ -167,4 +174,16 @@ struct cmp_level {
     int* d_lctr = nullptr;
     int list_grid = 0;
     int list_waves = 2;             // the work-list kernel's occupancy target (waves per SIMD)
+    // a level that mixes the two kernels: the tiles of the boxes the fixed-shape kernel keeps (box_ext
+    // 0 0 0), stepped by a k_stage launch beside the work-list launch over the other boxes
+    TileDev* d_ftiles = nullptr;
+    int nftiles = 0;
+    unsigned tiles_gen = 0;         // raised by every change of the above (a hierarchy's graph key)
+
+    unsigned long long uid = 0;     // unique per level created in this process
+    // the two-level fill's sources of every ghost cell (F2Cell, one per ghost cell in fd.gprefix
+    // order), for the coarse level f2_uid at ratio f2_ratio (0: none built)
+    void* d_f2plan = nullptr;
+    unsigned long long f2_uid = 0;
+    int f2_ratio = 0;
 };

@@ -78,24 +78,42 @@ static void box_tiles(const cmp_level* L, int lev, int b, const int* ext, std::v
             }
 }
 
-// The tiles of every box of L with the extents of box_ext (3 ints per box), level index lev.
+// The tiles of every box of L with the extents of box_ext (3 ints per box), level index lev; a box
+// whose extents are 0 0 0 is left to the fixed-shape kernel and has no tiles here.
 void level_list_tiles(const cmp_level* L, int lev, std::vector<ListTile>& out)
 {
-    for (int b = 0; b < L->nbox; ++b) box_tiles(L, lev, b, &L->box_ext[3 * (size_t)b], out);
+    for (int b = 0; b < L->nbox; ++b)
+        if (L->box_ext[3 * (size_t)b] > 0) box_tiles(L, lev, b, &L->box_ext[3 * (size_t)b], out);
 }
 
-// Builds L's work-list from L->box_ext: tiles, one-level table (fused ghost reads), counters.
+// Builds L's work-list from L->box_ext: tiles, one-level table (fused ghost reads), counters, and the
+// fixed-shape kernel's tiles of the boxes it keeps.
 static int level_build_list(cmp_level* L)
 {
+    ++L->tiles_gen;
     std::vector<ListTile> ht;
     level_list_tiles(L, 0, ht);
+    std::vector<TileDev> ft;
+    for (int b = 0; b < L->nbox; ++b)
+        if (L->box_ext[3 * (size_t)b] == 0)
+            for (int t = L->tile_start[b]; t < L->tile_start[b + 1]; ++t) ft.push_back(L->h_tiles[t]);
+    if (ht.empty()) return CMP_ERR_ARG;
     if (L->d_ltiles) { CMP_TRY(hipFree(L->d_ltiles)); L->d_ltiles = nullptr; }
-    CMP_TRY(hipMalloc(&L->d_ltiles, sizeof(ListTile) * std::max<size_t>(1, ht.size())));
+    CMP_TRY(hipMalloc(&L->d_ltiles, sizeof(ListTile) * ht.size()));
     CMP_TRY(hipMemcpy(L->d_ltiles, ht.data(), sizeof(ListTile) * ht.size(), hipMemcpyHostToDevice));
+    if (L->d_ftiles) { CMP_TRY(hipFree(L->d_ftiles)); L->d_ftiles = nullptr; }
+    L->nftiles = 0;
+    if (!ft.empty()) {
+        CMP_TRY(hipMalloc(&L->d_ftiles, sizeof(TileDev) * ft.size()));
+        CMP_TRY(hipMemcpy(L->d_ftiles, ft.data(), sizeof(TileDev) * ft.size(), hipMemcpyHostToDevice));
+        L->nftiles = (int)ft.size();
+    }
     if (!L->d_llev) {
-        CMP_TRY(hipMalloc(&L->d_llev, sizeof(ListLevel)));
-        const ListLevel lv{L->kp, L->d_boxes, L->d_fd, L->d_err};
-        CMP_TRY(hipMemcpy(L->d_llev, &lv, sizeof(ListLevel), hipMemcpyHostToDevice));
+        // [0] the stage reads the cells beyond its boxes where the ghost fill takes them from (a
+        // level stepped alone); [1] it reads the filled ghost cells (a level of a hierarchy)
+        CMP_TRY(hipMalloc(&L->d_llev, 2 * sizeof(ListLevel)));
+        const ListLevel lv[2] = {{L->kp, L->d_boxes, L->d_fd, L->d_err}, {L->kp, L->d_boxes, nullptr, L->d_err}};
+        CMP_TRY(hipMemcpy(L->d_llev, lv, 2 * sizeof(ListLevel), hipMemcpyHostToDevice));
     }
     if (!L->d_lctr) {
         CMP_TRY(hipMalloc(&L->d_lctr, sizeof(int) * LIST_NCTR));
@@ -107,17 +125,23 @@ static int level_build_list(cmp_level* L)
     return CMP_OK;
 }
 
-// One flow stage of L through its work-list (fused ghost reads, as cmp_level_rk3_step).
+// One flow stage of L through its work-list (fused: the ghost reads of cmp_level_rk3_step, else the
+// filled ghost cells, as in a hierarchy's stage); the boxes the
+// fixed-shape kernel keeps by a k_stage launch over their tiles first, on the same stream (the two
+// launches write disjoint cells and max-accumulate one c_max).
 int level_list_stage(cmp_level* L, int mode, int stage, const double* Us, const double* Un, double* Uout,
-                     const double* dt, double* cmax, hipStream_t s)
+                     const double* dt, double* cmax, hipStream_t s, bool fused)
 {
+    if (L->nftiles > 0)
+        CMP_TRY(launch_stage(L->lc, L->kp, mode, stage, L->d_ftiles, L->nftiles, L->d_boxes, nullptr, 0, Us, Un, Uout,
+                             dt, (unsigned long long*)cmax, L->d_err, s, nullptr, fused ? L->d_fd : nullptr));
     ListCall c{};
     c.Us[0] = Us;
     c.Un[0] = Un;
     c.Uout[0] = Uout;
     c.cmax[0] = (unsigned long long*)cmax;
-    CMP_TRY(launch_stage_list(L->lc, L->list_waves, L->d_llev, L->d_ltiles, (int)L->h_ltiles.size(), L->d_lctr, c, mode,
-                              stage, dt, L->list_grid, s));
+    CMP_TRY(launch_stage_list(L->lc, L->list_waves, L->d_llev + (fused ? 0 : 1), L->d_ltiles, (int)L->h_ltiles.size(),
+                              L->d_lctr, c, mode, stage, dt, L->list_grid, s));
     return CMP_OK;
 }
 
@@ -128,6 +152,9 @@ namespace {
 constexpr int MAX_CAND = 12;
 constexpr int NWAVES = 2;            // occupancy targets of the work-list kernel: 2, 3 waves per SIMD
 constexpr int WAVES[NWAVES] = {2, 3};
+// what a level that mixes the two kernels pays for its second launch per stage (ns), added to its
+// summed swept time when the tuner weighs it against the fixed or the work-list kernel alone
+constexpr double MIX_NS = 4000.0;
 
 struct TuneKey {
     int dim, scheme, limiter, riemann, thinc, nphase, ng, n[3];
@@ -323,15 +350,23 @@ void box_shape(const cmp_level* L, int b, int* n)
 }  // namespace
 
 // The cached choices for L (every box shape of L must be in the cache): per occupancy target, each
-// box's candidate of least time and the level's summed time; the fixed kernel's summed time.
+// box's candidate of least time and the level's summed time; the fixed kernel's summed time; with
+// ext_mixed / t_mixed, per occupancy target each box by the faster of the two kernels (0 0 0: the
+// fixed one) and that summed time, plus MIX_NS for the second launch when the level has boxes of both.
 // CMP_ERR_UNSUPPORTED when a box shape has no usable entry.
-int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, double* t_fixed)
+int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, double* t_fixed,
+                std::vector<int>* ext_mixed, double* t_mixed)
 {
     std::lock_guard<std::mutex> g(g_mu);
     *t_fixed = 0.0;
+    int nfix[NWAVES] = {};
     for (int w = 0; w < NWAVES; ++w) {
         ext[w].assign(3 * (size_t)L->nbox, 0);
         t_list[w] = 0.0;
+        if (ext_mixed) {
+            ext_mixed[w].assign(3 * (size_t)L->nbox, 0);
+            t_mixed[w] = 0.0;
+        }
     }
     for (int b = 0; b < L->nbox; ++b) {
         int n[3];
@@ -347,29 +382,93 @@ int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, doubl
             if (k < 0 || !tile_shape_ok(L->lc, E.cand[k])) return CMP_ERR_UNSUPPORTED;
             t_list[w] += cells * E.ns[k][w];
             for (int d = 0; d < 3; ++d) ext[w][3 * (size_t)b + d] = E.cand[k][d];
+            if (!ext_mixed) continue;
+            if (E.ns[k][w] < E.ns_fixed) {
+                t_mixed[w] += cells * E.ns[k][w];
+                for (int d = 0; d < 3; ++d) ext_mixed[w][3 * (size_t)b + d] = E.cand[k][d];
+            } else {
+                t_mixed[w] += cells * E.ns_fixed;
+                ++nfix[w];
+            }
         }
     }
+    if (ext_mixed)
+        for (int w = 0; w < NWAVES; ++w)
+            if (nfix[w] > 0 && nfix[w] < L->nbox) t_mixed[w] += MIX_NS;
     return CMP_OK;
 }
 
-// Applies the cached choices to L: the occupancy target of least summed time, each box's shape for
-// it, and the work-list kernel when it is faster in sum than the fixed kernel (force: 1 always, -1
-// never, 0 by the measured times).
-static int level_apply_tuning(cmp_level* L, int force)
+// Applies the cached choices to L (force: 1 the work-list kernel for every box at its better
+// occupancy target, -1 the fixed kernel). force 0: the summed sweeps give two candidates besides the
+// fixed kernel - the work-list kernel for every box, and each box by the faster of the two kernels,
+// each at its better occupancy target - and one RK stage of the level itself on U (HIP events, least
+// of 3) decides among the three: boxes of other shapes beside each other, and two launches in place
+// of one, do not cost what the sums of isolated sweeps say.
+static int level_apply_tuning(cmp_level* L, int force, const double* U, hipStream_t s)
 {
-    std::vector<int> ext[NWAVES];
-    double t_list[NWAVES], t_fixed = 0.0;
-    int r = level_tuned(L, ext, t_list, &t_fixed);
+    std::vector<int> ext[NWAVES], mix[NWAVES];
+    double t_list[NWAVES], t_mix[NWAVES], t_fixed = 0.0;
+    int r = level_tuned(L, ext, t_list, &t_fixed, mix, t_mix);
     if (r) return r;
-    const int w = (t_list[1] < t_list[0]) ? 1 : 0;
-    const bool use = force > 0 || (force == 0 && t_list[w] < t_fixed);
     L->gkey[0] = nullptr;   // a captured step is captured again
-    if (!use) {
-        L->use_list = false;
+    ++L->tiles_gen;
+    const int w = (t_list[1] < t_list[0]) ? 1 : 0;
+    const int wm = (t_mix[1] < t_mix[0]) ? 1 : 0;
+    L->use_list = false;
+    if (force < 0) return CMP_OK;
+    if (force > 0) {
+        L->box_ext = std::move(ext[w]);
+        L->list_waves = WAVES[w];
+        if ((r = level_build_list(L))) return r;
+        L->use_list = true;
         return CMP_OK;
     }
-    L->box_ext = std::move(ext[w]);
-    L->list_waves = WAVES[w];
+    struct Opt {
+        std::vector<int> e;
+        int w;
+    };
+    std::vector<Opt> opts;
+    opts.push_back({ext[w], w});
+    bool any = false;
+    for (size_t b = 0; b < mix[wm].size(); b += 3) any = any || mix[wm][b] > 0;
+    if (any && (mix[wm] != ext[w] || wm != w)) opts.push_back({mix[wm], wm});
+    long long total = 0;
+    for (const cmp_box& b : L->boxes) total += cmp_fab_size(L->P.dim, &b, L->ng, L->P.nstate);
+    // the level's error word as the measurement finds it, put back afterwards (a finer level's cells
+    // beyond its boxes are not all filled in U, so the measured stages may flag them)
+    int err0 = 0;
+    CMP_TRY(hipMemcpy(&err0, L->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    double *V = nullptr, *dt = nullptr;
+    CMP_TRY(hipMalloc(&V, sizeof(double) * (size_t)total));
+    r = check_hip(hipMalloc(&dt, sizeof(double)));
+    const double dtv = 1.0e-9;
+    if (!r) r = check_hip(hipMemcpy(dt, &dtv, sizeof(double), hipMemcpyHostToDevice));
+    double best = 0.0, ms = 0.0;
+    int pick = -1;
+    if (!r)
+        r = time_min([&]() -> int {
+            CMP_TRY(launch_stage(L->lc, L->kp, MODE_RK, 0, L->d_tiles, L->ntiles, L->d_boxes, nullptr, 0, U, nullptr, V,
+                                 dt, nullptr, L->d_err, s, nullptr, L->d_fd));
+            return CMP_OK;
+        }, 3, s, &best);
+    for (size_t k = 0; k < opts.size() && !r; ++k) {
+        L->box_ext = opts[k].e;
+        L->list_waves = WAVES[opts[k].w];
+        if ((r = level_build_list(L))) break;
+        r = time_min([&]() -> int { return level_list_stage(L, MODE_RK, 0, U, nullptr, V, dt, nullptr, s); }, 3, s, &ms);
+        if (!r && ms < best) {
+            best = ms;
+            pick = (int)k;
+        }
+    }
+    if (!r) r = check_hip(hipStreamSynchronize(s));
+    if (!r) r = check_hip(hipMemcpy(L->d_err, &err0, sizeof(int), hipMemcpyHostToDevice));
+    (void)hipFree(V);
+    if (dt) (void)hipFree(dt);
+    if (r) return r;
+    if (pick < 0) return CMP_OK;
+    L->box_ext = std::move(opts[pick].e);
+    L->list_waves = WAVES[opts[pick].w];
     if ((r = level_build_list(L))) return r;
     L->use_list = true;
     return CMP_OK;
@@ -386,18 +485,32 @@ int cmp_level_set_tiles(cmp_level* L, const int* ext, int waves)
     if (!L) return CMP_ERR_ARG;
     if (!ext) {
         L->use_list = false;
+        ++L->tiles_gen;
         L->gkey[0] = nullptr;
         return CMP_OK;
     }
     if (waves != 2 && waves != 3) return CMP_ERR_ARG;
     if (L->lc.riemann != 1 && L->lc.riemann != 2) return CMP_ERR_UNSUPPORTED;
     std::vector<int> e(3 * (size_t)L->nbox, 1);
+    int nlist = 0;
     for (int b = 0; b < L->nbox; ++b) {
+        const int* x = &ext[3 * (size_t)b];
+        if (x[0] == 0 && x[1] == 0 && x[2] == 0) {   // the fixed-shape kernel keeps this box
+            e[3 * (size_t)b] = e[3 * (size_t)b + 1] = e[3 * (size_t)b + 2] = 0;
+            continue;
+        }
         for (int d = 0; d < L->P.dim; ++d) {
             const int n = L->boxes[b].hi[d] - L->boxes[b].lo[d] + 1;
-            e[3 * (size_t)b + d] = std::min(ext[3 * (size_t)b + d], n);
+            e[3 * (size_t)b + d] = std::min(x[d], n);
         }
         if (!tile_shape_ok(L->lc, &e[3 * (size_t)b])) return CMP_ERR_ARG;
+        ++nlist;
+    }
+    if (nlist == 0) {
+        L->use_list = false;
+        ++L->tiles_gen;
+        L->gkey[0] = nullptr;
+        return CMP_OK;
     }
     L->box_ext = std::move(e);
     L->list_waves = waves;
@@ -467,12 +580,13 @@ int cmp_level_tune_tiles(cmp_level* L, const double* U, int nextra, const int* e
         std::lock_guard<std::mutex> g(g_mu);
         g_cache[key_of(L, n.data())] = E;
     }
+    const bool swept = (r == CMP_OK);
+    if (swept) r = level_apply_tuning(L, force, U, s);
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
-    if (r) return r;
-    r = level_apply_tuning(L, force);
-    if (r == CMP_ERR_UNSUPPORTED) {
+    if (swept && r == CMP_ERR_UNSUPPORTED) {
         L->use_list = false;
+        ++L->tiles_gen;
         return CMP_OK;
     }
     return r;

@@ -286,11 +286,14 @@ int cmp_level_info(const cmp_level* level, int* ntiles, int* tile_x, int* tile_y
  * cmp_level_set_tiles: the flow stage of cmp_level_rk3_step / cmp_level_advance (TVD-RK3) through
  * the work-list kernel, box b cut into tiles of ext[3b..3b+2] cells (clipped to the box; at most 256
  * cells, CMP_ERR_ARG for a shape the kernel's LDS layout does not hold), the kernel built for
- * `waves` (2 or 3) resident waves per SIMD; ext NULL: back to the fixed-shape kernel. Two-phase and
- * N-phase flow (not the Phase-Field flux). The results are those of the fixed-shape kernel bit for
- * bit.
+ * `waves` (2 or 3) resident waves per SIMD; ext NULL: back to the fixed-shape kernel. A box whose
+ * extents are 0 0 0 stays with the fixed-shape kernel: each stage is then a fixed-shape launch over
+ * those boxes' tiles and a work-list launch over the others (every box 0 0 0: the fixed kernel
+ * alone). Two-phase and N-phase flow (not the Phase-Field flux). The results are those of the
+ * fixed-shape kernel bit for bit.
  * cmp_level_tile_shapes: *use_list 1 when the work-list kernel is on, *waves its occupancy target
- * (0 when off); ext (3 ints per box, or NULL): each box's tile extents. */
+ * (0 when off); ext (3 ints per box, or NULL): each box's tile extents (0 0 0 for a box the
+ * fixed-shape kernel keeps in a level that mixes the two). */
 int cmp_level_set_tiles(cmp_level* level, const int* ext, int waves);
 int cmp_level_tile_shapes(const cmp_level* level, int* use_list, int* waves, int* ext);
 
@@ -299,9 +302,11 @@ int cmp_level_tile_shapes(const cmp_level* level, int* use_list, int* waves, int
  * THINC, nphase, ghost cells), a synthetic periodic level of copies of that box, its cells sampled
  * from the level's largest box in U, takes one RK stage per candidate tile shape with the work-list
  * kernel at both occupancy targets and one with the fixed-shape kernel (HIP events, least of 3); the
- * times per cell are cached. Then the level takes the occupancy target of least summed time, per box
- * its candidate of least time, and the work-list kernel when that sum is below the fixed kernel's
- * (force 1: always, -1: never). A box thinner than the ghost width is not swept, and a level holding
+ * times per cell are cached. The summed sweeps give two candidates besides the fixed kernel: the
+ * work-list kernel for every box (each box its candidate of least time) and each box by the faster
+ * of the two kernels, each at its better occupancy target; one RK stage of the level itself on U
+ * (HIP events, least of 3; the level's error word is left as it was) picks the fastest of the three
+ * (force 1: the work-list kernel for every box, -1: the fixed kernel). A box thinner than the ghost width is not swept, and a level holding
  * one keeps the fixed kernel. Allocates device memory while it runs; synchronises. */
 int cmp_level_tune_tiles(cmp_level* level, const double* U, int nextra, const int* extra, int force, void* stream);
 /* The cached sweep of one box shape (n[3]) for the level's scheme: the fixed-shape kernel's ns per
@@ -417,7 +422,11 @@ int cmp_level_average_down(const cmp_level* fine, const double* Uf, cmp_level* c
  * all components), the coarse neighbours read through the coarse level's own BCs. The fine domain
  * is the coarse one refined by ratio, with the same BCs; Uc's ghost cells are not read. A coarse
  * cell the coarse level does not hold is CMP_ERR_LAYOUT in the fine level's error word, and the
- * ghost keeps its value. Flow states only (two-phase or N-phase). */
+ * ghost keeps its value. Flow states only (two-phase or N-phase). The first call for a pair of
+ * levels and ratio finds every ghost cell's sources once (a device table the fine level keeps: it
+ * allocates, so that call is not made under stream capture; under capture without the table the
+ * sources are found per call) and every call after it gathers through the table, with the same
+ * values bit for bit; cmp_amr_advance builds the tables of its levels before it captures. */
 int cmp_level_fill_two_levels(cmp_level* fine, double* Uf, const cmp_level* coarse, const double* Uc, int ratio,
                               void* stream);
 

@@ -133,7 +133,26 @@ def test_tune_tiles_and_cache(gpu, tmp_path):
     assert np.array_equal(h, href) and np.array_equal(got, ref)
 
 
-def _hierarchy_steps(gpu, c, force, nsteps=3):
+def test_mixed_level_equals_fixed(gpu):
+    """A level that keeps some boxes with the fixed-shape kernel (extents 0 0 0) and steps the others
+    through the work-list kernel equals the fixed kernel bit for bit, and reports its choice."""
+    c = cases.nphase_rings_2d(128)
+    boxes = cases.uneven_boxes((128, 128), seed=7)
+    ref, href, _ = _steps(gpu, c, 3, boxes=boxes)
+    for waves in (2, 3):
+        ext = [v for b in range(len(boxes)) for v in ((0, 0, 0) if b % 3 == 0 else (24, 8, 1))]
+        got, h, (on, w, shapes) = _steps(gpu, c, 3, ext=ext, waves=waves, boxes=boxes)
+        assert on and w == waves
+        assert [s[0] == 0 for s in shapes] == [b % 3 == 0 for b in range(len(boxes))]
+        assert np.array_equal(h, href) and np.array_equal(got, ref), waves
+    # every box 0 0 0: the fixed kernel alone
+    _, _, (on, _, _) = _steps(gpu, c, 1, ext=[0, 0, 0] * len(boxes), boxes=boxes)
+    assert not on
+
+
+def _hierarchy_steps(gpu, c, force, nsteps=3, level_tiles=None):
+    """nsteps of the hierarchy; level_tiles: per level a function of the box index giving its tile
+    extents (0 0 0: the fixed kernel) and the levels' own choice instead of the tuner's."""
     import torch
     P = c["params"]
     levels, Us = [], []
@@ -141,8 +160,11 @@ def _hierarchy_steps(gpu, c, force, nsteps=3):
         lay = Layout(2, boxes, 3, P.nstate)
         levels.append(Level(P, g, lay))
         Us.append(torch.from_numpy(lay.pack(f)).to(gpu))
-    for lv, U in zip(levels, Us):
-        lv.tune_tiles(U)
+    for l, (lv, U) in enumerate(zip(levels, Us)):
+        if level_tiles is None:
+            lv.tune_tiles(U)
+        elif level_tiles[l] is not None:
+            lv.set_tiles([v for b in range(lv.layout.nbox) for v in level_tiles[l](b)], 3)
     amr = Amr(levels, 2)
     on, waves = amr.tiles(force)
     assert on == (force > 0)
@@ -172,6 +194,25 @@ def test_hierarchy_single_list_launch_equals_per_level(gpu, nphase):
                                   nphase=nphase, rings=bool(nphase), max_grid_size=16)
     ref, href = _hierarchy_steps(gpu, c, -1)
     got, h = _hierarchy_steps(gpu, c, 1)
+    assert np.array_equal(h, href)
+    for l, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a, b), (l, np.max(np.abs(a - b)))
+
+
+@pytest.mark.parametrize("nphase", [0, 3])
+def test_hierarchy_levels_mixed_tiles_equal_fixed(gpu, nphase):
+    """Each level of a hierarchy stepped with its own tiles (level 0 fixed, level 1 mixed, level 2
+    work-list only) through the per-level stages, which read the filled ghost cells, equals every
+    level on the fixed kernel bit for bit."""
+    def blocks(lo, hi, b):
+        return [([i, j], [i + b - 1, j + b - 1]) for j in range(lo, hi, b) for i in range(lo, hi, b)]
+
+    c = cases.interface_hierarchy(2, 32, [blocks(8, 56, 16), blocks(32, 80, 16) + blocks(80, 96, 8)[:1]],
+                                  nphase=nphase, rings=bool(nphase), max_grid_size=16)
+    none = [None, None, None]
+    ref, href = _hierarchy_steps(gpu, c, -1, level_tiles=none)
+    mixed = [None, lambda b: (0, 0, 0) if b % 2 else (8, 16, 1), lambda b: (16, 8, 1)]
+    got, h = _hierarchy_steps(gpu, c, -1, level_tiles=mixed)
     assert np.array_equal(h, href)
     for l, (a, b) in enumerate(zip(got, ref)):
         assert np.array_equal(a, b), (l, np.max(np.abs(a - b)))
