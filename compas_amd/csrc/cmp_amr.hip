@@ -649,9 +649,10 @@ int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* co
         CMP_TRY(hipStreamSynchronize(nullptr));
     }
     const int nlev = (int)A->L.size();
-    // the finer levels' two-level fill plans (they allocate, so before the capture)
+    // the finer levels' two-level fill and average-down plans (they allocate, so before the capture)
     for (int l = 1; l < nlev; ++l)
-        if ((r = level_fill2_plan(A->L[l], A->L[l - 1], A->ratio, cs))) return r;
+        if ((r = level_fill2_plan(A->L[l], A->L[l - 1], A->ratio, cs)) || (r = level_avg_plan(A->L[l], A->L[l - 1], A->ratio, cs)))
+            return r;
     std::vector<const void*> key;
     for (int l = 0; l < nlev; ++l) {
         key.push_back(Un[l]);
@@ -667,6 +668,8 @@ int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* co
         key.push_back((const void*)(uintptr_t)A->L[l]->tiles_gen);
         key.push_back(A->L[l]->d_f2plan);
         key.push_back((const void*)(uintptr_t)A->L[l]->f2_uid);
+        key.push_back(A->L[l]->d_avgplan);
+        key.push_back((const void*)(uintptr_t)A->L[l]->avg_uid);
     }
     if (!(A->gexec && A->gkey == key && A->gcfl == cfl && A->glimiter == limiter)) {
         if (A->gexec) { CMP_TRY(hipGraphExecDestroy(A->gexec)); A->gexec = nullptr; }

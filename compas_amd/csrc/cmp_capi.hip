@@ -66,10 +66,16 @@ template <int DIM>
 hipError_t launch_fill2_dim(int, const FillDev&, const FillDev*, const FillDev&, int, double*, const double*,
                             const double*, long long, int, int*, hipStream_t);
 template <int DIM>
+hipError_t launch_avgdown_plan_dim(const BoxDev*, const long long*, int, long long, const FillDev&, int, AvgCell*,
+                                   hipStream_t);
+template <int DIM>
+hipError_t launch_avgdown_apply_dim(const BoxDev*, const BoxDev*, const AvgCell*, long long, const double*, double*,
+                                    int, int, double, int*, hipStream_t);
+template <int DIM>
 hipError_t launch_fill2_plan_dim(const FillDev&, const FillDev&, int, void*, hipStream_t);
 template <int DIM>
-hipError_t launch_fill2_apply_dim(int, const FillDev&, const FillDev&, int, const void*, double*, const double*, int*,
-                                  hipStream_t);
+hipError_t launch_fill2_apply_dim(int, const FillDev&, const FillDev&, long long, int, const void*, double*,
+                                  const double*, int*, hipStream_t);
 template <int DIM>
 hipError_t launch_avgdown_dim(const BoxDev*, int, long long, const FillDev&, const double*, double*, int, int, double,
                               int*, hipStream_t);
@@ -892,7 +898,7 @@ int cmp_level_destroy(cmp_level* L)
     if (L->graph) (void)hipGraphDestroy(L->graph);
     if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
     void* bufs[] = {L->d_boxes, L->d_tiles, L->d_gprefix, L->d_owner, L->d_err, L->d_fd, L->d_toff, L->d_U3,
-                    L->d_clock, L->d_ltiles, L->d_llev, L->d_lctr, L->d_ftiles, L->d_f2plan};
+                    L->d_clock, L->d_ltiles, L->d_llev, L->d_lctr, L->d_ftiles, L->d_f2plan, L->d_avgprefix, L->d_avgplan};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     delete L;
@@ -1899,9 +1905,22 @@ int cmp_level_average_down(const cmp_level* F, const double* Uf, cmp_level* C, d
     double vol = 1.0;
     for (int d = 0; d < dim; ++d) vol *= ratio;
     const double volfrac = 1.0 / vol;
+    hipStream_t s = (hipStream_t)stream;
+    const bool have = F->d_avgplan && F->avg_uid == C->uid && F->avg_ratio == ratio;
+    if (!have) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        CMP_TRY(hipStreamIsCapturing(s, &cs));
+        int r;
+        if (cs == hipStreamCaptureStatusNone && (r = level_avg_plan(F, C, ratio, s))) return r;
+    }
+    if (F->d_avgplan && F->avg_uid == C->uid && F->avg_ratio == ratio) {
+        // one thread per covered coarse cell and component, through the plan
+        CMP_TRY((dim == 2 ? launch_avgdown_apply_dim<2> : launch_avgdown_apply_dim<3>)(
+            F->d_boxes, C->d_boxes, F->d_avgplan, F->avg_total, Uf, Uc, F->P.nstate, ratio, volfrac, C->d_err, s));
+        return CMP_OK;
+    }
     CMP_TRY((dim == 2 ? launch_avgdown_dim<2> : launch_avgdown_dim<3>)(F->d_boxes, F->nbox, maxc, C->fd, Uf, Uc,
-                                                                        F->P.nstate, ratio, volfrac, C->d_err,
-                                                                        (hipStream_t)stream));
+                                                                        F->P.nstate, ratio, volfrac, C->d_err, s));
     return CMP_OK;
 }
 
@@ -1924,6 +1943,35 @@ static int two_level_args(const cmp_level* F, const cmp_level* C, int ratio)
 }
 
 }  // extern "C"
+
+// The gather plan of F's average-down onto C at ratio (k_avgdown_plan; the boxes are aligned to ratio:
+// checked by the caller), built when F has none for that pair.
+int cmp::level_avg_plan(const cmp_level* F, const cmp_level* C, int ratio, hipStream_t s)
+{
+    if (F->d_avgplan && F->avg_uid == C->uid && F->avg_ratio == ratio) return CMP_OK;
+    for (void** p : {(void**)&F->d_avgprefix, (void**)&F->d_avgplan})
+        if (*p) {
+            CMP_TRY(hipFree(*p));
+            *p = nullptr;
+        }
+    F->avg_uid = 0;
+    F->avg_ratio = 0;
+    std::vector<long long> cp(F->nbox + 1, 0);
+    for (int b = 0; b < F->nbox; ++b) {
+        long long v = 1;
+        for (int d = 0; d < F->P.dim; ++d) v *= (F->boxes[b].hi[d] - F->boxes[b].lo[d] + 1) / ratio;
+        cp[b + 1] = cp[b] + v;
+    }
+    CMP_TRY(hipMalloc(&F->d_avgprefix, sizeof(long long) * cp.size()));
+    CMP_TRY(hipMemcpy(F->d_avgprefix, cp.data(), sizeof(long long) * cp.size(), hipMemcpyHostToDevice));
+    F->avg_total = cp[F->nbox];
+    CMP_TRY(hipMalloc(&F->d_avgplan, sizeof(AvgCell) * (size_t)std::max(1LL, F->avg_total)));
+    CMP_TRY((F->P.dim == 2 ? launch_avgdown_plan_dim<2> : launch_avgdown_plan_dim<3>)(
+        F->d_boxes, F->d_avgprefix, F->nbox, F->avg_total, C->fd, ratio, F->d_avgplan, s));
+    F->avg_uid = C->uid;
+    F->avg_ratio = ratio;
+    return CMP_OK;
+}
 
 // The gather plan of F's two-level fill from C at ratio (k_fill2_plan), built when F has none for that
 // pair; allocates device memory (not while the stream is captured).
@@ -1962,8 +2010,9 @@ int cmp_level_fill_two_levels(cmp_level* F, double* Uf, const cmp_level* C, cons
         if (cs == hipStreamCaptureStatusNone && (r = level_fill2_plan(F, C, ratio, s))) return r;
     }
     if (F->d_f2plan && F->f2_uid == C->uid && F->f2_ratio == ratio) {
-        CMP_TRY((dim == 2 ? launch_fill2_apply_dim<2> : launch_fill2_apply_dim<3>)(F->P.nstate, F->fd, C->fd, ratio,
-                                                                                 F->d_f2plan, Uf, Uc, F->d_err, s));
+        CMP_TRY((dim == 2 ? launch_fill2_apply_dim<2> : launch_fill2_apply_dim<3>)(F->P.nstate, F->fd, C->fd, F->nghost,
+                                                                                 ratio, F->d_f2plan, Uf, Uc, F->d_err,
+                                                                                 s));
         return CMP_OK;
     }
     // under capture without a plan: the search form (the same-level sources are the level's own boxes)

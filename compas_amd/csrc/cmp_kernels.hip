@@ -1356,6 +1356,83 @@ __global__ __launch_bounds__(NT) void k_avgdown(const BoxDev* __restrict__ fboxe
     }
 }
 
+// The average-down as a gather: k_avgdown_plan finds, once for a pair of levels, what k_avgdown finds
+// per call for each covered coarse cell (numbered box by box, cprefix: nfbox + 1 prefix sums of each
+// fine box's covered coarse cells, x fastest inside a box) - its fine box and the fab offset of its
+// first fine cell, its owning coarse box and its offset there (cbox < 0: error code -cbox) - and
+// k_avgdown_apply, one thread per covered cell and component, sums the same fine cells in the same
+// order as k_avgdown.
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_avgdown_plan(const BoxDev* __restrict__ fboxes,
+                                                     const long long* __restrict__ cprefix, int nfbox,
+                                                     long long total, FillDev cfd, int ratio,
+                                                     AvgCell* __restrict__ plan)
+{
+    const long long qg = (long long)blockIdx.x * NT + threadIdx.x;
+    if (qg >= total) return;
+    int lo = 0, hi = nfbox - 1;
+    while (lo < hi) {   // the fine box whose covered cells hold qg
+        const int mid = (lo + hi + 1) >> 1;
+        if (cprefix[mid] <= qg) lo = mid;
+        else hi = mid - 1;
+    }
+    const BoxDev fb = fboxes[lo];
+    int clo[3] = {0, 0, 0}, ce[3] = {1, 1, 1};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        clo[d] = fb.lo[d] / ratio;
+        ce[d] = (fb.hi[d] + 1) / ratio - clo[d];
+    }
+    int ic, jc, kc;
+    pos3(qg - cprefix[lo], ce[0], ce[1], ic, jc, kc);
+    ic += clo[0];
+    jc += clo[1];
+    kc += clo[2];
+    AvgCell a;
+    a.fbox = lo;
+    a.fcell = (int)fab_index(fb, ic * ratio, jc * ratio, (DIM == 3) ? kc * ratio : 0);
+    a.cbox = -CMP_ERR_LAYOUT;
+    a.ccell = 0;
+    const int cc[3] = {ic, jc, kc};
+    int bi = 0;
+    bool in = true;
+    for (int d = DIM - 1; d >= 0; --d) {
+        if (cc[d] < cfd.dlo[d] || cc[d] > cfd.dhi[d]) in = false;
+        else bi = bi * cfd.nb[d] + (cc[d] - cfd.dlo[d]) / cfd.bs;
+    }
+    if (in) {
+        const int own = cfd.owner[bi];
+        if (own >= 0) {
+            a.cbox = own;
+            a.ccell = (int)fab_index(cfd.boxes[own], ic, jc, kc);
+        }
+    }
+    plan[qg] = a;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(NT) void k_avgdown_apply(const BoxDev* __restrict__ fboxes,
+                                                      const BoxDev* __restrict__ cboxes,
+                                                      const AvgCell* __restrict__ plan, long long total,
+                                                      const double* __restrict__ Uf, double* Uc, int N, int ratio,
+                                                      double volfrac, int* err)
+{
+    const long long t = (long long)blockIdx.x * NT + threadIdx.x;
+    if (t >= total * N) return;
+    const int n = (int)(t / total);
+    const AvgCell a = plan[t - (long long)n * total];
+    if (a.cbox < 0) { atomicMax(err, -a.cbox); return; }
+    const BoxDev fb = fboxes[a.fbox];
+    const BoxDev cb = cboxes[a.cbox];
+    const double* __restrict__ f = Uf + fb.off + a.fcell + (long long)n * fb.sn;
+    const int rz = (DIM == 3) ? ratio : 1;
+    double c = 0.0;
+    for (int kr = 0; kr < rz; ++kr)
+        for (int jr = 0; jr < ratio; ++jr)
+            for (int ir = 0; ir < ratio; ++ir) c += f[ir + (long long)jr * fb.sj + (long long)kr * fb.sk];
+    Uc[cb.off + a.ccell + (long long)n * cb.sn] = volfrac * c;
+}
+
 // A coarse level's cell c (any index within one cell of its domain) as the coarse ghost fill would
 // give it: mapped through the coarse level's BCs, read from the box owning it, momenta flipped.
 template <int DIM, int N, int MOM>
@@ -1528,6 +1605,8 @@ void k_fill2_plan(FillDev fd, FillDev cd, int box0, int ratio, F2Cell<DIM>* __re
         p.kind = 0;
         p.err = 0;
         int src[3] = {bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z};
+        p.dbox = b;
+        p.dcell = (int)fab_index(bx, src[0], src[1], src[2]);
         unsigned flip;
         int e = map_cell<DIM>(fd, src, flip);
         p.flip = flip & 7u;
@@ -1559,26 +1638,16 @@ void k_fill2_plan(FillDev fd, FillDev cd, int box0, int ratio, F2Cell<DIM>* __re
 
 template <int DIM, int N>
 __global__ __launch_bounds__(NT)
-void k_fill2_apply(FillDev fd, FillDev cd, int box0, int ratio, const F2Cell<DIM>* __restrict__ plan, double* Uf,
-                   const double* __restrict__ Uc, int* err)
+void k_fill2_apply(FillDev fd, FillDev cd, long long nghost, int ratio, const F2Cell<DIM>* __restrict__ plan,
+                   double* Uf, const double* __restrict__ Uc, int* err)
 {
 #pragma clang fp contract(off)
     constexpr int MOM = (N == 4 + DIM) ? 2 : ((N > 3) ? (N - DIM - 1) / 2 : -100);
-    const int b = box0 + (int)blockIdx.y;
-    const BoxDev bx = fd.boxes[b];
-    const int ng = fd.ng;
-    const int nv0 = bx.hi[0] - bx.lo[0] + 1, nv1 = bx.hi[1] - bx.lo[1] + 1;
-    const int nv2 = (DIM == 3) ? bx.hi[2] - bx.lo[2] + 1 : 1;
-    const int ex = nv0 + 2 * ng, ey = nv1 + 2 * ng;
-    const int ez = (DIM == 3) ? nv2 + 2 * ng : 1;
-    const int ncell = ex * ey * ez - nv0 * nv1 * nv2;
-    const F2Cell<DIM>* __restrict__ pb = plan + fd.gprefix[b];
-    for (int g = (int)(blockIdx.x * NT + threadIdx.x); g < ncell; g += (int)(gridDim.x * NT)) {
-        const F2Cell<DIM> p = pb[g];
+    for (long long g = (long long)blockIdx.x * NT + threadIdx.x; g < nghost; g += (long long)gridDim.x * NT) {
+        const F2Cell<DIM> p = plan[g];
         if (p.err) { atomicMax(err, p.err); continue; }
-        int x, y, z;
-        ghost_xyz<DIM>(g, ng, ex, ey, ez, nv1, x, y, z);
-        const long long di = bx.off + fab_index(bx, bx.flo[0] + x, bx.flo[1] + y, bx.flo[2] + z);
+        const BoxDev bx = fd.boxes[p.dbox];
+        const long long di = bx.off + p.dcell;
         // the N components of source k of the entry (momenta flipped by its bits)
         auto load = [&](const FillDev& f, const double* __restrict__ U, int k, double* v) {
             const BoxDev so = f.boxes[p.box[k]];
@@ -1943,24 +2012,21 @@ hipError_t launch_fill2_plan_dim(const FillDev& fd, const FillDev& cd, int ratio
 }
 
 template <int DIM>
-hipError_t launch_fill2_apply_dim(int N, const FillDev& fd, const FillDev& cd, int ratio, const void* plan, double* Uf,
-                                  const double* Uc, int* err, hipStream_t s)
+hipError_t launch_fill2_apply_dim(int N, const FillDev& fd, const FillDev& cd, long long nghost, int ratio,
+                                  const void* plan, double* Uf, const double* Uc, int* err, hipStream_t s)
 {
     if (N != 4 + DIM && N != 5 + DIM && N != 7 + DIM) return hipErrorInvalidValue;
-    long long cols = (fd.maxghost + NT - 1) / NT;
-    if (cols > 65536) cols = 65536;
-    if (cols <= 0) return hipSuccess;
+    if (nghost <= 0) return hipSuccess;
+    long long blocks = (nghost + NT - 1) / NT;
+    if (blocks > (1LL << 20)) blocks = 1LL << 20;
+    const dim3 grid((unsigned)blocks);
     const F2Cell<DIM>* pl = (const F2Cell<DIM>*)plan;
-    for (int box0 = 0; box0 < fd.nbox; box0 += 65535) {
-        const int rows = (fd.nbox - box0 < 65535) ? fd.nbox - box0 : 65535;
-        const dim3 grid((unsigned)cols, (unsigned)rows);
-        if (N == 4 + DIM)
-            hipLaunchKernelGGL((k_fill2_apply<DIM, 4 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, pl, Uf, Uc, err);
-        else if (N == 5 + DIM)
-            hipLaunchKernelGGL((k_fill2_apply<DIM, 5 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, pl, Uf, Uc, err);
-        else
-            hipLaunchKernelGGL((k_fill2_apply<DIM, 7 + DIM>), grid, dim3(NT), 0, s, fd, cd, box0, ratio, pl, Uf, Uc, err);
-    }
+    if (N == 4 + DIM)
+        hipLaunchKernelGGL((k_fill2_apply<DIM, 4 + DIM>), grid, dim3(NT), 0, s, fd, cd, nghost, ratio, pl, Uf, Uc, err);
+    else if (N == 5 + DIM)
+        hipLaunchKernelGGL((k_fill2_apply<DIM, 5 + DIM>), grid, dim3(NT), 0, s, fd, cd, nghost, ratio, pl, Uf, Uc, err);
+    else
+        hipLaunchKernelGGL((k_fill2_apply<DIM, 7 + DIM>), grid, dim3(NT), 0, s, fd, cd, nghost, ratio, pl, Uf, Uc, err);
     return hipGetLastError();
 }
 
@@ -2000,6 +2066,28 @@ hipError_t launch_avgdown_dim(const BoxDev* fboxes, int nfbox, long long maxcell
 {
     const dim3 grid((unsigned)((maxcells + NT - 1) / NT), (unsigned)nfbox);
     hipLaunchKernelGGL((k_avgdown<DIM>), grid, dim3(NT), 0, s, fboxes, cfd, Uf, Uc, N, ratio, volfrac, err);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_avgdown_plan_dim(const BoxDev* fboxes, const long long* cprefix, int nfbox, long long total,
+                                   const FillDev& cfd, int ratio, AvgCell* plan, hipStream_t s)
+{
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_avgdown_plan<DIM>), dim3((unsigned)((total + NT - 1) / NT)), dim3(NT), 0, s, fboxes, cprefix,
+                       nfbox, total, cfd, ratio, plan);
+    return hipGetLastError();
+}
+
+template <int DIM>
+hipError_t launch_avgdown_apply_dim(const BoxDev* fboxes, const BoxDev* cboxes, const AvgCell* plan, long long total,
+                                    const double* Uf, double* Uc, int N, int ratio, double volfrac, int* err,
+                                    hipStream_t s)
+{
+    const long long n = total * N;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_avgdown_apply<DIM>), dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, s, fboxes, cboxes, plan,
+                       total, Uf, Uc, N, ratio, volfrac, err);
     return hipGetLastError();
 }
 
@@ -2075,10 +2163,15 @@ template hipError_t launch_fill2_dim<CMP_KERNEL_DIM>(int, const FillDev&, const 
                                                      double*, const double*, const double*, long long, int, int*,
                                                      hipStream_t);
 template hipError_t launch_fill2_plan_dim<CMP_KERNEL_DIM>(const FillDev&, const FillDev&, int, void*, hipStream_t);
-template hipError_t launch_fill2_apply_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev&, int, const void*, double*,
-                                                           const double*, int*, hipStream_t);
+template hipError_t launch_fill2_apply_dim<CMP_KERNEL_DIM>(int, const FillDev&, const FillDev&, long long, int,
+                                                           const void*, double*, const double*, int*, hipStream_t);
 template hipError_t launch_avgdown_dim<CMP_KERNEL_DIM>(const BoxDev*, int, long long, const FillDev&, const double*,
                                                        double*, int, int, double, int*, hipStream_t);
+template hipError_t launch_avgdown_plan_dim<CMP_KERNEL_DIM>(const BoxDev*, const long long*, int, long long,
+                                                            const FillDev&, int, AvgCell*, hipStream_t);
+template hipError_t launch_avgdown_apply_dim<CMP_KERNEL_DIM>(const BoxDev*, const BoxDev*, const AvgCell*, long long,
+                                                             const double*, double*, int, int, double, int*,
+                                                             hipStream_t);
 template hipError_t launch_tag_dim<CMP_KERNEL_DIM>(const LaunchCfg&, const KParams&, const BoxDev*, int, long long,
                                                    const long long*, const double*, const TagCriteria&,
                                                    unsigned char*, int*, hipStream_t);

@@ -85,9 +85,17 @@ struct F2Cell {
     static constexpr int NS = 1 + 2 * DIM;
     int box[NS];
     int cell[NS];
+    int dbox, dcell;   // the ghost cell itself: its box and its offset in that box's fab
     unsigned flip;
     int kind;   // bit 0: interpolated from the coarse level; bits 8 + 8d: offset in the coarse cell along d
     int err;    // nonzero: the fill's error code for this cell (nothing written)
+};
+
+// One covered coarse cell of an average-down, found once for a pair of levels (k_avgdown_plan): the
+// fine box and the fab offset of the cell's first fine cell, the owning coarse box and the cell's
+// offset there (cbox < 0: the error code -cbox, nothing written).
+struct AvgCell {
+    int fbox, fcell, cbox, ccell;
 };
 
 enum StageMode { MODE_RK = 0, MODE_RHS = 1, MODE_CMAX = 2 };

@@ -36,6 +36,9 @@ int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, doubl
 // The gather plan of fine level F's two-level fill from coarse level C at ratio, built when F holds
 // none for that pair (cmp_capi.hip); allocates device memory, so not under stream capture.
 int level_fill2_plan(cmp_level* F, const cmp_level* C, int ratio, hipStream_t s);
+// The gather plan of fine level F's average-down onto coarse level C at ratio, built when F holds
+// none for that pair (cmp_capi.hip); allocates device memory, so not under stream capture.
+int level_avg_plan(const cmp_level* F, const cmp_level* C, int ratio, hipStream_t s);
 
 }  // namespace cmp
 
@@ -186,4 +189,12 @@ struct cmp_level {
     void* d_f2plan = nullptr;
     unsigned long long f2_uid = 0;
     int f2_ratio = 0;
+    // the average-down onto the coarser level avg_uid at ratio avg_ratio (0: none built): per box the
+    // prefix sums of the coarse cells it covers (nbox + 1), avg_total in all, and each covered cell's
+    // sources (AvgCell)
+    mutable long long* d_avgprefix = nullptr;
+    mutable cmp::AvgCell* d_avgplan = nullptr;
+    mutable unsigned long long avg_uid = 0;
+    mutable int avg_ratio = 0;
+    mutable long long avg_total = 0;
 };

@@ -410,7 +410,10 @@ int cmp_level_tag(cmp_level* level, const double* U, const cmp_tag_criterion* cr
  * box takes the mean of its ratio^dim children (sum over the children, k slowest, times
  * 1/ratio^dim). The fine level's domain is the coarse one refined by ratio and its boxes are
  * aligned to ratio; a covered coarse cell outside every coarse box is CMP_ERR_LAYOUT in the coarse
- * level's error word. Uf (fine level buffer) and Uc (coarse) are device buffers. */
+ * level's error word. Uf (fine level buffer) and Uc (coarse) are device buffers. The first call for a
+ * pair of levels and ratio finds every covered cell's sources once (a device table the fine level
+ * keeps: it allocates, so that call is not made under stream capture; under capture without the
+ * table they are found per call); cmp_amr_advance builds its levels' tables before it captures. */
 int cmp_level_average_down(const cmp_level* fine, const double* Uf, cmp_level* coarse, double* Uc, int ratio,
                            void* stream);
 
