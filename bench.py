@@ -542,10 +542,10 @@ def served(args):
     W = args.warmup
     pf = _amr_pf(args)   # CAC-Adv, implicit RC (the served configuration's regularization)
     # warm-up: a few small cases through the same loop (kernels loaded, allocator primed)
-    serve.serve(max(rates), max(2, W), steps=5, seed=999, pf=pf)
+    serve.serve(max(rates), max(2, W), steps=5, seed=999, pf=pf, pf_threads=args.pf_threads)
     runs = []
     for r in rates:
-        res = serve.serve(r, args.cases, steps=args.case_steps, seed=1, pf=pf)
+        res = serve.serve(r, args.cases, steps=args.case_steps, seed=1, pf=pf, pf_threads=args.pf_threads)
         res.pop("finals")
         runs.append(res)
     alone = serve.one_at_a_time(args.cases, steps=args.case_steps, seed=1, pf=pf)
@@ -802,6 +802,8 @@ def main():
     ap.add_argument("--case-steps", type=int, default=200, help="served-stream: RK3 steps per case")
     ap.add_argument("--no-pf", action="store_true",
                     help="shock-bubble-amr, nphase-rings-amr, served-stream: the flow step alone, without the CAC Phase-Field step")
+    ap.add_argument("--pf-threads", type=int, default=0,
+                    help="served-stream: host workers driving the cases' Phase-Field steps (0: the library's default)")
     ap.add_argument("--regrid-int", type=int, default=10,
                     help="shock-bubble-amr, nphase-rings-amr: regrid every this many steps inside the timed run "
                          "(amr.regrid_int; 0: the hierarchy fixed)")
