@@ -708,17 +708,25 @@ int cmp_amr_advance(cmp_amr* A, double* const* Un, double* const* U1, double* co
 
 namespace {
 
-// valid cells (host): the index of cell c of box b of a level in its alpha layout (1 ghost, 1 comp)
-long long aidx(const cmp_level* L, int b, const int* c)
+// the alpha layout's box offsets of a level (1 ghost, 1 comp): nbox + 1 prefix sums of the fab sizes
+std::vector<long long> alpha_offsets(const cmp_level* L)
+{
+    std::vector<long long> off(L->nbox + 1, 0);
+    for (int q = 0; q < L->nbox; ++q) {
+        long long f = 1;
+        for (int d = 0; d < L->P.dim; ++d) f *= L->boxes[q].hi[d] - L->boxes[q].lo[d] + 3;
+        off[q + 1] = off[q] + f;
+    }
+    return off;
+}
+
+// valid cells (host): the index of cell c of box b of a level in its alpha layout (1 ghost, 1 comp);
+// aoff: that level's alpha_offsets
+long long aidx(const cmp_level* L, const std::vector<long long>& aoff, int b, const int* c)
 {
     const cmp_box& B = L->boxes[b];
     const int dim = L->P.dim;
-    long long off = 0;
-    for (int q = 0; q < b; ++q) {
-        long long f = 1;
-        for (int d = 0; d < dim; ++d) f *= L->boxes[q].hi[d] - L->boxes[q].lo[d] + 3;
-        off += f;
-    }
+    const long long off = aoff[b];
     long long idx = 0, st = 1;
     for (int d = 0; d < dim; ++d) {
         idx += (long long)(c[d] - B.lo[d] + 1) * st;
@@ -765,9 +773,10 @@ int amr_comp_build(cmp_amr* A)
     const int dim = A->L[0]->P.dim;
     std::vector<CompCell> cells;
     std::vector<CompLink> links;
-    std::vector<std::vector<long long>> foff(nlev);
+    std::vector<std::vector<long long>> foff(nlev), aoff(nlev);
     for (int l = 0; l < nlev; ++l) {
         const cmp_level* L = A->L[l];
+        aoff[l] = alpha_offsets(L);
         foff[l].resize((size_t)L->nbox * dim);
         cmp_face_layout(dim, L->nbox, L->boxes.data(), 1, foff[l].data(), nullptr);
     }
@@ -780,6 +789,8 @@ int amr_comp_build(cmp_amr* A)
     double vol_total = 0.0;
     long long ncomp = 0;
     P.d_mask.assign(nlev, nullptr);
+    std::vector<CompLink> lk;   // one cell's links (its storage kept from cell to cell)
+    lk.reserve(64);
     for (int l = 0; l < nlev; ++l) {
         const cmp_level* L = A->L[l];
         std::vector<int> mask((size_t)L->na, COMP_REGULAR);
@@ -792,7 +803,7 @@ int amr_comp_build(cmp_amr* A)
                 for (int j = B.lo[1]; j <= B.hi[1]; ++j)
                     for (int i = B.lo[0]; i <= B.hi[0]; ++i) {
                         const int c[3] = {i, j, k};
-                        const long long ci = aidx(L, b, c);
+                        const long long ci = aidx(L, aoff[l], b, c);
                         int f[3] = {0, 0, 0};
                         for (int d = 0; d < dim; ++d) f[d] = c[d] * ratio;
                         if (l + 1 < nlev && owner_of(A->L[l + 1], f) >= 0) {
@@ -801,7 +812,7 @@ int amr_comp_build(cmp_amr* A)
                         }
                         vol_total += vol;
                         ++ncomp;
-                        std::vector<CompLink> lk;
+                        lk.clear();
                         bool special = false;
                         for (int d = 0; d < dim; ++d)
                             for (int side = 0; side < 2; ++side) {
@@ -815,7 +826,7 @@ int amr_comp_build(cmp_amr* A)
                                 for (int e = 0; e < dim; ++e) fx[e] = x[e] * ratio;
                                 const bool xcov = nb >= 0 && l + 1 < nlev && owner_of(A->L[l + 1], fx) >= 0;
                                 if (nb >= 0 && !xcov) {
-                                    lk.push_back(CompLink{l, l, aidx(L, nb, x), aface(L, foff[l], b, d, fc),
+                                    lk.push_back(CompLink{l, l, aidx(L, aoff[l], nb, x), aface(L, foff[l], b, d, fc),
                                                           geo_of(l, d) / L->G.dx[d]});
                                     continue;
                                 }
@@ -839,7 +850,7 @@ int amr_comp_build(cmp_amr* A)
                                             gf[d] += side ? 0 : 1;
                                             const int fb = owner_of(F, g);
                                             if (fb < 0) return CMP_ERR_LAYOUT;
-                                            lk.push_back(CompLink{l + 1, l + 1, aidx(F, fb, g), aface(F, foff[l + 1], fb, d, gf),
+                                            lk.push_back(CompLink{l + 1, l + 1, aidx(F, aoff[l + 1], fb, g), aface(F, foff[l + 1], fb, d, gf),
                                                                   geo_of(l + 1, d) / dist});
                                         }
                                 } else {
@@ -851,7 +862,7 @@ int amr_comp_build(cmp_amr* A)
                                     if (!wrap_cell(C, p)) return CMP_ERR_LAYOUT;
                                     const int pb = owner_of(C, p);
                                     if (pb < 0) return CMP_ERR_LAYOUT;
-                                    lk.push_back(CompLink{l - 1, l, aidx(C, pb, p), aface(L, foff[l], b, d, fc),
+                                    lk.push_back(CompLink{l - 1, l, aidx(C, aoff[l - 1], pb, p), aface(L, foff[l], b, d, fc),
                                                           geo_of(l, d) / (0.5 * (L->G.dx[d] + C->G.dx[d]))});
                                 }
                             }
