@@ -81,6 +81,59 @@ def test_fill_two_levels_matches_oracle(gpu, dim, ratio, nphase):
     assert (got != fbuf).sum() > 0
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fill_and_average_down_captured_without_tables(gpu, dim):
+    """The two-level fill and the average-down keep per pair of levels a table of each cell's sources,
+    built by the first call that is not under stream capture. Captured into a graph before any call
+    has built them, they take their per-call source searches instead; replayed (several times, with
+    eager work between the replays) they equal the table forms bit for bit."""
+    import torch
+    ratio = 2
+    nc = 16 if dim == 2 else 8
+    P = make_params(dim, gamma=(1.4, 1.6))
+    bc = BCS[dim]
+    cg = make_geometry((nc,) * dim, (0.0,) * dim, (1.0,) * dim, bc["per"], lo_bc=bc["lo_bc"], hi_bc=bc["hi_bc"])
+    fg = make_geometry((nc * ratio,) * dim, (0.0,) * dim, (1.0,) * dim, bc["per"], lo_bc=bc["lo_bc"],
+                       hi_bc=bc["hi_bc"])
+    cl = Layout(dim, split_domain((nc,) * dim, max(4, nc // 2)), 3, P.nstate)
+    fl = Layout(dim, FILL2[(dim, ratio)], 3, P.nstate)
+    rng = np.random.default_rng(5 + dim)
+    cbuf = rng.uniform(0.2, 2.0, cl.total)
+    fbuf = rng.uniform(0.2, 2.0, fl.total)
+    # the table forms: eager calls on one pair of levels
+    lf, lc = Level(P, fg, fl), Level(P, cg, cl)
+    Uf_t = torch.from_numpy(fbuf).to(gpu)
+    Uc_t = torch.from_numpy(cbuf).to(gpu)
+    lf.FillPatchTwoLevels(Uf_t, lc, Uc_t, ratio)
+    lf.AverageDownTo(Uf_t, lc, Uc_t, ratio)
+    assert lf.check_errors() == 0 and lc.check_errors() == 0
+    assert not torch.equal(Uf_t, torch.from_numpy(fbuf).to(gpu))
+    assert not torch.equal(Uc_t, torch.from_numpy(cbuf).to(gpu))
+    # the search forms: a fresh pair of levels, captured first
+    lf2, lc2 = Level(P, fg, fl), Level(P, cg, cl)
+    Uf0 = torch.from_numpy(fbuf).to(gpu)
+    Uc0 = torch.from_numpy(cbuf).to(gpu)
+    Uf_g, Uc_g = Uf0.clone(), Uc0.clone()
+    g = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        cur = torch.cuda.current_stream()
+        lf2.FillPatchTwoLevels(Uf_g, lc2, Uc_g, ratio, stream=cur)
+        lf2.AverageDownTo(Uf_g, lc2, Uc_g, ratio, stream=cur)
+    for rep in range(3):
+        Uf_g.copy_(Uf0)
+        Uc_g.copy_(Uc0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(Uf_g, Uf_t), rep
+        assert torch.equal(Uc_g, Uc_t), rep
+    assert lf2.check_errors() == 0 and lc2.check_errors() == 0
+    del g
+    for lv in (lf, lc, lf2, lc2):
+        lv.close()
+
+
 # ---- the stage-coupled step of a hierarchy (cmp_amr_*) ---------------------------------------------
 
 def _hierarchy(dim, nc, fine_boxes, seed=3, third=None, **kw):
