@@ -399,11 +399,12 @@ int level_tuned(const cmp_level* L, std::vector<int>* ext, double* t_list, doubl
 }
 
 // Applies the cached choices to L (force: 1 the work-list kernel for every box at its better
-// occupancy target, -1 the fixed kernel). force 0: the summed sweeps give two candidates besides the
-// fixed kernel - the work-list kernel for every box, and each box by the faster of the two kernels,
-// each at its better occupancy target - and one RK stage of the level itself on U (HIP events, least
-// of 3) decides among the three: boxes of other shapes beside each other, and two launches in place
-// of one, do not cost what the sums of isolated sweeps say.
+// occupancy target, -1 the fixed kernel). force 0: the summed sweeps give up to two candidates
+// besides the fixed kernel - the work-list kernel for every box, and each box by the faster of the
+// two kernels, each at its better occupancy target, each only when its sum beats the fixed kernel's -
+// and one RK stage of the level itself on U (HIP events, least of 3) decides among them and the
+// fixed kernel: boxes of other shapes beside each other, and two launches in place of one, do not
+// cost what the sums of isolated sweeps say.
 static int level_apply_tuning(cmp_level* L, int force, const double* U, hipStream_t s)
 {
     std::vector<int> ext[NWAVES], mix[NWAVES];
@@ -427,11 +428,13 @@ static int level_apply_tuning(cmp_level* L, int force, const double* U, hipStrea
         std::vector<int> e;
         int w;
     };
+    // the candidates whose summed sweeps beat the fixed kernel's; none: the fixed kernel, unmeasured
     std::vector<Opt> opts;
-    opts.push_back({ext[w], w});
+    if (t_list[w] < t_fixed) opts.push_back({ext[w], w});
     bool any = false;
     for (size_t b = 0; b < mix[wm].size(); b += 3) any = any || mix[wm][b] > 0;
-    if (any && (mix[wm] != ext[w] || wm != w)) opts.push_back({mix[wm], wm});
+    if (any && t_mix[wm] < t_fixed && (opts.empty() || mix[wm] != ext[w] || wm != w)) opts.push_back({mix[wm], wm});
+    if (opts.empty()) return CMP_OK;
     long long total = 0;
     for (const cmp_box& b : L->boxes) total += cmp_fab_size(L->P.dim, &b, L->ng, L->P.nstate);
     // the level's error word as the measurement finds it, put back afterwards (a finer level's cells

@@ -302,11 +302,12 @@ int cmp_level_tile_shapes(const cmp_level* level, int* use_list, int* waves, int
  * THINC, nphase, ghost cells), a synthetic periodic level of copies of that box, its cells sampled
  * from the level's largest box in U, takes one RK stage per candidate tile shape with the work-list
  * kernel at both occupancy targets and one with the fixed-shape kernel (HIP events, least of 3); the
- * times per cell are cached. The summed sweeps give two candidates besides the fixed kernel: the
- * work-list kernel for every box (each box its candidate of least time) and each box by the faster
- * of the two kernels, each at its better occupancy target; one RK stage of the level itself on U
- * (HIP events, least of 3; the level's error word is left as it was) picks the fastest of the three
- * (force 1: the work-list kernel for every box, -1: the fixed kernel). A box thinner than the ghost width is not swept, and a level holding
+ * times per cell are cached. The summed sweeps give up to two candidates besides the fixed kernel,
+ * each kept only when its sum beats the fixed kernel's: the work-list kernel for every box (each box
+ * its candidate of least time) and each box by the faster of the two kernels, each at its better
+ * occupancy target; one RK stage of the level itself on U (HIP events, least of 3; the level's error
+ * word is left as it was) picks the fastest of them and the fixed kernel (force 1: the work-list
+ * kernel for every box, -1: the fixed kernel). A box thinner than the ghost width is not swept, and a level holding
  * one keeps the fixed kernel. Allocates device memory while it runs; synchronises. */
 int cmp_level_tune_tiles(cmp_level* level, const double* U, int nextra, const int* extra, int force, void* stream);
 /* The cached sweep of one box shape (n[3]) for the level's scheme: the fixed-shape kernel's ns per
