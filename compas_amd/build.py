@@ -53,6 +53,7 @@ def build(verbose=False):
         ("cmp_batch.hip", "batch.o", []),
         ("cmp_amr.hip", "amr.o", []),
         ("cmp_tune.hip", "tune.o", []),
+        ("cmp_cluster.hip", "cluster.o", []),
     ]
     procs = []
     for src, obj, extra in units:

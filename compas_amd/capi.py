@@ -155,6 +155,8 @@ SIGNATURES = {
     "cmp_tile_cache_save": (ctypes.c_int, [ctypes.c_char_p]),
     "cmp_tile_cache_load": (ctypes.c_int, [ctypes.c_char_p]),
     "cmp_tile_cache_clear": (ctypes.c_int, []),
+    "cmp_cluster_blocks": (ctypes.c_int, [P(ctypes.c_ubyte), ctypes.c_int, P(ctypes.c_int), ctypes.c_double, ctypes.c_int,
+                                          P(ctypes.c_int), P(ctypes.c_int)]),
     "cmp_face_layout": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(cmp_box), ctypes.c_int,
                                        P(ctypes.c_longlong), P(ctypes.c_longlong)]),
     "cmp_abi_pf_size": (ctypes.c_int, [P(ctypes.c_int)]),

@@ -19,24 +19,20 @@ import numpy as np
 
 
 def _grow(tags, n):
+    """tags grown by n cells along each axis in turn (no wrap across the array's ends)."""
     if n <= 0:
         return tags.astype(bool)
     out = tags.astype(bool).copy()
     for ax in range(tags.ndim):
         acc = out.copy()
-        for s in range(1, n + 1):
-            acc |= np.roll(out, s, axis=ax) & _not_wrapped(out.shape, ax, s)
-            acc |= np.roll(out, -s, axis=ax) & _not_wrapped(out.shape, ax, -s)
+        for s in range(1, min(n, out.shape[ax] - 1) + 1):
+            hi = [slice(None)] * out.ndim
+            lo = [slice(None)] * out.ndim
+            hi[ax], lo[ax] = slice(s, None), slice(None, -s)
+            acc[tuple(hi)] |= out[tuple(lo)]
+            acc[tuple(lo)] |= out[tuple(hi)]
         out = acc
     return out
-
-
-def _not_wrapped(shape, ax, s):
-    m = np.ones(shape, dtype=bool)
-    idx = [slice(None)] * len(shape)
-    idx[ax] = slice(0, s) if s > 0 else slice(shape[ax] + s, shape[ax])
-    m[tuple(idx)] = False
-    return m
 
 
 def _coarsen(tags, bf):
@@ -120,16 +116,39 @@ def _chop(lo, hi, maxb):
     return pieces
 
 
-def cluster(tags, blocking_factor=8, max_grid_size=64, grid_eff=0.7, n_error_buf=1, scale=1, n_cell=None):
+def _split_native(block, eff):
+    """_split of the whole block array by the library (cmp_cluster_blocks): the same boxes in the same
+    order."""
+    import ctypes
+    from . import capi
+    b = np.ascontiguousarray(block, dtype=np.uint8)
+    dim = b.ndim
+    nmax = max(1, int(b.sum()))
+    shape = (ctypes.c_int * dim)(*b.shape)
+    out = (ctypes.c_int * (6 * nmax))()
+    n = ctypes.c_int()
+    rc = capi.load().cmp_cluster_blocks(b.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), dim, shape,
+                                        float(eff), nmax, out, ctypes.byref(n))
+    if rc != capi.CMP_OK:
+        raise RuntimeError("cmp_cluster_blocks failed: %s" % capi.ERROR_NAMES.get(rc, rc))
+    return [(list(out[6 * q:6 * q + dim]), list(out[6 * q + 3:6 * q + 3 + dim])) for q in range(n.value)]
+
+
+def cluster(tags, blocking_factor=8, max_grid_size=64, grid_eff=0.7, n_error_buf=1, scale=1, n_cell=None,
+            native=True):
     """Boxes [(lo, hi)] (cell indices, x first, inclusive) covering every tag of `tags`
     ([(k,) j, i]), aligned to the blocking factor, at most max_grid_size cells a side. scale /
     n_cell: the tags stand for blocks of `scale` cells of a grid n_cell ([(k,) j, i]) in size, and
-    the boxes are returned in those cells."""
+    the boxes are returned in those cells. native: the split step in the library (the same boxes as
+    the Python _split, which native=False runs)."""
     dim = tags.ndim
     t = _grow(tags, n_error_buf)
     block = _coarsen(t, blocking_factor)
-    raw = []
-    _split(block, [0] * dim, [s - 1 for s in block.shape], grid_eff, raw)
+    if native:
+        raw = _split_native(block, grid_eff)
+    else:
+        raw = []
+        _split(block, [0] * dim, [s - 1 for s in block.shape], grid_eff, raw)
     maxb = max(1, max_grid_size // blocking_factor)
     out = []
     f = blocking_factor * scale

@@ -320,6 +320,19 @@ int cmp_tile_cache_save(const char* path);
 int cmp_tile_cache_load(const char* path);
 int cmp_tile_cache_clear(void);
 
+/* ---- host clustering of refinement tags ------------------------------------------------------
+ * No counterpart in the reference's sources: the reference hands its TagBoxArray to AMReX's
+ * Berger-Rigoutsos clustering (AmrCore::MakeNewGrids, amr.grid_eff). This is the split step of this
+ * library's own clustering (compas_amd/regrid.py): blocks (dim-dimensional, C order, shape[] with
+ * the slowest axis first; nonzero = a tagged block) are covered by boxes, each at least grid_eff
+ * tagged after shrinking to its tags' bounding box, else cut at the tag-free plane nearest its
+ * middle, else where the second difference of the tag signature changes sign most strongly, else in
+ * the middle of its longest side. Writes *nboxes boxes of 6 ints (lo[3], hi[3], block indices in the
+ * axis order of shape, unused axes 0); CMP_ERR_ARG when they would exceed max_boxes (the number of
+ * tagged blocks always suffices). Host memory only; no GPU work. */
+int cmp_cluster_blocks(const unsigned char* blocks, int dim, const int* shape, double grid_eff, int max_boxes,
+                       int* boxes, int* nboxes);
+
 /* ---- phase-field interface regularization of a level ------------------------------------------
  * Grad_Q face buffers below follow cmp_face_layout with one component per volume fraction: one for
  * the two-phase model (NPHASE = 1), nphase for the N-phase model (component k is phase k's Grad_Q,

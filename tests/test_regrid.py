@@ -104,3 +104,42 @@ def test_nest_in_3d_keeps_whole_blocks_inside_the_parent_with_a_margin():
     got = (m > 0).reshape(16, 4, 16, 4, 16, 4).all(axis=(1, 3, 5))
     assert np.all(got[blk])
     assert np.all(~(m > 0) | want)
+
+
+@pytest.mark.parametrize("dim,seed", [(2, 0), (2, 1), (2, 2), (3, 3), (3, 4)])
+def test_native_split_equals_python_split(dim, seed):
+    """The library's split step (cmp_cluster_blocks) gives the Python _split's boxes in its order, on
+    random sparse tags, on clustered tags with holes, and with grid_eff 0 and 1."""
+    from compas_amd import regrid
+    rng = np.random.default_rng(seed)
+    n = 64 if dim == 2 else 24
+    shape = (n,) * dim
+    sparse = rng.random(shape) < 0.02
+    grid = np.indices(shape)
+    c = np.array(shape)[:, None] / 2.0
+    r = np.sqrt(((grid.reshape(dim, -1) - c) ** 2).sum(axis=0)).reshape(shape)
+    rings = (np.abs(r - n * 0.2) < 1.5) | (np.abs(r - n * 0.35) < 1.0)
+    for tags in (sparse, rings, sparse | rings):
+        for eff in (0.0, 0.5, 0.7, 1.0):
+            for bf, nbuf in ((1, 0), (2, 1), (4, 1)):
+                a = regrid.cluster(tags.astype(np.uint8), blocking_factor=bf, max_grid_size=16 * bf,
+                                   grid_eff=eff, n_error_buf=nbuf, native=True)
+                b = regrid.cluster(tags.astype(np.uint8), blocking_factor=bf, max_grid_size=16 * bf,
+                                   grid_eff=eff, n_error_buf=nbuf, native=False)
+                assert a == b, (eff, bf, nbuf)
+    assert len(a) > 1
+
+
+def test_grow_is_the_dilation_without_wrap():
+    """_grow(tags, n): a cell is tagged when a tagged cell lies within n cells along one axis, axis
+    by axis (a box dilation of radius n), nothing carried across the array's ends."""
+    from compas_amd import regrid
+    rng = np.random.default_rng(0)
+    for shape in [(7, 9), (1, 5), (2, 2), (5, 6, 7), (40, 33)]:
+        t = rng.random(shape) < 0.1
+        for n in (0, 1, 2, 3, 8):
+            want = np.zeros(shape, dtype=bool)
+            for idx in zip(*np.nonzero(t)):
+                sl = tuple(slice(max(0, i - n), i + n + 1) for i in idx)
+                want[sl] = True
+            assert np.array_equal(regrid._grow(t, n), want), (shape, n)
